@@ -1,0 +1,307 @@
+"""ctypes wrapper of the CPU FP64 oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+bench.py's cpu_baseline leg as the checker. The product package never imports
+this module.
+
+`build_refined_keys` restates the recursive refinement of ``_build_tree``
+(TwoPhaseAMR.jl ``src/mesh/topology/mesh.jl:720-736``); the C++ side then
+restates balancing, leaf order and the face schedule.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build(force: bool = False) -> str:
+    path = os.path.join(_HERE, "liboracle.so")
+    if force or not os.path.exists(path):
+        subprocess.run(["make", "-C", _HERE, "-B" if force else "liboracle.so"],
+                       check=True, capture_output=True)
+    return path
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = C.CDLL(path)
+        _LIB.orc_last_error.restype = C.c_char_p
+    return _LIB
+
+
+class Stats(C.Structure):
+    _fields_ = [("cycles", C.c_int), ("converged", C.c_int), ("compatible", C.c_int),
+                ("pad", C.c_int), ("initial_l2", C.c_double), ("initial_linf", C.c_double),
+                ("final_l2", C.c_double), ("final_linf", C.c_double),
+                ("target_linf", C.c_double), ("residual_scale", C.c_double),
+                ("removed_rhs_mean", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def build_refined_keys(base_mesh, block_size, origin, extent, max_level, refine):
+    """Keys (level, i, j) for which ``refine(key, origin, extent)`` splits a node,
+    visiting every root depth first like ``_build_tree``."""
+    layout = (base_mesh[0] // block_size[0], base_mesh[1] // block_size[1])
+    root_extent = (extent[0] / layout[0], extent[1] / layout[1])
+    keys = []
+
+    def visit(level, i, j, o, w):
+        if level >= max_level:
+            return
+        if refine((level, i, j), o, w):
+            keys.append((level, i, j))
+            hw = (w[0] / 2, w[1] / 2)
+            for cj in (0, 1):
+                for ci in (0, 1):
+                    visit(level + 1, 2 * i + ci, 2 * j + cj,
+                          (o[0] + ci * hw[0], o[1] + cj * hw[1]), hw)
+
+    for j in range(layout[1]):
+        for i in range(layout[0]):
+            o = (origin[0] if i == 0 else origin[0] + i * root_extent[0],
+                 origin[1] if j == 0 else origin[1] + j * root_extent[1])
+            visit(0, i, j, o, root_extent)
+    return keys
+
+
+class OMesh:
+    def __init__(self, base_mesh, block_size=(16, 16), origin=(0.0, 0.0), extent=(1.0, 1.0),
+                 periodic=(True, True), max_level=0, refine=None, refined_keys=None):
+        L = lib()
+        if refined_keys is None:
+            refined_keys = [] if refine is None else build_refined_keys(
+                base_mesh, block_size, origin, extent, max_level, refine)
+        keys = np.asarray(refined_keys, dtype=np.int32).reshape(-1, 3)
+        base = np.asarray(base_mesh, np.int32)
+        bs = np.asarray(block_size, np.int32)
+        org = np.asarray(origin, np.float64)
+        ext = np.asarray(extent, np.float64)
+        per = np.asarray([int(bool(p)) for p in periodic], np.int32)
+        out = C.c_void_p()
+        _check(L.orc_mesh_new(_iptr(base), _iptr(bs), _dptr(org), _dptr(ext), _iptr(per),
+                              int(max_level), len(keys), _iptr(np.ascontiguousarray(keys)),
+                              C.byref(out)))
+        self._h = out
+        self._owned = True
+        self._load()
+
+    @classmethod
+    def _wrap(cls, handle):
+        self = cls.__new__(cls)
+        self._h = handle
+        self._owned = False
+        self._load()
+        return self
+
+    def _load(self):
+        L = lib()
+        s = np.zeros(5, np.int32)
+        L.orc_mesh_sizes(self._h, _iptr(s))
+        self.nblocks, self.ncells, self.nfaces, self.nbfaces, self.max_leaf_level = map(int, s)
+        self.block_keys = np.zeros((self.nblocks, 3), np.int32)
+        self.block_geom = np.zeros((self.nblocks, 7))
+        L.orc_mesh_blocks(self._h, _iptr(self.block_keys), _dptr(self.block_geom))
+        fi = np.zeros((self.nfaces, 3), np.int32)
+        fd = np.zeros((self.nfaces, 3))
+        L.orc_mesh_faces(self._h, _iptr(fi), _dptr(fd))
+        self.face_minus, self.face_plus, self.face_axis = fi[:, 0].copy(), fi[:, 1].copy(), fi[:, 2].copy()
+        self.face_center, self.face_area = fd[:, :2].copy(), fd[:, 2].copy()
+        bi = np.zeros((self.nbfaces, 3), np.int32)
+        bd = np.zeros((self.nbfaces, 3))
+        L.orc_mesh_bfaces(self._h, _iptr(bi), _dptr(bd))
+        self.bface_cell, self.bface_axis, self.bface_side = bi[:, 0].copy(), bi[:, 1].copy(), bi[:, 2].copy()
+        self.bface_center, self.bface_area = bd[:, :2].copy(), bd[:, 2].copy()
+        nb = None
+        self.volumes = np.repeat(self.block_geom[:, 6], self.cells_per_block) if self.nblocks else np.zeros(0)
+
+    @property
+    def cells_per_block(self):
+        return self.ncells // max(self.nblocks, 1)
+
+    def cell_centers(self):
+        nxny = self.cells_per_block
+        g = self.block_geom
+        # block size from the cell count and the spacing/extent ratio
+        nx = int(round(g[0, 2] / g[0, 4]))
+        ny = nxny // nx
+        ii, jj = np.meshgrid(np.arange(nx), np.arange(ny))
+        ii = ii.ravel()
+        jj = jj.ravel()
+        x = g[:, 0:1] + (ii[None, :] + 0.5) * g[:, 4:5]
+        y = g[:, 1:2] + (jj[None, :] + 0.5) * g[:, 5:6]
+        return x.ravel(), y.ravel()
+
+    def metric(self):
+        L = lib()
+        nnz = C.c_int()
+        diag = C.c_int()
+        _check(L.orc_metric(self._h, C.byref(nnz), C.byref(diag), None, None, None))
+        colptr = np.zeros(self.nfaces + 1, np.int32)
+        rowval = np.zeros(nnz.value, np.int32)
+        nzval = np.zeros(nnz.value)
+        _check(L.orc_metric(self._h, C.byref(nnz), C.byref(diag), _iptr(colptr), _iptr(rowval), _dptr(nzval)))
+        import scipy.sparse as sp
+        return sp.csc_matrix((nzval, rowval, colptr), shape=(self.nfaces, self.nfaces)), bool(diag.value)
+
+    def mixture_coefficients(self, c, rho_l, rho_g):
+        k = np.zeros(self.nfaces)
+        _check(lib().orc_mixture_coefficients(self._h, _dptr(np.ascontiguousarray(c, np.float64)),
+                                              C.c_double(rho_l), C.c_double(rho_g), _dptr(k)))
+        return k
+
+    def cgfm_jumps(self, c, curvature, sigma, csf=False):
+        out = np.zeros(self.nfaces)
+        curv = np.broadcast_to(np.asarray(curvature, np.float64), (self.nfaces,)).copy()
+        _check(lib().orc_cgfm_jumps(self._h, _dptr(np.ascontiguousarray(c, np.float64)), _dptr(curv),
+                                    C.c_double(sigma), int(csf), _dptr(out)))
+        return out
+
+    def divergence(self, u, ub=None):
+        out = np.zeros(self.ncells)
+        ubp = None if ub is None else _dptr(np.ascontiguousarray(ub, np.float64))
+        _check(lib().orc_divergence(self._h, _dptr(np.ascontiguousarray(u, np.float64)), ubp, _dptr(out)))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and self._h:
+            lib().orc_mesh_free(self._h)
+            self._h = None
+
+
+class OSolver:
+    """EllipticSolver(PoissonOperator(mesh; face_coefficient)) with the tree
+    geometric multigrid V-cycle (or plain CG with ``multigrid=False``)."""
+
+    def __init__(self, mesh: OMesh, kappa=None, coarse_size=64, pre_sweeps=2, post_sweeps=2,
+                 max_levels=32, multigrid=True):
+        self.mesh = mesh
+        k = np.ones(mesh.nfaces) if kappa is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(kappa, np.float64), (mesh.nfaces,)), np.float64)
+        out = C.c_void_p()
+        _check(lib().orc_solver_new(mesh._h, _dptr(k), coarse_size, pre_sweeps, post_sweeps,
+                                    max_levels, int(multigrid), C.byref(out)))
+        self._h = out
+        self.nlevels = lib().orc_solver_nlevels(self._h)
+
+    def level_info(self, l):
+        ints = np.zeros(5, np.int32)
+        om = C.c_double()
+        lib().orc_solver_level(self._h, l, _iptr(ints), C.byref(om))
+        return dict(ncells=int(ints[0]), nfaces=int(ints[1]), nnz_A=int(ints[2]), nnz_P=int(ints[3]),
+                    K_diagonal=bool(ints[4]), omega=om.value)
+
+    def level_matrix(self, l, which="A"):
+        import scipy.sparse as sp
+        info = self.level_info(l)
+        if which == "A":
+            m = n = info["ncells"]
+            nnz = info["nnz_A"]
+        else:
+            m = info["ncells"]
+            n = self.level_info(l + 1)["ncells"]
+            nnz = info["nnz_P"]
+        colptr = np.zeros(n + 1, np.int32)
+        rowval = np.zeros(nnz, np.int32)
+        nzval = np.zeros(nnz)
+        lib().orc_solver_level_matrix(self._h, l, 0 if which == "A" else 1, _iptr(colptr), _iptr(rowval), _dptr(nzval))
+        return sp.csc_matrix((nzval, rowval, colptr), shape=(m, n))
+
+    def level_kappa(self, l):
+        k = np.zeros(self.level_info(l)["nfaces"])
+        lib().orc_solver_level_kappa(self._h, l, _dptr(k))
+        return k
+
+    def level_mesh(self, l):
+        h = C.c_void_p()
+        lib().orc_solver_level_mesh(self._h, l, C.byref(h))
+        return OMesh._wrap(h)
+
+    def apply(self, x):
+        y = np.zeros(self.mesh.ncells)
+        _check(lib().orc_apply(self._h, _dptr(np.ascontiguousarray(x, np.float64)), _dptr(y)))
+        return y
+
+    def apply_level(self, l, x):
+        y = np.zeros(len(x))
+        _check(lib().orc_apply_level(self._h, l, _dptr(np.ascontiguousarray(x, np.float64)), _dptr(y)))
+        return y
+
+    def vcycle(self, r):
+        z = np.zeros(self.mesh.ncells)
+        _check(lib().orc_vcycle(self._h, _dptr(np.ascontiguousarray(r, np.float64)), _dptr(z)))
+        return z
+
+    def solve(self, rhs, x0=None, rtol=1e-10, atol=0.0, max_cycles=100, compatibility="project"):
+        x = np.zeros(self.mesh.ncells) if x0 is None else np.array(x0, np.float64)
+        st = Stats()
+        rm = np.zeros(max(self.mesh.ncells, 1))
+        _check(lib().orc_solve(self._h, _dptr(np.ascontiguousarray(rhs, np.float64)), _dptr(x),
+                               C.c_double(rtol), C.c_double(atol), int(max_cycles),
+                               int(compatibility == "error"), C.byref(st), _dptr(rm)))
+        return x, st.as_dict()
+
+    def weighted_gradient(self, p):
+        g = np.zeros(self.mesh.nfaces)
+        _check(lib().orc_weighted_gradient(self._h, _dptr(np.ascontiguousarray(p, np.float64)), _dptr(g)))
+        return g
+
+    def weighted_jump_force(self, jumps):
+        a = np.zeros(self.mesh.nfaces)
+        _check(lib().orc_weighted_jump_force(self._h, _dptr(np.ascontiguousarray(jumps, np.float64)), _dptr(a)))
+        return a
+
+    def project(self, u, p, dt, tolerance=np.sqrt(np.finfo(float).eps), max_cycles=100):
+        u = np.array(u, np.float64)
+        p = np.array(p, np.float64)
+        phi = np.zeros(self.mesh.ncells)
+        st = Stats()
+        _check(lib().orc_project(self._h, _dptr(u), _dptr(p), C.c_double(dt), C.c_double(tolerance),
+                                 int(max_cycles), C.byref(st), _dptr(phi)))
+        return u, p, phi, st.as_dict()
+
+    def adjoint(self, w, p, rtol=1e-12, max_cycles=200):
+        db = np.zeros(self.mesh.ncells)
+        dk = np.zeros(self.mesh.nfaces)
+        lam = np.zeros(self.mesh.ncells)
+        st = Stats()
+        _check(lib().orc_adjoint(self._h, _dptr(np.ascontiguousarray(w, np.float64)),
+                                 _dptr(np.ascontiguousarray(p, np.float64)), C.c_double(rtol),
+                                 int(max_cycles), _dptr(db), _dptr(dk), _dptr(lam), C.byref(st)))
+        return db, dk, lam, st.as_dict()
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_solver_free(self._h)
+            self._h = None
+
+
+def techref_noise(n):
+    """``solv_noise`` (docs/techref/figures/solvers.jl:26) at 1-based indices."""
+    idx = np.arange(1, n + 1, dtype=np.float64)
+    v = np.sin(12.9898 * idx) * 43758.5453
+    return v - np.floor(v) - 0.5

@@ -1,0 +1,1021 @@
+// Oracle: elliptic pressure operator, tree geometric multigrid, Krylov CG with
+// the reference's stopping rules, two-phase coefficients, projection and the
+// pressure adjoint. TEST INFRASTRUCTURE ONLY (see oracle.hpp).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <tuple>
+#include <unordered_map>
+
+#include "oracle.hpp"
+
+namespace orc {
+
+static const double EPS = std::numeric_limits<double>::epsilon();
+
+// COO -> CSC with duplicate summation in input order (SparseArrays `sparse`).
+// Explicit zeros are kept: the reference keeps cancelled stiffness entries in
+// the pattern (elliptic.jl:166-171).
+static CSC coo_to_csc(int m, int n, const std::vector<int>& r, const std::vector<int>& c,
+                      const std::vector<double>& v) {
+  std::vector<int> order(r.size());
+  for (size_t q = 0; q < order.size(); ++q) order[q] = int(q);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return c[a] != c[b] ? c[a] < c[b] : r[a] < r[b];
+  });
+  CSC A;
+  A.m = m;
+  A.n = n;
+  A.colptr.assign(n + 1, 0);
+  size_t q = 0;
+  for (int col = 0; col < n; ++col) {
+    while (q < order.size() && c[order[q]] == col) {
+      const int row = r[order[q]];
+      double s = v[order[q]];
+      ++q;
+      while (q < order.size() && c[order[q]] == col && r[order[q]] == row) {
+        s += v[order[q]];
+        ++q;
+      }
+      A.rowval.push_back(row);
+      A.nzval.push_back(s);
+    }
+    A.colptr[col + 1] = int(A.rowval.size());
+  }
+  return A;
+}
+
+static int csc_find(const CSC& A, int row, int col) {
+  const int* b = A.rowval.data() + A.colptr[col];
+  const int* e = A.rowval.data() + A.colptr[col + 1];
+  const int* p = std::lower_bound(b, e, row);
+  return (p != e && *p == row) ? int(p - A.rowval.data()) : -1;
+}
+
+// `_integrated_stiffness`, elliptic.jl:175-257: symmetric (A + A')/2 of
+// J' M J with M = kappa K (diagonal K) or sqrt(kappa) K sqrt(kappa).
+static CSC integrated_stiffness(const Mesh& M, const CSC& K, bool diag,
+                                const std::vector<double>& kappa,
+                                const std::vector<double>& sq) {
+  const int N = M.ncells();
+  std::vector<int> r, c;
+  std::vector<double> v;
+  r.reserve(4 * K.nnz() + N);
+  c.reserve(4 * K.nnz() + N);
+  v.reserve(4 * K.nnz() + N);
+  for (int rf = 0; rf < K.n; ++rf)
+    for (int p = K.colptr[rf]; p < K.colptr[rf + 1]; ++p) {
+      const int lf = K.rowval[p];
+      const double w = diag ? kappa[rf] * K.nzval[p] : (sq[lf] * K.nzval[p]) * sq[rf];
+      const Face& L = M.faces[lf];
+      const Face& R = M.faces[rf];
+      const int lc[2] = {L.minus, L.plus}, rc[2] = {R.minus, R.plus};
+      const double ls[2] = {-L.area, L.area}, rs[2] = {-R.area, R.area};
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+          r.push_back(lc[a]);
+          c.push_back(rc[b]);
+          v.push_back(ls[a] * w * rs[b]);
+        }
+    }
+  for (int cell = 0; cell < N; ++cell) {
+    r.push_back(cell);
+    c.push_back(cell);
+    v.push_back(0.0);
+  }
+  CSC A = coo_to_csc(N, N, r, c, v);
+  std::vector<double> sym(A.nnz());
+  for (int col = 0; col < N; ++col)
+    for (int p = A.colptr[col]; p < A.colptr[col + 1]; ++p) {
+      const int mirror = csc_find(A, col, A.rowval[p]);
+      if (mirror < 0) throw std::runtime_error("sparse topology is missing a required entry");
+      const double val = (A.nzval[p] + A.nzval[mirror]) / 2;
+      sym[p] = val == 0.0 ? 0.0 : val;
+    }
+  A.nzval = sym;
+  return A;
+}
+
+// `_projection_component_labels!` (two_phase/projection/common.jl:53-87) and
+// `_refresh_elliptic_components!` (elliptic_solver.jl:39-109) for an operator
+// with all cells active and homogeneous-Neumann (wall or periodic) sides:
+// every component is ungauged.
+static Components components(const Mesh& M, const std::vector<double>& kappa,
+                             const std::vector<double>& volumes) {
+  const int N = M.ncells();
+  std::vector<std::vector<int>> inc(N);
+  for (int f = 0; f < int(M.faces.size()); ++f) {
+    inc[M.faces[f].minus].push_back(f);
+    if (M.faces[f].plus != M.faces[f].minus) inc[M.faces[f].plus].push_back(f);
+  }
+  Components C;
+  C.labels.assign(N, -1);
+  std::vector<int> stack;
+  for (int root = 0; root < N; ++root) {
+    if (!(volumes[root] > 0) || C.labels[root] >= 0) continue;
+    C.labels[root] = root;
+    stack.assign(1, root);
+    while (!stack.empty()) {
+      const int cell = stack.back();
+      stack.pop_back();
+      for (int f : inc[cell]) {
+        if (kappa[f] == 0.0) continue;
+        const int nb = cell == M.faces[f].minus ? M.faces[f].plus : M.faces[f].minus;
+        if (C.labels[nb] >= 0 || !(volumes[nb] > 0)) continue;
+        C.labels[nb] = root;
+        stack.push_back(nb);
+      }
+    }
+  }
+  std::vector<double> vol(N, 0.0);
+  for (int i = 0; i < N; ++i)
+    if (C.labels[i] >= 0) vol[C.labels[i]] += volumes[i];
+  C.slots.assign(N, 0);
+  for (int i = 0; i < N; ++i)
+    if (C.labels[i] == i) {
+      C.anchors.push_back(i);
+      C.volumes.push_back(vol[i]);
+      C.slots[i] = int(C.anchors.size());
+    }
+  for (int i = 0; i < N; ++i)
+    if (C.labels[i] >= 0 && C.labels[i] != i) C.slots[i] = C.slots[C.labels[i]];
+  return C;
+}
+
+// EllipticOperator / PoissonOperator (elliptic.jl:683-800, 969-980) with
+// reaction 0, diffusion 1, all cells active, no open sides.
+static std::unique_ptr<Operator> make_operator(const Mesh* mesh, const std::vector<double>& kappa) {
+  auto op = std::make_unique<Operator>();
+  op->mesh = mesh;
+  if (kappa.size() != mesh->faces.size())
+    throw std::invalid_argument("one coefficient is required per interior face");
+  for (double k : kappa) {
+    if (!std::isfinite(k)) throw std::invalid_argument("face coefficients must be finite");
+    if (!(k >= 0)) throw std::invalid_argument("face coefficients must be nonnegative");
+  }
+  op->kappa = kappa;
+  op->sqrt_kappa.resize(kappa.size());
+  for (size_t f = 0; f < kappa.size(); ++f) op->sqrt_kappa[f] = std::sqrt(kappa[f]);
+  op->K = inverse_metric(*mesh, &op->K_diagonal);
+  op->volumes.resize(mesh->ncells());
+  const int nb = mesh->bs[0] * mesh->bs[1];
+  for (int c = 0; c < mesh->ncells(); ++c) op->volumes[c] = mesh->blocks[c / nb].volume;
+  op->A = integrated_stiffness(*mesh, op->K, op->K_diagonal, op->kappa, op->sqrt_kappa);
+  op->comp = components(*mesh, op->kappa, op->volumes);
+  op->singular = !op->comp.anchors.empty();
+  return op;
+}
+
+// Factorized action `mul!(::FactorizedEllipticOperator)`, elliptic.jl:1125-1165
+static void factorized_mul(const Operator& op, const double* x, double* y) {
+  const Mesh& M = *op.mesh;
+  const int F = int(M.faces.size());
+  std::vector<double> fin(F), fout(F, 0.0);
+  for (int f = 0; f < F; ++f) {
+    const Face& fa = M.faces[f];
+    fin[f] = op.sqrt_kappa[f] * fa.area * (x[fa.plus] - x[fa.minus]);
+  }
+  if (op.K_diagonal) {
+    for (int f = 0; f < F; ++f) fout[f] = op.K.nzval[op.K.colptr[f]] * fin[f];
+  } else {
+    // CSR row product on the symmetric K (column r == row r)
+    for (int f = 0; f < F; ++f) {
+      double s = 0;
+      for (int p = op.K.colptr[f]; p < op.K.colptr[f + 1]; ++p) s += op.K.nzval[p] * fin[op.K.rowval[p]];
+      fout[f] = s;
+    }
+  }
+  const int N = M.ncells();
+  for (int c = 0; c < N; ++c) y[c] = 0.0 * x[c];
+  // cell gather in face order (the cell-face incidence order)
+  for (int f = 0; f < F; ++f) {
+    const Face& fa = M.faces[f];
+    const double flux = 1.0 * fa.area * op.sqrt_kappa[f] * fout[f];
+    y[fa.minus] += -1.0 * flux;
+    y[fa.plus] += flux;
+  }
+}
+
+static void csc_sym_mul(const CSC& A, const double* x, double* y) {
+  for (int i = 0; i < A.n; ++i) {
+    double s = 0;
+    for (int p = A.colptr[i]; p < A.colptr[i + 1]; ++p) s += A.nzval[p] * x[A.rowval[p]];
+    y[i] = s;
+  }
+}
+
+// `_jacobi_weight!`, hierarchy.jl:8-30
+static double jacobi_weight(std::vector<double>& invdiag, const std::vector<double>& rowsum,
+                            bool skip_nonpositive, bool allow_zero, bool clamp, double zero_weight) {
+  double bound = 0;
+  for (size_t i = 0; i < invdiag.size(); ++i) {
+    const double d = invdiag[i];
+    if (d == 0.0 && allow_zero) continue;
+    if (!(d > 0)) {
+      if (skip_nonpositive) continue;
+      throw std::invalid_argument("elliptic matrix has a non-positive diagonal");
+    }
+    bound = std::max(bound, rowsum[i] / d);
+    invdiag[i] = 1.0 / d;
+  }
+  if (bound == 0.0) return zero_weight;
+  const double w = 4.0 / (3.0 * bound);
+  return clamp ? std::min(1.0, w) : w;
+}
+
+// ---------------------------------------------------------------------------
+// Tree coarsening, src/solvers/multigrid/tree_coarsening.jl
+
+// `_all_leaf_pruned_mesh`, tree_coarsening.jl:30-57
+static std::unique_ptr<Mesh> all_leaf_pruned(const Mesh& M) {
+  std::vector<Key> retained;
+  int maxl = -1;
+  for (uint64_t pk : M.internal) {
+    const int level = int(pk >> 56), i = int((pk >> 28) & 0xFFFFFFF), j = int(pk & 0xFFFFFFF);
+    bool has_internal_child = false;
+    for (int c = 0; c < 4 && !has_internal_child; ++c)
+      has_internal_child = M.internal.count(kpack(level + 1, 2 * i + (c & 1), 2 * j + (c >> 1))) > 0;
+    if (has_internal_child) {
+      retained.push_back({level, i, j});
+      maxl = std::max(maxl, level);
+    }
+  }
+  const bool per[2] = {M.periodic[0], M.periodic[1]};
+  return build_mesh(M.base, M.bs, M.origin, M.extent, per, maxl + 1, retained);
+}
+
+// `_coarsened_base_mesh`, tree_coarsening.jl:62-82
+static std::unique_ptr<Mesh> coarsened_base(const Mesh& M) {
+  if (M.layout[0] % 2 || M.layout[1] % 2) return nullptr;
+  const int cb[2] = {M.base[0] / 2, M.base[1] / 2};
+  for (int a = 0; a < 2; ++a)
+    if (M.periodic[a] && cb[a] < 4) return nullptr;
+  const bool per[2] = {M.periodic[0], M.periodic[1]};
+  return build_mesh(cb, M.bs, M.origin, M.extent, per, 0, {});
+}
+
+static void common_lattice(const Mesh& f, const Mesh& c, long lat[2]) {
+  for (int a = 0; a < 2; ++a)
+    lat[a] = std::max(long(f.base[a]) << f.max_level, long(c.base[a]) << c.max_level);
+}
+
+static void cell_bounds(const Mesh& M, int cell, const long lat[2], long low[2], long scale[2]) {
+  const int nb = M.bs[0] * M.bs[1];
+  const int b = cell / nb, r = cell % nb;
+  const int i = r % M.bs[0], j = r / M.bs[0];
+  const Key& k = M.blocks[b].key;
+  scale[0] = lat[0] / (long(M.base[0]) << k.level);
+  scale[1] = lat[1] / (long(M.base[1]) << k.level);
+  low[0] = (long(k.i) * M.bs[0] + i) * scale[0];
+  low[1] = (long(k.j) * M.bs[1] + j) * scale[1];
+}
+
+// `_tree_cell_prolongation`, tree_coarsening.jl:228-252: owner coarse cell of
+// each fine cell (the coarse cell containing the fine cell centre).
+static std::vector<int> tentative_owners(const Mesh& fine, const Mesh& coarse) {
+  long lat[2];
+  common_lattice(fine, coarse, lat);
+  std::unordered_map<uint64_t, int> leaf;
+  for (int b = 0; b < int(coarse.blocks.size()); ++b)
+    leaf[kpack(coarse.blocks[b].key.level, coarse.blocks[b].key.i, coarse.blocks[b].key.j)] = b;
+  const int maxl = coarse.max_leaf_level();
+  std::vector<int> owner(fine.ncells());
+  for (int cell = 0; cell < fine.ncells(); ++cell) {
+    long low[2], sc[2];
+    cell_bounds(fine, cell, lat, low, sc);
+    const long px = low[0] + sc[0] / 2, py = low[1] + sc[1] / 2;
+    int found = -1;
+    for (int l = 0; l <= maxl && found < 0; ++l) {
+      const long bw0 = lat[0] / (long(coarse.layout[0]) << l), bw1 = lat[1] / (long(coarse.layout[1]) << l);
+      auto it = leaf.find(kpack(l, int(px / bw0), int(py / bw1)));
+      if (it == leaf.end()) continue;
+      const int b = it->second;
+      const long cs0 = lat[0] / ((long(coarse.layout[0]) * coarse.bs[0]) << l);
+      const long cs1 = lat[1] / ((long(coarse.layout[1]) * coarse.bs[1]) << l);
+      const int ci = int((px / cs0) % coarse.bs[0]), cj = int((py / cs1) % coarse.bs[1]);
+      found = b * coarse.bs[0] * coarse.bs[1] + cj * coarse.bs[0] + ci;
+    }
+    if (found < 0) throw std::runtime_error("tentative transfer leaves a cell unowned");
+    owner[cell] = found;
+  }
+  return owner;
+}
+
+struct FaceRestriction {
+  std::vector<int> ptr, idx;
+  std::vector<double> w;
+};
+
+// `_face_coefficient_restriction`, tree_coarsening.jl:347-398
+static FaceRestriction face_restriction(const Mesh& coarse, const Mesh& fine) {
+  long lat[2];
+  common_lattice(fine, coarse, lat);
+  std::map<std::pair<int, long>, std::vector<std::tuple<long, long, int>>> lines;
+  for (int f = 0; f < int(fine.faces.size()); ++f) {
+    long n, lo, hi;
+    long ml[2], ms[2], pl[2], ps[2];
+    const Face& fa = fine.faces[f];
+    cell_bounds(fine, fa.minus, lat, ml, ms);
+    cell_bounds(fine, fa.plus, lat, pl, ps);
+    const int a = fa.axis, t = 1 - a;
+    n = ml[a] + ms[a];
+    if (fine.periodic[a]) n = ((n % lat[a]) + lat[a]) % lat[a];
+    lo = std::max(ml[t], pl[t]);
+    hi = std::min(ml[t] + ms[t], pl[t] + ps[t]);
+    lines[{a, n}].emplace_back(lo, hi, f);
+  }
+  for (auto& kv : lines)
+    std::stable_sort(kv.second.begin(), kv.second.end(),
+                     [](const auto& x, const auto& y) { return std::get<0>(x) < std::get<0>(y); });
+  FaceRestriction R;
+  R.ptr.push_back(0);
+  for (const Face& fa : coarse.faces) {
+    long ml[2], ms[2], pl[2], ps[2];
+    cell_bounds(coarse, fa.minus, lat, ml, ms);
+    cell_bounds(coarse, fa.plus, lat, pl, ps);
+    const int a = fa.axis, t = 1 - a;
+    long n = ml[a] + ms[a];
+    if (coarse.periodic[a]) n = ((n % lat[a]) + lat[a]) % lat[a];
+    const long lo = std::max(ml[t], pl[t]), hi = std::min(ml[t] + ms[t], pl[t] + ps[t]);
+    long covered = 0;
+    auto it = lines.find({a, n});
+    if (it != lines.end()) {
+      for (const auto& seg : it->second) {
+        const long flo = std::get<0>(seg), fhi = std::get<1>(seg);
+        if (fhi <= lo) continue;
+        if (flo >= hi) break;
+        const long overlap = std::min(hi, fhi) - std::max(lo, flo);
+        if (overlap > 0) {
+          R.idx.push_back(std::get<2>(seg));
+          R.w.push_back(double(overlap) / double(hi - lo));
+          covered += overlap;
+        }
+      }
+    }
+    if (covered != hi - lo) throw std::runtime_error("tree hierarchy could not cover a coarse face with fine faces");
+    R.ptr.push_back(int(R.idx.size()));
+  }
+  return R;
+}
+
+// `_smoothed_transfer_refresh` + `_refresh_smoothed_transfer!`,
+// hierarchy.jl:631-721: P = (I - w_P D^-1 S) P_t on the pattern
+// P_t union pattern(S) P_t.
+static CSC smoothed_transfer(const CSC& S, const std::vector<int>& owner, int ncoarse) {
+  const int nf = S.n;
+  std::vector<int> r, c;
+  std::vector<double> v;
+  for (int row = 0; row < nf; ++row) {
+    r.push_back(row);
+    c.push_back(owner[row]);
+    v.push_back(0.0);
+  }
+  for (int col = 0; col < nf; ++col)
+    for (int p = S.colptr[col]; p < S.colptr[col + 1]; ++p) {
+      r.push_back(S.rowval[p]);
+      c.push_back(owner[col]);
+      v.push_back(0.0);
+    }
+  CSC P = coo_to_csc(nf, ncoarse, r, c, v);
+  std::vector<double> invd(nf), rowsum(nf);
+  for (int col = 0; col < nf; ++col) {
+    double s = 0, d = 0;
+    for (int p = S.colptr[col]; p < S.colptr[col + 1]; ++p) {
+      s += std::fabs(S.nzval[p]);
+      if (S.rowval[p] == col) d = S.nzval[p];
+    }
+    rowsum[col] = s;
+    invd[col] = d;
+  }
+  const double w = jacobi_weight(invd, rowsum, true, false, false, 0.0);
+  for (int row = 0; row < nf; ++row) P.nzval[csc_find(P, row, owner[row])] = 1.0;
+  // contributions in stiffness-pointer order
+  for (int col = 0; col < nf; ++col)
+    for (int p = S.colptr[col]; p < S.colptr[col + 1]; ++p) {
+      const int row = S.rowval[p];
+      const int q = csc_find(P, row, owner[col]);
+      P.nzval[q] -= w * invd[row] * S.nzval[p];
+    }
+  return P;
+}
+
+static void level_components(Level& L) {
+  const Components& C = L.op->comp;
+  const int N = int(L.op->volumes.size());
+  L.comp_index.assign(N, 0);
+  L.comp_volumes.clear();
+  for (int cell = 0; cell < N; ++cell) {
+    const int root = C.labels[cell];
+    if (root < 0) continue;
+    if (cell == root) {
+      L.comp_volumes.push_back(C.volumes[C.slots[root] - 1]);
+      L.comp_index[cell] = int(L.comp_volumes.size());
+    } else {
+      L.comp_index[cell] = L.comp_index[root];
+    }
+  }
+}
+
+static void lu_factor(std::vector<double>& a, std::vector<int>& piv, int n) {
+  piv.resize(n);
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double best = std::fabs(a[size_t(k) * n + k]);
+    for (int i = k + 1; i < n; ++i)
+      if (std::fabs(a[size_t(i) * n + k]) > best) {
+        best = std::fabs(a[size_t(i) * n + k]);
+        p = i;
+      }
+    piv[k] = p;
+    if (p != k)
+      for (int j = 0; j < n; ++j) std::swap(a[size_t(k) * n + j], a[size_t(p) * n + j]);
+    const double d = a[size_t(k) * n + k];
+    if (d == 0.0) throw std::runtime_error("singular coarse matrix");
+    for (int i = k + 1; i < n; ++i) {
+      const double l = a[size_t(i) * n + k] / d;
+      a[size_t(i) * n + k] = l;
+      if (l != 0.0)
+        for (int j = k + 1; j < n; ++j) a[size_t(i) * n + j] -= l * a[size_t(k) * n + j];
+    }
+  }
+}
+
+static void lu_solve(const std::vector<double>& a, const std::vector<int>& piv, int n, double* x) {
+  for (int k = 0; k < n; ++k)
+    if (piv[k] != k) std::swap(x[k], x[piv[k]]);
+  for (int i = 0; i < n; ++i) {
+    double s = x[i];
+    for (int j = 0; j < i; ++j) s -= a[size_t(i) * n + j] * x[j];
+    x[i] = s;
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = x[i];
+    for (int j = i + 1; j < n; ++j) s -= a[size_t(i) * n + j] * x[j];
+    x[i] = s / a[size_t(i) * n + i];
+  }
+}
+
+// TreeGeometricMultigridPreconditioner constructor (pruning = :all, no
+// reuse, no geometry), geometric_preconditioner.jl:171-442, with
+// `_tree_next_coarse_mesh!` (tree_coarsening.jl:187-212).
+Solver* build_solver(const Mesh& fine, const std::vector<double>& kappa, int coarse_size, int pre,
+                     int post, int max_levels, bool use_mg) {
+  auto S = new Solver();
+  S->pre = pre;
+  S->post = post;
+  S->coarse_size = coarse_size;
+  S->max_levels = max_levels;
+  S->use_mg = use_mg;
+  S->ops.push_back(make_operator(&fine, kappa));
+  std::vector<CSC> prolongations;
+  if (use_mg) {
+    int active_level = fine.max_leaf_level();
+    const Mesh* current = &fine;
+    for (;;) {
+      std::unique_ptr<Mesh> coarse;
+      while (current->ncells() > coarse_size && int(S->ops.size()) < max_levels) {
+        std::unique_ptr<Mesh> cand = active_level == 0 ? coarsened_base(*current) : all_leaf_pruned(*current);
+        if (!cand) break;
+        if (cand->ncells() >= current->ncells()) break;
+        active_level = cand->max_leaf_level();
+        coarse = std::move(cand);
+        break;
+      }
+      if (!coarse) break;
+      const std::vector<int> owner = tentative_owners(*current, *coarse);
+      prolongations.push_back(smoothed_transfer(S->ops.back()->A, owner, coarse->ncells()));
+      const FaceRestriction R = face_restriction(*coarse, *current);
+      std::vector<double> ck(coarse->faces.size());
+      const std::vector<double>& fk = S->ops.back()->kappa;
+      for (size_t f = 0; f < ck.size(); ++f) {
+        double s = 0;
+        for (int q = R.ptr[f]; q < R.ptr[f + 1]; ++q) s += R.w[q] * fk[R.idx[q]];
+        ck[f] = s;
+      }
+      Mesh* cm = coarse.release();
+      auto op = make_operator(cm, ck);
+      op->owned_mesh.reset(cm);
+      S->ops.push_back(std::move(op));
+      current = cm;
+    }
+  }
+  const int nlev = int(S->ops.size());
+  S->levels.resize(nlev);
+  for (int l = 0; l < nlev; ++l) {
+    Level& L = S->levels[l];
+    L.op = S->ops[l].get();
+    const CSC& A = L.op->A;
+    const int N = A.n;
+    L.invdiag.assign(N, 0.0);
+    std::vector<double> rowsum(N, 0.0);
+    for (int col = 0; col < N; ++col)
+      for (int p = A.colptr[col]; p < A.colptr[col + 1]; ++p) {
+        rowsum[col] += std::fabs(A.nzval[p]);
+        if (A.rowval[p] == col) L.invdiag[col] = A.nzval[p];
+      }
+    L.omega = jacobi_weight(L.invdiag, rowsum, false, L.op->singular, true, 1.0);
+    if (l + 1 < nlev) L.P = prolongations[l];
+    level_components(L);
+    L.x.assign(N, 0.0);
+    L.b.assign(N, 0.0);
+    L.r.assign(N, 0.0);
+    L.t.assign(N, 0.0);
+  }
+  // coarse factorization, geometric_preconditioner.jl:144-169, 416-430
+  if (!use_mg) return S;
+  Level& C = S->levels.back();
+  const int n = int(C.op->volumes.size());
+  const int k = int(C.comp_volumes.size());
+  S->coarse_augmented = k > 0;
+  S->coarse_n = n + k;
+  S->lu.assign(size_t(n + k) * (n + k), 0.0);
+  const CSC& A = C.op->A;
+  for (int col = 0; col < n; ++col)
+    for (int p = A.colptr[col]; p < A.colptr[col + 1]; ++p)
+      S->lu[size_t(A.rowval[p]) * (n + k) + col] = A.nzval[p];
+  for (int cell = 0; cell < n; ++cell)
+    if (C.comp_index[cell]) {
+      const int comp = n + C.comp_index[cell] - 1;
+      S->lu[size_t(cell) * (n + k) + comp] = C.op->volumes[cell];
+      S->lu[size_t(comp) * (n + k) + cell] = C.op->volumes[cell];
+    }
+  lu_factor(S->lu, S->piv, n + k);
+  // fine factorized action incidence is implicit in factorized_mul
+  return S;
+}
+
+// ---------------------------------------------------------------------------
+// V-cycle, src/solvers/multigrid/cycles.jl:8-77 and hierarchy.jl:208-343
+
+static void project_component_rhs(Level& L, std::vector<double>& b) {
+  const size_t K = L.comp_volumes.size();
+  std::vector<double> s(K, 0.0);
+  for (size_t c = 0; c < b.size(); ++c)
+    if (L.comp_index[c]) s[L.comp_index[c] - 1] += b[c];
+  for (size_t q = 0; q < K; ++q) s[q] /= L.comp_volumes[q];
+  for (size_t c = 0; c < b.size(); ++c)
+    if (L.comp_index[c]) b[c] -= L.op->volumes[c] * s[L.comp_index[c] - 1];
+}
+
+static void remove_component_means(Level& L, std::vector<double>& x) {
+  const size_t K = L.comp_volumes.size();
+  std::vector<double> s(K, 0.0);
+  for (size_t c = 0; c < x.size(); ++c)
+    if (L.comp_index[c]) s[L.comp_index[c] - 1] += L.op->volumes[c] * x[c];
+  for (size_t q = 0; q < K; ++q) s[q] /= L.comp_volumes[q];
+  for (size_t c = 0; c < x.size(); ++c)
+    if (L.comp_index[c]) x[c] -= s[L.comp_index[c] - 1];
+}
+
+static void smooth(Level& L, int sweeps, bool singular) {
+  for (int s = 0; s < sweeps; ++s) {
+    csc_sym_mul(L.op->A, L.x.data(), L.t.data());
+    for (size_t i = 0; i < L.x.size(); ++i) L.x[i] += L.omega * L.invdiag[i] * (L.b[i] - L.t[i]);
+    if (singular) remove_component_means(L, L.x);
+  }
+}
+
+static void vcycle(Solver& S, int l) {
+  Level& L = S.levels[l];
+  const bool singular = !L.comp_volumes.empty();
+  if (singular) project_component_rhs(L, L.b);
+  if (l == int(S.levels.size()) - 1) {
+    const int n = int(L.x.size());
+    std::vector<double> rhs(S.coarse_n, 0.0);
+    std::copy(L.b.begin(), L.b.end(), rhs.begin());
+    lu_solve(S.lu, S.piv, S.coarse_n, rhs.data());
+    std::copy(rhs.begin(), rhs.begin() + n, L.x.begin());
+    return;
+  }
+  smooth(L, S.pre, singular);
+  csc_sym_mul(L.op->A, L.x.data(), L.t.data());
+  for (size_t i = 0; i < L.r.size(); ++i) L.r[i] = L.b[i] - L.t[i];
+  Level& C = S.levels[l + 1];
+  // `_restrict!`: CSC column dots with muladd, plus zero
+  for (int col = 0; col < L.P.n; ++col) {
+    double v = 0.0;
+    for (int p = L.P.colptr[col]; p < L.P.colptr[col + 1]; ++p) v = std::fma(L.P.nzval[p], L.r[L.P.rowval[p]], v);
+    C.b[col] = v + 0.0;
+  }
+  std::fill(C.x.begin(), C.x.end(), 0.0);
+  vcycle(S, l + 1);
+  // `_prolongate!`: rows in increasing column order with muladd
+  std::fill(L.t.begin(), L.t.end(), 0.0);
+  {
+    // build row form once per call (small cost for the oracle)
+    std::vector<int> cnt(L.P.m + 1, 0);
+    for (int p = 0; p < L.P.nnz(); ++p) cnt[L.P.rowval[p] + 1]++;
+    for (int i = 0; i < L.P.m; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int> cols(L.P.nnz()), ent(L.P.nnz()), next(cnt.begin(), cnt.end() - 1);
+    for (int col = 0; col < L.P.n; ++col)
+      for (int p = L.P.colptr[col]; p < L.P.colptr[col + 1]; ++p) {
+        const int row = L.P.rowval[p];
+        cols[next[row]] = col;
+        ent[next[row]] = p;
+        next[row]++;
+      }
+    for (int row = 0; row < L.P.m; ++row) {
+      double v = 0.0;
+      for (int q = cnt[row]; q < cnt[row + 1]; ++q) v = std::fma(L.P.nzval[ent[q]], C.x[cols[q]], v);
+      L.t[row] = v;
+    }
+  }
+  for (size_t i = 0; i < L.x.size(); ++i) L.x[i] += L.t[i];
+  if (singular) remove_component_means(L, L.x);
+  smooth(L, S.post, singular);
+}
+
+void apply_vcycle(Solver& S, const double* r, double* z) {
+  Level& F = S.levels[0];
+  std::copy(r, r + F.b.size(), F.b.begin());
+  std::fill(F.x.begin(), F.x.end(), 0.0);
+  vcycle(S, 0);
+  std::copy(F.x.begin(), F.x.end(), z);
+}
+
+// ---------------------------------------------------------------------------
+// Krylov driver, src/solvers/multigrid/elliptic_solver.jl:369-612 and
+// cycles.jl:80-260. Krylov.jl 0.10 `cg!` (third-party; parity unpinned
+// beyond iteration counts) is restated from its published algorithm.
+
+static double strong_max(const std::vector<double>& r, const std::vector<double>& V) {
+  double m = 0;
+  for (size_t i = 0; i < r.size(); ++i) {
+    if (!(V[i] > 0)) continue;
+    double q = std::fabs(r[i]) / V[i];
+    if (std::isnan(q)) return q;
+    m = std::max(m, q);
+  }
+  return m;
+}
+
+static void residual_norms(const std::vector<double>& r, const std::vector<double>& V, double& l2,
+                           double& linf) {
+  double sq = 0, tv = 0;
+  for (size_t i = 0; i < r.size(); ++i) {
+    if (!(V[i] > 0)) continue;
+    const double s = r[i] / V[i];
+    sq += V[i] * s * s;
+    tv += V[i];
+  }
+  l2 = std::sqrt(sq / tv);
+  linf = strong_max(r, V);
+}
+
+// factorized `_roundoff_residual_floor!`, cycles.jl:108-179
+static double roundoff_floor(const Operator& op, const std::vector<double>& x,
+                             const std::vector<double>& b) {
+  const Mesh& M = *op.mesh;
+  const int F = int(M.faces.size());
+  std::vector<double> fin(F), fout(F);
+  for (int f = 0; f < F; ++f)
+    fin[f] = op.sqrt_kappa[f] * M.faces[f].area * (std::fabs(x[M.faces[f].minus]) + std::fabs(x[M.faces[f].plus]));
+  for (int f = 0; f < F; ++f) {
+    double s = 0;
+    for (int p = op.K.colptr[f]; p < op.K.colptr[f + 1]; ++p) s += std::fabs(op.K.nzval[p]) * fin[op.K.rowval[p]];
+    fout[f] = s;
+  }
+  for (int f = 0; f < F; ++f) fin[f] = 1.0 * M.faces[f].area * op.sqrt_kappa[f] * fout[f];
+  const int N = M.ncells();
+  std::vector<double> bound(N, 0.0);
+  for (int c = 0; c < N; ++c) bound[c] = 0.0 * std::fabs(x[c]);
+  for (int f = 0; f < F; ++f) {
+    bound[M.faces[f].minus] += fin[f];
+    bound[M.faces[f].plus] += fin[f];
+  }
+  double scale = 0;
+  for (int c = 0; c < N; ++c) {
+    const double V = op.volumes[c];
+    if (!(V > 0)) continue;
+    scale = std::max(scale, std::max(bound[c] / V, std::fabs(b[c]) / V));
+  }
+  return 128 * EPS * std::max(1.0, scale);
+}
+
+static void remove_volume_means(const Operator& op, std::vector<double>& x) {
+  const Components& C = op.comp;
+  std::vector<double> s(C.anchors.size(), 0.0);
+  for (size_t c = 0; c < x.size(); ++c)
+    if (C.slots[c]) s[C.slots[c] - 1] += op.volumes[c] * x[c];
+  for (size_t q = 0; q < s.size(); ++q) s[q] /= C.volumes[q];
+  for (size_t c = 0; c < x.size(); ++c) {
+    if (C.labels[c] < 0) x[c] = 0.0;
+    else if (C.slots[c]) x[c] -= s[C.slots[c] - 1];
+  }
+}
+
+static double dotv(const std::vector<double>& a, const std::vector<double>& b) {
+  double s = 0;
+  for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+  return s;
+}
+
+// Krylov.jl cg! with M = V-cycle (ldiv), warm start x0, atol = rtol = 0,
+// itmax, and the strong-residual callback (elliptic_solver.jl:369-378).
+static int krylov_cg(Solver& S, const std::vector<double>& b, std::vector<double>& x0,
+                     double target, int itmax) {
+  Operator& op = *S.ops[0];
+  const std::vector<double>& V = op.volumes;
+  const size_t n = b.size();
+  std::vector<double> x(n, 0.0), r(n), p(n), Ap(n), z(n);
+  factorized_mul(op, x0.data(), r.data());
+  for (size_t i = 0; i < n; ++i) r[i] = b[i] - r[i];
+  auto precond = [&](const std::vector<double>& in, std::vector<double>& out) {
+    if (S.use_mg) apply_vcycle(S, in.data(), out.data());
+    else out = in;
+  };
+  precond(r, z);
+  p = z;
+  double gamma = dotv(r, z);
+  if (gamma < 0) throw std::runtime_error("Preconditioner is not positive definite");
+  double rnorm = std::sqrt(gamma);
+  if (gamma == 0) {
+    for (size_t i = 0; i < n; ++i) x0[i] = x[i] + x0[i];
+    return 0;
+  }
+  int iter = 0;
+  if (itmax == 0) itmax = 2 * int(n);
+  double pnorm2 = gamma;
+  const double eps_tol = 0.0;
+  bool solved = rnorm <= eps_tol, tired = iter >= itmax, zero_curv = false, user_exit = false;
+  while (!(solved || tired || zero_curv || user_exit)) {
+    factorized_mul(op, p.data(), Ap.data());
+    const double pAp = dotv(p, Ap);
+    if (pAp <= EPS * pnorm2) {
+      if (std::fabs(pAp) <= EPS * pnorm2) zero_curv = true;
+    }
+    if (zero_curv) continue;
+    const double alpha = gamma / pAp;
+    for (size_t i = 0; i < n; ++i) x[i] += alpha * p[i];
+    for (size_t i = 0; i < n; ++i) r[i] += -alpha * Ap[i];
+    precond(r, z);
+    const double gamma_next = dotv(r, z);
+    rnorm = std::sqrt(gamma_next);
+    const bool mach = rnorm + 1.0 <= 1.0;
+    solved = rnorm <= eps_tol || mach;
+    if (!solved) {
+      const double beta = gamma_next / gamma;
+      pnorm2 = gamma_next + beta * beta * pnorm2;
+      gamma = gamma_next;
+      for (size_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    }
+    ++iter;
+    tired = iter >= itmax;
+    user_exit = strong_max(r, V) <= target;
+  }
+  for (size_t i = 0; i < n; ++i) x0[i] = x[i] + x0[i];
+  return iter;
+}
+
+static double guarded_target(const Operator& op, const std::vector<double>& x,
+                             const std::vector<double>& b, double target, double initial,
+                             double current) {
+  if (!(std::isfinite(initial) && std::isfinite(current) && target < current && current < initial))
+    return target;
+  const double fl = roundoff_floor(op, x, b);
+  return (std::isfinite(fl) && fl < initial / 100) ? std::max(target, fl) : target;
+}
+
+// `solve!` (elliptic_solver.jl:628-675) with `_solve_krylov_with_stats!`
+// (:550-612) and `_run_krylov_reliably!` (:487-526). `rhs` is the strong
+// right-hand side; `x` holds the initial guess and receives the solution.
+Stats solve(Solver& S, const double* rhs, double* xio, double rtol, double atol, int max_cycles,
+            int compat_error, std::vector<double>* removed_means) {
+  Operator& op = *S.ops[0];
+  const std::vector<double>& V = op.volumes;
+  const size_t n = V.size();
+  std::vector<double> x(xio, xio + n), b(rhs, rhs + n);
+  for (size_t i = 0; i < n; ++i) {
+    if (V[i] == 0.0) {
+      b[i] = 0;
+      x[i] = 0;
+    } else {
+      b[i] *= V[i];
+    }
+  }
+  Stats st{};
+  double removed = 0;
+  if (op.singular) {
+    const Components& C = op.comp;
+    const size_t K = C.anchors.size();
+    std::vector<double> sums(K, 0.0), scales(K, 0.0);
+    for (size_t c = 0; c < n; ++c)
+      if (C.slots[c]) {
+        sums[C.slots[c] - 1] += b[c];
+        scales[C.slots[c] - 1] += std::fabs(b[c]);
+      }
+    double total_sum = 0, total_vol = 0;
+    for (size_t q = 0; q < K; ++q) {
+      const double tol = 256 * EPS * std::max(1.0, scales[q]);
+      if (compat_error && std::fabs(sums[q]) > tol)
+        throw std::invalid_argument("Poisson right-hand side has nonzero volume mean");
+      total_sum += sums[q];
+      total_vol += C.volumes[q];
+    }
+    if (removed_means) removed_means->clear();
+    for (size_t q = 0; q < K; ++q) {
+      sums[q] /= C.volumes[q];
+      if (removed_means) removed_means->push_back(sums[q]);
+    }
+    for (size_t c = 0; c < n; ++c)
+      if (C.slots[c]) b[c] -= sums[C.slots[c] - 1] * V[c];
+    removed = total_vol == 0 ? 0 : total_sum / total_vol;
+    remove_volume_means(op, x);
+  }
+  std::vector<double> prod(n), res(n);
+  auto true_residual = [&]() {
+    factorized_mul(op, x.data(), prod.data());
+    for (size_t i = 0; i < n; ++i) res[i] = b[i] - prod[i];
+  };
+  true_residual();
+  double il2, ilinf;
+  residual_norms(res, V, il2, ilinf);
+  const double floor0 = roundoff_floor(op, x, b);
+  double target = std::max(atol, std::max(rtol * std::max(ilinf, EPS), floor0));
+  int iterations = 0;
+  if (ilinf > target) {
+    int total = 0;
+    double final_linf = std::numeric_limits<double>::infinity();
+    while (total < max_cycles) {
+      const int it = krylov_cg(S, b, x, target, max_cycles - total);
+      total += it;
+      if (op.singular) remove_volume_means(op, x);
+      true_residual();
+      final_linf = strong_max(res, V);
+      const double eff = guarded_target(op, x, b, target, ilinf, final_linf);
+      if (final_linf <= eff) break;
+      if (!std::isfinite(final_linf) || it == 0) break;
+    }
+    iterations = total;
+  }
+  if (op.singular) remove_volume_means(op, x);
+  true_residual();
+  double fl2, flinf;
+  residual_norms(res, V, fl2, flinf);
+  target = guarded_target(op, x, b, target, ilinf, flinf);
+  std::copy(x.begin(), x.end(), xio);
+  st.cycles = iterations;
+  st.converged = std::isfinite(flinf) && std::isfinite(target) && flinf <= target;
+  st.compatible = 1;
+  st.initial_l2 = il2;
+  st.initial_linf = ilinf;
+  st.final_l2 = fl2;
+  st.final_linf = flinf;
+  st.target_linf = target;
+  st.residual_scale = std::max(ilinf, EPS);
+  st.removed_rhs_mean = removed;
+  return st;
+}
+
+// ---------------------------------------------------------------------------
+// Two-phase coefficients and jumps, src/two_phase/forces.jl
+
+// `_staggered_liquid_fraction` + `_mixture_density!` + `inv.(rho)`,
+// forces.jl:23-36,69-84,131
+void mixture_coefficients(const Mesh& M, const double* c, double rho_l, double rho_g, double* kappa) {
+  if (!(std::isfinite(rho_l) && rho_l > 0 && std::isfinite(rho_g) && rho_g > 0))
+    throw std::invalid_argument("phase densities must be finite and positive");
+  const int nb = M.bs[0] * M.bs[1];
+  for (size_t f = 0; f < M.faces.size(); ++f) {
+    const Face& fa = M.faces[f];
+    const double hm = M.blocks[fa.minus / nb].spacing[fa.axis];
+    const double hp = M.blocks[fa.plus / nb].spacing[fa.axis];
+    double cf = (hm * c[fa.minus] + hp * c[fa.plus]) / (hm + hp);
+    cf = std::min(std::max(cf, 0.0), 1.0);
+    const double rho = rho_l * cf + rho_g * (1 - cf);
+    kappa[f] = 1.0 / rho;
+  }
+}
+
+// `_surface_tension_jumps!` with CGFM (forces.jl:312-314,345-378) and given
+// face curvatures (NaN marks an unresolved face curvature).
+void cgfm_jumps(const Mesh& M, const double* c, const double* curvature, double sigma, int csf,
+                double* jumps) {
+  if (!(std::isfinite(sigma) && sigma >= 0)) throw std::invalid_argument("sigma must be finite and nonnegative");
+  for (size_t f = 0; f < M.faces.size(); ++f) {
+    const Face& fa = M.faces[f];
+    const double cm = c[fa.minus], cp = c[fa.plus];
+    const double jump = csf ? cp - cm : double(cp > 0.5) - double(cm > 0.5);
+    if (sigma == 0 || jump == 0) {
+      jumps[f] = 0;
+      continue;
+    }
+    const double k = curvature[f];
+    if (std::isnan(k)) {
+      jumps[f] = 0;
+      continue;
+    }
+    jumps[f] = sigma * k * jump * fa.area;
+  }
+}
+
+static void mul_metric(const Operator& op, const std::vector<double>& in, std::vector<double>& out) {
+  const int F = int(in.size());
+  out.assign(F, 0.0);
+  if (op.K_diagonal) {
+    for (int f = 0; f < F; ++f) out[f] = op.K.nzval[op.K.colptr[f]] * in[f];
+  } else {
+    for (int f = 0; f < F; ++f) {
+      double s = 0;
+      for (int p = op.K.colptr[f]; p < op.K.colptr[f + 1]; ++p) s += op.K.nzval[p] * in[op.K.rowval[p]];
+      out[f] = s;
+    }
+  }
+}
+
+// `_weighted_jump_force!` (forces.jl:179-209, no geometry)
+void weighted_jump_force(const Operator& op, const double* jumps, double* accel) {
+  const int F = int(op.kappa.size());
+  std::vector<double> in(F), out;
+  for (int f = 0; f < F; ++f) in[f] = op.sqrt_kappa[f] * jumps[f];
+  mul_metric(op, in, out);
+  for (int f = 0; f < F; ++f) accel[f] = op.sqrt_kappa[f] * out[f];
+}
+
+// `weighted_gradient!` (elliptic.jl:1272-1297): interior faces; boundary
+// faces of wall/periodic operators are zero.
+void weighted_gradient(const Operator& op, const double* p, double* g) {
+  const Mesh& M = *op.mesh;
+  const int F = int(M.faces.size());
+  std::vector<double> in(F), out;
+  for (int f = 0; f < F; ++f)
+    in[f] = op.sqrt_kappa[f] * (M.faces[f].area * (p[M.faces[f].plus] - p[M.faces[f].minus]));
+  mul_metric(op, in, out);
+  for (int f = 0; f < F; ++f) g[f] = op.sqrt_kappa[f] * out[f];
+}
+
+// `divergence!`, operators/advection/cell_advection.jl:205-220
+void divergence(const Mesh& M, const double* u, const double* ub, double* div) {
+  const int nb = M.bs[0] * M.bs[1];
+  for (int c = 0; c < M.ncells(); ++c) div[c] = 0.0;
+  for (size_t f = 0; f < M.faces.size(); ++f) {
+    const Face& fa = M.faces[f];
+    const double flux = u[f] * fa.area;
+    div[fa.minus] += flux / M.blocks[fa.minus / nb].volume;
+    div[fa.plus] += -flux / M.blocks[fa.plus / nb].volume;
+  }
+  if (ub)
+    for (size_t f = 0; f < M.bfaces.size(); ++f) {
+      const BFace& fa = M.bfaces[f];
+      const double flux = fa.side * ub[f] * fa.area;
+      div[fa.cell] += flux / M.blocks[fa.cell / nb].volume;
+    }
+}
+
+// `_project_prepared!` (navier_stokes/projection.jl:221-260) for a wall or
+// periodic domain with one component: div -> rhs = -div/dt -> solve with
+// rtol 0 and atol = tol/dt^2 -> u -= dt G phi -> p += phi.
+Stats project(Solver& S, double* u, double* p, double dt, double tol, int max_cycles,
+              double* phi_out) {
+  Operator& op = *S.ops[0];
+  const Mesh& M = *op.mesh;
+  const int N = M.ncells(), F = int(M.faces.size());
+  std::vector<double> div(N), phi(N, 0.0), g(F);
+  divergence(M, u, nullptr, div.data());
+  for (int c = 0; c < N; ++c) div[c] = -div[c] / dt;
+  Stats st = solve(S, div.data(), phi.data(), 0.0, tol / (dt * dt), max_cycles, 0, nullptr);
+  st.residual_scale = 1.0 / (dt * dt);
+  weighted_gradient(op, phi.data(), g.data());
+  const double scale = dt * 1.0;
+  for (int f = 0; f < F; ++f) u[f] -= scale * g[f];
+  for (int c = 0; c < N; ++c) p[c] += phi[c];
+  if (phi_out) std::copy(phi.begin(), phi.end(), phi_out);
+  return st;
+}
+
+// ---------------------------------------------------------------------------
+// Pressure adjoint (no reference counterpart; pinned by finite differences).
+// Forward: p = solve(b, beta) with zero volume mean. J = w' p.
+// dJ/db_i = V_i lambda_i where A lambda = Q w (Q w = w - V (1'w)/(1'V)),
+// lambda with zero volume mean; dJ/dbeta_f = -[y_l,f (K y_p)_f +
+// y_p,f (K y_l)_f] / (2 beta_f), y_x = sqrt(beta) |f| (x+ - x-).
+Stats adjoint(Solver& S, const double* w, const double* p, double rtol, int max_cycles,
+              double* dJ_db, double* dJ_dbeta, double* lambda_out) {
+  Operator& op = *S.ops[0];
+  const Mesh& M = *op.mesh;
+  const int N = M.ncells(), F = int(M.faces.size());
+  // solve() expects a strong RHS: pass w / V so that V * rhs = w
+  std::vector<double> rhs(N), lam(N, 0.0);
+  for (int c = 0; c < N; ++c) rhs[c] = w[c] / op.volumes[c];
+  Stats st = solve(S, rhs.data(), lam.data(), rtol, 0.0, max_cycles, 0, nullptr);
+  for (int c = 0; c < N; ++c) dJ_db[c] = op.volumes[c] * lam[c];
+  std::vector<double> yl(F), yp(F), Kyl, Kyp;
+  for (int f = 0; f < F; ++f) {
+    const Face& fa = M.faces[f];
+    yl[f] = op.sqrt_kappa[f] * fa.area * (lam[fa.plus] - lam[fa.minus]);
+    yp[f] = op.sqrt_kappa[f] * fa.area * (p[fa.plus] - p[fa.minus]);
+  }
+  mul_metric(op, yl, Kyl);
+  mul_metric(op, yp, Kyp);
+  for (int f = 0; f < F; ++f)
+    dJ_dbeta[f] = op.kappa[f] > 0 ? -(yl[f] * Kyp[f] + yp[f] * Kyl[f]) / (2 * op.kappa[f]) : 0.0;
+  if (lambda_out) std::copy(lam.begin(), lam.end(), lambda_out);
+  return st;
+}
+
+void apply_fine(Solver& S, const double* x, double* y) { factorized_mul(*S.ops[0], x, y); }
+void apply_level(Solver& S, int l, const double* x, double* y) { csc_sym_mul(S.levels[l].op->A, x, y); }
+
+}  // namespace orc
