@@ -1,0 +1,145 @@
+/* tpamr.h — C ABI of the MI355X (gfx950) pressure-projection solver for
+ * TwoPhaseAMR.jl's tree geometric multigrid.
+ *
+ * Every pointer argument is caller-owned memory; the library copies inputs
+ * and never retains a caller pointer. Host-pointer entry points accept plain
+ * host arrays (Julia `Vector{Float64}` under GC.@preserve, numpy arrays).
+ * The `_device` entry points take device pointers valid on the context's
+ * device and run on the context's stream.
+ *
+ * Cells are numbered in the reference's order (block-major, then j, then i;
+ * src/mesh/topology/mesh.jl:28-30); interior and boundary faces in the
+ * reference's face-schedule order (mesh.jl:874-927). All return codes are 0
+ * on success and negative on failure; tpamr_last_error gives the message.
+ * A context is single-threaded and not re-entrant, like the reference's
+ * FactorizedEllipticOperator (src/operators/elliptic.jl:1004-1007).
+ */
+#ifndef TPAMR_H
+#define TPAMR_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tpamr_ctx tpamr_ctx;
+
+enum {
+  TPAMR_OK = 0,
+  TPAMR_EINVAL = -1,       /* invalid argument (ArgumentError) */
+  TPAMR_ESIZE = -2,        /* size mismatch (DimensionMismatch) */
+  TPAMR_ENONFINITE = -3,   /* non-finite input */
+  TPAMR_EDEVICE = -4,      /* HIP runtime or no device */
+  TPAMR_ESTATE = -5,       /* call order (no mesh / no coefficients) */
+  TPAMR_EINCOMPATIBLE = -6 /* compatibility = error and an incompatible RHS */
+};
+
+/* precision of the multigrid V-cycle; the outer Krylov loop is always FP64 */
+enum { TPAMR_FP64 = 0, TPAMR_FP32_VCYCLE = 1 };
+
+/* LinearSolverStats, src/solvers/multigrid/stats.jl:14-36 */
+typedef struct {
+  int cycles, converged, compatible, reason;
+  double initial_l2, initial_linf, final_l2, final_linf, target_linf, residual_scale, removed_rhs_mean;
+  double solve_ms; /* device time of the solve, HIP events on the context stream */
+} tpamr_stats;
+
+/* MultigridOptions, src/solvers/multigrid/options.jl:61-95 */
+typedef struct {
+  int coarse_size, pre_sweeps, post_sweeps, max_levels;
+} tpamr_multigrid_options;
+
+/* Create a context on HIP device `device` (-1: host-only context that builds
+ * and inspects hierarchies but cannot solve). Replaces the construction of
+ * TreeGeometricMultigridPreconditioner + EllipticSolver
+ * (src/solvers/multigrid/elliptic_solver.jl:282-302). */
+int tpamr_create(tpamr_ctx** out, int device, int precision);
+void tpamr_destroy(tpamr_ctx* ctx);
+const char* tpamr_last_error(const tpamr_ctx* ctx);
+
+/* Block-quadtree mesh: BlockTreeMesh(base_mesh, block_size, origin, extent,
+ * periodic, max_level, refine) (mesh.jl:1059-1121). `refined` lists
+ * n_refined (level, i, j) triples of nodes the refine predicate split; the
+ * library 2:1-balances the tree (mesh.jl:673-714). Passing the ancestors of
+ * every leaf of an existing (balanced) BlockTreeMesh reproduces it. */
+int tpamr_set_mesh(tpamr_ctx* ctx, const int base_mesh[2], const int block_size[2], const double origin[2],
+                   const double extent[2], const int periodic[2], int max_level, int n_refined,
+                   const int* refined);
+/* sizes[5] = nblocks, ncells, nfaces, nboundary_faces, max leaf level */
+int tpamr_mesh_sizes(const tpamr_ctx* ctx, int* sizes);
+/* leaf keys (level,i,j) x nblocks in the reference block order */
+int tpamr_mesh_blocks(const tpamr_ctx* ctx, int* keys);
+/* interior faces: minus, plus (0-based cells), axis (0 = x), area */
+int tpamr_mesh_faces(const tpamr_ctx* ctx, int* minus, int* plus, int* axis, double* area);
+/* boundary faces: cell, axis, side (+1/-1), area */
+int tpamr_mesh_boundary_faces(const tpamr_ctx* ctx, int* cell, int* axis, int* side, double* area);
+
+/* MultigridOptions (defaults 64, 2, 2, 32). Takes effect at the next
+ * tpamr_set_face_coefficients. */
+int tpamr_set_multigrid(tpamr_ctx* ctx, const tpamr_multigrid_options* options);
+
+/* Face coefficients kappa_f (one per interior face, reference order):
+ * builds the hierarchy on first use, otherwise refreshes every coefficient-
+ * dependent value (replaces _refresh_geometric_multigrid!,
+ * src/solvers/multigrid/geometric_preconditioner.jl:483-594). */
+int tpamr_set_face_coefficients(tpamr_ctx* ctx, const double* kappa);
+
+/* hierarchy inspection */
+int tpamr_num_levels(const tpamr_ctx* ctx);
+/* ints[6] = ncells, nfaces, nblocks, irregular rows, nnz(A), max leaf level; dbl[2] = omega, omega_P */
+int tpamr_level_info(const tpamr_ctx* ctx, int level, int* ints, double* dbl);
+/* CSR of the level operator A (integrated, symmetric): pass NULL arrays to query nnz in *nnz */
+int tpamr_level_matrix(const tpamr_ctx* ctx, int level, int* nnz, int* rowptr, int* col, double* val);
+/* per fine block of `level`: coarse block and quadrant (-1: same block) of the tentative transfer */
+int tpamr_level_transfer(const tpamr_ctx* ctx, int level, int* xfer);
+int tpamr_level_kappa(const tpamr_ctx* ctx, int level, double* kappa);
+
+/* solve!(solution, solver, rhs; rtol, atol, max_cycles, compatibility)
+ * (src/solvers/multigrid/elliptic_solver.jl:628-675): strong rhs in, x holds
+ * the initial guess and receives the zero-volume-mean solution.
+ * compatibility: 0 = :project, 1 = :error. */
+int tpamr_solve(tpamr_ctx* ctx, const double* rhs, double* x, double rtol, double atol, int max_cycles,
+                int compatibility, tpamr_stats* stats);
+int tpamr_solve_device(tpamr_ctx* ctx, const double* d_rhs, double* d_x, double rtol, double atol, int max_cycles,
+                       int compatibility, tpamr_stats* stats);
+
+/* ldiv!(z, P, r) of the V-cycle preconditioner (cycles.jl:262-283), integrated r */
+int tpamr_vcycle(tpamr_ctx* ctx, const double* r, double* z);
+/* integrated operator y = A x (FactorizedEllipticOperator mul!, elliptic.jl:1125-1165) */
+int tpamr_apply(tpamr_ctx* ctx, const double* x, double* y);
+int tpamr_apply_device(tpamr_ctx* ctx, const double* d_x, double* d_y);
+
+/* two-phase coefficients, src/two_phase/forces.jl */
+/* inv.(mixture_density) with the staggered liquid fraction (forces.jl:23-36,69-84,131) */
+int tpamr_mixture_face_coefficients(tpamr_ctx* ctx, const double* fraction, double rho_liquid, double rho_gas,
+                                    double* kappa);
+/* _surface_tension_jumps! (forces.jl:345-378): formulation 0 = CGFM, 1 = CSF;
+ * face_curvature per face (NaN = unresolved) */
+int tpamr_surface_tension_jumps(tpamr_ctx* ctx, const double* fraction, const double* face_curvature, double sigma,
+                                int formulation, double* jumps);
+/* _weighted_jump_force! (forces.jl:179-209): accel = sqrt(k) K (sqrt(k) jumps) */
+int tpamr_weighted_jump_force(tpamr_ctx* ctx, const double* jumps, double* accel);
+
+/* divergence! (operators/advection/cell_advection.jl:205-220); u_boundary may be NULL */
+int tpamr_divergence(tpamr_ctx* ctx, const double* u, const double* u_boundary, double* div);
+/* weighted_gradient! interior faces (elliptic.jl:1272-1297) */
+int tpamr_weighted_gradient(tpamr_ctx* ctx, const double* p, double* g);
+/* _project_prepared! (navier_stokes/projection.jl:221-260) on a wall/periodic
+ * domain: updates u (interior faces) and p, writes the increment phi */
+int tpamr_project(tpamr_ctx* ctx, double* u, double* p, double dt, double tolerance, int max_cycles, double* phi,
+                  tpamr_stats* stats);
+
+/* Pressure adjoint (no reference counterpart): for J = w' p with p the
+ * zero-mean solution of the current operator, dJ/drhs (strong rhs) and
+ * dJ/dkappa per interior face; `p` is the stored forward solution. */
+int tpamr_adjoint(tpamr_ctx* ctx, const double* w, const double* p, double rtol, int max_cycles, double* dJ_drhs,
+                  double* dJ_dkappa, tpamr_stats* stats);
+
+/* Timing of the finest-level smoother kernel (HIP events, `reps` launches on
+ * the context stream); ms per launch in *avg_ms and its algorithmic bytes
+ * per launch in *bytes. */
+int tpamr_time_smoother(tpamr_ctx* ctx, int reps, double* avg_ms, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -90,9 +90,10 @@ def build_refined_keys(base_mesh, block_size, origin, extent, max_level, refine)
 
 
 class OMesh:
-    def __init__(self, base_mesh, block_size=(16, 16), origin=(0.0, 0.0), extent=(1.0, 1.0),
+    def __init__(self, base_mesh=None, block_size=(16, 16), origin=(0.0, 0.0), extent=(1.0, 1.0),
                  periodic=(True, True), max_level=0, refine=None, refined_keys=None):
         L = lib()
+        base_mesh = tuple(block_size) if base_mesh is None else tuple(base_mesh)
         if refined_keys is None:
             refined_keys = [] if refine is None else build_refined_keys(
                 base_mesh, block_size, origin, extent, max_level, refine)

@@ -1,0 +1,250 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU FP64 oracle.
+
+Tolerances: operator applications and coefficient kernels agree to round-off
+(1e-13 relative); an FP64 V-cycle to 1e-11 (its component-mean removal is
+applied once at the end of the cycle instead of after every sweep, which only
+moves round-off); FP64 solves to 1e-10 relative with identical iteration
+counts; FP32-V-cycle solves reach the same FP64 residual target and their
+solution is compared with the FP64 oracle solution.
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from cases import MESHES, DROP_CENTRE, DROP_RADIUS, disk_fraction, techref_noise
+
+pytestmark = pytest.mark.gpu
+
+import tpamr  # noqa: E402
+from oracle import OMesh, OSolver  # noqa: E402
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+_CACHE = {}
+
+
+def meshes(name):
+    if name not in _CACHE:
+        kw = MESHES[name]
+        _CACHE[name] = (OMesh(**kw), tpamr.BlockTreeMesh(**kw))
+    return _CACHE[name]
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def kappa_for(pm, kind):
+    if kind == "one":
+        return np.ones(pm.nfaces)
+    if kind == "smooth":
+        return np.exp(0.4 * np.sin(np.arange(1, pm.nfaces + 1)))
+    if kind == "drop":
+        x, y = pm.cell_centers()
+        dx, dy = pm.cell_spacing()
+        c = disk_fraction(x, y, dx, dy, DROP_CENTRE, DROP_RADIUS)
+        om, _ = None, None
+        return c
+    raise ValueError(kind)
+
+
+CASES = [("uniform64", "one"), ("amr32", "one"), ("amr_mixed", "smooth"), ("pruned_tree", "smooth"),
+         ("periodic_tg", "one"), ("circle16", "smooth")]
+
+
+@pytest.fixture(scope="module", params=["fp64", "fp32_vcycle"])
+def precision(request):
+    return request.param
+
+
+@pytest.mark.parametrize("name,kind", CASES)
+def test_apply_matches_oracle(name, kind):
+    _print("case apply", name, kind)
+    om, pm = meshes(name)
+    k = kappa_for(pm, kind)
+    os_ = OSolver(om, k)
+    gs = tpamr.EllipticSolver(tpamr.PoissonOperator(pm, k))
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(pm.ncells)
+    assert rel(gs.apply(x), os_.apply(x)) < 1e-13
+    gs.close()
+
+
+@pytest.mark.parametrize("name,kind", CASES)
+def test_vcycle_matches_oracle(name, kind, precision):
+    _print("case vcycle", name, kind, precision)
+    om, pm = meshes(name)
+    k = kappa_for(pm, kind)
+    os_ = OSolver(om, k)
+    gs = tpamr.EllipticSolver(tpamr.PoissonOperator(pm, k), precision=precision)
+    rng = np.random.default_rng(11)
+    r = rng.standard_normal(pm.ncells)
+    r -= pm.volumes * r.sum() / pm.volumes.sum()
+    zo = os_.vcycle(r)
+    zg = gs.ldiv(r)
+    tol = 1e-11 if precision == "fp64" else 2e-5
+    assert rel(zg, zo) < tol
+    # the V-cycle output has zero volume-weighted mean
+    assert abs(np.dot(pm.volumes, zg)) <= 1e-12 * np.sum(pm.volumes * np.abs(zg))
+    gs.close()
+
+
+@pytest.mark.parametrize("name,kind", CASES)
+def test_solve_matches_oracle(name, kind, precision):
+    _print("case solve", name, kind, precision)
+    om, pm = meshes(name)
+    k = kappa_for(pm, kind)
+    os_ = OSolver(om, k)
+    gs = tpamr.EllipticSolver(tpamr.PoissonOperator(pm, k), precision=precision)
+    b = techref_noise(pm.ncells)
+    xo, so = os_.solve(b, rtol=1e-10, max_cycles=200)
+    xg = np.zeros(pm.ncells)
+    sg = gs.solve(xg, b, rtol=1e-10, max_cycles=200)
+    assert sg.converged and so["converged"]
+    if precision == "fp64":
+        assert sg.cycles == so["cycles"]
+        assert rel(xg, xo) < 1e-10
+        assert abs(sg.initial_linf - so["initial_linf"]) <= 1e-12 * so["initial_linf"]
+    else:
+        assert abs(sg.cycles - so["cycles"]) <= 2
+        # both stop at the same strong-residual target; the solutions then
+        # differ by at most that residual through the operator's inverse
+        assert rel(xg, xo) < 1e-7
+    assert sg.final_linf <= sg.target_linf
+    gs.close()
+
+
+TECHREF = [("uniform64", 4096, 4, 10), ("uniform128", 16384, 5, 10), ("amr32", 7552, 5, 10), ("amr64", 19648, 6, 11)]
+
+
+@pytest.mark.parametrize("name,unknowns,levels,cycles", TECHREF)
+def test_published_iteration_counts_on_gpu(name, unknowns, levels, cycles):
+    """docs/techref/figures/generated/checks.tex:200-229 of the reference."""
+    _print("case techref", name)
+    _, pm = meshes(name)
+    gs = tpamr.EllipticSolver(tpamr.PoissonOperator(pm, 1.0))
+    assert pm.ncells == unknowns and len(gs.levels) == levels
+    x = np.zeros(pm.ncells)
+    st = gs.solve(x, techref_noise(pm.ncells), rtol=1e-10, max_cycles=20000)
+    assert st.converged and st.cycles == cycles
+    factor = (st.final_l2 / st.initial_l2) ** (1 / st.cycles)
+    assert round(factor, 2) in (0.08, 0.09)
+    gs.close()
+
+
+@pytest.mark.parametrize("name", ["uniform128", "amr64"])
+def test_drop_density_ratio_counts_on_gpu(name):
+    """drop 1000:1 rows of checks.tex:230-241: 15 and 16 PCG iterations."""
+    _print("case drop", name)
+    om, pm = meshes(name)
+    x, y = pm.cell_centers()
+    dx, dy = pm.cell_spacing()
+    c = disk_fraction(x, y, dx, dy, DROP_CENTRE, DROP_RADIUS)
+    gs = tpamr.EllipticSolver(tpamr.PoissonOperator(pm, 1.0))
+    k = tpamr.mixture_face_coefficients(gs, c, (1000.0, 1.0))
+    assert rel(k, om.mixture_coefficients(c, 1000.0, 1.0)) < 4e-16
+    tpamr.refresh_two_phase(gs, c, (1000.0, 1.0))
+    xs = np.zeros(pm.ncells)
+    st = gs.solve(xs, techref_noise(pm.ncells), rtol=1e-10, max_cycles=20000)
+    assert st.converged and st.cycles == {"uniform128": 15, "amr64": 16}[name]
+    gs.close()
+
+
+def test_two_phase_face_kernels_match_oracle():
+    _print("case two-phase kernels")
+    om, pm = meshes("amr64")
+    x, y = pm.cell_centers()
+    dx, dy = pm.cell_spacing()
+    c = disk_fraction(x, y, dx, dy, DROP_CENTRE, DROP_RADIUS)
+    gs = tpamr.EllipticSolver(tpamr.PoissonOperator(pm, 1.0))
+    k = tpamr.mixture_face_coefficients(gs, c, (1000.0, 1.0))
+    tpamr.refresh_two_phase(gs, c, (1000.0, 1.0))
+    os_ = OSolver(om, k)
+    curv = np.full(pm.nfaces, 5.0)
+    curv[::97] = np.nan
+    jg = tpamr.surface_tension_jumps(gs, c, curv, sigma=0.072)
+    jo = om.cgfm_jumps(c, curv, 0.072)
+    assert np.array_equal(jg, jo)
+    jc = tpamr.surface_tension_jumps(gs, c, curv, sigma=0.072, formulation=tpamr.CSF)
+    assert np.array_equal(jc, om.cgfm_jumps(c, curv, 0.072, csf=True))
+    assert rel(tpamr.weighted_jump_force(gs, jg), os_.weighted_jump_force(jo)) < 1e-13
+    p = np.sin(3 * x) * np.cos(2 * y)
+    assert rel(tpamr.weighted_gradient(gs, p), os_.weighted_gradient(p)) < 1e-13
+    u = np.cos(np.arange(pm.nfaces) * 0.37)
+    assert rel(tpamr.divergence(gs, u), om.divergence(u)) < 1e-13
+    gs.close()
+
+
+def test_projection_matches_oracle():
+    _print("case projection")
+    om, pm = meshes("amr_mixed")
+    k = kappa_for(pm, "smooth")
+    os_ = OSolver(om, k)
+    gs = tpamr.EllipticSolver(tpamr.PoissonOperator(pm, k))
+    u0 = np.cos(np.arange(pm.nfaces) * 0.61)
+    p0 = np.zeros(pm.ncells)
+    dt = 1e-3
+    uo, po, phio, so = os_.project(u0, p0, dt, tolerance=1e-12, max_cycles=200)
+    ug, pg = u0.copy(), p0.copy()
+    sg, phig = tpamr.project(ug, pg, gs, dt, tolerance=1e-12, max_cycles=200)
+    assert sg.converged and sg.cycles == so["cycles"]
+    assert rel(phig, phio) < 1e-10
+    assert rel(ug, uo) < 1e-10
+    assert rel(pg, po) < 1e-10
+    gs.close()
+
+
+def test_adjoint_matches_oracle_and_finite_differences():
+    _print("case adjoint")
+    om, pm = meshes("circle16")
+    k = kappa_for(pm, "smooth")
+    os_ = OSolver(om, k)
+    gs = tpamr.EllipticSolver(tpamr.PoissonOperator(pm, k))
+    rng = np.random.default_rng(3)
+    b = techref_noise(pm.ncells)
+    w = rng.standard_normal(pm.ncells)
+    p, pull = tpamr.pressure_solve_rrule(gs, b, rtol=1e-13)
+    db, dk = pull(w)
+    po, _ = os_.solve(b, rtol=1e-13, max_cycles=200)
+    dbo, dko, _, _ = os_.adjoint(w, po, rtol=1e-13)
+    assert rel(db, dbo) < 1e-9 and rel(dk, dko) < 1e-9
+    # central differences along random directions (J = w . p)
+    eps = 1e-6
+    for q in range(3):
+        d = rng.standard_normal(pm.ncells)
+        jp = w @ tpamr.pressure_solve(gs, b + eps * d, rtol=1e-13)[0]
+        jm = w @ tpamr.pressure_solve(gs, b - eps * d, rtol=1e-13)[0]
+        fd = (jp - jm) / (2 * eps)
+        assert abs(fd - db @ d) <= 1e-6 * max(abs(fd), 1e-30)
+    for q in range(2):
+        dkap = rng.standard_normal(pm.nfaces) * k
+        vals = []
+        for sgn in (1, -1):
+            gs.operator.replace_face_coefficient(k + sgn * eps * dkap)
+            vals.append(w @ tpamr.pressure_solve(gs, b, rtol=1e-13)[0])
+        gs.operator.replace_face_coefficient(k)
+        fd = (vals[0] - vals[1]) / (2 * eps)
+        assert abs(fd - dk @ dkap) <= 1e-6 * max(abs(fd), 1e-30)
+    gs.close()
+
+
+def test_errors_are_reported():
+    _print("case errors")
+    _, pm = meshes("uniform64")
+    gs = tpamr.EllipticSolver(tpamr.PoissonOperator(pm, 1.0))
+    x = np.zeros(pm.ncells)
+    with pytest.raises(ValueError):
+        gs.solve(x, np.full(pm.ncells, np.nan))
+    with pytest.raises(ValueError):
+        gs.solve(x, np.ones(pm.ncells), compatibility="error")
+    with pytest.raises(tpamr.DimensionMismatch):
+        gs.solve(x, np.ones(pm.ncells - 1))
+    st = gs.solve(x, techref_noise(pm.ncells), max_cycles=0)
+    assert not st.converged and st.cycles == 0
+    gs.close()

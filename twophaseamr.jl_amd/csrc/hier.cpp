@@ -1,0 +1,859 @@
+// Host-side topology and hierarchy construction (see hier.hpp).
+//
+// Reference behaviour followed here (TwoPhaseAMR.jl):
+//   tree, 2:1 balance, leaf order, faces   src/mesh/topology/mesh.jl:673-927, 1059-1121
+//   inverse face metric K                  src/operators/gradient.jl:107-408
+//   integrated stiffness J' sqrt(k) K sqrt(k) J   src/operators/elliptic.jl:175-257
+//   Jacobi weights                         src/solvers/multigrid/hierarchy.jl:8-30
+//   coarse meshes, P_t, kappa restriction  src/solvers/multigrid/tree_coarsening.jl:30-82, 187-252, 347-398
+//   smoothed transfer (applied matrix-free on the device)  hierarchy.jl:631-721
+//   augmented coarse system                geometric_preconditioner.jl:144-169
+#include "hier.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <stdexcept>
+#include <tuple>
+
+namespace tpamr {
+
+namespace {
+
+inline long wrapl(long x, long p) { return ((x % p) + p) % p; }
+
+void collect_leaves(const TreeMesh& M, std::vector<BlockKey>& leaves) {
+  leaves.clear();
+  std::vector<BlockKey> stack;
+  for (int j = 0; j < M.layout[1]; ++j)
+    for (int i = 0; i < M.layout[0]; ++i) stack.push_back({0, i, j});
+  while (!stack.empty()) {
+    BlockKey k = stack.back();
+    stack.pop_back();
+    if (M.internal.count(key64(k.level, k.i, k.j))) {
+      for (int c = 0; c < 4; ++c) stack.push_back({k.level + 1, 2 * k.i + (c & 1), 2 * k.j + (c >> 1)});
+    } else {
+      leaves.push_back(k);
+    }
+  }
+}
+
+// Minimal 2:1 (face and corner) balanced refinement of the tree: a leaf is
+// split whenever a level-(l+1) node touching it is itself refined.
+void balance(TreeMesh& M) {
+  std::vector<BlockKey> leaves, split;
+  for (;;) {
+    collect_leaves(M, leaves);
+    split.clear();
+    for (const BlockKey& L : leaves) {
+      const int l1 = L.level + 1;
+      const long px = long(M.layout[0]) << l1, py = long(M.layout[1]) << l1;
+      bool need = false;
+      for (int dy = -1; dy <= 2 && !need; ++dy)
+        for (int dx = -1; dx <= 2 && !need; ++dx) {
+          if ((dx == 0 || dx == 1) && (dy == 0 || dy == 1)) continue;
+          long x = 2L * L.i + dx, y = 2L * L.j + dy;
+          if (x < 0 || x >= px) {
+            if (!M.periodic[0]) continue;
+            x = wrapl(x, px);
+          }
+          if (y < 0 || y >= py) {
+            if (!M.periodic[1]) continue;
+            y = wrapl(y, py);
+          }
+          need = M.internal.count(key64(l1, int(x), int(y))) != 0;
+        }
+      if (need) split.push_back(L);
+    }
+    if (split.empty()) return;
+    for (const BlockKey& k : split) M.internal.insert(key64(k.level, k.i, k.j));
+  }
+}
+
+void block_geometry(const TreeMesh& M, const BlockKey& k, double* spacing) {
+  double w0 = M.root_extent[0], w1 = M.root_extent[1];
+  for (int l = 0; l < k.level; ++l) {
+    w0 /= 2;
+    w1 /= 2;
+  }
+  spacing[0] = w0 / M.bs[0];
+  spacing[1] = w1 / M.bs[1];
+}
+
+// Face schedule in the reference order: axis-major, then lattice line, then
+// increasing tangential position; boundary faces per line after the
+// interior faces of that line.
+void build_faces(TreeMesh& M) {
+  const int maxl = M.max_leaf;
+  const long lat[2] = {long(M.base[0]) << maxl, long(M.base[1]) << maxl};
+  const int bx = M.bs[0], by = M.bs[1], nb = bx * by;
+  const int nblk = int(M.blocks.size());
+  const long ncell = long(nblk) * nb;
+  struct Side {
+    long lo, hi;
+    int cell;
+  };
+  std::vector<long> xlo(ncell), ylo(ncell), sc(ncell);
+  for (int b = 0; b < nblk; ++b) {
+    const long s = 1L << (maxl - M.blocks[b].level);
+    for (int j = 0; j < by; ++j)
+      for (int i = 0; i < bx; ++i) {
+        const long c = long(b) * nb + j * bx + i;
+        xlo[c] = (long(M.blocks[b].i) * bx + i) * s;
+        ylo[c] = (long(M.blocks[b].j) * by + j) * s;
+        sc[c] = s;
+      }
+  }
+  for (int axis = 0; axis < 2; ++axis) {
+    const int t = 1 - axis;
+    const long lines = lat[axis];
+    std::vector<long> lcount(lines + 2, 0), ucount(lines + 2, 0);
+    auto low_of = [&](long c) { return axis == 0 ? xlo[c] : ylo[c]; };
+    auto tlo_of = [&](long c) { return axis == 0 ? ylo[c] : xlo[c]; };
+    auto canon = [&](long v) { return M.periodic[axis] ? wrapl(v, lines) : v; };
+    for (long c = 0; c < ncell; ++c) {
+      lcount[canon(low_of(c) + sc[c]) + 1]++;
+      ucount[canon(low_of(c)) + 1]++;
+    }
+    for (long q = 0; q <= lines; ++q) {
+      lcount[q + 1] += lcount[q];
+      ucount[q + 1] += ucount[q];
+    }
+    std::vector<Side> lower(ncell), upper(ncell);
+    {
+      std::vector<long> ln(lcount.begin(), lcount.end() - 1), un(ucount.begin(), ucount.end() - 1);
+      for (long c = 0; c < ncell; ++c) {
+        const Side s{tlo_of(c), tlo_of(c) + sc[c], int(c)};
+        lower[ln[canon(low_of(c) + sc[c])]++] = s;
+        upper[un[canon(low_of(c))]++] = s;
+      }
+    }
+    const double nscale = M.extent[axis] / double(lat[axis]);
+    const double tscale = M.extent[t] / double(lat[t]);
+    auto by_lo = [](const Side& a, const Side& b) { return a.lo < b.lo; };
+    for (long line = 0; line <= lines; ++line) {
+      Side* l0 = lower.data() + lcount[line];
+      Side* l1 = lower.data() + lcount[line + 1];
+      Side* u0 = upper.data() + ucount[line];
+      Side* u1 = upper.data() + ucount[line + 1];
+      std::sort(l0, l1, by_lo);
+      std::sort(u0, u1, by_lo);
+      Side *li = l0, *ui = u0;
+      while (li < l1 && ui < u1) {
+        const long lo = std::max(li->lo, ui->lo), hi = std::min(li->hi, ui->hi);
+        if (lo < hi) {
+          M.fminus.push_back(li->cell);
+          M.fplus.push_back(ui->cell);
+          M.faxis.push_back(int8_t(axis));
+          M.farea.push_back((hi - lo) * tscale);
+          const double nc = M.origin[axis] + line * nscale;
+          const double tc = M.origin[t] + (lo + hi) * tscale / 2;
+          M.fcx.push_back(axis == 0 ? nc : tc);
+          M.fcy.push_back(axis == 0 ? tc : nc);
+        }
+        const long lh = li->hi, uh = ui->hi;
+        if (lh <= uh) ++li;
+        if (uh <= lh) ++ui;
+      }
+      for (; li < l1; ++li) {
+        M.bcell.push_back(li->cell);
+        M.baxis.push_back(int8_t(axis));
+        M.bside.push_back(1);
+        M.barea.push_back((li->hi - li->lo) * M.extent[t] / double(lat[t]));
+      }
+      for (; ui < u1; ++ui) {
+        M.bcell.push_back(ui->cell);
+        M.baxis.push_back(int8_t(axis));
+        M.bside.push_back(-1);
+        M.barea.push_back((ui->hi - ui->lo) * M.extent[t] / double(lat[t]));
+      }
+    }
+  }
+}
+
+}  // namespace
+
+std::unique_ptr<TreeMesh> make_tree_mesh(const int base[2], const int bs[2], const double origin[2],
+                                         const double extent[2], const bool periodic[2], int max_level,
+                                         const std::vector<BlockKey>& refined) {
+  auto M = std::make_unique<TreeMesh>();
+  for (int a = 0; a < 2; ++a) {
+    if (bs[a] < 2 || bs[a] % 2) throw std::invalid_argument("block dimensions must be even and at least two");
+    if (base[a] < bs[a] || base[a] % bs[a])
+      throw std::invalid_argument("base mesh dimensions must be integer multiples of block dimensions");
+    if (!(extent[a] > 0)) throw std::invalid_argument("domain extents must be positive");
+    M->base[a] = base[a];
+    M->bs[a] = bs[a];
+    M->layout[a] = base[a] / bs[a];
+    M->origin[a] = origin[a];
+    M->extent[a] = extent[a];
+    M->root_extent[a] = extent[a] / M->layout[a];
+    M->periodic[a] = periodic[a];
+  }
+  if (max_level < 0) throw std::invalid_argument("max_level must be nonnegative");
+  M->max_level = max_level;
+  std::unordered_set<uint64_t> req;
+  req.reserve(refined.size() * 2 + 1);
+  for (const BlockKey& k : refined)
+    if (k.level < max_level) req.insert(key64(k.level, k.i, k.j));
+  std::vector<BlockKey> stack;
+  for (int j = 0; j < M->layout[1]; ++j)
+    for (int i = 0; i < M->layout[0]; ++i) stack.push_back({0, i, j});
+  while (!stack.empty()) {
+    BlockKey k = stack.back();
+    stack.pop_back();
+    if (k.level >= max_level || !req.count(key64(k.level, k.i, k.j))) continue;
+    M->internal.insert(key64(k.level, k.i, k.j));
+    for (int c = 0; c < 4; ++c) stack.push_back({k.level + 1, 2 * k.i + (c & 1), 2 * k.j + (c >> 1)});
+  }
+  balance(*M);
+  collect_leaves(*M, M->blocks);
+  std::sort(M->blocks.begin(), M->blocks.end(), [](const BlockKey& a, const BlockKey& b) {
+    return std::make_tuple(a.level, a.j, a.i) < std::make_tuple(b.level, b.j, b.i);
+  });
+  M->block_of.reserve(M->blocks.size() * 2);
+  M->bspacing.resize(2 * M->blocks.size());
+  M->max_leaf = 0;
+  for (int b = 0; b < int(M->blocks.size()); ++b) {
+    const BlockKey& k = M->blocks[b];
+    M->block_of[key64(k.level, k.i, k.j)] = b;
+    block_geometry(*M, k, &M->bspacing[2 * b]);
+    M->max_leaf = std::max(M->max_leaf, k.level);
+  }
+  build_faces(*M);
+  return M;
+}
+
+// ---------------------------------------------------------------------------
+// Inverse face metric.
+
+namespace {
+
+struct PatchFace {
+  int face, axis;
+  double area, d0, d1;
+};
+
+// Vertex-patch Riesz map (gradient.jl:194-232): L = N (sym M'N)^-1 N'
+// + tr/(n-2) (I - M (M'M)^-1 M'), symmetrised.
+std::vector<double> patch_metric(const std::vector<PatchFace>& F) {
+  const int n = int(F.size());
+  double mn00 = 0, mn01 = 0, mn10 = 0, mn11 = 0, g00 = 0, g01 = 0, g11 = 0;
+  std::vector<double> m0(n), m1(n), n0(n), n1(n);
+  for (int r = 0; r < n; ++r) {
+    m0[r] = F[r].area * F[r].d0;
+    m1[r] = F[r].area * F[r].d1;
+    n0[r] = F[r].axis == 0 ? 0.5 : 0.0;
+    n1[r] = F[r].axis == 1 ? 0.5 : 0.0;
+    mn00 += m0[r] * n0[r];
+    mn01 += m0[r] * n1[r];
+    mn10 += m1[r] * n0[r];
+    mn11 += m1[r] * n1[r];
+    g00 += m0[r] * m0[r];
+    g01 += m0[r] * m1[r];
+    g11 += m1[r] * m1[r];
+  }
+  const double s00 = mn00, s11 = mn11, s01 = (mn01 + mn10) / 2;
+  const double det_s = s00 * s11 - s01 * s01;
+  if (!(s00 > 0) || !(det_s > 0)) throw std::invalid_argument("degenerate vertex patch in gradient construction");
+  const double det_g = g00 * g11 - g01 * g01;
+  if (!(det_g > 0)) throw std::invalid_argument("degenerate vertex patch in gradient construction");
+  // inverses of the 2x2 symmetric matrices
+  const double si00 = s11 / det_s, si11 = s00 / det_s, si01 = -s01 / det_s;
+  const double gi00 = g11 / det_g, gi11 = g00 / det_g, gi01 = -g01 / det_g;
+  std::vector<double> C(n * n), L(n * n);
+  double trace = 0;
+  for (int r = 0; r < n; ++r)
+    for (int c = 0; c < n; ++c) {
+      const double v = n0[r] * (si00 * n0[c] + si01 * n1[c]) + n1[r] * (si01 * n0[c] + si11 * n1[c]);
+      C[r * n + c] = v;
+      if (r == c) trace += v;
+    }
+  const double stab = trace / (n - 2);
+  for (int r = 0; r < n; ++r)
+    for (int c = 0; c < n; ++c) {
+      const double proj = (r == c ? 1.0 : 0.0) -
+                          (m0[r] * (gi00 * m0[c] + gi01 * m1[c]) + m1[r] * (gi01 * m0[c] + gi11 * m1[c]));
+      L[r * n + c] = C[r * n + c] + stab * proj;
+    }
+  std::vector<double> out(n * n);
+  for (int r = 0; r < n; ++r)
+    for (int c = 0; c < n; ++c) out[r * n + c] = (L[r * n + c] + L[c * n + r]) / 2;
+  return out;
+}
+
+}  // namespace
+
+FaceMetric face_metric(const TreeMesh& M) {
+  const int F = M.nfaces();
+  const int bx = M.bs[0], by = M.bs[1], nb = bx * by;
+  const int maxl = M.max_leaf;
+  const long lat[2] = {long(M.base[0]) << maxl, long(M.base[1]) << maxl};
+  const int nblk = int(M.blocks.size());
+  // same-level leaf flags around each block: W E S N SW SE NW NE
+  std::vector<std::array<uint8_t, 8>> flag(nblk);
+  const int dxs[8] = {-1, 1, 0, 0, -1, 1, -1, 1}, dys[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+  for (int b = 0; b < nblk; ++b) {
+    const BlockKey& k = M.blocks[b];
+    const long px = long(M.layout[0]) << k.level, py = long(M.layout[1]) << k.level;
+    for (int q = 0; q < 8; ++q) {
+      long x = k.i + dxs[q], y = k.j + dys[q];
+      bool ok = true;
+      if (x < 0 || x >= px) {
+        if (M.periodic[0]) x = wrapl(x, px);
+        else ok = false;
+      }
+      if (y < 0 || y >= py) {
+        if (M.periodic[1]) y = wrapl(y, py);
+        else ok = false;
+      }
+      // outside a wall: the vertex is physical and handled before this test
+      flag[b][q] = ok ? uint8_t(M.block_of.count(key64(k.level, int(x), int(y))) != 0) : uint8_t(1);
+    }
+  }
+  FaceMetric K;
+  K.diag.assign(F, 0.0);
+  std::vector<std::array<double, 2>> disp(F);
+  std::unordered_map<uint64_t, std::vector<int>> cand;  // vertex -> faces
+  for (int f = 0; f < F; ++f) {
+    const int m = M.fminus[f], p = M.fplus[f];
+    const int a = M.faxis[f], t = 1 - a;
+    const int bm = m / nb, bp = p / nb;
+    const BlockKey& km = M.blocks[bm];
+    const BlockKey& kp = M.blocks[bp];
+    const double hm = M.bspacing[2 * bm + a], hp = M.bspacing[2 * bp + a];
+    const long sm = 1L << (maxl - km.level), sp = 1L << (maxl - kp.level);
+    const long mx = (long(km.i) * bx + (m % nb) % bx) * sm, my = (long(km.j) * by + (m % nb) / bx) * sm;
+    const long qx = (long(kp.i) * bx + (p % nb) % bx) * sp, qy = (long(kp.j) * by + (p % nb) / bx) * sp;
+    const long mlo[2] = {mx, my}, plo[2] = {qx, qy};
+    // displacement: normal = mean width; tangential = lattice centre offset
+    long offset = 2 * plo[t] + sp - 2 * mlo[t] - sm;
+    if (M.periodic[t]) {
+      if (offset > lat[t]) offset -= 2 * lat[t];
+      if (offset < -lat[t]) offset += 2 * lat[t];
+    }
+    disp[f][a] = (hm + hp) / 2;
+    disp[f][t] = double(offset) * ((M.extent[t] / double(lat[t])) / 2);
+    long normal = mlo[a] + sm;
+    if (M.periodic[a]) normal = wrapl(normal, lat[a]);
+    const long lo = std::max(mlo[t], plo[t]), hi = std::min(mlo[t] + sm, plo[t] + sp);
+    const double simple = 1.0 / (2.0 * M.farea[f] * disp[f][a]);
+    const bool same_level = km.level == kp.level;
+    for (int e = 0; e < 2; ++e) {
+      const long tv = e == 0 ? lo : hi;
+      long vx = a == 0 ? normal : tv, vy = a == 0 ? tv : normal;
+      if (M.periodic[0]) vx = wrapl(vx, lat[0]);
+      if (M.periodic[1]) vy = wrapl(vy, lat[1]);
+      const bool physical = (!M.periodic[0] && (vx == 0 || vx == lat[0])) ||
+                            (!M.periodic[1] && (vy == 0 || vy == lat[1]));
+      bool regular = physical;
+      if (!regular && same_level) {
+        // position of the vertex relative to the minus block, in its cells
+        const long bx0 = long(km.i) * bx * sm, by0 = long(km.j) * by * sm;
+        long lx = vx - bx0, ly = vy - by0;
+        const long spanx = long(bx) * sm, spany = long(by) * sm;
+        if (M.periodic[0]) {
+          if (lx < 0) lx += lat[0];
+          if (lx > spanx) lx -= lat[0];
+        }
+        if (M.periodic[1]) {
+          if (ly < 0) ly += lat[1];
+          if (ly > spany) ly -= lat[1];
+        }
+        const int ex = lx == 0 ? -1 : (lx == spanx ? 1 : 0);
+        const int ey = ly == 0 ? -1 : (ly == spany ? 1 : 0);
+        const auto& fl = flag[bm];
+        if (ex == 0 && ey == 0) regular = true;
+        else if (ey == 0) regular = fl[ex < 0 ? 0 : 1];
+        else if (ex == 0) regular = fl[ey < 0 ? 2 : 3];
+        else {
+          const int side_x = ex < 0 ? 0 : 1, side_y = ey < 0 ? 2 : 3;
+          const int corner = (ey < 0 ? 4 : 6) + (ex < 0 ? 0 : 1);
+          regular = fl[side_x] && fl[side_y] && fl[corner];
+        }
+      }
+      if (regular) {
+        K.diag[f] += simple;
+      } else {
+        cand[(uint64_t(vx) << 32) | uint64_t(vy)].push_back(f);
+      }
+    }
+  }
+  // irregular vertex patches
+  std::vector<std::vector<std::pair<int, double>>> off(F);
+  for (auto& kv : cand) {
+    std::vector<int>& faces = kv.second;
+    if (faces.size() <= 2) {
+      for (int f : faces) K.diag[f] += 1.0 / (2.0 * M.farea[f] * disp[f][M.faxis[f]]);
+      continue;
+    }
+    std::vector<PatchFace> pf;
+    for (int f : faces) pf.push_back({f, M.faxis[f], M.farea[f], disp[f][0], disp[f][1]});
+    // a patch of four same-level faces with square geometry is the MAC patch
+    bool mac = pf.size() == 4;
+    if (mac) {
+      int nax[2] = {0, 0};
+      for (const auto& q : pf) {
+        nax[q.axis]++;
+        const double tang = q.axis == 0 ? q.d1 : q.d0;
+        mac = mac && tang == 0.0 && M.block_level(M.fminus[q.face]) == M.block_level(M.fplus[q.face]) &&
+              M.block_level(M.fminus[q.face]) == M.block_level(M.fminus[pf[0].face]);
+      }
+      mac = mac && nax[0] == 2 && nax[1] == 2;
+    }
+    if (mac) {
+      for (const auto& q : pf) K.diag[q.face] += 1.0 / (2.0 * q.area * (q.axis == 0 ? q.d0 : q.d1));
+      continue;
+    }
+    std::sort(pf.begin(), pf.end(), [](const PatchFace& x, const PatchFace& y) { return x.face < y.face; });
+    const std::vector<double> L = patch_metric(pf);
+    const int n = int(pf.size());
+    for (int r = 0; r < n; ++r) {
+      K.diag[pf[r].face] += L[r * n + r];
+      for (int c = 0; c < n; ++c)
+        if (c != r) off[pf[r].face].push_back({pf[c].face, L[r * n + c]});
+    }
+  }
+  K.off_ptr.assign(F + 1, 0);
+  for (int f = 0; f < F; ++f) {
+    auto& v = off[f];
+    std::sort(v.begin(), v.end());
+    // merge duplicates (two faces meet in at most one patch, kept for safety)
+    std::vector<std::pair<int, double>> merged;
+    for (auto& e : v) {
+      if (!merged.empty() && merged.back().first == e.first) merged.back().second += e.second;
+      else merged.push_back(e);
+    }
+    for (auto& e : merged)
+      if (e.second != 0.0) {
+        K.off_face.push_back(e.first);
+        K.off_val.push_back(e.second);
+      }
+    K.off_ptr[f + 1] = int(K.off_face.size());
+  }
+  for (int f = 0; f < F; ++f)
+    if (!(K.diag[f] > 0)) throw std::invalid_argument("gradient Riesz map has a non-positive diagonal");
+  return K;
+}
+
+// ---------------------------------------------------------------------------
+// Levels
+
+namespace {
+
+std::unique_ptr<TreeMesh> pruned_mesh(const TreeMesh& M) {
+  std::vector<BlockKey> keep;
+  int maxl = -1;
+  for (uint64_t pk : M.internal) {
+    const int l = int(pk >> 56), i = int((pk >> 28) & 0xFFFFFFF), j = int(pk & 0xFFFFFFF);
+    bool child = false;
+    for (int c = 0; c < 4 && !child; ++c)
+      child = M.internal.count(key64(l + 1, 2 * i + (c & 1), 2 * j + (c >> 1))) != 0;
+    if (child) {
+      keep.push_back({l, i, j});
+      maxl = std::max(maxl, l);
+    }
+  }
+  return make_tree_mesh(M.base, M.bs, M.origin, M.extent, M.periodic, maxl + 1, keep);
+}
+
+std::unique_ptr<TreeMesh> agglomerated_mesh(const TreeMesh& M) {
+  if (M.layout[0] % 2 || M.layout[1] % 2) return nullptr;
+  const int cb[2] = {M.base[0] / 2, M.base[1] / 2};
+  for (int a = 0; a < 2; ++a)
+    if (M.periodic[a] && cb[a] < 4) return nullptr;
+  return make_tree_mesh(cb, M.bs, M.origin, M.extent, M.periodic, 0, {});
+}
+
+// Topology-only parts of a level: slots, regularity, neighbours, patterns.
+void level_topology(LevelData& L) {
+  const TreeMesh& M = *L.mesh;
+  const int bx = M.bs[0], by = M.bs[1], nb = bx * by;
+  L.nblocks = int(M.blocks.size());
+  L.ncells = M.ncells();
+  L.K = face_metric(M);
+  const int F = M.nfaces();
+  L.face_regular.assign(F, 0);
+  L.slot_a.assign(F, -1);
+  L.slot_b.assign(F, -1);
+  for (int f = 0; f < F; ++f) {
+    const int m = M.fminus[f], p = M.fplus[f];
+    const bool reg = M.block_level(m) == M.block_level(p) && L.K.off_ptr[f] == L.K.off_ptr[f + 1];
+    L.face_regular[f] = reg;
+    if (!reg) continue;
+    const int bm = m / nb, lm = m % nb, bp = p / nb, lp = p % nb;
+    if (M.faxis[f] == 0) {
+      L.slot_a[f] = int64_t(bm) * (bx + 1) * by + (lm / bx) * (bx + 1) + (lm % bx) + 1;
+      L.slot_b[f] = int64_t(bp) * (bx + 1) * by + (lp / bx) * (bx + 1) + (lp % bx);
+    } else {
+      L.slot_a[f] = int64_t(bm) * bx * (by + 1) + (lm / bx + 1) * bx + (lm % bx);
+      L.slot_b[f] = int64_t(bp) * bx * (by + 1) + (lp / bx) * bx + (lp % bx);
+    }
+  }
+  // neighbours
+  L.nbr.assign(4 * L.nblocks, -1);
+  for (int b = 0; b < L.nblocks; ++b) {
+    const BlockKey& k = M.blocks[b];
+    const long px = long(M.layout[0]) << k.level, py = long(M.layout[1]) << k.level;
+    const int dx[4] = {-1, 1, 0, 0}, dy[4] = {0, 0, -1, 1};
+    for (int s = 0; s < 4; ++s) {
+      long x = k.i + dx[s], y = k.j + dy[s];
+      if (x < 0 || x >= px) {
+        if (!M.periodic[0]) continue;
+        x = wrapl(x, px);
+      }
+      if (y < 0 || y >= py) {
+        if (!M.periodic[1]) continue;
+        y = wrapl(y, py);
+      }
+      auto it = M.block_of.find(key64(k.level, int(x), int(y)));
+      if (it != M.block_of.end()) L.nbr[4 * b + s] = it->second;
+    }
+  }
+  L.volume.resize(L.nblocks);
+  for (int b = 0; b < L.nblocks; ++b) L.volume[b] = M.bspacing[2 * b] * M.bspacing[2 * b + 1];
+  // irregular cells: touching a non-regular face
+  L.cell_irr.assign(L.ncells, -1);
+  std::vector<uint8_t> irr(L.ncells, 0);
+  for (int f = 0; f < F; ++f)
+    if (!L.face_regular[f]) irr[M.fminus[f]] = irr[M.fplus[f]] = 1;
+  L.blk_irr.assign(L.nblocks + 1, 0);
+  L.irr_local.clear();
+  for (int b = 0; b < L.nblocks; ++b) {
+    for (int q = 0; q < nb; ++q)
+      if (irr[b * nb + q]) {
+        L.cell_irr[b * nb + q] = int(L.irr_local.size());
+        L.irr_local.push_back(q);
+      }
+    L.blk_irr[b + 1] = int(L.irr_local.size());
+  }
+}
+
+// Coefficient-dependent values of a level.
+void level_values(LevelData& L) {
+  const TreeMesh& M = *L.mesh;
+  const int bx = M.bs[0], by = M.bs[1], nb = bx * by;
+  const int F = M.nfaces();
+  std::vector<double> sq(F);
+  for (int f = 0; f < F; ++f) sq[f] = std::sqrt(L.kappa[f]);
+  L.cx.assign(size_t(L.nblocks) * (bx + 1) * by, 0.0);
+  L.cy.assign(size_t(L.nblocks) * bx * (by + 1), 0.0);
+  for (int f = 0; f < F; ++f) {
+    if (!L.face_regular[f]) continue;
+    const double a = M.farea[f];
+    const double w = L.kappa[f] * L.K.diag[f] * a * a;
+    std::vector<double>& arr = M.faxis[f] == 0 ? L.cx : L.cy;
+    arr[L.slot_a[f]] = w;
+    arr[L.slot_b[f]] = w;
+  }
+  // irregular rows: full rows of A (and of the floor bound |J|'|M||J|)
+  const int nrows = int(L.irr_local.size());
+  std::vector<std::vector<int>> inc;  // faces per irregular row
+  inc.resize(nrows);
+  for (int f = 0; f < F; ++f) {
+    const int m = M.fminus[f], p = M.fplus[f];
+    if (L.cell_irr[m] >= 0) inc[L.cell_irr[m]].push_back(f);
+    if (p != m && L.cell_irr[p] >= 0) inc[L.cell_irr[p]].push_back(f);
+  }
+  L.irr_ptr.assign(nrows + 1, 0);
+  L.irr_col.clear();
+  L.irr_val.clear();
+  L.irr_bound.clear();
+  L.irr_diag.assign(nrows, 0.0);
+  double max_ratio = 0;
+  std::vector<std::pair<int, std::pair<double, double>>> row;
+  for (int b = 0; b < L.nblocks; ++b)
+    for (int r = L.blk_irr[b]; r < L.blk_irr[b + 1]; ++r) {
+      const int c = b * nb + L.irr_local[r];
+      row.clear();
+      for (int f : inc[r]) {
+        const double sf = (M.fminus[f] == c) ? -1.0 : 1.0;
+        const double af = M.farea[f];
+        auto add = [&](int g, double kfg, bool diag) {
+          const double ag = M.farea[g];
+          const double core = diag ? L.kappa[f] * kfg * af * ag : (sq[f] * kfg) * sq[g] * af * ag;
+          const double babs = diag ? core : (sq[f] * std::fabs(kfg)) * sq[g] * af * ag;
+          row.push_back({M.fminus[g], {sf * -1.0 * core, babs}});
+          row.push_back({M.fplus[g], {sf * 1.0 * core, babs}});
+        };
+        add(f, L.K.diag[f], true);
+        for (int q = L.K.off_ptr[f]; q < L.K.off_ptr[f + 1]; ++q) add(L.K.off_face[q], L.K.off_val[q], false);
+      }
+      std::sort(row.begin(), row.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+      double diag = 0, rowsum = 0;
+      size_t q = 0;
+      while (q < row.size()) {
+        const int col = row[q].first;
+        double v = 0, bv = 0;
+        while (q < row.size() && row[q].first == col) {
+          v += row[q].second.first;
+          bv += row[q].second.second;
+          ++q;
+        }
+        L.irr_col.push_back(col);
+        L.irr_val.push_back(v);
+        L.irr_bound.push_back(bv);
+        rowsum += std::fabs(v);
+        if (col == c) diag = v;
+      }
+      L.irr_ptr[r + 1] = int(L.irr_col.size());
+      L.irr_diag[r] = diag;
+      if (diag > 0) max_ratio = std::max(max_ratio, rowsum / diag);
+    }
+  // diagonal of regular rows and the Jacobi weights
+  L.dinv.assign(L.ncells, 0.0);
+  bool any_regular = false;
+  for (int b = 0; b < L.nblocks; ++b)
+    for (int q = 0; q < nb; ++q) {
+      const int c = b * nb + q;
+      double d;
+      if (L.cell_irr[c] >= 0) {
+        d = L.irr_diag[L.cell_irr[c]];
+      } else {
+        const int i = q % bx, j = q / bx;
+        const double* cx = &L.cx[size_t(b) * (bx + 1) * by];
+        const double* cy = &L.cy[size_t(b) * bx * (by + 1)];
+        d = cx[j * (bx + 1) + i] + cx[j * (bx + 1) + i + 1] + cy[j * bx + i] + cy[(j + 1) * bx + i];
+        if (d > 0) any_regular = true;
+      }
+      L.dinv[c] = d > 0 ? 1.0 / d : 0.0;
+    }
+  if (any_regular) max_ratio = std::max(max_ratio, 2.0);
+  if (max_ratio == 0) {
+    L.omega = 1.0;
+    L.omega_p = 0.0;
+  } else {
+    const double w = 4.0 / (3.0 * max_ratio);
+    L.omega = std::min(1.0, w);
+    L.omega_p = w;
+  }
+}
+
+void restriction_map(LevelData& C, const LevelData& Fl) {
+  // Fine and coarse faces are both ordered by (axis, line, lo) on their own
+  // lattices; map both onto the common lattice and merge.
+  const TreeMesh& fm = *Fl.mesh;
+  const TreeMesh& cm = *C.mesh;
+  long lat[2];
+  for (int a = 0; a < 2; ++a)
+    lat[a] = std::max(long(fm.base[a]) << fm.max_leaf, long(cm.base[a]) << cm.max_leaf);
+  auto geom = [&](const TreeMesh& M, int f, long& axis, long& normal, long& lo, long& hi) {
+    const int nb = M.nb(), bx = M.bs[0], by = M.bs[1];
+    const int m = M.fminus[f], p = M.fplus[f];
+    const BlockKey& km = M.blocks[m / nb];
+    const BlockKey& kp = M.blocks[p / nb];
+    const long sm0 = lat[0] / (long(M.base[0]) << km.level), sm1 = lat[1] / (long(M.base[1]) << km.level);
+    const long sp0 = lat[0] / (long(M.base[0]) << kp.level), sp1 = lat[1] / (long(M.base[1]) << kp.level);
+    const long ml[2] = {(long(km.i) * bx + (m % nb) % bx) * sm0, (long(km.j) * by + (m % nb) / bx) * sm1};
+    const long pl[2] = {(long(kp.i) * bx + (p % nb) % bx) * sp0, (long(kp.j) * by + (p % nb) / bx) * sp1};
+    const long ms[2] = {sm0, sm1}, ps[2] = {sp0, sp1};
+    const int a = M.faxis[f], t = 1 - a;
+    axis = a;
+    normal = ml[a] + ms[a];
+    if (M.periodic[a]) normal = wrapl(normal, lat[a]);
+    lo = std::max(ml[t], pl[t]);
+    hi = std::min(ml[t] + ms[t], pl[t] + ps[t]);
+  };
+  const int FF = fm.nfaces(), CF = cm.nfaces();
+  std::vector<std::array<long, 4>> fg(FF);
+  for (int f = 0; f < FF; ++f) geom(fm, f, fg[f][0], fg[f][1], fg[f][2], fg[f][3]);
+  C.rptr.assign(CF + 1, 0);
+  C.ridx.clear();
+  C.rw.clear();
+  int q = 0;
+  for (int f = 0; f < CF; ++f) {
+    long a, n, lo, hi;
+    geom(cm, f, a, n, lo, hi);
+    // advance the fine pointer to the first segment on this line ending after lo
+    auto before = [&](int g) {
+      return std::make_tuple(fg[g][0], fg[g][1], fg[g][3]) <= std::make_tuple(a, n, lo);
+    };
+    while (q < FF && before(q)) ++q;
+    long covered = 0;
+    int r = q;
+    while (r < FF && fg[r][0] == a && fg[r][1] == n && fg[r][2] < hi) {
+      const long ov = std::min(hi, fg[r][3]) - std::max(lo, fg[r][2]);
+      if (ov > 0) {
+        C.ridx.push_back(r);
+        C.rw.push_back(double(ov) / double(hi - lo));
+        covered += ov;
+      }
+      ++r;
+    }
+    if (covered != hi - lo) throw std::runtime_error("tree hierarchy could not cover a coarse face with fine faces");
+    C.rptr[f + 1] = int(C.ridx.size());
+  }
+}
+
+void transfer_descriptors(LevelData& Fl, const LevelData& C, bool agglomerated) {
+  const TreeMesh& fm = *Fl.mesh;
+  const TreeMesh& cm = *C.mesh;
+  Fl.xfer.assign(2 * Fl.nblocks, -1);
+  for (int b = 0; b < Fl.nblocks; ++b) {
+    const BlockKey& k = fm.blocks[b];
+    if (agglomerated) {
+      auto it = cm.block_of.find(key64(0, k.i >> 1, k.j >> 1));
+      if (it == cm.block_of.end()) throw std::runtime_error("agglomerated root block missing");
+      Fl.xfer[2 * b] = it->second;
+      Fl.xfer[2 * b + 1] = (k.i & 1) + 2 * (k.j & 1);
+      continue;
+    }
+    auto it = cm.block_of.find(key64(k.level, k.i, k.j));
+    if (it != cm.block_of.end()) {
+      Fl.xfer[2 * b] = it->second;
+      Fl.xfer[2 * b + 1] = -1;
+      continue;
+    }
+    if (k.level == 0) throw std::runtime_error("coarse mesh lost a root block");
+    auto jt = cm.block_of.find(key64(k.level - 1, k.i >> 1, k.j >> 1));
+    if (jt == cm.block_of.end()) throw std::runtime_error("coarse mesh is not a pruning of the fine mesh");
+    Fl.xfer[2 * b] = jt->second;
+    Fl.xfer[2 * b + 1] = (k.i & 1) + 2 * (k.j & 1);
+  }
+}
+
+void coarse_inverse(Hierarchy& H) {
+  LevelData& L = *H.levels.back();
+  const TreeMesh& M = *L.mesh;
+  const int n = L.ncells, nb = M.nb(), bx = M.bs[0], by = M.bs[1];
+  const int N = n + 1;  // one component (connected, no anchors)
+  std::vector<double> a(size_t(N) * N, 0.0);
+  for (int b = 0; b < L.nblocks; ++b)
+    for (int q = 0; q < nb; ++q) {
+      const int c = b * nb + q;
+      if (L.cell_irr[c] >= 0) {
+        const int r = L.cell_irr[c];
+        for (int e = L.irr_ptr[r]; e < L.irr_ptr[r + 1]; ++e) a[size_t(c) * N + L.irr_col[e]] += L.irr_val[e];
+      } else {
+        const int i = q % bx, j = q / bx;
+        const double* cx = &L.cx[size_t(b) * (bx + 1) * by];
+        const double* cy = &L.cy[size_t(b) * bx * (by + 1)];
+        const double w[4] = {cx[j * (bx + 1) + i], cx[j * (bx + 1) + i + 1], cy[j * bx + i], cy[(j + 1) * bx + i]};
+        // neighbours across the four sides
+        int nbc[4] = {-1, -1, -1, -1};
+        auto cell_at = [&](int bb, int ii, int jj) { return bb * nb + jj * bx + ii; };
+        nbc[0] = i > 0 ? cell_at(b, i - 1, j) : (L.nbr[4 * b] >= 0 ? cell_at(L.nbr[4 * b], bx - 1, j) : -1);
+        nbc[1] = i < bx - 1 ? cell_at(b, i + 1, j) : (L.nbr[4 * b + 1] >= 0 ? cell_at(L.nbr[4 * b + 1], 0, j) : -1);
+        nbc[2] = j > 0 ? cell_at(b, i, j - 1) : (L.nbr[4 * b + 2] >= 0 ? cell_at(L.nbr[4 * b + 2], i, by - 1) : -1);
+        nbc[3] = j < by - 1 ? cell_at(b, i, j + 1) : (L.nbr[4 * b + 3] >= 0 ? cell_at(L.nbr[4 * b + 3], i, 0) : -1);
+        for (int s = 0; s < 4; ++s) {
+          if (w[s] == 0.0) continue;
+          if (nbc[s] < 0) throw std::runtime_error("regular face without a neighbour");
+          a[size_t(c) * N + c] += w[s];
+          a[size_t(c) * N + nbc[s]] -= w[s];
+        }
+      }
+      const double V = L.volume[b];
+      a[size_t(c) * N + n] = V;
+      a[size_t(n) * N + c] = V;
+    }
+  // LU with partial pivoting, then the top-left n x n block of the inverse
+  std::vector<int> piv(N);
+  for (int k = 0; k < N; ++k) {
+    int p = k;
+    double best = std::fabs(a[size_t(k) * N + k]);
+    for (int i = k + 1; i < N; ++i)
+      if (std::fabs(a[size_t(i) * N + k]) > best) {
+        best = std::fabs(a[size_t(i) * N + k]);
+        p = i;
+      }
+    piv[k] = p;
+    if (best == 0.0) throw std::runtime_error("singular coarse operator");
+    if (p != k)
+      for (int j = 0; j < N; ++j) std::swap(a[size_t(k) * N + j], a[size_t(p) * N + j]);
+    const double d = a[size_t(k) * N + k];
+    for (int i = k + 1; i < N; ++i) {
+      const double l = a[size_t(i) * N + k] / d;
+      a[size_t(i) * N + k] = l;
+      if (l == 0.0) continue;
+      double* ri = &a[size_t(i) * N];
+      const double* rk = &a[size_t(k) * N];
+      for (int j = k + 1; j < N; ++j) ri[j] -= l * rk[j];
+    }
+  }
+  H.coarse_n = n;
+  H.coarse_inv.assign(size_t(n) * n, 0.0);
+  std::vector<double> x(N);
+  for (int col = 0; col < n; ++col) {
+    std::fill(x.begin(), x.end(), 0.0);
+    x[col] = 1.0;
+    for (int k = 0; k < N; ++k)
+      if (piv[k] != k) std::swap(x[k], x[piv[k]]);
+    for (int i = 0; i < N; ++i) {
+      double s = x[i];
+      const double* ri = &a[size_t(i) * N];
+      for (int j = 0; j < i; ++j) s -= ri[j] * x[j];
+      x[i] = s;
+    }
+    for (int i = N - 1; i >= 0; --i) {
+      double s = x[i];
+      const double* ri = &a[size_t(i) * N];
+      for (int j = i + 1; j < N; ++j) s -= ri[j] * x[j];
+      x[i] = s / ri[i];
+    }
+    // inverse column `col`; stored column-major: inv[col * n + row]
+    for (int row = 0; row < n; ++row) H.coarse_inv[size_t(col) * n + row] = x[row];
+  }
+}
+
+}  // namespace
+
+std::unique_ptr<Hierarchy> build_hierarchy(std::unique_ptr<TreeMesh> fine, const MultigridConfig& cfg) {
+  if (cfg.coarse_size <= 0) throw std::invalid_argument("coarse_size must be positive");
+  if (cfg.pre < 0 || cfg.post < 0) throw std::invalid_argument("sweep counts must be nonnegative");
+  if (cfg.max_levels <= 0) throw std::invalid_argument("max_levels must be positive");
+  auto H = std::make_unique<Hierarchy>();
+  auto L0 = std::make_unique<LevelData>();
+  L0->owned = std::move(fine);
+  L0->mesh = L0->owned.get();
+  level_topology(*L0);
+  H->levels.push_back(std::move(L0));
+  std::vector<bool> agg;
+  int active = H->levels[0]->mesh->max_leaf;
+  for (;;) {
+    const TreeMesh& cur = *H->levels.back()->mesh;
+    if (!(cur.ncells() > cfg.coarse_size) || int(H->levels.size()) >= cfg.max_levels) break;
+    const bool is_agg = active == 0;
+    std::unique_ptr<TreeMesh> cm = is_agg ? agglomerated_mesh(cur) : pruned_mesh(cur);
+    if (!cm || cm->ncells() >= cur.ncells()) break;
+    active = cm->max_leaf;
+    auto L = std::make_unique<LevelData>();
+    L->owned = std::move(cm);
+    L->mesh = L->owned.get();
+    level_topology(*L);
+    restriction_map(*L, *H->levels.back());
+    transfer_descriptors(*H->levels.back(), *L, is_agg);
+    H->levels.push_back(std::move(L));
+  }
+  const TreeMesh& M0 = *H->levels[0]->mesh;
+  H->total_volume = 0;
+  for (int b = 0; b < H->levels[0]->nblocks; ++b) H->total_volume += H->levels[0]->volume[b] * M0.nb();
+  return H;
+}
+
+void refresh_coefficients(Hierarchy& H, const double* kappa) {
+  LevelData& L0 = *H.levels[0];
+  const int F = L0.mesh->nfaces();
+  for (int f = 0; f < F; ++f) {
+    if (!std::isfinite(kappa[f])) throw std::invalid_argument("face coefficients must be finite");
+    if (!(kappa[f] > 0))
+      throw std::invalid_argument("face coefficients must be positive (disconnected components are not supported)");
+  }
+  L0.kappa.assign(kappa, kappa + F);
+  for (size_t l = 0; l < H.levels.size(); ++l) {
+    LevelData& L = *H.levels[l];
+    if (l > 0) {
+      const LevelData& P = *H.levels[l - 1];
+      L.kappa.assign(L.mesh->nfaces(), 0.0);
+      for (int f = 0; f < L.mesh->nfaces(); ++f) {
+        double s = 0;
+        for (int q = L.rptr[f]; q < L.rptr[f + 1]; ++q) s += L.rw[q] * P.kappa[L.ridx[q]];
+        L.kappa[f] = s;
+      }
+    }
+    level_values(L);
+  }
+  coarse_inverse(H);
+}
+
+}  // namespace tpamr

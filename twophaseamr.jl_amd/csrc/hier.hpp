@@ -1,0 +1,114 @@
+// Host-side topology and hierarchy construction for the GPU pressure solver.
+//
+// The library receives a block quadtree (the refined node set of a
+// BlockTreeMesh) and per-face coefficients in the reference's face order,
+// and builds, per multigrid level, the block-structured data the HIP
+// kernels consume: regular 5-point conductances, compact CSR rows for
+// the cells next to coarse-fine interfaces, block neighbour and transfer
+// descriptors, and the dense inverse of the coarsest augmented operator.
+#pragma once
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+namespace tpamr {
+
+struct BlockKey {
+  int level, i, j;
+};
+
+inline uint64_t key64(int level, int i, int j) {
+  return (uint64_t(uint32_t(level)) << 56) | (uint64_t(uint32_t(i)) << 28) | uint64_t(uint32_t(j));
+}
+
+// One mesh of the hierarchy (a BlockTreeMesh, src/mesh/topology/mesh.jl).
+struct TreeMesh {
+  int base[2], bs[2], layout[2];
+  double origin[2], extent[2], root_extent[2];
+  bool periodic[2];
+  int max_level;  // refinement ceiling
+  int max_leaf = 0;
+  std::unordered_set<uint64_t> internal;  // refined nodes
+  std::vector<BlockKey> blocks;           // leaves sorted by (level, j, i)
+  std::unordered_map<uint64_t, int> block_of;
+  std::vector<double> bspacing;  // 2 per block
+  // interior faces, reference order
+  std::vector<int> fminus, fplus;
+  std::vector<int8_t> faxis;
+  std::vector<double> farea, fcx, fcy;
+  // boundary faces, reference order
+  std::vector<int> bcell;
+  std::vector<int8_t> baxis, bside;
+  std::vector<double> barea;
+  int nb() const { return bs[0] * bs[1]; }
+  int ncells() const { return int(blocks.size()) * nb(); }
+  int nfaces() const { return int(fminus.size()); }
+  int block_level(int cell) const { return blocks[cell / nb()].level; }
+};
+
+std::unique_ptr<TreeMesh> make_tree_mesh(const int base[2], const int bs[2], const double origin[2],
+                                         const double extent[2], const bool periodic[2], int max_level,
+                                         const std::vector<BlockKey>& refined);
+
+// Inverse face metric K: diagonal plus symmetric off-diagonal pairs.
+struct FaceMetric {
+  std::vector<double> diag;               // per face
+  std::vector<int> off_ptr, off_face;     // CSR of off-diagonal entries by face
+  std::vector<double> off_val;
+  bool diagonal() const { return off_face.empty(); }
+};
+
+FaceMetric face_metric(const TreeMesh& m);
+
+// One multigrid level in device layout.
+struct LevelData {
+  std::unique_ptr<TreeMesh> owned;
+  const TreeMesh* mesh = nullptr;
+  FaceMetric K;
+  int nblocks = 0, ncells = 0;
+  std::vector<double> kappa;  // per face
+  std::vector<uint8_t> face_regular;
+  // face -> conductance slots: x faces in cx[(bx+1)*by] per block, y faces in cy[bx*(by+1)]
+  std::vector<int64_t> slot_a, slot_b;  // flat indices (or -1)
+  std::vector<double> cx, cy;           // conductances (0 where unused)
+  std::vector<int> nbr;                 // 4 per block: W, E, S, N same-level neighbour or -1
+  std::vector<double> volume;           // per block
+  // irregular rows, grouped by block
+  std::vector<int> blk_irr;             // nblocks + 1 row offsets
+  std::vector<int> irr_local;           // local cell index per row
+  std::vector<int> irr_ptr;             // row -> entries
+  std::vector<int> irr_col;
+  std::vector<double> irr_val, irr_bound;  // A values; factorized floor-bound values
+  std::vector<double> irr_diag;
+  std::vector<int> cell_irr;  // per cell: irregular row index or -1
+  std::vector<double> dinv;   // 1 / diag(A)
+  double omega = 1, omega_p = 0;
+  // transfer to the next coarser level (absent on the coarsest)
+  std::vector<int> xfer;  // 2 per block: coarse block, mode (-1 identity, 0..3 quadrant)
+  // face-coefficient restriction from the previous (finer) level
+  std::vector<int> rptr, ridx;
+  std::vector<double> rw;
+  // incidence of irregular cells: face lists
+};
+
+struct Hierarchy {
+  std::vector<std::unique_ptr<LevelData>> levels;
+  // coarsest augmented inverse (n x n, column major)
+  int coarse_n = 0;
+  std::vector<double> coarse_inv;
+  double total_volume = 0;
+};
+
+struct MultigridConfig {
+  int coarse_size = 64, pre = 2, post = 2, max_levels = 32;
+};
+
+// Build the whole hierarchy for fine mesh `fine` (takes ownership).
+std::unique_ptr<Hierarchy> build_hierarchy(std::unique_ptr<TreeMesh> fine, const MultigridConfig& cfg);
+// Refresh all coefficient-dependent values for new fine face coefficients.
+void refresh_coefficients(Hierarchy& H, const double* kappa);
+
+}  // namespace tpamr

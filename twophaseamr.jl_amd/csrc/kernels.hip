@@ -1,0 +1,890 @@
+// HIP kernels for gfx950 (CDNA4): block-tile multigrid operators, FP64 Krylov
+// vector kernels with two-stage deterministic reductions, and face kernels of
+// the two-phase coefficient / projection / adjoint path.
+//
+// Data layout (every multigrid level): cells are block-major, BX x BY cells
+// per block, j-major inside a block (the reference's cell order,
+// src/mesh/topology/mesh.jl:28-30). One workgroup owns one block: one
+// thread per cell, an LDS tile with a one-cell halo gathered from the four
+// same-level neighbour blocks. Conductances of regular faces live per block
+// in cx[(BX+1)*BY] (x faces) and cy[BX*(BY+1)] (y faces); rows next to
+// coarse-fine interfaces or off-diagonal metric entries are explicit CSR rows
+// grouped per block.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace tpamr {
+
+// ---------------------------------------------------------------------------
+// helpers
+
+template <typename T>
+__device__ inline T wave_sum(T v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+template <typename T>
+__device__ inline T wave_max(T v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  return v;
+}
+
+// Sum over the workgroup; result valid in thread 0. `scratch` holds one
+// value per wavefront.
+template <typename T, int NT>
+__device__ inline T block_sum(T v, T* scratch) {
+  constexpr int NW = (NT + 63) / 64;
+  v = wave_sum(v);
+  if constexpr (NW == 1) {
+    return v;
+  } else {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) scratch[w] = v;
+    __syncthreads();
+    T s = 0;
+    if (threadIdx.x == 0)
+      for (int q = 0; q < NW; ++q) s += scratch[q];
+    __syncthreads();
+    return s;
+  }
+}
+
+template <typename T, int NT>
+__device__ inline T block_max(T v, T* scratch) {
+  constexpr int NW = (NT + 63) / 64;
+  v = wave_max(v);
+  if constexpr (NW == 1) {
+    return v;
+  } else {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) scratch[w] = v;
+    __syncthreads();
+    T s = 0;
+    if (threadIdx.x == 0)
+      for (int q = 0; q < NW; ++q) s = fmax(s, scratch[q]);
+    __syncthreads();
+    return s;
+  }
+}
+
+// Global fine-cell index -> owning coarse cell under the tentative transfer
+// (identity block or one quadrant of a coarse block).
+template <int BX, int BY>
+__device__ inline int owner_of(int cell, const int* __restrict__ xfer) {
+  constexpr int NB = BX * BY;
+  const int blk = cell / NB, loc = cell - blk * NB;
+  const int cb = xfer[2 * blk], mode = xfer[2 * blk + 1];
+  if (mode < 0) return cb * NB + loc;
+  const int i = loc % BX, j = loc / BX;
+  const int ci = (mode & 1) * (BX / 2) + (i >> 1);
+  const int cj = (mode >> 1) * (BY / 2) + (j >> 1);
+  return cb * NB + cj * BX + ci;
+}
+
+// Value fetchers for the tile: a plain array, or the injected coarse value.
+template <typename T>
+struct ArrayFetch {
+  const T* __restrict__ v;
+  __device__ inline T operator()(int cell) const { return v[cell]; }
+};
+
+template <typename T, int BX, int BY>
+struct InjectFetch {
+  const T* __restrict__ xc;
+  const int* __restrict__ xfer;
+  __device__ inline T operator()(int cell) const { return xc[owner_of<BX, BY>(cell, xfer)]; }
+};
+
+// Shared-memory tile plus irregular-row scratch.
+template <typename T, int BX, int BY>
+struct Tile {
+  static constexpr int NB = BX * BY;
+  static constexpr int W = BX + 2;
+  T v[(BX + 2) * (BY + 2)];
+  T irr_ax[NB];
+  int irr_row[NB];
+  T red[16];
+};
+
+// Load the block's values and the one-cell halo, evaluate the irregular
+// rows (sum_j A_cj v_j over their CSR entries), and leave (A v)_c and the
+// diagonal D_c for this thread's cell in `ax`, `d`. `absmode` evaluates the
+// factorized floor bound sum_j B_cj |v_j| instead.
+template <typename T, int BX, int BY, bool ABS, class Fetch>
+__device__ inline void tile_apply(Tile<T, BX, BY>& S, const LevelView<T>& L, int b, Fetch fetch, T& vc,
+                                  T& ax, T& d) {
+  constexpr int NB = BX * BY, W = BX + 2;
+  const int t = threadIdx.x, i = t % BX, j = t / BX;
+  const size_t base = size_t(b) * NB;
+  T own = fetch(int(base) + t);
+  if (ABS) own = fabs(own);
+  S.v[(j + 1) * W + (i + 1)] = own;
+  S.irr_row[t] = -1;
+  if (t < BY) {
+    const int nb = L.nbr[4 * b + 0];
+    T h = nb >= 0 ? fetch(nb * NB + t * BX + (BX - 1)) : T(0);
+    S.v[(t + 1) * W + 0] = ABS ? fabs(h) : h;
+  } else if (t < 2 * BY) {
+    const int jj = t - BY, nb = L.nbr[4 * b + 1];
+    T h = nb >= 0 ? fetch(nb * NB + jj * BX) : T(0);
+    S.v[(jj + 1) * W + BX + 1] = ABS ? fabs(h) : h;
+  } else if (t < 2 * BY + BX) {
+    const int ii = t - 2 * BY, nb = L.nbr[4 * b + 2];
+    T h = nb >= 0 ? fetch(nb * NB + (BY - 1) * BX + ii) : T(0);
+    S.v[ii + 1] = ABS ? fabs(h) : h;
+  } else if (t < 2 * BY + 2 * BX) {
+    const int ii = t - 2 * BY - BX, nb = L.nbr[4 * b + 3];
+    T h = nb >= 0 ? fetch(nb * NB + ii) : T(0);
+    S.v[(BY + 1) * W + ii + 1] = ABS ? fabs(h) : h;
+  }
+  __syncthreads();
+  const int r0 = L.blk_irr[b], r1 = L.blk_irr[b + 1];
+  for (int r = r0 + t; r < r1; r += NB) {
+    const int loc = L.irr_local[r];
+    T s = 0;
+    const T* vals = ABS ? L.irr_bound : L.irr_val;
+    for (int e = L.irr_ptr[r]; e < L.irr_ptr[r + 1]; ++e) {
+      T xv = fetch(L.irr_col[e]);
+      if (ABS) xv = fabs(xv);
+      s += vals[e] * xv;
+    }
+    S.irr_ax[loc] = s;
+    S.irr_row[loc] = r;
+  }
+  __syncthreads();
+  vc = S.v[(j + 1) * W + (i + 1)];
+  const int r = S.irr_row[t];
+  if (r >= 0) {
+    ax = S.irr_ax[t];
+    d = L.irr_diag[r];
+  } else {
+    const T* cx = L.cx + size_t(b) * (BX + 1) * BY;
+    const T* cy = L.cy + size_t(b) * BX * (BY + 1);
+    const T cw = cx[j * (BX + 1) + i], ce = cx[j * (BX + 1) + i + 1];
+    const T cs = cy[j * BX + i], cn = cy[(j + 1) * BX + i];
+    d = (cw + ce) + (cs + cn);
+    const T vw = S.v[(j + 1) * W + i], ve = S.v[(j + 1) * W + i + 2];
+    const T vs = S.v[j * W + i + 1], vn = S.v[(j + 2) * W + i + 1];
+    if (ABS)
+      ax = d * vc + ((cw * vw + ce * ve) + (cs * vs + cn * vn));
+    else
+      ax = d * vc - ((cw * vw + ce * ve) + (cs * vs + cn * vn));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// multigrid level kernels
+
+// Damped Jacobi sweep x_out = x_in + w D^-1 (b - A x_in). With STATS (finest
+// level, last sweep of the V-cycle) the block also emits partial sums of
+// V z, r z and r for the Krylov update, where z = x_out and r is the FP64
+// outer residual.
+template <typename T, int BX, int BY, bool FINE, bool STATS>
+__global__ void __launch_bounds__(BX* BY) k_sweep(LevelView<T> L, const T* __restrict__ x_in, T* __restrict__ x_out,
+                                                 const T* __restrict__ rhs, double omega, const int* __restrict__ done,
+                                                 const double* __restrict__ r64, double* __restrict__ part) {
+  if (done && *done) return;
+  __shared__ Tile<T, BX, BY> S;
+  const int b = blockIdx.x;
+  T vc, ax, d;
+  tile_apply<T, BX, BY, false>(S, L, b, ArrayFetch<T>{x_in}, vc, ax, d);
+  const size_t c = size_t(b) * (BX * BY) + threadIdx.x;
+  const T z = vc + T(omega) * (rhs[c] - ax) / d;
+  x_out[c] = z;
+  if constexpr (STATS) {
+    __shared__ double red[16];
+    const double zd = double(z), rd = r64[c];
+    const double s_z = block_sum<double, BX * BY>(zd, red);
+    const double s_rz = block_sum<double, BX * BY>(rd * zd, red);
+    const double s_r = block_sum<double, BX * BY>(rd, red);
+    if (threadIdx.x == 0) {
+      const int nb = gridDim.x;
+      part[b] = s_z * L.volume[b];
+      part[nb + b] = s_rz;
+      part[2 * nb + b] = s_r;
+    }
+  }
+}
+
+// u = w_P D^-1 (b - A x): the scaled residual entering the smoothed restriction.
+template <typename T, int BX, int BY, bool FINE>
+__global__ void __launch_bounds__(BX* BY) k_residual(LevelView<T> L, const T* __restrict__ x, const T* __restrict__ rhs,
+                                                    T* __restrict__ u, double omega_p, const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ Tile<T, BX, BY> S;
+  const int b = blockIdx.x;
+  T vc, ax, d;
+  tile_apply<T, BX, BY, false>(S, L, b, ArrayFetch<T>{x}, vc, ax, d);
+  const size_t c = size_t(b) * (BX * BY) + threadIdx.x;
+  u[c] = T(omega_p) * (rhs[c] - ax) / d;
+}
+
+// Smoothed restriction b_c = P_t' (r - w_P S D^-1 r) with u = w_P D^-1 r:
+// w = D u / w_P - S u on the fine block, summed over the tentative owner.
+// Also starts the coarse smoother from zero: x_c = w_c D_c^-1 b_c.
+template <typename T, int BX, int BY, bool FINE>
+__global__ void __launch_bounds__(BX* BY) k_restrict(LevelView<T> L, const T* __restrict__ u, double omega_p,
+                                                    T* __restrict__ bc, T* __restrict__ xc,
+                                                    const T* __restrict__ dinv_c, double omega_c, int zero_start,
+                                                    const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ Tile<T, BX, BY> S;
+  __shared__ T wv[BX * BY];
+  const int b = blockIdx.x, t = threadIdx.x;
+  T vc, ax, d;
+  tile_apply<T, BX, BY, false>(S, L, b, ArrayFetch<T>{u}, vc, ax, d);
+  const T w = d * vc / T(omega_p) - ax;
+  const int cb = L.xfer[2 * b], mode = L.xfer[2 * b + 1];
+  constexpr int NB = BX * BY;
+  if (mode < 0) {
+    const int cc = cb * NB + t;
+    bc[cc] = w;
+    xc[cc] = zero_start ? T(omega_c) * dinv_c[cc] * w : T(0);
+    return;
+  }
+  wv[t] = w;
+  __syncthreads();
+  const int i = t % BX, j = t / BX;
+  if (i < BX / 2 && j < BY / 2) {
+    const T s = (wv[(2 * j) * BX + 2 * i] + wv[(2 * j) * BX + 2 * i + 1]) +
+                (wv[(2 * j + 1) * BX + 2 * i] + wv[(2 * j + 1) * BX + 2 * i + 1]);
+    const int ci = (mode & 1) * (BX / 2) + i, cj = (mode >> 1) * (BY / 2) + j;
+    const int cc = cb * NB + cj * BX + ci;
+    bc[cc] = s;
+    xc[cc] = zero_start ? T(omega_c) * dinv_c[cc] * s : T(0);
+  }
+}
+
+// Smoothed prolongation and correction: t = P_t x_c, x += t - w_P D^-1 S t.
+template <typename T, int BX, int BY, bool FINE>
+__global__ void __launch_bounds__(BX* BY) k_prolong(LevelView<T> L, T* __restrict__ x, const T* __restrict__ xc,
+                                                   double omega_p, const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ Tile<T, BX, BY> S;
+  const int b = blockIdx.x;
+  T vc, ax, d;
+  tile_apply<T, BX, BY, false>(S, L, b, InjectFetch<T, BX, BY>{xc, L.xfer}, vc, ax, d);
+  const size_t c = size_t(b) * (BX * BY) + threadIdx.x;
+  x[c] = x[c] + (vc - T(omega_p) * ax / d);
+}
+
+// Coarsest level: x = inv * b with the dense inverse of the augmented
+// (volume-constrained) operator, column-major, evaluated in FP64.
+template <typename T>
+__global__ void k_coarse_solve(int n, const double* __restrict__ inv, const T* __restrict__ b, T* __restrict__ x,
+                               const int* __restrict__ done) {
+  if (done && *done) return;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double s = 0;
+  for (int c = 0; c < n; ++c) s += inv[size_t(c) * n + r] * double(b[c]);
+  x[r] = T(s);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 outer Krylov kernels
+
+// y = A p (FP64, finest level) with per-block partial p.y.
+template <int BX, int BY>
+__global__ void __launch_bounds__(BX* BY) k_matvec_dot(LevelView<double> L, const double* __restrict__ p,
+                                                      double* __restrict__ y, double* __restrict__ part,
+                                                      const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ Tile<double, BX, BY> S;
+  __shared__ double red[16];
+  const int b = blockIdx.x;
+  double vc, ax, d;
+  tile_apply<double, BX, BY, false>(S, L, b, ArrayFetch<double>{p}, vc, ax, d);
+  const size_t c = size_t(b) * (BX * BY) + threadIdx.x;
+  y[c] = ax;
+  if (part) {
+    const double s = block_sum<double, BX * BY>(vc * ax, red);
+    if (threadIdx.x == 0) part[b] = s;
+  }
+}
+
+// Factorized round-off bound sum_j B_cj |x_j| per cell, and the per-block
+// maximum of max(bound/V, |b|/V).
+template <int BX, int BY>
+__global__ void __launch_bounds__(BX* BY) k_floor_bound(LevelView<double> L, const double* __restrict__ x,
+                                                       const double* __restrict__ rhs, double* __restrict__ part) {
+  __shared__ Tile<double, BX, BY> S;
+  __shared__ double red[16];
+  const int b = blockIdx.x;
+  double vc, ax, d;
+  tile_apply<double, BX, BY, true>(S, L, b, ArrayFetch<double>{x}, vc, ax, d);
+  const size_t c = size_t(b) * (BX * BY) + threadIdx.x;
+  const double V = L.volume[b];
+  double q = fmax(ax / V, fabs(rhs[c]) / V);
+  if (q != q) q = __longlong_as_double(0x7ff8000000000000LL);
+  const double m = block_max<double, BX * BY>(q, red);
+  if (threadIdx.x == 0) part[b] = m;
+}
+
+// Single-workgroup reduction of `n` partials at `part` (sum or max) into out.
+__global__ void k_reduce(int n, int nsets, const double* __restrict__ part, double* __restrict__ out, int is_max) {
+  __shared__ double red[16];
+  for (int s = 0; s < nsets; ++s) {
+    double v = 0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+      const double w = part[size_t(s) * n + q];
+      v = is_max ? fmax(v, w) : v + w;
+    }
+    v = is_max ? block_max<double, 1024>(v, red) : block_sum<double, 1024>(v, red);
+    if (threadIdx.x == 0) out[s] = v;
+    __syncthreads();
+  }
+}
+
+// Krylov scalar step 1: pAp and the step length (Krylov.jl cg semantics).
+__global__ void k_cg_alpha(int nb, const double* __restrict__ part, CgState* st) {
+  __shared__ double red[16];
+  if (st->done) return;
+  double v = 0;
+  for (int q = threadIdx.x; q < nb; q += blockDim.x) v += part[q];
+  v = block_sum<double, 1024>(v, red);
+  if (threadIdx.x == 0) {
+    const double pAp = v;
+    st->pAp = pAp;
+    const double eps = 2.220446049250313e-16;
+    if (pAp <= eps * st->pnorm2 && fabs(pAp) <= eps * st->pnorm2) {
+      st->done = 1;
+      st->reason = 3;  // zero curvature
+    } else {
+      st->alpha = st->gamma / pAp;
+    }
+  }
+}
+
+// x += alpha p; r -= alpha Ap; emits the strong residual maximum partials
+// and the V-cycle input on the finest level (b0 = r, x0 = w D^-1 r).
+template <typename TV>
+__global__ void k_cg_update_xr(int n, int nb_cells, const double* __restrict__ p, const double* __restrict__ Ap,
+                               double* __restrict__ x, double* __restrict__ r, const double* __restrict__ volume,
+                               TV* __restrict__ vb, TV* __restrict__ vx, const TV* __restrict__ dinv, double omega,
+                               double* __restrict__ part, CgState* st, int init) {
+  __shared__ double red[16];
+  if (st->done) return;
+  const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  double q = 0;
+  if (c < size_t(n)) {
+    double rv;
+    if (init) {
+      rv = r[c] - Ap[c];  // r holds b on entry; Ap holds A x0
+      r[c] = rv;
+    } else {
+      const double a = st->alpha;
+      x[c] += a * p[c];
+      rv = r[c] - a * Ap[c];
+      r[c] = rv;
+    }
+    const TV rt = TV(rv);
+    vb[c] = rt;
+    vx[c] = TV(omega) * dinv[c] * rt;
+    q = fabs(rv) / volume[c / nb_cells];
+    if (q != q) q = __longlong_as_double(0x7ff8000000000000LL);
+  }
+  const double m = block_max<double, 256>(q, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+// Krylov scalar step 2: z mean removal, gamma, beta and the stopping tests.
+__global__ void k_cg_beta(int nb, int nmax, const double* __restrict__ part, const double* __restrict__ maxpart,
+                          double total_volume, CgState* st, int init) {
+  __shared__ double red[16];
+  if (st->done) return;
+  double sz = 0, srz = 0, sr = 0, mx = 0;
+  for (int q = threadIdx.x; q < nb; q += blockDim.x) {
+    sz += part[q];
+    srz += part[nb + q];
+    sr += part[2 * nb + q];
+  }
+  for (int q = threadIdx.x; q < nmax; q += blockDim.x) mx = fmax(mx, maxpart[q]);
+  sz = block_sum<double, 1024>(sz, red);
+  srz = block_sum<double, 1024>(srz, red);
+  sr = block_sum<double, 1024>(sr, red);
+  mx = block_max<double, 1024>(mx, red);
+  if (threadIdx.x != 0) return;
+  const double m = sz / total_volume;
+  const double g = srz - m * sr;
+  st->zmean = m;
+  st->strong = mx;
+  if (init) {
+    st->gamma = g;
+    st->pnorm2 = g;
+    st->rnorm = sqrt(g);
+    st->beta = 0;
+    st->iter = 0;
+    if (g < 0) {
+      st->done = 1;
+      st->reason = 5;  // indefinite preconditioner
+    } else if (g == 0) {
+      st->done = 1;
+      st->reason = 1;
+    }
+    return;
+  }
+  const double rn = sqrt(g);
+  st->rnorm = rn;
+  const bool solved = rn + 1.0 <= 1.0;
+  if (!solved) {
+    const double beta = g / st->gamma;
+    st->pnorm2 = g + beta * beta * st->pnorm2;
+    st->gamma = g;
+    st->beta = beta;
+  }
+  st->iter += 1;
+  const bool tired = st->iter >= st->itmax;
+  const bool user = mx <= st->target;
+  if (solved || tired || user) {
+    st->done = 1;
+    st->reason = solved ? 1 : (user ? 2 : 4);
+  }
+}
+
+// p = (z - zmean) + beta p
+template <typename TV>
+__global__ void k_cg_update_p(int n, const TV* __restrict__ z, double* __restrict__ p, const CgState* st,
+                              const int* __restrict__ done) {
+  if (*done) return;
+  const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= size_t(n)) return;
+  p[c] = (double(z[c]) - st->zmean) + st->beta * p[c];
+}
+
+// Partial sums of V z, r z and r per mesh block (used when the V-cycle does
+// not end with a finest-level sweep).
+template <typename T, int BX, int BY>
+__global__ void __launch_bounds__(BX* BY) k_zstats(const T* __restrict__ z, const double* __restrict__ r64,
+                                                  const double* __restrict__ volume, double* __restrict__ part,
+                                                  const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ double red[16];
+  const int b = blockIdx.x;
+  const size_t c = size_t(b) * (BX * BY) + threadIdx.x;
+  const double zd = double(z[c]), rd = r64[c];
+  const double s_z = block_sum<double, BX * BY>(zd, red);
+  const double s_rz = block_sum<double, BX * BY>(rd * zd, red);
+  const double s_r = block_sum<double, BX * BY>(rd, red);
+  if (threadIdx.x == 0) {
+    const int nb = gridDim.x;
+    part[b] = s_z * volume[b];
+    part[nb + b] = s_rz;
+    part[2 * nb + b] = s_r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// elementwise / face kernels
+
+__global__ void k_axpby(int n, double a, const double* __restrict__ x, double b, double* __restrict__ y) {
+  const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c < size_t(n)) y[c] = a * x[c] + b * y[c];
+}
+
+// r = b - y with per-block strong maxima |r|/V
+__global__ void k_residual_strong(int n, int nb_cells, const double* __restrict__ b, const double* __restrict__ y,
+                                  double* __restrict__ r, const double* __restrict__ volume,
+                                  double* __restrict__ part) {
+  __shared__ double red[16];
+  const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  double q = 0, s2 = 0;
+  if (c < size_t(n)) {
+    const double rv = b[c] - y[c];
+    r[c] = rv;
+    const double V = volume[c / nb_cells];
+    q = fabs(rv) / V;
+    if (q != q) q = __longlong_as_double(0x7ff8000000000000LL);
+    s2 = V * (rv / V) * (rv / V);
+  }
+  const double m = block_max<double, 256>(q, red);
+  const double s = block_sum<double, 256>(s2, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = m;
+    part[gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// per-block sums of V x (for the volume-weighted mean) and of x
+__global__ void k_weighted_sums(int n, int nb_cells, const double* __restrict__ x, const double* __restrict__ volume,
+                                double* __restrict__ part) {
+  __shared__ double red[16];
+  const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  double v = 0, a = 0, s = 0;
+  if (c < size_t(n)) {
+    v = volume[c / nb_cells] * x[c];
+    a = fabs(x[c]);
+    s = x[c];
+  }
+  v = block_sum<double, 256>(v, red);
+  a = block_sum<double, 256>(a, red);
+  s = block_sum<double, 256>(s, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = v;
+    part[gridDim.x + blockIdx.x] = a;
+    part[2 * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// x -= shift (scalar from device memory), optionally scaled by the cell volume
+__global__ void k_shift(int n, int nb_cells, double* __restrict__ x, const double* __restrict__ shift, double scale,
+                        const double* __restrict__ volume, int by_volume) {
+  const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= size_t(n)) return;
+  const double s = *shift * scale;
+  x[c] -= by_volume ? volume[c / nb_cells] * s : s;
+}
+
+__global__ void k_scale_by_volume(int n, int nb_cells, const double* __restrict__ in, double* __restrict__ out,
+                                  const double* __restrict__ volume, int divide) {
+  const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= size_t(n)) return;
+  const double V = volume[c / nb_cells];
+  out[c] = divide ? in[c] / V : in[c] * V;
+}
+
+// K y on face vectors: diagonal plus off-diagonal CSR (symmetric)
+__global__ void k_metric_apply(int nf, const double* __restrict__ kdiag, const int* __restrict__ optr,
+                               const int* __restrict__ oface, const double* __restrict__ oval,
+                               const double* __restrict__ in, double* __restrict__ out) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  double s = kdiag[f] * in[f];
+  for (int q = optr[f]; q < optr[f + 1]; ++q) s += oval[q] * in[oface[q]];
+  out[f] = s;
+}
+
+// face jump sqrt(k) |f| (p+ - p-), or sqrt(k) * j when p == nullptr
+__global__ void k_face_jump(int nf, const int* __restrict__ fm, const int* __restrict__ fp,
+                            const double* __restrict__ area, const double* __restrict__ sqk,
+                            const double* __restrict__ p, const double* __restrict__ jumps, double* __restrict__ out) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  out[f] = p ? sqk[f] * (area[f] * (p[fp[f]] - p[fm[f]])) : sqk[f] * jumps[f];
+}
+
+__global__ void k_face_scale(int nf, const double* __restrict__ sqk, const double* __restrict__ in,
+                             double* __restrict__ out) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f < nf) out[f] = sqk[f] * in[f];
+}
+
+// mixture face coefficient kappa_f = 1 / (rho_l c_f + rho_g (1 - c_f))
+__global__ void k_mixture(int nf, int nb_cells, const int* __restrict__ fm, const int* __restrict__ fp,
+                          const signed char* __restrict__ axis, const double* __restrict__ spacing,
+                          const double* __restrict__ c, double rho_l, double rho_g, double* __restrict__ kappa) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const int m = fm[f], p = fp[f], a = axis[f];
+  const double hm = spacing[2 * (m / nb_cells) + a], hp = spacing[2 * (p / nb_cells) + a];
+  double cf = (hm * c[m] + hp * c[p]) / (hm + hp);
+  cf = fmin(fmax(cf, 0.0), 1.0);
+  kappa[f] = 1.0 / (rho_l * cf + rho_g * (1 - cf));
+}
+
+// CGFM (or CSF) surface-tension jump sigma kappa_f [H] |f|
+__global__ void k_surface_jump(int nf, const int* __restrict__ fm, const int* __restrict__ fp,
+                               const double* __restrict__ area, const double* __restrict__ c,
+                               const double* __restrict__ curv, double sigma, int csf, double* __restrict__ jumps) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const double cm = c[fm[f]], cp = c[fp[f]];
+  const double jump = csf ? cp - cm : double(cp > 0.5) - double(cm > 0.5);
+  const double k = curv[f];
+  jumps[f] = (sigma == 0 || jump == 0 || k != k) ? 0.0 : sigma * k * jump * area[f];
+}
+
+// divergence: gather of oriented face fluxes in face order, / V
+__global__ void k_divergence(int n, int nb_cells, const int* __restrict__ cptr, const int* __restrict__ cface,
+                             const signed char* __restrict__ csign, const double* __restrict__ area,
+                             const double* __restrict__ u, const int* __restrict__ bptr, const int* __restrict__ bface,
+                             const signed char* __restrict__ bside, const double* __restrict__ barea,
+                             const double* __restrict__ ub, const double* __restrict__ volume,
+                             double* __restrict__ div) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const double V = volume[c / nb_cells];
+  double s = 0;
+  for (int q = cptr[c]; q < cptr[c + 1]; ++q) {
+    const int f = cface[q];
+    const double flux = u[f] * area[f];
+    s += csign[q] < 0 ? flux / V : -flux / V;
+  }
+  if (ub)
+    for (int q = bptr[c]; q < bptr[c + 1]; ++q) {
+      const int f = bface[q];
+      s += (double(bside[f]) * ub[f] * barea[f]) / V;
+    }
+  div[c] = s;
+}
+
+// adjoint coefficient gradient -[y_l (K y_p) + y_p (K y_l)] / (2 kappa)
+__global__ void k_adjoint_face(int nf, const double* __restrict__ yl, const double* __restrict__ kyp,
+                               const double* __restrict__ yp, const double* __restrict__ kyl,
+                               const double* __restrict__ kappa, double* __restrict__ g) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  g[f] = kappa[f] > 0 ? -(yl[f] * kyp[f] + yp[f] * kyl[f]) / (2 * kappa[f]) : 0.0;
+}
+
+__global__ void k_cast_d2f(int n, const double* __restrict__ in, float* __restrict__ out) {
+  const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c < size_t(n)) out[c] = float(in[c]);
+}
+
+__global__ void k_cast_f2d(int n, const float* __restrict__ in, double* __restrict__ out) {
+  const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c < size_t(n)) out[c] = double(in[c]);
+}
+
+__global__ void k_copy(int n, const double* __restrict__ in, double* __restrict__ out) {
+  const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c < size_t(n)) out[c] = in[c];
+}
+
+__global__ void k_fill(int n, double v, double* __restrict__ out) {
+  const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c < size_t(n)) out[c] = v;
+}
+
+__global__ void k_fill_int(int n, int v, int* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) out[c] = v;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+
+static inline int cdiv(long a, long b) { return int((a + b - 1) / b); }
+
+template <typename T, int BX, int BY>
+static void launch_sweep_t(hipStream_t s, const LevelView<T>& L, const T* xi, T* xo, const T* b, double w,
+                           const int* done, bool fine, const double* r64, double* part) {
+  const int nb = L.nblocks;
+  if (part) {
+    hipLaunchKernelGGL((k_sweep<T, BX, BY, true, true>), dim3(nb), dim3(BX * BY), 0, s, L, xi, xo, b, w, done, r64,
+                       part);
+  } else if (fine) {
+    hipLaunchKernelGGL((k_sweep<T, BX, BY, true, false>), dim3(nb), dim3(BX * BY), 0, s, L, xi, xo, b, w, done,
+                       nullptr, nullptr);
+  } else {
+    hipLaunchKernelGGL((k_sweep<T, BX, BY, false, false>), dim3(nb), dim3(BX * BY), 0, s, L, xi, xo, b, w, done,
+                       nullptr, nullptr);
+  }
+}
+
+#define TPAMR_DISPATCH(BXV, CALL)                                  \
+  switch (BXV) {                                                   \
+    case 4: { constexpr int BX = 4, BY = 4; CALL; } break;         \
+    case 8: { constexpr int BX = 8, BY = 8; CALL; } break;         \
+    case 16: { constexpr int BX = 16, BY = 16; CALL; } break;      \
+    case 32: { constexpr int BX = 32, BY = 32; CALL; } break;      \
+    default: break;                                                \
+  }
+
+bool supported_block(int bx, int by) { return bx == by && (bx == 4 || bx == 8 || bx == 16 || bx == 32); }
+
+template <typename T>
+void launch_sweep(hipStream_t s, const LevelView<T>& L, const T* xi, T* xo, const T* b, double w, const int* done,
+                  bool fine, const double* r64, double* part) {
+  TPAMR_DISPATCH(L.bx, (launch_sweep_t<T, BX, BY>(s, L, xi, xo, b, w, done, fine, r64, part)));
+}
+
+template <typename T>
+void launch_residual(hipStream_t s, const LevelView<T>& L, const T* x, const T* b, T* u, double wp, const int* done,
+                     bool fine) {
+  TPAMR_DISPATCH(L.bx, ({
+                   if (fine)
+                     hipLaunchKernelGGL((k_residual<T, BX, BY, true>), dim3(L.nblocks), dim3(BX * BY), 0, s, L, x, b,
+                                        u, wp, done);
+                   else
+                     hipLaunchKernelGGL((k_residual<T, BX, BY, false>), dim3(L.nblocks), dim3(BX * BY), 0, s, L, x,
+                                        b, u, wp, done);
+                 }));
+}
+
+template <typename T>
+void launch_restrict(hipStream_t s, const LevelView<T>& L, const T* u, double wp, T* bc, T* xc, const T* dinv_c,
+                     double wc, int zero_start, const int* done, bool fine) {
+  TPAMR_DISPATCH(L.bx, ({
+                   if (fine)
+                     hipLaunchKernelGGL((k_restrict<T, BX, BY, true>), dim3(L.nblocks), dim3(BX * BY), 0, s, L, u, wp,
+                                        bc, xc, dinv_c, wc, zero_start, done);
+                   else
+                     hipLaunchKernelGGL((k_restrict<T, BX, BY, false>), dim3(L.nblocks), dim3(BX * BY), 0, s, L, u,
+                                        wp, bc, xc, dinv_c, wc, zero_start, done);
+                 }));
+}
+
+template <typename T>
+void launch_prolong(hipStream_t s, const LevelView<T>& L, T* x, const T* xc, double wp, const int* done, bool fine) {
+  TPAMR_DISPATCH(L.bx, ({
+                   if (fine)
+                     hipLaunchKernelGGL((k_prolong<T, BX, BY, true>), dim3(L.nblocks), dim3(BX * BY), 0, s, L, x, xc,
+                                        wp, done);
+                   else
+                     hipLaunchKernelGGL((k_prolong<T, BX, BY, false>), dim3(L.nblocks), dim3(BX * BY), 0, s, L, x, xc,
+                                        wp, done);
+                 }));
+}
+
+template <typename T>
+void launch_coarse(hipStream_t s, int n, const double* inv, const T* b, T* x, const int* done) {
+  hipLaunchKernelGGL((k_coarse_solve<T>), dim3(cdiv(n, 256)), dim3(256), 0, s, n, inv, b, x, done);
+}
+
+void launch_matvec_dot(hipStream_t s, const LevelView<double>& L, const double* p, double* y, double* part,
+                       const int* done) {
+  TPAMR_DISPATCH(L.bx, ({ hipLaunchKernelGGL((k_matvec_dot<BX, BY>), dim3(L.nblocks), dim3(BX * BY), 0, s, L, p, y,
+                                           part, done); }));
+}
+
+void launch_floor_bound(hipStream_t s, const LevelView<double>& L, const double* x, const double* b, double* part) {
+  TPAMR_DISPATCH(L.bx, ({ hipLaunchKernelGGL((k_floor_bound<BX, BY>), dim3(L.nblocks), dim3(BX * BY), 0, s, L, x, b,
+                                           part); }));
+}
+
+void launch_reduce(hipStream_t s, int n, int nsets, const double* part, double* out, bool is_max) {
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, n, nsets, part, out, int(is_max));
+}
+
+void launch_cg_alpha(hipStream_t s, int nb, const double* part, CgState* st) {
+  hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(1024), 0, s, nb, part, st);
+}
+
+template <typename TV>
+void launch_cg_update_xr(hipStream_t s, int n, int nbc, const double* p, const double* Ap, double* x, double* r,
+                         const double* vol, TV* vb, TV* vx, const TV* dinv, double omega, double* part, CgState* st,
+                         int init) {
+  hipLaunchKernelGGL((k_cg_update_xr<TV>), dim3(cdiv(n, 256)), dim3(256), 0, s, n, nbc, p, Ap, x, r, vol, vb, vx,
+                     dinv, omega, part, st, init);
+}
+
+void launch_cg_beta(hipStream_t s, int nb, int nmax, const double* part, const double* maxpart, double total_volume,
+                    CgState* st, int init) {
+  hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(1024), 0, s, nb, nmax, part, maxpart, total_volume, st, init);
+}
+
+template <typename TV>
+void launch_cg_update_p(hipStream_t s, int n, const TV* z, double* p, const CgState* st, const int* done) {
+  hipLaunchKernelGGL((k_cg_update_p<TV>), dim3(cdiv(n, 256)), dim3(256), 0, s, n, z, p, st, done);
+}
+
+void launch_residual_strong(hipStream_t s, int n, int nbc, const double* b, const double* y, double* r,
+                            const double* vol, double* part) {
+  hipLaunchKernelGGL(k_residual_strong, dim3(cdiv(n, 256)), dim3(256), 0, s, n, nbc, b, y, r, vol, part);
+}
+
+void launch_weighted_sums(hipStream_t s, int n, int nbc, const double* x, const double* vol, double* part) {
+  hipLaunchKernelGGL(k_weighted_sums, dim3(cdiv(n, 256)), dim3(256), 0, s, n, nbc, x, vol, part);
+}
+
+void launch_shift(hipStream_t s, int n, int nbc, double* x, const double* shift, double scale, const double* vol,
+                  bool by_volume) {
+  hipLaunchKernelGGL(k_shift, dim3(cdiv(n, 256)), dim3(256), 0, s, n, nbc, x, shift, scale, vol, int(by_volume));
+}
+
+void launch_scale_by_volume(hipStream_t s, int n, int nbc, const double* in, double* out, const double* vol,
+                            bool divide) {
+  hipLaunchKernelGGL(k_scale_by_volume, dim3(cdiv(n, 256)), dim3(256), 0, s, n, nbc, in, out, vol, int(divide));
+}
+
+void launch_metric_apply(hipStream_t s, int nf, const double* kd, const int* optr, const int* oface,
+                         const double* oval, const double* in, double* out) {
+  hipLaunchKernelGGL(k_metric_apply, dim3(cdiv(nf, 256)), dim3(256), 0, s, nf, kd, optr, oface, oval, in, out);
+}
+
+void launch_face_jump(hipStream_t s, int nf, const int* fm, const int* fp, const double* area, const double* sqk,
+                      const double* p, const double* jumps, double* out) {
+  hipLaunchKernelGGL(k_face_jump, dim3(cdiv(nf, 256)), dim3(256), 0, s, nf, fm, fp, area, sqk, p, jumps, out);
+}
+
+void launch_face_scale(hipStream_t s, int nf, const double* sqk, const double* in, double* out) {
+  hipLaunchKernelGGL(k_face_scale, dim3(cdiv(nf, 256)), dim3(256), 0, s, nf, sqk, in, out);
+}
+
+void launch_mixture(hipStream_t s, int nf, int nbc, const int* fm, const int* fp, const signed char* axis,
+                    const double* spacing, const double* c, double rl, double rg, double* kappa) {
+  hipLaunchKernelGGL(k_mixture, dim3(cdiv(nf, 256)), dim3(256), 0, s, nf, nbc, fm, fp, axis, spacing, c, rl, rg,
+                     kappa);
+}
+
+void launch_surface_jump(hipStream_t s, int nf, const int* fm, const int* fp, const double* area, const double* c,
+                         const double* curv, double sigma, int csf, double* jumps) {
+  hipLaunchKernelGGL(k_surface_jump, dim3(cdiv(nf, 256)), dim3(256), 0, s, nf, fm, fp, area, c, curv, sigma, csf,
+                     jumps);
+}
+
+void launch_divergence(hipStream_t s, int n, int nbc, const int* cptr, const int* cface, const signed char* csign,
+                       const double* area, const double* u, const int* bptr, const int* bface,
+                       const signed char* bside, const double* barea, const double* ub, const double* vol,
+                       double* div) {
+  hipLaunchKernelGGL(k_divergence, dim3(cdiv(n, 256)), dim3(256), 0, s, n, nbc, cptr, cface, csign, area, u, bptr,
+                     bface, bside, barea, ub, vol, div);
+}
+
+void launch_adjoint_face(hipStream_t s, int nf, const double* yl, const double* kyp, const double* yp,
+                         const double* kyl, const double* kappa, double* g) {
+  hipLaunchKernelGGL(k_adjoint_face, dim3(cdiv(nf, 256)), dim3(256), 0, s, nf, yl, kyp, yp, kyl, kappa, g);
+}
+
+template <typename T>
+void launch_zstats(hipStream_t s, int nblocks, int bx, const T* z, const double* r64, const double* vol, double* part,
+                   const int* done) {
+  TPAMR_DISPATCH(bx, ({ hipLaunchKernelGGL((k_zstats<T, BX, BY>), dim3(nblocks), dim3(BX * BY), 0, s, z, r64, vol,
+                                         part, done); }));
+}
+template void launch_zstats<float>(hipStream_t, int, int, const float*, const double*, const double*, double*,
+                                   const int*);
+template void launch_zstats<double>(hipStream_t, int, int, const double*, const double*, const double*, double*,
+                                    const int*);
+
+void launch_axpby(hipStream_t s, int n, double a, const double* x, double b, double* y) {
+  hipLaunchKernelGGL(k_axpby, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, x, b, y);
+}
+
+void launch_cast_d2f(hipStream_t s, int n, const double* in, float* out) {
+  hipLaunchKernelGGL(k_cast_d2f, dim3(cdiv(n, 256)), dim3(256), 0, s, n, in, out);
+}
+void launch_cast_f2d(hipStream_t s, int n, const float* in, double* out) {
+  hipLaunchKernelGGL(k_cast_f2d, dim3(cdiv(n, 256)), dim3(256), 0, s, n, in, out);
+}
+void launch_copy(hipStream_t s, int n, const double* in, double* out) {
+  hipLaunchKernelGGL(k_copy, dim3(cdiv(n, 256)), dim3(256), 0, s, n, in, out);
+}
+void launch_fill(hipStream_t s, int n, double v, double* out) {
+  hipLaunchKernelGGL(k_fill, dim3(cdiv(n, 256)), dim3(256), 0, s, n, v, out);
+}
+void launch_fill_int(hipStream_t s, int n, int v, int* out) {
+  hipLaunchKernelGGL(k_fill_int, dim3(cdiv(n, 256)), dim3(256), 0, s, n, v, out);
+}
+
+template void launch_sweep<float>(hipStream_t, const LevelView<float>&, const float*, float*, const float*, double,
+                                  const int*, bool, const double*, double*);
+template void launch_sweep<double>(hipStream_t, const LevelView<double>&, const double*, double*, const double*,
+                                   double, const int*, bool, const double*, double*);
+template void launch_residual<float>(hipStream_t, const LevelView<float>&, const float*, const float*, float*, double,
+                                     const int*, bool);
+template void launch_residual<double>(hipStream_t, const LevelView<double>&, const double*, const double*, double*,
+                                      double, const int*, bool);
+template void launch_restrict<float>(hipStream_t, const LevelView<float>&, const float*, double, float*, float*,
+                                     const float*, double, int, const int*, bool);
+template void launch_restrict<double>(hipStream_t, const LevelView<double>&, const double*, double, double*, double*,
+                                      const double*, double, int, const int*, bool);
+template void launch_prolong<float>(hipStream_t, const LevelView<float>&, float*, const float*, double, const int*,
+                                    bool);
+template void launch_prolong<double>(hipStream_t, const LevelView<double>&, double*, const double*, double,
+                                     const int*, bool);
+template void launch_coarse<float>(hipStream_t, int, const double*, const float*, float*, const int*);
+template void launch_coarse<double>(hipStream_t, int, const double*, const double*, double*, const int*);
+template void launch_cg_update_xr<float>(hipStream_t, int, int, const double*, const double*, double*, double*,
+                                         const double*, float*, float*, const float*, double, double*, CgState*, int);
+template void launch_cg_update_xr<double>(hipStream_t, int, int, const double*, const double*, double*, double*,
+                                          const double*, double*, double*, const double*, double, double*, CgState*,
+                                          int);
+template void launch_cg_update_p<float>(hipStream_t, int, const float*, double*, const CgState*, const int*);
+template void launch_cg_update_p<double>(hipStream_t, int, const double*, double*, const CgState*, const int*);
+
+}  // namespace tpamr

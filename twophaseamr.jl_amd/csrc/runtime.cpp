@@ -1,0 +1,1173 @@
+// Runtime behind the C ABI: device memory of the hierarchy, the V-cycle
+// launch sequence, the FP64 Krylov driver with the reference's stopping
+// rules, and the face-level two-phase / projection / adjoint operations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/tpamr.h"
+#include "hier.hpp"
+#include "kernels.hpp"
+
+using namespace tpamr;
+
+namespace {
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIPCHECK(x)                                                                                  \
+  do {                                                                                               \
+    hipError_t e_ = (x);                                                                             \
+    if (e_ != hipSuccess) throw HipError(std::string(#x) + ": " + hipGetErrorString(e_));           \
+  } while (0)
+
+const double EPS = std::numeric_limits<double>::epsilon();
+
+template <typename T>
+struct DevArray {
+  T* p = nullptr;
+  size_t n = 0;
+  DevArray() = default;
+  DevArray(const DevArray&) = delete;
+  DevArray& operator=(const DevArray&) = delete;
+  DevArray(DevArray&& o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DevArray& operator=(DevArray&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DevArray() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    n = count;
+    if (count) HIPCHECK(hipMalloc(&p, count * sizeof(T)));
+  }
+  template <typename S>
+  void upload(const std::vector<S>& v, hipStream_t s) {
+    alloc(v.size());
+    if (v.empty()) return;
+    if constexpr (std::is_same<S, T>::value) {
+      HIPCHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+      HIPCHECK(hipStreamSynchronize(s));
+    } else {
+      std::vector<T> tmp(v.begin(), v.end());
+      HIPCHECK(hipMemcpyAsync(p, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice, s));
+      HIPCHECK(hipStreamSynchronize(s));
+    }
+  }
+};
+
+struct DevTopo {
+  int nblocks = 0, ncells = 0;
+  DevArray<int> nbr, xfer, blk_irr, irr_local, irr_ptr, irr_col;
+  DevArray<double> volume;
+};
+
+template <typename T>
+struct DevLevel {
+  DevArray<T> cx, cy, irr_val, irr_bound, irr_diag, dinv;
+  DevArray<T> x, xs, b, u;
+  double omega = 1, omega_p = 0;
+  LevelView<T> view{};
+};
+
+}  // namespace
+
+struct tpamr_ctx {
+  int device = -1;
+  int precision = TPAMR_FP64;
+  hipStream_t stream = nullptr;
+  std::string err;
+  MultigridConfig cfg;
+  std::unique_ptr<Hierarchy> H;
+  std::unique_ptr<TreeMesh> mesh_pending;  // before the first coefficients
+  bool have_coeff = false;
+  // device data
+  std::vector<DevTopo> topo;
+  std::vector<DevLevel<double>> ld;  // FP64 levels (level 0 always, all in FP64 mode)
+  std::vector<DevLevel<float>> lf;   // FP32 levels (FP32 mode)
+  DevArray<double> coarse_inv;
+  DevArray<double> x, r, p, Ap, b, tmp, tmp2;
+  DevArray<double> part, maxpart, scal;
+  CgState* st = nullptr;      // device
+  CgState* st_host = nullptr; // pinned
+  CgState* st_host2 = nullptr;
+  // face data (finest level)
+  DevArray<int> fm, fp, kptr, kface, cptr, cface, bptr, bface;
+  DevArray<signed char> faxis, csign, bside;
+  DevArray<double> farea, kdiag, kval, kappa, sqk, spacing, barea;
+  DevArray<double> f1, f2, f3, f4;
+  // graph of one Krylov iteration
+  hipGraphExec_t graph = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evs[2] = {nullptr, nullptr};
+  const TreeMesh& mesh() const {
+    if (H) return *H->levels[0]->mesh;
+    if (mesh_pending) return *mesh_pending;
+    throw std::logic_error("no mesh: call tpamr_set_mesh first");
+  }
+};
+
+namespace {
+
+struct StateError : std::logic_error {
+  using std::logic_error::logic_error;
+};
+struct SizeError : std::invalid_argument {
+  using std::invalid_argument::invalid_argument;
+};
+struct IncompatibleError : std::invalid_argument {
+  using std::invalid_argument::invalid_argument;
+};
+
+template <class F>
+int guard(tpamr_ctx* ctx, F&& f) {
+  try {
+    f();
+    return TPAMR_OK;
+  } catch (const HipError& e) {
+    if (ctx) ctx->err = e.what();
+    return TPAMR_EDEVICE;
+  } catch (const StateError& e) {
+    if (ctx) ctx->err = e.what();
+    return TPAMR_ESTATE;
+  } catch (const SizeError& e) {
+    if (ctx) ctx->err = e.what();
+    return TPAMR_ESIZE;
+  } catch (const IncompatibleError& e) {
+    if (ctx) ctx->err = e.what();
+    return TPAMR_EINCOMPATIBLE;
+  } catch (const std::invalid_argument& e) {
+    if (ctx) ctx->err = e.what();
+    return TPAMR_EINVAL;
+  } catch (const std::exception& e) {
+    if (ctx) ctx->err = e.what();
+    return TPAMR_EINVAL;
+  }
+}
+
+void need_device(const tpamr_ctx* c) {
+  if (c->device < 0) throw StateError("host-only context: create the context on a HIP device to compute");
+}
+
+void need_solver(const tpamr_ctx* c) {
+  if (!c->H || !c->have_coeff) throw StateError("no operator: call tpamr_set_face_coefficients first");
+}
+
+void destroy_graph(tpamr_ctx* c) {
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+}
+
+template <typename T>
+void upload_level_values(tpamr_ctx* c, DevLevel<T>& D, const LevelData& L, const DevTopo& Tp, bool with_bound) {
+  hipStream_t s = c->stream;
+  D.cx.upload(L.cx, s);
+  D.cy.upload(L.cy, s);
+  D.irr_val.upload(L.irr_val, s);
+  if (with_bound) D.irr_bound.upload(L.irr_bound, s);
+  D.irr_diag.upload(L.irr_diag, s);
+  D.dinv.upload(L.dinv, s);
+  D.omega = L.omega;
+  D.omega_p = L.omega_p;
+  D.x.alloc(L.ncells);
+  D.xs.alloc(L.ncells);
+  D.b.alloc(L.ncells);
+  D.u.alloc(L.ncells);
+  LevelView<T>& v = D.view;
+  v.nblocks = L.nblocks;
+  v.ncells = L.ncells;
+  v.bx = L.mesh->bs[0];
+  v.by = L.mesh->bs[1];
+  v.cx = D.cx.p;
+  v.cy = D.cy.p;
+  v.irr_val = D.irr_val.p;
+  v.irr_bound = D.irr_bound.p;
+  v.irr_diag = D.irr_diag.p;
+  v.nbr = Tp.nbr.p;
+  v.xfer = Tp.xfer.p;
+  v.blk_irr = Tp.blk_irr.p;
+  v.irr_local = Tp.irr_local.p;
+  v.irr_ptr = Tp.irr_ptr.p;
+  v.irr_col = Tp.irr_col.p;
+  v.volume = Tp.volume.p;
+}
+
+// Upload topology (once per hierarchy) and values (every refresh).
+void upload_hierarchy(tpamr_ctx* c, bool topology) {
+  Hierarchy& H = *c->H;
+  hipStream_t s = c->stream;
+  const int nl = int(H.levels.size());
+  if (topology) {
+    c->topo.clear();
+    c->topo.resize(nl);
+    for (int l = 0; l < nl; ++l) {
+      const LevelData& L = *H.levels[l];
+      DevTopo& T = c->topo[l];
+      T.nblocks = L.nblocks;
+      T.ncells = L.ncells;
+      T.nbr.upload(L.nbr, s);
+      T.xfer.upload(L.xfer.empty() ? std::vector<int>(2 * L.nblocks, -1) : L.xfer, s);
+      T.blk_irr.upload(L.blk_irr, s);
+      T.irr_local.upload(L.irr_local, s);
+      T.irr_ptr.upload(L.irr_ptr, s);
+      T.irr_col.upload(L.irr_col, s);
+      T.volume.upload(L.volume, s);
+    }
+    c->ld.clear();
+    c->lf.clear();
+    c->ld.resize(c->precision == TPAMR_FP64 ? nl : 1);
+    if (c->precision == TPAMR_FP32_VCYCLE) c->lf.resize(nl);
+    // finest-level face data
+    const TreeMesh& M = *H.levels[0]->mesh;
+    const LevelData& L0 = *H.levels[0];
+    c->fm.upload(M.fminus, s);
+    c->fp.upload(M.fplus, s);
+    std::vector<signed char> ax(M.faxis.begin(), M.faxis.end());
+    c->faxis.upload(ax, s);
+    c->farea.upload(M.farea, s);
+    c->kdiag.upload(L0.K.diag, s);
+    c->kptr.upload(L0.K.off_ptr, s);
+    c->kface.upload(L0.K.off_face, s);
+    c->kval.upload(L0.K.off_val, s);
+    c->spacing.upload(M.bspacing, s);
+    // cell -> face incidence (face order) for the divergence gather
+    const int N = M.ncells(), F = M.nfaces();
+    std::vector<int> cnt(N + 1, 0);
+    for (int f = 0; f < F; ++f) {
+      cnt[M.fminus[f] + 1]++;
+      cnt[M.fplus[f] + 1]++;
+    }
+    for (int q = 0; q < N; ++q) cnt[q + 1] += cnt[q];
+    std::vector<int> cf(cnt[N]);
+    std::vector<signed char> cs(cnt[N]);
+    std::vector<int> next(cnt.begin(), cnt.end() - 1);
+    for (int f = 0; f < F; ++f) {
+      cf[next[M.fminus[f]]] = f;
+      cs[next[M.fminus[f]]++] = -1;
+      cf[next[M.fplus[f]]] = f;
+      cs[next[M.fplus[f]]++] = 1;
+    }
+    c->cptr.upload(cnt, s);
+    c->cface.upload(cf, s);
+    c->csign.upload(cs, s);
+    const int BF = int(M.bcell.size());
+    std::vector<int> bcnt(N + 1, 0);
+    for (int f = 0; f < BF; ++f) bcnt[M.bcell[f] + 1]++;
+    for (int q = 0; q < N; ++q) bcnt[q + 1] += bcnt[q];
+    std::vector<int> bf(std::max(BF, 1));
+    std::vector<int> bnext(bcnt.begin(), bcnt.end() - 1);
+    for (int f = 0; f < BF; ++f) bf[bnext[M.bcell[f]]++] = f;
+    c->bptr.upload(bcnt, s);
+    c->bface.upload(bf, s);
+    std::vector<signed char> bsd(M.bside.begin(), M.bside.end());
+    if (bsd.empty()) bsd.push_back(0);
+    c->bside.upload(bsd, s);
+    std::vector<double> ba(M.barea);
+    if (ba.empty()) ba.push_back(0);
+    c->barea.upload(ba, s);
+    // vectors and reduction scratch
+    c->x.alloc(N);
+    c->r.alloc(N);
+    c->p.alloc(N);
+    c->Ap.alloc(N);
+    c->b.alloc(N);
+    c->tmp.alloc(N);
+    c->tmp2.alloc(N);
+    c->f1.alloc(F);
+    c->f2.alloc(F);
+    c->f3.alloc(F);
+    c->f4.alloc(F);
+    const size_t nblk = size_t(L0.nblocks);
+    const size_t ncb = (size_t(N) + 255) / 256;
+    c->part.alloc(std::max(3 * nblk, 3 * ncb) + 16);
+    c->maxpart.alloc(std::max(nblk, ncb) + 16);
+    c->scal.alloc(16);
+    destroy_graph(c);
+  }
+  // kernel arguments of the captured iteration (weights, pointers) change
+  destroy_graph(c);
+  for (int l = 0; l < int(c->ld.size()); ++l) upload_level_values(c, c->ld[l], *H.levels[l], c->topo[l], l == 0);
+  for (int l = 0; l < int(c->lf.size()); ++l) upload_level_values(c, c->lf[l], *H.levels[l], c->topo[l], false);
+  c->coarse_inv.upload(H.coarse_inv, s);
+  c->kappa.upload(H.levels[0]->kappa, s);
+  std::vector<double> sq(H.levels[0]->kappa.size());
+  for (size_t f = 0; f < sq.size(); ++f) sq[f] = std::sqrt(H.levels[0]->kappa[f]);
+  c->sqk.upload(sq, s);
+}
+
+// ---------------------------------------------------------------------------
+// V-cycle launch sequence
+
+// Enqueue one V-cycle. On entry the finest level's b holds the residual and
+// its x holds the zero-start sweep (or zero). Returns the buffer holding z.
+template <typename T>
+T* enqueue_vcycle(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done, const double* r64, double* zpart) {
+  hipStream_t s = c->stream;
+  const int nl = int(lv.size());
+  const int pre = c->cfg.pre, post = c->cfg.post;
+  std::vector<T*> cur(nl), oth(nl);
+  for (int l = 0; l < nl; ++l) {
+    cur[l] = lv[l].x.p;
+    oth[l] = lv[l].xs.p;
+  }
+  if (nl == 1) {
+    launch_coarse<T>(s, c->H->coarse_n, c->coarse_inv.p, lv[0].b.p, lv[0].x.p, done);
+    if (zpart) launch_zstats<T>(s, lv[0].view.nblocks, lv[0].view.bx, lv[0].x.p, r64, lv[0].view.volume, zpart, done);
+    return lv[0].x.p;
+  }
+  for (int l = 0; l + 1 < nl; ++l) {
+    DevLevel<T>& L = lv[l];
+    for (int q = 1; q < pre; ++q) {
+      launch_sweep<T>(s, L.view, cur[l], oth[l], L.b.p, L.omega, done, l == 0, nullptr, nullptr);
+      std::swap(cur[l], oth[l]);
+    }
+    launch_residual<T>(s, L.view, cur[l], L.b.p, L.u.p, L.omega_p, done, l == 0);
+    DevLevel<T>& C = lv[l + 1];
+    const bool coarsest = l + 1 == nl - 1;
+    launch_restrict<T>(s, L.view, L.u.p, L.omega_p, C.b.p, C.x.p, C.dinv.p, C.omega, (!coarsest && pre > 0) ? 1 : 0,
+                       done, l == 0);
+  }
+  launch_coarse<T>(s, c->H->coarse_n, c->coarse_inv.p, lv[nl - 1].b.p, lv[nl - 1].x.p, done);
+  for (int l = nl - 2; l >= 0; --l) {
+    DevLevel<T>& L = lv[l];
+    launch_prolong<T>(s, L.view, cur[l], cur[l + 1], L.omega_p, done, l == 0);
+    const int sweeps = (l == 0 && zpart && post == 0) ? 1 : post;
+    for (int q = 0; q < sweeps; ++q) {
+      const bool last = l == 0 && q == sweeps - 1 && zpart;
+      const double w = (post == 0) ? 0.0 : L.omega;
+      launch_sweep<T>(s, L.view, cur[l], oth[l], L.b.p, w, done, l == 0, last ? r64 : nullptr,
+                      last ? zpart : nullptr);
+      std::swap(cur[l], oth[l]);
+    }
+  }
+  return cur[0];
+}
+
+// ---------------------------------------------------------------------------
+// FP64 helpers on the finest level
+
+LevelView<double>& fine64(tpamr_ctx* c) { return c->ld[0].view; }
+
+void matvec(tpamr_ctx* c, const double* x, double* y) {
+  launch_matvec_dot(c->stream, fine64(c), x, y, nullptr, nullptr);
+}
+
+double reduce_host(tpamr_ctx* c, int n, int nsets, const double* part, bool is_max, double* out_sets = nullptr) {
+  launch_reduce(c->stream, n, nsets, part, c->scal.p, is_max);
+  double h[16];
+  HIPCHECK(hipMemcpyAsync(h, c->scal.p, sizeof(double) * nsets, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  if (out_sets)
+    for (int q = 0; q < nsets; ++q) out_sets[q] = h[q];
+  return h[0];
+}
+
+int ncb(int n) { return (n + 255) / 256; }
+
+// true residual r = b - A x; returns (l2, linf) of the strong residual
+void true_residual(tpamr_ctx* c, double& l2, double& linf) {
+  const int N = c->topo[0].ncells, nb = c->H->levels[0]->mesh->nb();
+  matvec(c, c->x.p, c->Ap.p);
+  launch_residual_strong(c->stream, N, nb, c->b.p, c->Ap.p, c->r.p, c->topo[0].volume.p, c->part.p);
+  double sets[2];
+  reduce_host(c, ncb(N), 1, c->part.p, true, sets);
+  linf = sets[0];
+  double s2[1];
+  reduce_host(c, ncb(N), 1, c->part.p + ncb(N), false, s2);
+  l2 = std::sqrt(s2[0] / c->H->total_volume);
+}
+
+double roundoff_floor(tpamr_ctx* c) {
+  const int nblk = c->topo[0].nblocks;
+  launch_floor_bound(c->stream, fine64(c), c->x.p, c->b.p, c->part.p);
+  const double m = reduce_host(c, nblk, 1, c->part.p, true);
+  return 128 * EPS * std::max(1.0, m);
+}
+
+// subtract the volume-weighted mean (one connected, unanchored component)
+void remove_mean(tpamr_ctx* c, double* v) {
+  const int N = c->topo[0].ncells, nb = c->H->levels[0]->mesh->nb();
+  launch_weighted_sums(c->stream, N, nb, v, c->topo[0].volume.p, c->part.p);
+  launch_reduce(c->stream, ncb(N), 1, c->part.p, c->scal.p, false);
+  launch_shift(c->stream, N, nb, v, c->scal.p, 1.0 / c->H->total_volume, c->topo[0].volume.p, false);
+}
+
+double guarded_target(tpamr_ctx* c, double target, double initial, double current) {
+  if (!(std::isfinite(initial) && std::isfinite(current) && target < current && current < initial)) return target;
+  const double fl = roundoff_floor(c);
+  return (std::isfinite(fl) && fl < initial / 100) ? std::max(target, fl) : target;
+}
+
+// enqueue z = M r for the Krylov loop: V-cycle input from r, z stats partials
+template <typename T>
+T* enqueue_precondition(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done) {
+  return enqueue_vcycle<T>(c, lv, done, c->r.p, c->part.p);
+}
+
+template <typename T>
+void enqueue_iteration(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, bool init) {
+  hipStream_t s = c->stream;
+  const int N = c->topo[0].ncells, nb = c->H->levels[0]->mesh->nb(), nblk = c->topo[0].nblocks;
+  const int* done = &c->st->done;
+  DevLevel<T>& L0 = lv[0];
+  const double w0 = c->cfg.pre > 0 ? L0.omega : 0.0;
+  if (init) {
+    matvec(c, c->x.p, c->Ap.p);
+    launch_copy(s, N, c->b.p, c->r.p);
+    launch_cg_update_xr<T>(s, N, nb, nullptr, c->Ap.p, nullptr, c->r.p, c->topo[0].volume.p, L0.b.p, L0.x.p,
+                           L0.dinv.p, w0, c->maxpart.p, c->st, 1);
+  } else {
+    launch_matvec_dot(s, fine64(c), c->p.p, c->Ap.p, c->part.p, done);
+    launch_cg_alpha(s, nblk, c->part.p, c->st);
+    launch_cg_update_xr<T>(s, N, nb, c->p.p, c->Ap.p, c->x.p, c->r.p, c->topo[0].volume.p, L0.b.p, L0.x.p,
+                           L0.dinv.p, w0, c->maxpart.p, c->st, 0);
+  }
+  T* z = enqueue_precondition<T>(c, lv, done);
+  launch_cg_beta(s, nblk, ncb(N), c->part.p, c->maxpart.p, c->H->total_volume, c->st, init ? 1 : 0);
+  launch_cg_update_p<T>(s, N, z, c->p.p, c->st, done);
+}
+
+template <typename T>
+void build_graph(tpamr_ctx* c, std::vector<DevLevel<T>>& lv) {
+  destroy_graph(c);
+  hipGraph_t g;
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  enqueue_iteration<T>(c, lv, false);
+  HIPCHECK(hipStreamEndCapture(c->stream, &g));
+  HIPCHECK(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+  HIPCHECK(hipGraphDestroy(g));
+}
+
+// One run of Krylov CG (warm start from x) until the strong residual
+// callback, the iteration limit, or breakdown. Returns the iteration count.
+template <typename T>
+int run_cg(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double target, int itmax) {
+  hipStream_t s = c->stream;
+  const int N = c->topo[0].ncells;
+  CgState init{};
+  init.target = target;
+  init.itmax = itmax == 0 ? 2 * N : itmax;
+  *c->st_host = init;
+  HIPCHECK(hipMemcpyAsync(c->st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, s));
+  launch_fill(s, N, 0.0, c->p.p);
+  enqueue_iteration<T>(c, lv, true);
+  if (!c->graph) build_graph<T>(c, lv);
+  CgState* slot[2] = {c->st_host, c->st_host2};
+  int launched = 0;
+  for (;;) {
+    HIPCHECK(hipGraphLaunch(c->graph, s));
+    HIPCHECK(hipMemcpyAsync(slot[launched & 1], c->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipEventRecord(c->evs[launched & 1], s));
+    ++launched;
+    if (launched >= 2) {
+      HIPCHECK(hipEventSynchronize(c->evs[(launched - 2) & 1]));
+      if (slot[(launched - 2) & 1]->done) break;
+    }
+    if (launched > init.itmax + 2) break;
+  }
+  HIPCHECK(hipStreamSynchronize(s));
+  HIPCHECK(hipMemcpy(c->st_host, c->st, sizeof(CgState), hipMemcpyDeviceToHost));
+  return c->st_host->iter;
+}
+
+template <typename T>
+tpamr_stats solve_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double rtol, double atol, int max_cycles,
+                       int compat) {
+  hipStream_t s = c->stream;
+  const int N = c->topo[0].ncells, nb = c->H->levels[0]->mesh->nb();
+  const double* vol = c->topo[0].volume.p;
+  tpamr_stats st{};
+  HIPCHECK(hipEventRecord(c->ev0, s));
+  // integrated rhs
+  launch_scale_by_volume(s, N, nb, c->b.p, c->b.p, vol, false);
+  // compatible projection of the rhs (one unanchored component)
+  launch_weighted_sums(s, N, nb, c->b.p, vol, c->part.p);  // sums of V b (unused), |b|, b
+  double sums[3];
+  {
+    launch_reduce(s, ncb(N), 3, c->part.p, c->scal.p, false);
+    HIPCHECK(hipMemcpyAsync(sums, c->scal.p, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+  }
+  const double total_sum = sums[2], scale = sums[1];
+  if (!std::isfinite(total_sum)) throw std::invalid_argument("right-hand side must be finite");
+  if (compat == 1 && std::fabs(total_sum) > 256 * EPS * std::max(1.0, scale))
+    throw IncompatibleError("Poisson right-hand side has nonzero volume mean on component rooted at cell 1");
+  const double mean = total_sum / c->H->total_volume;
+  // b -= V * mean (scalar staged through device memory)
+  {
+    double h = mean;
+    HIPCHECK(hipMemcpyAsync(c->scal.p + 8, &h, sizeof(double), hipMemcpyHostToDevice, s));
+    launch_shift(s, N, nb, c->b.p, c->scal.p + 8, 1.0, vol, true);
+    HIPCHECK(hipStreamSynchronize(s));
+  }
+  st.removed_rhs_mean = mean;
+  remove_mean(c, c->x.p);
+  double il2, ilinf;
+  true_residual(c, il2, ilinf);
+  const double floor0 = roundoff_floor(c);
+  double target = std::max(atol, std::max(rtol * std::max(ilinf, EPS), floor0));
+  int iterations = 0;
+  int reason = 0;
+  if (ilinf > target) {
+    int total = 0;
+    while (total < max_cycles) {
+      const int it = run_cg<T>(c, lv, target, max_cycles - total);
+      reason = c->st_host->reason;
+      total += it;
+      remove_mean(c, c->x.p);
+      double l2, linf;
+      true_residual(c, l2, linf);
+      const double eff = guarded_target(c, target, ilinf, linf);
+      if (linf <= eff) break;
+      if (!std::isfinite(linf) || it == 0) break;
+    }
+    iterations = total;
+  }
+  remove_mean(c, c->x.p);
+  double fl2, flinf;
+  true_residual(c, fl2, flinf);
+  target = guarded_target(c, target, ilinf, flinf);
+  HIPCHECK(hipEventRecord(c->ev1, s));
+  HIPCHECK(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  st.cycles = iterations;
+  st.converged = std::isfinite(flinf) && std::isfinite(target) && flinf <= target;
+  st.compatible = 1;
+  st.reason = reason;
+  st.initial_l2 = il2;
+  st.initial_linf = ilinf;
+  st.final_l2 = fl2;
+  st.final_linf = flinf;
+  st.target_linf = target;
+  st.residual_scale = std::max(ilinf, EPS);
+  st.solve_ms = ms;
+  return st;
+}
+
+tpamr_stats solve_dispatch(tpamr_ctx* c, double rtol, double atol, int max_cycles, int compat) {
+  if (!(rtol >= 0 && rtol < 1)) throw std::invalid_argument("rtol must lie in [0, 1)");
+  if (!(atol >= 0)) throw std::invalid_argument("atol must be nonnegative");
+  if (max_cycles < 0) throw std::invalid_argument("max_cycles must be nonnegative");
+  if (compat != 0 && compat != 1) throw std::invalid_argument("compatibility must be :project or :error");
+  if (c->cfg.pre != c->cfg.post)
+    throw std::invalid_argument("CG requires equal pre- and post-smoothing sweep counts");
+  if (c->precision == TPAMR_FP32_VCYCLE) return solve_impl<float>(c, c->lf, rtol, atol, max_cycles, compat);
+  return solve_impl<double>(c, c->ld, rtol, atol, max_cycles, compat);
+}
+
+template <typename T>
+void vcycle_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv) {
+  // r in c->r (integrated); z into c->tmp with zero volume mean
+  hipStream_t s = c->stream;
+  const int N = c->topo[0].ncells, nb = c->H->levels[0]->mesh->nb();
+  CgState init{};
+  *c->st_host = init;
+  HIPCHECK(hipMemcpyAsync(c->st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, s));
+  DevLevel<T>& L0 = lv[0];
+  const double w0 = c->cfg.pre > 0 ? L0.omega : 0.0;
+  launch_fill(s, N, 0.0, c->Ap.p);
+  launch_cg_update_xr<T>(s, N, nb, nullptr, c->Ap.p, nullptr, c->r.p, c->topo[0].volume.p, L0.b.p, L0.x.p,
+                         L0.dinv.p, w0, c->maxpart.p, c->st, 1);
+  T* z = enqueue_vcycle<T>(c, lv, &c->st->done, nullptr, nullptr);
+  if constexpr (std::is_same<T, float>::value)
+    launch_cast_f2d(s, N, z, c->tmp.p);
+  else
+    launch_copy(s, N, z, c->tmp.p);
+  remove_mean(c, c->tmp.p);
+  HIPCHECK(hipStreamSynchronize(s));
+}
+
+size_t upload_vec(tpamr_ctx* c, double* dst, const double* src, size_t n) {
+  HIPCHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  return n;
+}
+
+void download_vec(tpamr_ctx* c, double* dst, const double* src, size_t n) {
+  HIPCHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+}
+
+void check_finite(const double* v, size_t n, const char* what) {
+  for (size_t q = 0; q < n; ++q)
+    if (!std::isfinite(v[q])) throw std::invalid_argument(std::string(what) + " must be finite");
+}
+
+// metric products on face vectors: out = sqrt(k) K (sqrt(k) in)
+void weighted_metric(tpamr_ctx* c, const double* in_scaled, double* out) {
+  const int F = c->mesh().nfaces();
+  launch_metric_apply(c->stream, F, c->kdiag.p, c->kptr.p, c->kface.p, c->kval.p, in_scaled, c->f4.p);
+  launch_face_scale(c->stream, F, c->sqk.p, c->f4.p, out);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+int tpamr_create(tpamr_ctx** out, int device, int precision) {
+  if (!out) return TPAMR_EINVAL;
+  *out = nullptr;
+  auto c = std::make_unique<tpamr_ctx>();
+  if (precision != TPAMR_FP64 && precision != TPAMR_FP32_VCYCLE) return TPAMR_EINVAL;
+  c->precision = precision;
+  c->device = device;
+  if (device >= 0) {
+    int rc = guard(c.get(), [&] {
+      int n = 0;
+      HIPCHECK(hipGetDeviceCount(&n));
+      if (device >= n) throw HipError("device index out of range");
+      HIPCHECK(hipSetDevice(device));
+      HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+      HIPCHECK(hipMalloc(&c->st, sizeof(CgState)));
+      HIPCHECK(hipHostMalloc(&c->st_host, sizeof(CgState), hipHostMallocDefault));
+      HIPCHECK(hipHostMalloc(&c->st_host2, sizeof(CgState), hipHostMallocDefault));
+      HIPCHECK(hipEventCreate(&c->ev0));
+      HIPCHECK(hipEventCreate(&c->ev1));
+      HIPCHECK(hipEventCreateWithFlags(&c->evs[0], hipEventDisableTiming));
+      HIPCHECK(hipEventCreateWithFlags(&c->evs[1], hipEventDisableTiming));
+    });
+    if (rc != TPAMR_OK) {
+      // keep the context so the caller can read the message
+      *out = c.release();
+      return rc;
+    }
+  }
+  *out = c.release();
+  return TPAMR_OK;
+}
+
+void tpamr_destroy(tpamr_ctx* c) {
+  if (!c) return;
+  if (c->device >= 0) {
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    destroy_graph(c);
+    c->topo.clear();
+    c->ld.clear();
+    c->lf.clear();
+    if (c->st) (void)hipFree(c->st);
+    if (c->st_host) (void)hipHostFree(c->st_host);
+    if (c->st_host2) (void)hipHostFree(c->st_host2);
+    for (hipEvent_t e : {c->ev0, c->ev1, c->evs[0], c->evs[1]})
+      if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+  }
+  delete c;
+}
+
+const char* tpamr_last_error(const tpamr_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int tpamr_set_mesh(tpamr_ctx* c, const int base_mesh[2], const int block_size[2], const double origin[2],
+                   const double extent[2], const int periodic[2], int max_level, int n_refined, const int* refined) {
+  if (!c) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    if (!base_mesh || !block_size || !origin || !extent || !periodic) throw std::invalid_argument("null argument");
+    if (n_refined < 0 || (n_refined > 0 && !refined)) throw std::invalid_argument("invalid refined key list");
+    if (!supported_block(block_size[0], block_size[1]))
+      throw std::invalid_argument("block_size must be square with 4, 8, 16 or 32 cells per side");
+    for (int a = 0; a < 2; ++a)
+      if (!std::isfinite(origin[a]) || !std::isfinite(extent[a]))
+        throw std::invalid_argument("domain origin and extent must be finite");
+    std::vector<BlockKey> keys(n_refined);
+    for (int q = 0; q < n_refined; ++q) keys[q] = {refined[3 * q], refined[3 * q + 1], refined[3 * q + 2]};
+    const bool per[2] = {periodic[0] != 0, periodic[1] != 0};
+    if (c->device >= 0) HIPCHECK(hipSetDevice(c->device));
+    destroy_graph(c);
+    c->H.reset();
+    c->have_coeff = false;
+    c->mesh_pending = make_tree_mesh(base_mesh, block_size, origin, extent, per, max_level, keys);
+  });
+}
+
+int tpamr_mesh_sizes(const tpamr_ctx* cc, int* sizes) {
+  auto* c = const_cast<tpamr_ctx*>(cc);
+  if (!c || !sizes) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    const TreeMesh& M = c->mesh();
+    sizes[0] = int(M.blocks.size());
+    sizes[1] = M.ncells();
+    sizes[2] = M.nfaces();
+    sizes[3] = int(M.bcell.size());
+    sizes[4] = M.max_leaf;
+  });
+}
+
+int tpamr_mesh_blocks(const tpamr_ctx* cc, int* keys) {
+  auto* c = const_cast<tpamr_ctx*>(cc);
+  if (!c || !keys) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    const TreeMesh& M = c->mesh();
+    for (size_t b = 0; b < M.blocks.size(); ++b) {
+      keys[3 * b] = M.blocks[b].level;
+      keys[3 * b + 1] = M.blocks[b].i;
+      keys[3 * b + 2] = M.blocks[b].j;
+    }
+  });
+}
+
+int tpamr_mesh_faces(const tpamr_ctx* cc, int* minus, int* plus, int* axis, double* area) {
+  auto* c = const_cast<tpamr_ctx*>(cc);
+  if (!c) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    const TreeMesh& M = c->mesh();
+    for (int f = 0; f < M.nfaces(); ++f) {
+      if (minus) minus[f] = M.fminus[f];
+      if (plus) plus[f] = M.fplus[f];
+      if (axis) axis[f] = M.faxis[f];
+      if (area) area[f] = M.farea[f];
+    }
+  });
+}
+
+int tpamr_mesh_boundary_faces(const tpamr_ctx* cc, int* cell, int* axis, int* side, double* area) {
+  auto* c = const_cast<tpamr_ctx*>(cc);
+  if (!c) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    const TreeMesh& M = c->mesh();
+    for (size_t f = 0; f < M.bcell.size(); ++f) {
+      if (cell) cell[f] = M.bcell[f];
+      if (axis) axis[f] = M.baxis[f];
+      if (side) side[f] = M.bside[f];
+      if (area) area[f] = M.barea[f];
+    }
+  });
+}
+
+int tpamr_set_multigrid(tpamr_ctx* c, const tpamr_multigrid_options* o) {
+  if (!c || !o) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    if (o->coarse_size <= 0) throw std::invalid_argument("coarse_size must be positive");
+    if (o->pre_sweeps < 0) throw std::invalid_argument("pre_sweeps must be nonnegative");
+    if (o->post_sweeps < 0) throw std::invalid_argument("post_sweeps must be nonnegative");
+    if (o->max_levels <= 0) throw std::invalid_argument("max_levels must be positive");
+    const bool topo_change = !c->H || o->coarse_size != c->cfg.coarse_size || o->max_levels != c->cfg.max_levels;
+    c->cfg.coarse_size = o->coarse_size;
+    c->cfg.pre = o->pre_sweeps;
+    c->cfg.post = o->post_sweeps;
+    c->cfg.max_levels = o->max_levels;
+    destroy_graph(c);
+    if (c->H && topo_change) {
+      // rebuild the hierarchy at the next coefficient update
+      c->mesh_pending = std::move(c->H->levels[0]->owned);
+      c->H.reset();
+      c->have_coeff = false;
+    }
+  });
+}
+
+int tpamr_set_face_coefficients(tpamr_ctx* c, const double* kappa) {
+  if (!c || !kappa) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    if (c->device >= 0) HIPCHECK(hipSetDevice(c->device));
+    bool topology = false;
+    if (!c->H) {
+      if (!c->mesh_pending) throw StateError("no mesh: call tpamr_set_mesh first");
+      c->H = build_hierarchy(std::move(c->mesh_pending), c->cfg);
+      topology = true;
+    }
+    refresh_coefficients(*c->H, kappa);
+    c->have_coeff = true;
+    if (c->device >= 0) upload_hierarchy(c, topology);
+  });
+}
+
+int tpamr_num_levels(const tpamr_ctx* c) { return (c && c->H) ? int(c->H->levels.size()) : 0; }
+
+int tpamr_level_info(const tpamr_ctx* cc, int level, int* ints, double* dbl) {
+  auto* c = const_cast<tpamr_ctx*>(cc);
+  if (!c) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_solver(c);
+    if (level < 0 || level >= int(c->H->levels.size())) throw std::invalid_argument("level out of range");
+    const LevelData& L = *c->H->levels[level];
+    if (ints) {
+      ints[0] = L.ncells;
+      ints[1] = L.mesh->nfaces();
+      ints[2] = L.nblocks;
+      ints[3] = int(L.irr_local.size());
+      long nnz = 0;
+      for (int b = 0; b < L.nblocks; ++b)
+        for (int q = 0; q < L.mesh->nb(); ++q) {
+          const int cidx = b * L.mesh->nb() + q;
+          if (L.cell_irr[cidx] >= 0) nnz += L.irr_ptr[L.cell_irr[cidx] + 1] - L.irr_ptr[L.cell_irr[cidx]];
+        }
+      ints[4] = int(nnz);
+      ints[5] = L.mesh->max_leaf;
+    }
+    if (dbl) {
+      dbl[0] = L.omega;
+      dbl[1] = L.omega_p;
+    }
+  });
+}
+
+int tpamr_level_matrix(const tpamr_ctx* cc, int level, int* nnz, int* rowptr, int* col, double* val) {
+  auto* c = const_cast<tpamr_ctx*>(cc);
+  if (!c || !nnz) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_solver(c);
+    if (level < 0 || level >= int(c->H->levels.size())) throw std::invalid_argument("level out of range");
+    const LevelData& L = *c->H->levels[level];
+    const TreeMesh& M = *L.mesh;
+    const int bx = M.bs[0], by = M.bs[1], nb = bx * by;
+    std::vector<int> rp(1, 0), cl;
+    std::vector<double> vl;
+    for (int b = 0; b < L.nblocks; ++b)
+      for (int q = 0; q < nb; ++q) {
+        const int cidx = b * nb + q;
+        std::vector<std::pair<int, double>> row;
+        if (L.cell_irr[cidx] >= 0) {
+          const int r = L.cell_irr[cidx];
+          for (int e = L.irr_ptr[r]; e < L.irr_ptr[r + 1]; ++e) row.push_back({L.irr_col[e], L.irr_val[e]});
+        } else {
+          const int i = q % bx, j = q / bx;
+          const double* cx = &L.cx[size_t(b) * (bx + 1) * by];
+          const double* cy = &L.cy[size_t(b) * bx * (by + 1)];
+          const double w[4] = {cx[j * (bx + 1) + i], cx[j * (bx + 1) + i + 1], cy[j * bx + i], cy[(j + 1) * bx + i]};
+          auto at = [&](int bb, int ii, int jj) { return bb * nb + jj * bx + ii; };
+          const int nbc[4] = {
+              i > 0 ? at(b, i - 1, j) : (L.nbr[4 * b] >= 0 ? at(L.nbr[4 * b], bx - 1, j) : -1),
+              i < bx - 1 ? at(b, i + 1, j) : (L.nbr[4 * b + 1] >= 0 ? at(L.nbr[4 * b + 1], 0, j) : -1),
+              j > 0 ? at(b, i, j - 1) : (L.nbr[4 * b + 2] >= 0 ? at(L.nbr[4 * b + 2], i, by - 1) : -1),
+              j < by - 1 ? at(b, i, j + 1) : (L.nbr[4 * b + 3] >= 0 ? at(L.nbr[4 * b + 3], i, 0) : -1)};
+          double d = (w[0] + w[1]) + (w[2] + w[3]);
+          row.push_back({cidx, d});
+          for (int sd = 0; sd < 4; ++sd)
+            if (w[sd] != 0.0) row.push_back({nbc[sd], -w[sd]});
+        }
+        std::sort(row.begin(), row.end());
+        for (size_t e = 0; e < row.size(); ++e) {
+          if (e > 0 && row[e].first == row[e - 1].first) {
+            vl.back() += row[e].second;
+            continue;
+          }
+          cl.push_back(row[e].first);
+          vl.push_back(row[e].second);
+        }
+        rp.push_back(int(cl.size()));
+      }
+    *nnz = int(cl.size());
+    if (rowptr) {
+      std::copy(rp.begin(), rp.end(), rowptr);
+      std::copy(cl.begin(), cl.end(), col);
+      std::copy(vl.begin(), vl.end(), val);
+    }
+  });
+}
+
+int tpamr_level_transfer(const tpamr_ctx* cc, int level, int* xfer) {
+  auto* c = const_cast<tpamr_ctx*>(cc);
+  if (!c || !xfer) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_solver(c);
+    if (level < 0 || level + 1 >= int(c->H->levels.size())) throw std::invalid_argument("level has no transfer");
+    const LevelData& L = *c->H->levels[level];
+    std::copy(L.xfer.begin(), L.xfer.end(), xfer);
+  });
+}
+
+int tpamr_level_kappa(const tpamr_ctx* cc, int level, double* kappa) {
+  auto* c = const_cast<tpamr_ctx*>(cc);
+  if (!c || !kappa) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_solver(c);
+    if (level < 0 || level >= int(c->H->levels.size())) throw std::invalid_argument("level out of range");
+    const LevelData& L = *c->H->levels[level];
+    std::copy(L.kappa.begin(), L.kappa.end(), kappa);
+  });
+}
+
+int tpamr_solve_device(tpamr_ctx* c, const double* d_rhs, double* d_x, double rtol, double atol, int max_cycles,
+                       int compatibility, tpamr_stats* stats) {
+  if (!c || !d_rhs || !d_x) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    need_solver(c);
+    HIPCHECK(hipSetDevice(c->device));
+    const int N = c->topo[0].ncells;
+    HIPCHECK(hipMemcpyAsync(c->b.p, d_rhs, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->x.p, d_x, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    tpamr_stats st = solve_dispatch(c, rtol, atol, max_cycles, compatibility);
+    HIPCHECK(hipMemcpyAsync(d_x, c->x.p, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (stats) *stats = st;
+  });
+}
+
+int tpamr_solve(tpamr_ctx* c, const double* rhs, double* x, double rtol, double atol, int max_cycles,
+                int compatibility, tpamr_stats* stats) {
+  if (!c || !rhs || !x) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    need_solver(c);
+    HIPCHECK(hipSetDevice(c->device));
+    const int N = c->topo[0].ncells;
+    check_finite(rhs, N, "right-hand side");
+    check_finite(x, N, "initial solution");
+    upload_vec(c, c->b.p, rhs, N);
+    upload_vec(c, c->x.p, x, N);
+    tpamr_stats st = solve_dispatch(c, rtol, atol, max_cycles, compatibility);
+    download_vec(c, x, c->x.p, N);
+    if (stats) *stats = st;
+  });
+}
+
+int tpamr_vcycle(tpamr_ctx* c, const double* r, double* z) {
+  if (!c || !r || !z) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    need_solver(c);
+    HIPCHECK(hipSetDevice(c->device));
+    const int N = c->topo[0].ncells;
+    upload_vec(c, c->r.p, r, N);
+    if (c->precision == TPAMR_FP32_VCYCLE)
+      vcycle_impl<float>(c, c->lf);
+    else
+      vcycle_impl<double>(c, c->ld);
+    download_vec(c, z, c->tmp.p, N);
+  });
+}
+
+int tpamr_apply_device(tpamr_ctx* c, const double* d_x, double* d_y) {
+  if (!c || !d_x || !d_y) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    need_solver(c);
+    HIPCHECK(hipSetDevice(c->device));
+    matvec(c, d_x, d_y);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int tpamr_apply(tpamr_ctx* c, const double* x, double* y) {
+  if (!c || !x || !y) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    need_solver(c);
+    HIPCHECK(hipSetDevice(c->device));
+    const int N = c->topo[0].ncells;
+    upload_vec(c, c->tmp.p, x, N);
+    matvec(c, c->tmp.p, c->tmp2.p);
+    download_vec(c, y, c->tmp2.p, N);
+  });
+}
+
+int tpamr_mixture_face_coefficients(tpamr_ctx* c, const double* fraction, double rl, double rg, double* kappa) {
+  if (!c || !fraction || !kappa) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    if (!c->H) throw StateError("no hierarchy: call tpamr_set_face_coefficients once before two-phase refreshes");
+    if (!(std::isfinite(rl) && rl > 0 && std::isfinite(rg) && rg > 0))
+      throw std::invalid_argument("phase densities must be finite and positive");
+    HIPCHECK(hipSetDevice(c->device));
+    const TreeMesh& M = c->mesh();
+    const int N = M.ncells(), F = M.nfaces();
+    upload_vec(c, c->tmp.p, fraction, N);
+    launch_mixture(c->stream, F, M.nb(), c->fm.p, c->fp.p, c->faxis.p, c->spacing.p, c->tmp.p, rl, rg, c->f1.p);
+    download_vec(c, kappa, c->f1.p, F);
+  });
+}
+
+int tpamr_surface_tension_jumps(tpamr_ctx* c, const double* fraction, const double* curvature, double sigma,
+                                int formulation, double* jumps) {
+  if (!c || !fraction || !curvature || !jumps) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    if (!c->H) throw StateError("no hierarchy: call tpamr_set_face_coefficients first");
+    if (!(std::isfinite(sigma) && sigma >= 0)) throw std::invalid_argument("sigma must be finite and nonnegative");
+    if (formulation != 0 && formulation != 1) throw std::invalid_argument("formulation must be CSF() or CGFM()");
+    HIPCHECK(hipSetDevice(c->device));
+    const TreeMesh& M = c->mesh();
+    const int N = M.ncells(), F = M.nfaces();
+    upload_vec(c, c->tmp.p, fraction, N);
+    upload_vec(c, c->f2.p, curvature, F);
+    launch_surface_jump(c->stream, F, c->fm.p, c->fp.p, c->farea.p, c->tmp.p, c->f2.p, sigma, formulation, c->f1.p);
+    download_vec(c, jumps, c->f1.p, F);
+  });
+}
+
+int tpamr_weighted_jump_force(tpamr_ctx* c, const double* jumps, double* accel) {
+  if (!c || !jumps || !accel) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    need_solver(c);
+    HIPCHECK(hipSetDevice(c->device));
+    const int F = c->mesh().nfaces();
+    upload_vec(c, c->f1.p, jumps, F);
+    launch_face_jump(c->stream, F, c->fm.p, c->fp.p, c->farea.p, c->sqk.p, nullptr, c->f1.p, c->f2.p);
+    weighted_metric(c, c->f2.p, c->f3.p);
+    download_vec(c, accel, c->f3.p, F);
+  });
+}
+
+int tpamr_divergence(tpamr_ctx* c, const double* u, const double* ub, double* div) {
+  if (!c || !u || !div) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    if (!c->H) throw StateError("no hierarchy: call tpamr_set_face_coefficients first");
+    HIPCHECK(hipSetDevice(c->device));
+    const TreeMesh& M = c->mesh();
+    const int N = M.ncells(), F = M.nfaces(), BF = int(M.bcell.size());
+    upload_vec(c, c->f1.p, u, F);
+    double* dub = nullptr;
+    if (ub && BF > 0) {
+      if (c->f2.n < size_t(BF)) throw SizeError("boundary face buffer too small");
+      upload_vec(c, c->f2.p, ub, BF);
+      dub = c->f2.p;
+    }
+    launch_divergence(c->stream, N, M.nb(), c->cptr.p, c->cface.p, c->csign.p, c->farea.p, c->f1.p, c->bptr.p,
+                      c->bface.p, c->bside.p, c->barea.p, dub, c->topo[0].volume.p, c->tmp.p);
+    download_vec(c, div, c->tmp.p, N);
+  });
+}
+
+int tpamr_weighted_gradient(tpamr_ctx* c, const double* p, double* g) {
+  if (!c || !p || !g) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    need_solver(c);
+    HIPCHECK(hipSetDevice(c->device));
+    const TreeMesh& M = c->mesh();
+    const int N = M.ncells(), F = M.nfaces();
+    upload_vec(c, c->tmp.p, p, N);
+    launch_face_jump(c->stream, F, c->fm.p, c->fp.p, c->farea.p, c->sqk.p, c->tmp.p, nullptr, c->f1.p);
+    weighted_metric(c, c->f1.p, c->f3.p);
+    download_vec(c, g, c->f3.p, F);
+  });
+}
+
+int tpamr_project(tpamr_ctx* c, double* u, double* p, double dt, double tol, int max_cycles, double* phi,
+                  tpamr_stats* stats) {
+  if (!c || !u || !p) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    need_solver(c);
+    if (!(dt > 0)) throw std::invalid_argument("dt must be strictly positive");
+    if (!(std::isfinite(tol) && tol > 0 && tol < 1)) throw std::invalid_argument("tolerance must be finite and lie in (0, 1)");
+    HIPCHECK(hipSetDevice(c->device));
+    const TreeMesh& M = c->mesh();
+    const int N = M.ncells(), F = M.nfaces(), nb = M.nb();
+    hipStream_t s = c->stream;
+    upload_vec(c, c->f1.p, u, F);
+    launch_divergence(s, N, nb, c->cptr.p, c->cface.p, c->csign.p, c->farea.p, c->f1.p, c->bptr.p, c->bface.p,
+                      c->bside.p, c->barea.p, nullptr, c->topo[0].volume.p, c->tmp.p);
+    // rhs = -div / dt
+    launch_axpby(s, N, -1.0 / dt, c->tmp.p, 0.0, c->b.p);
+    launch_fill(s, N, 0.0, c->x.p);
+    tpamr_stats st = solve_dispatch(c, 0.0, tol / (dt * dt), max_cycles, 0);
+    st.residual_scale = 1.0 / (dt * dt);
+    // u -= dt G phi
+    launch_face_jump(s, F, c->fm.p, c->fp.p, c->farea.p, c->sqk.p, c->x.p, nullptr, c->f2.p);
+    weighted_metric(c, c->f2.p, c->f3.p);
+    launch_axpby(s, F, -dt, c->f3.p, 1.0, c->f1.p);
+    download_vec(c, u, c->f1.p, F);
+    std::vector<double> ph(N);
+    download_vec(c, ph.data(), c->x.p, N);
+    for (int q = 0; q < N; ++q) p[q] += ph[q];
+    if (phi) std::copy(ph.begin(), ph.end(), phi);
+    if (stats) *stats = st;
+  });
+}
+
+int tpamr_adjoint(tpamr_ctx* c, const double* w, const double* p, double rtol, int max_cycles, double* dJ_drhs,
+                  double* dJ_dkappa, tpamr_stats* stats) {
+  if (!c || !w || !p || !dJ_drhs || !dJ_dkappa) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    need_solver(c);
+    HIPCHECK(hipSetDevice(c->device));
+    const TreeMesh& M = c->mesh();
+    const int N = M.ncells(), F = M.nfaces(), nb = M.nb();
+    hipStream_t s = c->stream;
+    // A lambda = Q w: strong rhs w / V
+    upload_vec(c, c->tmp.p, w, N);
+    launch_scale_by_volume(s, N, nb, c->tmp.p, c->b.p, c->topo[0].volume.p, true);
+    launch_fill(s, N, 0.0, c->x.p);
+    tpamr_stats st = solve_dispatch(c, rtol, 0.0, max_cycles, 0);
+    // dJ/drhs = V lambda
+    launch_scale_by_volume(s, N, nb, c->x.p, c->tmp.p, c->topo[0].volume.p, false);
+    download_vec(c, dJ_drhs, c->tmp.p, N);
+    // y_l, y_p, K y
+    upload_vec(c, c->tmp2.p, p, N);
+    launch_face_jump(s, F, c->fm.p, c->fp.p, c->farea.p, c->sqk.p, c->x.p, nullptr, c->f1.p);
+    launch_face_jump(s, F, c->fm.p, c->fp.p, c->farea.p, c->sqk.p, c->tmp2.p, nullptr, c->f2.p);
+    launch_metric_apply(s, F, c->kdiag.p, c->kptr.p, c->kface.p, c->kval.p, c->f1.p, c->f3.p);
+    launch_metric_apply(s, F, c->kdiag.p, c->kptr.p, c->kface.p, c->kval.p, c->f2.p, c->f4.p);
+    // f3 = K y_l, f4 = K y_p
+    launch_adjoint_face(s, F, c->f1.p, c->f4.p, c->f2.p, c->f3.p, c->kappa.p, c->f1.p);
+    download_vec(c, dJ_dkappa, c->f1.p, F);
+    if (stats) *stats = st;
+  });
+}
+
+int tpamr_time_smoother(tpamr_ctx* c, int reps, double* avg_ms, double* bytes) {
+  if (!c || !avg_ms || reps <= 0) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    need_solver(c);
+    HIPCHECK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int N = c->topo[0].ncells;
+    const LevelData& L = *c->H->levels[0];
+    const size_t nb = L.nblocks;
+    const size_t bx = L.mesh->bs[0], by = L.mesh->bs[1];
+    double elem = 8;
+    HIPCHECK(hipEventRecord(c->ev0, s));
+    if (c->precision == TPAMR_FP32_VCYCLE) {
+      DevLevel<float>& D = c->lf[0];
+      for (int q = 0; q < reps; ++q)
+        launch_sweep<float>(s, D.view, (q & 1) ? D.xs.p : D.x.p, (q & 1) ? D.x.p : D.xs.p, D.b.p, D.omega, nullptr,
+                            true, nullptr, nullptr);
+      elem = 4;
+    } else {
+      DevLevel<double>& D = c->ld[0];
+      for (int q = 0; q < reps; ++q)
+        launch_sweep<double>(s, D.view, (q & 1) ? D.xs.p : D.x.p, (q & 1) ? D.x.p : D.xs.p, D.b.p, D.omega, nullptr,
+                             true, nullptr, nullptr);
+    }
+    HIPCHECK(hipEventRecord(c->ev1, s));
+    HIPCHECK(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *avg_ms = double(ms) / reps;
+    if (bytes) {
+      // compulsory traffic: x, b read, x' write, the two conductance arrays
+      // (including the block-boundary faces) and the irregular rows
+      double cells = double(N) * 3 * elem;
+      double cond = double(nb) * ((bx + 1) * by + bx * (by + 1)) * elem;
+      double irr = double(L.irr_col.size()) * (4 + elem) + double(L.irr_local.size()) * (4 + 4 + elem);
+      *bytes = cells + cond + irr;
+    }
+  });
+}
+
+}  // extern "C"

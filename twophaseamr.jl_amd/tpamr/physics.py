@@ -1,0 +1,133 @@
+"""Two-phase pressure coefficients, surface-tension jumps, projection and the
+pressure adjoint, on the GPU.
+
+Reference interfaces (TwoPhaseAMR.jl):
+  mixture_density / PoissonOperator(::InterfaceField; density)  src/two_phase/forces.jl:9-140
+  _surface_tension_jumps! (CGFM / CSF)                          src/two_phase/forces.jl:312-378
+  _weighted_jump_force!                                         src/two_phase/forces.jl:179-209
+  divergence!                                                   src/operators/advection/cell_advection.jl:205-220
+  weighted_gradient!                                            src/operators/elliptic.jl:1272-1297
+  project! / _project_prepared!                                 src/navier_stokes/projection.jl:189-260
+The adjoint (`pressure_solve_rrule`) has no reference counterpart.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .solver import EllipticSolver, LinearSolverStats, PoissonOperator
+
+CGFM = 0
+CSF = 1
+
+
+def mixture_face_coefficients(solver: EllipticSolver, fraction, density=(1.0, 1.0)):
+    """inv.(mixture_density(field, density)): kappa_f = 1 / (rho_l c_f + rho_g (1 - c_f))."""
+    rho_l, rho_g = (float(v) for v in density)
+    if not (np.isfinite(rho_l) and rho_l > 0 and np.isfinite(rho_g) and rho_g > 0):
+        raise ValueError("phase densities must be finite and positive")
+    m = solver.operator.mesh
+    c = L.f64(fraction, m.ncells, "fraction")
+    k = np.zeros(m.nfaces)
+    L.check(L.lib().tpamr_mixture_face_coefficients(solver.ctx.h, L.ptr(c), rho_l, rho_g, L.ptr(k)), solver.ctx.h)
+    return k
+
+
+def refresh_two_phase(solver: EllipticSolver, fraction, density):
+    """`_refresh_pressure_substep!` (src/two_phase/stepping.jl:73-111): new
+    mixture coefficients into the operator and every multigrid level."""
+    solver.operator.replace_face_coefficient(mixture_face_coefficients(solver, fraction, density))
+    solver.refresh()
+    return solver
+
+
+def surface_tension_jumps(solver: EllipticSolver, fraction, face_curvature, *, sigma, formulation=CGFM):
+    """Integrated jumps sigma kappa_f [H]_f |f| (NaN curvature = unresolved face)."""
+    if not (np.isfinite(sigma) and sigma >= 0):
+        raise ValueError("sigma must be finite and nonnegative")
+    if formulation not in (CGFM, CSF):
+        raise ValueError("formulation must be CSF() or CGFM()")
+    m = solver.operator.mesh
+    c = L.f64(fraction, m.ncells, "fraction")
+    curv = np.ascontiguousarray(np.broadcast_to(np.asarray(face_curvature, np.float64), (m.nfaces,)))
+    out = np.zeros(m.nfaces)
+    L.check(L.lib().tpamr_surface_tension_jumps(solver.ctx.h, L.ptr(c), L.ptr(curv), float(sigma), int(formulation),
+                                                L.ptr(out)), solver.ctx.h)
+    return out
+
+
+def weighted_jump_force(solver: EllipticSolver, jumps):
+    solver._sync()
+    m = solver.operator.mesh
+    j = L.f64(jumps, m.nfaces, "jumps")
+    out = np.zeros(m.nfaces)
+    L.check(L.lib().tpamr_weighted_jump_force(solver.ctx.h, L.ptr(j), L.ptr(out)), solver.ctx.h)
+    return out
+
+
+def surface_tension(solver: EllipticSolver, fraction, face_curvature, *, sigma, formulation=CGFM):
+    """surface_tension! interior face accelerations (boundary faces are zero)."""
+    return weighted_jump_force(solver, surface_tension_jumps(solver, fraction, face_curvature, sigma=sigma,
+                                                             formulation=formulation))
+
+
+def divergence(solver: EllipticSolver, u, u_boundary=None):
+    m = solver.operator.mesh
+    uu = L.f64(u, m.nfaces, "u")
+    ub = None if u_boundary is None else L.f64(u_boundary, m.nboundary_faces, "u_boundary")
+    out = np.zeros(m.ncells)
+    L.check(L.lib().tpamr_divergence(solver.ctx.h, L.ptr(uu), L.ptr(ub), L.ptr(out)), solver.ctx.h)
+    return out
+
+
+def weighted_gradient(solver: EllipticSolver, p):
+    solver._sync()
+    m = solver.operator.mesh
+    pp = L.f64(p, m.ncells, "p")
+    out = np.zeros(m.nfaces)
+    L.check(L.lib().tpamr_weighted_gradient(solver.ctx.h, L.ptr(pp), L.ptr(out)), solver.ctx.h)
+    return out
+
+
+def project(u, p, solver: EllipticSolver, dt, *, tolerance=float(np.sqrt(np.finfo(float).eps)), max_cycles=100):
+    """project! on a wall/periodic domain: u and p are updated in place; returns
+    (stats, phi)."""
+    solver._sync()
+    m = solver.operator.mesh
+    if u.dtype != np.float64 or u.shape != (m.nfaces,) or not u.flags.c_contiguous:
+        raise L.DimensionMismatch("u must be a contiguous float64 vector over interior faces")
+    if p.dtype != np.float64 or p.shape != (m.ncells,) or not p.flags.c_contiguous:
+        raise L.DimensionMismatch("p must be a contiguous float64 vector over cells")
+    phi = np.zeros(m.ncells)
+    st = L.Stats()
+    L.check(L.lib().tpamr_project(solver.ctx.h, L.ptr(u), L.ptr(p), float(dt), float(tolerance), int(max_cycles),
+                                  L.ptr(phi), C.byref(st)), solver.ctx.h)
+    return LinearSolverStats._from(st, solver.precision), phi
+
+
+def pressure_solve(solver: EllipticSolver, rhs, *, rtol=1e-12, max_cycles=200):
+    p = np.zeros(solver.n)
+    st = solver.solve(p, rhs, rtol=rtol, max_cycles=max_cycles)
+    return p, st
+
+
+def pressure_solve_rrule(solver: EllipticSolver, rhs, *, rtol=1e-12, max_cycles=200):
+    """Forward pressure solve and its pullback, like a ChainRulesCore rrule.
+
+    Returns (p, pullback) with pullback(w) -> (dJ/drhs, dJ/dkappa) for J = w' p.
+    The adjoint solve reuses the same (symmetric) GPU multigrid."""
+    p, _ = pressure_solve(solver, rhs, rtol=rtol, max_cycles=max_cycles)
+
+    def pullback(w):
+        m = solver.operator.mesh
+        ww = L.f64(w, m.ncells, "w")
+        db = np.zeros(m.ncells)
+        dk = np.zeros(m.nfaces)
+        st = L.Stats()
+        L.check(L.lib().tpamr_adjoint(solver.ctx.h, L.ptr(ww), L.ptr(p), float(rtol), int(max_cycles), L.ptr(db),
+                                      L.ptr(dk), C.byref(st)), solver.ctx.h)
+        return db, dk
+
+    return p, pullback

@@ -1,0 +1,269 @@
+"""Pressure solver surface mirroring TwoPhaseAMR.jl's linear-solver API.
+
+Reference interfaces replaced here (file:line in TwoPhaseAMR.jl):
+  MultigridOptions       src/solvers/multigrid/options.jl:61-95
+  LinearSolverOptions    src/solvers/linear_options.jl:10-47
+  PoissonOperator        src/operators/elliptic.jl:969-980 (face_coefficient form)
+  EllipticSolver         src/solvers/multigrid/elliptic_solver.jl:282-302
+  solve!                 src/solvers/multigrid/elliptic_solver.jl:628-675
+  ldiv!(z, P, r)         src/solvers/multigrid/cycles.jl:275-283
+  LinearSolverStats      src/solvers/multigrid/stats.jl:14-36
+All computation runs in libtpamr_hip.so on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from .mesh import BlockTreeMesh
+
+REASONS = {0: "none", 1: "solved", 2: "strong residual target", 3: "zero curvature", 4: "iteration limit",
+           5: "indefinite preconditioner"}
+
+
+class MultigridOptions:
+    """MultigridOptions(; coarse_size=64, pre_sweeps=2, post_sweeps=2, max_levels=32)."""
+
+    def __init__(self, coarse_size=64, pre_sweeps=2, post_sweeps=2, max_levels=32):
+        vals = (coarse_size, pre_sweeps, post_sweeps, max_levels)
+        if not all(isinstance(v, (int, np.integer)) and not isinstance(v, bool) for v in vals):
+            raise ValueError("multigrid sizes and sweep counts must be integers")
+        if coarse_size <= 0:
+            raise ValueError("coarse_size must be positive")
+        if pre_sweeps < 0:
+            raise ValueError("pre_sweeps must be nonnegative")
+        if post_sweeps < 0:
+            raise ValueError("post_sweeps must be nonnegative")
+        if max_levels <= 0:
+            raise ValueError("max_levels must be positive")
+        self.coarse_size = int(coarse_size)
+        self.pre_sweeps = int(pre_sweeps)
+        self.post_sweeps = int(post_sweeps)
+        self.max_levels = int(max_levels)
+
+    def _c(self):
+        return L.MGOptions(self.coarse_size, self.pre_sweeps, self.post_sweeps, self.max_levels)
+
+    def __repr__(self):
+        return (f"MultigridOptions(coarse_size={self.coarse_size}, pre_sweeps={self.pre_sweeps}, "
+                f"post_sweeps={self.post_sweeps}, max_levels={self.max_levels})")
+
+
+_PRECONDITIONERS = (":auto", ":geometric", ":tree_geometric", ":amg", ":ilu", ":ichol")
+
+
+class LinearSolverOptions:
+    """LinearSolverOptions(; krylov=None, krylov_options={}, preconditioner=':auto',
+    preconditioner_options={}). This library implements CG with the tree
+    geometric multigrid; `preconditioner_options` may carry `precision`
+    ('fp64' or 'fp32_vcycle') and `device`."""
+
+    def __init__(self, krylov=None, krylov_options=None, preconditioner=":auto", preconditioner_options=None):
+        if isinstance(krylov, str):
+            raise ValueError("krylov must be None, a Krylov workspace, or a workspace constructor; "
+                             "symbols are not supported")
+        if isinstance(preconditioner, str) and preconditioner not in _PRECONDITIONERS:
+            raise ValueError("preconditioner must be :auto, :geometric, :tree_geometric, :amg, :ilu, :ichol, "
+                             "None, a factory, or a pre-built preconditioner")
+        krylov_options = {} if krylov_options is None else krylov_options
+        preconditioner_options = {} if preconditioner_options is None else preconditioner_options
+        if not isinstance(krylov_options, dict):
+            raise ValueError("krylov_options must be a named tuple")
+        if not isinstance(preconditioner_options, dict):
+            raise ValueError("preconditioner_options must be a named tuple")
+        self.krylov = krylov
+        self.krylov_options = dict(krylov_options)
+        self.preconditioner = preconditioner
+        self.preconditioner_options = dict(preconditioner_options)
+
+
+@dataclass
+class LinearSolverStats:
+    cycles: int = 0
+    converged: bool = False
+    compatible: bool = True
+    initial_l2: float = float("inf")
+    initial_linf: float = float("inf")
+    final_l2: float = float("inf")
+    final_linf: float = float("inf")
+    target_linf: float = float("inf")
+    residual_scale: float = 1.0
+    removed_rhs_mean: float = 0.0
+    removed_rhs_component_means: list = field(default_factory=list)
+    removed_rhs_component_anchors: list = field(default_factory=list)
+    method: str = "cg"
+    preconditioner: str = "tree_geometric_multigrid"
+    reason: str = "none"
+    solve_ms: float = 0.0
+
+    @classmethod
+    def _from(cls, st: L.Stats, precision):
+        return cls(cycles=st.cycles, converged=bool(st.converged), compatible=bool(st.compatible),
+                   initial_l2=st.initial_l2, initial_linf=st.initial_linf, final_l2=st.final_l2,
+                   final_linf=st.final_linf, target_linf=st.target_linf, residual_scale=st.residual_scale,
+                   removed_rhs_mean=st.removed_rhs_mean,
+                   removed_rhs_component_means=[st.removed_rhs_mean], removed_rhs_component_anchors=[1],
+                   preconditioner="tree_geometric_multigrid" + ("_fp32" if precision == L.TPAMR_FP32_VCYCLE else ""),
+                   reason=REASONS.get(st.reason, str(st.reason)), solve_ms=st.solve_ms)
+
+
+class PoissonOperator:
+    """PoissonOperator(mesh; face_coefficient=1): -div(kappa grad p) in integrated
+    finite-volume form with the AMR face Riesz map (homogeneous Neumann walls or
+    periodic sides). `face_coefficient` is a number, a per-face vector in the
+    mesh's face order, or a function (x, y, axis) evaluated at face centres."""
+
+    def __init__(self, mesh: BlockTreeMesh, face_coefficient=1.0):
+        self.mesh = mesh
+        self.face_coefficient = self._coefficients(face_coefficient)
+
+    def _coefficients(self, coefficient):
+        m = self.mesh
+        if np.isscalar(coefficient):
+            values = np.full(m.nfaces, float(coefficient))
+        elif callable(coefficient):
+            raise ValueError("pass face coefficients as a vector in face order")
+        else:
+            values = np.asarray(coefficient, dtype=np.float64)
+            if values.shape != (m.nfaces,):
+                raise L.DimensionMismatch("one coefficient is required per interior face")
+        if not np.all(np.isfinite(values)):
+            raise ValueError("face coefficients must be finite")
+        if np.any(values < 0):
+            raise ValueError("face coefficients must be nonnegative")
+        return np.ascontiguousarray(values)
+
+    def replace_face_coefficient(self, face_coefficient):
+        """`_replace_face_coefficient` (elliptic.jl:809-820)."""
+        self.face_coefficient = self._coefficients(face_coefficient)
+        return self
+
+
+class EllipticSolver:
+    """EllipticSolver(operator; multigrid=MultigridOptions(), precision='fp64',
+    device=0): FP64 CG preconditioned by one tree-GMG V-cycle per iteration,
+    with the V-cycle in FP64 ('fp64') or FP32 ('fp32_vcycle')."""
+
+    def __init__(self, operator: PoissonOperator, *, multigrid: MultigridOptions | None = None,
+                 precision: str = "fp64", device: int = 0, options: LinearSolverOptions | None = None, **kw):
+        if options is not None:
+            precision = options.preconditioner_options.get("precision", precision)
+            device = options.preconditioner_options.get("device", device)
+            if options.preconditioner not in (":auto", ":geometric", ":tree_geometric"):
+                raise ValueError("this solver provides the tree geometric multigrid preconditioner only")
+        if multigrid is None:
+            multigrid = MultigridOptions()
+        over = {k: kw.pop(k) for k in ("coarse_size", "pre_sweeps", "post_sweeps", "max_levels") if k in kw}
+        if kw:
+            raise TypeError(f"unknown keywords {sorted(kw)}")
+        if over:
+            multigrid = MultigridOptions(**{**vars(multigrid), **over})
+        if multigrid.pre_sweeps != multigrid.post_sweeps:
+            raise ValueError("CG requires equal pre- and post-smoothing sweep counts for a geometric V-cycle")
+        prec = {"fp64": L.TPAMR_FP64, "fp32_vcycle": L.TPAMR_FP32_VCYCLE, "fp32": L.TPAMR_FP32_VCYCLE}
+        if precision not in prec:
+            raise ValueError("precision must be 'fp64' or 'fp32_vcycle'")
+        self.operator = operator
+        self.multigrid = multigrid
+        self.precision = prec[precision]
+        self.ctx = L.Context(device, self.precision)
+        self.stats = LinearSolverStats()
+        self._coeff_version = None
+        operator.mesh.attach(self.ctx)
+        opts = multigrid._c()
+        L.check(L.lib().tpamr_set_multigrid(self.ctx.h, C.byref(opts)), self.ctx.h)
+        self.refresh()
+
+    @property
+    def n(self):
+        return self.operator.mesh.ncells
+
+    def refresh(self):
+        """Upload the operator's current face coefficients (refreshes every level)."""
+        k = self.operator.face_coefficient
+        L.check(L.lib().tpamr_set_face_coefficients(self.ctx.h, L.ptr(k)), self.ctx.h)
+        self._coeff = k.copy()
+
+    def _sync(self):
+        if not np.array_equal(self._coeff, self.operator.face_coefficient):
+            self.refresh()
+
+    @property
+    def levels(self):
+        return [self.level_info(l) for l in range(L.lib().tpamr_num_levels(self.ctx.h))]
+
+    def level_info(self, level):
+        ints = np.zeros(6, np.int32)
+        dbl = np.zeros(2)
+        L.check(L.lib().tpamr_level_info(self.ctx.h, level, L.ptr(ints), L.ptr(dbl)), self.ctx.h)
+        return dict(ncells=int(ints[0]), nfaces=int(ints[1]), nblocks=int(ints[2]), irregular_rows=int(ints[3]),
+                    irregular_nnz=int(ints[4]), max_leaf_level=int(ints[5]), omega=float(dbl[0]),
+                    omega_p=float(dbl[1]))
+
+    def apply(self, x):
+        """Integrated operator A x (FactorizedEllipticOperator mul!)."""
+        self._sync()
+        x = L.f64(x, self.n, "x")
+        y = np.zeros(self.n)
+        L.check(L.lib().tpamr_apply(self.ctx.h, L.ptr(x), L.ptr(y)), self.ctx.h)
+        return y
+
+    def ldiv(self, r):
+        """One V-cycle z = B r of the tree geometric multigrid (ldiv!)."""
+        self._sync()
+        r = L.f64(r, self.n, "r")
+        z = np.zeros(self.n)
+        L.check(L.lib().tpamr_vcycle(self.ctx.h, L.ptr(r), L.ptr(z)), self.ctx.h)
+        return z
+
+    def solve(self, solution, rhs, *, rtol=1.0e-10, atol=0.0, max_cycles=100, compatibility="project"):
+        """solve!(solution, solver, rhs; ...) in place on `solution` (strong form)."""
+        self._sync()
+        if not (0 <= rtol < 1):
+            raise ValueError("rtol must lie in [0, 1)")
+        if atol < 0:
+            raise ValueError("atol must be nonnegative")
+        if max_cycles < 0:
+            raise ValueError("max_cycles must be nonnegative")
+        if compatibility not in ("project", "error"):
+            raise ValueError("compatibility must be :project or :error")
+        rhs = L.f64(rhs, self.n, "rhs")
+        if solution.dtype != np.float64 or solution.shape != (self.n,) or not solution.flags.c_contiguous:
+            raise L.DimensionMismatch("solution must be a contiguous float64 vector of one value per cell")
+        st = L.Stats()
+        L.check(L.lib().tpamr_solve(self.ctx.h, L.ptr(rhs), L.ptr(solution), float(rtol), float(atol),
+                                    int(max_cycles), 1 if compatibility == "error" else 0, C.byref(st)),
+                self.ctx.h)
+        self.stats = LinearSolverStats._from(st, self.precision)
+        return self.stats
+
+    def solve_device(self, d_rhs, d_x, *, rtol=1.0e-10, atol=0.0, max_cycles=100, compatibility="project"):
+        """solve! on device buffers (raw device pointers, e.g. torch .data_ptr())."""
+        self._sync()
+        st = L.Stats()
+        L.check(L.lib().tpamr_solve_device(self.ctx.h, C.c_void_p(d_rhs), C.c_void_p(d_x), float(rtol),
+                                           float(atol), int(max_cycles), 1 if compatibility == "error" else 0,
+                                           C.byref(st)), self.ctx.h)
+        self.stats = LinearSolverStats._from(st, self.precision)
+        return self.stats
+
+    def time_smoother(self, reps=50):
+        ms = C.c_double()
+        by = C.c_double()
+        L.check(L.lib().tpamr_time_smoother(self.ctx.h, int(reps), C.byref(ms), C.byref(by)), self.ctx.h)
+        return ms.value, by.value
+
+    def close(self):
+        self.ctx.close()
+
+
+def solve(solution, solver: EllipticSolver, rhs, **kw):
+    """Functional form of solve!: returns the LinearSolverStats."""
+    return solver.solve(solution, rhs, **kw)
+
+
+def ldiv(solver: EllipticSolver, r):
+    return solver.ldiv(r)
