@@ -529,6 +529,61 @@ void level_topology(LevelData& L) {
   }
 }
 
+// Device form of the irregular rows (see LevelData): per block, the
+// irregular-cell bitmask, the LDS slot of every row entry and the list of
+// cells outside the block and its same-level halo.
+void build_ell(LevelData& L) {
+  const TreeMesh& M = *L.mesh;
+  const int bx = M.bs[0], by = M.bs[1], nb = bx * by, W = bx + 2;
+  const int tile = (bx + 2) * (by + 2);
+  const int nwords = (nb + 31) / 32;
+  const int nrows = int(L.irr_local.size());
+  int w = 0;
+  for (int r = 0; r < nrows; ++r) w = std::max(w, L.irr_ptr[r + 1] - L.irr_ptr[r]);
+  L.ell_w = w;
+  L.irr_mask.assign(size_t(L.nblocks) * nwords, 0u);
+  L.ell_ref.assign(size_t(nrows) * w, 0);
+  L.ell_val.assign(size_t(nrows) * w, 0.0);
+  L.ell_bound.assign(size_t(nrows) * w, 0.0);
+  L.blk_ext.assign(L.nblocks + 1, 0);
+  L.ext_col.clear();
+  std::unordered_map<int, int> ext;
+  for (int b = 0; b < L.nblocks; ++b) {
+    ext.clear();
+    const int* nbr = &L.nbr[4 * b];
+    for (int r = L.blk_irr[b]; r < L.blk_irr[b + 1]; ++r) {
+      const int loc = L.irr_local[r];
+      L.irr_mask[size_t(b) * nwords + loc / 32] |= 1u << (loc % 32);
+      const int own = ((loc / bx) + 1) * W + (loc % bx) + 1;
+      for (int k = 0; k < w; ++k) L.ell_ref[size_t(r) * w + k] = uint16_t(own);
+      int k = 0;
+      for (int e = L.irr_ptr[r]; e < L.irr_ptr[r + 1]; ++e, ++k) {
+        const int col = L.irr_col[e];
+        const int cb = col / nb, cl = col % nb, ci = cl % bx, cj = cl / bx;
+        int ref;
+        if (cb == b) ref = (cj + 1) * W + ci + 1;
+        else if (cb == nbr[0] && ci == bx - 1) ref = (cj + 1) * W;
+        else if (cb == nbr[1] && ci == 0) ref = (cj + 1) * W + bx + 1;
+        else if (cb == nbr[2] && cj == by - 1) ref = ci + 1;
+        else if (cb == nbr[3] && cj == 0) ref = (by + 1) * W + ci + 1;
+        else {
+          auto it = ext.find(col);
+          if (it == ext.end()) {
+            it = ext.emplace(col, int(ext.size())).first;
+            L.ext_col.push_back(col);
+          }
+          ref = tile + it->second;
+        }
+        L.ell_ref[size_t(r) * w + k] = uint16_t(ref);
+        L.ell_val[size_t(r) * w + k] = L.irr_val[e];
+        L.ell_bound[size_t(r) * w + k] = L.irr_bound[e];
+      }
+    }
+    if (int(ext.size()) > nb) throw std::runtime_error("too many external couplings in one block");
+    L.blk_ext[b + 1] = int(L.ext_col.size());
+  }
+}
+
 // Coefficient-dependent values of a level.
 void level_values(LevelData& L) {
   const TreeMesh& M = *L.mesh;
@@ -600,6 +655,7 @@ void level_values(LevelData& L) {
       L.irr_diag[r] = diag;
       if (diag > 0) max_ratio = std::max(max_ratio, rowsum / diag);
     }
+  build_ell(L);
   // diagonal of regular rows and the Jacobi weights
   L.dinv.assign(L.ncells, 0.0);
   bool any_regular = false;

@@ -85,6 +85,15 @@ struct LevelData {
   std::vector<double> irr_diag;
   std::vector<int> cell_irr;  // per cell: irregular row index or -1
   std::vector<double> dinv;   // 1 / diag(A)
+  // Device form of the irregular rows: a bitmask of irregular cells per
+  // block (NB/32 words), fixed-width rows (ell_w entries each, zero padded)
+  // whose columns are LDS slots of the block's tile: (BX+2)(BY+2) tile and
+  // halo positions, then the block's list of external cells.
+  int ell_w = 0;
+  std::vector<uint32_t> irr_mask;
+  std::vector<uint16_t> ell_ref;
+  std::vector<double> ell_val, ell_bound;
+  std::vector<int> blk_ext, ext_col;
   double omega = 1, omega_p = 0;
   // transfer to the next coarser level (absent on the coarsest)
   std::vector<int> xfer;  // 2 per block: coarse block, mode (-1 identity, 0..3 quadrant)

@@ -97,73 +97,99 @@ struct InjectFetch {
   __device__ inline T operator()(int cell) const { return xc[owner_of<BX, BY>(cell, xfer)]; }
 };
 
-// Shared-memory tile plus irregular-row scratch.
+// Logical block of workgroup `w`: consecutive blocks (same-level
+// neighbours) go to the same XCD, whose L2 then serves the halo reads.
+// Workgroups are dispatched to the 8 XCDs round-robin.
+__device__ inline int xcd_block(int w, int G) {
+  const int q = G >> 3, rem = G & 7, k = w & 7;
+  return k * q + min(k, rem) + (w >> 3);
+}
+
+// Shared-memory tile: the block, its one-cell halo and the block's external
+// cells (irregular-row columns outside block and halo).
 template <typename T, int BX, int BY>
 struct Tile {
   static constexpr int NB = BX * BY;
   static constexpr int W = BX + 2;
-  T v[(BX + 2) * (BY + 2)];
-  T irr_ax[NB];
-  int irr_row[NB];
-  T red[16];
+  static constexpr int TILE = (BX + 2) * (BY + 2);
+  T v[TILE + NB];
 };
 
-// Load the block's values and the one-cell halo, evaluate the irregular
-// rows (sum_j A_cj v_j over their CSR entries), and leave (A v)_c and the
-// diagonal D_c for this thread's cell in `ax`, `d`. `absmode` evaluates the
-// factorized floor bound sum_j B_cj |v_j| instead.
-template <typename T, int BX, int BY, bool ABS, class Fetch>
-__device__ inline void tile_apply(Tile<T, BX, BY>& S, const LevelView<T>& L, int b, Fetch fetch, T& vc,
+// Evaluate (A v)_c and D_c for this thread's cell c of block b. All global
+// loads of the thread (own value, halo or external slot, conductances,
+// irregular row, and the caller's `pre` loads) are issued before the single
+// barrier, so one memory latency covers the whole tile. ABS evaluates the
+// factorized round-off bound sum_j B_cj |v_j| instead.
+template <typename T, int BX, int BY, bool ABS, class Fetch, class Pre>
+__device__ inline void tile_apply(Tile<T, BX, BY>& S, const LevelView<T>& L, int b, Fetch fetch, Pre pre, T& vc,
                                   T& ax, T& d) {
-  constexpr int NB = BX * BY, W = BX + 2;
+  constexpr int NB = BX * BY, W = BX + 2, TILE = (BX + 2) * (BY + 2);
+  constexpr int NW = (NB + 31) / 32;
   const int t = threadIdx.x, i = t % BX, j = t / BX;
-  const size_t base = size_t(b) * NB;
-  T own = fetch(int(base) + t);
-  if (ABS) own = fabs(own);
-  S.v[(j + 1) * W + (i + 1)] = own;
-  S.irr_row[t] = -1;
+  const int base = b * NB;
+  T own = fetch(base + t);
+  // halo slot (one per thread on the block boundary)
+  T h = 0;
+  int hslot = -1;
   if (t < BY) {
     const int nb = L.nbr[4 * b + 0];
-    T h = nb >= 0 ? fetch(nb * NB + t * BX + (BX - 1)) : T(0);
-    S.v[(t + 1) * W + 0] = ABS ? fabs(h) : h;
+    hslot = (t + 1) * W;
+    if (nb >= 0) h = fetch(nb * NB + t * BX + (BX - 1));
   } else if (t < 2 * BY) {
     const int jj = t - BY, nb = L.nbr[4 * b + 1];
-    T h = nb >= 0 ? fetch(nb * NB + jj * BX) : T(0);
-    S.v[(jj + 1) * W + BX + 1] = ABS ? fabs(h) : h;
+    hslot = (jj + 1) * W + BX + 1;
+    if (nb >= 0) h = fetch(nb * NB + jj * BX);
   } else if (t < 2 * BY + BX) {
     const int ii = t - 2 * BY, nb = L.nbr[4 * b + 2];
-    T h = nb >= 0 ? fetch(nb * NB + (BY - 1) * BX + ii) : T(0);
-    S.v[ii + 1] = ABS ? fabs(h) : h;
+    hslot = ii + 1;
+    if (nb >= 0) h = fetch(nb * NB + (BY - 1) * BX + ii);
   } else if (t < 2 * BY + 2 * BX) {
     const int ii = t - 2 * BY - BX, nb = L.nbr[4 * b + 3];
-    T h = nb >= 0 ? fetch(nb * NB + ii) : T(0);
-    S.v[(BY + 1) * W + ii + 1] = ABS ? fabs(h) : h;
+    hslot = (BY + 1) * W + ii + 1;
+    if (nb >= 0) h = fetch(nb * NB + ii);
   }
-  __syncthreads();
-  const int r0 = L.blk_irr[b], r1 = L.blk_irr[b + 1];
-  for (int r = r0 + t; r < r1; r += NB) {
-    const int loc = L.irr_local[r];
-    T s = 0;
-    const T* vals = ABS ? L.irr_bound : L.irr_val;
-    for (int e = L.irr_ptr[r]; e < L.irr_ptr[r + 1]; ++e) {
-      T xv = fetch(L.irr_col[e]);
-      if (ABS) xv = fabs(xv);
-      s += vals[e] * xv;
+  // external slot
+  const int e0 = L.blk_ext[b], ne = L.blk_ext[b + 1] - e0;
+  T ev = 0;
+  if (t < ne) ev = fetch(L.ext_col[e0 + t]);
+  // conductances of the four faces of the cell
+  const T* cx = L.cx + size_t(b) * (BX + 1) * BY;
+  const T* cy = L.cy + size_t(b) * BX * (BY + 1);
+  const T cw = cx[j * (BX + 1) + i], ce = cx[j * (BX + 1) + i + 1];
+  const T cs = cy[j * BX + i], cn = cy[(j + 1) * BX + i];
+  // irregular row of this cell, if any
+  const unsigned mword = L.irr_mask[2 * (b * NW + (t >> 5))];
+  const bool irr = (mword >> (t & 31)) & 1u;
+  T rv[ELL_MAX];
+  unsigned short rr[ELL_MAX];
+  T rdiag = 0;
+  const int w = L.ell_w;
+  if (irr) {
+    const int r = int(L.irr_mask[2 * (b * NW + (t >> 5)) + 1]) + __popc(mword & ((1u << (t & 31)) - 1u));
+    const T* vals = ABS ? L.ell_bound : L.ell_val;
+#pragma unroll
+    for (int k = 0; k < ELL_MAX; ++k) {
+      if (k < w) {
+        rv[k] = vals[size_t(r) * w + k];
+        rr[k] = L.ell_ref[size_t(r) * w + k];
+      }
     }
-    S.irr_ax[loc] = s;
-    S.irr_row[loc] = r;
+    rdiag = L.irr_diag[r];
   }
+  pre();
+  S.v[(j + 1) * W + (i + 1)] = ABS ? fabs(own) : own;
+  if (hslot >= 0) S.v[hslot] = ABS ? fabs(h) : h;
+  if (t < ne) S.v[TILE + t] = ABS ? fabs(ev) : ev;
   __syncthreads();
   vc = S.v[(j + 1) * W + (i + 1)];
-  const int r = S.irr_row[t];
-  if (r >= 0) {
-    ax = S.irr_ax[t];
-    d = L.irr_diag[r];
+  if (irr) {
+    T s = 0;
+#pragma unroll
+    for (int k = 0; k < ELL_MAX; ++k)
+      if (k < w) s += rv[k] * S.v[rr[k]];
+    ax = s;
+    d = rdiag;
   } else {
-    const T* cx = L.cx + size_t(b) * (BX + 1) * BY;
-    const T* cy = L.cy + size_t(b) * BX * (BY + 1);
-    const T cw = cx[j * (BX + 1) + i], ce = cx[j * (BX + 1) + i + 1];
-    const T cs = cy[j * BX + i], cn = cy[(j + 1) * BX + i];
     d = (cw + ce) + (cs + cn);
     const T vw = S.v[(j + 1) * W + i], ve = S.v[(j + 1) * W + i + 2];
     const T vs = S.v[j * W + i + 1], vn = S.v[(j + 2) * W + i + 1];
@@ -173,6 +199,10 @@ __device__ inline void tile_apply(Tile<T, BX, BY>& S, const LevelView<T>& L, int
       ax = d * vc - ((cw * vw + ce * ve) + (cs * vs + cn * vn));
   }
 }
+
+struct NoPre {
+  __device__ inline void operator()() const {}
+};
 
 // ---------------------------------------------------------------------------
 // multigrid level kernels
@@ -187,15 +217,20 @@ __global__ void __launch_bounds__(BX* BY) k_sweep(LevelView<T> L, const T* __res
                                                  const double* __restrict__ r64, double* __restrict__ part) {
   if (done && *done) return;
   __shared__ Tile<T, BX, BY> S;
-  const int b = blockIdx.x;
-  T vc, ax, d;
-  tile_apply<T, BX, BY, false>(S, L, b, ArrayFetch<T>{x_in}, vc, ax, d);
+  const int b = xcd_block(blockIdx.x, gridDim.x);
   const size_t c = size_t(b) * (BX * BY) + threadIdx.x;
-  const T z = vc + T(omega) * (rhs[c] - ax) / d;
+  T bv;
+  double rd = 0;
+  T vc, ax, d;
+  tile_apply<T, BX, BY, false>(S, L, b, ArrayFetch<T>{x_in}, [&]() {
+    bv = rhs[c];
+    if constexpr (STATS) rd = r64[c];
+  }, vc, ax, d);
+  const T z = vc + T(omega) * (bv - ax) / d;
   x_out[c] = z;
   if constexpr (STATS) {
     __shared__ double red[16];
-    const double zd = double(z), rd = r64[c];
+    const double zd = double(z);
     const double s_z = block_sum<double, BX * BY>(zd, red);
     const double s_rz = block_sum<double, BX * BY>(rd * zd, red);
     const double s_r = block_sum<double, BX * BY>(rd, red);
@@ -214,11 +249,12 @@ __global__ void __launch_bounds__(BX* BY) k_residual(LevelView<T> L, const T* __
                                                     T* __restrict__ u, double omega_p, const int* __restrict__ done) {
   if (done && *done) return;
   __shared__ Tile<T, BX, BY> S;
-  const int b = blockIdx.x;
-  T vc, ax, d;
-  tile_apply<T, BX, BY, false>(S, L, b, ArrayFetch<T>{x}, vc, ax, d);
+  const int b = xcd_block(blockIdx.x, gridDim.x);
   const size_t c = size_t(b) * (BX * BY) + threadIdx.x;
-  u[c] = T(omega_p) * (rhs[c] - ax) / d;
+  T bv;
+  T vc, ax, d;
+  tile_apply<T, BX, BY, false>(S, L, b, ArrayFetch<T>{x}, [&]() { bv = rhs[c]; }, vc, ax, d);
+  u[c] = T(omega_p) * (bv - ax) / d;
 }
 
 // Smoothed restriction b_c = P_t' (r - w_P S D^-1 r) with u = w_P D^-1 r:
@@ -232,11 +268,11 @@ __global__ void __launch_bounds__(BX* BY) k_restrict(LevelView<T> L, const T* __
   if (done && *done) return;
   __shared__ Tile<T, BX, BY> S;
   __shared__ T wv[BX * BY];
-  const int b = blockIdx.x, t = threadIdx.x;
-  T vc, ax, d;
-  tile_apply<T, BX, BY, false>(S, L, b, ArrayFetch<T>{u}, vc, ax, d);
-  const T w = d * vc / T(omega_p) - ax;
+  const int b = xcd_block(blockIdx.x, gridDim.x), t = threadIdx.x;
   const int cb = L.xfer[2 * b], mode = L.xfer[2 * b + 1];
+  T vc, ax, d;
+  tile_apply<T, BX, BY, false>(S, L, b, ArrayFetch<T>{u}, NoPre{}, vc, ax, d);
+  const T w = d * vc / T(omega_p) - ax;
   constexpr int NB = BX * BY;
   if (mode < 0) {
     const int cc = cb * NB + t;
@@ -263,24 +299,41 @@ __global__ void __launch_bounds__(BX* BY) k_prolong(LevelView<T> L, T* __restric
                                                    double omega_p, const int* __restrict__ done) {
   if (done && *done) return;
   __shared__ Tile<T, BX, BY> S;
-  const int b = blockIdx.x;
-  T vc, ax, d;
-  tile_apply<T, BX, BY, false>(S, L, b, InjectFetch<T, BX, BY>{xc, L.xfer}, vc, ax, d);
+  const int b = xcd_block(blockIdx.x, gridDim.x);
   const size_t c = size_t(b) * (BX * BY) + threadIdx.x;
-  x[c] = x[c] + (vc - T(omega_p) * ax / d);
+  T xv;
+  T vc, ax, d;
+  tile_apply<T, BX, BY, false>(S, L, b, InjectFetch<T, BX, BY>{xc, L.xfer}, [&]() { xv = x[c]; }, vc, ax, d);
+  x[c] = xv + (vc - T(omega_p) * ax / d);
 }
 
 // Coarsest level: x = inv * b with the dense inverse of the augmented
-// (volume-constrained) operator, column-major, evaluated in FP64.
+// (volume-constrained) operator, column-major, evaluated in FP64. A
+// workgroup owns 64 rows; its 16 column groups are summed through LDS.
 template <typename T>
-__global__ void k_coarse_solve(int n, const double* __restrict__ inv, const T* __restrict__ b, T* __restrict__ x,
-                               const int* __restrict__ done) {
+__global__ void __launch_bounds__(1024) k_coarse_solve(int n, const double* __restrict__ inv, const T* __restrict__ b,
+                                                      T* __restrict__ x, const int* __restrict__ done) {
   if (done && *done) return;
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  double s = 0;
-  for (int c = 0; c < n; ++c) s += inv[size_t(c) * n + r] * double(b[c]);
-  x[r] = T(s);
+  __shared__ double part[16][65];
+  const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int r = blockIdx.x * 64 + lane;
+  double s0 = 0, s1 = 0;
+  if (r < n) {
+    int c = g;
+    for (; c + 16 < n; c += 32) {
+      s0 += inv[size_t(c) * n + r] * double(b[c]);
+      s1 += inv[size_t(c + 16) * n + r] * double(b[c + 16]);
+    }
+    if (c < n) s0 += inv[size_t(c) * n + r] * double(b[c]);
+  }
+  part[g][lane] = s0 + s1;
+  __syncthreads();
+  if (g == 0 && r < n) {
+    double s = 0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) s += part[q][lane];
+    x[r] = T(s);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -294,9 +347,9 @@ __global__ void __launch_bounds__(BX* BY) k_matvec_dot(LevelView<double> L, cons
   if (done && *done) return;
   __shared__ Tile<double, BX, BY> S;
   __shared__ double red[16];
-  const int b = blockIdx.x;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
   double vc, ax, d;
-  tile_apply<double, BX, BY, false>(S, L, b, ArrayFetch<double>{p}, vc, ax, d);
+  tile_apply<double, BX, BY, false>(S, L, b, ArrayFetch<double>{p}, NoPre{}, vc, ax, d);
   const size_t c = size_t(b) * (BX * BY) + threadIdx.x;
   y[c] = ax;
   if (part) {
@@ -312,39 +365,54 @@ __global__ void __launch_bounds__(BX* BY) k_floor_bound(LevelView<double> L, con
                                                        const double* __restrict__ rhs, double* __restrict__ part) {
   __shared__ Tile<double, BX, BY> S;
   __shared__ double red[16];
-  const int b = blockIdx.x;
-  double vc, ax, d;
-  tile_apply<double, BX, BY, true>(S, L, b, ArrayFetch<double>{x}, vc, ax, d);
+  const int b = xcd_block(blockIdx.x, gridDim.x);
   const size_t c = size_t(b) * (BX * BY) + threadIdx.x;
+  double bv = 0;
+  double vc, ax, d;
+  tile_apply<double, BX, BY, true>(S, L, b, ArrayFetch<double>{x}, [&]() { bv = rhs[c]; }, vc, ax, d);
   const double V = L.volume[b];
-  double q = fmax(ax / V, fabs(rhs[c]) / V);
+  double q = fmax(ax / V, fabs(bv) / V);
   if (q != q) q = __longlong_as_double(0x7ff8000000000000LL);
   const double m = block_max<double, BX * BY>(q, red);
   if (threadIdx.x == 0) part[b] = m;
 }
 
-// Single-workgroup reduction of `n` partials at `part` (sum or max) into out.
-__global__ void k_reduce(int n, int nsets, const double* __restrict__ part, double* __restrict__ out, int is_max) {
+// Sum (or max) of part[0..n) by one workgroup: 8 independent accumulators
+// per thread keep many loads in flight.
+template <bool MAX>
+__device__ inline double wg_reduce(int n, const double* __restrict__ part, double* red) {
+  double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int q = threadIdx.x;
+  for (; q + 7 * 1024 < n; q += 8 * 1024) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const double v = part[q + k * 1024];
+      a[k] = MAX ? fmax(a[k], v) : a[k] + v;
+    }
+  }
+  for (; q < n; q += 1024) a[0] = MAX ? fmax(a[0], part[q]) : a[0] + part[q];
+  double v = a[0];
+#pragma unroll
+  for (int k = 1; k < 8; ++k) v = MAX ? fmax(v, a[k]) : v + a[k];
+  return MAX ? block_max<double, 1024>(v, red) : block_sum<double, 1024>(v, red);
+}
+
+// Single-workgroup reduction of `nsets` sets of `n` partials (sum or max).
+__global__ void __launch_bounds__(1024) k_reduce(int n, int nsets, const double* __restrict__ part,
+                                                 double* __restrict__ out, int is_max) {
   __shared__ double red[16];
   for (int s = 0; s < nsets; ++s) {
-    double v = 0;
-    for (int q = threadIdx.x; q < n; q += blockDim.x) {
-      const double w = part[size_t(s) * n + q];
-      v = is_max ? fmax(v, w) : v + w;
-    }
-    v = is_max ? block_max<double, 1024>(v, red) : block_sum<double, 1024>(v, red);
+    const double v = is_max ? wg_reduce<true>(n, part + size_t(s) * n, red) : wg_reduce<false>(n, part + size_t(s) * n, red);
     if (threadIdx.x == 0) out[s] = v;
     __syncthreads();
   }
 }
 
 // Krylov scalar step 1: pAp and the step length (Krylov.jl cg semantics).
-__global__ void k_cg_alpha(int nb, const double* __restrict__ part, CgState* st) {
+__global__ void __launch_bounds__(1024) k_cg_alpha(int nb, const double* __restrict__ part, CgState* st) {
   __shared__ double red[16];
   if (st->done) return;
-  double v = 0;
-  for (int q = threadIdx.x; q < nb; q += blockDim.x) v += part[q];
-  v = block_sum<double, 1024>(v, red);
+  const double v = wg_reduce<false>(nb, part, red);
   if (threadIdx.x == 0) {
     const double pAp = v;
     st->pAp = pAp;
@@ -377,7 +445,10 @@ __global__ void k_cg_update_xr(int n, int nb_cells, const double* __restrict__ p
     } else {
       const double a = st->alpha;
       x[c] += a * p[c];
-      rv = r[c] - a * Ap[c];
+      // the volume mean of r (lagged by one iteration) is removed so the
+      // recursive residual stays compatible: sum(A p) is zero only up to
+      // round-off, and that drift cannot be reduced by the iteration
+      rv = (r[c] - a * Ap[c]) - volume[c / nb_cells] * st->rmean;
       r[c] = rv;
     }
     const TV rt = TV(rv);
@@ -391,23 +462,17 @@ __global__ void k_cg_update_xr(int n, int nb_cells, const double* __restrict__ p
 }
 
 // Krylov scalar step 2: z mean removal, gamma, beta and the stopping tests.
-__global__ void k_cg_beta(int nb, int nmax, const double* __restrict__ part, const double* __restrict__ maxpart,
+__global__ void __launch_bounds__(1024) k_cg_beta(int nb, int nmax, const double* __restrict__ part, const double* __restrict__ maxpart,
                           double total_volume, CgState* st, int init) {
   __shared__ double red[16];
   if (st->done) return;
-  double sz = 0, srz = 0, sr = 0, mx = 0;
-  for (int q = threadIdx.x; q < nb; q += blockDim.x) {
-    sz += part[q];
-    srz += part[nb + q];
-    sr += part[2 * nb + q];
-  }
-  for (int q = threadIdx.x; q < nmax; q += blockDim.x) mx = fmax(mx, maxpart[q]);
-  sz = block_sum<double, 1024>(sz, red);
-  srz = block_sum<double, 1024>(srz, red);
-  sr = block_sum<double, 1024>(sr, red);
-  mx = block_max<double, 1024>(mx, red);
+  const double sz = wg_reduce<false>(nb, part, red);
+  const double srz = wg_reduce<false>(nb, part + nb, red);
+  const double sr = wg_reduce<false>(nb, part + 2 * nb, red);
+  const double mx = wg_reduce<true>(nmax, maxpart, red);
   if (threadIdx.x != 0) return;
   const double m = sz / total_volume;
+  st->rmean = sr / total_volume;
   const double g = srz - m * sr;
   st->zmean = m;
   st->strong = mx;
@@ -732,7 +797,7 @@ void launch_prolong(hipStream_t s, const LevelView<T>& L, T* x, const T* xc, dou
 
 template <typename T>
 void launch_coarse(hipStream_t s, int n, const double* inv, const T* b, T* x, const int* done) {
-  hipLaunchKernelGGL((k_coarse_solve<T>), dim3(cdiv(n, 256)), dim3(256), 0, s, n, inv, b, x, done);
+  hipLaunchKernelGGL((k_coarse_solve<T>), dim3(cdiv(n, 64)), dim3(1024), 0, s, n, inv, b, x, done);
 }
 
 void launch_matvec_dot(hipStream_t s, const LevelView<double>& L, const double* p, double* y, double* part,

@@ -4,27 +4,30 @@
 
 namespace tpamr {
 
+// Longest irregular row the kernels accept (entries of one row).
+constexpr int ELL_MAX = 12;
+
 // One multigrid level as the kernels see it (device pointers).
 template <typename T>
 struct LevelView {
-  int nblocks, ncells, bx, by;
+  int nblocks, ncells, bx, by, ell_w;
   const T* cx;
   const T* cy;
-  const T* irr_val;
-  const T* irr_bound;  // floor-bound values (FP64 finest level only)
+  const T* ell_val;
+  const T* ell_bound;  // floor-bound values (FP64 finest level only)
   const T* irr_diag;
+  const unsigned short* ell_ref;
+  const unsigned* irr_mask;  // per block and 32-cell word: (mask, first row index)
+  const int* blk_ext;
+  const int* ext_col;
   const int* nbr;
   const int* xfer;
-  const int* blk_irr;
-  const int* irr_local;
-  const int* irr_ptr;
-  const int* irr_col;
   const double* volume;  // per block
 };
 
 // Device-resident Krylov state (Krylov.jl cg semantics).
 struct CgState {
-  double gamma, pnorm2, alpha, beta, pAp, zmean, rnorm, strong, target;
+  double gamma, pnorm2, alpha, beta, pAp, zmean, rnorm, strong, target, rmean;
   int iter, itmax, done, reason;
 };
 

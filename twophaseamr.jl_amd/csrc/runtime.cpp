@@ -82,13 +82,15 @@ struct DevArray {
 
 struct DevTopo {
   int nblocks = 0, ncells = 0;
-  DevArray<int> nbr, xfer, blk_irr, irr_local, irr_ptr, irr_col;
+  DevArray<int> nbr, xfer, blk_ext, ext_col;
+  DevArray<unsigned> irr_mask;
+  DevArray<unsigned short> ell_ref;
   DevArray<double> volume;
 };
 
 template <typename T>
 struct DevLevel {
-  DevArray<T> cx, cy, irr_val, irr_bound, irr_diag, dinv;
+  DevArray<T> cx, cy, ell_val, ell_bound, irr_diag, dinv;
   DevArray<T> x, xs, b, u;
   double omega = 1, omega_p = 0;
   LevelView<T> view{};
@@ -188,8 +190,8 @@ void upload_level_values(tpamr_ctx* c, DevLevel<T>& D, const LevelData& L, const
   hipStream_t s = c->stream;
   D.cx.upload(L.cx, s);
   D.cy.upload(L.cy, s);
-  D.irr_val.upload(L.irr_val, s);
-  if (with_bound) D.irr_bound.upload(L.irr_bound, s);
+  D.ell_val.upload(L.ell_val, s);
+  if (with_bound) D.ell_bound.upload(L.ell_bound, s);
   D.irr_diag.upload(L.irr_diag, s);
   D.dinv.upload(L.dinv, s);
   D.omega = L.omega;
@@ -205,15 +207,16 @@ void upload_level_values(tpamr_ctx* c, DevLevel<T>& D, const LevelData& L, const
   v.by = L.mesh->bs[1];
   v.cx = D.cx.p;
   v.cy = D.cy.p;
-  v.irr_val = D.irr_val.p;
-  v.irr_bound = D.irr_bound.p;
+  v.ell_w = L.ell_w;
+  v.ell_val = D.ell_val.p;
+  v.ell_bound = D.ell_bound.p;
   v.irr_diag = D.irr_diag.p;
+  v.ell_ref = Tp.ell_ref.p;
+  v.irr_mask = Tp.irr_mask.p;
+  v.blk_ext = Tp.blk_ext.p;
+  v.ext_col = Tp.ext_col.p;
   v.nbr = Tp.nbr.p;
   v.xfer = Tp.xfer.p;
-  v.blk_irr = Tp.blk_irr.p;
-  v.irr_local = Tp.irr_local.p;
-  v.irr_ptr = Tp.irr_ptr.p;
-  v.irr_col = Tp.irr_col.p;
   v.volume = Tp.volume.p;
 }
 
@@ -232,10 +235,23 @@ void upload_hierarchy(tpamr_ctx* c, bool topology) {
       T.ncells = L.ncells;
       T.nbr.upload(L.nbr, s);
       T.xfer.upload(L.xfer.empty() ? std::vector<int>(2 * L.nblocks, -1) : L.xfer, s);
-      T.blk_irr.upload(L.blk_irr, s);
-      T.irr_local.upload(L.irr_local, s);
-      T.irr_ptr.upload(L.irr_ptr, s);
-      T.irr_col.upload(L.irr_col, s);
+      if (L.ell_w > ELL_MAX) throw std::runtime_error("irregular row longer than the kernels support");
+      // (mask, index of the first irregular row of the word) per 32-cell word
+      const int nw = (L.mesh->nb() + 31) / 32;
+      std::vector<unsigned> m2(2 * size_t(L.nblocks) * nw);
+      for (int b = 0; b < L.nblocks; ++b) {
+        unsigned row = unsigned(L.blk_irr[b]);
+        for (int w = 0; w < nw; ++w) {
+          const unsigned m = L.irr_mask[size_t(b) * nw + w];
+          m2[2 * (size_t(b) * nw + w)] = m;
+          m2[2 * (size_t(b) * nw + w) + 1] = row;
+          row += unsigned(__builtin_popcount(m));
+        }
+      }
+      T.irr_mask.upload(m2, s);
+      T.ell_ref.upload(L.ell_ref.empty() ? std::vector<unsigned short>(1, 0) : L.ell_ref, s);
+      T.blk_ext.upload(L.blk_ext, s);
+      T.ext_col.upload(L.ext_col.empty() ? std::vector<int>(1, 0) : L.ext_col, s);
       T.volume.upload(L.volume, s);
     }
     c->ld.clear();
@@ -1160,12 +1176,15 @@ int tpamr_time_smoother(tpamr_ctx* c, int reps, double* avg_ms, double* bytes) {
     HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     *avg_ms = double(ms) / reps;
     if (bytes) {
-      // compulsory traffic: x, b read, x' write, the two conductance arrays
-      // (including the block-boundary faces) and the irregular rows
-      double cells = double(N) * 3 * elem;
-      double cond = double(nb) * ((bx + 1) * by + bx * (by + 1)) * elem;
-      double irr = double(L.irr_col.size()) * (4 + elem) + double(L.irr_local.size()) * (4 + 4 + elem);
-      *bytes = cells + cond + irr;
+      // compulsory traffic: x, b read, x' write, the two conductance
+      // arrays (including the block-boundary faces), the irregular-cell
+      // masks and the irregular rows with their external values
+      const double cells = double(N) * 3 * elem;
+      const double cond = double(nb) * ((bx + 1) * by + bx * (by + 1)) * elem;
+      const double mask = double(nb) * ((bx * by + 31) / 32) * 8;
+      const double irr = double(L.irr_local.size()) * (double(L.ell_w) * (elem + 2) + elem) +
+                         double(L.ext_col.size()) * (4 + elem);
+      *bytes = cells + cond + mask + irr;
     }
   });
 }
