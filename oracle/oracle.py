@@ -306,3 +306,14 @@ def techref_noise(n):
     idx = np.arange(1, n + 1, dtype=np.float64)
     v = np.sin(12.9898 * idx) * 43758.5453
     return v - np.floor(v) - 0.5
+
+
+def jacobi_weight(diagonal, row_sums, skip_nonpositive=False, allow_zero_diagonal=False, clamp_weight=True,
+                  zero_weight=1.0):
+    """`_jacobi_weight!`: returns (weight, diagonal with positive entries inverted)."""
+    d = np.array(diagonal, dtype=np.float64)
+    r = np.ascontiguousarray(row_sums, dtype=np.float64)
+    w = C.c_double()
+    _check(lib().orc_jacobi_weight(len(d), _dptr(d), _dptr(r), int(skip_nonpositive), int(allow_zero_diagonal),
+                                   int(clamp_weight), C.c_double(zero_weight), C.byref(w)))
+    return w.value, d

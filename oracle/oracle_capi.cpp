@@ -21,6 +21,8 @@ Stats project(Solver& S, double* u, double* p, double dt, double tol, int max_cy
 Stats adjoint(Solver& S, const double* w, const double* p, double rtol, int max_cycles,
               double* dJ_db, double* dJ_dbeta, double* lambda_out);
 void apply_fine(Solver& S, const double* x, double* y);
+double jacobi_weight_public(std::vector<double>& invdiag, const std::vector<double>& rowsum, bool skip_nonpositive,
+                            bool allow_zero, bool clamp, double zero_weight);
 void apply_level(Solver& S, int l, const double* x, double* y);
 }  // namespace orc
 
@@ -224,6 +226,17 @@ int orc_adjoint(void* sp, const double* w, const double* p, double rtol, int max
                 double* dJ_dbeta, double* lambda, Stats* st) {
   GUARD({
     *st = adjoint(*static_cast<Solver*>(sp), w, p, rtol, max_cycles, dJ_db, dJ_dbeta, lambda);
+    return 0;
+  })
+}
+
+// `_jacobi_weight!` known answers (hierarchy.jl:8-30): diag is replaced in place
+int orc_jacobi_weight(int n, double* diag, const double* rowsum, int skip_nonpositive, int allow_zero, int clamp,
+                      double zero_weight, double* weight) {
+  GUARD({
+    std::vector<double> d(diag, diag + n), r(rowsum, rowsum + n);
+    *weight = jacobi_weight_public(d, r, skip_nonpositive, allow_zero, clamp, zero_weight);
+    std::copy(d.begin(), d.end(), diag);
     return 0;
   })
 }

@@ -1016,6 +1016,10 @@ Stats adjoint(Solver& S, const double* w, const double* p, double rtol, int max_
 }
 
 void apply_fine(Solver& S, const double* x, double* y) { factorized_mul(*S.ops[0], x, y); }
+double jacobi_weight_public(std::vector<double>& invdiag, const std::vector<double>& rowsum, bool skip_nonpositive,
+                            bool allow_zero, bool clamp, double zero_weight) {
+  return jacobi_weight(invdiag, rowsum, skip_nonpositive, allow_zero, clamp, zero_weight);
+}
 void apply_level(Solver& S, int l, const double* x, double* y) { csc_sym_mul(S.levels[l].op->A, x, y); }
 
 }  // namespace orc

@@ -1,0 +1,52 @@
+"""The library's host hierarchy builder against the oracle (CPU, host-only
+context): identical block and face schedules, multigrid level sizes, Jacobi
+weights, level operators, tentative/smoothed transfers and restricted face
+coefficients."""
+import numpy as np
+import pytest
+
+import tpamr
+from cases import MESHES
+from oracle import OMesh, OSolver
+
+CASES = [("uniform64", "one"), ("periodic_tg", "one"), ("amr32", "one"), ("amr64", "one"),
+         ("amr_mixed", "smooth"), ("pruned_tree", "smooth"), ("circle16", "smooth")]
+
+
+def kappa(m, kind):
+    return np.ones(m.nfaces) if kind == "one" else np.exp(0.4 * np.sin(np.arange(1, m.nfaces + 1)))
+
+
+@pytest.mark.parametrize("name,kind", CASES)
+def test_hierarchy_matches_oracle(name, kind):
+    kw = MESHES[name]
+    om, pm = OMesh(**kw), tpamr.BlockTreeMesh(**kw)
+    assert (om.ncells, om.nfaces, om.nbfaces) == (pm.ncells, pm.nfaces, pm.nboundary_faces)
+    assert np.array_equal(om.block_keys, pm.block_keys)
+    for a, b in [(om.face_minus, pm.face_minus), (om.face_plus, pm.face_plus), (om.face_axis, pm.face_axis),
+                 (om.face_area, pm.face_area), (om.bface_cell, pm.bface_cell), (om.bface_side, pm.bface_side)]:
+        assert np.array_equal(a, b)
+    k = kappa(pm, kind)
+    os_, ph = OSolver(om, k), tpamr.HostHierarchy(pm, k)
+    assert [os_.level_info(l)["ncells"] for l in range(os_.nlevels)] == ph.sizes()
+    for l in range(os_.nlevels):
+        A, B = os_.level_matrix(l).tocsr(), ph.level_matrix(l)
+        assert abs(A - B).max() <= 1e-15 * abs(A).max()
+        assert abs(os_.level_info(l)["omega"] - ph.level_info(l)["omega"]) <= 1e-15
+        assert np.allclose(os_.level_kappa(l), ph.level_kappa(l), rtol=1e-14, atol=0)
+        if l + 1 < os_.nlevels:
+            assert abs(os_.level_matrix(l, "P").tocsr() - ph.prolongation(l)).max() <= 1e-15
+
+
+def test_invalid_inputs_raise():
+    with pytest.raises(ValueError):
+        tpamr.BlockTreeMesh(block_size=(6, 6), base_mesh=(12, 9))
+    with pytest.raises(ValueError):
+        tpamr.BlockTreeMesh(block_size=(5, 5))
+    m = tpamr.BlockTreeMesh(block_size=(8, 8))
+    with pytest.raises(ValueError):
+        tpamr.HostHierarchy(m, np.zeros(m.nfaces))  # disconnected components are rejected
+    with pytest.raises(ValueError):
+        tpamr.MultigridOptions(coarse_size=0)
+    with pytest.raises(ValueError):
+        tpamr.LinearSolverOptions(krylov="cg")
