@@ -657,6 +657,25 @@ void level_values(LevelData& L) {
     }
   build_ell(L);
   // diagonal of regular rows and the Jacobi weights
+  {
+    const size_t nbk = size_t(L.nblocks);
+    L.cw.assign(nbk * nb, 0.0);
+    L.cs.assign(nbk * nb, 0.0);
+    L.ce.assign(nbk * by, 0.0);
+    L.cn.assign(nbk * bx, 0.0);
+    for (size_t b = 0; b < nbk; ++b) {
+      const double* cx = &L.cx[b * (bx + 1) * by];
+      const double* cy = &L.cy[b * bx * (by + 1)];
+      for (int j = 0; j < by; ++j) {
+        for (int i = 0; i < bx; ++i) {
+          L.cw[b * nb + j * bx + i] = cx[j * (bx + 1) + i];
+          L.cs[b * nb + j * bx + i] = cy[j * bx + i];
+        }
+        L.ce[b * by + j] = cx[j * (bx + 1) + bx];
+      }
+      for (int i = 0; i < bx; ++i) L.cn[b * bx + i] = cy[by * bx + i];
+    }
+  }
   L.dinv.assign(L.ncells, 0.0);
   bool any_regular = false;
   for (int b = 0; b < L.nblocks; ++b)

@@ -74,6 +74,10 @@ struct LevelData {
   // face -> conductance slots: x faces in cx[(bx+1)*by] per block, y faces in cy[bx*(by+1)]
   std::vector<int64_t> slot_a, slot_b;  // flat indices (or -1)
   std::vector<double> cx, cy;           // conductances (0 where unused)
+  // aligned device layout of the same conductances: west face of every cell
+  // (nb per block), east face of the last column (by per block), south face
+  // of every cell (nb per block), north face of the top row (bx per block)
+  std::vector<double> cw, ce, cs, cn;
   std::vector<int> nbr;                 // 4 per block: W, E, S, N same-level neighbour or -1
   std::vector<double> volume;           // per block
   // irregular rows, grouped by block

@@ -4,12 +4,13 @@
 //
 // Data layout (every multigrid level): cells are block-major, BX x BY cells
 // per block, j-major inside a block (the reference's cell order,
-// src/mesh/topology/mesh.jl:28-30). One workgroup owns one block: one
-// thread per cell, an LDS tile with a one-cell halo gathered from the four
-// same-level neighbour blocks. Conductances of regular faces live per block
-// in cx[(BX+1)*BY] (x faces) and cy[BX*(BY+1)] (y faces); rows next to
-// coarse-fine interfaces or off-diagonal metric entries are explicit CSR rows
-// grouped per block.
+// src/mesh/topology/mesh.jl:28-30). A thread owns four consecutive cells of
+// one block row; a 256-thread workgroup owns 256/(BX*BY/4) blocks, each with
+// an LDS tile and a one-cell halo gathered from the four same-level neighbour
+// blocks. Conductances of regular faces are stored like the cells (west and
+// south face of each cell) plus the east column and north row of each block;
+// rows next to coarse-fine interfaces or off-diagonal metric entries are
+// fixed-width rows grouped per block whose columns are LDS slots.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -83,20 +84,6 @@ __device__ inline int owner_of(int cell, const int* __restrict__ xfer) {
   return cb * NB + cj * BX + ci;
 }
 
-// Value fetchers for the tile: a plain array, or the injected coarse value.
-template <typename T>
-struct ArrayFetch {
-  const T* __restrict__ v;
-  __device__ inline T operator()(int cell) const { return v[cell]; }
-};
-
-template <typename T, int BX, int BY>
-struct InjectFetch {
-  const T* __restrict__ xc;
-  const int* __restrict__ xfer;
-  __device__ inline T operator()(int cell) const { return xc[owner_of<BX, BY>(cell, xfer)]; }
-};
-
 // Logical block of workgroup `w`: consecutive blocks (same-level
 // neighbours) go to the same XCD, whose L2 then serves the halo reads.
 // Workgroups are dispatched to the 8 XCDs round-robin.
@@ -105,207 +92,10 @@ __device__ inline int xcd_block(int w, int G) {
   return k * q + min(k, rem) + (w >> 3);
 }
 
-// Shared-memory tile: the block, its one-cell halo and the block's external
-// cells (irregular-row columns outside block and halo).
-template <typename T, int BX, int BY>
-struct Tile {
-  static constexpr int NB = BX * BY;
-  static constexpr int W = BX + 2;
-  static constexpr int TILE = (BX + 2) * (BY + 2);
-  T v[TILE + NB];
-};
-
-// Evaluate (A v)_c and D_c for this thread's cell c of block b. All global
-// loads of the thread (own value, halo or external slot, conductances,
-// irregular row, and the caller's `pre` loads) are issued before the single
-// barrier, so one memory latency covers the whole tile. ABS evaluates the
-// factorized round-off bound sum_j B_cj |v_j| instead.
-template <typename T, int BX, int BY, bool ABS, class Fetch, class Pre>
-__device__ inline void tile_apply(Tile<T, BX, BY>& S, const LevelView<T>& L, int b, Fetch fetch, Pre pre, T& vc,
-                                  T& ax, T& d) {
-  constexpr int NB = BX * BY, W = BX + 2, TILE = (BX + 2) * (BY + 2);
-  constexpr int NW = (NB + 31) / 32;
-  const int t = threadIdx.x, i = t % BX, j = t / BX;
-  const int base = b * NB;
-  T own = fetch(base + t);
-  // halo slot (one per thread on the block boundary)
-  T h = 0;
-  int hslot = -1;
-  if (t < BY) {
-    const int nb = L.nbr[4 * b + 0];
-    hslot = (t + 1) * W;
-    if (nb >= 0) h = fetch(nb * NB + t * BX + (BX - 1));
-  } else if (t < 2 * BY) {
-    const int jj = t - BY, nb = L.nbr[4 * b + 1];
-    hslot = (jj + 1) * W + BX + 1;
-    if (nb >= 0) h = fetch(nb * NB + jj * BX);
-  } else if (t < 2 * BY + BX) {
-    const int ii = t - 2 * BY, nb = L.nbr[4 * b + 2];
-    hslot = ii + 1;
-    if (nb >= 0) h = fetch(nb * NB + (BY - 1) * BX + ii);
-  } else if (t < 2 * BY + 2 * BX) {
-    const int ii = t - 2 * BY - BX, nb = L.nbr[4 * b + 3];
-    hslot = (BY + 1) * W + ii + 1;
-    if (nb >= 0) h = fetch(nb * NB + ii);
-  }
-  // external slot
-  const int e0 = L.blk_ext[b], ne = L.blk_ext[b + 1] - e0;
-  T ev = 0;
-  if (t < ne) ev = fetch(L.ext_col[e0 + t]);
-  // conductances of the four faces of the cell
-  const T* cx = L.cx + size_t(b) * (BX + 1) * BY;
-  const T* cy = L.cy + size_t(b) * BX * (BY + 1);
-  const T cw = cx[j * (BX + 1) + i], ce = cx[j * (BX + 1) + i + 1];
-  const T cs = cy[j * BX + i], cn = cy[(j + 1) * BX + i];
-  // irregular row of this cell, if any
-  const unsigned mword = L.irr_mask[2 * (b * NW + (t >> 5))];
-  const bool irr = (mword >> (t & 31)) & 1u;
-  T rv[ELL_MAX];
-  unsigned short rr[ELL_MAX];
-  T rdiag = 0;
-  const int w = L.ell_w;
-  if (irr) {
-    const int r = int(L.irr_mask[2 * (b * NW + (t >> 5)) + 1]) + __popc(mword & ((1u << (t & 31)) - 1u));
-    const T* vals = ABS ? L.ell_bound : L.ell_val;
-#pragma unroll
-    for (int k = 0; k < ELL_MAX; ++k) {
-      if (k < w) {
-        rv[k] = vals[size_t(r) * w + k];
-        rr[k] = L.ell_ref[size_t(r) * w + k];
-      }
-    }
-    rdiag = L.irr_diag[r];
-  }
-  pre();
-  S.v[(j + 1) * W + (i + 1)] = ABS ? fabs(own) : own;
-  if (hslot >= 0) S.v[hslot] = ABS ? fabs(h) : h;
-  if (t < ne) S.v[TILE + t] = ABS ? fabs(ev) : ev;
-  __syncthreads();
-  vc = S.v[(j + 1) * W + (i + 1)];
-  if (irr) {
-    T s = 0;
-#pragma unroll
-    for (int k = 0; k < ELL_MAX; ++k)
-      if (k < w) s += rv[k] * S.v[rr[k]];
-    ax = s;
-    d = rdiag;
-  } else {
-    d = (cw + ce) + (cs + cn);
-    const T vw = S.v[(j + 1) * W + i], ve = S.v[(j + 1) * W + i + 2];
-    const T vs = S.v[j * W + i + 1], vn = S.v[(j + 2) * W + i + 1];
-    if (ABS)
-      ax = d * vc + ((cw * vw + ce * ve) + (cs * vs + cn * vn));
-    else
-      ax = d * vc - ((cw * vw + ce * ve) + (cs * vs + cn * vn));
-  }
-}
-
-struct NoPre {
-  __device__ inline void operator()() const {}
-};
+#include "tile.hpp"
 
 // ---------------------------------------------------------------------------
-// multigrid level kernels
-
-// Damped Jacobi sweep x_out = x_in + w D^-1 (b - A x_in). With STATS (finest
-// level, last sweep of the V-cycle) the block also emits partial sums of
-// V z, r z and r for the Krylov update, where z = x_out and r is the FP64
-// outer residual.
-template <typename T, int BX, int BY, bool FINE, bool STATS>
-__global__ void __launch_bounds__(BX* BY) k_sweep(LevelView<T> L, const T* __restrict__ x_in, T* __restrict__ x_out,
-                                                 const T* __restrict__ rhs, double omega, const int* __restrict__ done,
-                                                 const double* __restrict__ r64, double* __restrict__ part) {
-  if (done && *done) return;
-  __shared__ Tile<T, BX, BY> S;
-  const int b = xcd_block(blockIdx.x, gridDim.x);
-  const size_t c = size_t(b) * (BX * BY) + threadIdx.x;
-  T bv;
-  double rd = 0;
-  T vc, ax, d;
-  tile_apply<T, BX, BY, false>(S, L, b, ArrayFetch<T>{x_in}, [&]() {
-    bv = rhs[c];
-    if constexpr (STATS) rd = r64[c];
-  }, vc, ax, d);
-  const T z = vc + T(omega) * (bv - ax) / d;
-  x_out[c] = z;
-  if constexpr (STATS) {
-    __shared__ double red[16];
-    const double zd = double(z);
-    const double s_z = block_sum<double, BX * BY>(zd, red);
-    const double s_rz = block_sum<double, BX * BY>(rd * zd, red);
-    const double s_r = block_sum<double, BX * BY>(rd, red);
-    if (threadIdx.x == 0) {
-      const int nb = gridDim.x;
-      part[b] = s_z * L.volume[b];
-      part[nb + b] = s_rz;
-      part[2 * nb + b] = s_r;
-    }
-  }
-}
-
-// u = w_P D^-1 (b - A x): the scaled residual entering the smoothed restriction.
-template <typename T, int BX, int BY, bool FINE>
-__global__ void __launch_bounds__(BX* BY) k_residual(LevelView<T> L, const T* __restrict__ x, const T* __restrict__ rhs,
-                                                    T* __restrict__ u, double omega_p, const int* __restrict__ done) {
-  if (done && *done) return;
-  __shared__ Tile<T, BX, BY> S;
-  const int b = xcd_block(blockIdx.x, gridDim.x);
-  const size_t c = size_t(b) * (BX * BY) + threadIdx.x;
-  T bv;
-  T vc, ax, d;
-  tile_apply<T, BX, BY, false>(S, L, b, ArrayFetch<T>{x}, [&]() { bv = rhs[c]; }, vc, ax, d);
-  u[c] = T(omega_p) * (bv - ax) / d;
-}
-
-// Smoothed restriction b_c = P_t' (r - w_P S D^-1 r) with u = w_P D^-1 r:
-// w = D u / w_P - S u on the fine block, summed over the tentative owner.
-// Also starts the coarse smoother from zero: x_c = w_c D_c^-1 b_c.
-template <typename T, int BX, int BY, bool FINE>
-__global__ void __launch_bounds__(BX* BY) k_restrict(LevelView<T> L, const T* __restrict__ u, double omega_p,
-                                                    T* __restrict__ bc, T* __restrict__ xc,
-                                                    const T* __restrict__ dinv_c, double omega_c, int zero_start,
-                                                    const int* __restrict__ done) {
-  if (done && *done) return;
-  __shared__ Tile<T, BX, BY> S;
-  __shared__ T wv[BX * BY];
-  const int b = xcd_block(blockIdx.x, gridDim.x), t = threadIdx.x;
-  const int cb = L.xfer[2 * b], mode = L.xfer[2 * b + 1];
-  T vc, ax, d;
-  tile_apply<T, BX, BY, false>(S, L, b, ArrayFetch<T>{u}, NoPre{}, vc, ax, d);
-  const T w = d * vc / T(omega_p) - ax;
-  constexpr int NB = BX * BY;
-  if (mode < 0) {
-    const int cc = cb * NB + t;
-    bc[cc] = w;
-    xc[cc] = zero_start ? T(omega_c) * dinv_c[cc] * w : T(0);
-    return;
-  }
-  wv[t] = w;
-  __syncthreads();
-  const int i = t % BX, j = t / BX;
-  if (i < BX / 2 && j < BY / 2) {
-    const T s = (wv[(2 * j) * BX + 2 * i] + wv[(2 * j) * BX + 2 * i + 1]) +
-                (wv[(2 * j + 1) * BX + 2 * i] + wv[(2 * j + 1) * BX + 2 * i + 1]);
-    const int ci = (mode & 1) * (BX / 2) + i, cj = (mode >> 1) * (BY / 2) + j;
-    const int cc = cb * NB + cj * BX + ci;
-    bc[cc] = s;
-    xc[cc] = zero_start ? T(omega_c) * dinv_c[cc] * s : T(0);
-  }
-}
-
-// Smoothed prolongation and correction: t = P_t x_c, x += t - w_P D^-1 S t.
-template <typename T, int BX, int BY, bool FINE>
-__global__ void __launch_bounds__(BX* BY) k_prolong(LevelView<T> L, T* __restrict__ x, const T* __restrict__ xc,
-                                                   double omega_p, const int* __restrict__ done) {
-  if (done && *done) return;
-  __shared__ Tile<T, BX, BY> S;
-  const int b = xcd_block(blockIdx.x, gridDim.x);
-  const size_t c = size_t(b) * (BX * BY) + threadIdx.x;
-  T xv;
-  T vc, ax, d;
-  tile_apply<T, BX, BY, false>(S, L, b, InjectFetch<T, BX, BY>{xc, L.xfer}, [&]() { xv = x[c]; }, vc, ax, d);
-  x[c] = xv + (vc - T(omega_p) * ax / d);
-}
+// coarsest level
 
 // Coarsest level: x = inv * b with the dense inverse of the augmented
 // (volume-constrained) operator, column-major, evaluated in FP64. A
@@ -338,44 +128,6 @@ __global__ void __launch_bounds__(1024) k_coarse_solve(int n, const double* __re
 
 // ---------------------------------------------------------------------------
 // FP64 outer Krylov kernels
-
-// y = A p (FP64, finest level) with per-block partial p.y.
-template <int BX, int BY>
-__global__ void __launch_bounds__(BX* BY) k_matvec_dot(LevelView<double> L, const double* __restrict__ p,
-                                                      double* __restrict__ y, double* __restrict__ part,
-                                                      const int* __restrict__ done) {
-  if (done && *done) return;
-  __shared__ Tile<double, BX, BY> S;
-  __shared__ double red[16];
-  const int b = xcd_block(blockIdx.x, gridDim.x);
-  double vc, ax, d;
-  tile_apply<double, BX, BY, false>(S, L, b, ArrayFetch<double>{p}, NoPre{}, vc, ax, d);
-  const size_t c = size_t(b) * (BX * BY) + threadIdx.x;
-  y[c] = ax;
-  if (part) {
-    const double s = block_sum<double, BX * BY>(vc * ax, red);
-    if (threadIdx.x == 0) part[b] = s;
-  }
-}
-
-// Factorized round-off bound sum_j B_cj |x_j| per cell, and the per-block
-// maximum of max(bound/V, |b|/V).
-template <int BX, int BY>
-__global__ void __launch_bounds__(BX* BY) k_floor_bound(LevelView<double> L, const double* __restrict__ x,
-                                                       const double* __restrict__ rhs, double* __restrict__ part) {
-  __shared__ Tile<double, BX, BY> S;
-  __shared__ double red[16];
-  const int b = xcd_block(blockIdx.x, gridDim.x);
-  const size_t c = size_t(b) * (BX * BY) + threadIdx.x;
-  double bv = 0;
-  double vc, ax, d;
-  tile_apply<double, BX, BY, true>(S, L, b, ArrayFetch<double>{x}, [&]() { bv = rhs[c]; }, vc, ax, d);
-  const double V = L.volume[b];
-  double q = fmax(ax / V, fabs(bv) / V);
-  if (q != q) q = __longlong_as_double(0x7ff8000000000000LL);
-  const double m = block_max<double, BX * BY>(q, red);
-  if (threadIdx.x == 0) part[b] = m;
-}
 
 // Sum (or max) of part[0..n) by one workgroup: 8 independent accumulators
 // per thread keep many loads in flight.
@@ -724,22 +476,6 @@ __global__ void k_fill_int(int n, int v, int* __restrict__ out) {
 
 static inline int cdiv(long a, long b) { return int((a + b - 1) / b); }
 
-template <typename T, int BX, int BY>
-static void launch_sweep_t(hipStream_t s, const LevelView<T>& L, const T* xi, T* xo, const T* b, double w,
-                           const int* done, bool fine, const double* r64, double* part) {
-  const int nb = L.nblocks;
-  if (part) {
-    hipLaunchKernelGGL((k_sweep<T, BX, BY, true, true>), dim3(nb), dim3(BX * BY), 0, s, L, xi, xo, b, w, done, r64,
-                       part);
-  } else if (fine) {
-    hipLaunchKernelGGL((k_sweep<T, BX, BY, true, false>), dim3(nb), dim3(BX * BY), 0, s, L, xi, xo, b, w, done,
-                       nullptr, nullptr);
-  } else {
-    hipLaunchKernelGGL((k_sweep<T, BX, BY, false, false>), dim3(nb), dim3(BX * BY), 0, s, L, xi, xo, b, w, done,
-                       nullptr, nullptr);
-  }
-}
-
 #define TPAMR_DISPATCH(BXV, CALL)                                  \
   switch (BXV) {                                                   \
     case 4: { constexpr int BX = 4, BY = 4; CALL; } break;         \
@@ -751,6 +487,29 @@ static void launch_sweep_t(hipStream_t s, const LevelView<T>& L, const T* xi, T*
 
 bool supported_block(int bx, int by) { return bx == by && (bx == 4 || bx == 8 || bx == 16 || bx == 32); }
 
+// grid and workgroup of the block-tile kernels
+template <int BX, int BY>
+static inline dim3 tile_grid(int nblocks) {
+  return dim3(cdiv(nblocks, Geo<BX, BY>::BPW));
+}
+template <int BX, int BY>
+static inline dim3 tile_wg() {
+  return dim3(Geo<BX, BY>::WG);
+}
+
+template <typename T, int BX, int BY>
+static void launch_sweep_t(hipStream_t s, const LevelView<T>& L, const T* xi, T* xo, const T* b, double w,
+                           const int* done, bool fine, const double* r64, double* part) {
+  const dim3 g = tile_grid<BX, BY>(L.nblocks), t = tile_wg<BX, BY>();
+  if (part) {
+    hipLaunchKernelGGL((k_sweep<T, BX, BY, true, true>), g, t, 0, s, L, xi, xo, b, w, done, r64, part);
+  } else if (fine) {
+    hipLaunchKernelGGL((k_sweep<T, BX, BY, true, false>), g, t, 0, s, L, xi, xo, b, w, done, nullptr, nullptr);
+  } else {
+    hipLaunchKernelGGL((k_sweep<T, BX, BY, false, false>), g, t, 0, s, L, xi, xo, b, w, done, nullptr, nullptr);
+  }
+}
+
 template <typename T>
 void launch_sweep(hipStream_t s, const LevelView<T>& L, const T* xi, T* xo, const T* b, double w, const int* done,
                   bool fine, const double* r64, double* part) {
@@ -761,12 +520,11 @@ template <typename T>
 void launch_residual(hipStream_t s, const LevelView<T>& L, const T* x, const T* b, T* u, double wp, const int* done,
                      bool fine) {
   TPAMR_DISPATCH(L.bx, ({
+                   const dim3 g = tile_grid<BX, BY>(L.nblocks), t = tile_wg<BX, BY>();
                    if (fine)
-                     hipLaunchKernelGGL((k_residual<T, BX, BY, true>), dim3(L.nblocks), dim3(BX * BY), 0, s, L, x, b,
-                                        u, wp, done);
+                     hipLaunchKernelGGL((k_residual<T, BX, BY, true>), g, t, 0, s, L, x, b, u, wp, done);
                    else
-                     hipLaunchKernelGGL((k_residual<T, BX, BY, false>), dim3(L.nblocks), dim3(BX * BY), 0, s, L, x,
-                                        b, u, wp, done);
+                     hipLaunchKernelGGL((k_residual<T, BX, BY, false>), g, t, 0, s, L, x, b, u, wp, done);
                  }));
 }
 
@@ -774,24 +532,24 @@ template <typename T>
 void launch_restrict(hipStream_t s, const LevelView<T>& L, const T* u, double wp, T* bc, T* xc, const T* dinv_c,
                      double wc, int zero_start, const int* done, bool fine) {
   TPAMR_DISPATCH(L.bx, ({
+                   const dim3 g = tile_grid<BX, BY>(L.nblocks), t = tile_wg<BX, BY>();
                    if (fine)
-                     hipLaunchKernelGGL((k_restrict<T, BX, BY, true>), dim3(L.nblocks), dim3(BX * BY), 0, s, L, u, wp,
-                                        bc, xc, dinv_c, wc, zero_start, done);
+                     hipLaunchKernelGGL((k_restrict<T, BX, BY, true>), g, t, 0, s, L, u, wp, bc, xc, dinv_c, wc,
+                                        zero_start, done);
                    else
-                     hipLaunchKernelGGL((k_restrict<T, BX, BY, false>), dim3(L.nblocks), dim3(BX * BY), 0, s, L, u,
-                                        wp, bc, xc, dinv_c, wc, zero_start, done);
+                     hipLaunchKernelGGL((k_restrict<T, BX, BY, false>), g, t, 0, s, L, u, wp, bc, xc, dinv_c, wc,
+                                        zero_start, done);
                  }));
 }
 
 template <typename T>
 void launch_prolong(hipStream_t s, const LevelView<T>& L, T* x, const T* xc, double wp, const int* done, bool fine) {
   TPAMR_DISPATCH(L.bx, ({
+                   const dim3 g = tile_grid<BX, BY>(L.nblocks), t = tile_wg<BX, BY>();
                    if (fine)
-                     hipLaunchKernelGGL((k_prolong<T, BX, BY, true>), dim3(L.nblocks), dim3(BX * BY), 0, s, L, x, xc,
-                                        wp, done);
+                     hipLaunchKernelGGL((k_prolong<T, BX, BY, true>), g, t, 0, s, L, x, xc, wp, done);
                    else
-                     hipLaunchKernelGGL((k_prolong<T, BX, BY, false>), dim3(L.nblocks), dim3(BX * BY), 0, s, L, x, xc,
-                                        wp, done);
+                     hipLaunchKernelGGL((k_prolong<T, BX, BY, false>), g, t, 0, s, L, x, xc, wp, done);
                  }));
 }
 
@@ -802,13 +560,17 @@ void launch_coarse(hipStream_t s, int n, const double* inv, const T* b, T* x, co
 
 void launch_matvec_dot(hipStream_t s, const LevelView<double>& L, const double* p, double* y, double* part,
                        const int* done) {
-  TPAMR_DISPATCH(L.bx, ({ hipLaunchKernelGGL((k_matvec_dot<BX, BY>), dim3(L.nblocks), dim3(BX * BY), 0, s, L, p, y,
-                                           part, done); }));
+  TPAMR_DISPATCH(L.bx, ({
+                   const dim3 g = tile_grid<BX, BY>(L.nblocks), t = tile_wg<BX, BY>();
+                   hipLaunchKernelGGL((k_matvec_dot<BX, BY>), g, t, 0, s, L, p, y, part, done);
+                 }));
 }
 
 void launch_floor_bound(hipStream_t s, const LevelView<double>& L, const double* x, const double* b, double* part) {
-  TPAMR_DISPATCH(L.bx, ({ hipLaunchKernelGGL((k_floor_bound<BX, BY>), dim3(L.nblocks), dim3(BX * BY), 0, s, L, x, b,
-                                           part); }));
+  TPAMR_DISPATCH(L.bx, ({
+                   const dim3 g = tile_grid<BX, BY>(L.nblocks), t = tile_wg<BX, BY>();
+                   hipLaunchKernelGGL((k_floor_bound<BX, BY>), g, t, 0, s, L, x, b, part);
+                 }));
 }
 
 void launch_reduce(hipStream_t s, int n, int nsets, const double* part, double* out, bool is_max) {

@@ -11,13 +11,19 @@ constexpr int ELL_MAX = 12;
 template <typename T>
 struct LevelView {
   int nblocks, ncells, bx, by, ell_w;
-  const T* cx;
-  const T* cy;
+  // conductances: west and south face of every cell (block-major, like the
+  // cells), east face of each block's last column, north face of its top row
+  const T* cw;
+  const T* ce;
+  const T* cs;
+  const T* cn;
   const T* ell_val;
   const T* ell_bound;  // floor-bound values (FP64 finest level only)
   const T* irr_diag;
   const unsigned short* ell_ref;
   const unsigned* irr_mask;  // per block and 32-cell word: (mask, first row index)
+  const int* blk_irr;        // nblocks + 1 offsets of the irregular rows per block
+  const int* irr_local;      // local cell of every irregular row
   const int* blk_ext;
   const int* ext_col;
   const int* nbr;

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -82,7 +83,7 @@ struct DevArray {
 
 struct DevTopo {
   int nblocks = 0, ncells = 0;
-  DevArray<int> nbr, xfer, blk_ext, ext_col;
+  DevArray<int> nbr, xfer, blk_ext, ext_col, blk_irr, irr_local;
   DevArray<unsigned> irr_mask;
   DevArray<unsigned short> ell_ref;
   DevArray<double> volume;
@@ -90,7 +91,7 @@ struct DevTopo {
 
 template <typename T>
 struct DevLevel {
-  DevArray<T> cx, cy, ell_val, ell_bound, irr_diag, dinv;
+  DevArray<T> cw, ce, cs, cn, ell_val, ell_bound, irr_diag, dinv;
   DevArray<T> x, xs, b, u;
   double omega = 1, omega_p = 0;
   LevelView<T> view{};
@@ -188,8 +189,10 @@ void destroy_graph(tpamr_ctx* c) {
 template <typename T>
 void upload_level_values(tpamr_ctx* c, DevLevel<T>& D, const LevelData& L, const DevTopo& Tp, bool with_bound) {
   hipStream_t s = c->stream;
-  D.cx.upload(L.cx, s);
-  D.cy.upload(L.cy, s);
+  D.cw.upload(L.cw, s);
+  D.ce.upload(L.ce, s);
+  D.cs.upload(L.cs, s);
+  D.cn.upload(L.cn, s);
   D.ell_val.upload(L.ell_val, s);
   if (with_bound) D.ell_bound.upload(L.ell_bound, s);
   D.irr_diag.upload(L.irr_diag, s);
@@ -205,14 +208,18 @@ void upload_level_values(tpamr_ctx* c, DevLevel<T>& D, const LevelData& L, const
   v.ncells = L.ncells;
   v.bx = L.mesh->bs[0];
   v.by = L.mesh->bs[1];
-  v.cx = D.cx.p;
-  v.cy = D.cy.p;
+  v.cw = D.cw.p;
+  v.ce = D.ce.p;
+  v.cs = D.cs.p;
+  v.cn = D.cn.p;
   v.ell_w = L.ell_w;
   v.ell_val = D.ell_val.p;
   v.ell_bound = D.ell_bound.p;
   v.irr_diag = D.irr_diag.p;
   v.ell_ref = Tp.ell_ref.p;
   v.irr_mask = Tp.irr_mask.p;
+  v.blk_irr = Tp.blk_irr.p;
+  v.irr_local = Tp.irr_local.p;
   v.blk_ext = Tp.blk_ext.p;
   v.ext_col = Tp.ext_col.p;
   v.nbr = Tp.nbr.p;
@@ -251,6 +258,8 @@ void upload_hierarchy(tpamr_ctx* c, bool topology) {
       T.irr_mask.upload(m2, s);
       T.ell_ref.upload(L.ell_ref.empty() ? std::vector<unsigned short>(1, 0) : L.ell_ref, s);
       T.blk_ext.upload(L.blk_ext, s);
+      T.blk_irr.upload(L.blk_irr, s);
+      T.irr_local.upload(L.irr_local.empty() ? std::vector<int>(1, 0) : L.irr_local, s);
       T.ext_col.upload(L.ext_col.empty() ? std::vector<int>(1, 0) : L.ext_col, s);
       T.volume.upload(L.volume, s);
     }
@@ -979,7 +988,17 @@ int tpamr_apply_device(tpamr_ctx* c, const double* d_x, double* d_y) {
     need_device(c);
     need_solver(c);
     HIPCHECK(hipSetDevice(c->device));
-    matvec(c, d_x, d_y);
+    // the tile kernels move four cells per 16-byte (FP32) or 2x16-byte
+    // (FP64) access: misaligned caller buffers go through library buffers
+    const bool aligned = ((reinterpret_cast<uintptr_t>(d_x) | reinterpret_cast<uintptr_t>(d_y)) & 31u) == 0;
+    if (aligned) {
+      matvec(c, d_x, d_y);
+    } else {
+      const size_t N = size_t(c->topo[0].ncells);
+      HIPCHECK(hipMemcpyAsync(c->tmp.p, d_x, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      matvec(c, c->tmp.p, c->tmp2.p);
+      HIPCHECK(hipMemcpyAsync(d_y, c->tmp2.p, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
     HIPCHECK(hipStreamSynchronize(c->stream));
   });
 }

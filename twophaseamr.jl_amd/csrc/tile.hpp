@@ -1,0 +1,473 @@
+// Vectorised block tiles (included by kernels.hip inside namespace tpamr).
+//
+// A thread owns a 4-cell segment of one block row and moves it with 16-byte
+// loads and stores; a 256-thread workgroup holds BPW = 256 / (NB/4) blocks
+// (one wavefront per 16x16 block). Phase A issues every global load of the
+// thread (segment, conductances, halo or external slot, its irregular row,
+// the caller's operands); one barrier; phase B lets one lane per irregular
+// row sum its fixed-width row from LDS; phase C evaluates the 5-point stencil
+// of the four cells from LDS and registers.
+#pragma once
+
+template <typename T>
+struct Vec4 {
+  T v[4];
+};
+
+template <typename T>
+__device__ inline Vec4<T> ld4(const T* __restrict__ p) {
+  Vec4<T> r;
+  if constexpr (sizeof(T) == 4) {
+    const float4 q = *reinterpret_cast<const float4*>(p);
+    r.v[0] = q.x;
+    r.v[1] = q.y;
+    r.v[2] = q.z;
+    r.v[3] = q.w;
+  } else {
+    const double2 a = *reinterpret_cast<const double2*>(p);
+    const double2 c = *reinterpret_cast<const double2*>(p + 2);
+    r.v[0] = a.x;
+    r.v[1] = a.y;
+    r.v[2] = c.x;
+    r.v[3] = c.y;
+  }
+  return r;
+}
+
+template <typename T>
+__device__ inline void st4(T* __restrict__ p, const Vec4<T>& r) {
+  if constexpr (sizeof(T) == 4) {
+    *reinterpret_cast<float4*>(p) = make_float4(r.v[0], r.v[1], r.v[2], r.v[3]);
+  } else {
+    *reinterpret_cast<double2*>(p) = make_double2(r.v[0], r.v[1]);
+    *reinterpret_cast<double2*>(p + 2) = make_double2(r.v[2], r.v[3]);
+  }
+}
+
+template <int BX, int BY>
+struct Geo {
+  static constexpr int NB = BX * BY;
+  static constexpr int TPR = BX / 4;                 // threads per block row
+  static constexpr int TPB = NB / 4;                 // threads per block
+  static constexpr int BPW = TPB >= 256 ? 1 : 256 / TPB;  // blocks per workgroup
+  static constexpr int WG = BPW * TPB;
+  static constexpr int W = BX + 2;
+  static constexpr int TILE = (BX + 2) * (BY + 2);
+  static constexpr int HALO = 2 * (BX + BY);
+  static constexpr int EXTMAX = NB;                  // host limit on external cells per block
+  static constexpr int SEG = TPB < 64 ? TPB : 64;    // lanes of one block within a wavefront
+};
+
+template <typename T, int BX, int BY>
+struct Tile4 {
+  using G = Geo<BX, BY>;
+  T v[G::TILE + G::EXTMAX];
+  T cs[G::NB];
+  T ax[G::NB];
+};
+
+// plain array fetches
+template <typename T>
+struct Arr4 {
+  const T* __restrict__ v;
+  __device__ inline Vec4<T> seg(int cell0) const { return ld4(v + cell0); }
+  __device__ inline T one(int cell) const { return v[cell]; }
+};
+
+// injected coarse values t = P_t x_c (identity block or one quadrant)
+template <typename T, int BX, int BY>
+struct Inject4 {
+  const T* __restrict__ xc;
+  const int* __restrict__ xfer;
+  __device__ inline T one(int cell) const { return xc[owner_of<BX, BY>(cell, xfer)]; }
+  __device__ inline Vec4<T> seg(int cell0) const {
+    constexpr int NB = BX * BY;
+    const int blk = cell0 / NB, loc = cell0 - blk * NB;
+    const int cb = xfer[2 * blk], mode = xfer[2 * blk + 1];
+    if (mode < 0) return ld4(xc + cb * NB + loc);
+    const int i = loc % BX, j = loc / BX;
+    const int ci = (mode & 1) * (BX / 2) + (i >> 1), cj = (mode >> 1) * (BY / 2) + (j >> 1);
+    const T a = xc[cb * NB + cj * BX + ci], c = xc[cb * NB + cj * BX + ci + 1];
+    Vec4<T> r;
+    r.v[0] = a;
+    r.v[1] = a;
+    r.v[2] = c;
+    r.v[3] = c;
+    return r;
+  }
+};
+
+struct Pre0 {
+  __device__ inline void operator()() const {}
+};
+
+// Per-thread placement in the workgroup.
+template <int BX, int BY>
+struct Place {
+  int wg, b, lt, j, q, i0, cell0;
+  bool valid;
+  __device__ inline Place(int wg_, int nblocks) : wg(wg_) {
+    using G = Geo<BX, BY>;
+    const int t = threadIdx.x;
+    const int k = t / G::TPB;
+    lt = t - k * G::TPB;
+    b = wg * G::BPW + k;
+    valid = b < nblocks;
+    j = lt / G::TPR;
+    q = lt - j * G::TPR;
+    i0 = 4 * q;
+    cell0 = b * G::NB + j * BX + i0;
+  }
+  __device__ inline int slot() const { return threadIdx.x / Geo<BX, BY>::TPB; }
+};
+
+// Evaluate (A v) and D of the thread's four cells (or, with ABS, the
+// factorized round-off bound sum_j B_cj |v_j|). `vc` returns the cells' own
+// values (absolute values with ABS). Irregular cells are marked in a bitmask
+// word per 32 cells (with the index of the word's first irregular row); a
+// segment of four cells never straddles two words.
+template <typename T, int BX, int BY, bool ABS, class F, class Pre>
+__device__ inline void tile4(Tile4<T, BX, BY>* St, const LevelView<T>& L, const Place<BX, BY>& P, F fetch, Pre pre,
+                             Vec4<T>& vc, Vec4<T>& ax, Vec4<T>& d) {
+  using G = Geo<BX, BY>;
+  constexpr int W = G::W, TILE = G::TILE, NB = G::NB, TPB = G::TPB, TPR = G::TPR;
+  constexpr int NW = (NB + 31) / 32;
+  Tile4<T, BX, BY>& S = St[P.slot()];
+  const int b = P.b, lt = P.lt, j = P.j, q = P.q, i0 = P.i0;
+  // rows of the workgroup's blocks: blk_irr is a prefix sum over blocks
+  const int wb0 = P.wg * G::BPW, wb1 = min(wb0 + G::BPW, L.nblocks);
+  const bool any = L.blk_irr[wb1] > L.blk_irr[wb0];
+  Vec4<T> own{}, cw4{}, cs4{}, cn4{}, rd{};
+  T ceb = 0;
+  int nr = 0;
+  T rv[ELL_MAX];
+  unsigned short rr[ELL_MAX];
+  int rloc = 0;
+  unsigned bits = 0;
+  const int w = L.ell_w;
+  if (P.valid) {
+    own = fetch.seg(P.cell0);
+    cw4 = ld4(L.cw + P.cell0);
+    cs4 = ld4(L.cs + P.cell0);
+    if (q == TPR - 1) ceb = L.ce[b * BY + j];
+    if (j == BY - 1) cn4 = ld4(L.cn + b * BX + i0);
+    // halo values (W, E, S, N) of same-level neighbour blocks
+    for (int h = lt; h < G::HALO; h += TPB) {
+      int side, pos, slot, src;
+      if (h < BY) {
+        side = 0; pos = h; slot = (pos + 1) * W; src = pos * BX + (BX - 1);
+      } else if (h < 2 * BY) {
+        side = 1; pos = h - BY; slot = (pos + 1) * W + BX + 1; src = pos * BX;
+      } else if (h < 2 * BY + BX) {
+        side = 2; pos = h - 2 * BY; slot = pos + 1; src = (BY - 1) * BX + pos;
+      } else {
+        side = 3; pos = h - 2 * BY - BX; slot = (BY + 1) * W + pos + 1; src = pos;
+      }
+      const int nb = L.nbr[4 * b + side];
+      const T hv = nb >= 0 ? fetch.one(nb * NB + src) : T(0);
+      S.v[slot] = ABS ? fabs(hv) : hv;
+    }
+    if (any) {
+      // external cells of the irregular rows
+      const int e0 = L.blk_ext[b], ne = L.blk_ext[b + 1] - e0;
+      for (int e = lt; e < ne; e += TPB) {
+        const T ev = fetch.one(L.ext_col[e0 + e]);
+        S.v[TILE + e] = ABS ? fabs(ev) : ev;
+      }
+      // this lane's irregular row
+      const int r0 = L.blk_irr[b];
+      nr = L.blk_irr[b + 1] - r0;
+      if (lt < nr) {
+        const int r = r0 + lt;
+        const T* vals = ABS ? L.ell_bound : L.ell_val;
+#pragma unroll
+        for (int k = 0; k < ELL_MAX; ++k)
+          if (k < w) {
+            rv[k] = vals[size_t(r) * w + k];
+            rr[k] = L.ell_ref[size_t(r) * w + k];
+          }
+        rloc = L.irr_local[r];
+      }
+      // irregular cells of this segment and their diagonals
+      const int c0 = j * BX + i0, wd = c0 >> 5, bit = c0 & 31;
+      const unsigned mword = L.irr_mask[2 * (b * NW + wd)];
+      bits = (mword >> bit) & 15u;
+      if (bits) {
+        const int first = int(L.irr_mask[2 * (b * NW + wd) + 1]) + __popc(mword & ((1u << bit) - 1u));
+        int k = 0;
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+          if ((bits >> m) & 1u) rd.v[m] = L.irr_diag[first + k++];
+      }
+    }
+    pre();
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      if (ABS) own.v[m] = fabs(own.v[m]);
+      S.v[(j + 1) * W + i0 + 1 + m] = own.v[m];
+      S.cs[j * BX + i0 + m] = cs4.v[m];
+    }
+  }
+  __syncthreads();
+  if (any) {
+    if (P.valid) {
+      if (lt < nr) {
+        T s = 0;
+#pragma unroll
+        for (int k = 0; k < ELL_MAX; ++k)
+          if (k < w) s += rv[k] * S.v[rr[k]];
+        S.ax[rloc] = s;
+      }
+      // blocks with more irregular rows than lanes
+      for (int x = lt + TPB; x < nr; x += TPB) {
+        const int r = L.blk_irr[b] + x;
+        const T* vals = ABS ? L.ell_bound : L.ell_val;
+        T s = 0;
+        for (int k = 0; k < w; ++k) s += vals[size_t(r) * w + k] * S.v[L.ell_ref[size_t(r) * w + k]];
+        S.ax[L.irr_local[r]] = s;
+      }
+    }
+    __syncthreads();
+  }
+  const T cw_next = __shfl_down(cw4.v[0], 1, G::SEG);
+  vc = own;
+  if (!P.valid) return;
+  Vec4<T> axi{};
+  if (bits) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+      if ((bits >> m) & 1u) axi.v[m] = S.ax[j * BX + i0 + m];
+  }
+  // stencil of every cell, then a select for the irregular ones (keeps the
+  // results in registers)
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int i = i0 + m;
+    const T vw = m > 0 ? own.v[m - 1] : S.v[(j + 1) * W + i];
+    const T ve = m < 3 ? own.v[m + 1] : S.v[(j + 1) * W + i + 2];
+    const T vs = S.v[j * W + i + 1], vn = S.v[(j + 2) * W + i + 1];
+    const T cw = cw4.v[m];
+    const T ce = m < 3 ? cw4.v[m + 1] : (q < TPR - 1 ? cw_next : ceb);
+    const T cs = cs4.v[m];
+    const T cn = j < BY - 1 ? S.cs[(j + 1) * BX + i] : cn4.v[m];
+    const T dd = (cw + ce) + (cs + cn);
+    const T nbs = (cw * vw + ce * ve) + (cs * vs + cn * vn);
+    const T st = ABS ? dd * own.v[m] + nbs : dd * own.v[m] - nbs;
+    const bool irr = (bits >> m) & 1u;
+    ax.v[m] = irr ? axi.v[m] : st;
+    d.v[m] = irr ? rd.v[m] : dd;
+  }
+}
+
+// Sum of `v` over the lanes of one block; valid in lane lt == 0.
+template <typename T, int BX, int BY>
+__device__ inline T block4_sum(T v, T* scratch) {
+  using G = Geo<BX, BY>;
+  for (int o = G::SEG / 2; o > 0; o >>= 1) v += __shfl_down(v, o, G::SEG);
+  if constexpr (G::TPB <= 64) {
+    return v;
+  } else {
+    constexpr int NW = G::TPB / 64;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) scratch[wv] = v;
+    __syncthreads();
+    T s = 0;
+    if (threadIdx.x == 0)
+      for (int k = 0; k < NW; ++k) s += scratch[k];
+    __syncthreads();
+    return s;
+  }
+}
+
+template <typename T, int BX, int BY>
+__device__ inline T block4_max(T v, T* scratch) {
+  using G = Geo<BX, BY>;
+  for (int o = G::SEG / 2; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, G::SEG));
+  if constexpr (G::TPB <= 64) {
+    return v;
+  } else {
+    constexpr int NW = G::TPB / 64;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) scratch[wv] = v;
+    __syncthreads();
+    T s = 0;
+    if (threadIdx.x == 0)
+      for (int k = 0; k < NW; ++k) s = fmax(s, scratch[k]);
+    __syncthreads();
+    return s;
+  }
+}
+
+#define TILE4_SHARED(T, BX, BY) __shared__ Tile4<T, BX, BY> S_[Geo<BX, BY>::BPW]
+
+// Damped Jacobi sweep x_out = x_in + w D^-1 (b - A x_in); STATS: per-block
+// partial sums of V z, r z and r for the Krylov update.
+template <typename T, int BX, int BY, bool FINE, bool STATS>
+__global__ void __launch_bounds__(256) k_sweep(LevelView<T> L, const T* __restrict__ x_in, T* __restrict__ x_out,
+                                              const T* __restrict__ rhs, double omega, const int* __restrict__ done,
+                                              const double* __restrict__ r64, double* __restrict__ part) {
+  if (done && *done) return;
+  TILE4_SHARED(T, BX, BY);
+  const Place<BX, BY> P(xcd_block(blockIdx.x, gridDim.x), L.nblocks);
+  Vec4<T> bv{}, vc, ax, d;
+  Vec4<double> rd{};
+  tile4<T, BX, BY, false>(S_, L, P, Arr4<T>{x_in}, [&]() {
+    bv = ld4(rhs + P.cell0);
+    if constexpr (STATS) rd = ld4(r64 + P.cell0);
+  }, vc, ax, d);
+  Vec4<T> z;
+  double sz = 0, srz = 0, sr = 0;
+  if (P.valid) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      z.v[m] = vc.v[m] + T(omega) * (bv.v[m] - ax.v[m]) / d.v[m];
+      if constexpr (STATS) {
+        sz += double(z.v[m]);
+        srz += rd.v[m] * double(z.v[m]);
+        sr += rd.v[m];
+      }
+    }
+    st4(x_out + P.cell0, z);
+  }
+  if constexpr (STATS) {
+    __shared__ double red[4];
+    sz = block4_sum<double, BX, BY>(sz, red);
+    srz = block4_sum<double, BX, BY>(srz, red);
+    sr = block4_sum<double, BX, BY>(sr, red);
+    if (P.valid && P.lt == 0) {
+      const int nb = L.nblocks;
+      part[P.b] = sz * L.volume[P.b];
+      part[nb + P.b] = srz;
+      part[2 * nb + P.b] = sr;
+    }
+  }
+}
+
+// u = w_P D^-1 (b - A x)
+template <typename T, int BX, int BY, bool FINE>
+__global__ void __launch_bounds__(256) k_residual(LevelView<T> L, const T* __restrict__ x, const T* __restrict__ rhs,
+                                                 T* __restrict__ u, double omega_p, const int* __restrict__ done) {
+  if (done && *done) return;
+  TILE4_SHARED(T, BX, BY);
+  const Place<BX, BY> P(xcd_block(blockIdx.x, gridDim.x), L.nblocks);
+  Vec4<T> bv{}, vc, ax, d;
+  tile4<T, BX, BY, false>(S_, L, P, Arr4<T>{x}, [&]() { bv = ld4(rhs + P.cell0); }, vc, ax, d);
+  if (!P.valid) return;
+  Vec4<T> o;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) o.v[m] = T(omega_p) * (bv.v[m] - ax.v[m]) / d.v[m];
+  st4(u + P.cell0, o);
+}
+
+// Smoothed restriction b_c = P_t' (D u / w_P - S u), and the zero-start
+// coarse sweep x_c = w_c D_c^-1 b_c.
+template <typename T, int BX, int BY, bool FINE>
+__global__ void __launch_bounds__(256) k_restrict(LevelView<T> L, const T* __restrict__ u, double omega_p,
+                                                 T* __restrict__ bc, T* __restrict__ xc, const T* __restrict__ dinv_c,
+                                                 double omega_c, int zero_start, const int* __restrict__ done) {
+  if (done && *done) return;
+  using G = Geo<BX, BY>;
+  TILE4_SHARED(T, BX, BY);
+  const Place<BX, BY> P(xcd_block(blockIdx.x, gridDim.x), L.nblocks);
+  Vec4<T> vc, ax, d;
+  tile4<T, BX, BY, false>(S_, L, P, Arr4<T>{u}, Pre0{}, vc, ax, d);
+  Vec4<T> w;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) w.v[m] = P.valid ? d.v[m] * vc.v[m] / T(omega_p) - ax.v[m] : T(0);
+  const int cb = P.valid ? L.xfer[2 * P.b] : 0, mode = P.valid ? L.xfer[2 * P.b + 1] : -1;
+  // pair sums across the row segment, then across row pairs (same wavefront)
+  const T h0 = w.v[0] + w.v[1], h1 = w.v[2] + w.v[3];
+  const T n0 = __shfl_down(h0, G::TPR, G::SEG), n1 = __shfl_down(h1, G::TPR, G::SEG);
+  if (!P.valid) return;
+  if (mode < 0) {
+    const int cc = cb * G::NB + P.j * BX + P.i0;
+    st4(bc + cc, w);
+    Vec4<T> x0;
+    if (zero_start) {
+      const Vec4<T> di = ld4(dinv_c + cc);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) x0.v[m] = T(omega_c) * di.v[m] * w.v[m];
+    } else {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) x0.v[m] = T(0);
+    }
+    st4(xc + cc, x0);
+    return;
+  }
+  if (P.j & 1) return;
+  const T s0 = h0 + n0, s1 = h1 + n1;
+  const int ci = (mode & 1) * (BX / 2) + (P.i0 >> 1), cj = (mode >> 1) * (BY / 2) + (P.j >> 1);
+  const int cc = cb * G::NB + cj * BX + ci;
+  bc[cc] = s0;
+  bc[cc + 1] = s1;
+  if (zero_start) {
+    xc[cc] = T(omega_c) * dinv_c[cc] * s0;
+    xc[cc + 1] = T(omega_c) * dinv_c[cc + 1] * s1;
+  } else {
+    xc[cc] = T(0);
+    xc[cc + 1] = T(0);
+  }
+}
+
+// x += t - w_P D^-1 S t with t = P_t x_c
+template <typename T, int BX, int BY, bool FINE>
+__global__ void __launch_bounds__(256) k_prolong(LevelView<T> L, T* __restrict__ x, const T* __restrict__ xc,
+                                                double omega_p, const int* __restrict__ done) {
+  if (done && *done) return;
+  TILE4_SHARED(T, BX, BY);
+  const Place<BX, BY> P(xcd_block(blockIdx.x, gridDim.x), L.nblocks);
+  Vec4<T> xv{}, vc, ax, d;
+  tile4<T, BX, BY, false>(S_, L, P, Inject4<T, BX, BY>{xc, L.xfer}, [&]() { xv = ld4(x + P.cell0); }, vc, ax, d);
+  if (!P.valid) return;
+  Vec4<T> o;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) o.v[m] = xv.v[m] + (vc.v[m] - T(omega_p) * ax.v[m] / d.v[m]);
+  st4(x + P.cell0, o);
+}
+
+// y = A p (FP64) with per-block partial p.y
+template <int BX, int BY>
+__global__ void __launch_bounds__(256) k_matvec_dot(LevelView<double> L, const double* __restrict__ p,
+                                                   double* __restrict__ y, double* __restrict__ part,
+                                                   const int* __restrict__ done) {
+  if (done && *done) return;
+  TILE4_SHARED(double, BX, BY);
+  const Place<BX, BY> P(xcd_block(blockIdx.x, gridDim.x), L.nblocks);
+  Vec4<double> vc, ax, d;
+  tile4<double, BX, BY, false>(S_, L, P, Arr4<double>{p}, Pre0{}, vc, ax, d);
+  double s = 0;
+  if (P.valid) {
+    st4(y + P.cell0, ax);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) s += vc.v[m] * ax.v[m];
+  }
+  if (part) {
+    __shared__ double red[4];
+    s = block4_sum<double, BX, BY>(s, red);
+    if (P.valid && P.lt == 0) part[P.b] = s;
+  }
+}
+
+// Factorized round-off bound per cell and the per-block maximum of
+// max(bound / V, |b| / V).
+template <int BX, int BY>
+__global__ void __launch_bounds__(256) k_floor_bound(LevelView<double> L, const double* __restrict__ x,
+                                                    const double* __restrict__ rhs, double* __restrict__ part) {
+  TILE4_SHARED(double, BX, BY);
+  const Place<BX, BY> P(xcd_block(blockIdx.x, gridDim.x), L.nblocks);
+  Vec4<double> bv{}, vc, ax, d;
+  tile4<double, BX, BY, true>(S_, L, P, Arr4<double>{x}, [&]() { bv = ld4(rhs + P.cell0); }, vc, ax, d);
+  double qm = 0;
+  if (P.valid) {
+    const double V = L.volume[P.b];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      double q = fmax(ax.v[m] / V, fabs(bv.v[m]) / V);
+      if (q != q) q = __longlong_as_double(0x7ff8000000000000LL);
+      qm = (q != q || qm != qm) ? q : fmax(qm, q);
+    }
+  }
+  __shared__ double red[4];
+  qm = block4_max<double, BX, BY>(qm, red);
+  if (P.valid && P.lt == 0) part[P.b] = qm;
+}
