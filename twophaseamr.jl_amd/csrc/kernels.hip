@@ -160,13 +160,63 @@ __global__ void __launch_bounds__(1024) k_reduce(int n, int nsets, const double*
   }
 }
 
-// Krylov scalar step 1: pAp and the step length (Krylov.jl cg semantics).
-__global__ void __launch_bounds__(1024) k_cg_alpha(int nb, const double* __restrict__ part, CgState* st) {
-  __shared__ double red[16];
-  if (st->done) return;
-  const double v = wg_reduce<false>(nb, part, red);
+// Two-stage deterministic reductions for the Krylov scalar steps: each of
+// RED_WG workgroups reduces a fixed contiguous chunk of the per-block
+// partials, publishes its value, and takes a ticket; the workgroup holding
+// the last ticket reduces the RED_WG values in index order and finishes the
+// scalar step. Nothing waits on another workgroup.
+__device__ inline double chunk_sum(const double* __restrict__ part, int n, double* red) {
+  const int ch = (n + RED_WG - 1) / RED_WG, lo = blockIdx.x * ch, hi = min(n, lo + ch);
+  double a0 = 0, a1 = 0;
+  int q = lo + threadIdx.x;
+  for (; q + 256 < hi; q += 512) {
+    a0 += part[q];
+    a1 += part[q + 256];
+  }
+  if (q < hi) a0 += part[q];
+  return block_sum<double, 256>(a0 + a1, red);
+}
+
+__device__ inline double chunk_max(const double* __restrict__ part, int n, double* red) {
+  const int ch = (n + RED_WG - 1) / RED_WG, lo = blockIdx.x * ch, hi = min(n, lo + ch);
+  double a = 0;
+  for (int q = lo + threadIdx.x; q < hi; q += 256) a = fmax(a, part[q]);
+  return block_max<double, 256>(a, red);
+}
+
+// Publish this workgroup's values (thread 0) and report whether it holds
+// the last ticket; the ticket counter is left at zero for the next use.
+template <int NS>
+__device__ inline bool publish_last(const double* v, double* __restrict__ stage, int* ticket) {
+  __shared__ int last;
   if (threadIdx.x == 0) {
-    const double pAp = v;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) __hip_atomic_store(stage + k * RED_WG + blockIdx.x, v[k], __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+    const int t = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = t == int(gridDim.x) - 1;
+    if (last) __hip_atomic_store(ticket, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  return last;
+}
+
+template <bool MAX>
+__device__ inline double stage_reduce(const double* __restrict__ stage, double* red) {
+  double v = 0;
+  if (threadIdx.x < RED_WG) v = __hip_atomic_load(stage + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return MAX ? block_max<double, 256>(v, red) : block_sum<double, 256>(v, red);
+}
+
+// Krylov scalar step 1: pAp and the step length (Krylov.jl cg semantics).
+__global__ void __launch_bounds__(256) k_cg_alpha(int nb, const double* __restrict__ part, double* __restrict__ stage,
+                                                  CgState* st) {
+  __shared__ double red[4];
+  if (st->done) return;
+  const double v = chunk_sum(part, nb, red);
+  if (!publish_last<1>(&v, stage, &st->ticket)) return;
+  const double pAp = stage_reduce<false>(stage, red);
+  if (threadIdx.x == 0) {
     st->pAp = pAp;
     const double eps = 2.220446049250313e-16;
     if (pAp <= eps * st->pnorm2 && fabs(pAp) <= eps * st->pnorm2) {
@@ -214,14 +264,21 @@ __global__ void k_cg_update_xr(int n, int nb_cells, const double* __restrict__ p
 }
 
 // Krylov scalar step 2: z mean removal, gamma, beta and the stopping tests.
-__global__ void __launch_bounds__(1024) k_cg_beta(int nb, int nmax, const double* __restrict__ part, const double* __restrict__ maxpart,
-                          double total_volume, CgState* st, int init) {
-  __shared__ double red[16];
+__global__ void __launch_bounds__(256) k_cg_beta(int nb, int nmax, const double* __restrict__ part,
+                                                 const double* __restrict__ maxpart, double* __restrict__ stage,
+                                                 double total_volume, CgState* st, int init) {
+  __shared__ double red[4];
   if (st->done) return;
-  const double sz = wg_reduce<false>(nb, part, red);
-  const double srz = wg_reduce<false>(nb, part + nb, red);
-  const double sr = wg_reduce<false>(nb, part + 2 * nb, red);
-  const double mx = wg_reduce<true>(nmax, maxpart, red);
+  double v[4];
+  v[0] = chunk_sum(part, nb, red);
+  v[1] = chunk_sum(part + nb, nb, red);
+  v[2] = chunk_sum(part + 2 * nb, nb, red);
+  v[3] = chunk_max(maxpart, nmax, red);
+  if (!publish_last<4>(v, stage, &st->ticket)) return;
+  const double sz = stage_reduce<false>(stage, red);
+  const double srz = stage_reduce<false>(stage + RED_WG, red);
+  const double sr = stage_reduce<false>(stage + 2 * RED_WG, red);
+  const double mx = stage_reduce<true>(stage + 3 * RED_WG, red);
   if (threadIdx.x != 0) return;
   const double m = sz / total_volume;
   st->rmean = sr / total_volume;
@@ -577,8 +634,8 @@ void launch_reduce(hipStream_t s, int n, int nsets, const double* part, double* 
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, n, nsets, part, out, int(is_max));
 }
 
-void launch_cg_alpha(hipStream_t s, int nb, const double* part, CgState* st) {
-  hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(1024), 0, s, nb, part, st);
+void launch_cg_alpha(hipStream_t s, int nb, const double* part, double* stage, CgState* st) {
+  hipLaunchKernelGGL(k_cg_alpha, dim3(RED_WG), dim3(256), 0, s, nb, part, stage, st);
 }
 
 template <typename TV>
@@ -589,9 +646,10 @@ void launch_cg_update_xr(hipStream_t s, int n, int nbc, const double* p, const d
                      dinv, omega, part, st, init);
 }
 
-void launch_cg_beta(hipStream_t s, int nb, int nmax, const double* part, const double* maxpart, double total_volume,
-                    CgState* st, int init) {
-  hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(1024), 0, s, nb, nmax, part, maxpart, total_volume, st, init);
+void launch_cg_beta(hipStream_t s, int nb, int nmax, const double* part, const double* maxpart, double* stage,
+                    double total_volume, CgState* st, int init) {
+  hipLaunchKernelGGL(k_cg_beta, dim3(RED_WG), dim3(256), 0, s, nb, nmax, part, maxpart, stage, total_volume, st,
+                     init);
 }
 
 template <typename TV>

@@ -31,11 +31,15 @@ struct LevelView {
   const double* volume;  // per block
 };
 
-// Device-resident Krylov state (Krylov.jl cg semantics).
+// Device-resident Krylov state (Krylov.jl cg semantics). `ticket` counts
+// finished workgroups of a two-stage reduction and is zero between kernels.
 struct CgState {
   double gamma, pnorm2, alpha, beta, pAp, zmean, rnorm, strong, target, rmean;
-  int iter, itmax, done, reason;
+  int iter, itmax, done, reason, ticket;
 };
+
+// Workgroups of the Krylov scalar-step reductions (stage buffer: 4 * RED_WG).
+constexpr int RED_WG = 128;
 
 bool supported_block(int bx, int by);
 
@@ -57,13 +61,13 @@ void launch_matvec_dot(hipStream_t s, const LevelView<double>& L, const double* 
                        const int* done);
 void launch_floor_bound(hipStream_t s, const LevelView<double>& L, const double* x, const double* b, double* part);
 void launch_reduce(hipStream_t s, int n, int nsets, const double* part, double* out, bool is_max);
-void launch_cg_alpha(hipStream_t s, int nb, const double* part, CgState* st);
+void launch_cg_alpha(hipStream_t s, int nb, const double* part, double* stage, CgState* st);
 template <typename TV>
 void launch_cg_update_xr(hipStream_t s, int n, int nbc, const double* p, const double* Ap, double* x, double* r,
                          const double* vol, TV* vb, TV* vx, const TV* dinv, double omega, double* part, CgState* st,
                          int init);
-void launch_cg_beta(hipStream_t s, int nb, int nmax, const double* part, const double* maxpart, double total_volume,
-                    CgState* st, int init);
+void launch_cg_beta(hipStream_t s, int nb, int nmax, const double* part, const double* maxpart, double* stage,
+                    double total_volume, CgState* st, int init);
 template <typename TV>
 void launch_cg_update_p(hipStream_t s, int n, const TV* z, double* p, const CgState* st, const int* done);
 

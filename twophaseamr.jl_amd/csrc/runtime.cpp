@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -114,7 +115,7 @@ struct tpamr_ctx {
   std::vector<DevLevel<float>> lf;   // FP32 levels (FP32 mode)
   DevArray<double> coarse_inv;
   DevArray<double> x, r, p, Ap, b, tmp, tmp2;
-  DevArray<double> part, maxpart, scal;
+  DevArray<double> part, maxpart, scal, stage;
   CgState* st = nullptr;      // device
   CgState* st_host = nullptr; // pinned
   CgState* st_host2 = nullptr;
@@ -123,7 +124,9 @@ struct tpamr_ctx {
   DevArray<signed char> faxis, csign, bside;
   DevArray<double> farea, kdiag, kval, kappa, sqk, spacing, barea;
   DevArray<double> f1, f2, f3, f4;
-  // graph of one Krylov iteration
+  // graph of one Krylov iteration (TPAMR_GRAPH=0 in the environment launches
+  // the iteration's kernels one by one instead)
+  bool use_graph = true;
   hipGraphExec_t graph = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evs[2] = {nullptr, nullptr};
   const TreeMesh& mesh() const {
@@ -332,6 +335,7 @@ void upload_hierarchy(tpamr_ctx* c, bool topology) {
     c->part.alloc(std::max(3 * nblk, 3 * ncb) + 16);
     c->maxpart.alloc(std::max(nblk, ncb) + 16);
     c->scal.alloc(16);
+    c->stage.alloc(4 * RED_WG);
     destroy_graph(c);
   }
   // kernel arguments of the captured iteration (weights, pointers) change
@@ -468,12 +472,12 @@ void enqueue_iteration(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, bool init) {
                            L0.dinv.p, w0, c->maxpart.p, c->st, 1);
   } else {
     launch_matvec_dot(s, fine64(c), c->p.p, c->Ap.p, c->part.p, done);
-    launch_cg_alpha(s, nblk, c->part.p, c->st);
+    launch_cg_alpha(s, nblk, c->part.p, c->stage.p, c->st);
     launch_cg_update_xr<T>(s, N, nb, c->p.p, c->Ap.p, c->x.p, c->r.p, c->topo[0].volume.p, L0.b.p, L0.x.p,
                            L0.dinv.p, w0, c->maxpart.p, c->st, 0);
   }
   T* z = enqueue_precondition<T>(c, lv, done);
-  launch_cg_beta(s, nblk, ncb(N), c->part.p, c->maxpart.p, c->H->total_volume, c->st, init ? 1 : 0);
+  launch_cg_beta(s, nblk, ncb(N), c->part.p, c->maxpart.p, c->stage.p, c->H->total_volume, c->st, init ? 1 : 0);
   launch_cg_update_p<T>(s, N, z, c->p.p, c->st, done);
 }
 
@@ -502,11 +506,14 @@ int run_cg(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double target, int itmax)
   HIPCHECK(hipMemcpyAsync(c->st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, s));
   launch_fill(s, N, 0.0, c->p.p);
   enqueue_iteration<T>(c, lv, true);
-  if (!c->graph) build_graph<T>(c, lv);
+  if (c->use_graph && !c->graph) build_graph<T>(c, lv);
   CgState* slot[2] = {c->st_host, c->st_host2};
   int launched = 0;
   for (;;) {
-    HIPCHECK(hipGraphLaunch(c->graph, s));
+    if (c->use_graph)
+      HIPCHECK(hipGraphLaunch(c->graph, s));
+    else
+      enqueue_iteration<T>(c, lv, false);
     HIPCHECK(hipMemcpyAsync(slot[launched & 1], c->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipEventRecord(c->evs[launched & 1], s));
     ++launched;
@@ -665,6 +672,7 @@ int tpamr_create(tpamr_ctx** out, int device, int precision) {
   if (precision != TPAMR_FP64 && precision != TPAMR_FP32_VCYCLE) return TPAMR_EINVAL;
   c->precision = precision;
   c->device = device;
+  if (const char* g = std::getenv("TPAMR_GRAPH")) c->use_graph = std::strcmp(g, "0") != 0;
   if (device >= 0) {
     int rc = guard(c.get(), [&] {
       int n = 0;

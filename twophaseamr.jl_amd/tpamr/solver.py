@@ -118,6 +118,7 @@ class PoissonOperator:
 
     def __init__(self, mesh: BlockTreeMesh, face_coefficient=1.0):
         self.mesh = mesh
+        self.version = 0
         self.face_coefficient = self._coefficients(face_coefficient)
 
     def _coefficients(self, coefficient):
@@ -137,8 +138,11 @@ class PoissonOperator:
         return np.ascontiguousarray(values)
 
     def replace_face_coefficient(self, face_coefficient):
-        """`_replace_face_coefficient` (elliptic.jl:809-820)."""
+        """`_replace_face_coefficient` (elliptic.jl:809-820). Solvers built on
+        this operator pick the new coefficients up at their next call; after
+        changing `face_coefficient` in place, call the solver's refresh()."""
         self.face_coefficient = self._coefficients(face_coefficient)
+        self.version += 1
         return self
 
 
@@ -185,10 +189,11 @@ class EllipticSolver:
         """Upload the operator's current face coefficients (refreshes every level)."""
         k = self.operator.face_coefficient
         L.check(L.lib().tpamr_set_face_coefficients(self.ctx.h, L.ptr(k)), self.ctx.h)
-        self._coeff = k.copy()
+        self._coeff_version = self.operator.version
 
     def _sync(self):
-        if not np.array_equal(self._coeff, self.operator.face_coefficient):
+        # O(1) per call: coefficient replacements bump the operator's version
+        if self._coeff_version != self.operator.version:
             self.refresh()
 
     @property
