@@ -9,6 +9,22 @@ L-infinity stopping rule). The headline value uses the FP32 V-cycle; the
 all-FP64 GPU solver runs on the same system for comparison.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--levels 5 10]
+
+Roofline of the dominant kernel (the finest-level FP32 Jacobi sweep): its
+algorithmic bytes per launch are 24 B per cell (x, b, D^-1, two face
+coefficients read, x' written, 4 B each) plus 12 B per stored entry of the
+rows next to coarse-fine interfaces; the average launch time comes from HIP
+events around launches of that kernel on the solver's stream.
+
+Hardware-counter traffic (`roofline.traffic`) comes from two rocprofv3 PMC
+passes over `--probe` (FETCH_SIZE, then WRITE_SIZE), calibrated against a
+plain device copy of known size in the same run:
+
+  rocprofv3 --pmc FETCH_SIZE -d <dir_f> -o pmc -- python3 bench.py --probe
+  rocprofv3 --pmc WRITE_SIZE -d <dir_w> -o pmc -- python3 bench.py --probe
+  python3 bench.py --pmc-summary <dir_f>/pmc_results.db <dir_w>/pmc_results.db
+
+The summary is written to profiles/ and read back by later bench runs.
 """
 from __future__ import annotations
 
@@ -25,6 +41,15 @@ sys.path.insert(0, os.path.join(ROOT, "twophaseamr.jl_amd"))
 
 METRIC = "Mcells/s to rel. residual 1e-10, FP32-MG vs FP64 GPU solver; smoother HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+SWEEP_KERNEL = "k_sweep<float, 16, 16, true, false>"
+COPY_BYTES = 1 << 28  # per direction, calibration copy of the PMC probe
+TRAFFIC_MD = os.path.join(ROOT, "profiles", "k_sweep_fp32_circles64_levels5-10_traffic.md")
+
+
+def sweep_algorithmic_bytes(ncells, irregular_nnz, elem=4):
+    """Algorithmic bytes of one finest-level Jacobi sweep: 6 words per cell
+    (x, b, D^-1, two face coefficients, x') plus 12 B per irregular entry."""
+    return 6 * elem * ncells + 12 * irregular_nnz
 
 
 def log(*a):
@@ -51,9 +76,12 @@ def timed_solves(solver, d_rhs, d_x, steps, warmup, rtol):
     return float(np.mean(walls)), float(np.mean(dev)), st
 
 
-def cpu_baseline(levels, rtol):
+def cpu_baseline(levels, rtol, steps=3):
     """The oracle (CPU FP64 restatement, one thread) on a bounded sample of the
-    same workload family: the circles mesh at reduced maximum level."""
+    same workload family: the circles mesh at a reduced maximum level. The
+    same system is solved on the GPU in both precision modes and the
+    pressures are compared with the CPU FP64 solution."""
+    import torch
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import OMesh, OSolver
     import tpamr
@@ -63,17 +91,153 @@ def cpu_baseline(levels, rtol):
                max_level=mesh.max_level, refined_keys=mesh.refined_keys)
     c = W.liquid_fraction(mesh, circles)
     kappa = om.mixture_coefficients(c, 1000.0, 1.0)
-    u = W.smooth_velocity(mesh)
+    u = W.smooth_velocity(mesh, device="cuda:0")
     rhs = -om.divergence(u)
     t0 = time.perf_counter()
     s = OSolver(om, kappa)
     t1 = time.perf_counter()
-    x, st = s.solve(rhs, rtol=rtol, max_cycles=200)
+    x_cpu, st = s.solve(rhs, rtol=rtol, max_cycles=200)
     t2 = time.perf_counter()
-    return dict(value=mesh.ncells / (t2 - t1) / 1e6, unit="Mcells/s", cores=1, kind="port",
-                sample=f"oracle FP64 solve (CPU restatement, 1 thread) on the circles workload at levels "
-                       f"{levels[0]}-{levels[1]} ({mesh.ncells} cells, {st['cycles']} CG iterations, "
-                       f"solve {t2 - t1:.2f} s, hierarchy setup {t1 - t0:.2f} s excluded)")
+    del s
+    log(f"cpu baseline: {mesh.ncells} cells, setup {t1 - t0:.1f} s, solve {t2 - t1:.1f} s")
+    gpu = {"cells": mesh.ncells}
+    d_rhs = torch.as_tensor(rhs, device="cuda:0")
+    for prec in ("fp32_vcycle", "fp64"):
+        solver = tpamr.EllipticSolver(tpamr.PoissonOperator(mesh, kappa), precision=prec)
+        d_x = torch.zeros(mesh.ncells, dtype=torch.float64, device="cuda:0")
+        wall, _, gst = timed_solves(solver, d_rhs, d_x, steps, 1, rtol)
+        x = d_x.cpu().numpy()
+        gpu[prec] = {"mcells": mesh.ncells / wall / 1e6, "cycles": gst.cycles, "converged": bool(gst.converged),
+                     "pressure_error_vs_cpu_fp64": float(np.max(np.abs(x - x_cpu)) / np.max(np.abs(x_cpu)))}
+        solver.close()
+    gpu["speedup_fp32_vs_fp64"] = gpu["fp32_vcycle"]["mcells"] / gpu["fp64"]["mcells"]
+    cpu = dict(value=mesh.ncells / (t2 - t1) / 1e6, unit="Mcells/s", cores=1, kind="port",
+               sample=f"oracle FP64 solve (CPU restatement, 1 thread) on the circles workload at levels "
+                      f"{levels[0]}-{levels[1]} ({mesh.ncells} cells, {st['cycles']} CG iterations, "
+                      f"solve {t2 - t1:.2f} s, hierarchy setup {t1 - t0:.2f} s excluded)",
+               cpu_cycles=st["cycles"])
+    return cpu, gpu
+
+
+def build_case(levels, precision, device="cuda:0"):
+    """Mesh, two-phase solver and device rhs of the circles workload."""
+    import torch
+    import tpamr
+    from tpamr import workloads as W
+    mesh, circles = W.circles_mesh(tuple(levels))
+    c = W.liquid_fraction(mesh, circles)
+    u = W.smooth_velocity(mesh, device=device)
+    solver = tpamr.EllipticSolver(tpamr.PoissonOperator(mesh, 1.0), precision=precision)
+    tpamr.refresh_two_phase(solver, c, (1000.0, 1.0))
+    rhs = -tpamr.divergence(solver, u)
+    return mesh, solver, torch.as_tensor(rhs, device=device)
+
+
+def probe(levels):
+    """Short run for hardware-counter passes: one FP32-V-cycle solve, 10
+    finest-level sweeps, then 5 device copies of COPY_BYTES (calibration; the
+    last 5 dispatches of the process)."""
+    import torch
+    mesh, solver, d_rhs = build_case(levels, "fp32_vcycle")
+    d_x = torch.zeros(mesh.ncells, dtype=torch.float64, device="cuda:0")
+    st = solver.solve_device(d_rhs.data_ptr(), d_x.data_ptr(), rtol=1e-10, max_cycles=200)
+    log(f"probe: {mesh.ncells} cells, solve {st.cycles} iterations")
+    solver.time_smoother(reps=10)
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    info = solver.level_info(0)
+    print(json.dumps({"probe": True, "cells": mesh.ncells, "irregular_nnz": info["irregular_nnz"]}), flush=True)
+
+
+def pmc_summary(db_fetch, db_write, out_path=TRAFFIC_MD):
+    """Per-launch HBM traffic of the finest-level FP32 sweep from FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch), each scaled by what the counter reads for
+    the calibration copy (COPY_BYTES read and written)."""
+    import sqlite3
+
+    def rows(db, counter):
+        con = sqlite3.connect(db)
+        r = con.execute("select dispatch_id, kernel_name, grid_size, value from counters_collection "
+                        "where counter_name = ? order by dispatch_id", (counter,)).fetchall()
+        con.close()
+        return r
+
+    out = {}
+    for counter, db in (("FETCH_SIZE", db_fetch), ("WRITE_SIZE", db_write)):
+        r = rows(db, counter)
+        copy = r[-5:]
+        sweeps = [x for x in r if SWEEP_KERNEL in x[1]]
+        gmax = max(x[2] for x in sweeps)
+        sweeps = [x[3] for x in sweeps if x[2] == gmax]
+        copy_kib = float(np.mean([x[3] for x in copy]))
+        factor = (COPY_BYTES / 1024.0) / copy_kib
+        out[counter] = dict(sweep_kib=float(np.mean(sweeps)), n=len(sweeps), copy_kib=copy_kib, factor=factor,
+                            bytes=float(np.mean(sweeps)) * 1024.0 * factor)
+    f, w = out["FETCH_SIZE"], out["WRITE_SIZE"]
+    total = f["bytes"] + w["bytes"]
+    lines = [
+        "# HBM traffic of the finest-level FP32 Jacobi sweep, circles64 levels 5-10",
+        "",
+        "Commands (repository root, one MI355X; `cd /tmp && export TMPDIR=/tmp` first, paths absolute):",
+        "",
+        "```",
+        "rocprofv3 --pmc FETCH_SIZE -d <repo>/gpurun_out/pmc_f -o pmc -- python3 <repo>/bench.py --probe",
+        "rocprofv3 --pmc WRITE_SIZE -d <repo>/gpurun_out/pmc_w -o pmc -- python3 <repo>/bench.py --probe",
+        "python3 bench.py --pmc-summary gpurun_out/pmc_f/pmc_results.db gpurun_out/pmc_w/pmc_results.db",
+        "```",
+        "",
+        f"Kernel `{SWEEP_KERNEL}` at the finest level ({f['n']} dispatches per pass). Calibration: the last 5",
+        f"dispatches of the probe are a device copy reading and writing {COPY_BYTES} B each.",
+        "",
+        "| counter | sweep KiB/dispatch | copy KiB/dispatch | scale (known/counted) | sweep bytes/launch |",
+        "|---|---:|---:|---:|---:|",
+        f"| FETCH_SIZE | {f['sweep_kib']:.1f} | {f['copy_kib']:.1f} | {f['factor']:.4f} | {f['bytes']:.0f} |",
+        f"| WRITE_SIZE | {w['sweep_kib']:.1f} | {w['copy_kib']:.1f} | {w['factor']:.4f} | {w['bytes']:.0f} |",
+        "",
+        f"traffic_bytes_per_launch: {total:.0f}",
+        "",
+    ]
+    os.makedirs(os.path.dirname(out_path), exist_ok=True)
+    with open(out_path, "w") as fh:
+        fh.write("\n".join(lines))
+    print("\n".join(lines))
+
+
+def trace_summary(db, title, command, top=30):
+    """Markdown table of per-kernel totals from a rocprofv3 --kernel-trace
+    database (rocpd SQLite)."""
+    import sqlite3
+    con = sqlite3.connect(db)
+    tot = con.execute("select sum(end - start) / 1e6 from kernels").fetchone()[0]
+    rows = con.execute("select name, count(*), sum(end - start) / 1e6, avg(end - start) / 1e3, "
+                       "min(end - start) / 1e3, max(end - start) / 1e3, max(grid_x) from kernels "
+                       "group by name order by sum(end - start) desc limit ?", (top,)).fetchall()
+    con.close()
+    lines = [f"# {title}", "", "Command (repository root, one MI355X; `cd /tmp && export TMPDIR=/tmp` first):", "",
+             "```", command, "```", "", f"Total kernel time: {tot:.2f} ms", "",
+             "| kernel | calls | total ms | avg us | min us | max us | max grid |", "|---|---:|---:|---:|---:|---:|---:|"]
+    for r in rows:
+        name = r[0].replace("tpamr::", "").split("(")[0]
+        if len(name) > 90:
+            name = name[:87] + "..."
+        lines.append(f"| `{name}` | {r[1]} | {r[2]:.2f} | {r[3]:.2f} | {r[4]:.2f} | {r[5]:.2f} | {r[6]} |")
+    print("\n".join(lines))
+
+
+def committed_traffic():
+    """Bytes per launch recorded by the last --pmc-summary, or None."""
+    try:
+        with open(TRAFFIC_MD) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        return None
+    return None
 
 
 def main():
@@ -84,32 +248,35 @@ def main():
     ap.add_argument("--levels", type=int, nargs=2, default=(5, 10))
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-levels", type=int, nargs=2, default=(5, 7))
+    ap.add_argument("--cpu-levels", type=int, nargs=2, default=(5, 8))
+    ap.add_argument("--probe", action="store_true", help="short run for rocprofv3 --pmc passes")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DB", "WRITE_DB"))
+    ap.add_argument("--trace-summary", nargs=3, metavar=("DB", "TITLE", "COMMAND"))
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary)
+        return
+    if args.trace_summary:
+        trace_summary(*args.trace_summary)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); this build has no multi-GPU path",
               file=sys.stderr)
         sys.exit(2)
+    if args.probe:
+        probe(args.levels)
+        return
     import torch
-    import tpamr
-    from tpamr import workloads as W
 
     t0 = time.perf_counter()
-    mesh, circles = W.circles_mesh(tuple(args.levels))
-    c = W.liquid_fraction(mesh, circles)
-    log(f"mesh {mesh} ({time.perf_counter() - t0:.1f} s)")
-    u = W.smooth_velocity(mesh, device="cuda:0")
-    log(f"velocity ({time.perf_counter() - t0:.1f} s)")
-
     results = {}
     x_sol = {}
+    mesh = None
     for prec in ("fp32_vcycle", "fp64"):
         t1 = time.perf_counter()
-        solver = tpamr.EllipticSolver(tpamr.PoissonOperator(mesh, 1.0), precision=prec)
-        tpamr.refresh_two_phase(solver, c, (1000.0, 1.0))
-        log(f"{prec}: hierarchy {[lv['ncells'] for lv in solver.levels]} ({time.perf_counter() - t1:.1f} s)")
-        rhs = -tpamr.divergence(solver, u)
-        d_rhs = torch.as_tensor(rhs, device="cuda:0")
+        mesh, solver, d_rhs = build_case(args.levels, prec)
+        log(f"{prec}: {mesh.ncells} cells, hierarchy {[lv['ncells'] for lv in solver.levels]} "
+            f"({time.perf_counter() - t1:.1f} s)")
         d_x = torch.zeros(mesh.ncells, dtype=torch.float64, device="cuda:0")
         wall, dev, st = timed_solves(solver, d_rhs, d_x, args.steps, args.warmup, args.rtol)
         x_sol[prec] = d_x.cpu().numpy()
@@ -118,18 +285,23 @@ def main():
                    target_linf=st.target_linf, initial_linf=st.initial_linf)
         if prec == "fp32_vcycle":
             ms, nbytes = solver.time_smoother(reps=50)
+            info = solver.level_info(0)
             res["smoother_ms"] = ms
-            res["smoother_bytes"] = nbytes
+            res["smoother_kernel_bytes"] = nbytes
+            res["smoother_bytes"] = sweep_algorithmic_bytes(mesh.ncells, info["irregular_nnz"])
+            res["irregular_nnz"] = info["irregular_nnz"]
             res["levels"] = len(solver.levels)
         results[prec] = res
         log(f"{prec}: {res}")
         solver.close()
         del d_rhs, d_x
         torch.cuda.empty_cache()
+    log(f"timed workload done ({time.perf_counter() - t0:.1f} s)")
 
     f32, f64 = results["fp32_vcycle"], results["fp64"]
     err = float(np.max(np.abs(x_sol["fp32_vcycle"] - x_sol["fp64"])) / np.max(np.abs(x_sol["fp64"])))
     achieved = f32["smoother_bytes"] / (f32["smoother_ms"] * 1e-3) / 1e9
+    traffic = committed_traffic()
     out = {
         "metric": METRIC,
         "value": f32["mcells"],
@@ -151,14 +323,15 @@ def main():
         "fp64_solver": {"value": f64["mcells"], "ms_per_step": f64["ms_per_step"], "cycles": f64["cycles"],
                         "converged": bool(f64["converged"])},
         "speedup_vs_fp64": f32["mcells"] / f64["mcells"],
-        "pressure_error_fp32_vs_fp64": err,
+        "pressure_error_fp32_vs_fp64_gpu": err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_sweep<float,16,16,FINE>", "bytes_per_launch": f32["smoother_bytes"],
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": SWEEP_KERNEL, "bytes_per_launch": f32["smoother_bytes"],
+                     "kernel_compulsory_bytes": f32["smoother_kernel_bytes"],
                      "launch_ms": f32["smoother_ms"]},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(tuple(args.cpu_levels), args.rtol)
+        out["cpu_baseline"], out["cpu_sample_on_gpu"] = cpu_baseline(tuple(args.cpu_levels), args.rtol)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
