@@ -43,7 +43,7 @@ typedef struct {
   double solve_ms; /* device time of the solve, HIP events on the context stream */
 } tpamr_stats;
 
-/* MultigridOptions, src/solvers/multigrid/options.jl:61-95 */
+/* MultigridOptions, src/solvers/multigrid/options.jl:1-52 */
 typedef struct {
   int coarse_size, pre_sweeps, post_sweeps, max_levels;
 } tpamr_multigrid_options;
@@ -101,6 +101,18 @@ int tpamr_solve(tpamr_ctx* ctx, const double* rhs, double* x, double rtol, doubl
                 int compatibility, tpamr_stats* stats);
 int tpamr_solve_device(tpamr_ctx* ctx, const double* d_rhs, double* d_x, double rtol, double atol, int max_cycles,
                        int compatibility, tpamr_stats* stats);
+
+/* One Krylov run at the plugin seam: replaces the nine-argument
+ * TwoPhaseAMR._run_krylov!(workspace, matrix, rhs, x0, preconditioner,
+ * target, volumes, max_iterations, options) (elliptic_solver.jl:393-452)
+ * for a CG workspace: V-cycle-preconditioned CG on the integrated,
+ * compatible `rhs` from `x0` until max_i |r_i|/V_i <= target (the
+ * StrongResidualCallback), breakdown, or `max_iterations` (0: 2N). `x`
+ * receives Krylov.solution, *iterations Krylov.iteration_count; *reason
+ * (may be NULL): 1 solved, 2 callback, 3 zero curvature, 4 iteration limit,
+ * 5 indefinite preconditioner. */
+int tpamr_krylov(tpamr_ctx* ctx, const double* rhs, const double* x0, double target, int max_iterations, double* x,
+                 int* iterations, int* reason);
 
 /* ldiv!(z, P, r) of the V-cycle preconditioner (cycles.jl:262-283), integrated r */
 int tpamr_vcycle(tpamr_ctx* ctx, const double* r, double* z);

@@ -120,6 +120,34 @@ def test_solve_matches_oracle(name, kind, precision):
     gs.close()
 
 
+@pytest.mark.parametrize("name,kind", [("amr_mixed", "smooth"), ("circle16", "smooth")])
+def test_krylov_seam_matches_solve(name, kind, precision):
+    """tpamr_krylov (the _run_krylov! seam) from x0 = 0 on the integrated,
+    compatible rhs reproduces the first Krylov run of the full solve."""
+    _print("case krylov seam", name, kind, precision)
+    om, pm = meshes(name)
+    k = kappa_for(pm, kind)
+    os_ = OSolver(om, k)
+    gs = tpamr.EllipticSolver(tpamr.PoissonOperator(pm, k), precision=precision)
+    b = techref_noise(pm.ncells)
+    xs = np.zeros(pm.ncells)
+    st = gs.solve(xs, b, rtol=1e-10, max_cycles=200)
+    V = pm.volumes
+    bi = V * b
+    bi -= V * (bi.sum() / V.sum())
+    x, it, reason = gs.run_krylov(bi, np.zeros(pm.ncells), st.target_linf, 200)
+    assert reason == "strong residual target"
+    x -= np.dot(V, x) / V.sum()
+    assert it == st.cycles
+    assert rel(x, xs) < 1e-12
+    # the oracle's operator confirms the strong residual of the iterate
+    r = (bi - os_.apply(x)) / V
+    assert np.max(np.abs(r)) <= 1.01 * st.target_linf
+    with pytest.raises(ValueError):
+        gs.run_krylov(bi, np.zeros(pm.ncells), -1.0)
+    gs.close()
+
+
 TECHREF = [("uniform64", 4096, 4, 10), ("uniform128", 16384, 5, 10), ("amr32", 7552, 5, 10), ("amr64", 19648, 6, 11)]
 
 
@@ -231,6 +259,72 @@ def test_adjoint_matches_oracle_and_finite_differences():
         gs.operator.replace_face_coefficient(k)
         fd = (vals[0] - vals[1]) / (2 * eps)
         assert abs(fd - dk @ dkap) <= 1e-6 * max(abs(fd), 1e-30)
+    gs.close()
+
+
+def test_adjoint_two_phase_circle_twenty_directions():
+    """Differentiation case on the two-phase circle mesh (one 16x16 root, levels
+    3-7 within 4 finest cells of r = 0.15 at (0.5, 0.6), density 1000:1): J = w.p
+    with seeded w ~ N(0,1); the rrule gradient w.r.t. the rhs and the face
+    coefficients beta_f = 1/rho_f against the FP64 CPU (oracle) gradient and
+    against central differences (eps = 1e-6) along 20 seeded directions.
+
+    At 1000:1 a cold solve stops at the round-off floor, whose error divided
+    by 2 eps would swamp the difference quotient. Each perturbed solution is
+    therefore p0 + dp, one step of iterative refinement from the unperturbed
+    solution p0 (A dp = b - A p0 for the perturbed A or b), and the common
+    w.p0 is cancelled before rounding: J+ - J- = w.(dp+ - dp-)."""
+    from tpamr import workloads as W
+    _print("case adjoint two-phase circle")
+    mesh, circ = W.single_circle_mesh()
+    om = OMesh(base_mesh=mesh.base_mesh, block_size=mesh.block_size, periodic=mesh.periodic,
+               max_level=mesh.max_level, refined_keys=mesh.refined_keys)
+    c = W.liquid_fraction(mesh, circ)
+    k = om.mixture_coefficients(c, 1000.0, 1.0)
+    gs = tpamr.EllipticSolver(tpamr.PoissonOperator(mesh, k))
+    V = mesh.volumes
+    b = -om.divergence(W.smooth_velocity(mesh))
+    b -= np.dot(V, b) / V.sum()
+    rng = W.SplitMix64(W.SEED + 4)
+    w = np.array([rng.normal() for _ in range(mesh.ncells)])
+    p0, pull = tpamr.pressure_solve_rrule(gs, b, rtol=1e-13)
+    db, dk = pull(w)
+    os_ = OSolver(om, k)
+    po, _ = os_.solve(b, rtol=1e-13, max_cycles=200)
+    dbo, dko, _, _ = os_.adjoint(w, po, rtol=1e-13)
+    assert rel(p0, po) < 1e-10
+    assert rel(db, dbo) < 1e-9 and rel(dk, dko) < 1e-9
+
+    def correction(rhs):
+        r = rhs - gs.apply(p0) / V
+        r -= np.dot(V, r) / V.sum()
+        dp = np.zeros(mesh.ncells)
+        gs.solve(dp, r, rtol=1e-12, max_cycles=200)
+        return dp
+
+    eps = 1e-6
+    worst = 0.0
+    for q in range(20):
+        if q % 2 == 0:
+            d = np.array([rng.normal() for _ in range(mesh.ncells)])
+            d -= np.dot(V, d) / V.sum()
+            dpp, dpm = correction(b + eps * d), correction(b - eps * d)
+            ad = db @ d
+        else:
+            d = np.array([rng.normal() for _ in range(mesh.nfaces)]) * k
+            vals = []
+            for sgn in (1, -1):
+                gs.operator.replace_face_coefficient(k + sgn * eps * d)
+                vals.append(correction(b))
+            gs.operator.replace_face_coefficient(k)
+            dpp, dpm = vals
+            ad = dk @ d
+        fd = w @ (dpp - dpm) / (2 * eps)
+        err = abs(fd - ad) / max(abs(fd), 1e-30)
+        worst = max(worst, err)
+        _print(f"direction {q} ({'rhs' if q % 2 == 0 else 'beta'}): fd {fd:.12e} adjoint {ad:.12e} rel {err:.2e}")
+        assert err <= 1e-6, (q, fd, ad)
+    _print(f"adjoint vs central differences: worst relative difference {worst:.2e} over 20 directions")
     gs.close()
 
 

@@ -974,6 +974,28 @@ int tpamr_solve(tpamr_ctx* c, const double* rhs, double* x, double rtol, double 
   });
 }
 
+int tpamr_krylov(tpamr_ctx* c, const double* rhs, const double* x0, double target, int max_iterations, double* x,
+                 int* iterations, int* reason) {
+  if (!c || !rhs || !x0 || !x || !iterations) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    need_solver(c);
+    if (max_iterations < 0) throw std::invalid_argument("max_iterations must be nonnegative");
+    if (!(target >= 0)) throw std::invalid_argument("target must be a nonnegative number");
+    HIPCHECK(hipSetDevice(c->device));
+    const int N = c->topo[0].ncells;
+    check_finite(rhs, N, "right-hand side");
+    check_finite(x0, N, "initial solution");
+    upload_vec(c, c->b.p, rhs, N);
+    upload_vec(c, c->x.p, x0, N);
+    const int it = c->precision == TPAMR_FP32_VCYCLE ? run_cg<float>(c, c->lf, target, max_iterations)
+                                                     : run_cg<double>(c, c->ld, target, max_iterations);
+    download_vec(c, x, c->x.p, N);
+    *iterations = it;
+    if (reason) *reason = c->st_host->reason;
+  });
+}
+
 int tpamr_vcycle(tpamr_ctx* c, const double* r, double* z) {
   if (!c || !r || !z) return TPAMR_EINVAL;
   return guard(c, [&] {

@@ -1,0 +1,157 @@
+"""
+    TPAMRShim
+
+Julia side of the MI355X pressure solver: a Krylov workspace that TwoPhaseAMR.jl's
+`LinearSolverOptions(krylov = factory)` seam accepts (src/solvers/multigrid/cycles.jl:417-432),
+whose `_run_krylov!` method (src/solvers/multigrid/elliptic_solver.jl:393-452) runs
+V-cycle-preconditioned CG on the GPU through `libtpamr_hip.so` (include/tpamr.h).
+
+The reference keeps its own driver around the seam: compatible RHS, true residual, round-off
+floor, restart loop and stats (`_solve_krylov_with_stats!`, `_run_krylov_reliably!`). Each
+Krylov run goes to the device.
+
+Usage:
+
+    using TwoPhaseAMR, TPAMRShim
+    options = NavierStokesOptions(; pressure = LinearSolverOptions(
+        krylov = TPAMRShim.gpu_krylov(; precision = :fp32_vcycle), preconditioner = nothing))
+
+Loading the shim needs the built library at `ENV["TPAMR_LIB"]` or `../lib/libtpamr_hip.so`.
+"""
+module TPAMRShim
+
+using Krylov
+using TwoPhaseAMR
+
+const LIB = get(ENV, "TPAMR_LIB", joinpath(@__DIR__, "..", "lib", "libtpamr_hip.so"))
+
+const FP64 = Cint(0)
+const FP32_VCYCLE = Cint(1)
+
+function _check(ctx::Ptr{Cvoid}, rc::Cint)
+    rc == 0 && return nothing
+    msg = ctx == C_NULL ? "tpamr: error $rc" :
+        unsafe_string(ccall((:tpamr_last_error, LIB), Cstring, (Ptr{Cvoid},), ctx))
+    rc == -2 && throw(DimensionMismatch(msg))
+    rc in (-1, -3, -6) && throw(ArgumentError(msg))
+    throw(ErrorException(msg))
+end
+
+"""GPU CG workspace for the `krylov` seam. `m`, `n` are read by `_construct_krylov`."""
+mutable struct GpuCgWorkspace
+    ctx::Ptr{Cvoid}
+    m::Int
+    n::Int
+    x::Vector{Float64}
+    iterations::Int
+    mesh_generation::UInt
+    mesh_identity::Any
+    coefficient::Vector{Float64}
+end
+
+Krylov.solution(ws::GpuCgWorkspace) = ws.x
+Krylov.iteration_count(ws::GpuCgWorkspace) = ws.iterations
+
+function _destroy!(ws::GpuCgWorkspace)
+    ws.ctx == C_NULL || ccall((:tpamr_destroy, LIB), Cvoid, (Ptr{Cvoid},), ws.ctx)
+    ws.ctx = C_NULL
+    return nothing
+end
+
+"""
+    gpu_krylov(; device = 0, precision = :fp32_vcycle, multigrid = MultigridOptions())
+
+Factory `(matrix, rhs) -> GpuCgWorkspace` for `LinearSolverOptions(krylov = ...)`.
+"""
+function gpu_krylov(; device::Integer = 0, precision::Symbol = :fp32_vcycle,
+        multigrid = TwoPhaseAMR.MultigridOptions())
+    mode = precision === :fp64 ? FP64 : precision === :fp32_vcycle ? FP32_VCYCLE :
+        throw(ArgumentError("precision must be :fp64 or :fp32_vcycle"))
+    return function (matrix, rhs)
+        handle = Ref{Ptr{Cvoid}}(C_NULL)
+        _check(C_NULL, ccall((:tpamr_create, LIB), Cint, (Ref{Ptr{Cvoid}}, Cint, Cint), handle, device, mode))
+        opts = Cint[multigrid.coarse_size, multigrid.pre_sweeps, multigrid.post_sweeps, multigrid.max_levels]
+        _check(handle[], ccall((:tpamr_set_multigrid, LIB), Cint, (Ptr{Cvoid}, Ptr{Cint}), handle[], opts))
+        n = length(rhs)
+        ws = GpuCgWorkspace(handle[], n, n, zeros(n), 0, typemax(UInt), nothing, Float64[])
+        finalizer(_destroy!, ws)
+        return ws
+    end
+end
+
+# Refined nodes of the tree: every proper ancestor of every leaf. With the
+# library's 2:1 balance this reproduces the mesh (tpamr_set_mesh).
+function _refined_keys(mesh)
+    seen = Set{NTuple{3, Int}}()
+    for leaf in mesh.blocks
+        level, i, j = leaf.key.level, leaf.key.i, leaf.key.j
+        while level > 0
+            level -= 1
+            i >>= 1
+            j >>= 1
+            (level, i, j) in seen && break
+            push!(seen, (level, i, j))
+        end
+    end
+    keys = Vector{Cint}(undef, 3 * length(seen))
+    for (q, (l, i, j)) in enumerate(seen)
+        keys[3q - 2] = l
+        keys[3q - 1] = i
+        keys[3q] = j
+    end
+    return keys, length(seen)
+end
+
+function _sync!(ws::GpuCgWorkspace, operator)
+    mesh = operator.mesh
+    generation = mesh.topology_cache.generation
+    if ws.mesh_identity !== mesh || ws.mesh_generation != generation
+        keys, count = _refined_keys(mesh)
+        base = Cint[mesh.base_mesh...]
+        block = Cint[mesh.block_size...]
+        periodic = Cint[mesh.periodic...]
+        origin = Float64[mesh.origin...]
+        extent = Float64[mesh.extent...]
+        GC.@preserve keys base block periodic origin extent _check(ws.ctx, ccall((:tpamr_set_mesh, LIB), Cint,
+            (Ptr{Cvoid}, Ptr{Cint}, Ptr{Cint}, Ptr{Float64}, Ptr{Float64}, Ptr{Cint}, Cint, Cint, Ptr{Cint}),
+            ws.ctx, base, block, origin, extent, periodic, mesh.max_level, count, keys))
+        ws.mesh_identity = mesh
+        ws.mesh_generation = generation
+        empty!(ws.coefficient)
+    end
+    # two-phase steps rewrite the operator's coefficients in place
+    # (two_phase/stepping.jl:79-110 with _replace_face_coefficient): compare values
+    kappa = operator.face_coefficient
+    if ws.coefficient != kappa
+        GC.@preserve kappa _check(ws.ctx, ccall((:tpamr_set_face_coefficients, LIB), Cint,
+            (Ptr{Cvoid}, Ptr{Float64}), ws.ctx, kappa))
+        ws.coefficient = copy(kappa)
+    end
+    return ws
+end
+
+"""
+    TwoPhaseAMR._run_krylov!(ws::GpuCgWorkspace, matrix, rhs, x0, preconditioner, target,
+                             volumes, max_iterations, options)
+
+The nine-argument seam method (elliptic_solver.jl:403-419): `rhs` is the integrated,
+compatible right-hand side, `target` the strong L-infinity target of the
+StrongResidualCallback. The GPU owns the V-cycle, so `preconditioner` is ignored
+(configure `preconditioner = nothing`).
+"""
+function TwoPhaseAMR._run_krylov!(ws::GpuCgWorkspace, matrix, rhs, x0, preconditioner, target,
+        volumes, max_iterations, options)
+    _sync!(ws, matrix.operator)
+    length(rhs) == ws.n || throw(DimensionMismatch("right-hand side length"))
+    b = rhs isa Vector{Float64} ? rhs : Vector{Float64}(rhs)
+    x_init = x0 isa Vector{Float64} ? x0 : Vector{Float64}(x0)
+    iterations = Ref{Cint}(0)
+    reason = Ref{Cint}(0)
+    GC.@preserve b x_init _check(ws.ctx, ccall((:tpamr_krylov, LIB), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Float64, Cint, Ptr{Float64}, Ref{Cint}, Ref{Cint}),
+        ws.ctx, b, x_init, Float64(target), Cint(max_iterations), ws.x, iterations, reason))
+    ws.iterations = Int(iterations[])
+    return ws
+end
+
+end # module

@@ -71,6 +71,8 @@ _SIGNATURES = {
     "tpamr_solve_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int,
                                      C.c_int, C.POINTER(Stats)]),
     "tpamr_vcycle": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "tpamr_krylov": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p,
+                               C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "tpamr_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "tpamr_apply_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "tpamr_mixture_face_coefficients": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_void_p]),

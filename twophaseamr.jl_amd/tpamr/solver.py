@@ -1,7 +1,7 @@
 """Pressure solver surface mirroring TwoPhaseAMR.jl's linear-solver API.
 
 Reference interfaces replaced here (file:line in TwoPhaseAMR.jl):
-  MultigridOptions       src/solvers/multigrid/options.jl:61-95
+  MultigridOptions       src/solvers/multigrid/options.jl:1-52
   LinearSolverOptions    src/solvers/linear_options.jl:10-47
   PoissonOperator        src/operators/elliptic.jl:969-980 (face_coefficient form)
   EllipticSolver         src/solvers/multigrid/elliptic_solver.jl:282-302
@@ -244,6 +244,21 @@ class EllipticSolver:
                 self.ctx.h)
         self.stats = LinearSolverStats._from(st, self.precision)
         return self.stats
+
+    def run_krylov(self, rhs, x0, target, max_iterations=0):
+        """One Krylov run at the plugin seam (`_run_krylov!`,
+        elliptic_solver.jl:393-452): CG on the integrated, compatible `rhs`
+        from `x0`, preconditioned by one V-cycle, until the strong residual
+        max |r|/V <= target. Returns (x, iterations, reason)."""
+        self._sync()
+        rhs = L.f64(rhs, self.n, "rhs")
+        x0 = L.f64(x0, self.n, "x0")
+        x = np.zeros(self.n)
+        it = C.c_int(0)
+        reason = C.c_int(0)
+        L.check(L.lib().tpamr_krylov(self.ctx.h, L.ptr(rhs), L.ptr(x0), float(target), int(max_iterations),
+                                     L.ptr(x), C.byref(it), C.byref(reason)), self.ctx.h)
+        return x, it.value, REASONS.get(reason.value, str(reason.value))
 
     def solve_device(self, d_rhs, d_x, *, rtol=1.0e-10, atol=0.0, max_cycles=100, compatibility="project"):
         """solve! on device buffers (raw device pointers, e.g. torch .data_ptr())."""
