@@ -25,7 +25,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
 DEVICE_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                 "-munsafe-fp-atomics"]
 SOURCES = ["hier.cpp", "kernels.hip", "runtime.cpp"]
-HEADERS = ["hier.hpp", "kernels.hpp"]
+HEADERS = ["hier.hpp", "kernels.hpp", "tile.hpp"]
 
 
 def _newer(target, deps):

@@ -545,17 +545,30 @@ void build_ell(LevelData& L) {
   L.ell_ref.assign(size_t(nrows) * w, 0);
   L.ell_val.assign(size_t(nrows) * w, 0.0);
   L.ell_bound.assign(size_t(nrows) * w, 0.0);
+  L.ell_loc.assign(nrows, 0u);
   L.blk_ext.assign(L.nblocks + 1, 0);
   L.ext_col.clear();
   std::unordered_map<int, int> ext;
+  std::vector<int> order;
   for (int b = 0; b < L.nblocks; ++b) {
     ext.clear();
     const int* nbr = &L.nbr[4 * b];
-    for (int r = L.blk_irr[b]; r < L.blk_irr[b + 1]; ++r) {
+    const int r0 = L.blk_irr[b], nr = L.blk_irr[b + 1] - r0;
+    // rows of the block by decreasing length: the lanes reading column k of
+    // the block's column-major region are then a prefix
+    order.resize(nr);
+    for (int x = 0; x < nr; ++x) order[x] = r0 + x;
+    std::stable_sort(order.begin(), order.end(), [&](int p, int q) {
+      return L.irr_ptr[p + 1] - L.irr_ptr[p] > L.irr_ptr[q + 1] - L.irr_ptr[q];
+    });
+    for (int x = 0; x < nr; ++x) {
+      const int r = order[x];
       const int loc = L.irr_local[r];
+      const int len = L.irr_ptr[r + 1] - L.irr_ptr[r];
       L.irr_mask[size_t(b) * nwords + loc / 32] |= 1u << (loc % 32);
+      L.ell_loc[r0 + x] = uint32_t(loc) | (uint32_t(len) << 16);
       const int own = ((loc / bx) + 1) * W + (loc % bx) + 1;
-      for (int k = 0; k < w; ++k) L.ell_ref[size_t(r) * w + k] = uint16_t(own);
+      for (int k = 0; k < w; ++k) L.ell_ref[size_t(r0) * w + size_t(k) * nr + x] = uint16_t(own);
       int k = 0;
       for (int e = L.irr_ptr[r]; e < L.irr_ptr[r + 1]; ++e, ++k) {
         const int col = L.irr_col[e];
@@ -574,9 +587,10 @@ void build_ell(LevelData& L) {
           }
           ref = tile + it->second;
         }
-        L.ell_ref[size_t(r) * w + k] = uint16_t(ref);
-        L.ell_val[size_t(r) * w + k] = L.irr_val[e];
-        L.ell_bound[size_t(r) * w + k] = L.irr_bound[e];
+        const size_t at = size_t(r0) * w + size_t(k) * nr + x;
+        L.ell_ref[at] = uint16_t(ref);
+        L.ell_val[at] = L.irr_val[e];
+        L.ell_bound[at] = L.irr_bound[e];
       }
     }
     if (int(ext.size()) > nb) throw std::runtime_error("too many external couplings in one block");
@@ -792,6 +806,13 @@ void coarse_inverse(Hierarchy& H) {
   const TreeMesh& M = *L.mesh;
   const int n = L.ncells, nb = M.nb(), bx = M.bs[0], by = M.bs[1];
   const int N = n + 1;  // one component (connected, no anchors)
+  // The coarsest level is solved with a dense inverse. Tree coarsening ends
+  // at coarse_size cells when the root layout agglomerates down to one
+  // block; an odd root layout can stop far above that.
+  if (n > COARSE_MAX)
+    throw std::invalid_argument("coarsest multigrid level has " + std::to_string(n) +
+                                " cells; the dense coarse solve supports at most " + std::to_string(COARSE_MAX) +
+                                " (use a root-block layout that halves down to a few blocks)");
   std::vector<double> a(size_t(N) * N, 0.0);
   for (int b = 0; b < L.nblocks; ++b)
     for (int q = 0; q < nb; ++q) {

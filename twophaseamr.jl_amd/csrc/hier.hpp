@@ -90,11 +90,14 @@ struct LevelData {
   std::vector<int> cell_irr;  // per cell: irregular row index or -1
   std::vector<double> dinv;   // 1 / diag(A)
   // Device form of the irregular rows: a bitmask of irregular cells per
-  // block (NB/32 words), fixed-width rows (ell_w entries each, zero padded)
-  // whose columns are LDS slots of the block's tile: (BX+2)(BY+2) tile and
-  // halo positions, then the block's list of external cells.
+  // block (NB/32 words) and, per block, a column-major region of nr x ell_w
+  // entries (entry k of the block's row x at blk_irr[b]*ell_w + k*nr + x)
+  // with the rows ordered by decreasing length; ell_loc[blk_irr[b] + x] holds
+  // local cell | length << 16. Columns are LDS slots of the block's tile:
+  // (BX+2)(BY+2) tile and halo positions, then the block's external cells.
   int ell_w = 0;
   std::vector<uint32_t> irr_mask;
+  std::vector<uint32_t> ell_loc;
   std::vector<uint16_t> ell_ref;
   std::vector<double> ell_val, ell_bound;
   std::vector<int> blk_ext, ext_col;
@@ -114,6 +117,9 @@ struct Hierarchy {
   std::vector<double> coarse_inv;
   double total_volume = 0;
 };
+
+// Largest coarsest level (cells) of the dense coarse solve.
+constexpr int COARSE_MAX = 8192;
 
 struct MultigridConfig {
   int coarse_size = 64, pre = 2, post = 2, max_levels = 32;

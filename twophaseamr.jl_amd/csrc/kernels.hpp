@@ -20,10 +20,10 @@ struct LevelView {
   const T* ell_val;
   const T* ell_bound;  // floor-bound values (FP64 finest level only)
   const T* irr_diag;
-  const unsigned short* ell_ref;
+  const unsigned short* ell_ref;  // entry k of row x of block b at blk_irr[b] * ell_w + k * nrows_b + x
   const unsigned* irr_mask;  // per block and 32-cell word: (mask, first row index)
   const int* blk_irr;        // nblocks + 1 offsets of the irregular rows per block
-  const int* irr_local;      // local cell of every irregular row
+  const unsigned* ell_loc;   // per row in a block's length order: local cell | length << 16
   const int* blk_ext;
   const int* ext_col;
   const int* nbr;

@@ -33,6 +33,7 @@ struct HipError : std::runtime_error {
   } while (0)
 
 const double EPS = std::numeric_limits<double>::epsilon();
+constexpr int PIPE_MAX = 8;
 
 template <typename T>
 struct DevArray {
@@ -84,8 +85,8 @@ struct DevArray {
 
 struct DevTopo {
   int nblocks = 0, ncells = 0;
-  DevArray<int> nbr, xfer, blk_ext, ext_col, blk_irr, irr_local;
-  DevArray<unsigned> irr_mask;
+  DevArray<int> nbr, xfer, blk_ext, ext_col, blk_irr;
+  DevArray<unsigned> irr_mask, ell_loc;
   DevArray<unsigned short> ell_ref;
   DevArray<double> volume;
 };
@@ -118,7 +119,7 @@ struct tpamr_ctx {
   DevArray<double> part, maxpart, scal, stage;
   CgState* st = nullptr;      // device
   CgState* st_host = nullptr; // pinned
-  CgState* st_host2 = nullptr;
+  CgState* st_ring = nullptr; // pinned, one per Krylov iteration in flight
   // face data (finest level)
   DevArray<int> fm, fp, kptr, kface, cptr, cface, bptr, bface;
   DevArray<signed char> faxis, csign, bside;
@@ -128,7 +129,10 @@ struct tpamr_ctx {
   // the iteration's kernels one by one instead)
   bool use_graph = true;
   hipGraphExec_t graph = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, evs[2] = {nullptr, nullptr};
+  // Krylov iterations in flight before the host reads a done flag
+  // (TPAMR_PIPE in the environment, 2..PIPE_MAX)
+  int pipe = 3;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evs[PIPE_MAX] = {};
   const TreeMesh& mesh() const {
     if (H) return *H->levels[0]->mesh;
     if (mesh_pending) return *mesh_pending;
@@ -222,7 +226,7 @@ void upload_level_values(tpamr_ctx* c, DevLevel<T>& D, const LevelData& L, const
   v.ell_ref = Tp.ell_ref.p;
   v.irr_mask = Tp.irr_mask.p;
   v.blk_irr = Tp.blk_irr.p;
-  v.irr_local = Tp.irr_local.p;
+  v.ell_loc = Tp.ell_loc.p;
   v.blk_ext = Tp.blk_ext.p;
   v.ext_col = Tp.ext_col.p;
   v.nbr = Tp.nbr.p;
@@ -262,7 +266,7 @@ void upload_hierarchy(tpamr_ctx* c, bool topology) {
       T.ell_ref.upload(L.ell_ref.empty() ? std::vector<unsigned short>(1, 0) : L.ell_ref, s);
       T.blk_ext.upload(L.blk_ext, s);
       T.blk_irr.upload(L.blk_irr, s);
-      T.irr_local.upload(L.irr_local.empty() ? std::vector<int>(1, 0) : L.irr_local, s);
+      T.ell_loc.upload(L.ell_loc.empty() ? std::vector<uint32_t>(1, 0u) : L.ell_loc, s);
       T.ext_col.upload(L.ext_col.empty() ? std::vector<int>(1, 0) : L.ext_col, s);
       T.volume.upload(L.volume, s);
     }
@@ -507,21 +511,25 @@ int run_cg(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double target, int itmax)
   launch_fill(s, N, 0.0, c->p.p);
   enqueue_iteration<T>(c, lv, true);
   if (c->use_graph && !c->graph) build_graph<T>(c, lv);
-  CgState* slot[2] = {c->st_host, c->st_host2};
+  // `pipe` iterations stay queued: the host reads the done flag of the
+  // oldest one while the GPU runs the others; iterations queued after the
+  // last one exit at once on the device-side done flag.
+  const int P = c->pipe;
   int launched = 0;
   for (;;) {
     if (c->use_graph)
       HIPCHECK(hipGraphLaunch(c->graph, s));
     else
       enqueue_iteration<T>(c, lv, false);
-    HIPCHECK(hipMemcpyAsync(slot[launched & 1], c->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipEventRecord(c->evs[launched & 1], s));
+    HIPCHECK(hipMemcpyAsync(c->st_ring + launched % P, c->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipEventRecord(c->evs[launched % P], s));
     ++launched;
-    if (launched >= 2) {
-      HIPCHECK(hipEventSynchronize(c->evs[(launched - 2) & 1]));
-      if (slot[(launched - 2) & 1]->done) break;
+    if (launched >= P) {
+      const int q = (launched - P) % P;
+      HIPCHECK(hipEventSynchronize(c->evs[q]));
+      if (c->st_ring[q].done) break;
     }
-    if (launched > init.itmax + 2) break;
+    if (launched > init.itmax + P) break;
   }
   HIPCHECK(hipStreamSynchronize(s));
   HIPCHECK(hipMemcpy(c->st_host, c->st, sizeof(CgState), hipMemcpyDeviceToHost));
@@ -673,6 +681,7 @@ int tpamr_create(tpamr_ctx** out, int device, int precision) {
   c->precision = precision;
   c->device = device;
   if (const char* g = std::getenv("TPAMR_GRAPH")) c->use_graph = std::strcmp(g, "0") != 0;
+  if (const char* q = std::getenv("TPAMR_PIPE")) c->pipe = std::min(PIPE_MAX, std::max(2, std::atoi(q)));
   if (device >= 0) {
     int rc = guard(c.get(), [&] {
       int n = 0;
@@ -682,11 +691,10 @@ int tpamr_create(tpamr_ctx** out, int device, int precision) {
       HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
       HIPCHECK(hipMalloc(&c->st, sizeof(CgState)));
       HIPCHECK(hipHostMalloc(&c->st_host, sizeof(CgState), hipHostMallocDefault));
-      HIPCHECK(hipHostMalloc(&c->st_host2, sizeof(CgState), hipHostMallocDefault));
+      HIPCHECK(hipHostMalloc(&c->st_ring, PIPE_MAX * sizeof(CgState), hipHostMallocDefault));
       HIPCHECK(hipEventCreate(&c->ev0));
       HIPCHECK(hipEventCreate(&c->ev1));
-      HIPCHECK(hipEventCreateWithFlags(&c->evs[0], hipEventDisableTiming));
-      HIPCHECK(hipEventCreateWithFlags(&c->evs[1], hipEventDisableTiming));
+      for (int q = 0; q < PIPE_MAX; ++q) HIPCHECK(hipEventCreateWithFlags(&c->evs[q], hipEventDisableTiming));
     });
     if (rc != TPAMR_OK) {
       // keep the context so the caller can read the message
@@ -709,8 +717,10 @@ void tpamr_destroy(tpamr_ctx* c) {
     c->lf.clear();
     if (c->st) (void)hipFree(c->st);
     if (c->st_host) (void)hipHostFree(c->st_host);
-    if (c->st_host2) (void)hipHostFree(c->st_host2);
-    for (hipEvent_t e : {c->ev0, c->ev1, c->evs[0], c->evs[1]})
+    if (c->st_ring) (void)hipHostFree(c->st_ring);
+    for (hipEvent_t e : {c->ev0, c->ev1})
+      if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->evs)
       if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
   }
@@ -1231,7 +1241,8 @@ int tpamr_time_smoother(tpamr_ctx* c, int reps, double* avg_ms, double* bytes) {
       const double cells = double(N) * 3 * elem;
       const double cond = double(nb) * ((bx + 1) * by + bx * (by + 1)) * elem;
       const double mask = double(nb) * ((bx * by + 31) / 32) * 8;
-      const double irr = double(L.irr_local.size()) * (double(L.ell_w) * (elem + 2) + elem) +
+      const double nnz = L.irr_ptr.empty() ? 0.0 : double(L.irr_ptr.back());
+      const double irr = nnz * (elem + 2) + double(L.irr_local.size()) * (elem + 4) +
                          double(L.ext_col.size()) * (4 + elem);
       *bytes = cells + cond + mask + irr;
     }

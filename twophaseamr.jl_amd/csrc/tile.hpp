@@ -112,6 +112,8 @@ struct Place {
     const int k = t / G::TPB;
     lt = t - k * G::TPB;
     b = wg * G::BPW + k;
+    // a block fills whole wavefronts when TPB >= 64: keep its index scalar
+    if constexpr (G::TPB >= 64) b = __builtin_amdgcn_readfirstlane(b);
     valid = b < nblocks;
     j = lt / G::TPR;
     q = lt - j * G::TPR;
@@ -142,7 +144,7 @@ __device__ inline void tile4(Tile4<T, BX, BY>* St, const LevelView<T>& L, const 
   int nr = 0;
   T rv[ELL_MAX];
   unsigned short rr[ELL_MAX];
-  int rloc = 0;
+  int rloc = 0, rlen = 0;
   unsigned bits = 0;
   const int w = L.ell_w;
   if (P.valid) {
@@ -174,19 +176,27 @@ __device__ inline void tile4(Tile4<T, BX, BY>* St, const LevelView<T>& L, const 
         const T ev = fetch.one(L.ext_col[e0 + e]);
         S.v[TILE + e] = ABS ? fabs(ev) : ev;
       }
-      // this lane's irregular row
-      const int r0 = L.blk_irr[b];
+      // this lane's irregular row (rows in decreasing length order, entries
+      // column-major per block: the active lanes of column k are a prefix)
+      int r0 = L.blk_irr[b];
       nr = L.blk_irr[b + 1] - r0;
+      if constexpr (G::TPB >= 64) {
+        // uniform per wavefront: column offsets k * nr become scalar
+        r0 = __builtin_amdgcn_readfirstlane(r0);
+        nr = __builtin_amdgcn_readfirstlane(nr);
+      }
       if (lt < nr) {
-        const int r = r0 + lt;
+        const unsigned info = L.ell_loc[r0 + lt];
+        rloc = int(info & 0xffffu);
+        rlen = int(info >> 16);
+        const size_t base = size_t(r0) * w + lt;
         const T* vals = ABS ? L.ell_bound : L.ell_val;
 #pragma unroll
         for (int k = 0; k < ELL_MAX; ++k)
-          if (k < w) {
-            rv[k] = vals[size_t(r) * w + k];
-            rr[k] = L.ell_ref[size_t(r) * w + k];
+          if (k < rlen) {
+            rv[k] = vals[base + size_t(k) * nr];
+            rr[k] = L.ell_ref[base + size_t(k) * nr];
           }
-        rloc = L.irr_local[r];
       }
       // irregular cells of this segment and their diagonals
       const int c0 = j * BX + i0, wd = c0 >> 5, bit = c0 & 31;
@@ -211,20 +221,29 @@ __device__ inline void tile4(Tile4<T, BX, BY>* St, const LevelView<T>& L, const 
   __syncthreads();
   if (any) {
     if (P.valid) {
+      // Rows sum to zero (Neumann / periodic operator), so (A v)_i is
+      // evaluated in flux form sum_j a_ij (v_j - v_i): the diagonal and the
+      // zero padding drop out and rounding scales with the differences, as
+      // in the reference's face-based product. ABS keeps sum_j |b_ij||v_j|.
       if (lt < nr) {
+        const T vi = ABS ? T(0) : S.v[(rloc / BX + 1) * W + rloc % BX + 1];
         T s = 0;
 #pragma unroll
         for (int k = 0; k < ELL_MAX; ++k)
-          if (k < w) s += rv[k] * S.v[rr[k]];
+          if (k < rlen) s += rv[k] * (S.v[rr[k]] - vi);
         S.ax[rloc] = s;
       }
       // blocks with more irregular rows than lanes
       for (int x = lt + TPB; x < nr; x += TPB) {
-        const int r = L.blk_irr[b] + x;
+        const int r0 = L.blk_irr[b];
+        const unsigned info = L.ell_loc[r0 + x];
+        const int loc = int(info & 0xffffu), len = int(info >> 16);
+        const size_t base = size_t(r0) * w + x;
         const T* vals = ABS ? L.ell_bound : L.ell_val;
+        const T vi = ABS ? T(0) : S.v[(loc / BX + 1) * W + loc % BX + 1];
         T s = 0;
-        for (int k = 0; k < w; ++k) s += vals[size_t(r) * w + k] * S.v[L.ell_ref[size_t(r) * w + k]];
-        S.ax[L.irr_local[r]] = s;
+        for (int k = 0; k < len; ++k) s += vals[base + size_t(k) * nr] * (S.v[L.ell_ref[base + size_t(k) * nr]] - vi);
+        S.ax[loc] = s;
       }
     }
     __syncthreads();
@@ -251,8 +270,9 @@ __device__ inline void tile4(Tile4<T, BX, BY>* St, const LevelView<T>& L, const 
     const T cs = cs4.v[m];
     const T cn = j < BY - 1 ? S.cs[(j + 1) * BX + i] : cn4.v[m];
     const T dd = (cw + ce) + (cs + cn);
-    const T nbs = (cw * vw + ce * ve) + (cs * vs + cn * vn);
-    const T st = ABS ? dd * own.v[m] + nbs : dd * own.v[m] - nbs;
+    const T vi = own.v[m];
+    const T st = ABS ? dd * vi + ((cw * vw + ce * ve) + (cs * vs + cn * vn))
+                     : (cw * (vi - vw) + ce * (vi - ve)) + (cs * (vi - vs) + cn * (vi - vn));
     const bool irr = (bits >> m) & 1u;
     ax.v[m] = irr ? axi.v[m] : st;
     d.v[m] = irr ? rd.v[m] : dd;
