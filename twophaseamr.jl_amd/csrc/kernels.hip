@@ -92,6 +92,16 @@ __device__ inline int xcd_block(int w, int G) {
   return k * q + min(k, rem) + (w >> 3);
 }
 
+// Block group of tile workgroup `w` for a grid that is a multiple of 8*S:
+// the XCD (w mod 8) takes runs of S consecutive groups spread over the whole
+// block order. Neighbouring blocks in a run share the XCD's L2 for halo
+// reads, and every XCD sees a similar mix of block levels and
+// coarse-fine interfaces.
+__device__ inline int xcd_chunk_group(int w, int S) {
+  const int x = w & 7, s = w >> 3;
+  return ((s / S) * 8 + x) * S + s % S;
+}
+
 #include "tile.hpp"
 
 // ---------------------------------------------------------------------------
@@ -546,8 +556,8 @@ bool supported_block(int bx, int by) { return bx == by && (bx == 4 || bx == 8 ||
 
 // grid and workgroup of the block-tile kernels
 template <int BX, int BY>
-static inline dim3 tile_grid(int nblocks) {
-  return dim3(cdiv(nblocks, Geo<BX, BY>::BPW));
+static inline dim3 tile_grid(int wg_grid) {
+  return dim3(wg_grid);
 }
 template <int BX, int BY>
 static inline dim3 tile_wg() {
@@ -557,7 +567,7 @@ static inline dim3 tile_wg() {
 template <typename T, int BX, int BY>
 static void launch_sweep_t(hipStream_t s, const LevelView<T>& L, const T* xi, T* xo, const T* b, double w,
                            const int* done, bool fine, const double* r64, double* part) {
-  const dim3 g = tile_grid<BX, BY>(L.nblocks), t = tile_wg<BX, BY>();
+  const dim3 g = tile_grid<BX, BY>(L.wg_grid), t = tile_wg<BX, BY>();
   if (part) {
     hipLaunchKernelGGL((k_sweep<T, BX, BY, true, true>), g, t, 0, s, L, xi, xo, b, w, done, r64, part);
   } else if (fine) {
@@ -577,7 +587,7 @@ template <typename T>
 void launch_residual(hipStream_t s, const LevelView<T>& L, const T* x, const T* b, T* u, double wp, const int* done,
                      bool fine) {
   TPAMR_DISPATCH(L.bx, ({
-                   const dim3 g = tile_grid<BX, BY>(L.nblocks), t = tile_wg<BX, BY>();
+                   const dim3 g = tile_grid<BX, BY>(L.wg_grid), t = tile_wg<BX, BY>();
                    if (fine)
                      hipLaunchKernelGGL((k_residual<T, BX, BY, true>), g, t, 0, s, L, x, b, u, wp, done);
                    else
@@ -589,7 +599,7 @@ template <typename T>
 void launch_restrict(hipStream_t s, const LevelView<T>& L, const T* u, double wp, T* bc, T* xc, const T* dinv_c,
                      double wc, int zero_start, const int* done, bool fine) {
   TPAMR_DISPATCH(L.bx, ({
-                   const dim3 g = tile_grid<BX, BY>(L.nblocks), t = tile_wg<BX, BY>();
+                   const dim3 g = tile_grid<BX, BY>(L.wg_grid), t = tile_wg<BX, BY>();
                    if (fine)
                      hipLaunchKernelGGL((k_restrict<T, BX, BY, true>), g, t, 0, s, L, u, wp, bc, xc, dinv_c, wc,
                                         zero_start, done);
@@ -602,7 +612,7 @@ void launch_restrict(hipStream_t s, const LevelView<T>& L, const T* u, double wp
 template <typename T>
 void launch_prolong(hipStream_t s, const LevelView<T>& L, T* x, const T* xc, double wp, const int* done, bool fine) {
   TPAMR_DISPATCH(L.bx, ({
-                   const dim3 g = tile_grid<BX, BY>(L.nblocks), t = tile_wg<BX, BY>();
+                   const dim3 g = tile_grid<BX, BY>(L.wg_grid), t = tile_wg<BX, BY>();
                    if (fine)
                      hipLaunchKernelGGL((k_prolong<T, BX, BY, true>), g, t, 0, s, L, x, xc, wp, done);
                    else
@@ -618,14 +628,14 @@ void launch_coarse(hipStream_t s, int n, const double* inv, const T* b, T* x, co
 void launch_matvec_dot(hipStream_t s, const LevelView<double>& L, const double* p, double* y, double* part,
                        const int* done) {
   TPAMR_DISPATCH(L.bx, ({
-                   const dim3 g = tile_grid<BX, BY>(L.nblocks), t = tile_wg<BX, BY>();
+                   const dim3 g = tile_grid<BX, BY>(L.wg_grid), t = tile_wg<BX, BY>();
                    hipLaunchKernelGGL((k_matvec_dot<BX, BY>), g, t, 0, s, L, p, y, part, done);
                  }));
 }
 
 void launch_floor_bound(hipStream_t s, const LevelView<double>& L, const double* x, const double* b, double* part) {
   TPAMR_DISPATCH(L.bx, ({
-                   const dim3 g = tile_grid<BX, BY>(L.nblocks), t = tile_wg<BX, BY>();
+                   const dim3 g = tile_grid<BX, BY>(L.wg_grid), t = tile_wg<BX, BY>();
                    hipLaunchKernelGGL((k_floor_bound<BX, BY>), g, t, 0, s, L, x, b, part);
                  }));
 }

@@ -11,6 +11,9 @@ constexpr int ELL_MAX = 12;
 template <typename T>
 struct LevelView {
   int nblocks, ncells, bx, by, ell_w;
+  // tile-kernel grid: wg_grid workgroups (a multiple of 8 * wg_chunk); runs of
+  // wg_chunk consecutive block groups are dealt to the 8 XCDs in turn
+  int wg_chunk, wg_grid;
   // conductances: west and south face of every cell (block-major, like the
   // cells), east face of each block's last column, north face of its top row
   const T* cw;

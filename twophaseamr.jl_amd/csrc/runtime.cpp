@@ -215,6 +215,13 @@ void upload_level_values(tpamr_ctx* c, DevLevel<T>& D, const LevelData& L, const
   v.ncells = L.ncells;
   v.bx = L.mesh->bs[0];
   v.by = L.mesh->bs[1];
+  {
+    const int tpb = v.bx * v.by / 4, bpw = tpb >= 256 ? 1 : 256 / tpb;
+    const int groups = (L.nblocks + bpw - 1) / bpw;
+    v.wg_chunk = std::max(1, std::min(32, groups / 256));
+    const int step = 8 * v.wg_chunk;
+    v.wg_grid = (groups + step - 1) / step * step;
+  }
   v.cw = D.cw.p;
   v.ce = D.ce.p;
   v.cs = D.cs.p;

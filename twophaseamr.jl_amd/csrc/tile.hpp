@@ -328,7 +328,9 @@ __global__ void __launch_bounds__(256) k_sweep(LevelView<T> L, const T* __restri
                                               const double* __restrict__ r64, double* __restrict__ part) {
   if (done && *done) return;
   TILE4_SHARED(T, BX, BY);
-  const Place<BX, BY> P(xcd_block(blockIdx.x, gridDim.x), L.nblocks);
+  const int wg = xcd_chunk_group(blockIdx.x, L.wg_chunk);
+  if (wg * Geo<BX, BY>::BPW >= L.nblocks) return;  // grid padding
+  const Place<BX, BY> P(wg, L.nblocks);
   Vec4<T> bv{}, vc, ax, d;
   Vec4<double> rd{};
   tile4<T, BX, BY, false>(S_, L, P, Arr4<T>{x_in}, [&]() {
@@ -369,7 +371,9 @@ __global__ void __launch_bounds__(256) k_residual(LevelView<T> L, const T* __res
                                                  T* __restrict__ u, double omega_p, const int* __restrict__ done) {
   if (done && *done) return;
   TILE4_SHARED(T, BX, BY);
-  const Place<BX, BY> P(xcd_block(blockIdx.x, gridDim.x), L.nblocks);
+  const int wg = xcd_chunk_group(blockIdx.x, L.wg_chunk);
+  if (wg * Geo<BX, BY>::BPW >= L.nblocks) return;  // grid padding
+  const Place<BX, BY> P(wg, L.nblocks);
   Vec4<T> bv{}, vc, ax, d;
   tile4<T, BX, BY, false>(S_, L, P, Arr4<T>{x}, [&]() { bv = ld4(rhs + P.cell0); }, vc, ax, d);
   if (!P.valid) return;
@@ -388,7 +392,9 @@ __global__ void __launch_bounds__(256) k_restrict(LevelView<T> L, const T* __res
   if (done && *done) return;
   using G = Geo<BX, BY>;
   TILE4_SHARED(T, BX, BY);
-  const Place<BX, BY> P(xcd_block(blockIdx.x, gridDim.x), L.nblocks);
+  const int wg = xcd_chunk_group(blockIdx.x, L.wg_chunk);
+  if (wg * Geo<BX, BY>::BPW >= L.nblocks) return;  // grid padding
+  const Place<BX, BY> P(wg, L.nblocks);
   Vec4<T> vc, ax, d;
   tile4<T, BX, BY, false>(S_, L, P, Arr4<T>{u}, Pre0{}, vc, ax, d);
   Vec4<T> w;
@@ -435,7 +441,9 @@ __global__ void __launch_bounds__(256) k_prolong(LevelView<T> L, T* __restrict__
                                                 double omega_p, const int* __restrict__ done) {
   if (done && *done) return;
   TILE4_SHARED(T, BX, BY);
-  const Place<BX, BY> P(xcd_block(blockIdx.x, gridDim.x), L.nblocks);
+  const int wg = xcd_chunk_group(blockIdx.x, L.wg_chunk);
+  if (wg * Geo<BX, BY>::BPW >= L.nblocks) return;  // grid padding
+  const Place<BX, BY> P(wg, L.nblocks);
   Vec4<T> xv{}, vc, ax, d;
   tile4<T, BX, BY, false>(S_, L, P, Inject4<T, BX, BY>{xc, L.xfer}, [&]() { xv = ld4(x + P.cell0); }, vc, ax, d);
   if (!P.valid) return;
@@ -452,7 +460,9 @@ __global__ void __launch_bounds__(256) k_matvec_dot(LevelView<double> L, const d
                                                    const int* __restrict__ done) {
   if (done && *done) return;
   TILE4_SHARED(double, BX, BY);
-  const Place<BX, BY> P(xcd_block(blockIdx.x, gridDim.x), L.nblocks);
+  const int wg = xcd_chunk_group(blockIdx.x, L.wg_chunk);
+  if (wg * Geo<BX, BY>::BPW >= L.nblocks) return;  // grid padding
+  const Place<BX, BY> P(wg, L.nblocks);
   Vec4<double> vc, ax, d;
   tile4<double, BX, BY, false>(S_, L, P, Arr4<double>{p}, Pre0{}, vc, ax, d);
   double s = 0;
@@ -474,7 +484,9 @@ template <int BX, int BY>
 __global__ void __launch_bounds__(256) k_floor_bound(LevelView<double> L, const double* __restrict__ x,
                                                     const double* __restrict__ rhs, double* __restrict__ part) {
   TILE4_SHARED(double, BX, BY);
-  const Place<BX, BY> P(xcd_block(blockIdx.x, gridDim.x), L.nblocks);
+  const int wg = xcd_chunk_group(blockIdx.x, L.wg_chunk);
+  if (wg * Geo<BX, BY>::BPW >= L.nblocks) return;  // grid padding
+  const Place<BX, BY> P(wg, L.nblocks);
   Vec4<double> bv{}, vc, ax, d;
   tile4<double, BX, BY, true>(S_, L, P, Arr4<double>{x}, [&]() { bv = ld4(rhs + P.cell0); }, vc, ax, d);
   double qm = 0;
