@@ -536,16 +536,16 @@ void build_ell(LevelData& L) {
   const TreeMesh& M = *L.mesh;
   const int bx = M.bs[0], by = M.bs[1], nb = bx * by, W = bx + 2;
   const int tile = (bx + 2) * (by + 2);
-  const int nwords = (nb + 31) / 32;
   const int nrows = int(L.irr_local.size());
   int w = 0;
   for (int r = 0; r < nrows; ++r) w = std::max(w, L.irr_ptr[r + 1] - L.irr_ptr[r]);
   L.ell_w = w;
-  L.irr_mask.assign(size_t(L.nblocks) * nwords, 0u);
   L.ell_ref.assign(size_t(nrows) * w, 0);
   L.ell_val.assign(size_t(nrows) * w, 0.0);
   L.ell_bound.assign(size_t(nrows) * w, 0.0);
   L.ell_loc.assign(nrows, 0u);
+  L.ell_diag.assign(nrows, 0.0);
+  L.ext_max = 0;
   L.blk_ext.assign(L.nblocks + 1, 0);
   L.ext_col.clear();
   std::unordered_map<int, int> ext;
@@ -565,8 +565,8 @@ void build_ell(LevelData& L) {
       const int r = order[x];
       const int loc = L.irr_local[r];
       const int len = L.irr_ptr[r + 1] - L.irr_ptr[r];
-      L.irr_mask[size_t(b) * nwords + loc / 32] |= 1u << (loc % 32);
       L.ell_loc[r0 + x] = uint32_t(loc) | (uint32_t(len) << 16);
+      L.ell_diag[r0 + x] = L.irr_diag[r];
       const int own = ((loc / bx) + 1) * W + (loc % bx) + 1;
       for (int k = 0; k < w; ++k) L.ell_ref[size_t(r0) * w + size_t(k) * nr + x] = uint16_t(own);
       int k = 0;
@@ -594,6 +594,7 @@ void build_ell(LevelData& L) {
       }
     }
     if (int(ext.size()) > nb) throw std::runtime_error("too many external couplings in one block");
+    L.ext_max = std::max(L.ext_max, int(ext.size()));
     L.blk_ext[b + 1] = int(L.ext_col.size());
   }
 }

@@ -96,8 +96,9 @@ struct LevelData {
   // local cell | length << 16. Columns are LDS slots of the block's tile:
   // (BX+2)(BY+2) tile and halo positions, then the block's external cells.
   int ell_w = 0;
-  std::vector<uint32_t> irr_mask;
+  int ext_max = 0;  // most external cells of one block
   std::vector<uint32_t> ell_loc;
+  std::vector<double> ell_diag;  // diagonal of each row, same order as ell_loc
   std::vector<uint16_t> ell_ref;
   std::vector<double> ell_val, ell_bound;
   std::vector<int> blk_ext, ext_col;

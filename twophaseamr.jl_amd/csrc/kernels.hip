@@ -563,17 +563,23 @@ template <int BX, int BY>
 static inline dim3 tile_wg() {
   return dim3(Geo<BX, BY>::WG);
 }
+// dynamic LDS of a tile launch: one slot per block of the workgroup
+template <typename T, int BX, int BY>
+static inline unsigned tile_lds(const LevelView<T>& L) {
+  return unsigned(Geo<BX, BY>::BPW * Slot4<T, BX, BY>::bytes(L.ext_cap));
+}
 
 template <typename T, int BX, int BY>
 static void launch_sweep_t(hipStream_t s, const LevelView<T>& L, const T* xi, T* xo, const T* b, double w,
                            const int* done, bool fine, const double* r64, double* part) {
   const dim3 g = tile_grid<BX, BY>(L.wg_grid), t = tile_wg<BX, BY>();
+  const unsigned sh = tile_lds<T, BX, BY>(L);
   if (part) {
-    hipLaunchKernelGGL((k_sweep<T, BX, BY, true, true>), g, t, 0, s, L, xi, xo, b, w, done, r64, part);
+    hipLaunchKernelGGL((k_sweep<T, BX, BY, true, true>), g, t, sh, s, L, xi, xo, b, w, done, r64, part);
   } else if (fine) {
-    hipLaunchKernelGGL((k_sweep<T, BX, BY, true, false>), g, t, 0, s, L, xi, xo, b, w, done, nullptr, nullptr);
+    hipLaunchKernelGGL((k_sweep<T, BX, BY, true, false>), g, t, sh, s, L, xi, xo, b, w, done, nullptr, nullptr);
   } else {
-    hipLaunchKernelGGL((k_sweep<T, BX, BY, false, false>), g, t, 0, s, L, xi, xo, b, w, done, nullptr, nullptr);
+    hipLaunchKernelGGL((k_sweep<T, BX, BY, false, false>), g, t, sh, s, L, xi, xo, b, w, done, nullptr, nullptr);
   }
 }
 
@@ -588,10 +594,11 @@ void launch_residual(hipStream_t s, const LevelView<T>& L, const T* x, const T* 
                      bool fine) {
   TPAMR_DISPATCH(L.bx, ({
                    const dim3 g = tile_grid<BX, BY>(L.wg_grid), t = tile_wg<BX, BY>();
+                   const unsigned sh = tile_lds<T, BX, BY>(L);
                    if (fine)
-                     hipLaunchKernelGGL((k_residual<T, BX, BY, true>), g, t, 0, s, L, x, b, u, wp, done);
+                     hipLaunchKernelGGL((k_residual<T, BX, BY, true>), g, t, sh, s, L, x, b, u, wp, done);
                    else
-                     hipLaunchKernelGGL((k_residual<T, BX, BY, false>), g, t, 0, s, L, x, b, u, wp, done);
+                     hipLaunchKernelGGL((k_residual<T, BX, BY, false>), g, t, sh, s, L, x, b, u, wp, done);
                  }));
 }
 
@@ -600,11 +607,12 @@ void launch_restrict(hipStream_t s, const LevelView<T>& L, const T* u, double wp
                      double wc, int zero_start, const int* done, bool fine) {
   TPAMR_DISPATCH(L.bx, ({
                    const dim3 g = tile_grid<BX, BY>(L.wg_grid), t = tile_wg<BX, BY>();
+                   const unsigned sh = tile_lds<T, BX, BY>(L);
                    if (fine)
-                     hipLaunchKernelGGL((k_restrict<T, BX, BY, true>), g, t, 0, s, L, u, wp, bc, xc, dinv_c, wc,
+                     hipLaunchKernelGGL((k_restrict<T, BX, BY, true>), g, t, sh, s, L, u, wp, bc, xc, dinv_c, wc,
                                         zero_start, done);
                    else
-                     hipLaunchKernelGGL((k_restrict<T, BX, BY, false>), g, t, 0, s, L, u, wp, bc, xc, dinv_c, wc,
+                     hipLaunchKernelGGL((k_restrict<T, BX, BY, false>), g, t, sh, s, L, u, wp, bc, xc, dinv_c, wc,
                                         zero_start, done);
                  }));
 }
@@ -613,10 +621,11 @@ template <typename T>
 void launch_prolong(hipStream_t s, const LevelView<T>& L, T* x, const T* xc, double wp, const int* done, bool fine) {
   TPAMR_DISPATCH(L.bx, ({
                    const dim3 g = tile_grid<BX, BY>(L.wg_grid), t = tile_wg<BX, BY>();
+                   const unsigned sh = tile_lds<T, BX, BY>(L);
                    if (fine)
-                     hipLaunchKernelGGL((k_prolong<T, BX, BY, true>), g, t, 0, s, L, x, xc, wp, done);
+                     hipLaunchKernelGGL((k_prolong<T, BX, BY, true>), g, t, sh, s, L, x, xc, wp, done);
                    else
-                     hipLaunchKernelGGL((k_prolong<T, BX, BY, false>), g, t, 0, s, L, x, xc, wp, done);
+                     hipLaunchKernelGGL((k_prolong<T, BX, BY, false>), g, t, sh, s, L, x, xc, wp, done);
                  }));
 }
 
@@ -629,14 +638,16 @@ void launch_matvec_dot(hipStream_t s, const LevelView<double>& L, const double* 
                        const int* done) {
   TPAMR_DISPATCH(L.bx, ({
                    const dim3 g = tile_grid<BX, BY>(L.wg_grid), t = tile_wg<BX, BY>();
-                   hipLaunchKernelGGL((k_matvec_dot<BX, BY>), g, t, 0, s, L, p, y, part, done);
+                   const unsigned sh = tile_lds<double, BX, BY>(L);
+                   hipLaunchKernelGGL((k_matvec_dot<BX, BY>), g, t, sh, s, L, p, y, part, done);
                  }));
 }
 
 void launch_floor_bound(hipStream_t s, const LevelView<double>& L, const double* x, const double* b, double* part) {
   TPAMR_DISPATCH(L.bx, ({
                    const dim3 g = tile_grid<BX, BY>(L.wg_grid), t = tile_wg<BX, BY>();
-                   hipLaunchKernelGGL((k_floor_bound<BX, BY>), g, t, 0, s, L, x, b, part);
+                   const unsigned sh = tile_lds<double, BX, BY>(L);
+                   hipLaunchKernelGGL((k_floor_bound<BX, BY>), g, t, sh, s, L, x, b, part);
                  }));
 }
 

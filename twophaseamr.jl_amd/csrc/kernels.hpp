@@ -14,6 +14,7 @@ struct LevelView {
   // tile-kernel grid: wg_grid workgroups (a multiple of 8 * wg_chunk); runs of
   // wg_chunk consecutive block groups are dealt to the 8 XCDs in turn
   int wg_chunk, wg_grid;
+  int ext_cap;  // external-cell slots per block tile (most of any block of the level)
   // conductances: west and south face of every cell (block-major, like the
   // cells), east face of each block's last column, north face of its top row
   const T* cw;
@@ -22,9 +23,8 @@ struct LevelView {
   const T* cn;
   const T* ell_val;
   const T* ell_bound;  // floor-bound values (FP64 finest level only)
-  const T* irr_diag;
+  const T* ell_diag;  // diagonal per irregular row, in ell_loc order
   const unsigned short* ell_ref;  // entry k of row x of block b at blk_irr[b] * ell_w + k * nrows_b + x
-  const unsigned* irr_mask;  // per block and 32-cell word: (mask, first row index)
   const int* blk_irr;        // nblocks + 1 offsets of the irregular rows per block
   const unsigned* ell_loc;   // per row in a block's length order: local cell | length << 16
   const int* blk_ext;

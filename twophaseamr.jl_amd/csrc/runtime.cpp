@@ -86,14 +86,14 @@ struct DevArray {
 struct DevTopo {
   int nblocks = 0, ncells = 0;
   DevArray<int> nbr, xfer, blk_ext, ext_col, blk_irr;
-  DevArray<unsigned> irr_mask, ell_loc;
+  DevArray<unsigned> ell_loc;
   DevArray<unsigned short> ell_ref;
   DevArray<double> volume;
 };
 
 template <typename T>
 struct DevLevel {
-  DevArray<T> cw, ce, cs, cn, ell_val, ell_bound, irr_diag, dinv;
+  DevArray<T> cw, ce, cs, cn, ell_val, ell_bound, ell_diag, dinv;
   DevArray<T> x, xs, b, u;
   double omega = 1, omega_p = 0;
   LevelView<T> view{};
@@ -202,7 +202,7 @@ void upload_level_values(tpamr_ctx* c, DevLevel<T>& D, const LevelData& L, const
   D.cn.upload(L.cn, s);
   D.ell_val.upload(L.ell_val, s);
   if (with_bound) D.ell_bound.upload(L.ell_bound, s);
-  D.irr_diag.upload(L.irr_diag, s);
+  D.ell_diag.upload(L.ell_diag, s);
   D.dinv.upload(L.dinv, s);
   D.omega = L.omega;
   D.omega_p = L.omega_p;
@@ -222,6 +222,7 @@ void upload_level_values(tpamr_ctx* c, DevLevel<T>& D, const LevelData& L, const
     const int step = 8 * v.wg_chunk;
     v.wg_grid = (groups + step - 1) / step * step;
   }
+  v.ext_cap = std::max(4, (L.ext_max + 3) & ~3);
   v.cw = D.cw.p;
   v.ce = D.ce.p;
   v.cs = D.cs.p;
@@ -229,9 +230,8 @@ void upload_level_values(tpamr_ctx* c, DevLevel<T>& D, const LevelData& L, const
   v.ell_w = L.ell_w;
   v.ell_val = D.ell_val.p;
   v.ell_bound = D.ell_bound.p;
-  v.irr_diag = D.irr_diag.p;
+  v.ell_diag = D.ell_diag.p;
   v.ell_ref = Tp.ell_ref.p;
-  v.irr_mask = Tp.irr_mask.p;
   v.blk_irr = Tp.blk_irr.p;
   v.ell_loc = Tp.ell_loc.p;
   v.blk_ext = Tp.blk_ext.p;
@@ -257,19 +257,6 @@ void upload_hierarchy(tpamr_ctx* c, bool topology) {
       T.nbr.upload(L.nbr, s);
       T.xfer.upload(L.xfer.empty() ? std::vector<int>(2 * L.nblocks, -1) : L.xfer, s);
       if (L.ell_w > ELL_MAX) throw std::runtime_error("irregular row longer than the kernels support");
-      // (mask, index of the first irregular row of the word) per 32-cell word
-      const int nw = (L.mesh->nb() + 31) / 32;
-      std::vector<unsigned> m2(2 * size_t(L.nblocks) * nw);
-      for (int b = 0; b < L.nblocks; ++b) {
-        unsigned row = unsigned(L.blk_irr[b]);
-        for (int w = 0; w < nw; ++w) {
-          const unsigned m = L.irr_mask[size_t(b) * nw + w];
-          m2[2 * (size_t(b) * nw + w)] = m;
-          m2[2 * (size_t(b) * nw + w) + 1] = row;
-          row += unsigned(__builtin_popcount(m));
-        }
-      }
-      T.irr_mask.upload(m2, s);
       T.ell_ref.upload(L.ell_ref.empty() ? std::vector<unsigned short>(1, 0) : L.ell_ref, s);
       T.blk_ext.upload(L.blk_ext, s);
       T.blk_irr.upload(L.blk_irr, s);
@@ -1243,15 +1230,14 @@ int tpamr_time_smoother(tpamr_ctx* c, int reps, double* avg_ms, double* bytes) {
     *avg_ms = double(ms) / reps;
     if (bytes) {
       // compulsory traffic: x, b read, x' write, the two conductance
-      // arrays (including the block-boundary faces), the irregular-cell
-      // masks and the irregular rows with their external values
+      // arrays (including the block-boundary faces), and the irregular rows
+      // (entries, diagonal, cell/length word) with their external values
       const double cells = double(N) * 3 * elem;
       const double cond = double(nb) * ((bx + 1) * by + bx * (by + 1)) * elem;
-      const double mask = double(nb) * ((bx * by + 31) / 32) * 8;
       const double nnz = L.irr_ptr.empty() ? 0.0 : double(L.irr_ptr.back());
       const double irr = nnz * (elem + 2) + double(L.irr_local.size()) * (elem + 4) +
                          double(L.ext_col.size()) * (4 + elem);
-      *bytes = cells + cond + mask + irr;
+      *bytes = cells + cond + irr;
     }
   });
 }

@@ -58,12 +58,31 @@ struct Geo {
   static constexpr int SEG = TPB < 64 ? TPB : 64;    // lanes of one block within a wavefront
 };
 
+// LDS slot of one block: tile values (block, halo, then `ext` external
+// cells), the south conductances of the block (north exchange), and the
+// irregular rows' products, diagonals and flags. Slots are carved from the
+// dynamic LDS of the launch (slot_bytes each, BPW slots per workgroup).
 template <typename T, int BX, int BY>
-struct Tile4 {
-  using G = Geo<BX, BY>;
-  T v[G::TILE + G::EXTMAX];
-  T cs[G::NB];
-  T ax[G::NB];
+struct Slot4 {
+  T* v;
+  T* cs;
+  T* ax;
+  T* d;
+  unsigned* flag4;  // one byte per cell, four cells per word
+  static __host__ __device__ inline int bytes(int ext) {
+    constexpr int NB = BX * BY, TILE = (BX + 2) * (BY + 2);
+    const int vals = TILE + ((ext + 3) & ~3) + 3 * NB;
+    return ((vals * int(sizeof(T)) + NB + 15) / 16) * 16;
+  }
+  __device__ inline Slot4(unsigned char* lds, int slot, int ext) {
+    constexpr int NB = BX * BY, TILE = (BX + 2) * (BY + 2);
+    unsigned char* base = lds + size_t(slot) * bytes(ext);
+    v = reinterpret_cast<T*>(base);
+    cs = v + TILE + ((ext + 3) & ~3);
+    ax = cs + NB;
+    d = ax + NB;
+    flag4 = reinterpret_cast<unsigned*>(d + NB);
+  }
 };
 
 // plain array fetches
@@ -125,21 +144,21 @@ struct Place {
 
 // Evaluate (A v) and D of the thread's four cells (or, with ABS, the
 // factorized round-off bound sum_j B_cj |v_j|). `vc` returns the cells' own
-// values (absolute values with ABS). Irregular cells are marked in a bitmask
-// word per 32 cells (with the index of the word's first irregular row); a
-// segment of four cells never straddles two words.
+// values (absolute values with ABS). The lane that owns an irregular row
+// writes its product, diagonal and flag to LDS; the owners of the cells read
+// them after the second barrier.
 template <typename T, int BX, int BY, bool ABS, class F, class Pre>
-__device__ inline void tile4(Tile4<T, BX, BY>* St, const LevelView<T>& L, const Place<BX, BY>& P, F fetch, Pre pre,
+__device__ inline void tile4(unsigned char* lds, const LevelView<T>& L, const Place<BX, BY>& P, F fetch, Pre pre,
                              Vec4<T>& vc, Vec4<T>& ax, Vec4<T>& d) {
   using G = Geo<BX, BY>;
   constexpr int W = G::W, TILE = G::TILE, NB = G::NB, TPB = G::TPB, TPR = G::TPR;
-  constexpr int NW = (NB + 31) / 32;
-  Tile4<T, BX, BY>& S = St[P.slot()];
+  const Slot4<T, BX, BY> S(lds, P.slot(), L.ext_cap);
   const int b = P.b, lt = P.lt, j = P.j, q = P.q, i0 = P.i0;
   // rows of the workgroup's blocks: blk_irr is a prefix sum over blocks
   const int wb0 = P.wg * G::BPW, wb1 = min(wb0 + G::BPW, L.nblocks);
   const bool any = L.blk_irr[wb1] > L.blk_irr[wb0];
-  Vec4<T> own{}, cw4{}, cs4{}, cn4{}, rd{};
+  Vec4<T> own{}, cw4{}, cs4{}, cn4{};
+  T rdg = 0;
   T ceb = 0;
   int nr = 0;
   T rv[ELL_MAX];
@@ -198,17 +217,8 @@ __device__ inline void tile4(Tile4<T, BX, BY>* St, const LevelView<T>& L, const 
             rr[k] = L.ell_ref[base + size_t(k) * nr];
           }
       }
-      // irregular cells of this segment and their diagonals
-      const int c0 = j * BX + i0, wd = c0 >> 5, bit = c0 & 31;
-      const unsigned mword = L.irr_mask[2 * (b * NW + wd)];
-      bits = (mword >> bit) & 15u;
-      if (bits) {
-        const int first = int(L.irr_mask[2 * (b * NW + wd) + 1]) + __popc(mword & ((1u << bit) - 1u));
-        int k = 0;
-#pragma unroll
-        for (int m = 0; m < 4; ++m)
-          if ((bits >> m) & 1u) rd.v[m] = L.irr_diag[first + k++];
-      }
+      if (lt < nr) rdg = L.ell_diag[r0 + lt];
+      S.flag4[lt] = 0u;  // this lane's four cells
     }
     pre();
 #pragma unroll
@@ -232,6 +242,8 @@ __device__ inline void tile4(Tile4<T, BX, BY>* St, const LevelView<T>& L, const 
         for (int k = 0; k < ELL_MAX; ++k)
           if (k < rlen) s += rv[k] * (S.v[rr[k]] - vi);
         S.ax[rloc] = s;
+        S.d[rloc] = rdg;
+        reinterpret_cast<unsigned char*>(S.flag4)[rloc] = 1;
       }
       // blocks with more irregular rows than lanes
       for (int x = lt + TPB; x < nr; x += TPB) {
@@ -244,6 +256,8 @@ __device__ inline void tile4(Tile4<T, BX, BY>* St, const LevelView<T>& L, const 
         T s = 0;
         for (int k = 0; k < len; ++k) s += vals[base + size_t(k) * nr] * (S.v[L.ell_ref[base + size_t(k) * nr]] - vi);
         S.ax[loc] = s;
+        S.d[loc] = L.ell_diag[r0 + x];
+        reinterpret_cast<unsigned char*>(S.flag4)[loc] = 1;
       }
     }
     __syncthreads();
@@ -251,11 +265,18 @@ __device__ inline void tile4(Tile4<T, BX, BY>* St, const LevelView<T>& L, const 
   const T cw_next = __shfl_down(cw4.v[0], 1, G::SEG);
   vc = own;
   if (!P.valid) return;
-  Vec4<T> axi{};
-  if (bits) {
+  Vec4<T> axi{}, dgi{};
+  if (any) {
+    const unsigned f = S.flag4[lt];
+    bits = (f & 1u) | ((f >> 7) & 2u) | ((f >> 14) & 4u) | ((f >> 21) & 8u);
+    if (bits) {
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
-      if ((bits >> m) & 1u) axi.v[m] = S.ax[j * BX + i0 + m];
+      for (int m = 0; m < 4; ++m)
+        if ((bits >> m) & 1u) {
+          axi.v[m] = S.ax[j * BX + i0 + m];
+          dgi.v[m] = S.d[j * BX + i0 + m];
+        }
+    }
   }
   // stencil of every cell, then a select for the irregular ones (keeps the
   // results in registers)
@@ -275,7 +296,7 @@ __device__ inline void tile4(Tile4<T, BX, BY>* St, const LevelView<T>& L, const 
                      : (cw * (vi - vw) + ce * (vi - ve)) + (cs * (vi - vs) + cn * (vi - vn));
     const bool irr = (bits >> m) & 1u;
     ax.v[m] = irr ? axi.v[m] : st;
-    d.v[m] = irr ? rd.v[m] : dd;
+    d.v[m] = irr ? dgi.v[m] : dd;
   }
 }
 
@@ -318,7 +339,7 @@ __device__ inline T block4_max(T v, T* scratch) {
   }
 }
 
-#define TILE4_SHARED(T, BX, BY) __shared__ Tile4<T, BX, BY> S_[Geo<BX, BY>::BPW]
+#define TILE4_SHARED(T, BX, BY) extern __shared__ __align__(16) unsigned char S_[]
 
 // Damped Jacobi sweep x_out = x_in + w D^-1 (b - A x_in); STATS: per-block
 // partial sums of V z, r z and r for the Krylov update.
