@@ -33,14 +33,26 @@ enum {
   TPAMR_EINCOMPATIBLE = -6 /* compatibility = error and an incompatible RHS */
 };
 
-/* precision of the multigrid V-cycle; the outer Krylov loop is always FP64 */
-enum { TPAMR_FP64 = 0, TPAMR_FP32_VCYCLE = 1 };
+/* Precision of a context's solves.
+ * TPAMR_FP64: FP64 Krylov CG preconditioned by an FP64 V-cycle.
+ * TPAMR_FP32_VCYCLE: FP64 Krylov CG preconditioned by an FP32 V-cycle.
+ * TPAMR_FP32_REFINE: FP64 iterative refinement around a Krylov CG whose
+ *   vectors are FP32 (inner products and scalars FP64), preconditioned by the
+ *   FP32 V-cycle: each time the strong residual has fallen to 5e-3 of its
+ *   largest value since the last refinement step (the environment variable
+ *   TPAMR_IR_DELTA sets another factor), the FP32 correction moves into the
+ *   FP64 solution, the FP32 residual is replaced by the FP64 residual
+ *   b - A x and the CG recurrence restarts from it, all on the device.
+ *   The Krylov seam (tpamr_krylov) runs the TPAMR_FP32_VCYCLE iteration. */
+enum { TPAMR_FP64 = 0, TPAMR_FP32_VCYCLE = 1, TPAMR_FP32_REFINE = 2 };
 
 /* LinearSolverStats, src/solvers/multigrid/stats.jl:14-36 */
 typedef struct {
   int cycles, converged, compatible, reason;
   double initial_l2, initial_linf, final_l2, final_linf, target_linf, residual_scale, removed_rhs_mean;
   double solve_ms; /* device time of the solve, HIP events on the context stream */
+  int krylov_runs;  /* Krylov runs of the restart-and-verify loop */
+  int replacements; /* FP64 residual replacements of the inner FP32 iteration (TPAMR_FP32_REFINE) */
 } tpamr_stats;
 
 /* MultigridOptions, src/solvers/multigrid/options.jl:1-52 */

@@ -219,13 +219,19 @@ __device__ inline double stage_reduce(const double* __restrict__ stage, double* 
 }
 
 // Krylov scalar step 1: pAp and the step length (Krylov.jl cg semantics).
+// With `sums`, part also holds per-block sums of Ap (part[nb + b]) and the
+// volume mean the residual update subtracts becomes the exact mean of
+// r - alpha Ap (rmean holds the mean of r on entry).
 __global__ void __launch_bounds__(256) k_cg_alpha(int nb, const double* __restrict__ part, double* __restrict__ stage,
-                                                  CgState* st) {
+                                                  CgState* st, int sums, double total_volume) {
   __shared__ double red[4];
   if (st->done) return;
-  const double v = chunk_sum(part, nb, red);
-  if (!publish_last<1>(&v, stage, &st->ticket)) return;
+  double v[2];
+  v[0] = chunk_sum(part, nb, red);
+  v[1] = sums ? chunk_sum(part + nb, nb, red) : 0.0;
+  if (!publish_last<2>(v, stage, &st->ticket)) return;
   const double pAp = stage_reduce<false>(stage, red);
+  const double sap = sums ? stage_reduce<false>(stage + RED_WG, red) : 0.0;
   if (threadIdx.x == 0) {
     st->pAp = pAp;
     const double eps = 2.220446049250313e-16;
@@ -234,6 +240,7 @@ __global__ void __launch_bounds__(256) k_cg_alpha(int nb, const double* __restri
       st->reason = 3;  // zero curvature
     } else {
       st->alpha = st->gamma / pAp;
+      if (sums) st->rmean -= st->alpha * sap / total_volume;
     }
   }
 }
@@ -301,6 +308,7 @@ __global__ void __launch_bounds__(256) k_cg_beta(int nb, int nmax, const double*
     st->rnorm = sqrt(g);
     st->beta = 0;
     st->iter = 0;
+    st->rlast = mx;
     if (g < 0) {
       st->done = 1;
       st->reason = 5;  // indefinite preconditioner
@@ -312,19 +320,56 @@ __global__ void __launch_bounds__(256) k_cg_beta(int nb, int nmax, const double*
   }
   const double rn = sqrt(g);
   st->rnorm = rn;
-  const bool solved = rn + 1.0 <= 1.0;
+  // Krylov.jl's test on the unscaled residual norm (the refinement mode's
+  // inner vectors carry the factor `scale`)
+  const bool solved = rn / st->scale + 1.0 <= 1.0;
+  // A replaced residual (FP64 b - A x, while the inner iteration's operator
+  // is FP32) restarts the recurrence: p = z, as at a fresh start.
+  const bool restart = st->replace != 0;
   if (!solved) {
-    const double beta = g / st->gamma;
+    const double beta = restart ? 0.0 : g / st->gamma;
     st->pnorm2 = g + beta * beta * st->pnorm2;
     st->gamma = g;
     st->beta = beta;
   }
   st->iter += 1;
+  if (restart) {
+    st->rlast = mx;
+    st->replace = 0;
+    st->replaced += 1;
+  } else {
+    st->rlast = fmax(st->rlast, mx);
+  }
   const bool tired = st->iter >= st->itmax;
   const bool user = mx <= st->target;
   if (solved || tired || user) {
     st->done = 1;
     st->reason = solved ? 1 : (user ? 2 : 4);
+  } else if (st->delta > 0 && mx <= st->delta * st->rlast) {
+    // Refinement mode: the rounding of the FP32 correction and residual
+    // bounds the true residual at about 1e-3 of the largest strong residual
+    // since the last replacement, so the residual is replaced (and the
+    // recurrence restarted) once it has fallen by delta below that value.
+    st->replace = 1;
+  }
+}
+
+// After a residual replacement: the strong maximum of the FP64 residual
+// decides the stop (the Krylov callback's test) before the V-cycle runs;
+// the iteration's step is counted here when it stops.
+__global__ void __launch_bounds__(256) k_ir_check(int ng, const double* __restrict__ maxpart,
+                                                  double* __restrict__ stage, CgState* st) {
+  __shared__ double red[4];
+  if (st->done || !st->replace) return;
+  const double v = chunk_max(maxpart, ng, red);
+  if (!publish_last<1>(&v, stage, &st->ticket)) return;
+  const double mx = stage_reduce<true>(stage, red);
+  if (threadIdx.x == 0 && mx <= st->target) {
+    st->strong = mx;
+    st->iter += 1;
+    st->replaced += 1;
+    st->done = 1;
+    st->reason = 2;
   }
 }
 
@@ -338,17 +383,116 @@ __global__ void k_cg_update_p(int n, const TV* __restrict__ z, double* __restric
   p[c] = (double(z[c]) - st->zmean) + st->beta * p[c];
 }
 
+// ---------------------------------------------------------------------------
+// Inner FP32 Krylov iteration of the mixed-precision refinement loop: the
+// vectors e (correction), r (residual, also the finest level's V-cycle rhs),
+// p and Ap are FP32; scalars, inner products and the update arithmetic are
+// FP64. A thread owns four consecutive cells (n is a multiple of 4, cells
+// of one block share a volume).
+
+__device__ inline float4 ld_f4(const float* __restrict__ p) { return *reinterpret_cast<const float4*>(p); }
+__device__ inline void st_f4(float* __restrict__ p, float a, float b, float c, float d) {
+  *reinterpret_cast<float4*>(p) = make_float4(a, b, c, d);
+}
+
+// init: r = scale * r64, e = 0. Otherwise e += alpha p and
+// r = (r - alpha Ap) - V rmean, rmean being the exact volume mean of
+// r - alpha Ap from the Ap sums of the matvec (k_cg_alpha).
+// Both write the zero-start V-cycle iterate x0 = w D^-1 r and the per-
+// workgroup maxima of |r| / V.
+__global__ void __launch_bounds__(256) k_ir_update_xr(int n, int nb_cells, const float* __restrict__ p,
+                                                      const float* __restrict__ Ap, float* __restrict__ e,
+                                                      float* __restrict__ r, const double* __restrict__ volume,
+                                                      float* __restrict__ vx, const float* __restrict__ dinv,
+                                                      float omega, const double* __restrict__ r64,
+                                                      double* __restrict__ part, CgState* st, int init) {
+  __shared__ double red[4];
+  if (st->done) return;
+  const int c = 4 * (blockIdx.x * 256 + threadIdx.x);
+  double q = 0;
+  if (c < n) {
+    const double V = volume[c / nb_cells];
+    float rv[4];
+    if (init) {
+      const double scale = st->scale;
+      const double2 a = *reinterpret_cast<const double2*>(r64 + c);
+      const double2 b = *reinterpret_cast<const double2*>(r64 + c + 2);
+      rv[0] = float(scale * a.x);
+      rv[1] = float(scale * a.y);
+      rv[2] = float(scale * b.x);
+      rv[3] = float(scale * b.y);
+      st_f4(e + c, 0.f, 0.f, 0.f, 0.f);
+    } else {
+      const double al = st->alpha, sh = V * st->rmean;
+      const float4 pv = ld_f4(p + c), av = ld_f4(Ap + c), ev = ld_f4(e + c), ro = ld_f4(r + c);
+      st_f4(e + c, float(double(ev.x) + al * double(pv.x)), float(double(ev.y) + al * double(pv.y)),
+            float(double(ev.z) + al * double(pv.z)), float(double(ev.w) + al * double(pv.w)));
+      rv[0] = float((double(ro.x) - al * double(av.x)) - sh);
+      rv[1] = float((double(ro.y) - al * double(av.y)) - sh);
+      rv[2] = float((double(ro.z) - al * double(av.z)) - sh);
+      rv[3] = float((double(ro.w) - al * double(av.w)) - sh);
+    }
+    st_f4(r + c, rv[0], rv[1], rv[2], rv[3]);
+    const float4 dv = ld_f4(dinv + c);
+    st_f4(vx + c, omega * dv.x * rv[0], omega * dv.y * rv[1], omega * dv.z * rv[2], omega * dv.w * rv[3]);
+    const float m = fmaxf(fmaxf(fabsf(rv[0]), fabsf(rv[1])), fmaxf(fabsf(rv[2]), fabsf(rv[3])));
+    q = double(m) / V;
+    if (q != q || rv[0] != rv[0] || rv[1] != rv[1] || rv[2] != rv[2] || rv[3] != rv[3])
+      q = __longlong_as_double(0x7ff8000000000000LL);
+  }
+  const double m = block_max<double, 256>(q, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+// p = (z - zmean) + beta p
+__global__ void __launch_bounds__(256) k_ir_update_p(int n, const float* __restrict__ z, float* __restrict__ p,
+                                                     const CgState* st, const int* __restrict__ done) {
+  if (*done) return;
+  const int c = 4 * (blockIdx.x * 256 + threadIdx.x);
+  if (c >= n) return;
+  const double zm = st->zmean, be = st->beta;
+  const float4 zv = ld_f4(z + c), pv = ld_f4(p + c);
+  st_f4(p + c, float((double(zv.x) - zm) + be * double(pv.x)), float((double(zv.y) - zm) + be * double(pv.y)),
+        float((double(zv.z) - zm) + be * double(pv.z)), float((double(zv.w) - zm) + be * double(pv.w)));
+}
+
+// x += e / scale, e = 0 (the correction accumulated in FP32 since the last
+// replacement moves into the FP64 solution): when the state requests a
+// residual replacement, or always with `force`.
+__global__ void __launch_bounds__(256) k_ir_correct(int n, float* __restrict__ e, double* __restrict__ x,
+                                                    const CgState* st, int force) {
+  if (!force && (st->done || !st->replace)) return;
+  const int c = 4 * (blockIdx.x * 256 + threadIdx.x);
+  if (c >= n) return;
+  const double scale = 1.0 / st->scale;
+  const float4 ev = ld_f4(e + c);
+  st_f4(e + c, 0.f, 0.f, 0.f, 0.f);
+  double2 a = *reinterpret_cast<const double2*>(x + c);
+  double2 b = *reinterpret_cast<const double2*>(x + c + 2);
+  a.x += scale * double(ev.x);
+  a.y += scale * double(ev.y);
+  b.x += scale * double(ev.z);
+  b.y += scale * double(ev.w);
+  *reinterpret_cast<double2*>(x + c) = a;
+  *reinterpret_cast<double2*>(x + c + 2) = b;
+}
+
+__global__ void k_fill_f32(int n, float v, float* __restrict__ out) {
+  const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c < size_t(n)) out[c] = v;
+}
+
 // Partial sums of V z, r z and r per mesh block (used when the V-cycle does
 // not end with a finest-level sweep).
-template <typename T, int BX, int BY>
-__global__ void __launch_bounds__(BX* BY) k_zstats(const T* __restrict__ z, const double* __restrict__ r64,
+template <typename T, int BX, int BY, typename TR>
+__global__ void __launch_bounds__(BX* BY) k_zstats(const T* __restrict__ z, const TR* __restrict__ rz,
                                                   const double* __restrict__ volume, double* __restrict__ part,
                                                   const int* __restrict__ done) {
   if (done && *done) return;
   __shared__ double red[16];
   const int b = blockIdx.x;
   const size_t c = size_t(b) * (BX * BY) + threadIdx.x;
-  const double zd = double(z[c]), rd = r64[c];
+  const double zd = double(z[c]), rd = double(rz[c]);
   const double s_z = block_sum<double, BX * BY>(zd, red);
   const double s_rz = block_sum<double, BX * BY>(rd * zd, red);
   const double s_r = block_sum<double, BX * BY>(rd, red);
@@ -571,22 +715,26 @@ static inline unsigned tile_lds(const LevelView<T>& L) {
 
 template <typename T, int BX, int BY>
 static void launch_sweep_t(hipStream_t s, const LevelView<T>& L, const T* xi, T* xo, const T* b, double w,
-                           const int* done, bool fine, const double* r64, double* part) {
+                           const int* done, bool fine, const ZStats& zs) {
   const dim3 g = tile_grid<BX, BY>(L.wg_grid), t = tile_wg<BX, BY>();
   const unsigned sh = tile_lds<T, BX, BY>(L);
-  if (part) {
-    hipLaunchKernelGGL((k_sweep<T, BX, BY, true, true>), g, t, sh, s, L, xi, xo, b, w, done, r64, part);
+  const double* none = nullptr;
+  if (zs.part && zs.r32) {
+    hipLaunchKernelGGL((k_sweep<T, BX, BY, true, true, float>), g, t, sh, s, L, xi, xo, b, w, done, zs.r32, zs.part);
+  } else if (zs.part) {
+    hipLaunchKernelGGL((k_sweep<T, BX, BY, true, true, double>), g, t, sh, s, L, xi, xo, b, w, done, zs.r64,
+                       zs.part);
   } else if (fine) {
-    hipLaunchKernelGGL((k_sweep<T, BX, BY, true, false>), g, t, sh, s, L, xi, xo, b, w, done, nullptr, nullptr);
+    hipLaunchKernelGGL((k_sweep<T, BX, BY, true, false>), g, t, sh, s, L, xi, xo, b, w, done, none, nullptr);
   } else {
-    hipLaunchKernelGGL((k_sweep<T, BX, BY, false, false>), g, t, sh, s, L, xi, xo, b, w, done, nullptr, nullptr);
+    hipLaunchKernelGGL((k_sweep<T, BX, BY, false, false>), g, t, sh, s, L, xi, xo, b, w, done, none, nullptr);
   }
 }
 
 template <typename T>
 void launch_sweep(hipStream_t s, const LevelView<T>& L, const T* xi, T* xo, const T* b, double w, const int* done,
-                  bool fine, const double* r64, double* part) {
-  TPAMR_DISPATCH(L.bx, (launch_sweep_t<T, BX, BY>(s, L, xi, xo, b, w, done, fine, r64, part)));
+                  bool fine, const ZStats& zs) {
+  TPAMR_DISPATCH(L.bx, (launch_sweep_t<T, BX, BY>(s, L, xi, xo, b, w, done, fine, zs)));
 }
 
 template <typename T>
@@ -634,12 +782,16 @@ void launch_coarse(hipStream_t s, int n, const double* inv, const T* b, T* x, co
   hipLaunchKernelGGL((k_coarse_solve<T>), dim3(cdiv(n, 64)), dim3(1024), 0, s, n, inv, b, x, done);
 }
 
-void launch_matvec_dot(hipStream_t s, const LevelView<double>& L, const double* p, double* y, double* part,
-                       const int* done) {
+template <typename T>
+void launch_matvec_dot(hipStream_t s, const LevelView<T>& L, const T* p, T* y, double* part, const int* done,
+                       bool sums) {
   TPAMR_DISPATCH(L.bx, ({
                    const dim3 g = tile_grid<BX, BY>(L.wg_grid), t = tile_wg<BX, BY>();
-                   const unsigned sh = tile_lds<double, BX, BY>(L);
-                   hipLaunchKernelGGL((k_matvec_dot<BX, BY>), g, t, sh, s, L, p, y, part, done);
+                   const unsigned sh = tile_lds<T, BX, BY>(L);
+                   if (sums)
+                     hipLaunchKernelGGL((k_matvec_dot<T, BX, BY, true>), g, t, sh, s, L, p, y, part, done);
+                   else
+                     hipLaunchKernelGGL((k_matvec_dot<T, BX, BY, false>), g, t, sh, s, L, p, y, part, done);
                  }));
 }
 
@@ -655,8 +807,9 @@ void launch_reduce(hipStream_t s, int n, int nsets, const double* part, double* 
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, n, nsets, part, out, int(is_max));
 }
 
-void launch_cg_alpha(hipStream_t s, int nb, const double* part, double* stage, CgState* st) {
-  hipLaunchKernelGGL(k_cg_alpha, dim3(RED_WG), dim3(256), 0, s, nb, part, stage, st);
+void launch_cg_alpha(hipStream_t s, int nb, const double* part, double* stage, CgState* st, bool sums,
+                     double total_volume) {
+  hipLaunchKernelGGL(k_cg_alpha, dim3(RED_WG), dim3(256), 0, s, nb, part, stage, st, int(sums), total_volume);
 }
 
 template <typename TV>
@@ -737,15 +890,54 @@ void launch_adjoint_face(hipStream_t s, int nf, const double* yl, const double* 
 }
 
 template <typename T>
-void launch_zstats(hipStream_t s, int nblocks, int bx, const T* z, const double* r64, const double* vol, double* part,
+void launch_zstats(hipStream_t s, int nblocks, int bx, const T* z, const ZStats& zs, const double* vol,
                    const int* done) {
-  TPAMR_DISPATCH(bx, ({ hipLaunchKernelGGL((k_zstats<T, BX, BY>), dim3(nblocks), dim3(BX * BY), 0, s, z, r64, vol,
-                                         part, done); }));
+  TPAMR_DISPATCH(bx, ({
+                   if (zs.r32)
+                     hipLaunchKernelGGL((k_zstats<T, BX, BY, float>), dim3(nblocks), dim3(BX * BY), 0, s, z, zs.r32,
+                                        vol, zs.part, done);
+                   else
+                     hipLaunchKernelGGL((k_zstats<T, BX, BY, double>), dim3(nblocks), dim3(BX * BY), 0, s, z, zs.r64,
+                                        vol, zs.part, done);
+                 }));
 }
-template void launch_zstats<float>(hipStream_t, int, int, const float*, const double*, const double*, double*,
-                                   const int*);
-template void launch_zstats<double>(hipStream_t, int, int, const double*, const double*, const double*, double*,
-                                    const int*);
+template void launch_zstats<float>(hipStream_t, int, int, const float*, const ZStats&, const double*, const int*);
+template void launch_zstats<double>(hipStream_t, int, int, const double*, const ZStats&, const double*, const int*);
+
+int ir_groups(int n) { return cdiv(n, 1024); }
+
+void launch_ir_update_xr(hipStream_t s, int n, int nbc, const float* p, const float* Ap, float* e, float* r,
+                         const double* vol, float* vx, const float* dinv, double omega, const double* r64,
+                         double* part, CgState* st, int init) {
+  hipLaunchKernelGGL(k_ir_update_xr, dim3(ir_groups(n)), dim3(256), 0, s, n, nbc, p, Ap, e, r, vol, vx, dinv,
+                     float(omega), r64, part, st, init);
+}
+
+void launch_ir_replace(hipStream_t s, const LevelView<double>& L, const double* x, const double* b, float* r32,
+                       float* x0, const float* dinv, double omega, double* part, CgState* st) {
+  TPAMR_DISPATCH(L.bx, ({
+                   const dim3 g = tile_grid<BX, BY>(L.wg_grid), t = tile_wg<BX, BY>();
+                   const unsigned sh = tile_lds<double, BX, BY>(L);
+                   hipLaunchKernelGGL((k_ir_replace<BX, BY>), g, t, sh, s, L, x, b, r32, x0, dinv, float(omega),
+                                      part, st);
+                 }));
+}
+
+void launch_ir_check(hipStream_t s, int ng, const double* maxpart, double* stage, CgState* st) {
+  hipLaunchKernelGGL(k_ir_check, dim3(RED_WG), dim3(256), 0, s, ng, maxpart, stage, st);
+}
+
+void launch_ir_update_p(hipStream_t s, int n, const float* z, float* p, const CgState* st, const int* done) {
+  hipLaunchKernelGGL(k_ir_update_p, dim3(ir_groups(n)), dim3(256), 0, s, n, z, p, st, done);
+}
+
+void launch_ir_correct(hipStream_t s, int n, float* e, double* x, const CgState* st, bool force) {
+  hipLaunchKernelGGL(k_ir_correct, dim3(ir_groups(n)), dim3(256), 0, s, n, e, x, st, int(force));
+}
+
+void launch_fill_f32(hipStream_t s, int n, float v, float* out) {
+  hipLaunchKernelGGL(k_fill_f32, dim3(cdiv(n, 256)), dim3(256), 0, s, n, v, out);
+}
 
 void launch_axpby(hipStream_t s, int n, double a, const double* x, double b, double* y) {
   hipLaunchKernelGGL(k_axpby, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, x, b, y);
@@ -768,9 +960,13 @@ void launch_fill_int(hipStream_t s, int n, int v, int* out) {
 }
 
 template void launch_sweep<float>(hipStream_t, const LevelView<float>&, const float*, float*, const float*, double,
-                                  const int*, bool, const double*, double*);
+                                  const int*, bool, const ZStats&);
 template void launch_sweep<double>(hipStream_t, const LevelView<double>&, const double*, double*, const double*,
-                                   double, const int*, bool, const double*, double*);
+                                   double, const int*, bool, const ZStats&);
+template void launch_matvec_dot<float>(hipStream_t, const LevelView<float>&, const float*, float*, double*,
+                                       const int*, bool);
+template void launch_matvec_dot<double>(hipStream_t, const LevelView<double>&, const double*, double*, double*,
+                                        const int*, bool);
 template void launch_residual<float>(hipStream_t, const LevelView<float>&, const float*, const float*, float*, double,
                                      const int*, bool);
 template void launch_residual<double>(hipStream_t, const LevelView<double>&, const double*, const double*, double*,

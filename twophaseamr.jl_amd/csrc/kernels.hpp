@@ -36,19 +36,34 @@ struct LevelView {
 
 // Device-resident Krylov state (Krylov.jl cg semantics). `ticket` counts
 // finished workgroups of a two-stage reduction and is zero between kernels.
+// The inner FP32 iteration of the refinement mode also keeps its FP32
+// scaling (r32 = scale * r64), the largest strong residual since the last
+// FP64 residual replacement (rlast), the replacement threshold (a
+// replacement is requested once the strong residual falls to delta * rlast;
+// delta = 0: never), the request flag and the number of replacements.
 struct CgState {
   double gamma, pnorm2, alpha, beta, pAp, zmean, rnorm, strong, target, rmean;
-  int iter, itmax, done, reason, ticket;
+  double scale, rlast, delta;
+  int iter, itmax, done, reason, ticket, replace, replaced;
 };
 
 // Workgroups of the Krylov scalar-step reductions (stage buffer: 4 * RED_WG).
 constexpr int RED_WG = 128;
 
+// Krylov partial sums emitted by the V-cycle's last finest-level sweep:
+// sums of V z, r z and r per block, with r the FP64 Krylov residual (r64) or
+// the FP32 inner residual of the refinement loop (r32); part = nullptr: none.
+struct ZStats {
+  const double* r64 = nullptr;
+  const float* r32 = nullptr;
+  double* part = nullptr;
+};
+
 bool supported_block(int bx, int by);
 
 template <typename T>
 void launch_sweep(hipStream_t s, const LevelView<T>& L, const T* xi, T* xo, const T* b, double w, const int* done,
-                  bool fine, const double* r64, double* part);
+                  bool fine, const ZStats& zs);
 template <typename T>
 void launch_residual(hipStream_t s, const LevelView<T>& L, const T* x, const T* b, T* u, double wp, const int* done,
                      bool fine);
@@ -60,11 +75,15 @@ void launch_prolong(hipStream_t s, const LevelView<T>& L, T* x, const T* xc, dou
 template <typename T>
 void launch_coarse(hipStream_t s, int n, const double* inv, const T* b, T* x, const int* done);
 
-void launch_matvec_dot(hipStream_t s, const LevelView<double>& L, const double* p, double* y, double* part,
-                       const int* done);
+// y = A p with per-block partials of p.y (part[b]) and, with `sums`, of the
+// sum of y (part[nblocks + b])
+template <typename T>
+void launch_matvec_dot(hipStream_t s, const LevelView<T>& L, const T* p, T* y, double* part, const int* done,
+                       bool sums = false);
 void launch_floor_bound(hipStream_t s, const LevelView<double>& L, const double* x, const double* b, double* part);
 void launch_reduce(hipStream_t s, int n, int nsets, const double* part, double* out, bool is_max);
-void launch_cg_alpha(hipStream_t s, int nb, const double* part, double* stage, CgState* st);
+void launch_cg_alpha(hipStream_t s, int nb, const double* part, double* stage, CgState* st, bool sums = false,
+                     double total_volume = 1.0);
 template <typename TV>
 void launch_cg_update_xr(hipStream_t s, int n, int nbc, const double* p, const double* Ap, double* x, double* r,
                          const double* vol, TV* vb, TV* vx, const TV* dinv, double omega, double* part, CgState* st,
@@ -73,6 +92,27 @@ void launch_cg_beta(hipStream_t s, int nb, int nmax, const double* part, const d
                     double total_volume, CgState* st, int init);
 template <typename TV>
 void launch_cg_update_p(hipStream_t s, int n, const TV* z, double* p, const CgState* st, const int* done);
+
+// Inner FP32 Krylov iteration of the mixed-precision refinement loop. Four
+// cells per thread; the strong-residual maxima go to `part`, one per
+// workgroup of ir_groups(n).
+int ir_groups(int n);
+void launch_ir_update_xr(hipStream_t s, int n, int nbc, const float* p, const float* Ap, float* e, float* r,
+                         const double* vol, float* vx, const float* dinv, double omega, const double* r64,
+                         double* part, CgState* st, int init);
+void launch_ir_update_p(hipStream_t s, int n, const float* z, float* p, const CgState* st, const int* done);
+// x += e / scale, e = 0: when a residual replacement is requested, or always
+// with force
+void launch_ir_correct(hipStream_t s, int n, float* e, double* x, const CgState* st, bool force);
+// FP64 residual replacement (when requested): r32 = scale (b - A x) with the
+// FP64 finest-level operator L64, the V-cycle's zero-start iterate and the
+// strong-residual maxima (one per 1024 cells, like launch_ir_update_xr)
+void launch_ir_replace(hipStream_t s, const LevelView<double>& L64, const double* x, const double* b, float* r32,
+                       float* x0, const float* dinv, double omega, double* part, CgState* st);
+// stop test on a replaced residual: done when its strong maximum (ng
+// partials) meets the target
+void launch_ir_check(hipStream_t s, int ng, const double* maxpart, double* stage, CgState* st);
+void launch_fill_f32(hipStream_t s, int n, float v, float* out);
 
 void launch_residual_strong(hipStream_t s, int n, int nbc, const double* b, const double* y, double* r,
                             const double* vol, double* part);
@@ -97,7 +137,7 @@ void launch_divergence(hipStream_t s, int n, int nbc, const int* cptr, const int
 void launch_adjoint_face(hipStream_t s, int nf, const double* yl, const double* kyp, const double* yp,
                          const double* kyl, const double* kappa, double* g);
 template <typename T>
-void launch_zstats(hipStream_t s, int nblocks, int bx, const T* z, const double* r64, const double* vol, double* part,
+void launch_zstats(hipStream_t s, int nblocks, int bx, const T* z, const ZStats& zs, const double* vol,
                    const int* done);
 void launch_axpby(hipStream_t s, int n, double a, const double* x, double b, double* y);
 void launch_cast_d2f(hipStream_t s, int n, const double* in, float* out);

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -128,7 +129,16 @@ struct tpamr_ctx {
   // graph of one Krylov iteration (TPAMR_GRAPH=0 in the environment launches
   // the iteration's kernels one by one instead)
   bool use_graph = true;
+  bool trace = false;  // TPAMR_TRACE=1: print the Krylov state of every iteration to stderr
   hipGraphExec_t graph = nullptr;
+  hipGraphExec_t graph_ir = nullptr;  // one inner FP32 Krylov iteration
+  // TPAMR_FP32_REFINE: the inner FP32 Krylov iteration replaces its
+  // residual by the FP64 residual b - A x once the strong residual has
+  // fallen by ir_delta since the last replacement (TPAMR_IR_DELTA in the
+  // environment)
+  double ir_delta = 5e-3;
+  DevArray<float> p32, Ap32, e32;  // inner Krylov vectors (TPAMR_FP32_REFINE)
+  bool fp32() const { return precision != TPAMR_FP64; }
   // Krylov iterations in flight before the host reads a done flag
   // (TPAMR_PIPE in the environment, 2..PIPE_MAX)
   int pipe = 3;
@@ -190,6 +200,10 @@ void destroy_graph(tpamr_ctx* c) {
   if (c->graph) {
     (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
+  }
+  if (c->graph_ir) {
+    (void)hipGraphExecDestroy(c->graph_ir);
+    c->graph_ir = nullptr;
   }
 }
 
@@ -266,8 +280,8 @@ void upload_hierarchy(tpamr_ctx* c, bool topology) {
     }
     c->ld.clear();
     c->lf.clear();
-    c->ld.resize(c->precision == TPAMR_FP64 ? nl : 1);
-    if (c->precision == TPAMR_FP32_VCYCLE) c->lf.resize(nl);
+    c->ld.resize(c->fp32() ? 1 : nl);
+    if (c->fp32()) c->lf.resize(nl);
     // finest-level face data
     const TreeMesh& M = *H.levels[0]->mesh;
     const LevelData& L0 = *H.levels[0];
@@ -324,6 +338,11 @@ void upload_hierarchy(tpamr_ctx* c, bool topology) {
     c->b.alloc(N);
     c->tmp.alloc(N);
     c->tmp2.alloc(N);
+    if (c->precision == TPAMR_FP32_REFINE) {
+      c->p32.alloc(N);
+      c->Ap32.alloc(N);
+      c->e32.alloc(N);
+    }
     c->f1.alloc(F);
     c->f2.alloc(F);
     c->f3.alloc(F);
@@ -353,7 +372,7 @@ void upload_hierarchy(tpamr_ctx* c, bool topology) {
 // Enqueue one V-cycle. On entry the finest level's b holds the residual and
 // its x holds the zero-start sweep (or zero). Returns the buffer holding z.
 template <typename T>
-T* enqueue_vcycle(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done, const double* r64, double* zpart) {
+T* enqueue_vcycle(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done, const ZStats& zs) {
   hipStream_t s = c->stream;
   const int nl = int(lv.size());
   const int pre = c->cfg.pre, post = c->cfg.post;
@@ -364,13 +383,13 @@ T* enqueue_vcycle(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done, c
   }
   if (nl == 1) {
     launch_coarse<T>(s, c->H->coarse_n, c->coarse_inv.p, lv[0].b.p, lv[0].x.p, done);
-    if (zpart) launch_zstats<T>(s, lv[0].view.nblocks, lv[0].view.bx, lv[0].x.p, r64, lv[0].view.volume, zpart, done);
+    if (zs.part) launch_zstats<T>(s, lv[0].view.nblocks, lv[0].view.bx, lv[0].x.p, zs, lv[0].view.volume, done);
     return lv[0].x.p;
   }
   for (int l = 0; l + 1 < nl; ++l) {
     DevLevel<T>& L = lv[l];
     for (int q = 1; q < pre; ++q) {
-      launch_sweep<T>(s, L.view, cur[l], oth[l], L.b.p, L.omega, done, l == 0, nullptr, nullptr);
+      launch_sweep<T>(s, L.view, cur[l], oth[l], L.b.p, L.omega, done, l == 0, ZStats{});
       std::swap(cur[l], oth[l]);
     }
     launch_residual<T>(s, L.view, cur[l], L.b.p, L.u.p, L.omega_p, done, l == 0);
@@ -383,12 +402,11 @@ T* enqueue_vcycle(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done, c
   for (int l = nl - 2; l >= 0; --l) {
     DevLevel<T>& L = lv[l];
     launch_prolong<T>(s, L.view, cur[l], cur[l + 1], L.omega_p, done, l == 0);
-    const int sweeps = (l == 0 && zpart && post == 0) ? 1 : post;
+    const int sweeps = (l == 0 && zs.part && post == 0) ? 1 : post;
     for (int q = 0; q < sweeps; ++q) {
-      const bool last = l == 0 && q == sweeps - 1 && zpart;
+      const bool last = l == 0 && q == sweeps - 1 && zs.part;
       const double w = (post == 0) ? 0.0 : L.omega;
-      launch_sweep<T>(s, L.view, cur[l], oth[l], L.b.p, w, done, l == 0, last ? r64 : nullptr,
-                      last ? zpart : nullptr);
+      launch_sweep<T>(s, L.view, cur[l], oth[l], L.b.p, w, done, l == 0, last ? zs : ZStats{});
       std::swap(cur[l], oth[l]);
     }
   }
@@ -401,7 +419,7 @@ T* enqueue_vcycle(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done, c
 LevelView<double>& fine64(tpamr_ctx* c) { return c->ld[0].view; }
 
 void matvec(tpamr_ctx* c, const double* x, double* y) {
-  launch_matvec_dot(c->stream, fine64(c), x, y, nullptr, nullptr);
+  launch_matvec_dot<double>(c->stream, fine64(c), x, y, nullptr, nullptr);
 }
 
 double reduce_host(tpamr_ctx* c, int n, int nsets, const double* part, bool is_max, double* out_sets = nullptr) {
@@ -453,7 +471,10 @@ double guarded_target(tpamr_ctx* c, double target, double initial, double curren
 // enqueue z = M r for the Krylov loop: V-cycle input from r, z stats partials
 template <typename T>
 T* enqueue_precondition(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done) {
-  return enqueue_vcycle<T>(c, lv, done, c->r.p, c->part.p);
+  ZStats zs;
+  zs.r64 = c->r.p;
+  zs.part = c->part.p;
+  return enqueue_vcycle<T>(c, lv, done, zs);
 }
 
 template <typename T>
@@ -469,7 +490,7 @@ void enqueue_iteration(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, bool init) {
     launch_cg_update_xr<T>(s, N, nb, nullptr, c->Ap.p, nullptr, c->r.p, c->topo[0].volume.p, L0.b.p, L0.x.p,
                            L0.dinv.p, w0, c->maxpart.p, c->st, 1);
   } else {
-    launch_matvec_dot(s, fine64(c), c->p.p, c->Ap.p, c->part.p, done);
+    launch_matvec_dot<double>(s, fine64(c), c->p.p, c->Ap.p, c->part.p, done);
     launch_cg_alpha(s, nblk, c->part.p, c->stage.p, c->st);
     launch_cg_update_xr<T>(s, N, nb, c->p.p, c->Ap.p, c->x.p, c->r.p, c->topo[0].volume.p, L0.b.p, L0.x.p,
                            L0.dinv.p, w0, c->maxpart.p, c->st, 0);
@@ -479,32 +500,31 @@ void enqueue_iteration(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, bool init) {
   launch_cg_update_p<T>(s, N, z, c->p.p, c->st, done);
 }
 
-template <typename T>
-void build_graph(tpamr_ctx* c, std::vector<DevLevel<T>>& lv) {
-  destroy_graph(c);
+// Capture one non-initial Krylov iteration into `graph`.
+template <class Enq>
+void capture_iteration(tpamr_ctx* c, hipGraphExec_t& graph, Enq&& enqueue) {
+  if (graph) {
+    (void)hipGraphExecDestroy(graph);
+    graph = nullptr;
+  }
   hipGraph_t g;
   HIPCHECK(hipStreamSynchronize(c->stream));
   HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  enqueue_iteration<T>(c, lv, false);
+  enqueue(false);
   HIPCHECK(hipStreamEndCapture(c->stream, &g));
-  HIPCHECK(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+  HIPCHECK(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
   HIPCHECK(hipGraphDestroy(g));
 }
 
-// One run of Krylov CG (warm start from x) until the strong residual
-// callback, the iteration limit, or breakdown. Returns the iteration count.
-template <typename T>
-int run_cg(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double target, int itmax) {
+// Run Krylov iterations until the device-side done flag (strong residual
+// callback, iteration limit or breakdown). `enqueue(init)` enqueues one
+// iteration; the state must already hold target and itmax. Returns the
+// iteration count.
+template <class Enq>
+int run_iterations(tpamr_ctx* c, hipGraphExec_t& graph, int itmax, Enq&& enqueue) {
   hipStream_t s = c->stream;
-  const int N = c->topo[0].ncells;
-  CgState init{};
-  init.target = target;
-  init.itmax = itmax == 0 ? 2 * N : itmax;
-  *c->st_host = init;
-  HIPCHECK(hipMemcpyAsync(c->st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, s));
-  launch_fill(s, N, 0.0, c->p.p);
-  enqueue_iteration<T>(c, lv, true);
-  if (c->use_graph && !c->graph) build_graph<T>(c, lv);
+  enqueue(true);
+  if (c->use_graph && !graph) capture_iteration(c, graph, enqueue);
   // `pipe` iterations stay queued: the host reads the done flag of the
   // oldest one while the GPU runs the others; iterations queued after the
   // last one exit at once on the device-side done flag.
@@ -512,22 +532,92 @@ int run_cg(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double target, int itmax)
   int launched = 0;
   for (;;) {
     if (c->use_graph)
-      HIPCHECK(hipGraphLaunch(c->graph, s));
+      HIPCHECK(hipGraphLaunch(graph, s));
     else
-      enqueue_iteration<T>(c, lv, false);
+      enqueue(false);
     HIPCHECK(hipMemcpyAsync(c->st_ring + launched % P, c->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipEventRecord(c->evs[launched % P], s));
     ++launched;
     if (launched >= P) {
       const int q = (launched - P) % P;
       HIPCHECK(hipEventSynchronize(c->evs[q]));
+      if (c->trace) {
+        const CgState& t = c->st_ring[q];
+        std::fprintf(stderr, "tpamr: iter %d strong %.6e gamma %.6e rlast %.6e replace %d replaced %d done %d\n",
+                     t.iter, t.strong, t.gamma, t.rlast, t.replace, t.replaced, t.done);
+      }
       if (c->st_ring[q].done) break;
     }
-    if (launched > init.itmax + P) break;
+    if (launched > itmax + P) break;
   }
   HIPCHECK(hipStreamSynchronize(s));
   HIPCHECK(hipMemcpy(c->st_host, c->st, sizeof(CgState), hipMemcpyDeviceToHost));
   return c->st_host->iter;
+}
+
+int reset_state(tpamr_ctx* c, double target, int itmax, double scale = 1.0, double delta = 0.0) {
+  const int N = c->topo[0].ncells;
+  CgState init{};
+  init.target = target;
+  init.itmax = itmax == 0 ? 2 * N : itmax;
+  init.scale = scale;
+  init.delta = delta;
+  init.rlast = 1.0;
+  *c->st_host = init;
+  HIPCHECK(hipMemcpyAsync(c->st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+  return init.itmax;
+}
+
+// One run of Krylov CG (warm start from x) until the strong residual
+// callback, the iteration limit, or breakdown. Returns the iteration count.
+template <typename T>
+int run_cg(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double target, int itmax) {
+  const int n = reset_state(c, target, itmax);
+  launch_fill(c->stream, c->topo[0].ncells, 0.0, c->p.p);
+  return run_iterations(c, c->graph, n, [&](bool init) { enqueue_iteration<T>(c, lv, init); });
+}
+
+// One inner FP32 Krylov iteration of the refinement mode: e (correction
+// since the last replacement), r (the finest level's V-cycle rhs), p and Ap
+// are FP32; pAp, the z statistics and every scalar are FP64. init: e = 0,
+// r = scale * c->r. When the previous iteration requested it, the FP32
+// correction moves into the FP64 solution c->x, r is replaced by
+// scale * (b - A x) in FP64 and its strong norm decides the stop before the
+// V-cycle (the three kernels exit at once otherwise).
+void enqueue_ir_iteration(tpamr_ctx* c, std::vector<DevLevel<float>>& lv, bool init) {
+  hipStream_t s = c->stream;
+  const int N = c->topo[0].ncells, nb = c->H->levels[0]->mesh->nb(), nblk = c->topo[0].nblocks;
+  const int* done = &c->st->done;
+  DevLevel<float>& L0 = lv[0];
+  const double w0 = c->cfg.pre > 0 ? L0.omega : 0.0;
+  if (!init) {
+    launch_matvec_dot<float>(s, L0.view, c->p32.p, c->Ap32.p, c->part.p, done, true);
+    launch_cg_alpha(s, nblk, c->part.p, c->stage.p, c->st, true, c->H->total_volume);
+  }
+  launch_ir_update_xr(s, N, nb, c->p32.p, c->Ap32.p, c->e32.p, L0.b.p, c->topo[0].volume.p, L0.x.p, L0.dinv.p, w0,
+                      c->r.p, c->maxpart.p, c->st, init ? 1 : 0);
+  if (!init) {
+    launch_ir_correct(s, N, c->e32.p, c->x.p, c->st, false);
+    launch_ir_replace(s, fine64(c), c->x.p, c->b.p, L0.b.p, L0.x.p, L0.dinv.p, w0, c->maxpart.p, c->st);
+    launch_ir_check(s, ir_groups(N), c->maxpart.p, c->stage.p, c->st);
+  }
+  ZStats zs;
+  zs.r32 = L0.b.p;
+  zs.part = c->part.p;
+  float* z = enqueue_vcycle<float>(c, lv, done, zs);
+  launch_cg_beta(s, nblk, ir_groups(N), c->part.p, c->maxpart.p, c->stage.p, c->H->total_volume, c->st,
+                 init ? 1 : 0);
+  launch_ir_update_p(s, N, z, c->p32.p, c->st, done);
+}
+
+// One run of the refinement mode's Krylov iteration from the FP64 residual
+// in c->r (scaled by `scale` into FP32) until the strong residual reaches
+// `target` (scaled units). Returns the iteration count; the correction since
+// the last replacement is left in c->e32.
+int run_ir(tpamr_ctx* c, std::vector<DevLevel<float>>& lv, double target, double scale, int itmax) {
+  const int n = reset_state(c, target, itmax, scale, c->ir_delta);
+  launch_fill_f32(c->stream, c->topo[0].ncells, 0.f, c->p32.p);
+  return run_iterations(c, c->graph_ir, n, [&](bool init) { enqueue_ir_iteration(c, lv, init); });
 }
 
 template <typename T>
@@ -568,9 +658,35 @@ tpamr_stats solve_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double rtol, 
   double target = std::max(atol, std::max(rtol * std::max(ilinf, EPS), floor0));
   int iterations = 0;
   int reason = 0;
-  if (ilinf > target) {
+  int runs = 0, replaced = 0;
+  constexpr bool fp32 = std::is_same<T, float>::value;
+  if (ilinf > target && fp32 && c->precision == TPAMR_FP32_REFINE) {
+    // FP32 Krylov CG with FP64 residual replacement; each run starts from
+    // the FP64 residual scaled to a unit strong norm, and its end is checked
+    // against the FP64 true residual (the reference's restart-and-verify
+    // loop).
+    int total = 0;
+    double linf = ilinf;
+    while (total < max_cycles) {
+      ++runs;
+      const double scale = 1.0 / linf;
+      const int it = run_ir(c, c->lf, target * scale, scale, max_cycles - total);
+      reason = c->st_host->reason;
+      replaced += c->st_host->replaced;
+      total += it;
+      launch_ir_correct(s, N, c->e32.p, c->x.p, c->st, true);
+      remove_mean(c, c->x.p);
+      double l2;
+      true_residual(c, l2, linf);
+      const double eff = guarded_target(c, target, ilinf, linf);
+      if (linf <= eff) break;
+      if (!std::isfinite(linf) || it == 0) break;
+    }
+    iterations = total;
+  } else if (ilinf > target) {
     int total = 0;
     while (total < max_cycles) {
+      ++runs;
       const int it = run_cg<T>(c, lv, target, max_cycles - total);
       reason = c->st_host->reason;
       total += it;
@@ -602,6 +718,8 @@ tpamr_stats solve_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double rtol, 
   st.target_linf = target;
   st.residual_scale = std::max(ilinf, EPS);
   st.solve_ms = ms;
+  st.krylov_runs = runs;
+  st.replacements = replaced;
   return st;
 }
 
@@ -612,7 +730,7 @@ tpamr_stats solve_dispatch(tpamr_ctx* c, double rtol, double atol, int max_cycle
   if (compat != 0 && compat != 1) throw std::invalid_argument("compatibility must be :project or :error");
   if (c->cfg.pre != c->cfg.post)
     throw std::invalid_argument("CG requires equal pre- and post-smoothing sweep counts");
-  if (c->precision == TPAMR_FP32_VCYCLE) return solve_impl<float>(c, c->lf, rtol, atol, max_cycles, compat);
+  if (c->fp32()) return solve_impl<float>(c, c->lf, rtol, atol, max_cycles, compat);
   return solve_impl<double>(c, c->ld, rtol, atol, max_cycles, compat);
 }
 
@@ -622,6 +740,7 @@ void vcycle_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv) {
   hipStream_t s = c->stream;
   const int N = c->topo[0].ncells, nb = c->H->levels[0]->mesh->nb();
   CgState init{};
+  init.scale = 1.0;
   *c->st_host = init;
   HIPCHECK(hipMemcpyAsync(c->st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, s));
   DevLevel<T>& L0 = lv[0];
@@ -629,7 +748,7 @@ void vcycle_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv) {
   launch_fill(s, N, 0.0, c->Ap.p);
   launch_cg_update_xr<T>(s, N, nb, nullptr, c->Ap.p, nullptr, c->r.p, c->topo[0].volume.p, L0.b.p, L0.x.p,
                          L0.dinv.p, w0, c->maxpart.p, c->st, 1);
-  T* z = enqueue_vcycle<T>(c, lv, &c->st->done, nullptr, nullptr);
+  T* z = enqueue_vcycle<T>(c, lv, &c->st->done, ZStats{});
   if constexpr (std::is_same<T, float>::value)
     launch_cast_f2d(s, N, z, c->tmp.p);
   else
@@ -671,11 +790,17 @@ int tpamr_create(tpamr_ctx** out, int device, int precision) {
   if (!out) return TPAMR_EINVAL;
   *out = nullptr;
   auto c = std::make_unique<tpamr_ctx>();
-  if (precision != TPAMR_FP64 && precision != TPAMR_FP32_VCYCLE) return TPAMR_EINVAL;
+  if (precision != TPAMR_FP64 && precision != TPAMR_FP32_VCYCLE && precision != TPAMR_FP32_REFINE)
+    return TPAMR_EINVAL;
   c->precision = precision;
   c->device = device;
   if (const char* g = std::getenv("TPAMR_GRAPH")) c->use_graph = std::strcmp(g, "0") != 0;
   if (const char* q = std::getenv("TPAMR_PIPE")) c->pipe = std::min(PIPE_MAX, std::max(2, std::atoi(q)));
+  if (const char* q = std::getenv("TPAMR_TRACE")) c->trace = std::strcmp(q, "0") != 0;
+  if (const char* q = std::getenv("TPAMR_IR_DELTA")) {
+    const double v = std::atof(q);
+    if (v > 0 && v < 1) c->ir_delta = v;
+  }
   if (device >= 0) {
     int rc = guard(c.get(), [&] {
       int n = 0;
@@ -992,7 +1117,7 @@ int tpamr_krylov(tpamr_ctx* c, const double* rhs, const double* x0, double targe
     check_finite(x0, N, "initial solution");
     upload_vec(c, c->b.p, rhs, N);
     upload_vec(c, c->x.p, x0, N);
-    const int it = c->precision == TPAMR_FP32_VCYCLE ? run_cg<float>(c, c->lf, target, max_iterations)
+    const int it = c->fp32() ? run_cg<float>(c, c->lf, target, max_iterations)
                                                      : run_cg<double>(c, c->ld, target, max_iterations);
     download_vec(c, x, c->x.p, N);
     *iterations = it;
@@ -1008,7 +1133,7 @@ int tpamr_vcycle(tpamr_ctx* c, const double* r, double* z) {
     HIPCHECK(hipSetDevice(c->device));
     const int N = c->topo[0].ncells;
     upload_vec(c, c->r.p, r, N);
-    if (c->precision == TPAMR_FP32_VCYCLE)
+    if (c->fp32())
       vcycle_impl<float>(c, c->lf);
     else
       vcycle_impl<double>(c, c->ld);
@@ -1211,17 +1336,17 @@ int tpamr_time_smoother(tpamr_ctx* c, int reps, double* avg_ms, double* bytes) {
     const size_t bx = L.mesh->bs[0], by = L.mesh->bs[1];
     double elem = 8;
     HIPCHECK(hipEventRecord(c->ev0, s));
-    if (c->precision == TPAMR_FP32_VCYCLE) {
+    if (c->fp32()) {
       DevLevel<float>& D = c->lf[0];
       for (int q = 0; q < reps; ++q)
         launch_sweep<float>(s, D.view, (q & 1) ? D.xs.p : D.x.p, (q & 1) ? D.x.p : D.xs.p, D.b.p, D.omega, nullptr,
-                            true, nullptr, nullptr);
+                            true, ZStats{});
       elem = 4;
     } else {
       DevLevel<double>& D = c->ld[0];
       for (int q = 0; q < reps; ++q)
         launch_sweep<double>(s, D.view, (q & 1) ? D.xs.p : D.x.p, (q & 1) ? D.x.p : D.xs.p, D.b.p, D.omega, nullptr,
-                             true, nullptr, nullptr);
+                             true, ZStats{});
     }
     HIPCHECK(hipEventRecord(c->ev1, s));
     HIPCHECK(hipEventSynchronize(c->ev1));

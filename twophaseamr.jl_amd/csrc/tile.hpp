@@ -342,11 +342,12 @@ __device__ inline T block4_max(T v, T* scratch) {
 #define TILE4_SHARED(T, BX, BY) extern __shared__ __align__(16) unsigned char S_[]
 
 // Damped Jacobi sweep x_out = x_in + w D^-1 (b - A x_in); STATS: per-block
-// partial sums of V z, r z and r for the Krylov update.
-template <typename T, int BX, int BY, bool FINE, bool STATS>
+// partial sums of V z, r z and r for the Krylov update (r of type TR: the
+// FP64 Krylov residual or the FP32 inner residual).
+template <typename T, int BX, int BY, bool FINE, bool STATS, typename TR = double>
 __global__ void __launch_bounds__(256) k_sweep(LevelView<T> L, const T* __restrict__ x_in, T* __restrict__ x_out,
                                               const T* __restrict__ rhs, double omega, const int* __restrict__ done,
-                                              const double* __restrict__ r64, double* __restrict__ part) {
+                                              const TR* __restrict__ rz, double* __restrict__ part) {
   if (done && *done) return;
   TILE4_SHARED(T, BX, BY);
   const int wg = xcd_chunk_group(blockIdx.x, L.wg_chunk);
@@ -356,7 +357,11 @@ __global__ void __launch_bounds__(256) k_sweep(LevelView<T> L, const T* __restri
   Vec4<double> rd{};
   tile4<T, BX, BY, false>(S_, L, P, Arr4<T>{x_in}, [&]() {
     bv = ld4(rhs + P.cell0);
-    if constexpr (STATS) rd = ld4(r64 + P.cell0);
+    if constexpr (STATS) {
+      const Vec4<TR> q = ld4(rz + P.cell0);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) rd.v[m] = double(q.v[m]);
+    }
   }, vc, ax, d);
   Vec4<T> z;
   double sz = 0, srz = 0, sr = 0;
@@ -474,29 +479,79 @@ __global__ void __launch_bounds__(256) k_prolong(LevelView<T> L, T* __restrict__
   st4(x + P.cell0, o);
 }
 
-// y = A p (FP64) with per-block partial p.y
-template <int BX, int BY>
-__global__ void __launch_bounds__(256) k_matvec_dot(LevelView<double> L, const double* __restrict__ p,
-                                                   double* __restrict__ y, double* __restrict__ part,
-                                                   const int* __restrict__ done) {
+// y = A p with per-block partial p.y accumulated in FP64 (FP64 Krylov
+// operator, or the FP32 finest level of the inner Krylov iteration)
+// SUM: also the per-block sums of y (part[nb + b]).
+template <typename T, int BX, int BY, bool SUM>
+__global__ void __launch_bounds__(256) k_matvec_dot(LevelView<T> L, const T* __restrict__ p, T* __restrict__ y,
+                                                   double* __restrict__ part, const int* __restrict__ done) {
   if (done && *done) return;
-  TILE4_SHARED(double, BX, BY);
+  TILE4_SHARED(T, BX, BY);
   const int wg = xcd_chunk_group(blockIdx.x, L.wg_chunk);
   if (wg * Geo<BX, BY>::BPW >= L.nblocks) return;  // grid padding
   const Place<BX, BY> P(wg, L.nblocks);
-  Vec4<double> vc, ax, d;
-  tile4<double, BX, BY, false>(S_, L, P, Arr4<double>{p}, Pre0{}, vc, ax, d);
-  double s = 0;
+  Vec4<T> vc, ax, d;
+  tile4<T, BX, BY, false>(S_, L, P, Arr4<T>{p}, Pre0{}, vc, ax, d);
+  double s = 0, sy = 0;
   if (P.valid) {
     st4(y + P.cell0, ax);
 #pragma unroll
-    for (int m = 0; m < 4; ++m) s += vc.v[m] * ax.v[m];
+    for (int m = 0; m < 4; ++m) {
+      s += double(vc.v[m]) * double(ax.v[m]);
+      if constexpr (SUM) sy += double(ax.v[m]);
+    }
   }
   if (part) {
     __shared__ double red[4];
     s = block4_sum<double, BX, BY>(s, red);
     if (P.valid && P.lt == 0) part[P.b] = s;
+    if constexpr (SUM) {
+      sy = block4_sum<double, BX, BY>(sy, red);
+      if (P.valid && P.lt == 0) part[L.nblocks + P.b] = sy;
+    }
   }
+}
+
+// FP64 residual replacement of the inner FP32 Krylov iteration (runs when
+// the state requests one): r32 = scale (b - A x) with the FP64 operator, the
+// V-cycle's zero-start iterate x0 = w D^-1 r32 and the workgroup's maximum of
+// |r32| / V (a workgroup holds 1024 cells, the grouping of k_ir_update_xr).
+template <int BX, int BY>
+__global__ void __launch_bounds__(256) k_ir_replace(LevelView<double> L, const double* __restrict__ x,
+                                                   const double* __restrict__ rhs, float* __restrict__ r32,
+                                                   float* __restrict__ x0, const float* __restrict__ dinv,
+                                                   float omega, double* __restrict__ maxpart, CgState* st) {
+  if (st->done || !st->replace) return;
+  TILE4_SHARED(double, BX, BY);
+  const int wg = xcd_chunk_group(blockIdx.x, L.wg_chunk);
+  if (wg * Geo<BX, BY>::BPW >= L.nblocks) return;  // grid padding
+  const Place<BX, BY> P(wg, L.nblocks);
+  Vec4<double> bv{}, vc, ax, d;
+  Vec4<float> dv{};
+  tile4<double, BX, BY, false>(S_, L, P, Arr4<double>{x}, [&]() {
+    bv = ld4(rhs + P.cell0);
+    dv = ld4(dinv + P.cell0);
+  }, vc, ax, d);
+  double q = 0;
+  if (P.valid) {
+    const double sc = st->scale;
+    Vec4<float> rv, xv;
+    bool nan = false;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      rv.v[m] = float(sc * (bv.v[m] - ax.v[m]));
+      xv.v[m] = omega * dv.v[m] * rv.v[m];
+      q = fmax(q, double(fabsf(rv.v[m])));
+      nan = nan || rv.v[m] != rv.v[m];
+    }
+    st4(r32 + P.cell0, rv);
+    st4(x0 + P.cell0, xv);
+    q /= L.volume[P.b];
+    if (nan) q = __longlong_as_double(0x7ff8000000000000LL);
+  }
+  __shared__ double red[4];
+  q = block_max<double, 256>(q, red);
+  if (threadIdx.x == 0) maxpart[wg] = q;
 }
 
 // Factorized round-off bound per cell and the per-block maximum of

@@ -27,6 +27,26 @@ const LIB = get(ENV, "TPAMR_LIB", joinpath(@__DIR__, "..", "lib", "libtpamr_hip.
 
 const FP64 = Cint(0)
 const FP32_VCYCLE = Cint(1)
+const FP32_REFINE = Cint(2)
+
+"""Mirror of `tpamr_stats` (include/tpamr.h), field for field; the first eleven
+fields are `LinearSolverStats` (src/solvers/multigrid/stats.jl:14-36)."""
+struct TpamrStats
+    cycles::Cint
+    converged::Cint
+    compatible::Cint
+    reason::Cint
+    initial_l2::Float64
+    initial_linf::Float64
+    final_l2::Float64
+    final_linf::Float64
+    target_linf::Float64
+    residual_scale::Float64
+    removed_rhs_mean::Float64
+    solve_ms::Float64
+    krylov_runs::Cint
+    replacements::Cint
+end
 
 function _check(ctx::Ptr{Cvoid}, rc::Cint)
     rc == 0 && return nothing

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libtpamr_hip.so")
 
 TPAMR_FP64 = 0
 TPAMR_FP32_VCYCLE = 1
+TPAMR_FP32_REFINE = 2
 
 ERRORS = {
     -1: ValueError,       # ArgumentError
@@ -38,7 +39,8 @@ class Stats(C.Structure):
     _fields_ = [("cycles", C.c_int), ("converged", C.c_int), ("compatible", C.c_int), ("reason", C.c_int),
                 ("initial_l2", C.c_double), ("initial_linf", C.c_double), ("final_l2", C.c_double),
                 ("final_linf", C.c_double), ("target_linf", C.c_double), ("residual_scale", C.c_double),
-                ("removed_rhs_mean", C.c_double), ("solve_ms", C.c_double)]
+                ("removed_rhs_mean", C.c_double), ("solve_ms", C.c_double), ("krylov_runs", C.c_int),
+                ("replacements", C.c_int)]
 
 
 class MGOptions(C.Structure):

@@ -98,6 +98,8 @@ class LinearSolverStats:
     preconditioner: str = "tree_geometric_multigrid"
     reason: str = "none"
     solve_ms: float = 0.0
+    krylov_runs: int = 0
+    replacements: int = 0
 
     @classmethod
     def _from(cls, st: L.Stats, precision):
@@ -106,8 +108,10 @@ class LinearSolverStats:
                    final_linf=st.final_linf, target_linf=st.target_linf, residual_scale=st.residual_scale,
                    removed_rhs_mean=st.removed_rhs_mean,
                    removed_rhs_component_means=[st.removed_rhs_mean], removed_rhs_component_anchors=[1],
-                   preconditioner="tree_geometric_multigrid" + ("_fp32" if precision == L.TPAMR_FP32_VCYCLE else ""),
-                   reason=REASONS.get(st.reason, str(st.reason)), solve_ms=st.solve_ms)
+                   preconditioner="tree_geometric_multigrid" + ("" if precision == L.TPAMR_FP64 else "_fp32"),
+                   method="cg" if precision != L.TPAMR_FP32_REFINE else "refinement(cg_fp32)",
+                   reason=REASONS.get(st.reason, str(st.reason)), solve_ms=st.solve_ms,
+                   krylov_runs=st.krylov_runs, replacements=st.replacements)
 
 
 class PoissonOperator:
@@ -148,8 +152,11 @@ class PoissonOperator:
 
 class EllipticSolver:
     """EllipticSolver(operator; multigrid=MultigridOptions(), precision='fp64',
-    device=0): FP64 CG preconditioned by one tree-GMG V-cycle per iteration,
-    with the V-cycle in FP64 ('fp64') or FP32 ('fp32_vcycle')."""
+    device=0): CG preconditioned by one tree-GMG V-cycle per iteration.
+    'fp64': FP64 CG, FP64 V-cycle. 'fp32_vcycle': FP64 CG, FP32 V-cycle.
+    'fp32_refine': FP64 iterative refinement (FP64 residual and solution)
+    around an inner CG with FP32 vectors, FP64 inner products and the FP32
+    V-cycle; `cycles` counts the inner iterations."""
 
     def __init__(self, operator: PoissonOperator, *, multigrid: MultigridOptions | None = None,
                  precision: str = "fp64", device: int = 0, options: LinearSolverOptions | None = None, **kw):
@@ -167,9 +174,10 @@ class EllipticSolver:
             multigrid = MultigridOptions(**{**vars(multigrid), **over})
         if multigrid.pre_sweeps != multigrid.post_sweeps:
             raise ValueError("CG requires equal pre- and post-smoothing sweep counts for a geometric V-cycle")
-        prec = {"fp64": L.TPAMR_FP64, "fp32_vcycle": L.TPAMR_FP32_VCYCLE, "fp32": L.TPAMR_FP32_VCYCLE}
+        prec = {"fp64": L.TPAMR_FP64, "fp32_vcycle": L.TPAMR_FP32_VCYCLE, "fp32": L.TPAMR_FP32_VCYCLE,
+                "fp32_refine": L.TPAMR_FP32_REFINE}
         if precision not in prec:
-            raise ValueError("precision must be 'fp64' or 'fp32_vcycle'")
+            raise ValueError("precision must be 'fp64', 'fp32_vcycle' or 'fp32_refine'")
         self.operator = operator
         self.multigrid = multigrid
         self.precision = prec[precision]
