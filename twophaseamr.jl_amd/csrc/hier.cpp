@@ -529,20 +529,26 @@ void level_topology(LevelData& L) {
   }
 }
 
-// Device form of the irregular rows (see LevelData): per block, the
-// irregular-cell bitmask, the LDS slot of every row entry and the list of
-// cells outside the block and its same-level halo.
+// Device form of the irregular rows (see LevelData): per block, the rows in
+// decreasing length order with their entries in groups of four (entry 4g+t of
+// row x of the block at (r0 * ng + g * nr + x) * 4 + t, ng = ceil(w / 4)),
+// each entry's LDS slot in the block tile, and the cells outside the block and
+// its same-level halo. Tile slots: interior cells j * bx + i, then the halo
+// strips W (bx*by + j), E (+ by), S (+ 2 by + i), N (+ 2 by + bx + i), then
+// the external cells (+ 2 (bx + by) + e). Padding entries: value 0, the row's
+// own slot.
 void build_ell(LevelData& L) {
   const TreeMesh& M = *L.mesh;
-  const int bx = M.bs[0], by = M.bs[1], nb = bx * by, W = bx + 2;
-  const int tile = (bx + 2) * (by + 2);
+  const int bx = M.bs[0], by = M.bs[1], nb = bx * by;
+  const int hw = nb, he = nb + by, hs = nb + 2 * by, hn = nb + 2 * by + bx, ext0 = nb + 2 * (bx + by);
   const int nrows = int(L.irr_local.size());
   int w = 0;
   for (int r = 0; r < nrows; ++r) w = std::max(w, L.irr_ptr[r + 1] - L.irr_ptr[r]);
   L.ell_w = w;
-  L.ell_ref.assign(size_t(nrows) * w, 0);
-  L.ell_val.assign(size_t(nrows) * w, 0.0);
-  L.ell_bound.assign(size_t(nrows) * w, 0.0);
+  const int ng = (w + 3) / 4, w4 = 4 * ng;
+  L.ell_ref.assign(size_t(nrows) * w4, 0);
+  L.ell_val.assign(size_t(nrows) * w4, 0.0);
+  L.ell_bound.assign(size_t(nrows) * w4, 0.0);
   L.ell_loc.assign(nrows, 0u);
   L.ell_diag.assign(nrows, 0.0);
   L.ext_max = 0;
@@ -554,40 +560,40 @@ void build_ell(LevelData& L) {
     ext.clear();
     const int* nbr = &L.nbr[4 * b];
     const int r0 = L.blk_irr[b], nr = L.blk_irr[b + 1] - r0;
-    // rows of the block by decreasing length: the lanes reading column k of
-    // the block's column-major region are then a prefix
+    // rows of the block by decreasing length: the lanes reading group g of
+    // the block's region are then a prefix
     order.resize(nr);
     for (int x = 0; x < nr; ++x) order[x] = r0 + x;
     std::stable_sort(order.begin(), order.end(), [&](int p, int q) {
       return L.irr_ptr[p + 1] - L.irr_ptr[p] > L.irr_ptr[q + 1] - L.irr_ptr[q];
     });
+    auto slot = [&](int x, int k) { return (size_t(r0) * ng + size_t(k / 4) * nr + x) * 4 + k % 4; };
     for (int x = 0; x < nr; ++x) {
       const int r = order[x];
       const int loc = L.irr_local[r];
       const int len = L.irr_ptr[r + 1] - L.irr_ptr[r];
       L.ell_loc[r0 + x] = uint32_t(loc) | (uint32_t(len) << 16);
       L.ell_diag[r0 + x] = L.irr_diag[r];
-      const int own = ((loc / bx) + 1) * W + (loc % bx) + 1;
-      for (int k = 0; k < w; ++k) L.ell_ref[size_t(r0) * w + size_t(k) * nr + x] = uint16_t(own);
+      for (int k = 0; k < w4; ++k) L.ell_ref[slot(x, k)] = uint16_t(loc);
       int k = 0;
       for (int e = L.irr_ptr[r]; e < L.irr_ptr[r + 1]; ++e, ++k) {
         const int col = L.irr_col[e];
         const int cb = col / nb, cl = col % nb, ci = cl % bx, cj = cl / bx;
         int ref;
-        if (cb == b) ref = (cj + 1) * W + ci + 1;
-        else if (cb == nbr[0] && ci == bx - 1) ref = (cj + 1) * W;
-        else if (cb == nbr[1] && ci == 0) ref = (cj + 1) * W + bx + 1;
-        else if (cb == nbr[2] && cj == by - 1) ref = ci + 1;
-        else if (cb == nbr[3] && cj == 0) ref = (by + 1) * W + ci + 1;
+        if (cb == b) ref = cl;
+        else if (cb == nbr[0] && ci == bx - 1) ref = hw + cj;
+        else if (cb == nbr[1] && ci == 0) ref = he + cj;
+        else if (cb == nbr[2] && cj == by - 1) ref = hs + ci;
+        else if (cb == nbr[3] && cj == 0) ref = hn + ci;
         else {
           auto it = ext.find(col);
           if (it == ext.end()) {
             it = ext.emplace(col, int(ext.size())).first;
             L.ext_col.push_back(col);
           }
-          ref = tile + it->second;
+          ref = ext0 + it->second;
         }
-        const size_t at = size_t(r0) * w + size_t(k) * nr + x;
+        const size_t at = slot(x, k);
         L.ell_ref[at] = uint16_t(ref);
         L.ell_val[at] = L.irr_val[e];
         L.ell_bound[at] = L.irr_bound[e];

@@ -89,12 +89,12 @@ struct LevelData {
   std::vector<double> irr_diag;
   std::vector<int> cell_irr;  // per cell: irregular row index or -1
   std::vector<double> dinv;   // 1 / diag(A)
-  // Device form of the irregular rows: a bitmask of irregular cells per
-  // block (NB/32 words) and, per block, a column-major region of nr x ell_w
-  // entries (entry k of the block's row x at blk_irr[b]*ell_w + k*nr + x)
-  // with the rows ordered by decreasing length; ell_loc[blk_irr[b] + x] holds
-  // local cell | length << 16. Columns are LDS slots of the block's tile:
-  // (BX+2)(BY+2) tile and halo positions, then the block's external cells.
+  // Device form of the irregular rows: per block, the rows ordered by
+  // decreasing length with their entries in groups of four (entry 4g+t of the
+  // block's row x at (blk_irr[b] * ng + g * nr + x) * 4 + t, ng =
+  // ceil(ell_w / 4)); ell_loc[blk_irr[b] + x] holds local cell | length << 16.
+  // Columns are LDS slots of the block's tile: interior cells, the W, E, S, N
+  // halo strips, then the block's external cells (build_ell).
   int ell_w = 0;
   int ext_max = 0;  // most external cells of one block
   std::vector<uint32_t> ell_loc;

@@ -24,7 +24,9 @@ struct LevelView {
   const T* ell_val;
   const T* ell_bound;  // floor-bound values (FP64 finest level only)
   const T* ell_diag;  // diagonal per irregular row, in ell_loc order
-  const unsigned short* ell_ref;  // entry k of row x of block b at blk_irr[b] * ell_w + k * nrows_b + x
+  // irregular rows of block b (nr_b rows, ng = ceil(ell_w / 4) groups of four
+  // entries per row): entry 4 g + t of row x at (blk_irr[b] * ng + g * nr_b + x) * 4 + t
+  const unsigned short* ell_ref;
   const int* blk_irr;        // nblocks + 1 offsets of the irregular rows per block
   const unsigned* ell_loc;   // per row in a block's length order: local cell | length << 16
   const int* blk_ext;

@@ -5,8 +5,9 @@
 // (one wavefront per 16x16 block). Phase A issues every global load of the
 // thread (segment, conductances, halo or external slot, its irregular row,
 // the caller's operands); one barrier; phase B lets one lane per irregular
-// row sum its fixed-width row from LDS; phase C evaluates the 5-point stencil
-// of the four cells from LDS and registers.
+// row sum its row from LDS; phase C evaluates the 5-point stencil of the four
+// cells from registers, 16-byte LDS reads of the rows above and below, and
+// lane shuffles for the west and east neighbours.
 #pragma once
 
 template <typename T>
@@ -51,17 +52,27 @@ struct Geo {
   static constexpr int TPB = NB / 4;                 // threads per block
   static constexpr int BPW = TPB >= 256 ? 1 : 256 / TPB;  // blocks per workgroup
   static constexpr int WG = BPW * TPB;
-  static constexpr int W = BX + 2;
-  static constexpr int TILE = (BX + 2) * (BY + 2);
   static constexpr int HALO = 2 * (BX + BY);
-  static constexpr int EXTMAX = NB;                  // host limit on external cells per block
   static constexpr int SEG = TPB < 64 ? TPB : 64;    // lanes of one block within a wavefront
+  // LDS value slots of a block tile: the interior cells (row-major, like the
+  // cells in HBM), the halo strips W and E (one value per block row), S and N
+  // (one per block column), then the external cells of the irregular rows.
+  // The irregular rows' column references (built on the host) use this
+  // numbering.
+  static constexpr int HW = NB, HE = NB + BY, HS = NB + 2 * BY, HN = NB + 2 * BY + BX, EXT = NB + HALO;
 };
 
-// LDS slot of one block: tile values (block, halo, then `ext` external
+// Irregular rows: entries in groups of four per row (ELL_G groups at most),
+// each group one 16-byte load of values and one 8-byte load of four 16-bit
+// LDS slot references; padding entries have value 0 and reference the row's
+// own cell.
+constexpr int ELL_G = ELL_MAX / 4;
+
+// LDS slot of one block: tile values (interior, halo strips, `ext` external
 // cells), the south conductances of the block (north exchange), and the
 // irregular rows' products, diagonals and flags. Slots are carved from the
-// dynamic LDS of the launch (slot_bytes each, BPW slots per workgroup).
+// dynamic LDS of the launch (bytes(ext) each, BPW slots per workgroup); every
+// array starts 16-byte aligned.
 template <typename T, int BX, int BY>
 struct Slot4 {
   T* v;
@@ -70,18 +81,18 @@ struct Slot4 {
   T* d;
   unsigned* flag4;  // one byte per cell, four cells per word
   static __host__ __device__ inline int bytes(int ext) {
-    constexpr int NB = BX * BY, TILE = (BX + 2) * (BY + 2);
-    const int vals = TILE + ((ext + 3) & ~3) + 3 * NB;
-    return ((vals * int(sizeof(T)) + NB + 15) / 16) * 16;
+    using G = Geo<BX, BY>;
+    const int vals = G::EXT + ((ext + 3) & ~3) + 3 * G::NB;
+    return ((vals * int(sizeof(T)) + G::NB + 15) / 16) * 16;
   }
   __device__ inline Slot4(unsigned char* lds, int slot, int ext) {
-    constexpr int NB = BX * BY, TILE = (BX + 2) * (BY + 2);
-    unsigned char* base = lds + size_t(slot) * bytes(ext);
+    using G = Geo<BX, BY>;
+    unsigned char* base = lds + slot * bytes(ext);
     v = reinterpret_cast<T*>(base);
-    cs = v + TILE + ((ext + 3) & ~3);
-    ax = cs + NB;
-    d = ax + NB;
-    flag4 = reinterpret_cast<unsigned*>(d + NB);
+    cs = v + G::EXT + ((ext + 3) & ~3);
+    ax = cs + G::NB;
+    d = ax + G::NB;
+    flag4 = reinterpret_cast<unsigned*>(d + G::NB);
   }
 };
 
@@ -151,56 +162,59 @@ template <typename T, int BX, int BY, bool ABS, class F, class Pre>
 __device__ inline void tile4(unsigned char* lds, const LevelView<T>& L, const Place<BX, BY>& P, F fetch, Pre pre,
                              Vec4<T>& vc, Vec4<T>& ax, Vec4<T>& d) {
   using G = Geo<BX, BY>;
-  constexpr int W = G::W, TILE = G::TILE, NB = G::NB, TPB = G::TPB, TPR = G::TPR;
+  constexpr int NB = G::NB, TPB = G::TPB, TPR = G::TPR;
   const Slot4<T, BX, BY> S(lds, P.slot(), L.ext_cap);
   const int b = P.b, lt = P.lt, j = P.j, q = P.q, i0 = P.i0;
+  const int seg = 4 * lt;  // the segment's first interior slot (j * BX + i0)
   // rows of the workgroup's blocks: blk_irr is a prefix sum over blocks
   const int wb0 = P.wg * G::BPW, wb1 = min(wb0 + G::BPW, L.nblocks);
   const bool any = L.blk_irr[wb1] > L.blk_irr[wb0];
+  const int ng = (L.ell_w + 3) >> 2;  // entry groups per row
+  const T* __restrict__ vals = ABS ? L.ell_bound : L.ell_val;
   Vec4<T> own{}, cw4{}, cs4{}, cn4{};
   T rdg = 0;
   T ceb = 0;
-  int nr = 0;
-  T rv[ELL_MAX];
-  unsigned short rr[ELL_MAX];
+  int nr = 0, r0 = 0;
+  Vec4<T> rv[ELL_G];
+  uint2 rr[ELL_G];
   int rloc = 0, rlen = 0;
   unsigned bits = 0;
-  const int w = L.ell_w;
   if (P.valid) {
     own = fetch.seg(P.cell0);
     cw4 = ld4(L.cw + P.cell0);
     cs4 = ld4(L.cs + P.cell0);
     if (q == TPR - 1) ceb = L.ce[b * BY + j];
     if (j == BY - 1) cn4 = ld4(L.cn + b * BX + i0);
-    // halo values (W, E, S, N) of same-level neighbour blocks
+    // halo values (W, E, S, N) of same-level neighbour blocks: halo index h
+    // is LDS slot NB + h
     for (int h = lt; h < G::HALO; h += TPB) {
-      int side, pos, slot, src;
+      int side, src;
       if (h < BY) {
-        side = 0; pos = h; slot = (pos + 1) * W; src = pos * BX + (BX - 1);
+        side = 0; src = h * BX + (BX - 1);
       } else if (h < 2 * BY) {
-        side = 1; pos = h - BY; slot = (pos + 1) * W + BX + 1; src = pos * BX;
+        side = 1; src = (h - BY) * BX;
       } else if (h < 2 * BY + BX) {
-        side = 2; pos = h - 2 * BY; slot = pos + 1; src = (BY - 1) * BX + pos;
+        side = 2; src = (BY - 1) * BX + (h - 2 * BY);
       } else {
-        side = 3; pos = h - 2 * BY - BX; slot = (BY + 1) * W + pos + 1; src = pos;
+        side = 3; src = h - 2 * BY - BX;
       }
       const int nb = L.nbr[4 * b + side];
       const T hv = nb >= 0 ? fetch.one(nb * NB + src) : T(0);
-      S.v[slot] = ABS ? fabs(hv) : hv;
+      S.v[NB + h] = ABS ? fabs(hv) : hv;
     }
     if (any) {
       // external cells of the irregular rows
       const int e0 = L.blk_ext[b], ne = L.blk_ext[b + 1] - e0;
       for (int e = lt; e < ne; e += TPB) {
         const T ev = fetch.one(L.ext_col[e0 + e]);
-        S.v[TILE + e] = ABS ? fabs(ev) : ev;
+        S.v[G::EXT + e] = ABS ? fabs(ev) : ev;
       }
-      // this lane's irregular row (rows in decreasing length order, entries
-      // column-major per block: the active lanes of column k are a prefix)
-      int r0 = L.blk_irr[b];
+      // this lane's irregular row (rows in decreasing length order; group g
+      // of row x of the block at (r0 * ng + g * nr + x) * 4)
+      r0 = L.blk_irr[b];
       nr = L.blk_irr[b + 1] - r0;
       if constexpr (G::TPB >= 64) {
-        // uniform per wavefront: column offsets k * nr become scalar
+        // uniform per wavefront: group offsets become scalar
         r0 = __builtin_amdgcn_readfirstlane(r0);
         nr = __builtin_amdgcn_readfirstlane(nr);
       }
@@ -208,25 +222,25 @@ __device__ inline void tile4(unsigned char* lds, const LevelView<T>& L, const Pl
         const unsigned info = L.ell_loc[r0 + lt];
         rloc = int(info & 0xffffu);
         rlen = int(info >> 16);
-        const size_t base = size_t(r0) * w + lt;
-        const T* vals = ABS ? L.ell_bound : L.ell_val;
+        rdg = L.ell_diag[r0 + lt];
+        const int base = (r0 * ng + lt) * 4;
 #pragma unroll
-        for (int k = 0; k < ELL_MAX; ++k)
-          if (k < rlen) {
-            rv[k] = vals[base + size_t(k) * nr];
-            rr[k] = L.ell_ref[base + size_t(k) * nr];
+        for (int g = 0; g < ELL_G; ++g)
+          if (g < ng && 4 * g < rlen) {
+            const int at = base + g * nr * 4;
+            rv[g] = ld4(vals + at);
+            rr[g] = *reinterpret_cast<const uint2*>(L.ell_ref + at);
           }
       }
-      if (lt < nr) rdg = L.ell_diag[r0 + lt];
       S.flag4[lt] = 0u;  // this lane's four cells
     }
     pre();
+    if constexpr (ABS) {
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      if (ABS) own.v[m] = fabs(own.v[m]);
-      S.v[(j + 1) * W + i0 + 1 + m] = own.v[m];
-      S.cs[j * BX + i0 + m] = cs4.v[m];
+      for (int m = 0; m < 4; ++m) own.v[m] = fabs(own.v[m]);
     }
+    st4(S.v + seg, own);
+    st4(S.cs + seg, cs4);
   }
   __syncthreads();
   if (any) {
@@ -236,25 +250,35 @@ __device__ inline void tile4(unsigned char* lds, const LevelView<T>& L, const Pl
       // zero padding drop out and rounding scales with the differences, as
       // in the reference's face-based product. ABS keeps sum_j |b_ij||v_j|.
       if (lt < nr) {
-        const T vi = ABS ? T(0) : S.v[(rloc / BX + 1) * W + rloc % BX + 1];
+        const T vi = ABS ? T(0) : S.v[rloc];
         T s = 0;
 #pragma unroll
-        for (int k = 0; k < ELL_MAX; ++k)
-          if (k < rlen) s += rv[k] * (S.v[rr[k]] - vi);
+        for (int g = 0; g < ELL_G; ++g)
+          if (g < ng && 4 * g < rlen) {
+            s += rv[g].v[0] * (S.v[rr[g].x & 0xffffu] - vi);
+            s += rv[g].v[1] * (S.v[rr[g].x >> 16] - vi);
+            s += rv[g].v[2] * (S.v[rr[g].y & 0xffffu] - vi);
+            s += rv[g].v[3] * (S.v[rr[g].y >> 16] - vi);
+          }
         S.ax[rloc] = s;
         S.d[rloc] = rdg;
         reinterpret_cast<unsigned char*>(S.flag4)[rloc] = 1;
       }
       // blocks with more irregular rows than lanes
       for (int x = lt + TPB; x < nr; x += TPB) {
-        const int r0 = L.blk_irr[b];
         const unsigned info = L.ell_loc[r0 + x];
         const int loc = int(info & 0xffffu), len = int(info >> 16);
-        const size_t base = size_t(r0) * w + x;
-        const T* vals = ABS ? L.ell_bound : L.ell_val;
-        const T vi = ABS ? T(0) : S.v[(loc / BX + 1) * W + loc % BX + 1];
+        const T vi = ABS ? T(0) : S.v[loc];
         T s = 0;
-        for (int k = 0; k < len; ++k) s += vals[base + size_t(k) * nr] * (S.v[L.ell_ref[base + size_t(k) * nr]] - vi);
+        for (int g = 0; 4 * g < len; ++g) {
+          const int at = (r0 * ng + x) * 4 + g * nr * 4;
+          const Vec4<T> v = ld4(vals + at);
+          const uint2 r = *reinterpret_cast<const uint2*>(L.ell_ref + at);
+          s += v.v[0] * (S.v[r.x & 0xffffu] - vi);
+          s += v.v[1] * (S.v[r.x >> 16] - vi);
+          s += v.v[2] * (S.v[r.y & 0xffffu] - vi);
+          s += v.v[3] * (S.v[r.y >> 16] - vi);
+        }
         S.ax[loc] = s;
         S.d[loc] = L.ell_diag[r0 + x];
         reinterpret_cast<unsigned char*>(S.flag4)[loc] = 1;
@@ -262,34 +286,37 @@ __device__ inline void tile4(unsigned char* lds, const LevelView<T>& L, const Pl
     }
     __syncthreads();
   }
+  // west / east neighbours inside the block row come from the next lanes
   const T cw_next = __shfl_down(cw4.v[0], 1, G::SEG);
+  const T v_prev = __shfl_up(own.v[3], 1, G::SEG);
+  const T v_next = __shfl_down(own.v[0], 1, G::SEG);
   vc = own;
   if (!P.valid) return;
+  const T vwest = q > 0 ? v_prev : S.v[G::HW + j];
+  const T veast = q < TPR - 1 ? v_next : S.v[G::HE + j];
+  const Vec4<T> vs4 = ld4(S.v + (j > 0 ? seg - BX : G::HS + i0));
+  const Vec4<T> vn4 = ld4(S.v + (j < BY - 1 ? seg + BX : G::HN + i0));
+  if (j < BY - 1) cn4 = ld4(S.cs + seg + BX);
   Vec4<T> axi{}, dgi{};
   if (any) {
     const unsigned f = S.flag4[lt];
     bits = (f & 1u) | ((f >> 7) & 2u) | ((f >> 14) & 4u) | ((f >> 21) & 8u);
     if (bits) {
-#pragma unroll
-      for (int m = 0; m < 4; ++m)
-        if ((bits >> m) & 1u) {
-          axi.v[m] = S.ax[j * BX + i0 + m];
-          dgi.v[m] = S.d[j * BX + i0 + m];
-        }
+      axi = ld4(S.ax + seg);
+      dgi = ld4(S.d + seg);
     }
   }
   // stencil of every cell, then a select for the irregular ones (keeps the
   // results in registers)
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
-    const int i = i0 + m;
-    const T vw = m > 0 ? own.v[m - 1] : S.v[(j + 1) * W + i];
-    const T ve = m < 3 ? own.v[m + 1] : S.v[(j + 1) * W + i + 2];
-    const T vs = S.v[j * W + i + 1], vn = S.v[(j + 2) * W + i + 1];
+    const T vw = m > 0 ? own.v[m - 1] : vwest;
+    const T ve = m < 3 ? own.v[m + 1] : veast;
+    const T vs = vs4.v[m], vn = vn4.v[m];
     const T cw = cw4.v[m];
     const T ce = m < 3 ? cw4.v[m + 1] : (q < TPR - 1 ? cw_next : ceb);
     const T cs = cs4.v[m];
-    const T cn = j < BY - 1 ? S.cs[(j + 1) * BX + i] : cn4.v[m];
+    const T cn = cn4.v[m];
     const T dd = (cw + ce) + (cs + cn);
     const T vi = own.v[m];
     const T st = ABS ? dd * vi + ((cw * vw + ce * ve) + (cs * vs + cn * vn))
@@ -341,11 +368,16 @@ __device__ inline T block4_max(T v, T* scratch) {
 
 #define TILE4_SHARED(T, BX, BY) extern __shared__ __align__(16) unsigned char S_[]
 
+// Minimum waves per SIMD the tile kernels are compiled for: FP32 tiles fit
+// 8 (64 VGPRs; their LDS slots allow 8 workgroups per CU), FP64 tiles keep
+// the compiler's choice (their LDS slots allow 4).
+#define TILE_WAVES(T) (sizeof(T) == 4 ? 8 : 1)
+
 // Damped Jacobi sweep x_out = x_in + w D^-1 (b - A x_in); STATS: per-block
 // partial sums of V z, r z and r for the Krylov update (r of type TR: the
 // FP64 Krylov residual or the FP32 inner residual).
 template <typename T, int BX, int BY, bool FINE, bool STATS, typename TR = double>
-__global__ void __launch_bounds__(256) k_sweep(LevelView<T> L, const T* __restrict__ x_in, T* __restrict__ x_out,
+__global__ void __launch_bounds__(256, TILE_WAVES(T)) k_sweep(LevelView<T> L, const T* __restrict__ x_in, T* __restrict__ x_out,
                                               const T* __restrict__ rhs, double omega, const int* __restrict__ done,
                                               const TR* __restrict__ rz, double* __restrict__ part) {
   if (done && *done) return;
@@ -393,7 +425,7 @@ __global__ void __launch_bounds__(256) k_sweep(LevelView<T> L, const T* __restri
 
 // u = w_P D^-1 (b - A x)
 template <typename T, int BX, int BY, bool FINE>
-__global__ void __launch_bounds__(256) k_residual(LevelView<T> L, const T* __restrict__ x, const T* __restrict__ rhs,
+__global__ void __launch_bounds__(256, TILE_WAVES(T)) k_residual(LevelView<T> L, const T* __restrict__ x, const T* __restrict__ rhs,
                                                  T* __restrict__ u, double omega_p, const int* __restrict__ done) {
   if (done && *done) return;
   TILE4_SHARED(T, BX, BY);
@@ -412,7 +444,7 @@ __global__ void __launch_bounds__(256) k_residual(LevelView<T> L, const T* __res
 // Smoothed restriction b_c = P_t' (D u / w_P - S u), and the zero-start
 // coarse sweep x_c = w_c D_c^-1 b_c.
 template <typename T, int BX, int BY, bool FINE>
-__global__ void __launch_bounds__(256) k_restrict(LevelView<T> L, const T* __restrict__ u, double omega_p,
+__global__ void __launch_bounds__(256, TILE_WAVES(T)) k_restrict(LevelView<T> L, const T* __restrict__ u, double omega_p,
                                                  T* __restrict__ bc, T* __restrict__ xc, const T* __restrict__ dinv_c,
                                                  double omega_c, int zero_start, const int* __restrict__ done) {
   if (done && *done) return;
@@ -463,7 +495,7 @@ __global__ void __launch_bounds__(256) k_restrict(LevelView<T> L, const T* __res
 
 // x += t - w_P D^-1 S t with t = P_t x_c
 template <typename T, int BX, int BY, bool FINE>
-__global__ void __launch_bounds__(256) k_prolong(LevelView<T> L, T* __restrict__ x, const T* __restrict__ xc,
+__global__ void __launch_bounds__(256, TILE_WAVES(T)) k_prolong(LevelView<T> L, T* __restrict__ x, const T* __restrict__ xc,
                                                 double omega_p, const int* __restrict__ done) {
   if (done && *done) return;
   TILE4_SHARED(T, BX, BY);
@@ -483,7 +515,7 @@ __global__ void __launch_bounds__(256) k_prolong(LevelView<T> L, T* __restrict__
 // operator, or the FP32 finest level of the inner Krylov iteration)
 // SUM: also the per-block sums of y (part[nb + b]).
 template <typename T, int BX, int BY, bool SUM>
-__global__ void __launch_bounds__(256) k_matvec_dot(LevelView<T> L, const T* __restrict__ p, T* __restrict__ y,
+__global__ void __launch_bounds__(256, TILE_WAVES(T)) k_matvec_dot(LevelView<T> L, const T* __restrict__ p, T* __restrict__ y,
                                                    double* __restrict__ part, const int* __restrict__ done) {
   if (done && *done) return;
   TILE4_SHARED(T, BX, BY);
