@@ -13,6 +13,9 @@
 // fixed-width rows grouped per block whose columns are LDS slots.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <stdexcept>
+
 #include "kernels.hpp"
 
 namespace tpamr {
@@ -781,6 +784,44 @@ template <typename T>
 void launch_coarse(hipStream_t s, int n, const double* inv, const T* b, T* x, const int* done) {
   hipLaunchKernelGGL((k_coarse_solve<T>), dim3(cdiv(n, 64)), dim3(1024), 0, s, n, inv, b, x, done);
 }
+
+// Dynamic LDS a workgroup may use (the device limit less the tail kernel's
+// static LDS of the coarse solve).
+static int lds_budget() {
+  static int budget = -1;
+  if (budget < 0) {
+    int dev = 0, v = 65536;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      v = 65536;
+    budget = v - 16 * 65 * 8 - 256;
+  }
+  return budget;
+}
+
+template <typename T, int BX, int BY>
+static void launch_vtail_t(hipStream_t s, const TailArgs<T>& A, const int* done) {
+  int slot = 0;
+  for (int l = 0; l + 1 < A.n; ++l) slot = std::max(slot, Slot4<T, BX, BY>::bytes(A.L[l].ext_cap));
+  const int per_quarter = Geo<BX, BY>::BPW * slot;
+  int nq = 4;
+  while (nq > 1 && nq * per_quarter > lds_budget()) nq >>= 1;
+  if (nq * per_quarter > lds_budget()) throw std::runtime_error("tail levels need more LDS than a workgroup has");
+  const unsigned lds = unsigned(nq * per_quarter);
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vtail<T, BX, BY>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_budget());
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_vtail<T, BX, BY>), dim3(1), dim3(256 * nq), lds, s, A, done);
+}
+
+template <typename T>
+void launch_vtail(hipStream_t s, const TailArgs<T>& A, const int* done) {
+  TPAMR_DISPATCH(A.L[0].bx, (launch_vtail_t<T, BX, BY>(s, A, done)));
+}
+template void launch_vtail<float>(hipStream_t, const TailArgs<float>&, const int*);
+template void launch_vtail<double>(hipStream_t, const TailArgs<double>&, const int*);
 
 template <typename T>
 void launch_matvec_dot(hipStream_t s, const LevelView<T>& L, const T* p, T* y, double* part, const int* done,

@@ -77,6 +77,24 @@ void launch_prolong(hipStream_t s, const LevelView<T>& L, T* x, const T* xc, dou
 template <typename T>
 void launch_coarse(hipStream_t s, int n, const double* inv, const T* b, T* x, const int* done);
 
+// The V-cycle of the small levels in one workgroup (k_vtail): levels
+// 0..n-1 of the tail (the last one the coarsest), their vectors and weights.
+constexpr int TAIL_MAX = 8;
+template <typename T>
+struct TailArgs {
+  LevelView<T> L[TAIL_MAX];
+  T* x[TAIL_MAX];
+  T* xs[TAIL_MAX];
+  T* b[TAIL_MAX];
+  T* u[TAIL_MAX];
+  const T* dinv[TAIL_MAX];
+  double omega[TAIL_MAX], omega_p[TAIL_MAX];
+  const double* inv;
+  int n, pre, post, coarse_n;
+};
+template <typename T>
+void launch_vtail(hipStream_t s, const TailArgs<T>& A, const int* done);
+
 // y = A p with per-block partials of p.y (part[b]) and, with `sums`, of the
 // sum of y (part[nblocks + b])
 template <typename T>

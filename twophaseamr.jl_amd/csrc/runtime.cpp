@@ -130,6 +130,10 @@ struct tpamr_ctx {
   // the iteration's kernels one by one instead)
   bool use_graph = true;
   bool trace = false;  // TPAMR_TRACE=1: print the Krylov state of every iteration to stderr
+  int graph_iters = 1;  // Krylov iterations per captured graph (TPAMR_GRAPH_ITERS)
+  // V-cycle levels with at most this many blocks run in one workgroup
+  // (TPAMR_TAIL_BLOCKS; 0: every level is launched on its own)
+  int tail_blocks = 16;
   hipGraphExec_t graph = nullptr;
   hipGraphExec_t graph_ir = nullptr;  // one inner FP32 Krylov iteration
   // TPAMR_FP32_REFINE: the inner FP32 Krylov iteration replaces its
@@ -386,7 +390,15 @@ T* enqueue_vcycle(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done, c
     if (zs.part) launch_zstats<T>(s, lv[0].view.nblocks, lv[0].view.bx, lv[0].x.p, zs, lv[0].view.volume, done);
     return lv[0].x.p;
   }
-  for (int l = 0; l + 1 < nl; ++l) {
+  // levels from `ts` on (at most tail_blocks blocks each) run in one
+  // workgroup; ts = nl - 1: no tail
+  int ts = nl - 1;
+  for (int l = 1; l + 1 < nl; ++l)
+    if (lv[l].view.nblocks <= c->tail_blocks && nl - l <= TAIL_MAX) {
+      ts = l;
+      break;
+    }
+  for (int l = 0; l < ts; ++l) {
     DevLevel<T>& L = lv[l];
     for (int q = 1; q < pre; ++q) {
       launch_sweep<T>(s, L.view, cur[l], oth[l], L.b.p, L.omega, done, l == 0, ZStats{});
@@ -398,8 +410,31 @@ T* enqueue_vcycle(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done, c
     launch_restrict<T>(s, L.view, L.u.p, L.omega_p, C.b.p, C.x.p, C.dinv.p, C.omega, (!coarsest && pre > 0) ? 1 : 0,
                        done, l == 0);
   }
-  launch_coarse<T>(s, c->H->coarse_n, c->coarse_inv.p, lv[nl - 1].b.p, lv[nl - 1].x.p, done);
-  for (int l = nl - 2; l >= 0; --l) {
+  if (ts == nl - 1) {
+    launch_coarse<T>(s, c->H->coarse_n, c->coarse_inv.p, lv[nl - 1].b.p, lv[nl - 1].x.p, done);
+  } else {
+    TailArgs<T> A{};
+    A.n = nl - ts;
+    for (int k = 0; k < A.n; ++k) {
+      DevLevel<T>& L = lv[ts + k];
+      A.L[k] = L.view;
+      A.x[k] = L.x.p;
+      A.xs[k] = L.xs.p;
+      A.b[k] = L.b.p;
+      A.u[k] = L.u.p;
+      A.dinv[k] = L.dinv.p;
+      A.omega[k] = L.omega;
+      A.omega_p[k] = L.omega_p;
+    }
+    A.inv = c->coarse_inv.p;
+    A.coarse_n = c->H->coarse_n;
+    A.pre = pre;
+    A.post = post;
+    launch_vtail<T>(s, A, done);
+    // level ts's result: the buffer after its (pre - 1) + post sweeps
+    if ((std::max(pre - 1, 0) + post) & 1) std::swap(cur[ts], oth[ts]);
+  }
+  for (int l = ts - 1; l >= 0; --l) {
     DevLevel<T>& L = lv[l];
     launch_prolong<T>(s, L.view, cur[l], cur[l + 1], L.omega_p, done, l == 0);
     const int sweeps = (l == 0 && zs.part && post == 0) ? 1 : post;
@@ -500,7 +535,7 @@ void enqueue_iteration(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, bool init) {
   launch_cg_update_p<T>(s, N, z, c->p.p, c->st, done);
 }
 
-// Capture one non-initial Krylov iteration into `graph`.
+// Capture `graph_iters` non-initial Krylov iterations into `graph`.
 template <class Enq>
 void capture_iteration(tpamr_ctx* c, hipGraphExec_t& graph, Enq&& enqueue) {
   if (graph) {
@@ -510,7 +545,7 @@ void capture_iteration(tpamr_ctx* c, hipGraphExec_t& graph, Enq&& enqueue) {
   hipGraph_t g;
   HIPCHECK(hipStreamSynchronize(c->stream));
   HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  enqueue(false);
+  for (int k = 0; k < c->graph_iters; ++k) enqueue(false);
   HIPCHECK(hipStreamEndCapture(c->stream, &g));
   HIPCHECK(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
   HIPCHECK(hipGraphDestroy(g));
@@ -529,6 +564,7 @@ int run_iterations(tpamr_ctx* c, hipGraphExec_t& graph, int itmax, Enq&& enqueue
   // oldest one while the GPU runs the others; iterations queued after the
   // last one exit at once on the device-side done flag.
   const int P = c->pipe;
+  const int per = c->use_graph ? c->graph_iters : 1;
   int launched = 0;
   for (;;) {
     if (c->use_graph)
@@ -548,7 +584,7 @@ int run_iterations(tpamr_ctx* c, hipGraphExec_t& graph, int itmax, Enq&& enqueue
       }
       if (c->st_ring[q].done) break;
     }
-    if (launched > itmax + P) break;
+    if (launched * per > itmax + P * per) break;
   }
   HIPCHECK(hipStreamSynchronize(s));
   HIPCHECK(hipMemcpy(c->st_host, c->st, sizeof(CgState), hipMemcpyDeviceToHost));
@@ -797,6 +833,8 @@ int tpamr_create(tpamr_ctx** out, int device, int precision) {
   if (const char* g = std::getenv("TPAMR_GRAPH")) c->use_graph = std::strcmp(g, "0") != 0;
   if (const char* q = std::getenv("TPAMR_PIPE")) c->pipe = std::min(PIPE_MAX, std::max(2, std::atoi(q)));
   if (const char* q = std::getenv("TPAMR_TRACE")) c->trace = std::strcmp(q, "0") != 0;
+  if (const char* q = std::getenv("TPAMR_GRAPH_ITERS")) c->graph_iters = std::min(16, std::max(1, std::atoi(q)));
+  if (const char* q = std::getenv("TPAMR_TAIL_BLOCKS")) c->tail_blocks = std::max(0, std::atoi(q));
   if (const char* q = std::getenv("TPAMR_IR_DELTA")) {
     const double v = std::atof(q);
     if (v > 0 && v < 1) c->ir_delta = v;

@@ -131,14 +131,19 @@ struct Pre0 {
   __device__ inline void operator()() const {}
 };
 
-// Per-thread placement in the workgroup.
+// Per-thread placement: thread t (0..255) of tile group `wg` (BPW blocks),
+// using the LDS slots of quarter `quarter` (a tail launch runs `span` tile
+// groups at once, one per 256 threads: groups wg - quarter ... + span - 1).
+// The irregular-row phase runs when any block of those groups has irregular
+// rows, so its barrier is reached by the whole workgroup.
 template <int BX, int BY>
 struct Place {
-  int wg, b, lt, j, q, i0, cell0;
+  int wg, b, lt, j, q, i0, cell0, slot_, blk_lo, blk_hi;
   bool valid;
-  __device__ inline Place(int wg_, int nblocks) : wg(wg_) {
+  __device__ inline Place(int wg_, int nblocks, int t, int quarter, int span = 1) : wg(wg_) {
     using G = Geo<BX, BY>;
-    const int t = threadIdx.x;
+    blk_lo = min((wg - quarter) * G::BPW, nblocks);
+    blk_hi = min(blk_lo + span * G::BPW, nblocks);
     const int k = t / G::TPB;
     lt = t - k * G::TPB;
     b = wg * G::BPW + k;
@@ -149,8 +154,9 @@ struct Place {
     q = lt - j * G::TPR;
     i0 = 4 * q;
     cell0 = b * G::NB + j * BX + i0;
+    slot_ = quarter * G::BPW + k;
   }
-  __device__ inline int slot() const { return threadIdx.x / Geo<BX, BY>::TPB; }
+  __device__ inline int slot() const { return slot_; }
 };
 
 // Evaluate (A v) and D of the thread's four cells (or, with ABS, the
@@ -167,8 +173,7 @@ __device__ inline void tile4(unsigned char* lds, const LevelView<T>& L, const Pl
   const int b = P.b, lt = P.lt, j = P.j, q = P.q, i0 = P.i0;
   const int seg = 4 * lt;  // the segment's first interior slot (j * BX + i0)
   // rows of the workgroup's blocks: blk_irr is a prefix sum over blocks
-  const int wb0 = P.wg * G::BPW, wb1 = min(wb0 + G::BPW, L.nblocks);
-  const bool any = L.blk_irr[wb1] > L.blk_irr[wb0];
+  const bool any = L.blk_irr[P.blk_hi] > L.blk_irr[P.blk_lo];
   const int ng = (L.ell_w + 3) >> 2;  // entry groups per row
   const T* __restrict__ vals = ABS ? L.ell_bound : L.ell_val;
   Vec4<T> own{}, cw4{}, cs4{}, cn4{};
@@ -376,18 +381,13 @@ __device__ inline T block4_max(T v, T* scratch) {
 // Damped Jacobi sweep x_out = x_in + w D^-1 (b - A x_in); STATS: per-block
 // partial sums of V z, r z and r for the Krylov update (r of type TR: the
 // FP64 Krylov residual or the FP32 inner residual).
-template <typename T, int BX, int BY, bool FINE, bool STATS, typename TR = double>
-__global__ void __launch_bounds__(256, TILE_WAVES(T)) k_sweep(LevelView<T> L, const T* __restrict__ x_in, T* __restrict__ x_out,
-                                              const T* __restrict__ rhs, double omega, const int* __restrict__ done,
-                                              const TR* __restrict__ rz, double* __restrict__ part) {
-  if (done && *done) return;
-  TILE4_SHARED(T, BX, BY);
-  const int wg = xcd_chunk_group(blockIdx.x, L.wg_chunk);
-  if (wg * Geo<BX, BY>::BPW >= L.nblocks) return;  // grid padding
-  const Place<BX, BY> P(wg, L.nblocks);
+template <typename T, int BX, int BY, bool STATS, typename TR>
+__device__ inline void sweep_op(unsigned char* lds, const LevelView<T>& L, const Place<BX, BY>& P,
+                                const T* __restrict__ x_in, T* __restrict__ x_out, const T* __restrict__ rhs,
+                                double omega, const TR* __restrict__ rz, double* __restrict__ part) {
   Vec4<T> bv{}, vc, ax, d;
   Vec4<double> rd{};
-  tile4<T, BX, BY, false>(S_, L, P, Arr4<T>{x_in}, [&]() {
+  tile4<T, BX, BY, false>(lds, L, P, Arr4<T>{x_in}, [&]() {
     bv = ld4(rhs + P.cell0);
     if constexpr (STATS) {
       const Vec4<TR> q = ld4(rz + P.cell0);
@@ -424,16 +424,12 @@ __global__ void __launch_bounds__(256, TILE_WAVES(T)) k_sweep(LevelView<T> L, co
 }
 
 // u = w_P D^-1 (b - A x)
-template <typename T, int BX, int BY, bool FINE>
-__global__ void __launch_bounds__(256, TILE_WAVES(T)) k_residual(LevelView<T> L, const T* __restrict__ x, const T* __restrict__ rhs,
-                                                 T* __restrict__ u, double omega_p, const int* __restrict__ done) {
-  if (done && *done) return;
-  TILE4_SHARED(T, BX, BY);
-  const int wg = xcd_chunk_group(blockIdx.x, L.wg_chunk);
-  if (wg * Geo<BX, BY>::BPW >= L.nblocks) return;  // grid padding
-  const Place<BX, BY> P(wg, L.nblocks);
+template <typename T, int BX, int BY>
+__device__ inline void residual_op(unsigned char* lds, const LevelView<T>& L, const Place<BX, BY>& P,
+                                   const T* __restrict__ x, const T* __restrict__ rhs, T* __restrict__ u,
+                                   double omega_p) {
   Vec4<T> bv{}, vc, ax, d;
-  tile4<T, BX, BY, false>(S_, L, P, Arr4<T>{x}, [&]() { bv = ld4(rhs + P.cell0); }, vc, ax, d);
+  tile4<T, BX, BY, false>(lds, L, P, Arr4<T>{x}, [&]() { bv = ld4(rhs + P.cell0); }, vc, ax, d);
   if (!P.valid) return;
   Vec4<T> o;
 #pragma unroll
@@ -443,18 +439,13 @@ __global__ void __launch_bounds__(256, TILE_WAVES(T)) k_residual(LevelView<T> L,
 
 // Smoothed restriction b_c = P_t' (D u / w_P - S u), and the zero-start
 // coarse sweep x_c = w_c D_c^-1 b_c.
-template <typename T, int BX, int BY, bool FINE>
-__global__ void __launch_bounds__(256, TILE_WAVES(T)) k_restrict(LevelView<T> L, const T* __restrict__ u, double omega_p,
-                                                 T* __restrict__ bc, T* __restrict__ xc, const T* __restrict__ dinv_c,
-                                                 double omega_c, int zero_start, const int* __restrict__ done) {
-  if (done && *done) return;
+template <typename T, int BX, int BY>
+__device__ inline void restrict_op(unsigned char* lds, const LevelView<T>& L, const Place<BX, BY>& P,
+                                   const T* __restrict__ u, double omega_p, T* __restrict__ bc, T* __restrict__ xc,
+                                   const T* __restrict__ dinv_c, double omega_c, int zero_start) {
   using G = Geo<BX, BY>;
-  TILE4_SHARED(T, BX, BY);
-  const int wg = xcd_chunk_group(blockIdx.x, L.wg_chunk);
-  if (wg * Geo<BX, BY>::BPW >= L.nblocks) return;  // grid padding
-  const Place<BX, BY> P(wg, L.nblocks);
   Vec4<T> vc, ax, d;
-  tile4<T, BX, BY, false>(S_, L, P, Arr4<T>{u}, Pre0{}, vc, ax, d);
+  tile4<T, BX, BY, false>(lds, L, P, Arr4<T>{u}, Pre0{}, vc, ax, d);
   Vec4<T> w;
 #pragma unroll
   for (int m = 0; m < 4; ++m) w.v[m] = P.valid ? d.v[m] * vc.v[m] / T(omega_p) - ax.v[m] : T(0);
@@ -494,21 +485,155 @@ __global__ void __launch_bounds__(256, TILE_WAVES(T)) k_restrict(LevelView<T> L,
 }
 
 // x += t - w_P D^-1 S t with t = P_t x_c
-template <typename T, int BX, int BY, bool FINE>
-__global__ void __launch_bounds__(256, TILE_WAVES(T)) k_prolong(LevelView<T> L, T* __restrict__ x, const T* __restrict__ xc,
-                                                double omega_p, const int* __restrict__ done) {
-  if (done && *done) return;
-  TILE4_SHARED(T, BX, BY);
-  const int wg = xcd_chunk_group(blockIdx.x, L.wg_chunk);
-  if (wg * Geo<BX, BY>::BPW >= L.nblocks) return;  // grid padding
-  const Place<BX, BY> P(wg, L.nblocks);
+template <typename T, int BX, int BY>
+__device__ inline void prolong_op(unsigned char* lds, const LevelView<T>& L, const Place<BX, BY>& P, T* __restrict__ x,
+                                  const T* __restrict__ xc, double omega_p) {
   Vec4<T> xv{}, vc, ax, d;
-  tile4<T, BX, BY, false>(S_, L, P, Inject4<T, BX, BY>{xc, L.xfer}, [&]() { xv = ld4(x + P.cell0); }, vc, ax, d);
+  tile4<T, BX, BY, false>(lds, L, P, Inject4<T, BX, BY>{xc, L.xfer}, [&]() { xv = ld4(x + P.cell0); }, vc, ax, d);
   if (!P.valid) return;
   Vec4<T> o;
 #pragma unroll
   for (int m = 0; m < 4; ++m) o.v[m] = xv.v[m] + (vc.v[m] - T(omega_p) * ax.v[m] / d.v[m]);
   st4(x + P.cell0, o);
+}
+
+// Tile group of a standalone tile launch (padding workgroups return).
+#define TILE_GROUP(L)                                       \
+  const int wg = xcd_chunk_group(blockIdx.x, (L).wg_chunk);  \
+  if (wg * Geo<BX, BY>::BPW >= (L).nblocks) return;           \
+  const Place<BX, BY> P(wg, (L).nblocks, threadIdx.x, 0)
+
+template <typename T, int BX, int BY, bool FINE, bool STATS, typename TR = double>
+__global__ void __launch_bounds__(256, TILE_WAVES(T))
+    k_sweep(LevelView<T> L, const T* __restrict__ x_in, T* __restrict__ x_out, const T* __restrict__ rhs, double omega,
+            const int* __restrict__ done, const TR* __restrict__ rz, double* __restrict__ part) {
+  if (done && *done) return;
+  TILE4_SHARED(T, BX, BY);
+  TILE_GROUP(L);
+  sweep_op<T, BX, BY, STATS, TR>(S_, L, P, x_in, x_out, rhs, omega, rz, part);
+}
+
+template <typename T, int BX, int BY, bool FINE>
+__global__ void __launch_bounds__(256, TILE_WAVES(T))
+    k_residual(LevelView<T> L, const T* __restrict__ x, const T* __restrict__ rhs, T* __restrict__ u, double omega_p,
+               const int* __restrict__ done) {
+  if (done && *done) return;
+  TILE4_SHARED(T, BX, BY);
+  TILE_GROUP(L);
+  residual_op<T, BX, BY>(S_, L, P, x, rhs, u, omega_p);
+}
+
+template <typename T, int BX, int BY, bool FINE>
+__global__ void __launch_bounds__(256, TILE_WAVES(T))
+    k_restrict(LevelView<T> L, const T* __restrict__ u, double omega_p, T* __restrict__ bc, T* __restrict__ xc,
+               const T* __restrict__ dinv_c, double omega_c, int zero_start, const int* __restrict__ done) {
+  if (done && *done) return;
+  TILE4_SHARED(T, BX, BY);
+  TILE_GROUP(L);
+  restrict_op<T, BX, BY>(S_, L, P, u, omega_p, bc, xc, dinv_c, omega_c, zero_start);
+}
+
+template <typename T, int BX, int BY, bool FINE>
+__global__ void __launch_bounds__(256, TILE_WAVES(T))
+    k_prolong(LevelView<T> L, T* __restrict__ x, const T* __restrict__ xc, double omega_p,
+              const int* __restrict__ done) {
+  if (done && *done) return;
+  TILE4_SHARED(T, BX, BY);
+  TILE_GROUP(L);
+  prolong_op<T, BX, BY>(S_, L, P, x, xc, omega_p);
+}
+
+// Coarsest level: x = inv * b with the dense inverse of the augmented
+// (volume-constrained) operator, column-major, evaluated in FP64 by the
+// whole workgroup: 64 rows at a time, blockDim / 64 column groups summed
+// through LDS.
+template <typename T>
+__device__ inline void coarse_op(int n, const double* __restrict__ inv, const T* __restrict__ b, T* __restrict__ x) {
+  __shared__ double part[16][65];
+  const int lane = threadIdx.x & 63, g = threadIdx.x >> 6, ng = blockDim.x >> 6;
+  for (int r0 = 0; r0 < n; r0 += 64) {
+    const int r = r0 + lane;
+    double s0 = 0, s1 = 0;
+    if (r < n) {
+      int c = g;
+      for (; c + ng < n; c += 2 * ng) {
+        s0 += inv[size_t(c) * n + r] * double(b[c]);
+        s1 += inv[size_t(c + ng) * n + r] * double(b[c + ng]);
+      }
+      if (c < n) s0 += inv[size_t(c) * n + r] * double(b[c]);
+    }
+    part[g][lane] = s0 + s1;
+    __syncthreads();
+    if (g == 0 && r < n) {
+      double s = 0;
+      for (int q = 0; q < ng; ++q) s += part[q][lane];
+      x[r] = T(s);
+    }
+    __syncthreads();
+  }
+}
+
+// The V-cycle below a level small enough for one workgroup: levels
+// 0..n-1 of `A` (the last one the coarsest), tile passes of up to
+// blockDim / 256 tile groups at once separated by workgroup barriers. On
+// entry b[0] and the zero-start x[0] come from the restriction of the next
+// finer level; on exit level 0's result is in x[0] or xs[0] by the parity
+// of its (pre - 1) + post sweeps, as in the launch sequence.
+template <typename T, int BX, int BY>
+__global__ void __launch_bounds__(1024) k_vtail(TailArgs<T> A, const int* __restrict__ done) {
+  if (done && *done) return;
+  TILE4_SHARED(T, BX, BY);
+  using G = Geo<BX, BY>;
+  const int nq = blockDim.x >> 8, quarter = threadIdx.x >> 8, t = threadIdx.x & 255;
+  unsigned par = 0;  // bit l: level l's current iterate is in xs
+  auto cur = [&](int l) { return ((par >> l) & 1u) ? A.xs[l] : A.x[l]; };
+  auto oth = [&](int l) { return ((par >> l) & 1u) ? A.x[l] : A.xs[l]; };
+  // one tile pass over level l (every thread runs the same number of tiles)
+  auto pass = [&](int l, auto&& op) {
+    const LevelView<T>& L = A.L[l];
+    const int groups = (L.nblocks + G::BPW - 1) / G::BPW;
+    for (int g0 = 0; g0 < groups; g0 += nq) {
+      const Place<BX, BY> P(g0 + quarter, L.nblocks, t, quarter, nq);
+      op(L, P);
+    }
+    __syncthreads();
+  };
+  for (int l = 0; l + 1 < A.n; ++l) {
+    for (int q = 1; q < A.pre; ++q) {
+      T* xi = cur(l);
+      T* xo = oth(l);
+      pass(l, [&](const LevelView<T>& L, const Place<BX, BY>& P) {
+        sweep_op<T, BX, BY, false, double>(S_, L, P, xi, xo, A.b[l], A.omega[l], nullptr, nullptr);
+      });
+      par ^= 1u << l;
+    }
+    T* xl = cur(l);
+    pass(l, [&](const LevelView<T>& L, const Place<BX, BY>& P) {
+      residual_op<T, BX, BY>(S_, L, P, xl, A.b[l], A.u[l], A.omega_p[l]);
+    });
+    const int zero_start = (l + 2 < A.n && A.pre > 0) ? 1 : 0;
+    pass(l, [&](const LevelView<T>& L, const Place<BX, BY>& P) {
+      restrict_op<T, BX, BY>(S_, L, P, A.u[l], A.omega_p[l], A.b[l + 1], A.x[l + 1], A.dinv[l + 1], A.omega[l + 1],
+                             zero_start);
+    });
+  }
+  coarse_op<T>(A.coarse_n, A.inv, A.b[A.n - 1], A.x[A.n - 1]);
+  __syncthreads();
+  for (int l = A.n - 2; l >= 0; --l) {
+    T* xl = cur(l);
+    const T* xc = cur(l + 1);
+    pass(l, [&](const LevelView<T>& L, const Place<BX, BY>& P) {
+      prolong_op<T, BX, BY>(S_, L, P, xl, xc, A.omega_p[l]);
+    });
+    for (int q = 0; q < A.post; ++q) {
+      T* xi = cur(l);
+      T* xo = oth(l);
+      pass(l, [&](const LevelView<T>& L, const Place<BX, BY>& P) {
+        sweep_op<T, BX, BY, false, double>(S_, L, P, xi, xo, A.b[l], A.omega[l], nullptr, nullptr);
+      });
+      par ^= 1u << l;
+    }
+  }
 }
 
 // y = A p with per-block partial p.y accumulated in FP64 (FP64 Krylov
@@ -521,7 +646,7 @@ __global__ void __launch_bounds__(256, TILE_WAVES(T)) k_matvec_dot(LevelView<T> 
   TILE4_SHARED(T, BX, BY);
   const int wg = xcd_chunk_group(blockIdx.x, L.wg_chunk);
   if (wg * Geo<BX, BY>::BPW >= L.nblocks) return;  // grid padding
-  const Place<BX, BY> P(wg, L.nblocks);
+  const Place<BX, BY> P(wg, L.nblocks, threadIdx.x, 0);
   Vec4<T> vc, ax, d;
   tile4<T, BX, BY, false>(S_, L, P, Arr4<T>{p}, Pre0{}, vc, ax, d);
   double s = 0, sy = 0;
@@ -557,7 +682,7 @@ __global__ void __launch_bounds__(256) k_ir_replace(LevelView<double> L, const d
   TILE4_SHARED(double, BX, BY);
   const int wg = xcd_chunk_group(blockIdx.x, L.wg_chunk);
   if (wg * Geo<BX, BY>::BPW >= L.nblocks) return;  // grid padding
-  const Place<BX, BY> P(wg, L.nblocks);
+  const Place<BX, BY> P(wg, L.nblocks, threadIdx.x, 0);
   Vec4<double> bv{}, vc, ax, d;
   Vec4<float> dv{};
   tile4<double, BX, BY, false>(S_, L, P, Arr4<double>{x}, [&]() {
@@ -594,7 +719,7 @@ __global__ void __launch_bounds__(256) k_floor_bound(LevelView<double> L, const 
   TILE4_SHARED(double, BX, BY);
   const int wg = xcd_chunk_group(blockIdx.x, L.wg_chunk);
   if (wg * Geo<BX, BY>::BPW >= L.nblocks) return;  // grid padding
-  const Place<BX, BY> P(wg, L.nblocks);
+  const Place<BX, BY> P(wg, L.nblocks, threadIdx.x, 0);
   Vec4<double> bv{}, vc, ax, d;
   tile4<double, BX, BY, true>(S_, L, P, Arr4<double>{x}, [&]() { bv = ld4(rhs + P.cell0); }, vc, ax, d);
   double qm = 0;
