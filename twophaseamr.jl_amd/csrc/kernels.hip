@@ -559,6 +559,58 @@ __global__ void k_weighted_sums(int n, int nb_cells, const double* __restrict__ 
   }
 }
 
+// Start of a solve: the integrated rhs b <- V b in place, and per-workgroup
+// partial sums of |b|, b (integrated), V x and |x| (four sets).
+__global__ void k_rhs_prepare(int n, int nb_cells, double* __restrict__ b, const double* __restrict__ x,
+                              const double* __restrict__ volume, double* __restrict__ part) {
+  __shared__ double red[16];
+  const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  double a = 0, s = 0, vx = 0, ax = 0;
+  if (c < size_t(n)) {
+    const double V = volume[c / nb_cells];
+    const double bi = b[c] * V;
+    b[c] = bi;
+    a = fabs(bi);
+    s = bi;
+    vx = V * x[c];
+    ax = fabs(x[c]);
+  }
+  a = block_sum<double, 256>(a, red);
+  s = block_sum<double, 256>(s, red);
+  vx = block_sum<double, 256>(vx, red);
+  ax = block_sum<double, 256>(ax, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[gridDim.x + blockIdx.x] = s;
+    part[2 * gridDim.x + blockIdx.x] = vx;
+    part[3 * gridDim.x + blockIdx.x] = ax;
+  }
+}
+
+// b -= V mean; with a zero solution the residual is r = b: per-workgroup
+// maxima of |r| / V and sums of V (r / V)^2.
+__global__ void k_shift_residual(int n, int nb_cells, double* __restrict__ b, double mean,
+                                 const double* __restrict__ volume, double* __restrict__ r, double* __restrict__ part) {
+  __shared__ double red[16];
+  const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  double q = 0, s2 = 0;
+  if (c < size_t(n)) {
+    const double V = volume[c / nb_cells];
+    const double rv = b[c] - V * mean;
+    b[c] = rv;
+    r[c] = rv;
+    q = fabs(rv) / V;
+    if (q != q) q = __longlong_as_double(0x7ff8000000000000LL);
+    s2 = V * (rv / V) * (rv / V);
+  }
+  const double m = block_max<double, 256>(q, red);
+  const double s = block_sum<double, 256>(s2, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = m;
+    part[gridDim.x + blockIdx.x] = s;
+  }
+}
+
 // x -= shift (scalar from device memory), optionally scaled by the cell volume
 __global__ void k_shift(int n, int nb_cells, double* __restrict__ x, const double* __restrict__ shift, double scale,
                         const double* __restrict__ volume, int by_volume) {
@@ -879,6 +931,15 @@ void launch_residual_strong(hipStream_t s, int n, int nbc, const double* b, cons
 
 void launch_weighted_sums(hipStream_t s, int n, int nbc, const double* x, const double* vol, double* part) {
   hipLaunchKernelGGL(k_weighted_sums, dim3(cdiv(n, 256)), dim3(256), 0, s, n, nbc, x, vol, part);
+}
+
+void launch_rhs_prepare(hipStream_t s, int n, int nbc, double* b, const double* x, const double* vol, double* part) {
+  hipLaunchKernelGGL(k_rhs_prepare, dim3(cdiv(n, 256)), dim3(256), 0, s, n, nbc, b, x, vol, part);
+}
+
+void launch_shift_residual(hipStream_t s, int n, int nbc, double* b, double mean, const double* vol, double* r,
+                           double* part) {
+  hipLaunchKernelGGL(k_shift_residual, dim3(cdiv(n, 256)), dim3(256), 0, s, n, nbc, b, mean, vol, r, part);
 }
 
 void launch_shift(hipStream_t s, int n, int nbc, double* x, const double* shift, double scale, const double* vol,

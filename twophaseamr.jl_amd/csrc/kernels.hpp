@@ -137,6 +137,11 @@ void launch_fill_f32(hipStream_t s, int n, float v, float* out);
 void launch_residual_strong(hipStream_t s, int n, int nbc, const double* b, const double* y, double* r,
                             const double* vol, double* part);
 void launch_weighted_sums(hipStream_t s, int n, int nbc, const double* x, const double* vol, double* part);
+// b <- V b and partial sums of |b|, b, V x, |x| (four sets of ceil(n/256))
+void launch_rhs_prepare(hipStream_t s, int n, int nbc, double* b, const double* x, const double* vol, double* part);
+// b -= V mean, r = b, partial strong maxima and squared-norm sums of r
+void launch_shift_residual(hipStream_t s, int n, int nbc, double* b, double mean, const double* vol, double* r,
+                           double* part);
 void launch_shift(hipStream_t s, int n, int nbc, double* x, const double* shift, double scale, const double* vol,
                   bool by_volume);
 void launch_scale_by_volume(hipStream_t s, int n, int nbc, const double* in, double* out, const double* vol,

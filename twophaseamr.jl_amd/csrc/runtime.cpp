@@ -353,7 +353,9 @@ void upload_hierarchy(tpamr_ctx* c, bool topology) {
     c->f4.alloc(F);
     const size_t nblk = size_t(L0.nblocks);
     const size_t ncb = (size_t(N) + 255) / 256;
-    c->part.alloc(std::max(3 * nblk, 3 * ncb) + 16);
+    // per-block partials (3 sets: z statistics) or per-256-cell partials
+    // (4 sets: the rhs preparation sums)
+    c->part.alloc(std::max(3 * nblk, 4 * ncb) + 16);
     c->maxpart.alloc(std::max(nblk, ncb) + 16);
     c->scal.alloc(16);
     c->stage.alloc(4 * RED_WG);
@@ -664,34 +666,47 @@ tpamr_stats solve_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double rtol, 
   const double* vol = c->topo[0].volume.p;
   tpamr_stats st{};
   HIPCHECK(hipEventRecord(c->ev0, s));
-  // integrated rhs
-  launch_scale_by_volume(s, N, nb, c->b.p, c->b.p, vol, false);
-  // compatible projection of the rhs (one unanchored component)
-  launch_weighted_sums(s, N, nb, c->b.p, vol, c->part.p);  // sums of V b (unused), |b|, b
-  double sums[3];
+  // integrated rhs, its sums for the compatible projection, and whether the
+  // initial solution is zero (one pass, one host synchronisation)
+  launch_rhs_prepare(s, N, nb, c->b.p, c->x.p, vol, c->part.p);
+  double sums[4];
   {
-    launch_reduce(s, ncb(N), 3, c->part.p, c->scal.p, false);
-    HIPCHECK(hipMemcpyAsync(sums, c->scal.p, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
+    launch_reduce(s, ncb(N), 4, c->part.p, c->scal.p, false);
+    HIPCHECK(hipMemcpyAsync(sums, c->scal.p, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
   }
-  const double total_sum = sums[2], scale = sums[1];
+  const double scale = sums[0], total_sum = sums[1];
+  const bool zero_start = sums[3] == 0.0;
   if (!std::isfinite(total_sum)) throw std::invalid_argument("right-hand side must be finite");
   if (compat == 1 && std::fabs(total_sum) > 256 * EPS * std::max(1.0, scale))
     throw IncompatibleError("Poisson right-hand side has nonzero volume mean on component rooted at cell 1");
   const double mean = total_sum / c->H->total_volume;
-  // b -= V * mean (scalar staged through device memory)
-  {
+  st.removed_rhs_mean = mean;
+  double il2, ilinf, floor0;
+  if (zero_start) {
+    // b -= V mean; the residual of x = 0 is b and its round-off floor bound
+    // reduces to max |b / V|
+    launch_shift_residual(s, N, nb, c->b.p, mean, vol, c->r.p, c->part.p);
+    double sets[2];
+    launch_reduce(s, ncb(N), 1, c->part.p, c->scal.p, true);
+    launch_reduce(s, ncb(N), 1, c->part.p + ncb(N), c->scal.p + 1, false);
+    HIPCHECK(hipMemcpyAsync(sets, c->scal.p, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    ilinf = sets[0];
+    il2 = std::sqrt(sets[1] / c->H->total_volume);
+    floor0 = 128 * EPS * std::max(1.0, ilinf);
+  } else {
+    // b -= V * mean (scalar staged through device memory)
     double h = mean;
     HIPCHECK(hipMemcpyAsync(c->scal.p + 8, &h, sizeof(double), hipMemcpyHostToDevice, s));
     launch_shift(s, N, nb, c->b.p, c->scal.p + 8, 1.0, vol, true);
-    HIPCHECK(hipStreamSynchronize(s));
+    remove_mean(c, c->x.p);
+    true_residual(c, il2, ilinf);
+    floor0 = roundoff_floor(c);
   }
-  st.removed_rhs_mean = mean;
-  remove_mean(c, c->x.p);
-  double il2, ilinf;
-  true_residual(c, il2, ilinf);
-  const double floor0 = roundoff_floor(c);
   double target = std::max(atol, std::max(rtol * std::max(ilinf, EPS), floor0));
+  // final residual norms and the guarded target of the last check
+  double fl2 = il2, flinf = ilinf, ftarget = target;
   int iterations = 0;
   int reason = 0;
   int runs = 0, replaced = 0;
@@ -712,10 +727,10 @@ tpamr_stats solve_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double rtol, 
       total += it;
       launch_ir_correct(s, N, c->e32.p, c->x.p, c->st, true);
       remove_mean(c, c->x.p);
-      double l2;
-      true_residual(c, l2, linf);
-      const double eff = guarded_target(c, target, ilinf, linf);
-      if (linf <= eff) break;
+      true_residual(c, fl2, linf);
+      flinf = linf;
+      ftarget = guarded_target(c, target, ilinf, linf);
+      if (linf <= ftarget) break;
       if (!std::isfinite(linf) || it == 0) break;
     }
     iterations = total;
@@ -727,18 +742,14 @@ tpamr_stats solve_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double rtol, 
       reason = c->st_host->reason;
       total += it;
       remove_mean(c, c->x.p);
-      double l2, linf;
-      true_residual(c, l2, linf);
-      const double eff = guarded_target(c, target, ilinf, linf);
-      if (linf <= eff) break;
-      if (!std::isfinite(linf) || it == 0) break;
+      true_residual(c, fl2, flinf);
+      ftarget = guarded_target(c, target, ilinf, flinf);
+      if (flinf <= ftarget) break;
+      if (!std::isfinite(flinf) || it == 0) break;
     }
     iterations = total;
   }
-  remove_mean(c, c->x.p);
-  double fl2, flinf;
-  true_residual(c, fl2, flinf);
-  target = guarded_target(c, target, ilinf, flinf);
+  target = ftarget;
   HIPCHECK(hipEventRecord(c->ev1, s));
   HIPCHECK(hipEventSynchronize(c->ev1));
   float ms = 0;
