@@ -147,6 +147,24 @@ int tpamr_weighted_jump_force(tpamr_ctx* ctx, const double* jumps, double* accel
 int tpamr_divergence(tpamr_ctx* ctx, const double* u, const double* u_boundary, double* div);
 /* weighted_gradient! interior faces (elliptic.jl:1272-1297) */
 int tpamr_weighted_gradient(tpamr_ctx* ctx, const double* p, double* g);
+
+/* Halo refresh of a cell field after the projection: fill_halo!(field,
+ * ZeroGradient()) or fill_halo!(field, Dirichlet(f)) (src/mesh/variables.jl:205-229)
+ * with the schedule of _build_halo_schedule (src/mesh/topology/mesh.jl:944-1037):
+ * same-level copy, fine-to-coarse quarter average, coarse-to-fine injection,
+ * periodic wrap, physical sides from the nearest interior cell. `cells` in
+ * the reference cell order; `haloed` receives, per block, the
+ * (bx + 2 halo) x (by + 2 halo) array of CellField.blocks[b] (x index
+ * fastest). `physical` = NULL: zero-gradient physical halos; otherwise one
+ * value per physical halo entry in schedule order (what a Dirichlet callback
+ * returns at the entry centres). tpamr_halo_sizes reports the schedule's
+ * entry and physical-entry counts. */
+int tpamr_halo_sizes(tpamr_ctx* ctx, int halo, int* entries, int* physical);
+/* The schedule itself (host-only contexts too): per entry the output offset
+ * dest[e], donors donor[ptr[e] .. ptr[e+1]) with weights, and phys[e] (rank
+ * among the physical entries, -1 otherwise); ptr has entries + 1 values. */
+int tpamr_halo_schedule(tpamr_ctx* ctx, int halo, int* dest, int* ptr, int* donor, double* weight, int* phys);
+int tpamr_fill_halo(tpamr_ctx* ctx, int halo, const double* cells, const double* physical, double* haloed);
 /* _project_prepared! (navier_stokes/projection.jl:221-260) on a wall/periodic
  * domain: updates u (interior faces) and p, writes the increment phi */
 int tpamr_project(tpamr_ctx* ctx, double* u, double* p, double dt, double tolerance, int max_cycles, double* phi,

@@ -81,6 +81,10 @@ std::unique_ptr<Mesh> build_mesh(const int base[2], const int bs[2],
 
 CSC inverse_metric(const Mesh& mesh, bool* diagonal);
 
+// fill_halo! (src/mesh/variables.jl:205-229) with the halo schedule of
+// mesh.jl:944-1037; returns the number of physical halo entries
+int fill_halo(const Mesh& M, int halo, const double* cells, const double* physical_values, double* out);
+
 struct Components {
   std::vector<int> labels;  // root cell index, -1 for inactive
   std::vector<int> anchors;  // ungauged component roots (increasing)

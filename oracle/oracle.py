@@ -109,6 +109,7 @@ class OMesh:
                               C.byref(out)))
         self._h = out
         self._owned = True
+        self.block_size = tuple(int(v) for v in block_size)
         self._load()
 
     @classmethod
@@ -116,6 +117,7 @@ class OMesh:
         self = cls.__new__(cls)
         self._h = handle
         self._owned = False
+        self.block_size = None
         self._load()
         return self
 
@@ -180,6 +182,19 @@ class OMesh:
         curv = np.broadcast_to(np.asarray(curvature, np.float64), (self.nfaces,)).copy()
         _check(lib().orc_cgfm_jumps(self._h, _dptr(np.ascontiguousarray(c, np.float64)), _dptr(curv),
                                     C.c_double(sigma), int(csf), _dptr(out)))
+        return out
+
+    def fill_halo(self, cells, halo=1, physical=None):
+        """fill_halo! (src/mesh/variables.jl:205-229): per block the haloed
+        array, shape (nblocks, by + 2 halo, bx + 2 halo); `physical`: one value
+        per physical halo entry in schedule order (None: zero gradient)."""
+        bx, by = self.block_size
+        out = np.zeros((self.nblocks, by + 2 * halo, bx + 2 * halo))
+        n = C.c_int()
+        php = None if physical is None else _dptr(np.ascontiguousarray(physical, np.float64))
+        _check(lib().orc_fill_halo(self._h, int(halo), _dptr(np.ascontiguousarray(cells, np.float64)), php,
+                                   _dptr(out), C.byref(n)))
+        self.physical_halo_entries = n.value
         return out
 
     def divergence(self, u, ub=None):

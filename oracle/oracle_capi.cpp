@@ -3,6 +3,8 @@
 #include <exception>
 #include <string>
 
+#include <stdexcept>
+
 #include "oracle.hpp"
 
 namespace orc {
@@ -211,6 +213,14 @@ int orc_weighted_jump_force(void* sp, const double* jumps, double* accel) {
 
 int orc_weighted_gradient(void* sp, const double* p, double* g) {
   GUARD({ weighted_gradient(*static_cast<Solver*>(sp)->ops[0], p, g); return 0; })
+}
+
+int orc_fill_halo(void* mp, int halo, const double* cells, const double* physical, double* out, int* nphysical) {
+  GUARD({
+    if (halo < 1) throw std::invalid_argument("halo must be positive");
+    *nphysical = fill_halo(*static_cast<Mesh*>(mp), halo, cells, physical, out);
+    return 0;
+  })
 }
 
 int orc_divergence(void* mp, const double* u, const double* ub, double* div) {

@@ -496,4 +496,80 @@ CSC inverse_metric(const Mesh& M, bool* diagonal) {
   return K;
 }
 
+// Halo refresh: `_build_halo_schedule` (src/mesh/topology/mesh.jl:944-1037),
+// `_cellref_at_lattice` (mesh.jl:534-541, leaf lookup by tree descent), and
+// `fill_halo!` (src/mesh/variables.jl:205-229) with `ZeroGradient` or, when
+// `physical_values` is given, one prescribed value per physical entry in
+// schedule order (the values a `Dirichlet` callback returns). Output: per
+// block the haloed (bx + 2h) x (by + 2h) array with the x index fastest
+// (`blockdata` of a CellField). Returns the number of physical entries.
+int fill_halo(const Mesh& M, int h, const double* cells, const double* physical_values, double* out) {
+  const int nx = M.bs[0], ny = M.bs[1], nb = nx * ny;
+  const int W = nx + 2 * h, H = ny + 2 * h;
+  const int L = M.max_leaf_level();
+  std::unordered_map<uint64_t, int> index;
+  for (size_t b = 0; b < M.blocks.size(); ++b) index[kpack(M.blocks[b].key.level, M.blocks[b].key.i, M.blocks[b].key.j)] = int(b);
+  const long lattice[2] = {long(M.base[0]) << L, long(M.base[1]) << L};
+  const long quarter[2] = {4 * lattice[0], 4 * lattice[1]};
+  // leaf cell containing lattice cell p (finest-level units): descend from the root
+  auto cell_at = [&](long px, long py) {
+    for (int l = 0; l <= L; ++l) {
+      const long cx = px >> (L - l), cy = py >> (L - l);
+      auto it = index.find(kpack(l, int(cx / nx), int(cy / ny)));
+      if (it != index.end()) return it->second * nb + int(cy % ny) * nx + int(cx % nx);
+    }
+    throw std::logic_error("halo sample outside every leaf");
+  };
+  auto wrap = [](long x, long period, bool periodic, bool& outside) {
+    if (periodic) {
+      outside = false;
+      return ((x % period) + period) % period;
+    }
+    outside = x < 0 || x >= period;
+    return std::min(std::max(x, 0L), period - 1);
+  };
+  int nphys = 0;
+  for (size_t b = 0; b < M.blocks.size(); ++b) {
+    const Block& B = M.blocks[b];
+    double* o = out + b * size_t(W) * H;
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) o[(i + h) + (j + h) * W] = cells[b * nb + j * nx + i];
+    const long scale = 1L << (L - B.key.level);
+    for (int j = 1 - h; j <= ny + h; ++j)
+      for (int i = 1 - h; i <= nx + h; ++i) {
+        if (i >= 1 && i <= nx && j >= 1 && j <= ny) continue;
+        const long xb = 4 * (long(B.key.i) * nx + i - 1) * scale;
+        const long yb = 4 * (long(B.key.j) * ny + j - 1) * scale;
+        bool ox, oy;
+        const long cx = wrap(xb + 2 * scale, quarter[0], M.periodic[0], ox);
+        const long cy = wrap(yb + 2 * scale, quarter[1], M.periodic[1], oy);
+        double v;
+        if (ox || oy) {
+          v = physical_values ? physical_values[nphys] : cells[cell_at(cx / 4, cy / 4)];
+          ++nphys;
+        } else {
+          int donor[4];
+          for (int smp = 0; smp < 4; ++smp) {
+            const int qx = (smp % 2 == 0) ? 1 : 3, qy = smp < 2 ? 1 : 3;
+            bool dummy;
+            const long x = wrap(xb + qx * scale, quarter[0], M.periodic[0], dummy);
+            const long y = wrap(yb + qy * scale, quarter[1], M.periodic[1], dummy);
+            donor[smp] = cell_at(x / 4, y / 4);
+          }
+          v = 0.0;
+          for (int d = 0; d < 4; ++d) {
+            bool seen = false;
+            for (int e = 0; e < d; ++e) seen |= donor[e] == donor[d];
+            if (seen) continue;
+            int mult = 0;
+            for (int e = 0; e < 4; ++e) mult += donor[e] == donor[d];
+            v += (double(mult) / 4) * cells[donor[d]];
+          }
+        }
+        o[(i - 1 + h) + (j - 1 + h) * W] = v;
+      }
+  }
+  return nphys;
+}
+
 }  // namespace orc

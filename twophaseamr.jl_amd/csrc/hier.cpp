@@ -959,4 +959,79 @@ void refresh_coefficients(Hierarchy& H, const double* kappa) {
   coarse_inverse(H);
 }
 
+// Halo schedule of `_build_halo_schedule` (src/mesh/topology/mesh.jl:944-1037):
+// a halo position of a block at level l covers a square of 2^(L-l) finest
+// lattice cells (L the finest leaf level); its value is the average over the
+// four quarter points of that square of the leaf cells containing them
+// (same-level copy, coarse-to-fine injection, fine-to-coarse average; the
+// periodic axes wrap). A position whose centre lies outside a non-periodic
+// side is a physical entry fed by the leaf cell at the clamped centre.
+HaloSchedule halo_schedule(const TreeMesh& m, int h) {
+  if (h < 1) throw std::invalid_argument("halo must be positive");
+  const int nx = m.bs[0], ny = m.bs[1], nb = nx * ny, W = nx + 2 * h, H = ny + 2 * h;
+  int L = 0;
+  for (const BlockKey& k : m.blocks) L = std::max(L, k.level);
+  const long qp[2] = {4L * m.layout[0] * nx << L, 4L * m.layout[1] * ny << L};  // quarter-point periods
+  // leaf cell of finest-lattice cell (px, py): the ancestor keys at every
+  // level, finest first (a block's neighbours are within one level)
+  auto leaf_cell = [&](long px, long py) {
+    for (int l = L; l >= 0; --l) {
+      const long cx = px >> (L - l), cy = py >> (L - l);
+      auto it = m.block_of.find(key64(l, int(cx / nx), int(cy / ny)));
+      if (it != m.block_of.end()) return it->second * nb + int(cy % ny) * nx + int(cx % nx);
+    }
+    throw std::logic_error("halo point outside the mesh");
+  };
+  auto fold = [&](long v, int axis, bool& out) {
+    const long p = qp[axis];
+    if (m.periodic[axis]) {
+      out = false;
+      v %= p;
+      return v < 0 ? v + p : v;
+    }
+    out = v < 0 || v >= p;
+    return v < 0 ? 0 : (v >= p ? p - 1 : v);
+  };
+  HaloSchedule S;
+  S.halo = h;
+  S.ptr.push_back(0);
+  for (int b = 0; b < int(m.blocks.size()); ++b) {
+    const BlockKey& k = m.blocks[b];
+    const long sc = 1L << (L - k.level);
+    for (int j = -h; j < ny + h; ++j)
+      for (int i = -h; i < nx + h; ++i) {
+        if (i >= 0 && i < nx && j >= 0 && j < ny) continue;
+        S.dest.push_back(b * W * H + (j + h) * W + (i + h));
+        const long x0 = 4 * (long(k.i) * nx + i) * sc, y0 = 4 * (long(k.j) * ny + j) * sc;
+        bool ox, oy;
+        const long cx = fold(x0 + 2 * sc, 0, ox), cy = fold(y0 + 2 * sc, 1, oy);
+        if (ox || oy) {
+          S.donor.push_back(leaf_cell(cx / 4, cy / 4));
+          S.weight.push_back(1.0);
+          S.phys.push_back(S.nphysical++);
+        } else {
+          // quarter points (1,1), (3,1), (1,3), (3,3); repeated cells merge
+          // (first occurrence order) with weight multiplicity / 4
+          int cells[4], mult[4] = {0, 0, 0, 0}, n = 0;
+          for (int q = 0; q < 4; ++q) {
+            bool o;
+            const long px = fold(x0 + (q & 1 ? 3 : 1) * sc, 0, o), py = fold(y0 + (q & 2 ? 3 : 1) * sc, 1, o);
+            const int c = leaf_cell(px / 4, py / 4);
+            int at = 0;
+            while (at < n && cells[at] != c) ++at;
+            if (at == n) cells[n++] = c;
+            ++mult[at];
+          }
+          for (int q = 0; q < n; ++q) {
+            S.donor.push_back(cells[q]);
+            S.weight.push_back(mult[q] / 4.0);
+          }
+          S.phys.push_back(-1);
+        }
+        S.ptr.push_back(int(S.donor.size()));
+      }
+  }
+  return S;
+}
+
 }  // namespace tpamr

@@ -119,6 +119,23 @@ struct Hierarchy {
   double total_volume = 0;
 };
 
+// Halo refresh schedule of a haloed cell field (fill_halo!, halo width h):
+// per block the (bx + 2h) x (by + 2h) array with the x index fastest; every
+// halo position is an entry with donor cells and weights (ptr / donor /
+// weight), or a physical-boundary entry (one donor: the nearest interior
+// cell; phys = its rank among the physical entries, -1 otherwise). Entries
+// are in the reference's schedule order.
+struct HaloSchedule {
+  int halo = 0;
+  std::vector<int> dest;  // offset in the haloed output
+  std::vector<int> ptr, donor;
+  std::vector<double> weight;
+  std::vector<int> phys;
+  int nphysical = 0;
+};
+
+HaloSchedule halo_schedule(const TreeMesh& m, int halo);
+
 // Largest coarsest level (cells) of the dense coarse solve.
 constexpr int COARSE_MAX = 8192;
 

@@ -611,6 +611,36 @@ __global__ void k_shift_residual(int n, int nb_cells, double* __restrict__ b, do
   }
 }
 
+// Haloed cell field: interior values (cell c of block b to position
+// (i + h, j + h) of the block's (bx + 2h) x (by + 2h) array) ...
+__global__ void k_halo_interior(int n, int bx, int by, int h, const double* __restrict__ cells,
+                                double* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int nb = bx * by, b = c / nb, l = c - b * nb, j = l / bx, i = l - j * bx;
+  const int W = bx + 2 * h, H = by + 2 * h;
+  out[size_t(b) * W * H + (j + h) * W + (i + h)] = cells[c];
+}
+
+// ... and the halo positions: the weighted donor sum in schedule order, or a
+// physical entry (the nearest interior cell, or a prescribed value).
+__global__ void k_halo_fill(int ne, const int* __restrict__ dest, const int* __restrict__ ptr,
+                            const int* __restrict__ donor, const double* __restrict__ weight,
+                            const int* __restrict__ phys, const double* __restrict__ cells,
+                            const double* __restrict__ physical, double* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int p0 = ptr[e], p1 = ptr[e + 1], k = phys[e];
+  double v;
+  if (k >= 0) {
+    v = physical ? physical[k] : cells[donor[p0]];
+  } else {
+    v = 0.0;
+    for (int q = p0; q < p1; ++q) v += weight[q] * cells[donor[q]];
+  }
+  out[dest[e]] = v;
+}
+
 // x -= shift (scalar from device memory), optionally scaled by the cell volume
 __global__ void k_shift(int n, int nb_cells, double* __restrict__ x, const double* __restrict__ shift, double scale,
                         const double* __restrict__ volume, int by_volume) {
@@ -931,6 +961,15 @@ void launch_residual_strong(hipStream_t s, int n, int nbc, const double* b, cons
 
 void launch_weighted_sums(hipStream_t s, int n, int nbc, const double* x, const double* vol, double* part) {
   hipLaunchKernelGGL(k_weighted_sums, dim3(cdiv(n, 256)), dim3(256), 0, s, n, nbc, x, vol, part);
+}
+
+void launch_fill_halo(hipStream_t s, int n, int bx, int by, int h, int ne, const int* dest, const int* ptr,
+                      const int* donor, const double* weight, const int* phys, const double* cells,
+                      const double* physical, double* out) {
+  hipLaunchKernelGGL(k_halo_interior, dim3(cdiv(n, 256)), dim3(256), 0, s, n, bx, by, h, cells, out);
+  if (ne > 0)
+    hipLaunchKernelGGL(k_halo_fill, dim3(cdiv(ne, 256)), dim3(256), 0, s, ne, dest, ptr, donor, weight, phys, cells,
+                       physical, out);
 }
 
 void launch_rhs_prepare(hipStream_t s, int n, int nbc, double* b, const double* x, const double* vol, double* part) {

@@ -137,6 +137,10 @@ void launch_fill_f32(hipStream_t s, int n, float v, float* out);
 void launch_residual_strong(hipStream_t s, int n, int nbc, const double* b, const double* y, double* r,
                             const double* vol, double* part);
 void launch_weighted_sums(hipStream_t s, int n, int nbc, const double* x, const double* vol, double* part);
+// haloed cell field: interior copy and the ne halo entries of a schedule
+void launch_fill_halo(hipStream_t s, int n, int bx, int by, int h, int ne, const int* dest, const int* ptr,
+                      const int* donor, const double* weight, const int* phys, const double* cells,
+                      const double* physical, double* out);
 // b <- V b and partial sums of |b|, b, V x, |x| (four sets of ceil(n/256))
 void launch_rhs_prepare(hipStream_t s, int n, int nbc, double* b, const double* x, const double* vol, double* part);
 // b -= V mean, r = b, partial strong maxima and squared-norm sums of r

@@ -126,6 +126,11 @@ struct tpamr_ctx {
   DevArray<signed char> faxis, csign, bside;
   DevArray<double> farea, kdiag, kval, kappa, sqk, spacing, barea;
   DevArray<double> f1, f2, f3, f4;
+  // halo refresh schedule (built on first use for a halo width)
+  HaloSchedule halo_host;
+  int halo_dev = 0;  // halo width of the uploaded schedule (0: none)
+  DevArray<int> h_dest, h_ptr, h_donor, h_phys;
+  DevArray<double> h_weight, h_out, h_phys_values;
   // graph of one Krylov iteration (TPAMR_GRAPH=0 in the environment launches
   // the iteration's kernels one by one instead)
   bool use_graph = true;
@@ -342,6 +347,8 @@ void upload_hierarchy(tpamr_ctx* c, bool topology) {
     c->b.alloc(N);
     c->tmp.alloc(N);
     c->tmp2.alloc(N);
+    c->halo_host = HaloSchedule{};
+    c->halo_dev = 0;
     if (c->precision == TPAMR_FP32_REFINE) {
       c->p32.alloc(N);
       c->Ap32.alloc(N);
@@ -1369,6 +1376,65 @@ int tpamr_adjoint(tpamr_ctx* c, const double* w, const double* p, double rtol, i
     launch_adjoint_face(s, F, c->f1.p, c->f4.p, c->f2.p, c->f3.p, c->kappa.p, c->f1.p);
     download_vec(c, dJ_dkappa, c->f1.p, F);
     if (stats) *stats = st;
+  });
+}
+
+int tpamr_halo_sizes(tpamr_ctx* c, int halo, int* entries, int* physical) {
+  if (!c || !entries || !physical) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    if (!c->H) throw StateError("no hierarchy: call tpamr_set_face_coefficients first");
+    if (c->halo_host.halo != halo) c->halo_host = halo_schedule(c->mesh(), halo);
+    *entries = int(c->halo_host.dest.size());
+    *physical = c->halo_host.nphysical;
+  });
+}
+
+int tpamr_halo_schedule(tpamr_ctx* c, int halo, int* dest, int* ptr, int* donor, double* weight, int* phys) {
+  if (!c || !dest || !ptr || !donor || !weight || !phys) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    if (!c->H) throw StateError("no hierarchy: call tpamr_set_face_coefficients first");
+    if (c->halo_host.halo != halo) c->halo_host = halo_schedule(c->mesh(), halo);
+    const HaloSchedule& S = c->halo_host;
+    std::copy(S.dest.begin(), S.dest.end(), dest);
+    std::copy(S.ptr.begin(), S.ptr.end(), ptr);
+    std::copy(S.donor.begin(), S.donor.end(), donor);
+    std::copy(S.weight.begin(), S.weight.end(), weight);
+    std::copy(S.phys.begin(), S.phys.end(), phys);
+  });
+}
+
+int tpamr_fill_halo(tpamr_ctx* c, int halo, const double* cells, const double* physical, double* haloed) {
+  if (!c || !cells || !haloed) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    if (!c->H) throw StateError("no hierarchy: call tpamr_set_face_coefficients first");
+    HIPCHECK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const TreeMesh& M = c->mesh();
+    if (c->halo_dev != halo) {
+      if (c->halo_host.halo != halo) c->halo_host = halo_schedule(M, halo);
+      const HaloSchedule& S = c->halo_host;
+      c->h_dest.upload(S.dest.empty() ? std::vector<int>(1, 0) : S.dest, s);
+      c->h_ptr.upload(S.ptr, s);
+      c->h_donor.upload(S.donor.empty() ? std::vector<int>(1, 0) : S.donor, s);
+      c->h_weight.upload(S.weight.empty() ? std::vector<double>(1, 0.0) : S.weight, s);
+      c->h_phys.upload(S.phys.empty() ? std::vector<int>(1, -1) : S.phys, s);
+      c->halo_dev = halo;
+    }
+    const HaloSchedule& S = c->halo_host;
+    const int N = M.ncells(), bx = M.bs[0], by = M.bs[1];
+    const size_t total = size_t(M.blocks.size()) * (bx + 2 * halo) * (by + 2 * halo);
+    c->h_out.alloc(total);
+    upload_vec(c, c->tmp.p, cells, N);
+    const double* dphys = nullptr;
+    if (physical && S.nphysical > 0) {
+      c->h_phys_values.alloc(S.nphysical);
+      upload_vec(c, c->h_phys_values.p, physical, S.nphysical);
+      dphys = c->h_phys_values.p;
+    }
+    launch_fill_halo(s, N, bx, by, halo, int(S.dest.size()), c->h_dest.p, c->h_ptr.p, c->h_donor.p, c->h_weight.p,
+                     c->h_phys.p, c->tmp.p, dphys, c->h_out.p);
+    download_vec(c, haloed, c->h_out.p, total);
   });
 }
 

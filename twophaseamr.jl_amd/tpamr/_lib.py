@@ -83,6 +83,10 @@ _SIGNATURES = {
     "tpamr_weighted_jump_force": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "tpamr_divergence": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "tpamr_weighted_gradient": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "tpamr_halo_sizes": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "tpamr_halo_schedule": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
+    "tpamr_fill_halo": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "tpamr_project": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int,
                                 C.c_void_p, C.POINTER(Stats)]),
     "tpamr_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p,

@@ -8,6 +8,7 @@ Reference interfaces (TwoPhaseAMR.jl):
   divergence!                                                   src/operators/advection/cell_advection.jl:205-220
   weighted_gradient!                                            src/operators/elliptic.jl:1272-1297
   project! / _project_prepared!                                 src/navier_stokes/projection.jl:189-260
+  fill_halo! / _build_halo_schedule                             src/mesh/variables.jl:205-229, mesh.jl:944-1037
 The adjoint (`pressure_solve_rrule`) has no reference counterpart.
 """
 from __future__ import annotations
@@ -131,3 +132,42 @@ def pressure_solve_rrule(solver: EllipticSolver, rhs, *, rtol=1e-12, max_cycles=
         return db, dk
 
     return p, pullback
+
+
+def halo_schedule(ctx: L.Context, halo=1):
+    """The halo refresh schedule of `_build_halo_schedule`
+    (src/mesh/topology/mesh.jl:944-1037) as built by the library: per entry
+    its offset in the haloed output, donor cells with weights (CSR) and its
+    rank among the physical-boundary entries (-1 otherwise)."""
+    ne, nphys = C.c_int(), C.c_int()
+    L.check(L.lib().tpamr_halo_sizes(ctx.h, int(halo), C.byref(ne), C.byref(nphys)), ctx.h)
+    dest = np.zeros(ne.value, np.int32)
+    ptr = np.zeros(ne.value + 1, np.int32)
+    phys = np.zeros(ne.value, np.int32)
+    probe_n = max(ne.value * 4, 1)
+    donor = np.zeros(probe_n, np.int32)
+    weight = np.zeros(probe_n)
+    L.check(L.lib().tpamr_halo_schedule(ctx.h, int(halo), L.ptr(dest), L.ptr(ptr), L.ptr(donor), L.ptr(weight),
+                                        L.ptr(phys)), ctx.h)
+    nd = int(ptr[-1])
+    return dict(dest=dest, ptr=ptr, donor=donor[:nd], weight=weight[:nd], phys=phys, nphysical=nphys.value)
+
+
+def fill_halo(solver: EllipticSolver, cells, halo=1, physical=None):
+    """fill_halo!(field, ZeroGradient()) or, with `physical` (one value per
+    physical halo entry in schedule order), fill_halo!(field, Dirichlet(f))
+    (src/mesh/variables.jl:205-229) on the GPU: returns per block the haloed
+    array, shape (nblocks, by + 2 halo, bx + 2 halo) (CellField.blocks with the
+    x index fastest)."""
+    m = solver.operator.mesh
+    bx, by = m.block_size
+    cells = L.f64(cells, m.ncells, "cell values")
+    out = np.zeros((m.nblocks, by + 2 * halo, bx + 2 * halo))
+    php = None
+    if physical is not None:
+        ne, nphys = C.c_int(), C.c_int()
+        L.check(L.lib().tpamr_halo_sizes(solver.ctx.h, int(halo), C.byref(ne), C.byref(nphys)), solver.ctx.h)
+        physical = L.f64(physical, nphys.value, "physical halo values")
+        php = L.ptr(physical)
+    L.check(L.lib().tpamr_fill_halo(solver.ctx.h, int(halo), L.ptr(cells), php, L.ptr(out)), solver.ctx.h)
+    return out
