@@ -44,7 +44,7 @@ sys.path.insert(0, os.path.join(ROOT, "twophaseamr.jl_amd"))
 
 METRIC = "Mcells/s to rel. residual 1e-10, FP32-MG vs FP64 GPU solver; smoother HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-SWEEP_KERNEL = "k_sweep<float, 16, 16, true, false>"
+SWEEP_KERNEL = "k_sweep<float, 16, 16, true, false, double>"
 COPY_BYTES = 1 << 28  # per direction, calibration copy of the PMC probe
 TRAFFIC_MD = os.path.join(ROOT, "profiles", "k_sweep_fp32_circles64_levels5-10_traffic.md")
 # precision modes solved on the workload; the headline value is the first
