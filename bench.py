@@ -257,6 +257,95 @@ def committed_traffic():
     return None
 
 
+def wave_main(args):
+    """Stokes-wave-like run: [0, 2] x [0, 1], 32x32 blocks, levels 5-11 near
+    the wave surface, density (1000, 1.2), gravity (0, -9.81); `args.steps`
+    steps (default 10) of gravity! plus project! with the interface at each
+    step's time (dt = 1e-4, the surface following its linear standing-wave
+    phase), pressures checkpointed, J = kinetic energy after the last step
+    and its gradient with respect to the first step's pressure rhs by the
+    adjoint run. The forward run is timed in the headline mode and in
+    all-FP64; `value` is the headline mode's solve throughput (cells x steps
+    / summed device solve time)."""
+    import torch
+    import tpamr
+    from tpamr import workloads as W
+    t0 = time.perf_counter()
+    mesh = W.wave_mesh(tuple(args.levels))
+    log(f"wave: {mesh.ncells} cells, {mesh.nblocks} blocks ({time.perf_counter() - t0:.1f} s)")
+    nsteps, dt = args.steps, 1e-4
+    u0 = 0.01 * W.smooth_velocity(mesh, device="cuda:0")
+    steps = frac_end = None
+    res, runs = {}, {}
+    for prec in (HEADLINE, "fp64"):
+        t1 = time.perf_counter()
+        s = tpamr.EllipticSolver(tpamr.PoissonOperator(mesh, 1.0), precision=prec)
+        log(f"{prec}: hierarchy {[lv['ncells'] for lv in s.levels]} ({time.perf_counter() - t1:.1f} s)")
+        if steps is None:
+            t1 = time.perf_counter()
+            steps, frac_end = W.wave_steps(s, nsteps, dt)
+            log(f"interface data of {nsteps} steps ({time.perf_counter() - t1:.1f} s)")
+        run = tpamr.ProjectionRun(s, dt, rtol=args.rtol, adjoint_rtol=args.rtol)
+        t1 = time.perf_counter()
+        uN = run.forward(u0, steps)
+        wall = time.perf_counter() - t1
+        solve_s = sum(st.solve_ms for st in run.stats) * 1e-3
+        J, ub = tpamr.kinetic_energy(mesh, uN, frac_end, W.WAVE_DENSITY)
+        r = dict(mcells=mesh.ncells * nsteps / solve_s / 1e6, solve_ms_per_step=solve_s / nsteps * 1e3,
+                 wall_s=wall, cycles=[st.cycles for st in run.stats],
+                 converged=all(st.converged for st in run.stats), J=J,
+                 refresh_s_per_step=float(np.mean([t["refresh_s"] for t in run.timings])),
+                 project_s_per_step=float(np.mean([t["project_s"] for t in run.timings])))
+        log(f"{prec}: forward {r}")
+        if prec == HEADLINE:
+            t1 = time.perf_counter()
+            g = run.backward(ub)
+            r["adjoint_wall_s"] = time.perf_counter() - t1
+            ast = g["stats"]
+            r["adjoint_mcells"] = mesh.ncells * nsteps / (sum(st.solve_ms for st in ast) * 1e-3) / 1e6
+            r["adjoint_cycles"] = [st.cycles for st in ast]
+            r["adjoint_converged"] = all(st.converged for st in ast)
+            r["grad_rhs_l2"] = float(np.linalg.norm(g["rhs"]))
+            grad = g["rhs"]
+            log(f"{prec}: adjoint {r['adjoint_wall_s']:.1f} s, {r['adjoint_mcells']:.1f} Mcells/s")
+            runs[prec] = [c.copy() for c in run.checkpoints]
+            if args.wave_fd:
+                # J is quadratic in the rhs: central differences are exact for any step
+                d = W.smooth_velocity(mesh, seed=W.SEED + 3, device="cuda:0")
+                d = tpamr.divergence(s, d)
+                eps = float(np.max(np.abs(tpamr.divergence(s, u0)))) / dt / float(np.max(np.abs(d)))
+                run.rtol = 1e-12
+                jp = tpamr.kinetic_energy(mesh, run.forward(u0, steps, rhs_shift=eps * d), frac_end,
+                                          W.WAVE_DENSITY)[0]
+                jm = tpamr.kinetic_energy(mesh, run.forward(u0, steps, rhs_shift=-eps * d), frac_end,
+                                          W.WAVE_DENSITY)[0]
+                fd = (jp - jm) / (2 * eps)
+                r["fd_check"] = dict(fd=fd, adjoint=float(grad @ d), rel=abs(fd - grad @ d) / abs(fd))
+                log(f"{prec}: finite differences {r['fd_check']}")
+        if prec not in runs:
+            runs[prec] = [c.copy() for c in run.checkpoints]
+        res[prec] = r
+        s.close()
+        torch.cuda.empty_cache()
+    perr = max(float(np.max(np.abs(a - b)) / np.max(np.abs(b))) for a, b in zip(runs[HEADLINE], runs["fp64"]))
+    head, f64 = res[HEADLINE], res["fp64"]
+    out = {
+        "metric": METRIC, "value": head["mcells"], "unit": "Mcells/s", "n_gpus": 1, "steps": nsteps,
+        "warmup": 0, "ms_per_step": head["solve_ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 V-cycle and inner CG vectors / f64 refinement, inner products",
+        "data": "synthetic",
+        "config": {"workload": f"stokes_wave_levels{args.levels[0]}-{args.levels[1]}_rho1000-1.2",
+                   "cells": mesh.ncells, "blocks": mesh.nblocks, "block_size": list(mesh.block_size),
+                   "dt": dt, "rtol": args.rtol, "precision": HEADLINE},
+        "fp64_solver": f64, "headline": head,
+        "speedup_vs_fp64": head["mcells"] / f64["mcells"],
+        "checkpoint_pressure_error_vs_fp64": perr,
+        "kinetic_energy_rel_diff_vs_fp64": abs(head["J"] - f64["J"]) / abs(f64["J"]),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -269,6 +358,9 @@ def main():
     ap.add_argument("--probe", action="store_true", help="short run for rocprofv3 --pmc passes")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DB", "WRITE_DB"))
     ap.add_argument("--trace-summary", nargs=3, metavar=("DB", "TITLE", "COMMAND"))
+    ap.add_argument("--workload", choices=("circles", "wave"), default="circles",
+                    help="circles: the headline solve benchmark; wave: the checkpointed multi-step run")
+    ap.add_argument("--wave-fd", action="store_true", help="wave: one finite-difference check of the gradient")
     args = ap.parse_args()
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary)
@@ -282,6 +374,13 @@ def main():
         sys.exit(2)
     if args.probe:
         probe(args.levels)
+        return
+    if args.workload == "wave":
+        if "--levels" not in sys.argv:
+            args.levels = (5, 11)
+        if "--steps" not in sys.argv:
+            args.steps = 10
+        wave_main(args)
         return
     import torch
 

@@ -143,6 +143,23 @@ int tpamr_surface_tension_jumps(tpamr_ctx* ctx, const double* fraction, const do
 /* _weighted_jump_force! (forces.jl:179-209): accel = sqrt(k) K (sqrt(k) jumps) */
 int tpamr_weighted_jump_force(tpamr_ctx* ctx, const double* jumps, double* accel);
 
+/* gravity! (forces.jl:528-580) face jumps rho_f |f| (phi+ - phi-): potential
+ * phi = gx x + gy y at the cell centres, moved in mixed cells to the
+ * interface centroid; `offsets` (2 per cell, NULL = none) hold that centroid
+ * relative to the cell centre in units of the cell spacing (the reference's
+ * _interface_centroid of the reconstruction, zero in pure cells); rho_f =
+ * rho_l a_l + rho_g a_g with the phase face apertures of the reconstruction
+ * (_current_phase_face_apertures!, two_velocity/transport_geometry.jl:491-515),
+ * one value per interior face each. Nonzero gravity along a periodic axis is
+ * rejected. */
+int tpamr_gravity_jumps(tpamr_ctx* ctx, const double* offsets, const double* liquid_aperture,
+                        const double* gas_aperture, double rho_liquid, double rho_gas, double gx, double gy,
+                        double* jumps);
+/* gravity! face accelerations: the jumps above through _weighted_jump_force!
+ * (forces.jl:179-209), accel = sqrt(k) K (sqrt(k) jumps) */
+int tpamr_gravity(tpamr_ctx* ctx, const double* offsets, const double* liquid_aperture, const double* gas_aperture,
+                  double rho_liquid, double rho_gas, double gx, double gy, double* accel);
+
 /* divergence! (operators/advection/cell_advection.jl:205-220); u_boundary may be NULL */
 int tpamr_divergence(tpamr_ctx* ctx, const double* u, const double* u_boundary, double* div);
 /* weighted_gradient! interior faces (elliptic.jl:1272-1297) */
@@ -175,6 +192,23 @@ int tpamr_project(tpamr_ctx* ctx, double* u, double* p, double dt, double tolera
  * dJ/dkappa per interior face; `p` is the stored forward solution. */
 int tpamr_adjoint(tpamr_ctx* ctx, const double* w, const double* p, double rtol, int max_cycles, double* dJ_drhs,
                   double* dJ_dkappa, tpamr_stats* stats);
+
+/* Adjoint of one projection step (no reference counterpart). The forward
+ * step is tpamr_project from u* = u + dt a (a = the step's force
+ * accelerations sqrt(k) K (sqrt(k) jumps)): u' = u* - dt G phi with
+ * A phi = V b, b = -div(u*) / dt. Given ubar = dJ/du' (interior faces) it
+ * solves A lambda = J^T M ubar (J x = |f| (x+ - x-), M = sqrt(k) K sqrt(k))
+ * with the current operator and writes
+ *   ubar_star = dJ/du* = ubar - J lambda            (interior faces),
+ *   dJ_drhs   = dJ/db  = -dt V lambda               (cells; NULL: skipped),
+ *   dJ_dkappa = dJ/dk_f = dt [W_f (K Z)_f + Z_f (K W)_f] / (2 k_f)
+ *               with W = sqrt(k) ubar_star, Z = sqrt(k) (jumps - J phi)
+ *               (faces; NULL: skipped; needs the checkpointed phi of the
+ *               step, `jumps` NULL = no force).
+ * rtol is the relative tolerance of the adjoint solve. */
+int tpamr_project_adjoint(tpamr_ctx* ctx, const double* ubar, const double* phi, const double* jumps, double dt,
+                          double rtol, int max_cycles, double* ubar_star, double* dJ_drhs, double* dJ_dkappa,
+                          tpamr_stats* stats);
 
 /* Timing of the finest-level smoother kernel (HIP events, `reps` launches on
  * the context stream); ms per launch in *avg_ms and its algorithmic bytes

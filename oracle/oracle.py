@@ -184,6 +184,16 @@ class OMesh:
                                     C.c_double(sigma), int(csf), _dptr(out)))
         return out
 
+    def gravity_jumps(self, liquid, gas, density, gravity, offsets=None):
+        """gravity! face jumps (src/two_phase/forces.jl:528-580)."""
+        out = np.zeros(self.nfaces)
+        off = None if offsets is None else _dptr(np.ascontiguousarray(offsets, np.float64))
+        _check(lib().orc_gravity_jumps(self._h, off, _dptr(np.ascontiguousarray(liquid, np.float64)),
+                                       _dptr(np.ascontiguousarray(gas, np.float64)), C.c_double(density[0]),
+                                       C.c_double(density[1]), C.c_double(gravity[0]), C.c_double(gravity[1]),
+                                       _dptr(out)))
+        return out
+
     def fill_halo(self, cells, halo=1, physical=None):
         """fill_halo! (src/mesh/variables.jl:205-229): per block the haloed
         array, shape (nblocks, by + 2 halo, bx + 2 halo); `physical`: one value

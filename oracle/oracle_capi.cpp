@@ -14,6 +14,8 @@ void apply_vcycle(Solver& S, const double* r, double* z);
 Stats solve(Solver& S, const double* rhs, double* xio, double rtol, double atol, int max_cycles,
             int compat_error, std::vector<double>* removed_means);
 void mixture_coefficients(const Mesh& M, const double* c, double rho_l, double rho_g, double* kappa);
+void gravity_jumps(const Mesh& M, const double* offsets, const double* liquid, const double* gas, double rho_l,
+                   double rho_g, double gx, double gy, double* jumps);
 void cgfm_jumps(const Mesh& M, const double* c, const double* curvature, double sigma, int csf,
                 double* jumps);
 void weighted_jump_force(const Operator& op, const double* jumps, double* accel);
@@ -205,6 +207,11 @@ int orc_mixture_coefficients(void* mp, const double* c, double rho_l, double rho
 
 int orc_cgfm_jumps(void* mp, const double* c, const double* curvature, double sigma, int csf, double* out) {
   GUARD({ cgfm_jumps(*static_cast<Mesh*>(mp), c, curvature, sigma, csf, out); return 0; })
+}
+
+int orc_gravity_jumps(void* mp, const double* offsets, const double* liquid, const double* gas, double rho_l,
+                      double rho_g, double gx, double gy, double* jumps) {
+  GUARD({ gravity_jumps(*static_cast<Mesh*>(mp), offsets, liquid, gas, rho_l, rho_g, gx, gy, jumps); return 0; })
 }
 
 int orc_weighted_jump_force(void* sp, const double* jumps, double* accel) {

@@ -911,6 +911,41 @@ void cgfm_jumps(const Mesh& M, const double* c, const double* curvature, double 
   }
 }
 
+// `gravity!` jumps (forces.jl:528-580): potential g . x at the cell centre
+// (`_leaf_cell_center`, mesh.jl:1142-1145), moved by the interface centroid
+// offset (in cell spacings; offsets == nullptr: none), then per face
+// rho |f| (phi+ - phi-) with rho = (rho_l a_l + rho_g a_g) / open, open = 1.
+void gravity_jumps(const Mesh& M, const double* offsets, const double* liquid, const double* gas, double rho_l,
+                   double rho_g, double gx, double gy, double* jumps) {
+  if (!(std::isfinite(rho_l) && rho_l > 0 && std::isfinite(rho_g) && rho_g > 0))
+    throw std::invalid_argument("densities must be finite and positive");
+  if (!(std::isfinite(gx) && std::isfinite(gy))) throw std::invalid_argument("gravity must be finite");
+  if ((M.periodic[0] && gx != 0) || (M.periodic[1] && gy != 0))
+    throw std::invalid_argument("gravity potential requires nonperiodic boundaries along gravity");
+  const int bx = M.bs[0], by = M.bs[1], nb = bx * by;
+  std::vector<double> phi(M.ncells());
+  for (size_t b = 0; b < M.blocks.size(); ++b) {
+    const Block& B = M.blocks[b];
+    for (int j = 1; j <= by; ++j)
+      for (int i = 1; i <= bx; ++i) {
+        const size_t cell = b * nb + size_t(j - 1) * bx + (i - 1);
+        double x = B.origin[0] + (i - 0.5) * B.spacing[0];
+        double y = B.origin[1] + (j - 0.5) * B.spacing[1];
+        if (offsets) {
+          x += offsets[2 * cell] * B.spacing[0];
+          y += offsets[2 * cell + 1] * B.spacing[1];
+        }
+        phi[cell] = gx * x + gy * y;
+      }
+  }
+  for (size_t f = 0; f < M.faces.size(); ++f) {
+    const Face& fa = M.faces[f];
+    const double open = 1.0;
+    const double rho = (rho_l * liquid[f] + rho_g * gas[f]) / open;
+    jumps[f] = rho * fa.area * (phi[fa.plus] - phi[fa.minus]);
+  }
+}
+
 static void mul_metric(const Operator& op, const std::vector<double>& in, std::vector<double>& out) {
   const int F = int(in.size());
   out.assign(F, 0.0);

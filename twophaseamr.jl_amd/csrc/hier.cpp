@@ -80,6 +80,25 @@ void block_geometry(const TreeMesh& M, const BlockKey& k, double* spacing) {
   spacing[1] = w1 / M.bs[1];
 }
 
+// Lower-left corner of block k: the root rectangle's corner
+// (`_build_root_grid`, src/mesh/topology/mesh.jl:736-757), then per level the
+// child rule corner + c * width / 2, as `_leaf_blocks` (mesh.jl:759-773) reads
+// it from the tree nodes.
+void block_origin(const TreeMesh& M, const BlockKey& k, double* origin) {
+  const int ri = k.i >> k.level, rj = k.j >> k.level;
+  double o0 = ri == 0 ? M.origin[0] : M.origin[0] + ri * M.root_extent[0];
+  double o1 = rj == 0 ? M.origin[1] : M.origin[1] + rj * M.root_extent[1];
+  double w0 = M.root_extent[0], w1 = M.root_extent[1];
+  for (int l = 1; l <= k.level; ++l) {
+    w0 /= 2;
+    w1 /= 2;
+    o0 += ((k.i >> (k.level - l)) & 1) * w0;
+    o1 += ((k.j >> (k.level - l)) & 1) * w1;
+  }
+  origin[0] = o0;
+  origin[1] = o1;
+}
+
 // Face schedule in the reference order: axis-major, then lattice line, then
 // increasing tangential position; boundary faces per line after the
 // interior faces of that line.
@@ -213,11 +232,13 @@ std::unique_ptr<TreeMesh> make_tree_mesh(const int base[2], const int bs[2], con
   });
   M->block_of.reserve(M->blocks.size() * 2);
   M->bspacing.resize(2 * M->blocks.size());
+  M->borigin.resize(2 * M->blocks.size());
   M->max_leaf = 0;
   for (int b = 0; b < int(M->blocks.size()); ++b) {
     const BlockKey& k = M->blocks[b];
     M->block_of[key64(k.level, k.i, k.j)] = b;
     block_geometry(*M, k, &M->bspacing[2 * b]);
+    block_origin(*M, k, &M->borigin[2 * b]);
     M->max_leaf = std::max(M->max_leaf, k.level);
   }
   build_faces(*M);
@@ -808,7 +829,9 @@ void transfer_descriptors(LevelData& Fl, const LevelData& C, bool agglomerated) 
   }
 }
 
-void coarse_inverse(Hierarchy& H) {
+// Dense augmented coarsest operator [A V; V^T 0] (one connected component,
+// no anchors), row-major N x N with N = n + 1, into H.coarse_aug.
+void coarse_matrix(Hierarchy& H) {
   LevelData& L = *H.levels.back();
   const TreeMesh& M = *L.mesh;
   const int n = L.ncells, nb = M.nb(), bx = M.bs[0], by = M.bs[1];
@@ -820,7 +843,9 @@ void coarse_inverse(Hierarchy& H) {
     throw std::invalid_argument("coarsest multigrid level has " + std::to_string(n) +
                                 " cells; the dense coarse solve supports at most " + std::to_string(COARSE_MAX) +
                                 " (use a root-block layout that halves down to a few blocks)");
-  std::vector<double> a(size_t(N) * N, 0.0);
+  H.coarse_n = n;
+  std::vector<double>& a = H.coarse_aug;
+  a.assign(size_t(N) * N, 0.0);
   for (int b = 0; b < L.nblocks; ++b)
     for (int q = 0; q < nb; ++q) {
       const int c = b * nb + q;
@@ -850,53 +875,6 @@ void coarse_inverse(Hierarchy& H) {
       a[size_t(c) * N + n] = V;
       a[size_t(n) * N + c] = V;
     }
-  // LU with partial pivoting, then the top-left n x n block of the inverse
-  std::vector<int> piv(N);
-  for (int k = 0; k < N; ++k) {
-    int p = k;
-    double best = std::fabs(a[size_t(k) * N + k]);
-    for (int i = k + 1; i < N; ++i)
-      if (std::fabs(a[size_t(i) * N + k]) > best) {
-        best = std::fabs(a[size_t(i) * N + k]);
-        p = i;
-      }
-    piv[k] = p;
-    if (best == 0.0) throw std::runtime_error("singular coarse operator");
-    if (p != k)
-      for (int j = 0; j < N; ++j) std::swap(a[size_t(k) * N + j], a[size_t(p) * N + j]);
-    const double d = a[size_t(k) * N + k];
-    for (int i = k + 1; i < N; ++i) {
-      const double l = a[size_t(i) * N + k] / d;
-      a[size_t(i) * N + k] = l;
-      if (l == 0.0) continue;
-      double* ri = &a[size_t(i) * N];
-      const double* rk = &a[size_t(k) * N];
-      for (int j = k + 1; j < N; ++j) ri[j] -= l * rk[j];
-    }
-  }
-  H.coarse_n = n;
-  H.coarse_inv.assign(size_t(n) * n, 0.0);
-  std::vector<double> x(N);
-  for (int col = 0; col < n; ++col) {
-    std::fill(x.begin(), x.end(), 0.0);
-    x[col] = 1.0;
-    for (int k = 0; k < N; ++k)
-      if (piv[k] != k) std::swap(x[k], x[piv[k]]);
-    for (int i = 0; i < N; ++i) {
-      double s = x[i];
-      const double* ri = &a[size_t(i) * N];
-      for (int j = 0; j < i; ++j) s -= ri[j] * x[j];
-      x[i] = s;
-    }
-    for (int i = N - 1; i >= 0; --i) {
-      double s = x[i];
-      const double* ri = &a[size_t(i) * N];
-      for (int j = i + 1; j < N; ++j) s -= ri[j] * x[j];
-      x[i] = s / ri[i];
-    }
-    // inverse column `col`; stored column-major: inv[col * n + row]
-    for (int row = 0; row < n; ++row) H.coarse_inv[size_t(col) * n + row] = x[row];
-  }
 }
 
 }  // namespace
@@ -956,7 +934,7 @@ void refresh_coefficients(Hierarchy& H, const double* kappa) {
     }
     level_values(L);
   }
-  coarse_inverse(H);
+  coarse_matrix(H);
 }
 
 // Halo schedule of `_build_halo_schedule` (src/mesh/topology/mesh.jl:944-1037):

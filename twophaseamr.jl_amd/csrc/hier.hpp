@@ -35,6 +35,7 @@ struct TreeMesh {
   std::vector<BlockKey> blocks;           // leaves sorted by (level, j, i)
   std::unordered_map<uint64_t, int> block_of;
   std::vector<double> bspacing;  // 2 per block
+  std::vector<double> borigin;   // 2 per block: lower-left corner
   // interior faces, reference order
   std::vector<int> fminus, fplus;
   std::vector<int8_t> faxis;
@@ -113,9 +114,10 @@ struct LevelData {
 
 struct Hierarchy {
   std::vector<std::unique_ptr<LevelData>> levels;
-  // coarsest augmented inverse (n x n, column major)
+  // coarsest level: the dense augmented operator [A V; V^T 0] ((n + 1) x
+  // (n + 1), row major); the device inverts it
   int coarse_n = 0;
-  std::vector<double> coarse_inv;
+  std::vector<double> coarse_aug;
   double total_volume = 0;
 };
 

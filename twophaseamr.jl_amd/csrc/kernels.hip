@@ -675,7 +675,8 @@ __global__ void k_face_jump(int nf, const int* __restrict__ fm, const int* __res
                             const double* __restrict__ p, const double* __restrict__ jumps, double* __restrict__ out) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
-  out[f] = p ? sqk[f] * (area[f] * (p[fp[f]] - p[fm[f]])) : sqk[f] * jumps[f];
+  const double w = sqk ? sqk[f] : 1.0;
+  out[f] = p ? w * (area[f] * (p[fp[f]] - p[fm[f]])) : w * jumps[f];
 }
 
 __global__ void k_face_scale(int nf, const double* __restrict__ sqk, const double* __restrict__ in,
@@ -695,6 +696,141 @@ __global__ void k_mixture(int nf, int nb_cells, const int* __restrict__ fm, cons
   double cf = (hm * c[m] + hp * c[p]) / (hm + hp);
   cf = fmin(fmax(cf, 0.0), 1.0);
   kappa[f] = 1.0 / (rho_l * cf + rho_g * (1 - cf));
+}
+
+// Dense inverse of the augmented coarsest operator, Gauss-Jordan in place
+// with partial pivoting (rows swapped, pivot = first largest |a_ik|). Step k:
+// k_gj_pivot (one workgroup) picks and swaps the pivot row, scales it by
+// 1 / a_kk (a_kk <- 1 / a_kk) and copies column k (the multipliers); then
+// k_gj_update eliminates column k from every other row (a_ik <- -m_i / a_kk).
+// The row swaps come back as column swaps in reverse order (k_gj_unpermute).
+__global__ void __launch_bounds__(1024) k_gj_pivot(int N, int k, double* __restrict__ a, int* __restrict__ piv,
+                                                   double* __restrict__ col, int* __restrict__ fail) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  double best = -1.0;
+  int bi = N;
+  for (int i = k + t; i < N; i += 1024) {
+    const double v = fabs(a[size_t(i) * N + k]);
+    if (v > best) {
+      best = v;
+      bi = i;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ob = __shfl_down(best, off);
+    const int oi = __shfl_down(bi, off);
+    if (ob > best || (ob == best && oi < bi)) {
+      best = ob;
+      bi = oi;
+    }
+  }
+  __shared__ double sb[16];
+  __shared__ int si[16];
+  __shared__ int sp;
+  __shared__ double sv;
+  if (lane == 0) {
+    sb[w] = best;
+    si[w] = bi;
+  }
+  __syncthreads();
+  if (t == 0) {
+    for (int q = 1; q < 16; ++q)
+      if (sb[q] > sb[0] || (sb[q] == sb[0] && si[q] < si[0])) {
+        sb[0] = sb[q];
+        si[0] = si[q];
+      }
+    const bool ok = sb[0] > 0.0 && isfinite(sb[0]) && si[0] < N;
+    sp = ok ? si[0] : k;
+    sv = ok ? sb[0] : 0.0;
+    piv[k] = sp;
+    if (!ok) *fail = 1;
+  }
+  __syncthreads();
+  if (!(sv > 0.0)) return;
+  const int p = sp;
+  double* rk = a + size_t(k) * N;
+  if (p != k) {
+    double* rp = a + size_t(p) * N;
+    for (int j = t; j < N; j += 1024) {
+      const double x = rk[j];
+      rk[j] = rp[j];
+      rp[j] = x;
+    }
+  }
+  __syncthreads();
+  const double inv = 1.0 / rk[k];
+  __syncthreads();
+  for (int i = t; i < N; i += 1024) col[i] = i == k ? 0.0 : a[size_t(i) * N + k];
+  for (int j = t; j < N; j += 1024) rk[j] = j == k ? inv : rk[j] * inv;
+}
+
+// one row per blockIdx.y; rows with a zero multiplier are left alone
+__global__ void k_gj_update(int N, int k, double* __restrict__ a, const double* __restrict__ col) {
+  const int i = blockIdx.y;
+  if (i == k) return;
+  const double m = col[i];
+  if (m == 0.0) return;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const double* rk = a + size_t(k) * N;
+  double* ri = a + size_t(i) * N;
+  ri[j] = j == k ? -m * rk[k] : ri[j] - m * rk[j];
+}
+
+__global__ void k_gj_unpermute(int N, double* __restrict__ a, const int* __restrict__ piv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double* ri = a + size_t(i) * N;
+  for (int k = N - 1; k >= 0; --k) {
+    const int p = piv[k];
+    if (p != k) {
+      const double x = ri[k];
+      ri[k] = ri[p];
+      ri[p] = x;
+    }
+  }
+}
+
+// top-left n x n block, column major
+__global__ void k_gj_extract(int n, int N, const double* __restrict__ a, double* __restrict__ inv) {
+  const size_t q = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= size_t(n) * n) return;
+  const size_t c = q / n, r = q % n;
+  inv[q] = a[r * N + c];
+}
+
+// gravity! potential (src/two_phase/forces.jl:545-556): g . x at the cell
+// centre origin + (i + 1/2) h, moved by offset * h (the interface centroid of
+// a mixed cell relative to its centre, in units of the cell spacing; nullptr:
+// none). Products and sums round one at a time, as the reference's do.
+__global__ void k_gravity_potential(int n, int bx, int nb_cells, const double* __restrict__ origin,
+                                    const double* __restrict__ spacing, const double* __restrict__ offset, double gx,
+                                    double gy, double* __restrict__ phi) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int b = c / nb_cells, l = c - b * nb_cells, i = l % bx, j = l / bx;
+  const double hx = spacing[2 * b], hy = spacing[2 * b + 1];
+  double x = origin[2 * b] + (i + 0.5) * hx;
+  double y = origin[2 * b + 1] + (j + 0.5) * hy;
+  if (offset) {
+    x += offset[2 * c] * hx;
+    y += offset[2 * c + 1] * hy;
+  }
+  phi[c] = gx * x + gy * y;
+}
+
+// gravity! jumps (forces.jl:560-576): rho_f |f| (phi+ - phi-) with the
+// open-face phase mixture rho_f = rho_l a_l + rho_g a_g (open aperture 1)
+__global__ void k_gravity_jump(int nf, const int* __restrict__ fm, const int* __restrict__ fp,
+                               const double* __restrict__ area, const double* __restrict__ liquid,
+                               const double* __restrict__ gas, double rho_l, double rho_g,
+                               const double* __restrict__ phi, double* __restrict__ jumps) {
+#pragma clang fp contract(off)
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const double rho = rho_l * liquid[f] + rho_g * gas[f];
+  jumps[f] = rho * area[f] * (phi[fp[f]] - phi[fm[f]]);
 }
 
 // CGFM (or CSF) surface-tension jump sigma kappa_f [H] |f|
@@ -733,13 +869,14 @@ __global__ void k_divergence(int n, int nb_cells, const int* __restrict__ cptr, 
   div[c] = s;
 }
 
-// adjoint coefficient gradient -[y_l (K y_p) + y_p (K y_l)] / (2 kappa)
+// adjoint coefficient gradient scale [y_l (K y_p) + y_p (K y_l)] / (2 kappa); g may alias kyl
 __global__ void k_adjoint_face(int nf, const double* __restrict__ yl, const double* __restrict__ kyp,
-                               const double* __restrict__ yp, const double* __restrict__ kyl,
-                               const double* __restrict__ kappa, double* __restrict__ g) {
+                               const double* __restrict__ yp,
+                               const double* kyl, const double* __restrict__ kappa, double scale, double* g) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
-  g[f] = kappa[f] > 0 ? -(yl[f] * kyp[f] + yp[f] * kyl[f]) / (2 * kappa[f]) : 0.0;
+  const double v = kappa[f] > 0 ? scale * (yl[f] * kyp[f] + yp[f] * kyl[f]) / (2 * kappa[f]) : 0.0;
+  g[f] = v;
 }
 
 __global__ void k_cast_d2f(int n, const double* __restrict__ in, float* __restrict__ out) {
@@ -1011,6 +1148,25 @@ void launch_mixture(hipStream_t s, int nf, int nbc, const int* fm, const int* fp
                      kappa);
 }
 
+void launch_dense_inverse(hipStream_t s, int N, double* a, int* piv, double* col, int* fail, int n, double* inv) {
+  for (int k = 0; k < N; ++k) {
+    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, s, N, k, a, piv, col, fail);
+    hipLaunchKernelGGL(k_gj_update, dim3(cdiv(N, 256), N), dim3(256), 0, s, N, k, a, col);
+  }
+  hipLaunchKernelGGL(k_gj_unpermute, dim3(cdiv(N, 256)), dim3(256), 0, s, N, a, piv);
+  hipLaunchKernelGGL(k_gj_extract, dim3(cdiv(n * n, 256)), dim3(256), 0, s, n, N, a, inv);
+}
+
+void launch_gravity_jumps(hipStream_t s, int n, int bx, int nbc, const double* origin, const double* spacing,
+                          const double* offset, double gx, double gy, double* phi, int nf, const int* fm,
+                          const int* fp, const double* area, const double* liquid, const double* gas, double rl,
+                          double rg, double* jumps) {
+  hipLaunchKernelGGL(k_gravity_potential, dim3(cdiv(n, 256)), dim3(256), 0, s, n, bx, nbc, origin, spacing, offset,
+                     gx, gy, phi);
+  hipLaunchKernelGGL(k_gravity_jump, dim3(cdiv(nf, 256)), dim3(256), 0, s, nf, fm, fp, area, liquid, gas, rl, rg, phi,
+                     jumps);
+}
+
 void launch_surface_jump(hipStream_t s, int nf, const int* fm, const int* fp, const double* area, const double* c,
                          const double* curv, double sigma, int csf, double* jumps) {
   hipLaunchKernelGGL(k_surface_jump, dim3(cdiv(nf, 256)), dim3(256), 0, s, nf, fm, fp, area, c, curv, sigma, csf,
@@ -1026,8 +1182,8 @@ void launch_divergence(hipStream_t s, int n, int nbc, const int* cptr, const int
 }
 
 void launch_adjoint_face(hipStream_t s, int nf, const double* yl, const double* kyp, const double* yp,
-                         const double* kyl, const double* kappa, double* g) {
-  hipLaunchKernelGGL(k_adjoint_face, dim3(cdiv(nf, 256)), dim3(256), 0, s, nf, yl, kyp, yp, kyl, kappa, g);
+                         const double* kyl, const double* kappa, double scale, double* g) {
+  hipLaunchKernelGGL(k_adjoint_face, dim3(cdiv(nf, 256)), dim3(256), 0, s, nf, yl, kyp, yp, kyl, kappa, scale, g);
 }
 
 template <typename T>

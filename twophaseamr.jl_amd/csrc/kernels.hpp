@@ -76,6 +76,10 @@ template <typename T>
 void launch_prolong(hipStream_t s, const LevelView<T>& L, T* x, const T* xc, double wp, const int* done, bool fine);
 template <typename T>
 void launch_coarse(hipStream_t s, int n, const double* inv, const T* b, T* x, const int* done);
+// inverse of the dense N x N row-major matrix `a` (overwritten; Gauss-Jordan
+// with partial pivoting, piv and col N-entry scratch); its top-left n x n
+// block goes to `inv` column major; *fail is set on a zero or non-finite pivot
+void launch_dense_inverse(hipStream_t s, int N, double* a, int* piv, double* col, int* fail, int n, double* inv);
 
 // The V-cycle of the small levels in one workgroup (k_vtail): levels
 // 0..n-1 of the tail (the last one the coarsest), their vectors and weights.
@@ -152,8 +156,14 @@ void launch_scale_by_volume(hipStream_t s, int n, int nbc, const double* in, dou
                             bool divide);
 void launch_metric_apply(hipStream_t s, int nf, const double* kd, const int* optr, const int* oface,
                          const double* oval, const double* in, double* out);
+// out = sqrt(k) |f| (p+ - p-) (p given) or sqrt(k) jumps; sqk = nullptr: unit weights
 void launch_face_jump(hipStream_t s, int nf, const int* fm, const int* fp, const double* area, const double* sqk,
                       const double* p, const double* jumps, double* out);
+// gravity potential of n cells (phi) and the nf face jumps rho_f |f| (phi+ - phi-)
+void launch_gravity_jumps(hipStream_t s, int n, int bx, int nbc, const double* origin, const double* spacing,
+                          const double* offset, double gx, double gy, double* phi, int nf, const int* fm,
+                          const int* fp, const double* area, const double* liquid, const double* gas, double rl,
+                          double rg, double* jumps);
 void launch_face_scale(hipStream_t s, int nf, const double* sqk, const double* in, double* out);
 void launch_mixture(hipStream_t s, int nf, int nbc, const int* fm, const int* fp, const signed char* axis,
                     const double* spacing, const double* c, double rl, double rg, double* kappa);
@@ -163,8 +173,9 @@ void launch_divergence(hipStream_t s, int n, int nbc, const int* cptr, const int
                        const double* area, const double* u, const int* bptr, const int* bface,
                        const signed char* bside, const double* barea, const double* ub, const double* vol,
                        double* div);
+// g = scale [yl (K yp) + yp (K yl)] / (2 kappa) per face (g may alias kyl)
 void launch_adjoint_face(hipStream_t s, int nf, const double* yl, const double* kyp, const double* yp,
-                         const double* kyl, const double* kappa, double* g);
+                         const double* kyl, const double* kappa, double scale, double* g);
 template <typename T>
 void launch_zstats(hipStream_t s, int nblocks, int bx, const T* z, const ZStats& zs, const double* vol,
                    const int* done);

@@ -115,7 +115,8 @@ struct tpamr_ctx {
   std::vector<DevTopo> topo;
   std::vector<DevLevel<double>> ld;  // FP64 levels (level 0 always, all in FP64 mode)
   std::vector<DevLevel<float>> lf;   // FP32 levels (FP32 mode)
-  DevArray<double> coarse_inv;
+  DevArray<double> coarse_inv, coarse_work, coarse_col;
+  DevArray<int> coarse_piv, coarse_fail;
   DevArray<double> x, r, p, Ap, b, tmp, tmp2;
   DevArray<double> part, maxpart, scal, stage;
   CgState* st = nullptr;      // device
@@ -125,6 +126,7 @@ struct tpamr_ctx {
   DevArray<int> fm, fp, kptr, kface, cptr, cface, bptr, bface;
   DevArray<signed char> faxis, csign, bside;
   DevArray<double> farea, kdiag, kval, kappa, sqk, spacing, barea;
+  DevArray<double> origin, cell2;  // 2 per block: lower-left corner; 2 per cell: staging
   DevArray<double> f1, f2, f3, f4;
   // halo refresh schedule (built on first use for a halo width)
   HaloSchedule halo_host;
@@ -304,6 +306,7 @@ void upload_hierarchy(tpamr_ctx* c, bool topology) {
     c->kface.upload(L0.K.off_face, s);
     c->kval.upload(L0.K.off_val, s);
     c->spacing.upload(M.bspacing, s);
+    c->origin.upload(M.borigin, s);
     // cell -> face incidence (face order) for the divergence gather
     const int N = M.ncells(), F = M.nfaces();
     std::vector<int> cnt(N + 1, 0);
@@ -372,7 +375,22 @@ void upload_hierarchy(tpamr_ctx* c, bool topology) {
   destroy_graph(c);
   for (int l = 0; l < int(c->ld.size()); ++l) upload_level_values(c, c->ld[l], *H.levels[l], c->topo[l], l == 0);
   for (int l = 0; l < int(c->lf.size()); ++l) upload_level_values(c, c->lf[l], *H.levels[l], c->topo[l], false);
-  c->coarse_inv.upload(H.coarse_inv, s);
+  {
+    // dense coarse inverse on the device
+    const int n = H.coarse_n, N = n + 1;
+    c->coarse_work.upload(H.coarse_aug, s);
+    c->coarse_col.alloc(N);
+    c->coarse_piv.alloc(N);
+    c->coarse_fail.alloc(1);
+    c->coarse_inv.alloc(size_t(n) * n);
+    HIPCHECK(hipMemsetAsync(c->coarse_fail.p, 0, sizeof(int), s));
+    launch_dense_inverse(s, N, c->coarse_work.p, c->coarse_piv.p, c->coarse_col.p, c->coarse_fail.p, n,
+                         c->coarse_inv.p);
+    int fail = 0;
+    HIPCHECK(hipMemcpyAsync(&fail, c->coarse_fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (fail) throw std::runtime_error("singular coarse operator");
+  }
   c->kappa.upload(H.levels[0]->kappa, s);
   std::vector<double> sq(H.levels[0]->kappa.size());
   for (size_t f = 0; f < sq.size(); ++f) sq[f] = std::sqrt(H.levels[0]->kappa[f]);
@@ -1279,6 +1297,60 @@ int tpamr_weighted_jump_force(tpamr_ctx* c, const double* jumps, double* accel) 
   });
 }
 
+namespace {
+// gravity! jumps into c->f3 (liquid and gas apertures staged in f1, f2)
+void gravity_jumps(tpamr_ctx* c, const double* offsets, const double* liquid, const double* gas, double rl,
+                   double rg, double gx, double gy) {
+  if (!(std::isfinite(rl) && rl > 0 && std::isfinite(rg) && rg > 0))
+    throw std::invalid_argument("densities must be finite and positive");
+  if (!(std::isfinite(gx) && std::isfinite(gy))) throw std::invalid_argument("gravity must be finite");
+  const TreeMesh& M = c->mesh();
+  if ((M.periodic[0] && gx != 0) || (M.periodic[1] && gy != 0))
+    throw std::invalid_argument("gravity potential requires nonperiodic boundaries along gravity");
+  const int N = M.ncells(), F = M.nfaces();
+  check_finite(liquid, F, "liquid aperture");
+  check_finite(gas, F, "gas aperture");
+  upload_vec(c, c->f1.p, liquid, F);
+  upload_vec(c, c->f2.p, gas, F);
+  const double* doff = nullptr;
+  if (offsets) {
+    check_finite(offsets, 2 * size_t(N), "interface centroid offsets");
+    if (c->cell2.n < 2 * size_t(N)) c->cell2.alloc(2 * size_t(N));
+    upload_vec(c, c->cell2.p, offsets, 2 * size_t(N));
+    doff = c->cell2.p;
+  }
+  launch_gravity_jumps(c->stream, N, M.bs[0], M.nb(), c->origin.p, c->spacing.p, doff, gx, gy, c->tmp.p, F, c->fm.p,
+                       c->fp.p, c->farea.p, c->f1.p, c->f2.p, rl, rg, c->f3.p);
+}
+}  // namespace
+
+int tpamr_gravity_jumps(tpamr_ctx* c, const double* offsets, const double* liquid, const double* gas,
+                        double rho_liquid, double rho_gas, double gx, double gy, double* jumps) {
+  if (!c || !liquid || !gas || !jumps) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    if (!c->H) throw StateError("no hierarchy: call tpamr_set_face_coefficients first");
+    HIPCHECK(hipSetDevice(c->device));
+    gravity_jumps(c, offsets, liquid, gas, rho_liquid, rho_gas, gx, gy);
+    download_vec(c, jumps, c->f3.p, c->mesh().nfaces());
+  });
+}
+
+int tpamr_gravity(tpamr_ctx* c, const double* offsets, const double* liquid, const double* gas, double rho_liquid,
+                  double rho_gas, double gx, double gy, double* accel) {
+  if (!c || !liquid || !gas || !accel) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    need_solver(c);
+    HIPCHECK(hipSetDevice(c->device));
+    gravity_jumps(c, offsets, liquid, gas, rho_liquid, rho_gas, gx, gy);
+    const int F = c->mesh().nfaces();
+    launch_face_jump(c->stream, F, c->fm.p, c->fp.p, c->farea.p, c->sqk.p, nullptr, c->f3.p, c->f1.p);
+    weighted_metric(c, c->f1.p, c->f2.p);
+    download_vec(c, accel, c->f2.p, F);
+  });
+}
+
 int tpamr_divergence(tpamr_ctx* c, const double* u, const double* ub, double* div) {
   if (!c || !u || !div) return TPAMR_EINVAL;
   return guard(c, [&] {
@@ -1373,8 +1445,64 @@ int tpamr_adjoint(tpamr_ctx* c, const double* w, const double* p, double rtol, i
     launch_metric_apply(s, F, c->kdiag.p, c->kptr.p, c->kface.p, c->kval.p, c->f1.p, c->f3.p);
     launch_metric_apply(s, F, c->kdiag.p, c->kptr.p, c->kface.p, c->kval.p, c->f2.p, c->f4.p);
     // f3 = K y_l, f4 = K y_p
-    launch_adjoint_face(s, F, c->f1.p, c->f4.p, c->f2.p, c->f3.p, c->kappa.p, c->f1.p);
+    launch_adjoint_face(s, F, c->f1.p, c->f4.p, c->f2.p, c->f3.p, c->kappa.p, -1.0, c->f1.p);
     download_vec(c, dJ_dkappa, c->f1.p, F);
+    if (stats) *stats = st;
+  });
+}
+
+int tpamr_project_adjoint(tpamr_ctx* c, const double* ubar, const double* phi, const double* jumps, double dt,
+                          double rtol, int max_cycles, double* ubar_star, double* dJ_drhs, double* dJ_dkappa,
+                          tpamr_stats* stats) {
+  if (!c || !ubar || !ubar_star || (dJ_dkappa && !phi)) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    need_solver(c);
+    if (!(dt > 0 && std::isfinite(dt))) throw std::invalid_argument("dt must be finite and strictly positive");
+    HIPCHECK(hipSetDevice(c->device));
+    const TreeMesh& M = c->mesh();
+    const int N = M.ncells(), F = M.nfaces(), nb = M.nb();
+    hipStream_t s = c->stream;
+    check_finite(ubar, F, "velocity adjoint");
+    upload_vec(c, c->f1.p, ubar, F);
+    // A lambda = J^T M ubar = -V div(M ubar), M = sqrt(k) K sqrt(k)
+    launch_face_scale(s, F, c->sqk.p, c->f1.p, c->f2.p);
+    weighted_metric(c, c->f2.p, c->f3.p);
+    launch_divergence(s, N, nb, c->cptr.p, c->cface.p, c->csign.p, c->farea.p, c->f3.p, c->bptr.p, c->bface.p,
+                      c->bside.p, c->barea.p, nullptr, c->topo[0].volume.p, c->tmp.p);
+    launch_axpby(s, N, -1.0, c->tmp.p, 0.0, c->b.p);
+    launch_fill(s, N, 0.0, c->x.p);
+    tpamr_stats st = solve_dispatch(c, rtol, 0.0, max_cycles, 0);
+    if (dJ_drhs) {
+      // dJ/drhs = -dt V lambda
+      launch_scale_by_volume(s, N, nb, c->x.p, c->tmp.p, c->topo[0].volume.p, false);
+      launch_axpby(s, N, -dt, c->tmp.p, 0.0, c->tmp.p);
+      download_vec(c, dJ_drhs, c->tmp.p, N);
+    }
+    // ubar* = ubar - J lambda, J x = |f| (x+ - x-)
+    launch_face_jump(s, F, c->fm.p, c->fp.p, c->farea.p, nullptr, c->x.p, nullptr, c->f2.p);
+    launch_axpby(s, F, -1.0, c->f2.p, 1.0, c->f1.p);
+    download_vec(c, ubar_star, c->f1.p, F);
+    if (dJ_dkappa) {
+      // z = jumps - J phi; dJ/dk_f = dt [W_f (K Z)_f + Z_f (K W)_f] / (2 k_f)
+      // with W = sqrt(k) ubar*, Z = sqrt(k) z
+      check_finite(phi, N, "checkpointed pressure");
+      upload_vec(c, c->tmp2.p, phi, N);
+      launch_face_jump(s, F, c->fm.p, c->fp.p, c->farea.p, nullptr, c->tmp2.p, nullptr, c->f3.p);
+      if (jumps) {
+        check_finite(jumps, F, "force jumps");
+        upload_vec(c, c->f2.p, jumps, F);
+      } else {
+        launch_fill(s, F, 0.0, c->f2.p);
+      }
+      launch_axpby(s, F, -1.0, c->f3.p, 1.0, c->f2.p);
+      launch_face_scale(s, F, c->sqk.p, c->f2.p, c->f3.p);  // Z
+      launch_face_scale(s, F, c->sqk.p, c->f1.p, c->f2.p);  // W
+      launch_metric_apply(s, F, c->kdiag.p, c->kptr.p, c->kface.p, c->kval.p, c->f2.p, c->f1.p);
+      launch_metric_apply(s, F, c->kdiag.p, c->kptr.p, c->kface.p, c->kval.p, c->f3.p, c->f4.p);
+      launch_adjoint_face(s, F, c->f2.p, c->f4.p, c->f3.p, c->f1.p, c->kappa.p, dt, c->f1.p);
+      download_vec(c, dJ_dkappa, c->f1.p, F);
+    }
     if (stats) *stats = st;
   });
 }

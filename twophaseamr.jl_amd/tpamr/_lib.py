@@ -81,6 +81,10 @@ _SIGNATURES = {
     "tpamr_surface_tension_jumps": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int,
                                               C.c_void_p]),
     "tpamr_weighted_jump_force": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "tpamr_gravity_jumps": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                      C.c_double, C.c_double, C.c_void_p]),
+    "tpamr_gravity": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                C.c_double, C.c_double, C.c_void_p]),
     "tpamr_divergence": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "tpamr_weighted_gradient": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "tpamr_halo_sizes": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
@@ -91,6 +95,8 @@ _SIGNATURES = {
                                 C.c_void_p, C.POINTER(Stats)]),
     "tpamr_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p,
                                 C.c_void_p, C.POINTER(Stats)]),
+    "tpamr_project_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                        C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Stats)]),
     "tpamr_time_smoother": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
 }
 

@@ -43,7 +43,7 @@ def _refined_keys(base_mesh, block_size, origin, extent, max_level, refine, refi
     oy = np.where(cj == 0, float(origin[1]), origin[1] + cj * rex[1])
     wx, wy = rex
     out = []
-    while lv < max_level and len(ci):
+    while lv < max_level and len(ci) and (refine is not None or refine_many is not None):
         if refine_many is not None:
             split = np.asarray(refine_many(np.full(len(ci), lv), ci, cj, ox, oy,
                                            np.full(len(ci), wx), np.full(len(ci), wy)), dtype=bool)

@@ -9,7 +9,9 @@ Reference interfaces (TwoPhaseAMR.jl):
   weighted_gradient!                                            src/operators/elliptic.jl:1272-1297
   project! / _project_prepared!                                 src/navier_stokes/projection.jl:189-260
   fill_halo! / _build_halo_schedule                             src/mesh/variables.jl:205-229, mesh.jl:944-1037
-The adjoint (`pressure_solve_rrule`) has no reference counterpart.
+  gravity!                                                      src/two_phase/forces.jl:528-580
+The adjoints (`pressure_solve_rrule`, `project_adjoint`) have no reference
+counterpart.
 """
 from __future__ import annotations
 
@@ -74,6 +76,42 @@ def surface_tension(solver: EllipticSolver, fraction, face_curvature, *, sigma, 
                                                              formulation=formulation))
 
 
+def _gravity_args(solver, liquid_aperture, gas_aperture, density, gravity, centroid_offsets):
+    m = solver.operator.mesh
+    rho_l, rho_g = (float(v) for v in density)
+    gx, gy = (float(v) for v in gravity)
+    la = L.f64(liquid_aperture, m.nfaces, "liquid aperture")
+    ga = L.f64(gas_aperture, m.nfaces, "gas aperture")
+    off = None
+    if centroid_offsets is not None:
+        off = np.ascontiguousarray(centroid_offsets, np.float64)
+        if off.shape not in ((m.ncells, 2), (2 * m.ncells,)):
+            raise L.DimensionMismatch(f"centroid offsets must have shape ({m.ncells}, 2)")
+    return m, (L.ptr(off), L.ptr(la), L.ptr(ga), rho_l, rho_g, gx, gy), (off, la, ga)
+
+
+def gravity_jumps(solver: EllipticSolver, liquid_aperture, gas_aperture, *, density, gravity, centroid_offsets=None):
+    """The face jumps of gravity! (src/two_phase/forces.jl:528-580):
+    rho_f |f| (phi+ - phi-) with phi = g . x at the cell centres, moved in mixed
+    cells by `centroid_offsets` (per cell, the interface centroid relative to
+    the cell centre in units of the cell spacing; None: cell centres) and
+    rho_f = rho_l a_l + rho_g a_g from the phase face apertures."""
+    m, args, _keep = _gravity_args(solver, liquid_aperture, gas_aperture, density, gravity, centroid_offsets)
+    out = np.zeros(m.nfaces)
+    L.check(L.lib().tpamr_gravity_jumps(solver.ctx.h, *args, L.ptr(out)), solver.ctx.h)
+    return out
+
+
+def gravity(solver: EllipticSolver, liquid_aperture, gas_aperture, *, density, gravity, centroid_offsets=None):
+    """gravity! interior face accelerations: the jumps of `gravity_jumps`
+    through `_weighted_jump_force!`."""
+    solver._sync()
+    m, args, _keep = _gravity_args(solver, liquid_aperture, gas_aperture, density, gravity, centroid_offsets)
+    out = np.zeros(m.nfaces)
+    L.check(L.lib().tpamr_gravity(solver.ctx.h, *args, L.ptr(out)), solver.ctx.h)
+    return out
+
+
 def divergence(solver: EllipticSolver, u, u_boundary=None):
     m = solver.operator.mesh
     uu = L.f64(u, m.nfaces, "u")
@@ -132,6 +170,30 @@ def pressure_solve_rrule(solver: EllipticSolver, rhs, *, rtol=1e-12, max_cycles=
         return db, dk
 
     return p, pullback
+
+
+def project_adjoint(solver: EllipticSolver, ubar, dt, *, phi=None, jumps=None, rtol=1e-12, max_cycles=200):
+    """Pullback of one forcing-plus-projection step u' = P(u + dt a), a =
+    weighted_jump_force(jumps), through `project` with the current operator.
+
+    Given ubar = dJ/du' returns (dJ/du, dJ/drhs, dJ/dkappa, stats): dJ/du is
+    also dJ/du* (u* = u + dt a); dJ/drhs is the gradient with respect to the
+    step's strong pressure rhs b = -div(u*) / dt; dJ/dkappa (None without
+    `phi`, the step's checkpointed pressure increment) is the gradient with
+    respect to the face coefficients of the step (operator, gradient and force)."""
+    solver._sync()
+    m = solver.operator.mesh
+    ub = L.f64(ubar, m.nfaces, "ubar")
+    ph = None if phi is None else L.f64(phi, m.ncells, "phi")
+    jp = None if jumps is None else L.f64(jumps, m.nfaces, "jumps")
+    us = np.zeros(m.nfaces)
+    db = np.zeros(m.ncells)
+    dk = None if phi is None else np.zeros(m.nfaces)
+    st = L.Stats()
+    L.check(L.lib().tpamr_project_adjoint(solver.ctx.h, L.ptr(ub), L.ptr(ph), L.ptr(jp), float(dt), float(rtol),
+                                          int(max_cycles), L.ptr(us), L.ptr(db), L.ptr(dk), C.byref(st)),
+            solver.ctx.h)
+    return us, db, dk, LinearSolverStats._from(st, solver.precision)
 
 
 def halo_schedule(ctx: L.Context, halo=1):

@@ -221,3 +221,173 @@ def smooth_velocity(mesh: BlockTreeMesh, modes=8, seed=SEED + 1, device=None, ch
         uy = np.sum((np.cos(X) @ b) * np.sin(Y), axis=1)
         u[sl] = np.where(ax[sl] == 0, ux, uy)
     return u
+
+
+# interfaces y = eta(x): cell fractions, phase face apertures, centroids -------
+
+def _clamped_integral(x0, x1, l0, l1, hi):
+    """Exact integral over [x0, x1] of clamp(l, 0, hi) for l linear from l0 at
+    x0 to l1 at x1 (the clamp makes it piecewise linear: integrate between
+    the points where l crosses 0 and hi)."""
+    dl = l1 - l0
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t0 = np.where(dl != 0, -l0 / dl, 2.0)
+        t1 = np.where(dl != 0, (hi - l0) / dl, 2.0)
+    ta = np.clip(np.minimum(t0, t1), 0.0, 1.0)
+    tb = np.clip(np.maximum(t0, t1), 0.0, 1.0)
+    total = 0.0
+    for a, b in ((0.0, ta), (ta, tb), (tb, 1.0)):
+        fa = np.clip(l0 + a * dl, 0.0, hi)
+        fb = np.clip(l0 + b * dl, 0.0, hi)
+        total = total + (b - a) * (fa + fb) / 2
+    return total * (x1 - x0)
+
+
+class InterfaceGeometry:
+    """Interface data of the phase below (or above) a curve y = eta(x) on one
+    mesh, with the curve taken as its tangent line at each cell centre for
+    the cell fraction and the interface centroid, at the face position for
+    vertical faces and as its chord over horizontal faces (all exact for a
+    straight interface). Only cells that the tangent line crosses, and their
+    faces, are computed in full; the mesh arrays are kept between calls."""
+
+    def __init__(self, mesh: BlockTreeMesh):
+        self.mesh = mesh
+        self.x, self.y = mesh.cell_centers()
+        self.hx, self.hy = mesh.cell_spacing()
+
+    def __call__(self, eta, deta, liquid_below=True):
+        """Returns (fraction per cell, liquid aperture per interior face, gas
+        aperture, centroid offsets (ncells, 2) in cell spacings, zero in pure
+        cells)."""
+        mesh = self.mesh
+        x, y, hx, hy = self.x, self.y, self.hx, self.hy
+        ec_all = eta(x)
+        s_all = deta(x)
+        cut = np.abs(ec_all - y) <= hy / 2 + np.abs(s_all) * hx / 2
+        below = (ec_all > y).astype(np.float64)
+        ci = np.nonzero(cut)[0]
+        xc, yc, ec, s, hxc, hyc = x[ci], y[ci], ec_all[ci], s_all[ci], hx[ci], hy[ci]
+        x0, x1, y0 = xc - hxc / 2, xc + hxc / 2, yc - hyc / 2
+        fb = _clamped_integral(x0, x1, ec - s * hxc / 2 - y0, ec + s * hxc / 2 - y0, hyc) / (hxc * hyc)
+        below[ci] = np.clip(fb, 0.0, 1.0)
+        frac = below if liquid_below else 1.0 - below
+        # interface chord of the tangent line inside the cell: the x range
+        # where the line lies between the bottom and the top of the cell
+        with np.errstate(divide="ignore", invalid="ignore"):
+            xa = np.where(s != 0, xc + (y0 - ec) / s, -np.inf)
+            xb = np.where(s != 0, xc + (y0 + hyc - ec) / s, np.inf)
+        lo = np.maximum(np.minimum(xa, xb), x0)
+        hi = np.minimum(np.maximum(xa, xb), x1)
+        mid = (lo + hi) / 2
+        fc = frac[ci]
+        mixed = (fc > 0) & (fc < 1) & (hi >= lo)
+        off = np.zeros((mesh.ncells, 2))
+        off[ci[mixed], 0] = (mid[mixed] - xc[mixed]) / hxc[mixed]
+        off[ci[mixed], 1] = (ec[mixed] + s[mixed] * (mid[mixed] - xc[mixed]) - yc[mixed]) / hyc[mixed]
+        np.clip(off, -0.5, 0.5, out=off)
+        # faces: pure unless next to a crossed cell
+        m, p = mesh.face_minus, mesh.face_plus
+        lb = below[m].copy()
+        fi = np.nonzero(cut[m] | cut[p])[0]
+        fm, fp, ax = m[fi], p[fi], mesh.face_axis[fi]
+        vert = ax == 0
+        fx = np.where(vert, x[fm] + hx[fm] / 2,
+                      0.5 * (np.maximum(x[fm] - hx[fm] / 2, x[fp] - hx[fp] / 2)
+                             + np.minimum(x[fm] + hx[fm] / 2, x[fp] + hx[fp] / 2)))
+        fy = np.where(~vert, y[fm] + hy[fm] / 2,
+                      0.5 * (np.maximum(y[fm] - hy[fm] / 2, y[fp] - hy[fp] / 2)
+                             + np.minimum(y[fm] + hy[fm] / 2, y[fp] + hy[fp] / 2)))
+        ext = np.where(vert, np.minimum(hy[fm], hy[fp]), np.minimum(hx[fm], hx[fp]))
+        a = np.empty(len(fi))
+        a[vert] = np.clip((eta(fx[vert]) - (fy[vert] - ext[vert] / 2)) / ext[vert], 0.0, 1.0)
+        h = ~vert
+        e0 = eta(fx[h] - ext[h] / 2) - fy[h]
+        e1 = eta(fx[h] + ext[h] / 2) - fy[h]
+        # part of the face under the chord (phase below)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            t = np.where(e1 != e0, -e0 / (e1 - e0), 0.0)
+        t = np.clip(t, 0.0, 1.0)
+        a[h] = np.where((e0 > 0) & (e1 > 0), 1.0,
+                        np.where((e0 <= 0) & (e1 <= 0), 0.0, np.where(e1 > e0, 1.0 - t, t)))
+        lb[fi] = a
+        liquid = lb if liquid_below else 1.0 - lb
+        return frac, liquid, 1.0 - liquid, off
+
+
+def interface_geometry(mesh: BlockTreeMesh, eta, deta, liquid_below=True):
+    """One-shot InterfaceGeometry(mesh)(eta, deta, liquid_below)."""
+    return InterfaceGeometry(mesh)(eta, deta, liquid_below)
+
+
+# Stokes-wave-like two-phase case ---------------------------------------------
+
+WAVE_DENSITY = (1000.0, 1.2)
+WAVE_GRAVITY = (0.0, -9.81)
+
+
+def wave_surface(seed=SEED + 2):
+    """eta(x, t) = 0.5 + cos(w t) (0.08 cos(pi x) + 0.01 (a2 cos(2 pi x) + a3
+    cos(3 pi x))), a2, a3 ~ U(-1, 1) (seeded), w = sqrt(g pi tanh(pi / 2)) (the
+    linear standing-wave frequency of the first mode at depth 1/2). Returns
+    (eta, deta) as functions of (x, t)."""
+    rng = SplitMix64(seed)
+    a2 = 2 * rng.uniform() - 1
+    a3 = 2 * rng.uniform() - 1
+    w = math.sqrt(9.81 * math.pi * math.tanh(math.pi / 2))
+
+    def eta(x, t=0.0):
+        return 0.5 + math.cos(w * t) * (0.08 * np.cos(np.pi * x)
+                                        + 0.01 * (a2 * np.cos(2 * np.pi * x) + a3 * np.cos(3 * np.pi * x)))
+
+    def deta(x, t=0.0):
+        return -math.cos(w * t) * np.pi * (0.08 * np.sin(np.pi * x)
+                                           + 0.01 * (2 * a2 * np.sin(2 * np.pi * x) + 3 * a3 * np.sin(3 * np.pi * x)))
+
+    return eta, deta
+
+
+def wave_mesh(levels=(5, 11), block=32, band_cells=4, drift=0.0):
+    """[0, 2] x [0, 1] with walls, block x block cells per block (two roots),
+    uniform to levels[0] and refined to levels[1] within `band_cells` finest
+    cells (plus `drift`, the largest interface displacement of the run) of
+    the wave surface at t = 0."""
+    eta, _ = wave_surface()
+    lo, hi = levels
+    band = band_cells / (block * (1 << hi)) + drift
+    lip = 0.08 * math.pi + 0.01 * (2 * math.pi + 3 * math.pi)  # bound of |eta'|
+
+    def refine_many(level, i, j, ox, oy, ex, ey):
+        out = level < lo
+        rest = ~out
+        if rest.any():
+            xs = ox[rest][:, None] + ex[rest][:, None] * np.linspace(0.0, 1.0, 17)[None, :]
+            e = eta(xs)
+            slack = lip * ex[rest] / 32 + band
+            emin = e.min(axis=1) - slack
+            emax = e.max(axis=1) + slack
+            out[rest] = (emax >= oy[rest]) & (emin <= oy[rest] + ey[rest])
+        return out
+
+    return BlockTreeMesh(base_mesh=(2 * block, block), block_size=(block, block), extent=(2.0, 1.0),
+                         periodic=(False, False), max_level=hi, refine_many=refine_many)
+
+
+def wave_steps(solver, steps, dt, density=WAVE_DENSITY, gravity=WAVE_GRAVITY):
+    """Per step n (time n dt): the mixture face coefficients of the surface
+    at that time and the gravity! jumps, as ProjectionRun takes them, plus
+    the final fraction (for the kinetic energy)."""
+    from .physics import gravity_jumps, mixture_face_coefficients
+    eta, deta = wave_surface()
+    geom = InterfaceGeometry(solver.operator.mesh)
+    out = []
+    frac = None
+    for n in range(steps + 1):
+        t = n * dt
+        frac, la, ga, off = geom(lambda x: eta(x, t), lambda x: deta(x, t))
+        if n == steps:
+            break
+        kappa = mixture_face_coefficients(solver, frac, density)
+        jumps = gravity_jumps(solver, la, ga, density=density, gravity=gravity, centroid_offsets=off)
+        out.append((kappa, jumps))
+    return out, frac
