@@ -111,32 +111,28 @@ __device__ inline int xcd_chunk_group(int w, int S) {
 // coarsest level
 
 // Coarsest level: x = inv * b with the dense inverse of the augmented
-// (volume-constrained) operator, column-major, evaluated in FP64. A
-// workgroup owns 64 rows; its 16 column groups are summed through LDS.
+// (volume-constrained) operator, evaluated in FP64. The inverse is symmetric
+// (so is the augmented operator), so its column-major storage is read row by
+// row: one wavefront per row streams that row contiguously and sums with
+// shuffles; four rows per workgroup.
 template <typename T>
-__global__ void __launch_bounds__(1024) k_coarse_solve(int n, const double* __restrict__ inv, const T* __restrict__ b,
-                                                      T* __restrict__ x, const int* __restrict__ done) {
+__global__ void __launch_bounds__(256) k_coarse_solve(int n, const double* __restrict__ inv, const T* __restrict__ b,
+                                                     T* __restrict__ x, const int* __restrict__ done) {
   if (done && *done) return;
-  __shared__ double part[16][65];
-  const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
-  const int r = blockIdx.x * 64 + lane;
+  const int lane = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= n) return;
+  const double* row = inv + size_t(r) * n;
   double s0 = 0, s1 = 0;
-  if (r < n) {
-    int c = g;
-    for (; c + 16 < n; c += 32) {
-      s0 += inv[size_t(c) * n + r] * double(b[c]);
-      s1 += inv[size_t(c + 16) * n + r] * double(b[c + 16]);
-    }
-    if (c < n) s0 += inv[size_t(c) * n + r] * double(b[c]);
+  int c = lane;
+  for (; c + 64 < n; c += 128) {
+    s0 += row[c] * double(b[c]);
+    s1 += row[c + 64] * double(b[c + 64]);
   }
-  part[g][lane] = s0 + s1;
-  __syncthreads();
-  if (g == 0 && r < n) {
-    double s = 0;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) s += part[q][lane];
-    x[r] = T(s);
-  }
+  if (c < n) s0 += row[c] * double(b[c]);
+  double s = s0 + s1;
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+  if (lane == 0) x[r] = T(s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1001,7 +997,7 @@ void launch_prolong(hipStream_t s, const LevelView<T>& L, T* x, const T* xc, dou
 
 template <typename T>
 void launch_coarse(hipStream_t s, int n, const double* inv, const T* b, T* x, const int* done) {
-  hipLaunchKernelGGL((k_coarse_solve<T>), dim3(cdiv(n, 64)), dim3(1024), 0, s, n, inv, b, x, done);
+  hipLaunchKernelGGL((k_coarse_solve<T>), dim3(cdiv(n, 4)), dim3(256), 0, s, n, inv, b, x, done);
 }
 
 // Dynamic LDS a workgroup may use (the device limit less the tail kernel's

@@ -83,7 +83,12 @@ void launch_dense_inverse(hipStream_t s, int N, double* a, int* piv, double* col
 
 // The V-cycle of the small levels in one workgroup (k_vtail): levels
 // 0..n-1 of the tail (the last one the coarsest), their vectors and weights.
+// phase 0: the whole tail cycle; 1: the way down to the coarsest level's
+// rhs; 2: the way up from the coarsest level's solution (a coarsest level of
+// more than COARSE_INLINE cells is solved between the two by the
+// many-workgroup coarse kernel).
 constexpr int TAIL_MAX = 8;
+constexpr int COARSE_INLINE = 256;
 template <typename T>
 struct TailArgs {
   LevelView<T> L[TAIL_MAX];
@@ -94,7 +99,7 @@ struct TailArgs {
   const T* dinv[TAIL_MAX];
   double omega[TAIL_MAX], omega_p[TAIL_MAX];
   const double* inv;
-  int n, pre, post, coarse_n;
+  int n, pre, post, coarse_n, phase;
 };
 template <typename T>
 void launch_vtail(hipStream_t s, const TailArgs<T>& A, const int* done);

@@ -457,7 +457,16 @@ T* enqueue_vcycle(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done, c
     A.coarse_n = c->H->coarse_n;
     A.pre = pre;
     A.post = post;
-    launch_vtail<T>(s, A, done);
+    if (A.coarse_n <= COARSE_INLINE) {
+      A.phase = 0;
+      launch_vtail<T>(s, A, done);
+    } else {
+      A.phase = 1;
+      launch_vtail<T>(s, A, done);
+      launch_coarse<T>(s, A.coarse_n, c->coarse_inv.p, lv[nl - 1].b.p, lv[nl - 1].x.p, done);
+      A.phase = 2;
+      launch_vtail<T>(s, A, done);
+    }
     // level ts's result: the buffer after its (pre - 1) + post sweeps
     if ((std::max(pre - 1, 0) + post) & 1) std::swap(cur[ts], oth[ts]);
   }

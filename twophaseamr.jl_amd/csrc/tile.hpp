@@ -578,14 +578,17 @@ __device__ inline void coarse_op(int n, const double* __restrict__ inv, const T*
 // blockDim / 256 tile groups at once separated by workgroup barriers. On
 // entry b[0] and the zero-start x[0] come from the restriction of the next
 // finer level; on exit level 0's result is in x[0] or xs[0] by the parity
-// of its (pre - 1) + post sweeps, as in the launch sequence.
+// of its (pre - 1) + post sweeps, as in the launch sequence. Phase 1 stops
+// once the coarsest level's rhs is restricted, phase 2 starts from its
+// solution (the parities after the way down are fixed by `pre`).
 template <typename T, int BX, int BY>
 __global__ void __launch_bounds__(1024) k_vtail(TailArgs<T> A, const int* __restrict__ done) {
   if (done && *done) return;
   TILE4_SHARED(T, BX, BY);
   using G = Geo<BX, BY>;
   const int nq = blockDim.x >> 8, quarter = threadIdx.x >> 8, t = threadIdx.x & 255;
-  unsigned par = 0;  // bit l: level l's current iterate is in xs
+  // bit l: level l's current iterate is in xs
+  unsigned par = (A.phase == 2 && (max(A.pre - 1, 0) & 1)) ? (1u << (A.n - 1)) - 1u : 0u;
   auto cur = [&](int l) { return ((par >> l) & 1u) ? A.xs[l] : A.x[l]; };
   auto oth = [&](int l) { return ((par >> l) & 1u) ? A.x[l] : A.xs[l]; };
   // one tile pass over level l (every thread runs the same number of tiles)
@@ -598,7 +601,7 @@ __global__ void __launch_bounds__(1024) k_vtail(TailArgs<T> A, const int* __rest
     }
     __syncthreads();
   };
-  for (int l = 0; l + 1 < A.n; ++l) {
+  for (int l = 0; l + 1 < A.n && A.phase != 2; ++l) {
     for (int q = 1; q < A.pre; ++q) {
       T* xi = cur(l);
       T* xo = oth(l);
@@ -617,8 +620,11 @@ __global__ void __launch_bounds__(1024) k_vtail(TailArgs<T> A, const int* __rest
                              zero_start);
     });
   }
-  coarse_op<T>(A.coarse_n, A.inv, A.b[A.n - 1], A.x[A.n - 1]);
-  __syncthreads();
+  if (A.phase == 1) return;
+  if (A.phase == 0) {
+    coarse_op<T>(A.coarse_n, A.inv, A.b[A.n - 1], A.x[A.n - 1]);
+    __syncthreads();
+  }
   for (int l = A.n - 2; l >= 0; --l) {
     T* xl = cur(l);
     const T* xc = cur(l + 1);
