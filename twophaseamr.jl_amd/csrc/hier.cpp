@@ -14,6 +14,7 @@
 #include <array>
 #include <cmath>
 #include <stdexcept>
+#include <thread>
 #include <tuple>
 
 namespace tpamr {
@@ -550,6 +551,23 @@ void level_topology(LevelData& L) {
   }
 }
 
+// Run f(lo, hi, t) over [0, n) in contiguous chunks on up to 16 threads
+// (t = chunk index); small ranges run inline.
+template <typename F>
+void parallel_for(long n, F&& f) {
+  const long grain = 1 << 14;
+  int nt = int(std::min<long>(16, std::max<long>(1, std::thread::hardware_concurrency())));
+  nt = int(std::min<long>(nt, (n + grain - 1) / grain));
+  if (nt <= 1) {
+    f(0L, n, 0);
+    return;
+  }
+  std::vector<std::thread> pool;
+  pool.reserve(nt);
+  for (int t = 0; t < nt; ++t) pool.emplace_back([&, t] { f(n * t / nt, n * (t + 1) / nt, t); });
+  for (auto& th : pool) th.join();
+}
+
 // Device form of the irregular rows (see LevelData): per block, the rows in
 // decreasing length order with their entries in groups of four (entry 4g+t of
 // row x of the block at (r0 * ng + g * nr + x) * 4 + t, ng = ceil(w / 4)),
@@ -557,7 +575,8 @@ void level_topology(LevelData& L) {
 // its same-level halo. Tile slots: interior cells j * bx + i, then the halo
 // strips W (bx*by + j), E (+ by), S (+ 2 by + i), N (+ 2 by + bx + i), then
 // the external cells (+ 2 (bx + by) + e). Padding entries: value 0, the row's
-// own slot.
+// own slot. The structure only: each entry's value position (ell_pos) and each
+// row's position (ell_row) are kept for the value passes.
 void build_ell(LevelData& L) {
   const TreeMesh& M = *L.mesh;
   const int bx = M.bs[0], by = M.bs[1], nb = bx * by;
@@ -572,6 +591,8 @@ void build_ell(LevelData& L) {
   L.ell_bound.assign(size_t(nrows) * w4, 0.0);
   L.ell_loc.assign(nrows, 0u);
   L.ell_diag.assign(nrows, 0.0);
+  L.ell_pos.assign(L.irr_col.size(), 0);
+  L.ell_row.assign(nrows, 0);
   L.ext_max = 0;
   L.blk_ext.assign(L.nblocks + 1, 0);
   L.ext_col.clear();
@@ -594,7 +615,7 @@ void build_ell(LevelData& L) {
       const int loc = L.irr_local[r];
       const int len = L.irr_ptr[r + 1] - L.irr_ptr[r];
       L.ell_loc[r0 + x] = uint32_t(loc) | (uint32_t(len) << 16);
-      L.ell_diag[r0 + x] = L.irr_diag[r];
+      L.ell_row[r] = r0 + x;
       for (int k = 0; k < w4; ++k) L.ell_ref[slot(x, k)] = uint16_t(loc);
       int k = 0;
       for (int e = L.irr_ptr[r]; e < L.irr_ptr[r + 1]; ++e, ++k) {
@@ -616,8 +637,7 @@ void build_ell(LevelData& L) {
         }
         const size_t at = slot(x, k);
         L.ell_ref[at] = uint16_t(ref);
-        L.ell_val[at] = L.irr_val[e];
-        L.ell_bound[at] = L.irr_bound[e];
+        L.ell_pos[e] = int(at);
       }
     }
     if (int(ext.size()) > nb) throw std::runtime_error("too many external couplings in one block");
@@ -626,27 +646,15 @@ void build_ell(LevelData& L) {
   }
 }
 
-// Coefficient-dependent values of a level.
-void level_values(LevelData& L) {
+// Structure of the irregular rows (built once per topology): the columns of
+// every row (CSR, increasing column) and, per entry, the face-metric terms
+// that sum to its value in the order the value pass adds them.
+void level_pattern(LevelData& L) {
   const TreeMesh& M = *L.mesh;
-  const int bx = M.bs[0], by = M.bs[1], nb = bx * by;
+  const int nb = M.nb();
   const int F = M.nfaces();
-  std::vector<double> sq(F);
-  for (int f = 0; f < F; ++f) sq[f] = std::sqrt(L.kappa[f]);
-  L.cx.assign(size_t(L.nblocks) * (bx + 1) * by, 0.0);
-  L.cy.assign(size_t(L.nblocks) * bx * (by + 1), 0.0);
-  for (int f = 0; f < F; ++f) {
-    if (!L.face_regular[f]) continue;
-    const double a = M.farea[f];
-    const double w = L.kappa[f] * L.K.diag[f] * a * a;
-    std::vector<double>& arr = M.faxis[f] == 0 ? L.cx : L.cy;
-    arr[L.slot_a[f]] = w;
-    arr[L.slot_b[f]] = w;
-  }
-  // irregular rows: full rows of A (and of the floor bound |J|'|M||J|)
   const int nrows = int(L.irr_local.size());
-  std::vector<std::vector<int>> inc;  // faces per irregular row
-  inc.resize(nrows);
+  std::vector<std::vector<int>> inc(nrows);  // faces per irregular row
   for (int f = 0; f < F; ++f) {
     const int m = M.fminus[f], p = M.fplus[f];
     if (L.cell_irr[m] >= 0) inc[L.cell_irr[m]].push_back(f);
@@ -654,58 +662,128 @@ void level_values(LevelData& L) {
   }
   L.irr_ptr.assign(nrows + 1, 0);
   L.irr_col.clear();
-  L.irr_val.clear();
-  L.irr_bound.clear();
-  L.irr_diag.assign(nrows, 0.0);
-  double max_ratio = 0;
+  L.irr_diag_e.assign(nrows, -1);
+  L.term_ptr.assign(1, 0);
+  L.term_f.clear();
+  L.term_g.clear();
+  L.term_k.clear();
+  L.term_s.clear();
+  L.term_diag.clear();
+  // (column, (term index, 0)): sorted by column only, as the values were
   std::vector<std::pair<int, std::pair<double, double>>> row;
+  struct Term {
+    int f, g;
+    double k, s;
+    bool diag;
+  };
+  std::vector<Term> terms;
   for (int b = 0; b < L.nblocks; ++b)
     for (int r = L.blk_irr[b]; r < L.blk_irr[b + 1]; ++r) {
       const int c = b * nb + L.irr_local[r];
       row.clear();
+      terms.clear();
       for (int f : inc[r]) {
         const double sf = (M.fminus[f] == c) ? -1.0 : 1.0;
-        const double af = M.farea[f];
         auto add = [&](int g, double kfg, bool diag) {
-          const double ag = M.farea[g];
-          const double core = diag ? L.kappa[f] * kfg * af * ag : (sq[f] * kfg) * sq[g] * af * ag;
-          const double babs = diag ? core : (sq[f] * std::fabs(kfg)) * sq[g] * af * ag;
-          row.push_back({M.fminus[g], {sf * -1.0 * core, babs}});
-          row.push_back({M.fplus[g], {sf * 1.0 * core, babs}});
+          terms.push_back({f, g, kfg, sf * -1.0, diag});
+          row.push_back({M.fminus[g], {double(terms.size() - 1), 0.0}});
+          terms.push_back({f, g, kfg, sf * 1.0, diag});
+          row.push_back({M.fplus[g], {double(terms.size() - 1), 0.0}});
         };
         add(f, L.K.diag[f], true);
         for (int q = L.K.off_ptr[f]; q < L.K.off_ptr[f + 1]; ++q) add(L.K.off_face[q], L.K.off_val[q], false);
       }
       std::sort(row.begin(), row.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
-      double diag = 0, rowsum = 0;
       size_t q = 0;
       while (q < row.size()) {
         const int col = row[q].first;
-        double v = 0, bv = 0;
         while (q < row.size() && row[q].first == col) {
-          v += row[q].second.first;
-          bv += row[q].second.second;
+          const Term& t = terms[size_t(row[q].second.first)];
+          L.term_f.push_back(t.f);
+          L.term_g.push_back(t.g);
+          L.term_k.push_back(t.k);
+          L.term_s.push_back(t.s);
+          L.term_diag.push_back(t.diag);
           ++q;
         }
+        if (col == c) L.irr_diag_e[r] = int(L.irr_col.size());
         L.irr_col.push_back(col);
-        L.irr_val.push_back(v);
-        L.irr_bound.push_back(bv);
-        rowsum += std::fabs(v);
-        if (col == c) diag = v;
+        L.term_ptr.push_back(int(L.term_f.size()));
       }
       L.irr_ptr[r + 1] = int(L.irr_col.size());
-      L.irr_diag[r] = diag;
-      if (diag > 0) max_ratio = std::max(max_ratio, rowsum / diag);
     }
+  L.irr_val.assign(L.irr_col.size(), 0.0);
+  L.irr_bound.assign(L.irr_col.size(), 0.0);
+  L.irr_diag.assign(nrows, 0.0);
   build_ell(L);
-  // diagonal of regular rows and the Jacobi weights
-  {
-    const size_t nbk = size_t(L.nblocks);
-    L.cw.assign(nbk * nb, 0.0);
-    L.cs.assign(nbk * nb, 0.0);
-    L.ce.assign(nbk * by, 0.0);
-    L.cn.assign(nbk * bx, 0.0);
-    for (size_t b = 0; b < nbk; ++b) {
+  L.planned = true;
+}
+
+// Coefficient-dependent values of a level.
+void level_values(LevelData& L) {
+  const TreeMesh& M = *L.mesh;
+  const int bx = M.bs[0], by = M.bs[1], nb = bx * by;
+  const int F = M.nfaces();
+  if (!L.planned) level_pattern(L);
+  L.cx.resize(size_t(L.nblocks) * (bx + 1) * by);
+  L.cy.resize(size_t(L.nblocks) * bx * (by + 1));
+  parallel_for(long(L.cx.size()), [&](long lo, long hi, int) { std::fill(&L.cx[lo], &L.cx[0] + hi, 0.0); });
+  parallel_for(long(L.cy.size()), [&](long lo, long hi, int) { std::fill(&L.cy[lo], &L.cy[0] + hi, 0.0); });
+  parallel_for(F, [&](long lo, long hi, int) {
+    for (long f = lo; f < hi; ++f) {
+      if (!L.face_regular[f]) continue;
+      const double a = M.farea[f];
+      const double w = L.kappa[f] * L.K.diag[f] * a * a;
+      std::vector<double>& arr = M.faxis[f] == 0 ? L.cx : L.cy;
+      arr[L.slot_a[f]] = w;
+      arr[L.slot_b[f]] = w;
+    }
+  });
+  // irregular rows: full rows of A (and of the floor bound |J|'|M||J|)
+  const int nrows = int(L.irr_local.size());
+  std::vector<double> ratio(16, 0.0);
+  parallel_for(nrows, [&](long lo, long hi, int t) {
+    double mr = 0;
+    for (long r = lo; r < hi; ++r) {
+      double rowsum = 0;
+      for (int e = L.irr_ptr[r]; e < L.irr_ptr[r + 1]; ++e) {
+        double v = 0, bv = 0;
+        for (int q = L.term_ptr[e]; q < L.term_ptr[e + 1]; ++q) {
+          const int f = L.term_f[q], g = L.term_g[q];
+          const double kfg = L.term_k[q], af = M.farea[f], ag = M.farea[g];
+          const bool diag = L.term_diag[q];
+          const double sf = diag ? 0.0 : std::sqrt(L.kappa[f]), sg = diag ? 0.0 : std::sqrt(L.kappa[g]);
+          const double core = diag ? L.kappa[f] * kfg * af * ag : (sf * kfg) * sg * af * ag;
+          const double babs = diag ? core : (sf * std::fabs(kfg)) * sg * af * ag;
+          v += L.term_s[q] * core;
+          bv += babs;
+        }
+        L.irr_val[e] = v;
+        L.irr_bound[e] = bv;
+        L.ell_val[L.ell_pos[e]] = v;
+        L.ell_bound[L.ell_pos[e]] = bv;
+        rowsum += std::fabs(v);
+      }
+      const int de = L.irr_diag_e[r];
+      const double diag = de >= 0 ? L.irr_val[de] : 0.0;
+      L.irr_diag[r] = diag;
+      L.ell_diag[L.ell_row[r]] = diag;
+      if (diag > 0) mr = std::max(mr, rowsum / diag);
+    }
+    ratio[t] = mr;
+  });
+  double max_ratio = *std::max_element(ratio.begin(), ratio.end());
+  // device layout of the regular conductances, the diagonal and the Jacobi weights
+  const size_t nbk = size_t(L.nblocks);
+  // every entry is written below
+  L.cw.resize(nbk * nb);
+  L.cs.resize(nbk * nb);
+  L.ce.resize(nbk * by);
+  L.cn.resize(nbk * bx);
+  L.dinv.resize(L.ncells);
+  std::vector<uint8_t> regular(16, 0);
+  parallel_for(long(nbk), [&](long lo, long hi, int t) {
+    for (long b = lo; b < hi; ++b) {
       const double* cx = &L.cx[b * (bx + 1) * by];
       const double* cy = &L.cy[b * bx * (by + 1)];
       for (int j = 0; j < by; ++j) {
@@ -716,25 +794,21 @@ void level_values(LevelData& L) {
         L.ce[b * by + j] = cx[j * (bx + 1) + bx];
       }
       for (int i = 0; i < bx; ++i) L.cn[b * bx + i] = cy[by * bx + i];
-    }
-  }
-  L.dinv.assign(L.ncells, 0.0);
-  bool any_regular = false;
-  for (int b = 0; b < L.nblocks; ++b)
-    for (int q = 0; q < nb; ++q) {
-      const int c = b * nb + q;
-      double d;
-      if (L.cell_irr[c] >= 0) {
-        d = L.irr_diag[L.cell_irr[c]];
-      } else {
-        const int i = q % bx, j = q / bx;
-        const double* cx = &L.cx[size_t(b) * (bx + 1) * by];
-        const double* cy = &L.cy[size_t(b) * bx * (by + 1)];
-        d = cx[j * (bx + 1) + i] + cx[j * (bx + 1) + i + 1] + cy[j * bx + i] + cy[(j + 1) * bx + i];
-        if (d > 0) any_regular = true;
+      for (int q = 0; q < nb; ++q) {
+        const long c = b * nb + q;
+        double d;
+        if (L.cell_irr[c] >= 0) {
+          d = L.irr_diag[L.cell_irr[c]];
+        } else {
+          const int i = q % bx, j = q / bx;
+          d = cx[j * (bx + 1) + i] + cx[j * (bx + 1) + i + 1] + cy[j * bx + i] + cy[(j + 1) * bx + i];
+          if (d > 0) regular[t] = 1;
+        }
+        L.dinv[c] = d > 0 ? 1.0 / d : 0.0;
       }
-      L.dinv[c] = d > 0 ? 1.0 / d : 0.0;
     }
+  });
+  const bool any_regular = *std::max_element(regular.begin(), regular.end()) != 0;
   if (any_regular) max_ratio = std::max(max_ratio, 2.0);
   if (max_ratio == 0) {
     L.omega = 1.0;
@@ -925,12 +999,14 @@ void refresh_coefficients(Hierarchy& H, const double* kappa) {
     LevelData& L = *H.levels[l];
     if (l > 0) {
       const LevelData& P = *H.levels[l - 1];
-      L.kappa.assign(L.mesh->nfaces(), 0.0);
-      for (int f = 0; f < L.mesh->nfaces(); ++f) {
-        double s = 0;
-        for (int q = L.rptr[f]; q < L.rptr[f + 1]; ++q) s += L.rw[q] * P.kappa[L.ridx[q]];
-        L.kappa[f] = s;
-      }
+      L.kappa.resize(L.mesh->nfaces());
+      parallel_for(L.mesh->nfaces(), [&](long lo, long hi, int) {
+        for (long f = lo; f < hi; ++f) {
+          double s = 0;
+          for (int q = L.rptr[f]; q < L.rptr[f + 1]; ++q) s += L.rw[q] * P.kappa[L.ridx[q]];
+          L.kappa[f] = s;
+        }
+      });
     }
     level_values(L);
   }

@@ -103,6 +103,15 @@ struct LevelData {
   std::vector<uint16_t> ell_ref;
   std::vector<double> ell_val, ell_bound;
   std::vector<int> blk_ext, ext_col;
+  // structure of the value refresh (level_pattern, built once per topology):
+  // per irregular entry its face-metric terms [term_ptr[e], term_ptr[e+1])
+  // (faces f, g, K entry, sign, diagonal flag) and its ELL value position;
+  // per row its diagonal entry and its position in ell_loc order
+  bool planned = false;
+  std::vector<int> term_ptr, term_f, term_g;
+  std::vector<double> term_k, term_s;
+  std::vector<uint8_t> term_diag;
+  std::vector<int> ell_pos, ell_row, irr_diag_e;
   double omega = 1, omega_p = 0;
   // transfer to the next coarser level (absent on the coarsest)
   std::vector<int> xfer;  // 2 per block: coarse block, mode (-1 identity, 0..3 quadrant)
