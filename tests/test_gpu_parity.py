@@ -373,3 +373,51 @@ def test_errors_are_reported():
     st = gs.solve(x, techref_noise(pm.ncells), max_cycles=0)
     assert not st.converged and st.cycles == 0
     gs.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_vcycle", "fp32_refine"])
+@pytest.mark.parametrize("name", ["amr_mixed", "circle16", "pruned_tree"])
+def test_device_refresh_matches_host_refresh(name, precision):
+    """A coefficient refresh after the first runs on the device (restriction,
+    conductances, irregular rows, Jacobi diagonal and weights, FP32 copies); it
+    performs the host pass's operations in the same order, so a solve after it
+    is bit for bit the solve after a host refresh."""
+    _, pm = meshes(name)
+    k1 = kappa_for(pm, "smooth")
+    k2 = np.exp(0.7 * np.cos(0.37 * np.arange(1, pm.nfaces + 1)))
+    x, y = pm.cell_centers()
+    rhs = np.sin(3 * x) * np.cos(5 * y) + 0.1 * np.cos(7 * x * y)
+    out = {}
+    for host in (True, False):
+        if host:
+            os.environ["TPAMR_HOST_REFRESH"] = "1"
+        try:
+            s = tpamr.EllipticSolver(tpamr.PoissonOperator(pm, k1), precision=precision, device=0)
+        finally:
+            os.environ.pop("TPAMR_HOST_REFRESH", None)
+        s.operator.replace_face_coefficient(k2)
+        s.refresh()
+        p = np.zeros(pm.ncells)
+        st = s.solve(p, rhs, rtol=1e-10)
+        nf1 = s.level_info(1)["nfaces"]
+        k_l1 = np.zeros(nf1)
+        tpamr._lib.check(tpamr.lib().tpamr_level_kappa(s.ctx.h, 1, tpamr._lib.ptr(k_l1)), s.ctx.h)
+        out[host] = (p, st.cycles, [lv["omega"] for lv in s.levels], k_l1)
+        if not host:
+            # host copies of the level values follow on demand
+            assert np.array_equal(k_l1, out[True][3])
+            # a zero coefficient passes the operator's check and is refused
+            # by the device pass before it changes any level
+            with pytest.raises(ValueError, match="positive"):
+                s.operator.replace_face_coefficient(np.where(np.arange(pm.nfaces) == 3, 0.0, k2))
+                s.refresh()
+            s.operator.replace_face_coefficient(k2)
+            s.refresh()
+            q = np.zeros(pm.ncells)
+            s.solve(q, rhs, rtol=1e-10)
+            assert np.array_equal(q, p)
+        s.close()
+    _print(name, precision, out[True][1], out[False][1])
+    assert out[True][1] == out[False][1]
+    assert out[True][2] == out[False][2]
+    assert np.array_equal(out[True][0], out[False][0])

@@ -986,6 +986,13 @@ std::unique_ptr<Hierarchy> build_hierarchy(std::unique_ptr<TreeMesh> fine, const
   return H;
 }
 
+void refresh_coarsest(Hierarchy& H, const double* kappa_coarsest) {
+  LevelData& L = *H.levels.back();
+  L.kappa.assign(kappa_coarsest, kappa_coarsest + L.mesh->nfaces());
+  level_values(L);
+  coarse_matrix(H);
+}
+
 void refresh_coefficients(Hierarchy& H, const double* kappa) {
   LevelData& L0 = *H.levels[0];
   const int F = L0.mesh->nfaces();

@@ -158,5 +158,8 @@ struct MultigridConfig {
 std::unique_ptr<Hierarchy> build_hierarchy(std::unique_ptr<TreeMesh> fine, const MultigridConfig& cfg);
 // Refresh all coefficient-dependent values for new fine face coefficients.
 void refresh_coefficients(Hierarchy& H, const double* kappa);
+// Values of the coarsest level alone from its own face coefficients, and its
+// dense augmented matrix (the device refreshes the other levels).
+void refresh_coarsest(Hierarchy& H, const double* kappa_coarsest);
 
 }  // namespace tpamr

@@ -795,6 +795,120 @@ __global__ void k_gj_extract(int n, int N, const double* __restrict__ a, double*
   inv[q] = a[r * N + c];
 }
 
+// ---------------------------------------------------------------------------
+// Coefficient refresh of one level on the device (the value pass of the
+// host's refresh_coefficients, same operations in the same order, each
+// product and sum rounded on its own)
+
+// face coefficients of level l from level l - 1: k[f] = sum_q rw_q kprev[ridx_q]
+__global__ void k_ref_restrict(int nf, const int* __restrict__ rptr, const int* __restrict__ ridx,
+                               const double* __restrict__ rw, const double* __restrict__ kprev,
+                               double* __restrict__ k) {
+#pragma clang fp contract(off)
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  double s = 0;
+  for (int q = rptr[f]; q < rptr[f + 1]; ++q) s += rw[q] * kprev[ridx[q]];
+  k[f] = s;
+}
+
+// regular faces: conductance k K_ff |f|^2 into its two slots of the device
+// layout (dst code = index * 4 + array: 0 cw, 1 ce, 2 cs, 3 cn; -1: none)
+__global__ void k_ref_regular(int nf, const double* __restrict__ k, const double* __restrict__ kdiag,
+                              const double* __restrict__ area, const int* __restrict__ dst, double* cw, double* ce,
+                              double* cs, double* cn) {
+#pragma clang fp contract(off)
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const int d0 = dst[2 * f], d1 = dst[2 * f + 1];
+  if (d0 < 0) return;
+  const double a = area[f];
+  const double w = k[f] * kdiag[f] * a * a;
+  double* arr[4] = {cw, ce, cs, cn};
+  arr[d0 & 3][d0 >> 2] = w;
+  arr[d1 & 3][d1 >> 2] = w;
+}
+
+// irregular rows: every entry is the sum of its face-metric terms; the
+// row's diagonal and the largest row ratio sum |a_rj| / a_rr (as the bits of
+// a nonnegative double, whose order is the numeric order)
+__global__ void k_ref_rows(int nrows, const int* __restrict__ irr_ptr, const int* __restrict__ term_ptr,
+                           const int* __restrict__ term_f, const int* __restrict__ term_g,
+                           const double* __restrict__ term_k, const double* __restrict__ term_s,
+                           const unsigned char* __restrict__ term_diag, const double* __restrict__ k,
+                           const double* __restrict__ area, const int* __restrict__ ell_pos,
+                           const int* __restrict__ ell_row, const int* __restrict__ diag_e,
+                           double* __restrict__ ell_val, double* __restrict__ ell_bound,
+                           double* __restrict__ ell_diag, double* __restrict__ irr_diag,
+                           unsigned long long* __restrict__ ratio) {
+#pragma clang fp contract(off)
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  double rowsum = 0, diag = 0;
+  const int de = diag_e[r];
+  for (int e = irr_ptr[r]; e < irr_ptr[r + 1]; ++e) {
+    double v = 0, bv = 0;
+    for (int q = term_ptr[e]; q < term_ptr[e + 1]; ++q) {
+      const int f = term_f[q], g = term_g[q];
+      const double kfg = term_k[q], af = area[f], ag = area[g];
+      double core, babs;
+      if (term_diag[q]) {
+        core = k[f] * kfg * af * ag;
+        babs = core;
+      } else {
+        const double sf = sqrt(k[f]), sg = sqrt(k[g]);
+        core = (sf * kfg) * sg * af * ag;
+        babs = (sf * fabs(kfg)) * sg * af * ag;
+      }
+      v += term_s[q] * core;
+      bv += babs;
+    }
+    ell_val[ell_pos[e]] = v;
+    if (ell_bound) ell_bound[ell_pos[e]] = bv;
+    rowsum += fabs(v);
+    if (e == de) diag = v;
+  }
+  irr_diag[r] = diag;
+  ell_diag[ell_row[r]] = diag;
+  if (diag > 0) atomicMax(ratio, (unsigned long long)__double_as_longlong(rowsum / diag));
+}
+
+// Jacobi diagonal inverse of every cell: the irregular row's diagonal, or the
+// sum of the four conductances; flags a positive regular diagonal
+__global__ void k_ref_dinv(int n, int bx, int by, const double* __restrict__ cw, const double* __restrict__ ce,
+                           const double* __restrict__ cs, const double* __restrict__ cn,
+                           const int* __restrict__ cell_irr, const double* __restrict__ irr_diag,
+                           double* __restrict__ dinv, int* __restrict__ regular) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int nb = bx * by, b = c / nb, q = c - b * nb, i = q % bx, j = q / bx;
+  double d;
+  const int r = cell_irr[c];
+  if (r >= 0) {
+    d = irr_diag[r];
+  } else {
+    const double e = i < bx - 1 ? cw[c + 1] : ce[b * by + j];
+    const double nn = j < by - 1 ? cs[c + bx] : cn[b * bx + i];
+    d = cw[c] + e + cs[c] + nn;
+    if (d > 0) *regular = 1;
+  }
+  dinv[c] = d > 0 ? 1.0 / d : 0.0;
+}
+
+// any face coefficient not finite or not positive
+__global__ void k_ref_check(int nf, const double* __restrict__ k, int* __restrict__ bad) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const double v = k[f];
+  if (!(v > 0) || !isfinite(v)) *bad = 1;
+}
+
+__global__ void k_sqrt(int n, const double* __restrict__ in, double* __restrict__ out) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f < n) out[f] = sqrt(in[f]);
+}
+
 // gravity! potential (src/two_phase/forces.jl:545-556): g . x at the cell
 // centre origin + (i + 1/2) h, moved by offset * h (the interface centroid of
 // a mixed cell relative to its centre, in units of the cell spacing; nullptr:
@@ -1142,6 +1256,37 @@ void launch_mixture(hipStream_t s, int nf, int nbc, const int* fm, const int* fp
                     const double* spacing, const double* c, double rl, double rg, double* kappa) {
   hipLaunchKernelGGL(k_mixture, dim3(cdiv(nf, 256)), dim3(256), 0, s, nf, nbc, fm, fp, axis, spacing, c, rl, rg,
                      kappa);
+}
+
+void launch_ref_restrict(hipStream_t s, int nf, const int* rptr, const int* ridx, const double* rw,
+                         const double* kprev, double* k) {
+  if (nf > 0) hipLaunchKernelGGL(k_ref_restrict, dim3(cdiv(nf, 256)), dim3(256), 0, s, nf, rptr, ridx, rw, kprev, k);
+}
+void launch_ref_regular(hipStream_t s, int nf, const double* k, const double* kdiag, const double* area,
+                        const int* dst, double* cw, double* ce, double* cs, double* cn) {
+  if (nf > 0)
+    hipLaunchKernelGGL(k_ref_regular, dim3(cdiv(nf, 256)), dim3(256), 0, s, nf, k, kdiag, area, dst, cw, ce, cs, cn);
+}
+void launch_ref_rows(hipStream_t s, int nrows, const int* irr_ptr, const int* term_ptr, const int* term_f,
+                     const int* term_g, const double* term_k, const double* term_s, const unsigned char* term_diag,
+                     const double* k, const double* area, const int* ell_pos, const int* ell_row, const int* diag_e,
+                     double* ell_val, double* ell_bound, double* ell_diag, double* irr_diag,
+                     unsigned long long* ratio) {
+  if (nrows > 0)
+    hipLaunchKernelGGL(k_ref_rows, dim3(cdiv(nrows, 256)), dim3(256), 0, s, nrows, irr_ptr, term_ptr, term_f,
+                       term_g, term_k, term_s, term_diag, k, area, ell_pos, ell_row, diag_e, ell_val, ell_bound,
+                       ell_diag, irr_diag, ratio);
+}
+void launch_ref_dinv(hipStream_t s, int n, int bx, int by, const double* cw, const double* ce, const double* cs,
+                     const double* cn, const int* cell_irr, const double* irr_diag, double* dinv, int* regular) {
+  hipLaunchKernelGGL(k_ref_dinv, dim3(cdiv(n, 256)), dim3(256), 0, s, n, bx, by, cw, ce, cs, cn, cell_irr, irr_diag,
+                     dinv, regular);
+}
+void launch_ref_check(hipStream_t s, int nf, const double* k, int* bad) {
+  if (nf > 0) hipLaunchKernelGGL(k_ref_check, dim3(cdiv(nf, 256)), dim3(256), 0, s, nf, k, bad);
+}
+void launch_sqrt(hipStream_t s, int n, const double* in, double* out) {
+  if (n > 0) hipLaunchKernelGGL(k_sqrt, dim3(cdiv(n, 256)), dim3(256), 0, s, n, in, out);
 }
 
 void launch_dense_inverse(hipStream_t s, int N, double* a, int* piv, double* col, int* fail, int n, double* inv) {

@@ -76,6 +76,26 @@ template <typename T>
 void launch_prolong(hipStream_t s, const LevelView<T>& L, T* x, const T* xc, double wp, const int* done, bool fine);
 template <typename T>
 void launch_coarse(hipStream_t s, int n, const double* inv, const T* b, T* x, const int* done);
+// Coefficient refresh of one level (the host value pass, on the device):
+// restricted face coefficients, regular conductances into the device layout
+// (dst: 2 codes per face, index * 4 + array 0 cw / 1 ce / 2 cs / 3 cn, -1
+// none), irregular rows from their face-metric terms (ratio: the largest row
+// ratio as double bits), the Jacobi diagonal inverse (regular: set when a
+// regular cell has a positive diagonal), a check of the coefficients (bad:
+// set when one is not finite and positive)
+void launch_ref_restrict(hipStream_t s, int nf, const int* rptr, const int* ridx, const double* rw,
+                         const double* kprev, double* k);
+void launch_ref_regular(hipStream_t s, int nf, const double* k, const double* kdiag, const double* area,
+                        const int* dst, double* cw, double* ce, double* cs, double* cn);
+void launch_ref_rows(hipStream_t s, int nrows, const int* irr_ptr, const int* term_ptr, const int* term_f,
+                     const int* term_g, const double* term_k, const double* term_s, const unsigned char* term_diag,
+                     const double* k, const double* area, const int* ell_pos, const int* ell_row, const int* diag_e,
+                     double* ell_val, double* ell_bound, double* ell_diag, double* irr_diag,
+                     unsigned long long* ratio);
+void launch_ref_dinv(hipStream_t s, int n, int bx, int by, const double* cw, const double* ce, const double* cs,
+                     const double* cn, const int* cell_irr, const double* irr_diag, double* dinv, int* regular);
+void launch_ref_check(hipStream_t s, int nf, const double* k, int* bad);
+void launch_sqrt(hipStream_t s, int n, const double* in, double* out);
 // inverse of the dense N x N row-major matrix `a` (overwritten; Gauss-Jordan
 // with partial pivoting, piv and col N-entry scratch); its top-left n x n
 // block goes to `inv` column major; *fail is set on a zero or non-finite pivot

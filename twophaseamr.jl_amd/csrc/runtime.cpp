@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdint>
 #include <cstdlib>
+#include <chrono>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -92,6 +93,18 @@ struct DevTopo {
   DevArray<double> volume;
 };
 
+// Device form of a level's coefficient refresh (uploaded once per topology):
+// restriction from the finer level, per-face data, the regular faces' slots
+// in the device layout, the irregular rows' terms, the level's face
+// coefficients and, for levels without an FP64 copy, FP64 staging values.
+struct DevPlan {
+  int nfaces = 0, nrows = 0;
+  DevArray<int> rptr, ridx, dst, irr_ptr, term_ptr, term_f, term_g, ell_pos, ell_row, diag_e, cell_irr;
+  DevArray<double> rw, kdiag, area, kappa, term_k, term_s, irr_diag;
+  DevArray<unsigned char> term_diag;
+  DevArray<double> cw, ce, cs, cn, ell_val, ell_diag, dinv;
+};
+
 template <typename T>
 struct DevLevel {
   DevArray<T> cw, ce, cs, cn, ell_val, ell_bound, ell_diag, dinv;
@@ -117,6 +130,12 @@ struct tpamr_ctx {
   std::vector<DevLevel<float>> lf;   // FP32 levels (FP32 mode)
   DevArray<double> coarse_inv, coarse_work, coarse_col;
   DevArray<int> coarse_piv, coarse_fail;
+  // coefficient refresh on the device (after the first, host-built one);
+  // TPAMR_HOST_REFRESH=1 in the environment keeps every refresh on the host
+  std::vector<DevPlan> plan;
+  bool plan_ready = false, host_refresh = false, host_stale = false;
+  DevArray<unsigned long long> ref_ratio;
+  DevArray<int> ref_flags;
   DevArray<double> x, r, p, Ap, b, tmp, tmp2;
   DevArray<double> part, maxpart, scal, stage;
   CgState* st = nullptr;      // device
@@ -266,6 +285,8 @@ void upload_level_values(tpamr_ctx* c, DevLevel<T>& D, const LevelData& L, const
   v.volume = Tp.volume.p;
 }
 
+void coarse_inverse_device(tpamr_ctx* c);
+
 // Upload topology (once per hierarchy) and values (every refresh).
 void upload_hierarchy(tpamr_ctx* c, bool topology) {
   Hierarchy& H = *c->H;
@@ -375,22 +396,7 @@ void upload_hierarchy(tpamr_ctx* c, bool topology) {
   destroy_graph(c);
   for (int l = 0; l < int(c->ld.size()); ++l) upload_level_values(c, c->ld[l], *H.levels[l], c->topo[l], l == 0);
   for (int l = 0; l < int(c->lf.size()); ++l) upload_level_values(c, c->lf[l], *H.levels[l], c->topo[l], false);
-  {
-    // dense coarse inverse on the device
-    const int n = H.coarse_n, N = n + 1;
-    c->coarse_work.upload(H.coarse_aug, s);
-    c->coarse_col.alloc(N);
-    c->coarse_piv.alloc(N);
-    c->coarse_fail.alloc(1);
-    c->coarse_inv.alloc(size_t(n) * n);
-    HIPCHECK(hipMemsetAsync(c->coarse_fail.p, 0, sizeof(int), s));
-    launch_dense_inverse(s, N, c->coarse_work.p, c->coarse_piv.p, c->coarse_col.p, c->coarse_fail.p, n,
-                         c->coarse_inv.p);
-    int fail = 0;
-    HIPCHECK(hipMemcpyAsync(&fail, c->coarse_fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipStreamSynchronize(s));
-    if (fail) throw std::runtime_error("singular coarse operator");
-  }
+  coarse_inverse_device(c);
   c->kappa.upload(H.levels[0]->kappa, s);
   std::vector<double> sq(H.levels[0]->kappa.size());
   for (size_t f = 0; f < sq.size(); ++f) sq[f] = std::sqrt(H.levels[0]->kappa[f]);
@@ -860,6 +866,194 @@ void weighted_metric(tpamr_ctx* c, const double* in_scaled, double* out) {
   launch_face_scale(c->stream, F, c->sqk.p, c->f4.p, out);
 }
 
+// Dense coarse inverse of H.coarse_aug on the device into c->coarse_inv.
+void coarse_inverse_device(tpamr_ctx* c) {
+  Hierarchy& H = *c->H;
+  hipStream_t s = c->stream;
+  const int n = H.coarse_n, N = n + 1;
+  c->coarse_work.upload(H.coarse_aug, s);
+  c->coarse_col.alloc(N);
+  c->coarse_piv.alloc(N);
+  c->coarse_fail.alloc(1);
+  c->coarse_inv.alloc(size_t(n) * n);
+  HIPCHECK(hipMemsetAsync(c->coarse_fail.p, 0, sizeof(int), s));
+  launch_dense_inverse(s, N, c->coarse_work.p, c->coarse_piv.p, c->coarse_col.p, c->coarse_fail.p, n,
+                       c->coarse_inv.p);
+  int fail = 0;
+  HIPCHECK(hipMemcpyAsync(&fail, c->coarse_fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  if (fail) throw std::runtime_error("singular coarse operator");
+}
+
+void upload_plan(tpamr_ctx* c) {
+  Hierarchy& H = *c->H;
+  hipStream_t s = c->stream;
+  const int nl = int(H.levels.size());
+  c->plan.clear();
+  c->plan.resize(nl);
+  for (int l = 0; l < nl; ++l) {
+    const LevelData& L = *H.levels[l];
+    const TreeMesh& M = *L.mesh;
+    DevPlan& P = c->plan[l];
+    const int F = M.nfaces(), bx = M.bs[0], by = M.bs[1], nb = bx * by;
+    P.nfaces = F;
+    P.nrows = int(L.irr_local.size());
+    if (l > 0) {
+      P.rptr.upload(L.rptr, s);
+      P.ridx.upload(L.ridx.empty() ? std::vector<int>(1, 0) : L.ridx, s);
+      P.rw.upload(L.rw.empty() ? std::vector<double>(1, 0.0) : L.rw, s);
+    }
+    P.kdiag.upload(L.K.diag, s);
+    P.area.upload(M.farea, s);
+    P.kappa.alloc(F);
+    // regular faces: slot in cx / cy -> code index * 4 + array in the device layout
+    std::vector<int> dst(2 * size_t(F), -1);
+    auto code = [&](int axis, int64_t slot) {
+      if (axis == 0) {
+        const int64_t per = int64_t(bx + 1) * by, b = slot / per, rem = slot % per, j = rem / (bx + 1),
+                      i = rem % (bx + 1);
+        return i < bx ? int((b * nb + j * bx + i) * 4 + 0) : int((b * by + j) * 4 + 1);
+      }
+      const int64_t per = int64_t(bx) * (by + 1), b = slot / per, rem = slot % per, j = rem / bx, i = rem % bx;
+      return j < by ? int((b * nb + j * bx + i) * 4 + 2) : int((b * bx + i) * 4 + 3);
+    };
+    for (int f = 0; f < F; ++f)
+      if (L.face_regular[f]) {
+        dst[2 * size_t(f)] = code(M.faxis[f], L.slot_a[f]);
+        dst[2 * size_t(f) + 1] = code(M.faxis[f], L.slot_b[f]);
+      }
+    P.dst.upload(dst, s);
+    auto nz = [](const auto& v, auto z) { return v.empty() ? std::vector<decltype(z)>(1, z) : v; };
+    P.irr_ptr.upload(L.irr_ptr, s);
+    P.term_ptr.upload(L.term_ptr, s);
+    P.term_f.upload(nz(L.term_f, 0), s);
+    P.term_g.upload(nz(L.term_g, 0), s);
+    P.term_k.upload(nz(L.term_k, 0.0), s);
+    P.term_s.upload(nz(L.term_s, 0.0), s);
+    P.term_diag.upload(nz(L.term_diag, (unsigned char)0), s);
+    P.ell_pos.upload(nz(L.ell_pos, 0), s);
+    P.ell_row.upload(nz(L.ell_row, 0), s);
+    P.diag_e.upload(nz(L.irr_diag_e, 0), s);
+    P.cell_irr.upload(L.cell_irr, s);
+    P.irr_diag.alloc(std::max(P.nrows, 1));
+    if (l >= int(c->ld.size())) {
+      P.cw.upload(L.cw, s);
+      P.ce.upload(L.ce, s);
+      P.cs.upload(L.cs, s);
+      P.cn.upload(L.cn, s);
+      P.ell_val.upload(nz(L.ell_val, 0.0), s);
+      P.ell_diag.upload(nz(L.ell_diag, 0.0), s);
+      P.dinv.upload(L.dinv, s);
+    }
+  }
+  c->ref_ratio.alloc(nl);
+  c->ref_flags.alloc(nl + 1);
+  c->plan_ready = true;
+}
+
+// A coefficient refresh computed on the device (topology unchanged): the
+// value pass of every level, FP32 copies, the finest level's sqrt(k), the
+// Jacobi weights (one read-back) and the dense coarse inverse. The host keeps
+// the coarsest level's values (for its dense matrix); the other levels' host
+// values are refreshed only when asked for (host_stale).
+void device_refresh(tpamr_ctx* c, const double* kappa) {
+  Hierarchy& H = *c->H;
+  hipStream_t s = c->stream;
+  const int nl = int(H.levels.size());
+  if (!c->plan_ready) upload_plan(c);
+  DevPlan& P0 = c->plan[0];
+  upload_vec(c, P0.kappa.p, kappa, P0.nfaces);
+  HIPCHECK(hipMemsetAsync(c->ref_ratio.p, 0, sizeof(unsigned long long) * nl, s));
+  HIPCHECK(hipMemsetAsync(c->ref_flags.p, 0, sizeof(int) * (nl + 1), s));
+  launch_ref_check(s, P0.nfaces, P0.kappa.p, c->ref_flags.p + nl);
+  int bad = 0;
+  HIPCHECK(hipMemcpyAsync(&bad, c->ref_flags.p + nl, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  if (bad)
+    throw std::invalid_argument(
+        "face coefficients must be finite and positive (disconnected components are not supported)");
+  destroy_graph(c);
+  for (int l = 0; l < nl; ++l) {
+    const LevelData& L = *H.levels[l];
+    DevPlan& P = c->plan[l];
+    const int bx = L.mesh->bs[0], by = L.mesh->bs[1];
+    if (l > 0) launch_ref_restrict(s, P.nfaces, P.rptr.p, P.ridx.p, P.rw.p, c->plan[l - 1].kappa.p, P.kappa.p);
+    const bool has64 = l < int(c->ld.size());
+    DevLevel<double>* D = has64 ? &c->ld[l] : nullptr;
+    double* cw = has64 ? D->cw.p : P.cw.p;
+    double* ce = has64 ? D->ce.p : P.ce.p;
+    double* cs = has64 ? D->cs.p : P.cs.p;
+    double* cn = has64 ? D->cn.p : P.cn.p;
+    double* ev = has64 ? D->ell_val.p : P.ell_val.p;
+    double* ed = has64 ? D->ell_diag.p : P.ell_diag.p;
+    double* di = has64 ? D->dinv.p : P.dinv.p;
+    double* eb = (has64 && l == 0) ? D->ell_bound.p : nullptr;
+    launch_fill(s, int(L.cw.size()), 0.0, cw);
+    launch_fill(s, int(L.ce.size()), 0.0, ce);
+    launch_fill(s, int(L.cs.size()), 0.0, cs);
+    launch_fill(s, int(L.cn.size()), 0.0, cn);
+    launch_ref_regular(s, P.nfaces, P.kappa.p, P.kdiag.p, P.area.p, P.dst.p, cw, ce, cs, cn);
+    launch_ref_rows(s, P.nrows, P.irr_ptr.p, P.term_ptr.p, P.term_f.p, P.term_g.p, P.term_k.p, P.term_s.p,
+                    P.term_diag.p, P.kappa.p, P.area.p, P.ell_pos.p, P.ell_row.p, P.diag_e.p, ev, eb, ed,
+                    P.irr_diag.p, c->ref_ratio.p + l);
+    launch_ref_dinv(s, L.ncells, bx, by, cw, ce, cs, cn, P.cell_irr.p, P.irr_diag.p, di, c->ref_flags.p + l);
+    if (l < int(c->lf.size())) {
+      DevLevel<float>& Fv = c->lf[l];
+      launch_cast_d2f(s, int(L.cw.size()), cw, Fv.cw.p);
+      launch_cast_d2f(s, int(L.ce.size()), ce, Fv.ce.p);
+      launch_cast_d2f(s, int(L.cs.size()), cs, Fv.cs.p);
+      launch_cast_d2f(s, int(L.cn.size()), cn, Fv.cn.p);
+      if (!L.ell_val.empty()) launch_cast_d2f(s, int(L.ell_val.size()), ev, Fv.ell_val.p);
+      if (!L.ell_diag.empty()) launch_cast_d2f(s, int(L.ell_diag.size()), ed, Fv.ell_diag.p);
+      launch_cast_d2f(s, L.ncells, di, Fv.dinv.p);
+    }
+  }
+  HIPCHECK(hipMemcpyAsync(c->kappa.p, P0.kappa.p, sizeof(double) * P0.nfaces, hipMemcpyDeviceToDevice, s));
+  launch_sqrt(s, P0.nfaces, P0.kappa.p, c->sqk.p);
+  // Jacobi weights: 4 / (3 max ratio), the ratio at least 2 with a regular row
+  std::vector<unsigned long long> ratio(nl);
+  std::vector<int> regular(nl);
+  HIPCHECK(hipMemcpyAsync(ratio.data(), c->ref_ratio.p, sizeof(unsigned long long) * nl, hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipMemcpyAsync(regular.data(), c->ref_flags.p, sizeof(int) * nl, hipMemcpyDeviceToHost, s));
+  // the coarsest level's face coefficients, for its dense matrix on the host
+  std::vector<double> kc(c->plan[nl - 1].nfaces);
+  HIPCHECK(hipMemcpyAsync(kc.data(), c->plan[nl - 1].kappa.p, sizeof(double) * kc.size(), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  for (int l = 0; l < nl; ++l) {
+    double mr;
+    std::memcpy(&mr, &ratio[l], sizeof(double));
+    if (regular[l]) mr = std::max(mr, 2.0);
+    double w = 1.0, wp = 0.0;
+    if (mr != 0) {
+      wp = 4.0 / (3.0 * mr);
+      w = std::min(1.0, wp);
+    }
+    LevelData& L = *H.levels[l];
+    L.omega = w;
+    L.omega_p = wp;
+    if (l < int(c->ld.size())) {
+      c->ld[l].omega = w;
+      c->ld[l].omega_p = wp;
+    }
+    if (l < int(c->lf.size())) {
+      c->lf[l].omega = w;
+      c->lf[l].omega_p = wp;
+    }
+  }
+  refresh_coarsest(H, kc.data());
+  coarse_inverse_device(c);
+  c->host_stale = true;
+}
+
+// Host copies of the level values after device refreshes (for inspection).
+void ensure_host_values(tpamr_ctx* c) {
+  if (!c->host_stale) return;
+  std::vector<double> k0(c->plan[0].nfaces);
+  download_vec(c, k0.data(), c->kappa.p, k0.size());
+  refresh_coefficients(*c->H, k0.data());
+  c->host_stale = false;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -878,6 +1072,7 @@ int tpamr_create(tpamr_ctx** out, int device, int precision) {
   if (const char* g = std::getenv("TPAMR_GRAPH")) c->use_graph = std::strcmp(g, "0") != 0;
   if (const char* q = std::getenv("TPAMR_PIPE")) c->pipe = std::min(PIPE_MAX, std::max(2, std::atoi(q)));
   if (const char* q = std::getenv("TPAMR_TRACE")) c->trace = std::strcmp(q, "0") != 0;
+  if (const char* q = std::getenv("TPAMR_HOST_REFRESH")) c->host_refresh = std::strcmp(q, "0") != 0;
   if (const char* q = std::getenv("TPAMR_GRAPH_ITERS")) c->graph_iters = std::min(16, std::max(1, std::atoi(q)));
   if (const char* q = std::getenv("TPAMR_TAIL_BLOCKS")) c->tail_blocks = std::max(0, std::atoi(q));
   if (const char* q = std::getenv("TPAMR_IR_DELTA")) {
@@ -1039,9 +1234,25 @@ int tpamr_set_face_coefficients(tpamr_ctx* c, const double* kappa) {
       c->H = build_hierarchy(std::move(c->mesh_pending), c->cfg);
       topology = true;
     }
-    refresh_coefficients(*c->H, kappa);
+    const auto t0 = std::chrono::steady_clock::now();
+    const bool on_device = c->device >= 0 && !topology && !c->host_refresh;
+    if (!on_device) {
+      refresh_coefficients(*c->H, kappa);
+      c->host_stale = false;
+    }
     c->have_coeff = true;
-    if (c->device >= 0) upload_hierarchy(c, topology);
+    const auto t1 = std::chrono::steady_clock::now();
+    if (on_device)
+      device_refresh(c, kappa);
+    else if (c->device >= 0)
+      upload_hierarchy(c, topology);
+    if (c->trace) {
+      const auto t2 = std::chrono::steady_clock::now();
+      std::fprintf(stderr, "tpamr refresh (%s): host values %.1f ms, device part %.1f ms\n",
+                   on_device ? "device" : "host",
+                   std::chrono::duration<double, std::milli>(t1 - t0).count(),
+                   std::chrono::duration<double, std::milli>(t2 - t1).count());
+    }
   });
 }
 
@@ -1081,6 +1292,7 @@ int tpamr_level_matrix(const tpamr_ctx* cc, int level, int* nnz, int* rowptr, in
   return guard(c, [&] {
     need_solver(c);
     if (level < 0 || level >= int(c->H->levels.size())) throw std::invalid_argument("level out of range");
+    ensure_host_values(c);
     const LevelData& L = *c->H->levels[level];
     const TreeMesh& M = *L.mesh;
     const int bx = M.bs[0], by = M.bs[1], nb = bx * by;
@@ -1146,6 +1358,7 @@ int tpamr_level_kappa(const tpamr_ctx* cc, int level, double* kappa) {
   return guard(c, [&] {
     need_solver(c);
     if (level < 0 || level >= int(c->H->levels.size())) throw std::invalid_argument("level out of range");
+    ensure_host_values(c);
     const LevelData& L = *c->H->levels[level];
     std::copy(L.kappa.begin(), L.kappa.end(), kappa);
   });
