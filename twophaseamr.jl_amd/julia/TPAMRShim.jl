@@ -174,4 +174,42 @@ function TwoPhaseAMR._run_krylov!(ws::GpuCgWorkspace, matrix, rhs, x0, precondit
     return ws
 end
 
+"""
+    gravity!(ws, acceleration, offsets, liquid, gas; density, gravity)
+
+`gravity!` (src/two_phase/forces.jl:528-580) on the GPU: `offsets` holds, per cell,
+the interface centroid relative to the cell centre in cell spacings (zero in pure
+cells; the reconstruction's `_interface_centroid`), `liquid` and `gas` the phase
+face apertures (`_current_phase_face_apertures!`). Writes the face accelerations.
+"""
+function gravity!(ws::GpuCgWorkspace, acceleration::Vector{Float64}, offsets::Vector{Float64},
+        liquid::Vector{Float64}, gas::Vector{Float64}; density::NTuple{2}, gravity::NTuple{2})
+    GC.@preserve acceleration offsets liquid gas _check(ws.ctx, ccall((:tpamr_gravity, LIB), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Float64, Float64, Ptr{Float64}),
+        ws.ctx, offsets, liquid, gas, density[1], density[2], gravity[1], gravity[2], acceleration))
+    return acceleration
+end
+
+"""
+    project_pullback(ws, ubar, phi, jumps, dt; rtol = 1e-10, max_cycles = 200)
+
+Pullback of one force-plus-projection step with the operator's current
+coefficients (no reference counterpart): returns `(dJ/du, dJ/db, dJ/dκ, stats)`
+for `ubar = dJ/du'`, the step's checkpointed pressure increment `phi` and its
+force jumps (`nothing`: no force).
+"""
+function project_pullback(ws::GpuCgWorkspace, ubar::Vector{Float64}, phi::Vector{Float64}, jumps, dt::Real;
+        rtol::Real = 1e-10, max_cycles::Integer = 200)
+    ustar = similar(ubar)
+    db = Vector{Float64}(undef, length(phi))
+    dk = similar(ubar)
+    st = Ref{TpamrStats}()
+    j = jumps === nothing ? C_NULL : jumps
+    GC.@preserve ubar phi jumps ustar db dk _check(ws.ctx, ccall((:tpamr_project_adjoint, LIB), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Cint, Ptr{Float64}, Ptr{Float64},
+         Ptr{Float64}, Ref{TpamrStats}),
+        ws.ctx, ubar, phi, j, Float64(dt), Float64(rtol), Cint(max_cycles), ustar, db, dk, st))
+    return ustar, db, dk, st[]
+end
+
 end # module
