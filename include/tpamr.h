@@ -140,6 +140,16 @@ int tpamr_mixture_face_coefficients(tpamr_ctx* ctx, const double* fraction, doub
  * face_curvature per face (NaN = unresolved) */
 int tpamr_surface_tension_jumps(tpamr_ctx* ctx, const double* fraction, const double* face_curvature, double sigma,
                                 int formulation, double* jumps);
+/* _surface_tension_jumps! with the face curvature of _face_curvature
+ * (forces.jl:211-300,345-378): per cell the reconstruction's curvature (NaN =
+ * none) and its height-function coverage in [0, 1]; the face takes the
+ * LHF-first c(1-c) interpolant of its two cells, or, when both are pure, the
+ * c(1-c)-weighted extension of the cell curvatures within 3 h of the face
+ * centre. `unresolved` (may be NULL) receives the number of jump faces left
+ * without a curvature (their jump is zero). */
+int tpamr_surface_tension_jumps_cells(tpamr_ctx* ctx, const double* fraction, const double* cell_curvature,
+                                      const double* curvature_quality, double sigma, int formulation, double* jumps,
+                                      int* unresolved);
 /* _weighted_jump_force! (forces.jl:179-209): accel = sqrt(k) K (sqrt(k) jumps) */
 int tpamr_weighted_jump_force(tpamr_ctx* ctx, const double* jumps, double* accel);
 

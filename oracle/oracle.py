@@ -184,6 +184,28 @@ class OMesh:
                                     C.c_double(sigma), int(csf), _dptr(out)))
         return out
 
+    def face_curvature(self, c, kappa, quality, faces):
+        """`_face_curvature` (src/two_phase/forces.jl:211-300) of the given
+        faces; NaN for `nothing`."""
+        faces = np.ascontiguousarray(faces, np.int32)
+        out = np.zeros(len(faces))
+        _check(lib().orc_face_curvature(self._h, _dptr(np.ascontiguousarray(c, np.float64)),
+                                        _dptr(np.ascontiguousarray(kappa, np.float64)),
+                                        _dptr(np.ascontiguousarray(quality, np.float64)), len(faces), _iptr(faces),
+                                        _dptr(out)))
+        return out
+
+    def surface_tension_jumps_cells(self, c, kappa, quality, sigma, csf=False):
+        """`_surface_tension_jumps!` with `_face_curvature`; returns (jumps,
+        unresolved faces)."""
+        out = np.zeros(self.nfaces)
+        n = C.c_int()
+        _check(lib().orc_surface_tension_jumps_cells(self._h, _dptr(np.ascontiguousarray(c, np.float64)),
+                                                     _dptr(np.ascontiguousarray(kappa, np.float64)),
+                                                     _dptr(np.ascontiguousarray(quality, np.float64)),
+                                                     C.c_double(sigma), int(csf), _dptr(out), C.byref(n)))
+        return out, n.value
+
     def gravity_jumps(self, liquid, gas, density, gravity, offsets=None):
         """gravity! face jumps (src/two_phase/forces.jl:528-580)."""
         out = np.zeros(self.nfaces)

@@ -14,6 +14,9 @@ void apply_vcycle(Solver& S, const double* r, double* z);
 Stats solve(Solver& S, const double* rhs, double* xio, double rtol, double atol, int max_cycles,
             int compat_error, std::vector<double>* removed_means);
 void mixture_coefficients(const Mesh& M, const double* c, double rho_l, double rho_g, double* kappa);
+double face_curvature(const Mesh& M, const double* c, const double* kappa, const double* quality, int fi);
+int surface_tension_jumps_cells(const Mesh& M, const double* c, const double* kappa, const double* quality,
+                                double sigma, int csf, double* jumps);
 void gravity_jumps(const Mesh& M, const double* offsets, const double* liquid, const double* gas, double rho_l,
                    double rho_g, double gx, double gy, double* jumps);
 void cgfm_jumps(const Mesh& M, const double* c, const double* curvature, double sigma, int csf,
@@ -207,6 +210,23 @@ int orc_mixture_coefficients(void* mp, const double* c, double rho_l, double rho
 
 int orc_cgfm_jumps(void* mp, const double* c, const double* curvature, double sigma, int csf, double* out) {
   GUARD({ cgfm_jumps(*static_cast<Mesh*>(mp), c, curvature, sigma, csf, out); return 0; })
+}
+
+int orc_face_curvature(void* mp, const double* c, const double* kappa, const double* quality, int n,
+                       const int* faces, double* out) {
+  GUARD({
+    const Mesh& M = *static_cast<Mesh*>(mp);
+    for (int q = 0; q < n; ++q) out[q] = face_curvature(M, c, kappa, quality, faces[q]);
+    return 0;
+  })
+}
+
+int orc_surface_tension_jumps_cells(void* mp, const double* c, const double* kappa, const double* quality,
+                                    double sigma, int csf, double* jumps, int* unresolved) {
+  GUARD({
+    *unresolved = surface_tension_jumps_cells(*static_cast<Mesh*>(mp), c, kappa, quality, sigma, csf, jumps);
+    return 0;
+  })
 }
 
 int orc_gravity_jumps(void* mp, const double* offsets, const double* liquid, const double* gas, double rho_l,

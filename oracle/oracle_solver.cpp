@@ -911,6 +911,101 @@ void cgfm_jumps(const Mesh& M, const double* c, const double* curvature, double 
   }
 }
 
+// `_wrapped_delta` (operators/gradient.jl:97-100): Julia's round is
+// round-half-even, as rint in the default rounding mode.
+static double wrapped_delta(double d, double extent, bool periodic) {
+  return periodic ? d - std::rint(d / extent) * extent : d;
+}
+
+// `_face_curvature` (forces.jl:211-300): the LHF-first c(1-c) interpolant of
+// the two adjacent cell curvatures (NaN curvature = none), else the
+// c(1-c)-weighted extension of the cell curvatures within 3 h of the face
+// centre (blocks and cells in order); NaN for `nothing`.
+double face_curvature(const Mesh& M, const double* c, const double* kappa, const double* quality, int fi) {
+  const Face& fa = M.faces[fi];
+  const double km = kappa[fa.minus], kp = kappa[fa.plus];
+  const double cm = c[fa.minus], cp = c[fa.plus];
+  const double wm = std::isfinite(km) ? cm * (1 - cm) : 0.0;
+  const double wp = std::isfinite(kp) ? cp * (1 - cp) : 0.0;
+  if (wm > 0 || wp > 0) {
+    const double qm = wm > 0 ? quality[fa.minus] : 0.0;
+    const double qp = wp > 0 ? quality[fa.plus] : 0.0;
+    const double pref = std::max(qm, qp);
+    double total = 0, weight = 0;
+    if (wm > 0) {
+      const double w = wm * (qm + (1 - pref) * (1 - qm));
+      total += w * km;
+      weight += w;
+    }
+    if (wp > 0) {
+      const double w = wp * (qp + (1 - pref) * (1 - qp));
+      total += w * kp;
+      weight += w;
+    }
+    return total / weight;
+  }
+  const int bx = M.bs[0], by = M.bs[1], nb = bx * by;
+  const Block& bm = M.blocks[fa.minus / nb];
+  const Block& bp = M.blocks[fa.plus / nb];
+  const double local = std::max(std::max(bm.spacing[0], bm.spacing[1]), std::max(bp.spacing[0], bp.spacing[1]));
+  const double radius = 3 * local;
+  const double emax = std::max(M.extent[0], M.extent[1]);
+  double total = 0, weight = 0;
+  for (size_t b = 0; b < M.blocks.size(); ++b) {
+    const Block& B = M.blocks[b];
+    const double hs = std::max(B.spacing[0], B.spacing[1]);
+    const double dx = std::fabs(wrapped_delta(B.origin[0] + B.extent[0] / 2 - fa.center[0], M.extent[0], M.periodic[0]));
+    const double dy = std::fabs(wrapped_delta(B.origin[1] + B.extent[1] / 2 - fa.center[1], M.extent[1], M.periodic[1]));
+    const double margin = 16 * 2.220446049250313e-16 *
+                          std::max(std::max(std::max(emax, std::fabs(B.origin[0])), std::max(std::fabs(B.origin[1]),
+                                   std::fabs(fa.center[0]))), std::fabs(fa.center[1]));
+    const double reach = radius + hs + margin;
+    if (dx > B.extent[0] / 2 + reach || dy > B.extent[1] / 2 + reach) continue;
+    for (int j = 1; j <= by; ++j)
+      for (int i = 1; i <= bx; ++i) {
+        const size_t cell = b * nb + size_t(j - 1) * bx + (i - 1);
+        const double cc = c[cell], k = kappa[cell];
+        const double sw = cc * (1 - cc);
+        if (!(sw > 0 && std::isfinite(k))) continue;
+        const double px = B.origin[0] + (i - 0.5) * B.spacing[0];
+        const double py = B.origin[1] + (j - 0.5) * B.spacing[1];
+        const double ddx = wrapped_delta(px - fa.center[0], M.extent[0], M.periodic[0]);
+        const double ddy = wrapped_delta(py - fa.center[1], M.extent[1], M.periodic[1]);
+        const double dist = std::hypot(ddx, ddy);
+        if (!(dist <= radius + hs)) continue;
+        const double gw = sw / std::max(dist * dist, (hs / 4) * (hs / 4));
+        total += gw * k;
+        weight += gw;
+      }
+  }
+  return weight > 0 ? total / weight : std::nan("");
+}
+
+// `_surface_tension_jumps!` (forces.jl:345-378) with the face curvature of
+// `_face_curvature`; returns the number of unresolved faces.
+int surface_tension_jumps_cells(const Mesh& M, const double* c, const double* kappa, const double* quality,
+                                double sigma, int csf, double* jumps) {
+  if (!(std::isfinite(sigma) && sigma >= 0)) throw std::invalid_argument("sigma must be finite and nonnegative");
+  int unresolved = 0;
+  for (size_t f = 0; f < M.faces.size(); ++f) {
+    const Face& fa = M.faces[f];
+    const double cm = c[fa.minus], cp = c[fa.plus];
+    const double jump = csf ? cp - cm : double(cp > 0.5) - double(cm > 0.5);
+    if (sigma == 0 || jump == 0) {
+      jumps[f] = 0;
+      continue;
+    }
+    const double k = face_curvature(M, c, kappa, quality, int(f));
+    if (std::isnan(k)) {
+      ++unresolved;
+      jumps[f] = 0;
+      continue;
+    }
+    jumps[f] = sigma * k * jump * fa.area;
+  }
+  return unresolved;
+}
+
 // `gravity!` jumps (forces.jl:528-580): potential g . x at the cell centre
 // (`_leaf_cell_center`, mesh.jl:1142-1145), moved by the interface centroid
 // offset (in cell spacings; offsets == nullptr: none), then per face

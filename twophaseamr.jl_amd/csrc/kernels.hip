@@ -506,9 +506,11 @@ __global__ void __launch_bounds__(BX* BY) k_zstats(const T* __restrict__ z, cons
 // ---------------------------------------------------------------------------
 // elementwise / face kernels
 
-__global__ void k_axpby(int n, double a, const double* __restrict__ x, double b, double* __restrict__ y) {
+// y = a x + b y; with b = 0, y is only written (it may hold anything, NaN
+// included, before); x may alias y
+__global__ void k_axpby(int n, double a, const double* x, double b, double* y) {
   const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c < size_t(n)) y[c] = a * x[c] + b * y[c];
+  if (c < size_t(n)) y[c] = b != 0.0 ? a * x[c] + b * y[c] : a * x[c];
 }
 
 // r = b - y with per-block strong maxima |r|/V
@@ -793,6 +795,111 @@ __global__ void k_gj_extract(int n, int N, const double* __restrict__ a, double*
   if (q >= size_t(n) * n) return;
   const size_t c = q / n, r = q % n;
   inv[q] = a[r * N + c];
+}
+
+// ---------------------------------------------------------------------------
+// Surface-tension jumps with the face curvature of `_face_curvature`
+// (src/two_phase/forces.jl:211-300, 345-378)
+
+struct CurvMesh {
+  const double* origin;   // 2 per block
+  const double* spacing;  // 2 per block
+  const double* fcx;      // face centres
+  const double* fcy;
+  int nblocks, bx, by;
+  double extent[2];
+  int periodic[2];
+};
+
+__device__ inline double wrapped_delta(double d, double extent, int periodic) {
+#pragma clang fp contract(off)
+  return periodic ? d - rint(d / extent) * extent : d;
+}
+
+// The LHF-first c(1-c) interpolant of the two cells' curvatures (NaN = none),
+// else the c(1-c)-weighted extension of the cell curvatures within 3 h of the
+// face centre over all blocks in order (rare: both cells pure); NaN when
+// there is none. Rounded operation by operation like the reference.
+__device__ double face_curvature(const CurvMesh& M, int f, int m, int p, const double* __restrict__ c,
+                                 const double* __restrict__ kappa, const double* __restrict__ quality) {
+#pragma clang fp contract(off)
+  const double km = kappa[m], kp = kappa[p], cm = c[m], cp = c[p];
+  const double wm = isfinite(km) ? cm * (1 - cm) : 0.0;
+  const double wp = isfinite(kp) ? cp * (1 - cp) : 0.0;
+  if (wm > 0 || wp > 0) {
+    const double qm = wm > 0 ? quality[m] : 0.0;
+    const double qp = wp > 0 ? quality[p] : 0.0;
+    const double pref = fmax(qm, qp);
+    double total = 0, weight = 0;
+    if (wm > 0) {
+      const double w = wm * (qm + (1 - pref) * (1 - qm));
+      total += w * km;
+      weight += w;
+    }
+    if (wp > 0) {
+      const double w = wp * (qp + (1 - pref) * (1 - qp));
+      total += w * kp;
+      weight += w;
+    }
+    return total / weight;
+  }
+  const int nb = M.bx * M.by, bm = m / nb, bp = p / nb;
+  const double local = fmax(fmax(M.spacing[2 * bm], M.spacing[2 * bm + 1]), fmax(M.spacing[2 * bp], M.spacing[2 * bp + 1]));
+  const double radius = 3 * local;
+  const double fx = M.fcx[f], fy = M.fcy[f];
+  const double emax = fmax(M.extent[0], M.extent[1]);
+  double total = 0, weight = 0;
+  for (int b = 0; b < M.nblocks; ++b) {
+    const double hx = M.spacing[2 * b], hy = M.spacing[2 * b + 1];
+    const double ox = M.origin[2 * b], oy = M.origin[2 * b + 1];
+    const double ex = hx * M.bx, ey = hy * M.by;
+    const double hs = fmax(hx, hy);
+    const double dx = fabs(wrapped_delta(ox + ex / 2 - fx, M.extent[0], M.periodic[0]));
+    const double dy = fabs(wrapped_delta(oy + ey / 2 - fy, M.extent[1], M.periodic[1]));
+    const double margin = 16 * 2.220446049250313e-16 *
+                          fmax(fmax(fmax(emax, fabs(ox)), fmax(fabs(oy), fabs(fx))), fabs(fy));
+    const double reach = radius + hs + margin;
+    if (dx > ex / 2 + reach || dy > ey / 2 + reach) continue;
+    for (int j = 0; j < M.by; ++j)
+      for (int i = 0; i < M.bx; ++i) {
+        const int cell = b * nb + j * M.bx + i;
+        const double cc = c[cell], k = kappa[cell];
+        const double sw = cc * (1 - cc);
+        if (!(sw > 0 && isfinite(k))) continue;
+        const double px = ox + (i + 0.5) * hx, py = oy + (j + 0.5) * hy;
+        const double ddx = wrapped_delta(px - fx, M.extent[0], M.periodic[0]);
+        const double ddy = wrapped_delta(py - fy, M.extent[1], M.periodic[1]);
+        const double dist = hypot(ddx, ddy);
+        if (!(dist <= radius + hs)) continue;
+        const double gw = sw / fmax(dist * dist, (hs / 4) * (hs / 4));
+        total += gw * k;
+        weight += gw;
+      }
+  }
+  return weight > 0 ? total / weight : __builtin_nan("");
+}
+
+__global__ void k_surface_jump_cells(int nf, const int* __restrict__ fm, const int* __restrict__ fp,
+                                     const double* __restrict__ area, CurvMesh M, const double* __restrict__ c,
+                                     const double* __restrict__ kappa, const double* __restrict__ quality,
+                                     double sigma, int csf, double* __restrict__ jumps, int* __restrict__ unresolved) {
+#pragma clang fp contract(off)
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const int m = fm[f], p = fp[f];
+  const double cm = c[m], cp = c[p];
+  const double jump = csf ? cp - cm : double(cp > 0.5) - double(cm > 0.5);
+  if (sigma == 0 || jump == 0) {
+    jumps[f] = 0.0;
+    return;
+  }
+  const double k = face_curvature(M, f, m, p, c, kappa, quality);
+  if (k != k) {
+    atomicAdd(unresolved, 1);
+    jumps[f] = 0.0;
+    return;
+  }
+  jumps[f] = sigma * k * jump * area[f];
 }
 
 // ---------------------------------------------------------------------------
@@ -1256,6 +1363,16 @@ void launch_mixture(hipStream_t s, int nf, int nbc, const int* fm, const int* fp
                     const double* spacing, const double* c, double rl, double rg, double* kappa) {
   hipLaunchKernelGGL(k_mixture, dim3(cdiv(nf, 256)), dim3(256), 0, s, nf, nbc, fm, fp, axis, spacing, c, rl, rg,
                      kappa);
+}
+
+void launch_surface_jump_cells(hipStream_t s, int nf, const int* fm, const int* fp, const double* area,
+                               const double* origin, const double* spacing, const double* fcx, const double* fcy,
+                               int nblocks, int bx, int by, const double extent[2], const bool periodic[2],
+                               const double* c, const double* kappa, const double* quality, double sigma, int csf,
+                               double* jumps, int* unresolved) {
+  CurvMesh M{origin, spacing, fcx, fcy, nblocks, bx, by, {extent[0], extent[1]}, {periodic[0], periodic[1]}};
+  hipLaunchKernelGGL(k_surface_jump_cells, dim3(cdiv(nf, 256)), dim3(256), 0, s, nf, fm, fp, area, M, c, kappa,
+                     quality, sigma, csf, jumps, unresolved);
 }
 
 void launch_ref_restrict(hipStream_t s, int nf, const int* rptr, const int* ridx, const double* rw,

@@ -194,6 +194,14 @@ void launch_mixture(hipStream_t s, int nf, int nbc, const int* fm, const int* fp
                     const double* spacing, const double* c, double rl, double rg, double* kappa);
 void launch_surface_jump(hipStream_t s, int nf, const int* fm, const int* fp, const double* area, const double* c,
                          const double* curv, double sigma, int csf, double* jumps);
+// CGFM / CSF jumps with the face curvature of `_face_curvature` from cell
+// curvatures (NaN = none) and their height-function coverage; counts the
+// faces left without a curvature
+void launch_surface_jump_cells(hipStream_t s, int nf, const int* fm, const int* fp, const double* area,
+                               const double* origin, const double* spacing, const double* fcx, const double* fcy,
+                               int nblocks, int bx, int by, const double extent[2], const bool periodic[2],
+                               const double* c, const double* kappa, const double* quality, double sigma, int csf,
+                               double* jumps, int* unresolved);
 void launch_divergence(hipStream_t s, int n, int nbc, const int* cptr, const int* cface, const signed char* csign,
                        const double* area, const double* u, const int* bptr, const int* bface,
                        const signed char* bside, const double* barea, const double* ub, const double* vol,
@@ -204,6 +212,7 @@ void launch_adjoint_face(hipStream_t s, int nf, const double* yl, const double* 
 template <typename T>
 void launch_zstats(hipStream_t s, int nblocks, int bx, const T* z, const ZStats& zs, const double* vol,
                    const int* done);
+// y = a x + b y (b = 0: y not read)
 void launch_axpby(hipStream_t s, int n, double a, const double* x, double b, double* y);
 void launch_cast_d2f(hipStream_t s, int n, const double* in, float* out);
 void launch_cast_f2d(hipStream_t s, int n, const float* in, double* out);

@@ -146,6 +146,8 @@ struct tpamr_ctx {
   DevArray<signed char> faxis, csign, bside;
   DevArray<double> farea, kdiag, kval, kappa, sqk, spacing, barea;
   DevArray<double> origin, cell2;  // 2 per block: lower-left corner; 2 per cell: staging
+  DevArray<double> fcx, fcy;       // face centres
+  DevArray<int> counter;
   DevArray<double> f1, f2, f3, f4;
   // halo refresh schedule (built on first use for a halo width)
   HaloSchedule halo_host;
@@ -328,6 +330,9 @@ void upload_hierarchy(tpamr_ctx* c, bool topology) {
     c->kval.upload(L0.K.off_val, s);
     c->spacing.upload(M.bspacing, s);
     c->origin.upload(M.borigin, s);
+    c->fcx.upload(M.fcx, s);
+    c->fcy.upload(M.fcy, s);
+    c->counter.alloc(1);
     // cell -> face incidence (face order) for the divergence gather
     const int N = M.ncells(), F = M.nfaces();
     std::vector<int> cnt(N + 1, 0);
@@ -1502,6 +1507,33 @@ int tpamr_surface_tension_jumps(tpamr_ctx* c, const double* fraction, const doub
     upload_vec(c, c->f2.p, curvature, F);
     launch_surface_jump(c->stream, F, c->fm.p, c->fp.p, c->farea.p, c->tmp.p, c->f2.p, sigma, formulation, c->f1.p);
     download_vec(c, jumps, c->f1.p, F);
+  });
+}
+
+int tpamr_surface_tension_jumps_cells(tpamr_ctx* c, const double* fraction, const double* cell_curvature,
+                                      const double* curvature_quality, double sigma, int formulation, double* jumps,
+                                      int* unresolved) {
+  if (!c || !fraction || !cell_curvature || !curvature_quality || !jumps) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    if (!c->H) throw StateError("no hierarchy: call tpamr_set_face_coefficients first");
+    if (!(std::isfinite(sigma) && sigma >= 0)) throw std::invalid_argument("sigma must be finite and nonnegative");
+    if (formulation != 0 && formulation != 1) throw std::invalid_argument("formulation must be CSF() or CGFM()");
+    HIPCHECK(hipSetDevice(c->device));
+    const TreeMesh& M = c->mesh();
+    const int N = M.ncells(), F = M.nfaces();
+    if (c->cell2.n < size_t(N)) c->cell2.alloc(2 * size_t(N));
+    upload_vec(c, c->tmp.p, fraction, N);
+    upload_vec(c, c->tmp2.p, cell_curvature, N);
+    upload_vec(c, c->cell2.p, curvature_quality, N);
+    HIPCHECK(hipMemsetAsync(c->counter.p, 0, sizeof(int), c->stream));
+    launch_surface_jump_cells(c->stream, F, c->fm.p, c->fp.p, c->farea.p, c->origin.p, c->spacing.p, c->fcx.p,
+                              c->fcy.p, int(M.blocks.size()), M.bs[0], M.bs[1], M.extent, M.periodic, c->tmp.p,
+                              c->tmp2.p, c->cell2.p, sigma, formulation, c->f1.p, c->counter.p);
+    int n = 0;
+    HIPCHECK(hipMemcpyAsync(&n, c->counter.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    download_vec(c, jumps, c->f1.p, F);
+    if (unresolved) *unresolved = n;
   });
 }
 

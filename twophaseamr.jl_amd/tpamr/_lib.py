@@ -80,6 +80,8 @@ _SIGNATURES = {
     "tpamr_mixture_face_coefficients": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_void_p]),
     "tpamr_surface_tension_jumps": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int,
                                               C.c_void_p]),
+    "tpamr_surface_tension_jumps_cells": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                                    C.c_int, C.c_void_p, C.c_void_p]),
     "tpamr_weighted_jump_force": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "tpamr_gravity_jumps": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
                                       C.c_double, C.c_double, C.c_void_p]),

@@ -3,7 +3,7 @@ pressure adjoint, on the GPU.
 
 Reference interfaces (TwoPhaseAMR.jl):
   mixture_density / PoissonOperator(::InterfaceField; density)  src/two_phase/forces.jl:9-140
-  _surface_tension_jumps! (CGFM / CSF)                          src/two_phase/forces.jl:312-378
+  _surface_tension_jumps! (CGFM / CSF), _face_curvature         src/two_phase/forces.jl:211-378
   _weighted_jump_force!                                         src/two_phase/forces.jl:179-209
   divergence!                                                   src/operators/advection/cell_advection.jl:205-220
   weighted_gradient!                                            src/operators/elliptic.jl:1272-1297
@@ -59,6 +59,27 @@ def surface_tension_jumps(solver: EllipticSolver, fraction, face_curvature, *, s
     L.check(L.lib().tpamr_surface_tension_jumps(solver.ctx.h, L.ptr(c), L.ptr(curv), float(sigma), int(formulation),
                                                 L.ptr(out)), solver.ctx.h)
     return out
+
+
+def surface_tension_jumps_from_cells(solver: EllipticSolver, fraction, cell_curvature, curvature_quality, *, sigma,
+                                     formulation=CGFM):
+    """`_surface_tension_jumps!` (src/two_phase/forces.jl:345-378) with the
+    face curvature of `_face_curvature` (forces.jl:211-300) from the
+    reconstruction's cell curvatures (NaN = none) and their height-function
+    coverage. Returns (jumps, number of unresolved faces)."""
+    if not (np.isfinite(sigma) and sigma >= 0):
+        raise ValueError("sigma must be finite and nonnegative")
+    if formulation not in (CGFM, CSF):
+        raise ValueError("formulation must be CSF() or CGFM()")
+    m = solver.operator.mesh
+    c = L.f64(fraction, m.ncells, "fraction")
+    k = L.f64(cell_curvature, m.ncells, "cell curvature")
+    q = L.f64(curvature_quality, m.ncells, "curvature quality")
+    out = np.zeros(m.nfaces)
+    n = C.c_int()
+    L.check(L.lib().tpamr_surface_tension_jumps_cells(solver.ctx.h, L.ptr(c), L.ptr(k), L.ptr(q), float(sigma),
+                                                      int(formulation), L.ptr(out), C.byref(n)), solver.ctx.h)
+    return out, n.value
 
 
 def weighted_jump_force(solver: EllipticSolver, jumps):
