@@ -192,10 +192,21 @@ int tpamr_halo_sizes(tpamr_ctx* ctx, int halo, int* entries, int* physical);
  * among the physical entries, -1 otherwise); ptr has entries + 1 values. */
 int tpamr_halo_schedule(tpamr_ctx* ctx, int halo, int* dest, int* ptr, int* donor, double* weight, int* phys);
 int tpamr_fill_halo(tpamr_ctx* ctx, int halo, const double* cells, const double* physical, double* haloed);
+/* _pressure_predictor! (single_phase/stepping.jl:144-168): u -= dt G p with
+ * the weighted gradient of the pressure operator (interior faces) */
+int tpamr_pressure_predictor(tpamr_ctx* ctx, double* u, const double* p, double dt);
 /* _project_prepared! (navier_stokes/projection.jl:221-260) on a wall/periodic
  * domain: updates u (interior faces) and p, writes the increment phi */
 int tpamr_project(tpamr_ctx* ctx, double* u, double* p, double dt, double tolerance, int max_cycles, double* phi,
                   tpamr_stats* stats);
+/* The same with project!'s internal arguments _inherited_divergence and
+ * _reference_dt (projection.jl:130-156,189-214): a step shortened from
+ * reference_dt (>= dt) keeps (reference_dt - dt) / (dt reference_dt) times the
+ * inherited divergence (per cell, strong; NULL: none) in its rhs. Both
+ * entry points set stats->compatible as _projection_compatibility!
+ * (projection.jl:100-127): dt^2 |removed rhs mean| <= tolerance. */
+int tpamr_project_ex(tpamr_ctx* ctx, double* u, double* p, double dt, double tolerance, int max_cycles,
+                     const double* inherited_divergence, double reference_dt, double* phi, tpamr_stats* stats);
 
 /* Pressure adjoint (no reference counterpart): for J = w' p with p the
  * zero-mean solution of the current operator, dJ/drhs (strong rhs) and

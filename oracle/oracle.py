@@ -323,13 +323,16 @@ class OSolver:
         _check(lib().orc_weighted_jump_force(self._h, _dptr(np.ascontiguousarray(jumps, np.float64)), _dptr(a)))
         return a
 
-    def project(self, u, p, dt, tolerance=np.sqrt(np.finfo(float).eps), max_cycles=100):
+    def project(self, u, p, dt, tolerance=np.sqrt(np.finfo(float).eps), max_cycles=100, inherited=None,
+                reference_dt=None):
         u = np.array(u, np.float64)
         p = np.array(p, np.float64)
         phi = np.zeros(self.mesh.ncells)
         st = Stats()
-        _check(lib().orc_project(self._h, _dptr(u), _dptr(p), C.c_double(dt), C.c_double(tolerance),
-                                 int(max_cycles), C.byref(st), _dptr(phi)))
+        inh = None if inherited is None else _dptr(np.ascontiguousarray(inherited, np.float64))
+        _check(lib().orc_project_ex(self._h, _dptr(u), _dptr(p), C.c_double(dt), C.c_double(tolerance),
+                                    int(max_cycles), C.byref(st), _dptr(phi), inh,
+                                    C.c_double(dt if reference_dt is None else reference_dt)))
         return u, p, phi, st.as_dict()
 
     def adjoint(self, w, p, rtol=1e-12, max_cycles=200):

@@ -24,7 +24,8 @@ void cgfm_jumps(const Mesh& M, const double* c, const double* curvature, double 
 void weighted_jump_force(const Operator& op, const double* jumps, double* accel);
 void weighted_gradient(const Operator& op, const double* p, double* g);
 void divergence(const Mesh& M, const double* u, const double* ub, double* div);
-Stats project(Solver& S, double* u, double* p, double dt, double tol, int max_cycles, double* phi_out);
+Stats project(Solver& S, double* u, double* p, double dt, double tol, int max_cycles, double* phi_out,
+              const double* inherited, double reference_dt);
 Stats adjoint(Solver& S, const double* w, const double* p, double rtol, int max_cycles,
               double* dJ_db, double* dJ_dbeta, double* lambda_out);
 void apply_fine(Solver& S, const double* x, double* y);
@@ -256,7 +257,15 @@ int orc_divergence(void* mp, const double* u, const double* ub, double* div) {
 
 int orc_project(void* sp, double* u, double* p, double dt, double tol, int max_cycles, Stats* st,
                 double* phi) {
-  GUARD({ *st = project(*static_cast<Solver*>(sp), u, p, dt, tol, max_cycles, phi); return 0; })
+  GUARD({ *st = project(*static_cast<Solver*>(sp), u, p, dt, tol, max_cycles, phi, nullptr, dt); return 0; })
+}
+
+int orc_project_ex(void* sp, double* u, double* p, double dt, double tol, int max_cycles, Stats* st, double* phi,
+                   const double* inherited, double reference_dt) {
+  GUARD({
+    *st = project(*static_cast<Solver*>(sp), u, p, dt, tol, max_cycles, phi, inherited, reference_dt);
+    return 0;
+  })
 }
 
 int orc_adjoint(void* sp, const double* w, const double* p, double rtol, int max_cycles, double* dJ_db,

@@ -1097,16 +1097,28 @@ void divergence(const Mesh& M, const double* u, const double* ub, double* div) {
 // `_project_prepared!` (navier_stokes/projection.jl:221-260) for a wall or
 // periodic domain with one component: div -> rhs = -div/dt -> solve with
 // rtol 0 and atol = tol/dt^2 -> u -= dt G phi -> p += phi.
+// With `inherited` (a shortened step, `_projection_rhs!` projection.jl:130-156)
+// the rhs keeps (tau - dt) / (dt tau) of the inherited divergence; the stats'
+// `compatible` is `_projection_compatibility!` (:100-127).
 Stats project(Solver& S, double* u, double* p, double dt, double tol, int max_cycles,
-              double* phi_out) {
+              double* phi_out, const double* inherited, double reference_dt) {
   Operator& op = *S.ops[0];
   const Mesh& M = *op.mesh;
   const int N = M.ncells(), F = int(M.faces.size());
+  if (!(std::isfinite(reference_dt) && reference_dt >= dt))
+    throw std::invalid_argument("reference dt must be finite and at least dt");
   std::vector<double> div(N), phi(N, 0.0), g(F);
   divergence(M, u, nullptr, div.data());
-  for (int c = 0; c < N; ++c) div[c] = -div[c] / dt;
+  if (inherited) {
+    const double w = (reference_dt - dt) / (dt * reference_dt);
+    for (int c = 0; c < N; ++c) div[c] = -div[c] / dt + w * inherited[c];
+  } else {
+    for (int c = 0; c < N; ++c) div[c] = -div[c] / dt;
+  }
   Stats st = solve(S, div.data(), phi.data(), 0.0, tol / (dt * dt), max_cycles, 0, nullptr);
   st.residual_scale = 1.0 / (dt * dt);
+  const double mean = std::fabs(st.removed_rhs_mean);
+  st.compatible = (mean == 0 ? 0.0 : dt * dt * mean) <= tol;
   weighted_gradient(op, phi.data(), g.data());
   const double scale = dt * 1.0;
   for (int f = 0; f < F; ++f) u[f] -= scale * g[f];

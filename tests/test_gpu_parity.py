@@ -421,3 +421,47 @@ def test_device_refresh_matches_host_refresh(name, precision):
     assert out[True][1] == out[False][1]
     assert out[True][2] == out[False][2]
     assert np.array_equal(out[True][0], out[False][0])
+
+
+def test_projection_with_inherited_divergence_matches_oracle():
+    """project! on a shortened step (dt < reference dt) keeps part of the
+    inherited divergence (projection.jl:130-156); stats.compatible follows
+    _projection_compatibility! (:100-127)."""
+    om, pm = meshes("amr_mixed")
+    k = kappa_for(pm, "smooth")
+    os_ = OSolver(om, k)
+    gs = tpamr.EllipticSolver(tpamr.PoissonOperator(pm, k))
+    u0 = np.cos(np.arange(pm.nfaces) * 0.61)
+    inh = om.divergence(np.sin(np.arange(pm.nfaces) * 0.37))
+    dt, tau = 1e-3, 2.5e-3
+    uo, po, phio, so = os_.project(u0, np.zeros(pm.ncells), dt, tolerance=1e-12, max_cycles=200, inherited=inh,
+                                   reference_dt=tau)
+    ug, pg = u0.copy(), np.zeros(pm.ncells)
+    sg, phig = tpamr.project(ug, pg, gs, dt, tolerance=1e-12, max_cycles=200, inherited_divergence=inh,
+                             reference_dt=tau)
+    assert sg.converged and sg.cycles == so["cycles"] and sg.compatible == bool(so["compatible"])
+    assert rel(phig, phio) < 1e-10 and rel(ug, uo) < 1e-10
+    # a constant inherited divergence cannot be projected out
+    _, _, _, sc = os_.project(u0, np.zeros(pm.ncells), dt, tolerance=1e-12, inherited=np.ones(pm.ncells),
+                              reference_dt=tau)
+    sgc, _ = tpamr.project(u0.copy(), np.zeros(pm.ncells), gs, dt, tolerance=1e-12,
+                           inherited_divergence=np.ones(pm.ncells), reference_dt=tau)
+    assert not sc["compatible"] and not sgc.compatible
+    with pytest.raises(ValueError):
+        tpamr.project(u0.copy(), np.zeros(pm.ncells), gs, dt, reference_dt=0.5 * dt)
+    gs.close()
+
+
+def test_pressure_predictor_matches_oracle():
+    """_pressure_predictor! (single_phase/stepping.jl:144-168): u -= dt G p."""
+    om, pm = meshes("amr_mixed")
+    k = kappa_for(pm, "smooth")
+    os_ = OSolver(om, k)
+    gs = tpamr.EllipticSolver(tpamr.PoissonOperator(pm, k))
+    x, y = pm.cell_centers()
+    p = np.sin(3 * x) * np.cos(2 * y)
+    u0 = np.cos(np.arange(pm.nfaces) * 0.61)
+    ug = tpamr.pressure_predictor(u0.copy(), p, gs, 1e-3)
+    uo = u0 - 1e-3 * os_.weighted_gradient(p)
+    assert rel(ug, uo) < 1e-13
+    gs.close()

@@ -298,3 +298,25 @@ def test_adjoint_matches_finite_differences():
     vals = [w @ OSolver(m, k + sg * eps * dkap).solve(b, rtol=1e-13, max_cycles=200)[0] for sg in (1, -1)]
     fd = (vals[0] - vals[1]) / (2 * eps)
     assert abs(fd - dk @ dkap) <= 1e-6 * abs(fd)
+
+
+def test_oracle_shortened_projection_keeps_inherited_divergence():
+    """With the inherited divergence equal to the velocity's own divergence d,
+    the rhs is -d / tau: the step removes dt / tau of it and leaves
+    (1 - dt / tau) d (projection.jl:130-156); with tau = dt the projection is
+    the plain one."""
+    from cases import MESHES
+    from oracle import OMesh, OSolver
+    om = OMesh(**MESHES["amr_mixed"])
+    s = OSolver(om, np.exp(0.4 * np.sin(np.arange(1, om.nfaces + 1))))
+    u = np.cos(np.arange(om.nfaces) * 0.61)
+    d = om.divergence(u)
+    dt, tau = 1e-3, 4e-3
+    un, _, _, st = s.project(u, np.zeros(om.ncells), dt, tolerance=1e-14, max_cycles=200, inherited=d,
+                             reference_dt=tau)
+    assert st["converged"] and st["compatible"]
+    assert np.max(np.abs(om.divergence(un) - (1 - dt / tau) * d)) <= 1e-9 * np.max(np.abs(d))
+    u1, _, _, _ = s.project(u, np.zeros(om.ncells), dt, tolerance=1e-14, max_cycles=200, inherited=d,
+                            reference_dt=dt)
+    u2, _, _, _ = s.project(u, np.zeros(om.ncells), dt, tolerance=1e-14, max_cycles=200)
+    assert np.array_equal(u1, u2)

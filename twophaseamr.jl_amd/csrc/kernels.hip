@@ -506,6 +506,18 @@ __global__ void __launch_bounds__(BX* BY) k_zstats(const T* __restrict__ z, cons
 // ---------------------------------------------------------------------------
 // elementwise / face kernels
 
+// `_projection_rhs!` (navier_stokes/projection.jl:130-156): rhs = -div / dt,
+// plus w * inherited with w = (tau - dt) / (dt tau) when a shortened step
+// keeps part of the inherited divergence (inherited = nullptr: none)
+__global__ void k_projection_rhs(int n, const double* __restrict__ div, const double* __restrict__ inherited,
+                                 double dt, double w, double* __restrict__ rhs) {
+#pragma clang fp contract(off)
+  const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= size_t(n)) return;
+  const double v = -div[c] / dt;
+  rhs[c] = inherited ? v + w * inherited[c] : v;
+}
+
 // y = a x + b y; with b = 0, y is only written (it may hold anything, NaN
 // included, before); x may alias y
 __global__ void k_axpby(int n, double a, const double* x, double b, double* y) {
@@ -1492,6 +1504,11 @@ void launch_ir_correct(hipStream_t s, int n, float* e, double* x, const CgState*
 
 void launch_fill_f32(hipStream_t s, int n, float v, float* out) {
   hipLaunchKernelGGL(k_fill_f32, dim3(cdiv(n, 256)), dim3(256), 0, s, n, v, out);
+}
+
+void launch_projection_rhs(hipStream_t s, int n, const double* div, const double* inherited, double dt, double w,
+                           double* rhs) {
+  hipLaunchKernelGGL(k_projection_rhs, dim3(cdiv(n, 256)), dim3(256), 0, s, n, div, inherited, dt, w, rhs);
 }
 
 void launch_axpby(hipStream_t s, int n, double a, const double* x, double b, double* y) {

@@ -212,6 +212,9 @@ void launch_adjoint_face(hipStream_t s, int nf, const double* yl, const double* 
 template <typename T>
 void launch_zstats(hipStream_t s, int nblocks, int bx, const T* z, const ZStats& zs, const double* vol,
                    const int* done);
+// projection rhs -div / dt (+ w inherited; inherited may be nullptr)
+void launch_projection_rhs(hipStream_t s, int n, const double* div, const double* inherited, double dt, double w,
+                           double* rhs);
 // y = a x + b y (b = 0: y not read)
 void launch_axpby(hipStream_t s, int n, double a, const double* x, double b, double* y);
 void launch_cast_d2f(hipStream_t s, int n, const double* in, float* out);

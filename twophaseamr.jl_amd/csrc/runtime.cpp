@@ -1641,26 +1641,38 @@ int tpamr_weighted_gradient(tpamr_ctx* c, const double* p, double* g) {
   });
 }
 
-int tpamr_project(tpamr_ctx* c, double* u, double* p, double dt, double tol, int max_cycles, double* phi,
-                  tpamr_stats* stats) {
+int tpamr_project_ex(tpamr_ctx* c, double* u, double* p, double dt, double tol, int max_cycles,
+                     const double* inherited_divergence, double reference_dt, double* phi, tpamr_stats* stats) {
   if (!c || !u || !p) return TPAMR_EINVAL;
   return guard(c, [&] {
     need_device(c);
     need_solver(c);
     if (!(dt > 0)) throw std::invalid_argument("dt must be strictly positive");
     if (!(std::isfinite(tol) && tol > 0 && tol < 1)) throw std::invalid_argument("tolerance must be finite and lie in (0, 1)");
+    if (!(std::isfinite(reference_dt) && reference_dt >= dt))
+      throw std::invalid_argument("reference dt must be finite and at least dt");
     HIPCHECK(hipSetDevice(c->device));
     const TreeMesh& M = c->mesh();
     const int N = M.ncells(), F = M.nfaces(), nb = M.nb();
     hipStream_t s = c->stream;
+    check_finite(u, F, "velocity");
     upload_vec(c, c->f1.p, u, F);
     launch_divergence(s, N, nb, c->cptr.p, c->cface.p, c->csign.p, c->farea.p, c->f1.p, c->bptr.p, c->bface.p,
                       c->bside.p, c->barea.p, nullptr, c->topo[0].volume.p, c->tmp.p);
-    // rhs = -div / dt
-    launch_axpby(s, N, -1.0 / dt, c->tmp.p, 0.0, c->b.p);
+    const double* inh = nullptr;
+    if (inherited_divergence) {
+      check_finite(inherited_divergence, N, "inherited divergence");
+      upload_vec(c, c->tmp2.p, inherited_divergence, N);
+      inh = c->tmp2.p;
+    }
+    launch_projection_rhs(s, N, c->tmp.p, inh, dt, (reference_dt - dt) / (dt * reference_dt), c->b.p);
     launch_fill(s, N, 0.0, c->x.p);
     tpamr_stats st = solve_dispatch(c, 0.0, tol / (dt * dt), max_cycles, 0);
     st.residual_scale = 1.0 / (dt * dt);
+    // _projection_compatibility! (projection.jl:100-127): the removed strong
+    // rhs mean, times dt^2, against the tolerance
+    const double mean = std::fabs(st.removed_rhs_mean);
+    st.compatible = (mean == 0 ? 0.0 : dt * dt * mean) <= tol;
     // u -= dt G phi
     launch_face_jump(s, F, c->fm.p, c->fp.p, c->farea.p, c->sqk.p, c->x.p, nullptr, c->f2.p);
     weighted_metric(c, c->f2.p, c->f3.p);
@@ -1672,6 +1684,30 @@ int tpamr_project(tpamr_ctx* c, double* u, double* p, double dt, double tol, int
     if (phi) std::copy(ph.begin(), ph.end(), phi);
     if (stats) *stats = st;
   });
+}
+
+int tpamr_pressure_predictor(tpamr_ctx* c, double* u, const double* p, double dt) {
+  if (!c || !u || !p) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    need_solver(c);
+    if (!(dt > 0 && std::isfinite(dt))) throw std::invalid_argument("dt must be finite and strictly positive");
+    HIPCHECK(hipSetDevice(c->device));
+    const TreeMesh& M = c->mesh();
+    const int N = M.ncells(), F = M.nfaces();
+    hipStream_t s = c->stream;
+    upload_vec(c, c->f1.p, u, F);
+    upload_vec(c, c->tmp.p, p, N);
+    launch_face_jump(s, F, c->fm.p, c->fp.p, c->farea.p, c->sqk.p, c->tmp.p, nullptr, c->f2.p);
+    weighted_metric(c, c->f2.p, c->f3.p);
+    launch_axpby(s, F, -dt, c->f3.p, 1.0, c->f1.p);
+    download_vec(c, u, c->f1.p, F);
+  });
+}
+
+int tpamr_project(tpamr_ctx* c, double* u, double* p, double dt, double tol, int max_cycles, double* phi,
+                  tpamr_stats* stats) {
+  return tpamr_project_ex(c, u, p, dt, tol, max_cycles, nullptr, dt, phi, stats);
 }
 
 int tpamr_adjoint(tpamr_ctx* c, const double* w, const double* p, double rtol, int max_cycles, double* dJ_drhs,

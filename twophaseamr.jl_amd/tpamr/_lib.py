@@ -95,6 +95,9 @@ _SIGNATURES = {
     "tpamr_fill_halo": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "tpamr_project": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int,
                                 C.c_void_p, C.POINTER(Stats)]),
+    "tpamr_pressure_predictor": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]),
+    "tpamr_project_ex": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int,
+                                   C.c_void_p, C.c_double, C.c_void_p, C.POINTER(Stats)]),
     "tpamr_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p,
                                 C.c_void_p, C.POINTER(Stats)]),
     "tpamr_project_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,

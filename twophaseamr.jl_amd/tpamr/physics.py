@@ -7,7 +7,8 @@ Reference interfaces (TwoPhaseAMR.jl):
   _weighted_jump_force!                                         src/two_phase/forces.jl:179-209
   divergence!                                                   src/operators/advection/cell_advection.jl:205-220
   weighted_gradient!                                            src/operators/elliptic.jl:1272-1297
-  project! / _project_prepared!                                 src/navier_stokes/projection.jl:189-260
+  project! / _project_prepared! / _projection_rhs!              src/navier_stokes/projection.jl:100-260
+  _pressure_predictor!                                          src/single_phase/stepping.jl:144-168
   fill_halo! / _build_halo_schedule                             src/mesh/variables.jl:205-229, mesh.jl:944-1037
   gravity!                                                      src/two_phase/forces.jl:528-580
 The adjoints (`pressure_solve_rrule`, `project_adjoint`) have no reference
@@ -151,19 +152,36 @@ def weighted_gradient(solver: EllipticSolver, p):
     return out
 
 
-def project(u, p, solver: EllipticSolver, dt, *, tolerance=float(np.sqrt(np.finfo(float).eps)), max_cycles=100):
+def pressure_predictor(u, p, solver: EllipticSolver, dt):
+    """`_pressure_predictor!` (src/single_phase/stepping.jl:144-168): u -= dt
+    G p with the operator's weighted gradient, in place; returns u."""
+    solver._sync()
+    m = solver.operator.mesh
+    if u.dtype != np.float64 or u.shape != (m.nfaces,) or not u.flags.c_contiguous:
+        raise L.DimensionMismatch("u must be a contiguous float64 vector over interior faces")
+    pp = L.f64(p, m.ncells, "p")
+    L.check(L.lib().tpamr_pressure_predictor(solver.ctx.h, L.ptr(u), L.ptr(pp), float(dt)), solver.ctx.h)
+    return u
+
+
+def project(u, p, solver: EllipticSolver, dt, *, tolerance=float(np.sqrt(np.finfo(float).eps)), max_cycles=100,
+            inherited_divergence=None, reference_dt=None):
     """project! on a wall/periodic domain: u and p are updated in place; returns
-    (stats, phi)."""
+    (stats, phi). `inherited_divergence` and `reference_dt` are project!'s
+    `_inherited_divergence` and `_reference_dt` (a shortened step keeps part of
+    the inherited divergence, projection.jl:130-156)."""
     solver._sync()
     m = solver.operator.mesh
     if u.dtype != np.float64 or u.shape != (m.nfaces,) or not u.flags.c_contiguous:
         raise L.DimensionMismatch("u must be a contiguous float64 vector over interior faces")
     if p.dtype != np.float64 or p.shape != (m.ncells,) or not p.flags.c_contiguous:
         raise L.DimensionMismatch("p must be a contiguous float64 vector over cells")
+    inh = None if inherited_divergence is None else L.f64(inherited_divergence, m.ncells, "inherited divergence")
     phi = np.zeros(m.ncells)
     st = L.Stats()
-    L.check(L.lib().tpamr_project(solver.ctx.h, L.ptr(u), L.ptr(p), float(dt), float(tolerance), int(max_cycles),
-                                  L.ptr(phi), C.byref(st)), solver.ctx.h)
+    L.check(L.lib().tpamr_project_ex(solver.ctx.h, L.ptr(u), L.ptr(p), float(dt), float(tolerance), int(max_cycles),
+                                     L.ptr(inh), float(dt if reference_dt is None else reference_dt), L.ptr(phi),
+                                     C.byref(st)), solver.ctx.h)
     return LinearSolverStats._from(st, solver.precision), phi
 
 
