@@ -214,6 +214,24 @@ int tpamr_project_ex(tpamr_ctx* ctx, double* u, double* p, double dt, double tol
 int tpamr_adjoint(tpamr_ctx* ctx, const double* w, const double* p, double rtol, int max_cycles, double* dJ_drhs,
                   double* dJ_dkappa, tpamr_stats* stats);
 
+/* Device-memory twins for a time loop that keeps its fields in HBM (every
+ * pointer is device memory of the context's GPU; the call returns when the
+ * results are written):
+ * - tpamr_set_face_coefficients_device: the coefficient refresh from device
+ *   face coefficients (after one tpamr_set_face_coefficients);
+ * - tpamr_apply_force_device: the force substep u += dt sqrt(k) K (sqrt(k)
+ *   jumps) (_apply_phase_forces! with _weighted_jump_force!,
+ *   two_phase/stepping.jl:181-201);
+ * - tpamr_divergence_device: divergence! of interior face velocities;
+ * - tpamr_project_device: tpamr_project_ex on device u (and p += phi when p
+ *   is not NULL, phi copied out when not NULL). */
+int tpamr_set_face_coefficients_device(tpamr_ctx* ctx, const double* d_kappa);
+int tpamr_apply_force_device(tpamr_ctx* ctx, const double* d_jumps, double dt, double* d_u);
+int tpamr_divergence_device(tpamr_ctx* ctx, const double* d_u, double* d_div);
+int tpamr_project_device(tpamr_ctx* ctx, double* d_u, double* d_p, double dt, double tolerance, int max_cycles,
+                         const double* d_inherited_divergence, double reference_dt, double* d_phi,
+                         tpamr_stats* stats);
+
 /* Adjoint of one projection step (no reference counterpart). The forward
  * step is tpamr_project from u* = u + dt a (a = the step's force
  * accelerations sqrt(k) K (sqrt(k) jumps)): u' = u* - dt G phi with
@@ -230,6 +248,10 @@ int tpamr_adjoint(tpamr_ctx* ctx, const double* w, const double* p, double rtol,
 int tpamr_project_adjoint(tpamr_ctx* ctx, const double* ubar, const double* phi, const double* jumps, double dt,
                           double rtol, int max_cycles, double* ubar_star, double* dJ_drhs, double* dJ_dkappa,
                           tpamr_stats* stats);
+/* The same on device memory (no reference counterpart). */
+int tpamr_project_adjoint_device(tpamr_ctx* ctx, const double* d_ubar, const double* d_phi, const double* d_jumps,
+                                 double dt, double rtol, int max_cycles, double* d_ubar_star, double* d_dJ_drhs,
+                                 double* d_dJ_dkappa, tpamr_stats* stats);
 
 /* Timing of the finest-level smoother kernel (HIP events, `reps` launches on
  * the context stream); ms per launch in *avg_ms and its algorithmic bytes

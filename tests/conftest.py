@@ -15,6 +15,12 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and the built library")
     config.addinivalue_line("markers", "slow: larger CPU cases")
+    # torch carries its own HIP runtime; it finds the GPU only when it
+    # initialises before the library's runtime has opened the device, so GPU
+    # sessions (which hand torch tensors to the library) start it first
+    markexpr = getattr(config.option, "markexpr", "") or ""
+    if "gpu" in markexpr and "not gpu" not in markexpr:
+        gpu_available()
 
 
 def gpu_available():

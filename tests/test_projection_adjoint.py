@@ -188,3 +188,32 @@ def test_run_gradient_in_mixed_precision(case, precision):
         assert np.max(np.abs(run.checkpoints[n] - ref.checkpoints[n])) <= 1e-8 * np.max(np.abs(ref.checkpoints[n]))
     assert np.max(np.abs(g["rhs"] - g64["rhs"])) <= 1e-8 * np.max(np.abs(g64["rhs"]))
     assert np.max(np.abs(g["kappa"][0] - g64["kappa"][0])) <= 1e-8 * np.max(np.abs(g64["kappa"][0]))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+def test_device_run_matches_host_run(case, precision):
+    """The run kept in HBM (device-memory entry points: coefficient refresh,
+    force, projection, pullback) is the host-array run, bit for bit."""
+    import torch
+    pm, om, gs, steps, frac_end, u0 = case
+    s = gs if precision == "fp64" else tpamr.EllipticSolver(tpamr.PoissonOperator(pm, steps[0][0]),
+                                                             precision=precision, device=0)
+    for rtol in (None, 1e-11):
+        host = tpamr.ProjectionRun(s, DT, tolerance=1e-14, rtol=rtol, adjoint_rtol=1e-13)
+        uh = host.forward(u0, steps)
+        _, ubh = tpamr.kinetic_energy(pm, uh, frac_end, W.WAVE_DENSITY)
+        gh = host.backward(ubh)
+        dev = torch.device("cuda:0")
+        dsteps = [(torch.as_tensor(k, device=dev), torch.as_tensor(j, device=dev)) for k, j in steps]
+        run = tpamr.DeviceProjectionRun(s, DT, tolerance=1e-14, rtol=rtol, adjoint_rtol=1e-13)
+        ud = run.forward(torch.as_tensor(u0, device=dev), dsteps)
+        assert np.array_equal(ud.cpu().numpy(), uh)
+        for a, b in zip(run.checkpoints, host.checkpoints):
+            assert np.array_equal(a.cpu().numpy(), b)
+        gd = run.backward(torch.as_tensor(ubh, device=dev))
+        assert np.array_equal(gd["rhs"].cpu().numpy(), gh["rhs"])
+        assert np.array_equal(gd["u0"].cpu().numpy(), gh["u0"])
+        for a, b in zip(gd["kappa"], gh["kappa"]):
+            assert np.array_equal(a.cpu().numpy(), b)
+    if s is not gs:
+        s.close()

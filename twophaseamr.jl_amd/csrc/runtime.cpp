@@ -961,13 +961,16 @@ void upload_plan(tpamr_ctx* c) {
 // Jacobi weights (one read-back) and the dense coarse inverse. The host keeps
 // the coarsest level's values (for its dense matrix); the other levels' host
 // values are refreshed only when asked for (host_stale).
-void device_refresh(tpamr_ctx* c, const double* kappa) {
+void device_refresh(tpamr_ctx* c, const double* kappa, bool kappa_on_device = false) {
   Hierarchy& H = *c->H;
   hipStream_t s = c->stream;
   const int nl = int(H.levels.size());
   if (!c->plan_ready) upload_plan(c);
   DevPlan& P0 = c->plan[0];
-  upload_vec(c, P0.kappa.p, kappa, P0.nfaces);
+  if (kappa_on_device)
+    HIPCHECK(hipMemcpyAsync(P0.kappa.p, kappa, sizeof(double) * P0.nfaces, hipMemcpyDeviceToDevice, s));
+  else
+    upload_vec(c, P0.kappa.p, kappa, P0.nfaces);
   HIPCHECK(hipMemsetAsync(c->ref_ratio.p, 0, sizeof(unsigned long long) * nl, s));
   HIPCHECK(hipMemsetAsync(c->ref_flags.p, 0, sizeof(int) * (nl + 1), s));
   launch_ref_check(s, P0.nfaces, P0.kappa.p, c->ref_flags.p + nl);
@@ -1258,6 +1261,17 @@ int tpamr_set_face_coefficients(tpamr_ctx* c, const double* kappa) {
                    std::chrono::duration<double, std::milli>(t1 - t0).count(),
                    std::chrono::duration<double, std::milli>(t2 - t1).count());
     }
+  });
+}
+
+int tpamr_set_face_coefficients_device(tpamr_ctx* c, const double* d_kappa) {
+  if (!c || !d_kappa) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    if (!c->H || c->topo.empty())
+      throw StateError("no hierarchy: call tpamr_set_face_coefficients once before device refreshes");
+    HIPCHECK(hipSetDevice(c->device));
+    device_refresh(c, d_kappa, true);
   });
 }
 
@@ -1641,48 +1655,129 @@ int tpamr_weighted_gradient(tpamr_ctx* c, const double* p, double* g) {
   });
 }
 
+namespace {
+// copy into / out of the library's buffers: host memory (upload, download) or
+// the caller's device memory
+void copy_in(tpamr_ctx* c, double* dst, const double* src, size_t n, bool host) {
+  if (host)
+    upload_vec(c, dst, src, n);
+  else
+    HIPCHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+}
+void copy_out(tpamr_ctx* c, double* dst, const double* src, size_t n, bool host) {
+  if (host)
+    download_vec(c, dst, src, n);
+  else
+    HIPCHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+}
+
+void project_checks(double dt, double tol, double reference_dt) {
+  if (!(dt > 0)) throw std::invalid_argument("dt must be strictly positive");
+  if (!(std::isfinite(tol) && tol > 0 && tol < 1)) throw std::invalid_argument("tolerance must be finite and lie in (0, 1)");
+  if (!(std::isfinite(reference_dt) && reference_dt >= dt))
+    throw std::invalid_argument("reference dt must be finite and at least dt");
+}
+
+// The projection on the velocity in c->f1: rhs from its divergence (and the
+// inherited divergence in c->tmp2 when `inherited`), the increment in c->x,
+// f1 -= dt G x.
+tpamr_stats project_core(tpamr_ctx* c, double dt, double tol, int max_cycles, bool inherited, double reference_dt) {
+  const TreeMesh& M = c->mesh();
+  const int N = M.ncells(), F = M.nfaces(), nb = M.nb();
+  hipStream_t s = c->stream;
+  launch_divergence(s, N, nb, c->cptr.p, c->cface.p, c->csign.p, c->farea.p, c->f1.p, c->bptr.p, c->bface.p,
+                    c->bside.p, c->barea.p, nullptr, c->topo[0].volume.p, c->tmp.p);
+  launch_projection_rhs(s, N, c->tmp.p, inherited ? c->tmp2.p : nullptr, dt, (reference_dt - dt) / (dt * reference_dt),
+                        c->b.p);
+  launch_fill(s, N, 0.0, c->x.p);
+  tpamr_stats st = solve_dispatch(c, 0.0, tol / (dt * dt), max_cycles, 0);
+  st.residual_scale = 1.0 / (dt * dt);
+  // _projection_compatibility! (projection.jl:100-127): the removed strong
+  // rhs mean, times dt^2, against the tolerance
+  const double mean = std::fabs(st.removed_rhs_mean);
+  st.compatible = (mean == 0 ? 0.0 : dt * dt * mean) <= tol;
+  // u -= dt G phi
+  launch_face_jump(s, F, c->fm.p, c->fp.p, c->farea.p, c->sqk.p, c->x.p, nullptr, c->f2.p);
+  weighted_metric(c, c->f2.p, c->f3.p);
+  launch_axpby(s, F, -dt, c->f3.p, 1.0, c->f1.p);
+  return st;
+}
+}  // namespace
+
 int tpamr_project_ex(tpamr_ctx* c, double* u, double* p, double dt, double tol, int max_cycles,
                      const double* inherited_divergence, double reference_dt, double* phi, tpamr_stats* stats) {
   if (!c || !u || !p) return TPAMR_EINVAL;
   return guard(c, [&] {
     need_device(c);
     need_solver(c);
-    if (!(dt > 0)) throw std::invalid_argument("dt must be strictly positive");
-    if (!(std::isfinite(tol) && tol > 0 && tol < 1)) throw std::invalid_argument("tolerance must be finite and lie in (0, 1)");
-    if (!(std::isfinite(reference_dt) && reference_dt >= dt))
-      throw std::invalid_argument("reference dt must be finite and at least dt");
+    project_checks(dt, tol, reference_dt);
     HIPCHECK(hipSetDevice(c->device));
     const TreeMesh& M = c->mesh();
-    const int N = M.ncells(), F = M.nfaces(), nb = M.nb();
-    hipStream_t s = c->stream;
+    const int N = M.ncells(), F = M.nfaces();
     check_finite(u, F, "velocity");
     upload_vec(c, c->f1.p, u, F);
-    launch_divergence(s, N, nb, c->cptr.p, c->cface.p, c->csign.p, c->farea.p, c->f1.p, c->bptr.p, c->bface.p,
-                      c->bside.p, c->barea.p, nullptr, c->topo[0].volume.p, c->tmp.p);
-    const double* inh = nullptr;
     if (inherited_divergence) {
       check_finite(inherited_divergence, N, "inherited divergence");
       upload_vec(c, c->tmp2.p, inherited_divergence, N);
-      inh = c->tmp2.p;
     }
-    launch_projection_rhs(s, N, c->tmp.p, inh, dt, (reference_dt - dt) / (dt * reference_dt), c->b.p);
-    launch_fill(s, N, 0.0, c->x.p);
-    tpamr_stats st = solve_dispatch(c, 0.0, tol / (dt * dt), max_cycles, 0);
-    st.residual_scale = 1.0 / (dt * dt);
-    // _projection_compatibility! (projection.jl:100-127): the removed strong
-    // rhs mean, times dt^2, against the tolerance
-    const double mean = std::fabs(st.removed_rhs_mean);
-    st.compatible = (mean == 0 ? 0.0 : dt * dt * mean) <= tol;
-    // u -= dt G phi
-    launch_face_jump(s, F, c->fm.p, c->fp.p, c->farea.p, c->sqk.p, c->x.p, nullptr, c->f2.p);
-    weighted_metric(c, c->f2.p, c->f3.p);
-    launch_axpby(s, F, -dt, c->f3.p, 1.0, c->f1.p);
+    tpamr_stats st = project_core(c, dt, tol, max_cycles, inherited_divergence != nullptr, reference_dt);
     download_vec(c, u, c->f1.p, F);
     std::vector<double> ph(N);
     download_vec(c, ph.data(), c->x.p, N);
     for (int q = 0; q < N; ++q) p[q] += ph[q];
     if (phi) std::copy(ph.begin(), ph.end(), phi);
     if (stats) *stats = st;
+  });
+}
+
+int tpamr_project_device(tpamr_ctx* c, double* d_u, double* d_p, double dt, double tol, int max_cycles,
+                         const double* d_inherited_divergence, double reference_dt, double* d_phi,
+                         tpamr_stats* stats) {
+  if (!c || !d_u) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    need_solver(c);
+    project_checks(dt, tol, reference_dt);
+    HIPCHECK(hipSetDevice(c->device));
+    const TreeMesh& M = c->mesh();
+    const int N = M.ncells(), F = M.nfaces();
+    hipStream_t s = c->stream;
+    copy_in(c, c->f1.p, d_u, F, false);
+    if (d_inherited_divergence) copy_in(c, c->tmp2.p, d_inherited_divergence, N, false);
+    tpamr_stats st = project_core(c, dt, tol, max_cycles, d_inherited_divergence != nullptr, reference_dt);
+    copy_out(c, d_u, c->f1.p, F, false);
+    if (d_p) launch_axpby(s, N, 1.0, c->x.p, 1.0, d_p);
+    if (d_phi) copy_out(c, d_phi, c->x.p, N, false);
+    HIPCHECK(hipStreamSynchronize(s));
+    if (stats) *stats = st;
+  });
+}
+
+int tpamr_apply_force_device(tpamr_ctx* c, const double* d_jumps, double dt, double* d_u) {
+  if (!c || !d_jumps || !d_u) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    need_solver(c);
+    if (!std::isfinite(dt)) throw std::invalid_argument("dt must be finite");
+    HIPCHECK(hipSetDevice(c->device));
+    const int F = c->mesh().nfaces();
+    launch_face_jump(c->stream, F, c->fm.p, c->fp.p, c->farea.p, c->sqk.p, nullptr, d_jumps, c->f1.p);
+    weighted_metric(c, c->f1.p, c->f2.p);
+    launch_axpby(c->stream, F, dt, c->f2.p, 1.0, d_u);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int tpamr_divergence_device(tpamr_ctx* c, const double* d_u, double* d_div) {
+  if (!c || !d_u || !d_div) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    if (!c->H) throw StateError("no hierarchy: call tpamr_set_face_coefficients first");
+    HIPCHECK(hipSetDevice(c->device));
+    const TreeMesh& M = c->mesh();
+    launch_divergence(c->stream, M.ncells(), M.nb(), c->cptr.p, c->cface.p, c->csign.p, c->farea.p, d_u, c->bptr.p,
+                      c->bface.p, c->bside.p, c->barea.p, nullptr, c->topo[0].volume.p, d_div);
+    HIPCHECK(hipStreamSynchronize(c->stream));
   });
 }
 
@@ -1741,6 +1836,61 @@ int tpamr_adjoint(tpamr_ctx* c, const double* w, const double* p, double rtol, i
   });
 }
 
+namespace {
+// Pullback of one projection step; inputs and outputs in host memory (host)
+// or device memory
+tpamr_stats project_adjoint_core(tpamr_ctx* c, const double* ubar, const double* phi, const double* jumps, double dt,
+                                 double rtol, int max_cycles, double* ubar_star, double* dJ_drhs, double* dJ_dkappa,
+                                 bool host) {
+  if (!(dt > 0 && std::isfinite(dt))) throw std::invalid_argument("dt must be finite and strictly positive");
+  const TreeMesh& M = c->mesh();
+  const int N = M.ncells(), F = M.nfaces(), nb = M.nb();
+  hipStream_t s = c->stream;
+  if (host) check_finite(ubar, F, "velocity adjoint");
+  copy_in(c, c->f1.p, ubar, F, host);
+  // A lambda = J^T M ubar = -V div(M ubar), M = sqrt(k) K sqrt(k)
+  launch_face_scale(s, F, c->sqk.p, c->f1.p, c->f2.p);
+  weighted_metric(c, c->f2.p, c->f3.p);
+  launch_divergence(s, N, nb, c->cptr.p, c->cface.p, c->csign.p, c->farea.p, c->f3.p, c->bptr.p, c->bface.p,
+                    c->bside.p, c->barea.p, nullptr, c->topo[0].volume.p, c->tmp.p);
+  launch_axpby(s, N, -1.0, c->tmp.p, 0.0, c->b.p);
+  launch_fill(s, N, 0.0, c->x.p);
+  tpamr_stats st = solve_dispatch(c, rtol, 0.0, max_cycles, 0);
+  if (dJ_drhs) {
+    // dJ/drhs = -dt V lambda
+    launch_scale_by_volume(s, N, nb, c->x.p, c->tmp.p, c->topo[0].volume.p, false);
+    launch_axpby(s, N, -dt, c->tmp.p, 0.0, c->tmp.p);
+    copy_out(c, dJ_drhs, c->tmp.p, N, host);
+  }
+  // ubar* = ubar - J lambda, J x = |f| (x+ - x-)
+  launch_face_jump(s, F, c->fm.p, c->fp.p, c->farea.p, nullptr, c->x.p, nullptr, c->f2.p);
+  launch_axpby(s, F, -1.0, c->f2.p, 1.0, c->f1.p);
+  copy_out(c, ubar_star, c->f1.p, F, host);
+  if (dJ_dkappa) {
+    // z = jumps - J phi; dJ/dk_f = dt [W_f (K Z)_f + Z_f (K W)_f] / (2 k_f)
+    // with W = sqrt(k) ubar*, Z = sqrt(k) z
+    if (host) check_finite(phi, N, "checkpointed pressure");
+    copy_in(c, c->tmp2.p, phi, N, host);
+    launch_face_jump(s, F, c->fm.p, c->fp.p, c->farea.p, nullptr, c->tmp2.p, nullptr, c->f3.p);
+    if (jumps) {
+      if (host) check_finite(jumps, F, "force jumps");
+      copy_in(c, c->f2.p, jumps, F, host);
+    } else {
+      launch_fill(s, F, 0.0, c->f2.p);
+    }
+    launch_axpby(s, F, -1.0, c->f3.p, 1.0, c->f2.p);
+    launch_face_scale(s, F, c->sqk.p, c->f2.p, c->f3.p);  // Z
+    launch_face_scale(s, F, c->sqk.p, c->f1.p, c->f2.p);  // W
+    launch_metric_apply(s, F, c->kdiag.p, c->kptr.p, c->kface.p, c->kval.p, c->f2.p, c->f1.p);
+    launch_metric_apply(s, F, c->kdiag.p, c->kptr.p, c->kface.p, c->kval.p, c->f3.p, c->f4.p);
+    launch_adjoint_face(s, F, c->f2.p, c->f4.p, c->f3.p, c->f1.p, c->kappa.p, dt, c->f1.p);
+    copy_out(c, dJ_dkappa, c->f1.p, F, host);
+  }
+  HIPCHECK(hipStreamSynchronize(s));
+  return st;
+}
+}  // namespace
+
 int tpamr_project_adjoint(tpamr_ctx* c, const double* ubar, const double* phi, const double* jumps, double dt,
                           double rtol, int max_cycles, double* ubar_star, double* dJ_drhs, double* dJ_dkappa,
                           tpamr_stats* stats) {
@@ -1748,51 +1898,23 @@ int tpamr_project_adjoint(tpamr_ctx* c, const double* ubar, const double* phi, c
   return guard(c, [&] {
     need_device(c);
     need_solver(c);
-    if (!(dt > 0 && std::isfinite(dt))) throw std::invalid_argument("dt must be finite and strictly positive");
     HIPCHECK(hipSetDevice(c->device));
-    const TreeMesh& M = c->mesh();
-    const int N = M.ncells(), F = M.nfaces(), nb = M.nb();
-    hipStream_t s = c->stream;
-    check_finite(ubar, F, "velocity adjoint");
-    upload_vec(c, c->f1.p, ubar, F);
-    // A lambda = J^T M ubar = -V div(M ubar), M = sqrt(k) K sqrt(k)
-    launch_face_scale(s, F, c->sqk.p, c->f1.p, c->f2.p);
-    weighted_metric(c, c->f2.p, c->f3.p);
-    launch_divergence(s, N, nb, c->cptr.p, c->cface.p, c->csign.p, c->farea.p, c->f3.p, c->bptr.p, c->bface.p,
-                      c->bside.p, c->barea.p, nullptr, c->topo[0].volume.p, c->tmp.p);
-    launch_axpby(s, N, -1.0, c->tmp.p, 0.0, c->b.p);
-    launch_fill(s, N, 0.0, c->x.p);
-    tpamr_stats st = solve_dispatch(c, rtol, 0.0, max_cycles, 0);
-    if (dJ_drhs) {
-      // dJ/drhs = -dt V lambda
-      launch_scale_by_volume(s, N, nb, c->x.p, c->tmp.p, c->topo[0].volume.p, false);
-      launch_axpby(s, N, -dt, c->tmp.p, 0.0, c->tmp.p);
-      download_vec(c, dJ_drhs, c->tmp.p, N);
-    }
-    // ubar* = ubar - J lambda, J x = |f| (x+ - x-)
-    launch_face_jump(s, F, c->fm.p, c->fp.p, c->farea.p, nullptr, c->x.p, nullptr, c->f2.p);
-    launch_axpby(s, F, -1.0, c->f2.p, 1.0, c->f1.p);
-    download_vec(c, ubar_star, c->f1.p, F);
-    if (dJ_dkappa) {
-      // z = jumps - J phi; dJ/dk_f = dt [W_f (K Z)_f + Z_f (K W)_f] / (2 k_f)
-      // with W = sqrt(k) ubar*, Z = sqrt(k) z
-      check_finite(phi, N, "checkpointed pressure");
-      upload_vec(c, c->tmp2.p, phi, N);
-      launch_face_jump(s, F, c->fm.p, c->fp.p, c->farea.p, nullptr, c->tmp2.p, nullptr, c->f3.p);
-      if (jumps) {
-        check_finite(jumps, F, "force jumps");
-        upload_vec(c, c->f2.p, jumps, F);
-      } else {
-        launch_fill(s, F, 0.0, c->f2.p);
-      }
-      launch_axpby(s, F, -1.0, c->f3.p, 1.0, c->f2.p);
-      launch_face_scale(s, F, c->sqk.p, c->f2.p, c->f3.p);  // Z
-      launch_face_scale(s, F, c->sqk.p, c->f1.p, c->f2.p);  // W
-      launch_metric_apply(s, F, c->kdiag.p, c->kptr.p, c->kface.p, c->kval.p, c->f2.p, c->f1.p);
-      launch_metric_apply(s, F, c->kdiag.p, c->kptr.p, c->kface.p, c->kval.p, c->f3.p, c->f4.p);
-      launch_adjoint_face(s, F, c->f2.p, c->f4.p, c->f3.p, c->f1.p, c->kappa.p, dt, c->f1.p);
-      download_vec(c, dJ_dkappa, c->f1.p, F);
-    }
+    tpamr_stats st = project_adjoint_core(c, ubar, phi, jumps, dt, rtol, max_cycles, ubar_star, dJ_drhs, dJ_dkappa,
+                                          true);
+    if (stats) *stats = st;
+  });
+}
+
+int tpamr_project_adjoint_device(tpamr_ctx* c, const double* d_ubar, const double* d_phi, const double* d_jumps,
+                                 double dt, double rtol, int max_cycles, double* d_ubar_star, double* d_dJ_drhs,
+                                 double* d_dJ_dkappa, tpamr_stats* stats) {
+  if (!c || !d_ubar || !d_ubar_star || (d_dJ_dkappa && !d_phi)) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    need_solver(c);
+    HIPCHECK(hipSetDevice(c->device));
+    tpamr_stats st = project_adjoint_core(c, d_ubar, d_phi, d_jumps, dt, rtol, max_cycles, d_ubar_star, d_dJ_drhs,
+                                          d_dJ_dkappa, false);
     if (stats) *stats = st;
   });
 }

@@ -10,12 +10,13 @@ from .physics import (CGFM, CSF, divergence, fill_halo, gravity, gravity_jumps, 
                       refresh_two_phase, surface_tension, surface_tension_jumps, surface_tension_jumps_from_cells,
                       weighted_gradient,
                       weighted_jump_force)
-from .stepping import ProjectionRun, face_dual_volumes, kinetic_energy, staggered_liquid_fraction
+from .stepping import DeviceProjectionRun, ProjectionRun, face_dual_volumes, kinetic_energy, staggered_liquid_fraction
 from .solver import (EllipticSolver, LinearSolverOptions, LinearSolverStats, MultigridOptions, PoissonOperator, ldiv,
                      solve)
 
 __all__ = [
-    "BlockKey", "BlockTreeMesh", "CGFM", "CSF", "Context", "DimensionMismatch", "EllipticSolver", "HostHierarchy",
+    "BlockKey", "BlockTreeMesh", "CGFM", "CSF", "Context", "DeviceProjectionRun", "DimensionMismatch", "EllipticSolver",
+    "HostHierarchy",
     "LIB_PATH", "LinearSolverOptions", "LinearSolverStats", "MultigridOptions", "PoissonOperator", "ProjectionRun",
     "TPAMR_FP32_REFINE", "TPAMR_FP32_VCYCLE", "TPAMR_FP64", "divergence", "face_dual_volumes", "fill_halo",
     "gravity", "gravity_jumps", "halo_schedule", "kinetic_energy", "ldiv", "lib", "mixture_face_coefficients",

@@ -30,7 +30,7 @@ import numpy as np
 
 from .mesh import BlockTreeMesh
 from .physics import divergence, project, project_adjoint, weighted_gradient, weighted_jump_force
-from .solver import EllipticSolver
+from .solver import EllipticSolver, LinearSolverStats
 
 
 def face_dual_volumes(mesh: BlockTreeMesh):
@@ -142,4 +142,102 @@ class ProjectionRun:
             self._load(kappa)
             ub, db, gk[n], stats[n] = project_adjoint(s, ub, self.dt, phi=self.checkpoints[n], jumps=jumps,
                                                       rtol=self.adjoint_rtol, max_cycles=self.max_cycles)
+        return dict(rhs=db, kappa=gk, u0=ub, stats=stats)
+
+
+class DeviceProjectionRun:
+    """ProjectionRun with every field in HBM: the velocity, the steps' face
+    coefficients and force jumps, the checkpoints and the adjoint vectors are
+    torch tensors (float64) on the solver's device, and each step calls the
+    device-memory entry points (coefficient refresh, force, projection, its
+    pullback), so nothing crosses PCIe inside the loop. `steps` is a sequence
+    of (face coefficients, force jumps or None) tensors."""
+
+    def __init__(self, solver: EllipticSolver, dt, *, tolerance=1e-12, rtol=None, max_cycles=200,
+                 adjoint_rtol=1e-12):
+        if not dt > 0:
+            raise ValueError("dt must be strictly positive")
+        self.solver = solver
+        self.dt = float(dt)
+        self.tolerance = float(tolerance)
+        self.rtol = None if rtol is None else float(rtol)
+        self.max_cycles = int(max_cycles)
+        self.adjoint_rtol = float(adjoint_rtol)
+        self.steps = None
+        self.checkpoints = []
+        self.stats = []
+
+    def _check(self, rc):
+        from . import _lib as L
+        L.check(rc, self.solver.ctx.h)
+
+    @staticmethod
+    def _ptr(t):
+        return None if t is None else t.data_ptr()
+
+    def _tensor(self, t, n, what):
+        import torch
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.shape == (n,)
+                and t.is_contiguous()):
+            raise ValueError(f"{what} must be a contiguous float64 device tensor of length {n}")
+        return t
+
+    def forward(self, u0, steps):
+        import ctypes as C
+        import torch
+        from . import _lib as L
+        s = self.solver
+        s._sync()
+        m = s.operator.mesh
+        lib = L.lib()
+        h = s.ctx.h
+        u = self._tensor(u0, m.nfaces, "u0").clone()
+        div = torch.empty(m.ncells, dtype=torch.float64, device=u.device) if self.rtol is not None else None
+        self.steps = [(self._tensor(k, m.nfaces, "face coefficients"),
+                       None if j is None else self._tensor(j, m.nfaces, "force jumps")) for k, j in steps]
+        self.checkpoints = []
+        self.stats = []
+        for kappa, jumps in self.steps:
+            self._check(lib.tpamr_set_face_coefficients_device(h, kappa.data_ptr()))
+            if jumps is not None:
+                self._check(lib.tpamr_apply_force_device(h, jumps.data_ptr(), self.dt, u.data_ptr()))
+            tol = self.tolerance
+            if self.rtol is not None:
+                self._check(lib.tpamr_divergence_device(h, u.data_ptr(), div.data_ptr()))
+                tol = min(max(self.rtol * float(div.abs().max()) * self.dt, 1e-300), 0.5)
+            phi = torch.empty(m.ncells, dtype=torch.float64, device=u.device)
+            st = L.Stats()
+            self._check(lib.tpamr_project_device(h, u.data_ptr(), None, self.dt, tol, self.max_cycles, None, self.dt,
+                                                 phi.data_ptr(), C.byref(st)))
+            self.checkpoints.append(phi)
+            self.stats.append(LinearSolverStats._from(st, s.precision))
+        return u
+
+    def backward(self, ubar):
+        import ctypes as C
+        import torch
+        from . import _lib as L
+        if self.steps is None:
+            raise RuntimeError("forward() must run before backward()")
+        s = self.solver
+        m = s.operator.mesh
+        lib = L.lib()
+        h = s.ctx.h
+        ub = self._tensor(ubar, m.nfaces, "ubar").clone()
+        gk = [None] * len(self.steps)
+        stats = [None] * len(self.steps)
+        db = torch.empty(m.ncells, dtype=torch.float64, device=ub.device)
+        for n in range(len(self.steps) - 1, -1, -1):
+            kappa, jumps = self.steps[n]
+            self._check(lib.tpamr_set_face_coefficients_device(h, kappa.data_ptr()))
+            us = torch.empty_like(ub)
+            gk[n] = torch.empty_like(ub)
+            st = L.Stats()
+            self._check(lib.tpamr_project_adjoint_device(h, ub.data_ptr(), self.checkpoints[n].data_ptr(),
+                                                         self._ptr(jumps), self.dt, self.adjoint_rtol,
+                                                         self.max_cycles, us.data_ptr(),
+                                                         db.data_ptr() if n == 0 else None, gk[n].data_ptr(),
+                                                         C.byref(st)))
+            stats[n] = LinearSolverStats._from(st, s.precision)
+            ub = us
         return dict(rhs=db, kappa=gk, u0=ub, stats=stats)
