@@ -52,12 +52,18 @@ def staggered_liquid_fraction(mesh: BlockTreeMesh, fraction):
     return np.clip((hm * c[m] + hp * c[p]) / (hm + hp), 0.0, 1.0)
 
 
+def kinetic_energy_weights(mesh: BlockTreeMesh, fraction, density):
+    """rho_f V_f per face: the staggered mixture density times the face dual
+    volume."""
+    cf = staggered_liquid_fraction(mesh, fraction)
+    rho = density[0] * cf + density[1] * (1.0 - cf)
+    return rho * face_dual_volumes(mesh)
+
+
 def kinetic_energy(mesh: BlockTreeMesh, u, fraction, density):
     """sum_f rho_f u_f^2 V_f / 2 with the staggered mixture density; returns
     (J, dJ/du)."""
-    cf = staggered_liquid_fraction(mesh, fraction)
-    rho = density[0] * cf + density[1] * (1.0 - cf)
-    w = rho * face_dual_volumes(mesh)
+    w = kinetic_energy_weights(mesh, fraction, density)
     u = np.asarray(u, np.float64)
     return float(np.sum(w * u * u) / 2), w * u
 
