@@ -465,3 +465,34 @@ def test_pressure_predictor_matches_oracle():
     uo = u0 - 1e-3 * os_.weighted_gradient(p)
     assert rel(ug, uo) < 1e-13
     gs.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+def test_coarse_newton_schulz_update_matches_gauss_jordan(precision):
+    """After the first refresh the dense coarse inverse is updated from the
+    previous one by Newton-Schulz steps (GEMMs); a Gauss-Jordan inverse at
+    every refresh (TPAMR_COARSE_GJ=1) gives the same solves."""
+    _, pm = meshes("pruned_tree")
+    k1 = kappa_for(pm, "smooth")
+    x, y = pm.cell_centers()
+    rhs = np.sin(3 * x) * np.cos(5 * y)
+    out = []
+    for exact in (True, False):
+        if exact:
+            os.environ["TPAMR_COARSE_GJ"] = "1"
+        try:
+            s = tpamr.EllipticSolver(tpamr.PoissonOperator(pm, k1), precision=precision, device=0)
+        finally:
+            os.environ.pop("TPAMR_COARSE_GJ", None)
+        sols = []
+        for r in range(3):
+            s.operator.replace_face_coefficient(k1 * (1 + 0.01 * (r + 1) * np.cos(np.arange(pm.nfaces))))
+            s.refresh()
+            p = np.zeros(pm.ncells)
+            st = s.solve(p, rhs, rtol=1e-10)
+            sols.append((p, st.cycles))
+        out.append(sols)
+        s.close()
+    for (pg, cg), (pn, cn) in zip(*out):
+        assert cg == cn
+        assert rel(pn, pg) < 1e-12

@@ -1418,6 +1418,30 @@ void launch_sqrt(hipStream_t s, int n, const double* in, double* out) {
   if (n > 0) hipLaunchKernelGGL(k_sqrt, dim3(cdiv(n, 256)), dim3(256), 0, s, n, in, out);
 }
 
+// max |I - T|_ij of an N x N matrix as nonnegative-double bits (non-finite
+// entries count as infinity)
+__global__ void k_identity_residual(int N, const double* __restrict__ T, unsigned long long* __restrict__ out) {
+  const size_t q = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t NN = size_t(N) * N;
+  double m = 0;
+  for (size_t e = q; e < NN; e += size_t(gridDim.x) * blockDim.x) {
+    const size_t i = e / N, j = e % N;
+    double v = fabs((i == j ? 1.0 : 0.0) - T[e]);
+    if (!(v <= 1.7976931348623157e308)) v = __builtin_inf();
+    m = fmax(m, v);
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+void launch_identity_residual(hipStream_t s, int N, const double* T, unsigned long long* out) {
+  hipLaunchKernelGGL(k_identity_residual, dim3(1024), dim3(256), 0, s, N, T, out);
+}
+
+void launch_extract_block(hipStream_t s, int n, int N, const double* a, double* inv) {
+  hipLaunchKernelGGL(k_gj_extract, dim3(cdiv(n * n, 256)), dim3(256), 0, s, n, N, a, inv);
+}
+
 void launch_dense_inverse(hipStream_t s, int N, double* a, int* piv, double* col, int* fail, int n, double* inv) {
   for (int k = 0; k < N; ++k) {
     hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, s, N, k, a, piv, col, fail);

@@ -100,6 +100,10 @@ void launch_sqrt(hipStream_t s, int n, const double* in, double* out);
 // with partial pivoting, piv and col N-entry scratch); its top-left n x n
 // block goes to `inv` column major; *fail is set on a zero or non-finite pivot
 void launch_dense_inverse(hipStream_t s, int N, double* a, int* piv, double* col, int* fail, int n, double* inv);
+// *out = max(*out, max_ij |I - T|_ij) as the bits of a nonnegative double
+void launch_identity_residual(hipStream_t s, int N, const double* T, unsigned long long* out);
+// top-left n x n block of the row-major N x N `a`, column major
+void launch_extract_block(hipStream_t s, int n, int N, const double* a, double* inv);
 
 // The V-cycle of the small levels in one workgroup (k_vtail): levels
 // 0..n-1 of the tail (the last one the coarsest), their vectors and weights.

@@ -2,6 +2,7 @@
 // launch sequence, the FP64 Krylov driver with the reference's stopping
 // rules, and the face-level two-phase / projection / adjoint operations.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -130,6 +131,14 @@ struct tpamr_ctx {
   std::vector<DevLevel<float>> lf;   // FP32 levels (FP32 mode)
   DevArray<double> coarse_inv, coarse_work, coarse_col;
   DevArray<int> coarse_piv, coarse_fail;
+  // full inverse of the last coarse refresh (the start of the next one's
+  // Newton-Schulz update), its identity residual, GEMM scratch
+  DevArray<double> coarse_x, coarse_t, coarse_xn;
+  DevArray<unsigned long long> coarse_res;
+  bool coarse_x_valid = false;
+  bool coarse_exact = false;  // TPAMR_COARSE_GJ=1: Gauss-Jordan at every refresh
+  double coarse_gj_residual = 0;
+  rocblas_handle blas = nullptr;
   // coefficient refresh on the device (after the first, host-built one);
   // TPAMR_HOST_REFRESH=1 in the environment keeps every refresh on the host
   std::vector<DevPlan> plan;
@@ -871,23 +880,104 @@ void weighted_metric(tpamr_ctx* c, const double* in_scaled, double* out) {
   launch_face_scale(c->stream, F, c->sqk.p, c->f4.p, out);
 }
 
-// Dense coarse inverse of H.coarse_aug on the device into c->coarse_inv.
+// max |I - T| of the N x N product in c->coarse_t
+double identity_residual(tpamr_ctx* c, int N) {
+  hipStream_t s = c->stream;
+  c->coarse_res.alloc(1);
+  HIPCHECK(hipMemsetAsync(c->coarse_res.p, 0, sizeof(unsigned long long), s));
+  launch_identity_residual(s, N, c->coarse_t.p, c->coarse_res.p);
+  unsigned long long bits = 0;
+  HIPCHECK(hipMemcpyAsync(&bits, c->coarse_res.p, sizeof(bits), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  double r;
+  std::memcpy(&r, &bits, sizeof(r));
+  return r;
+}
+
+void blas_check(rocblas_status st) {
+  if (st != rocblas_status_success) throw HipError("rocBLAS call failed (status " + std::to_string(int(st)) + ")");
+}
+
+// T = A X for row-major N x N A (c->coarse_work) and X (c->coarse_x): the
+// column-major product X^T A^T of the same buffers
+void coarse_product(tpamr_ctx* c, int N) {
+  const double one = 1.0, zero = 0.0;
+  blas_check(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, N, N, N, &one, c->coarse_x.p, N,
+                           c->coarse_work.p, N, &zero, c->coarse_t.p, N));
+}
+
+// Newton-Schulz update X <- X (2 I - A X) of the previous refresh's inverse,
+// while its identity residual falls; true when the residual reaches the
+// level of a Gauss-Jordan inverse (4x its residual, at least 1e-13).
+bool coarse_newton_schulz(tpamr_ctx* c, int N) {
+  hipStream_t s = c->stream;
+  if (!c->blas) {
+    blas_check(rocblas_create_handle(&c->blas));
+    blas_check(rocblas_set_pointer_mode(c->blas, rocblas_pointer_mode_host));
+  }
+  blas_check(rocblas_set_stream(c->blas, s));
+  c->coarse_t.alloc(size_t(N) * N);
+  c->coarse_xn.alloc(size_t(N) * N);
+  const double accept = std::max(1e-13, 4 * c->coarse_gj_residual);
+  double prev = 0;
+  for (int it = 0; it < 6; ++it) {
+    coarse_product(c, N);
+    const double r = identity_residual(c, N);
+    if (r <= accept) return true;
+    if (!(r < 0.25) || (it > 0 && !(r < 0.25 * prev))) return false;
+    prev = r;
+    // X_n = 2 X - X T (row major) = 2 X^T - T^T X^T (column major)
+    const double m1 = -1.0, two = 2.0;
+    HIPCHECK(hipMemcpyAsync(c->coarse_xn.p, c->coarse_x.p, sizeof(double) * N * size_t(N), hipMemcpyDeviceToDevice,
+                            s));
+    blas_check(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, N, N, N, &m1, c->coarse_t.p, N,
+                             c->coarse_x.p, N, &two, c->coarse_xn.p, N));
+    std::swap(c->coarse_x.p, c->coarse_xn.p);
+  }
+  return false;
+}
+
+// Dense coarse inverse of H.coarse_aug on the device into c->coarse_inv: a
+// Newton-Schulz update of the previous refresh's inverse when it converges
+// (the coefficients of consecutive steps differ little), else Gauss-Jordan.
 void coarse_inverse_device(tpamr_ctx* c) {
   Hierarchy& H = *c->H;
   hipStream_t s = c->stream;
   const int n = H.coarse_n, N = n + 1;
+  const size_t NN = size_t(N) * N;
   c->coarse_work.upload(H.coarse_aug, s);
+  c->coarse_inv.alloc(size_t(n) * n);
+  if (c->coarse_x_valid && c->coarse_x.n == NN && !c->coarse_exact && coarse_newton_schulz(c, N)) {
+    launch_extract_block(s, n, N, c->coarse_x.p, c->coarse_inv.p);
+    return;
+  }
   c->coarse_col.alloc(N);
   c->coarse_piv.alloc(N);
   c->coarse_fail.alloc(1);
-  c->coarse_inv.alloc(size_t(n) * n);
+  c->coarse_x.alloc(NN);
+  // Gauss-Jordan on a copy; A stays in coarse_work for the residual
+  HIPCHECK(hipMemcpyAsync(c->coarse_x.p, c->coarse_work.p, sizeof(double) * NN, hipMemcpyDeviceToDevice, s));
   HIPCHECK(hipMemsetAsync(c->coarse_fail.p, 0, sizeof(int), s));
-  launch_dense_inverse(s, N, c->coarse_work.p, c->coarse_piv.p, c->coarse_col.p, c->coarse_fail.p, n,
+  launch_dense_inverse(s, N, c->coarse_x.p, c->coarse_piv.p, c->coarse_col.p, c->coarse_fail.p, n,
                        c->coarse_inv.p);
   int fail = 0;
   HIPCHECK(hipMemcpyAsync(&fail, c->coarse_fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHECK(hipStreamSynchronize(s));
-  if (fail) throw std::runtime_error("singular coarse operator");
+  if (fail) {
+    c->coarse_x_valid = false;
+    throw std::runtime_error("singular coarse operator");
+  }
+  c->coarse_x_valid = true;
+  if (!c->coarse_exact) {
+    if (!c->blas) {
+      blas_check(rocblas_create_handle(&c->blas));
+      blas_check(rocblas_set_pointer_mode(c->blas, rocblas_pointer_mode_host));
+    }
+    blas_check(rocblas_set_stream(c->blas, s));
+    c->coarse_t.alloc(NN);
+    coarse_product(c, N);
+    c->coarse_gj_residual = identity_residual(c, N);
+  }
 }
 
 void upload_plan(tpamr_ctx* c) {
@@ -1081,6 +1171,7 @@ int tpamr_create(tpamr_ctx** out, int device, int precision) {
   if (const char* q = std::getenv("TPAMR_PIPE")) c->pipe = std::min(PIPE_MAX, std::max(2, std::atoi(q)));
   if (const char* q = std::getenv("TPAMR_TRACE")) c->trace = std::strcmp(q, "0") != 0;
   if (const char* q = std::getenv("TPAMR_HOST_REFRESH")) c->host_refresh = std::strcmp(q, "0") != 0;
+  if (const char* q = std::getenv("TPAMR_COARSE_GJ")) c->coarse_exact = std::strcmp(q, "0") != 0;
   if (const char* q = std::getenv("TPAMR_GRAPH_ITERS")) c->graph_iters = std::min(16, std::max(1, std::atoi(q)));
   if (const char* q = std::getenv("TPAMR_TAIL_BLOCKS")) c->tail_blocks = std::max(0, std::atoi(q));
   if (const char* q = std::getenv("TPAMR_IR_DELTA")) {
@@ -1117,6 +1208,7 @@ void tpamr_destroy(tpamr_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     destroy_graph(c);
+    if (c->blas) (void)rocblas_destroy_handle(c->blas);
     c->topo.clear();
     c->ld.clear();
     c->lf.clear();
