@@ -12,8 +12,14 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <stdexcept>
+#include <string>
 #include <thread>
 #include <tuple>
 
@@ -21,7 +27,104 @@ namespace tpamr {
 
 namespace {
 
+// TPAMR_TRACE=1: wall time of each setup phase to stderr.
+struct PhaseTimer {
+  bool on;
+  std::chrono::steady_clock::time_point t;
+  PhaseTimer() {
+    const char* q = std::getenv("TPAMR_TRACE");
+    on = q && std::strcmp(q, "0") != 0;
+    t = std::chrono::steady_clock::now();
+  }
+  void mark(const char* what, long n = -1) {
+    if (!on) return;
+    const auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "tpamr setup: %-28s %9.1f ms", what, std::chrono::duration<double, std::milli>(now - t).count());
+    if (n >= 0) std::fprintf(stderr, "  (%ld)", n);
+    std::fprintf(stderr, "\n");
+    t = now;
+  }
+};
+
 inline long wrapl(long x, long p) { return ((x % p) + p) % p; }
+
+// Workers for n items: up to 16 threads, at least `grain` items each.
+inline int worker_count(long n, long grain = 1 << 14) {
+  const long hw = std::max<long>(1, std::thread::hardware_concurrency());
+  return int(std::max<long>(1, std::min<long>({16L, hw, (n + grain - 1) / grain})));
+}
+
+// Run f(lo, hi, t) for t < nt over the contiguous chunks [n t / nt, n (t + 1)
+// / nt); the split depends on n and nt only, so two passes with the same nt
+// see the same chunks.
+template <typename F>
+void run_chunks(long n, int nt, F&& f) {
+  if (nt <= 1) {
+    f(0L, n, 0);
+    return;
+  }
+  std::vector<std::thread> pool;
+  pool.reserve(nt);
+  for (int t = 0; t < nt; ++t) pool.emplace_back([&, t] { f(n * t / nt, n * (t + 1) / nt, t); });
+  for (auto& th : pool) th.join();
+}
+
+template <typename F>
+void parallel_for(long n, F&& f, long grain = 1 << 14) {
+  run_chunks(n, worker_count(n, grain), f);
+}
+
+// Run f(k) for every task k < n on up to 16 threads, taking tasks in turn.
+template <typename F>
+void parallel_tasks(long n, F&& f) {
+  const int nt = worker_count(n, 1);
+  if (nt <= 1) {
+    for (long k = 0; k < n; ++k) f(k);
+    return;
+  }
+  std::atomic<long> next{0};
+  std::vector<std::thread> pool;
+  pool.reserve(nt);
+  for (int t = 0; t < nt; ++t)
+    pool.emplace_back([&] {
+      for (long k; (k = next.fetch_add(1)) < n;) f(k);
+    });
+  for (auto& th : pool) th.join();
+}
+
+// Sort v by the strict total order `less` in parallel: bucket(x) in [0, nbk)
+// must not decrease along the order, so the elements are scattered to their
+// buckets (per-thread counts keep the scatter deterministic) and every bucket
+// is sorted on its own. The result is the one std::sort gives.
+template <typename V, typename B, typename C>
+void bucket_sort(V& v, int nbk, B bucket, C less) {
+  const long n = long(v.size());
+  const int nt = worker_count(n, 1 << 16);
+  if (nt <= 1) {
+    std::sort(v.begin(), v.end(), less);
+    return;
+  }
+  std::vector<std::vector<long>> h(nt, std::vector<long>(nbk, 0));
+  run_chunks(n, nt, [&](long a, long b, int w) {
+    for (long q = a; q < b; ++q) h[w][bucket(v[q])]++;
+  });
+  std::vector<long> start(nbk + 1, 0);
+  for (int k = 0; k < nbk; ++k) {
+    long at = start[k];
+    for (int w = 0; w < nt; ++w) {
+      const long c = h[w][k];
+      h[w][k] = at;
+      at += c;
+    }
+    start[k + 1] = at;
+  }
+  hvec<typename V::value_type> out(n);
+  run_chunks(n, nt, [&](long a, long b, int w) {
+    for (long q = a; q < b; ++q) out[h[w][bucket(v[q])]++] = v[q];
+  });
+  parallel_tasks(nbk, [&](long k) { std::sort(out.begin() + start[k], out.begin() + start[k + 1], less); });
+  run_chunks(n, nt, [&](long a, long b, int) { std::copy(out.begin() + a, out.begin() + b, v.begin() + a); });
+}
 
 void collect_leaves(const TreeMesh& M, std::vector<BlockKey>& leaves) {
   leaves.clear();
@@ -102,7 +205,11 @@ void block_origin(const TreeMesh& M, const BlockKey& k, double* origin) {
 
 // Face schedule in the reference order: axis-major, then lattice line, then
 // increasing tangential position; boundary faces per line after the
-// interior faces of that line.
+// interior faces of that line. Per axis, every cell is bucketed by the line
+// of its lower and of its upper side (per-thread histograms, so the buckets
+// fill in parallel in cell order); each line's two buckets are sorted by
+// tangential position and merged, once to count the faces of every line and
+// once to write them at the line's offset.
 void build_faces(TreeMesh& M) {
   const int maxl = M.max_leaf;
   const long lat[2] = {long(M.base[0]) << maxl, long(M.base[1]) << maxl};
@@ -113,81 +220,142 @@ void build_faces(TreeMesh& M) {
     long lo, hi;
     int cell;
   };
-  std::vector<long> xlo(ncell), ylo(ncell), sc(ncell);
-  for (int b = 0; b < nblk; ++b) {
-    const long s = 1L << (maxl - M.blocks[b].level);
-    for (int j = 0; j < by; ++j)
-      for (int i = 0; i < bx; ++i) {
-        const long c = long(b) * nb + j * bx + i;
-        xlo[c] = (long(M.blocks[b].i) * bx + i) * s;
-        ylo[c] = (long(M.blocks[b].j) * by + j) * s;
-        sc[c] = s;
-      }
-  }
+  hvec<long> xlo(ncell), ylo(ncell), sc(ncell);
+  parallel_for(nblk, [&](long b0, long b1, int) {
+    for (long b = b0; b < b1; ++b) {
+      const long s = 1L << (maxl - M.blocks[b].level);
+      for (int j = 0; j < by; ++j)
+        for (int i = 0; i < bx; ++i) {
+          const long c = b * nb + j * bx + i;
+          xlo[c] = (long(M.blocks[b].i) * bx + i) * s;
+          ylo[c] = (long(M.blocks[b].j) * by + j) * s;
+          sc[c] = s;
+        }
+    }
+  }, 256);
+  hvec<Side> lower(ncell), upper(ncell);
+  // a line's merge yields at most |lower| + |upper| - 1 faces and leaves at
+  // most that many boundary pieces: 2 ncell of each per axis
+  for (auto* v : {&M.fminus, &M.fplus}) v->reserve(M.fminus.size() + 4 * size_t(ncell));
+  M.faxis.reserve(M.faxis.size() + 4 * size_t(ncell));
+  for (auto* v : {&M.farea, &M.fcx, &M.fcy}) v->reserve(v->size() + 4 * size_t(ncell));
+  M.bcell.reserve(M.bcell.size() + 4 * size_t(ncell));
+  for (auto* v : {&M.baxis, &M.bside}) v->reserve(v->size() + 4 * size_t(ncell));
+  M.barea.reserve(M.barea.size() + 4 * size_t(ncell));
   for (int axis = 0; axis < 2; ++axis) {
     const int t = 1 - axis;
     const long lines = lat[axis];
-    std::vector<long> lcount(lines + 2, 0), ucount(lines + 2, 0);
-    auto low_of = [&](long c) { return axis == 0 ? xlo[c] : ylo[c]; };
-    auto tlo_of = [&](long c) { return axis == 0 ? ylo[c] : xlo[c]; };
+    const hvec<long>& low = axis == 0 ? xlo : ylo;
+    const hvec<long>& tlo = axis == 0 ? ylo : xlo;
     auto canon = [&](long v) { return M.periodic[axis] ? wrapl(v, lines) : v; };
-    for (long c = 0; c < ncell; ++c) {
-      lcount[canon(low_of(c) + sc[c]) + 1]++;
-      ucount[canon(low_of(c)) + 1]++;
-    }
-    for (long q = 0; q <= lines; ++q) {
-      lcount[q + 1] += lcount[q];
-      ucount[q + 1] += ucount[q];
-    }
-    std::vector<Side> lower(ncell), upper(ncell);
-    {
-      std::vector<long> ln(lcount.begin(), lcount.end() - 1), un(ucount.begin(), ucount.end() - 1);
-      for (long c = 0; c < ncell; ++c) {
-        const Side s{tlo_of(c), tlo_of(c) + sc[c], int(c)};
-        lower[ln[canon(low_of(c) + sc[c])]++] = s;
-        upper[un[canon(low_of(c))]++] = s;
+    // per-thread bucket counts, then each thread's start in every bucket
+    const int nt = worker_count(ncell);
+    std::vector<std::vector<long>> lh(nt, std::vector<long>(lines + 1, 0)), uh(nt, std::vector<long>(lines + 1, 0));
+    run_chunks(ncell, nt, [&](long c0, long c1, int w) {
+      for (long c = c0; c < c1; ++c) {
+        lh[w][canon(low[c] + sc[c])]++;
+        uh[w][canon(low[c])]++;
       }
+    });
+    std::vector<long> lcount(lines + 2, 0), ucount(lines + 2, 0);
+    for (long q = 0; q <= lines; ++q) {
+      long ls = lcount[q], us = ucount[q];
+      for (int w = 0; w < nt; ++w) {
+        const long a = lh[w][q], b = uh[w][q];
+        lh[w][q] = ls;
+        uh[w][q] = us;
+        ls += a;
+        us += b;
+      }
+      lcount[q + 1] = ls;
+      ucount[q + 1] = us;
     }
+    run_chunks(ncell, nt, [&](long c0, long c1, int w) {
+      for (long c = c0; c < c1; ++c) {
+        const Side s{tlo[c], tlo[c] + sc[c], int(c)};
+        lower[lh[w][canon(low[c] + sc[c])]++] = s;
+        upper[uh[w][canon(low[c])]++] = s;
+      }
+    });
+    lh.clear();
+    uh.clear();
     const double nscale = M.extent[axis] / double(lat[axis]);
     const double tscale = M.extent[t] / double(lat[t]);
     auto by_lo = [](const Side& a, const Side& b) { return a.lo < b.lo; };
-    for (long line = 0; line <= lines; ++line) {
+    // faces and boundary faces of each line: counted, then written
+    std::vector<long> nf(lines + 2, 0), nbf(lines + 2, 0);
+    const size_t f0 = M.fminus.size(), bf0 = M.bcell.size();
+    auto merge = [&](long line, bool write) {
       Side* l0 = lower.data() + lcount[line];
       Side* l1 = lower.data() + lcount[line + 1];
       Side* u0 = upper.data() + ucount[line];
       Side* u1 = upper.data() + ucount[line + 1];
-      std::sort(l0, l1, by_lo);
-      std::sort(u0, u1, by_lo);
+      if (!write) {
+        std::sort(l0, l1, by_lo);
+        std::sort(u0, u1, by_lo);
+      }
+      size_t fo = write ? f0 + nf[line] : 0, bo = write ? bf0 + nbf[line] : 0;
+      long count = 0;
       Side *li = l0, *ui = u0;
       while (li < l1 && ui < u1) {
         const long lo = std::max(li->lo, ui->lo), hi = std::min(li->hi, ui->hi);
         if (lo < hi) {
-          M.fminus.push_back(li->cell);
-          M.fplus.push_back(ui->cell);
-          M.faxis.push_back(int8_t(axis));
-          M.farea.push_back((hi - lo) * tscale);
-          const double nc = M.origin[axis] + line * nscale;
-          const double tc = M.origin[t] + (lo + hi) * tscale / 2;
-          M.fcx.push_back(axis == 0 ? nc : tc);
-          M.fcy.push_back(axis == 0 ? tc : nc);
+          if (write) {
+            M.fminus[fo] = li->cell;
+            M.fplus[fo] = ui->cell;
+            M.faxis[fo] = int8_t(axis);
+            M.farea[fo] = (hi - lo) * tscale;
+            const double nc = M.origin[axis] + line * nscale;
+            const double tc = M.origin[t] + (lo + hi) * tscale / 2;
+            M.fcx[fo] = axis == 0 ? nc : tc;
+            M.fcy[fo] = axis == 0 ? tc : nc;
+            ++fo;
+          }
+          ++count;
         }
-        const long lh = li->hi, uh = ui->hi;
-        if (lh <= uh) ++li;
-        if (uh <= lh) ++ui;
+        const long lh_ = li->hi, uh_ = ui->hi;
+        if (lh_ <= uh_) ++li;
+        if (uh_ <= lh_) ++ui;
       }
-      for (; li < l1; ++li) {
-        M.bcell.push_back(li->cell);
-        M.baxis.push_back(int8_t(axis));
-        M.bside.push_back(1);
-        M.barea.push_back((li->hi - li->lo) * M.extent[t] / double(lat[t]));
+      if (!write) {
+        nf[line + 1] = count;
+        nbf[line + 1] = (l1 - li) + (u1 - ui);
+        return;
       }
-      for (; ui < u1; ++ui) {
-        M.bcell.push_back(ui->cell);
-        M.baxis.push_back(int8_t(axis));
-        M.bside.push_back(-1);
-        M.barea.push_back((ui->hi - ui->lo) * M.extent[t] / double(lat[t]));
+      for (; li < l1; ++li, ++bo) {
+        M.bcell[bo] = li->cell;
+        M.baxis[bo] = int8_t(axis);
+        M.bside[bo] = 1;
+        M.barea[bo] = (li->hi - li->lo) * M.extent[t] / double(lat[t]);
       }
+      for (; ui < u1; ++ui, ++bo) {
+        M.bcell[bo] = ui->cell;
+        M.baxis[bo] = int8_t(axis);
+        M.bside[bo] = -1;
+        M.barea[bo] = (ui->hi - ui->lo) * M.extent[t] / double(lat[t]);
+      }
+    };
+    parallel_for(lines + 1, [&](long q0, long q1, int) {
+      for (long q = q0; q < q1; ++q) merge(q, false);
+    }, 16);
+    for (long q = 0; q <= lines; ++q) {
+      nf[q + 1] += nf[q];
+      nbf[q + 1] += nbf[q];
     }
+    const size_t nfa = f0 + nf[lines + 1], nba = bf0 + nbf[lines + 1];
+    M.fminus.resize(nfa);
+    M.fplus.resize(nfa);
+    M.faxis.resize(nfa);
+    M.farea.resize(nfa);
+    M.fcx.resize(nfa);
+    M.fcy.resize(nfa);
+    M.bcell.resize(nba);
+    M.baxis.resize(nba);
+    M.bside.resize(nba);
+    M.barea.resize(nba);
+    parallel_for(lines + 1, [&](long q0, long q1, int) {
+      for (long q = q0; q < q1; ++q) merge(q, true);
+    }, 16);
   }
 }
 
@@ -226,7 +394,9 @@ std::unique_ptr<TreeMesh> make_tree_mesh(const int base[2], const int bs[2], con
     M->internal.insert(key64(k.level, k.i, k.j));
     for (int c = 0; c < 4; ++c) stack.push_back({k.level + 1, 2 * k.i + (c & 1), 2 * k.j + (c >> 1)});
   }
+  PhaseTimer pt;
   balance(*M);
+  pt.mark("balance");
   collect_leaves(*M, M->blocks);
   std::sort(M->blocks.begin(), M->blocks.end(), [](const BlockKey& a, const BlockKey& b) {
     return std::make_tuple(a.level, a.j, a.i) < std::make_tuple(b.level, b.j, b.i);
@@ -242,7 +412,9 @@ std::unique_ptr<TreeMesh> make_tree_mesh(const int base[2], const int bs[2], con
     block_origin(*M, k, &M->borigin[2 * b]);
     M->max_leaf = std::max(M->max_leaf, k.level);
   }
+  pt.mark("leaves and block geometry", long(M->blocks.size()));
   build_faces(*M);
+  pt.mark("faces", long(M->nfaces()));
   return M;
 }
 
@@ -315,146 +487,217 @@ FaceMetric face_metric(const TreeMesh& M) {
   // same-level leaf flags around each block: W E S N SW SE NW NE
   std::vector<std::array<uint8_t, 8>> flag(nblk);
   const int dxs[8] = {-1, 1, 0, 0, -1, 1, -1, 1}, dys[8] = {0, 0, -1, 1, -1, -1, 1, 1};
-  for (int b = 0; b < nblk; ++b) {
-    const BlockKey& k = M.blocks[b];
-    const long px = long(M.layout[0]) << k.level, py = long(M.layout[1]) << k.level;
-    for (int q = 0; q < 8; ++q) {
-      long x = k.i + dxs[q], y = k.j + dys[q];
-      bool ok = true;
-      if (x < 0 || x >= px) {
-        if (M.periodic[0]) x = wrapl(x, px);
-        else ok = false;
+  parallel_for(nblk, [&](long b0, long b1, int) {
+    for (long b = b0; b < b1; ++b) {
+      const BlockKey& k = M.blocks[b];
+      const long px = long(M.layout[0]) << k.level, py = long(M.layout[1]) << k.level;
+      for (int q = 0; q < 8; ++q) {
+        long x = k.i + dxs[q], y = k.j + dys[q];
+        bool ok = true;
+        if (x < 0 || x >= px) {
+          if (M.periodic[0]) x = wrapl(x, px);
+          else ok = false;
+        }
+        if (y < 0 || y >= py) {
+          if (M.periodic[1]) y = wrapl(y, py);
+          else ok = false;
+        }
+        // outside a wall: the vertex is physical and handled before this test
+        flag[b][q] = ok ? uint8_t(M.block_of.count(key64(k.level, int(x), int(y))) != 0) : uint8_t(1);
       }
-      if (y < 0 || y >= py) {
-        if (M.periodic[1]) y = wrapl(y, py);
-        else ok = false;
-      }
-      // outside a wall: the vertex is physical and handled before this test
-      flag[b][q] = ok ? uint8_t(M.block_of.count(key64(k.level, int(x), int(y))) != 0) : uint8_t(1);
     }
-  }
+  }, 256);
   FaceMetric K;
-  K.diag.assign(F, 0.0);
-  std::vector<std::array<double, 2>> disp(F);
-  std::unordered_map<uint64_t, std::vector<int>> cand;  // vertex -> faces
-  for (int f = 0; f < F; ++f) {
-    const int m = M.fminus[f], p = M.fplus[f];
-    const int a = M.faxis[f], t = 1 - a;
-    const int bm = m / nb, bp = p / nb;
-    const BlockKey& km = M.blocks[bm];
-    const BlockKey& kp = M.blocks[bp];
-    const double hm = M.bspacing[2 * bm + a], hp = M.bspacing[2 * bp + a];
-    const long sm = 1L << (maxl - km.level), sp = 1L << (maxl - kp.level);
-    const long mx = (long(km.i) * bx + (m % nb) % bx) * sm, my = (long(km.j) * by + (m % nb) / bx) * sm;
-    const long qx = (long(kp.i) * bx + (p % nb) % bx) * sp, qy = (long(kp.j) * by + (p % nb) / bx) * sp;
-    const long mlo[2] = {mx, my}, plo[2] = {qx, qy};
-    // displacement: normal = mean width; tangential = lattice centre offset
-    long offset = 2 * plo[t] + sp - 2 * mlo[t] - sm;
-    if (M.periodic[t]) {
-      if (offset > lat[t]) offset -= 2 * lat[t];
-      if (offset < -lat[t]) offset += 2 * lat[t];
+  K.diag.resize(F);
+  hvec<std::array<double, 2>> disp(F);
+  // (vertex, face) pairs of the irregular vertices, per thread in face order
+  const int nt = worker_count(F);
+  std::vector<std::vector<std::pair<uint64_t, int>>> cand_t(nt);
+  run_chunks(F, nt, [&](long fa, long fb, int w) {
+    auto& cand = cand_t[w];
+    for (long f = fa; f < fb; ++f) {
+      const int m = M.fminus[f], p = M.fplus[f];
+      const int a = M.faxis[f], t = 1 - a;
+      const int bm = m / nb, bp = p / nb;
+      const BlockKey& km = M.blocks[bm];
+      const BlockKey& kp = M.blocks[bp];
+      const double hm = M.bspacing[2 * bm + a], hp = M.bspacing[2 * bp + a];
+      const long sm = 1L << (maxl - km.level), sp = 1L << (maxl - kp.level);
+      const long mx = (long(km.i) * bx + (m % nb) % bx) * sm, my = (long(km.j) * by + (m % nb) / bx) * sm;
+      const long qx = (long(kp.i) * bx + (p % nb) % bx) * sp, qy = (long(kp.j) * by + (p % nb) / bx) * sp;
+      const long mlo[2] = {mx, my}, plo[2] = {qx, qy};
+      // displacement: normal = mean width; tangential = lattice centre offset
+      long offset = 2 * plo[t] + sp - 2 * mlo[t] - sm;
+      if (M.periodic[t]) {
+        if (offset > lat[t]) offset -= 2 * lat[t];
+        if (offset < -lat[t]) offset += 2 * lat[t];
+      }
+      disp[f][a] = (hm + hp) / 2;
+      disp[f][t] = double(offset) * ((M.extent[t] / double(lat[t])) / 2);
+      K.diag[f] = 0.0;
+      long normal = mlo[a] + sm;
+      if (M.periodic[a]) normal = wrapl(normal, lat[a]);
+      const long lo = std::max(mlo[t], plo[t]), hi = std::min(mlo[t] + sm, plo[t] + sp);
+      const double simple = 1.0 / (2.0 * M.farea[f] * disp[f][a]);
+      const bool same_level = km.level == kp.level;
+      for (int e = 0; e < 2; ++e) {
+        const long tv = e == 0 ? lo : hi;
+        long vx = a == 0 ? normal : tv, vy = a == 0 ? tv : normal;
+        if (M.periodic[0]) vx = wrapl(vx, lat[0]);
+        if (M.periodic[1]) vy = wrapl(vy, lat[1]);
+        const bool physical = (!M.periodic[0] && (vx == 0 || vx == lat[0])) ||
+                              (!M.periodic[1] && (vy == 0 || vy == lat[1]));
+        bool regular = physical;
+        if (!regular && same_level) {
+          // position of the vertex relative to the minus block, in its cells
+          const long bx0 = long(km.i) * bx * sm, by0 = long(km.j) * by * sm;
+          long lx = vx - bx0, ly = vy - by0;
+          const long spanx = long(bx) * sm, spany = long(by) * sm;
+          if (M.periodic[0]) {
+            if (lx < 0) lx += lat[0];
+            if (lx > spanx) lx -= lat[0];
+          }
+          if (M.periodic[1]) {
+            if (ly < 0) ly += lat[1];
+            if (ly > spany) ly -= lat[1];
+          }
+          const int ex = lx == 0 ? -1 : (lx == spanx ? 1 : 0);
+          const int ey = ly == 0 ? -1 : (ly == spany ? 1 : 0);
+          const auto& fl = flag[bm];
+          if (ex == 0 && ey == 0) regular = true;
+          else if (ey == 0) regular = fl[ex < 0 ? 0 : 1];
+          else if (ex == 0) regular = fl[ey < 0 ? 2 : 3];
+          else {
+            const int side_x = ex < 0 ? 0 : 1, side_y = ey < 0 ? 2 : 3;
+            const int corner = (ey < 0 ? 4 : 6) + (ex < 0 ? 0 : 1);
+            regular = fl[side_x] && fl[side_y] && fl[corner];
+          }
+        }
+        if (regular) {
+          K.diag[f] += simple;
+        } else {
+          cand.push_back({(uint64_t(vx) << 32) | uint64_t(vy), int(f)});
+        }
+      }
     }
-    disp[f][a] = (hm + hp) / 2;
-    disp[f][t] = double(offset) * ((M.extent[t] / double(lat[t])) / 2);
-    long normal = mlo[a] + sm;
-    if (M.periodic[a]) normal = wrapl(normal, lat[a]);
-    const long lo = std::max(mlo[t], plo[t]), hi = std::min(mlo[t] + sm, plo[t] + sp);
-    const double simple = 1.0 / (2.0 * M.farea[f] * disp[f][a]);
-    const bool same_level = km.level == kp.level;
-    for (int e = 0; e < 2; ++e) {
-      const long tv = e == 0 ? lo : hi;
-      long vx = a == 0 ? normal : tv, vy = a == 0 ? tv : normal;
-      if (M.periodic[0]) vx = wrapl(vx, lat[0]);
-      if (M.periodic[1]) vy = wrapl(vy, lat[1]);
-      const bool physical = (!M.periodic[0] && (vx == 0 || vx == lat[0])) ||
-                            (!M.periodic[1] && (vy == 0 || vy == lat[1]));
-      bool regular = physical;
-      if (!regular && same_level) {
-        // position of the vertex relative to the minus block, in its cells
-        const long bx0 = long(km.i) * bx * sm, by0 = long(km.j) * by * sm;
-        long lx = vx - bx0, ly = vy - by0;
-        const long spanx = long(bx) * sm, spany = long(by) * sm;
-        if (M.periodic[0]) {
-          if (lx < 0) lx += lat[0];
-          if (lx > spanx) lx -= lat[0];
-        }
-        if (M.periodic[1]) {
-          if (ly < 0) ly += lat[1];
-          if (ly > spany) ly -= lat[1];
-        }
-        const int ex = lx == 0 ? -1 : (lx == spanx ? 1 : 0);
-        const int ey = ly == 0 ? -1 : (ly == spany ? 1 : 0);
-        const auto& fl = flag[bm];
-        if (ex == 0 && ey == 0) regular = true;
-        else if (ey == 0) regular = fl[ex < 0 ? 0 : 1];
-        else if (ex == 0) regular = fl[ey < 0 ? 2 : 3];
-        else {
-          const int side_x = ex < 0 ? 0 : 1, side_y = ey < 0 ? 2 : 3;
-          const int corner = (ey < 0 ? 4 : 6) + (ex < 0 ? 0 : 1);
-          regular = fl[side_x] && fl[side_y] && fl[corner];
-        }
-      }
-      if (regular) {
-        K.diag[f] += simple;
-      } else {
-        cand[(uint64_t(vx) << 32) | uint64_t(vy)].push_back(f);
-      }
+  });
+  // vertex patches: every vertex's faces in increasing face order, vertices in
+  // increasing key order
+  std::vector<std::pair<uint64_t, int>> cand;
+  {
+    size_t total = 0;
+    for (auto& v : cand_t) total += v.size();
+    cand.reserve(total);
+    for (auto& v : cand_t) {
+      cand.insert(cand.end(), v.begin(), v.end());
+      std::vector<std::pair<uint64_t, int>>().swap(v);
     }
   }
-  // irregular vertex patches
-  std::vector<std::vector<std::pair<int, double>>> off(F);
-  for (auto& kv : cand) {
-    std::vector<int>& faces = kv.second;
-    if (faces.size() <= 2) {
-      for (int f : faces) K.diag[f] += 1.0 / (2.0 * M.farea[f] * disp[f][M.faxis[f]]);
-      continue;
-    }
-    std::vector<PatchFace> pf;
-    for (int f : faces) pf.push_back({f, M.faxis[f], M.farea[f], disp[f][0], disp[f][1]});
-    // a patch of four same-level faces with square geometry is the MAC patch
-    bool mac = pf.size() == 4;
-    if (mac) {
-      int nax[2] = {0, 0};
-      for (const auto& q : pf) {
-        nax[q.axis]++;
-        const double tang = q.axis == 0 ? q.d1 : q.d0;
-        mac = mac && tang == 0.0 && M.block_level(M.fminus[q.face]) == M.block_level(M.fplus[q.face]) &&
-              M.block_level(M.fminus[q.face]) == M.block_level(M.fminus[pf[0].face]);
-      }
-      mac = mac && nax[0] == 2 && nax[1] == 2;
-    }
-    if (mac) {
-      for (const auto& q : pf) K.diag[q.face] += 1.0 / (2.0 * q.area * (q.axis == 0 ? q.d0 : q.d1));
-      continue;
-    }
-    std::sort(pf.begin(), pf.end(), [](const PatchFace& x, const PatchFace& y) { return x.face < y.face; });
-    const std::vector<double> L = patch_metric(pf);
-    const int n = int(pf.size());
-    for (int r = 0; r < n; ++r) {
-      K.diag[pf[r].face] += L[r * n + r];
-      for (int c = 0; c < n; ++c)
-        if (c != r) off[pf[r].face].push_back({pf[c].face, L[r * n + c]});
-    }
+  {
+    const int nbk = 64 * worker_count(long(cand.size()), 1 << 16);
+    const uint64_t span = uint64_t(lat[0]) + 1;
+    bucket_sort(cand, nbk, [&](const std::pair<uint64_t, int>& x) { return int((x.first >> 32) * nbk / span); },
+                [](const std::pair<uint64_t, int>& x, const std::pair<uint64_t, int>& y) { return x < y; });
   }
+  std::vector<size_t> vstart;
+  for (size_t q = 0; q < cand.size(); ++q)
+    if (q == 0 || cand[q].first != cand[q - 1].first) vstart.push_back(q);
+  vstart.push_back(cand.size());
+  const long nv = long(vstart.size()) - 1;
+  // each face meets two vertices, so its diagonal collects two terms from
+  // zero: the sum does not depend on the order in which patches are applied
+  struct Entry {
+    int f, g;
+    double v;
+  };
+  const int pt_n = worker_count(nv, 256);
+  std::vector<std::vector<std::pair<int, double>>> dg_t(pt_n);
+  std::vector<std::vector<Entry>> off_t(pt_n);
+  std::vector<std::string> err_t(pt_n);
+  run_chunks(nv, pt_n, [&](long v0, long v1, int w) {
+    try {
+      std::vector<PatchFace> pf;
+      for (long v = v0; v < v1; ++v) {
+        const size_t q0 = vstart[v], q1 = vstart[v + 1];
+        if (q1 - q0 <= 2) {
+          for (size_t q = q0; q < q1; ++q) {
+            const int f = cand[q].second;
+            dg_t[w].push_back({f, 1.0 / (2.0 * M.farea[f] * disp[f][M.faxis[f]])});
+          }
+          continue;
+        }
+        pf.clear();
+        for (size_t q = q0; q < q1; ++q) {
+          const int f = cand[q].second;
+          pf.push_back({f, M.faxis[f], M.farea[f], disp[f][0], disp[f][1]});
+        }
+        // a patch of four same-level faces with square geometry is the MAC patch
+        bool mac = pf.size() == 4;
+        if (mac) {
+          int nax[2] = {0, 0};
+          for (const auto& e : pf) {
+            nax[e.axis]++;
+            const double tang = e.axis == 0 ? e.d1 : e.d0;
+            mac = mac && tang == 0.0 && M.block_level(M.fminus[e.face]) == M.block_level(M.fplus[e.face]) &&
+                  M.block_level(M.fminus[e.face]) == M.block_level(M.fminus[pf[0].face]);
+          }
+          mac = mac && nax[0] == 2 && nax[1] == 2;
+        }
+        if (mac) {
+          for (const auto& e : pf) dg_t[w].push_back({e.face, 1.0 / (2.0 * e.area * (e.axis == 0 ? e.d0 : e.d1))});
+          continue;
+        }
+        const std::vector<double> L = patch_metric(pf);
+        const int n = int(pf.size());
+        for (int r = 0; r < n; ++r) {
+          dg_t[w].push_back({pf[r].face, L[r * n + r]});
+          for (int c = 0; c < n; ++c)
+            if (c != r) off_t[w].push_back({pf[r].face, pf[c].face, L[r * n + c]});
+        }
+      }
+    } catch (const std::exception& e) {
+      err_t[w] = e.what();
+    }
+  });
+  for (const auto& e : err_t)
+    if (!e.empty()) throw std::invalid_argument(e);
+  for (const auto& v : dg_t)
+    for (const auto& d : v) K.diag[d.first] += d.second;
+  std::vector<Entry> off;
+  {
+    size_t total = 0;
+    for (auto& v : off_t) total += v.size();
+    off.reserve(total);
+    for (auto& v : off_t) off.insert(off.end(), v.begin(), v.end());
+  }
+  {
+    const int nbk = 64 * worker_count(long(off.size()), 1 << 16);
+    bucket_sort(off, nbk, [&](const Entry& x) { return int(long(x.f) * nbk / F); },
+                [](const Entry& x, const Entry& y) { return std::tie(x.f, x.g, x.v) < std::tie(y.f, y.g, y.v); });
+  }
+  // merge duplicates (two faces meet in at most one patch, kept for safety)
   K.off_ptr.assign(F + 1, 0);
-  for (int f = 0; f < F; ++f) {
-    auto& v = off[f];
-    std::sort(v.begin(), v.end());
-    // merge duplicates (two faces meet in at most one patch, kept for safety)
-    std::vector<std::pair<int, double>> merged;
-    for (auto& e : v) {
-      if (!merged.empty() && merged.back().first == e.first) merged.back().second += e.second;
-      else merged.push_back(e);
+  for (size_t q = 0; q < off.size();) {
+    const int f = off[q].f, g = off[q].g;
+    double v = 0;
+    bool first = true;
+    for (; q < off.size() && off[q].f == f && off[q].g == g; ++q) {
+      v = first ? off[q].v : v + off[q].v;
+      first = false;
     }
-    for (auto& e : merged)
-      if (e.second != 0.0) {
-        K.off_face.push_back(e.first);
-        K.off_val.push_back(e.second);
-      }
-    K.off_ptr[f + 1] = int(K.off_face.size());
+    if (v != 0.0) {
+      K.off_face.push_back(g);
+      K.off_val.push_back(v);
+      K.off_ptr[f + 1]++;
+    }
   }
-  for (int f = 0; f < F; ++f)
-    if (!(K.diag[f] > 0)) throw std::invalid_argument("gradient Riesz map has a non-positive diagonal");
+  for (int f = 0; f < F; ++f) K.off_ptr[f + 1] += K.off_ptr[f];
+  std::atomic<bool> nonpositive{false};
+  parallel_for(F, [&](long f0, long f1, int) {
+    for (long f = f0; f < f1; ++f)
+      if (!(K.diag[f] > 0)) nonpositive = true;
+  });
+  if (nonpositive) throw std::invalid_argument("gradient Riesz map has a non-positive diagonal");
   return K;
 }
 
@@ -493,45 +736,52 @@ void level_topology(LevelData& L) {
   const int bx = M.bs[0], by = M.bs[1], nb = bx * by;
   L.nblocks = int(M.blocks.size());
   L.ncells = M.ncells();
+  PhaseTimer pt;
   L.K = face_metric(M);
+  pt.mark("face metric", long(M.nfaces()));
   const int F = M.nfaces();
-  L.face_regular.assign(F, 0);
-  L.slot_a.assign(F, -1);
-  L.slot_b.assign(F, -1);
-  for (int f = 0; f < F; ++f) {
-    const int m = M.fminus[f], p = M.fplus[f];
-    const bool reg = M.block_level(m) == M.block_level(p) && L.K.off_ptr[f] == L.K.off_ptr[f + 1];
-    L.face_regular[f] = reg;
-    if (!reg) continue;
-    const int bm = m / nb, lm = m % nb, bp = p / nb, lp = p % nb;
-    if (M.faxis[f] == 0) {
-      L.slot_a[f] = int64_t(bm) * (bx + 1) * by + (lm / bx) * (bx + 1) + (lm % bx) + 1;
-      L.slot_b[f] = int64_t(bp) * (bx + 1) * by + (lp / bx) * (bx + 1) + (lp % bx);
-    } else {
-      L.slot_a[f] = int64_t(bm) * bx * (by + 1) + (lm / bx + 1) * bx + (lm % bx);
-      L.slot_b[f] = int64_t(bp) * bx * (by + 1) + (lp / bx) * bx + (lp % bx);
+  L.face_regular.resize(F);
+  L.slot_a.resize(F);
+  L.slot_b.resize(F);
+  parallel_for(F, [&](long f0, long f1, int) {
+    for (long f = f0; f < f1; ++f) {
+      const int m = M.fminus[f], p = M.fplus[f];
+      const bool reg = M.block_level(m) == M.block_level(p) && L.K.off_ptr[f] == L.K.off_ptr[f + 1];
+      L.face_regular[f] = reg;
+      L.slot_a[f] = L.slot_b[f] = -1;
+      if (!reg) continue;
+      const int bm = m / nb, lm = m % nb, bp = p / nb, lp = p % nb;
+      if (M.faxis[f] == 0) {
+        L.slot_a[f] = int64_t(bm) * (bx + 1) * by + (lm / bx) * (bx + 1) + (lm % bx) + 1;
+        L.slot_b[f] = int64_t(bp) * (bx + 1) * by + (lp / bx) * (bx + 1) + (lp % bx);
+      } else {
+        L.slot_a[f] = int64_t(bm) * bx * (by + 1) + (lm / bx + 1) * bx + (lm % bx);
+        L.slot_b[f] = int64_t(bp) * bx * (by + 1) + (lp / bx) * bx + (lp % bx);
+      }
     }
-  }
+  });
   // neighbours
   L.nbr.assign(4 * L.nblocks, -1);
-  for (int b = 0; b < L.nblocks; ++b) {
-    const BlockKey& k = M.blocks[b];
-    const long px = long(M.layout[0]) << k.level, py = long(M.layout[1]) << k.level;
-    const int dx[4] = {-1, 1, 0, 0}, dy[4] = {0, 0, -1, 1};
-    for (int s = 0; s < 4; ++s) {
-      long x = k.i + dx[s], y = k.j + dy[s];
-      if (x < 0 || x >= px) {
-        if (!M.periodic[0]) continue;
-        x = wrapl(x, px);
+  parallel_for(L.nblocks, [&](long b0, long b1, int) {
+    for (long b = b0; b < b1; ++b) {
+      const BlockKey& k = M.blocks[b];
+      const long px = long(M.layout[0]) << k.level, py = long(M.layout[1]) << k.level;
+      const int dx[4] = {-1, 1, 0, 0}, dy[4] = {0, 0, -1, 1};
+      for (int s = 0; s < 4; ++s) {
+        long x = k.i + dx[s], y = k.j + dy[s];
+        if (x < 0 || x >= px) {
+          if (!M.periodic[0]) continue;
+          x = wrapl(x, px);
+        }
+        if (y < 0 || y >= py) {
+          if (!M.periodic[1]) continue;
+          y = wrapl(y, py);
+        }
+        auto it = M.block_of.find(key64(k.level, int(x), int(y)));
+        if (it != M.block_of.end()) L.nbr[4 * b + s] = it->second;
       }
-      if (y < 0 || y >= py) {
-        if (!M.periodic[1]) continue;
-        y = wrapl(y, py);
-      }
-      auto it = M.block_of.find(key64(k.level, int(x), int(y)));
-      if (it != M.block_of.end()) L.nbr[4 * b + s] = it->second;
     }
-  }
+  }, 256);
   L.volume.resize(L.nblocks);
   for (int b = 0; b < L.nblocks; ++b) L.volume[b] = M.bspacing[2 * b] * M.bspacing[2 * b + 1];
   // irregular cells: touching a non-regular face
@@ -549,23 +799,7 @@ void level_topology(LevelData& L) {
       }
     L.blk_irr[b + 1] = int(L.irr_local.size());
   }
-}
-
-// Run f(lo, hi, t) over [0, n) in contiguous chunks on up to 16 threads
-// (t = chunk index); small ranges run inline.
-template <typename F>
-void parallel_for(long n, F&& f) {
-  const long grain = 1 << 14;
-  int nt = int(std::min<long>(16, std::max<long>(1, std::thread::hardware_concurrency())));
-  nt = int(std::min<long>(nt, (n + grain - 1) / grain));
-  if (nt <= 1) {
-    f(0L, n, 0);
-    return;
-  }
-  std::vector<std::thread> pool;
-  pool.reserve(nt);
-  for (int t = 0; t < nt; ++t) pool.emplace_back([&, t] { f(n * t / nt, n * (t + 1) / nt, t); });
-  for (auto& th : pool) th.join();
+  pt.mark("level topology");
 }
 
 // Device form of the irregular rows (see LevelData): per block, the rows in
@@ -582,10 +816,10 @@ void build_ell(LevelData& L) {
   const int bx = M.bs[0], by = M.bs[1], nb = bx * by;
   const int hw = nb, he = nb + by, hs = nb + 2 * by, hn = nb + 2 * by + bx, ext0 = nb + 2 * (bx + by);
   const int nrows = int(L.irr_local.size());
-  int w = 0;
-  for (int r = 0; r < nrows; ++r) w = std::max(w, L.irr_ptr[r + 1] - L.irr_ptr[r]);
-  L.ell_w = w;
-  const int ng = (w + 3) / 4, w4 = 4 * ng;
+  int width = 0;
+  for (int r = 0; r < nrows; ++r) width = std::max(width, L.irr_ptr[r + 1] - L.irr_ptr[r]);
+  L.ell_w = width;
+  const int ng = (width + 3) / 4, w4 = 4 * ng;
   L.ell_ref.assign(size_t(nrows) * w4, 0);
   L.ell_val.assign(size_t(nrows) * w4, 0.0);
   L.ell_bound.assign(size_t(nrows) * w4, 0.0);
@@ -596,53 +830,72 @@ void build_ell(LevelData& L) {
   L.ext_max = 0;
   L.blk_ext.assign(L.nblocks + 1, 0);
   L.ext_col.clear();
-  std::unordered_map<int, int> ext;
-  std::vector<int> order;
-  for (int b = 0; b < L.nblocks; ++b) {
-    ext.clear();
-    const int* nbr = &L.nbr[4 * b];
-    const int r0 = L.blk_irr[b], nr = L.blk_irr[b + 1] - r0;
-    // rows of the block by decreasing length: the lanes reading group g of
-    // the block's region are then a prefix
-    order.resize(nr);
-    for (int x = 0; x < nr; ++x) order[x] = r0 + x;
-    std::stable_sort(order.begin(), order.end(), [&](int p, int q) {
-      return L.irr_ptr[p + 1] - L.irr_ptr[p] > L.irr_ptr[q + 1] - L.irr_ptr[q];
-    });
-    auto slot = [&](int x, int k) { return (size_t(r0) * ng + size_t(k / 4) * nr + x) * 4 + k % 4; };
-    for (int x = 0; x < nr; ++x) {
-      const int r = order[x];
-      const int loc = L.irr_local[r];
-      const int len = L.irr_ptr[r + 1] - L.irr_ptr[r];
-      L.ell_loc[r0 + x] = uint32_t(loc) | (uint32_t(len) << 16);
-      L.ell_row[r] = r0 + x;
-      for (int k = 0; k < w4; ++k) L.ell_ref[slot(x, k)] = uint16_t(loc);
-      int k = 0;
-      for (int e = L.irr_ptr[r]; e < L.irr_ptr[r + 1]; ++e, ++k) {
-        const int col = L.irr_col[e];
-        const int cb = col / nb, cl = col % nb, ci = cl % bx, cj = cl / bx;
-        int ref;
-        if (cb == b) ref = cl;
-        else if (cb == nbr[0] && ci == bx - 1) ref = hw + cj;
-        else if (cb == nbr[1] && ci == 0) ref = he + cj;
-        else if (cb == nbr[2] && cj == by - 1) ref = hs + ci;
-        else if (cb == nbr[3] && cj == 0) ref = hn + ci;
-        else {
-          auto it = ext.find(col);
-          if (it == ext.end()) {
-            it = ext.emplace(col, int(ext.size())).first;
-            L.ext_col.push_back(col);
+  // blocks in parallel chunks; each chunk's external cells are appended in
+  // block order below
+  const int nt = worker_count(L.nblocks, 256);
+  std::vector<std::vector<int>> ext_t(nt), next_t(nt);
+  std::vector<int> emax(nt, 0);
+  std::vector<uint8_t> over(nt, 0);
+  run_chunks(L.nblocks, nt, [&](long b0, long b1, int w) {
+    std::unordered_map<int, int> ext;
+    std::vector<int> order;
+    for (long b = b0; b < b1; ++b) {
+      ext.clear();
+      const int* nbr = &L.nbr[4 * b];
+      const int r0 = L.blk_irr[b], nr = L.blk_irr[b + 1] - r0;
+      // rows of the block by decreasing length: the lanes reading group g of
+      // the block's region are then a prefix
+      order.resize(nr);
+      for (int x = 0; x < nr; ++x) order[x] = r0 + x;
+      std::stable_sort(order.begin(), order.end(), [&](int p, int q) {
+        return L.irr_ptr[p + 1] - L.irr_ptr[p] > L.irr_ptr[q + 1] - L.irr_ptr[q];
+      });
+      auto slot = [&](int x, int k) { return (size_t(r0) * ng + size_t(k / 4) * nr + x) * 4 + k % 4; };
+      for (int x = 0; x < nr; ++x) {
+        const int r = order[x];
+        const int loc = L.irr_local[r];
+        const int len = L.irr_ptr[r + 1] - L.irr_ptr[r];
+        L.ell_loc[r0 + x] = uint32_t(loc) | (uint32_t(len) << 16);
+        L.ell_row[r] = r0 + x;
+        for (int k = 0; k < w4; ++k) L.ell_ref[slot(x, k)] = uint16_t(loc);
+        int k = 0;
+        for (int e = L.irr_ptr[r]; e < L.irr_ptr[r + 1]; ++e, ++k) {
+          const int col = L.irr_col[e];
+          const int cb = col / nb, cl = col % nb, ci = cl % bx, cj = cl / bx;
+          int ref;
+          if (cb == b) ref = cl;
+          else if (cb == nbr[0] && ci == bx - 1) ref = hw + cj;
+          else if (cb == nbr[1] && ci == 0) ref = he + cj;
+          else if (cb == nbr[2] && cj == by - 1) ref = hs + ci;
+          else if (cb == nbr[3] && cj == 0) ref = hn + ci;
+          else {
+            auto it = ext.find(col);
+            if (it == ext.end()) {
+              it = ext.emplace(col, int(ext.size())).first;
+              ext_t[w].push_back(col);
+            }
+            ref = ext0 + it->second;
           }
-          ref = ext0 + it->second;
+          const size_t at = slot(x, k);
+          L.ell_ref[at] = uint16_t(ref);
+          L.ell_pos[e] = int(at);
         }
-        const size_t at = slot(x, k);
-        L.ell_ref[at] = uint16_t(ref);
-        L.ell_pos[e] = int(at);
       }
+      if (int(ext.size()) > nb) over[w] = 1;
+      emax[w] = std::max(emax[w], int(ext.size()));
+      next_t[w].push_back(int(ext.size()));
     }
-    if (int(ext.size()) > nb) throw std::runtime_error("too many external couplings in one block");
-    L.ext_max = std::max(L.ext_max, int(ext.size()));
-    L.blk_ext[b + 1] = int(L.ext_col.size());
+  });
+  for (uint8_t o : over)
+    if (o) throw std::runtime_error("too many external couplings in one block");
+  long b = 0;
+  for (int w = 0; w < nt; ++w) {
+    for (int n : next_t[w]) {
+      L.blk_ext[b + 1] = L.blk_ext[b] + n;
+      ++b;
+    }
+    L.ext_col.insert(L.ext_col.end(), ext_t[w].begin(), ext_t[w].end());
+    L.ext_max = std::max(L.ext_max, emax[w]);
   }
 }
 
@@ -654,35 +907,54 @@ void level_pattern(LevelData& L) {
   const int nb = M.nb();
   const int F = M.nfaces();
   const int nrows = int(L.irr_local.size());
-  std::vector<std::vector<int>> inc(nrows);  // faces per irregular row
-  for (int f = 0; f < F; ++f) {
-    const int m = M.fminus[f], p = M.fplus[f];
-    if (L.cell_irr[m] >= 0) inc[L.cell_irr[m]].push_back(f);
-    if (p != m && L.cell_irr[p] >= 0) inc[L.cell_irr[p]].push_back(f);
+  // faces per irregular row, in increasing face order (CSR)
+  std::vector<int> inc_ptr(nrows + 1, 0), inc;
+  {
+    const int nt = worker_count(F);
+    std::vector<std::vector<std::pair<int, int>>> pr(nt);
+    run_chunks(F, nt, [&](long f0, long f1, int w) {
+      for (long f = f0; f < f1; ++f) {
+        const int m = M.fminus[f], p = M.fplus[f];
+        if (L.cell_irr[m] >= 0) pr[w].push_back({L.cell_irr[m], int(f)});
+        if (p != m && L.cell_irr[p] >= 0) pr[w].push_back({L.cell_irr[p], int(f)});
+      }
+    });
+    for (const auto& v : pr)
+      for (const auto& e : v) inc_ptr[e.first + 1]++;
+    for (int r = 0; r < nrows; ++r) inc_ptr[r + 1] += inc_ptr[r];
+    inc.resize(inc_ptr[nrows]);
+    std::vector<int> at(inc_ptr.begin(), inc_ptr.end() - 1);
+    for (const auto& v : pr)
+      for (const auto& e : v) inc[at[e.first]++] = e.second;
   }
-  L.irr_ptr.assign(nrows + 1, 0);
-  L.irr_col.clear();
-  L.irr_diag_e.assign(nrows, -1);
-  L.term_ptr.assign(1, 0);
-  L.term_f.clear();
-  L.term_g.clear();
-  L.term_k.clear();
-  L.term_s.clear();
-  L.term_diag.clear();
-  // (column, (term index, 0)): sorted by column only, as the values were
-  std::vector<std::pair<int, std::pair<double, double>>> row;
+  std::vector<int> row_blk(nrows);
+  for (int b = 0; b < L.nblocks; ++b)
+    for (int r = L.blk_irr[b]; r < L.blk_irr[b + 1]; ++r) row_blk[r] = b;
   struct Term {
     int f, g;
     double k, s;
     bool diag;
   };
-  std::vector<Term> terms;
-  for (int b = 0; b < L.nblocks; ++b)
-    for (int r = L.blk_irr[b]; r < L.blk_irr[b + 1]; ++r) {
-      const int c = b * nb + L.irr_local[r];
+  // rows in parallel chunks; each chunk's columns and terms are appended in
+  // row order below
+  struct Part {
+    std::vector<int> col, ncol, nterm, diag_e, f, g;
+    std::vector<double> k, s;
+    std::vector<uint8_t> dg;
+  };
+  const int nt = worker_count(nrows, 2048);
+  std::vector<Part> part(nt);
+  run_chunks(nrows, nt, [&](long r0, long r1, int w) {
+    Part& P = part[w];
+    // (column, (term index, 0)): sorted by column only, as the values were
+    std::vector<std::pair<int, std::pair<double, double>>> row;
+    std::vector<Term> terms;
+    for (long r = r0; r < r1; ++r) {
+      const int c = row_blk[r] * nb + L.irr_local[r];
       row.clear();
       terms.clear();
-      for (int f : inc[r]) {
+      for (int q = inc_ptr[r]; q < inc_ptr[r + 1]; ++q) {
+        const int f = inc[q];
         const double sf = (M.fminus[f] == c) ? -1.0 : 1.0;
         auto add = [&](int g, double kfg, bool diag) {
           terms.push_back({f, g, kfg, sf * -1.0, diag});
@@ -691,27 +963,73 @@ void level_pattern(LevelData& L) {
           row.push_back({M.fplus[g], {double(terms.size() - 1), 0.0}});
         };
         add(f, L.K.diag[f], true);
-        for (int q = L.K.off_ptr[f]; q < L.K.off_ptr[f + 1]; ++q) add(L.K.off_face[q], L.K.off_val[q], false);
+        for (int o = L.K.off_ptr[f]; o < L.K.off_ptr[f + 1]; ++o) add(L.K.off_face[o], L.K.off_val[o], false);
       }
       std::sort(row.begin(), row.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+      int de = -1, nc = 0;
       size_t q = 0;
       while (q < row.size()) {
         const int col = row[q].first;
+        int n = 0;
         while (q < row.size() && row[q].first == col) {
           const Term& t = terms[size_t(row[q].second.first)];
-          L.term_f.push_back(t.f);
-          L.term_g.push_back(t.g);
-          L.term_k.push_back(t.k);
-          L.term_s.push_back(t.s);
-          L.term_diag.push_back(t.diag);
+          P.f.push_back(t.f);
+          P.g.push_back(t.g);
+          P.k.push_back(t.k);
+          P.s.push_back(t.s);
+          P.dg.push_back(t.diag);
           ++q;
+          ++n;
         }
-        if (col == c) L.irr_diag_e[r] = int(L.irr_col.size());
-        L.irr_col.push_back(col);
-        L.term_ptr.push_back(int(L.term_f.size()));
+        if (col == c) de = int(P.col.size());
+        P.col.push_back(col);
+        P.nterm.push_back(n);
+        ++nc;
       }
-      L.irr_ptr[r + 1] = int(L.irr_col.size());
+      P.ncol.push_back(nc);
+      P.diag_e.push_back(de);
     }
+  });
+  // offsets of every chunk's rows, columns and terms, then a parallel copy
+  std::vector<long> row_off(nt + 1, 0), col_off(nt + 1, 0), term_off(nt + 1, 0);
+  for (int w = 0; w < nt; ++w) {
+    row_off[w + 1] = row_off[w] + long(part[w].ncol.size());
+    col_off[w + 1] = col_off[w] + long(part[w].col.size());
+    term_off[w + 1] = term_off[w] + long(part[w].f.size());
+  }
+  const long ncols = col_off[nt], nterms = term_off[nt];
+  L.irr_ptr.assign(nrows + 1, 0);
+  L.irr_col.resize(ncols);
+  L.irr_diag_e.assign(nrows, -1);
+  L.term_ptr.resize(ncols + 1);
+  L.term_ptr[0] = 0;
+  L.term_f.resize(nterms);
+  L.term_g.resize(nterms);
+  L.term_k.resize(nterms);
+  L.term_s.resize(nterms);
+  L.term_diag.resize(nterms);
+  parallel_tasks(nt, [&](long w) {
+    Part& P = part[w];
+    long cum = col_off[w];
+    for (size_t x = 0; x < P.ncol.size(); ++x) {
+      const long r = row_off[w] + long(x);
+      cum += P.ncol[x];
+      L.irr_ptr[r + 1] = int(cum);
+      L.irr_diag_e[r] = P.diag_e[x] >= 0 ? int(col_off[w] + P.diag_e[x]) : -1;
+    }
+    long tcum = term_off[w];
+    for (size_t e = 0; e < P.nterm.size(); ++e) {
+      tcum += P.nterm[e];
+      L.term_ptr[col_off[w] + long(e) + 1] = int(tcum);
+    }
+    std::copy(P.col.begin(), P.col.end(), L.irr_col.begin() + col_off[w]);
+    std::copy(P.f.begin(), P.f.end(), L.term_f.begin() + term_off[w]);
+    std::copy(P.g.begin(), P.g.end(), L.term_g.begin() + term_off[w]);
+    std::copy(P.k.begin(), P.k.end(), L.term_k.begin() + term_off[w]);
+    std::copy(P.s.begin(), P.s.end(), L.term_s.begin() + term_off[w]);
+    std::copy(P.dg.begin(), P.dg.end(), L.term_diag.begin() + term_off[w]);
+    P = Part();
+  });
   L.irr_val.assign(L.irr_col.size(), 0.0);
   L.irr_bound.assign(L.irr_col.size(), 0.0);
   L.irr_diag.assign(nrows, 0.0);
@@ -724,7 +1042,11 @@ void level_values(LevelData& L) {
   const TreeMesh& M = *L.mesh;
   const int bx = M.bs[0], by = M.bs[1], nb = bx * by;
   const int F = M.nfaces();
-  if (!L.planned) level_pattern(L);
+  if (!L.planned) {
+    PhaseTimer pt;
+    level_pattern(L);
+    pt.mark("irregular-row pattern", long(L.irr_col.size()));
+  }
   L.cx.resize(size_t(L.nblocks) * (bx + 1) * by);
   L.cy.resize(size_t(L.nblocks) * bx * (by + 1));
   parallel_for(long(L.cx.size()), [&](long lo, long hi, int) { std::fill(&L.cx[lo], &L.cx[0] + hi, 0.0); });
@@ -846,34 +1168,65 @@ void restriction_map(LevelData& C, const LevelData& Fl) {
     hi = std::min(ml[t] + ms[t], pl[t] + ps[t]);
   };
   const int FF = fm.nfaces(), CF = cm.nfaces();
-  std::vector<std::array<long, 4>> fg(FF);
-  for (int f = 0; f < FF; ++f) geom(fm, f, fg[f][0], fg[f][1], fg[f][2], fg[f][3]);
-  C.rptr.assign(CF + 1, 0);
-  C.ridx.clear();
-  C.rw.clear();
-  int q = 0;
-  for (int f = 0; f < CF; ++f) {
-    long a, n, lo, hi;
-    geom(cm, f, a, n, lo, hi);
-    // advance the fine pointer to the first segment on this line ending after lo
-    auto before = [&](int g) {
-      return std::make_tuple(fg[g][0], fg[g][1], fg[g][3]) <= std::make_tuple(a, n, lo);
-    };
-    while (q < FF && before(q)) ++q;
-    long covered = 0;
-    int r = q;
-    while (r < FF && fg[r][0] == a && fg[r][1] == n && fg[r][2] < hi) {
-      const long ov = std::min(hi, fg[r][3]) - std::max(lo, fg[r][2]);
-      if (ov > 0) {
-        C.ridx.push_back(r);
-        C.rw.push_back(double(ov) / double(hi - lo));
-        covered += ov;
+  hvec<std::array<long, 4>> fg(FF);
+  parallel_for(FF, [&](long f0, long f1, int) {
+    for (long f = f0; f < f1; ++f) geom(fm, int(f), fg[f][0], fg[f][1], fg[f][2], fg[f][3]);
+  });
+  // Fine faces run in (axis, line, lo) order and cover each line with
+  // disjoint segments, so (axis, line, hi) increases too: each chunk of
+  // coarse faces finds its first fine segment by binary search and then
+  // walks forward.
+  const int nt = worker_count(CF, 4096);
+  std::vector<std::vector<int>> idx_t(nt), cnt_t(nt);
+  std::vector<std::vector<double>> w_t(nt);
+  std::vector<uint8_t> bad(nt, 0);
+  run_chunks(CF, nt, [&](long c0, long c1, int w) {
+    auto& idx = idx_t[w];
+    auto& wt = w_t[w];
+    auto& cnt = cnt_t[w];
+    long q = -1;
+    for (long f = c0; f < c1; ++f) {
+      long a, n, lo, hi;
+      geom(cm, int(f), a, n, lo, hi);
+      const auto key = std::make_tuple(a, n, lo);
+      auto before = [&](const std::array<long, 4>& g) { return std::make_tuple(g[0], g[1], g[3]) <= key; };
+      if (q < 0) q = std::partition_point(fg.begin(), fg.end(), before) - fg.begin();
+      while (q < FF && before(fg[q])) ++q;
+      long covered = 0;
+      int k = 0;
+      for (long r = q; r < FF && fg[r][0] == a && fg[r][1] == n && fg[r][2] < hi; ++r) {
+        const long ov = std::min(hi, fg[r][3]) - std::max(lo, fg[r][2]);
+        if (ov > 0) {
+          idx.push_back(int(r));
+          wt.push_back(double(ov) / double(hi - lo));
+          covered += ov;
+          ++k;
+        }
       }
-      ++r;
+      if (covered != hi - lo) bad[w] = 1;
+      cnt.push_back(k);
     }
-    if (covered != hi - lo) throw std::runtime_error("tree hierarchy could not cover a coarse face with fine faces");
-    C.rptr[f + 1] = int(C.ridx.size());
+  });
+  for (uint8_t b : bad)
+    if (b) throw std::runtime_error("tree hierarchy could not cover a coarse face with fine faces");
+  // every chunk's coarse faces and entries at their offsets
+  std::vector<long> face_off(nt + 1, 0), ent_off(nt + 1, 0);
+  for (int w = 0; w < nt; ++w) {
+    face_off[w + 1] = face_off[w] + long(cnt_t[w].size());
+    ent_off[w + 1] = ent_off[w] + long(idx_t[w].size());
   }
+  C.rptr.assign(CF + 1, 0);
+  C.ridx.resize(ent_off[nt]);
+  C.rw.resize(ent_off[nt]);
+  parallel_tasks(nt, [&](long w) {
+    long cum = ent_off[w];
+    for (size_t x = 0; x < cnt_t[w].size(); ++x) {
+      cum += cnt_t[w][x];
+      C.rptr[face_off[w] + long(x) + 1] = int(cum);
+    }
+    std::copy(idx_t[w].begin(), idx_t[w].end(), C.ridx.begin() + ent_off[w]);
+    std::copy(w_t[w].begin(), w_t[w].end(), C.rw.begin() + ent_off[w]);
+  });
 }
 
 void transfer_descriptors(LevelData& Fl, const LevelData& C, bool agglomerated) {
@@ -969,15 +1322,20 @@ std::unique_ptr<Hierarchy> build_hierarchy(std::unique_ptr<TreeMesh> fine, const
     const TreeMesh& cur = *H->levels.back()->mesh;
     if (!(cur.ncells() > cfg.coarse_size) || int(H->levels.size()) >= cfg.max_levels) break;
     const bool is_agg = active == 0;
+    PhaseTimer pt;
     std::unique_ptr<TreeMesh> cm = is_agg ? agglomerated_mesh(cur) : pruned_mesh(cur);
+    pt.mark("coarse mesh", cm ? long(cm->ncells()) : 0L);
     if (!cm || cm->ncells() >= cur.ncells()) break;
     active = cm->max_leaf;
     auto L = std::make_unique<LevelData>();
     L->owned = std::move(cm);
     L->mesh = L->owned.get();
     level_topology(*L);
+    pt.t = std::chrono::steady_clock::now();
     restriction_map(*L, *H->levels.back());
+    pt.mark("restriction map");
     transfer_descriptors(*H->levels.back(), *L, is_agg);
+    pt.mark("transfer descriptors");
     H->levels.push_back(std::move(L));
   }
   const TreeMesh& M0 = *H->levels[0]->mesh;

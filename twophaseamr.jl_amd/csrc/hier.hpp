@@ -9,12 +9,38 @@
 #pragma once
 #include <cstdint>
 #include <memory>
+#include <utility>
 #include <string>
+#include <type_traits>
 #include <unordered_map>
 #include <unordered_set>
 #include <vector>
 
 namespace tpamr {
+
+// Allocator whose value-less construct leaves elements default-initialised:
+// resizing a large array does not zero it on one thread, and the threads that
+// fill it touch its pages first.
+template <typename T>
+struct NoInit : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = NoInit<U>;
+  };
+  NoInit() = default;
+  template <typename U>
+  NoInit(const NoInit<U>&) noexcept {}
+  template <typename U, typename... A>
+  void construct(U* p, A&&... a) {
+    ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+  }
+  template <typename U>
+  void construct(U* p) noexcept(std::is_nothrow_default_constructible<U>::value) {
+    ::new (static_cast<void*>(p)) U;
+  }
+};
+template <typename T>
+using hvec = std::vector<T, NoInit<T>>;
 
 struct BlockKey {
   int level, i, j;
@@ -37,13 +63,13 @@ struct TreeMesh {
   std::vector<double> bspacing;  // 2 per block
   std::vector<double> borigin;   // 2 per block: lower-left corner
   // interior faces, reference order
-  std::vector<int> fminus, fplus;
-  std::vector<int8_t> faxis;
-  std::vector<double> farea, fcx, fcy;
+  hvec<int> fminus, fplus;
+  hvec<int8_t> faxis;
+  hvec<double> farea, fcx, fcy;
   // boundary faces, reference order
-  std::vector<int> bcell;
-  std::vector<int8_t> baxis, bside;
-  std::vector<double> barea;
+  hvec<int> bcell;
+  hvec<int8_t> baxis, bside;
+  hvec<double> barea;
   int nb() const { return bs[0] * bs[1]; }
   int ncells() const { return int(blocks.size()) * nb(); }
   int nfaces() const { return int(fminus.size()); }
@@ -56,7 +82,7 @@ std::unique_ptr<TreeMesh> make_tree_mesh(const int base[2], const int bs[2], con
 
 // Inverse face metric K: diagonal plus symmetric off-diagonal pairs.
 struct FaceMetric {
-  std::vector<double> diag;               // per face
+  hvec<double> diag;                      // per face
   std::vector<int> off_ptr, off_face;     // CSR of off-diagonal entries by face
   std::vector<double> off_val;
   bool diagonal() const { return off_face.empty(); }
@@ -71,9 +97,9 @@ struct LevelData {
   FaceMetric K;
   int nblocks = 0, ncells = 0;
   std::vector<double> kappa;  // per face
-  std::vector<uint8_t> face_regular;
+  hvec<uint8_t> face_regular;
   // face -> conductance slots: x faces in cx[(bx+1)*by] per block, y faces in cy[bx*(by+1)]
-  std::vector<int64_t> slot_a, slot_b;  // flat indices (or -1)
+  hvec<int64_t> slot_a, slot_b;  // flat indices (or -1)
   std::vector<double> cx, cy;           // conductances (0 where unused)
   // aligned device layout of the same conductances: west face of every cell
   // (nb per block), east face of the last column (by per block), south face
@@ -108,16 +134,17 @@ struct LevelData {
   // (faces f, g, K entry, sign, diagonal flag) and its ELL value position;
   // per row its diagonal entry and its position in ell_loc order
   bool planned = false;
-  std::vector<int> term_ptr, term_f, term_g;
-  std::vector<double> term_k, term_s;
-  std::vector<uint8_t> term_diag;
+  hvec<int> term_ptr, term_f, term_g;
+  hvec<double> term_k, term_s;
+  hvec<uint8_t> term_diag;
   std::vector<int> ell_pos, ell_row, irr_diag_e;
   double omega = 1, omega_p = 0;
   // transfer to the next coarser level (absent on the coarsest)
   std::vector<int> xfer;  // 2 per block: coarse block, mode (-1 identity, 0..3 quadrant)
   // face-coefficient restriction from the previous (finer) level
-  std::vector<int> rptr, ridx;
-  std::vector<double> rw;
+  std::vector<int> rptr;
+  hvec<int> ridx;
+  hvec<double> rw;
   // incidence of irregular cells: face lists
 };
 

@@ -71,8 +71,8 @@ struct DevArray {
     n = count;
     if (count) HIPCHECK(hipMalloc(&p, count * sizeof(T)));
   }
-  template <typename S>
-  void upload(const std::vector<S>& v, hipStream_t s) {
+  template <typename S, typename A>
+  void upload(const std::vector<S, A>& v, hipStream_t s) {
     alloc(v.size());
     if (v.empty()) return;
     if constexpr (std::is_same<S, T>::value) {
@@ -374,7 +374,7 @@ void upload_hierarchy(tpamr_ctx* c, bool topology) {
     std::vector<signed char> bsd(M.bside.begin(), M.bside.end());
     if (bsd.empty()) bsd.push_back(0);
     c->bside.upload(bsd, s);
-    std::vector<double> ba(M.barea);
+    std::vector<double> ba(M.barea.begin(), M.barea.end());
     if (ba.empty()) ba.push_back(0);
     c->barea.upload(ba, s);
     // vectors and reduction scratch
@@ -995,8 +995,13 @@ void upload_plan(tpamr_ctx* c) {
     P.nrows = int(L.irr_local.size());
     if (l > 0) {
       P.rptr.upload(L.rptr, s);
-      P.ridx.upload(L.ridx.empty() ? std::vector<int>(1, 0) : L.ridx, s);
-      P.rw.upload(L.rw.empty() ? std::vector<double>(1, 0.0) : L.rw, s);
+      if (L.ridx.empty()) {
+        P.ridx.upload(std::vector<int>(1, 0), s);
+        P.rw.upload(std::vector<double>(1, 0.0), s);
+      } else {
+        P.ridx.upload(L.ridx, s);
+        P.rw.upload(L.rw, s);
+      }
     }
     P.kdiag.upload(L.K.diag, s);
     P.area.upload(M.farea, s);
@@ -1018,17 +1023,21 @@ void upload_plan(tpamr_ctx* c) {
         dst[2 * size_t(f) + 1] = code(M.faxis[f], L.slot_b[f]);
       }
     P.dst.upload(dst, s);
-    auto nz = [](const auto& v, auto z) { return v.empty() ? std::vector<decltype(z)>(1, z) : v; };
+    // empty arrays upload one zero (kernels take valid pointers)
+    auto up = [&](auto& dst, const auto& v, auto z) {
+      if (v.empty()) dst.upload(std::vector<decltype(z)>(1, z), s);
+      else dst.upload(v, s);
+    };
     P.irr_ptr.upload(L.irr_ptr, s);
     P.term_ptr.upload(L.term_ptr, s);
-    P.term_f.upload(nz(L.term_f, 0), s);
-    P.term_g.upload(nz(L.term_g, 0), s);
-    P.term_k.upload(nz(L.term_k, 0.0), s);
-    P.term_s.upload(nz(L.term_s, 0.0), s);
-    P.term_diag.upload(nz(L.term_diag, (unsigned char)0), s);
-    P.ell_pos.upload(nz(L.ell_pos, 0), s);
-    P.ell_row.upload(nz(L.ell_row, 0), s);
-    P.diag_e.upload(nz(L.irr_diag_e, 0), s);
+    up(P.term_f, L.term_f, 0);
+    up(P.term_g, L.term_g, 0);
+    up(P.term_k, L.term_k, 0.0);
+    up(P.term_s, L.term_s, 0.0);
+    up(P.term_diag, L.term_diag, (unsigned char)0);
+    up(P.ell_pos, L.ell_pos, 0);
+    up(P.ell_row, L.ell_row, 0);
+    up(P.diag_e, L.irr_diag_e, 0);
     P.cell_irr.upload(L.cell_irr, s);
     P.irr_diag.alloc(std::max(P.nrows, 1));
     if (l >= int(c->ld.size())) {
@@ -1036,8 +1045,8 @@ void upload_plan(tpamr_ctx* c) {
       P.ce.upload(L.ce, s);
       P.cs.upload(L.cs, s);
       P.cn.upload(L.cn, s);
-      P.ell_val.upload(nz(L.ell_val, 0.0), s);
-      P.ell_diag.upload(nz(L.ell_diag, 0.0), s);
+      up(P.ell_val, L.ell_val, 0.0);
+      up(P.ell_diag, L.ell_diag, 0.0);
       P.dinv.upload(L.dinv, s);
     }
   }
