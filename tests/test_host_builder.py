@@ -2,6 +2,10 @@
 context): identical block and face schedules, multigrid level sizes, Jacobi
 weights, level operators, tentative/smoothed transfers and restricted face
 coefficients."""
+import os
+import subprocess
+import sys
+
 import numpy as np
 import pytest
 
@@ -50,3 +54,22 @@ def test_invalid_inputs_raise():
         tpamr.MultigridOptions(coarse_size=0)
     with pytest.raises(ValueError):
         tpamr.LinearSolverOptions(krylov="cg")
+
+
+def test_threaded_builder_does_not_depend_on_thread_count(tmp_path):
+    """The setup passes split their work by the host thread count; the faces,
+    level operators, weights, restricted coefficients and transfers are the
+    same, bit for bit, on 1 and on 16 threads (a mesh large enough that every
+    pass runs threaded)."""
+    here = os.path.dirname(__file__)
+    outs = []
+    for threads in (1, 16):
+        path = tmp_path / f"h{threads}.npz"
+        env = dict(os.environ, TPAMR_HOST_THREADS=str(threads))
+        subprocess.run([sys.executable, os.path.join(here, "_host_dump.py"), str(path)], env=env, check=True,
+                       cwd=here, timeout=600)
+        outs.append(np.load(path))
+    a, b = outs
+    assert sorted(a.files) == sorted(b.files)
+    for k in a.files:
+        assert np.array_equal(a[k], b[k]), k

@@ -48,10 +48,23 @@ struct PhaseTimer {
 
 inline long wrapl(long x, long p) { return ((x % p) + p) % p; }
 
-// Workers for n items: up to 16 threads, at least `grain` items each.
+// Host threads of the setup passes: the hardware threads, at most 16, or
+// TPAMR_HOST_THREADS (1 to 64). Every pass splits its work by this count
+// alone, and its results do not depend on it.
+inline long host_threads() {
+  static const long n = [] {
+    if (const char* q = std::getenv("TPAMR_HOST_THREADS")) {
+      const long v = std::atol(q);
+      if (v >= 1) return std::min<long>(v, 64);
+    }
+    return std::max<long>(1, std::min<long>(16, std::thread::hardware_concurrency()));
+  }();
+  return n;
+}
+
+// Workers for n items: up to host_threads(), at least `grain` items each.
 inline int worker_count(long n, long grain = 1 << 14) {
-  const long hw = std::max<long>(1, std::thread::hardware_concurrency());
-  return int(std::max<long>(1, std::min<long>({16L, hw, (n + grain - 1) / grain})));
+  return int(std::max<long>(1, std::min<long>(host_threads(), (n + grain - 1) / grain)));
 }
 
 // Run f(lo, hi, t) for t < nt over the contiguous chunks [n t / nt, n (t + 1)
@@ -74,7 +87,8 @@ void parallel_for(long n, F&& f, long grain = 1 << 14) {
   run_chunks(n, worker_count(n, grain), f);
 }
 
-// Run f(k) for every task k < n on up to 16 threads, taking tasks in turn.
+// Run f(k) for every task k < n on up to host_threads() threads, taking tasks
+// in turn.
 template <typename F>
 void parallel_tasks(long n, F&& f) {
   const int nt = worker_count(n, 1);
@@ -1063,7 +1077,7 @@ void level_values(LevelData& L) {
   });
   // irregular rows: full rows of A (and of the floor bound |J|'|M||J|)
   const int nrows = int(L.irr_local.size());
-  std::vector<double> ratio(16, 0.0);
+  std::vector<double> ratio(host_threads(), 0.0);
   parallel_for(nrows, [&](long lo, long hi, int t) {
     double mr = 0;
     for (long r = lo; r < hi; ++r) {
@@ -1103,7 +1117,7 @@ void level_values(LevelData& L) {
   L.ce.resize(nbk * by);
   L.cn.resize(nbk * bx);
   L.dinv.resize(L.ncells);
-  std::vector<uint8_t> regular(16, 0);
+  std::vector<uint8_t> regular(host_threads(), 0);
   parallel_for(long(nbk), [&](long lo, long hi, int t) {
     for (long b = lo; b < hi; ++b) {
       const double* cx = &L.cx[b * (bx + 1) * by];
