@@ -323,8 +323,10 @@ __global__ void __launch_bounds__(256) k_cg_beta(int nb, int nmax, const double*
   // inner vectors carry the factor `scale`)
   const bool solved = rn / st->scale + 1.0 <= 1.0;
   // A replaced residual (FP64 b - A x, while the inner iteration's operator
-  // is FP32) restarts the recurrence: p = z, as at a fresh start.
-  const bool restart = st->replace != 0;
+  // is FP32) restarts the recurrence (p = z, as at a fresh start), or with
+  // `keep` continues it (a reliable update: beta from the replaced residual).
+  const bool replaced_now = st->replace != 0;
+  const bool restart = replaced_now && !st->keep;
   if (!solved) {
     const double beta = restart ? 0.0 : g / st->gamma;
     st->pnorm2 = g + beta * beta * st->pnorm2;
@@ -332,7 +334,7 @@ __global__ void __launch_bounds__(256) k_cg_beta(int nb, int nmax, const double*
     st->beta = beta;
   }
   st->iter += 1;
-  if (restart) {
+  if (replaced_now) {
     st->rlast = mx;
     st->replace = 0;
     st->replaced += 1;

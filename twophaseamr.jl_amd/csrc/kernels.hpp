@@ -47,6 +47,7 @@ struct CgState {
   double gamma, pnorm2, alpha, beta, pAp, zmean, rnorm, strong, target, rmean;
   double scale, rlast, delta;
   int iter, itmax, done, reason, ticket, replace, replaced;
+  int keep;  // refinement mode: a residual replacement keeps the recurrence (p, beta)
 };
 
 // Workgroups of the Krylov scalar-step reductions (stage buffer: 4 * RED_WG).

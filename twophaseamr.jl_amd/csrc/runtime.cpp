@@ -178,6 +178,7 @@ struct tpamr_ctx {
   // fallen by ir_delta since the last replacement (TPAMR_IR_DELTA in the
   // environment)
   double ir_delta = 5e-3;
+  bool ir_keep = false;  // TPAMR_IR_RESTART=0: replacements keep the recurrence
   DevArray<float> p32, Ap32, e32;  // inner Krylov vectors (TPAMR_FP32_REFINE)
   bool fp32() const { return precision != TPAMR_FP64; }
   // Krylov iterations in flight before the host reads a done flag
@@ -654,6 +655,7 @@ int reset_state(tpamr_ctx* c, double target, int itmax, double scale = 1.0, doub
   init.itmax = itmax == 0 ? 2 * N : itmax;
   init.scale = scale;
   init.delta = delta;
+  init.keep = c->ir_keep ? 1 : 0;
   init.rlast = 1.0;
   *c->st_host = init;
   HIPCHECK(hipMemcpyAsync(c->st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
@@ -1183,6 +1185,7 @@ int tpamr_create(tpamr_ctx** out, int device, int precision) {
   if (const char* q = std::getenv("TPAMR_COARSE_GJ")) c->coarse_exact = std::strcmp(q, "0") != 0;
   if (const char* q = std::getenv("TPAMR_GRAPH_ITERS")) c->graph_iters = std::min(16, std::max(1, std::atoi(q)));
   if (const char* q = std::getenv("TPAMR_TAIL_BLOCKS")) c->tail_blocks = std::max(0, std::atoi(q));
+  if (const char* q = std::getenv("TPAMR_IR_RESTART")) c->ir_keep = std::strcmp(q, "0") == 0;
   if (const char* q = std::getenv("TPAMR_IR_DELTA")) {
     const double v = std::atof(q);
     if (v > 0 && v < 1) c->ir_delta = v;
