@@ -279,7 +279,7 @@ def wave_main(args):
     nsteps, dt = args.steps, 1e-4
     u0 = 0.01 * W.smooth_velocity(mesh, device="cuda:0")
     d_u0 = torch.as_tensor(u0, device=dev)
-    steps = d_steps = frac_end = None
+    d_ifaces = frac_end = None
     res, runs = {}, {}
     grad = None
     for prec in (HEADLINE, "fp64"):
@@ -287,17 +287,25 @@ def wave_main(args):
         s = tpamr.EllipticSolver(tpamr.PoissonOperator(mesh, 1.0), precision=prec)
         setup_s[prec] = time.perf_counter() - t1
         log(f"{prec}: hierarchy {[lv['ncells'] for lv in s.levels]} ({setup_s[prec]:.1f} s)")
-        if steps is None:
+        if d_ifaces is None:
+            # the interface of every step (fraction, apertures, centroid
+            # offsets) in HBM: each timed step computes its coefficients and
+            # gravity jumps on the GPU
             t1 = time.perf_counter()
-            steps, frac_end = W.wave_steps(s, nsteps, dt)
-            d_steps = [(torch.as_tensor(k, device=dev), torch.as_tensor(j, device=dev)) for k, j in steps]
+            d_ifaces = []
+            for n, data in enumerate(W.wave_interfaces(mesh, nsteps, dt)):
+                if n == nsteps:
+                    frac_end = data[0]
+                    break
+                d_ifaces.append(tuple(torch.as_tensor(a, device=dev) for a in data))
             log(f"interface data of {nsteps} steps ({time.perf_counter() - t1:.1f} s)")
         w = torch.as_tensor(tpamr.stepping.kinetic_energy_weights(mesh, frac_end, W.WAVE_DENSITY), device=dev)
         run = tpamr.DeviceProjectionRun(s, dt, rtol=args.rtol, adjoint_rtol=args.rtol)
-        run.forward(d_u0, d_steps[:1])  # first device refresh uploads the refresh plan
+        phys = dict(density=W.WAVE_DENSITY, gravity=W.WAVE_GRAVITY)
+        run.forward_interface(d_u0, d_ifaces[:1], **phys)  # first device refresh uploads the refresh plan
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        uN = run.forward(d_u0, d_steps)
+        uN = run.forward_interface(d_u0, d_ifaces, **phys)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t1
         solve_s = sum(st.solve_ms for st in run.stats) * 1e-3
@@ -321,6 +329,7 @@ def wave_main(args):
             runs[prec] = [c.cpu().numpy() for c in run.checkpoints]
             if args.wave_fd:
                 # J is quadratic in the rhs: central differences are exact for any step
+                steps = [(k.cpu().numpy(), j.cpu().numpy()) for k, j in run.steps]
                 host = tpamr.ProjectionRun(s, dt, rtol=1e-12, adjoint_rtol=args.rtol)
                 d = W.smooth_velocity(mesh, seed=W.SEED + 3, device="cuda:0")
                 d = tpamr.divergence(s, d)

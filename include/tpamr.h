@@ -224,8 +224,20 @@ int tpamr_adjoint(tpamr_ctx* ctx, const double* w, const double* p, double rtol,
  *   two_phase/stepping.jl:181-201);
  * - tpamr_divergence_device: divergence! of interior face velocities;
  * - tpamr_project_device: tpamr_project_ex on device u (and p += phi when p
- *   is not NULL, phi copied out when not NULL). */
+ *   is not NULL, phi copied out when not NULL);
+ * - tpamr_mixture_face_coefficients_device: tpamr_mixture_face_coefficients
+ *   (inv.(mixture_density), forces.jl:23-36,69-84,131) from a device cell
+ *   fraction into device face coefficients;
+ * - tpamr_gravity_jumps_device: tpamr_gravity_jumps (gravity!,
+ *   forces.jl:528-580) from device apertures and offsets (NULL = none).
+ * The twins do not scan their inputs for non-finite values; a non-finite
+ * coefficient is rejected by the refresh that consumes it. */
 int tpamr_set_face_coefficients_device(tpamr_ctx* ctx, const double* d_kappa);
+int tpamr_mixture_face_coefficients_device(tpamr_ctx* ctx, const double* d_fraction, double rho_liquid,
+                                           double rho_gas, double* d_kappa);
+int tpamr_gravity_jumps_device(tpamr_ctx* ctx, const double* d_offsets, const double* d_liquid_aperture,
+                               const double* d_gas_aperture, double rho_liquid, double rho_gas, double gx, double gy,
+                               double* d_jumps);
 int tpamr_apply_force_device(tpamr_ctx* ctx, const double* d_jumps, double dt, double* d_u);
 int tpamr_divergence_device(tpamr_ctx* ctx, const double* d_u, double* d_div);
 int tpamr_project_device(tpamr_ctx* ctx, double* d_u, double* d_p, double dt, double tolerance, int max_cycles,

@@ -186,3 +186,28 @@ def test_hydrostatic_balance_on_gpu(precision, kind, gravity):
     st, _ = tpamr.project(u, np.zeros(pm.ncells), gs, 0.01, tolerance=512 * EPS)
     assert np.max(np.abs(u)) <= 2048 * EPS, (np.max(np.abs(u)), st)
     gs.close()
+
+
+@pytest.mark.gpu
+def test_device_twins_of_mixture_and_gravity_match_host_entry_points():
+    """tpamr_mixture_face_coefficients_device and tpamr_gravity_jumps_device
+    run the host entry points' kernels on device memory: bit for bit the same
+    coefficients and jumps."""
+    import torch
+    pm, om, frac, la, ga, off = wave_case()
+    gs = tpamr.EllipticSolver(tpamr.PoissonOperator(pm, np.ones(pm.nfaces)), precision="fp64", device=0)
+    dev = torch.device("cuda:0")
+    kd = tpamr.mixture_face_coefficients_device(gs, torch.as_tensor(frac, device=dev), W.WAVE_DENSITY)
+    assert np.array_equal(kd.cpu().numpy(), tpamr.mixture_face_coefficients(gs, frac, W.WAVE_DENSITY))
+    for o in (off, None):
+        jd = tpamr.gravity_jumps_device(gs, torch.as_tensor(la, device=dev), torch.as_tensor(ga, device=dev),
+                                        density=W.WAVE_DENSITY, gravity=W.WAVE_GRAVITY,
+                                        centroid_offsets=None if o is None else torch.as_tensor(o, device=dev))
+        jh = tpamr.gravity_jumps(gs, la, ga, density=W.WAVE_DENSITY, gravity=W.WAVE_GRAVITY, centroid_offsets=o)
+        assert np.array_equal(jd.cpu().numpy(), jh)
+    with pytest.raises(ValueError):
+        tpamr.gravity_jumps_device(gs, torch.as_tensor(la, device=dev), torch.as_tensor(ga, device=dev),
+                                   density=(1000.0, -1.0), gravity=W.WAVE_GRAVITY)
+    with pytest.raises(ValueError):
+        tpamr.mixture_face_coefficients_device(gs, torch.as_tensor(frac[:-1], device=dev), W.WAVE_DENSITY)
+    gs.close()

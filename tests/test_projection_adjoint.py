@@ -224,3 +224,36 @@ def test_device_run_matches_host_run(case, precision):
         for a, b in zip(gd["kappa"], gh["kappa"]):
             assert np.array_equal(a.cpu().numpy(), b)
         s.close()
+
+
+def test_device_run_from_interface_data_matches_run_from_steps(case):
+    """DeviceProjectionRun.forward_interface computes each step's mixture
+    coefficients and gravity jumps on the GPU from the step's interface data;
+    the run equals the one from host-built steps (wave_steps), bit for bit."""
+    import torch
+    pm, om, gs, steps, frac_end, u0 = case
+    dev = torch.device("cuda:0")
+    ifaces = []
+    for n, data in enumerate(W.wave_interfaces(pm, STEPS, DT)):
+        if n == STEPS:
+            assert np.array_equal(data[0], frac_end)
+            break
+        ifaces.append(tuple(torch.as_tensor(a, device=dev) for a in data))
+    outs = []
+    for how in ("steps", "interface"):
+        s = tpamr.EllipticSolver(tpamr.PoissonOperator(pm, steps[0][0]), precision="fp64", device=0)
+        run = tpamr.DeviceProjectionRun(s, DT, tolerance=1e-14, adjoint_rtol=1e-13)
+        if how == "steps":
+            uN = run.forward(torch.as_tensor(u0, device=dev), [(torch.as_tensor(k, device=dev),
+                                                                 torch.as_tensor(j, device=dev)) for k, j in steps])
+        else:
+            uN = run.forward_interface(torch.as_tensor(u0, device=dev), ifaces, density=W.WAVE_DENSITY,
+                                       gravity=W.WAVE_GRAVITY)
+            for (k, j), (kh, jh) in zip(run.steps, steps):
+                assert np.array_equal(k.cpu().numpy(), kh) and np.array_equal(j.cpu().numpy(), jh)
+        g = run.backward(uN.clone())
+        outs.append((uN.cpu().numpy(), [c.cpu().numpy() for c in run.checkpoints], g["rhs"].cpu().numpy()))
+        s.close()
+    (ua, ca, ga), (ub, cb, gb) = outs
+    assert np.array_equal(ua, ub) and np.array_equal(ga, gb)
+    assert all(np.array_equal(a, b) for a, b in zip(ca, cb))

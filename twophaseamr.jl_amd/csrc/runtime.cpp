@@ -1671,14 +1671,27 @@ int tpamr_weighted_jump_force(tpamr_ctx* c, const double* jumps, double* accel) 
 
 namespace {
 // gravity! jumps into c->f3 (liquid and gas apertures staged in f1, f2)
-void gravity_jumps(tpamr_ctx* c, const double* offsets, const double* liquid, const double* gas, double rl,
-                   double rg, double gx, double gy) {
+void gravity_checks(tpamr_ctx* c, double rl, double rg, double gx, double gy) {
   if (!(std::isfinite(rl) && rl > 0 && std::isfinite(rg) && rg > 0))
     throw std::invalid_argument("densities must be finite and positive");
   if (!(std::isfinite(gx) && std::isfinite(gy))) throw std::invalid_argument("gravity must be finite");
   const TreeMesh& M = c->mesh();
   if ((M.periodic[0] && gx != 0) || (M.periodic[1] && gy != 0))
     throw std::invalid_argument("gravity potential requires nonperiodic boundaries along gravity");
+}
+
+// Jumps of gravity! from device apertures and offsets (nullable) into `out`.
+void launch_gravity(tpamr_ctx* c, const double* d_off, const double* d_liquid, const double* d_gas, double rl,
+                    double rg, double gx, double gy, double* out) {
+  const TreeMesh& M = c->mesh();
+  launch_gravity_jumps(c->stream, M.ncells(), M.bs[0], M.nb(), c->origin.p, c->spacing.p, d_off, gx, gy, c->tmp.p,
+                       M.nfaces(), c->fm.p, c->fp.p, c->farea.p, d_liquid, d_gas, rl, rg, out);
+}
+
+void gravity_jumps(tpamr_ctx* c, const double* offsets, const double* liquid, const double* gas, double rl,
+                   double rg, double gx, double gy) {
+  gravity_checks(c, rl, rg, gx, gy);
+  const TreeMesh& M = c->mesh();
   const int N = M.ncells(), F = M.nfaces();
   check_finite(liquid, F, "liquid aperture");
   check_finite(gas, F, "gas aperture");
@@ -1691,8 +1704,7 @@ void gravity_jumps(tpamr_ctx* c, const double* offsets, const double* liquid, co
     upload_vec(c, c->cell2.p, offsets, 2 * size_t(N));
     doff = c->cell2.p;
   }
-  launch_gravity_jumps(c->stream, N, M.bs[0], M.nb(), c->origin.p, c->spacing.p, doff, gx, gy, c->tmp.p, F, c->fm.p,
-                       c->fp.p, c->farea.p, c->f1.p, c->f2.p, rl, rg, c->f3.p);
+  launch_gravity(c, doff, c->f1.p, c->f2.p, rl, rg, gx, gy, c->f3.p);
 }
 }  // namespace
 
@@ -1868,6 +1880,35 @@ int tpamr_apply_force_device(tpamr_ctx* c, const double* d_jumps, double dt, dou
     launch_face_jump(c->stream, F, c->fm.p, c->fp.p, c->farea.p, c->sqk.p, nullptr, d_jumps, c->f1.p);
     weighted_metric(c, c->f1.p, c->f2.p);
     launch_axpby(c->stream, F, dt, c->f2.p, 1.0, d_u);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int tpamr_mixture_face_coefficients_device(tpamr_ctx* c, const double* d_fraction, double rl, double rg,
+                                           double* d_kappa) {
+  if (!c || !d_fraction || !d_kappa) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    if (!c->H) throw StateError("no hierarchy: call tpamr_set_face_coefficients once before two-phase refreshes");
+    if (!(std::isfinite(rl) && rl > 0 && std::isfinite(rg) && rg > 0))
+      throw std::invalid_argument("phase densities must be finite and positive");
+    HIPCHECK(hipSetDevice(c->device));
+    const TreeMesh& M = c->mesh();
+    launch_mixture(c->stream, M.nfaces(), M.nb(), c->fm.p, c->fp.p, c->faxis.p, c->spacing.p, d_fraction, rl, rg,
+                   d_kappa);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int tpamr_gravity_jumps_device(tpamr_ctx* c, const double* d_offsets, const double* d_liquid, const double* d_gas,
+                               double rho_liquid, double rho_gas, double gx, double gy, double* d_jumps) {
+  if (!c || !d_liquid || !d_gas || !d_jumps) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    if (!c->H) throw StateError("no hierarchy: call tpamr_set_face_coefficients first");
+    gravity_checks(c, rho_liquid, rho_gas, gx, gy);
+    HIPCHECK(hipSetDevice(c->device));
+    launch_gravity(c, d_offsets, d_liquid, d_gas, rho_liquid, rho_gas, gx, gy, d_jumps);
     HIPCHECK(hipStreamSynchronize(c->stream));
   });
 }

@@ -105,6 +105,10 @@ _SIGNATURES = {
     "tpamr_set_face_coefficients_device": (C.c_int, [C.c_void_p, C.c_void_p]),
     "tpamr_apply_force_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),
     "tpamr_divergence_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "tpamr_mixture_face_coefficients_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                                         C.c_void_p]),
+    "tpamr_gravity_jumps_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                             C.c_double, C.c_double, C.c_double, C.c_void_p]),
     "tpamr_project_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int,
                                        C.c_void_p, C.c_double, C.c_void_p, C.POINTER(Stats)]),
     "tpamr_project_adjoint_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,

@@ -134,6 +134,48 @@ def gravity(solver: EllipticSolver, liquid_aperture, gas_aperture, *, density, g
     return out
 
 
+def _device_vec(t, n, what, width=1):
+    import torch
+    shape = (n,) if width == 1 else (n, width)
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+            and (tuple(t.shape) == shape or (width > 1 and tuple(t.shape) == (n * width,)))):
+        raise ValueError(f"{what} must be a contiguous float64 device tensor of shape {shape}")
+    return t
+
+
+def mixture_face_coefficients_device(solver: EllipticSolver, fraction, density=(1.0, 1.0), out=None):
+    """`mixture_face_coefficients` on device memory: `fraction` (cells) and the
+    result (faces, a new tensor unless `out` is given) are float64 tensors on
+    the solver's GPU."""
+    import torch
+    rho_l, rho_g = (float(v) for v in density)
+    m = solver.operator.mesh
+    c = _device_vec(fraction, m.ncells, "fraction")
+    k = torch.empty(m.nfaces, dtype=torch.float64, device=c.device) if out is None else \
+        _device_vec(out, m.nfaces, "out")
+    L.check(L.lib().tpamr_mixture_face_coefficients_device(solver.ctx.h, c.data_ptr(), rho_l, rho_g, k.data_ptr()),
+            solver.ctx.h)
+    return k
+
+
+def gravity_jumps_device(solver: EllipticSolver, liquid_aperture, gas_aperture, *, density, gravity,
+                         centroid_offsets=None, out=None):
+    """`gravity_jumps` on device memory: apertures (faces), offsets (cells x 2
+    or None) and the result are float64 tensors on the solver's GPU."""
+    import torch
+    m = solver.operator.mesh
+    rho_l, rho_g = (float(v) for v in density)
+    gx, gy = (float(v) for v in gravity)
+    la = _device_vec(liquid_aperture, m.nfaces, "liquid aperture")
+    ga = _device_vec(gas_aperture, m.nfaces, "gas aperture")
+    off = None if centroid_offsets is None else _device_vec(centroid_offsets, m.ncells, "centroid offsets", 2)
+    j = torch.empty(m.nfaces, dtype=torch.float64, device=la.device) if out is None else \
+        _device_vec(out, m.nfaces, "out")
+    L.check(L.lib().tpamr_gravity_jumps_device(solver.ctx.h, None if off is None else off.data_ptr(), la.data_ptr(),
+                                               ga.data_ptr(), rho_l, rho_g, gx, gy, j.data_ptr()), solver.ctx.h)
+    return j
+
+
 def divergence(solver: EllipticSolver, u, u_boundary=None):
     m = solver.operator.mesh
     uu = L.f64(u, m.nfaces, "u")

@@ -219,6 +219,24 @@ class DeviceProjectionRun:
             self.stats.append(LinearSolverStats._from(st, s.precision))
         return u
 
+    def forward_interface(self, u0, interfaces, *, density, gravity):
+        """`forward` with each step's face coefficients and gravity jumps
+        computed on the GPU from that step's interface data: `interfaces` is a
+        sequence of (cell fraction, liquid face aperture, gas face aperture,
+        centroid offsets or None) device tensors; the coefficients are the
+        mixture's (`mixture_face_coefficients_device`) and the jumps
+        gravity!'s (`gravity_jumps_device`). They are kept as the run's steps
+        for `backward`."""
+        from .physics import gravity_jumps_device, mixture_face_coefficients_device
+        s = self.solver
+        s._sync()
+        steps = []
+        for fraction, liquid, gas, offsets in interfaces:
+            kappa = mixture_face_coefficients_device(s, fraction, density)
+            jumps = gravity_jumps_device(s, liquid, gas, density=density, gravity=gravity, centroid_offsets=offsets)
+            steps.append((kappa, jumps))
+        return self.forward(u0, steps)
+
     def backward(self, ubar):
         import ctypes as C
         import torch

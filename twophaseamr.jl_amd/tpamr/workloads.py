@@ -373,18 +373,25 @@ def wave_mesh(levels=(5, 11), block=32, band_cells=4, drift=0.0):
                          periodic=(False, False), max_level=hi, refine_many=refine_many)
 
 
+def wave_interfaces(mesh: BlockTreeMesh, steps, dt):
+    """Yields, for n = 0 .. steps (time n dt), the interface data of the wave
+    surface at that time: (cell fraction, liquid face aperture, gas face
+    aperture, centroid offsets), one step at a time."""
+    eta, deta = wave_surface()
+    geom = InterfaceGeometry(mesh)
+    for n in range(steps + 1):
+        t = n * dt
+        yield geom(lambda x: eta(x, t), lambda x: deta(x, t))
+
+
 def wave_steps(solver, steps, dt, density=WAVE_DENSITY, gravity=WAVE_GRAVITY):
     """Per step n (time n dt): the mixture face coefficients of the surface
     at that time and the gravity! jumps, as ProjectionRun takes them, plus
     the final fraction (for the kinetic energy)."""
     from .physics import gravity_jumps, mixture_face_coefficients
-    eta, deta = wave_surface()
-    geom = InterfaceGeometry(solver.operator.mesh)
     out = []
     frac = None
-    for n in range(steps + 1):
-        t = n * dt
-        frac, la, ga, off = geom(lambda x: eta(x, t), lambda x: deta(x, t))
+    for n, (frac, la, ga, off) in enumerate(wave_interfaces(solver.operator.mesh, steps, dt)):
         if n == steps:
             break
         kappa = mixture_face_coefficients(solver, frac, density)
