@@ -199,12 +199,13 @@ def pmc_summary(db_fetch, db_write, out_path=TRAFFIC_MD):
     lines = [
         "# HBM traffic of the finest-level FP32 Jacobi sweep, circles64 levels 5-10",
         "",
-        "Commands (repository root, one MI355X; `cd /tmp && export TMPDIR=/tmp` first, paths absolute):",
+        "Commands (one MI355X; `cd /tmp && export TMPDIR=/tmp` first; `<repo>` the repository root, `<out>` an "
+        "output folder):",
         "",
         "```",
-        "rocprofv3 --pmc FETCH_SIZE -d <repo>/gpurun_out/pmc_f -o pmc -- python3 <repo>/bench.py --probe",
-        "rocprofv3 --pmc WRITE_SIZE -d <repo>/gpurun_out/pmc_w -o pmc -- python3 <repo>/bench.py --probe",
-        "python3 bench.py --pmc-summary gpurun_out/pmc_f/pmc_results.db gpurun_out/pmc_w/pmc_results.db",
+        "rocprofv3 --pmc FETCH_SIZE -d <out>/pmc_f -o pmc -- python3 <repo>/bench.py --probe",
+        "rocprofv3 --pmc WRITE_SIZE -d <out>/pmc_w -o pmc -- python3 <repo>/bench.py --probe",
+        "python3 bench.py --pmc-summary <out>/pmc_f/pmc_results.db <out>/pmc_w/pmc_results.db",
         "```",
         "",
         f"Kernel `{SWEEP_KERNEL}` at the finest level ({f['n']} dispatches per pass). Calibration: the last 5",
@@ -234,7 +235,8 @@ def trace_summary(db, title, command, top=30):
                        "min(end - start) / 1e3, max(end - start) / 1e3, max(grid_x) from kernels "
                        "group by name order by sum(end - start) desc limit ?", (top,)).fetchall()
     con.close()
-    lines = [f"# {title}", "", "Command (repository root, one MI355X; `cd /tmp && export TMPDIR=/tmp` first):", "",
+    lines = [f"# {title}", "", "Command (one MI355X; `cd /tmp && export TMPDIR=/tmp` first; `<repo>` the repository "
+             "root, `<out>` an output folder):", "",
              "```", command, "```", "", f"Total kernel time: {tot:.2f} ms", "",
              "| kernel | calls | total ms | avg us | min us | max us | max grid |", "|---|---:|---:|---:|---:|---:|---:|"]
     for r in rows:
