@@ -496,3 +496,36 @@ def test_coarse_newton_schulz_update_matches_gauss_jordan(precision):
     for (pg, cg), (pn, cn) in zip(*out):
         assert cg == cn
         assert rel(pn, pg) < 1e-12
+
+
+def test_solvers_built_and_closed_repeatedly_release_device_memory():
+    """A regridding run builds a new pressure solver at every regrid
+    (`_rebuild_pressure_solver!`, src/navier_stokes/workspace.jl:82-112):
+    building, refreshing (device refresh, Newton-Schulz coarse update),
+    solving and closing solvers again and again leaves the device's free
+    memory where it was."""
+    import torch
+    from tpamr import workloads as W
+    pm = W.circles_mesh(levels=(3, 7), n=16)[0]
+    x, y = pm.cell_centers()
+    rhs = np.sin(3 * x) * np.cos(5 * y)
+    k = np.exp(0.3 * np.sin(np.arange(pm.nfaces)))
+
+    def cycle():
+        for precision in ("fp64", "fp32_refine"):
+            s = tpamr.EllipticSolver(tpamr.PoissonOperator(pm, k), precision=precision, device=0)
+            for r in range(3):
+                s.operator.replace_face_coefficient(k * (1 + 0.01 * r))
+                s.refresh()
+                p = np.zeros(pm.ncells)
+                assert s.solve(p, rhs, rtol=1e-10).converged
+            s.close()
+        torch.cuda.synchronize()
+        return torch.cuda.mem_get_info()[0]
+
+    cycle()  # one-time runtime and library state
+    free0 = cycle()
+    for _ in range(3):
+        free = cycle()
+    print("free device memory after the second and the fifth cycle:", free0, free, flush=True)
+    assert abs(free - free0) <= 16 << 20
