@@ -247,6 +247,26 @@ def trace_summary(db, title, command, top=30):
     print("\n".join(lines))
 
 
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """Bytes read plus written per second (GB/s) of a plain device copy of
+    `nbytes`, timed with HIP events on torch's stream: what a streaming kernel
+    reaches on this GPU."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2.0 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return gbs
+
+
 def committed_traffic():
     """Bytes per launch recorded by the last --pmc-summary, or None."""
     try:
@@ -450,6 +470,7 @@ def main():
         return float(np.max(np.abs(x_sol[prec] - x_sol["fp64"])) / np.max(np.abs(x_sol["fp64"])))
 
     achieved = head["smoother_bytes"] / (head["smoother_ms"] * 1e-3) / 1e9
+    copy_gbs = copy_bandwidth()
     traffic = committed_traffic()
     out = {
         "metric": METRIC,
@@ -485,7 +506,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": SWEEP_KERNEL, "bytes_per_launch": head["smoother_bytes"],
                      "kernel_compulsory_bytes": head["smoother_kernel_bytes"],
-                     "launch_ms": head["smoother_ms"]},
+                     "launch_ms": head["smoother_ms"], "copy_GBps": copy_gbs,
+                     "frac_of_copy": achieved / copy_gbs},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"], out["cpu_sample_on_gpu"] = cpu_baseline(tuple(args.cpu_levels), args.rtol)

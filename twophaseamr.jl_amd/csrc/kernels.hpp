@@ -28,6 +28,7 @@ struct LevelView {
   // entries per row): entry 4 g + t of row x at (blk_irr[b] * ng + g * nr_b + x) * 4 + t
   const unsigned short* ell_ref;
   const int* blk_irr;        // nblocks + 1 offsets of the irregular rows per block
+  const int* blk_ng;         // per block: entry groups of its longest irregular row
   const unsigned* ell_loc;   // per row in a block's length order: local cell | length << 16
   const int* blk_ext;
   const int* ext_col;

@@ -88,7 +88,7 @@ struct DevArray {
 
 struct DevTopo {
   int nblocks = 0, ncells = 0;
-  DevArray<int> nbr, xfer, blk_ext, ext_col, blk_irr;
+  DevArray<int> nbr, xfer, blk_ext, ext_col, blk_irr, blk_ng;
   DevArray<unsigned> ell_loc;
   DevArray<unsigned short> ell_ref;
   DevArray<double> volume;
@@ -289,6 +289,7 @@ void upload_level_values(tpamr_ctx* c, DevLevel<T>& D, const LevelData& L, const
   v.ell_diag = D.ell_diag.p;
   v.ell_ref = Tp.ell_ref.p;
   v.blk_irr = Tp.blk_irr.p;
+  v.blk_ng = Tp.blk_ng.p;
   v.ell_loc = Tp.ell_loc.p;
   v.blk_ext = Tp.blk_ext.p;
   v.ext_col = Tp.ext_col.p;
@@ -318,6 +319,13 @@ void upload_hierarchy(tpamr_ctx* c, bool topology) {
       T.ell_ref.upload(L.ell_ref.empty() ? std::vector<unsigned short>(1, 0) : L.ell_ref, s);
       T.blk_ext.upload(L.blk_ext, s);
       T.blk_irr.upload(L.blk_irr, s);
+      {
+        // a block's rows run in decreasing length: its first row is the longest
+        std::vector<int> ngb(L.nblocks, 0);
+        for (int b = 0; b < L.nblocks; ++b)
+          if (L.blk_irr[b + 1] > L.blk_irr[b]) ngb[b] = int(((L.ell_loc[L.blk_irr[b]] >> 16) + 3) / 4);
+        T.blk_ng.upload(ngb, s);
+      }
       T.ell_loc.upload(L.ell_loc.empty() ? std::vector<uint32_t>(1, 0u) : L.ell_loc, s);
       T.ext_col.upload(L.ext_col.empty() ? std::vector<int>(1, 0) : L.ext_col, s);
       T.volume.upload(L.volume, s);

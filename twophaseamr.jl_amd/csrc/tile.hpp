@@ -190,9 +190,10 @@ __device__ inline void tile4(unsigned char* lds, const LevelView<T>& L, const Pl
     cs4 = ld4(L.cs + P.cell0);
     if (q == TPR - 1) ceb = L.ce[b * BY + j];
     if (j == BY - 1) cn4 = ld4(L.cn + b * BX + i0);
+    pre();  // the kernel's own loads (no LDS)
     // halo values (W, E, S, N) of same-level neighbour blocks: halo index h
     // is LDS slot NB + h
-    for (int h = lt; h < G::HALO; h += TPB) {
+    auto halo_value = [&](int h) {
       int side, src;
       if (h < BY) {
         side = 0; src = h * BX + (BX - 1);
@@ -204,42 +205,62 @@ __device__ inline void tile4(unsigned char* lds, const LevelView<T>& L, const Pl
         side = 3; src = h - 2 * BY - BX;
       }
       const int nb = L.nbr[4 * b + side];
-      const T hv = nb >= 0 ? fetch.one(nb * NB + src) : T(0);
-      S.v[NB + h] = ABS ? fabs(hv) : hv;
-    }
+      return nb >= 0 ? fetch.one(nb * NB + src) : T(0);
+    };
+    // the lane's first halo value and first external cell are loaded into
+    // registers and written to LDS after every load of the tile is issued,
+    // so the irregular rows' loads do not wait for the halo values
+    const T hv0 = lt < G::HALO ? halo_value(lt) : T(0);
+    T ev0 = 0;
+    int e0 = 0, ne = 0;
     if (any) {
-      // external cells of the irregular rows
-      const int e0 = L.blk_ext[b], ne = L.blk_ext[b + 1] - e0;
-      for (int e = lt; e < ne; e += TPB) {
-        const T ev = fetch.one(L.ext_col[e0 + e]);
-        S.v[G::EXT + e] = ABS ? fabs(ev) : ev;
-      }
       // this lane's irregular row (rows in decreasing length order; group g
-      // of row x of the block at (r0 * ng + g * nr + x) * 4)
+      // of row x of the block at (r0 * ng + g * nr + x) * 4). Its loads are
+      // predicated on the block's group count, not on the row's own length,
+      // so they wait on no other load than the block's row offsets; padding
+      // entries of shorter rows are loaded, not used.
       r0 = L.blk_irr[b];
       nr = L.blk_irr[b + 1] - r0;
+      int ngb = L.blk_ng[b];
       if constexpr (G::TPB >= 64) {
         // uniform per wavefront: group offsets become scalar
         r0 = __builtin_amdgcn_readfirstlane(r0);
         nr = __builtin_amdgcn_readfirstlane(nr);
+        ngb = __builtin_amdgcn_readfirstlane(ngb);
       }
+      unsigned info = 0;
       if (lt < nr) {
-        const unsigned info = L.ell_loc[r0 + lt];
-        rloc = int(info & 0xffffu);
-        rlen = int(info >> 16);
+        info = L.ell_loc[r0 + lt];
         rdg = L.ell_diag[r0 + lt];
         const int base = (r0 * ng + lt) * 4;
 #pragma unroll
         for (int g = 0; g < ELL_G; ++g)
-          if (g < ng && 4 * g < rlen) {
+          if (g < ngb) {
             const int at = base + g * nr * 4;
             rv[g] = ld4(vals + at);
             rr[g] = *reinterpret_cast<const uint2*>(L.ell_ref + at);
           }
       }
+      // external cells of the irregular rows
+      e0 = L.blk_ext[b];
+      ne = L.blk_ext[b + 1] - e0;
+      if (lt < ne) ev0 = fetch.one(L.ext_col[e0 + lt]);
+      rloc = int(info & 0xffffu);
+      rlen = int(info >> 16);
+    }
+    if (lt < G::HALO) S.v[NB + lt] = ABS ? fabs(hv0) : hv0;
+    for (int h = lt + TPB; h < G::HALO; h += TPB) {
+      const T hv = halo_value(h);
+      S.v[NB + h] = ABS ? fabs(hv) : hv;
+    }
+    if (any) {
+      if (lt < ne) S.v[G::EXT + lt] = ABS ? fabs(ev0) : ev0;
+      for (int e = lt + TPB; e < ne; e += TPB) {
+        const T ev = fetch.one(L.ext_col[e0 + e]);
+        S.v[G::EXT + e] = ABS ? fabs(ev) : ev;
+      }
       S.flag4[lt] = 0u;  // this lane's four cells
     }
-    pre();
     if constexpr (ABS) {
 #pragma unroll
       for (int m = 0; m < 4; ++m) own.v[m] = fabs(own.v[m]);
