@@ -950,16 +950,44 @@ __global__ void k_ref_regular(int nf, const double* __restrict__ k, const double
   arr[d1 & 3][d1 >> 2] = w;
 }
 
-// irregular rows: every entry is the sum of its face-metric terms; the
-// row's diagonal and the largest row ratio sum |a_rj| / a_rr (as the bits of
-// a nonnegative double, whose order is the numeric order)
-__global__ void k_ref_rows(int nrows, const int* __restrict__ irr_ptr, const int* __restrict__ term_ptr,
-                           const int* __restrict__ term_f, const int* __restrict__ term_g,
-                           const double* __restrict__ term_k, const double* __restrict__ term_s,
-                           const unsigned char* __restrict__ term_diag, const double* __restrict__ k,
-                           const double* __restrict__ area, const int* __restrict__ ell_pos,
-                           const int* __restrict__ ell_row, const int* __restrict__ diag_e,
-                           double* __restrict__ ell_val, double* __restrict__ ell_bound,
+// irregular rows, one thread per entry: the entry is the sum of its
+// face-metric terms, in the host pass's order (also kept in val_e, the
+// entries in row order)
+__global__ void k_ref_entries(int nnz, const int* __restrict__ term_ptr, const int* __restrict__ term_f,
+                              const int* __restrict__ term_g, const double* __restrict__ term_k,
+                              const double* __restrict__ term_s, const unsigned char* __restrict__ term_diag,
+                              const double* __restrict__ k, const double* __restrict__ area,
+                              const int* __restrict__ ell_pos, double* __restrict__ ell_val,
+                              double* __restrict__ ell_bound, double* __restrict__ val_e) {
+#pragma clang fp contract(off)
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnz) return;
+  double v = 0, bv = 0;
+  for (int q = term_ptr[e]; q < term_ptr[e + 1]; ++q) {
+    const int f = term_f[q], g = term_g[q];
+    const double kfg = term_k[q], af = area[f], ag = area[g];
+    double core, babs;
+    if (term_diag[q]) {
+      core = k[f] * kfg * af * ag;
+      babs = core;
+    } else {
+      const double sf = sqrt(k[f]), sg = sqrt(k[g]);
+      core = (sf * kfg) * sg * af * ag;
+      babs = (sf * fabs(kfg)) * sg * af * ag;
+    }
+    v += term_s[q] * core;
+    bv += babs;
+  }
+  ell_val[ell_pos[e]] = v;
+  if (ell_bound) ell_bound[ell_pos[e]] = bv;
+  val_e[e] = v;
+}
+
+// irregular rows, one thread per row: the row's diagonal and the largest row
+// ratio sum |a_rj| / a_rr (as the bits of a nonnegative double, whose order is
+// the numeric order), summed in entry order
+__global__ void k_ref_rows(int nrows, const int* __restrict__ irr_ptr, const int* __restrict__ diag_e,
+                           const int* __restrict__ ell_row, const double* __restrict__ val_e,
                            double* __restrict__ ell_diag, double* __restrict__ irr_diag,
                            unsigned long long* __restrict__ ratio) {
 #pragma clang fp contract(off)
@@ -968,24 +996,7 @@ __global__ void k_ref_rows(int nrows, const int* __restrict__ irr_ptr, const int
   double rowsum = 0, diag = 0;
   const int de = diag_e[r];
   for (int e = irr_ptr[r]; e < irr_ptr[r + 1]; ++e) {
-    double v = 0, bv = 0;
-    for (int q = term_ptr[e]; q < term_ptr[e + 1]; ++q) {
-      const int f = term_f[q], g = term_g[q];
-      const double kfg = term_k[q], af = area[f], ag = area[g];
-      double core, babs;
-      if (term_diag[q]) {
-        core = k[f] * kfg * af * ag;
-        babs = core;
-      } else {
-        const double sf = sqrt(k[f]), sg = sqrt(k[g]);
-        core = (sf * kfg) * sg * af * ag;
-        babs = (sf * fabs(kfg)) * sg * af * ag;
-      }
-      v += term_s[q] * core;
-      bv += babs;
-    }
-    ell_val[ell_pos[e]] = v;
-    if (ell_bound) ell_bound[ell_pos[e]] = bv;
+    const double v = val_e[e];
     rowsum += fabs(v);
     if (e == de) diag = v;
   }
@@ -1401,11 +1412,13 @@ void launch_ref_regular(hipStream_t s, int nf, const double* k, const double* kd
 void launch_ref_rows(hipStream_t s, int nrows, const int* irr_ptr, const int* term_ptr, const int* term_f,
                      const int* term_g, const double* term_k, const double* term_s, const unsigned char* term_diag,
                      const double* k, const double* area, const int* ell_pos, const int* ell_row, const int* diag_e,
-                     double* ell_val, double* ell_bound, double* ell_diag, double* irr_diag,
-                     unsigned long long* ratio) {
+                     double* ell_val, double* ell_bound, double* ell_diag, double* irr_diag, double* val_e,
+                     int nnz, unsigned long long* ratio) {
+  if (nnz > 0)
+    hipLaunchKernelGGL(k_ref_entries, dim3(cdiv(nnz, 256)), dim3(256), 0, s, nnz, term_ptr, term_f, term_g, term_k,
+                       term_s, term_diag, k, area, ell_pos, ell_val, ell_bound, val_e);
   if (nrows > 0)
-    hipLaunchKernelGGL(k_ref_rows, dim3(cdiv(nrows, 256)), dim3(256), 0, s, nrows, irr_ptr, term_ptr, term_f,
-                       term_g, term_k, term_s, term_diag, k, area, ell_pos, ell_row, diag_e, ell_val, ell_bound,
+    hipLaunchKernelGGL(k_ref_rows, dim3(cdiv(nrows, 256)), dim3(256), 0, s, nrows, irr_ptr, diag_e, ell_row, val_e,
                        ell_diag, irr_diag, ratio);
 }
 void launch_ref_dinv(hipStream_t s, int n, int bx, int by, const double* cw, const double* ce, const double* cs,

@@ -92,8 +92,8 @@ void launch_ref_regular(hipStream_t s, int nf, const double* k, const double* kd
 void launch_ref_rows(hipStream_t s, int nrows, const int* irr_ptr, const int* term_ptr, const int* term_f,
                      const int* term_g, const double* term_k, const double* term_s, const unsigned char* term_diag,
                      const double* k, const double* area, const int* ell_pos, const int* ell_row, const int* diag_e,
-                     double* ell_val, double* ell_bound, double* ell_diag, double* irr_diag,
-                     unsigned long long* ratio);
+                     double* ell_val, double* ell_bound, double* ell_diag, double* irr_diag, double* val_e,
+                     int nnz, unsigned long long* ratio);
 void launch_ref_dinv(hipStream_t s, int n, int bx, int by, const double* cw, const double* ce, const double* cs,
                      const double* cn, const int* cell_irr, const double* irr_diag, double* dinv, int* regular);
 void launch_ref_check(hipStream_t s, int nf, const double* k, int* bad);

@@ -101,7 +101,8 @@ struct DevTopo {
 struct DevPlan {
   int nfaces = 0, nrows = 0;
   DevArray<int> rptr, ridx, dst, irr_ptr, term_ptr, term_f, term_g, ell_pos, ell_row, diag_e, cell_irr;
-  DevArray<double> rw, kdiag, area, kappa, term_k, term_s, irr_diag;
+  DevArray<double> rw, kdiag, area, kappa, term_k, term_s, irr_diag, val_e;
+  int nnz = 0;  // irregular entries
   DevArray<unsigned char> term_diag;
   DevArray<double> cw, ce, cs, cn, ell_val, ell_diag, dinv;
 };
@@ -1050,6 +1051,8 @@ void upload_plan(tpamr_ctx* c) {
     up(P.diag_e, L.irr_diag_e, 0);
     P.cell_irr.upload(L.cell_irr, s);
     P.irr_diag.alloc(std::max(P.nrows, 1));
+    P.nnz = int(L.irr_col.size());
+    P.val_e.alloc(std::max(P.nnz, 1));
     if (l >= int(c->ld.size())) {
       P.cw.upload(L.cw, s);
       P.ce.upload(L.ce, s);
@@ -1112,7 +1115,7 @@ void device_refresh(tpamr_ctx* c, const double* kappa, bool kappa_on_device = fa
     launch_ref_regular(s, P.nfaces, P.kappa.p, P.kdiag.p, P.area.p, P.dst.p, cw, ce, cs, cn);
     launch_ref_rows(s, P.nrows, P.irr_ptr.p, P.term_ptr.p, P.term_f.p, P.term_g.p, P.term_k.p, P.term_s.p,
                     P.term_diag.p, P.kappa.p, P.area.p, P.ell_pos.p, P.ell_row.p, P.diag_e.p, ev, eb, ed,
-                    P.irr_diag.p, c->ref_ratio.p + l);
+                    P.irr_diag.p, P.val_e.p, P.nnz, c->ref_ratio.p + l);
     launch_ref_dinv(s, L.ncells, bx, by, cw, ce, cs, cn, P.cell_irr.p, P.irr_diag.p, di, c->ref_flags.p + l);
     if (l < int(c->lf.size())) {
       DevLevel<float>& Fv = c->lf[l];
