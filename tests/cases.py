@@ -45,6 +45,11 @@ MESHES = {
     "circle16": dict(base_mesh=(16, 16), block_size=(16, 16), periodic=(False, False), max_level=4,
                      refine=lambda k, o, e: k[0] < 2 or near_circle(o, e, centre=(0.5, 0.6), radius=0.15,
                                                                      band=4.0 / (16 * 16))),
+    # the smallest and the largest block tiles of the kernels
+    "amr_block4": dict(base_mesh=(32, 32), block_size=(4, 4), periodic=(False, True), max_level=2,
+                       refine=lambda k, o, e: near_circle(o, e)),
+    "amr_block32": dict(base_mesh=(64, 64), block_size=(32, 32), periodic=(True, False), max_level=2,
+                        refine=lambda k, o, e: k[0] == 0 and k[1] == 0 or (k[0] == 1 and o[1] < 0.3)),
 }
 
 

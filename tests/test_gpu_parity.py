@@ -56,7 +56,7 @@ def kappa_for(pm, kind):
 
 
 CASES = [("uniform64", "one"), ("amr32", "one"), ("amr_mixed", "smooth"), ("pruned_tree", "smooth"),
-         ("periodic_tg", "one"), ("circle16", "smooth")]
+         ("periodic_tg", "one"), ("circle16", "smooth"), ("amr_block4", "smooth"), ("amr_block32", "smooth")]
 
 
 @pytest.fixture(scope="module", params=["fp64", "fp32_vcycle", "fp32_refine"])

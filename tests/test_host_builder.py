@@ -14,7 +14,8 @@ from cases import MESHES
 from oracle import OMesh, OSolver
 
 CASES = [("uniform64", "one"), ("periodic_tg", "one"), ("amr32", "one"), ("amr64", "one"),
-         ("amr_mixed", "smooth"), ("pruned_tree", "smooth"), ("circle16", "smooth")]
+         ("amr_mixed", "smooth"), ("pruned_tree", "smooth"), ("circle16", "smooth"), ("amr_block4", "smooth"),
+         ("amr_block32", "smooth")]
 
 
 def kappa(m, kind):

@@ -895,7 +895,9 @@ void build_ell(LevelData& L) {
           L.ell_pos[e] = int(at);
         }
       }
-      if (int(ext.size()) > nb) over[w] = 1;
+      // LDS slots of a block tile: up to max(nb, 32) external cells (4x4
+      // tiles at coarse-fine interfaces reach beyond their own 16 cells)
+      if (int(ext.size()) > std::max(nb, 32)) over[w] = 1;
       emax[w] = std::max(emax[w], int(ext.size()));
       next_t[w].push_back(int(ext.size()));
     }
