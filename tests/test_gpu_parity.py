@@ -476,6 +476,10 @@ def test_coarse_newton_schulz_update_matches_gauss_jordan(precision):
     k1 = kappa_for(pm, "smooth")
     x, y = pm.cell_centers()
     rhs = np.sin(3 * x) * np.cos(5 * y)
+    # small changes (Newton-Schulz steps), then a 1000:1 jump (the update
+    # diverges: Gauss-Jordan), then a small change from there
+    jump = np.where(x[pm.face_minus] < 0.4, 1.0, 1e-3) * k1
+    seq = [k1 * (1 + 0.01 * (r + 1) * np.cos(np.arange(pm.nfaces))) for r in range(3)] + [jump, jump * 1.01]
     out = []
     for exact in (True, False):
         if exact:
@@ -485,8 +489,8 @@ def test_coarse_newton_schulz_update_matches_gauss_jordan(precision):
         finally:
             os.environ.pop("TPAMR_COARSE_GJ", None)
         sols = []
-        for r in range(3):
-            s.operator.replace_face_coefficient(k1 * (1 + 0.01 * (r + 1) * np.cos(np.arange(pm.nfaces))))
+        for kk in seq:
+            s.operator.replace_face_coefficient(kk)
             s.refresh()
             p = np.zeros(pm.ncells)
             st = s.solve(p, rhs, rtol=1e-10)
