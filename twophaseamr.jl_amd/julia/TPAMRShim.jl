@@ -212,4 +212,44 @@ function project_pullback(ws::GpuCgWorkspace, ubar::Vector{Float64}, phi::Vector
     return ustar, db, dk, st[]
 end
 
+# Device-memory twins: a time loop whose fields live in HBM (AMDGPU.jl arrays,
+# passed as `pointer(a)`) runs every step without PCIe copies. Each call
+# returns when its results are written.
+
+"""
+    step_device!(ws, d_u, d_fraction, d_liquid, d_gas, d_offsets, d_kappa, d_jumps, d_phi, dt;
+                 density, gravity, tolerance = 1e-12, max_cycles = 200)
+
+One `gravity!` plus `project!` step on device pointers: mixture coefficients
+from the cell fraction (forces.jl:23-36,69-84,131) into `d_kappa`, the
+coefficient refresh (`_refresh_pressure_substep!`, two_phase/stepping.jl:73-111),
+gravity!'s jumps (forces.jl:528-580) into `d_jumps`, the force substep
+(`_apply_phase_forces!`, stepping.jl:181-201) and the projection
+(projection.jl:189-260) of `d_u` in place, the pressure increment into `d_phi`
+(C_NULL: not kept). `d_offsets` may be C_NULL. Returns the solve stats. The
+workspace needs its hierarchy first: one host-side refresh (`_sync!`, which the
+Krylov seam runs) of the same mesh.
+"""
+function step_device!(ws::GpuCgWorkspace, d_u::Ptr{Float64}, d_fraction::Ptr{Float64}, d_liquid::Ptr{Float64},
+        d_gas::Ptr{Float64}, d_offsets::Ptr{Float64}, d_kappa::Ptr{Float64}, d_jumps::Ptr{Float64},
+        d_phi::Ptr{Float64}, dt::Real; density::NTuple{2}, gravity::NTuple{2}, tolerance::Real = 1e-12,
+        max_cycles::Integer = 200)
+    _check(ws.ctx, ccall((:tpamr_mixture_face_coefficients_device, LIB), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Float64, Float64, Ptr{Float64}), ws.ctx, d_fraction, density[1], density[2],
+        d_kappa))
+    _check(ws.ctx, ccall((:tpamr_set_face_coefficients_device, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), ws.ctx,
+        d_kappa))
+    _check(ws.ctx, ccall((:tpamr_gravity_jumps_device, LIB), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Float64, Float64, Ptr{Float64}),
+        ws.ctx, d_offsets, d_liquid, d_gas, density[1], density[2], gravity[1], gravity[2], d_jumps))
+    _check(ws.ctx, ccall((:tpamr_apply_force_device, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Float64, Ptr{Float64}),
+        ws.ctx, d_jumps, Float64(dt), d_u))
+    st = Ref{TpamrStats}()
+    _check(ws.ctx, ccall((:tpamr_project_device, LIB), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Cint, Ptr{Float64}, Float64, Ptr{Float64},
+         Ref{TpamrStats}),
+        ws.ctx, d_u, C_NULL, Float64(dt), Float64(tolerance), Cint(max_cycles), C_NULL, Float64(dt), d_phi, st))
+    return st[]
+end
+
 end # module
