@@ -247,6 +247,25 @@ def trace_summary(db, title, command, top=30):
     print("\n".join(lines))
 
 
+def uniform_sweep(n, reps=50):
+    """Finest-level Jacobi sweep on a uniform n x n mesh (16x16 blocks, walls:
+    no irregular rows) in FP32 and FP64, beside a plain device copy: the
+    streaming rate the regular stencil path reaches. One JSON line."""
+    import torch
+    import tpamr
+    torch.cuda.init()  # torch's HIP runtime first (it finds no GPU once the library has opened it)
+    mesh = tpamr.BlockTreeMesh(base_mesh=(n, n), block_size=(16, 16), periodic=(False, False))
+    out = {"workload": f"uniform{n}x{n}_block16", "cells": mesh.ncells}
+    for prec in ("fp32_refine", "fp64"):
+        s = tpamr.EllipticSolver(tpamr.PoissonOperator(mesh, np.ones(mesh.nfaces)), precision=prec)
+        ms, nbytes = s.time_smoother(reps=reps)
+        out[prec] = {"sweep_us": ms * 1e3, "compulsory_bytes": nbytes, "GBps": nbytes / (ms * 1e-3) / 1e9,
+                     "irregular_nnz": s.level_info(0)["irregular_nnz"]}
+        s.close()
+    out["copy_GBps"] = copy_bandwidth()
+    print(json.dumps(out), flush=True)
+
+
 def copy_bandwidth(nbytes=1 << 30, reps=10):
     """Bytes read plus written per second (GB/s) of a plain device copy of
     `nbytes`, timed with HIP events on torch's stream: what a streaming kernel
@@ -406,12 +425,17 @@ def main():
     ap.add_argument("--workload", choices=("circles", "wave"), default="circles",
                     help="circles: the headline solve benchmark; wave: the checkpointed multi-step run")
     ap.add_argument("--wave-fd", action="store_true", help="wave: one finite-difference check of the gradient")
+    ap.add_argument("--uniform", type=int, metavar="N",
+                    help="time the finest-level sweep on a uniform N x N mesh of 16x16 blocks (no irregular rows)")
     args = ap.parse_args()
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary)
         return
     if args.trace_summary:
         trace_summary(*args.trace_summary)
+        return
+    if args.uniform:
+        uniform_sweep(args.uniform)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); this build has no multi-GPU path",
