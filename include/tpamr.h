@@ -5,7 +5,11 @@
  * and never retains a caller pointer. Host-pointer entry points accept plain
  * host arrays (Julia `Vector{Float64}` under GC.@preserve, numpy arrays).
  * The `_device` entry points take device pointers valid on the context's
- * device and run on the context's stream.
+ * device and run on the context's stream. That stream first waits for the
+ * work queued so far on the caller's stream (tpamr_set_caller_stream; the
+ * null stream by default), so inputs written there are complete before the
+ * library reads them; each `_device` call returns when its outputs are
+ * written.
  *
  * Cells are numbered in the reference's order (block-major, then j, then i;
  * src/mesh/topology/mesh.jl:28-30); interior and boundary faces in the
@@ -67,6 +71,12 @@ typedef struct {
 int tpamr_create(tpamr_ctx** out, int device, int precision);
 void tpamr_destroy(tpamr_ctx* ctx);
 const char* tpamr_last_error(const tpamr_ctx* ctx);
+/* The stream (a hipStream_t of the context's device; NULL = the null stream)
+ * on which the caller produces the inputs of the `_device` entry points:
+ * each of them orders the context's stream after the work queued on it at
+ * the time of the call. No reference counterpart: the reference's arrays
+ * live in host memory. */
+int tpamr_set_caller_stream(tpamr_ctx* ctx, void* stream);
 
 /* Block-quadtree mesh: BlockTreeMesh(base_mesh, block_size, origin, extent,
  * periodic, max_level, refine) (mesh.jl:1059-1121). `refined` lists

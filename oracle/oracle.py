@@ -246,13 +246,20 @@ class OSolver:
     geometric multigrid V-cycle (or plain CG with ``multigrid=False``)."""
 
     def __init__(self, mesh: OMesh, kappa=None, coarse_size=64, pre_sweeps=2, post_sweeps=2,
-                 max_levels=32, multigrid=True):
+                 max_levels=32, multigrid=True, volumes=None):
         self.mesh = mesh
         k = np.ones(mesh.nfaces) if kappa is None else np.ascontiguousarray(
             np.broadcast_to(np.asarray(kappa, np.float64), (mesh.nfaces,)), np.float64)
         out = C.c_void_p()
-        _check(lib().orc_solver_new(mesh._h, _dptr(k), coarse_size, pre_sweeps, post_sweeps,
-                                    max_levels, int(multigrid), C.byref(out)))
+        if volumes is None:
+            _check(lib().orc_solver_new(mesh._h, _dptr(k), coarse_size, pre_sweeps, post_sweeps,
+                                        max_levels, int(multigrid), C.byref(out)))
+        else:
+            v = np.ascontiguousarray(volumes, np.float64)
+            assert v.shape == (mesh.ncells,)
+            self._volumes = v
+            _check(lib().orc_solver_new_ex(mesh._h, _dptr(k), _dptr(v), coarse_size, pre_sweeps, post_sweeps,
+                                           max_levels, int(multigrid), C.byref(out)))
         self._h = out
         self.nlevels = lib().orc_solver_nlevels(self._h)
 
@@ -305,6 +312,8 @@ class OSolver:
         return z
 
     def solve(self, rhs, x0=None, rtol=1e-10, atol=0.0, max_cycles=100, compatibility="project"):
+        if compatibility not in ("project", "error"):
+            raise ValueError("compatibility must be :project or :error")
         x = np.zeros(self.mesh.ncells) if x0 is None else np.array(x0, np.float64)
         st = Stats()
         rm = np.zeros(max(self.mesh.ncells, 1))
@@ -312,6 +321,16 @@ class OSolver:
                                C.c_double(rtol), C.c_double(atol), int(max_cycles),
                                int(compatibility == "error"), C.byref(st), _dptr(rm)))
         return x, st.as_dict()
+
+    def floor(self, x, b_integrated, assembled=False):
+        """`_roundoff_residual_floor!` of iterate x with the integrated rhs:
+        the factorized action's (cycles.jl:108-179) or, with `assembled`, the
+        assembled matrix's (cycles.jl:80-101)."""
+        out = C.c_double()
+        _check(lib().orc_solver_floor(self._h, _dptr(np.ascontiguousarray(x, np.float64)),
+                                      _dptr(np.ascontiguousarray(b_integrated, np.float64)), int(assembled),
+                                      C.byref(out)))
+        return out.value
 
     def weighted_gradient(self, p):
         g = np.zeros(self.mesh.nfaces)
@@ -349,6 +368,44 @@ class OSolver:
         if getattr(self, "_h", None):
             lib().orc_solver_free(self._h)
             self._h = None
+
+
+def _csc(matrix):
+    import scipy.sparse as sp
+    A = sp.csc_matrix(np.asarray(matrix, np.float64))
+    return (A.shape[0], np.ascontiguousarray(A.indptr, np.int32), np.ascontiguousarray(A.indices, np.int32),
+            np.ascontiguousarray(A.data, np.float64))
+
+
+def assembled_floor(matrix, volumes, x, b, single=False):
+    """assembled `_roundoff_residual_floor!` (cycles.jl:80-101) of a dense matrix."""
+    n, colptr, rowval, nz = _csc(matrix)
+    out = C.c_double()
+    _check(lib().orc_assembled_floor(int(single), n, _iptr(colptr), _iptr(rowval), _dptr(nz),
+                                     _dptr(np.ascontiguousarray(volumes, np.float64)),
+                                     _dptr(np.ascontiguousarray(x, np.float64)),
+                                     _dptr(np.ascontiguousarray(b, np.float64)), C.byref(out)))
+    return out.value
+
+
+def scripted_solve(matrix, volumes, b, iterates, *, rtol, atol=0.0, max_cycles=100, relative_to_rhs=False,
+                   x0=None, single=False):
+    """`_solve_krylov_with_stats!` with a scripted Krylov method (the
+    reference's mock workspaces, test/solvers.jl:698-751): Krylov run k leaves
+    iterates[min(k, len)] as its solution and reports one iteration. `matrix`
+    is a dense array (stored as CSC), `b` the integrated right-hand side;
+    `single` runs the driver in Float32. Returns (x, stats, runs)."""
+    n, colptr, rowval, nz = _csc(matrix)
+    V = np.ascontiguousarray(volumes, np.float64)
+    bb = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64)
+    its = np.ascontiguousarray(np.asarray(iterates, np.float64).reshape(-1, n))
+    st = Stats()
+    passes = C.c_int(0)
+    _check(lib().orc_scripted_solve(int(single), n, _iptr(colptr), _iptr(rowval), _dptr(nz), _dptr(V), _dptr(bb),
+                                    _dptr(x), C.c_double(rtol), C.c_double(atol), int(max_cycles),
+                                    int(relative_to_rhs), its.shape[0], _dptr(its), C.byref(st), C.byref(passes)))
+    return x, st.as_dict(), passes.value
 
 
 def techref_noise(n):

@@ -9,7 +9,13 @@
 
 namespace orc {
 Solver* build_solver(const Mesh& fine, const std::vector<double>& kappa, int coarse_size, int pre,
-                     int post, int max_levels, bool use_mg);
+                     int post, int max_levels, bool use_mg, const double* volumes);
+double solver_floor(Solver& S, const double* x, const double* b, bool assembled);
+double csc_floor(bool single, int n, const int* colptr, const int* rowval, const double* nzval,
+                 const double* volumes, const double* x, const double* b);
+Stats scripted_solve(bool single, int n, const int* colptr, const int* rowval, const double* nzval,
+                     const double* volumes, const double* b, double* x, double rtol, double atol, int max_cycles,
+                     bool relative_to_rhs, int npasses, const double* iterates, int* passes);
 void apply_vcycle(Solver& S, const double* r, double* z);
 Stats solve(Solver& S, const double* rhs, double* xio, double rtol, double atol, int max_cycles,
             int compat_error, std::vector<double>* removed_means);
@@ -138,7 +144,48 @@ int orc_solver_new(void* mp, const double* kappa, int coarse_size, int pre, int 
   GUARD({
     Mesh* m = static_cast<Mesh*>(mp);
     std::vector<double> k(kappa, kappa + m->faces.size());
-    *out = build_solver(*m, k, coarse_size, pre, post, max_levels, use_mg != 0);
+    *out = build_solver(*m, k, coarse_size, pre, post, max_levels, use_mg != 0, nullptr);
+    return 0;
+  })
+}
+
+// the same with per-cell volumes of the fine operator (NULL: geometric)
+int orc_solver_new_ex(void* mp, const double* kappa, const double* volumes, int coarse_size, int pre, int post,
+                      int max_levels, int use_mg, void** out) {
+  GUARD({
+    Mesh* m = static_cast<Mesh*>(mp);
+    std::vector<double> k(kappa, kappa + m->faces.size());
+    *out = build_solver(*m, k, coarse_size, pre, post, max_levels, use_mg != 0, volumes);
+    return 0;
+  })
+}
+
+// round-off residual floor of iterate x with integrated rhs b: the
+// factorized operator's (assembled = 0) or the assembled matrix's
+int orc_solver_floor(void* sp, const double* x, const double* b, int assembled, double* out) {
+  GUARD({
+    *out = solver_floor(*static_cast<Solver*>(sp), x, b, assembled != 0);
+    return 0;
+  })
+}
+
+// assembled `_roundoff_residual_floor!` of a CSC matrix (single = 1: Float32)
+int orc_assembled_floor(int single, int n, const int* colptr, const int* rowval, const double* nzval,
+                        const double* volumes, const double* x, const double* b, double* out) {
+  GUARD({
+    *out = csc_floor(single != 0, n, colptr, rowval, nzval, volumes, x, b);
+    return 0;
+  })
+}
+
+// the solve driver with scripted Krylov iterates on an assembled CSC matrix
+// (single = 1: Float32 arithmetic); *passes receives the Krylov runs made
+int orc_scripted_solve(int single, int n, const int* colptr, const int* rowval, const double* nzval,
+                       const double* volumes, const double* b, double* x, double rtol, double atol, int max_cycles,
+                       int relative_to_rhs, int npasses, const double* iterates, Stats* st, int* passes) {
+  GUARD({
+    *st = scripted_solve(single != 0, n, colptr, rowval, nzval, volumes, b, x, rtol, atol, max_cycles,
+                         relative_to_rhs != 0, npasses, iterates, passes);
     return 0;
   })
 }

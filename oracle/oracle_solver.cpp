@@ -145,8 +145,11 @@ static Components components(const Mesh& M, const std::vector<double>& kappa,
 }
 
 // EllipticOperator / PoissonOperator (elliptic.jl:683-800, 969-980) with
-// reaction 0, diffusion 1, all cells active, no open sides.
-static std::unique_ptr<Operator> make_operator(const Mesh* mesh, const std::vector<double>& kappa) {
+// reaction 0, diffusion 1, all cells active, no open sides. `volumes`
+// (NULL: the cells' geometric volumes) is the operator's `volumes` keyword
+// (elliptic.jl:683-700); the stiffness does not depend on it.
+static std::unique_ptr<Operator> make_operator(const Mesh* mesh, const std::vector<double>& kappa,
+                                               const double* volumes = nullptr) {
   auto op = std::make_unique<Operator>();
   op->mesh = mesh;
   if (kappa.size() != mesh->faces.size())
@@ -161,7 +164,9 @@ static std::unique_ptr<Operator> make_operator(const Mesh* mesh, const std::vect
   op->K = inverse_metric(*mesh, &op->K_diagonal);
   op->volumes.resize(mesh->ncells());
   const int nb = mesh->bs[0] * mesh->bs[1];
-  for (int c = 0; c < mesh->ncells(); ++c) op->volumes[c] = mesh->blocks[c / nb].volume;
+  for (int c = 0; c < mesh->ncells(); ++c) op->volumes[c] = volumes ? volumes[c] : mesh->blocks[c / nb].volume;
+  for (double v : op->volumes)
+    if (!(std::isfinite(v) && v >= 0)) throw std::invalid_argument("cell volumes must be finite and nonnegative");
   op->A = integrated_stiffness(*mesh, op->K, op->K_diagonal, op->kappa, op->sqrt_kappa);
   op->comp = components(*mesh, op->kappa, op->volumes);
   op->singular = !op->comp.anchors.empty();
@@ -461,14 +466,16 @@ static void lu_solve(const std::vector<double>& a, const std::vector<int>& piv, 
 // reuse, no geometry), geometric_preconditioner.jl:171-442, with
 // `_tree_next_coarse_mesh!` (tree_coarsening.jl:187-212).
 Solver* build_solver(const Mesh& fine, const std::vector<double>& kappa, int coarse_size, int pre,
-                     int post, int max_levels, bool use_mg) {
+                     int post, int max_levels, bool use_mg, const double* volumes) {
   auto S = new Solver();
   S->pre = pre;
   S->post = post;
   S->coarse_size = coarse_size;
   S->max_levels = max_levels;
   S->use_mg = use_mg;
-  S->ops.push_back(make_operator(&fine, kappa));
+  // coarse levels keep their geometric volumes (`_cell_volumes` of each
+  // coarse mesh, geometric_preconditioner.jl:75-105)
+  S->ops.push_back(make_operator(&fine, kappa, volumes));
   std::vector<CSC> prolongations;
   if (use_mg) {
     int active_level = fine.max_leaf_level();
@@ -640,28 +647,48 @@ void apply_vcycle(Solver& S, const double* r, double* z) {
 // cycles.jl:80-260. Krylov.jl 0.10 `cg!` (third-party; parity unpinned
 // beyond iteration counts) is restated from its published algorithm.
 
-static double strong_max(const std::vector<double>& r, const std::vector<double>& V) {
-  double m = 0;
+// `_strong_residual_maximum` / `_strong_cells_maximum` (cycles.jl:226-260):
+// max |r_i| / V_i over cells of positive volume; a NaN term is returned.
+template <typename T>
+static T strong_max(const std::vector<T>& r, const std::vector<T>& V) {
+  T m = 0;
   for (size_t i = 0; i < r.size(); ++i) {
     if (!(V[i] > 0)) continue;
-    double q = std::fabs(r[i]) / V[i];
+    T q = std::fabs(r[i]) / V[i];
     if (std::isnan(q)) return q;
     m = std::max(m, q);
   }
   return m;
 }
 
-static void residual_norms(const std::vector<double>& r, const std::vector<double>& V, double& l2,
-                           double& linf) {
-  double sq = 0, tv = 0;
+// `_integrated_residual_norms` (cycles.jl:181-196)
+template <typename T>
+static void residual_norms(const std::vector<T>& r, const std::vector<T>& V, T& l2, T& linf) {
+  T sq = 0, tv = 0;
   for (size_t i = 0; i < r.size(); ++i) {
     if (!(V[i] > 0)) continue;
-    const double s = r[i] / V[i];
+    const T s = r[i] / V[i];
     sq += V[i] * s * s;
     tv += V[i];
   }
   l2 = std::sqrt(sq / tv);
   linf = strong_max(r, V);
+}
+
+// assembled `_roundoff_residual_floor!` (cycles.jl:80-101):
+// 128 eps max(1, max_i (|A||x|)_i / V_i, max_i |b_i| / V_i)
+template <typename T>
+static T assembled_floor(int n, const int* colptr, const int* rowval, const T* nzval, const std::vector<T>& x,
+                         const std::vector<T>& b, const std::vector<T>& V) {
+  std::vector<T> t(n, T(0));
+  for (int col = 0; col < n; ++col)
+    for (int p = colptr[col]; p < colptr[col + 1]; ++p) t[rowval[p]] += std::fabs(nzval[p]) * std::fabs(x[col]);
+  T scale = 1;
+  for (int i = 0; i < n; ++i) {
+    if (!(V[i] > 0)) continue;
+    scale = std::max(scale, std::max(t[i] / V[i], std::fabs(b[i] / V[i])));
+  }
+  return T(128) * std::numeric_limits<T>::epsilon() * scale;
 }
 
 // factorized `_roundoff_residual_floor!`, cycles.jl:108-179
@@ -692,6 +719,16 @@ static double roundoff_floor(const Operator& op, const std::vector<double>& x,
     scale = std::max(scale, std::max(bound[c] / V, std::fabs(b[c]) / V));
   }
   return 128 * EPS * std::max(1.0, scale);
+}
+
+double solver_floor(Solver& S, const double* x, const double* b, bool assembled) {
+  const Operator& op = *S.ops[0];
+  const size_t n = op.volumes.size();
+  std::vector<double> xv(x, x + n), bv(b, b + n);
+  if (assembled)
+    return assembled_floor<double>(int(n), op.A.colptr.data(), op.A.rowval.data(), op.A.nzval.data(), xv, bv,
+                                   op.volumes);
+  return roundoff_floor(op, xv, bv);
 }
 
 static void remove_volume_means(const Operator& op, std::vector<double>& x) {
@@ -769,13 +806,147 @@ static int krylov_cg(Solver& S, const std::vector<double>& b, std::vector<double
   return iter;
 }
 
-static double guarded_target(const Operator& op, const std::vector<double>& x,
-                             const std::vector<double>& b, double target, double initial,
-                             double current) {
+// `_guarded_roundoff_target!` (elliptic_solver.jl:454-473): the iterate's
+// floor replaces the target only after progress (target < current <
+// initial) and only while it is below initial / 100.
+template <typename T, class Floor>
+static T guarded_target(Floor&& floor, T target, T initial, T current) {
   if (!(std::isfinite(initial) && std::isfinite(current) && target < current && current < initial))
     return target;
-  const double fl = roundoff_floor(op, x, b);
+  const T fl = floor();
   return (std::isfinite(fl) && fl < initial / 100) ? std::max(target, fl) : target;
+}
+
+template <typename T>
+struct DriverStats {
+  int cycles;
+  bool converged;
+  T initial_l2, initial_linf, final_l2, final_linf, target_linf, residual_scale;
+};
+
+// `_solve_krylov_with_stats!` (elliptic_solver.jl:550-612) with
+// `_run_krylov_reliably!` (:487-526), independent of the operator and the
+// Krylov method. `b` is the integrated (compatible) right-hand side and `x`
+// the prepared initial iterate; `mul(x, y)` is the action y = A x,
+// `floor(x)` the round-off floor of the iterate, `krylov(x, target, itmax)`
+// one Krylov run that leaves Krylov.solution in `x` and returns
+// Krylov.iteration_count, `gauge(x)` the component mean removal.
+// `relative_to_rhs` selects the shifted-diffusion scale max |b_i| / V_i (atol
+// unused) instead of the initial residual.
+template <typename T, class Mul, class Floor, class Krylov, class Gauge>
+static DriverStats<T> krylov_driver(const std::vector<T>& V, const std::vector<T>& b, std::vector<T>& x, T rtol,
+                                    T atol, int max_cycles, bool relative_to_rhs, Mul&& mul, Floor&& floor,
+                                    Krylov&& krylov, Gauge&& gauge) {
+  const size_t n = V.size();
+  const T eps = std::numeric_limits<T>::epsilon();
+  std::vector<T> prod(n), res(n);
+  auto true_residual = [&]() {
+    mul(x, prod);
+    for (size_t i = 0; i < n; ++i) res[i] = b[i] - prod[i];
+  };
+  true_residual();
+  DriverStats<T> st{};
+  residual_norms(res, V, st.initial_l2, st.initial_linf);
+  const T ilinf = st.initial_linf;
+  const T floor0 = floor(x);
+  T target, scale;
+  if (relative_to_rhs) {
+    scale = strong_max(b, V);
+    target = std::max(rtol * scale, floor0);
+  } else {
+    scale = std::max(ilinf, eps);
+    target = std::max(atol, std::max(rtol * std::max(ilinf, eps), floor0));
+  }
+  int iterations = 0;
+  if (ilinf > target) {
+    int total = 0;
+    while (total < max_cycles) {
+      const int it = krylov(x, target, max_cycles - total);
+      total += it;
+      gauge(x);
+      true_residual();
+      const T final_linf = strong_max(res, V);
+      const T eff = guarded_target([&] { return floor(x); }, target, ilinf, final_linf);
+      if (final_linf <= eff) break;
+      if (!std::isfinite(final_linf) || it == 0) break;
+    }
+    iterations = total;
+  }
+  gauge(x);
+  true_residual();
+  residual_norms(res, V, st.final_l2, st.final_linf);
+  target = guarded_target([&] { return floor(x); }, target, ilinf, st.final_linf);
+  st.cycles = iterations;
+  st.converged = std::isfinite(st.final_linf) && std::isfinite(target) && st.final_linf <= target;
+  st.target_linf = target;
+  st.residual_scale = scale;
+  return st;
+}
+
+// The driver above with a scripted Krylov method, as the reference's mock
+// workspaces (test/solvers.jl:698-751): run k (1-based) leaves
+// iterates[min(k, npasses)] as its solution and reports one iteration. The
+// operator is an assembled CSC matrix with its assembled floor.
+template <typename T>
+static DriverStats<T> scripted_driver(int n, const int* colptr, const int* rowval, const std::vector<T>& nz,
+                                      const std::vector<T>& V, const std::vector<T>& b, std::vector<T>& x, T rtol,
+                                      T atol, int max_cycles, bool relative_to_rhs, int npasses,
+                                      const std::vector<T>& iterates, int* passes) {
+  int pass = 0;
+  auto mul = [&](const std::vector<T>& in, std::vector<T>& out) {
+    std::fill(out.begin(), out.end(), T(0));
+    for (int col = 0; col < n; ++col)
+      for (int p = colptr[col]; p < colptr[col + 1]; ++p) out[rowval[p]] += nz[p] * in[col];
+  };
+  auto floor = [&](const std::vector<T>& in) { return assembled_floor<T>(n, colptr, rowval, nz.data(), in, b, V); };
+  auto krylov = [&](std::vector<T>& io, T, int) {
+    ++pass;
+    const int k = std::min(pass, npasses) - 1;
+    std::copy(iterates.begin() + size_t(k) * n, iterates.begin() + size_t(k + 1) * n, io.begin());
+    return 1;
+  };
+  auto gauge = [](std::vector<T>&) {};
+  auto st = krylov_driver<T>(V, b, x, rtol, atol, max_cycles, relative_to_rhs, mul, floor, krylov, gauge);
+  *passes = pass;
+  return st;
+}
+
+double csc_floor(bool single, int n, const int* colptr, const int* rowval, const double* nzval,
+                 const double* volumes, const double* x, const double* b) {
+  auto run = [&](auto zero) {
+    using T = decltype(zero);
+    std::vector<T> nz(nzval, nzval + colptr[n]), V(volumes, volumes + n), xv(x, x + n), bv(b, b + n);
+    return double(assembled_floor<T>(n, colptr, rowval, nz.data(), xv, bv, V));
+  };
+  return single ? run(0.0f) : run(0.0);
+}
+
+Stats scripted_solve(bool single, int n, const int* colptr, const int* rowval, const double* nzval,
+                     const double* volumes, const double* b, double* x, double rtol, double atol, int max_cycles,
+                     bool relative_to_rhs, int npasses, const double* iterates, int* passes) {
+  if (npasses < 1) throw std::invalid_argument("at least one scripted iterate is required");
+  auto run = [&](auto zero) {
+    using T = decltype(zero);
+    const int nnz = colptr[n];
+    std::vector<T> nz(nzval, nzval + nnz), V(volumes, volumes + n), bv(b, b + n), xv(x, x + n),
+        it(iterates, iterates + size_t(npasses) * n);
+    auto d = scripted_driver<T>(n, colptr, rowval, nz, V, bv, xv, T(rtol), T(atol), max_cycles, relative_to_rhs,
+                                npasses, it, passes);
+    std::copy(xv.begin(), xv.end(), x);
+    Stats st{};
+    st.cycles = d.cycles;
+    st.converged = d.converged;
+    st.compatible = 1;
+    st.initial_l2 = d.initial_l2;
+    st.initial_linf = d.initial_linf;
+    st.final_l2 = d.final_l2;
+    st.final_linf = d.final_linf;
+    st.target_linf = d.target_linf;
+    st.residual_scale = d.residual_scale;
+    st.removed_rhs_mean = 0;
+    return st;
+  };
+  return single ? run(0.0f) : run(0.0);
 }
 
 // `solve!` (elliptic_solver.jl:628-675) with `_solve_krylov_with_stats!`
@@ -795,7 +966,6 @@ Stats solve(Solver& S, const double* rhs, double* xio, double rtol, double atol,
       b[i] *= V[i];
     }
   }
-  Stats st{};
   double removed = 0;
   if (op.singular) {
     const Components& C = op.comp;
@@ -824,47 +994,24 @@ Stats solve(Solver& S, const double* rhs, double* xio, double rtol, double atol,
     removed = total_vol == 0 ? 0 : total_sum / total_vol;
     remove_volume_means(op, x);
   }
-  std::vector<double> prod(n), res(n);
-  auto true_residual = [&]() {
-    factorized_mul(op, x.data(), prod.data());
-    for (size_t i = 0; i < n; ++i) res[i] = b[i] - prod[i];
+  auto mul = [&](const std::vector<double>& in, std::vector<double>& out) { factorized_mul(op, in.data(), out.data()); };
+  auto floor = [&](const std::vector<double>& in) { return roundoff_floor(op, in, b); };
+  auto krylov = [&](std::vector<double>& io, double target, int itmax) { return krylov_cg(S, b, io, target, itmax); };
+  auto gauge = [&](std::vector<double>& io) {
+    if (op.singular) remove_volume_means(op, io);
   };
-  true_residual();
-  double il2, ilinf;
-  residual_norms(res, V, il2, ilinf);
-  const double floor0 = roundoff_floor(op, x, b);
-  double target = std::max(atol, std::max(rtol * std::max(ilinf, EPS), floor0));
-  int iterations = 0;
-  if (ilinf > target) {
-    int total = 0;
-    double final_linf = std::numeric_limits<double>::infinity();
-    while (total < max_cycles) {
-      const int it = krylov_cg(S, b, x, target, max_cycles - total);
-      total += it;
-      if (op.singular) remove_volume_means(op, x);
-      true_residual();
-      final_linf = strong_max(res, V);
-      const double eff = guarded_target(op, x, b, target, ilinf, final_linf);
-      if (final_linf <= eff) break;
-      if (!std::isfinite(final_linf) || it == 0) break;
-    }
-    iterations = total;
-  }
-  if (op.singular) remove_volume_means(op, x);
-  true_residual();
-  double fl2, flinf;
-  residual_norms(res, V, fl2, flinf);
-  target = guarded_target(op, x, b, target, ilinf, flinf);
+  const auto d = krylov_driver<double>(V, b, x, rtol, atol, max_cycles, false, mul, floor, krylov, gauge);
   std::copy(x.begin(), x.end(), xio);
-  st.cycles = iterations;
-  st.converged = std::isfinite(flinf) && std::isfinite(target) && flinf <= target;
+  Stats st{};
+  st.cycles = d.cycles;
+  st.converged = d.converged;
   st.compatible = 1;
-  st.initial_l2 = il2;
-  st.initial_linf = ilinf;
-  st.final_l2 = fl2;
-  st.final_linf = flinf;
-  st.target_linf = target;
-  st.residual_scale = std::max(ilinf, EPS);
+  st.initial_l2 = d.initial_l2;
+  st.initial_linf = d.initial_linf;
+  st.final_l2 = d.final_l2;
+  st.final_linf = d.final_linf;
+  st.target_linf = d.target_linf;
+  st.residual_scale = d.residual_scale;
   st.removed_rhs_mean = removed;
   return st;
 }

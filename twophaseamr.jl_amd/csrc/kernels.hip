@@ -701,12 +701,14 @@ __global__ void k_face_scale(int nf, const double* __restrict__ sqk, const doubl
 __global__ void k_mixture(int nf, int nb_cells, const int* __restrict__ fm, const int* __restrict__ fp,
                           const signed char* __restrict__ axis, const double* __restrict__ spacing,
                           const double* __restrict__ c, double rho_l, double rho_g, double* __restrict__ kappa) {
+#pragma clang fp contract(off)
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
   const int m = fm[f], p = fp[f], a = axis[f];
   const double hm = spacing[2 * (m / nb_cells) + a], hp = spacing[2 * (p / nb_cells) + a];
   double cf = (hm * c[m] + hp * c[p]) / (hm + hp);
-  cf = fmin(fmax(cf, 0.0), 1.0);
+  // Julia's clamp: a NaN fraction stays NaN
+  cf = cf > 1.0 ? 1.0 : (cf < 0.0 ? 0.0 : cf);
   kappa[f] = 1.0 / (rho_l * cf + rho_g * (1 - cf));
 }
 

@@ -186,6 +186,11 @@ struct tpamr_ctx {
   // (TPAMR_PIPE in the environment, 2..PIPE_MAX)
   int pipe = 3;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evs[PIPE_MAX] = {};
+  // the `_device` entry points wait for the work queued on the caller's
+  // stream (tpamr_set_caller_stream; default the null stream) before they
+  // touch its buffers
+  hipStream_t caller = nullptr;
+  hipEvent_t ev_caller = nullptr;
   const TreeMesh& mesh() const {
     if (H) return *H->levels[0]->mesh;
     if (mesh_pending) return *mesh_pending;
@@ -235,8 +240,30 @@ void need_device(const tpamr_ctx* c) {
   if (c->device < 0) throw StateError("host-only context: create the context on a HIP device to compute");
 }
 
+// Order the context stream after everything queued so far on the caller's
+// stream, so device inputs written there are complete when the library
+// reads them.
+void order_after_caller(tpamr_ctx* c) {
+  HIPCHECK(hipEventRecord(c->ev_caller, c->caller));
+  HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_caller, 0));
+}
+
 void need_solver(const tpamr_ctx* c) {
   if (!c->H || !c->have_coeff) throw StateError("no operator: call tpamr_set_face_coefficients first");
+}
+
+void destroy_graph(tpamr_ctx* c);
+
+// A new mesh or hierarchy invalidates everything derived from the old one's
+// topology: the captured iteration, the device refresh plan (face slots,
+// irregular-row terms, level sizes) and the previous coarse inverse that
+// starts a Newton-Schulz update.
+void reset_topology_state(tpamr_ctx* c) {
+  destroy_graph(c);
+  c->plan.clear();
+  c->plan_ready = false;
+  c->coarse_x_valid = false;
+  c->host_stale = false;
 }
 
 void destroy_graph(tpamr_ctx* c) {
@@ -307,6 +334,7 @@ void upload_hierarchy(tpamr_ctx* c, bool topology) {
   hipStream_t s = c->stream;
   const int nl = int(H.levels.size());
   if (topology) {
+    reset_topology_state(c);
     c->topo.clear();
     c->topo.resize(nl);
     for (int l = 0; l < nl; ++l) {
@@ -742,7 +770,10 @@ tpamr_stats solve_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double rtol, 
   }
   const double scale = sums[0], total_sum = sums[1];
   const bool zero_start = sums[3] == 0.0;
-  if (!std::isfinite(total_sum)) throw std::invalid_argument("right-hand side must be finite");
+  // sums of |b| and |x| are finite exactly when every entry is (device
+  // inputs are not scanned on the host)
+  if (!std::isfinite(scale) || !std::isfinite(total_sum)) throw std::invalid_argument("right-hand side must be finite");
+  if (!std::isfinite(sums[3])) throw std::invalid_argument("initial solution must be finite");
   if (compat == 1 && std::fabs(total_sum) > 256 * EPS * std::max(1.0, scale))
     throw IncompatibleError("Poisson right-hand side has nonzero volume mean on component rooted at cell 1");
   const double mean = total_sum / c->H->total_volume;
@@ -1078,6 +1109,8 @@ void device_refresh(tpamr_ctx* c, const double* kappa, bool kappa_on_device = fa
   hipStream_t s = c->stream;
   const int nl = int(H.levels.size());
   if (!c->plan_ready) upload_plan(c);
+  if (int(c->plan.size()) != nl || c->plan[0].nfaces != c->mesh().nfaces() || int(c->topo.size()) != nl)
+    throw StateError("device refresh plan does not match the current hierarchy");
   DevPlan& P0 = c->plan[0];
   if (kappa_on_device)
     HIPCHECK(hipMemcpyAsync(P0.kappa.p, kappa, sizeof(double) * P0.nfaces, hipMemcpyDeviceToDevice, s));
@@ -1214,6 +1247,7 @@ int tpamr_create(tpamr_ctx** out, int device, int precision) {
       HIPCHECK(hipEventCreate(&c->ev0));
       HIPCHECK(hipEventCreate(&c->ev1));
       for (int q = 0; q < PIPE_MAX; ++q) HIPCHECK(hipEventCreateWithFlags(&c->evs[q], hipEventDisableTiming));
+      HIPCHECK(hipEventCreateWithFlags(&c->ev_caller, hipEventDisableTiming));
     });
     if (rc != TPAMR_OK) {
       // keep the context so the caller can read the message
@@ -1238,7 +1272,7 @@ void tpamr_destroy(tpamr_ctx* c) {
     if (c->st) (void)hipFree(c->st);
     if (c->st_host) (void)hipHostFree(c->st_host);
     if (c->st_ring) (void)hipHostFree(c->st_ring);
-    for (hipEvent_t e : {c->ev0, c->ev1})
+    for (hipEvent_t e : {c->ev0, c->ev1, c->ev_caller})
       if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->evs)
       if (e) (void)hipEventDestroy(e);
@@ -1248,6 +1282,14 @@ void tpamr_destroy(tpamr_ctx* c) {
 }
 
 const char* tpamr_last_error(const tpamr_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int tpamr_set_caller_stream(tpamr_ctx* c, void* stream) {
+  if (!c) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    c->caller = static_cast<hipStream_t>(stream);
+  });
+}
 
 int tpamr_set_mesh(tpamr_ctx* c, const int base_mesh[2], const int block_size[2], const double origin[2],
                    const double extent[2], const int periodic[2], int max_level, int n_refined, const int* refined) {
@@ -1264,7 +1306,7 @@ int tpamr_set_mesh(tpamr_ctx* c, const int base_mesh[2], const int block_size[2]
     for (int q = 0; q < n_refined; ++q) keys[q] = {refined[3 * q], refined[3 * q + 1], refined[3 * q + 2]};
     const bool per[2] = {periodic[0] != 0, periodic[1] != 0};
     if (c->device >= 0) HIPCHECK(hipSetDevice(c->device));
-    destroy_graph(c);
+    reset_topology_state(c);
     c->H.reset();
     c->have_coeff = false;
     c->mesh_pending = make_tree_mesh(base_mesh, block_size, origin, extent, per, max_level, keys);
@@ -1340,6 +1382,7 @@ int tpamr_set_multigrid(tpamr_ctx* c, const tpamr_multigrid_options* o) {
     destroy_graph(c);
     if (c->H && topo_change) {
       // rebuild the hierarchy at the next coefficient update
+      reset_topology_state(c);
       c->mesh_pending = std::move(c->H->levels[0]->owned);
       c->H.reset();
       c->have_coeff = false;
@@ -1386,6 +1429,7 @@ int tpamr_set_face_coefficients_device(tpamr_ctx* c, const double* d_kappa) {
     if (!c->H || c->topo.empty())
       throw StateError("no hierarchy: call tpamr_set_face_coefficients once before device refreshes");
     HIPCHECK(hipSetDevice(c->device));
+    order_after_caller(c);
     device_refresh(c, d_kappa, true);
   });
 }
@@ -1505,6 +1549,7 @@ int tpamr_solve_device(tpamr_ctx* c, const double* d_rhs, double* d_x, double rt
     need_device(c);
     need_solver(c);
     HIPCHECK(hipSetDevice(c->device));
+    order_after_caller(c);
     const int N = c->topo[0].ncells;
     HIPCHECK(hipMemcpyAsync(c->b.p, d_rhs, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     HIPCHECK(hipMemcpyAsync(c->x.p, d_x, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
@@ -1577,6 +1622,7 @@ int tpamr_apply_device(tpamr_ctx* c, const double* d_x, double* d_y) {
     need_device(c);
     need_solver(c);
     HIPCHECK(hipSetDevice(c->device));
+    order_after_caller(c);
     // the tile kernels move four cells per 16-byte (FP32) or 2x16-byte
     // (FP64) access: misaligned caller buffers go through library buffers
     const bool aligned = ((reinterpret_cast<uintptr_t>(d_x) | reinterpret_cast<uintptr_t>(d_y)) & 31u) == 0;
@@ -1866,6 +1912,7 @@ int tpamr_project_device(tpamr_ctx* c, double* d_u, double* d_p, double dt, doub
     need_solver(c);
     project_checks(dt, tol, reference_dt);
     HIPCHECK(hipSetDevice(c->device));
+    order_after_caller(c);
     const TreeMesh& M = c->mesh();
     const int N = M.ncells(), F = M.nfaces();
     hipStream_t s = c->stream;
@@ -1887,6 +1934,7 @@ int tpamr_apply_force_device(tpamr_ctx* c, const double* d_jumps, double dt, dou
     need_solver(c);
     if (!std::isfinite(dt)) throw std::invalid_argument("dt must be finite");
     HIPCHECK(hipSetDevice(c->device));
+    order_after_caller(c);
     const int F = c->mesh().nfaces();
     launch_face_jump(c->stream, F, c->fm.p, c->fp.p, c->farea.p, c->sqk.p, nullptr, d_jumps, c->f1.p);
     weighted_metric(c, c->f1.p, c->f2.p);
@@ -1904,6 +1952,7 @@ int tpamr_mixture_face_coefficients_device(tpamr_ctx* c, const double* d_fractio
     if (!(std::isfinite(rl) && rl > 0 && std::isfinite(rg) && rg > 0))
       throw std::invalid_argument("phase densities must be finite and positive");
     HIPCHECK(hipSetDevice(c->device));
+    order_after_caller(c);
     const TreeMesh& M = c->mesh();
     launch_mixture(c->stream, M.nfaces(), M.nb(), c->fm.p, c->fp.p, c->faxis.p, c->spacing.p, d_fraction, rl, rg,
                    d_kappa);
@@ -1919,6 +1968,7 @@ int tpamr_gravity_jumps_device(tpamr_ctx* c, const double* d_offsets, const doub
     if (!c->H) throw StateError("no hierarchy: call tpamr_set_face_coefficients first");
     gravity_checks(c, rho_liquid, rho_gas, gx, gy);
     HIPCHECK(hipSetDevice(c->device));
+    order_after_caller(c);
     launch_gravity(c, d_offsets, d_liquid, d_gas, rho_liquid, rho_gas, gx, gy, d_jumps);
     HIPCHECK(hipStreamSynchronize(c->stream));
   });
@@ -1930,6 +1980,7 @@ int tpamr_divergence_device(tpamr_ctx* c, const double* d_u, double* d_div) {
     need_device(c);
     if (!c->H) throw StateError("no hierarchy: call tpamr_set_face_coefficients first");
     HIPCHECK(hipSetDevice(c->device));
+    order_after_caller(c);
     const TreeMesh& M = c->mesh();
     launch_divergence(c->stream, M.ncells(), M.nb(), c->cptr.p, c->cface.p, c->csign.p, c->farea.p, d_u, c->bptr.p,
                       c->bface.p, c->bside.p, c->barea.p, nullptr, c->topo[0].volume.p, d_div);
@@ -2069,6 +2120,7 @@ int tpamr_project_adjoint_device(tpamr_ctx* c, const double* d_ubar, const doubl
     need_device(c);
     need_solver(c);
     HIPCHECK(hipSetDevice(c->device));
+    order_after_caller(c);
     tpamr_stats st = project_adjoint_core(c, d_ubar, d_phi, d_jumps, dt, rtol, max_cycles, d_ubar_star, d_dJ_drhs,
                                           d_dJ_dkappa, false);
     if (stats) *stats = st;

@@ -6,15 +6,21 @@ Julia side of the MI355X pressure solver: a Krylov workspace that TwoPhaseAMR.jl
 whose `_run_krylov!` method (src/solvers/multigrid/elliptic_solver.jl:393-452) runs
 V-cycle-preconditioned CG on the GPU through `libtpamr_hip.so` (include/tpamr.h).
 
-The reference keeps its own driver around the seam: compatible RHS, true residual, round-off
-floor, restart loop and stats (`_solve_krylov_with_stats!`, `_run_krylov_reliably!`). Each
-Krylov run goes to the device.
+Two entry paths, both selected by the same option:
+
+- `solve!` on an `EllipticSolver` whose Krylov workspace is a `GpuCgWorkspace` (the projection's
+  call, navier_stokes/projection.jl:221-260) runs the whole solve on the GPU in the workspace's
+  precision, including the FP32 refinement mode (`precision = :fp32_refine`): compatible RHS,
+  Krylov runs, true residual, round-off floor, restart loop and stats (`tpamr_solve`).
+- Any other caller of the nine-argument `_run_krylov!` seam keeps the reference's own driver
+  around it (`_solve_krylov_with_stats!`, `_run_krylov_reliably!`); each Krylov run goes to the
+  device with an FP32 V-cycle (FP64 V-cycle for `:fp64`).
 
 Usage:
 
     using TwoPhaseAMR, TPAMRShim
     options = NavierStokesOptions(; pressure = LinearSolverOptions(
-        krylov = TPAMRShim.gpu_krylov(; precision = :fp32_vcycle), preconditioner = nothing))
+        krylov = TPAMRShim.gpu_krylov(; precision = :fp32_refine), preconditioner = nothing))
 
 Loading the shim needs the built library at `ENV["TPAMR_LIB"]` or `../lib/libtpamr_hip.so`.
 """
@@ -82,11 +88,15 @@ end
     gpu_krylov(; device = 0, precision = :fp32_vcycle, multigrid = MultigridOptions())
 
 Factory `(matrix, rhs) -> GpuCgWorkspace` for `LinearSolverOptions(krylov = ...)`.
+`precision`: `:fp64` (FP64 CG and V-cycle), `:fp32_vcycle` (FP64 CG, FP32 V-cycle) or
+`:fp32_refine` (FP64 refinement around an FP32-vector CG with FP64 inner products; `solve!`
+only, the `_run_krylov!` seam then runs the `:fp32_vcycle` iteration).
 """
 function gpu_krylov(; device::Integer = 0, precision::Symbol = :fp32_vcycle,
         multigrid = TwoPhaseAMR.MultigridOptions())
     mode = precision === :fp64 ? FP64 : precision === :fp32_vcycle ? FP32_VCYCLE :
-        throw(ArgumentError("precision must be :fp64 or :fp32_vcycle"))
+        precision === :fp32_refine ? FP32_REFINE :
+        throw(ArgumentError("precision must be :fp64, :fp32_vcycle or :fp32_refine"))
     return function (matrix, rhs)
         handle = Ref{Ptr{Cvoid}}(C_NULL)
         _check(C_NULL, ccall((:tpamr_create, LIB), Cint, (Ref{Ptr{Cvoid}}, Cint, Cint), handle, device, mode))
@@ -175,6 +185,63 @@ function TwoPhaseAMR._run_krylov!(ws::GpuCgWorkspace, matrix, rhs, x0, precondit
 end
 
 """
+    TwoPhaseAMR.solve!(solution, solver::EllipticSolver{Float64, O, A, GpuCgWorkspace}, rhs;
+                       rtol = 1e-10, atol = 0, max_cycles = 100, compatibility = :project)
+
+`solve!` (src/solvers/multigrid/elliptic_solver.jl:628-675) with the whole solve on the GPU:
+the library takes the strong RHS and the initial pressure, integrates and projects the RHS,
+runs the Krylov runs of `_run_krylov_reliably!` with the strong-residual callback, checks the
+true residual with the guarded round-off floor and returns the zero-mean pressure, in the
+precision of the workspace (`tpamr_solve`). The stats are those of
+`_solve_krylov_with_stats!` (`cycles` counts Krylov iterations over every run). One connected
+component without anchors or covered cells (the library rejects others).
+"""
+function TwoPhaseAMR.solve!(solution::TwoPhaseAMR.CellField{Float64},
+        solver::TwoPhaseAMR.EllipticSolver{Float64, O, A, GpuCgWorkspace},
+        right_hand_side::TwoPhaseAMR.CellField{Float64}; rtol = 1.0e-10, atol = 0,
+        max_cycles = 100, compatibility = :project) where {O, A}
+    solution.mesh === right_hand_side.mesh === solver.operator.mesh ||
+        throw(ArgumentError("fields and multigrid solver must share a mesh"))
+    0 <= rtol < 1 || throw(ArgumentError("rtol must lie in [0, 1)"))
+    atol >= 0 || throw(ArgumentError("atol must be nonnegative"))
+    max_cycles >= 0 || throw(ArgumentError("max_cycles must be nonnegative"))
+    compatibility in (:project, :error) ||
+        throw(ArgumentError("compatibility must be :project or :error"))
+    ws = solver.krylov
+    _sync!(ws, solver.operator)
+    references = solver.operator.references
+    TwoPhaseAMR._copy_from_field!(solver.solution, solution, references)
+    TwoPhaseAMR._copy_from_field!(solver.right_hand_side, right_hand_side, references)
+    x = solver.solution
+    b = solver.right_hand_side
+    st = Ref{TpamrStats}()
+    GC.@preserve x b _check(ws.ctx, ccall((:tpamr_solve, LIB), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Cint, Cint, Ref{TpamrStats}),
+        ws.ctx, b, x, Float64(rtol), Float64(atol), Cint(max_cycles),
+        Cint(compatibility === :error ? 1 : 0), st))
+    TwoPhaseAMR._copy_to_field!(solution, x, references)
+    r = st[]
+    stats = solver.stats
+    stats.cycles = Int(r.cycles)
+    stats.converged = r.converged != 0
+    stats.compatible = true
+    stats.initial_l2 = r.initial_l2
+    stats.initial_linf = r.initial_linf
+    stats.final_l2 = r.final_l2
+    stats.final_linf = r.final_linf
+    stats.target_linf = r.target_linf
+    stats.residual_scale = r.residual_scale
+    stats.removed_rhs_mean = r.removed_rhs_mean
+    empty!(stats.removed_rhs_component_means)
+    empty!(stats.removed_rhs_component_anchors)
+    push!(stats.removed_rhs_component_means, r.removed_rhs_mean)
+    push!(stats.removed_rhs_component_anchors, 1)
+    stats.method = :cg
+    stats.preconditioner = :tree_geometric
+    return stats
+end
+
+"""
     gravity!(ws, acceleration, offsets, liquid, gas; density, gravity)
 
 `gravity!` (src/two_phase/forces.jl:528-580) on the GPU: `offsets` holds, per cell,
@@ -214,7 +281,18 @@ end
 
 # Device-memory twins: a time loop whose fields live in HBM (AMDGPU.jl arrays,
 # passed as `pointer(a)`) runs every step without PCIe copies. Each call
-# returns when its results are written.
+# returns when its results are written, and first waits for the work queued on
+# the caller's stream (`set_caller_stream!`; the null stream by default).
+
+"""
+    set_caller_stream!(ws, stream::Ptr{Cvoid})
+
+The HIP stream on which the caller writes the device twins' inputs (for AMDGPU.jl,
+`AMDGPU.stream().stream`): every later device call of `ws` orders itself after the work
+queued there (`tpamr_set_caller_stream`).
+"""
+set_caller_stream!(ws::GpuCgWorkspace, stream::Ptr{Cvoid}) =
+    _check(ws.ctx, ccall((:tpamr_set_caller_stream, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ws.ctx, stream))
 
 """
     step_device!(ws, d_u, d_fraction, d_liquid, d_gas, d_offsets, d_kappa, d_jumps, d_phi, dt;

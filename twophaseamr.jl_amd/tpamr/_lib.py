@@ -103,6 +103,7 @@ _SIGNATURES = {
     "tpamr_project_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
                                         C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Stats)]),
     "tpamr_set_face_coefficients_device": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "tpamr_set_caller_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
     "tpamr_apply_force_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),
     "tpamr_divergence_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "tpamr_mixture_face_coefficients_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double,
@@ -142,6 +143,15 @@ def f64(a, n=None, name="array"):
     if n is not None and arr.shape != (n,):
         raise DimensionMismatch(f"{name} must have length {n}, got shape {arr.shape}")
     return arr
+
+
+def order_after_torch(handle, device):
+    """Make the library's device-memory entry points on this context wait for
+    the work queued so far on torch's current stream of `device` (where the
+    caller wrote their inputs)."""
+    import torch
+    stream = torch.cuda.current_stream(device)
+    check(lib().tpamr_set_caller_stream(handle, C.c_void_p(stream.cuda_stream)), handle)
 
 
 def check(rc, handle):

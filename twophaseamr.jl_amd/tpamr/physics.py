@@ -153,6 +153,7 @@ def mixture_face_coefficients_device(solver: EllipticSolver, fraction, density=(
     c = _device_vec(fraction, m.ncells, "fraction")
     k = torch.empty(m.nfaces, dtype=torch.float64, device=c.device) if out is None else \
         _device_vec(out, m.nfaces, "out")
+    L.order_after_torch(solver.ctx.h, c.device)
     L.check(L.lib().tpamr_mixture_face_coefficients_device(solver.ctx.h, c.data_ptr(), rho_l, rho_g, k.data_ptr()),
             solver.ctx.h)
     return k
@@ -171,6 +172,7 @@ def gravity_jumps_device(solver: EllipticSolver, liquid_aperture, gas_aperture, 
     off = None if centroid_offsets is None else _device_vec(centroid_offsets, m.ncells, "centroid offsets", 2)
     j = torch.empty(m.nfaces, dtype=torch.float64, device=la.device) if out is None else \
         _device_vec(out, m.nfaces, "out")
+    L.order_after_torch(solver.ctx.h, la.device)
     L.check(L.lib().tpamr_gravity_jumps_device(solver.ctx.h, None if off is None else off.data_ptr(), la.data_ptr(),
                                                ga.data_ptr(), rho_l, rho_g, gx, gy, j.data_ptr()), solver.ctx.h)
     return j

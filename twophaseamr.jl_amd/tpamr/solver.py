@@ -269,8 +269,11 @@ class EllipticSolver:
         return x, it.value, REASONS.get(reason.value, str(reason.value))
 
     def solve_device(self, d_rhs, d_x, *, rtol=1.0e-10, atol=0.0, max_cycles=100, compatibility="project"):
-        """solve! on device buffers (raw device pointers, e.g. torch .data_ptr())."""
+        """solve! on device buffers (raw device pointers, e.g. torch .data_ptr()
+        of float64 tensors on the solver's GPU). The solve waits for the work
+        queued on torch's current stream of that GPU."""
         self._sync()
+        L.order_after_torch(self.ctx.h, self.ctx.device)
         st = L.Stats()
         L.check(L.lib().tpamr_solve_device(self.ctx.h, C.c_void_p(d_rhs), C.c_void_p(d_x), float(rtol),
                                            float(atol), int(max_cycles), 1 if compatibility == "error" else 0,

@@ -198,6 +198,7 @@ class DeviceProjectionRun:
         lib = L.lib()
         h = s.ctx.h
         u = self._tensor(u0, m.nfaces, "u0").clone()
+        L.order_after_torch(h, u.device)
         div = torch.empty(m.ncells, dtype=torch.float64, device=u.device) if self.rtol is not None else None
         self.steps = [(self._tensor(k, m.nfaces, "face coefficients"),
                        None if j is None else self._tensor(j, m.nfaces, "force jumps")) for k, j in steps]
@@ -248,6 +249,7 @@ class DeviceProjectionRun:
         lib = L.lib()
         h = s.ctx.h
         ub = self._tensor(ubar, m.nfaces, "ubar").clone()
+        L.order_after_torch(h, ub.device)
         gk = [None] * len(self.steps)
         stats = [None] * len(self.steps)
         db = torch.empty(m.ncells, dtype=torch.float64, device=ub.device)
