@@ -123,6 +123,19 @@ int tpamr_solve(tpamr_ctx* ctx, const double* rhs, double* x, double rtol, doubl
                 int compatibility, tpamr_stats* stats);
 int tpamr_solve_device(tpamr_ctx* ctx, const double* d_rhs, double* d_x, double rtol, double atol, int max_cycles,
                        int compatibility, tpamr_stats* stats);
+/* tpamr_solve_device without any allocation during the call: no hipMalloc,
+ * hipFree, graph instantiation or allocating library routine. No counterpart
+ * in the reference; it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid it). The Krylov
+ * state and the reduction partials live in the library's static device
+ * scratch of 8 MiB per GPU (held until the process ends), which bounds the
+ * finest level at about 5e7 cells: larger meshes are rejected with
+ * TPAMR_ESIZE. The hierarchy and vectors are the context's, allocated at
+ * set-up (tpamr_set_face_coefficients). Results equal tpamr_solve_device's
+ * bit for bit. Every call on one GPU shares that GPU's scratch: calls that
+ * overlap on one GPU (two host threads, two contexts on one device) race. */
+int tpamr_solve_device_noalloc(tpamr_ctx* ctx, const double* d_rhs, double* d_x, double rtol, double atol,
+                               int max_cycles, int compatibility, tpamr_stats* stats);
 
 /* One Krylov run at the plugin seam: replaces the nine-argument
  * TwoPhaseAMR._run_krylov!(workspace, matrix, rhs, x0, preconditioner,

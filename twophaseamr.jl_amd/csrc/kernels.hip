@@ -14,7 +14,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <stdexcept>
+#include <vector>
 
 #include "kernels.hpp"
 
@@ -1601,5 +1603,28 @@ template void launch_cg_update_xr<double>(hipStream_t, int, int, const double*, 
                                           int);
 template void launch_cg_update_p<float>(hipStream_t, int, const float*, double*, const CgState*, const int*);
 template void launch_cg_update_p<double>(hipStream_t, int, const double*, double*, const CgState*, const int*);
+
+// Scratch of tpamr_solve_device_noalloc: the Krylov state and the reduction
+// partials of one solve, carved at 256-byte offsets. Each GPU holds its own
+// copy; host code reaches it only through the address that
+// hipGetSymbolAddress returns for that device (the name itself is a
+// host-side stand-in, not a device address).
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+hipError_t noalloc_scratch_address(int device, void** ptr) {
+  static std::mutex mu;
+  static std::vector<void*> addr;
+  if (device < 0 || !ptr) return hipErrorInvalidValue;
+  std::lock_guard<std::mutex> lock(mu);
+  if (size_t(device) >= addr.size()) addr.resize(size_t(device) + 1, nullptr);
+  if (!addr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = p;
+  }
+  *ptr = addr[device];
+  return hipSuccess;
+}
 
 }  // namespace tpamr

@@ -229,4 +229,12 @@ void launch_copy(hipStream_t s, int n, const double* in, double* out);
 void launch_fill(hipStream_t s, int n, double v, double* out);
 void launch_fill_int(hipStream_t s, int n, int v, int* out);
 
+// Device scratch of the allocation-free solve entry point: a file-scope
+// array of this many bytes on every GPU the library runs on, held until the
+// process ends.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(8) << 20;
+// Its address on `device`, which must be the current device: looked up once
+// per device with hipGetSymbolAddress; returns that call's status.
+hipError_t noalloc_scratch_address(int device, void** ptr);
+
 }  // namespace tpamr

@@ -148,7 +148,14 @@ struct tpamr_ctx {
   DevArray<int> ref_flags;
   DevArray<double> x, r, p, Ap, b, tmp, tmp2;
   DevArray<double> part, maxpart, scal, stage;
-  CgState* st = nullptr;      // device
+  CgState* st_own = nullptr;  // device
+  // Reduction scratch and Krylov state of the solve path: the context's own
+  // buffers above, or (for the duration of tpamr_solve_device_noalloc) the
+  // library's static device scratch.
+  struct Reduce {
+    double *part = nullptr, *maxpart = nullptr, *scal = nullptr, *stage = nullptr;
+    CgState* st = nullptr;
+  } rs;
   CgState* st_host = nullptr; // pinned
   CgState* st_ring = nullptr; // pinned, one per Krylov iteration in flight
   // face data (finest level)
@@ -191,6 +198,8 @@ struct tpamr_ctx {
   // touch its buffers
   hipStream_t caller = nullptr;
   hipEvent_t ev_caller = nullptr;
+  // the library's static device scratch on this context's device
+  void* noalloc = nullptr;
   const TreeMesh& mesh() const {
     if (H) return *H->levels[0]->mesh;
     if (mesh_pending) return *mesh_pending;
@@ -442,6 +451,10 @@ void upload_hierarchy(tpamr_ctx* c, bool topology) {
     c->maxpart.alloc(std::max(nblk, ncb) + 16);
     c->scal.alloc(16);
     c->stage.alloc(4 * RED_WG);
+    c->rs.part = c->part.p;
+    c->rs.maxpart = c->maxpart.p;
+    c->rs.scal = c->scal.p;
+    c->rs.stage = c->stage.p;
     destroy_graph(c);
   }
   // kernel arguments of the captured iteration (weights, pointers) change
@@ -552,9 +565,9 @@ void matvec(tpamr_ctx* c, const double* x, double* y) {
 }
 
 double reduce_host(tpamr_ctx* c, int n, int nsets, const double* part, bool is_max, double* out_sets = nullptr) {
-  launch_reduce(c->stream, n, nsets, part, c->scal.p, is_max);
+  launch_reduce(c->stream, n, nsets, part, c->rs.scal, is_max);
   double h[16];
-  HIPCHECK(hipMemcpyAsync(h, c->scal.p, sizeof(double) * nsets, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipMemcpyAsync(h, c->rs.scal, sizeof(double) * nsets, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   if (out_sets)
     for (int q = 0; q < nsets; ++q) out_sets[q] = h[q];
@@ -567,28 +580,28 @@ int ncb(int n) { return (n + 255) / 256; }
 void true_residual(tpamr_ctx* c, double& l2, double& linf) {
   const int N = c->topo[0].ncells, nb = c->H->levels[0]->mesh->nb();
   matvec(c, c->x.p, c->Ap.p);
-  launch_residual_strong(c->stream, N, nb, c->b.p, c->Ap.p, c->r.p, c->topo[0].volume.p, c->part.p);
+  launch_residual_strong(c->stream, N, nb, c->b.p, c->Ap.p, c->r.p, c->topo[0].volume.p, c->rs.part);
   double sets[2];
-  reduce_host(c, ncb(N), 1, c->part.p, true, sets);
+  reduce_host(c, ncb(N), 1, c->rs.part, true, sets);
   linf = sets[0];
   double s2[1];
-  reduce_host(c, ncb(N), 1, c->part.p + ncb(N), false, s2);
+  reduce_host(c, ncb(N), 1, c->rs.part + ncb(N), false, s2);
   l2 = std::sqrt(s2[0] / c->H->total_volume);
 }
 
 double roundoff_floor(tpamr_ctx* c) {
   const int nblk = c->topo[0].nblocks;
-  launch_floor_bound(c->stream, fine64(c), c->x.p, c->b.p, c->part.p);
-  const double m = reduce_host(c, nblk, 1, c->part.p, true);
+  launch_floor_bound(c->stream, fine64(c), c->x.p, c->b.p, c->rs.part);
+  const double m = reduce_host(c, nblk, 1, c->rs.part, true);
   return 128 * EPS * std::max(1.0, m);
 }
 
 // subtract the volume-weighted mean (one connected, unanchored component)
 void remove_mean(tpamr_ctx* c, double* v) {
   const int N = c->topo[0].ncells, nb = c->H->levels[0]->mesh->nb();
-  launch_weighted_sums(c->stream, N, nb, v, c->topo[0].volume.p, c->part.p);
-  launch_reduce(c->stream, ncb(N), 1, c->part.p, c->scal.p, false);
-  launch_shift(c->stream, N, nb, v, c->scal.p, 1.0 / c->H->total_volume, c->topo[0].volume.p, false);
+  launch_weighted_sums(c->stream, N, nb, v, c->topo[0].volume.p, c->rs.part);
+  launch_reduce(c->stream, ncb(N), 1, c->rs.part, c->rs.scal, false);
+  launch_shift(c->stream, N, nb, v, c->rs.scal, 1.0 / c->H->total_volume, c->topo[0].volume.p, false);
 }
 
 double guarded_target(tpamr_ctx* c, double target, double initial, double current) {
@@ -602,7 +615,7 @@ template <typename T>
 T* enqueue_precondition(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done) {
   ZStats zs;
   zs.r64 = c->r.p;
-  zs.part = c->part.p;
+  zs.part = c->rs.part;
   return enqueue_vcycle<T>(c, lv, done, zs);
 }
 
@@ -610,23 +623,23 @@ template <typename T>
 void enqueue_iteration(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, bool init) {
   hipStream_t s = c->stream;
   const int N = c->topo[0].ncells, nb = c->H->levels[0]->mesh->nb(), nblk = c->topo[0].nblocks;
-  const int* done = &c->st->done;
+  const int* done = &c->rs.st->done;
   DevLevel<T>& L0 = lv[0];
   const double w0 = c->cfg.pre > 0 ? L0.omega : 0.0;
   if (init) {
     matvec(c, c->x.p, c->Ap.p);
     launch_copy(s, N, c->b.p, c->r.p);
     launch_cg_update_xr<T>(s, N, nb, nullptr, c->Ap.p, nullptr, c->r.p, c->topo[0].volume.p, L0.b.p, L0.x.p,
-                           L0.dinv.p, w0, c->maxpart.p, c->st, 1);
+                           L0.dinv.p, w0, c->rs.maxpart, c->rs.st, 1);
   } else {
-    launch_matvec_dot<double>(s, fine64(c), c->p.p, c->Ap.p, c->part.p, done);
-    launch_cg_alpha(s, nblk, c->part.p, c->stage.p, c->st);
+    launch_matvec_dot<double>(s, fine64(c), c->p.p, c->Ap.p, c->rs.part, done);
+    launch_cg_alpha(s, nblk, c->rs.part, c->rs.stage, c->rs.st);
     launch_cg_update_xr<T>(s, N, nb, c->p.p, c->Ap.p, c->x.p, c->r.p, c->topo[0].volume.p, L0.b.p, L0.x.p,
-                           L0.dinv.p, w0, c->maxpart.p, c->st, 0);
+                           L0.dinv.p, w0, c->rs.maxpart, c->rs.st, 0);
   }
   T* z = enqueue_precondition<T>(c, lv, done);
-  launch_cg_beta(s, nblk, ncb(N), c->part.p, c->maxpart.p, c->stage.p, c->H->total_volume, c->st, init ? 1 : 0);
-  launch_cg_update_p<T>(s, N, z, c->p.p, c->st, done);
+  launch_cg_beta(s, nblk, ncb(N), c->rs.part, c->rs.maxpart, c->rs.stage, c->H->total_volume, c->rs.st, init ? 1 : 0);
+  launch_cg_update_p<T>(s, N, z, c->p.p, c->rs.st, done);
 }
 
 // Capture `graph_iters` non-initial Krylov iterations into `graph`.
@@ -665,7 +678,7 @@ int run_iterations(tpamr_ctx* c, hipGraphExec_t& graph, int itmax, Enq&& enqueue
       HIPCHECK(hipGraphLaunch(graph, s));
     else
       enqueue(false);
-    HIPCHECK(hipMemcpyAsync(c->st_ring + launched % P, c->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(c->st_ring + launched % P, c->rs.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipEventRecord(c->evs[launched % P], s));
     ++launched;
     if (launched >= P) {
@@ -681,7 +694,7 @@ int run_iterations(tpamr_ctx* c, hipGraphExec_t& graph, int itmax, Enq&& enqueue
     if (launched * per > itmax + P * per) break;
   }
   HIPCHECK(hipStreamSynchronize(s));
-  HIPCHECK(hipMemcpy(c->st_host, c->st, sizeof(CgState), hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(c->st_host, c->rs.st, sizeof(CgState), hipMemcpyDeviceToHost));
   return c->st_host->iter;
 }
 
@@ -695,7 +708,7 @@ int reset_state(tpamr_ctx* c, double target, int itmax, double scale = 1.0, doub
   init.keep = c->ir_keep ? 1 : 0;
   init.rlast = 1.0;
   *c->st_host = init;
-  HIPCHECK(hipMemcpyAsync(c->st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->rs.st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
   return init.itmax;
 }
 
@@ -718,27 +731,27 @@ int run_cg(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double target, int itmax)
 void enqueue_ir_iteration(tpamr_ctx* c, std::vector<DevLevel<float>>& lv, bool init) {
   hipStream_t s = c->stream;
   const int N = c->topo[0].ncells, nb = c->H->levels[0]->mesh->nb(), nblk = c->topo[0].nblocks;
-  const int* done = &c->st->done;
+  const int* done = &c->rs.st->done;
   DevLevel<float>& L0 = lv[0];
   const double w0 = c->cfg.pre > 0 ? L0.omega : 0.0;
   if (!init) {
-    launch_matvec_dot<float>(s, L0.view, c->p32.p, c->Ap32.p, c->part.p, done, true);
-    launch_cg_alpha(s, nblk, c->part.p, c->stage.p, c->st, true, c->H->total_volume);
+    launch_matvec_dot<float>(s, L0.view, c->p32.p, c->Ap32.p, c->rs.part, done, true);
+    launch_cg_alpha(s, nblk, c->rs.part, c->rs.stage, c->rs.st, true, c->H->total_volume);
   }
   launch_ir_update_xr(s, N, nb, c->p32.p, c->Ap32.p, c->e32.p, L0.b.p, c->topo[0].volume.p, L0.x.p, L0.dinv.p, w0,
-                      c->r.p, c->maxpart.p, c->st, init ? 1 : 0);
+                      c->r.p, c->rs.maxpart, c->rs.st, init ? 1 : 0);
   if (!init) {
-    launch_ir_correct(s, N, c->e32.p, c->x.p, c->st, false);
-    launch_ir_replace(s, fine64(c), c->x.p, c->b.p, L0.b.p, L0.x.p, L0.dinv.p, w0, c->maxpart.p, c->st);
-    launch_ir_check(s, ir_groups(N), c->maxpart.p, c->stage.p, c->st);
+    launch_ir_correct(s, N, c->e32.p, c->x.p, c->rs.st, false);
+    launch_ir_replace(s, fine64(c), c->x.p, c->b.p, L0.b.p, L0.x.p, L0.dinv.p, w0, c->rs.maxpart, c->rs.st);
+    launch_ir_check(s, ir_groups(N), c->rs.maxpart, c->rs.stage, c->rs.st);
   }
   ZStats zs;
   zs.r32 = L0.b.p;
-  zs.part = c->part.p;
+  zs.part = c->rs.part;
   float* z = enqueue_vcycle<float>(c, lv, done, zs);
-  launch_cg_beta(s, nblk, ir_groups(N), c->part.p, c->maxpart.p, c->stage.p, c->H->total_volume, c->st,
+  launch_cg_beta(s, nblk, ir_groups(N), c->rs.part, c->rs.maxpart, c->rs.stage, c->H->total_volume, c->rs.st,
                  init ? 1 : 0);
-  launch_ir_update_p(s, N, z, c->p32.p, c->st, done);
+  launch_ir_update_p(s, N, z, c->p32.p, c->rs.st, done);
 }
 
 // One run of the refinement mode's Krylov iteration from the FP64 residual
@@ -761,11 +774,11 @@ tpamr_stats solve_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double rtol, 
   HIPCHECK(hipEventRecord(c->ev0, s));
   // integrated rhs, its sums for the compatible projection, and whether the
   // initial solution is zero (one pass, one host synchronisation)
-  launch_rhs_prepare(s, N, nb, c->b.p, c->x.p, vol, c->part.p);
+  launch_rhs_prepare(s, N, nb, c->b.p, c->x.p, vol, c->rs.part);
   double sums[4];
   {
-    launch_reduce(s, ncb(N), 4, c->part.p, c->scal.p, false);
-    HIPCHECK(hipMemcpyAsync(sums, c->scal.p, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
+    launch_reduce(s, ncb(N), 4, c->rs.part, c->rs.scal, false);
+    HIPCHECK(hipMemcpyAsync(sums, c->rs.scal, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
   }
   const double scale = sums[0], total_sum = sums[1];
@@ -782,11 +795,11 @@ tpamr_stats solve_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double rtol, 
   if (zero_start) {
     // b -= V mean; the residual of x = 0 is b and its round-off floor bound
     // reduces to max |b / V|
-    launch_shift_residual(s, N, nb, c->b.p, mean, vol, c->r.p, c->part.p);
+    launch_shift_residual(s, N, nb, c->b.p, mean, vol, c->r.p, c->rs.part);
     double sets[2];
-    launch_reduce(s, ncb(N), 1, c->part.p, c->scal.p, true);
-    launch_reduce(s, ncb(N), 1, c->part.p + ncb(N), c->scal.p + 1, false);
-    HIPCHECK(hipMemcpyAsync(sets, c->scal.p, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+    launch_reduce(s, ncb(N), 1, c->rs.part, c->rs.scal, true);
+    launch_reduce(s, ncb(N), 1, c->rs.part + ncb(N), c->rs.scal + 1, false);
+    HIPCHECK(hipMemcpyAsync(sets, c->rs.scal, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     ilinf = sets[0];
     il2 = std::sqrt(sets[1] / c->H->total_volume);
@@ -794,8 +807,8 @@ tpamr_stats solve_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double rtol, 
   } else {
     // b -= V * mean (scalar staged through device memory)
     double h = mean;
-    HIPCHECK(hipMemcpyAsync(c->scal.p + 8, &h, sizeof(double), hipMemcpyHostToDevice, s));
-    launch_shift(s, N, nb, c->b.p, c->scal.p + 8, 1.0, vol, true);
+    HIPCHECK(hipMemcpyAsync(c->rs.scal + 8, &h, sizeof(double), hipMemcpyHostToDevice, s));
+    launch_shift(s, N, nb, c->b.p, c->rs.scal + 8, 1.0, vol, true);
     remove_mean(c, c->x.p);
     true_residual(c, il2, ilinf);
     floor0 = roundoff_floor(c);
@@ -821,7 +834,7 @@ tpamr_stats solve_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double rtol, 
       reason = c->st_host->reason;
       replaced += c->st_host->replaced;
       total += it;
-      launch_ir_correct(s, N, c->e32.p, c->x.p, c->st, true);
+      launch_ir_correct(s, N, c->e32.p, c->x.p, c->rs.st, true);
       remove_mean(c, c->x.p);
       true_residual(c, fl2, linf);
       flinf = linf;
@@ -885,13 +898,13 @@ void vcycle_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv) {
   CgState init{};
   init.scale = 1.0;
   *c->st_host = init;
-  HIPCHECK(hipMemcpyAsync(c->st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, s));
+  HIPCHECK(hipMemcpyAsync(c->rs.st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, s));
   DevLevel<T>& L0 = lv[0];
   const double w0 = c->cfg.pre > 0 ? L0.omega : 0.0;
   launch_fill(s, N, 0.0, c->Ap.p);
   launch_cg_update_xr<T>(s, N, nb, nullptr, c->Ap.p, nullptr, c->r.p, c->topo[0].volume.p, L0.b.p, L0.x.p,
-                         L0.dinv.p, w0, c->maxpart.p, c->st, 1);
-  T* z = enqueue_vcycle<T>(c, lv, &c->st->done, ZStats{});
+                         L0.dinv.p, w0, c->rs.maxpart, c->rs.st, 1);
+  T* z = enqueue_vcycle<T>(c, lv, &c->rs.st->done, ZStats{});
   if constexpr (std::is_same<T, float>::value)
     launch_cast_f2d(s, N, z, c->tmp.p);
   else
@@ -1241,13 +1254,15 @@ int tpamr_create(tpamr_ctx** out, int device, int precision) {
       if (device >= n) throw HipError("device index out of range");
       HIPCHECK(hipSetDevice(device));
       HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-      HIPCHECK(hipMalloc(&c->st, sizeof(CgState)));
+      HIPCHECK(hipMalloc(&c->st_own, sizeof(CgState)));
+      c->rs.st = c->st_own;
       HIPCHECK(hipHostMalloc(&c->st_host, sizeof(CgState), hipHostMallocDefault));
       HIPCHECK(hipHostMalloc(&c->st_ring, PIPE_MAX * sizeof(CgState), hipHostMallocDefault));
       HIPCHECK(hipEventCreate(&c->ev0));
       HIPCHECK(hipEventCreate(&c->ev1));
       for (int q = 0; q < PIPE_MAX; ++q) HIPCHECK(hipEventCreateWithFlags(&c->evs[q], hipEventDisableTiming));
       HIPCHECK(hipEventCreateWithFlags(&c->ev_caller, hipEventDisableTiming));
+      HIPCHECK(noalloc_scratch_address(device, &c->noalloc));
     });
     if (rc != TPAMR_OK) {
       // keep the context so the caller can read the message
@@ -1269,7 +1284,7 @@ void tpamr_destroy(tpamr_ctx* c) {
     c->topo.clear();
     c->ld.clear();
     c->lf.clear();
-    if (c->st) (void)hipFree(c->st);
+    if (c->st_own) (void)hipFree(c->st_own);
     if (c->st_host) (void)hipHostFree(c->st_host);
     if (c->st_ring) (void)hipHostFree(c->st_ring);
     for (hipEvent_t e : {c->ev0, c->ev1, c->ev_caller})
@@ -1551,6 +1566,61 @@ int tpamr_solve_device(tpamr_ctx* c, const double* d_rhs, double* d_x, double rt
     HIPCHECK(hipSetDevice(c->device));
     order_after_caller(c);
     const int N = c->topo[0].ncells;
+    HIPCHECK(hipMemcpyAsync(c->b.p, d_rhs, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->x.p, d_x, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    tpamr_stats st = solve_dispatch(c, rtol, atol, max_cycles, compatibility);
+    HIPCHECK(hipMemcpyAsync(d_x, c->x.p, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (stats) *stats = st;
+  });
+}
+
+int tpamr_solve_device_noalloc(tpamr_ctx* c, const double* d_rhs, double* d_x, double rtol, double atol,
+                               int max_cycles, int compatibility, tpamr_stats* stats) {
+  if (!c || !d_rhs || !d_x) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    need_solver(c);
+    HIPCHECK(hipSetDevice(c->device));
+    if (!c->noalloc) throw StateError("no device scratch address on this context's device");
+    // layout of the scratch: Krylov state, scalars, stage, partials (the
+    // sizes of the context's own buffers)
+    const size_t N = size_t(c->topo[0].ncells), nblk = size_t(c->topo[0].nblocks), nc = (N + 255) / 256;
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    size_t off = 0;
+    const size_t o_st = off;
+    off += up(sizeof(CgState));
+    const size_t o_scal = off;
+    off += up(16 * sizeof(double));
+    const size_t o_stage = off;
+    off += up(4 * RED_WG * sizeof(double));
+    const size_t o_part = off;
+    off += up((std::max(3 * nblk, 4 * nc) + 16) * sizeof(double));
+    const size_t o_max = off;
+    off += up((std::max(nblk, nc) + 16) * sizeof(double));
+    if (off > NOALLOC_SCRATCH_BYTES)
+      throw SizeError("mesh too large for the allocation-free solve: it needs " + std::to_string(off) +
+                      " bytes of device scratch, the limit is " + std::to_string(NOALLOC_SCRATCH_BYTES) +
+                      " (about 5e7 cells); use tpamr_solve_device");
+    order_after_caller(c);
+    char* base = static_cast<char*>(c->noalloc);
+    // the solve runs on the scratch and launches its iterations one by one
+    // (no graph instantiation); the context's own buffers come back after
+    struct Swap {
+      tpamr_ctx* c;
+      tpamr_ctx::Reduce saved;
+      bool graph;
+      ~Swap() {
+        c->rs = saved;
+        c->use_graph = graph;
+      }
+    } swap{c, c->rs, c->use_graph};
+    c->rs.st = reinterpret_cast<CgState*>(base + o_st);
+    c->rs.scal = reinterpret_cast<double*>(base + o_scal);
+    c->rs.stage = reinterpret_cast<double*>(base + o_stage);
+    c->rs.part = reinterpret_cast<double*>(base + o_part);
+    c->rs.maxpart = reinterpret_cast<double*>(base + o_max);
+    c->use_graph = false;
     HIPCHECK(hipMemcpyAsync(c->b.p, d_rhs, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     HIPCHECK(hipMemcpyAsync(c->x.p, d_x, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     tpamr_stats st = solve_dispatch(c, rtol, atol, max_cycles, compatibility);

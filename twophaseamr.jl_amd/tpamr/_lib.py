@@ -72,6 +72,8 @@ _SIGNATURES = {
                               C.POINTER(Stats)]),
     "tpamr_solve_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int,
                                      C.c_int, C.POINTER(Stats)]),
+    "tpamr_solve_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int,
+                                             C.c_int, C.POINTER(Stats)]),
     "tpamr_vcycle": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "tpamr_krylov": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p,
                                C.POINTER(C.c_int), C.POINTER(C.c_int)]),
