@@ -82,30 +82,62 @@ def timed_solves(solver, d_rhs, d_x, steps, warmup, rtol):
     return float(np.mean(walls)), float(np.mean(dev)), st
 
 
-def cpu_baseline(levels, rtol, steps=3):
-    """The oracle (CPU FP64 restatement, one thread) on a bounded sample of the
-    same workload family: the circles mesh at a reduced maximum level. The
-    same system is solved on the GPU in both precision modes and the
-    pressures are compared with the CPU FP64 solution."""
-    import torch
+def host_cpu():
+    """(model name, threads this process may run on; at most 16, the GPU
+    box's CPU share per GPU)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return model, max(1, min(16, n))
+
+
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from oracle import OMesh, OSolver
+    import oracle
+    return oracle
+
+
+def cpu_baseline(levels, rtol, steps=3):
+    """The oracle (CPU FP64 restatement) on a bounded sample of the same
+    workload family: the circles mesh at a reduced maximum level, solved on
+    the host's threads (OpenMP; results independent of the thread count) and
+    again on one thread. The same system is solved on the GPU in every
+    precision mode and the pressures are compared with the CPU FP64 solution."""
+    import torch
+    O = _oracle()
     import tpamr
     from tpamr import workloads as W
+    model, threads = host_cpu()
     mesh, circles = W.circles_mesh(levels)
-    om = OMesh(base_mesh=mesh.base_mesh, block_size=mesh.block_size, periodic=mesh.periodic,
-               max_level=mesh.max_level, refined_keys=mesh.refined_keys)
+    om = O.OMesh(base_mesh=mesh.base_mesh, block_size=mesh.block_size, periodic=mesh.periodic,
+                 max_level=mesh.max_level, refined_keys=mesh.refined_keys)
     c = W.liquid_fraction(mesh, circles)
     kappa = om.mixture_coefficients(c, 1000.0, 1.0)
     u = W.smooth_velocity(mesh, device="cuda:0")
     rhs = -om.divergence(u)
+    O.set_threads(threads)
     t0 = time.perf_counter()
-    s = OSolver(om, kappa)
+    s = O.OSolver(om, kappa)
     t1 = time.perf_counter()
     x_cpu, st = s.solve(rhs, rtol=rtol, max_cycles=200)
     t2 = time.perf_counter()
+    O.set_threads(1)
+    x_one, st_one = s.solve(rhs, rtol=rtol, max_cycles=200)
+    t3 = time.perf_counter()
+    O.set_threads(threads)
     del s
-    log(f"cpu baseline: {mesh.ncells} cells, setup {t1 - t0:.1f} s, solve {t2 - t1:.1f} s")
+    log(f"cpu baseline: {mesh.ncells} cells, setup {t1 - t0:.1f} s, solve {t2 - t1:.1f} s on {threads} threads, "
+        f"{t3 - t2:.1f} s on 1")
     gpu = {"cells": mesh.ncells}
     d_rhs = torch.as_tensor(rhs, device="cuda:0")
     for prec in MODES:
@@ -117,12 +149,41 @@ def cpu_baseline(levels, rtol, steps=3):
                      "pressure_error_vs_cpu_fp64": float(np.max(np.abs(x - x_cpu)) / np.max(np.abs(x_cpu)))}
         solver.close()
     gpu["speedup_fp32_refine_vs_fp64"] = gpu["fp32_refine"]["mcells"] / gpu["fp64"]["mcells"]
-    cpu = dict(value=mesh.ncells / (t2 - t1) / 1e6, unit="Mcells/s", cores=1, kind="port",
-               sample=f"oracle FP64 solve (CPU restatement, 1 thread) on the circles workload at levels "
-                      f"{levels[0]}-{levels[1]} ({mesh.ncells} cells, {st['cycles']} CG iterations, "
+    cpu = dict(value=mesh.ncells / (t2 - t1) / 1e6, unit="Mcells/s", cores=threads, kind="port",
+               cpu_model=model,
+               sample=f"oracle FP64 solve (CPU restatement, OpenMP on {threads} threads) on the circles workload "
+                      f"at levels {levels[0]}-{levels[1]} ({mesh.ncells} cells, {st['cycles']} CG iterations, "
                       f"solve {t2 - t1:.2f} s, hierarchy setup {t1 - t0:.2f} s excluded)",
-               cpu_cycles=st["cycles"])
+               cpu_cycles=st["cycles"],
+               value_1_thread=mesh.ncells / (t3 - t2) / 1e6,
+               thread_count_independent=bool(np.array_equal(x_cpu, x_one) and st["cycles"] == st_one["cycles"]))
     return cpu, gpu
+
+
+def cpu_full(case, x_sol, rtol):
+    """The oracle's FP64 CPU solve of the benchmark system itself (the full
+    mesh, on the host's threads), and each GPU mode's pressure error against
+    it."""
+    O = _oracle()
+    model, threads = host_cpu()
+    O.set_threads(threads)
+    mesh = case.mesh
+    t0 = time.perf_counter()
+    om = O.OMesh(base_mesh=mesh.base_mesh, block_size=mesh.block_size, periodic=mesh.periodic,
+                 max_level=mesh.max_level, refined_keys=mesh.refined_keys)
+    kappa = om.mixture_coefficients(case.fraction, 1000.0, 1.0)
+    s = O.OSolver(om, kappa)
+    t1 = time.perf_counter()
+    log(f"cpu full: setup {t1 - t0:.1f} s")
+    rhs = case.d_rhs.cpu().numpy()
+    x_cpu, st = s.solve(rhs, rtol=rtol, max_cycles=200)
+    t2 = time.perf_counter()
+    log(f"cpu full: solve {t2 - t1:.1f} s, {st['cycles']} iterations")
+    ref = np.max(np.abs(x_cpu))
+    return dict(cells=mesh.ncells, threads=threads, cpu_model=model, setup_s=t1 - t0, solve_s=t2 - t1,
+                mcells=mesh.ncells / (t2 - t1) / 1e6, cycles=st["cycles"], converged=bool(st["converged"]),
+                target_linf=st["target_linf"], final_linf=st["final_linf"],
+                pressure_error_vs_cpu_fp64={p: float(np.max(np.abs(x - x_cpu)) / ref) for p, x in x_sol.items()})
 
 
 class Case:
@@ -249,8 +310,9 @@ def trace_summary(db, title, command, top=30):
 
 def uniform_sweep(n, reps=50):
     """Finest-level Jacobi sweep on a uniform n x n mesh (16x16 blocks, walls:
-    no irregular rows) in FP32 and FP64, beside a plain device copy: the
-    streaming rate the regular stencil path reaches. One JSON line."""
+    no irregular rows) in FP32 and FP64, beside the library's streaming
+    passes (a copy, and five reads per write): the streaming rate the regular
+    stencil path reaches. One JSON line."""
     import torch
     import tpamr
     torch.cuda.init()  # torch's HIP runtime first (it finds no GPU once the library has opened it)
@@ -262,28 +324,28 @@ def uniform_sweep(n, reps=50):
         out[prec] = {"sweep_us": ms * 1e3, "compulsory_bytes": nbytes, "GBps": nbytes / (ms * 1e-3) / 1e9,
                      "irregular_nnz": s.level_info(0)["irregular_nnz"]}
         s.close()
-    out["copy_GBps"] = copy_bandwidth()
+    out["stream"] = stream_rates()
     print(json.dumps(out), flush=True)
 
 
-def copy_bandwidth(nbytes=1 << 30, reps=10):
-    """Bytes read plus written per second (GB/s) of a plain device copy of
-    `nbytes`, timed with HIP events on torch's stream: what a streaming kernel
-    reaches on this GPU."""
-    import torch
-    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
-    b = torch.empty_like(a)
-    b.copy_(a)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        b.copy_(a)
-    e1.record()
-    torch.cuda.synchronize()
-    gbs = 2.0 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
-    del a, b
-    torch.cuda.empty_cache()
-    return gbs
+def stream_rates(n=1 << 28, reps=10):
+    """HBM streaming rates (bytes read plus written per second, GB/s) of the
+    library's float4 pass over n-float arrays, timed with HIP events on the
+    context stream: a plain copy (working set 2n floats) and the Jacobi
+    sweep's mix of five reads per write (6n floats); n = 2^28 puts both far
+    beyond the 256 MB Infinity Cache."""
+    import ctypes as C
+    from tpamr import _lib as L
+    ctx = L.Context(0, L.TPAMR_FP32_REFINE)
+    out = {}
+    try:
+        for name, reads in (("copy", 1), ("sweep_mix", 5)):
+            ms, nb = C.c_double(), C.c_double()
+            L.check(L.lib().tpamr_time_stream(ctx.h, n, reads, reps, C.byref(ms), C.byref(nb)), ctx.h)
+            out[name] = dict(GBps=nb.value / (ms.value * 1e-3) / 1e9, bytes=nb.value, ms=ms.value)
+    finally:
+        ctx.close()
+    return out
 
 
 def committed_traffic():
@@ -419,6 +481,9 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-levels", type=int, nargs=2, default=(5, 8))
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="also solve the benchmark system itself with the oracle on the host's threads and "
+                         "report every mode's pressure error against it (minutes of CPU setup)")
     ap.add_argument("--probe", action="store_true", help="short run for rocprofv3 --pmc passes")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DB", "WRITE_DB"))
     ap.add_argument("--trace-summary", nargs=3, metavar=("DB", "TITLE", "COMMAND"))
@@ -494,7 +559,7 @@ def main():
         return float(np.max(np.abs(x_sol[prec] - x_sol["fp64"])) / np.max(np.abs(x_sol["fp64"])))
 
     achieved = head["smoother_bytes"] / (head["smoother_ms"] * 1e-3) / 1e9
-    copy_gbs = copy_bandwidth()
+    stream = stream_rates()
     traffic = committed_traffic()
     out = {
         "metric": METRIC,
@@ -530,13 +595,18 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": SWEEP_KERNEL, "bytes_per_launch": head["smoother_bytes"],
                      "kernel_compulsory_bytes": head["smoother_kernel_bytes"],
-                     "launch_ms": head["smoother_ms"], "copy_GBps": copy_gbs,
-                     "frac_of_copy": achieved / copy_gbs},
+                     "launch_ms": head["smoother_ms"], "copy_GBps": stream["copy"]["GBps"],
+                     "stream_GBps": stream["sweep_mix"]["GBps"],
+                     "frac_of_stream": achieved / stream["sweep_mix"]["GBps"],
+                     "stream_note": "hand-written float4 passes over 2^28-float arrays (beyond the Infinity "
+                                    "Cache): copy, and five reads per write as the sweep"},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"], out["cpu_sample_on_gpu"] = cpu_baseline(tuple(args.cpu_levels), args.rtol)
     else:
         out["cpu_baseline"] = None
+    if args.cpu_full:
+        out["cpu_full"] = cpu_full(case, x_sol, args.rtol)
     print(json.dumps(out), flush=True)
 
 

@@ -292,6 +292,13 @@ int tpamr_project_adjoint_device(tpamr_ctx* ctx, const double* d_ubar, const dou
  * the context stream); ms per launch in *avg_ms and its algorithmic bytes
  * per launch in *bytes. */
 int tpamr_time_smoother(tpamr_ctx* ctx, int reps, double* avg_ms, double* bytes);
+/* HBM streaming rate of the smoother's access mix (no reference
+ * counterpart): a float4 pass over `reads` (1: a copy; 5: the Jacobi
+ * sweep's five reads per write) input arrays of n floats each into one
+ * output, `reps` launches timed with HIP events on the context stream after
+ * one untimed pass; ms per launch in *avg_ms, bytes read plus written per
+ * launch in *bytes. Allocates (reads + 1) n floats for the call. */
+int tpamr_time_stream(tpamr_ctx* ctx, long long n, int reads, int reps, double* avg_ms, double* bytes);
 
 #ifdef __cplusplus
 }

@@ -99,6 +99,10 @@ struct Operator {
   bool K_diagonal;
   std::vector<double> kappa, sqrt_kappa, volumes;
   CSC A;  // assembled symmetric stiffness (pressure: matrix == stiffness)
+  // cell-face incidence in increasing face order (a face's minus entry
+  // before its plus entry): the order a face loop scatters to a cell
+  std::vector<int> inc_ptr, inc_face;
+  std::vector<signed char> inc_sign;
   Components comp;
   bool singular;
 };
@@ -111,6 +115,9 @@ struct Level {
   std::vector<int> comp_index;  // per cell: 1-based component slot or 0
   std::vector<double> comp_volumes;
   std::vector<double> x, b, r, t;
+  // row form of P (built on first use): per fine row, its entries in
+  // increasing column order
+  std::vector<int> prow_ptr, prow_col, prow_ent;
 };
 
 struct Solver {

@@ -50,6 +50,14 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
+def set_threads(n):
+    """OpenMP threads of the oracle's loops; returns the previous count (the
+    oracle's results do not depend on it)."""
+    prev = lib().orc_get_threads()
+    lib().orc_set_threads(int(n))
+    return prev
+
+
 def _check(rc):
     if rc != 0:
         raise RuntimeError(lib().orc_last_error().decode())

@@ -1,5 +1,6 @@
 // Oracle C API for the Python test harness (ctypes). TEST INFRASTRUCTURE ONLY.
 #include <cstring>
+#include <omp.h>
 #include <exception>
 #include <string>
 
@@ -55,6 +56,10 @@ static thread_local std::string g_err;
 extern "C" {
 
 const char* orc_last_error() { return g_err.c_str(); }
+
+// OpenMP threads of the oracle's loops (results do not depend on the count)
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads() { return omp_get_max_threads(); }
 
 // refined: n_refined triples (level, i, j) of blocks the refine predicate split
 int orc_mesh_new(const int* base, const int* bs, const double* origin, const double* extent,

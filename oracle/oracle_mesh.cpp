@@ -1,6 +1,7 @@
 // Oracle: block-quadtree mesh, face schedule and inverse face metric K.
 // TEST INFRASTRUCTURE ONLY (see oracle.hpp).
 #include <algorithm>
+#include <parallel/algorithm>
 #include <cmath>
 #include <cstdio>
 #include <map>
@@ -398,7 +399,7 @@ CSC inverse_metric(const Mesh& M, bool* diagonal) {
       eps[2 * idx + e] = {i, j, 2 * idx + e};
     }
   }
-  std::stable_sort(eps.begin(), eps.end(), [](const EP& a, const EP& b) {
+  __gnu_parallel::stable_sort(eps.begin(), eps.end(), [](const EP& a, const EP& b) {
     return a.k1 != b.k1 ? a.k1 < b.k1 : a.k2 < b.k2;
   });
   std::vector<int> rows, cols;
@@ -455,7 +456,7 @@ CSC inverse_metric(const Mesh& M, bool* diagonal) {
   // sparse(rows, cols, vals): duplicates summed in input order; dropzeros!
   std::vector<int> order(rows.size());
   for (size_t q = 0; q < order.size(); ++q) order[q] = int(q);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+  __gnu_parallel::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     return cols[a] != cols[b] ? cols[a] < cols[b] : rows[a] < rows[b];
   });
   CSC K;

@@ -2,6 +2,8 @@
 // the reference's stopping rules, two-phase coefficients, projection and the
 // pressure adjoint. TEST INFRASTRUCTURE ONLY (see oracle.hpp).
 #include <algorithm>
+#include <parallel/algorithm>
+#include <omp.h>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -15,6 +17,39 @@ namespace orc {
 
 static const double EPS = std::numeric_limits<double>::epsilon();
 
+// `_chunked_slot_sums!` (src/threading.jl:202-283): items summed in index
+// order within fixed chunks of 1024, chunk sums added in chunk order, so the
+// result does not depend on the thread count (the chunks run on OpenMP
+// threads). `slot(i)` is the 1-based slot of item i (0 skips it).
+static constexpr long REDUCTION_CHUNK = 1024;
+
+template <class Slot, class Term>
+static std::vector<double> chunked_slot_sums(long n, size_t K, Slot&& slot, Term&& term) {
+  std::vector<double> sums(K, 0.0);
+  const long chunks = (n + REDUCTION_CHUNK - 1) / REDUCTION_CHUNK;
+  if (chunks <= 1) {
+    for (long i = 0; i < n; ++i) {
+      const int k = slot(i);
+      if (k) sums[k - 1] += term(i);
+    }
+    return sums;
+  }
+  std::vector<double> part(size_t(chunks) * K, 0.0);
+#pragma omp parallel for schedule(static)
+  for (long ch = 0; ch < chunks; ++ch) {
+    double* pp = &part[size_t(ch) * K];
+    const long hi = std::min(n, (ch + 1) * REDUCTION_CHUNK);
+    for (long i = ch * REDUCTION_CHUNK; i < hi; ++i) {
+      const int k = slot(i);
+      if (k) pp[k - 1] += term(i);
+    }
+  }
+  for (long ch = 0; ch < chunks; ++ch)
+    for (size_t k = 0; k < K; ++k) sums[k] += part[size_t(ch) * K + k];
+  return sums;
+}
+
+
 // COO -> CSC with duplicate summation in input order (SparseArrays `sparse`).
 // Explicit zeros are kept: the reference keeps cancelled stiffness entries in
 // the pattern (elliptic.jl:166-171).
@@ -22,7 +57,7 @@ static CSC coo_to_csc(int m, int n, const std::vector<int>& r, const std::vector
                       const std::vector<double>& v) {
   std::vector<int> order(r.size());
   for (size_t q = 0; q < order.size(); ++q) order[q] = int(q);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+  __gnu_parallel::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     return c[a] != c[b] ? c[a] < c[b] : r[a] < r[b];
   });
   CSC A;
@@ -168,6 +203,25 @@ static std::unique_ptr<Operator> make_operator(const Mesh* mesh, const std::vect
   for (double v : op->volumes)
     if (!(std::isfinite(v) && v >= 0)) throw std::invalid_argument("cell volumes must be finite and nonnegative");
   op->A = integrated_stiffness(*mesh, op->K, op->K_diagonal, op->kappa, op->sqrt_kappa);
+  {
+    const int N = mesh->ncells(), F = int(mesh->faces.size());
+    op->inc_ptr.assign(N + 1, 0);
+    for (int f = 0; f < F; ++f) {
+      op->inc_ptr[mesh->faces[f].minus + 1]++;
+      op->inc_ptr[mesh->faces[f].plus + 1]++;
+    }
+    for (int c = 0; c < N; ++c) op->inc_ptr[c + 1] += op->inc_ptr[c];
+    op->inc_face.resize(op->inc_ptr[N]);
+    op->inc_sign.resize(op->inc_ptr[N]);
+    std::vector<int> next(op->inc_ptr.begin(), op->inc_ptr.end() - 1);
+    for (int f = 0; f < F; ++f) {
+      const int m = mesh->faces[f].minus, pl = mesh->faces[f].plus;
+      op->inc_face[next[m]] = f;
+      op->inc_sign[next[m]++] = -1;
+      op->inc_face[next[pl]] = f;
+      op->inc_sign[next[pl]++] = 1;
+    }
+  }
   op->comp = components(*mesh, op->kappa, op->volumes);
   op->singular = !op->comp.anchors.empty();
   return op;
@@ -178,32 +232,41 @@ static void factorized_mul(const Operator& op, const double* x, double* y) {
   const Mesh& M = *op.mesh;
   const int F = int(M.faces.size());
   std::vector<double> fin(F), fout(F, 0.0);
+#pragma omp parallel for schedule(static)
   for (int f = 0; f < F; ++f) {
     const Face& fa = M.faces[f];
     fin[f] = op.sqrt_kappa[f] * fa.area * (x[fa.plus] - x[fa.minus]);
   }
   if (op.K_diagonal) {
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < F; ++f) fout[f] = op.K.nzval[op.K.colptr[f]] * fin[f];
   } else {
     // CSR row product on the symmetric K (column r == row r)
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < F; ++f) {
       double s = 0;
       for (int p = op.K.colptr[f]; p < op.K.colptr[f + 1]; ++p) s += op.K.nzval[p] * fin[op.K.rowval[p]];
       fout[f] = s;
     }
   }
+#pragma omp parallel for schedule(static)
+  for (int f = 0; f < F; ++f) fin[f] = 1.0 * M.faces[f].area * op.sqrt_kappa[f] * fout[f];
+  // cell gather in face order (the cell-face incidence order): the sums of
+  // the face loop's scatter, term for term
   const int N = M.ncells();
-  for (int c = 0; c < N; ++c) y[c] = 0.0 * x[c];
-  // cell gather in face order (the cell-face incidence order)
-  for (int f = 0; f < F; ++f) {
-    const Face& fa = M.faces[f];
-    const double flux = 1.0 * fa.area * op.sqrt_kappa[f] * fout[f];
-    y[fa.minus] += -1.0 * flux;
-    y[fa.plus] += flux;
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < N; ++c) {
+    double v = 0.0 * x[c];
+    for (int q = op.inc_ptr[c]; q < op.inc_ptr[c + 1]; ++q) {
+      const double flux = fin[op.inc_face[q]];
+      v += op.inc_sign[q] < 0 ? -1.0 * flux : flux;
+    }
+    y[c] = v;
   }
 }
 
 static void csc_sym_mul(const CSC& A, const double* x, double* y) {
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < A.n; ++i) {
     double s = 0;
     for (int p = A.colptr[i]; p < A.colptr[i + 1]; ++p) s += A.nzval[p] * x[A.rowval[p]];
@@ -556,30 +619,37 @@ Solver* build_solver(const Mesh& fine, const std::vector<double>& kappa, int coa
 // ---------------------------------------------------------------------------
 // V-cycle, src/solvers/multigrid/cycles.jl:8-77 and hierarchy.jl:208-343
 
+// `_multigrid_project_component_rhs!` (hierarchy.jl:326-343)
 static void project_component_rhs(Level& L, std::vector<double>& b) {
   const size_t K = L.comp_volumes.size();
-  std::vector<double> s(K, 0.0);
-  for (size_t c = 0; c < b.size(); ++c)
-    if (L.comp_index[c]) s[L.comp_index[c] - 1] += b[c];
+  const int* idx = L.comp_index.data();
+  std::vector<double> s = chunked_slot_sums(long(b.size()), K, [&](long c) { return idx[c]; },
+                                            [&](long c) { return b[c]; });
   for (size_t q = 0; q < K; ++q) s[q] /= L.comp_volumes[q];
-  for (size_t c = 0; c < b.size(); ++c)
-    if (L.comp_index[c]) b[c] -= L.op->volumes[c] * s[L.comp_index[c] - 1];
+  const double* V = L.op->volumes.data();
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < long(b.size()); ++c)
+    if (idx[c]) b[c] -= V[c] * s[idx[c] - 1];
 }
 
+// `_multigrid_remove_component_means!` (hierarchy.jl:309-324)
 static void remove_component_means(Level& L, std::vector<double>& x) {
   const size_t K = L.comp_volumes.size();
-  std::vector<double> s(K, 0.0);
-  for (size_t c = 0; c < x.size(); ++c)
-    if (L.comp_index[c]) s[L.comp_index[c] - 1] += L.op->volumes[c] * x[c];
+  const int* idx = L.comp_index.data();
+  const double* V = L.op->volumes.data();
+  std::vector<double> s = chunked_slot_sums(long(x.size()), K, [&](long c) { return idx[c]; },
+                                            [&](long c) { return V[c] * x[c]; });
   for (size_t q = 0; q < K; ++q) s[q] /= L.comp_volumes[q];
-  for (size_t c = 0; c < x.size(); ++c)
-    if (L.comp_index[c]) x[c] -= s[L.comp_index[c] - 1];
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < long(x.size()); ++c)
+    if (idx[c]) x[c] -= s[idx[c] - 1];
 }
 
 static void smooth(Level& L, int sweeps, bool singular) {
   for (int s = 0; s < sweeps; ++s) {
     csc_sym_mul(L.op->A, L.x.data(), L.t.data());
-    for (size_t i = 0; i < L.x.size(); ++i) L.x[i] += L.omega * L.invdiag[i] * (L.b[i] - L.t[i]);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < long(L.x.size()); ++i) L.x[i] += L.omega * L.invdiag[i] * (L.b[i] - L.t[i]);
     if (singular) remove_component_means(L, L.x);
   }
 }
@@ -598,9 +668,11 @@ static void vcycle(Solver& S, int l) {
   }
   smooth(L, S.pre, singular);
   csc_sym_mul(L.op->A, L.x.data(), L.t.data());
-  for (size_t i = 0; i < L.r.size(); ++i) L.r[i] = L.b[i] - L.t[i];
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < long(L.r.size()); ++i) L.r[i] = L.b[i] - L.t[i];
   Level& C = S.levels[l + 1];
   // `_restrict!`: CSC column dots with muladd, plus zero
+#pragma omp parallel for schedule(static)
   for (int col = 0; col < L.P.n; ++col) {
     double v = 0.0;
     for (int p = L.P.colptr[col]; p < L.P.colptr[col + 1]; ++p) v = std::fma(L.P.nzval[p], L.r[L.P.rowval[p]], v);
@@ -609,9 +681,7 @@ static void vcycle(Solver& S, int l) {
   std::fill(C.x.begin(), C.x.end(), 0.0);
   vcycle(S, l + 1);
   // `_prolongate!`: rows in increasing column order with muladd
-  std::fill(L.t.begin(), L.t.end(), 0.0);
-  {
-    // build row form once per call (small cost for the oracle)
+  if (L.prow_ptr.empty()) {
     std::vector<int> cnt(L.P.m + 1, 0);
     for (int p = 0; p < L.P.nnz(); ++p) cnt[L.P.rowval[p] + 1]++;
     for (int i = 0; i < L.P.m; ++i) cnt[i + 1] += cnt[i];
@@ -623,13 +693,19 @@ static void vcycle(Solver& S, int l) {
         ent[next[row]] = p;
         next[row]++;
       }
-    for (int row = 0; row < L.P.m; ++row) {
-      double v = 0.0;
-      for (int q = cnt[row]; q < cnt[row + 1]; ++q) v = std::fma(L.P.nzval[ent[q]], C.x[cols[q]], v);
-      L.t[row] = v;
-    }
+    L.prow_ptr = std::move(cnt);
+    L.prow_col = std::move(cols);
+    L.prow_ent = std::move(ent);
   }
-  for (size_t i = 0; i < L.x.size(); ++i) L.x[i] += L.t[i];
+#pragma omp parallel for schedule(static)
+  for (int row = 0; row < L.P.m; ++row) {
+    double v = 0.0;
+    for (int q = L.prow_ptr[row]; q < L.prow_ptr[row + 1]; ++q)
+      v = std::fma(L.P.nzval[L.prow_ent[q]], C.x[L.prow_col[q]], v);
+    L.t[row] = v;
+  }
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < long(L.x.size()); ++i) L.x[i] += L.t[i];
   if (singular) remove_component_means(L, L.x);
   smooth(L, S.post, singular);
 }
@@ -651,12 +727,28 @@ void apply_vcycle(Solver& S, const double* r, double* z) {
 // max |r_i| / V_i over cells of positive volume; a NaN term is returned.
 template <typename T>
 static T strong_max(const std::vector<T>& r, const std::vector<T>& V) {
+  const long n = long(r.size());
+  const long chunks = (n + REDUCTION_CHUNK - 1) / REDUCTION_CHUNK;
+  std::vector<T> part(std::max(chunks, 1L), T(0));
+#pragma omp parallel for schedule(static)
+  for (long ch = 0; ch < chunks; ++ch) {
+    T m = 0;
+    const long hi = std::min(n, (ch + 1) * REDUCTION_CHUNK);
+    for (long i = ch * REDUCTION_CHUNK; i < hi; ++i) {
+      if (!(V[i] > 0)) continue;
+      const T q = std::fabs(r[i]) / V[i];
+      if (std::isnan(q)) {
+        m = q;
+        break;
+      }
+      m = std::max(m, q);
+    }
+    part[ch] = m;
+  }
   T m = 0;
-  for (size_t i = 0; i < r.size(); ++i) {
-    if (!(V[i] > 0)) continue;
-    T q = std::fabs(r[i]) / V[i];
-    if (std::isnan(q)) return q;
-    m = std::max(m, q);
+  for (long ch = 0; ch < chunks; ++ch) {
+    if (std::isnan(part[ch])) return part[ch];
+    m = std::max(m, part[ch]);
   }
   return m;
 }
@@ -697,20 +789,26 @@ static double roundoff_floor(const Operator& op, const std::vector<double>& x,
   const Mesh& M = *op.mesh;
   const int F = int(M.faces.size());
   std::vector<double> fin(F), fout(F);
+#pragma omp parallel for schedule(static)
   for (int f = 0; f < F; ++f)
     fin[f] = op.sqrt_kappa[f] * M.faces[f].area * (std::fabs(x[M.faces[f].minus]) + std::fabs(x[M.faces[f].plus]));
+#pragma omp parallel for schedule(static)
   for (int f = 0; f < F; ++f) {
     double s = 0;
     for (int p = op.K.colptr[f]; p < op.K.colptr[f + 1]; ++p) s += std::fabs(op.K.nzval[p]) * fin[op.K.rowval[p]];
     fout[f] = s;
   }
+#pragma omp parallel for schedule(static)
   for (int f = 0; f < F; ++f) fin[f] = 1.0 * M.faces[f].area * op.sqrt_kappa[f] * fout[f];
+  // each cell gathers its face bounds in incidence (face) order, the order of
+  // the face loop's scatter (cycles.jl:108-112)
   const int N = M.ncells();
-  std::vector<double> bound(N, 0.0);
-  for (int c = 0; c < N; ++c) bound[c] = 0.0 * std::fabs(x[c]);
-  for (int f = 0; f < F; ++f) {
-    bound[M.faces[f].minus] += fin[f];
-    bound[M.faces[f].plus] += fin[f];
+  std::vector<double> bound(N);
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < N; ++c) {
+    double v = 0.0 * std::fabs(x[c]);
+    for (int q = op.inc_ptr[c]; q < op.inc_ptr[c + 1]; ++q) v += fin[op.inc_face[q]];
+    bound[c] = v;
   }
   double scale = 0;
   for (int c = 0; c < N; ++c) {
@@ -731,18 +829,24 @@ double solver_floor(Solver& S, const double* x, const double* b, bool assembled)
   return roundoff_floor(op, xv, bv);
 }
 
+// `_remove_component_volume_means!` (elliptic_solver.jl:130-148)
 static void remove_volume_means(const Operator& op, std::vector<double>& x) {
   const Components& C = op.comp;
-  std::vector<double> s(C.anchors.size(), 0.0);
-  for (size_t c = 0; c < x.size(); ++c)
-    if (C.slots[c]) s[C.slots[c] - 1] += op.volumes[c] * x[c];
+  const int* slots = C.slots.data();
+  const double* V = op.volumes.data();
+  std::vector<double> s = chunked_slot_sums(long(x.size()), C.anchors.size(), [&](long c) { return slots[c]; },
+                                            [&](long c) { return V[c] * x[c]; });
   for (size_t q = 0; q < s.size(); ++q) s[q] /= C.volumes[q];
-  for (size_t c = 0; c < x.size(); ++c) {
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < long(x.size()); ++c) {
     if (C.labels[c] < 0) x[c] = 0.0;
-    else if (C.slots[c]) x[c] -= s[C.slots[c] - 1];
+    else if (slots[c]) x[c] -= s[slots[c] - 1];
   }
 }
 
+// Krylov.jl's dot (a BLAS dot, whose summation order is the library's):
+// index order. Fixed chunks instead would move plain CG on the techref
+// composite 64^2 mesh from the published 780 iterations to 781.
 static double dotv(const std::vector<double>& a, const std::vector<double>& b) {
   double s = 0;
   for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
@@ -758,7 +862,8 @@ static int krylov_cg(Solver& S, const std::vector<double>& b, std::vector<double
   const size_t n = b.size();
   std::vector<double> x(n, 0.0), r(n), p(n), Ap(n), z(n);
   factorized_mul(op, x0.data(), r.data());
-  for (size_t i = 0; i < n; ++i) r[i] = b[i] - r[i];
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < long(n); ++i) r[i] = b[i] - r[i];
   auto precond = [&](const std::vector<double>& in, std::vector<double>& out) {
     if (S.use_mg) apply_vcycle(S, in.data(), out.data());
     else out = in;
@@ -769,7 +874,8 @@ static int krylov_cg(Solver& S, const std::vector<double>& b, std::vector<double
   if (gamma < 0) throw std::runtime_error("Preconditioner is not positive definite");
   double rnorm = std::sqrt(gamma);
   if (gamma == 0) {
-    for (size_t i = 0; i < n; ++i) x0[i] = x[i] + x0[i];
+  #pragma omp parallel for schedule(static)
+  for (long i = 0; i < long(n); ++i) x0[i] = x[i] + x0[i];
     return 0;
   }
   int iter = 0;
@@ -785,8 +891,11 @@ static int krylov_cg(Solver& S, const std::vector<double>& b, std::vector<double
     }
     if (zero_curv) continue;
     const double alpha = gamma / pAp;
-    for (size_t i = 0; i < n; ++i) x[i] += alpha * p[i];
-    for (size_t i = 0; i < n; ++i) r[i] += -alpha * Ap[i];
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < long(n); ++i) {
+      x[i] += alpha * p[i];
+      r[i] += -alpha * Ap[i];
+    }
     precond(r, z);
     const double gamma_next = dotv(r, z);
     rnorm = std::sqrt(gamma_next);
@@ -796,13 +905,14 @@ static int krylov_cg(Solver& S, const std::vector<double>& b, std::vector<double
       const double beta = gamma_next / gamma;
       pnorm2 = gamma_next + beta * beta * pnorm2;
       gamma = gamma_next;
-      for (size_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+#pragma omp parallel for schedule(static)
+      for (long i = 0; i < long(n); ++i) p[i] = z[i] + beta * p[i];
     }
     ++iter;
     tired = iter >= itmax;
     user_exit = strong_max(r, V) <= target;
   }
-  for (size_t i = 0; i < n; ++i) x0[i] = x[i] + x0[i];
+  for (long i = 0; i < long(n); ++i) x0[i] = x[i] + x0[i];
   return iter;
 }
 
@@ -842,7 +952,8 @@ static DriverStats<T> krylov_driver(const std::vector<T>& V, const std::vector<T
   std::vector<T> prod(n), res(n);
   auto true_residual = [&]() {
     mul(x, prod);
-    for (size_t i = 0; i < n; ++i) res[i] = b[i] - prod[i];
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < long(n); ++i) res[i] = b[i] - prod[i];
   };
   true_residual();
   DriverStats<T> st{};
@@ -970,12 +1081,12 @@ Stats solve(Solver& S, const double* rhs, double* xio, double rtol, double atol,
   if (op.singular) {
     const Components& C = op.comp;
     const size_t K = C.anchors.size();
-    std::vector<double> sums(K, 0.0), scales(K, 0.0);
-    for (size_t c = 0; c < n; ++c)
-      if (C.slots[c]) {
-        sums[C.slots[c] - 1] += b[c];
-        scales[C.slots[c] - 1] += std::fabs(b[c]);
-      }
+    // `_project_component_rhs!` (elliptic_solver.jl:150-190)
+    const int* slots = C.slots.data();
+    std::vector<double> sums = chunked_slot_sums(long(n), K, [&](long c) { return slots[c]; },
+                                                 [&](long c) { return b[c]; });
+    std::vector<double> scales = chunked_slot_sums(long(n), K, [&](long c) { return slots[c]; },
+                                                   [&](long c) { return std::fabs(b[c]); });
     double total_sum = 0, total_vol = 0;
     for (size_t q = 0; q < K; ++q) {
       const double tol = 256 * EPS * std::max(1.0, scales[q]);

@@ -7,6 +7,11 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+# the oracle's OpenMP loops: idle threads yield instead of spinning, and
+# parallel test workers share the host's cores
+os.environ.setdefault("OMP_WAIT_POLICY", "passive")
+if os.environ.get("PYTEST_XDIST_WORKER"):
+    os.environ.setdefault("OMP_NUM_THREADS", "2")
 sys.path.insert(0, os.path.join(ROOT, "twophaseamr.jl_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))

@@ -152,9 +152,10 @@ def test_stopping_statistics_match_oracle(name, centre, radius, precision):
     assert sg.final_linf <= sg.target_linf
     if precision == "fp64":
         assert sg.cycles == so["cycles"]
-        # the last iterate's residual carries the round-off of both codes
-        assert math.isclose(sg.final_linf, so["final_linf"], rel_tol=1e-3)
-        assert math.isclose(sg.final_l2, so["final_l2"], rel_tol=1e-3)
+        # the last iterate's residual (here ~3e-11 of the initial one at
+        # 1000:1) carries the round-off of both codes' V-cycles
+        assert math.isclose(sg.final_linf, so["final_linf"], rel_tol=1e-2)
+        assert math.isclose(sg.final_l2, so["final_l2"], rel_tol=1e-2)
     gs.close()
 
 

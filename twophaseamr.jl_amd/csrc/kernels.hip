@@ -1604,6 +1604,40 @@ template void launch_cg_update_xr<double>(hipStream_t, int, int, const double*, 
 template void launch_cg_update_p<float>(hipStream_t, int, const float*, double*, const CgState*, const int*);
 template void launch_cg_update_p<double>(hipStream_t, int, const double*, double*, const CgState*, const int*);
 
+// Streaming pass for the HBM ceiling of a given access mix: out = sum of R
+// input arrays, one float4 of each per thread (R = 1: a plain copy; R = 5:
+// the Jacobi sweep's five reads per write).
+template <int R>
+__global__ void __launch_bounds__(256) k_stream(size_t n4, size_t stride4, const float4* __restrict__ in,
+                                                float4* __restrict__ out) {
+  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n4) return;
+  float4 v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = in[size_t(r) * stride4 + i];
+  float4 s = v[0];
+#pragma unroll
+  for (int r = 1; r < R; ++r) {
+    s.x += v[r].x;
+    s.y += v[r].y;
+    s.z += v[r].z;
+    s.w += v[r].w;
+  }
+  out[i] = s;
+}
+
+void launch_stream(hipStream_t s, int reads, size_t n4, const float* in, float* out) {
+  const dim3 g(unsigned((n4 + 255) / 256)), t(256);
+  const float4* a = reinterpret_cast<const float4*>(in);
+  float4* o = reinterpret_cast<float4*>(out);
+  if (reads == 1)
+    hipLaunchKernelGGL(k_stream<1>, g, t, 0, s, n4, n4, a, o);
+  else if (reads == 5)
+    hipLaunchKernelGGL(k_stream<5>, g, t, 0, s, n4, n4, a, o);
+  else
+    throw std::invalid_argument("stream pass: 1 or 5 input arrays");
+}
+
 // Scratch of tpamr_solve_device_noalloc: the Krylov state and the reduction
 // partials of one solve, carved at 256-byte offsets. Each GPU holds its own
 // copy; host code reaches it only through the address that

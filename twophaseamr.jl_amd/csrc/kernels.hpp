@@ -229,6 +229,9 @@ void launch_copy(hipStream_t s, int n, const double* in, double* out);
 void launch_fill(hipStream_t s, int n, double v, double* out);
 void launch_fill_int(hipStream_t s, int n, int v, int* out);
 
+// out = sum of `reads` (1 or 5) consecutive input arrays of n4 float4 each
+void launch_stream(hipStream_t s, int reads, size_t n4, const float* in, float* out);
+
 // Device scratch of the allocation-free solve entry point: a file-scope
 // array of this many bytes on every GPU the library runs on, held until the
 // process ends.

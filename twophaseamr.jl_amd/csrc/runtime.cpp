@@ -2256,6 +2256,30 @@ int tpamr_fill_halo(tpamr_ctx* c, int halo, const double* cells, const double* p
   });
 }
 
+int tpamr_time_stream(tpamr_ctx* c, long long n, int reads, int reps, double* avg_ms, double* bytes) {
+  if (!c || !avg_ms || reps <= 0) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    if (reads != 1 && reads != 5) throw std::invalid_argument("reads must be 1 or 5");
+    if (n <= 0 || n % 4 != 0) throw std::invalid_argument("n must be a positive multiple of 4");
+    HIPCHECK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    DevArray<float> in, out;
+    in.alloc(size_t(n) * reads);
+    out.alloc(size_t(n));
+    HIPCHECK(hipMemsetAsync(in.p, 0, sizeof(float) * size_t(n) * reads, s));
+    launch_stream(s, reads, size_t(n) / 4, in.p, out.p);  // first touch
+    HIPCHECK(hipEventRecord(c->ev0, s));
+    for (int q = 0; q < reps; ++q) launch_stream(s, reads, size_t(n) / 4, in.p, out.p);
+    HIPCHECK(hipEventRecord(c->ev1, s));
+    HIPCHECK(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *avg_ms = double(ms) / reps;
+    if (bytes) *bytes = double(n) * 4.0 * (reads + 1);
+  });
+}
+
 int tpamr_time_smoother(tpamr_ctx* c, int reps, double* avg_ms, double* bytes) {
   if (!c || !avg_ms || reps <= 0) return TPAMR_EINVAL;
   return guard(c, [&] {
