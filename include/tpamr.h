@@ -107,6 +107,12 @@ int tpamr_set_face_coefficients(tpamr_ctx* ctx, const double* kappa);
 
 /* hierarchy inspection */
 int tpamr_num_levels(const tpamr_ctx* ctx);
+/* Levels the last hierarchy build took over from the previous mesh's
+ * hierarchy (a mesh replaced by tpamr_set_mesh, same multigrid options):
+ * levels new_start.. of this hierarchy are old_start.. of that one, as the
+ * reference's _tree_reuse_suffix! (tree_coarsening.jl:214-226) with
+ * _same_tree_topology (:84-121); -1, -1 when nothing was reused. */
+int tpamr_hierarchy_reuse(const tpamr_ctx* ctx, int* new_start, int* old_start);
 /* ints[6] = ncells, nfaces, nblocks, irregular rows, nnz(A), max leaf level; dbl[2] = omega, omega_P */
 int tpamr_level_info(const tpamr_ctx* ctx, int level, int* ints, double* dbl);
 /* CSR of the level operator A (integrated, symmetric): pass NULL arrays to query nnz in *nnz */

@@ -67,6 +67,11 @@ def test_one_context_across_meshes(precision):
             k_first = k * np.exp(0.3 * np.sin(np.arange(1, pm.nfaces + 1)))
             for kk in (k_first, k, k):  # build + host refresh, then two device refreshes
                 L.check(L.lib().tpamr_set_face_coefficients(ctx.h, L.ptr(np.ascontiguousarray(kk))), ctx.h)
+            ns, os_start = C.c_int(), C.c_int()
+            L.check(L.lib().tpamr_hierarchy_reuse(ctx.h, C.byref(ns), C.byref(os_start)), ctx.h)
+            # amr32 after amr64 takes over the previous agglomerated levels
+            # 1024, 256 and 64 (and the coarse inverse as a Newton-Schulz start)
+            assert (ns.value, os_start.value) == ((2, 3) if name == "amr32" else (-1, -1))
             b = techref_noise(pm.ncells)
             x = np.zeros(pm.ncells)
             st = raw_solve(ctx, b, x, 1e-10)
@@ -257,3 +262,34 @@ def test_shim_solve_override_mirror():
         assert math.isclose(st.target_linf, so["target_linf"], rel_tol=1e-11)
     finally:
         lib.tpamr_destroy(h)
+
+
+def test_coarsest_level_above_8192_cells():
+    """A 3 x 3 layout of 32^2 roots cannot agglomerate: the hierarchy stops at
+    9216 cells, solved by the dense inverse from the blocked Gauss-Jordan.
+    The pressure equals the exact solution of the oracle's assembled operator
+    (sparse direct solve of the augmented system [A V; V^T 0])."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    _print("case coarse 9216")
+    kw = dict(base_mesh=(96, 96), block_size=(32, 32), periodic=(False, False), max_level=1,
+              refine=lambda k, o, e: k[0] == 0 and (k[1] + k[2]) % 2 == 0)
+    om, pm = OMesh(**kw), tpamr.BlockTreeMesh(**kw)
+    k = np.exp(0.5 * np.sin(np.arange(1, pm.nfaces + 1)))
+    A = OSolver(om, k, multigrid=False).level_matrix(0).tocsc()
+    V = om.volumes
+    n = pm.ncells
+    aug = sp.bmat([[A, sp.csc_matrix(V.reshape(-1, 1))], [sp.csc_matrix(V.reshape(1, -1)), None]]).tocsc()
+    b = techref_noise(n)
+    bi = b * V
+    bi -= V * bi.sum() / V.sum()
+    exact = spla.spsolve(aug, np.concatenate([bi, [0.0]]))[:n]
+    for precision in ("fp64", "fp32_refine"):
+        gs = tpamr.EllipticSolver(tpamr.PoissonOperator(pm, k), precision=precision)
+        sizes = [lv["ncells"] for lv in gs.levels]
+        assert sizes[-1] == 9216, sizes
+        x = np.zeros(n)
+        st = gs.solve(x, b, rtol=1e-12, max_cycles=100)
+        assert st.converged
+        assert rel(x, exact) < 1e-9, (precision, rel(x, exact))
+        gs.close()

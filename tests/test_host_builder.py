@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import tpamr
-from cases import MESHES
+from cases import MESHES, near_circle
 from oracle import OMesh, OSolver
 
 CASES = [("uniform64", "one"), ("periodic_tg", "one"), ("amr32", "one"), ("amr64", "one"),
@@ -74,3 +74,38 @@ def test_threaded_builder_does_not_depend_on_thread_count(tmp_path):
     assert sorted(a.files) == sorted(b.files)
     for k in a.files:
         assert np.array_equal(a[k], b[k]), k
+
+
+def _circle_mesh(centre):
+    return dict(base_mesh=(16, 16), block_size=(16, 16), periodic=(False, False), max_level=4,
+                refine=lambda k, o, e: k[0] < 2 or near_circle(o, e, centre=centre, radius=0.15, band=4.0 / 256))
+
+
+@pytest.mark.parametrize("seq", [[(0.5, 0.6), (0.53, 0.58)], [(0.5, 0.6), (0.3, 0.3), (0.5, 0.6)]])
+def test_hierarchy_reuse_after_remesh(seq):
+    """A context that changes mesh takes over the coarse levels whose tree
+    topology is unchanged (_tree_reuse_suffix!, tree_coarsening.jl:214-226):
+    the hierarchy equals a fresh build of the new mesh and the oracle's."""
+    h = None
+    for centre in seq:
+        kw = _circle_mesh(centre)
+        pm = tpamr.BlockTreeMesh(**kw)
+        k = kappa(pm, "smooth")
+        if h is None:
+            h = tpamr.HostHierarchy(pm, k)
+            assert h.reuse() == (-1, -1)
+            continue
+        h.remesh(pm, k)
+        new_start, old_start = h.reuse()
+        assert new_start > 0 and old_start >= 0
+        fresh = tpamr.HostHierarchy(pm, k)
+        os_ = OSolver(OMesh(**kw), k)
+        assert h.sizes() == fresh.sizes() == [os_.level_info(l)["ncells"] for l in range(os_.nlevels)]
+        for l in range(h.nlevels):
+            A, B = h.level_matrix(l), fresh.level_matrix(l)
+            assert (A != B).nnz == 0
+            assert abs(os_.level_matrix(l).tocsr() - A).max() <= 1e-15 * abs(A).max()
+            assert np.array_equal(h.level_kappa(l), fresh.level_kappa(l))
+            assert h.level_info(l) == fresh.level_info(l)
+            if l + 1 < h.nlevels:
+                assert np.array_equal(h.level_transfer(l), fresh.level_transfer(l))

@@ -1322,7 +1322,21 @@ void coarse_matrix(Hierarchy& H) {
 
 }  // namespace
 
-std::unique_ptr<Hierarchy> build_hierarchy(std::unique_ptr<TreeMesh> fine, const MultigridConfig& cfg) {
+bool same_tree_topology(const TreeMesh& a, const TreeMesh& b) {
+  for (int q = 0; q < 2; ++q)
+    if (a.base[q] != b.base[q] || a.bs[q] != b.bs[q] || a.periodic[q] != b.periodic[q] ||
+        !(a.origin[q] == b.origin[q]) || !(a.extent[q] == b.extent[q]))
+      return false;
+  if (a.blocks.size() != b.blocks.size() || a.nfaces() != b.nfaces() || a.bcell.size() != b.bcell.size())
+    return false;
+  for (size_t k = 0; k < a.blocks.size(); ++k)
+    if (a.blocks[k].level != b.blocks[k].level || a.blocks[k].i != b.blocks[k].i || a.blocks[k].j != b.blocks[k].j)
+      return false;
+  return true;
+}
+
+std::unique_ptr<Hierarchy> build_hierarchy(std::unique_ptr<TreeMesh> fine, const MultigridConfig& cfg,
+                                           Hierarchy* previous) {
   if (cfg.coarse_size <= 0) throw std::invalid_argument("coarse_size must be positive");
   if (cfg.pre < 0 || cfg.post < 0) throw std::invalid_argument("sweep counts must be nonnegative");
   if (cfg.max_levels <= 0) throw std::invalid_argument("max_levels must be positive");
@@ -1343,6 +1357,26 @@ std::unique_ptr<Hierarchy> build_hierarchy(std::unique_ptr<TreeMesh> fine, const
     pt.mark("coarse mesh", cm ? long(cm->ncells()) : 0L);
     if (!cm || cm->ncells() >= cur.ncells()) break;
     active = cm->max_leaf;
+    if (previous) {
+      int match = -1;
+      for (int l = 0; l < int(previous->levels.size()) && match < 0; ++l)
+        if (previous->levels[l] && same_tree_topology(*previous->levels[l]->mesh, *cm)) match = l;
+      const int suffix = match < 0 ? 0 : int(previous->levels.size()) - match;
+      if (match >= 0 && int(H->levels.size()) + suffix <= cfg.max_levels) {
+        // the matching level's restriction and the transfer into it come
+        // from the new finer level; everything coarser is unchanged
+        H->reused_new = int(H->levels.size());
+        H->reused_old = match;
+        std::unique_ptr<LevelData> L = std::move(previous->levels[match]);
+        restriction_map(*L, *H->levels.back());
+        transfer_descriptors(*H->levels.back(), *L, is_agg);
+        H->levels.push_back(std::move(L));
+        for (int l = match + 1; l < int(previous->levels.size()); ++l)
+          H->levels.push_back(std::move(previous->levels[l]));
+        previous->levels.resize(match);
+        break;
+      }
+    }
     auto L = std::make_unique<LevelData>();
     L->owned = std::move(cm);
     L->mesh = L->owned.get();

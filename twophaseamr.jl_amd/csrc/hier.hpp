@@ -155,6 +155,10 @@ struct Hierarchy {
   int coarse_n = 0;
   std::vector<double> coarse_aug;
   double total_volume = 0;
+  // levels taken over from the previous hierarchy (_tree_reuse_suffix!):
+  // this hierarchy's levels from reused_new on are the previous one's from
+  // reused_old on (-1: none)
+  int reused_new = -1, reused_old = -1;
 };
 
 // Halo refresh schedule of a haloed cell field (fill_halo!, halo width h):
@@ -174,15 +178,26 @@ struct HaloSchedule {
 
 HaloSchedule halo_schedule(const TreeMesh& m, int halo);
 
-// Largest coarsest level (cells) of the dense coarse solve.
-constexpr int COARSE_MAX = 8192;
+// Largest coarsest level (cells) of the dense coarse solve: its augmented
+// inverse is 2.1 GB of FP64 at this size, built by a blocked Gauss-Jordan
+// on the device and applied as one matrix-vector product per V-cycle.
+constexpr int COARSE_MAX = 16384;
 
 struct MultigridConfig {
   int coarse_size = 64, pre = 2, post = 2, max_levels = 32;
 };
 
-// Build the whole hierarchy for fine mesh `fine` (takes ownership).
-std::unique_ptr<Hierarchy> build_hierarchy(std::unique_ptr<TreeMesh> fine, const MultigridConfig& cfg);
+// Multigrid hierarchy of a tree mesh (TreeGeometricMultigridPreconditioner,
+// geometric_preconditioner.jl:171-442). With `previous` (a hierarchy of an
+// earlier mesh built with the same options), the first coarse mesh whose
+// tree topology equals one of its levels ends the coarsening: that level and
+// the coarser ones are taken over from it (_tree_reuse_suffix!,
+// tree_coarsening.jl:214-226), and `previous` loses them.
+std::unique_ptr<Hierarchy> build_hierarchy(std::unique_ptr<TreeMesh> fine, const MultigridConfig& cfg,
+                                           Hierarchy* previous = nullptr);
+// _same_tree_topology (tree_coarsening.jl:84-121): domain, block size and
+// leaf keys (faces and geometry follow from them)
+bool same_tree_topology(const TreeMesh& a, const TreeMesh& b);
 // Refresh all coefficient-dependent values for new fine face coefficients.
 void refresh_coefficients(Hierarchy& H, const double* kappa);
 // Values of the coarsest level alone from its own face coefficients, and its

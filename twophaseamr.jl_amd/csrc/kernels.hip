@@ -793,6 +793,113 @@ __global__ void k_gj_update(int N, int k, double* __restrict__ a, const double* 
   ri[j] = j == k ? -m * rk[k] : ri[j] - m * rk[j];
 }
 
+// Blocked Gauss-Jordan: the columns [c0, c0 + nb) form a panel. For each
+// column k of the panel, k_gjb_pivot picks the pivot row among k..N-1,
+// swaps rows k and p inside the panel's columns only, scales row k's panel
+// entries by 1 / a_kk (a_kk <- 1 / a_kk) and copies column k (the
+// multipliers); k_gjb_update eliminates column k from the panel's columns
+// of every other row. The panel then holds the panel columns of the
+// elimination matrix E (E A's panel columns are the identity's); the other
+// columns receive the panel's row swaps (k_gjb_swap_rest) and the update
+// A_rest <- A_rest + (E_panel - J) A_rest[panel rows] (J: the identity's panel
+// columns) as two DGEMMs, from the copies k_gjb_prep makes.
+__global__ void __launch_bounds__(1024) k_gjb_pivot(int N, int k, int c0, int nb, double* __restrict__ a,
+                                                    int* __restrict__ piv, double* __restrict__ col,
+                                                    int* __restrict__ fail) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  double best = -1.0;
+  int bi = N;
+  for (int i = k + t; i < N; i += 1024) {
+    const double v = fabs(a[size_t(i) * N + k]);
+    if (v > best) {
+      best = v;
+      bi = i;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ob = __shfl_down(best, off);
+    const int oi = __shfl_down(bi, off);
+    if (ob > best || (ob == best && oi < bi)) {
+      best = ob;
+      bi = oi;
+    }
+  }
+  __shared__ double sb[16];
+  __shared__ int si[16];
+  __shared__ int sp;
+  __shared__ double sv;
+  if (lane == 0) {
+    sb[w] = best;
+    si[w] = bi;
+  }
+  __syncthreads();
+  if (t == 0) {
+    for (int q = 1; q < 16; ++q)
+      if (sb[q] > sb[0] || (sb[q] == sb[0] && si[q] < si[0])) {
+        sb[0] = sb[q];
+        si[0] = si[q];
+      }
+    const bool ok = sb[0] > 0.0 && isfinite(sb[0]) && si[0] < N;
+    sp = ok ? si[0] : k;
+    sv = ok ? sb[0] : 0.0;
+    piv[k] = sp;
+    if (!ok) *fail = 1;
+  }
+  __syncthreads();
+  if (!(sv > 0.0)) return;
+  const int p = sp;
+  double* rk = a + size_t(k) * N;
+  if (p != k) {
+    double* rp = a + size_t(p) * N;
+    for (int j = c0 + t; j < c0 + nb; j += 1024) {
+      const double x = rk[j];
+      rk[j] = rp[j];
+      rp[j] = x;
+    }
+  }
+  __syncthreads();
+  const double inv = 1.0 / rk[k];
+  __syncthreads();
+  for (int i = t; i < N; i += 1024) col[i] = i == k ? 0.0 : a[size_t(i) * N + k];
+  for (int j = c0 + t; j < c0 + nb; j += 1024) rk[j] = j == k ? inv : rk[j] * inv;
+}
+
+// one row per blockIdx.x, the panel's columns across the workgroup
+__global__ void k_gjb_update(int N, int k, int c0, int nb, double* __restrict__ a, const double* __restrict__ col) {
+  const int i = blockIdx.x;
+  if (i == k) return;
+  const double m = col[i];
+  if (m == 0.0) return;
+  const double* rk = a + size_t(k) * N;
+  double* ri = a + size_t(i) * N;
+  for (int j = c0 + threadIdx.x; j < c0 + nb; j += blockDim.x) ri[j] = j == k ? -m * rk[k] : ri[j] - m * rk[j];
+}
+
+// the panel's row swaps, in order, on every column outside the panel
+__global__ void k_gjb_swap_rest(int N, int c0, int nb, double* __restrict__ a, const int* __restrict__ piv) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N || (j >= c0 && j < c0 + nb)) return;
+  for (int t = 0; t < nb; ++t) {
+    const int k = c0 + t, p = piv[k];
+    if (p != k) {
+      const double x = a[size_t(k) * N + j];
+      a[size_t(k) * N + j] = a[size_t(p) * N + j];
+      a[size_t(p) * N + j] = x;
+    }
+  }
+}
+
+// R = rows c0 .. c0 + nb of a (nb x N, row major); M = E_panel - J (N x nb)
+__global__ void k_gjb_prep(int N, int c0, int nb, const double* __restrict__ a, double* __restrict__ R,
+                           double* __restrict__ M) {
+  const size_t q = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= size_t(N) * nb) return;
+  const int t = int(q / N), j = int(q % N);
+  R[q] = a[size_t(c0 + t) * N + j];
+  const int i = int(q / nb), u = int(q % nb);
+  M[q] = a[size_t(i) * N + c0 + u] - (i == c0 + u ? 1.0 : 0.0);
+}
+
 __global__ void k_gj_unpermute(int N, double* __restrict__ a, const int* __restrict__ piv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -996,17 +1103,23 @@ __global__ void k_ref_rows(int nrows, const int* __restrict__ irr_ptr, const int
                            unsigned long long* __restrict__ ratio) {
 #pragma clang fp contract(off)
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrows) return;
-  double rowsum = 0, diag = 0;
-  const int de = diag_e[r];
-  for (int e = irr_ptr[r]; e < irr_ptr[r + 1]; ++e) {
-    const double v = val_e[e];
-    rowsum += fabs(v);
-    if (e == de) diag = v;
+  // the largest row ratio of the wavefront, then one atomic per wavefront
+  // (nonnegative doubles order as their bit patterns)
+  double q = 0;
+  if (r < nrows) {
+    double rowsum = 0, diag = 0;
+    const int de = diag_e[r];
+    for (int e = irr_ptr[r]; e < irr_ptr[r + 1]; ++e) {
+      const double v = val_e[e];
+      rowsum += fabs(v);
+      if (e == de) diag = v;
+    }
+    irr_diag[r] = diag;
+    ell_diag[ell_row[r]] = diag;
+    if (diag > 0) q = rowsum / diag;
   }
-  irr_diag[r] = diag;
-  ell_diag[ell_row[r]] = diag;
-  if (diag > 0) atomicMax(ratio, (unsigned long long)__double_as_longlong(rowsum / diag));
+  for (int off = 32; off > 0; off >>= 1) q = fmax(q, __shfl_xor(q, off, 64));
+  if ((threadIdx.x & 63) == 0 && q > 0) atomicMax(ratio, (unsigned long long)__double_as_longlong(q));
 }
 
 // Jacobi diagonal inverse of every cell: the irregular row's diagonal, or the
@@ -1017,19 +1130,23 @@ __global__ void k_ref_dinv(int n, int bx, int by, const double* __restrict__ cw,
                            double* __restrict__ dinv, int* __restrict__ regular) {
 #pragma clang fp contract(off)
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  const int nb = bx * by, b = c / nb, q = c - b * nb, i = q % bx, j = q / bx;
-  double d;
-  const int r = cell_irr[c];
-  if (r >= 0) {
-    d = irr_diag[r];
-  } else {
-    const double e = i < bx - 1 ? cw[c + 1] : ce[b * by + j];
-    const double nn = j < by - 1 ? cs[c + bx] : cn[b * bx + i];
-    d = cw[c] + e + cs[c] + nn;
-    if (d > 0) *regular = 1;
+  bool reg = false;
+  if (c < n) {
+    const int nb = bx * by, b = c / nb, q = c - b * nb, i = q % bx, j = q / bx;
+    double d;
+    const int r = cell_irr[c];
+    if (r >= 0) {
+      d = irr_diag[r];
+    } else {
+      const double e = i < bx - 1 ? cw[c + 1] : ce[b * by + j];
+      const double nn = j < by - 1 ? cs[c + bx] : cn[b * bx + i];
+      d = cw[c] + e + cs[c] + nn;
+      reg = d > 0;
+    }
+    dinv[c] = d > 0 ? 1.0 / d : 0.0;
   }
-  dinv[c] = d > 0 ? 1.0 / d : 0.0;
+  // one store per wavefront that has a positive regular diagonal
+  if (__any(reg) && (threadIdx.x & 63) == 0) *regular = 1;
 }
 
 // any face coefficient not finite or not positive
@@ -1459,6 +1576,24 @@ void launch_identity_residual(hipStream_t s, int N, const double* T, unsigned lo
 
 void launch_extract_block(hipStream_t s, int n, int N, const double* a, double* inv) {
   hipLaunchKernelGGL(k_gj_extract, dim3(cdiv(n * n, 256)), dim3(256), 0, s, n, N, a, inv);
+}
+
+void launch_gjb_panel(hipStream_t s, int N, int c0, int nb, double* a, int* piv, double* col, int* fail) {
+  for (int k = c0; k < c0 + nb; ++k) {
+    hipLaunchKernelGGL(k_gjb_pivot, dim3(1), dim3(1024), 0, s, N, k, c0, nb, a, piv, col, fail);
+    hipLaunchKernelGGL(k_gjb_update, dim3(N), dim3(64), 0, s, N, k, c0, nb, a, col);
+  }
+  hipLaunchKernelGGL(k_gjb_swap_rest, dim3(cdiv(N, 256)), dim3(256), 0, s, N, c0, nb, a, piv);
+}
+
+void launch_gjb_prep(hipStream_t s, int N, int c0, int nb, const double* a, double* R, double* M) {
+  const size_t q = size_t(N) * nb;
+  hipLaunchKernelGGL(k_gjb_prep, dim3(unsigned((q + 255) / 256)), dim3(256), 0, s, N, c0, nb, a, R, M);
+}
+
+void launch_gj_finish(hipStream_t s, int N, double* a, const int* piv, int n, double* inv) {
+  hipLaunchKernelGGL(k_gj_unpermute, dim3(cdiv(N, 256)), dim3(256), 0, s, N, a, piv);
+  hipLaunchKernelGGL(k_gj_extract, dim3(unsigned((size_t(n) * n + 255) / 256)), dim3(256), 0, s, n, N, a, inv);
 }
 
 void launch_dense_inverse(hipStream_t s, int N, double* a, int* piv, double* col, int* fail, int n, double* inv) {

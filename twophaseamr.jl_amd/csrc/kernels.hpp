@@ -102,6 +102,13 @@ void launch_sqrt(hipStream_t s, int n, const double* in, double* out);
 // with partial pivoting, piv and col N-entry scratch); its top-left n x n
 // block goes to `inv` column major; *fail is set on a zero or non-finite pivot
 void launch_dense_inverse(hipStream_t s, int N, double* a, int* piv, double* col, int* fail, int n, double* inv);
+// Blocked Gauss-Jordan pieces (panel of columns [c0, c0 + nb)): the panel's
+// elimination with its pivots, then its row swaps on the other columns;
+// R (nb x N) and M (N x nb) for the update of the other columns; the final
+// column unpermutation and the n x n column-major block of the inverse.
+void launch_gjb_panel(hipStream_t s, int N, int c0, int nb, double* a, int* piv, double* col, int* fail);
+void launch_gjb_prep(hipStream_t s, int N, int c0, int nb, const double* a, double* R, double* M);
+void launch_gj_finish(hipStream_t s, int N, double* a, const int* piv, int n, double* inv);
 // *out = max(*out, max_ij |I - T|_ij) as the bits of a nonnegative double
 void launch_identity_residual(hipStream_t s, int N, const double* T, unsigned long long* out);
 // top-left n x n block of the row-major N x N `a`, column major

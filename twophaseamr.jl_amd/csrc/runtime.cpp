@@ -125,6 +125,9 @@ struct tpamr_ctx {
   MultigridConfig cfg;
   std::unique_ptr<Hierarchy> H;
   std::unique_ptr<TreeMesh> mesh_pending;  // before the first coefficients
+  // the hierarchy of the previous mesh, whose coarse suffix the next build
+  // may take over (same multigrid options)
+  std::unique_ptr<Hierarchy> H_prev;
   bool have_coeff = false;
   // device data
   std::vector<DevTopo> topo;
@@ -135,6 +138,7 @@ struct tpamr_ctx {
   // full inverse of the last coarse refresh (the start of the next one's
   // Newton-Schulz update), its identity residual, GEMM scratch
   DevArray<double> coarse_x, coarse_t, coarse_xn;
+  DevArray<double> gj_R, gj_M;  // blocked Gauss-Jordan: panel rows, panel update factor
   DevArray<unsigned long long> coarse_res;
   bool coarse_x_valid = false;
   bool coarse_exact = false;  // TPAMR_COARSE_GJ=1: Gauss-Jordan at every refresh
@@ -271,7 +275,6 @@ void reset_topology_state(tpamr_ctx* c) {
   destroy_graph(c);
   c->plan.clear();
   c->plan_ready = false;
-  c->coarse_x_valid = false;
   c->host_stale = false;
 }
 
@@ -344,6 +347,9 @@ void upload_hierarchy(tpamr_ctx* c, bool topology) {
   const int nl = int(H.levels.size());
   if (topology) {
     reset_topology_state(c);
+    // the previous coarse inverse starts the next Newton-Schulz update only
+    // when the coarsest level was taken over from the previous hierarchy
+    if (H.reused_new < 0) c->coarse_x_valid = false;
     c->topo.clear();
     c->topo.resize(nl);
     for (int l = 0; l < nl; ++l) {
@@ -953,6 +959,14 @@ void blas_check(rocblas_status st) {
   if (st != rocblas_status_success) throw HipError("rocBLAS call failed (status " + std::to_string(int(st)) + ")");
 }
 
+void blas_ready(tpamr_ctx* c) {
+  if (!c->blas) {
+    blas_check(rocblas_create_handle(&c->blas));
+    blas_check(rocblas_set_pointer_mode(c->blas, rocblas_pointer_mode_host));
+  }
+  blas_check(rocblas_set_stream(c->blas, c->stream));
+}
+
 // T = A X for row-major N x N A (c->coarse_work) and X (c->coarse_x): the
 // column-major product X^T A^T of the same buffers
 void coarse_product(tpamr_ctx* c, int N) {
@@ -966,11 +980,7 @@ void coarse_product(tpamr_ctx* c, int N) {
 // level of a Gauss-Jordan inverse (4x its residual, at least 1e-13).
 bool coarse_newton_schulz(tpamr_ctx* c, int N) {
   hipStream_t s = c->stream;
-  if (!c->blas) {
-    blas_check(rocblas_create_handle(&c->blas));
-    blas_check(rocblas_set_pointer_mode(c->blas, rocblas_pointer_mode_host));
-  }
-  blas_check(rocblas_set_stream(c->blas, s));
+  blas_ready(c);
   c->coarse_t.alloc(size_t(N) * N);
   c->coarse_xn.alloc(size_t(N) * N);
   const double accept = std::max(1e-13, 4 * c->coarse_gj_residual);
@@ -990,6 +1000,34 @@ bool coarse_newton_schulz(tpamr_ctx* c, int N) {
     std::swap(c->coarse_x.p, c->coarse_xn.p);
   }
   return false;
+}
+
+// Blocked Gauss-Jordan with partial pivoting of the N x N row-major matrix
+// in c->coarse_x (in place; its n x n leading block of the inverse goes to
+// c->coarse_inv, column major): per panel of GJ_PANEL columns, the panel's
+// own elimination (two small kernels per column), its row swaps on the other
+// columns, and their update A_rest += (E_panel - J) A_rest[panel rows] as two
+// DGEMMs (the row-major blocks seen column major: C^T += R^T M^T).
+void dense_inverse_blocked(tpamr_ctx* c, int N, int n) {
+  constexpr int GJ_PANEL = 64;
+  hipStream_t s = c->stream;
+  blas_ready(c);
+  c->gj_R.alloc(size_t(GJ_PANEL) * N);
+  c->gj_M.alloc(size_t(N) * GJ_PANEL);
+  double* a = c->coarse_x.p;
+  const double one = 1.0;
+  for (int c0 = 0; c0 < N; c0 += GJ_PANEL) {
+    const int nb = std::min(GJ_PANEL, N - c0), c1 = c0 + nb;
+    launch_gjb_panel(s, N, c0, nb, a, c->coarse_piv.p, c->coarse_col.p, c->coarse_fail.p);
+    launch_gjb_prep(s, N, c0, nb, a, c->gj_R.p, c->gj_M.p);
+    if (c0 > 0)
+      blas_check(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, c0, N, nb, &one, c->gj_R.p,
+                               N, c->gj_M.p, nb, &one, a, N));
+    if (c1 < N)
+      blas_check(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, N - c1, N, nb, &one,
+                               c->gj_R.p + c1, N, c->gj_M.p, nb, &one, a + c1, N));
+  }
+  launch_gj_finish(s, N, a, c->coarse_piv.p, n, c->coarse_inv.p);
 }
 
 // Dense coarse inverse of H.coarse_aug on the device into c->coarse_inv: a
@@ -1013,8 +1051,7 @@ void coarse_inverse_device(tpamr_ctx* c) {
   // Gauss-Jordan on a copy; A stays in coarse_work for the residual
   HIPCHECK(hipMemcpyAsync(c->coarse_x.p, c->coarse_work.p, sizeof(double) * NN, hipMemcpyDeviceToDevice, s));
   HIPCHECK(hipMemsetAsync(c->coarse_fail.p, 0, sizeof(int), s));
-  launch_dense_inverse(s, N, c->coarse_x.p, c->coarse_piv.p, c->coarse_col.p, c->coarse_fail.p, n,
-                       c->coarse_inv.p);
+  dense_inverse_blocked(c, N, n);
   int fail = 0;
   HIPCHECK(hipMemcpyAsync(&fail, c->coarse_fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHECK(hipStreamSynchronize(s));
@@ -1024,11 +1061,7 @@ void coarse_inverse_device(tpamr_ctx* c) {
   }
   c->coarse_x_valid = true;
   if (!c->coarse_exact) {
-    if (!c->blas) {
-      blas_check(rocblas_create_handle(&c->blas));
-      blas_check(rocblas_set_pointer_mode(c->blas, rocblas_pointer_mode_host));
-    }
-    blas_check(rocblas_set_stream(c->blas, s));
+    blas_ready(c);
     c->coarse_t.alloc(NN);
     coarse_product(c, N);
     c->coarse_gj_residual = identity_residual(c, N);
@@ -1322,7 +1355,7 @@ int tpamr_set_mesh(tpamr_ctx* c, const int base_mesh[2], const int block_size[2]
     const bool per[2] = {periodic[0] != 0, periodic[1] != 0};
     if (c->device >= 0) HIPCHECK(hipSetDevice(c->device));
     reset_topology_state(c);
-    c->H.reset();
+    if (c->H) c->H_prev = std::move(c->H);
     c->have_coeff = false;
     c->mesh_pending = make_tree_mesh(base_mesh, block_size, origin, extent, per, max_level, keys);
   });
@@ -1396,10 +1429,12 @@ int tpamr_set_multigrid(tpamr_ctx* c, const tpamr_multigrid_options* o) {
     c->cfg.max_levels = o->max_levels;
     destroy_graph(c);
     if (c->H && topo_change) {
-      // rebuild the hierarchy at the next coefficient update
+      // rebuild the hierarchy at the next coefficient update (nothing is
+      // reused across different options)
       reset_topology_state(c);
       c->mesh_pending = std::move(c->H->levels[0]->owned);
       c->H.reset();
+      c->H_prev.reset();
       c->have_coeff = false;
     }
   });
@@ -1412,7 +1447,8 @@ int tpamr_set_face_coefficients(tpamr_ctx* c, const double* kappa) {
     bool topology = false;
     if (!c->H) {
       if (!c->mesh_pending) throw StateError("no mesh: call tpamr_set_mesh first");
-      c->H = build_hierarchy(std::move(c->mesh_pending), c->cfg);
+      c->H = build_hierarchy(std::move(c->mesh_pending), c->cfg, c->H_prev.get());
+      c->H_prev.reset();
       topology = true;
     }
     const auto t0 = std::chrono::steady_clock::now();
@@ -1450,6 +1486,14 @@ int tpamr_set_face_coefficients_device(tpamr_ctx* c, const double* d_kappa) {
 }
 
 int tpamr_num_levels(const tpamr_ctx* c) { return (c && c->H) ? int(c->H->levels.size()) : 0; }
+
+int tpamr_hierarchy_reuse(const tpamr_ctx* c, int* new_start, int* old_start) {
+  if (!c || !new_start || !old_start) return TPAMR_EINVAL;
+  if (!c->H) return TPAMR_ESTATE;
+  *new_start = c->H->reused_new;
+  *old_start = c->H->reused_old;
+  return TPAMR_OK;
+}
 
 int tpamr_level_info(const tpamr_ctx* cc, int level, int* ints, double* dbl) {
   auto* c = const_cast<tpamr_ctx*>(cc);

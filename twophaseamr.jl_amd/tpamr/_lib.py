@@ -106,6 +106,7 @@ _SIGNATURES = {
                                         C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Stats)]),
     "tpamr_set_face_coefficients_device": (C.c_int, [C.c_void_p, C.c_void_p]),
     "tpamr_set_caller_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "tpamr_hierarchy_reuse": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "tpamr_time_stream": (C.c_int, [C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.POINTER(C.c_double),
                                     C.POINTER(C.c_double)]),
     "tpamr_apply_force_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),

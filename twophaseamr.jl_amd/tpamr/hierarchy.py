@@ -27,6 +27,23 @@ class HostHierarchy:
         L.check(L.lib().tpamr_set_face_coefficients(self.ctx.h, L.ptr(k)), self.ctx.h)
         self.nlevels = L.lib().tpamr_num_levels(self.ctx.h)
 
+    def remesh(self, mesh: BlockTreeMesh, face_coefficient=1.0):
+        """Attach a new mesh to the same context and build its hierarchy,
+        taking over the coarse levels whose tree topology is unchanged
+        (`_tree_reuse_suffix!`)."""
+        self.mesh = mesh
+        mesh.attach(self.ctx)
+        k = np.ascontiguousarray(np.broadcast_to(np.asarray(face_coefficient, np.float64), (mesh.nfaces,)))
+        L.check(L.lib().tpamr_set_face_coefficients(self.ctx.h, L.ptr(k)), self.ctx.h)
+        self.nlevels = L.lib().tpamr_num_levels(self.ctx.h)
+
+    def reuse(self):
+        """(first reused level of this hierarchy, its level in the previous
+        one), or (-1, -1)."""
+        a, b = C.c_int(), C.c_int()
+        L.check(L.lib().tpamr_hierarchy_reuse(self.ctx.h, C.byref(a), C.byref(b)), self.ctx.h)
+        return a.value, b.value
+
     def level_info(self, level):
         ints = np.zeros(6, np.int32)
         dbl = np.zeros(2)
