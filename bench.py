@@ -286,25 +286,71 @@ def pmc_summary(db_fetch, db_write, out_path=TRAFFIC_MD):
     print("\n".join(lines))
 
 
-def trace_summary(db, title, command, top=30):
-    """Markdown table of per-kernel totals from a rocprofv3 --kernel-trace
-    database (rocpd SQLite)."""
+def _trace_rows(path):
+    """(name, start ns, end ns, grid x) of every kernel dispatch of a
+    rocprofv3 --kernel-trace output: the CSV kernel trace or the rocpd
+    SQLite database."""
+    if path.endswith(".csv"):
+        import csv
+        with open(path) as fh:
+            return [(r["Kernel_Name"], int(r["Start_Timestamp"]), int(r["End_Timestamp"]), int(r["Grid_Size_X"]))
+                    for r in csv.DictReader(fh)]
     import sqlite3
-    con = sqlite3.connect(db)
-    tot = con.execute("select sum(end - start) / 1e6 from kernels").fetchone()[0]
-    rows = con.execute("select name, count(*), sum(end - start) / 1e6, avg(end - start) / 1e3, "
-                       "min(end - start) / 1e3, max(end - start) / 1e3, max(grid_x) from kernels "
-                       "group by name order by sum(end - start) desc limit ?", (top,)).fetchall()
+    con = sqlite3.connect(path)
+    rows = con.execute("select name, start, end, grid_x from kernels").fetchall()
     con.close()
+    return rows
+
+
+def trace_summary(db, title, command, top=30):
+    """Markdown summary of a rocprofv3 --kernel-trace run: per-kernel totals;
+    the finest-level sweep's mean over full launches (launches queued after a
+    solve has converged exit at once and are left out) beside its mean over
+    every launch; and, for the refinement mode, the split of the median
+    Krylov iteration (between consecutive k_ir_update_p launches)."""
+    import statistics
+    rows = sorted(_trace_rows(db), key=lambda r: r[1])
+    tot = sum(e - s for _, s, e, _ in rows) / 1e6
+    agg = {}
+    for name, s0, e0, g in rows:
+        a = agg.setdefault(name, [0, 0.0, float("inf"), 0.0, 0])
+        d = (e0 - s0) / 1e3
+        a[0] += 1
+        a[1] += d
+        a[2] = min(a[2], d)
+        a[3] = max(a[3], d)
+        a[4] = max(a[4], g)
     lines = [f"# {title}", "", "Command (one MI355X; `cd /tmp && export TMPDIR=/tmp` first; `<repo>` the repository "
              "root, `<out>` an output folder):", "",
              "```", command, "```", "", f"Total kernel time: {tot:.2f} ms", "",
              "| kernel | calls | total ms | avg us | min us | max us | max grid |", "|---|---:|---:|---:|---:|---:|---:|"]
-    for r in rows:
-        name = r[0].replace("tpamr::", "").split("(")[0]
-        if len(name) > 90:
-            name = name[:87] + "..."
-        lines.append(f"| `{name}` | {r[1]} | {r[2]:.2f} | {r[3]:.2f} | {r[4]:.2f} | {r[5]:.2f} | {r[6]} |")
+    for name, (n, t, lo, hi, g) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
+        short = name.replace("tpamr::", "").split("(")[0]
+        if len(short) > 90:
+            short = short[:87] + "..."
+        lines.append(f"| `{short}` | {n} | {t / 1e3:.2f} | {t / n:.2f} | {lo:.2f} | {hi:.2f} | {g} |")
+    sweep = [(e0 - s0) / 1e3 for name, s0, e0, g in rows if SWEEP_KERNEL.replace(" ", "") in name.replace(" ", "")]
+    if sweep:
+        med = statistics.median(sweep)
+        full = [d for d in sweep if d >= 0.5 * med]
+        lines += ["", f"`{SWEEP_KERNEL}` (finest level): mean {statistics.mean(sweep):.2f} us over all "
+                      f"{len(sweep)} launches, {statistics.mean(full):.2f} us over the {len(full)} full launches "
+                      f"(at least half the median), median {med:.2f} us."]
+    ends = [i for i, r in enumerate(rows) if "k_ir_update_p" in r[0]]
+    its = []
+    for i0, i1 in zip(ends, ends[1:]):
+        seg = rows[i0 + 1:i1 + 1]
+        wall = seg[-1][2] - rows[i0][2]
+        if wall < 5e6:
+            its.append((wall, seg))
+    if its:
+        wall, seg = sorted(its, key=lambda t: t[0])[len(its) // 2]
+        busy = sum(e0 - s0 for _, s0, e0, _ in seg)
+        lines += ["", f"Median refinement-mode Krylov iteration ({len(its)} iterations): {wall / 1e3:.1f} us from "
+                      f"the end of one k_ir_update_p to the next, kernels busy {busy / 1e3:.1f} us, "
+                      f"{len(seg)} launches, in order:", "", "| us | kernel |", "|---:|---|"]
+        for name, s0, e0, _ in seg:
+            lines.append(f"| {(e0 - s0) / 1e3:.1f} | `{name.replace('tpamr::', '').split('(')[0][:80]}` |")
     print("\n".join(lines))
 
 

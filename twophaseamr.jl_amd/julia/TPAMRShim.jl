@@ -161,6 +161,23 @@ function _sync!(ws::GpuCgWorkspace, operator)
 end
 
 """
+    TwoPhaseAMR._resize_krylov_workspace!(ws::GpuCgWorkspace, number_of_unknowns)
+
+After an AMR regrid `_reconfigure_elliptic_solver!` (elliptic_solver.jl:311-361) resizes the
+Krylov workspace, or builds a new one through the factory when this returns `nothing`
+(cycles.jl:465-480). The GPU workspace keeps its context: the next `_sync!` attaches the new mesh,
+and the library's hierarchy build takes over the previous mesh's coarse levels whose tree topology
+is unchanged (`tpamr_hierarchy_reuse`).
+"""
+function TwoPhaseAMR._resize_krylov_workspace!(ws::GpuCgWorkspace, number_of_unknowns::Int)
+    ws.m = ws.n = number_of_unknowns
+    resize!(ws.x, number_of_unknowns)
+    fill!(ws.x, 0.0)
+    ws.iterations = 0
+    return ws
+end
+
+"""
     TwoPhaseAMR._run_krylov!(ws::GpuCgWorkspace, matrix, rhs, x0, preconditioner, target,
                              volumes, max_iterations, options)
 
