@@ -185,6 +185,8 @@ struct tpamr_ctx {
   int tail_blocks = 16;
   hipGraphExec_t graph = nullptr;
   hipGraphExec_t graph_ir = nullptr;  // one inner FP32 Krylov iteration
+  // the first iteration of a Krylov run (its initial residual and V-cycle)
+  hipGraphExec_t graph0 = nullptr, graph0_ir = nullptr;
   // TPAMR_FP32_REFINE: the inner FP32 Krylov iteration replaces its
   // residual by the FP64 residual b - A x once the strong residual has
   // fallen by ir_delta since the last replacement (TPAMR_IR_DELTA in the
@@ -283,6 +285,11 @@ void destroy_graph(tpamr_ctx* c) {
     (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
   }
+  for (hipGraphExec_t* g : {&c->graph0, &c->graph0_ir})
+    if (*g) {
+      (void)hipGraphExecDestroy(*g);
+      *g = nullptr;
+    }
   if (c->graph_ir) {
     (void)hipGraphExecDestroy(c->graph_ir);
     c->graph_ir = nullptr;
@@ -650,7 +657,7 @@ void enqueue_iteration(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, bool init) {
 
 // Capture `graph_iters` non-initial Krylov iterations into `graph`.
 template <class Enq>
-void capture_iteration(tpamr_ctx* c, hipGraphExec_t& graph, Enq&& enqueue) {
+void capture_iteration(tpamr_ctx* c, hipGraphExec_t& graph, Enq&& enqueue, bool init = false) {
   if (graph) {
     (void)hipGraphExecDestroy(graph);
     graph = nullptr;
@@ -658,7 +665,10 @@ void capture_iteration(tpamr_ctx* c, hipGraphExec_t& graph, Enq&& enqueue) {
   hipGraph_t g;
   HIPCHECK(hipStreamSynchronize(c->stream));
   HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  for (int k = 0; k < c->graph_iters; ++k) enqueue(false);
+  if (init)
+    enqueue(true);
+  else
+    for (int k = 0; k < c->graph_iters; ++k) enqueue(false);
   HIPCHECK(hipStreamEndCapture(c->stream, &g));
   HIPCHECK(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
   HIPCHECK(hipGraphDestroy(g));
@@ -669,10 +679,15 @@ void capture_iteration(tpamr_ctx* c, hipGraphExec_t& graph, Enq&& enqueue) {
 // iteration; the state must already hold target and itmax. Returns the
 // iteration count.
 template <class Enq>
-int run_iterations(tpamr_ctx* c, hipGraphExec_t& graph, int itmax, Enq&& enqueue) {
+int run_iterations(tpamr_ctx* c, hipGraphExec_t& graph0, hipGraphExec_t& graph, int itmax, Enq&& enqueue) {
   hipStream_t s = c->stream;
-  enqueue(true);
-  if (c->use_graph && !graph) capture_iteration(c, graph, enqueue);
+  if (c->use_graph) {
+    if (!graph0) capture_iteration(c, graph0, enqueue, true);
+    HIPCHECK(hipGraphLaunch(graph0, s));
+    if (!graph) capture_iteration(c, graph, enqueue);
+  } else {
+    enqueue(true);
+  }
   // `pipe` iterations stay queued: the host reads the done flag of the
   // oldest one while the GPU runs the others; iterations queued after the
   // last one exit at once on the device-side done flag.
@@ -724,7 +739,7 @@ template <typename T>
 int run_cg(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double target, int itmax) {
   const int n = reset_state(c, target, itmax);
   launch_fill(c->stream, c->topo[0].ncells, 0.0, c->p.p);
-  return run_iterations(c, c->graph, n, [&](bool init) { enqueue_iteration<T>(c, lv, init); });
+  return run_iterations(c, c->graph0, c->graph, n, [&](bool init) { enqueue_iteration<T>(c, lv, init); });
 }
 
 // One inner FP32 Krylov iteration of the refinement mode: e (correction
@@ -767,7 +782,7 @@ void enqueue_ir_iteration(tpamr_ctx* c, std::vector<DevLevel<float>>& lv, bool i
 int run_ir(tpamr_ctx* c, std::vector<DevLevel<float>>& lv, double target, double scale, int itmax) {
   const int n = reset_state(c, target, itmax, scale, c->ir_delta);
   launch_fill_f32(c->stream, c->topo[0].ncells, 0.f, c->p32.p);
-  return run_iterations(c, c->graph_ir, n, [&](bool init) { enqueue_ir_iteration(c, lv, init); });
+  return run_iterations(c, c->graph0_ir, c->graph_ir, n, [&](bool init) { enqueue_ir_iteration(c, lv, init); });
 }
 
 template <typename T>
