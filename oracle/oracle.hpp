@@ -103,6 +103,12 @@ struct Operator {
   // before its plus entry): the order a face loop scatters to a cell
   std::vector<int> inc_ptr, inc_face;
   std::vector<signed char> inc_sign;
+  // open (homogeneous Dirichlet) pressure sides left, right, bottom, top;
+  // per boundary face its coefficient and gradient factor 2 k_b / h, per
+  // cell the boundary diagonal sum |f| 2 k_b / h over its open faces
+  // (elliptic.jl:76-152)
+  bool open[4] = {false, false, false, false};
+  std::vector<double> bcoef, bgrad, bdiag;
   Components comp;
   bool singular;
 };

@@ -254,12 +254,25 @@ class OSolver:
     geometric multigrid V-cycle (or plain CG with ``multigrid=False``)."""
 
     def __init__(self, mesh: OMesh, kappa=None, coarse_size=64, pre_sweeps=2, post_sweeps=2,
-                 max_levels=32, multigrid=True, volumes=None):
+                 max_levels=32, multigrid=True, volumes=None, open_sides=None, boundary_coefficient=None):
+        """``open_sides``: (left, right, bottom, top) open pressure sides of
+        ``velocity_boundary`` (elliptic.jl:76-88); ``boundary_coefficient``:
+        one value per boundary face (None: extended from ``kappa``)."""
         self.mesh = mesh
         k = np.ones(mesh.nfaces) if kappa is None else np.ascontiguousarray(
             np.broadcast_to(np.asarray(kappa, np.float64), (mesh.nfaces,)), np.float64)
         out = C.c_void_p()
-        if volumes is None:
+        if open_sides is not None:
+            ob = np.asarray([int(bool(v)) for v in open_sides], np.int32)
+            assert ob.shape == (4,)
+            v = None if volumes is None else np.ascontiguousarray(volumes, np.float64)
+            bc = None if boundary_coefficient is None else np.ascontiguousarray(
+                np.broadcast_to(np.asarray(boundary_coefficient, np.float64), (mesh.nbfaces,)), np.float64)
+            self._keep = (v, bc)
+            _check(lib().orc_solver_new_open(mesh._h, _dptr(k), None if v is None else _dptr(v), _iptr(ob),
+                                             None if bc is None else _dptr(bc), coarse_size, pre_sweeps,
+                                             post_sweeps, max_levels, int(multigrid), C.byref(out)))
+        elif volumes is None:
             _check(lib().orc_solver_new(mesh._h, _dptr(k), coarse_size, pre_sweeps, post_sweeps,
                                         max_levels, int(multigrid), C.byref(out)))
         else:
@@ -298,6 +311,33 @@ class OSolver:
         k = np.zeros(self.level_info(l)["nfaces"])
         lib().orc_solver_level_kappa(self._h, l, _dptr(k))
         return k
+
+    def level_boundary(self, l):
+        """Level l's boundary face coefficients and boundary diagonal, and
+        whether its operator is singular and the coarse solve augmented."""
+        m = self.level_mesh(l)
+        bc = np.zeros(m.nbfaces)
+        bd = np.zeros(m.ncells)
+        fl = np.zeros(2, np.int32)
+        _check(lib().orc_solver_level_boundary(self._h, l, _dptr(bc), _dptr(bd), _iptr(fl)))
+        return dict(coefficient=bc, diagonal=bd, singular=bool(fl[0]), coarse_augmented=bool(fl[1]))
+
+    def weighted_gradient_boundary(self, p):
+        g = np.zeros(self.mesh.nbfaces)
+        _check(lib().orc_weighted_gradient_boundary(self._h, _dptr(np.ascontiguousarray(p, np.float64)),
+                                                    _dptr(g)))
+        return g
+
+    def project_open(self, u, ub, p, dt, tolerance=np.sqrt(np.finfo(float).eps), max_cycles=100):
+        """The projection with boundary face velocities (open sides)."""
+        u = np.array(u, np.float64)
+        ub = np.array(ub, np.float64)
+        p = np.array(p, np.float64)
+        phi = np.zeros(self.mesh.ncells)
+        st = Stats()
+        _check(lib().orc_project_open(self._h, _dptr(u), _dptr(ub), _dptr(p), C.c_double(dt),
+                                      C.c_double(tolerance), int(max_cycles), C.byref(st), _dptr(phi)))
+        return u, ub, p, phi, st.as_dict()
 
     def level_mesh(self, l):
         h = C.c_void_p()

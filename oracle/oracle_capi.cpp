@@ -10,7 +10,9 @@
 
 namespace orc {
 Solver* build_solver(const Mesh& fine, const std::vector<double>& kappa, int coarse_size, int pre,
-                     int post, int max_levels, bool use_mg, const double* volumes);
+                     int post, int max_levels, bool use_mg, const double* volumes, const bool* open,
+                     const double* bcoef);
+void weighted_gradient_boundary(const Operator& op, const double* p, double* gb);
 double solver_floor(Solver& S, const double* x, const double* b, bool assembled);
 double csc_floor(bool single, int n, const int* colptr, const int* rowval, const double* nzval,
                  const double* volumes, const double* x, const double* b);
@@ -31,7 +33,7 @@ void cgfm_jumps(const Mesh& M, const double* c, const double* curvature, double 
 void weighted_jump_force(const Operator& op, const double* jumps, double* accel);
 void weighted_gradient(const Operator& op, const double* p, double* g);
 void divergence(const Mesh& M, const double* u, const double* ub, double* div);
-Stats project(Solver& S, double* u, double* p, double dt, double tol, int max_cycles, double* phi_out,
+Stats project(Solver& S, double* u, double* ub, double* p, double dt, double tol, int max_cycles, double* phi_out,
               const double* inherited, double reference_dt);
 Stats adjoint(Solver& S, const double* w, const double* p, double rtol, int max_cycles,
               double* dJ_db, double* dJ_dbeta, double* lambda_out);
@@ -149,7 +151,7 @@ int orc_solver_new(void* mp, const double* kappa, int coarse_size, int pre, int 
   GUARD({
     Mesh* m = static_cast<Mesh*>(mp);
     std::vector<double> k(kappa, kappa + m->faces.size());
-    *out = build_solver(*m, k, coarse_size, pre, post, max_levels, use_mg != 0, nullptr);
+    *out = build_solver(*m, k, coarse_size, pre, post, max_levels, use_mg != 0, nullptr, nullptr, nullptr);
     return 0;
   })
 }
@@ -160,7 +162,35 @@ int orc_solver_new_ex(void* mp, const double* kappa, const double* volumes, int 
   GUARD({
     Mesh* m = static_cast<Mesh*>(mp);
     std::vector<double> k(kappa, kappa + m->faces.size());
-    *out = build_solver(*m, k, coarse_size, pre, post, max_levels, use_mg != 0, volumes);
+    *out = build_solver(*m, k, coarse_size, pre, post, max_levels, use_mg != 0, volumes, nullptr, nullptr);
+    return 0;
+  })
+}
+
+// the same with open pressure sides open[4] (left, right, bottom, top) and
+// boundary face coefficients (NULL: extended from kappa)
+int orc_solver_new_open(void* mp, const double* kappa, const double* volumes, const int* open, const double* bcoef,
+                        int coarse_size, int pre, int post, int max_levels, int use_mg, void** out) {
+  GUARD({
+    Mesh* m = static_cast<Mesh*>(mp);
+    std::vector<double> k(kappa, kappa + m->faces.size());
+    const bool ob[4] = {open[0] != 0, open[1] != 0, open[2] != 0, open[3] != 0};
+    *out = build_solver(*m, k, coarse_size, pre, post, max_levels, use_mg != 0, volumes, ob, bcoef);
+    return 0;
+  })
+}
+
+// level l's boundary face coefficients, boundary diagonal, and flags[2] =
+// (singular, coarse augmented)
+int orc_solver_level_boundary(void* sp, int l, double* bcoef, double* bdiag, int* flags) {
+  GUARD({
+    Solver* s = static_cast<Solver*>(sp);
+    if (l < 0 || l >= int(s->levels.size())) throw std::invalid_argument("level out of range");
+    const Operator& op = *s->levels[l].op;
+    if (bcoef) std::copy(op.bcoef.begin(), op.bcoef.end(), bcoef);
+    if (bdiag) std::copy(op.bdiag.begin(), op.bdiag.end(), bdiag);
+    flags[0] = op.singular ? 1 : 0;
+    flags[1] = s->coarse_augmented ? 1 : 0;
     return 0;
   })
 }
@@ -309,15 +339,25 @@ int orc_divergence(void* mp, const double* u, const double* ub, double* div) {
 
 int orc_project(void* sp, double* u, double* p, double dt, double tol, int max_cycles, Stats* st,
                 double* phi) {
-  GUARD({ *st = project(*static_cast<Solver*>(sp), u, p, dt, tol, max_cycles, phi, nullptr, dt); return 0; })
+  GUARD({ *st = project(*static_cast<Solver*>(sp), u, nullptr, p, dt, tol, max_cycles, phi, nullptr, dt); return 0; })
 }
 
 int orc_project_ex(void* sp, double* u, double* p, double dt, double tol, int max_cycles, Stats* st, double* phi,
                    const double* inherited, double reference_dt) {
   GUARD({
-    *st = project(*static_cast<Solver*>(sp), u, p, dt, tol, max_cycles, phi, inherited, reference_dt);
+    *st = project(*static_cast<Solver*>(sp), u, nullptr, p, dt, tol, max_cycles, phi, inherited, reference_dt);
     return 0;
   })
+}
+
+// the projection with boundary face velocities ub (open sides)
+int orc_project_open(void* sp, double* u, double* ub, double* p, double dt, double tol, int max_cycles, Stats* st,
+                     double* phi) {
+  GUARD({ *st = project(*static_cast<Solver*>(sp), u, ub, p, dt, tol, max_cycles, phi, nullptr, dt); return 0; })
+}
+
+int orc_weighted_gradient_boundary(void* sp, const double* p, double* gb) {
+  GUARD({ weighted_gradient_boundary(*static_cast<Solver*>(sp)->ops[0], p, gb); return 0; })
 }
 
 int orc_adjoint(void* sp, const double* w, const double* p, double rtol, int max_cycles, double* dJ_db,

@@ -135,10 +135,11 @@ static CSC integrated_stiffness(const Mesh& M, const CSC& K, bool diag,
 
 // `_projection_component_labels!` (two_phase/projection/common.jl:53-87) and
 // `_refresh_elliptic_components!` (elliptic_solver.jl:39-109) for an operator
-// with all cells active and homogeneous-Neumann (wall or periodic) sides:
-// every component is ungauged.
+// with all cells active: a component is gauged (has no slot) when one of its
+// cells has a nonzero boundary diagonal (an open Dirichlet side), ungauged
+// otherwise.
 static Components components(const Mesh& M, const std::vector<double>& kappa,
-                             const std::vector<double>& volumes) {
+                             const std::vector<double>& volumes, const std::vector<double>& bdiag) {
   const int N = M.ncells();
   std::vector<std::vector<int>> inc(N);
   for (int f = 0; f < int(M.faces.size()); ++f) {
@@ -165,11 +166,15 @@ static Components components(const Mesh& M, const std::vector<double>& kappa,
     }
   }
   std::vector<double> vol(N, 0.0);
+  std::vector<unsigned char> gauged(N, 0);
   for (int i = 0; i < N; ++i)
-    if (C.labels[i] >= 0) vol[C.labels[i]] += volumes[i];
+    if (C.labels[i] >= 0) {
+      vol[C.labels[i]] += volumes[i];
+      if (bdiag[i] != 0.0) gauged[C.labels[i]] = 1;
+    }
   C.slots.assign(N, 0);
   for (int i = 0; i < N; ++i)
-    if (C.labels[i] == i) {
+    if (C.labels[i] == i && !gauged[i]) {
       C.anchors.push_back(i);
       C.volumes.push_back(vol[i]);
       C.slots[i] = int(C.anchors.size());
@@ -179,12 +184,53 @@ static Components components(const Mesh& M, const std::vector<double>& kappa,
   return C;
 }
 
+// `_boundary_material` (elliptic.jl:90-131) for a vector of interior
+// coefficients: every open boundary face takes the area-weighted mean of the
+// interior faces on the opposite side of its cell along the same axis, summed
+// in face order; closed faces get 0.
+static std::vector<double> boundary_material(const Mesh& M, const std::vector<double>& kappa, const bool open[4]) {
+  const int BF = int(M.bfaces.size());
+  std::vector<double> values(BF, 0.0), weights(BF, 0.0);
+  auto is_open = [&](const BFace& b) { return open[2 * b.axis + (b.side > 0 ? 1 : 0)]; };
+  bool any = false;
+  for (int s = 0; s < 4; ++s) any = any || open[s];
+  if (!any) return values;
+  std::map<std::tuple<int, int, int>, int> index;
+  for (int f = 0; f < BF; ++f)
+    if (is_open(M.bfaces[f])) index[{M.bfaces[f].cell, M.bfaces[f].axis, M.bfaces[f].side}] = f;
+  for (int f = 0; f < int(M.faces.size()); ++f) {
+    const Face& fa = M.faces[f];
+    const std::pair<int, int> ends[2] = {{fa.minus, -1}, {fa.plus, 1}};
+    for (const auto& e : ends) {
+      auto it = index.find({e.first, fa.axis, e.second});
+      if (it == index.end()) continue;
+      values[it->second] += fa.area * kappa[f];
+      weights[it->second] += fa.area;
+    }
+  }
+  for (int f = 0; f < BF; ++f) {
+    if (!is_open(M.bfaces[f])) continue;
+    if (!(weights[f] > 0))
+      throw std::invalid_argument(
+          "cannot extend an interior coefficient to an open face without an opposite interior face; supply "
+          "boundary_face_coefficient");
+    values[f] /= weights[f];
+  }
+  return values;
+}
+
 // EllipticOperator / PoissonOperator (elliptic.jl:683-800, 969-980) with
-// reaction 0, diffusion 1, all cells active, no open sides. `volumes`
-// (NULL: the cells' geometric volumes) is the operator's `volumes` keyword
-// (elliptic.jl:683-700); the stiffness does not depend on it.
+// reaction 0, diffusion 1 and all cells active. `volumes` (NULL: the cells'
+// geometric volumes) is the operator's `volumes` keyword (elliptic.jl:683-700);
+// the stiffness does not depend on it. `open` (NULL: none) are the open
+// pressure sides (left, right, bottom, top) of `velocity_boundary`
+// (elliptic.jl:76-88) and `bcoef` (NULL: extended from `kappa`,
+// `_boundary_material`) the boundary face coefficients; the boundary diagonal
+// of `_elliptic_boundary_data!` (elliptic.jl:133-152) joins the stiffness
+// diagonal (elliptic.jl:752-764).
 static std::unique_ptr<Operator> make_operator(const Mesh* mesh, const std::vector<double>& kappa,
-                                               const double* volumes = nullptr) {
+                                               const double* volumes = nullptr, const bool* open = nullptr,
+                                               const double* bcoef = nullptr) {
   auto op = std::make_unique<Operator>();
   op->mesh = mesh;
   if (kappa.size() != mesh->faces.size())
@@ -222,7 +268,27 @@ static std::unique_ptr<Operator> make_operator(const Mesh* mesh, const std::vect
       op->inc_sign[next[pl]++] = 1;
     }
   }
-  op->comp = components(*mesh, op->kappa, op->volumes);
+  // open pressure sides and the boundary diagonal
+  const int BF = int(mesh->bfaces.size());
+  for (int s = 0; s < 4; ++s) op->open[s] = open ? open[s] : false;
+  if (bcoef) {
+    op->bcoef.assign(bcoef, bcoef + BF);
+  } else {
+    op->bcoef = boundary_material(*mesh, op->kappa, op->open);
+  }
+  for (double k : op->bcoef)
+    if (!(std::isfinite(k) && k >= 0)) throw std::invalid_argument("boundary coefficients must be finite and nonnegative");
+  op->bgrad.assign(BF, 0.0);
+  op->bdiag.assign(mesh->ncells(), 0.0);
+  for (int f = 0; f < BF; ++f) {
+    const BFace& b = mesh->bfaces[f];
+    if (!op->open[2 * b.axis + (b.side > 0 ? 1 : 0)]) continue;
+    const double factor = 2 * op->bcoef[f] / mesh->blocks[b.cell / nb].spacing[b.axis];
+    op->bgrad[f] = factor;
+    op->bdiag[b.cell] += b.area * 1.0 * factor;
+  }
+  for (int c = 0; c < mesh->ncells(); ++c) op->A.nzval[csc_find(op->A, c, c)] += op->bdiag[c];
+  op->comp = components(*mesh, op->kappa, op->volumes, op->bdiag);
   op->singular = !op->comp.anchors.empty();
   return op;
 }
@@ -256,7 +322,8 @@ static void factorized_mul(const Operator& op, const double* x, double* y) {
   const int N = M.ncells();
 #pragma omp parallel for schedule(static)
   for (int c = 0; c < N; ++c) {
-    double v = 0.0 * x[c];
+    // (reaction V + diffusion B) x with reaction 0, diffusion 1
+    double v = (0.0 * op.volumes[c] + 1.0 * op.bdiag[c]) * x[c];
     for (int q = op.inc_ptr[c]; q < op.inc_ptr[c + 1]; ++q) {
       const double flux = fin[op.inc_face[q]];
       v += op.inc_sign[q] < 0 ? -1.0 * flux : flux;
@@ -428,6 +495,56 @@ static FaceRestriction face_restriction(const Mesh& coarse, const Mesh& fine) {
   return R;
 }
 
+// `_face_coefficient_restriction(...; boundary = true)`
+// (tree_coarsening.jl:347-398) with the boundary-face lattice geometry of
+// mesh.jl:476-485: each coarse boundary face takes the overlap-weighted fine
+// boundary faces on its line (axis, normal coordinate).
+static FaceRestriction boundary_restriction(const Mesh& coarse, const Mesh& fine) {
+  long lat[2];
+  common_lattice(fine, coarse, lat);
+  auto geom = [&](const Mesh& M, const BFace& b, long& n, long& lo, long& hi) {
+    long low[2], sc[2];
+    cell_bounds(M, b.cell, lat, low, sc);
+    const int a = b.axis, t = 1 - a;
+    n = low[a] + (b.side > 0 ? sc[a] : 0);
+    lo = low[t];
+    hi = low[t] + sc[t];
+  };
+  std::map<std::pair<int, long>, std::vector<std::tuple<long, long, int>>> lines;
+  for (int f = 0; f < int(fine.bfaces.size()); ++f) {
+    long n, lo, hi;
+    geom(fine, fine.bfaces[f], n, lo, hi);
+    lines[{fine.bfaces[f].axis, n}].emplace_back(lo, hi, f);
+  }
+  for (auto& kv : lines)
+    std::stable_sort(kv.second.begin(), kv.second.end(),
+                     [](const auto& x, const auto& y) { return std::get<0>(x) < std::get<0>(y); });
+  FaceRestriction R;
+  R.ptr.push_back(0);
+  for (const BFace& b : coarse.bfaces) {
+    long n, lo, hi;
+    geom(coarse, b, n, lo, hi);
+    long covered = 0;
+    auto it = lines.find({b.axis, n});
+    if (it != lines.end()) {
+      for (const auto& seg : it->second) {
+        const long flo = std::get<0>(seg), fhi = std::get<1>(seg);
+        if (fhi <= lo) continue;
+        if (flo >= hi) break;
+        const long overlap = std::min(hi, fhi) - std::max(lo, flo);
+        if (overlap > 0) {
+          R.idx.push_back(std::get<2>(seg));
+          R.w.push_back(double(overlap) / double(hi - lo));
+          covered += overlap;
+        }
+      }
+    }
+    if (covered != hi - lo) throw std::runtime_error("tree hierarchy could not cover a coarse face with fine faces");
+    R.ptr.push_back(int(R.idx.size()));
+  }
+  return R;
+}
+
 // `_smoothed_transfer_refresh` + `_refresh_smoothed_transfer!`,
 // hierarchy.jl:631-721: P = (I - w_P D^-1 S) P_t on the pattern
 // P_t union pattern(S) P_t.
@@ -469,6 +586,8 @@ static CSC smoothed_transfer(const CSC& S, const std::vector<int>& owner, int nc
   return P;
 }
 
+// `_refresh_multigrid_components!` (hierarchy.jl:276-298): the level's
+// ungauged components only
 static void level_components(Level& L) {
   const Components& C = L.op->comp;
   const int N = int(L.op->volumes.size());
@@ -476,7 +595,7 @@ static void level_components(Level& L) {
   L.comp_volumes.clear();
   for (int cell = 0; cell < N; ++cell) {
     const int root = C.labels[cell];
-    if (root < 0) continue;
+    if (root < 0 || C.slots[root] == 0) continue;
     if (cell == root) {
       L.comp_volumes.push_back(C.volumes[C.slots[root] - 1]);
       L.comp_index[cell] = int(L.comp_volumes.size());
@@ -529,7 +648,8 @@ static void lu_solve(const std::vector<double>& a, const std::vector<int>& piv, 
 // reuse, no geometry), geometric_preconditioner.jl:171-442, with
 // `_tree_next_coarse_mesh!` (tree_coarsening.jl:187-212).
 Solver* build_solver(const Mesh& fine, const std::vector<double>& kappa, int coarse_size, int pre,
-                     int post, int max_levels, bool use_mg, const double* volumes) {
+                     int post, int max_levels, bool use_mg, const double* volumes, const bool* open,
+                     const double* bcoef) {
   auto S = new Solver();
   S->pre = pre;
   S->post = post;
@@ -538,7 +658,7 @@ Solver* build_solver(const Mesh& fine, const std::vector<double>& kappa, int coa
   S->use_mg = use_mg;
   // coarse levels keep their geometric volumes (`_cell_volumes` of each
   // coarse mesh, geometric_preconditioner.jl:75-105)
-  S->ops.push_back(make_operator(&fine, kappa, volumes));
+  S->ops.push_back(make_operator(&fine, kappa, volumes, open, bcoef));
   std::vector<CSC> prolongations;
   if (use_mg) {
     int active_level = fine.max_leaf_level();
@@ -564,8 +684,18 @@ Solver* build_solver(const Mesh& fine, const std::vector<double>& kappa, int coa
         for (int q = R.ptr[f]; q < R.ptr[f + 1]; ++q) s += R.w[q] * fk[R.idx[q]];
         ck[f] = s;
       }
+      // coarse boundary coefficients (geometric_preconditioner.jl:249-251,
+      // 283-287): overlap-weighted sums in restriction order
+      const FaceRestriction RB = boundary_restriction(*coarse, *current);
+      std::vector<double> cb(coarse->bfaces.size());
+      const std::vector<double>& fb = S->ops.back()->bcoef;
+      for (size_t f = 0; f < cb.size(); ++f) {
+        double s = 0;
+        for (int q = RB.ptr[f]; q < RB.ptr[f + 1]; ++q) s += RB.w[q] * fb[RB.idx[q]];
+        cb[f] = s;
+      }
       Mesh* cm = coarse.release();
-      auto op = make_operator(cm, ck);
+      auto op = make_operator(cm, ck, nullptr, S->ops[0]->open, cb.data());
       op->owned_mesh.reset(cm);
       S->ops.push_back(std::move(op));
       current = cm;
@@ -806,7 +936,7 @@ static double roundoff_floor(const Operator& op, const std::vector<double>& x,
   std::vector<double> bound(N);
 #pragma omp parallel for schedule(static)
   for (int c = 0; c < N; ++c) {
-    double v = 0.0 * std::fabs(x[c]);
+    double v = (0.0 * op.volumes[c] + 1.0 * op.bdiag[c] + 0.0) * std::fabs(x[c]);
     for (int q = op.inc_ptr[c]; q < op.inc_ptr[c + 1]; ++q) v += fin[op.inc_face[q]];
     bound[c] = v;
   }
@@ -1334,6 +1464,14 @@ void weighted_gradient(const Operator& op, const double* p, double* g) {
   for (int f = 0; f < F; ++f) g[f] = op.sqrt_kappa[f] * out[f];
 }
 
+// `_elliptic_boundary_gradient!` (elliptic.jl:1254-1263): boundary faces
+// -side * (2 k_b / h) * p_cell on open sides, zero on closed ones
+void weighted_gradient_boundary(const Operator& op, const double* p, double* gb) {
+  const Mesh& M = *op.mesh;
+  for (size_t f = 0; f < M.bfaces.size(); ++f)
+    gb[f] = -double(M.bfaces[f].side) * op.bgrad[f] * p[M.bfaces[f].cell];
+}
+
 // `divergence!`, operators/advection/cell_advection.jl:205-220
 void divergence(const Mesh& M, const double* u, const double* ub, double* div) {
   const int nb = M.bs[0] * M.bs[1];
@@ -1352,13 +1490,15 @@ void divergence(const Mesh& M, const double* u, const double* ub, double* div) {
     }
 }
 
-// `_project_prepared!` (navier_stokes/projection.jl:221-260) for a wall or
-// periodic domain with one component: div -> rhs = -div/dt -> solve with
-// rtol 0 and atol = tol/dt^2 -> u -= dt G phi -> p += phi.
+// `_project_prepared!` (navier_stokes/projection.jl:221-260) with one
+// component: div -> rhs = -div/dt -> solve with rtol 0 and atol = tol/dt^2 ->
+// u -= dt G phi -> p += phi. `ub` (NULL: none) are the boundary face
+// velocities of an operator with open sides: they enter the divergence and
+// receive the boundary gradient's correction.
 // With `inherited` (a shortened step, `_projection_rhs!` projection.jl:130-156)
 // the rhs keeps (tau - dt) / (dt tau) of the inherited divergence; the stats'
 // `compatible` is `_projection_compatibility!` (:100-127).
-Stats project(Solver& S, double* u, double* p, double dt, double tol, int max_cycles,
+Stats project(Solver& S, double* u, double* ub, double* p, double dt, double tol, int max_cycles,
               double* phi_out, const double* inherited, double reference_dt) {
   Operator& op = *S.ops[0];
   const Mesh& M = *op.mesh;
@@ -1366,7 +1506,7 @@ Stats project(Solver& S, double* u, double* p, double dt, double tol, int max_cy
   if (!(std::isfinite(reference_dt) && reference_dt >= dt))
     throw std::invalid_argument("reference dt must be finite and at least dt");
   std::vector<double> div(N), phi(N, 0.0), g(F);
-  divergence(M, u, nullptr, div.data());
+  divergence(M, u, ub, div.data());
   if (inherited) {
     const double w = (reference_dt - dt) / (dt * reference_dt);
     for (int c = 0; c < N; ++c) div[c] = -div[c] / dt + w * inherited[c];
@@ -1380,6 +1520,11 @@ Stats project(Solver& S, double* u, double* p, double dt, double tol, int max_cy
   weighted_gradient(op, phi.data(), g.data());
   const double scale = dt * 1.0;
   for (int f = 0; f < F; ++f) u[f] -= scale * g[f];
+  if (ub) {
+    std::vector<double> gb(M.bfaces.size());
+    weighted_gradient_boundary(op, phi.data(), gb.data());
+    for (size_t f = 0; f < gb.size(); ++f) ub[f] -= scale * gb[f];
+  }
   for (int c = 0; c < N; ++c) p[c] += phi[c];
   if (phi_out) std::copy(phi.begin(), phi.end(), phi_out);
   return st;
