@@ -105,8 +105,42 @@ int tpamr_set_multigrid(tpamr_ctx* ctx, const tpamr_multigrid_options* options);
  * src/solvers/multigrid/geometric_preconditioner.jl:483-594). */
 int tpamr_set_face_coefficients(tpamr_ctx* ctx, const double* kappa);
 
+/* Open (homogeneous Dirichlet) pressure sides: PoissonOperator's
+ * velocity_boundary and boundary_face_coefficient (src/operators/elliptic.jl:
+ * 76-152, 683-800). open[4] = left, right, bottom, top (the Outflow and
+ * ZeroGradientVelocity sides of VelocityBoundaryConditions; all zero: walls
+ * and periodic sides only, the default). boundary_coefficient: one value per
+ * boundary face (reference order, nonnegative), or NULL to extend the next
+ * face coefficients to the open faces (_boundary_material: each open face
+ * takes the area-weighted mean of the interior coefficients on the opposite
+ * side of its cell). Every open face adds |f| 2 k_b / h to its cell's
+ * diagonal; with one positive, the operator is nonsingular: solves take the
+ * rhs without compatibility projection (removed_rhs_mean 0) and return no
+ * gauge-shifted solution, and the coarsest level is solved without the
+ * volume constraint. Takes effect at the next tpamr_set_face_coefficients
+ * (or _device); changing the sides rebuilds the hierarchy there
+ * (geometric_preconditioner.jl:494-496). A new mesh keeps the sides and
+ * extends the coefficients again unless they are given anew. */
+int tpamr_set_pressure_boundary(tpamr_ctx* ctx, const int open[4], const double* boundary_coefficient);
+/* _elliptic_boundary_gradient! (elliptic.jl:1254-1263): per boundary face
+ * -side (2 k_b / h) p_cell on open faces, 0 on closed ones (host arrays) */
+int tpamr_weighted_gradient_boundary(tpamr_ctx* ctx, const double* p, double* gb);
+/* _project_prepared! (navier_stokes/projection.jl:221-260) for an operator with
+ * open sides: `ub` (one velocity per boundary face) enters the divergence and
+ * its open faces receive -dt g_b; u (interior faces) and p as tpamr_project.
+ * The interior-face projections (tpamr_project, _ex, _device and the
+ * projection adjoints) reject an operator with open sides, and this one an
+ * operator without them (_validate_projection_boundary, projection.jl:37-49). */
+int tpamr_project_open(tpamr_ctx* ctx, double* u, double* ub, double* p, double dt, double tolerance, int max_cycles,
+                       double* phi, tpamr_stats* stats);
+
 /* hierarchy inspection */
 int tpamr_num_levels(const tpamr_ctx* ctx);
+/* Level `level`'s boundary face coefficients (restricted from the finest
+ * level's, tree_coarsening.jl:347-398,538-555; NULL: skipped) and flags[4] =
+ * (level singular, coarsest level solved with the volume constraint, open
+ * boundary faces of the level, boundary faces of the level). */
+int tpamr_level_boundary(const tpamr_ctx* ctx, int level, double* boundary_coefficient, int* flags);
 /* Levels the last hierarchy build took over from the previous mesh's
  * hierarchy (a mesh replaced by tpamr_set_mesh, same multigrid options):
  * levels new_start.. of this hierarchy are old_start.. of that one, as the

@@ -85,3 +85,30 @@ def disk_fraction(x, y, dx, dy, centre, radius):
     area = (_quad_area(a1, b1, radius) - _quad_area(a0, b1, radius) - _quad_area(a1, b0, radius)
             + _quad_area(a0, b0, radius))
     return np.clip(area / (dx * dy), 0.0, 1.0)
+
+
+# Open pressure sides (left, right, bottom, top): meshes, open sides, and the
+# boundary coefficient kind ("material": 0.8 + 0.2 y + 0.1 x at the boundary
+# face centres; "extended": the interior coefficients extended to the open
+# faces).
+OPEN_LR = (True, True, False, False)
+
+
+def _kw(refine, periodic):
+    return dict(base_mesh=(16, 8), block_size=(4, 4), extent=(2.0, 1.0), periodic=periodic, max_level=1,
+                refine=refine)
+
+
+OPEN_CASES = {
+    "outflow_column": (_kw(lambda k, o, e: k[0] == 0 and k[1] == 3, (False, False)), OPEN_LR, "material"),
+    "outflow_column_periodic_y": (_kw(lambda k, o, e: k[0] == 0 and k[1] == 3, (False, True)), OPEN_LR, "material"),
+    "uniform": (_kw(lambda k, o, e: False, (False, False)), OPEN_LR, "material"),
+    "corner_refined": (_kw(lambda k, o, e: k[0] == 0 and k[1] == 0 and k[2] == 0, (False, False)),
+                       (True, True, True, True), "material"),
+    "amr64_all_open": (dict(base_mesh=(64, 64), block_size=(8, 8), periodic=(False, False), max_level=2,
+                            refine=lambda k, o, e: (o[0] < 0.3 and o[1] < 0.4) or (k[0] == 0 and o[0] > 0.7)),
+                       (True, True, True, True), "extended"),
+    "vertical_periodic_x": (dict(base_mesh=(16, 8), block_size=(4, 4), extent=(3.0, 1.0), periodic=(True, False),
+                                 max_level=1, refine=lambda k, o, e: k[0] == 0 and k[1] == 1 and k[2] == 0),
+                            (False, False, True, True), "material"),
+}

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -813,6 +814,44 @@ void level_topology(LevelData& L) {
       }
     L.blk_irr[b + 1] = int(L.irr_local.size());
   }
+  // open boundary faces: a regular cell's conductance slot beyond the side,
+  // or its irregular row (the row's extra ELL entries, build_ell)
+  L.bopen.clear();
+  L.bslot.clear();
+  L.bdst.clear();
+  const int BF = int(M.bcell.size()), nrows = int(L.irr_local.size());
+  std::vector<int> rcount(nrows + 1, 0);
+  for (int f = 0; f < BF; ++f) {
+    if (!L.open[2 * M.baxis[f] + (M.bside[f] > 0 ? 1 : 0)]) continue;
+    const int c = M.bcell[f], b = c / nb, q = c % nb, i = q % bx, j = q / bx;
+    L.bopen.push_back(f);
+    if (L.cell_irr[c] >= 0) {
+      rcount[L.cell_irr[c] + 1]++;
+      L.bslot.push_back(-1);
+      L.bdst.push_back(-1);
+      continue;
+    }
+    if (M.baxis[f] == 0) {
+      const int ii = M.bside[f] > 0 ? bx : 0;
+      L.bslot.push_back(int64_t(b) * (bx + 1) * by + int64_t(j) * (bx + 1) + ii);
+      L.bdst.push_back(M.bside[f] > 0 ? (b * by + j) * 4 + 1 : c * 4 + 0);
+    } else {
+      const int jj = M.bside[f] > 0 ? by : 0;
+      L.bslot.push_back(int64_t(b) * bx * (by + 1) + int64_t(jj) * bx + i);
+      L.bdst.push_back(M.bside[f] > 0 ? (b * bx + i) * 4 + 3 : c * 4 + 2);
+    }
+  }
+  for (int r = 0; r < nrows; ++r) rcount[r + 1] += rcount[r];
+  L.irr_bptr = rcount;
+  L.irr_bface.assign(rcount[nrows], 0);
+  {
+    std::vector<int> at(rcount.begin(), rcount.end() - 1);
+    for (int k = 0; k < int(L.bopen.size()); ++k) {
+      const int r = L.cell_irr[M.bcell[L.bopen[k]]];
+      if (r >= 0) L.irr_bface[at[r]++] = k;
+    }
+  }
+  L.bpos.assign(L.bopen.size(), -1);
   pt.mark("level topology");
 }
 
@@ -830,8 +869,10 @@ void build_ell(LevelData& L) {
   const int bx = M.bs[0], by = M.bs[1], nb = bx * by;
   const int hw = nb, he = nb + by, hs = nb + 2 * by, hn = nb + 2 * by + bx, ext0 = nb + 2 * (bx + by);
   const int nrows = int(L.irr_local.size());
+  // a row's length: its entries of A, then one entry per open boundary face
+  auto row_len = [&](int r) { return L.irr_ptr[r + 1] - L.irr_ptr[r] + L.irr_bptr[r + 1] - L.irr_bptr[r]; };
   int width = 0;
-  for (int r = 0; r < nrows; ++r) width = std::max(width, L.irr_ptr[r + 1] - L.irr_ptr[r]);
+  for (int r = 0; r < nrows; ++r) width = std::max(width, row_len(r));
   L.ell_w = width;
   const int ng = (width + 3) / 4, w4 = 4 * ng;
   L.ell_ref.assign(size_t(nrows) * w4, 0);
@@ -861,14 +902,12 @@ void build_ell(LevelData& L) {
       // the block's region are then a prefix
       order.resize(nr);
       for (int x = 0; x < nr; ++x) order[x] = r0 + x;
-      std::stable_sort(order.begin(), order.end(), [&](int p, int q) {
-        return L.irr_ptr[p + 1] - L.irr_ptr[p] > L.irr_ptr[q + 1] - L.irr_ptr[q];
-      });
+      std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return row_len(p) > row_len(q); });
       auto slot = [&](int x, int k) { return (size_t(r0) * ng + size_t(k / 4) * nr + x) * 4 + k % 4; };
       for (int x = 0; x < nr; ++x) {
         const int r = order[x];
         const int loc = L.irr_local[r];
-        const int len = L.irr_ptr[r + 1] - L.irr_ptr[r];
+        const int len = row_len(r);
         L.ell_loc[r0 + x] = uint32_t(loc) | (uint32_t(len) << 16);
         L.ell_row[r] = r0 + x;
         for (int k = 0; k < w4; ++k) L.ell_ref[slot(x, k)] = uint16_t(loc);
@@ -893,6 +932,18 @@ void build_ell(LevelData& L) {
           const size_t at = slot(x, k);
           L.ell_ref[at] = uint16_t(ref);
           L.ell_pos[e] = int(at);
+        }
+        // open boundary faces: the halo slot beyond the side holds zero (no
+        // same-level neighbour there), so the entry -bw contributes bw v_i
+        const int li = loc % bx, lj = loc / bx;
+        for (int q = L.irr_bptr[r]; q < L.irr_bptr[r + 1]; ++q, ++k) {
+          const int kf = L.irr_bface[q], f = L.bopen[kf];
+          const int side = 2 * M.baxis[f] + (M.bside[f] > 0 ? 1 : 0);
+          if (nbr[side] >= 0) throw std::logic_error("open boundary face with a neighbour block");
+          const int ref = side == 0 ? hw + lj : side == 1 ? he + lj : side == 2 ? hs + li : hn + li;
+          const size_t at = slot(x, k);
+          L.ell_ref[at] = uint16_t(ref);
+          L.bpos[kf] = int(at);
         }
       }
       // LDS slots of a block tile: up to max(nb, 32) external cells (4x4
@@ -1077,6 +1128,13 @@ void level_values(LevelData& L) {
       arr[L.slot_b[f]] = w;
     }
   });
+  // open boundary faces of regular cells: their conductance in the slot
+  // beyond the side (the halo value there is zero)
+  const bool opened = L.any_open();
+  if (opened && L.bw.size() != L.bopen.size())
+    throw std::logic_error("open pressure sides without boundary coefficients");
+  for (size_t k = 0; k < L.bopen.size(); ++k)
+    if (L.bslot[k] >= 0) (M.baxis[L.bopen[k]] == 0 ? L.cx : L.cy)[L.bslot[k]] = L.bw[k];
   // irregular rows: full rows of A (and of the floor bound |J|'|M||J|)
   const int nrows = int(L.irr_local.size());
   std::vector<double> ratio(host_threads(), 0.0);
@@ -1084,6 +1142,7 @@ void level_values(LevelData& L) {
     double mr = 0;
     for (long r = lo; r < hi; ++r) {
       double rowsum = 0;
+      const int de = L.irr_diag_e[r];
       for (int e = L.irr_ptr[r]; e < L.irr_ptr[r + 1]; ++e) {
         double v = 0, bv = 0;
         for (int q = L.term_ptr[e]; q < L.term_ptr[e + 1]; ++q) {
@@ -1096,13 +1155,23 @@ void level_values(LevelData& L) {
           v += L.term_s[q] * core;
           bv += babs;
         }
+        if (opened && e == de) {
+          // the boundary diagonal joins the row's diagonal (and its bound)
+          v = v + L.irr_bd[r];
+          bv = bv + L.irr_bd[r];
+        }
         L.irr_val[e] = v;
         L.irr_bound[e] = bv;
         L.ell_val[L.ell_pos[e]] = v;
         L.ell_bound[L.ell_pos[e]] = bv;
         rowsum += std::fabs(v);
       }
-      const int de = L.irr_diag_e[r];
+      // the open faces' entries: -bw at the zero halo slot beyond the side
+      for (int q = L.irr_bptr[r]; q < L.irr_bptr[r + 1]; ++q) {
+        const int k = L.irr_bface[q];
+        L.ell_val[L.bpos[k]] = -L.bw[k];
+        L.ell_bound[L.bpos[k]] = 0.0;
+      }
       const double diag = de >= 0 ? L.irr_val[de] : 0.0;
       L.irr_diag[r] = diag;
       L.ell_diag[L.ell_row[r]] = diag;
@@ -1120,6 +1189,9 @@ void level_values(LevelData& L) {
   L.cn.resize(nbk * bx);
   L.dinv.resize(L.ncells);
   std::vector<uint8_t> regular(host_threads(), 0);
+  // regular cells with a positive open-face conductance: their own row ratio
+  // (diagonal plus the conductances of their interior faces, over the diagonal)
+  std::vector<double> bratio(host_threads(), 0.0);
   parallel_for(long(nbk), [&](long lo, long hi, int t) {
     for (long b = lo; b < hi; ++b) {
       const double* cx = &L.cx[b * (bx + 1) * by];
@@ -1139,8 +1211,20 @@ void level_values(LevelData& L) {
           d = L.irr_diag[L.cell_irr[c]];
         } else {
           const int i = q % bx, j = q / bx;
-          d = cx[j * (bx + 1) + i] + cx[j * (bx + 1) + i + 1] + cy[j * bx + i] + cy[(j + 1) * bx + i];
-          if (d > 0) regular[t] = 1;
+          const double cwv = cx[j * (bx + 1) + i], cev = cx[j * (bx + 1) + i + 1], csv = cy[j * bx + i],
+                       cnv = cy[(j + 1) * bx + i];
+          d = cwv + cev + csv + cnv;
+          const bool ow = i == 0 && L.nbr[4 * b] < 0, oe = i == bx - 1 && L.nbr[4 * b + 1] < 0,
+                     os = j == 0 && L.nbr[4 * b + 2] < 0, on = j == by - 1 && L.nbr[4 * b + 3] < 0;
+          const bool bnd = (ow && cwv > 0) || (oe && cev > 0) || (os && csv > 0) || (on && cnv > 0);
+          if (d > 0) {
+            if (!bnd) {
+              regular[t] = 1;
+            } else {
+              const double isum = (ow ? 0.0 : cwv) + (oe ? 0.0 : cev) + (os ? 0.0 : csv) + (on ? 0.0 : cnv);
+              bratio[t] = std::max(bratio[t], (d + isum) / d);
+            }
+          }
         }
         L.dinv[c] = d > 0 ? 1.0 / d : 0.0;
       }
@@ -1148,6 +1232,7 @@ void level_values(LevelData& L) {
   });
   const bool any_regular = *std::max_element(regular.begin(), regular.end()) != 0;
   if (any_regular) max_ratio = std::max(max_ratio, 2.0);
+  max_ratio = std::max(max_ratio, *std::max_element(bratio.begin(), bratio.end()));
   if (max_ratio == 0) {
     L.omega = 1.0;
     L.omega_p = 0.0;
@@ -1245,6 +1330,84 @@ void restriction_map(LevelData& C, const LevelData& Fl) {
   });
 }
 
+// Boundary-coefficient restriction from the finer level
+// (`_face_coefficient_restriction(...; boundary = true)`,
+// tree_coarsening.jl:347-398, with the boundary-face lattice geometry of
+// mesh.jl:476-485): every coarse boundary face covers fine boundary faces on
+// its line (axis, normal coordinate on the common lattice), weighted by the
+// overlap fraction of its length.
+void boundary_restriction_map(LevelData& C, const LevelData& Fl) {
+  const TreeMesh& fm = *Fl.mesh;
+  const TreeMesh& cm = *C.mesh;
+  long lat[2];
+  for (int a = 0; a < 2; ++a)
+    lat[a] = std::max(long(fm.base[a]) << fm.max_leaf, long(cm.base[a]) << cm.max_leaf);
+  auto geom = [&](const TreeMesh& M, int f, long& n, long& lo, long& hi) {
+    const int nb = M.nb(), bx = M.bs[0], by = M.bs[1];
+    const int c = M.bcell[f];
+    const BlockKey& k = M.blocks[c / nb];
+    const long sc[2] = {lat[0] / (long(M.base[0]) << k.level), lat[1] / (long(M.base[1]) << k.level)};
+    const long low[2] = {(long(k.i) * bx + (c % nb) % bx) * sc[0], (long(k.j) * by + (c % nb) / bx) * sc[1]};
+    const int a = M.baxis[f], t = 1 - a;
+    n = low[a] + (M.bside[f] > 0 ? sc[a] : 0);
+    lo = low[t];
+    hi = low[t] + sc[t];
+  };
+  std::map<std::pair<int, long>, std::vector<std::array<long, 3>>> lines;
+  for (int f = 0; f < int(fm.bcell.size()); ++f) {
+    long n, lo, hi;
+    geom(fm, f, n, lo, hi);
+    lines[{int(fm.baxis[f]), n}].push_back({lo, hi, long(f)});
+  }
+  for (auto& kv : lines)
+    std::stable_sort(kv.second.begin(), kv.second.end(), [](const auto& x, const auto& y) { return x[0] < y[0]; });
+  C.brptr.assign(1, 0);
+  C.bridx.clear();
+  C.brw.clear();
+  for (int f = 0; f < int(cm.bcell.size()); ++f) {
+    long n, lo, hi;
+    geom(cm, f, n, lo, hi);
+    long covered = 0;
+    auto it = lines.find({int(cm.baxis[f]), n});
+    if (it != lines.end())
+      for (const auto& seg : it->second) {
+        if (seg[1] <= lo) continue;
+        if (seg[0] >= hi) break;
+        const long ov = std::min(hi, seg[1]) - std::max(lo, seg[0]);
+        if (ov > 0) {
+          C.bridx.push_back(int(seg[2]));
+          C.brw.push_back(double(ov) / double(hi - lo));
+          covered += ov;
+        }
+      }
+    if (covered != hi - lo) throw std::runtime_error("tree hierarchy could not cover a coarse face with fine faces");
+    C.brptr.push_back(int(C.bridx.size()));
+  }
+}
+
+// Open-face conductances |f| 2 kb / h (the boundary diagonal terms of
+// `_elliptic_boundary_data!`, elliptic.jl:133-152: factor 2 kb / h, then
+// |f| * aperture 1 * factor), the irregular rows' boundary diagonals (sums
+// in face order) and the level's singularity (no open face conducts).
+void boundary_values(LevelData& L) {
+  const TreeMesh& M = *L.mesh;
+  const int nb = M.nb();
+  L.bw.resize(L.bopen.size());
+  bool gauged = false;
+  for (size_t k = 0; k < L.bopen.size(); ++k) {
+    const int f = L.bopen[k];
+    const double h = M.bspacing[2 * (M.bcell[f] / nb) + M.baxis[f]];
+    const double factor = 2 * L.kb[f] / h;
+    L.bw[k] = M.barea[f] * 1.0 * factor;
+    gauged = gauged || L.bw[k] != 0.0;
+  }
+  const int nrows = int(L.irr_local.size());
+  L.irr_bd.assign(nrows, 0.0);
+  for (int r = 0; r < nrows; ++r)
+    for (int q = L.irr_bptr[r]; q < L.irr_bptr[r + 1]; ++q) L.irr_bd[r] += L.bw[L.irr_bface[q]];
+  L.singular = !gauged;
+}
+
 void transfer_descriptors(LevelData& Fl, const LevelData& C, bool agglomerated) {
   const TreeMesh& fm = *Fl.mesh;
   const TreeMesh& cm = *C.mesh;
@@ -1272,13 +1435,15 @@ void transfer_descriptors(LevelData& Fl, const LevelData& C, bool agglomerated) 
   }
 }
 
-// Dense augmented coarsest operator [A V; V^T 0] (one connected component,
-// no anchors), row-major N x N with N = n + 1, into H.coarse_aug.
+// Dense coarsest operator into H.coarse_aug, row-major N x N: the augmented
+// [A V; V^T 0] (N = n + 1) of a singular level (one connected, ungauged
+// component), A itself (N = n) of a level that open sides anchor
+// (geometric_preconditioner.jl:144-169, 415-430).
 void coarse_matrix(Hierarchy& H) {
   LevelData& L = *H.levels.back();
   const TreeMesh& M = *L.mesh;
   const int n = L.ncells, nb = M.nb(), bx = M.bs[0], by = M.bs[1];
-  const int N = n + 1;  // one component (connected, no anchors)
+  const int N = L.singular ? n + 1 : n;
   // The coarsest level is solved with a dense inverse. Tree coarsening ends
   // at coarse_size cells when the root layout agglomerates down to one
   // block; an odd root layout can stop far above that.
@@ -1287,6 +1452,7 @@ void coarse_matrix(Hierarchy& H) {
                                 " cells; the dense coarse solve supports at most " + std::to_string(COARSE_MAX) +
                                 " (use a root-block layout that halves down to a few blocks)");
   H.coarse_n = n;
+  H.coarse_N = N;
   std::vector<double>& a = H.coarse_aug;
   a.assign(size_t(N) * N, 0.0);
   for (int b = 0; b < L.nblocks; ++b)
@@ -1307,16 +1473,19 @@ void coarse_matrix(Hierarchy& H) {
         nbc[1] = i < bx - 1 ? cell_at(b, i + 1, j) : (L.nbr[4 * b + 1] >= 0 ? cell_at(L.nbr[4 * b + 1], 0, j) : -1);
         nbc[2] = j > 0 ? cell_at(b, i, j - 1) : (L.nbr[4 * b + 2] >= 0 ? cell_at(L.nbr[4 * b + 2], i, by - 1) : -1);
         nbc[3] = j < by - 1 ? cell_at(b, i, j + 1) : (L.nbr[4 * b + 3] >= 0 ? cell_at(L.nbr[4 * b + 3], i, 0) : -1);
+        // a regular cell's slot without a neighbour is an open boundary face:
+        // its conductance is the boundary diagonal term
         for (int s = 0; s < 4; ++s) {
           if (w[s] == 0.0) continue;
-          if (nbc[s] < 0) throw std::runtime_error("regular face without a neighbour");
           a[size_t(c) * N + c] += w[s];
-          a[size_t(c) * N + nbc[s]] -= w[s];
+          if (nbc[s] >= 0) a[size_t(c) * N + nbc[s]] -= w[s];
         }
       }
-      const double V = L.volume[b];
-      a[size_t(c) * N + n] = V;
-      a[size_t(n) * N + c] = V;
+      if (N > n) {
+        const double V = L.volume[b];
+        a[size_t(c) * N + n] = V;
+        a[size_t(n) * N + c] = V;
+      }
     }
 }
 
@@ -1336,14 +1505,17 @@ bool same_tree_topology(const TreeMesh& a, const TreeMesh& b) {
 }
 
 std::unique_ptr<Hierarchy> build_hierarchy(std::unique_ptr<TreeMesh> fine, const MultigridConfig& cfg,
-                                           Hierarchy* previous) {
+                                           Hierarchy* previous, const bool* open) {
   if (cfg.coarse_size <= 0) throw std::invalid_argument("coarse_size must be positive");
   if (cfg.pre < 0 || cfg.post < 0) throw std::invalid_argument("sweep counts must be nonnegative");
   if (cfg.max_levels <= 0) throw std::invalid_argument("max_levels must be positive");
+  bool sides[4] = {false, false, false, false};
+  for (int q = 0; q < 4 && open; ++q) sides[q] = open[q];
   auto H = std::make_unique<Hierarchy>();
   auto L0 = std::make_unique<LevelData>();
   L0->owned = std::move(fine);
   L0->mesh = L0->owned.get();
+  std::copy(sides, sides + 4, L0->open);
   level_topology(*L0);
   H->levels.push_back(std::move(L0));
   std::vector<bool> agg;
@@ -1360,7 +1532,9 @@ std::unique_ptr<Hierarchy> build_hierarchy(std::unique_ptr<TreeMesh> fine, const
     if (previous) {
       int match = -1;
       for (int l = 0; l < int(previous->levels.size()) && match < 0; ++l)
-        if (previous->levels[l] && same_tree_topology(*previous->levels[l]->mesh, *cm)) match = l;
+        if (previous->levels[l] && same_tree_topology(*previous->levels[l]->mesh, *cm) &&
+            std::equal(sides, sides + 4, previous->levels[l]->open))
+          match = l;
       const int suffix = match < 0 ? 0 : int(previous->levels.size()) - match;
       if (match >= 0 && int(H->levels.size()) + suffix <= cfg.max_levels) {
         // the matching level's restriction and the transfer into it come
@@ -1369,6 +1543,7 @@ std::unique_ptr<Hierarchy> build_hierarchy(std::unique_ptr<TreeMesh> fine, const
         H->reused_old = match;
         std::unique_ptr<LevelData> L = std::move(previous->levels[match]);
         restriction_map(*L, *H->levels.back());
+        boundary_restriction_map(*L, *H->levels.back());
         transfer_descriptors(*H->levels.back(), *L, is_agg);
         H->levels.push_back(std::move(L));
         for (int l = match + 1; l < int(previous->levels.size()); ++l)
@@ -1380,9 +1555,11 @@ std::unique_ptr<Hierarchy> build_hierarchy(std::unique_ptr<TreeMesh> fine, const
     auto L = std::make_unique<LevelData>();
     L->owned = std::move(cm);
     L->mesh = L->owned.get();
+    std::copy(sides, sides + 4, L->open);
     level_topology(*L);
     pt.t = std::chrono::steady_clock::now();
     restriction_map(*L, *H->levels.back());
+    boundary_restriction_map(*L, *H->levels.back());
     pt.mark("restriction map");
     transfer_descriptors(*H->levels.back(), *L, is_agg);
     pt.mark("transfer descriptors");
@@ -1392,6 +1569,66 @@ std::unique_ptr<Hierarchy> build_hierarchy(std::unique_ptr<TreeMesh> fine, const
   H->total_volume = 0;
   for (int b = 0; b < H->levels[0]->nblocks; ++b) H->total_volume += H->levels[0]->volume[b] * M0.nb();
   return H;
+}
+
+void set_boundary_coefficients(Hierarchy& H, const double* kb) {
+  LevelData& L0 = *H.levels[0];
+  const int BF = int(L0.mesh->bcell.size());
+  for (int f = 0; f < BF; ++f)
+    if (!(std::isfinite(kb[f]) && kb[f] >= 0))
+      throw std::invalid_argument("boundary coefficients must be finite and nonnegative");
+  L0.kb.assign(kb, kb + BF);
+  for (size_t l = 0; l < H.levels.size(); ++l) {
+    LevelData& L = *H.levels[l];
+    if (l > 0) {
+      // _restrict_face_coefficients! (tree_coarsening.jl:538-555)
+      const LevelData& P = *H.levels[l - 1];
+      L.kb.resize(L.mesh->bcell.size());
+      for (size_t f = 0; f < L.kb.size(); ++f) {
+        double v = 0;
+        for (int q = L.brptr[f]; q < L.brptr[f + 1]; ++q) v += L.brw[q] * P.kb[L.bridx[q]];
+        L.kb[f] = v;
+      }
+    }
+    boundary_values(L);
+  }
+  // one connected component: every level is anchored exactly when the
+  // finest one is (restriction weights are positive)
+  H.singular = L0.singular;
+  for (const auto& L : H.levels)
+    if (L->singular != H.singular) throw std::logic_error("levels disagree on the pressure gauge");
+}
+
+std::vector<double> boundary_material(const TreeMesh& m, const bool open[4], const double* kappa) {
+  const int BF = int(m.bcell.size());
+  std::vector<double> values(BF, 0.0), weights(BF, 0.0);
+  auto is_open = [&](int f) { return open[2 * m.baxis[f] + (m.bside[f] > 0 ? 1 : 0)]; };
+  // (cell, axis, side) of every open face
+  std::unordered_map<uint64_t, int> index;
+  auto key = [](int cell, int axis, int side) {
+    return (uint64_t(uint32_t(cell)) << 2) | (uint64_t(axis) << 1) | uint64_t(side > 0 ? 1 : 0);
+  };
+  for (int f = 0; f < BF; ++f)
+    if (is_open(f)) index[key(m.bcell[f], m.baxis[f], m.bside[f])] = f;
+  if (index.empty()) return values;
+  for (int f = 0; f < m.nfaces(); ++f) {
+    const int ends[2][2] = {{m.fminus[f], -1}, {m.fplus[f], 1}};
+    for (const auto& e : ends) {
+      auto it = index.find(key(e[0], m.faxis[f], e[1]));
+      if (it == index.end()) continue;
+      values[it->second] += m.farea[f] * kappa[f];
+      weights[it->second] += m.farea[f];
+    }
+  }
+  for (int f = 0; f < BF; ++f) {
+    if (!is_open(f)) continue;
+    if (!(weights[f] > 0))
+      throw std::invalid_argument(
+          "cannot extend an interior coefficient to an open face without an opposite interior face; supply "
+          "boundary_face_coefficient");
+    values[f] /= weights[f];
+  }
+  return values;
 }
 
 void refresh_coarsest(Hierarchy& H, const double* kappa_coarsest) {
@@ -1409,6 +1646,8 @@ void refresh_coefficients(Hierarchy& H, const double* kappa) {
     if (!(kappa[f] > 0))
       throw std::invalid_argument("face coefficients must be positive (disconnected components are not supported)");
   }
+  if (L0.any_open() && L0.kb.size() != L0.mesh->bcell.size())
+    throw std::logic_error("open pressure sides need boundary coefficients before the coefficient refresh");
   L0.kappa.assign(kappa, kappa + F);
   for (size_t l = 0; l < H.levels.size(); ++l) {
     LevelData& L = *H.levels[l];

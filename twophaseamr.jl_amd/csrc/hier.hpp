@@ -145,15 +145,42 @@ struct LevelData {
   std::vector<int> rptr;
   hvec<int> ridx;
   hvec<double> rw;
-  // incidence of irregular cells: face lists
+  // Open (homogeneous Dirichlet) pressure sides, left / right / bottom / top
+  // (PoissonOperator's velocity_boundary, elliptic.jl:76-88). Per boundary
+  // face its coefficient kb (the finest level: the caller's; coarser ones:
+  // the overlap-weighted restriction brptr/bridx/brw of the finer level's);
+  // the open faces (bopen, boundary-face order) with their conductance
+  // |f| 2 kb / h (bw, the boundary diagonal term of elliptic.jl:133-152) and
+  // destination: a regular cell's conductance slot beyond that side (bslot:
+  // flat index into cx (x faces) or cy (y faces); bdst: the device code
+  // index * 4 + array of the regular faces), or an irregular row's extra ELL
+  // entry of value -bw whose column is the zero halo slot beyond that side
+  // (bpos: its ELL position; irr_bptr / irr_bface: each row's open faces).
+  // irr_bd: per irregular row the sum of its open faces' conductances (face
+  // order), added to the row's diagonal. A level with an open face of
+  // positive conductance is nonsingular.
+  bool open[4] = {false, false, false, false};
+  std::vector<double> kb;
+  std::vector<int> brptr, bridx;
+  std::vector<double> brw;
+  std::vector<int> bopen, bdst, bpos, irr_bptr, irr_bface;
+  std::vector<int64_t> bslot;
+  std::vector<double> bw, irr_bd;
+  bool singular = true;
+  bool any_open() const { return open[0] || open[1] || open[2] || open[3]; }
 };
 
 struct Hierarchy {
   std::vector<std::unique_ptr<LevelData>> levels;
-  // coarsest level: the dense augmented operator [A V; V^T 0] ((n + 1) x
-  // (n + 1), row major); the device inverts it
-  int coarse_n = 0;
+  // coarsest level: the dense operator, augmented [A V; V^T 0] ((n + 1) x
+  // (n + 1)) when the levels are singular, A itself (n x n) when open sides
+  // anchor them; row major, coarse_N rows; the device inverts it
+  int coarse_n = 0, coarse_N = 0;
   std::vector<double> coarse_aug;
+  // every level singular (wall / periodic sides, or open sides whose
+  // boundary coefficients are all zero): the solve projects the rhs and
+  // removes volume means; otherwise no mean is removed
+  bool singular = true;
   double total_volume = 0;
   // levels taken over from the previous hierarchy (_tree_reuse_suffix!):
   // this hierarchy's levels from reused_new on are the previous one's from
@@ -193,8 +220,21 @@ struct MultigridConfig {
 // tree topology equals one of its levels ends the coarsening: that level and
 // the coarser ones are taken over from it (_tree_reuse_suffix!,
 // tree_coarsening.jl:214-226), and `previous` loses them.
+// `open` (NULL: none) are the open pressure sides (left, right, bottom, top);
+// they shape the irregular rows, so a change of open sides needs a new
+// hierarchy, and `previous` must have been built with the same sides.
 std::unique_ptr<Hierarchy> build_hierarchy(std::unique_ptr<TreeMesh> fine, const MultigridConfig& cfg,
-                                           Hierarchy* previous = nullptr);
+                                           Hierarchy* previous = nullptr, const bool* open = nullptr);
+// Boundary coefficients of the finest level's boundary faces (one per
+// boundary face, reference order): restricted to every level, with each
+// level's open-face conductances, row boundary diagonals and singular flags;
+// the next refresh_coefficients / level value pass uses them.
+void set_boundary_coefficients(Hierarchy& H, const double* kb);
+// `_boundary_material` (elliptic.jl:90-131) for interior face coefficients:
+// every open boundary face of `m` takes the area-weighted mean of the
+// interior coefficients on the opposite side of its cell along its axis;
+// closed faces 0.
+std::vector<double> boundary_material(const TreeMesh& m, const bool open[4], const double* kappa);
 // _same_tree_topology (tree_coarsening.jl:84-121): domain, block size and
 // leaf keys (faces and geometry follow from them)
 bool same_tree_topology(const TreeMesh& a, const TreeMesh& b);

@@ -524,6 +524,20 @@ __global__ void k_projection_rhs(int n, const double* __restrict__ div, const do
 
 // y = a x + b y; with b = 0, y is only written (it may hold anything, NaN
 // included, before); x may alias y
+// open boundary faces after a projection (_project_prepared!,
+// projection.jl:248-255): ub -= dt g_b with the boundary weighted gradient
+// g_b = (-side * bgrad) * phi_cell (_elliptic_boundary_gradient!,
+// elliptic.jl:1254-1263); closed faces have bgrad 0
+__global__ void k_boundary_velocity(int nbf, const int* __restrict__ bcell, const signed char* __restrict__ bside,
+                                    const double* __restrict__ bgrad, const double* __restrict__ phi, double dt,
+                                    double* __restrict__ ub) {
+#pragma clang fp contract(off)
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nbf) return;
+  const double g = (-double(bside[f]) * bgrad[f]) * phi[bcell[f]];
+  ub[f] = ub[f] - dt * g;
+}
+
 __global__ void k_axpby(int n, double a, const double* x, double b, double* y) {
   const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (c < size_t(n)) y[c] = b != 0.0 ? a * x[c] + b * y[c] : a * x[c];
@@ -1094,13 +1108,34 @@ __global__ void k_ref_entries(int nnz, const int* __restrict__ term_ptr, const i
   val_e[e] = v;
 }
 
-// irregular rows, one thread per row: the row's diagonal and the largest row
+// open boundary faces, one thread per face: the conductance bw into the
+// regular cell's slot beyond the side (dst code index * 4 + array), or -bw
+// into the irregular row's extra ELL entry (pos)
+__global__ void k_ref_boundary(int nopen, const double* __restrict__ bw, const int* __restrict__ dst,
+                               const int* __restrict__ pos, double* cw, double* ce, double* cs, double* cn,
+                               double* __restrict__ ell_val, double* __restrict__ ell_bound) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nopen) return;
+  const int d = dst[k];
+  if (d >= 0) {
+    double* arr[4] = {cw, ce, cs, cn};
+    arr[d & 3][d >> 2] = bw[k];
+  } else {
+    ell_val[pos[k]] = -bw[k];
+    if (ell_bound) ell_bound[pos[k]] = 0.0;
+  }
+}
+
+// irregular rows, one thread per row: the row's diagonal (with the row's
+// boundary diagonal bd when open sides are present) and the largest row
 // ratio sum |a_rj| / a_rr (as the bits of a nonnegative double, whose order is
 // the numeric order), summed in entry order
 __global__ void k_ref_rows(int nrows, const int* __restrict__ irr_ptr, const int* __restrict__ diag_e,
                            const int* __restrict__ ell_row, const double* __restrict__ val_e,
                            double* __restrict__ ell_diag, double* __restrict__ irr_diag,
-                           unsigned long long* __restrict__ ratio) {
+                           unsigned long long* __restrict__ ratio, const double* __restrict__ bd,
+                           const int* __restrict__ ell_pos, double* __restrict__ ell_val,
+                           double* __restrict__ ell_bound) {
 #pragma clang fp contract(off)
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   // the largest row ratio of the wavefront, then one atomic per wavefront
@@ -1110,9 +1145,14 @@ __global__ void k_ref_rows(int nrows, const int* __restrict__ irr_ptr, const int
     double rowsum = 0, diag = 0;
     const int de = diag_e[r];
     for (int e = irr_ptr[r]; e < irr_ptr[r + 1]; ++e) {
-      const double v = val_e[e];
+      double v = val_e[e];
+      if (bd && e == de) v = v + bd[r];
       rowsum += fabs(v);
       if (e == de) diag = v;
+    }
+    if (bd && de >= 0) {
+      ell_val[ell_pos[de]] = diag;
+      if (ell_bound) ell_bound[ell_pos[de]] = ell_bound[ell_pos[de]] + bd[r];
     }
     irr_diag[r] = diag;
     ell_diag[ell_row[r]] = diag;
@@ -1127,26 +1167,48 @@ __global__ void k_ref_rows(int nrows, const int* __restrict__ irr_ptr, const int
 __global__ void k_ref_dinv(int n, int bx, int by, const double* __restrict__ cw, const double* __restrict__ ce,
                            const double* __restrict__ cs, const double* __restrict__ cn,
                            const int* __restrict__ cell_irr, const double* __restrict__ irr_diag,
-                           double* __restrict__ dinv, int* __restrict__ regular) {
+                           double* __restrict__ dinv, int* __restrict__ regular, const int* __restrict__ nbr,
+                           unsigned long long* __restrict__ ratio) {
 #pragma clang fp contract(off)
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   bool reg = false;
+  double q = 0;
   if (c < n) {
-    const int nb = bx * by, b = c / nb, q = c - b * nb, i = q % bx, j = q / bx;
+    const int nb = bx * by, b = c / nb, qq = c - b * nb, i = qq % bx, j = qq / bx;
     double d;
     const int r = cell_irr[c];
     if (r >= 0) {
       d = irr_diag[r];
     } else {
-      const double e = i < bx - 1 ? cw[c + 1] : ce[b * by + j];
-      const double nn = j < by - 1 ? cs[c + bx] : cn[b * bx + i];
-      d = cw[c] + e + cs[c] + nn;
-      reg = d > 0;
+      const double w = cw[c], e = i < bx - 1 ? cw[c + 1] : ce[b * by + j];
+      const double s = cs[c], nn = j < by - 1 ? cs[c + bx] : cn[b * bx + i];
+      d = w + e + s + nn;
+      // a slot at the block edge without a neighbour block is a physical
+      // side: with a positive conductance, an open boundary face
+      bool ow = false, oe = false, os = false, on = false;
+      if (nbr) {
+        ow = i == 0 && nbr[4 * b] < 0;
+        oe = i == bx - 1 && nbr[4 * b + 1] < 0;
+        os = j == 0 && nbr[4 * b + 2] < 0;
+        on = j == by - 1 && nbr[4 * b + 3] < 0;
+      }
+      const bool bnd = (ow && w > 0) || (oe && e > 0) || (os && s > 0) || (on && nn > 0);
+      if (d > 0) {
+        if (!bnd) {
+          reg = true;
+        } else {
+          const double isum = (ow ? 0.0 : w) + (oe ? 0.0 : e) + (os ? 0.0 : s) + (on ? 0.0 : nn);
+          q = (d + isum) / d;
+        }
+      }
     }
     dinv[c] = d > 0 ? 1.0 / d : 0.0;
   }
   // one store per wavefront that has a positive regular diagonal
   if (__any(reg) && (threadIdx.x & 63) == 0) *regular = 1;
+  // the largest open-boundary row ratio of the wavefront, one atomic
+  for (int off = 32; off > 0; off >>= 1) q = fmax(q, __shfl_xor(q, off, 64));
+  if ((threadIdx.x & 63) == 0 && q > 0) atomicMax(ratio, (unsigned long long)__double_as_longlong(q));
 }
 
 // any face coefficient not finite or not positive
@@ -1530,22 +1592,29 @@ void launch_ref_regular(hipStream_t s, int nf, const double* k, const double* kd
   if (nf > 0)
     hipLaunchKernelGGL(k_ref_regular, dim3(cdiv(nf, 256)), dim3(256), 0, s, nf, k, kdiag, area, dst, cw, ce, cs, cn);
 }
+void launch_ref_boundary(hipStream_t s, int nopen, const double* bw, const int* dst, const int* pos, double* cw,
+                         double* ce, double* cs, double* cn, double* ell_val, double* ell_bound) {
+  if (nopen > 0)
+    hipLaunchKernelGGL(k_ref_boundary, dim3(cdiv(nopen, 256)), dim3(256), 0, s, nopen, bw, dst, pos, cw, ce, cs, cn,
+                       ell_val, ell_bound);
+}
 void launch_ref_rows(hipStream_t s, int nrows, const int* irr_ptr, const int* term_ptr, const int* term_f,
                      const int* term_g, const double* term_k, const double* term_s, const unsigned char* term_diag,
                      const double* k, const double* area, const int* ell_pos, const int* ell_row, const int* diag_e,
                      double* ell_val, double* ell_bound, double* ell_diag, double* irr_diag, double* val_e,
-                     int nnz, unsigned long long* ratio) {
+                     int nnz, unsigned long long* ratio, const double* bd) {
   if (nnz > 0)
     hipLaunchKernelGGL(k_ref_entries, dim3(cdiv(nnz, 256)), dim3(256), 0, s, nnz, term_ptr, term_f, term_g, term_k,
                        term_s, term_diag, k, area, ell_pos, ell_val, ell_bound, val_e);
   if (nrows > 0)
     hipLaunchKernelGGL(k_ref_rows, dim3(cdiv(nrows, 256)), dim3(256), 0, s, nrows, irr_ptr, diag_e, ell_row, val_e,
-                       ell_diag, irr_diag, ratio);
+                       ell_diag, irr_diag, ratio, bd, ell_pos, ell_val, ell_bound);
 }
 void launch_ref_dinv(hipStream_t s, int n, int bx, int by, const double* cw, const double* ce, const double* cs,
-                     const double* cn, const int* cell_irr, const double* irr_diag, double* dinv, int* regular) {
+                     const double* cn, const int* cell_irr, const double* irr_diag, double* dinv, int* regular,
+                     const int* nbr, unsigned long long* ratio) {
   hipLaunchKernelGGL(k_ref_dinv, dim3(cdiv(n, 256)), dim3(256), 0, s, n, bx, by, cw, ce, cs, cn, cell_irr, irr_diag,
-                     dinv, regular);
+                     dinv, regular, nbr, ratio);
 }
 void launch_ref_check(hipStream_t s, int nf, const double* k, int* bad) {
   if (nf > 0) hipLaunchKernelGGL(k_ref_check, dim3(cdiv(nf, 256)), dim3(256), 0, s, nf, k, bad);
@@ -1689,6 +1758,12 @@ void launch_projection_rhs(hipStream_t s, int n, const double* div, const double
   hipLaunchKernelGGL(k_projection_rhs, dim3(cdiv(n, 256)), dim3(256), 0, s, n, div, inherited, dt, w, rhs);
 }
 
+void launch_boundary_velocity(hipStream_t s, int nbf, const int* bcell, const signed char* bside, const double* bgrad,
+                              const double* phi, double dt, double* ub) {
+  if (nbf > 0)
+    hipLaunchKernelGGL(k_boundary_velocity, dim3(cdiv(nbf, 256)), dim3(256), 0, s, nbf, bcell, bside, bgrad, phi, dt,
+                       ub);
+}
 void launch_axpby(hipStream_t s, int n, double a, const double* x, double b, double* y) {
   hipLaunchKernelGGL(k_axpby, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, x, b, y);
 }

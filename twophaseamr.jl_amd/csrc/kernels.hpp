@@ -89,13 +89,21 @@ void launch_ref_restrict(hipStream_t s, int nf, const int* rptr, const int* ridx
                          const double* kprev, double* k);
 void launch_ref_regular(hipStream_t s, int nf, const double* k, const double* kdiag, const double* area,
                         const int* dst, double* cw, double* ce, double* cs, double* cn);
+// open boundary faces: bw into a regular cell's slot (dst >= 0) or -bw into an
+// irregular row's extra ELL entry (pos)
+void launch_ref_boundary(hipStream_t s, int nopen, const double* bw, const int* dst, const int* pos, double* cw,
+                         double* ce, double* cs, double* cn, double* ell_val, double* ell_bound);
+// bd (NULL without open sides): per row the boundary diagonal added to its diagonal
 void launch_ref_rows(hipStream_t s, int nrows, const int* irr_ptr, const int* term_ptr, const int* term_f,
                      const int* term_g, const double* term_k, const double* term_s, const unsigned char* term_diag,
                      const double* k, const double* area, const int* ell_pos, const int* ell_row, const int* diag_e,
                      double* ell_val, double* ell_bound, double* ell_diag, double* irr_diag, double* val_e,
-                     int nnz, unsigned long long* ratio);
+                     int nnz, unsigned long long* ratio, const double* bd);
+// nbr (NULL without open sides): regular cells with an open boundary slot add
+// their row ratio to `ratio`
 void launch_ref_dinv(hipStream_t s, int n, int bx, int by, const double* cw, const double* ce, const double* cs,
-                     const double* cn, const int* cell_irr, const double* irr_diag, double* dinv, int* regular);
+                     const double* cn, const int* cell_irr, const double* irr_diag, double* dinv, int* regular,
+                     const int* nbr, unsigned long long* ratio);
 void launch_ref_check(hipStream_t s, int nf, const double* k, int* bad);
 void launch_sqrt(hipStream_t s, int n, const double* in, double* out);
 // inverse of the dense N x N row-major matrix `a` (overwritten; Gauss-Jordan
@@ -230,6 +238,9 @@ void launch_projection_rhs(hipStream_t s, int n, const double* div, const double
                            double* rhs);
 // y = a x + b y (b = 0: y not read)
 void launch_axpby(hipStream_t s, int n, double a, const double* x, double b, double* y);
+// ub -= dt (-side bgrad) phi_cell on every boundary face
+void launch_boundary_velocity(hipStream_t s, int nbf, const int* bcell, const signed char* bside, const double* bgrad,
+                              const double* phi, double dt, double* ub);
 void launch_cast_d2f(hipStream_t s, int n, const double* in, float* out);
 void launch_cast_f2d(hipStream_t s, int n, const float* in, double* out);
 void launch_copy(hipStream_t s, int n, const double* in, double* out);

@@ -105,6 +105,12 @@ struct DevPlan {
   int nnz = 0;  // irregular entries
   DevArray<unsigned char> term_diag;
   DevArray<double> cw, ce, cs, cn, ell_val, ell_diag, dinv;
+  // open boundary faces: conductance and destination (slot code or ELL
+  // position), and every irregular row's boundary diagonal
+  int nopen = 0;
+  bool open = false;
+  DevArray<int> bdst, bpos;
+  DevArray<double> bw, irr_bd;
 };
 
 template <typename T>
@@ -129,6 +135,22 @@ struct tpamr_ctx {
   // may take over (same multigrid options)
   std::unique_ptr<Hierarchy> H_prev;
   bool have_coeff = false;
+  // open (homogeneous Dirichlet) pressure sides, left / right / bottom / top,
+  // and the finest level's boundary face coefficients: the caller's
+  // (kb_user), or, with kb_extend, extended from the face coefficients of the
+  // next coefficient update; kb_dirty: the device refresh plan's boundary
+  // values are older than the hierarchy's
+  // kb_pending: not yet applied to the hierarchy (the next coefficient
+  // update applies it); bgrad_dirty: the device boundary gradient factors
+  // are older than the hierarchy's coefficients
+  bool open[4] = {false, false, false, false};
+  std::vector<double> kb_user;
+  bool kb_extend = true, kb_dirty = false, kb_pending = false, bgrad_dirty = false;
+  bool any_open() const { return open[0] || open[1] || open[2] || open[3]; }
+  // finest level's boundary faces on the device: cell, gradient factor
+  // 2 kb / h (0 on closed faces), and the boundary velocities of a projection
+  DevArray<int> bcell;
+  DevArray<double> bgrad, fb;
   // device data
   std::vector<DevTopo> topo;
   std::vector<DevLevel<double>> ld;  // FP64 levels (level 0 always, all in FP64 mode)
@@ -431,6 +453,13 @@ void upload_hierarchy(tpamr_ctx* c, bool topology) {
     for (int f = 0; f < BF; ++f) bf[bnext[M.bcell[f]]++] = f;
     c->bptr.upload(bcnt, s);
     c->bface.upload(bf, s);
+    {
+      std::vector<int> bc(M.bcell.begin(), M.bcell.end());
+      if (bc.empty()) bc.push_back(0);
+      c->bcell.upload(bc, s);
+      c->fb.alloc(std::max(BF, 1));
+      c->bgrad_dirty = true;
+    }
     std::vector<signed char> bsd(M.bside.begin(), M.bside.end());
     if (bsd.empty()) bsd.push_back(0);
     c->bside.upload(bsd, s);
@@ -609,8 +638,18 @@ double roundoff_floor(tpamr_ctx* c) {
   return 128 * EPS * std::max(1.0, m);
 }
 
-// subtract the volume-weighted mean (one connected, unanchored component)
+// The volume that divides the means the Krylov kernels remove (z, and the
+// residual's compatibility drift): the total volume of a singular operator
+// (one connected, ungauged component), infinity for one that open sides
+// anchor, whose means are then 0 / inf = 0 and nothing is removed.
+double gauge_volume(const tpamr_ctx* c) {
+  return c->H->singular ? c->H->total_volume : std::numeric_limits<double>::infinity();
+}
+
+// subtract the volume-weighted mean (one connected, unanchored component;
+// nothing for an operator that open sides anchor)
 void remove_mean(tpamr_ctx* c, double* v) {
+  if (!c->H->singular) return;
   const int N = c->topo[0].ncells, nb = c->H->levels[0]->mesh->nb();
   launch_weighted_sums(c->stream, N, nb, v, c->topo[0].volume.p, c->rs.part);
   launch_reduce(c->stream, ncb(N), 1, c->rs.part, c->rs.scal, false);
@@ -651,7 +690,7 @@ void enqueue_iteration(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, bool init) {
                            L0.dinv.p, w0, c->rs.maxpart, c->rs.st, 0);
   }
   T* z = enqueue_precondition<T>(c, lv, done);
-  launch_cg_beta(s, nblk, ncb(N), c->rs.part, c->rs.maxpart, c->rs.stage, c->H->total_volume, c->rs.st, init ? 1 : 0);
+  launch_cg_beta(s, nblk, ncb(N), c->rs.part, c->rs.maxpart, c->rs.stage, gauge_volume(c), c->rs.st, init ? 1 : 0);
   launch_cg_update_p<T>(s, N, z, c->p.p, c->rs.st, done);
 }
 
@@ -757,7 +796,7 @@ void enqueue_ir_iteration(tpamr_ctx* c, std::vector<DevLevel<float>>& lv, bool i
   const double w0 = c->cfg.pre > 0 ? L0.omega : 0.0;
   if (!init) {
     launch_matvec_dot<float>(s, L0.view, c->p32.p, c->Ap32.p, c->rs.part, done, true);
-    launch_cg_alpha(s, nblk, c->rs.part, c->rs.stage, c->rs.st, true, c->H->total_volume);
+    launch_cg_alpha(s, nblk, c->rs.part, c->rs.stage, c->rs.st, true, gauge_volume(c));
   }
   launch_ir_update_xr(s, N, nb, c->p32.p, c->Ap32.p, c->e32.p, L0.b.p, c->topo[0].volume.p, L0.x.p, L0.dinv.p, w0,
                       c->r.p, c->rs.maxpart, c->rs.st, init ? 1 : 0);
@@ -770,7 +809,7 @@ void enqueue_ir_iteration(tpamr_ctx* c, std::vector<DevLevel<float>>& lv, bool i
   zs.r32 = L0.b.p;
   zs.part = c->rs.part;
   float* z = enqueue_vcycle<float>(c, lv, done, zs);
-  launch_cg_beta(s, nblk, ir_groups(N), c->rs.part, c->rs.maxpart, c->rs.stage, c->H->total_volume, c->rs.st,
+  launch_cg_beta(s, nblk, ir_groups(N), c->rs.part, c->rs.maxpart, c->rs.stage, gauge_volume(c), c->rs.st,
                  init ? 1 : 0);
   launch_ir_update_p(s, N, z, c->p32.p, c->rs.st, done);
 }
@@ -808,9 +847,13 @@ tpamr_stats solve_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double rtol, 
   // inputs are not scanned on the host)
   if (!std::isfinite(scale) || !std::isfinite(total_sum)) throw std::invalid_argument("right-hand side must be finite");
   if (!std::isfinite(sums[3])) throw std::invalid_argument("initial solution must be finite");
-  if (compat == 1 && std::fabs(total_sum) > 256 * EPS * std::max(1.0, scale))
+  // the compatible projection (_project_component_rhs!,
+  // elliptic_solver.jl:150-190) of a singular operator; an anchored one
+  // takes the rhs as it is
+  const bool singular = c->H->singular;
+  if (singular && compat == 1 && std::fabs(total_sum) > 256 * EPS * std::max(1.0, scale))
     throw IncompatibleError("Poisson right-hand side has nonzero volume mean on component rooted at cell 1");
-  const double mean = total_sum / c->H->total_volume;
+  const double mean = singular ? total_sum / c->H->total_volume : 0.0;
   st.removed_rhs_mean = mean;
   double il2, ilinf, floor0;
   if (zero_start) {
@@ -827,9 +870,11 @@ tpamr_stats solve_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double rtol, 
     floor0 = 128 * EPS * std::max(1.0, ilinf);
   } else {
     // b -= V * mean (scalar staged through device memory)
-    double h = mean;
-    HIPCHECK(hipMemcpyAsync(c->rs.scal + 8, &h, sizeof(double), hipMemcpyHostToDevice, s));
-    launch_shift(s, N, nb, c->b.p, c->rs.scal + 8, 1.0, vol, true);
+    if (singular) {
+      double h = mean;
+      HIPCHECK(hipMemcpyAsync(c->rs.scal + 8, &h, sizeof(double), hipMemcpyHostToDevice, s));
+      launch_shift(s, N, nb, c->b.p, c->rs.scal + 8, 1.0, vol, true);
+    }
     remove_mean(c, c->x.p);
     true_residual(c, il2, ilinf);
     floor0 = roundoff_floor(c);
@@ -1051,7 +1096,7 @@ void dense_inverse_blocked(tpamr_ctx* c, int N, int n) {
 void coarse_inverse_device(tpamr_ctx* c) {
   Hierarchy& H = *c->H;
   hipStream_t s = c->stream;
-  const int n = H.coarse_n, N = n + 1;
+  const int n = H.coarse_n, N = H.coarse_N;
   const size_t NN = size_t(N) * N;
   c->coarse_work.upload(H.coarse_aug, s);
   c->coarse_inv.alloc(size_t(n) * n);
@@ -1154,10 +1199,33 @@ void upload_plan(tpamr_ctx* c) {
       up(P.ell_diag, L.ell_diag, 0.0);
       P.dinv.upload(L.dinv, s);
     }
+    P.open = L.any_open();
+    P.nopen = int(L.bopen.size());
+    up(P.bdst, L.bdst, 0);
+    up(P.bpos, L.bpos, 0);
   }
   c->ref_ratio.alloc(nl);
   c->ref_flags.alloc(nl + 1);
   c->plan_ready = true;
+  c->kb_dirty = true;
+}
+
+// The boundary values of every level (conductances and row boundary
+// diagonals; they change with the boundary coefficients only) into the
+// device refresh plan.
+void upload_plan_boundary(tpamr_ctx* c) {
+  Hierarchy& H = *c->H;
+  for (size_t l = 0; l < H.levels.size(); ++l) {
+    const LevelData& L = *H.levels[l];
+    DevPlan& P = c->plan[l];
+    if (!P.open) continue;
+    if (L.bw.size() != L.bopen.size()) throw std::logic_error("open pressure sides without boundary coefficients");
+    if (L.bw.empty()) P.bw.upload(std::vector<double>(1, 0.0), c->stream);
+    else P.bw.upload(L.bw, c->stream);
+    if (L.irr_bd.empty()) P.irr_bd.upload(std::vector<double>(1, 0.0), c->stream);
+    else P.irr_bd.upload(L.irr_bd, c->stream);
+  }
+  c->kb_dirty = false;
 }
 
 // A coefficient refresh computed on the device (topology unchanged): the
@@ -1172,6 +1240,7 @@ void device_refresh(tpamr_ctx* c, const double* kappa, bool kappa_on_device = fa
   if (!c->plan_ready) upload_plan(c);
   if (int(c->plan.size()) != nl || c->plan[0].nfaces != c->mesh().nfaces() || int(c->topo.size()) != nl)
     throw StateError("device refresh plan does not match the current hierarchy");
+  if (c->kb_dirty) upload_plan_boundary(c);
   DevPlan& P0 = c->plan[0];
   if (kappa_on_device)
     HIPCHECK(hipMemcpyAsync(P0.kappa.p, kappa, sizeof(double) * P0.nfaces, hipMemcpyDeviceToDevice, s));
@@ -1207,10 +1276,12 @@ void device_refresh(tpamr_ctx* c, const double* kappa, bool kappa_on_device = fa
     launch_fill(s, int(L.cs.size()), 0.0, cs);
     launch_fill(s, int(L.cn.size()), 0.0, cn);
     launch_ref_regular(s, P.nfaces, P.kappa.p, P.kdiag.p, P.area.p, P.dst.p, cw, ce, cs, cn);
+    if (P.open) launch_ref_boundary(s, P.nopen, P.bw.p, P.bdst.p, P.bpos.p, cw, ce, cs, cn, ev, eb);
     launch_ref_rows(s, P.nrows, P.irr_ptr.p, P.term_ptr.p, P.term_f.p, P.term_g.p, P.term_k.p, P.term_s.p,
                     P.term_diag.p, P.kappa.p, P.area.p, P.ell_pos.p, P.ell_row.p, P.diag_e.p, ev, eb, ed,
-                    P.irr_diag.p, P.val_e.p, P.nnz, c->ref_ratio.p + l);
-    launch_ref_dinv(s, L.ncells, bx, by, cw, ce, cs, cn, P.cell_irr.p, P.irr_diag.p, di, c->ref_flags.p + l);
+                    P.irr_diag.p, P.val_e.p, P.nnz, c->ref_ratio.p + l, P.open ? P.irr_bd.p : nullptr);
+    launch_ref_dinv(s, L.ncells, bx, by, cw, ce, cs, cn, P.cell_irr.p, P.irr_diag.p, di, c->ref_flags.p + l,
+                    P.open ? c->topo[l].nbr.p : nullptr, c->ref_ratio.p + l);
     if (l < int(c->lf.size())) {
       DevLevel<float>& Fv = c->lf[l];
       launch_cast_d2f(s, int(L.cw.size()), cw, Fv.cw.p);
@@ -1257,6 +1328,38 @@ void device_refresh(tpamr_ctx* c, const double* kappa, bool kappa_on_device = fa
   refresh_coarsest(H, kc.data());
   coarse_inverse_device(c);
   c->host_stale = true;
+}
+
+// Pending boundary coefficients of open sides into the hierarchy (before its
+// value pass): the caller's, or extended from the face coefficients
+// (`_boundary_material`); `rebuilt`: a new hierarchy needs them again.
+void apply_boundary(tpamr_ctx* c, const double* kappa_host, bool rebuilt) {
+  if (!c->any_open() || (!c->kb_pending && !rebuilt)) return;
+  const TreeMesh& M = *c->H->levels[0]->mesh;
+  std::vector<double> kb = c->kb_extend ? boundary_material(M, c->open, kappa_host) : c->kb_user;
+  if (kb.size() != M.bcell.size()) throw SizeError("one boundary coefficient is required per boundary face");
+  set_boundary_coefficients(*c->H, kb.data());
+  c->kb_user = std::move(kb);
+  c->kb_extend = false;
+  c->kb_pending = false;
+  c->kb_dirty = true;
+  c->bgrad_dirty = true;
+}
+
+// The finest level's boundary gradient factors 2 kb / h (0 on closed faces)
+// on the device.
+void upload_boundary_gradient(tpamr_ctx* c) {
+  if (!c->bgrad_dirty || c->device < 0 || c->topo.empty()) return;
+  const LevelData& L0 = *c->H->levels[0];
+  const TreeMesh& M = *L0.mesh;
+  const int BF = int(M.bcell.size()), nb = M.nb();
+  std::vector<double> g(std::max(BF, 1), 0.0);
+  if (L0.any_open())
+    for (int f = 0; f < BF; ++f)
+      if (L0.open[2 * M.baxis[f] + (M.bside[f] > 0 ? 1 : 0)])
+        g[f] = 2 * L0.kb[f] / M.bspacing[2 * (M.bcell[f] / nb) + M.baxis[f]];
+  c->bgrad.upload(g, c->stream);
+  c->bgrad_dirty = false;
 }
 
 // Host copies of the level values after device refreshes (for inspection).
@@ -1373,6 +1476,11 @@ int tpamr_set_mesh(tpamr_ctx* c, const int base_mesh[2], const int block_size[2]
     if (c->H) c->H_prev = std::move(c->H);
     c->have_coeff = false;
     c->mesh_pending = make_tree_mesh(base_mesh, block_size, origin, extent, per, max_level, keys);
+    // open sides stay; their boundary coefficients on the new mesh are
+    // extended from its first face coefficients unless given again
+    c->kb_user.clear();
+    c->kb_extend = true;
+    c->kb_pending = c->any_open();
   });
 }
 
@@ -1462,10 +1570,11 @@ int tpamr_set_face_coefficients(tpamr_ctx* c, const double* kappa) {
     bool topology = false;
     if (!c->H) {
       if (!c->mesh_pending) throw StateError("no mesh: call tpamr_set_mesh first");
-      c->H = build_hierarchy(std::move(c->mesh_pending), c->cfg, c->H_prev.get());
+      c->H = build_hierarchy(std::move(c->mesh_pending), c->cfg, c->H_prev.get(), c->open);
       c->H_prev.reset();
       topology = true;
     }
+    apply_boundary(c, kappa, topology);
     const auto t0 = std::chrono::steady_clock::now();
     const bool on_device = c->device >= 0 && !topology && !c->host_refresh;
     if (!on_device) {
@@ -1478,6 +1587,7 @@ int tpamr_set_face_coefficients(tpamr_ctx* c, const double* kappa) {
       device_refresh(c, kappa);
     else if (c->device >= 0)
       upload_hierarchy(c, topology);
+    upload_boundary_gradient(c);
     if (c->trace) {
       const auto t2 = std::chrono::steady_clock::now();
       std::fprintf(stderr, "tpamr refresh (%s): host values %.1f ms, device part %.1f ms\n",
@@ -1496,7 +1606,72 @@ int tpamr_set_face_coefficients_device(tpamr_ctx* c, const double* d_kappa) {
       throw StateError("no hierarchy: call tpamr_set_face_coefficients once before device refreshes");
     HIPCHECK(hipSetDevice(c->device));
     order_after_caller(c);
+    if (c->any_open() && c->kb_pending) {
+      std::vector<double> k;
+      if (c->kb_extend) {
+        k.resize(c->mesh().nfaces());
+        download_vec(c, k.data(), d_kappa, k.size());
+      }
+      apply_boundary(c, k.data(), false);
+    }
     device_refresh(c, d_kappa, true);
+    upload_boundary_gradient(c);
+  });
+}
+
+int tpamr_set_pressure_boundary(tpamr_ctx* c, const int open[4], const double* boundary_coefficient) {
+  if (!c || !open) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    if (!c->H && !c->mesh_pending) throw StateError("no mesh: call tpamr_set_mesh first");
+    const TreeMesh& M = c->mesh();
+    const int BF = int(M.bcell.size());
+    bool sides[4];
+    for (int q = 0; q < 4; ++q) sides[q] = open[q] != 0;
+    if (boundary_coefficient) {
+      for (int f = 0; f < BF; ++f)
+        if (!(std::isfinite(boundary_coefficient[f]) && boundary_coefficient[f] >= 0))
+          throw std::invalid_argument("boundary coefficients must be finite and nonnegative");
+      c->kb_user.assign(boundary_coefficient, boundary_coefficient + BF);
+      c->kb_extend = false;
+    } else {
+      c->kb_user.clear();
+      c->kb_extend = true;
+    }
+    c->kb_pending = true;
+    const bool changed = !std::equal(sides, sides + 4, c->open);
+    std::copy(sides, sides + 4, c->open);
+    if (changed && c->H) {
+      // the open sides shape the irregular rows: rebuild the hierarchy at the
+      // next coefficient update (geometric_preconditioner.jl:494-496)
+      if (c->device >= 0) HIPCHECK(hipSetDevice(c->device));
+      reset_topology_state(c);
+      c->mesh_pending = std::move(c->H->levels[0]->owned);
+      c->H.reset();
+      c->H_prev.reset();
+      c->have_coeff = false;
+    }
+  });
+}
+
+int tpamr_level_boundary(const tpamr_ctx* cc, int level, double* boundary_coefficient, int* flags) {
+  auto* c = const_cast<tpamr_ctx*>(cc);
+  if (!c) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_solver(c);
+    if (level < 0 || level >= int(c->H->levels.size())) throw std::invalid_argument("level out of range");
+    const LevelData& L = *c->H->levels[level];
+    if (boundary_coefficient) {
+      if (L.kb.size() == L.mesh->bcell.size())
+        std::copy(L.kb.begin(), L.kb.end(), boundary_coefficient);
+      else
+        std::fill(boundary_coefficient, boundary_coefficient + L.mesh->bcell.size(), 0.0);
+    }
+    if (flags) {
+      flags[0] = L.singular ? 1 : 0;
+      flags[1] = c->H->coarse_N > c->H->coarse_n ? 1 : 0;
+      flags[2] = int(L.bopen.size());
+      flags[3] = int(L.mesh->bcell.size());
+    }
   });
 }
 
@@ -1570,8 +1745,9 @@ int tpamr_level_matrix(const tpamr_ctx* cc, int level, int* nnz, int* rowptr, in
               j < by - 1 ? at(b, i, j + 1) : (L.nbr[4 * b + 3] >= 0 ? at(L.nbr[4 * b + 3], i, 0) : -1)};
           double d = (w[0] + w[1]) + (w[2] + w[3]);
           row.push_back({cidx, d});
+          // a slot without a neighbour is an open boundary face (diagonal only)
           for (int sd = 0; sd < 4; ++sd)
-            if (w[sd] != 0.0) row.push_back({nbc[sd], -w[sd]});
+            if (w[sd] != 0.0 && nbc[sd] >= 0) row.push_back({nbc[sd], -w[sd]});
         }
         std::sort(row.begin(), row.end());
         for (size_t e = 0; e < row.size(); ++e) {
@@ -1957,6 +2133,22 @@ int tpamr_weighted_gradient(tpamr_ctx* c, const double* p, double* g) {
   });
 }
 
+int tpamr_weighted_gradient_boundary(tpamr_ctx* c, const double* p, double* gb) {
+  if (!c || !p || !gb) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_solver(c);
+    const LevelData& L0 = *c->H->levels[0];
+    const TreeMesh& M = *L0.mesh;
+    const int BF = int(M.bcell.size()), nb = M.nb();
+    check_finite(p, M.ncells(), "pressure");
+    for (int f = 0; f < BF; ++f) {
+      const bool open = L0.open[2 * M.baxis[f] + (M.bside[f] > 0 ? 1 : 0)];
+      const double factor = open ? 2 * L0.kb[f] / M.bspacing[2 * (M.bcell[f] / nb) + M.baxis[f]] : 0.0;
+      gb[f] = (-double(M.bside[f]) * factor) * p[M.bcell[f]];
+    }
+  });
+}
+
 namespace {
 // copy into / out of the library's buffers: host memory (upload, download) or
 // the caller's device memory
@@ -1983,12 +2175,15 @@ void project_checks(double dt, double tol, double reference_dt) {
 // The projection on the velocity in c->f1: rhs from its divergence (and the
 // inherited divergence in c->tmp2 when `inherited`), the increment in c->x,
 // f1 -= dt G x.
-tpamr_stats project_core(tpamr_ctx* c, double dt, double tol, int max_cycles, bool inherited, double reference_dt) {
+tpamr_stats project_core(tpamr_ctx* c, double dt, double tol, int max_cycles, bool inherited, double reference_dt,
+                         bool boundary = false) {
   const TreeMesh& M = c->mesh();
   const int N = M.ncells(), F = M.nfaces(), nb = M.nb();
   hipStream_t s = c->stream;
+  // `boundary`: the boundary face velocities in c->fb enter the divergence
+  // and receive the open faces' correction
   launch_divergence(s, N, nb, c->cptr.p, c->cface.p, c->csign.p, c->farea.p, c->f1.p, c->bptr.p, c->bface.p,
-                    c->bside.p, c->barea.p, nullptr, c->topo[0].volume.p, c->tmp.p);
+                    c->bside.p, c->barea.p, boundary ? c->fb.p : nullptr, c->topo[0].volume.p, c->tmp.p);
   launch_projection_rhs(s, N, c->tmp.p, inherited ? c->tmp2.p : nullptr, dt, (reference_dt - dt) / (dt * reference_dt),
                         c->b.p);
   launch_fill(s, N, 0.0, c->x.p);
@@ -2002,7 +2197,22 @@ tpamr_stats project_core(tpamr_ctx* c, double dt, double tol, int max_cycles, bo
   launch_face_jump(s, F, c->fm.p, c->fp.p, c->farea.p, c->sqk.p, c->x.p, nullptr, c->f2.p);
   weighted_metric(c, c->f2.p, c->f3.p);
   launch_axpby(s, F, -dt, c->f3.p, 1.0, c->f1.p);
+  if (boundary)
+    launch_boundary_velocity(s, int(M.bcell.size()), c->bcell.p, c->bside.p, c->bgrad.p, c->x.p, dt * 1.0, c->fb.p);
   return st;
+}
+
+// _validate_projection_boundary (navier_stokes/projection.jl:37-49): the
+// interior-face projections serve operators without open sides,
+// tpamr_project_open those with them.
+void projection_sides(const tpamr_ctx* c, bool open_api) {
+  if (c->any_open() && !open_api)
+    throw std::invalid_argument(
+        "pressure operator and velocity boundary have different open sides; project with tpamr_project_open");
+  if (!c->any_open() && open_api)
+    throw std::invalid_argument(
+        "pressure operator and velocity boundary have different open sides; set them with "
+        "tpamr_set_pressure_boundary");
 }
 }  // namespace
 
@@ -2013,6 +2223,7 @@ int tpamr_project_ex(tpamr_ctx* c, double* u, double* p, double dt, double tol, 
     need_device(c);
     need_solver(c);
     project_checks(dt, tol, reference_dt);
+    projection_sides(c, false);
     HIPCHECK(hipSetDevice(c->device));
     const TreeMesh& M = c->mesh();
     const int N = M.ncells(), F = M.nfaces();
@@ -2032,6 +2243,32 @@ int tpamr_project_ex(tpamr_ctx* c, double* u, double* p, double dt, double tol, 
   });
 }
 
+int tpamr_project_open(tpamr_ctx* c, double* u, double* ub, double* p, double dt, double tol, int max_cycles,
+                       double* phi, tpamr_stats* stats) {
+  if (!c || !u || !ub || !p) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    need_solver(c);
+    project_checks(dt, tol, dt);
+    projection_sides(c, true);
+    HIPCHECK(hipSetDevice(c->device));
+    const TreeMesh& M = c->mesh();
+    const int N = M.ncells(), F = M.nfaces(), BF = int(M.bcell.size());
+    check_finite(u, F, "velocity");
+    check_finite(ub, BF, "boundary velocity");
+    upload_vec(c, c->f1.p, u, F);
+    if (BF > 0) upload_vec(c, c->fb.p, ub, BF);
+    tpamr_stats st = project_core(c, dt, tol, max_cycles, false, dt, true);
+    download_vec(c, u, c->f1.p, F);
+    if (BF > 0) download_vec(c, ub, c->fb.p, BF);
+    std::vector<double> ph(N);
+    download_vec(c, ph.data(), c->x.p, N);
+    for (int q = 0; q < N; ++q) p[q] += ph[q];
+    if (phi) std::copy(ph.begin(), ph.end(), phi);
+    if (stats) *stats = st;
+  });
+}
+
 int tpamr_project_device(tpamr_ctx* c, double* d_u, double* d_p, double dt, double tol, int max_cycles,
                          const double* d_inherited_divergence, double reference_dt, double* d_phi,
                          tpamr_stats* stats) {
@@ -2040,6 +2277,7 @@ int tpamr_project_device(tpamr_ctx* c, double* d_u, double* d_p, double dt, doub
     need_device(c);
     need_solver(c);
     project_checks(dt, tol, reference_dt);
+    projection_sides(c, false);
     HIPCHECK(hipSetDevice(c->device));
     order_after_caller(c);
     const TreeMesh& M = c->mesh();
@@ -2179,6 +2417,7 @@ tpamr_stats project_adjoint_core(tpamr_ctx* c, const double* ubar, const double*
                                  double rtol, int max_cycles, double* ubar_star, double* dJ_drhs, double* dJ_dkappa,
                                  bool host) {
   if (!(dt > 0 && std::isfinite(dt))) throw std::invalid_argument("dt must be finite and strictly positive");
+  projection_sides(c, false);
   const TreeMesh& M = c->mesh();
   const int N = M.ncells(), F = M.nfaces(), nb = M.nb();
   hipStream_t s = c->stream;

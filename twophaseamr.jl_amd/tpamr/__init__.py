@@ -6,16 +6,19 @@ from .hierarchy import HostHierarchy
 from .mesh import BlockKey, BlockTreeMesh
 from .physics import (CGFM, CSF, divergence, fill_halo, gravity, gravity_jumps, gravity_jumps_device, halo_schedule,
                       mixture_face_coefficients, mixture_face_coefficients_device, pressure_predictor, pressure_solve, pressure_solve_rrule, project,
-                      project_adjoint,
+                      project_adjoint, project_open, weighted_gradient_boundary,
                       refresh_two_phase, surface_tension, surface_tension_jumps, surface_tension_jumps_from_cells,
                       weighted_gradient,
                       weighted_jump_force)
 from .stepping import DeviceProjectionRun, ProjectionRun, face_dual_volumes, kinetic_energy, staggered_liquid_fraction
-from .solver import (EllipticSolver, LinearSolverOptions, LinearSolverStats, MultigridOptions, PoissonOperator, ldiv,
-                     solve)
+from .solver import (EllipticSolver, FreeSlip, LinearSolverOptions, LinearSolverStats, MultigridOptions, NoSlip,
+                     Outflow, PoissonOperator, VelocityBoundaryConditions, ZeroGradientVelocity, ldiv,
+                     pressure_boundary_sides, solve)
 
 __all__ = [
     "BlockKey", "BlockTreeMesh", "CGFM", "CSF", "Context", "DeviceProjectionRun", "DimensionMismatch", "EllipticSolver",
+    "FreeSlip", "NoSlip", "Outflow", "VelocityBoundaryConditions", "ZeroGradientVelocity", "pressure_boundary_sides",
+    "project_open", "weighted_gradient_boundary",
     "HostHierarchy",
     "LIB_PATH", "LinearSolverOptions", "LinearSolverStats", "MultigridOptions", "PoissonOperator", "ProjectionRun",
     "TPAMR_FP32_REFINE", "TPAMR_FP32_VCYCLE", "TPAMR_FP64", "divergence", "face_dual_volumes", "fill_halo",

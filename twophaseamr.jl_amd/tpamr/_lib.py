@@ -121,6 +121,11 @@ _SIGNATURES = {
                                                C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                                C.POINTER(Stats)]),
     "tpamr_time_smoother": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "tpamr_set_pressure_boundary": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "tpamr_weighted_gradient_boundary": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "tpamr_project_open": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                     C.c_int, C.c_void_p, C.c_void_p]),
+    "tpamr_level_boundary": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
 }
 
 

@@ -17,15 +17,38 @@ from .solver import MultigridOptions
 
 
 class HostHierarchy:
-    def __init__(self, mesh: BlockTreeMesh, face_coefficient=1.0, multigrid: MultigridOptions | None = None):
+    """`velocity_boundary` and `boundary_face_coefficient` as PoissonOperator's
+    (open pressure sides; the boundary coefficients of a vector
+    `face_coefficient` default to the library's extension)."""
+
+    def __init__(self, mesh: BlockTreeMesh, face_coefficient=1.0, multigrid: MultigridOptions | None = None,
+                 velocity_boundary=None, boundary_face_coefficient=None):
+        from .solver import pressure_boundary_sides
         self.mesh = mesh
         self.ctx = L.Context(-1)
         mesh.attach(self.ctx)
         mg = multigrid or MultigridOptions()
         L.check(L.lib().tpamr_set_multigrid(self.ctx.h, C.byref(mg._c())), self.ctx.h)
+        sides = pressure_boundary_sides(velocity_boundary)
+        if any(sides):
+            ob = np.asarray([int(v) for v in sides], np.int32)
+            kb = None if boundary_face_coefficient is None else np.ascontiguousarray(
+                np.broadcast_to(np.asarray(boundary_face_coefficient, np.float64), (mesh.nboundary_faces,)))
+            L.check(L.lib().tpamr_set_pressure_boundary(self.ctx.h, L.ptr(ob), None if kb is None else L.ptr(kb)),
+                    self.ctx.h)
         k = np.ascontiguousarray(np.broadcast_to(np.asarray(face_coefficient, np.float64), (mesh.nfaces,)))
         L.check(L.lib().tpamr_set_face_coefficients(self.ctx.h, L.ptr(k)), self.ctx.h)
         self.nlevels = L.lib().tpamr_num_levels(self.ctx.h)
+
+    def level_boundary(self, level):
+        """Boundary face coefficients of the level and its flags (singular,
+        coarse_augmented, open_faces)."""
+        fl = np.zeros(4, np.int32)
+        L.check(L.lib().tpamr_level_boundary(self.ctx.h, level, None, L.ptr(fl)), self.ctx.h)
+        kb = np.zeros(max(int(fl[3]), 1))
+        L.check(L.lib().tpamr_level_boundary(self.ctx.h, level, L.ptr(kb), L.ptr(fl)), self.ctx.h)
+        return dict(coefficient=kb[:int(fl[3])], singular=bool(fl[0]), coarse_augmented=bool(fl[1]),
+                    open_faces=int(fl[2]))
 
     def remesh(self, mesh: BlockTreeMesh, face_coefficient=1.0):
         """Attach a new mesh to the same context and build its hierarchy,

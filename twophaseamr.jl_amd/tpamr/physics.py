@@ -196,6 +196,34 @@ def weighted_gradient(solver: EllipticSolver, p):
     return out
 
 
+def weighted_gradient_boundary(solver: EllipticSolver, p):
+    """`_elliptic_boundary_gradient!` (elliptic.jl:1254-1263): per boundary
+    face -side (2 k_b / h) p_cell on open faces, 0 on closed ones."""
+    solver._sync()
+    m = solver.operator.mesh
+    pp = L.f64(p, m.ncells, "p")
+    out = np.zeros(max(m.nboundary_faces, 1))
+    L.check(L.lib().tpamr_weighted_gradient_boundary(solver.ctx.h, L.ptr(pp), L.ptr(out)), solver.ctx.h)
+    return out[:m.nboundary_faces]
+
+
+def project_open(u, ub, p, solver: EllipticSolver, dt, *, tolerance=float(np.sqrt(np.finfo(float).eps)),
+                 max_cycles=100):
+    """project! on a domain with open pressure sides: the boundary face
+    velocities `ub` enter the divergence and the open faces receive
+    -dt g_b; u, ub and p are updated in place; returns (stats, phi)."""
+    solver._sync()
+    m = solver.operator.mesh
+    for name, v, n in (("u", u, m.nfaces), ("ub", ub, m.nboundary_faces), ("p", p, m.ncells)):
+        if v.dtype != np.float64 or v.shape != (n,) or not v.flags.c_contiguous:
+            raise L.DimensionMismatch(f"{name} must be a contiguous float64 vector of {n} values")
+    phi = np.zeros(m.ncells)
+    st = L.Stats()
+    L.check(L.lib().tpamr_project_open(solver.ctx.h, L.ptr(u), L.ptr(ub), L.ptr(p), float(dt), float(tolerance),
+                                       int(max_cycles), L.ptr(phi), C.byref(st)), solver.ctx.h)
+    return LinearSolverStats._from(st, solver.precision), phi
+
+
 def pressure_predictor(u, p, solver: EllipticSolver, dt):
     """`_pressure_predictor!` (src/single_phase/stepping.jl:144-168): u -= dt
     G p with the operator's weighted gradient, in place; returns u."""

@@ -3,7 +3,10 @@
 Reference interfaces replaced here (file:line in TwoPhaseAMR.jl):
   MultigridOptions       src/solvers/multigrid/options.jl:1-52
   LinearSolverOptions    src/solvers/linear_options.jl:10-47
-  PoissonOperator        src/operators/elliptic.jl:969-980 (face_coefficient form)
+  PoissonOperator        src/operators/elliptic.jl:969-980 (face_coefficient form;
+                         velocity_boundary / boundary_face_coefficient, :683-800)
+  VelocityBoundaryConditions, NoSlip, FreeSlip, Outflow, ZeroGradientVelocity
+                         (the open pressure sides they imply, elliptic.jl:76-88)
   EllipticSolver         src/solvers/multigrid/elliptic_solver.jl:282-302
   solve!                 src/solvers/multigrid/elliptic_solver.jl:628-675
   ldiv!(z, P, r)         src/solvers/multigrid/cycles.jl:275-283
@@ -114,16 +117,80 @@ class LinearSolverStats:
                    krylov_runs=st.krylov_runs, replacements=st.replacements)
 
 
-class PoissonOperator:
-    """PoissonOperator(mesh; face_coefficient=1): -div(kappa grad p) in integrated
-    finite-volume form with the AMR face Riesz map (homogeneous Neumann walls or
-    periodic sides). `face_coefficient` is a number, a per-face vector in the
-    mesh's face order, or a function (x, y, axis) evaluated at face centres."""
+class NoSlip:
+    """Wall: homogeneous Neumann pressure side."""
 
-    def __init__(self, mesh: BlockTreeMesh, face_coefficient=1.0):
+
+class FreeSlip:
+    """Slip wall: homogeneous Neumann pressure side."""
+
+
+class Outflow:
+    """Open side: homogeneous Dirichlet pressure side."""
+
+
+class ZeroGradientVelocity:
+    """Open side: homogeneous Dirichlet pressure side."""
+
+
+class VelocityBoundaryConditions:
+    """VelocityBoundaryConditions(; left, right, bottom, top): the pressure
+    operator reads only which sides are open (Outflow, ZeroGradientVelocity)."""
+
+    def __init__(self, left=None, right=None, bottom=None, top=None):
+        self.left = NoSlip() if left is None else left
+        self.right = NoSlip() if right is None else right
+        self.bottom = NoSlip() if bottom is None else bottom
+        self.top = NoSlip() if top is None else top
+
+
+def pressure_boundary_sides(boundary):
+    """`_pressure_boundary_sides` (elliptic.jl:76-84): (left, right, bottom,
+    top) open flags of None, a wall condition, a 4-tuple of flags, or
+    VelocityBoundaryConditions."""
+    if boundary is None or isinstance(boundary, (NoSlip, FreeSlip)):
+        return (False, False, False, False)
+    if isinstance(boundary, VelocityBoundaryConditions):
+        return tuple(isinstance(c, (Outflow, ZeroGradientVelocity))
+                     for c in (boundary.left, boundary.right, boundary.bottom, boundary.top))
+    sides = tuple(bool(v) for v in boundary)
+    if len(sides) != 4:
+        raise ValueError("open sides are four flags: left, right, bottom, top")
+    return sides
+
+
+class PoissonOperator:
+    """PoissonOperator(mesh; face_coefficient=1, velocity_boundary=None,
+    boundary_face_coefficient=None): -div(kappa grad p) in integrated
+    finite-volume form with the AMR face Riesz map. Sides are homogeneous
+    Neumann (walls) or periodic, except the open sides of `velocity_boundary`
+    (homogeneous Dirichlet, boundary distance half the cell spacing).
+    `face_coefficient` is a number or a per-face vector in the mesh's face
+    order. `boundary_face_coefficient` (one per boundary face) defaults to a
+    scalar face coefficient on the open faces, or, for a vector, to the
+    area-weighted interior coefficients on the opposite side of each open
+    face's cell (computed by the library)."""
+
+    def __init__(self, mesh: BlockTreeMesh, face_coefficient=1.0, velocity_boundary=None,
+                 boundary_face_coefficient=None):
         self.mesh = mesh
         self.version = 0
+        self.boundary_sides = pressure_boundary_sides(velocity_boundary)
         self.face_coefficient = self._coefficients(face_coefficient)
+        if boundary_face_coefficient is None and np.isscalar(face_coefficient) and any(self.boundary_sides):
+            open_face = np.asarray([self.boundary_sides[2 * a + (1 if sd > 0 else 0)]
+                                    for a, sd in zip(mesh.bface_axis, mesh.bface_side)], bool)
+            boundary_face_coefficient = np.where(open_face, float(face_coefficient), 0.0)
+        self.boundary_face_coefficient = self._boundary(boundary_face_coefficient)
+
+    def _boundary(self, coefficient):
+        if coefficient is None:
+            return None
+        m = self.mesh
+        values = np.asarray(np.broadcast_to(np.asarray(coefficient, np.float64), (m.nboundary_faces,)), np.float64)
+        if not np.all(np.isfinite(values)) or np.any(values < 0):
+            raise ValueError("boundary coefficients must be finite and nonnegative")
+        return np.ascontiguousarray(values)
 
     def _coefficients(self, coefficient):
         m = self.mesh
@@ -141,11 +208,16 @@ class PoissonOperator:
             raise ValueError("face coefficients must be nonnegative")
         return np.ascontiguousarray(values)
 
-    def replace_face_coefficient(self, face_coefficient):
-        """`_replace_face_coefficient` (elliptic.jl:809-820). Solvers built on
-        this operator pick the new coefficients up at their next call; after
-        changing `face_coefficient` in place, call the solver's refresh()."""
+    def replace_face_coefficient(self, face_coefficient, boundary_face_coefficient=None):
+        """`_replace_face_coefficient` (elliptic.jl:809-820): new interior
+        coefficients and, when given, new boundary coefficients (kept
+        otherwise). Solvers built on this operator pick them up at their next
+        call; after changing `face_coefficient` in place, call the solver's
+        refresh()."""
         self.face_coefficient = self._coefficients(face_coefficient)
+        if boundary_face_coefficient is not None:
+            self.boundary_face_coefficient = self._boundary(boundary_face_coefficient)
+            self.boundary_version = getattr(self, "boundary_version", 0) + 1
         self.version += 1
         return self
 
@@ -194,10 +266,20 @@ class EllipticSolver:
         return self.operator.mesh.ncells
 
     def refresh(self):
-        """Upload the operator's current face coefficients (refreshes every level)."""
-        k = self.operator.face_coefficient
+        """Upload the operator's current face coefficients (refreshes every
+        level), with its open sides and boundary coefficients when they
+        changed."""
+        op = self.operator
+        bstate = (op.boundary_sides, getattr(op, "boundary_version", 0))
+        if bstate != getattr(self, "_boundary_state", ((False,) * 4, 0)):
+            sides = np.asarray([int(v) for v in op.boundary_sides], np.int32)
+            kb = op.boundary_face_coefficient
+            L.check(L.lib().tpamr_set_pressure_boundary(self.ctx.h, L.ptr(sides), None if kb is None else L.ptr(kb)),
+                    self.ctx.h)
+            self._boundary_state = bstate
+        k = op.face_coefficient
         L.check(L.lib().tpamr_set_face_coefficients(self.ctx.h, L.ptr(k)), self.ctx.h)
-        self._coeff_version = self.operator.version
+        self._coeff_version = op.version
 
     def _sync(self):
         # O(1) per call: coefficient replacements bump the operator's version
@@ -215,6 +297,17 @@ class EllipticSolver:
         return dict(ncells=int(ints[0]), nfaces=int(ints[1]), nblocks=int(ints[2]), irregular_rows=int(ints[3]),
                     irregular_nnz=int(ints[4]), max_leaf_level=int(ints[5]), omega=float(dbl[0]),
                     omega_p=float(dbl[1]))
+
+    def level_boundary(self, level):
+        """Level `level`'s boundary face coefficients (restricted from the
+        finest level's) and flags: singular, coarse_augmented (the coarsest
+        solve carries the volume constraint), open_faces."""
+        fl = np.zeros(4, np.int32)
+        L.check(L.lib().tpamr_level_boundary(self.ctx.h, level, None, L.ptr(fl)), self.ctx.h)
+        kb = np.zeros(max(int(fl[3]), 1))
+        L.check(L.lib().tpamr_level_boundary(self.ctx.h, level, L.ptr(kb), L.ptr(fl)), self.ctx.h)
+        return dict(coefficient=kb[:int(fl[3])], singular=bool(fl[0]), coarse_augmented=bool(fl[1]),
+                    open_faces=int(fl[2]))
 
     def apply(self, x):
         """Integrated operator A x (FactorizedEllipticOperator mul!)."""
