@@ -291,6 +291,7 @@ class EllipticSolver:
         return [self.level_info(l) for l in range(L.lib().tpamr_num_levels(self.ctx.h))]
 
     def level_info(self, level):
+        self._sync()
         ints = np.zeros(6, np.int32)
         dbl = np.zeros(2)
         L.check(L.lib().tpamr_level_info(self.ctx.h, level, L.ptr(ints), L.ptr(dbl)), self.ctx.h)
@@ -302,6 +303,7 @@ class EllipticSolver:
         """Level `level`'s boundary face coefficients (restricted from the
         finest level's) and flags: singular, coarse_augmented (the coarsest
         solve carries the volume constraint), open_faces."""
+        self._sync()
         fl = np.zeros(4, np.int32)
         L.check(L.lib().tpamr_level_boundary(self.ctx.h, level, None, L.ptr(fl)), self.ctx.h)
         kb = np.zeros(max(int(fl[3]), 1))
