@@ -368,7 +368,22 @@ class OSolver:
         _check(lib().orc_solve(self._h, _dptr(np.ascontiguousarray(rhs, np.float64)), _dptr(x),
                                C.c_double(rtol), C.c_double(atol), int(max_cycles),
                                int(compatibility == "error"), C.byref(st), _dptr(rm)))
-        return x, st.as_dict()
+        d = st.as_dict()
+        _, anchors = self.components(0)
+        d["removed_rhs_component_anchors"] = [int(a) + 1 for a in anchors]
+        d["removed_rhs_component_means"] = [float(v) for v in rm[:len(anchors)]]
+        return x, d
+
+    def components(self, l=0):
+        """Level l's component labels (0-based root cell per cell, -1 for an
+        inactive cell) and the roots of its ungauged components."""
+        n = self.level_info(l)["ncells"]
+        labels = np.zeros(n, np.int32)
+        cnt = C.c_int()
+        _check(lib().orc_solver_components(self._h, l, _iptr(labels), None, C.byref(cnt)))
+        anchors = np.zeros(max(cnt.value, 1), np.int32)
+        _check(lib().orc_solver_components(self._h, l, None, _iptr(anchors), C.byref(cnt)))
+        return labels, anchors[:cnt.value]
 
     def floor(self, x, b_integrated, assembled=False):
         """`_roundoff_residual_floor!` of iterate x with the integrated rhs:

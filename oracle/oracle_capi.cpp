@@ -180,6 +180,20 @@ int orc_solver_new_open(void* mp, const double* kappa, const double* volumes, co
   })
 }
 
+// level l's component labels (0-based root cell, -1 inactive) and the
+// ungauged components' roots (anchors, increasing; *count of them)
+int orc_solver_components(void* sp, int l, int* labels, int* anchors, int* count) {
+  GUARD({
+    Solver* s = static_cast<Solver*>(sp);
+    if (l < 0 || l >= int(s->levels.size())) throw std::invalid_argument("level out of range");
+    const Components& C = s->levels[l].op->comp;
+    if (labels) std::copy(C.labels.begin(), C.labels.end(), labels);
+    if (anchors) std::copy(C.anchors.begin(), C.anchors.end(), anchors);
+    *count = int(C.anchors.size());
+    return 0;
+  })
+}
+
 // level l's boundary face coefficients, boundary diagonal, and flags[2] =
 // (singular, coarse augmented)
 int orc_solver_level_boundary(void* sp, int l, double* bcoef, double* bdiag, int* flags) {
