@@ -99,10 +99,15 @@ int tpamr_mesh_boundary_faces(const tpamr_ctx* ctx, int* cell, int* axis, int* s
  * tpamr_set_face_coefficients. */
 int tpamr_set_multigrid(tpamr_ctx* ctx, const tpamr_multigrid_options* options);
 
-/* Face coefficients kappa_f (one per interior face, reference order):
- * builds the hierarchy on first use, otherwise refreshes every coefficient-
- * dependent value (replaces _refresh_geometric_multigrid!,
- * src/solvers/multigrid/geometric_preconditioner.jl:483-594). */
+/* Face coefficients kappa_f (one per interior face, reference order,
+ * nonnegative): builds the hierarchy on first use, otherwise refreshes every
+ * coefficient-dependent value (replaces _refresh_geometric_multigrid!,
+ * src/solvers/multigrid/geometric_preconditioner.jl:483-594). Zero
+ * coefficients split the domain into pressure components
+ * (_refresh_elliptic_components!, elliptic_solver.jl:39-109): every component
+ * without an open face is projected and gauged on its own, on every
+ * multigrid level (the V-cycle then runs level by level); such a refresh
+ * runs on the host. */
 int tpamr_set_face_coefficients(tpamr_ctx* ctx, const double* kappa);
 
 /* Open (homogeneous Dirichlet) pressure sides: PoissonOperator's
@@ -134,8 +139,19 @@ int tpamr_weighted_gradient_boundary(tpamr_ctx* ctx, const double* p, double* gb
 int tpamr_project_open(tpamr_ctx* ctx, double* u, double* ub, double* p, double dt, double tolerance, int max_cycles,
                        double* phi, tpamr_stats* stats);
 
+/* The last solve's removed component means and their anchors
+ * (LinearSolverStats.removed_rhs_component_means / _anchors, stats.jl):
+ * *count components, roots 1-based smallest cells (NULL arrays: count
+ * only). A connected singular operator reports one component rooted at 1;
+ * an anchored one none. */
+int tpamr_component_means(const tpamr_ctx* ctx, int* count, int* roots, double* means);
+
 /* hierarchy inspection */
 int tpamr_num_levels(const tpamr_ctx* ctx);
+/* Level `level`'s ungauged components: *count, per cell its 1-based slot (0:
+ * gauged; NULL: skipped) and per slot its root (1-based smallest cell; NULL:
+ * skipped). */
+int tpamr_level_components(const tpamr_ctx* ctx, int level, int* count, int* slot, int* roots);
 /* Level `level`'s boundary face coefficients (restricted from the finest
  * level's, tree_coarsening.jl:347-398,538-555; NULL: skipped) and flags[4] =
  * (level singular, coarsest level solved with the volume constraint, open
@@ -273,7 +289,8 @@ int tpamr_project_ex(tpamr_ctx* ctx, double* u, double* p, double dt, double tol
 
 /* Pressure adjoint (no reference counterpart): for J = w' p with p the
  * zero-mean solution of the current operator, dJ/drhs (strong rhs) and
- * dJ/dkappa per interior face; `p` is the stored forward solution. */
+ * dJ/dkappa per interior face (boundary coefficients held fixed); `p` is the
+ * stored forward solution. The adjoints need one connected component. */
 int tpamr_adjoint(tpamr_ctx* ctx, const double* w, const double* p, double rtol, int max_cycles, double* dJ_drhs,
                   double* dJ_dkappa, tpamr_stats* stats);
 

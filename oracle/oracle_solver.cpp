@@ -1513,9 +1513,12 @@ Stats project(Solver& S, double* u, double* ub, double* p, double dt, double tol
   } else {
     for (int c = 0; c < N; ++c) div[c] = -div[c] / dt;
   }
-  Stats st = solve(S, div.data(), phi.data(), 0.0, tol / (dt * dt), max_cycles, 0, nullptr);
+  std::vector<double> means;
+  Stats st = solve(S, div.data(), phi.data(), 0.0, tol / (dt * dt), max_cycles, 0, &means);
   st.residual_scale = 1.0 / (dt * dt);
-  const double mean = std::fabs(st.removed_rhs_mean);
+  // `_projection_compatibility!`: the largest removed component mean
+  double mean = std::fabs(st.removed_rhs_mean);
+  for (double m : means) mean = std::max(mean, std::fabs(m));
   st.compatible = (mean == 0 ? 0.0 : dt * dt * mean) <= tol;
   weighted_gradient(op, phi.data(), g.data());
   const double scale = dt * 1.0;
@@ -1526,6 +1529,22 @@ Stats project(Solver& S, double* u, double* ub, double* p, double dt, double tol
     for (size_t f = 0; f < gb.size(); ++f) ub[f] -= scale * gb[f];
   }
   for (int c = 0; c < N; ++c) p[c] += phi[c];
+  // `_accumulate_pressure_increment!` (projection.jl:52-97): on a singular
+  // operator with several components every ungauged component's volume mean
+  // of the accumulated pressure is removed (sums in cell order)
+  const Components& C = op.comp;
+  int ncomponents = 0;
+  for (int c = 0; c < N; ++c) ncomponents += C.labels[c] == c ? 1 : 0;
+  if (op.singular && ncomponents > 1) {
+    std::vector<double> sums(C.anchors.size(), 0.0);
+    for (int c = 0; c < N; ++c)
+      if (C.slots[c]) sums[C.slots[c] - 1] += op.volumes[c] * p[c];
+    for (size_t q = 0; q < sums.size(); ++q) sums[q] /= C.volumes[q];
+    for (int c = 0; c < N; ++c) {
+      if (C.labels[c] < 0) p[c] = 0.0;
+      else if (C.slots[c]) p[c] = p[c] - sums[C.slots[c] - 1];
+    }
+  }
   if (phi_out) std::copy(phi.begin(), phi.end(), phi_out);
   return st;
 }

@@ -50,7 +50,7 @@ def test_invalid_inputs_raise():
         tpamr.BlockTreeMesh(block_size=(5, 5))
     m = tpamr.BlockTreeMesh(block_size=(8, 8))
     with pytest.raises(ValueError):
-        tpamr.HostHierarchy(m, np.zeros(m.nfaces))  # disconnected components are rejected
+        tpamr.HostHierarchy(m, -np.ones(m.nfaces))  # negative coefficients are rejected
     with pytest.raises(ValueError):
         tpamr.MultigridOptions(coarse_size=0)
     with pytest.raises(ValueError):

@@ -799,6 +799,8 @@ void level_topology(LevelData& L) {
   }, 256);
   L.volume.resize(L.nblocks);
   for (int b = 0; b < L.nblocks; ++b) L.volume[b] = M.bspacing[2 * b] * M.bspacing[2 * b + 1];
+  L.total_volume = 0;
+  for (int b = 0; b < L.nblocks; ++b) L.total_volume += L.volume[b] * nb;
   // irregular cells: touching a non-regular face
   L.cell_irr.assign(L.ncells, -1);
   std::vector<uint8_t> irr(L.ncells, 0);
@@ -1408,6 +1410,109 @@ void boundary_values(LevelData& L) {
   L.singular = !gauged;
 }
 
+// The level's pressure components (see LevelData): with `positive` every
+// face coefficient of the level is positive, so the level is one connected
+// component, gauged exactly when an open face conducts; otherwise the
+// components are labelled over the faces with a nonzero coefficient (root =
+// smallest cell, as the reference's depth-first labelling from increasing
+// roots gives) and the chunked-sum plan is built.
+void level_components(LevelData& L, bool positive) {
+  const TreeMesh& M = *L.mesh;
+  const int N = L.ncells, nb = M.nb();
+  std::vector<uint8_t> anchored;
+  bool any_anchor = false;
+  for (size_t k = 0; k < L.bopen.size() && k < L.bw.size(); ++k)
+    if (L.bw[k] != 0.0) any_anchor = true;
+  L.pair_chunk.clear();
+  L.pair_slot.clear();
+  L.slot_pair_ptr.clear();
+  L.slot_pairs.clear();
+  if (positive) {
+    L.connected = true;
+    L.ncomp = any_anchor ? 0 : 1;
+    L.comp_slot.clear();
+    L.comp_root.assign(L.ncomp, 0);
+    L.comp_vol.assign(L.ncomp, L.total_volume);
+    L.singular = L.ncomp > 0;
+    return;
+  }
+  // union-find, the smaller root wins
+  std::vector<int> parent(N);
+  for (int c = 0; c < N; ++c) parent[c] = c;
+  auto find = [&](int c) {
+    while (parent[c] != c) {
+      parent[c] = parent[parent[c]];
+      c = parent[c];
+    }
+    return c;
+  };
+  for (int f = 0; f < M.nfaces(); ++f) {
+    if (L.kappa[f] == 0.0) continue;
+    int a = find(M.fminus[f]), b = find(M.fplus[f]);
+    if (a == b) continue;
+    if (a > b) std::swap(a, b);
+    parent[b] = a;
+  }
+  std::vector<int> root(N);
+  for (int c = 0; c < N; ++c) root[c] = find(c);
+  anchored.assign(N, 0);
+  for (size_t k = 0; k < L.bopen.size() && k < L.bw.size(); ++k)
+    if (L.bw[k] != 0.0) anchored[root[M.bcell[L.bopen[k]]]] = 1;
+  int ncomponents = 0;
+  std::vector<int> slot_of_root(N, 0);
+  L.comp_root.clear();
+  for (int c = 0; c < N; ++c)
+    if (root[c] == c) {
+      ++ncomponents;
+      if (!anchored[c]) {
+        L.comp_root.push_back(c);
+        slot_of_root[c] = int(L.comp_root.size());
+      }
+    }
+  L.ncomp = int(L.comp_root.size());
+  L.connected = ncomponents == 1;
+  L.singular = L.ncomp > 0;
+  L.comp_slot.assign(N, 0);
+  L.comp_vol.assign(L.ncomp, 0.0);
+  for (int c = 0; c < N; ++c) {
+    const int q = slot_of_root[root[c]];
+    L.comp_slot[c] = q;
+    if (q) L.comp_vol[q - 1] += L.volume[c / nb];
+  }
+  if (L.ncomp == 0) return;
+  // chunked-sum plan: the slots present in every chunk, increasing
+  const int nchunks = (N + GAUGE_CHUNK - 1) / GAUGE_CHUNK;
+  std::vector<int> seen(L.ncomp + 1, -1), present;
+  for (int ch = 0; ch < nchunks; ++ch) {
+    present.clear();
+    for (int c = ch * GAUGE_CHUNK; c < std::min(N, (ch + 1) * GAUGE_CHUNK); ++c) {
+      const int q = L.comp_slot[c];
+      if (q && seen[q] != ch) {
+        seen[q] = ch;
+        present.push_back(q);
+      }
+    }
+    std::sort(present.begin(), present.end());
+    for (int q : present) {
+      L.pair_chunk.push_back(ch);
+      L.pair_slot.push_back(q);
+    }
+  }
+  L.slot_pair_ptr.assign(L.ncomp + 1, 0);
+  for (int q : L.pair_slot) L.slot_pair_ptr[q]++;
+  for (int q = 0; q < L.ncomp; ++q) L.slot_pair_ptr[q + 1] += L.slot_pair_ptr[q];
+  L.slot_pairs.assign(L.pair_slot.size(), 0);
+  std::vector<int> at(L.slot_pair_ptr.begin(), L.slot_pair_ptr.end() - 1);
+  for (size_t p = 0; p < L.pair_slot.size(); ++p) L.slot_pairs[at[L.pair_slot[p] - 1]++] = int(p);
+}
+
+// The hierarchy's gauge flags from its levels' components.
+void finish_gauge(Hierarchy& H) {
+  H.singular = H.levels[0]->singular;
+  H.general_gauge = false;
+  for (const auto& L : H.levels) H.general_gauge = H.general_gauge || !L->simple_gauge();
+}
+
 void transfer_descriptors(LevelData& Fl, const LevelData& C, bool agglomerated) {
   const TreeMesh& fm = *Fl.mesh;
   const TreeMesh& cm = *C.mesh;
@@ -1435,15 +1540,16 @@ void transfer_descriptors(LevelData& Fl, const LevelData& C, bool agglomerated) 
   }
 }
 
-// Dense coarsest operator into H.coarse_aug, row-major N x N: the augmented
-// [A V; V^T 0] (N = n + 1) of a singular level (one connected, ungauged
-// component), A itself (N = n) of a level that open sides anchor
-// (geometric_preconditioner.jl:144-169, 415-430).
+// Dense coarsest operator into H.coarse_aug, row-major N x N: A augmented
+// with one volume constraint per ungauged component, [A C; C^T 0]
+// (N = n + ncomp; C's column of a component holds its cells' volumes), or A
+// itself when every component is gauged (geometric_preconditioner.jl:144-169,
+// 415-430).
 void coarse_matrix(Hierarchy& H) {
   LevelData& L = *H.levels.back();
   const TreeMesh& M = *L.mesh;
   const int n = L.ncells, nb = M.nb(), bx = M.bs[0], by = M.bs[1];
-  const int N = L.singular ? n + 1 : n;
+  const int N = n + L.ncomp;
   // The coarsest level is solved with a dense inverse. Tree coarsening ends
   // at coarse_size cells when the root layout agglomerates down to one
   // block; an odd root layout can stop far above that.
@@ -1481,10 +1587,11 @@ void coarse_matrix(Hierarchy& H) {
           if (nbc[s] >= 0) a[size_t(c) * N + nbc[s]] -= w[s];
         }
       }
-      if (N > n) {
+      const int slot = L.comp_slot.empty() ? L.ncomp : L.comp_slot[c];
+      if (slot > 0) {
         const double V = L.volume[b];
-        a[size_t(c) * N + n] = V;
-        a[size_t(n) * N + c] = V;
+        a[size_t(c) * N + n + slot - 1] = V;
+        a[size_t(n + slot - 1) * N + c] = V;
       }
     }
 }
@@ -1592,11 +1699,6 @@ void set_boundary_coefficients(Hierarchy& H, const double* kb) {
     }
     boundary_values(L);
   }
-  // one connected component: every level is anchored exactly when the
-  // finest one is (restriction weights are positive)
-  H.singular = L0.singular;
-  for (const auto& L : H.levels)
-    if (L->singular != H.singular) throw std::logic_error("levels disagree on the pressure gauge");
 }
 
 std::vector<double> boundary_material(const TreeMesh& m, const bool open[4], const double* kappa) {
@@ -1641,10 +1743,11 @@ void refresh_coarsest(Hierarchy& H, const double* kappa_coarsest) {
 void refresh_coefficients(Hierarchy& H, const double* kappa) {
   LevelData& L0 = *H.levels[0];
   const int F = L0.mesh->nfaces();
+  bool positive = true;
   for (int f = 0; f < F; ++f) {
     if (!std::isfinite(kappa[f])) throw std::invalid_argument("face coefficients must be finite");
-    if (!(kappa[f] > 0))
-      throw std::invalid_argument("face coefficients must be positive (disconnected components are not supported)");
+    if (!(kappa[f] >= 0)) throw std::invalid_argument("face coefficients must be nonnegative");
+    positive = positive && kappa[f] > 0;
   }
   if (L0.any_open() && L0.kb.size() != L0.mesh->bcell.size())
     throw std::logic_error("open pressure sides need boundary coefficients before the coefficient refresh");
@@ -1663,8 +1766,15 @@ void refresh_coefficients(Hierarchy& H, const double* kappa) {
       });
     }
     level_values(L);
+    level_components(L, positive);
   }
+  finish_gauge(H);
   coarse_matrix(H);
+}
+
+void connected_components(Hierarchy& H) {
+  for (auto& L : H.levels) level_components(*L, true);
+  finish_gauge(H);
 }
 
 // Halo schedule of `_build_halo_schedule` (src/mesh/topology/mesh.jl:944-1037):

@@ -166,9 +166,31 @@ struct LevelData {
   std::vector<int> bopen, bdst, bpos, irr_bptr, irr_bface;
   std::vector<int64_t> bslot;
   std::vector<double> bw, irr_bd;
+  // Pressure components of the level (`_refresh_elliptic_components!`,
+  // elliptic_solver.jl:39-109): connected over faces with a nonzero
+  // coefficient; a component with an open face of positive conductance is
+  // gauged, the others (ncomp, slots 1..ncomp in increasing root order) are
+  // projected and gauged each on its own. connected: one component holding
+  // every cell. comp_slot per cell (0: gauged), comp_root (0-based smallest
+  // cell) and comp_vol (cell volumes summed in cell order) per slot. The
+  // component sums of a level with several components run in chunks of
+  // GAUGE_CHUNK cells (`_chunked_slot_sums!`, threading.jl:202-283): one
+  // partial per (chunk, slot) pair present, chunk-major (pair_chunk,
+  // pair_slot), and per slot its pairs in chunk order (slot_pair_ptr /
+  // slot_pairs). total_volume: the level's cell volumes (per block).
+  int ncomp = 1;
+  bool connected = true;
+  double total_volume = 0;
+  std::vector<int> comp_slot, comp_root, pair_chunk, pair_slot, slot_pair_ptr, slot_pairs;
+  std::vector<double> comp_vol;
   bool singular = true;
   bool any_open() const { return open[0] || open[1] || open[2] || open[3]; }
+  // one ungauged component holding every cell, or none: no per-level projection
+  bool simple_gauge() const { return ncomp == 0 || (ncomp == 1 && connected); }
 };
+
+// Cells per chunk of the component sums.
+constexpr int GAUGE_CHUNK = 1024;
 
 struct Hierarchy {
   std::vector<std::unique_ptr<LevelData>> levels;
@@ -177,10 +199,12 @@ struct Hierarchy {
   // anchor them; row major, coarse_N rows; the device inverts it
   int coarse_n = 0, coarse_N = 0;
   std::vector<double> coarse_aug;
-  // every level singular (wall / periodic sides, or open sides whose
-  // boundary coefficients are all zero): the solve projects the rhs and
-  // removes volume means; otherwise no mean is removed
-  bool singular = true;
+  // the finest level singular (an ungauged component): the solve projects
+  // the rhs and removes volume means; otherwise no mean is removed.
+  // general_gauge: some level has several components or gauged cells beside
+  // an ungauged component, so the V-cycle projects and gauges every such
+  // level's components (cycles.jl:8-77)
+  bool singular = true, general_gauge = false;
   double total_volume = 0;
   // levels taken over from the previous hierarchy (_tree_reuse_suffix!):
   // this hierarchy's levels from reused_new on are the previous one's from
@@ -238,8 +262,13 @@ std::vector<double> boundary_material(const TreeMesh& m, const bool open[4], con
 // _same_tree_topology (tree_coarsening.jl:84-121): domain, block size and
 // leaf keys (faces and geometry follow from them)
 bool same_tree_topology(const TreeMesh& a, const TreeMesh& b);
-// Refresh all coefficient-dependent values for new fine face coefficients.
+// Refresh all coefficient-dependent values for new fine face coefficients
+// (nonnegative; zeros split the domain into components).
 void refresh_coefficients(Hierarchy& H, const double* kappa);
+// The components of every level for positive face coefficients on every
+// level (one connected component; gauged by open faces), and the
+// hierarchy's gauge flags: the state after a device refresh.
+void connected_components(Hierarchy& H);
 // Values of the coarsest level alone from its own face coefficients, and its
 // dense augmented matrix (the device refreshes the other levels).
 void refresh_coarsest(Hierarchy& H, const double* kappa_coarsest);

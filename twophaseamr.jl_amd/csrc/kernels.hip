@@ -1212,11 +1212,76 @@ __global__ void k_ref_dinv(int n, int bx, int by, const double* __restrict__ cw,
 }
 
 // any face coefficient not finite or not positive
+// face coefficients: bad[0] = 1 for a negative or non-finite one, bad[1] =
+// 1 for a zero one (a split into components)
 __global__ void k_ref_check(int nf, const double* __restrict__ k, int* __restrict__ bad) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
   const double v = k[f];
-  if (!(v > 0) || !isfinite(v)) *bad = 1;
+  if (!isfinite(v) || v < 0) bad[0] = 1;
+  else if (v == 0) bad[1] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// Pressure components: sums over each ungauged component in chunks of
+// GAUGE_CHUNK_CELLS cells (`_chunked_slot_sums!`, threading.jl:202-283), then the
+// projection of a right-hand side or the removal of volume means
+// (hierarchy.jl:300-343, elliptic_solver.jl:130-190).
+
+// one wavefront per (chunk, slot) pair: the sum of term(c) over the chunk's
+// cells of that slot; mode 0: x, 1: V x, 2: |x|. Lanes take every 64th cell,
+// then a fixed shuffle tree (deterministic)
+template <typename T>
+__global__ void __launch_bounds__(256) k_comp_partial(int npairs, int n, int nb_cells, const int* __restrict__ pair_chunk,
+                                                      const int* __restrict__ pair_slot, const int* __restrict__ slot,
+                                                      const T* __restrict__ x, const double* __restrict__ volume,
+                                                      int mode, double* __restrict__ partial) {
+  const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  if (w >= npairs) return;  // whole wavefronts
+  const int c0 = pair_chunk[w] * GAUGE_CHUNK_CELLS, q = pair_slot[w];
+  double s = 0;
+  for (int i = lane; i < GAUGE_CHUNK_CELLS; i += 64) {
+    const int c = c0 + i;
+    if (c < n && slot[c] == q) {
+      const double v = double(x[c]);
+      s += mode == 0 ? v : (mode == 1 ? volume[c / nb_cells] * v : fabs(v));
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) partial[w] = s;
+}
+
+// per slot: its pairs' partials in chunk order; divided by the component's
+// volume with `divide` (a mean)
+__global__ void k_comp_total(int ncomp, const int* __restrict__ slot_pair_ptr, const int* __restrict__ slot_pairs,
+                             const double* __restrict__ partial, const double* __restrict__ vol, int divide,
+                             double* __restrict__ out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ncomp) return;
+  double s = 0;
+  for (int p = slot_pair_ptr[q]; p < slot_pair_ptr[q + 1]; ++p) s += partial[slot_pairs[p]];
+  out[q] = divide ? s / vol[q] : s;
+}
+
+// mode 0: x -= mean (remove volume means); 1: x -= V mean (project a rhs)
+template <typename T>
+__global__ void k_comp_shift(int n, int nb_cells, const int* __restrict__ slot, const double* __restrict__ mean,
+                             const double* __restrict__ volume, int mode, T* __restrict__ x) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int q = slot[c];
+  if (!q) return;
+  const double m = mean[q - 1];
+  x[c] = T(double(x[c]) - (mode ? volume[c / nb_cells] * m : m));
+}
+
+// the first sweep from zero: x = (w D^-1) b
+template <typename T>
+__global__ void k_zero_sweep(int n, double omega, const T* __restrict__ dinv, const T* __restrict__ b,
+                             T* __restrict__ x) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  x[c] = (T(omega) * dinv[c]) * b[c];
 }
 
 __global__ void k_sqrt(int n, const double* __restrict__ in, double* __restrict__ out) {
@@ -1616,6 +1681,38 @@ void launch_ref_dinv(hipStream_t s, int n, int bx, int by, const double* cw, con
   hipLaunchKernelGGL(k_ref_dinv, dim3(cdiv(n, 256)), dim3(256), 0, s, n, bx, by, cw, ce, cs, cn, cell_irr, irr_diag,
                      dinv, regular, nbr, ratio);
 }
+template <typename T>
+void launch_comp_sums(hipStream_t s, const GaugeView& G, int n, int nb_cells, const T* x, const double* volume,
+                      int mode, int divide, double* out) {
+  if (G.ncomp == 0) return;
+  if (G.npairs > 0)
+    hipLaunchKernelGGL(k_comp_partial<T>, dim3(cdiv(G.npairs, 4)), dim3(256), 0, s, G.npairs, n, nb_cells,
+                       G.pair_chunk, G.pair_slot, G.slot, x, volume, mode, G.partial);
+  hipLaunchKernelGGL(k_comp_total, dim3(cdiv(G.ncomp, 256)), dim3(256), 0, s, G.ncomp, G.slot_pair_ptr,
+                     G.slot_pairs, G.partial, G.vol, divide, out);
+}
+template <typename T>
+void launch_comp_shift(hipStream_t s, const GaugeView& G, int n, int nb_cells, const double* mean,
+                       const double* volume, int mode, T* x) {
+  if (G.ncomp == 0) return;
+  hipLaunchKernelGGL(k_comp_shift<T>, dim3(cdiv(n, 256)), dim3(256), 0, s, n, nb_cells, G.slot, mean, volume, mode,
+                     x);
+}
+template <typename T>
+void launch_zero_sweep(hipStream_t s, int n, double omega, const T* dinv, const T* b, T* x) {
+  hipLaunchKernelGGL(k_zero_sweep<T>, dim3(cdiv(n, 256)), dim3(256), 0, s, n, omega, dinv, b, x);
+}
+template void launch_comp_sums<double>(hipStream_t, const GaugeView&, int, int, const double*, const double*, int,
+                                       int, double*);
+template void launch_comp_sums<float>(hipStream_t, const GaugeView&, int, int, const float*, const double*, int,
+                                      int, double*);
+template void launch_comp_shift<double>(hipStream_t, const GaugeView&, int, int, const double*, const double*, int,
+                                        double*);
+template void launch_comp_shift<float>(hipStream_t, const GaugeView&, int, int, const double*, const double*, int,
+                                       float*);
+template void launch_zero_sweep<double>(hipStream_t, int, double, const double*, const double*, double*);
+template void launch_zero_sweep<float>(hipStream_t, int, double, const float*, const float*, float*);
+
 void launch_ref_check(hipStream_t s, int nf, const double* k, int* bad) {
   if (nf > 0) hipLaunchKernelGGL(k_ref_check, dim3(cdiv(nf, 256)), dim3(256), 0, s, nf, k, bad);
 }

@@ -54,6 +54,24 @@ struct CgState {
 // Workgroups of the Krylov scalar-step reductions (stage buffer: 4 * RED_WG).
 constexpr int RED_WG = 128;
 
+// Cells per chunk of the component sums (as the host's plan).
+constexpr int GAUGE_CHUNK_CELLS = 1024;
+
+// A level's ungauged pressure components on the device (device pointers):
+// slot per cell (0: gauged), component volumes, the chunked-sum plan of
+// (chunk, slot) pairs and per slot its pairs in chunk order, and a partial
+// per pair.
+struct GaugeView {
+  int ncomp = 0, npairs = 0;
+  const int* slot = nullptr;
+  const double* vol = nullptr;
+  const int* pair_chunk = nullptr;
+  const int* pair_slot = nullptr;
+  const int* slot_pair_ptr = nullptr;
+  const int* slot_pairs = nullptr;
+  double* partial = nullptr;
+};
+
 // Krylov partial sums emitted by the V-cycle's last finest-level sweep:
 // sums of V z, r z and r per block, with r the FP64 Krylov residual (r64) or
 // the FP32 inner residual of the refinement loop (r32); part = nullptr: none.
@@ -104,7 +122,20 @@ void launch_ref_rows(hipStream_t s, int nrows, const int* irr_ptr, const int* te
 void launch_ref_dinv(hipStream_t s, int n, int bx, int by, const double* cw, const double* ce, const double* cs,
                      const double* cn, const int* cell_irr, const double* irr_diag, double* dinv, int* regular,
                      const int* nbr, unsigned long long* ratio);
+// bad[0]: a negative or non-finite coefficient; bad[1]: a zero one
 void launch_ref_check(hipStream_t s, int nf, const double* k, int* bad);
+// component sums of x (mode 0: x, 1: V x, 2: |x|) into out[ncomp], divided
+// by the component volumes with `divide`
+template <typename T>
+void launch_comp_sums(hipStream_t s, const GaugeView& G, int n, int nb_cells, const T* x, const double* volume,
+                      int mode, int divide, double* out);
+// x -= mean[slot] (mode 0) or x -= V mean[slot] (mode 1) on ungauged cells
+template <typename T>
+void launch_comp_shift(hipStream_t s, const GaugeView& G, int n, int nb_cells, const double* mean,
+                       const double* volume, int mode, T* x);
+// x = (w D^-1) b
+template <typename T>
+void launch_zero_sweep(hipStream_t s, int n, double omega, const T* dinv, const T* b, T* x);
 void launch_sqrt(hipStream_t s, int n, const double* in, double* out);
 // inverse of the dense N x N row-major matrix `a` (overwritten; Gauss-Jordan
 // with partial pivoting, piv and col N-entry scratch); its top-left n x n

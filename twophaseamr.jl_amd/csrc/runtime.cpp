@@ -113,6 +113,18 @@ struct DevPlan {
   DevArray<double> bw, irr_bd;
 };
 
+// A level's ungauged pressure components (uploaded after a host refresh of
+// a hierarchy with several components): the arrays behind its GaugeView and
+// the component means of the last projection or gauge.
+struct DevGauge {
+  int ncomp = 0;
+  DevArray<int> slot, pair_chunk, pair_slot, slot_pair_ptr, slot_pairs;
+  DevArray<double> vol, partial, mean;
+  GaugeView view;
+};
+
+static_assert(GAUGE_CHUNK == GAUGE_CHUNK_CELLS, "host and device component chunks differ");
+
 template <typename T>
 struct DevLevel {
   DevArray<T> cw, ce, cs, cn, ell_val, ell_bound, ell_diag, dinv;
@@ -151,6 +163,11 @@ struct tpamr_ctx {
   // 2 kb / h (0 on closed faces), and the boundary velocities of a projection
   DevArray<int> bcell;
   DevArray<double> bgrad, fb;
+  // components of every level (Hierarchy::general_gauge) and the last
+  // solve's removed component means with their roots (1-based)
+  std::vector<DevGauge> gauge;
+  std::vector<int> last_roots;
+  std::vector<double> last_means;
   // device data
   std::vector<DevTopo> topo;
   std::vector<DevLevel<double>> ld;  // FP64 levels (level 0 always, all in FP64 mode)
@@ -597,6 +614,126 @@ T* enqueue_vcycle(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done, c
   return cur[0];
 }
 
+// The V-cycle of a hierarchy with several components, literally as the
+// reference runs it (cycles.jl:8-77): on every level with ungauged
+// components the rhs is projected on entry and the component means are
+// removed after every sweep and after the coarse correction; levels are
+// launched one by one (no tail kernel) and the first sweep from zero runs
+// after the projection. The Krylov statistics follow the last removal.
+// Level l's component sums and shifts (a hierarchy with several
+// components): remove each ungauged component's volume mean from x
+// (`_multigrid_remove_component_means!`, hierarchy.jl:309-324), or project a
+// rhs b (`_multigrid_project_component_rhs!`, :326-343).
+template <typename T>
+void gauge_remove(tpamr_ctx* c, int l, T* x) {
+  DevGauge& G = c->gauge[l];
+  if (!G.ncomp) return;
+  const int N = c->topo[l].ncells, nb = c->H->levels[l]->mesh->nb();
+  launch_comp_sums<T>(c->stream, G.view, N, nb, x, c->topo[l].volume.p, 1, 1, G.mean.p);
+  launch_comp_shift<T>(c->stream, G.view, N, nb, G.mean.p, c->topo[l].volume.p, 0, x);
+}
+template <typename T>
+void gauge_project(tpamr_ctx* c, int l, T* b) {
+  DevGauge& G = c->gauge[l];
+  if (!G.ncomp) return;
+  const int N = c->topo[l].ncells, nb = c->H->levels[l]->mesh->nb();
+  launch_comp_sums<T>(c->stream, G.view, N, nb, b, c->topo[l].volume.p, 0, 1, G.mean.p);
+  launch_comp_shift<T>(c->stream, G.view, N, nb, G.mean.p, c->topo[l].volume.p, 1, b);
+}
+
+// The components of every level on the device (after a host refresh of a
+// hierarchy with several components).
+void upload_gauge(tpamr_ctx* c) {
+  Hierarchy& H = *c->H;
+  c->gauge.clear();
+  c->gauge.resize(H.levels.size());
+  if (!H.general_gauge) return;
+  hipStream_t s = c->stream;
+  for (size_t l = 0; l < H.levels.size(); ++l) {
+    const LevelData& L = *H.levels[l];
+    DevGauge& G = c->gauge[l];
+    G.ncomp = L.ncomp;
+    if (!G.ncomp) continue;
+    if (L.comp_slot.size() != size_t(L.ncells) || L.slot_pair_ptr.size() != size_t(L.ncomp) + 1)
+      throw std::logic_error("components without their reduction plan");
+    G.slot.upload(L.comp_slot, s);
+    G.vol.upload(L.comp_vol, s);
+    G.pair_chunk.upload(L.pair_chunk, s);
+    G.pair_slot.upload(L.pair_slot, s);
+    G.slot_pair_ptr.upload(L.slot_pair_ptr, s);
+    G.slot_pairs.upload(L.slot_pairs, s);
+    G.partial.alloc(L.pair_slot.size());
+    G.mean.alloc(L.ncomp);
+    G.view.ncomp = L.ncomp;
+    G.view.npairs = int(L.pair_slot.size());
+    G.view.slot = G.slot.p;
+    G.view.vol = G.vol.p;
+    G.view.pair_chunk = G.pair_chunk.p;
+    G.view.pair_slot = G.pair_slot.p;
+    G.view.slot_pair_ptr = G.slot_pair_ptr.p;
+    G.view.slot_pairs = G.slot_pairs.p;
+    G.view.partial = G.partial.p;
+  }
+}
+
+template <typename T>
+void launch_fill_t(hipStream_t s, int n, T* x) {
+  if constexpr (std::is_same<T, float>::value)
+    launch_fill_f32(s, n, 0.f, x);
+  else
+    launch_fill(s, n, 0.0, x);
+}
+
+template <typename T>
+T* enqueue_vcycle_general(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done, const ZStats& zs) {
+  hipStream_t s = c->stream;
+  const int nl = int(lv.size());
+  const int pre = c->cfg.pre, post = c->cfg.post;
+  std::vector<T*> cur(nl), oth(nl);
+  for (int l = 0; l < nl; ++l) {
+    cur[l] = lv[l].x.p;
+    oth[l] = lv[l].xs.p;
+  }
+  for (int l = 0; l + 1 < nl; ++l) {
+    DevLevel<T>& L = lv[l];
+    const int N = L.view.ncells;
+    gauge_project<T>(c, l, L.b.p);
+    if (pre > 0) {
+      launch_zero_sweep<T>(s, N, L.omega, L.dinv.p, L.b.p, cur[l]);
+      gauge_remove<T>(c, l, cur[l]);
+    } else {
+      launch_fill_t<T>(s, N, cur[l]);
+    }
+    for (int q = 1; q < pre; ++q) {
+      launch_sweep<T>(s, L.view, cur[l], oth[l], L.b.p, L.omega, done, l == 0, ZStats{});
+      std::swap(cur[l], oth[l]);
+      gauge_remove<T>(c, l, cur[l]);
+    }
+    launch_residual<T>(s, L.view, cur[l], L.b.p, L.u.p, L.omega_p, done, l == 0);
+    DevLevel<T>& C = lv[l + 1];
+    launch_restrict<T>(s, L.view, L.u.p, L.omega_p, C.b.p, C.x.p, C.dinv.p, C.omega, 0, done, l == 0);
+  }
+  gauge_project<T>(c, nl - 1, lv[nl - 1].b.p);
+  launch_coarse<T>(s, c->H->coarse_n, c->coarse_inv.p, lv[nl - 1].b.p, lv[nl - 1].x.p, done);
+  for (int l = nl - 2; l >= 0; --l) {
+    DevLevel<T>& L = lv[l];
+    launch_prolong<T>(s, L.view, cur[l], cur[l + 1], L.omega_p, done, l == 0);
+    gauge_remove<T>(c, l, cur[l]);
+    for (int q = 0; q < post; ++q) {
+      launch_sweep<T>(s, L.view, cur[l], oth[l], L.b.p, L.omega, done, l == 0, ZStats{});
+      std::swap(cur[l], oth[l]);
+      gauge_remove<T>(c, l, cur[l]);
+    }
+  }
+  if (zs.part) launch_zstats<T>(s, lv[0].view.nblocks, lv[0].view.bx, cur[0], zs, lv[0].view.volume, done);
+  return cur[0];
+}
+
+template <typename T>
+T* enqueue_vcycle_any(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done, const ZStats& zs) {
+  return c->H->general_gauge ? enqueue_vcycle_general<T>(c, lv, done, zs) : enqueue_vcycle<T>(c, lv, done, zs);
+}
+
 // ---------------------------------------------------------------------------
 // FP64 helpers on the finest level
 
@@ -642,13 +779,20 @@ double roundoff_floor(tpamr_ctx* c) {
 // residual's compatibility drift): the total volume of a singular operator
 // (one connected, ungauged component), infinity for one that open sides
 // anchor, whose means are then 0 / inf = 0 and nothing is removed.
+// A hierarchy with several components gauges them in the V-cycle itself,
+// so the Krylov kernels remove nothing either.
 double gauge_volume(const tpamr_ctx* c) {
-  return c->H->singular ? c->H->total_volume : std::numeric_limits<double>::infinity();
+  return (c->H->singular && !c->H->general_gauge) ? c->H->total_volume : std::numeric_limits<double>::infinity();
 }
 
 // subtract the volume-weighted mean (one connected, unanchored component;
-// nothing for an operator that open sides anchor)
+// each ungauged component's own with several; nothing for an operator that
+// open sides anchor)
 void remove_mean(tpamr_ctx* c, double* v) {
+  if (c->H->general_gauge) {
+    gauge_remove<double>(c, 0, v);
+    return;
+  }
   if (!c->H->singular) return;
   const int N = c->topo[0].ncells, nb = c->H->levels[0]->mesh->nb();
   launch_weighted_sums(c->stream, N, nb, v, c->topo[0].volume.p, c->rs.part);
@@ -668,7 +812,7 @@ T* enqueue_precondition(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* d
   ZStats zs;
   zs.r64 = c->r.p;
   zs.part = c->rs.part;
-  return enqueue_vcycle<T>(c, lv, done, zs);
+  return enqueue_vcycle_any<T>(c, lv, done, zs);
 }
 
 template <typename T>
@@ -808,7 +952,7 @@ void enqueue_ir_iteration(tpamr_ctx* c, std::vector<DevLevel<float>>& lv, bool i
   ZStats zs;
   zs.r32 = L0.b.p;
   zs.part = c->rs.part;
-  float* z = enqueue_vcycle<float>(c, lv, done, zs);
+  float* z = enqueue_vcycle_any<float>(c, lv, done, zs);
   launch_cg_beta(s, nblk, ir_groups(N), c->rs.part, c->rs.maxpart, c->rs.stage, gauge_volume(c), c->rs.st,
                  init ? 1 : 0);
   launch_ir_update_p(s, N, z, c->p32.p, c->rs.st, done);
@@ -850,11 +994,48 @@ tpamr_stats solve_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double rtol, 
   // the compatible projection (_project_component_rhs!,
   // elliptic_solver.jl:150-190) of a singular operator; an anchored one
   // takes the rhs as it is
-  const bool singular = c->H->singular;
-  if (singular && compat == 1 && std::fabs(total_sum) > 256 * EPS * std::max(1.0, scale))
+  const bool singular = c->H->singular, general = singular && c->H->general_gauge;
+  if (singular && !general && compat == 1 && std::fabs(total_sum) > 256 * EPS * std::max(1.0, scale))
     throw IncompatibleError("Poisson right-hand side has nonzero volume mean on component rooted at cell 1");
-  const double mean = singular ? total_sum / c->H->total_volume : 0.0;
+  const double mean = (singular && !general) ? total_sum / c->H->total_volume : 0.0;
   st.removed_rhs_mean = mean;
+  c->last_roots.clear();
+  c->last_means.clear();
+  if (singular && !general) {
+    c->last_roots.push_back(1);
+    c->last_means.push_back(mean);
+  }
+  if (general) {
+    // each ungauged component's own projection (_project_component_rhs!,
+    // elliptic_solver.jl:150-190)
+    const LevelData& L0 = *c->H->levels[0];
+    DevGauge& G = c->gauge[0];
+    const int K = G.ncomp;
+    std::vector<double> sums(K), scales(K);
+    launch_comp_sums<double>(s, G.view, N, nb, c->b.p, vol, 0, 0, G.mean.p);
+    HIPCHECK(hipMemcpyAsync(sums.data(), G.mean.p, sizeof(double) * K, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    launch_comp_sums<double>(s, G.view, N, nb, c->b.p, vol, 2, 0, G.mean.p);
+    HIPCHECK(hipMemcpyAsync(scales.data(), G.mean.p, sizeof(double) * K, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    double tsum = 0, tvol = 0;
+    for (int q = 0; q < K; ++q) {
+      if (compat == 1 && std::fabs(sums[q]) > 256 * EPS * std::max(1.0, scales[q]))
+        throw IncompatibleError("Poisson right-hand side has nonzero volume mean on component rooted at cell " +
+                                std::to_string(L0.comp_root[q] + 1));
+      tsum += sums[q];
+      tvol += L0.comp_vol[q];
+    }
+    for (int q = 0; q < K; ++q) {
+      sums[q] /= L0.comp_vol[q];
+      c->last_roots.push_back(L0.comp_root[q] + 1);
+      c->last_means.push_back(sums[q]);
+    }
+    HIPCHECK(hipMemcpyAsync(G.mean.p, sums.data(), sizeof(double) * K, hipMemcpyHostToDevice, s));
+    launch_comp_shift<double>(s, G.view, N, nb, G.mean.p, vol, 1, c->b.p);
+    HIPCHECK(hipStreamSynchronize(s));
+    st.removed_rhs_mean = tvol == 0 ? 0.0 : tsum / tvol;
+  }
   double il2, ilinf, floor0;
   if (zero_start) {
     // b -= V mean; the residual of x = 0 is b and its round-off floor bound
@@ -870,7 +1051,7 @@ tpamr_stats solve_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double rtol, 
     floor0 = 128 * EPS * std::max(1.0, ilinf);
   } else {
     // b -= V * mean (scalar staged through device memory)
-    if (singular) {
+    if (singular && !general) {
       double h = mean;
       HIPCHECK(hipMemcpyAsync(c->rs.scal + 8, &h, sizeof(double), hipMemcpyHostToDevice, s));
       launch_shift(s, N, nb, c->b.p, c->rs.scal + 8, 1.0, vol, true);
@@ -970,7 +1151,7 @@ void vcycle_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv) {
   launch_fill(s, N, 0.0, c->Ap.p);
   launch_cg_update_xr<T>(s, N, nb, nullptr, c->Ap.p, nullptr, c->r.p, c->topo[0].volume.p, L0.b.p, L0.x.p,
                          L0.dinv.p, w0, c->rs.maxpart, c->rs.st, 1);
-  T* z = enqueue_vcycle<T>(c, lv, &c->rs.st->done, ZStats{});
+  T* z = enqueue_vcycle_any<T>(c, lv, &c->rs.st->done, ZStats{});
   if constexpr (std::is_same<T, float>::value)
     launch_cast_f2d(s, N, z, c->tmp.p);
   else
@@ -1205,7 +1386,7 @@ void upload_plan(tpamr_ctx* c) {
     up(P.bpos, L.bpos, 0);
   }
   c->ref_ratio.alloc(nl);
-  c->ref_flags.alloc(nl + 1);
+  c->ref_flags.alloc(nl + 2);
   c->plan_ready = true;
   c->kb_dirty = true;
 }
@@ -1232,8 +1413,10 @@ void upload_plan_boundary(tpamr_ctx* c) {
 // value pass of every level, FP32 copies, the finest level's sqrt(k), the
 // Jacobi weights (one read-back) and the dense coarse inverse. The host keeps
 // the coarsest level's values (for its dense matrix); the other levels' host
-// values are refreshed only when asked for (host_stale).
-void device_refresh(tpamr_ctx* c, const double* kappa, bool kappa_on_device = false) {
+// values are refreshed only when asked for (host_stale). Returns false, with
+// nothing changed, when a face coefficient is zero: the components of such
+// an operator are labelled by the host refresh.
+bool device_refresh(tpamr_ctx* c, const double* kappa, bool kappa_on_device = false) {
   Hierarchy& H = *c->H;
   hipStream_t s = c->stream;
   const int nl = int(H.levels.size());
@@ -1247,14 +1430,13 @@ void device_refresh(tpamr_ctx* c, const double* kappa, bool kappa_on_device = fa
   else
     upload_vec(c, P0.kappa.p, kappa, P0.nfaces);
   HIPCHECK(hipMemsetAsync(c->ref_ratio.p, 0, sizeof(unsigned long long) * nl, s));
-  HIPCHECK(hipMemsetAsync(c->ref_flags.p, 0, sizeof(int) * (nl + 1), s));
+  HIPCHECK(hipMemsetAsync(c->ref_flags.p, 0, sizeof(int) * (nl + 2), s));
   launch_ref_check(s, P0.nfaces, P0.kappa.p, c->ref_flags.p + nl);
-  int bad = 0;
-  HIPCHECK(hipMemcpyAsync(&bad, c->ref_flags.p + nl, sizeof(int), hipMemcpyDeviceToHost, s));
+  int bad[2] = {0, 0};
+  HIPCHECK(hipMemcpyAsync(bad, c->ref_flags.p + nl, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHECK(hipStreamSynchronize(s));
-  if (bad)
-    throw std::invalid_argument(
-        "face coefficients must be finite and positive (disconnected components are not supported)");
+  if (bad[0]) throw std::invalid_argument("face coefficients must be finite and nonnegative");
+  if (bad[1]) return false;
   destroy_graph(c);
   for (int l = 0; l < nl; ++l) {
     const LevelData& L = *H.levels[l];
@@ -1325,9 +1507,13 @@ void device_refresh(tpamr_ctx* c, const double* kappa, bool kappa_on_device = fa
       c->lf[l].omega_p = wp;
     }
   }
+  // positive coefficients: one connected component per level
+  connected_components(H);
+  upload_gauge(c);
   refresh_coarsest(H, kc.data());
   coarse_inverse_device(c);
   c->host_stale = true;
+  return true;
 }
 
 // Pending boundary coefficients of open sides into the hierarchy (before its
@@ -1576,17 +1762,19 @@ int tpamr_set_face_coefficients(tpamr_ctx* c, const double* kappa) {
     }
     apply_boundary(c, kappa, topology);
     const auto t0 = std::chrono::steady_clock::now();
-    const bool on_device = c->device >= 0 && !topology && !c->host_refresh;
+    bool on_device = c->device >= 0 && !topology && !c->host_refresh;
+    // zero coefficients (components) take the host refresh
+    if (on_device && !device_refresh(c, kappa)) on_device = false;
     if (!on_device) {
       refresh_coefficients(*c->H, kappa);
       c->host_stale = false;
     }
     c->have_coeff = true;
     const auto t1 = std::chrono::steady_clock::now();
-    if (on_device)
-      device_refresh(c, kappa);
-    else if (c->device >= 0)
+    if (!on_device && c->device >= 0) {
       upload_hierarchy(c, topology);
+      upload_gauge(c);
+    }
     upload_boundary_gradient(c);
     if (c->trace) {
       const auto t2 = std::chrono::steady_clock::now();
@@ -1614,7 +1802,9 @@ int tpamr_set_face_coefficients_device(tpamr_ctx* c, const double* d_kappa) {
       }
       apply_boundary(c, k.data(), false);
     }
-    device_refresh(c, d_kappa, true);
+    if (!device_refresh(c, d_kappa, true))
+      throw std::invalid_argument(
+          "zero face coefficients split the domain into components: refresh with tpamr_set_face_coefficients");
     upload_boundary_gradient(c);
   });
 }
@@ -1672,6 +1862,33 @@ int tpamr_level_boundary(const tpamr_ctx* cc, int level, double* boundary_coeffi
       flags[2] = int(L.bopen.size());
       flags[3] = int(L.mesh->bcell.size());
     }
+  });
+}
+
+int tpamr_component_means(const tpamr_ctx* c, int* count, int* roots, double* means) {
+  if (!c || !count) return TPAMR_EINVAL;
+  *count = int(c->last_means.size());
+  if (roots) std::copy(c->last_roots.begin(), c->last_roots.end(), roots);
+  if (means) std::copy(c->last_means.begin(), c->last_means.end(), means);
+  return TPAMR_OK;
+}
+
+int tpamr_level_components(const tpamr_ctx* cc, int level, int* count, int* slot, int* roots) {
+  auto* c = const_cast<tpamr_ctx*>(cc);
+  if (!c || !count) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_solver(c);
+    if (level < 0 || level >= int(c->H->levels.size())) throw std::invalid_argument("level out of range");
+    const LevelData& L = *c->H->levels[level];
+    *count = L.ncomp;
+    if (slot) {
+      if (L.comp_slot.empty())
+        std::fill(slot, slot + L.ncells, L.ncomp);
+      else
+        std::copy(L.comp_slot.begin(), L.comp_slot.end(), slot);
+    }
+    if (roots)
+      for (int q = 0; q < L.ncomp; ++q) roots[q] = L.comp_root[q] + 1;
   });
 }
 
@@ -2189,9 +2406,10 @@ tpamr_stats project_core(tpamr_ctx* c, double dt, double tol, int max_cycles, bo
   launch_fill(s, N, 0.0, c->x.p);
   tpamr_stats st = solve_dispatch(c, 0.0, tol / (dt * dt), max_cycles, 0);
   st.residual_scale = 1.0 / (dt * dt);
-  // _projection_compatibility! (projection.jl:100-127): the removed strong
-  // rhs mean, times dt^2, against the tolerance
-  const double mean = std::fabs(st.removed_rhs_mean);
+  // _projection_compatibility! (projection.jl:100-127): the largest removed
+  // strong component mean, times dt^2, against the tolerance
+  double mean = std::fabs(st.removed_rhs_mean);
+  for (double m : c->last_means) mean = std::max(mean, std::fabs(m));
   st.compatible = (mean == 0 ? 0.0 : dt * dt * mean) <= tol;
   // u -= dt G phi
   launch_face_jump(s, F, c->fm.p, c->fp.p, c->farea.p, c->sqk.p, c->x.p, nullptr, c->f2.p);
@@ -2200,6 +2418,21 @@ tpamr_stats project_core(tpamr_ctx* c, double dt, double tol, int max_cycles, bo
   if (boundary)
     launch_boundary_velocity(s, int(M.bcell.size()), c->bcell.p, c->bside.p, c->bgrad.p, c->x.p, dt * 1.0, c->fb.p);
   return st;
+}
+
+// _accumulate_pressure_increment! (navier_stokes/projection.jl:52-97) after
+// p += phi: with several components each ungauged one's volume mean of the
+// accumulated pressure is removed (cell order), the gauged ones keep theirs.
+void gauge_pressure_host(const tpamr_ctx* c, double* p) {
+  const LevelData& L0 = *c->H->levels[0];
+  if (!L0.singular || L0.connected) return;
+  const int N = L0.ncells, nb = L0.mesh->nb();
+  std::vector<double> sums(L0.ncomp, 0.0);
+  for (int q = 0; q < N; ++q)
+    if (L0.comp_slot[q]) sums[L0.comp_slot[q] - 1] += L0.volume[q / nb] * p[q];
+  for (int k = 0; k < L0.ncomp; ++k) sums[k] /= L0.comp_vol[k];
+  for (int q = 0; q < N; ++q)
+    if (L0.comp_slot[q]) p[q] = p[q] - sums[L0.comp_slot[q] - 1];
 }
 
 // _validate_projection_boundary (navier_stokes/projection.jl:37-49): the
@@ -2238,6 +2471,7 @@ int tpamr_project_ex(tpamr_ctx* c, double* u, double* p, double dt, double tol, 
     std::vector<double> ph(N);
     download_vec(c, ph.data(), c->x.p, N);
     for (int q = 0; q < N; ++q) p[q] += ph[q];
+    gauge_pressure_host(c, p);
     if (phi) std::copy(ph.begin(), ph.end(), phi);
     if (stats) *stats = st;
   });
@@ -2264,6 +2498,7 @@ int tpamr_project_open(tpamr_ctx* c, double* u, double* ub, double* p, double dt
     std::vector<double> ph(N);
     download_vec(c, ph.data(), c->x.p, N);
     for (int q = 0; q < N; ++q) p[q] += ph[q];
+    gauge_pressure_host(c, p);
     if (phi) std::copy(ph.begin(), ph.end(), phi);
     if (stats) *stats = st;
   });
@@ -2287,7 +2522,13 @@ int tpamr_project_device(tpamr_ctx* c, double* d_u, double* d_p, double dt, doub
     if (d_inherited_divergence) copy_in(c, c->tmp2.p, d_inherited_divergence, N, false);
     tpamr_stats st = project_core(c, dt, tol, max_cycles, d_inherited_divergence != nullptr, reference_dt);
     copy_out(c, d_u, c->f1.p, F, false);
-    if (d_p) launch_axpby(s, N, 1.0, c->x.p, 1.0, d_p);
+    if (d_p) {
+      launch_axpby(s, N, 1.0, c->x.p, 1.0, d_p);
+      // _accumulate_pressure_increment!: each ungauged component's own gauge
+      // when the domain has several components
+      const LevelData& L0 = *c->H->levels[0];
+      if (L0.singular && !L0.connected) gauge_remove<double>(c, 0, d_p);
+    }
     if (d_phi) copy_out(c, d_phi, c->x.p, N, false);
     HIPCHECK(hipStreamSynchronize(s));
     if (stats) *stats = st;
@@ -2386,6 +2627,7 @@ int tpamr_adjoint(tpamr_ctx* c, const double* w, const double* p, double rtol, i
     need_device(c);
     need_solver(c);
     HIPCHECK(hipSetDevice(c->device));
+    if (c->H->general_gauge) throw std::invalid_argument("the adjoint needs one connected pressure component");
     const TreeMesh& M = c->mesh();
     const int N = M.ncells(), F = M.nfaces(), nb = M.nb();
     hipStream_t s = c->stream;
@@ -2418,6 +2660,8 @@ tpamr_stats project_adjoint_core(tpamr_ctx* c, const double* ubar, const double*
                                  bool host) {
   if (!(dt > 0 && std::isfinite(dt))) throw std::invalid_argument("dt must be finite and strictly positive");
   projection_sides(c, false);
+  if (c->H->general_gauge)
+    throw std::invalid_argument("the projection adjoint needs one connected pressure component");
   const TreeMesh& M = c->mesh();
   const int N = M.ncells(), F = M.nfaces(), nb = M.nb();
   hipStream_t s = c->stream;

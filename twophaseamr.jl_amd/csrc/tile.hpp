@@ -421,7 +421,10 @@ __device__ inline void sweep_op(unsigned char* lds, const LevelView<T>& L, const
   if (P.valid) {
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      z.v[m] = vc.v[m] + T(omega) * (bv.v[m] - ax.v[m]) / d.v[m];
+      // a cell without conducting faces (zero diagonal, its own component)
+      // keeps its value, as with the reference's zero inverse diagonal
+      const T step = T(omega) * (bv.v[m] - ax.v[m]) / d.v[m];
+      z.v[m] = d.v[m] > T(0) ? vc.v[m] + step : vc.v[m];
       if constexpr (STATS) {
         sz += double(z.v[m]);
         srz += rd.v[m] * double(z.v[m]);
@@ -444,7 +447,9 @@ __device__ inline void sweep_op(unsigned char* lds, const LevelView<T>& L, const
   }
 }
 
-// u = w_P D^-1 (b - A x)
+// u = w_P D^-1 (b - A x); a cell without conducting faces (zero diagonal)
+// keeps its residual b - A x in u instead, which its neighbours' products do
+// not see (their couplings to it are zero)
 template <typename T, int BX, int BY>
 __device__ inline void residual_op(unsigned char* lds, const LevelView<T>& L, const Place<BX, BY>& P,
                                    const T* __restrict__ x, const T* __restrict__ rhs, T* __restrict__ u,
@@ -454,12 +459,16 @@ __device__ inline void residual_op(unsigned char* lds, const LevelView<T>& L, co
   if (!P.valid) return;
   Vec4<T> o;
 #pragma unroll
-  for (int m = 0; m < 4; ++m) o.v[m] = T(omega_p) * (bv.v[m] - ax.v[m]) / d.v[m];
+  for (int m = 0; m < 4; ++m) {
+    const T r = bv.v[m] - ax.v[m];
+    o.v[m] = d.v[m] > T(0) ? T(omega_p) * r / d.v[m] : r;
+  }
   st4(u + P.cell0, o);
 }
 
-// Smoothed restriction b_c = P_t' (D u / w_P - S u), and the zero-start
-// coarse sweep x_c = w_c D_c^-1 b_c.
+// Smoothed restriction b_c = P_t' (r - S u) with r = D u / w_P (the residual
+// itself for a zero diagonal, whose smoothed transfer row is the tentative
+// one), and the zero-start coarse sweep x_c = w_c D_c^-1 b_c.
 template <typename T, int BX, int BY>
 __device__ inline void restrict_op(unsigned char* lds, const LevelView<T>& L, const Place<BX, BY>& P,
                                    const T* __restrict__ u, double omega_p, T* __restrict__ bc, T* __restrict__ xc,
@@ -469,7 +478,8 @@ __device__ inline void restrict_op(unsigned char* lds, const LevelView<T>& L, co
   tile4<T, BX, BY, false>(lds, L, P, Arr4<T>{u}, Pre0{}, vc, ax, d);
   Vec4<T> w;
 #pragma unroll
-  for (int m = 0; m < 4; ++m) w.v[m] = P.valid ? d.v[m] * vc.v[m] / T(omega_p) - ax.v[m] : T(0);
+  for (int m = 0; m < 4; ++m)
+    w.v[m] = P.valid ? (d.v[m] > T(0) ? d.v[m] * vc.v[m] / T(omega_p) - ax.v[m] : vc.v[m]) : T(0);
   const int cb = P.valid ? L.xfer[2 * P.b] : 0, mode = P.valid ? L.xfer[2 * P.b + 1] : -1;
   // pair sums across the row segment, then across row pairs (same wavefront)
   const T h0 = w.v[0] + w.v[1], h1 = w.v[2] + w.v[3];
@@ -514,7 +524,7 @@ __device__ inline void prolong_op(unsigned char* lds, const LevelView<T>& L, con
   if (!P.valid) return;
   Vec4<T> o;
 #pragma unroll
-  for (int m = 0; m < 4; ++m) o.v[m] = xv.v[m] + (vc.v[m] - T(omega_p) * ax.v[m] / d.v[m]);
+  for (int m = 0; m < 4; ++m) o.v[m] = xv.v[m] + (d.v[m] > T(0) ? vc.v[m] - T(omega_p) * ax.v[m] / d.v[m] : vc.v[m]);
   st4(x + P.cell0, o);
 }
 

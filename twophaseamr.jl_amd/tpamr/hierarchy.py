@@ -40,6 +40,12 @@ class HostHierarchy:
         L.check(L.lib().tpamr_set_face_coefficients(self.ctx.h, L.ptr(k)), self.ctx.h)
         self.nlevels = L.lib().tpamr_num_levels(self.ctx.h)
 
+    def level_components(self, level):
+        """Per cell its 1-based ungauged-component slot (0: gauged) and per slot
+        its root (1-based smallest cell)."""
+        from .solver import _level_components
+        return _level_components(self.ctx, level, self.level_info(level)["ncells"])
+
     def level_boundary(self, level):
         """Boundary face coefficients of the level and its flags (singular,
         coarse_augmented, open_faces)."""
@@ -124,5 +130,6 @@ class HostHierarchy:
         owner = self.tentative_owner(level)
         nc = self.level_info(level + 1)["ncells"]
         Pt = sp.csr_matrix((np.ones(len(owner)), (np.arange(len(owner)), owner)), shape=(len(owner), nc))
-        dinv = 1.0 / A.diagonal()
+        d = A.diagonal()
+        dinv = np.divide(1.0, d, out=np.zeros_like(d), where=d > 0)  # zero diagonal: tentative row
         return (Pt - info["omega_p"] * sp.diags(dinv) @ A @ Pt).tocsr()

@@ -221,7 +221,7 @@ def project_open(u, ub, p, solver: EllipticSolver, dt, *, tolerance=float(np.sqr
     st = L.Stats()
     L.check(L.lib().tpamr_project_open(solver.ctx.h, L.ptr(u), L.ptr(ub), L.ptr(p), float(dt), float(tolerance),
                                        int(max_cycles), L.ptr(phi), C.byref(st)), solver.ctx.h)
-    return LinearSolverStats._from(st, solver.precision), phi
+    return LinearSolverStats._from(st, solver.precision, solver.ctx.h), phi
 
 
 def pressure_predictor(u, p, solver: EllipticSolver, dt):
@@ -254,7 +254,7 @@ def project(u, p, solver: EllipticSolver, dt, *, tolerance=float(np.sqrt(np.finf
     L.check(L.lib().tpamr_project_ex(solver.ctx.h, L.ptr(u), L.ptr(p), float(dt), float(tolerance), int(max_cycles),
                                      L.ptr(inh), float(dt if reference_dt is None else reference_dt), L.ptr(phi),
                                      C.byref(st)), solver.ctx.h)
-    return LinearSolverStats._from(st, solver.precision), phi
+    return LinearSolverStats._from(st, solver.precision, solver.ctx.h), phi
 
 
 def pressure_solve(solver: EllipticSolver, rhs, *, rtol=1e-12, max_cycles=200):

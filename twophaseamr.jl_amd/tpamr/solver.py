@@ -105,12 +105,21 @@ class LinearSolverStats:
     replacements: int = 0
 
     @classmethod
-    def _from(cls, st: L.Stats, precision):
+    def _from(cls, st: L.Stats, precision, handle=None):
+        means, anchors = [st.removed_rhs_mean], [1]
+        if handle is not None:
+            n = C.c_int()
+            L.check(L.lib().tpamr_component_means(handle, C.byref(n), None, None), handle)
+            roots = np.zeros(max(n.value, 1), np.int32)
+            vals = np.zeros(max(n.value, 1))
+            L.check(L.lib().tpamr_component_means(handle, C.byref(n), L.ptr(roots), L.ptr(vals)), handle)
+            means = [float(v) for v in vals[:n.value]]
+            anchors = [int(r) for r in roots[:n.value]]
         return cls(cycles=st.cycles, converged=bool(st.converged), compatible=bool(st.compatible),
                    initial_l2=st.initial_l2, initial_linf=st.initial_linf, final_l2=st.final_l2,
                    final_linf=st.final_linf, target_linf=st.target_linf, residual_scale=st.residual_scale,
                    removed_rhs_mean=st.removed_rhs_mean,
-                   removed_rhs_component_means=[st.removed_rhs_mean], removed_rhs_component_anchors=[1],
+                   removed_rhs_component_means=means, removed_rhs_component_anchors=anchors,
                    preconditioner="tree_geometric_multigrid" + ("" if precision == L.TPAMR_FP64 else "_fp32"),
                    method="cg" if precision != L.TPAMR_FP32_REFINE else "refinement(cg_fp32)",
                    reason=REASONS.get(st.reason, str(st.reason)), solve_ms=st.solve_ms,
@@ -222,6 +231,15 @@ class PoissonOperator:
         return self
 
 
+def _level_components(ctx, level, ncells):
+    n = C.c_int()
+    L.check(L.lib().tpamr_level_components(ctx.h, level, C.byref(n), None, None), ctx.h)
+    slot = np.zeros(ncells, np.int32)
+    roots = np.zeros(max(n.value, 1), np.int32)
+    L.check(L.lib().tpamr_level_components(ctx.h, level, C.byref(n), L.ptr(slot), L.ptr(roots)), ctx.h)
+    return slot, roots[:n.value]
+
+
 class EllipticSolver:
     """EllipticSolver(operator; multigrid=MultigridOptions(), precision='fp64',
     device=0): CG preconditioned by one tree-GMG V-cycle per iteration.
@@ -299,6 +317,12 @@ class EllipticSolver:
                     irregular_nnz=int(ints[4]), max_leaf_level=int(ints[5]), omega=float(dbl[0]),
                     omega_p=float(dbl[1]))
 
+    def level_components(self, level):
+        """Level `level`'s ungauged components: per cell its 1-based slot (0:
+        gauged) and per slot its root (1-based smallest cell)."""
+        self._sync()
+        return _level_components(self.ctx, level, self.level_info(level)["ncells"])
+
     def level_boundary(self, level):
         """Level `level`'s boundary face coefficients (restricted from the
         finest level's) and flags: singular, coarse_augmented (the coarsest
@@ -345,7 +369,7 @@ class EllipticSolver:
         L.check(L.lib().tpamr_solve(self.ctx.h, L.ptr(rhs), L.ptr(solution), float(rtol), float(atol),
                                     int(max_cycles), 1 if compatibility == "error" else 0, C.byref(st)),
                 self.ctx.h)
-        self.stats = LinearSolverStats._from(st, self.precision)
+        self.stats = LinearSolverStats._from(st, self.precision, self.ctx.h)
         return self.stats
 
     def run_krylov(self, rhs, x0, target, max_iterations=0):
@@ -373,7 +397,7 @@ class EllipticSolver:
         L.check(L.lib().tpamr_solve_device(self.ctx.h, C.c_void_p(d_rhs), C.c_void_p(d_x), float(rtol),
                                            float(atol), int(max_cycles), 1 if compatibility == "error" else 0,
                                            C.byref(st)), self.ctx.h)
-        self.stats = LinearSolverStats._from(st, self.precision)
+        self.stats = LinearSolverStats._from(st, self.precision, self.ctx.h)
         return self.stats
 
     def time_smoother(self, reps=50):
