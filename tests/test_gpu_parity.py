@@ -413,16 +413,29 @@ def test_device_refresh_matches_host_refresh(name, precision):
         if not host:
             # host copies of the level values follow on demand
             assert np.array_equal(k_l1, out[True][3])
-            # a zero coefficient passes the operator's check and is refused
-            # by the device pass before it changes any level
-            with pytest.raises(ValueError, match="positive"):
-                s.operator.replace_face_coefficient(np.where(np.arange(pm.nfaces) == 3, 0.0, k2))
-                s.refresh()
+            # a negative coefficient is refused by the device pass before it
+            # changes any level
+            kneg = np.where(np.arange(pm.nfaces) == 3, -1.0, k2)
+            with pytest.raises(ValueError, match="nonnegative"):
+                tpamr._lib.check(tpamr.lib().tpamr_set_face_coefficients(s.ctx.h, tpamr._lib.ptr(kneg)), s.ctx.h)
+            # a zero one takes the host refresh (zeros may split the domain
+            # into components): the solve agrees with a fresh solver's (the
+            # refreshed coarse inverse may come from a Newton-Schulz update)
+            kzero = np.where(np.arange(pm.nfaces) == 3, 0.0, k2)
+            s.operator.replace_face_coefficient(kzero)
+            pz = np.zeros(pm.ncells)
+            sz = s.solve(pz, rhs, rtol=1e-10)
+            fresh = tpamr.EllipticSolver(tpamr.PoissonOperator(pm, kzero), precision=precision, device=0)
+            pf = np.zeros(pm.ncells)
+            fresh.solve(pf, rhs, rtol=1e-10)
+            fresh.close()
+            assert sz.converged and sz.removed_rhs_component_anchors == [1]
+            assert np.max(np.abs(pz - pf)) <= 1e-8 * np.max(np.abs(pf))
             s.operator.replace_face_coefficient(k2)
             s.refresh()
             q = np.zeros(pm.ncells)
             s.solve(q, rhs, rtol=1e-10)
-            assert np.array_equal(q, p)
+            assert np.max(np.abs(q - p)) <= 1e-8 * np.max(np.abs(p))
         s.close()
     _print(name, precision, out[True][1], out[False][1])
     assert out[True][1] == out[False][1]
