@@ -73,6 +73,9 @@ mutable struct GpuCgWorkspace
     mesh_generation::UInt
     mesh_identity::Any
     coefficient::Vector{Float64}
+    # open pressure sides and boundary coefficients last given to the context
+    sides::Vector{Cint}
+    boundary::Vector{Float64}
 end
 
 Krylov.solution(ws::GpuCgWorkspace) = ws.x
@@ -103,7 +106,8 @@ function gpu_krylov(; device::Integer = 0, precision::Symbol = :fp32_vcycle,
         opts = Cint[multigrid.coarse_size, multigrid.pre_sweeps, multigrid.post_sweeps, multigrid.max_levels]
         _check(handle[], ccall((:tpamr_set_multigrid, LIB), Cint, (Ptr{Cvoid}, Ptr{Cint}), handle[], opts))
         n = length(rhs)
-        ws = GpuCgWorkspace(handle[], n, n, zeros(n), 0, typemax(UInt), nothing, Float64[])
+        ws = GpuCgWorkspace(handle[], n, n, zeros(n), 0, typemax(UInt), nothing, Float64[], Cint[0, 0, 0, 0],
+            Float64[])
         finalizer(_destroy!, ws)
         return ws
     end
@@ -147,6 +151,17 @@ function _sync!(ws::GpuCgWorkspace, operator)
             ws.ctx, base, block, origin, extent, periodic, mesh.max_level, count, keys))
         ws.mesh_identity = mesh
         ws.mesh_generation = generation
+        empty!(ws.coefficient)
+    end
+    # open pressure sides and their boundary coefficients (elliptic.jl:76-152):
+    # given before the face coefficients they shape
+    sides = Cint[operator.boundary_sides...]
+    boundary = operator.boundary_face_coefficient
+    if ws.sides != sides || (any(!iszero, sides) && ws.boundary != boundary)
+        GC.@preserve sides boundary _check(ws.ctx, ccall((:tpamr_set_pressure_boundary, LIB), Cint,
+            (Ptr{Cvoid}, Ptr{Cint}, Ptr{Float64}), ws.ctx, sides, boundary))
+        ws.sides = sides
+        ws.boundary = copy(boundary)
         empty!(ws.coefficient)
     end
     # two-phase steps rewrite the operator's coefficients in place
@@ -210,8 +225,8 @@ the library takes the strong RHS and the initial pressure, integrates and projec
 runs the Krylov runs of `_run_krylov_reliably!` with the strong-residual callback, checks the
 true residual with the guarded round-off floor and returns the zero-mean pressure, in the
 precision of the workspace (`tpamr_solve`). The stats are those of
-`_solve_krylov_with_stats!` (`cycles` counts Krylov iterations over every run). One connected
-component without anchors or covered cells (the library rejects others).
+`_solve_krylov_with_stats!` (`cycles` counts Krylov iterations over every run). Open sides and
+components split by zero coefficients are the library's; covered (inactive) cells are not.
 """
 function TwoPhaseAMR.solve!(solution::TwoPhaseAMR.CellField{Float64},
         solver::TwoPhaseAMR.EllipticSolver{Float64, O, A, GpuCgWorkspace},
@@ -249,10 +264,15 @@ function TwoPhaseAMR.solve!(solution::TwoPhaseAMR.CellField{Float64},
     stats.target_linf = r.target_linf
     stats.residual_scale = r.residual_scale
     stats.removed_rhs_mean = r.removed_rhs_mean
-    empty!(stats.removed_rhs_component_means)
-    empty!(stats.removed_rhs_component_anchors)
-    push!(stats.removed_rhs_component_means, r.removed_rhs_mean)
-    push!(stats.removed_rhs_component_anchors, 1)
+    count = Ref{Cint}(0)
+    _check(ws.ctx, ccall((:tpamr_component_means, LIB), Cint, (Ptr{Cvoid}, Ref{Cint}, Ptr{Cint}, Ptr{Float64}),
+        ws.ctx, count, C_NULL, C_NULL))
+    roots = Vector{Cint}(undef, count[])
+    means = Vector{Float64}(undef, count[])
+    _check(ws.ctx, ccall((:tpamr_component_means, LIB), Cint, (Ptr{Cvoid}, Ref{Cint}, Ptr{Cint}, Ptr{Float64}),
+        ws.ctx, count, roots, means))
+    resize!(stats.removed_rhs_component_means, length(means)) .= means
+    resize!(stats.removed_rhs_component_anchors, length(roots)) .= roots
     stats.method = :cg
     stats.preconditioner = :tree_geometric
     return stats
