@@ -816,8 +816,8 @@ void level_topology(LevelData& L) {
       }
     L.blk_irr[b + 1] = int(L.irr_local.size());
   }
-  // open boundary faces: a regular cell's conductance slot beyond the side,
-  // or its irregular row (the row's extra ELL entries, build_ell)
+  // open boundary faces: the cell's conductance slot beyond the side (for
+  // irregular cells too: the stencil holds their regular and open faces)
   L.bopen.clear();
   L.bslot.clear();
   L.bdst.clear();
@@ -827,12 +827,7 @@ void level_topology(LevelData& L) {
     if (!L.open[2 * M.baxis[f] + (M.bside[f] > 0 ? 1 : 0)]) continue;
     const int c = M.bcell[f], b = c / nb, q = c % nb, i = q % bx, j = q / bx;
     L.bopen.push_back(f);
-    if (L.cell_irr[c] >= 0) {
-      rcount[L.cell_irr[c] + 1]++;
-      L.bslot.push_back(-1);
-      L.bdst.push_back(-1);
-      continue;
-    }
+    if (L.cell_irr[c] >= 0) rcount[L.cell_irr[c] + 1]++;
     if (M.baxis[f] == 0) {
       const int ii = M.bside[f] > 0 ? bx : 0;
       L.bslot.push_back(int64_t(b) * (bx + 1) * by + int64_t(j) * (bx + 1) + ii);
@@ -853,7 +848,6 @@ void level_topology(LevelData& L) {
       if (r >= 0) L.irr_bface[at[r]++] = k;
     }
   }
-  L.bpos.assign(L.bopen.size(), -1);
   pt.mark("level topology");
 }
 
@@ -871,8 +865,12 @@ void build_ell(LevelData& L) {
   const int bx = M.bs[0], by = M.bs[1], nb = bx * by;
   const int hw = nb, he = nb + by, hs = nb + 2 * by, hn = nb + 2 * by + bx, ext0 = nb + 2 * (bx + by);
   const int nrows = int(L.irr_local.size());
-  // a row's length: its entries of A, then one entry per open boundary face
-  auto row_len = [&](int r) { return L.irr_ptr[r + 1] - L.irr_ptr[r] + L.irr_bptr[r + 1] - L.irr_bptr[r]; };
+  // a row's ELL entries: its off-diagonal entries with a non-regular face's term
+  std::vector<int> rlen(nrows, 0);
+  for (int r = 0; r < nrows; ++r)
+    for (int e = L.irr_ptr[r]; e < L.irr_ptr[r + 1]; ++e)
+      if (L.ell_part[e] && e != L.irr_diag_e[r]) ++rlen[r];
+  auto row_len = [&](int r) { return rlen[r]; };
   int width = 0;
   for (int r = 0; r < nrows; ++r) width = std::max(width, row_len(r));
   L.ell_w = width;
@@ -882,7 +880,9 @@ void build_ell(LevelData& L) {
   L.ell_bound.assign(size_t(nrows) * w4, 0.0);
   L.ell_loc.assign(nrows, 0u);
   L.ell_diag.assign(nrows, 0.0);
-  L.ell_pos.assign(L.irr_col.size(), 0);
+  L.ell_bdiag.assign(nrows, 0.0);
+  L.ell_pos.assign(L.irr_col.size(), -1);
+  L.ell_dpos.assign(L.irr_col.size(), -1);
   L.ell_row.assign(nrows, 0);
   L.ext_max = 0;
   L.blk_ext.assign(L.nblocks + 1, 0);
@@ -912,9 +912,11 @@ void build_ell(LevelData& L) {
         const int len = row_len(r);
         L.ell_loc[r0 + x] = uint32_t(loc) | (uint32_t(len) << 16);
         L.ell_row[r] = r0 + x;
+        if (L.irr_diag_e[r] >= 0) L.ell_dpos[L.irr_diag_e[r]] = r0 + x;
         for (int k = 0; k < w4; ++k) L.ell_ref[slot(x, k)] = uint16_t(loc);
         int k = 0;
-        for (int e = L.irr_ptr[r]; e < L.irr_ptr[r + 1]; ++e, ++k) {
+        for (int e = L.irr_ptr[r]; e < L.irr_ptr[r + 1]; ++e) {
+          if (!L.ell_part[e] || e == L.irr_diag_e[r]) continue;
           const int col = L.irr_col[e];
           const int cb = col / nb, cl = col % nb, ci = cl % bx, cj = cl / bx;
           int ref;
@@ -934,18 +936,7 @@ void build_ell(LevelData& L) {
           const size_t at = slot(x, k);
           L.ell_ref[at] = uint16_t(ref);
           L.ell_pos[e] = int(at);
-        }
-        // open boundary faces: the halo slot beyond the side holds zero (no
-        // same-level neighbour there), so the entry -bw contributes bw v_i
-        const int li = loc % bx, lj = loc / bx;
-        for (int q = L.irr_bptr[r]; q < L.irr_bptr[r + 1]; ++q, ++k) {
-          const int kf = L.irr_bface[q], f = L.bopen[kf];
-          const int side = 2 * M.baxis[f] + (M.bside[f] > 0 ? 1 : 0);
-          if (nbr[side] >= 0) throw std::logic_error("open boundary face with a neighbour block");
-          const int ref = side == 0 ? hw + lj : side == 1 ? he + lj : side == 2 ? hs + li : hn + li;
-          const size_t at = slot(x, k);
-          L.ell_ref[at] = uint16_t(ref);
-          L.bpos[kf] = int(at);
+          ++k;
         }
       }
       // LDS slots of a block tile: up to max(nb, 32) external cells (4x4
@@ -1102,6 +1093,17 @@ void level_pattern(LevelData& L) {
   L.irr_val.assign(L.irr_col.size(), 0.0);
   L.irr_bound.assign(L.irr_col.size(), 0.0);
   L.irr_diag.assign(nrows, 0.0);
+  // entries with a term of a non-regular face (the others are regular faces'
+  // conductances, which the stencil slots hold)
+  L.ell_part.assign(L.irr_col.size(), 0);
+  parallel_for(long(L.irr_col.size()), [&](long e0, long e1, int) {
+    for (long e = e0; e < e1; ++e)
+      for (int q = L.term_ptr[e]; q < L.term_ptr[e + 1]; ++q)
+        if (!L.face_regular[L.term_f[q]]) {
+          L.ell_part[e] = 1;
+          break;
+        }
+  });
   build_ell(L);
   L.planned = true;
 }
@@ -1130,8 +1132,8 @@ void level_values(LevelData& L) {
       arr[L.slot_b[f]] = w;
     }
   });
-  // open boundary faces of regular cells: their conductance in the slot
-  // beyond the side (the halo value there is zero)
+  // open boundary faces: their conductance in the cell's slot beyond the
+  // side (the halo value there is zero)
   const bool opened = L.any_open();
   if (opened && L.bw.size() != L.bopen.size())
     throw std::logic_error("open pressure sides without boundary coefficients");
@@ -1146,7 +1148,8 @@ void level_values(LevelData& L) {
       double rowsum = 0;
       const int de = L.irr_diag_e[r];
       for (int e = L.irr_ptr[r]; e < L.irr_ptr[r + 1]; ++e) {
-        double v = 0, bv = 0;
+        // the full entry (v, bound bv) and its non-regular faces' part (vp, bvp)
+        double v = 0, bv = 0, vp = 0, bvp = 0;
         for (int q = L.term_ptr[e]; q < L.term_ptr[e + 1]; ++q) {
           const int f = L.term_f[q], g = L.term_g[q];
           const double kfg = L.term_k[q], af = M.farea[f], ag = M.farea[g];
@@ -1156,23 +1159,24 @@ void level_values(LevelData& L) {
           const double babs = diag ? core : (sf * std::fabs(kfg)) * sg * af * ag;
           v += L.term_s[q] * core;
           bv += babs;
+          if (!L.face_regular[f]) {
+            vp += L.term_s[q] * core;
+            bvp += babs;
+          }
         }
         if (opened && e == de) {
-          // the boundary diagonal joins the row's diagonal (and its bound)
+          // the boundary diagonal joins the row's full diagonal (and bound)
           v = v + L.irr_bd[r];
           bv = bv + L.irr_bd[r];
         }
         L.irr_val[e] = v;
         L.irr_bound[e] = bv;
-        L.ell_val[L.ell_pos[e]] = v;
-        L.ell_bound[L.ell_pos[e]] = bv;
+        if (L.ell_pos[e] >= 0) {
+          L.ell_val[L.ell_pos[e]] = vp;
+          L.ell_bound[L.ell_pos[e]] = bvp;
+        }
+        if (L.ell_dpos[e] >= 0) L.ell_bdiag[L.ell_dpos[e]] = bvp;
         rowsum += std::fabs(v);
-      }
-      // the open faces' entries: -bw at the zero halo slot beyond the side
-      for (int q = L.irr_bptr[r]; q < L.irr_bptr[r + 1]; ++q) {
-        const int k = L.irr_bface[q];
-        L.ell_val[L.bpos[k]] = -L.bw[k];
-        L.ell_bound[L.bpos[k]] = 0.0;
       }
       const double diag = de >= 0 ? L.irr_val[de] : 0.0;
       L.irr_diag[r] = diag;

@@ -116,18 +116,29 @@ struct LevelData {
   std::vector<double> irr_diag;
   std::vector<int> cell_irr;  // per cell: irregular row index or -1
   std::vector<double> dinv;   // 1 / diag(A)
-  // Device form of the irregular rows: per block, the rows ordered by
-  // decreasing length with their entries in groups of four (entry 4g+t of the
-  // block's row x at (blk_irr[b] * ng + g * nr + x) * 4 + t, ng =
-  // ceil(ell_w / 4)); ell_loc[blk_irr[b] + x] holds local cell | length << 16.
-  // Columns are LDS slots of the block's tile: interior cells, the W, E, S, N
-  // halo strips, then the block's external cells (build_ell).
+  // Device form of the irregular rows. A row of A splits into the
+  // conductances of the cell's regular (and open boundary) faces, which its
+  // stencil slots hold like any cell's, and the couplings of its non-regular
+  // faces (coarse-fine faces and faces with off-diagonal metric terms). The
+  // ELL keeps the off-diagonal entries of that second part only: per block,
+  // the rows ordered by decreasing length with their entries in groups of
+  // four (entry 4g+t of the block's row x at (blk_irr[b] * ng + g * nr + x) *
+  // 4 + t, ng = ceil(ell_w / 4)); ell_loc[blk_irr[b] + x] holds local cell |
+  // length << 16. Columns are LDS slots of the block's tile: interior cells,
+  // the W, E, S, N halo strips, then the block's external cells (build_ell).
+  // ell_diag is the full diagonal of A (the Jacobi diagonal), ell_bdiag the
+  // non-regular faces' part of the floor-bound diagonal.
   int ell_w = 0;
   int ext_max = 0;  // most external cells of one block
   std::vector<uint32_t> ell_loc;
-  std::vector<double> ell_diag;  // diagonal of each row, same order as ell_loc
+  std::vector<double> ell_diag, ell_bdiag;  // per row, same order as ell_loc
   std::vector<uint16_t> ell_ref;
   std::vector<double> ell_val, ell_bound;
+  // per CSR entry: whether a term comes from a non-regular face (its
+  // off-diagonal entries are the ELL's); ell_dpos: for a row's diagonal entry
+  // its row's position in ell_loc order (-1 for the others)
+  std::vector<uint8_t> ell_part;
+  std::vector<int> ell_dpos;
   std::vector<int> blk_ext, ext_col;
   // structure of the value refresh (level_pattern, built once per topology):
   // per irregular entry its face-metric terms [term_ptr[e], term_ptr[e+1])
@@ -150,20 +161,18 @@ struct LevelData {
   // face its coefficient kb (the finest level: the caller's; coarser ones:
   // the overlap-weighted restriction brptr/bridx/brw of the finer level's);
   // the open faces (bopen, boundary-face order) with their conductance
-  // |f| 2 kb / h (bw, the boundary diagonal term of elliptic.jl:133-152) and
-  // destination: a regular cell's conductance slot beyond that side (bslot:
-  // flat index into cx (x faces) or cy (y faces); bdst: the device code
-  // index * 4 + array of the regular faces), or an irregular row's extra ELL
-  // entry of value -bw whose column is the zero halo slot beyond that side
-  // (bpos: its ELL position; irr_bptr / irr_bface: each row's open faces).
-  // irr_bd: per irregular row the sum of its open faces' conductances (face
-  // order), added to the row's diagonal. A level with an open face of
-  // positive conductance is nonsingular.
+  // |f| 2 kb / h (bw, the boundary diagonal term of elliptic.jl:133-152) in
+  // the cell's conductance slot beyond that side (bslot: flat index into cx
+  // (x faces) or cy (y faces); bdst: the device code index * 4 + array of the
+  // regular faces): the halo value there is zero. irr_bptr / irr_bface: each
+  // irregular row's open faces; irr_bd: their conductances summed in face
+  // order, which the row's full diagonal includes. A level with an open face
+  // of positive conductance is nonsingular.
   bool open[4] = {false, false, false, false};
   std::vector<double> kb;
   std::vector<int> brptr, bridx;
   std::vector<double> brw;
-  std::vector<int> bopen, bdst, bpos, irr_bptr, irr_bface;
+  std::vector<int> bopen, bdst, irr_bptr, irr_bface;
   std::vector<int64_t> bslot;
   std::vector<double> bw, irr_bd;
   // Pressure components of the level (`_refresh_elliptic_components!`,

@@ -1076,18 +1076,22 @@ __global__ void k_ref_regular(int nf, const double* __restrict__ k, const double
 }
 
 // irregular rows, one thread per entry: the entry is the sum of its
-// face-metric terms, in the host pass's order (also kept in val_e, the
-// entries in row order)
+// face-metric terms, in the host pass's order (kept in val_e, the entries in
+// row order); the terms of non-regular faces (dst[2 f] < 0) also sum to the
+// entry's ELL part (value and floor bound) when it has an ELL slot, and a
+// diagonal entry's ELL bound part goes to its row's ell_bdiag
 __global__ void k_ref_entries(int nnz, const int* __restrict__ term_ptr, const int* __restrict__ term_f,
                               const int* __restrict__ term_g, const double* __restrict__ term_k,
                               const double* __restrict__ term_s, const unsigned char* __restrict__ term_diag,
                               const double* __restrict__ k, const double* __restrict__ area,
-                              const int* __restrict__ ell_pos, double* __restrict__ ell_val,
-                              double* __restrict__ ell_bound, double* __restrict__ val_e) {
+                              const int* __restrict__ dst, const int* __restrict__ ell_pos,
+                              const int* __restrict__ ell_dpos, double* __restrict__ ell_val,
+                              double* __restrict__ ell_bound, double* __restrict__ ell_bdiag,
+                              double* __restrict__ val_e) {
 #pragma clang fp contract(off)
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nnz) return;
-  double v = 0, bv = 0;
+  double v = 0, bv = 0, vp = 0, bvp = 0;
   for (int q = term_ptr[e]; q < term_ptr[e + 1]; ++q) {
     const int f = term_f[q], g = term_g[q];
     const double kfg = term_k[q], af = area[f], ag = area[g];
@@ -1102,28 +1106,29 @@ __global__ void k_ref_entries(int nnz, const int* __restrict__ term_ptr, const i
     }
     v += term_s[q] * core;
     bv += babs;
+    if (dst[2 * f] < 0) {
+      vp += term_s[q] * core;
+      bvp += babs;
+    }
   }
-  ell_val[ell_pos[e]] = v;
-  if (ell_bound) ell_bound[ell_pos[e]] = bv;
+  const int pos = ell_pos[e];
+  if (pos >= 0) {
+    ell_val[pos] = vp;
+    if (ell_bound) ell_bound[pos] = bvp;
+  }
+  if (ell_bdiag && ell_dpos[e] >= 0) ell_bdiag[ell_dpos[e]] = bvp;
   val_e[e] = v;
 }
 
 // open boundary faces, one thread per face: the conductance bw into the
-// regular cell's slot beyond the side (dst code index * 4 + array), or -bw
-// into the irregular row's extra ELL entry (pos)
-__global__ void k_ref_boundary(int nopen, const double* __restrict__ bw, const int* __restrict__ dst,
-                               const int* __restrict__ pos, double* cw, double* ce, double* cs, double* cn,
-                               double* __restrict__ ell_val, double* __restrict__ ell_bound) {
+// cell's slot beyond the side (dst code index * 4 + array)
+__global__ void k_ref_boundary(int nopen, const double* __restrict__ bw, const int* __restrict__ dst, double* cw,
+                               double* ce, double* cs, double* cn) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nopen) return;
   const int d = dst[k];
-  if (d >= 0) {
-    double* arr[4] = {cw, ce, cs, cn};
-    arr[d & 3][d >> 2] = bw[k];
-  } else {
-    ell_val[pos[k]] = -bw[k];
-    if (ell_bound) ell_bound[pos[k]] = 0.0;
-  }
+  double* arr[4] = {cw, ce, cs, cn};
+  arr[d & 3][d >> 2] = bw[k];
 }
 
 // irregular rows, one thread per row: the row's diagonal (with the row's
@@ -1133,9 +1138,7 @@ __global__ void k_ref_boundary(int nopen, const double* __restrict__ bw, const i
 __global__ void k_ref_rows(int nrows, const int* __restrict__ irr_ptr, const int* __restrict__ diag_e,
                            const int* __restrict__ ell_row, const double* __restrict__ val_e,
                            double* __restrict__ ell_diag, double* __restrict__ irr_diag,
-                           unsigned long long* __restrict__ ratio, const double* __restrict__ bd,
-                           const int* __restrict__ ell_pos, double* __restrict__ ell_val,
-                           double* __restrict__ ell_bound) {
+                           unsigned long long* __restrict__ ratio, const double* __restrict__ bd) {
 #pragma clang fp contract(off)
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   // the largest row ratio of the wavefront, then one atomic per wavefront
@@ -1149,10 +1152,6 @@ __global__ void k_ref_rows(int nrows, const int* __restrict__ irr_ptr, const int
       if (bd && e == de) v = v + bd[r];
       rowsum += fabs(v);
       if (e == de) diag = v;
-    }
-    if (bd && de >= 0) {
-      ell_val[ell_pos[de]] = diag;
-      if (ell_bound) ell_bound[ell_pos[de]] = ell_bound[ell_pos[de]] + bd[r];
     }
     irr_diag[r] = diag;
     ell_diag[ell_row[r]] = diag;
@@ -1657,23 +1656,23 @@ void launch_ref_regular(hipStream_t s, int nf, const double* k, const double* kd
   if (nf > 0)
     hipLaunchKernelGGL(k_ref_regular, dim3(cdiv(nf, 256)), dim3(256), 0, s, nf, k, kdiag, area, dst, cw, ce, cs, cn);
 }
-void launch_ref_boundary(hipStream_t s, int nopen, const double* bw, const int* dst, const int* pos, double* cw,
-                         double* ce, double* cs, double* cn, double* ell_val, double* ell_bound) {
+void launch_ref_boundary(hipStream_t s, int nopen, const double* bw, const int* dst, double* cw, double* ce,
+                         double* cs, double* cn) {
   if (nopen > 0)
-    hipLaunchKernelGGL(k_ref_boundary, dim3(cdiv(nopen, 256)), dim3(256), 0, s, nopen, bw, dst, pos, cw, ce, cs, cn,
-                       ell_val, ell_bound);
+    hipLaunchKernelGGL(k_ref_boundary, dim3(cdiv(nopen, 256)), dim3(256), 0, s, nopen, bw, dst, cw, ce, cs, cn);
 }
 void launch_ref_rows(hipStream_t s, int nrows, const int* irr_ptr, const int* term_ptr, const int* term_f,
                      const int* term_g, const double* term_k, const double* term_s, const unsigned char* term_diag,
-                     const double* k, const double* area, const int* ell_pos, const int* ell_row, const int* diag_e,
-                     double* ell_val, double* ell_bound, double* ell_diag, double* irr_diag, double* val_e,
-                     int nnz, unsigned long long* ratio, const double* bd) {
+                     const double* k, const double* area, const int* dst, const int* ell_pos, const int* ell_dpos,
+                     const int* ell_row, const int* diag_e, double* ell_val, double* ell_bound, double* ell_bdiag,
+                     double* ell_diag, double* irr_diag, double* val_e, int nnz, unsigned long long* ratio,
+                     const double* bd) {
   if (nnz > 0)
     hipLaunchKernelGGL(k_ref_entries, dim3(cdiv(nnz, 256)), dim3(256), 0, s, nnz, term_ptr, term_f, term_g, term_k,
-                       term_s, term_diag, k, area, ell_pos, ell_val, ell_bound, val_e);
+                       term_s, term_diag, k, area, dst, ell_pos, ell_dpos, ell_val, ell_bound, ell_bdiag, val_e);
   if (nrows > 0)
     hipLaunchKernelGGL(k_ref_rows, dim3(cdiv(nrows, 256)), dim3(256), 0, s, nrows, irr_ptr, diag_e, ell_row, val_e,
-                       ell_diag, irr_diag, ratio, bd, ell_pos, ell_val, ell_bound);
+                       ell_diag, irr_diag, ratio, bd);
 }
 void launch_ref_dinv(hipStream_t s, int n, int bx, int by, const double* cw, const double* ce, const double* cs,
                      const double* cn, const int* cell_irr, const double* irr_diag, double* dinv, int* regular,

@@ -21,9 +21,13 @@ struct LevelView {
   const T* ce;
   const T* cs;
   const T* cn;
+  // irregular rows: the ELL part of the non-regular faces (off-diagonal
+  // entries; their floor-bound values and the bound's diagonal part for the
+  // FP64 finest level only) and the full diagonal per row, in ell_loc order
   const T* ell_val;
-  const T* ell_bound;  // floor-bound values (FP64 finest level only)
-  const T* ell_diag;  // diagonal per irregular row, in ell_loc order
+  const T* ell_bound;
+  const T* ell_bdiag;
+  const T* ell_diag;
   // irregular rows of block b (nr_b rows, ng = ceil(ell_w / 4) groups of four
   // entries per row): entry 4 g + t of row x at (blk_irr[b] * ng + g * nr_b + x) * 4 + t
   const unsigned short* ell_ref;
@@ -107,16 +111,19 @@ void launch_ref_restrict(hipStream_t s, int nf, const int* rptr, const int* ridx
                          const double* kprev, double* k);
 void launch_ref_regular(hipStream_t s, int nf, const double* k, const double* kdiag, const double* area,
                         const int* dst, double* cw, double* ce, double* cs, double* cn);
-// open boundary faces: bw into a regular cell's slot (dst >= 0) or -bw into an
-// irregular row's extra ELL entry (pos)
-void launch_ref_boundary(hipStream_t s, int nopen, const double* bw, const int* dst, const int* pos, double* cw,
-                         double* ce, double* cs, double* cn, double* ell_val, double* ell_bound);
-// bd (NULL without open sides): per row the boundary diagonal added to its diagonal
+// open boundary faces: bw into the cell's slot beyond the side
+void launch_ref_boundary(hipStream_t s, int nopen, const double* bw, const int* dst, double* cw, double* ce,
+                         double* cs, double* cn);
+// irregular rows: full entries (val_e, diagonals, row ratios) and the ELL
+// part of the non-regular faces (ell_val, ell_bound, ell_bdiag; bound arrays
+// may be NULL); dst marks the regular faces; bd (NULL without open sides):
+// per row the boundary diagonal added to its full diagonal
 void launch_ref_rows(hipStream_t s, int nrows, const int* irr_ptr, const int* term_ptr, const int* term_f,
                      const int* term_g, const double* term_k, const double* term_s, const unsigned char* term_diag,
-                     const double* k, const double* area, const int* ell_pos, const int* ell_row, const int* diag_e,
-                     double* ell_val, double* ell_bound, double* ell_diag, double* irr_diag, double* val_e,
-                     int nnz, unsigned long long* ratio, const double* bd);
+                     const double* k, const double* area, const int* dst, const int* ell_pos, const int* ell_dpos,
+                     const int* ell_row, const int* diag_e, double* ell_val, double* ell_bound, double* ell_bdiag,
+                     double* ell_diag, double* irr_diag, double* val_e, int nnz, unsigned long long* ratio,
+                     const double* bd);
 // nbr (NULL without open sides): regular cells with an open boundary slot add
 // their row ratio to `ratio`
 void launch_ref_dinv(hipStream_t s, int n, int bx, int by, const double* cw, const double* ce, const double* cs,

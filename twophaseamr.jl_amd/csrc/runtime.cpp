@@ -100,7 +100,7 @@ struct DevTopo {
 // coefficients and, for levels without an FP64 copy, FP64 staging values.
 struct DevPlan {
   int nfaces = 0, nrows = 0;
-  DevArray<int> rptr, ridx, dst, irr_ptr, term_ptr, term_f, term_g, ell_pos, ell_row, diag_e, cell_irr;
+  DevArray<int> rptr, ridx, dst, irr_ptr, term_ptr, term_f, term_g, ell_pos, ell_dpos, ell_row, diag_e, cell_irr;
   DevArray<double> rw, kdiag, area, kappa, term_k, term_s, irr_diag, val_e;
   int nnz = 0;  // irregular entries
   DevArray<unsigned char> term_diag;
@@ -109,7 +109,7 @@ struct DevPlan {
   // position), and every irregular row's boundary diagonal
   int nopen = 0;
   bool open = false;
-  DevArray<int> bdst, bpos;
+  DevArray<int> bdst;
   DevArray<double> bw, irr_bd;
 };
 
@@ -127,7 +127,7 @@ static_assert(GAUGE_CHUNK == GAUGE_CHUNK_CELLS, "host and device component chunk
 
 template <typename T>
 struct DevLevel {
-  DevArray<T> cw, ce, cs, cn, ell_val, ell_bound, ell_diag, dinv;
+  DevArray<T> cw, ce, cs, cn, ell_val, ell_bound, ell_bdiag, ell_diag, dinv;
   DevArray<T> x, xs, b, u;
   double omega = 1, omega_p = 0;
   LevelView<T> view{};
@@ -343,7 +343,10 @@ void upload_level_values(tpamr_ctx* c, DevLevel<T>& D, const LevelData& L, const
   D.cs.upload(L.cs, s);
   D.cn.upload(L.cn, s);
   D.ell_val.upload(L.ell_val, s);
-  if (with_bound) D.ell_bound.upload(L.ell_bound, s);
+  if (with_bound) {
+    D.ell_bound.upload(L.ell_bound, s);
+    D.ell_bdiag.upload(L.ell_bdiag.empty() ? std::vector<double>(1, 0.0) : L.ell_bdiag, s);
+  }
   D.ell_diag.upload(L.ell_diag, s);
   D.dinv.upload(L.dinv, s);
   D.omega = L.omega;
@@ -372,6 +375,7 @@ void upload_level_values(tpamr_ctx* c, DevLevel<T>& D, const LevelData& L, const
   v.ell_w = L.ell_w;
   v.ell_val = D.ell_val.p;
   v.ell_bound = D.ell_bound.p;
+  v.ell_bdiag = D.ell_bdiag.p;
   v.ell_diag = D.ell_diag.p;
   v.ell_ref = Tp.ell_ref.p;
   v.blk_irr = Tp.blk_irr.p;
@@ -1365,6 +1369,7 @@ void upload_plan(tpamr_ctx* c) {
     up(P.term_s, L.term_s, 0.0);
     up(P.term_diag, L.term_diag, (unsigned char)0);
     up(P.ell_pos, L.ell_pos, 0);
+    up(P.ell_dpos, L.ell_dpos, 0);
     up(P.ell_row, L.ell_row, 0);
     up(P.diag_e, L.irr_diag_e, 0);
     P.cell_irr.upload(L.cell_irr, s);
@@ -1383,7 +1388,6 @@ void upload_plan(tpamr_ctx* c) {
     P.open = L.any_open();
     P.nopen = int(L.bopen.size());
     up(P.bdst, L.bdst, 0);
-    up(P.bpos, L.bpos, 0);
   }
   c->ref_ratio.alloc(nl);
   c->ref_flags.alloc(nl + 2);
@@ -1453,15 +1457,16 @@ bool device_refresh(tpamr_ctx* c, const double* kappa, bool kappa_on_device = fa
     double* ed = has64 ? D->ell_diag.p : P.ell_diag.p;
     double* di = has64 ? D->dinv.p : P.dinv.p;
     double* eb = (has64 && l == 0) ? D->ell_bound.p : nullptr;
+    double* ebd = (has64 && l == 0) ? D->ell_bdiag.p : nullptr;
     launch_fill(s, int(L.cw.size()), 0.0, cw);
     launch_fill(s, int(L.ce.size()), 0.0, ce);
     launch_fill(s, int(L.cs.size()), 0.0, cs);
     launch_fill(s, int(L.cn.size()), 0.0, cn);
     launch_ref_regular(s, P.nfaces, P.kappa.p, P.kdiag.p, P.area.p, P.dst.p, cw, ce, cs, cn);
-    if (P.open) launch_ref_boundary(s, P.nopen, P.bw.p, P.bdst.p, P.bpos.p, cw, ce, cs, cn, ev, eb);
+    if (P.open) launch_ref_boundary(s, P.nopen, P.bw.p, P.bdst.p, cw, ce, cs, cn);
     launch_ref_rows(s, P.nrows, P.irr_ptr.p, P.term_ptr.p, P.term_f.p, P.term_g.p, P.term_k.p, P.term_s.p,
-                    P.term_diag.p, P.kappa.p, P.area.p, P.ell_pos.p, P.ell_row.p, P.diag_e.p, ev, eb, ed,
-                    P.irr_diag.p, P.val_e.p, P.nnz, c->ref_ratio.p + l, P.open ? P.irr_bd.p : nullptr);
+                    P.term_diag.p, P.kappa.p, P.area.p, P.dst.p, P.ell_pos.p, P.ell_dpos.p, P.ell_row.p, P.diag_e.p,
+                    ev, eb, ebd, ed, P.irr_diag.p, P.val_e.p, P.nnz, c->ref_ratio.p + l, P.open ? P.irr_bd.p : nullptr);
     launch_ref_dinv(s, L.ncells, bx, by, cw, ce, cs, cn, P.cell_irr.p, P.irr_diag.p, di, c->ref_flags.p + l,
                     P.open ? c->topo[l].nbr.p : nullptr, c->ref_ratio.p + l);
     if (l < int(c->lf.size())) {

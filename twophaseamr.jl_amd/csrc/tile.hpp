@@ -177,7 +177,7 @@ __device__ inline void tile4(unsigned char* lds, const LevelView<T>& L, const Pl
   const int ng = (L.ell_w + 3) >> 2;  // entry groups per row
   const T* __restrict__ vals = ABS ? L.ell_bound : L.ell_val;
   Vec4<T> own{}, cw4{}, cs4{}, cn4{};
-  T rdg = 0;
+  T rdg = 0, rbd = 0;
   T ceb = 0;
   int nr = 0, r0 = 0;
   Vec4<T> rv[ELL_G];
@@ -232,6 +232,7 @@ __device__ inline void tile4(unsigned char* lds, const LevelView<T>& L, const Pl
       if (lt < nr) {
         info = L.ell_loc[r0 + lt];
         rdg = L.ell_diag[r0 + lt];
+        if constexpr (ABS) rbd = L.ell_bdiag[r0 + lt];
         const int base = (r0 * ng + lt) * 4;
 #pragma unroll
         for (int g = 0; g < ELL_G; ++g)
@@ -271,10 +272,13 @@ __device__ inline void tile4(unsigned char* lds, const LevelView<T>& L, const Pl
   __syncthreads();
   if (any) {
     if (P.valid) {
-      // Rows sum to zero (Neumann / periodic operator), so (A v)_i is
-      // evaluated in flux form sum_j a_ij (v_j - v_i): the diagonal and the
-      // zero padding drop out and rounding scales with the differences, as
-      // in the reference's face-based product. ABS keeps sum_j |b_ij||v_j|.
+      // The row's ELL part (the couplings of its non-regular faces) sums to
+      // zero, like each face's contribution, so it is evaluated in flux form
+      // sum_j a_ij (v_j - v_i) over its off-diagonal entries: the zero padding
+      // drops out and rounding scales with the differences, as in the
+      // reference's face-based product. The stencil adds the cell's regular
+      // and open faces in phase C. ABS keeps sum_j |b_ij||v_j| with the
+      // bound's diagonal part rbd |v_i|.
       if (lt < nr) {
         const T vi = ABS ? T(0) : S.v[rloc];
         T s = 0;
@@ -286,6 +290,7 @@ __device__ inline void tile4(unsigned char* lds, const LevelView<T>& L, const Pl
             s += rv[g].v[2] * (S.v[rr[g].y & 0xffffu] - vi);
             s += rv[g].v[3] * (S.v[rr[g].y >> 16] - vi);
           }
+        if constexpr (ABS) s += rbd * S.v[rloc];
         S.ax[rloc] = s;
         S.d[rloc] = rdg;
         reinterpret_cast<unsigned char*>(S.flag4)[rloc] = 1;
@@ -305,6 +310,7 @@ __device__ inline void tile4(unsigned char* lds, const LevelView<T>& L, const Pl
           s += v.v[2] * (S.v[r.y & 0xffffu] - vi);
           s += v.v[3] * (S.v[r.y >> 16] - vi);
         }
+        if constexpr (ABS) s += L.ell_bdiag[r0 + x] * S.v[loc];
         S.ax[loc] = s;
         S.d[loc] = L.ell_diag[r0 + x];
         reinterpret_cast<unsigned char*>(S.flag4)[loc] = 1;
@@ -332,8 +338,8 @@ __device__ inline void tile4(unsigned char* lds, const LevelView<T>& L, const Pl
       dgi = ld4(S.d + seg);
     }
   }
-  // stencil of every cell, then a select for the irregular ones (keeps the
-  // results in registers)
+  // stencil of every cell (an irregular cell's regular and open faces), plus
+  // the ELL part of the irregular ones (keeps the results in registers)
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     const T vw = m > 0 ? own.v[m - 1] : vwest;
@@ -348,7 +354,7 @@ __device__ inline void tile4(unsigned char* lds, const LevelView<T>& L, const Pl
     const T st = ABS ? dd * vi + ((cw * vw + ce * ve) + (cs * vs + cn * vn))
                      : (cw * (vi - vw) + ce * (vi - ve)) + (cs * (vi - vs) + cn * (vi - vn));
     const bool irr = (bits >> m) & 1u;
-    ax.v[m] = irr ? axi.v[m] : st;
+    ax.v[m] = irr ? st + axi.v[m] : st;
     d.v[m] = irr ? dgi.v[m] : dd;
   }
 }
