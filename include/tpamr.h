@@ -215,6 +215,12 @@ int tpamr_apply_device(tpamr_ctx* ctx, const double* d_x, double* d_y);
 /* inv.(mixture_density) with the staggered liquid fraction (forces.jl:23-36,69-84,131) */
 int tpamr_mixture_face_coefficients(tpamr_ctx* ctx, const double* fraction, double rho_liquid, double rho_gas,
                                     double* kappa);
+/* _mixture_boundary_coefficient (forces.jl:103-111): per boundary face
+ * inv(rho_l c + rho_g (1 - c)) of its cell's fraction c, the boundary
+ * coefficients of the two-phase operator's open sides (host arrays; for
+ * tpamr_set_pressure_boundary) */
+int tpamr_mixture_boundary_coefficients(tpamr_ctx* ctx, const double* fraction, double rho_liquid, double rho_gas,
+                                        double* boundary_coefficient);
 /* _surface_tension_jumps! (forces.jl:345-378): formulation 0 = CGFM, 1 = CSF;
  * face_curvature per face (NaN = unresolved) */
 int tpamr_surface_tension_jumps(tpamr_ctx* ctx, const double* fraction, const double* face_curvature, double sigma,

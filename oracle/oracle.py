@@ -185,6 +185,12 @@ class OMesh:
                                               C.c_double(rho_l), C.c_double(rho_g), _dptr(k)))
         return k
 
+    def mixture_boundary_coefficients(self, c, rho_l, rho_g):
+        kb = np.zeros(self.nbfaces)
+        _check(lib().orc_mixture_boundary_coefficients(self._h, _dptr(np.ascontiguousarray(c, np.float64)),
+                                                       C.c_double(rho_l), C.c_double(rho_g), _dptr(kb)))
+        return kb
+
     def cgfm_jumps(self, c, curvature, sigma, csf=False):
         out = np.zeros(self.nfaces)
         curv = np.broadcast_to(np.asarray(curvature, np.float64), (self.nfaces,)).copy()

@@ -23,6 +23,7 @@ void apply_vcycle(Solver& S, const double* r, double* z);
 Stats solve(Solver& S, const double* rhs, double* xio, double rtol, double atol, int max_cycles,
             int compat_error, std::vector<double>* removed_means);
 void mixture_coefficients(const Mesh& M, const double* c, double rho_l, double rho_g, double* kappa);
+void mixture_boundary_coefficients(const Mesh& M, const double* c, double rho_l, double rho_g, double* kb);
 double face_curvature(const Mesh& M, const double* c, const double* kappa, const double* quality, int fi);
 int surface_tension_jumps_cells(const Mesh& M, const double* c, const double* kappa, const double* quality,
                                 double sigma, int csf, double* jumps);
@@ -299,6 +300,10 @@ int orc_solve(void* sp, const double* rhs, double* x, double rtol, double atol, 
       for (size_t q = 0; q < rm.size(); ++q) removed_means[q] = rm[q];
     return 0;
   })
+}
+
+int orc_mixture_boundary_coefficients(void* mp, const double* c, double rho_l, double rho_g, double* kb) {
+  GUARD({ mixture_boundary_coefficients(*static_cast<Mesh*>(mp), c, rho_l, rho_g, kb); return 0; })
 }
 
 int orc_mixture_coefficients(void* mp, const double* c, double rho_l, double rho_g, double* kappa) {

@@ -1262,6 +1262,18 @@ Stats solve(Solver& S, const double* rhs, double* xio, double rtol, double atol,
 
 // `_staggered_liquid_fraction` + `_mixture_density!` + `inv.(rho)`,
 // forces.jl:23-36,69-84,131
+// `_mixture_boundary_coefficient` (forces.jl:103-111): per boundary face
+// inv(rho_l c + rho_g (1 - c)) with c the fraction of its cell (no embedded
+// geometry: the conditional fraction is the fraction)
+void mixture_boundary_coefficients(const Mesh& M, const double* c, double rho_l, double rho_g, double* kb) {
+  if (!(std::isfinite(rho_l) && rho_l > 0 && std::isfinite(rho_g) && rho_g > 0))
+    throw std::invalid_argument("phase densities must be finite and positive");
+  for (size_t f = 0; f < M.bfaces.size(); ++f) {
+    const double cf = c[M.bfaces[f].cell];
+    kb[f] = 1.0 / (rho_l * cf + rho_g * (1.0 - cf));
+  }
+}
+
 void mixture_coefficients(const Mesh& M, const double* c, double rho_l, double rho_g, double* kappa) {
   if (!(std::isfinite(rho_l) && rho_l > 0 && std::isfinite(rho_g) && rho_g > 0))
     throw std::invalid_argument("phase densities must be finite and positive");

@@ -197,8 +197,7 @@ def test_open_circles_levels5_8_against_oracle():
     om = OMesh(base_mesh=mesh.base_mesh, block_size=mesh.block_size, origin=mesh.origin, extent=mesh.extent,
                periodic=mesh.periodic, max_level=mesh.max_level, refined_keys=mesh.refined_keys)
     kappa = om.mixture_coefficients(frac, 1000.0, 1.0)
-    fb = frac[mesh.bface_cell]
-    kb = 1.0 / (1000.0 * fb + 1.0 * (1.0 - fb))
+    kb = om.mixture_boundary_coefficients(frac, 1000.0, 1.0)
     sides = (True, True, False, False)
     u = W.smooth_velocity(mesh, device="cuda:0")
     s0 = tpamr.EllipticSolver(tpamr.PoissonOperator(mesh, kappa), precision="fp64")
@@ -207,8 +206,11 @@ def test_open_circles_levels5_8_against_oracle():
     xo, so = OSolver(om, kappa, open_sides=sides, boundary_coefficient=kb).solve(rhs, rtol=1e-10, max_cycles=200)
     assert so["converged"]
     for prec in MODES:
-        s = tpamr.EllipticSolver(tpamr.PoissonOperator(mesh, kappa, velocity_boundary=sides,
-                                                       boundary_face_coefficient=kb), precision=prec)
+        # the two-phase refresh sets the mixture's interior and boundary coefficients
+        s = tpamr.EllipticSolver(tpamr.PoissonOperator(mesh, 1.0, velocity_boundary=sides), precision=prec)
+        tpamr.refresh_two_phase(s, frac, (1000.0, 1.0))
+        assert np.array_equal(s.operator.boundary_face_coefficient, kb)
+        assert np.array_equal(s.operator.face_coefficient, kappa)
         d_rhs = torch.as_tensor(rhs, device="cuda:0")
         d_x = torch.zeros(mesh.ncells, dtype=torch.float64, device="cuda:0")
         st = s.solve_device(d_rhs.data_ptr(), d_x.data_ptr(), rtol=1e-10, max_cycles=200)

@@ -77,3 +77,20 @@ def test_open_side_errors():
                                           bottom=tpamr.NoSlip(), top=tpamr.FreeSlip())
     assert tpamr.pressure_boundary_sides(vb) == OPEN_LR
     assert tpamr.pressure_boundary_sides(None) == (False,) * 4
+
+
+def test_mixture_boundary_coefficients_match_oracle():
+    """`_mixture_boundary_coefficient` (forces.jl:103-111): 1 / (rho_l c + rho_g
+    (1 - c)) of each boundary face's cell, bit for bit with the oracle (and
+    the closed form)."""
+    import ctypes as C
+    from tpamr import _lib as L
+    kw, _, _ = CASES["amr64_all_open"]
+    om, pm = OMesh(**kw), tpamr.BlockTreeMesh(**kw)
+    x, y = pm.cell_centers()
+    c = np.clip(0.5 + 2.0 * (0.3 - np.hypot(x - 0.5, y - 0.5)), 0.0, 1.0)
+    kb = np.zeros(pm.nboundary_faces)
+    L.check(L.lib().tpamr_mixture_boundary_coefficients(pm._host.h, L.ptr(c), 1000.0, 1.2, L.ptr(kb)), pm._host.h)
+    assert np.array_equal(kb, om.mixture_boundary_coefficients(c, 1000.0, 1.2))
+    cb = c[pm.bface_cell]
+    assert np.array_equal(kb, 1.0 / (1000.0 * cb + 1.2 * (1.0 - cb)))

@@ -2194,6 +2194,20 @@ int tpamr_mixture_face_coefficients(tpamr_ctx* c, const double* fraction, double
   });
 }
 
+int tpamr_mixture_boundary_coefficients(tpamr_ctx* c, const double* fraction, double rl, double rg, double* kb) {
+  if (!c || !fraction || !kb) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    if (!(std::isfinite(rl) && rl > 0 && std::isfinite(rg) && rg > 0))
+      throw std::invalid_argument("phase densities must be finite and positive");
+    const TreeMesh& M = c->mesh();
+    check_finite(fraction, M.ncells(), "fraction");
+    for (size_t f = 0; f < M.bcell.size(); ++f) {
+      const double cf = fraction[M.bcell[f]];
+      kb[f] = 1.0 / (rl * cf + rg * (1.0 - cf));
+    }
+  });
+}
+
 int tpamr_surface_tension_jumps(tpamr_ctx* c, const double* fraction, const double* curvature, double sigma,
                                 int formulation, double* jumps) {
   if (!c || !fraction || !curvature || !jumps) return TPAMR_EINVAL;

@@ -5,7 +5,8 @@ from ._lib import LIB_PATH, TPAMR_FP32_REFINE, TPAMR_FP32_VCYCLE, TPAMR_FP64, Co
 from .hierarchy import HostHierarchy
 from .mesh import BlockKey, BlockTreeMesh
 from .physics import (CGFM, CSF, divergence, fill_halo, gravity, gravity_jumps, gravity_jumps_device, halo_schedule,
-                      mixture_face_coefficients, mixture_face_coefficients_device, pressure_predictor, pressure_solve, pressure_solve_rrule, project,
+                      mixture_boundary_coefficients, mixture_face_coefficients, mixture_face_coefficients_device,
+                      pressure_predictor, pressure_solve, pressure_solve_rrule, project,
                       project_adjoint, project_open, weighted_gradient_boundary,
                       refresh_two_phase, surface_tension, surface_tension_jumps, surface_tension_jumps_from_cells,
                       weighted_gradient,
@@ -23,7 +24,7 @@ __all__ = [
     "LIB_PATH", "LinearSolverOptions", "LinearSolverStats", "MultigridOptions", "PoissonOperator", "ProjectionRun",
     "TPAMR_FP32_REFINE", "TPAMR_FP32_VCYCLE", "TPAMR_FP64", "divergence", "face_dual_volumes", "fill_halo",
     "gravity", "gravity_jumps", "gravity_jumps_device", "halo_schedule", "kinetic_energy", "ldiv", "lib",
-    "mixture_face_coefficients", "mixture_face_coefficients_device",
+    "mixture_boundary_coefficients", "mixture_face_coefficients", "mixture_face_coefficients_device",
     "pressure_predictor", "pressure_solve", "pressure_solve_rrule", "project", "project_adjoint", "refresh_two_phase", "solve",
     "staggered_liquid_fraction", "surface_tension", "surface_tension_jumps", "surface_tension_jumps_from_cells",
     "weighted_gradient",

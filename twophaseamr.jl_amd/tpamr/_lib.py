@@ -127,6 +127,8 @@ _SIGNATURES = {
                                      C.c_int, C.c_void_p, C.c_void_p]),
     "tpamr_level_boundary": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "tpamr_component_means": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_void_p, C.c_void_p]),
+    "tpamr_mixture_boundary_coefficients": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                                      C.c_void_p]),
     "tpamr_level_components": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.c_void_p, C.c_void_p]),
 }
 

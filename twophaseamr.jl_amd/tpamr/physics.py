@@ -39,10 +39,26 @@ def mixture_face_coefficients(solver: EllipticSolver, fraction, density=(1.0, 1.
     return k
 
 
+def mixture_boundary_coefficients(solver: EllipticSolver, fraction, density=(1.0, 1.0)):
+    """`_mixture_boundary_coefficient` (forces.jl:103-111): per boundary face
+    1 / (rho_l c + rho_g (1 - c)) of its cell's fraction."""
+    rho_l, rho_g = (float(v) for v in density)
+    m = solver.operator.mesh
+    c = L.f64(fraction, m.ncells, "fraction")
+    kb = np.zeros(max(m.nboundary_faces, 1))
+    L.check(L.lib().tpamr_mixture_boundary_coefficients(solver.ctx.h, L.ptr(c), rho_l, rho_g, L.ptr(kb)),
+            solver.ctx.h)
+    return kb[:m.nboundary_faces]
+
+
 def refresh_two_phase(solver: EllipticSolver, fraction, density):
-    """`_refresh_pressure_substep!` (src/two_phase/stepping.jl:73-111): new
-    mixture coefficients into the operator and every multigrid level."""
-    solver.operator.replace_face_coefficient(mixture_face_coefficients(solver, fraction, density))
+    """`_refresh_pressure_substep!` (src/two_phase/stepping.jl:73-111) with
+    `PoissonOperator(field; density, template)` (forces.jl:112-140): new
+    mixture coefficients into the operator and every multigrid level, and,
+    with open sides, the boundary faces' mixture coefficients."""
+    k = mixture_face_coefficients(solver, fraction, density)
+    kb = mixture_boundary_coefficients(solver, fraction, density) if any(solver.operator.boundary_sides) else None
+    solver.operator.replace_face_coefficient(k, boundary_face_coefficient=kb)
     solver.refresh()
     return solver
 
