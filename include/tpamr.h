@@ -330,6 +330,61 @@ int tpamr_project_device(tpamr_ctx* ctx, double* d_u, double* d_p, double dt, do
                          const double* d_inherited_divergence, double reference_dt, double* d_phi,
                          tpamr_stats* stats);
 
+/* One-velocity momentum step on a uniform mesh periodic in both directions
+ * (the reference's uniform staggered schedule,
+ * operators/advection/staggered_schedules.jl:226-264,396-458). Other meshes
+ * are rejected with TPAMR_EINVAL. The context's pressure operator is the
+ * step's (face coefficient 1 / rho). Velocities are interior-face vectors.
+ *
+ * tpamr_set_momentum: the momentum parts of NavierStokesSystem /
+ *   NavierStokesWorkspace (navier_stokes/system.jl, workspace.jl): the
+ *   staggered transport workspace (StaggeredVelocityAdvectionWorkspace,
+ *   staggered_schedules.jl:925-934; its x/y sweep order starts forward) and
+ *   the staggered viscous operator StaggeredDiffusionOperator(mesh;
+ *   viscosity) (operators/staggered_diffusion.jl:765-811) with kinematic
+ *   viscosity `viscosity`; scheme 0 Upwind, 1 Centred, 2 LaxWendroff (the
+ *   default of AdvectionOptions, navier_stokes/options.jl:1-35). A new mesh
+ *   (tpamr_set_mesh) drops them. */
+int tpamr_set_momentum(tpamr_ctx* ctx, double viscosity, int scheme);
+/* advect!(field, advecting, dt, workspace) (staggered_update.jl:265-326) in
+ * place on `field`; advecting == field is the one-velocity shorthand
+ * advect!(velocity, dt, workspace). Alternates the sweep order after every
+ * split step with dt != 0. */
+int tpamr_advect(tpamr_ctx* ctx, double* field, const double* advecting, double dt);
+/* diffusion!(out, u, operator) (staggered_diffusion.jl:880-900): nu L u */
+int tpamr_diffusion(tpamr_ctx* ctx, const double* u, double* out);
+/* implicit_midpoint_diffusion!(dst, convected, old, operator, dt, ...)
+ * (navier_stokes/implicit_diffusion.jl:916-965): solves
+ * (I - dt nu L / 2) dst = convected + dt nu L old / 2 per component, from the
+ * initial guess in dst, to the strong residual `tolerance` relative to the
+ * right-hand side (relative_to_rhs, elliptic_solver.jl:550-627); stats (x,
+ * then y component; may be NULL) as LinearSolverStats. The Krylov method is
+ * CG preconditioned by the shifted matrix's diagonal (the reference's
+ * TreeStaggeredGeometricMultigridPreconditioner is not used). Raises when a
+ * component does not converge. */
+int tpamr_implicit_diffusion(tpamr_ctx* ctx, double* dst, const double* convected, const double* old, double dt,
+                             double tolerance, int max_cycles, tpamr_stats stats[2]);
+/* stable_timestep(velocity, advection workspace; cfl)
+ * (staggered_update.jl:426-455) */
+int tpamr_stable_timestep(tpamr_ctx* ctx, const double* u, double cfl, double* dt_out);
+/* step!(velocity, pressure, solver, dt, workspace; force, tolerance,
+ * max_cycles) (single_phase/stepping.jl:194-264) of one phase: the old
+ * velocity checked against the advective bound at cfl 1 (times
+ * 1 + 1e-6 + 32 eps), transported with itself frozen, the body-force face
+ * values `force` (sampled at the midpoint time; NULL: none) added times dt,
+ * the implicit-midpoint viscous substep, the pressure predictor with the old
+ * pressure and the incremental projection (p += phi). `stats` receives the
+ * projection's LinearSolverStats; tpamr_step_viscous_stats those of the
+ * step's viscous solves. */
+int tpamr_step(tpamr_ctx* ctx, double* u, double* p, double dt, const double* force, double tolerance,
+               int max_cycles, tpamr_stats* stats);
+int tpamr_step_device(tpamr_ctx* ctx, double* d_u, double* d_p, double dt, const double* d_force, double tolerance,
+                      int max_cycles, tpamr_stats* stats);
+int tpamr_step_viscous_stats(const tpamr_ctx* ctx, tpamr_stats stats[2]);
+/* The transport's sweep order flag: 1 when the next split step sweeps x
+ * first. */
+int tpamr_momentum_sweep_forward(const tpamr_ctx* ctx, int* forward);
+
 /* Adjoint of one projection step (no reference counterpart). The forward
  * step is tpamr_project from u* = u + dt a (a = the step's force
  * accelerations sqrt(k) K (sqrt(k) jumps)): u' = u* - dt G phi with

@@ -40,3 +40,15 @@ def test_host_only_context_refuses_compute():
     assert rc == -5 and b"host-only" in _lib.lib().tpamr_last_error(ctx.h)
     rc = _lib.lib().tpamr_apply(ctx.h, _lib.ptr(x), _lib.ptr(x))
     assert rc == -5
+
+
+def test_host_only_context_refuses_momentum():
+    m = tpamr.BlockTreeMesh(base_mesh=(16, 16), block_size=(8, 8))
+    ctx = _lib.Context(-1)
+    m.attach(ctx)
+    from tpamr import navier_stokes as ns
+    rc = ns._lib().tpamr_set_momentum(ctx.h, 0.01, 2)
+    assert rc == -5 and b"host-only" in _lib.lib().tpamr_last_error(ctx.h)
+    u = np.zeros(m.nfaces)
+    rc = ns._lib().tpamr_advect(ctx.h, _lib.ptr(u), _lib.ptr(u), 0.1)
+    assert rc == -5

@@ -20,6 +20,7 @@
 #include "../../include/tpamr.h"
 #include "hier.hpp"
 #include "kernels.hpp"
+#include "momentum.hpp"
 
 using namespace tpamr;
 
@@ -245,6 +246,12 @@ struct tpamr_ctx {
   hipEvent_t ev_caller = nullptr;
   // the library's static device scratch on this context's device
   void* noalloc = nullptr;
+  // momentum operators of the one-velocity step (tpamr_set_momentum), the
+  // step's old and transported velocities and a pressure staging vector,
+  // and the last step's viscous solve statistics
+  std::unique_ptr<Momentum> mom;
+  DevArray<double> mold, mconv, mp, mforce;
+  tpamr_stats vstats[2] = {};
   const TreeMesh& mesh() const {
     if (H) return *H->levels[0]->mesh;
     if (mesh_pending) return *mesh_pending;
@@ -270,6 +277,9 @@ int guard(tpamr_ctx* ctx, F&& f) {
     f();
     return TPAMR_OK;
   } catch (const HipError& e) {
+    if (ctx) ctx->err = e.what();
+    return TPAMR_EDEVICE;
+  } catch (const MomentumDeviceError& e) {
     if (ctx) ctx->err = e.what();
     return TPAMR_EDEVICE;
   } catch (const StateError& e) {
@@ -1621,6 +1631,7 @@ void tpamr_destroy(tpamr_ctx* c) {
   if (c->device >= 0) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->mom.reset();
     destroy_graph(c);
     if (c->blas) (void)rocblas_destroy_handle(c->blas);
     c->topo.clear();
@@ -1664,6 +1675,7 @@ int tpamr_set_mesh(tpamr_ctx* c, const int base_mesh[2], const int block_size[2]
     const bool per[2] = {periodic[0] != 0, periodic[1] != 0};
     if (c->device >= 0) HIPCHECK(hipSetDevice(c->device));
     reset_topology_state(c);
+    c->mom.reset();
     if (c->H) c->H_prev = std::move(c->H);
     c->have_coeff = false;
     c->mesh_pending = make_tree_mesh(base_mesh, block_size, origin, extent, per, max_level, keys);
@@ -2882,6 +2894,255 @@ int tpamr_time_smoother(tpamr_ctx* c, int reps, double* avg_ms, double* bytes) {
                          double(L.ext_col.size()) * (4 + elem);
       *bytes = cells + cond + irr;
     }
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// One-velocity momentum step (uniform, fully periodic meshes)
+
+namespace {
+
+Momentum& need_momentum(tpamr_ctx* c) {
+  if (!c->mom) throw StateError("no momentum operators: call tpamr_set_momentum first");
+  return *c->mom;
+}
+
+tpamr_stats viscous_stats(const ViscousStats& v) {
+  tpamr_stats st{};
+  st.cycles = v.cycles;
+  st.converged = v.converged;
+  st.compatible = 1;
+  st.initial_l2 = v.initial_l2;
+  st.initial_linf = v.initial_linf;
+  st.final_l2 = v.final_l2;
+  st.final_linf = v.final_linf;
+  st.target_linf = v.target_linf;
+  st.residual_scale = v.residual_scale;
+  st.krylov_runs = 1;
+  return st;
+}
+
+void momentum_buffers(tpamr_ctx* c) {
+  const TreeMesh& M = c->mesh();
+  c->mold.alloc(M.nfaces());
+  c->mconv.alloc(M.nfaces());
+  c->f1.alloc(M.nfaces());
+  c->mp.alloc(M.ncells());
+}
+
+// the implicit viscous substep; a component that does not converge raises
+// after dst and the statistics are written (implicit_diffusion.jl:953-963)
+void implicit_checked(tpamr_ctx* c, double* dst, const double* conv, const double* old, double dt, double tol,
+                      int max_cycles) {
+  ViscousStats vs[2];
+  c->mom->implicit(dst, conv, old, dt, tol, max_cycles, vs);
+  c->vstats[0] = viscous_stats(vs[0]);
+  c->vstats[1] = viscous_stats(vs[1]);
+  for (int a = 0; a < 2; ++a)
+    if (!vs[a].converged) {
+      char msg[256];
+      std::snprintf(msg, sizeof(msg),
+                    "staggered diffusion solve did not converge (component %d: %d cycles, residual %.3e, target "
+                    "%.3e; dt=%g, max_cycles=%d)",
+                    a + 1, vs[a].cycles, vs[a].final_linf, vs[a].target_linf, dt, max_cycles);
+      throw std::runtime_error(msg);
+    }
+}
+
+// step! on the velocity in c->f1 and the device pressure dp
+tpamr_stats step_core(tpamr_ctx* c, double* dp, double dt, const double* dforce, double tol, int max_cycles) {
+  Momentum& Mo = need_momentum(c);
+  const TreeMesh& M = c->mesh();
+  const int N = M.ncells(), F = M.nfaces();
+  hipStream_t s = c->stream;
+  launch_copy(s, F, c->f1.p, c->mold.p);
+  // _check_transport_timestep (single_phase/stepping.jl:95-103)
+  const double bound = Mo.stable_timestep(c->mold.p, 1.0);
+  if (!(dt <= bound * (1 + 1.0e-6 + 32 * EPS))) {
+    char msg[160];
+    std::snprintf(msg, sizeof(msg), "time step %.17g exceeds advective stability bound %.17g", dt, bound);
+    throw std::invalid_argument(msg);
+  }
+  // transport with the old velocity frozen, body force, viscous substep
+  launch_copy(s, F, c->mold.p, c->mconv.p);
+  Mo.advect(c->mconv.p, c->mold.p, dt);
+  if (dforce) launch_axpby(s, F, dt, dforce, 1.0, c->mconv.p);
+  implicit_checked(c, c->f1.p, c->mconv.p, c->mold.p, dt, tol, max_cycles);
+  // _pressure_predictor! with the old pressure, then the projection
+  launch_face_jump(s, F, c->fm.p, c->fp.p, c->farea.p, c->sqk.p, dp, nullptr, c->f2.p);
+  weighted_metric(c, c->f2.p, c->f3.p);
+  launch_axpby(s, F, -dt, c->f3.p, 1.0, c->f1.p);
+  tpamr_stats st = project_core(c, dt, tol, max_cycles, false, dt);
+  launch_axpby(s, N, 1.0, c->x.p, 1.0, dp);
+  const LevelData& L0 = *c->H->levels[0];
+  if (L0.singular && !L0.connected) gauge_remove<double>(c, 0, dp);
+  return st;
+}
+
+void step_checks(tpamr_ctx* c, double dt, double tol, int max_cycles) {
+  need_device(c);
+  need_solver(c);
+  need_momentum(c);
+  if (!(std::isfinite(dt) && dt > 0)) throw std::invalid_argument("dt must be finite and strictly positive");
+  if (max_cycles < 0) throw std::invalid_argument("max_cycles must be nonnegative");
+  project_checks(dt, tol, dt);
+  projection_sides(c, false);
+}
+
+}  // namespace
+
+extern "C" {
+
+int tpamr_set_momentum(tpamr_ctx* c, double viscosity, int scheme) {
+  if (!c) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    HIPCHECK(hipSetDevice(c->device));
+    c->mom.reset();
+    c->mom.reset(new Momentum(c->mesh(), viscosity, scheme, c->stream));
+    momentum_buffers(c);
+  });
+}
+
+int tpamr_momentum_sweep_forward(const tpamr_ctx* cc, int* forward) {
+  auto* c = const_cast<tpamr_ctx*>(cc);
+  if (!c || !forward) return TPAMR_EINVAL;
+  return guard(c, [&] { *forward = need_momentum(c).sweep_forward() ? 1 : 0; });
+}
+
+int tpamr_advect(tpamr_ctx* c, double* field, const double* advecting, double dt) {
+  if (!c || !field || !advecting) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    Momentum& Mo = need_momentum(c);
+    if (!(dt >= 0)) throw std::invalid_argument("dt must be nonnegative");
+    HIPCHECK(hipSetDevice(c->device));
+    const int F = c->mesh().nfaces();
+    upload_vec(c, c->mconv.p, field, F);
+    const bool self = advecting == field;
+    if (!self) upload_vec(c, c->mold.p, advecting, F);
+    Mo.advect(c->mconv.p, self ? c->mconv.p : c->mold.p, dt);
+    download_vec(c, field, c->mconv.p, F);
+  });
+}
+
+int tpamr_diffusion(tpamr_ctx* c, const double* u, double* out) {
+  if (!c || !u || !out) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    Momentum& Mo = need_momentum(c);
+    HIPCHECK(hipSetDevice(c->device));
+    const int F = c->mesh().nfaces();
+    upload_vec(c, c->mold.p, u, F);
+    Mo.diffusion(c->mold.p, c->mconv.p);
+    download_vec(c, out, c->mconv.p, F);
+  });
+}
+
+int tpamr_implicit_diffusion(tpamr_ctx* c, double* dst, const double* convected, const double* old, double dt,
+                             double tolerance, int max_cycles, tpamr_stats stats[2]) {
+  if (!c || !dst || !convected || !old) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    need_momentum(c);
+    if (!(std::isfinite(dt) && dt > 0)) throw std::invalid_argument("dt must be finite and strictly positive");
+    if (!(std::isfinite(tolerance) && tolerance > 0 && tolerance < 1))
+      throw std::invalid_argument("tolerance must be finite and lie in (0, 1)");
+    if (max_cycles < 0) throw std::invalid_argument("max_cycles must be nonnegative");
+    HIPCHECK(hipSetDevice(c->device));
+    const int F = c->mesh().nfaces();
+    upload_vec(c, c->f1.p, dst, F);
+    upload_vec(c, c->mconv.p, convected, F);
+    upload_vec(c, c->mold.p, old, F);
+    try {
+      implicit_checked(c, c->f1.p, c->mconv.p, c->mold.p, dt, tolerance, max_cycles);
+    } catch (const std::runtime_error& e) {
+      download_vec(c, dst, c->f1.p, F);
+      if (stats) {
+        stats[0] = c->vstats[0];
+        stats[1] = c->vstats[1];
+      }
+      throw std::invalid_argument(e.what());
+    }
+    download_vec(c, dst, c->f1.p, F);
+    if (stats) {
+      stats[0] = c->vstats[0];
+      stats[1] = c->vstats[1];
+    }
+  });
+}
+
+int tpamr_stable_timestep(tpamr_ctx* c, const double* u, double cfl, double* dt_out) {
+  if (!c || !u || !dt_out) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    need_device(c);
+    Momentum& Mo = need_momentum(c);
+    HIPCHECK(hipSetDevice(c->device));
+    upload_vec(c, c->mold.p, u, c->mesh().nfaces());
+    *dt_out = Mo.stable_timestep(c->mold.p, cfl);
+  });
+}
+
+int tpamr_step(tpamr_ctx* c, double* u, double* p, double dt, const double* force, double tolerance, int max_cycles,
+               tpamr_stats* stats) {
+  if (!c || !u || !p) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    step_checks(c, dt, tolerance, max_cycles);
+    HIPCHECK(hipSetDevice(c->device));
+    const TreeMesh& M = c->mesh();
+    const int N = M.ncells(), F = M.nfaces();
+    check_finite(u, F, "velocity");
+    check_finite(p, N, "pressure");
+    upload_vec(c, c->f1.p, u, F);
+    upload_vec(c, c->mp.p, p, N);
+    // the force values have their own staging buffer (f2..f4 are the
+    // projection's scratch)
+    if (force) {
+      check_finite(force, F, "force");
+      c->mforce.alloc(F);
+      upload_vec(c, c->mforce.p, force, F);
+    }
+    tpamr_stats st;
+    try {
+      st = step_core(c, c->mp.p, dt, force ? c->mforce.p : nullptr, tolerance, max_cycles);
+    } catch (const std::runtime_error& e) {
+      throw std::invalid_argument(e.what());
+    }
+    download_vec(c, u, c->f1.p, F);
+    download_vec(c, p, c->mp.p, N);
+    if (stats) *stats = st;
+  });
+}
+
+int tpamr_step_device(tpamr_ctx* c, double* d_u, double* d_p, double dt, const double* d_force, double tolerance,
+                      int max_cycles, tpamr_stats* stats) {
+  if (!c || !d_u || !d_p) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    step_checks(c, dt, tolerance, max_cycles);
+    HIPCHECK(hipSetDevice(c->device));
+    order_after_caller(c);
+    const int F = c->mesh().nfaces();
+    copy_in(c, c->f1.p, d_u, F, false);
+    tpamr_stats st;
+    try {
+      st = step_core(c, d_p, dt, d_force, tolerance, max_cycles);
+    } catch (const std::runtime_error& e) {
+      throw std::invalid_argument(e.what());
+    }
+    copy_out(c, d_u, c->f1.p, F, false);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (stats) *stats = st;
+  });
+}
+
+int tpamr_step_viscous_stats(const tpamr_ctx* cc, tpamr_stats stats[2]) {
+  auto* c = const_cast<tpamr_ctx*>(cc);
+  if (!c || !stats) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    stats[0] = c->vstats[0];
+    stats[1] = c->vstats[1];
   });
 }
 

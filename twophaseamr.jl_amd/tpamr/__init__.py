@@ -11,6 +11,7 @@ from .physics import (CGFM, CSF, divergence, fill_halo, gravity, gravity_jumps, 
                       refresh_two_phase, surface_tension, surface_tension_jumps, surface_tension_jumps_from_cells,
                       weighted_gradient,
                       weighted_jump_force)
+from .navier_stokes import NavierStokesSystem, step, step_device
 from .stepping import DeviceProjectionRun, ProjectionRun, face_dual_volumes, kinetic_energy, staggered_liquid_fraction
 from .solver import (EllipticSolver, FreeSlip, LinearSolverOptions, LinearSolverStats, MultigridOptions, NoSlip,
                      Outflow, PoissonOperator, VelocityBoundaryConditions, ZeroGradientVelocity, ldiv,
@@ -21,7 +22,8 @@ __all__ = [
     "FreeSlip", "NoSlip", "Outflow", "VelocityBoundaryConditions", "ZeroGradientVelocity", "pressure_boundary_sides",
     "project_open", "weighted_gradient_boundary",
     "HostHierarchy",
-    "LIB_PATH", "LinearSolverOptions", "LinearSolverStats", "MultigridOptions", "PoissonOperator", "ProjectionRun",
+    "LIB_PATH", "LinearSolverOptions", "LinearSolverStats", "MultigridOptions", "NavierStokesSystem",
+    "PoissonOperator", "ProjectionRun", "step", "step_device",
     "TPAMR_FP32_REFINE", "TPAMR_FP32_VCYCLE", "TPAMR_FP64", "divergence", "face_dual_volumes", "fill_halo",
     "gravity", "gravity_jumps", "gravity_jumps_device", "halo_schedule", "kinetic_energy", "ldiv", "lib",
     "mixture_boundary_coefficients", "mixture_face_coefficients", "mixture_face_coefficients_device",
