@@ -325,7 +325,7 @@ def trace_summary(db, title, command, top=30):
              "```", command, "```", "", f"Total kernel time: {tot:.2f} ms", "",
              "| kernel | calls | total ms | avg us | min us | max us | max grid |", "|---|---:|---:|---:|---:|---:|---:|"]
     for name, (n, t, lo, hi, g) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
-        short = name.replace("tpamr::", "").split("(")[0]
+        short = name.replace("tpamr::", "").replace("(anonymous namespace)::", "").split("(")[0]
         if len(short) > 90:
             short = short[:87] + "..."
         lines.append(f"| `{short}` | {n} | {t / 1e3:.2f} | {t / n:.2f} | {lo:.2f} | {hi:.2f} | {g} |")
@@ -350,7 +350,8 @@ def trace_summary(db, title, command, top=30):
                       f"the end of one k_ir_update_p to the next, kernels busy {busy / 1e3:.1f} us, "
                       f"{len(seg)} launches, in order:", "", "| us | kernel |", "|---:|---|"]
         for name, s0, e0, _ in seg:
-            lines.append(f"| {(e0 - s0) / 1e3:.1f} | `{name.replace('tpamr::', '').split('(')[0][:80]}` |")
+            short = name.replace("tpamr::", "").replace("(anonymous namespace)::", "").split("(")[0]
+            lines.append(f"| {(e0 - s0) / 1e3:.1f} | `{short[:80]}` |")
     print("\n".join(lines))
 
 
@@ -518,6 +519,84 @@ def wave_main(args):
     print(json.dumps(out), flush=True)
 
 
+def taylor_green_run(n=128, block=16, steps=2, warmup=1, nu=0.02, dt=1e-3, tol=1e-12, oracle_check=True):
+    """The plain-method Taylor-Green configuration: periodic [0, 2 pi]^2, n^2
+    cells in block^2 blocks, density 1, viscosity nu, dt, `steps` timed
+    one-velocity steps (step! with velocity and pressure in HBM) after
+    `warmup` steps from the same initial field. With oracle_check, the
+    oracle runs the same steps on the host (its FP64 restatement: numpy
+    transport and viscous operator, C++ pressure solver) and the velocity
+    and pressure are compared field for field; its wall time per step is the
+    CPU figure."""
+    import torch
+    import tpamr
+    kw = dict(base_mesh=(n, n), block_size=(block, block), extent=(2 * np.pi, 2 * np.pi), periodic=(True, True))
+    mesh = tpamr.BlockTreeMesh(**kw)
+    O = _oracle()
+    om = O.OMesh(**kw)
+    ax = np.asarray(om.face_axis)
+    c = om.face_center
+    u0 = np.where(ax == 0, np.sin(c[:, 0]) * np.cos(c[:, 1]), -np.cos(c[:, 0]) * np.sin(c[:, 1]))
+    system = tpamr.NavierStokesSystem(mesh, density=1.0, dynamic_viscosity=nu)
+
+    def run(k):
+        du = torch.tensor(u0, device="cuda:0")
+        dp = torch.zeros(mesh.ncells, dtype=torch.float64, device="cuda:0")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        st = None
+        for _ in range(k):
+            st = tpamr.step_device(du, dp, system, dt, tolerance=tol)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, du, dp, st
+
+    for _ in range(warmup):
+        run(steps)
+    # a new system starts the transport's sweep order forward, as the
+    # oracle's does
+    system.close()
+    system = tpamr.NavierStokesSystem(mesh, density=1.0, dynamic_viscosity=nu)
+    wall, du, dp, st = run(steps)
+    vx, vy = system.viscous_stats()
+    out = {"cells": mesh.ncells, "block_size": [block, block], "steps": steps, "dt": dt, "viscosity": nu,
+           "ms_per_step": wall / steps * 1e3, "mcells": mesh.ncells * steps / wall / 1e6,
+           "pressure_cycles": st.cycles, "viscous_cycles": [vx.cycles, vy.cycles], "converged": bool(st.converged)}
+    system.close()
+    if oracle_check:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_ns as N
+        model, threads = host_cpu()
+        O.set_threads(threads)
+        S = O.OSolver(om, np.ones(om.nfaces))
+        adv, D = N.StaggeredAdvection(om), N.StaggeredDiffusion(om, nu)
+        uo, po = u0.copy(), np.zeros(om.ncells)
+        t = time.perf_counter()
+        for _ in range(steps):
+            uo, po, _ = N.step(uo, po, S, adv, D, dt, tolerance=tol)
+        cpu = (time.perf_counter() - t) / steps
+        ug, pg = du.cpu().numpy(), dp.cpu().numpy()
+        out["velocity_error_vs_oracle"] = float(np.max(np.abs(ug - uo)) / np.max(np.abs(uo)))
+        out["pressure_error_vs_oracle"] = float(np.max(np.abs(pg - po)) / np.max(np.abs(po)))
+        out["cpu_oracle_ms_per_step"] = cpu * 1e3
+        out["cpu_oracle_threads"] = threads
+        out["cpu_oracle_note"] = ("oracle restatement (numpy transport and viscous operator with an exact sparse "
+                                  "viscous solve, C++ pressure solver on the host's threads), FP64")
+    return out
+
+
+def taylor_green_main(args):
+    n = args.tg_cells
+    res = taylor_green_run(n=n, steps=args.steps, warmup=args.warmup, oracle_check=n <= 512)
+    out = {"metric": "Mcells/s of the one-velocity step (cells x steps / wall time)", "value": res["mcells"],
+           "unit": "Mcells/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"taylor_green_{n}x{n}_periodic", "cells": res["cells"], "viscosity": 0.02,
+                      "dt": 1e-3},
+           "detail": res}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -533,8 +612,10 @@ def main():
     ap.add_argument("--probe", action="store_true", help="short run for rocprofv3 --pmc passes")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DB", "WRITE_DB"))
     ap.add_argument("--trace-summary", nargs=3, metavar=("DB", "TITLE", "COMMAND"))
-    ap.add_argument("--workload", choices=("circles", "wave"), default="circles",
-                    help="circles: the headline solve benchmark; wave: the checkpointed multi-step run")
+    ap.add_argument("--workload", choices=("circles", "wave", "taylor_green"), default="circles",
+                    help="circles: the headline solve benchmark; wave: the checkpointed multi-step run; "
+                         "taylor_green: one-velocity steps on the periodic Taylor-Green field")
+    ap.add_argument("--tg-cells", type=int, default=128, help="taylor_green: cells per side")
     ap.add_argument("--wave-fd", action="store_true", help="wave: one finite-difference check of the gradient")
     ap.add_argument("--uniform", type=int, metavar="N",
                     help="time the finest-level sweep on a uniform N x N mesh of 16x16 blocks (no irregular rows)")
@@ -554,6 +635,9 @@ def main():
         sys.exit(2)
     if args.probe:
         probe(args.levels)
+        return
+    if args.workload == "taylor_green":
+        taylor_green_main(args)
         return
     if args.workload == "wave":
         if "--levels" not in sys.argv:
@@ -653,6 +737,9 @@ def main():
         out["cpu_baseline"] = None
     if args.cpu_full:
         out["cpu_full"] = cpu_full(case, x_sol, args.rtol)
+    # the plain-method configuration's one-velocity step (128^2 cells, two
+    # steps), field for field against the oracle
+    out["taylor_green_step"] = taylor_green_run()
     print(json.dumps(out), flush=True)
 
 

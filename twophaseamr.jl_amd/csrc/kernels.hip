@@ -1091,7 +1091,7 @@ __global__ void k_ref_entries(int nnz, const int* __restrict__ term_ptr, const i
 #pragma clang fp contract(off)
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nnz) return;
-  double v = 0, bv = 0, vp = 0, bvp = 0;
+  double v = 0, vp = 0, bvp = 0;
   for (int q = term_ptr[e]; q < term_ptr[e + 1]; ++q) {
     const int f = term_f[q], g = term_g[q];
     const double kfg = term_k[q], af = area[f], ag = area[g];
@@ -1105,7 +1105,6 @@ __global__ void k_ref_entries(int nnz, const int* __restrict__ term_ptr, const i
       babs = (sf * fabs(kfg)) * sg * af * ag;
     }
     v += term_s[q] * core;
-    bv += babs;
     if (dst[2 * f] < 0) {
       vp += term_s[q] * core;
       bvp += babs;

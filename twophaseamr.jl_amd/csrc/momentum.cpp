@@ -8,7 +8,6 @@
 
 #include <algorithm>
 #include <cmath>
-#include <cstring>
 #include <limits>
 #include <stdexcept>
 #include <string>
@@ -58,34 +57,38 @@ struct Momentum::Impl {
   Buf<double> cx, cy, ox, oy;              // implicit substep: convected, old
   Buf<double> b, r, z, p, Ap;              // viscous Krylov vectors (one component)
   Buf<double> part, scal;
-  Buf<unsigned long long> word;
   double host_scal[12] = {};
 
   void read_scal() {
     MCHECK(hipMemcpyAsync(host_scal, scal.p, sizeof(host_scal), hipMemcpyDeviceToHost, s));
     MCHECK(hipStreamSynchronize(s));
   }
-  // strong residual max, l2, round-off floor and rhs scale of b - M x
-  void residual(const double* bb, double* x, double& linf, double& l2, double& floor, double& bscale) {
+  // the true residual r = b - M x and its statistics
+  struct Res {
+    double linf, l2, floor, bscale;
+  };
+  Res residual(const double* bb, double* x) {
     launch_st_residual(s, g, k, bb, x, r.p, part.p);
     launch_st_reduce(s, st_parts(n), 4, 0b1101, part.p, scal.p, 0);
     read_scal();
-    linf = host_scal[4];
-    l2 = std::sqrt(host_scal[5] / (double(n) * (g.dx * g.dy)));
-    floor = 128.0 * EPS * std::max({1.0, host_scal[6], host_scal[7]});
-    bscale = host_scal[7];
+    Res q;
+    q.linf = host_scal[4];
+    q.l2 = std::sqrt(host_scal[5] / (double(n) * (g.dx * g.dy)));
+    // _roundoff_residual_floor! (cycles.jl:78-101): 128 eps max(1, |M||x| / V, |b| / V)
+    q.floor = 128.0 * EPS * std::max({1.0, host_scal[6], host_scal[7]});
+    q.bscale = host_scal[7];
+    return q;
   }
-  double guarded(const double* bb, double* x, double target, double initial, double current) {
+  // _guarded_roundoff_target! (elliptic_solver.jl:462-485) with the floor of
+  // the current iterate
+  static double guarded(double floor, double target, double initial, double current) {
     if (!(std::isfinite(initial) && std::isfinite(current) && target < current && current < initial)) return target;
-    double linf, l2, fl, bs;
-    residual(bb, x, linf, l2, fl, bs);
-    return (std::isfinite(fl) && fl < initial / 100) ? std::max(target, fl) : target;
+    return (std::isfinite(floor) && floor < initial / 100) ? std::max(target, floor) : target;
   }
-  // one Krylov run (Krylov.jl cg with a diagonal preconditioner) from x
-  // until the strong recursive residual reaches `target` or `itmax`
-  int cg_run(const double* bb, double* x, double target, int itmax) {
-    double linf, l2, fl, bs;
-    residual(bb, x, linf, l2, fl, bs);
+  // one Krylov run (Krylov.jl cg with a diagonal preconditioner) from x,
+  // whose true residual is in r, until the strong recursive residual
+  // reaches `target` or `itmax`
+  int cg_run(double* x, double target, int itmax) {
     launch_st_cg_start(s, n, k, r.p, z.p, p.p, part.p);
     launch_st_reduce(s, st_parts(n), 1, 0, part.p, scal.p, 1);
     read_scal();
@@ -104,35 +107,33 @@ struct Momentum::Impl {
     }
     return it;
   }
+  // _solve_krylov_with_stats! with relative_to_rhs (elliptic_solver.jl:550-627)
   ViscousStats solve(const double* bb, double* x, double tol, int max_cycles) {
     ViscousStats st;
-    double initial_linf, initial_l2, fl, bscale;
-    residual(bb, x, initial_linf, initial_l2, fl, bscale);
-    const double target0 = std::max(tol * bscale, fl);
+    const Res r0 = residual(bb, x);
+    const double target0 = std::max(tol * r0.bscale, r0.floor);
+    Res cur = r0;
     int iterations = 0;
-    if (initial_linf > target0) {
-      // _run_krylov_reliably! (elliptic_solver.jl:487-526)
+    if (r0.linf > target0) {
+      // _run_krylov_reliably! (elliptic_solver.jl:487-526); every run starts
+      // from the true residual of its iterate
       while (iterations < max_cycles) {
-        const int its = cg_run(bb, x, target0, max_cycles - iterations);
+        const int its = cg_run(x, target0, max_cycles - iterations);
         iterations += its;
-        double linf, l2, f2, bs;
-        residual(bb, x, linf, l2, f2, bs);
-        const double eff = guarded(bb, x, target0, initial_linf, linf);
-        if (linf <= eff) break;
-        if (!std::isfinite(linf) || its == 0) break;
+        cur = residual(bb, x);
+        if (cur.linf <= guarded(cur.floor, target0, r0.linf, cur.linf)) break;
+        if (!std::isfinite(cur.linf) || its == 0) break;
       }
     }
-    double final_linf, final_l2, f3, bs3;
-    residual(bb, x, final_linf, final_l2, f3, bs3);
-    const double target = guarded(bb, x, target0, initial_linf, final_linf);
+    const double target = guarded(cur.floor, target0, r0.linf, cur.linf);
     st.cycles = iterations;
-    st.converged = std::isfinite(final_linf) && std::isfinite(target) && final_linf <= target;
-    st.initial_l2 = initial_l2;
-    st.initial_linf = initial_linf;
-    st.final_l2 = final_l2;
-    st.final_linf = final_linf;
+    st.converged = std::isfinite(cur.linf) && std::isfinite(target) && cur.linf <= target;
+    st.initial_l2 = r0.l2;
+    st.initial_linf = r0.linf;
+    st.final_l2 = cur.l2;
+    st.final_linf = cur.linf;
     st.target_linf = target;
-    st.residual_scale = bscale;
+    st.residual_scale = r0.bscale;
     return st;
   }
 };
@@ -206,9 +207,8 @@ Momentum::Momentum(const TreeMesh& m, double viscosity, int scheme, hipStream_t 
   for (Buf<double>* b : {&D.X, &D.Y, &D.U, &D.V, &D.X2, &D.Y2, &D.cx, &D.cy, &D.ox, &D.oy, &D.b, &D.r, &D.z, &D.p,
                          &D.Ap})
     b->alloc(N);
-  D.part.alloc(size_t(st_parts(N)) * 4);
+  D.part.alloc(size_t(st_parts(2 * N)) * 4);
   D.scal.alloc(12);
-  D.word.alloc(1);
   MCHECK(hipStreamSynchronize(stream));
 }
 
@@ -287,13 +287,10 @@ double Momentum::stable_timestep(const double* u, double cfl) {
   Impl& D = *d_;
   if (!(cfl > 0 && cfl <= 1)) throw std::invalid_argument("cfl must lie in (0, 1]");
   launch_st_gather(D.s, D.n, D.fx.p, D.fy.p, u, D.U.p, D.V.p);
-  launch_st_clear(D.s, D.word.p);
-  launch_st_rates(D.s, D.g, D.U.p, D.V.p, D.word.p);
-  unsigned long long bits = 0;
-  MCHECK(hipMemcpyAsync(&bits, D.word.p, sizeof(bits), hipMemcpyDeviceToHost, D.s));
-  MCHECK(hipStreamSynchronize(D.s));
-  double rate;
-  std::memcpy(&rate, &bits, sizeof(rate));
+  launch_st_rates(D.s, D.g, D.U.p, D.V.p, D.part.p);
+  launch_st_reduce(D.s, st_parts(2 * D.n), 1, 1, D.part.p, D.scal.p, 0);
+  D.read_scal();
+  const double rate = D.host_scal[4];
   return rate == 0 ? std::numeric_limits<double>::infinity() : cfl / rate;
 }
 

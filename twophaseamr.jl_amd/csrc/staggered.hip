@@ -15,6 +15,7 @@
 // bit (contraction off).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "staggered.hpp"
@@ -144,34 +145,25 @@ __global__ void k_st_sweep(StagGrid g, int sweep, int scheme, double dt, const d
   (comp ? Yo : Xo)[cell] = Q[cell] - dt * tend;
 }
 
-__global__ void k_st_clear(unsigned long long* w) { *w = 0ull; }
+// rate of an unknown: sum of |dual flux| over its four dual faces / V;
+// per-workgroup maxima (grid-stride over both components' unknowns)
+template <int NS>
+__device__ __forceinline__ void wg_partials(double (&v)[NS], int max_mask, double* __restrict__ part);
 
-// rate of an unknown: sum of |dual flux| over its four dual faces / V
-__global__ void k_st_rates(StagGrid g, const double* __restrict__ U, const double* __restrict__ V,
-                           unsigned long long* __restrict__ bits) {
+__global__ void __launch_bounds__(STAG_WG) k_st_rates(StagGrid g, const double* __restrict__ U,
+                                                      const double* __restrict__ V, double* __restrict__ part) {
 #pragma clang fp contract(off)
-  __shared__ double red[STAG_WG / 64];
   const int n = g.nx * g.ny;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  double rate = 0.0;
-  if (t < 2 * n) {
+  double v[1] = {0.0};
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 2 * n; t += gridDim.x * blockDim.x) {
     const int comp = t >= n ? 1 : 0;
     const int cell = t - comp * n;
     const int i = cell % g.nx, j = cell / g.nx;
     const double a = fabs(dual_flux(g, comp, 1, j, i - 1, U, V)) + fabs(dual_flux(g, comp, 1, j, i, U, V));
     const double b = fabs(dual_flux(g, comp, 2, i, j - 1, U, V)) + fabs(dual_flux(g, comp, 2, i, j, U, V));
-    rate = (a + b) * g.inv_vol;
+    v[0] = fmax(v[0], (a + b) * g.inv_vol);
   }
-  for (int o = 32; o > 0; o >>= 1) rate = fmax(rate, __shfl_xor(rate, o, 64));
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) red[w] = rate;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = red[0];
-    for (int k = 1; k < int(blockDim.x) / 64; ++k) m = fmax(m, red[k]);
-    // non-negative doubles order like their bit patterns
-    atomicMax(bits, static_cast<unsigned long long>(__double_as_longlong(m)));
-  }
+  wg_partials<1>(v, 1, part);
 }
 
 // nu (L q) at one unknown: diagonal, then x neighbours, then y neighbours
@@ -252,24 +244,23 @@ __global__ void __launch_bounds__(STAG_WG) k_st_residual(StagGrid g, StagCoef k,
                                                          double* __restrict__ part) {
 #pragma clang fp contract(off)
   const int n = g.nx * g.ny;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const double vol = g.dx * g.dy;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
-  if (t < n) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
     const int i = t % g.nx, j = t / g.nx;
     const double rr = b[t] - shifted(g, k, x, i, j);
     r[t] = rr;
     const double strong = rr / vol;
-    v[0] = fabs(strong);
-    v[1] = vol * (strong * strong);
+    v[0] = fmax(v[0], fabs(strong));
+    v[1] = v[1] + vol * (strong * strong);
     // round-off bound row: sum |M_ij| |x_j|
     double bound = fabs(k.md) * fabs(x[t]);
     bound = bound + fabs(k.mx) * fabs(x[j * g.nx + wrap(i - 1, g.nx)]);
     bound = bound + fabs(k.mx) * fabs(x[j * g.nx + wrap(i + 1, g.nx)]);
     bound = bound + fabs(k.my) * fabs(x[wrap(j - 1, g.ny) * g.nx + i]);
     bound = bound + fabs(k.my) * fabs(x[wrap(j + 1, g.ny) * g.nx + i]);
-    v[2] = bound / vol;
-    v[3] = fabs(b[t] / vol);
+    v[2] = fmax(v[2], bound / vol);
+    v[3] = fmax(v[3], fabs(b[t] / vol));
   }
   wg_partials<4>(v, 0b1101, part);
 }
@@ -278,13 +269,12 @@ __global__ void __launch_bounds__(STAG_WG) k_st_cg_start(int n, StagCoef k, cons
                                                          double* __restrict__ z, double* __restrict__ p,
                                                          double* __restrict__ part) {
 #pragma clang fp contract(off)
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   double v[1] = {0.0};
-  if (t < n) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
     const double zz = k.mdinv * r[t];
     z[t] = zz;
     p[t] = zz;
-    v[0] = r[t] * zz;
+    v[0] = v[0] + r[t] * zz;
   }
   wg_partials<1>(v, 0, part);
 }
@@ -293,12 +283,11 @@ __global__ void __launch_bounds__(STAG_WG) k_st_matvec_dot(StagGrid g, StagCoef 
                                                            double* __restrict__ Ap, double* __restrict__ part) {
 #pragma clang fp contract(off)
   const int n = g.nx * g.ny;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   double v[1] = {0.0};
-  if (t < n) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
     const double a = shifted(g, k, p, t % g.nx, t / g.nx);
     Ap[t] = a;
-    v[0] = p[t] * a;
+    v[0] = v[0] + p[t] * a;
   }
   wg_partials<1>(v, 0, part);
 }
@@ -310,17 +299,16 @@ __global__ void __launch_bounds__(STAG_WG) k_st_cg_update(int n, StagCoef k, dou
                                                           const double* __restrict__ scal,
                                                           double* __restrict__ part) {
 #pragma clang fp contract(off)
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const double alpha = scal[1];
   double v[2] = {0.0, 0.0};
-  if (t < n) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
     x[t] = x[t] + alpha * p[t];
     const double rr = r[t] - alpha * Ap[t];
     r[t] = rr;
     const double zz = k.mdinv * rr;
     z[t] = zz;
-    v[0] = rr * zz;
-    v[1] = fabs(rr * inv_vol);
+    v[0] = v[0] + rr * zz;
+    v[1] = fmax(v[1], fabs(rr * inv_vol));
   }
   wg_partials<2>(v, 0b10, part);
 }
@@ -376,7 +364,8 @@ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 }  // namespace
 
-int st_parts(int n) { return cdiv(n, STAG_WG); }
+// workgroups of the reduction kernels (grid-stride beyond STAG_MAX_WG)
+int st_parts(int n) { return std::min(cdiv(n, STAG_WG), STAG_MAX_WG); }
 
 void launch_st_gather(hipStream_t s, int n, const int* fx, const int* fy, const double* u, double* X, double* Y) {
   hipLaunchKernelGGL(k_st_gather, dim3(cdiv(n, 256)), dim3(256), 0, s, n, fx, fy, u, X, Y);
@@ -393,14 +382,8 @@ void launch_st_sweep(hipStream_t s, const StagGrid& g, int sweep, int scheme, do
   hipLaunchKernelGGL(k_st_sweep, dim3(cdiv(n2, 256)), dim3(256), 0, s, g, sweep, scheme, dt, Xi, Yi, U, V, Xo, Yo);
 }
 
-void launch_st_clear(hipStream_t s, unsigned long long* word) {
-  hipLaunchKernelGGL(k_st_clear, dim3(1), dim3(1), 0, s, word);
-}
-
-void launch_st_rates(hipStream_t s, const StagGrid& g, const double* U, const double* V,
-                     unsigned long long* rate_bits) {
-  const int n2 = 2 * g.nx * g.ny;
-  hipLaunchKernelGGL(k_st_rates, dim3(cdiv(n2, STAG_WG)), dim3(STAG_WG), 0, s, g, U, V, rate_bits);
+void launch_st_rates(hipStream_t s, const StagGrid& g, const double* U, const double* V, double* part) {
+  hipLaunchKernelGGL(k_st_rates, dim3(st_parts(2 * g.nx * g.ny)), dim3(STAG_WG), 0, s, g, U, V, part);
 }
 
 void launch_st_diffusion(hipStream_t s, const StagGrid& g, const StagCoef& k, double nu, const double* X,

@@ -30,8 +30,10 @@ struct StagCoef {
 // Advection schemes of the staggered transport (staggered_schemes.jl).
 constexpr int STAG_UPWIND = 0, STAG_CENTRED = 1, STAG_LW = 2;
 
-// Unknowns of one component grid handled per workgroup of the reductions.
+// Threads per workgroup of the reduction kernels, and their largest grid
+// (grid-stride loops beyond it keep the partials few).
 constexpr int STAG_WG = 256;
+constexpr int STAG_MAX_WG = 1024;
 
 void launch_st_gather(hipStream_t s, int n, const int* fx, const int* fy, const double* u, double* X, double* Y);
 void launch_st_scatter(hipStream_t s, int n, const int* fx, const int* fy, const double* X, const double* Y,
@@ -43,12 +45,9 @@ void launch_st_scatter(hipStream_t s, int n, const int* fx, const int* fy, const
 // the grids (U, V).
 void launch_st_sweep(hipStream_t s, const StagGrid& g, int sweep, int scheme, double dt, const double* Xi,
                      const double* Yi, const double* U, const double* V, double* Xo, double* Yo);
-// Largest rate sum(|dual flux|) / V over both components, as the bits of a
-// positive double, atomically maximised into *rate_bits (cleared first by
-// launch_st_clear).
-void launch_st_clear(hipStream_t s, unsigned long long* word);
-void launch_st_rates(hipStream_t s, const StagGrid& g, const double* U, const double* V,
-                     unsigned long long* rate_bits);
+// Per-workgroup maxima (st_parts(2 n) partials) of the rate
+// sum(|dual flux|) / V over both components' unknowns.
+void launch_st_rates(hipStream_t s, const StagGrid& g, const double* U, const double* V, double* part);
 // out = nu L q for both component grids.
 void launch_st_diffusion(hipStream_t s, const StagGrid& g, const StagCoef& k, double nu, const double* X,
                          const double* Y, double* oX, double* oY);
