@@ -273,7 +273,7 @@ __global__ void k_cg_update_xr(int n, int nb_cells, const double* __restrict__ p
     }
     const TV rt = TV(rv);
     vb[c] = rt;
-    vx[c] = TV(omega) * dinv[c] * rt;
+    if (vx) vx[c] = TV(omega) * dinv[c] * rt;
     q = fabs(rv) / volume[c / nb_cells];
     if (q != q) q = __longlong_as_double(0x7ff8000000000000LL);
   }
@@ -436,8 +436,10 @@ __global__ void __launch_bounds__(256) k_ir_update_xr(int n, int nb_cells, const
       rv[3] = float((double(ro.w) - al * double(av.w)) - sh);
     }
     st_f4(r + c, rv[0], rv[1], rv[2], rv[3]);
-    const float4 dv = ld_f4(dinv + c);
-    st_f4(vx + c, omega * dv.x * rv[0], omega * dv.y * rv[1], omega * dv.z * rv[2], omega * dv.w * rv[3]);
+    if (vx) {
+      const float4 dv = ld_f4(dinv + c);
+      st_f4(vx + c, omega * dv.x * rv[0], omega * dv.y * rv[1], omega * dv.z * rv[2], omega * dv.w * rv[3]);
+    }
     const float m = fmaxf(fmaxf(fabsf(rv[0]), fabsf(rv[1])), fmaxf(fabsf(rv[2]), fabsf(rv[3])));
     q = double(m) / V;
     if (q != q || rv[0] != rv[0] || rv[1] != rv[1] || rv[2] != rv[2] || rv[3] != rv[3])
@@ -1425,26 +1427,36 @@ static inline unsigned tile_lds(const LevelView<T>& L) {
 
 template <typename T, int BX, int BY>
 static void launch_sweep_t(hipStream_t s, const LevelView<T>& L, const T* xi, T* xo, const T* b, double w,
-                           const int* done, bool fine, const ZStats& zs) {
+                           const int* done, bool fine, const ZStats& zs, const T* x0_dinv, double x0_w) {
   const dim3 g = tile_grid<BX, BY>(L.wg_grid), t = tile_wg<BX, BY>();
   const unsigned sh = tile_lds<T, BX, BY>(L);
   const double* none = nullptr;
   if (zs.part && zs.r32) {
-    hipLaunchKernelGGL((k_sweep<T, BX, BY, true, true, float>), g, t, sh, s, L, xi, xo, b, w, done, zs.r32, zs.part);
+    hipLaunchKernelGGL((k_sweep<T, BX, BY, true, true, float>), g, t, sh, s, L, xi, xo, b, w, done, zs.r32, zs.part,
+                       nullptr, 0.0);
   } else if (zs.part) {
     hipLaunchKernelGGL((k_sweep<T, BX, BY, true, true, double>), g, t, sh, s, L, xi, xo, b, w, done, zs.r64,
-                       zs.part);
+                       zs.part, nullptr, 0.0);
+  } else if (x0_dinv) {
+    if (fine)
+      hipLaunchKernelGGL((k_sweep<T, BX, BY, true, false, double, true>), g, t, sh, s, L, xi, xo, b, w, done, none,
+                         nullptr, x0_dinv, x0_w);
+    else
+      hipLaunchKernelGGL((k_sweep<T, BX, BY, false, false, double, true>), g, t, sh, s, L, xi, xo, b, w, done, none,
+                         nullptr, x0_dinv, x0_w);
   } else if (fine) {
-    hipLaunchKernelGGL((k_sweep<T, BX, BY, true, false>), g, t, sh, s, L, xi, xo, b, w, done, none, nullptr);
+    hipLaunchKernelGGL((k_sweep<T, BX, BY, true, false>), g, t, sh, s, L, xi, xo, b, w, done, none, nullptr, nullptr,
+                       0.0);
   } else {
-    hipLaunchKernelGGL((k_sweep<T, BX, BY, false, false>), g, t, sh, s, L, xi, xo, b, w, done, none, nullptr);
+    hipLaunchKernelGGL((k_sweep<T, BX, BY, false, false>), g, t, sh, s, L, xi, xo, b, w, done, none, nullptr, nullptr,
+                       0.0);
   }
 }
 
 template <typename T>
 void launch_sweep(hipStream_t s, const LevelView<T>& L, const T* xi, T* xo, const T* b, double w, const int* done,
-                  bool fine, const ZStats& zs) {
-  TPAMR_DISPATCH(L.bx, (launch_sweep_t<T, BX, BY>(s, L, xi, xo, b, w, done, fine, zs)));
+                  bool fine, const ZStats& zs, const T* x0_dinv, double x0_w) {
+  TPAMR_DISPATCH(L.bx, (launch_sweep_t<T, BX, BY>(s, L, xi, xo, b, w, done, fine, zs, x0_dinv, x0_w)));
 }
 
 template <typename T>
@@ -1880,9 +1892,9 @@ void launch_fill_int(hipStream_t s, int n, int v, int* out) {
 }
 
 template void launch_sweep<float>(hipStream_t, const LevelView<float>&, const float*, float*, const float*, double,
-                                  const int*, bool, const ZStats&);
+                                  const int*, bool, const ZStats&, const float*, double);
 template void launch_sweep<double>(hipStream_t, const LevelView<double>&, const double*, double*, const double*,
-                                   double, const int*, bool, const ZStats&);
+                                   double, const int*, bool, const ZStats&, const double*, double);
 template void launch_matvec_dot<float>(hipStream_t, const LevelView<float>&, const float*, float*, double*,
                                        const int*, bool);
 template void launch_matvec_dot<double>(hipStream_t, const LevelView<double>&, const double*, double*, double*,

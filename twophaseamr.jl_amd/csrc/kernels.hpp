@@ -89,7 +89,7 @@ bool supported_block(int bx, int by);
 
 template <typename T>
 void launch_sweep(hipStream_t s, const LevelView<T>& L, const T* xi, T* xo, const T* b, double w, const int* done,
-                  bool fine, const ZStats& zs);
+                  bool fine, const ZStats& zs, const T* x0_dinv = nullptr, double x0_w = 0.0);
 template <typename T>
 void launch_residual(hipStream_t s, const LevelView<T>& L, const T* x, const T* b, T* u, double wp, const int* done,
                      bool fine);
@@ -192,6 +192,8 @@ void launch_floor_bound(hipStream_t s, const LevelView<double>& L, const double*
 void launch_reduce(hipStream_t s, int n, int nsets, const double* part, double* out, bool is_max);
 void launch_cg_alpha(hipStream_t s, int nb, const double* part, double* stage, CgState* st, bool sums = false,
                      double total_volume = 1.0);
+// vx = NULL: the zero-start iterate x0 = w D^-1 r is not stored (the
+// V-cycle's first sweep forms it)
 template <typename TV>
 void launch_cg_update_xr(hipStream_t s, int n, int nbc, const double* p, const double* Ap, double* x, double* r,
                          const double* vol, TV* vb, TV* vx, const TV* dinv, double omega, double* part, CgState* st,
@@ -203,7 +205,7 @@ void launch_cg_update_p(hipStream_t s, int n, const TV* z, double* p, const CgSt
 
 // Inner FP32 Krylov iteration of the mixed-precision refinement loop. Four
 // cells per thread; the strong-residual maxima go to `part`, one per
-// workgroup of ir_groups(n).
+// workgroup of ir_groups(n); vx = NULL: x0 is not stored.
 int ir_groups(int n);
 void launch_ir_update_xr(hipStream_t s, int n, int nbc, const float* p, const float* Ap, float* e, float* r,
                          const double* vol, float* vx, const float* dinv, double omega, const double* r64,

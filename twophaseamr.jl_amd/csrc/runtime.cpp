@@ -546,8 +546,11 @@ void upload_hierarchy(tpamr_ctx* c, bool topology) {
 
 // Enqueue one V-cycle. On entry the finest level's b holds the residual and
 // its x holds the zero-start sweep (or zero). Returns the buffer holding z.
+// x0_fused: the finest level's x is not stored; its first pre-sweep forms
+// the zero-start iterate from b (pre >= 2).
 template <typename T>
-T* enqueue_vcycle(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done, const ZStats& zs) {
+T* enqueue_vcycle(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done, const ZStats& zs,
+                  bool x0_fused = false) {
   hipStream_t s = c->stream;
   const int nl = int(lv.size());
   const int pre = c->cfg.pre, post = c->cfg.post;
@@ -572,7 +575,9 @@ T* enqueue_vcycle(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done, c
   for (int l = 0; l < ts; ++l) {
     DevLevel<T>& L = lv[l];
     for (int q = 1; q < pre; ++q) {
-      launch_sweep<T>(s, L.view, cur[l], oth[l], L.b.p, L.omega, done, l == 0, ZStats{});
+      const bool zs0 = x0_fused && l == 0 && q == 1;
+      launch_sweep<T>(s, L.view, cur[l], oth[l], L.b.p, L.omega, done, l == 0, ZStats{}, zs0 ? L.dinv.p : nullptr,
+                      L.omega);
       std::swap(cur[l], oth[l]);
     }
     launch_residual<T>(s, L.view, cur[l], L.b.p, L.u.p, L.omega_p, done, l == 0);
@@ -744,8 +749,10 @@ T* enqueue_vcycle_general(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int*
 }
 
 template <typename T>
-T* enqueue_vcycle_any(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done, const ZStats& zs) {
-  return c->H->general_gauge ? enqueue_vcycle_general<T>(c, lv, done, zs) : enqueue_vcycle<T>(c, lv, done, zs);
+T* enqueue_vcycle_any(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done, const ZStats& zs,
+                      bool x0_fused = false) {
+  return c->H->general_gauge ? enqueue_vcycle_general<T>(c, lv, done, zs)
+                             : enqueue_vcycle<T>(c, lv, done, zs, x0_fused);
 }
 
 // ---------------------------------------------------------------------------
@@ -820,13 +827,18 @@ double guarded_target(tpamr_ctx* c, double target, double initial, double curren
   return (std::isfinite(fl) && fl < initial / 100) ? std::max(target, fl) : target;
 }
 
+// The Krylov iterations leave the finest level's zero-start iterate to the
+// V-cycle's first pre-sweep (it needs a second pre-sweep to fuse into, and
+// the level-by-level V-cycle of several components projects b first).
+bool x0_fused(const tpamr_ctx* c) { return c->cfg.pre >= 2 && !c->H->general_gauge; }
+
 // enqueue z = M r for the Krylov loop: V-cycle input from r, z stats partials
 template <typename T>
-T* enqueue_precondition(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done) {
+T* enqueue_precondition(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, const int* done, bool fused = false) {
   ZStats zs;
   zs.r64 = c->r.p;
   zs.part = c->rs.part;
-  return enqueue_vcycle_any<T>(c, lv, done, zs);
+  return enqueue_vcycle_any<T>(c, lv, done, zs, fused);
 }
 
 template <typename T>
@@ -836,18 +848,20 @@ void enqueue_iteration(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, bool init) {
   const int* done = &c->rs.st->done;
   DevLevel<T>& L0 = lv[0];
   const double w0 = c->cfg.pre > 0 ? L0.omega : 0.0;
+  const bool fused = x0_fused(c);
+  T* x0 = fused ? nullptr : L0.x.p;
   if (init) {
     matvec(c, c->x.p, c->Ap.p);
     launch_copy(s, N, c->b.p, c->r.p);
-    launch_cg_update_xr<T>(s, N, nb, nullptr, c->Ap.p, nullptr, c->r.p, c->topo[0].volume.p, L0.b.p, L0.x.p,
-                           L0.dinv.p, w0, c->rs.maxpart, c->rs.st, 1);
+    launch_cg_update_xr<T>(s, N, nb, nullptr, c->Ap.p, nullptr, c->r.p, c->topo[0].volume.p, L0.b.p, x0, L0.dinv.p,
+                           w0, c->rs.maxpart, c->rs.st, 1);
   } else {
     launch_matvec_dot<double>(s, fine64(c), c->p.p, c->Ap.p, c->rs.part, done);
     launch_cg_alpha(s, nblk, c->rs.part, c->rs.stage, c->rs.st);
-    launch_cg_update_xr<T>(s, N, nb, c->p.p, c->Ap.p, c->x.p, c->r.p, c->topo[0].volume.p, L0.b.p, L0.x.p,
-                           L0.dinv.p, w0, c->rs.maxpart, c->rs.st, 0);
+    launch_cg_update_xr<T>(s, N, nb, c->p.p, c->Ap.p, c->x.p, c->r.p, c->topo[0].volume.p, L0.b.p, x0, L0.dinv.p,
+                           w0, c->rs.maxpart, c->rs.st, 0);
   }
-  T* z = enqueue_precondition<T>(c, lv, done);
+  T* z = enqueue_precondition<T>(c, lv, done, fused);
   launch_cg_beta(s, nblk, ncb(N), c->rs.part, c->rs.maxpart, c->rs.stage, gauge_volume(c), c->rs.st, init ? 1 : 0);
   launch_cg_update_p<T>(s, N, z, c->p.p, c->rs.st, done);
 }
@@ -956,17 +970,20 @@ void enqueue_ir_iteration(tpamr_ctx* c, std::vector<DevLevel<float>>& lv, bool i
     launch_matvec_dot<float>(s, L0.view, c->p32.p, c->Ap32.p, c->rs.part, done, true);
     launch_cg_alpha(s, nblk, c->rs.part, c->rs.stage, c->rs.st, true, gauge_volume(c));
   }
-  launch_ir_update_xr(s, N, nb, c->p32.p, c->Ap32.p, c->e32.p, L0.b.p, c->topo[0].volume.p, L0.x.p, L0.dinv.p, w0,
+  const bool fused = x0_fused(c);
+  launch_ir_update_xr(s, N, nb, c->p32.p, c->Ap32.p, c->e32.p, L0.b.p, c->topo[0].volume.p, fused ? nullptr : L0.x.p,
+                      L0.dinv.p, w0,
                       c->r.p, c->rs.maxpart, c->rs.st, init ? 1 : 0);
   if (!init) {
     launch_ir_correct(s, N, c->e32.p, c->x.p, c->rs.st, false);
-    launch_ir_replace(s, fine64(c), c->x.p, c->b.p, L0.b.p, L0.x.p, L0.dinv.p, w0, c->rs.maxpart, c->rs.st);
+    launch_ir_replace(s, fine64(c), c->x.p, c->b.p, L0.b.p, fused ? nullptr : L0.x.p, L0.dinv.p, w0, c->rs.maxpart,
+                      c->rs.st);
     launch_ir_check(s, ir_groups(N), c->rs.maxpart, c->rs.stage, c->rs.st);
   }
   ZStats zs;
   zs.r32 = L0.b.p;
   zs.part = c->rs.part;
-  float* z = enqueue_vcycle_any<float>(c, lv, done, zs);
+  float* z = enqueue_vcycle_any<float>(c, lv, done, zs, fused);
   launch_cg_beta(s, nblk, ir_groups(N), c->rs.part, c->rs.maxpart, c->rs.stage, gauge_volume(c), c->rs.st,
                  init ? 1 : 0);
   launch_ir_update_p(s, N, z, c->p32.p, c->rs.st, done);

@@ -104,6 +104,23 @@ struct Arr4 {
   __device__ inline T one(int cell) const { return v[cell]; }
 };
 
+// the zero-start Jacobi iterate x0 = (w D^-1) b of a level, formed where it
+// is read (the same products as the Krylov update that would store it)
+template <typename T>
+struct ZeroStart4 {
+  const T* __restrict__ b;
+  const T* __restrict__ dinv;
+  T w;
+  __device__ inline Vec4<T> seg(int cell0) const {
+    const Vec4<T> bv = ld4(b + cell0), dv = ld4(dinv + cell0);
+    Vec4<T> r;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) r.v[m] = (w * dv.v[m]) * bv.v[m];
+    return r;
+  }
+  __device__ inline T one(int cell) const { return (w * dinv[cell]) * b[cell]; }
+};
+
 // injected coarse values t = P_t x_c (identity block or one quadrant)
 template <typename T, int BX, int BY>
 struct Inject4 {
@@ -408,13 +425,13 @@ __device__ inline T block4_max(T v, T* scratch) {
 // Damped Jacobi sweep x_out = x_in + w D^-1 (b - A x_in); STATS: per-block
 // partial sums of V z, r z and r for the Krylov update (r of type TR: the
 // FP64 Krylov residual or the FP32 inner residual).
-template <typename T, int BX, int BY, bool STATS, typename TR>
-__device__ inline void sweep_op(unsigned char* lds, const LevelView<T>& L, const Place<BX, BY>& P,
-                                const T* __restrict__ x_in, T* __restrict__ x_out, const T* __restrict__ rhs,
-                                double omega, const TR* __restrict__ rz, double* __restrict__ part) {
+template <typename T, int BX, int BY, bool STATS, typename TR, class F = Arr4<T>>
+__device__ inline void sweep_op(unsigned char* lds, const LevelView<T>& L, const Place<BX, BY>& P, F x_in,
+                                T* __restrict__ x_out, const T* __restrict__ rhs, double omega,
+                                const TR* __restrict__ rz, double* __restrict__ part) {
   Vec4<T> bv{}, vc, ax, d;
   Vec4<double> rd{};
-  tile4<T, BX, BY, false>(lds, L, P, Arr4<T>{x_in}, [&]() {
+  tile4<T, BX, BY, false>(lds, L, P, x_in, [&]() {
     bv = ld4(rhs + P.cell0);
     if constexpr (STATS) {
       const Vec4<TR> q = ld4(rz + P.cell0);
@@ -540,14 +557,21 @@ __device__ inline void prolong_op(unsigned char* lds, const LevelView<T>& L, con
   if (wg * Geo<BX, BY>::BPW >= (L).nblocks) return;           \
   const Place<BX, BY> P(wg, (L).nblocks, threadIdx.x, 0)
 
-template <typename T, int BX, int BY, bool FINE, bool STATS, typename TR = double>
+// ZS: the sweep starts from the zero-start iterate x0 = (w0 D^-1) rhs,
+// formed from rhs and the level's D^-1 (x_in unused): two damped Jacobi
+// sweeps from zero in one pass.
+template <typename T, int BX, int BY, bool FINE, bool STATS, typename TR = double, bool ZS = false>
 __global__ void __launch_bounds__(256, TILE_WAVES(T))
     k_sweep(LevelView<T> L, const T* __restrict__ x_in, T* __restrict__ x_out, const T* __restrict__ rhs, double omega,
-            const int* __restrict__ done, const TR* __restrict__ rz, double* __restrict__ part) {
+            const int* __restrict__ done, const TR* __restrict__ rz, double* __restrict__ part,
+            const T* __restrict__ dinv0, double w0) {
   if (done && *done) return;
   TILE4_SHARED(T, BX, BY);
   TILE_GROUP(L);
-  sweep_op<T, BX, BY, STATS, TR>(S_, L, P, x_in, x_out, rhs, omega, rz, part);
+  if constexpr (ZS)
+    sweep_op<T, BX, BY, STATS, TR>(S_, L, P, ZeroStart4<T>{rhs, dinv0, T(w0)}, x_out, rhs, omega, rz, part);
+  else
+    sweep_op<T, BX, BY, STATS, TR>(S_, L, P, Arr4<T>{x_in}, x_out, rhs, omega, rz, part);
 }
 
 template <typename T, int BX, int BY, bool FINE>
@@ -643,7 +667,7 @@ __global__ void __launch_bounds__(1024) k_vtail(TailArgs<T> A, const int* __rest
       T* xi = cur(l);
       T* xo = oth(l);
       pass(l, [&](const LevelView<T>& L, const Place<BX, BY>& P) {
-        sweep_op<T, BX, BY, false, double>(S_, L, P, xi, xo, A.b[l], A.omega[l], nullptr, nullptr);
+        sweep_op<T, BX, BY, false, double>(S_, L, P, Arr4<T>{xi}, xo, A.b[l], A.omega[l], nullptr, nullptr);
       });
       par ^= 1u << l;
     }
@@ -672,7 +696,7 @@ __global__ void __launch_bounds__(1024) k_vtail(TailArgs<T> A, const int* __rest
       T* xi = cur(l);
       T* xo = oth(l);
       pass(l, [&](const LevelView<T>& L, const Place<BX, BY>& P) {
-        sweep_op<T, BX, BY, false, double>(S_, L, P, xi, xo, A.b[l], A.omega[l], nullptr, nullptr);
+        sweep_op<T, BX, BY, false, double>(S_, L, P, Arr4<T>{xi}, xo, A.b[l], A.omega[l], nullptr, nullptr);
       });
       par ^= 1u << l;
     }
@@ -715,7 +739,8 @@ __global__ void __launch_bounds__(256, TILE_WAVES(T)) k_matvec_dot(LevelView<T> 
 // FP64 residual replacement of the inner FP32 Krylov iteration (runs when
 // the state requests one): r32 = scale (b - A x) with the FP64 operator, the
 // V-cycle's zero-start iterate x0 = w D^-1 r32 and the workgroup's maximum of
-// |r32| / V (a workgroup holds 1024 cells, the grouping of k_ir_update_xr).
+// |r32| / V (a workgroup holds 1024 cells, the grouping of k_ir_update_xr);
+// x0 = NULL: the zero-start iterate is left to the V-cycle's first sweep.
 template <int BX, int BY>
 __global__ void __launch_bounds__(256) k_ir_replace(LevelView<double> L, const double* __restrict__ x,
                                                    const double* __restrict__ rhs, float* __restrict__ r32,
@@ -730,7 +755,7 @@ __global__ void __launch_bounds__(256) k_ir_replace(LevelView<double> L, const d
   Vec4<float> dv{};
   tile4<double, BX, BY, false>(S_, L, P, Arr4<double>{x}, [&]() {
     bv = ld4(rhs + P.cell0);
-    dv = ld4(dinv + P.cell0);
+    if (x0) dv = ld4(dinv + P.cell0);
   }, vc, ax, d);
   double q = 0;
   if (P.valid) {
@@ -745,7 +770,7 @@ __global__ void __launch_bounds__(256) k_ir_replace(LevelView<double> L, const d
       nan = nan || rv.v[m] != rv.v[m];
     }
     st4(r32 + P.cell0, rv);
-    st4(x0 + P.cell0, xv);
+    if (x0) st4(x0 + P.cell0, xv);
     q /= L.volume[P.b];
     if (nan) q = __longlong_as_double(0x7ff8000000000000LL);
   }
