@@ -400,7 +400,9 @@ __device__ inline void st_f4(float* __restrict__ p, float a, float b, float c, f
 
 // init: r = scale * r64, e = 0. Otherwise e += alpha p and
 // r = (r - alpha Ap) - V rmean, rmean being the exact volume mean of
-// r - alpha Ap from the Ap sums of the matvec (k_cg_alpha).
+// r - alpha Ap from the Ap sums of the matvec (k_cg_alpha); when the state
+// requests a residual replacement and x64 is given, e then moves into the
+// FP64 solution x64 (k_ir_correct's step, folded in).
 // Both write the zero-start V-cycle iterate x0 = w D^-1 r and the per-
 // workgroup maxima of |r| / V.
 __global__ void __launch_bounds__(256) k_ir_update_xr(int n, int nb_cells, const float* __restrict__ p,
@@ -408,7 +410,8 @@ __global__ void __launch_bounds__(256) k_ir_update_xr(int n, int nb_cells, const
                                                       float* __restrict__ r, const double* __restrict__ volume,
                                                       float* __restrict__ vx, const float* __restrict__ dinv,
                                                       float omega, const double* __restrict__ r64,
-                                                      double* __restrict__ part, CgState* st, int init) {
+                                                      double* __restrict__ part, CgState* st, int init,
+                                                      double* __restrict__ x64) {
   __shared__ double red[4];
   if (st->done) return;
   const int c = 4 * (blockIdx.x * 256 + threadIdx.x);
@@ -428,8 +431,24 @@ __global__ void __launch_bounds__(256) k_ir_update_xr(int n, int nb_cells, const
     } else {
       const double al = st->alpha, sh = V * st->rmean;
       const float4 pv = ld_f4(p + c), av = ld_f4(Ap + c), ev = ld_f4(e + c), ro = ld_f4(r + c);
-      st_f4(e + c, float(double(ev.x) + al * double(pv.x)), float(double(ev.y) + al * double(pv.y)),
-            float(double(ev.z) + al * double(pv.z)), float(double(ev.w) + al * double(pv.w)));
+      const float en[4] = {float(double(ev.x) + al * double(pv.x)), float(double(ev.y) + al * double(pv.y)),
+                           float(double(ev.z) + al * double(pv.z)), float(double(ev.w) + al * double(pv.w))};
+      if (x64 && st->replace) {
+        // a residual replacement follows: the correction moves into the
+        // FP64 solution now (x += e / scale, e = 0)
+        const double sc = 1.0 / st->scale;
+        double2 a = *reinterpret_cast<const double2*>(x64 + c);
+        double2 b = *reinterpret_cast<const double2*>(x64 + c + 2);
+        a.x += sc * double(en[0]);
+        a.y += sc * double(en[1]);
+        b.x += sc * double(en[2]);
+        b.y += sc * double(en[3]);
+        *reinterpret_cast<double2*>(x64 + c) = a;
+        *reinterpret_cast<double2*>(x64 + c + 2) = b;
+        st_f4(e + c, 0.f, 0.f, 0.f, 0.f);
+      } else {
+        st_f4(e + c, en[0], en[1], en[2], en[3]);
+      }
       rv[0] = float((double(ro.x) - al * double(av.x)) - sh);
       rv[1] = float((double(ro.y) - al * double(av.y)) - sh);
       rv[2] = float((double(ro.z) - al * double(av.z)) - sh);
@@ -1829,9 +1848,9 @@ int ir_groups(int n) { return cdiv(n, 1024); }
 
 void launch_ir_update_xr(hipStream_t s, int n, int nbc, const float* p, const float* Ap, float* e, float* r,
                          const double* vol, float* vx, const float* dinv, double omega, const double* r64,
-                         double* part, CgState* st, int init) {
+                         double* part, CgState* st, int init, double* x64) {
   hipLaunchKernelGGL(k_ir_update_xr, dim3(ir_groups(n)), dim3(256), 0, s, n, nbc, p, Ap, e, r, vol, vx, dinv,
-                     float(omega), r64, part, st, init);
+                     float(omega), r64, part, st, init, x64);
 }
 
 void launch_ir_replace(hipStream_t s, const LevelView<double>& L, const double* x, const double* b, float* r32,

@@ -207,9 +207,11 @@ void launch_cg_update_p(hipStream_t s, int n, const TV* z, double* p, const CgSt
 // cells per thread; the strong-residual maxima go to `part`, one per
 // workgroup of ir_groups(n); vx = NULL: x0 is not stored.
 int ir_groups(int n);
+// x64 (not NULL): when a residual replacement is requested, the correction
+// e moves into this FP64 solution (x += e / scale, e = 0) in the same pass
 void launch_ir_update_xr(hipStream_t s, int n, int nbc, const float* p, const float* Ap, float* e, float* r,
                          const double* vol, float* vx, const float* dinv, double omega, const double* r64,
-                         double* part, CgState* st, int init);
+                         double* part, CgState* st, int init, double* x64 = nullptr);
 void launch_ir_update_p(hipStream_t s, int n, const float* z, float* p, const CgState* st, const int* done);
 // x += e / scale, e = 0: when a residual replacement is requested, or always
 // with force

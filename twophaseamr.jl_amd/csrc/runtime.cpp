@@ -957,9 +957,9 @@ int run_cg(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double target, int itmax)
 // since the last replacement), r (the finest level's V-cycle rhs), p and Ap
 // are FP32; pAp, the z statistics and every scalar are FP64. init: e = 0,
 // r = scale * c->r. When the previous iteration requested it, the FP32
-// correction moves into the FP64 solution c->x, r is replaced by
-// scale * (b - A x) in FP64 and its strong norm decides the stop before the
-// V-cycle (the three kernels exit at once otherwise).
+// correction moves into the FP64 solution c->x (in the Krylov update), r is
+// replaced by scale * (b - A x) in FP64 and its strong norm decides the stop
+// before the V-cycle (the replacement kernels exit at once otherwise).
 void enqueue_ir_iteration(tpamr_ctx* c, std::vector<DevLevel<float>>& lv, bool init) {
   hipStream_t s = c->stream;
   const int N = c->topo[0].ncells, nb = c->H->levels[0]->mesh->nb(), nblk = c->topo[0].nblocks;
@@ -971,11 +971,11 @@ void enqueue_ir_iteration(tpamr_ctx* c, std::vector<DevLevel<float>>& lv, bool i
     launch_cg_alpha(s, nblk, c->rs.part, c->rs.stage, c->rs.st, true, gauge_volume(c));
   }
   const bool fused = x0_fused(c);
+  // a requested replacement first moves the FP32 correction into the FP64
+  // solution (in the update's pass)
   launch_ir_update_xr(s, N, nb, c->p32.p, c->Ap32.p, c->e32.p, L0.b.p, c->topo[0].volume.p, fused ? nullptr : L0.x.p,
-                      L0.dinv.p, w0,
-                      c->r.p, c->rs.maxpart, c->rs.st, init ? 1 : 0);
+                      L0.dinv.p, w0, c->r.p, c->rs.maxpart, c->rs.st, init ? 1 : 0, init ? nullptr : c->x.p);
   if (!init) {
-    launch_ir_correct(s, N, c->e32.p, c->x.p, c->rs.st, false);
     launch_ir_replace(s, fine64(c), c->x.p, c->b.p, L0.b.p, fused ? nullptr : L0.x.p, L0.dinv.p, w0, c->rs.maxpart,
                       c->rs.st);
     launch_ir_check(s, ir_groups(N), c->rs.maxpart, c->rs.stage, c->rs.st);
