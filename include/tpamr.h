@@ -359,9 +359,11 @@ int tpamr_diffusion(tpamr_ctx* ctx, const double* u, double* out);
  * initial guess in dst, to the strong residual `tolerance` relative to the
  * right-hand side (relative_to_rhs, elliptic_solver.jl:550-627); stats (x,
  * then y component; may be NULL) as LinearSolverStats. The Krylov method is
- * CG preconditioned by the shifted matrix's diagonal (the reference's
- * TreeStaggeredGeometricMultigridPreconditioner is not used). Raises when a
- * component does not converge. */
+ * CG preconditioned by one V-cycle of the staggered tree multigrid
+ * (TreeStaggeredGeometricMultigridPreconditioner with the context's
+ * multigrid options; a coarsest level above 1024 unknowns, which an odd
+ * root layout can leave, falls back to the operator's diagonal). Raises
+ * when a component does not converge. */
 int tpamr_implicit_diffusion(tpamr_ctx* ctx, double* dst, const double* convected, const double* old, double dt,
                              double tolerance, int max_cycles, tpamr_stats stats[2]);
 /* stable_timestep(velocity, advection workspace; cfl)
@@ -384,6 +386,11 @@ int tpamr_step_viscous_stats(const tpamr_ctx* ctx, tpamr_stats stats[2]);
 /* The transport's sweep order flag: 1 when the next split step sweeps x
  * first. */
 int tpamr_momentum_sweep_forward(const tpamr_ctx* ctx, int* forward);
+/* Cells per level of the viscous solve's staggered multigrid
+ * (TreeStaggeredGeometricMultigridPreconditioner's levels,
+ * implicit_diffusion.jl:419-541): *count levels, sizes in `cells` when not
+ * NULL (at most 64 entries). */
+int tpamr_momentum_levels(const tpamr_ctx* ctx, int* count, int* cells);
 
 /* Adjoint of one projection step (no reference counterpart). The forward
  * step is tpamr_project from u* = u + dt a (a = the step's force

@@ -15,7 +15,7 @@ import pytest
 
 import tpamr
 from oracle import OMesh, OSolver
-from oracle_ns import StaggeredAdvection, StaggeredDiffusion, step as ostep
+from oracle_ns import StaggeredAdvection, StaggeredDiffusion, implicit_midpoint_mg, step as ostep
 
 pytestmark = pytest.mark.gpu
 
@@ -124,7 +124,12 @@ def test_implicit_midpoint_diffusion_matches_oracle(name):
     sysg.close()
 
 
-def run_steps(om, pm, u0, steps, dt, nu, tol, force=None):
+def mg_params(name):
+    kw = MESHES[name]
+    return kw["base_mesh"], kw["block_size"], kw.get("max_level", 0)
+
+
+def run_steps(om, pm, u0, steps, dt, nu, tol, force=None, name=None):
     sysg = tpamr.NavierStokesSystem(pm, dynamic_viscosity=nu)
     ug, pg = u0.copy(), np.zeros(pm.ncells)
     S = OSolver(om, np.ones(om.nfaces))
@@ -133,8 +138,10 @@ def run_steps(om, pm, u0, steps, dt, nu, tol, force=None):
     stats = []
     for _ in range(steps):
         st = tpamr.step(ug, pg, sysg, dt, force=force, tolerance=tol)
-        uo, po, so = ostep(uo, po, S, oadv, D, dt, force=force, tolerance=tol)
-        stats.append((st, so, sysg.viscous_stats()))
+        vo = []
+        uo, po, so = ostep(uo, po, S, oadv, D, dt, force=force, tolerance=tol,
+                           mg_params=None if name is None else mg_params(name), viscous_stats=vo)
+        stats.append((st, so, sysg.viscous_stats(), vo[0] if vo else None))
     sysg.close()
     return ug, pg, uo, po, stats
 
@@ -145,11 +152,13 @@ def test_taylor_green_two_steps_field_for_field():
     steps, velocity and pressure against the oracle."""
     om, pm = pair("taylor_green_128")
     nu, dt = 0.02, 1e-3
-    ug, pg, uo, po, stats = run_steps(om, pm, tg(om), 2, dt, nu, 1e-12)
-    for st, so, (vx, vy) in stats:
+    ug, pg, uo, po, stats = run_steps(om, pm, tg(om), 2, dt, nu, 1e-12, name="taylor_green_128")
+    for st, so, (vx, vy), vo in stats:
         assert st.converged and so["converged"]
         assert st.cycles == so["cycles"]
         assert vx.converged and vy.converged
+        # the viscous solves take the reference's multigrid iteration counts
+        assert (vx.cycles, vy.cycles) == (vo[0]["cycles"], vo[1]["cycles"])
     assert rel(ug, uo) < 1e-10
     assert rel(pg, po) < 1e-9
     # the velocity stays discretely divergence-free and follows the decaying
@@ -165,7 +174,7 @@ def test_step_with_body_force_matches_oracle():
                 lambda x, y: -0.5 * np.cos(np.pi * x) * np.sin(2 * np.pi * y))
     f = sample(om, lambda x, y: np.cos(2 * np.pi * y), lambda x, y: 0.3 * np.sin(np.pi * x))
     dt = 0.4 * StaggeredAdvection(om).stable_timestep(u0, cfl=1.0)
-    ug, pg, uo, po, _ = run_steps(om, pm, u0, 3, dt, nu, 1e-12, force=f)
+    ug, pg, uo, po, _ = run_steps(om, pm, u0, 3, dt, nu, 1e-12, force=f, name="roots_8")
     assert rel(ug, uo) < 1e-10 and rel(pg, po) < 1e-9
 
 
@@ -233,3 +242,63 @@ def test_step_rejects_what_the_reference_rejects():
                                 refine=lambda k, o, e: k.i == 0 and k.j == 0)
     with pytest.raises(ValueError, match="uniform"):
         tpamr.NavierStokesSystem(mixed, dynamic_viscosity=0.01)
+
+
+def test_staggered_multigrid_levels():
+    """'staggered base-mesh multigrid' (test/operators/staggered_diffusion.jl:
+    1752-1766), periodic: base meshes (16,16), (8,8), (4,4) with coarse_size
+    16; a single root block keeps one level (:1694-1736); leaf levels are
+    pruned before the base mesh is halved; the Taylor-Green mesh."""
+    cases = [
+        (dict(base_mesh=(16, 16), block_size=(4, 4)), 16, [256, 64, 16]),
+        (dict(base_mesh=(16, 16), block_size=(16, 16)), 64, [256]),
+        (MESHES["refined_4"], 64, [1024, 256, 64]),
+        (MESHES["taylor_green_128"], 64, [16384, 4096, 1024, 256]),
+    ]
+    for kw, coarse, want in cases:
+        pm = tpamr.BlockTreeMesh(**dict(kw, periodic=(True, True)))
+        sysg = tpamr.NavierStokesSystem(pm, dynamic_viscosity=0.2, coarse_size=coarse)
+        assert sysg.viscous_levels == want, (kw, sysg.viscous_levels)
+        sysg.close()
+
+
+@pytest.mark.parametrize("name", ["refined_4", "roots_8", "taylor_green_128"])
+def test_multigrid_viscous_solve_matches_oracle(name):
+    """The viscous solve preconditioned by the staggered multigrid: the same
+    iteration counts per component as the oracle's restatement of the
+    reference's solver, and the same solution (1e-12)."""
+    om, pm = pair(name)
+    nu, dt = 0.5, 0.02
+    Lx, Ly = MESHES[name]["extent"]
+    old = sample(om, lambda x, y: np.sin(2 * np.pi * x / Lx) * np.cos(4 * np.pi * y / Ly),
+                 lambda x, y: np.cos(2 * np.pi * x / Lx))
+    conv = old + 0.05 * np.sin(np.arange(om.nfaces) * 0.37)
+    sysg = tpamr.NavierStokesSystem(pm, dynamic_viscosity=nu)
+    dst = old.copy()
+    sx, sy = sysg.implicit_midpoint_diffusion(dst, conv, old, dt, tolerance=1e-11)
+    want, st = implicit_midpoint_mg(StaggeredDiffusion(om, nu), mg_params(name), conv, old, old, dt, tol=1e-11)
+    assert (sx.cycles, sy.cycles) == (st[0]["cycles"], st[1]["cycles"])
+    assert sx.cycles >= 2
+    assert rel(dst, want) < 1e-12
+    sysg.close()
+
+
+def test_implicit_midpoint_amplification_on_one_root():
+    """'implicit midpoint staggered elliptic solve'
+    (test/operators/staggered_diffusion.jl:1694-1728): a 16^2 root block
+    (one level: the dense coarse solve), viscosity 0.1, dt 0.1, a
+    sin(2 pi x) / cos(2 pi y) field, default tolerance: the discrete
+    amplification factor to 2e-13."""
+    kw = dict(block_size=(16, 16), periodic=(True, True))
+    om, pm = OMesh(**kw), tpamr.BlockTreeMesh(**kw)
+    nu, dt = 0.1, 0.1
+    sysg = tpamr.NavierStokesSystem(pm, dynamic_viscosity=nu)
+    old = sample(om, lambda x, y: np.sin(2 * np.pi * x), lambda x, y: np.cos(2 * np.pi * y))
+    out = np.zeros(om.nfaces)
+    sx, sy = sysg.implicit_midpoint_diffusion(out, old, old, dt)
+    dx = 1.0 / 16
+    lam = -4 * np.sin(np.pi / 16) ** 2 / dx ** 2
+    g = (1 + dt * nu * lam / 2) / (1 - dt * nu * lam / 2)
+    assert np.max(np.abs(out - g * old)) <= 2e-13 + 2e-13 * np.max(np.abs(g * old))
+    assert sx.converged and sy.converged and sysg.viscous_levels == [256]
+    sysg.close()

@@ -44,6 +44,16 @@ const double EPS = std::numeric_limits<double>::epsilon();
 
 }  // namespace
 
+// One level of the viscous solve's staggered multigrid: an nx x ny periodic
+// component grid, its operator, Jacobi weight and vectors.
+struct StagLevel {
+  int nx = 0, ny = 0;
+  double hx = 0, hy = 0;
+  StagCoef k{};
+  double w = 1;
+  Buf<double> x, xs, b, r;
+};
+
 struct Momentum::Impl {
   hipStream_t s = nullptr;
   StagGrid g{};
@@ -52,12 +62,145 @@ struct Momentum::Impl {
   int scheme = STAG_LW;
   bool forward = true;
   int n = 0, nfaces = 0;
+  MultigridConfig mg;
   Buf<int> fx, fy;
   Buf<double> X, Y, U, V, X2, Y2;          // transported grids, advecting grids, stage outputs
   Buf<double> cx, cy, ox, oy;              // implicit substep: convected, old
   Buf<double> b, r, z, p, Ap;              // viscous Krylov vectors (one component)
   Buf<double> part, scal;
   double host_scal[12] = {};
+  // staggered multigrid: levels (level 0 shares the Krylov grid size) and
+  // the dense inverse of the coarsest operator for the current shift
+  std::vector<std::unique_ptr<StagLevel>> lv;
+  Buf<double> coarse_inv;
+  double coarse_half = -1;
+  int comp = 0;
+  bool use_mg = true;
+
+  // level operators for the shift h = dt nu / 2: the finest as the Krylov
+  // operator V (I - h L), the coarser ones h K + V
+  // (_refresh_staggered_component!, implicit_diffusion.jl:213-261); Jacobi
+  // weights (_jacobi_weight!, hierarchy.jl:8-30); dense coarse inverse
+  void set_shift(double half) {
+    for (size_t l = 0; l < lv.size(); ++l) {
+      StagLevel& L = *lv[l];
+      const double hx = L.hx, hy = L.hy, vol = hx * hy;
+      const double kx = ((hy * hx) * (1.0 / hx)) * (1.0 / hx);
+      const double ky = ((hx * hy) * (1.0 / hy)) * (1.0 / hy);
+      const double sd = ((kx + kx) + ky) + ky;
+      if (l == 0) {
+        const double iv = 1.0 / vol;
+        const double ld = -iv * sd, lx = iv * kx, ly = iv * ky;
+        L.k.md = vol * (1.0 - half * ld);
+        L.k.mx = vol * (0.0 - half * lx);
+        L.k.my = vol * (0.0 - half * ly);
+      } else {
+        L.k.md = half * sd + vol;
+        L.k.mx = half * -kx;
+        L.k.my = half * -ky;
+      }
+      L.k.mdinv = 1.0 / L.k.md;
+      const double rowsum = (((std::fabs(L.k.md) + std::fabs(L.k.mx)) + std::fabs(L.k.mx)) + std::fabs(L.k.my)) +
+                            std::fabs(L.k.my);
+      const double bound = rowsum / L.k.md;
+      L.w = std::min(1.0, 4.0 / (3.0 * bound));
+    }
+    k.md = lv[0]->k.md;
+    k.mx = lv[0]->k.mx;
+    k.my = lv[0]->k.my;
+    k.mdinv = lv[0]->k.mdinv;
+    coarse_half = half;
+    if (!use_mg) return;
+    // dense inverse of the coarsest operator (Gauss-Jordan, partial pivoting)
+    const StagLevel& C = *lv.back();
+    const int nc = C.nx * C.ny;
+    std::vector<double> a(size_t(nc) * nc, 0.0), inv(size_t(nc) * nc, 0.0);
+    for (int j = 0; j < C.ny; ++j)
+      for (int i = 0; i < C.nx; ++i) {
+        const int r0 = j * C.nx + i;
+        auto add = [&](int ii, int jj, double v) {
+          const int c0 = ((jj + C.ny) % C.ny) * C.nx + ((ii + C.nx) % C.nx);
+          a[size_t(r0) * nc + c0] += v;
+        };
+        add(i, j, C.k.md);
+        add(i - 1, j, C.k.mx);
+        add(i + 1, j, C.k.mx);
+        add(i, j - 1, C.k.my);
+        add(i, j + 1, C.k.my);
+      }
+    for (int q = 0; q < nc; ++q) inv[size_t(q) * nc + q] = 1.0;
+    for (int col = 0; col < nc; ++col) {
+      int piv = col;
+      for (int q = col + 1; q < nc; ++q)
+        if (std::fabs(a[size_t(q) * nc + col]) > std::fabs(a[size_t(piv) * nc + col])) piv = q;
+      if (a[size_t(piv) * nc + col] == 0.0) throw std::invalid_argument("singular coarsest viscous operator");
+      if (piv != col)
+        for (int c2 = 0; c2 < nc; ++c2) {
+          std::swap(a[size_t(piv) * nc + c2], a[size_t(col) * nc + c2]);
+          std::swap(inv[size_t(piv) * nc + c2], inv[size_t(col) * nc + c2]);
+        }
+      const double d = 1.0 / a[size_t(col) * nc + col];
+      for (int c2 = 0; c2 < nc; ++c2) {
+        a[size_t(col) * nc + c2] *= d;
+        inv[size_t(col) * nc + c2] *= d;
+      }
+      for (int q = 0; q < nc; ++q) {
+        if (q == col) continue;
+        const double f = a[size_t(q) * nc + col];
+        if (f == 0.0) continue;
+        for (int c2 = 0; c2 < nc; ++c2) {
+          a[size_t(q) * nc + c2] -= f * a[size_t(col) * nc + c2];
+          inv[size_t(q) * nc + c2] -= f * inv[size_t(col) * nc + c2];
+        }
+      }
+    }
+    coarse_inv.alloc(inv.size());
+    MCHECK(hipMemcpyAsync(coarse_inv.p, inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    MCHECK(hipStreamSynchronize(s));
+  }
+
+  // z = V-cycle(rhs) of component `comp` (_vcycle!, cycles.jl:40-77): every
+  // level ping-pongs between its own two buffers; the finest result is
+  // copied to z
+  void vcycle(const double* rhs, double* z) {
+    const int nl = int(lv.size());
+    if (nl == 1) {
+      launch_st_dense_solve(s, n, coarse_inv.p, rhs, z);
+      return;
+    }
+    std::vector<double*> cur(nl), oth(nl);
+    std::vector<const double*> bl(nl);
+    for (int l = 0; l < nl; ++l) {
+      cur[l] = lv[l]->x.p;
+      oth[l] = lv[l]->xs.p;
+      bl[l] = l ? lv[l]->b.p : rhs;
+    }
+    for (int l = 0; l + 1 < nl; ++l) {
+      StagLevel& L = *lv[l];
+      // the first sweep starts from zero (x = 0 + w D^-1 (b - 0))
+      for (int q = 0; q < mg.pre; ++q) {
+        launch_st_mg_smooth(s, L.nx, L.ny, L.k, L.w, q == 0 ? nullptr : cur[l], bl[l], oth[l]);
+        std::swap(cur[l], oth[l]);
+      }
+      if (mg.pre == 0) launch_st_mg_smooth(s, L.nx, L.ny, L.k, 0.0, nullptr, bl[l], cur[l]);
+      launch_st_mg_residual(s, L.nx, L.ny, L.k, bl[l], cur[l], L.r.p);
+      launch_st_mg_restrict(s, lv[l + 1]->nx, lv[l + 1]->ny, comp, L.r.p, lv[l + 1]->b.p);
+    }
+    const StagLevel& C = *lv.back();
+    launch_st_dense_solve(s, C.nx * C.ny, coarse_inv.p, bl[nl - 1], cur[nl - 1]);
+    for (int l = nl - 2; l >= 0; --l) {
+      StagLevel& L = *lv[l];
+      launch_st_mg_prolong(s, lv[l + 1]->nx, lv[l + 1]->ny, comp, cur[l + 1], cur[l]);
+      for (int q = 0; q < mg.post; ++q) {
+        launch_st_mg_smooth(s, L.nx, L.ny, L.k, L.w, cur[l], bl[l], oth[l]);
+        std::swap(cur[l], oth[l]);
+      }
+    }
+    launch_copy_grid(cur[0], z);
+  }
+  void launch_copy_grid(const double* from, double* to) {
+    MCHECK(hipMemcpyAsync(to, from, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  }
 
   void read_scal() {
     MCHECK(hipMemcpyAsync(host_scal, scal.p, sizeof(host_scal), hipMemcpyDeviceToHost, s));
@@ -85,24 +228,35 @@ struct Momentum::Impl {
     if (!(std::isfinite(initial) && std::isfinite(current) && target < current && current < initial)) return target;
     return (std::isfinite(floor) && floor < initial / 100) ? std::max(target, floor) : target;
   }
-  // one Krylov run (Krylov.jl cg with a diagonal preconditioner) from x,
-  // whose true residual is in r, until the strong recursive residual
-  // reaches `target` or `itmax`
+  // one Krylov run (Krylov.jl cg preconditioned by the staggered V-cycle)
+  // from x, whose true residual is in r, until the strong recursive
+  // residual reaches `target` or `itmax`
+  void precondition(const double* rr, double* zz) {
+    if (use_mg)
+      vcycle(rr, zz);
+    else
+      launch_st_mg_smooth(s, g.nx, g.ny, k, 1.0, nullptr, rr, zz);
+  }
   int cg_run(double* x, double target, int itmax) {
-    launch_st_cg_start(s, n, k, r.p, z.p, p.p, part.p);
+    precondition(r.p, z.p);
+    launch_st_dot(s, n, r.p, z.p, part.p);
     launch_st_reduce(s, st_parts(n), 1, 0, part.p, scal.p, 1);
+    launch_copy_grid(z.p, p.p);
     read_scal();
     if (host_scal[0] == 0.0) return 0;
     int it = 0;
     while (it < itmax) {
       launch_st_matvec_dot(s, g, k, p.p, Ap.p, part.p);
       launch_st_reduce(s, st_parts(n), 1, 0, part.p, scal.p, 2);
-      launch_st_cg_update(s, n, k, g.inv_vol, x, r.p, z.p, p.p, Ap.p, scal.p, part.p);
-      launch_st_reduce(s, st_parts(n), 2, 0b10, part.p, scal.p, 3);
+      launch_st_cg_update_xr(s, n, g.inv_vol, x, r.p, p.p, Ap.p, scal.p, part.p);
+      launch_st_reduce(s, st_parts(n), 1, 1, part.p, scal.p, 0);
       read_scal();
       ++it;
-      const double strong = host_scal[3];
+      const double strong = host_scal[4];
       if (!std::isfinite(host_scal[1]) || strong <= target || !std::isfinite(strong)) break;
+      precondition(r.p, z.p);
+      launch_st_dot(s, n, r.p, z.p, part.p);
+      launch_st_reduce(s, st_parts(n), 1, 0, part.p, scal.p, 4);
       launch_st_cg_direction(s, n, z.p, p.p, scal.p);
     }
     return it;
@@ -138,7 +292,8 @@ struct Momentum::Impl {
   }
 };
 
-Momentum::Momentum(const TreeMesh& m, double viscosity, int scheme, hipStream_t stream) : d_(new Impl) {
+Momentum::Momentum(const TreeMesh& m, double viscosity, int scheme, const MultigridConfig& mg, hipStream_t stream)
+    : d_(new Impl) {
   Impl& D = *d_;
   if (!(std::isfinite(viscosity) && viscosity > 0))
     throw std::invalid_argument("constant viscosity must be strictly positive");
@@ -209,7 +364,49 @@ Momentum::Momentum(const TreeMesh& m, double viscosity, int scheme, hipStream_t 
     b->alloc(N);
   D.part.alloc(size_t(st_parts(2 * N)) * 4);
   D.scal.alloc(12);
+  // the staggered tree-coarsening plan of a uniform periodic mesh
+  // (_tree_next_coarse_mesh! with _TreeCoarseningPlan{:staggered},
+  // tree_coarsening.jl:145-201): prune leaf levels, then halve the base mesh
+  // while the root layout is even and every periodic axis keeps four base
+  // cells, down to coarse_size unknowns, max_levels, or no progress
+  D.mg = mg;
+  int base0 = m.base[0], base1 = m.base[1], lev = level;
+  auto add_level = [&](int nxl, int nyl, double hxl, double hyl) {
+    std::unique_ptr<StagLevel> L(new StagLevel);
+    L->nx = nxl;
+    L->ny = nyl;
+    L->hx = hxl;
+    L->hy = hyl;
+    const size_t nl = size_t(nxl) * nyl;
+    L->x.alloc(nl);
+    L->xs.alloc(nl);
+    L->r.alloc(nl);
+    if (!D.lv.empty()) L->b.alloc(nl);
+    D.lv.push_back(std::move(L));
+  };
+  add_level(nx, ny, hx, hy);
+  while (int64_t(D.lv.back()->nx) * D.lv.back()->ny > mg.coarse_size && int(D.lv.size()) < mg.max_levels) {
+    if (lev == 0) {
+      if ((base0 / BX) % 2 || (base1 / BY) % 2) break;
+      if (base0 / 2 < 4 || base1 / 2 < 4) break;
+      base0 /= 2;
+      base1 /= 2;
+    } else {
+      --lev;
+    }
+    const StagLevel& F = *D.lv.back();
+    add_level(F.nx / 2, F.ny / 2, 2 * F.hx, 2 * F.hy);
+  }
+  // the coarsest level's dense inverse is formed on the host; above this
+  // size the Krylov solve takes the operator's diagonal as preconditioner
+  D.use_mg = int64_t(D.lv.back()->nx) * D.lv.back()->ny <= STAG_DENSE_MAX;
   MCHECK(hipStreamSynchronize(stream));
+}
+
+std::vector<int> Momentum::level_cells() const {
+  std::vector<int> out;
+  for (const auto& L : d_->lv) out.push_back(L->nx * L->ny);
+  return out;
 }
 
 Momentum::~Momentum() = default;
@@ -261,21 +458,14 @@ void Momentum::implicit(double* dst, const double* convected, const double* old,
   Impl& D = *d_;
   if (!(dt > 0)) throw std::invalid_argument("dt must be strictly positive");
   if (max_cycles < 0) throw std::invalid_argument("max_cycles must be nonnegative");
-  // the shifted matrix h K + V, h = dt nu / 2 (_update_staggered_elliptic!
-  // with _staggered_stable_matrix, reaction 1)
+  // the shifted operators for h = dt nu / 2 (_update_staggered_elliptic!)
   const double half = dt * D.nu / 2;
-  const double hx = D.g.dx, hy = D.g.dy;
-  const double kx = ((hy * hx) * (1.0 / hx)) * (1.0 / hx);
-  const double ky = ((hx * hy) * (1.0 / hy)) * (1.0 / hy);
-  const double sd = ((kx + kx) + ky) + ky;
-  D.k.md = half * sd + hx * hy;
-  D.k.mx = half * -kx;
-  D.k.my = half * -ky;
-  D.k.mdinv = 1.0 / D.k.md;
+  if (half != D.coarse_half) D.set_shift(half);
   launch_st_gather(D.s, D.n, D.fx.p, D.fy.p, convected, D.cx.p, D.cy.p);
   launch_st_gather(D.s, D.n, D.fx.p, D.fy.p, old, D.ox.p, D.oy.p);
   launch_st_gather(D.s, D.n, D.fx.p, D.fy.p, dst, D.X.p, D.Y.p);
   for (int a = 0; a < 2; ++a) {
+    D.comp = a;
     launch_st_midpoint_rhs(D.s, D.g, D.k, D.nu, dt, a ? D.cy.p : D.cx.p, a ? D.oy.p : D.ox.p, D.b.p);
     stats[a] = D.solve(D.b.p, a ? D.Y.p : D.X.p, tol, max_cycles);
   }

@@ -9,6 +9,7 @@
 
 #include <memory>
 #include <stdexcept>
+#include <vector>
 
 #include "hier.hpp"
 
@@ -28,8 +29,9 @@ struct ViscousStats {
 class Momentum {
  public:
   // Throws std::invalid_argument unless the mesh is uniform and periodic in
-  // both directions (the reference's _uniform_periodic_cell_faces).
-  Momentum(const TreeMesh& mesh, double viscosity, int scheme, hipStream_t stream);
+  // both directions (the reference's _uniform_periodic_cell_faces). The
+  // viscous solve's staggered multigrid takes the multigrid options.
+  Momentum(const TreeMesh& mesh, double viscosity, int scheme, const MultigridConfig& mg, hipStream_t stream);
   ~Momentum();
   Momentum(const Momentum&) = delete;
   Momentum& operator=(const Momentum&) = delete;
@@ -49,6 +51,8 @@ class Momentum {
                 int max_cycles, ViscousStats stats[2]);
   // stable_timestep(velocity, workspace; cfl) (staggered_update.jl:426-455)
   double stable_timestep(const double* u, double cfl);
+  // cells per level of the viscous solve's staggered hierarchy
+  std::vector<int> level_cells() const;
 
  private:
   struct Impl;

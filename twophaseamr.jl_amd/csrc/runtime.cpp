@@ -3018,7 +3018,7 @@ int tpamr_set_momentum(tpamr_ctx* c, double viscosity, int scheme) {
     need_device(c);
     HIPCHECK(hipSetDevice(c->device));
     c->mom.reset();
-    c->mom.reset(new Momentum(c->mesh(), viscosity, scheme, c->stream));
+    c->mom.reset(new Momentum(c->mesh(), viscosity, scheme, c->cfg, c->stream));
     momentum_buffers(c);
   });
 }
@@ -3027,6 +3027,17 @@ int tpamr_momentum_sweep_forward(const tpamr_ctx* cc, int* forward) {
   auto* c = const_cast<tpamr_ctx*>(cc);
   if (!c || !forward) return TPAMR_EINVAL;
   return guard(c, [&] { *forward = need_momentum(c).sweep_forward() ? 1 : 0; });
+}
+
+int tpamr_momentum_levels(const tpamr_ctx* cc, int* count, int* cells) {
+  auto* c = const_cast<tpamr_ctx*>(cc);
+  if (!c || !count) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    const std::vector<int> v = need_momentum(c).level_cells();
+    *count = int(v.size());
+    if (cells)
+      for (size_t q = 0; q < v.size() && q < 64; ++q) cells[q] = v[q];
+  });
 }
 
 int tpamr_advect(tpamr_ctx* c, double* field, const double* advecting, double dt) {

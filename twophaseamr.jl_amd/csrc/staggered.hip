@@ -356,10 +356,169 @@ __global__ void __launch_bounds__(STAG_WG) k_st_reduce(int nparts, int nsets, in
       scal[2] = s0 / scal[0];
       scal[0] = s0;
       scal[3] = red[1][0];
+    } else if (step == 4) {
+      scal[2] = s0 / scal[0];
+      scal[0] = s0;
     }
   }
 }
 
+// ---------------------------------------------------------------------------
+// Staggered tree multigrid (one component grid per level)
+
+__device__ __forceinline__ double mg_apply(int nx, int ny, const StagCoef& k, const double* __restrict__ Q, int i,
+                                           int j) {
+#pragma clang fp contract(off)
+  double acc = k.md * Q[j * nx + i];
+  acc = acc + k.mx * Q[j * nx + wrap(i - 1, nx)];
+  acc = acc + k.mx * Q[j * nx + wrap(i + 1, nx)];
+  acc = acc + k.my * Q[wrap(j - 1, ny) * nx + i];
+  acc = acc + k.my * Q[wrap(j + 1, ny) * nx + i];
+  return acc;
+}
+
+__global__ void k_st_mg_smooth(int nx, int ny, StagCoef k, double w, const double* __restrict__ x_in,
+                               const double* __restrict__ b, double* __restrict__ x_out) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nx * ny) return;
+  if (!x_in) {
+    x_out[t] = (w * k.mdinv) * b[t];
+    return;
+  }
+  const double mx = mg_apply(nx, ny, k, x_in, t % nx, t / nx);
+  x_out[t] = x_in[t] + (w * k.mdinv) * (b[t] - mx);
+}
+
+__global__ void k_st_mg_residual(int nx, int ny, StagCoef k, const double* __restrict__ b,
+                                 const double* __restrict__ x, double* __restrict__ r) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nx * ny) return;
+  r[t] = b[t] - mg_apply(nx, ny, k, x, t % nx, t / nx);
+}
+
+// coarse unknown (ic, jc) of component 0 gathers the fine x-faces of the two
+// fine rows 2 jc, 2 jc + 1: the face on its line (weight 1) and the mid-line
+// faces on either side (weight 1/2); component 1 likewise along columns
+__global__ void k_st_mg_restrict(int ncx, int ncy, int comp, const double* __restrict__ rf,
+                                 double* __restrict__ bc) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncx * ncy) return;
+  const int ic = t % ncx, jc = t / ncx;
+  const int nfx = 2 * ncx, nfy = 2 * ncy;
+  double acc = 0.0;
+  for (int s = 0; s < 2; ++s) {
+    if (comp == 0) {
+      const int jf = 2 * jc + s, i0 = 2 * ic;
+      acc = acc + rf[jf * nfx + i0];
+      acc = acc + 0.5 * rf[jf * nfx + i0 + 1];
+      acc = acc + 0.5 * rf[jf * nfx + wrap(i0 - 1, nfx)];
+    } else {
+      const int ifc = 2 * ic + s, j0 = 2 * jc;
+      acc = acc + rf[j0 * nfx + ifc];
+      acc = acc + 0.5 * rf[(j0 + 1) * nfx + ifc];
+      acc = acc + 0.5 * rf[wrap(j0 - 1, nfy) * nfx + ifc];
+    }
+  }
+  bc[t] = acc;
+}
+
+__global__ void k_st_mg_prolong(int ncx, int ncy, int comp, const double* __restrict__ xc,
+                                double* __restrict__ xf) {
+#pragma clang fp contract(off)
+  const int nfx = 2 * ncx, nfy = 2 * ncy;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nfx * nfy) return;
+  const int i = t % nfx, j = t / nfx;
+  double v;
+  if (comp == 0) {
+    const int jc = j >> 1;
+    if ((i & 1) == 0) {
+      v = xc[jc * ncx + (i >> 1)];
+    } else {
+      const int lo = i >> 1;
+      v = 0.5 * xc[jc * ncx + lo] + 0.5 * xc[jc * ncx + wrap(lo + 1, ncx)];
+    }
+  } else {
+    const int ic = i >> 1;
+    if ((j & 1) == 0) {
+      v = xc[(j >> 1) * ncx + ic];
+    } else {
+      const int lo = j >> 1;
+      v = 0.5 * xc[lo * ncx + ic] + 0.5 * xc[wrap(lo + 1, ncy) * ncx + ic];
+    }
+  }
+  xf[t] = xf[t] + v;
+}
+
+// one wavefront per row of the dense inverse
+__global__ void __launch_bounds__(256) k_st_dense_solve(int n, const double* __restrict__ inv,
+                                                        const double* __restrict__ b, double* __restrict__ x) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= n) return;
+  double acc = 0.0;
+  for (int c = lane; c < n; c += 64) acc += inv[size_t(row) * n + c] * b[c];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (lane == 0) x[row] = acc;
+}
+
+__global__ void __launch_bounds__(STAG_WG) k_st_cg_update_xr(int n, double inv_vol, double* __restrict__ x,
+                                                             double* __restrict__ r, const double* __restrict__ p,
+                                                             const double* __restrict__ Ap,
+                                                             const double* __restrict__ scal,
+                                                             double* __restrict__ part) {
+#pragma clang fp contract(off)
+  const double alpha = scal[1];
+  double v[1] = {0.0};
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    x[t] = x[t] + alpha * p[t];
+    const double rr = r[t] - alpha * Ap[t];
+    r[t] = rr;
+    v[0] = fmax(v[0], fabs(rr * inv_vol));
+  }
+  wg_partials<1>(v, 1, part);
+}
+
+__global__ void __launch_bounds__(STAG_WG) k_st_dot(int n, const double* __restrict__ r, const double* __restrict__ z,
+                                                    double* __restrict__ part) {
+#pragma clang fp contract(off)
+  double v[1] = {0.0};
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) v[0] = v[0] + r[t] * z[t];
+  wg_partials<1>(v, 0, part);
+}
+
+inline int cdiv2(int a, int b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+void launch_st_mg_smooth(hipStream_t s, int nx, int ny, const StagCoef& k, double w, const double* x_in,
+                         const double* b, double* x_out) {
+  hipLaunchKernelGGL(k_st_mg_smooth, dim3(cdiv2(nx * ny, 256)), dim3(256), 0, s, nx, ny, k, w, x_in, b, x_out);
+}
+void launch_st_mg_residual(hipStream_t s, int nx, int ny, const StagCoef& k, const double* b, const double* x,
+                           double* r) {
+  hipLaunchKernelGGL(k_st_mg_residual, dim3(cdiv2(nx * ny, 256)), dim3(256), 0, s, nx, ny, k, b, x, r);
+}
+void launch_st_mg_restrict(hipStream_t s, int ncx, int ncy, int comp, const double* r_f, double* b_c) {
+  hipLaunchKernelGGL(k_st_mg_restrict, dim3(cdiv2(ncx * ncy, 256)), dim3(256), 0, s, ncx, ncy, comp, r_f, b_c);
+}
+void launch_st_mg_prolong(hipStream_t s, int ncx, int ncy, int comp, const double* x_c, double* x_f) {
+  hipLaunchKernelGGL(k_st_mg_prolong, dim3(cdiv2(4 * ncx * ncy, 256)), dim3(256), 0, s, ncx, ncy, comp, x_c, x_f);
+}
+void launch_st_dense_solve(hipStream_t s, int n, const double* inv, const double* b, double* x) {
+  hipLaunchKernelGGL(k_st_dense_solve, dim3(cdiv2(n, 4)), dim3(256), 0, s, n, inv, b, x);
+}
+void launch_st_cg_update_xr(hipStream_t s, int n, double inv_vol, double* x, double* r, const double* p,
+                            const double* Ap, const double* scal, double* part) {
+  hipLaunchKernelGGL(k_st_cg_update_xr, dim3(st_parts(n)), dim3(STAG_WG), 0, s, n, inv_vol, x, r, p, Ap, scal, part);
+}
+void launch_st_dot(hipStream_t s, int n, const double* r, const double* z, double* part) {
+  hipLaunchKernelGGL(k_st_dot, dim3(st_parts(n)), dim3(STAG_WG), 0, s, n, r, z, part);
+}
+
+namespace {
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 }  // namespace

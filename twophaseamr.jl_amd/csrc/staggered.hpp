@@ -34,6 +34,9 @@ constexpr int STAG_UPWIND = 0, STAG_CENTRED = 1, STAG_LW = 2;
 // (grid-stride loops beyond it keep the partials few).
 constexpr int STAG_WG = 256;
 constexpr int STAG_MAX_WG = 1024;
+// Largest coarsest level of the viscous staggered multigrid (its dense
+// inverse is formed on the host).
+constexpr int STAG_DENSE_MAX = 1024;
 
 void launch_st_gather(hipStream_t s, int n, const int* fx, const int* fy, const double* u, double* X, double* Y);
 void launch_st_scatter(hipStream_t s, int n, const int* fx, const int* fy, const double* X, const double* Y,
@@ -75,9 +78,32 @@ void launch_st_cg_direction(hipStream_t s, int n, const double* z, double* p, co
 // of part i at part[i * nsets + k]); set k is a maximum when bit k of
 // max_mask is set, else a sum. Then the Krylov scalar step `step`: 0 none;
 // 1 gamma = s0; 2 alpha = gamma / s0 (s0 = pAp); 3 beta = s0 / gamma,
-// gamma = s0, strong = s1.
+// gamma = s0, strong = s1; 4 beta = s0 / gamma, gamma = s0.
 // scal: [0] gamma, [1] alpha, [2] beta, [3] strong, [4..] the reduced sets.
 void launch_st_reduce(hipStream_t s, int nparts, int nsets, int max_mask, const double* part, double* scal, int step);
 int st_parts(int n);
+
+// Staggered tree multigrid of the viscous solve (one component grid per
+// level; levels are nested periodic grids, the coarse one half as fine per
+// axis). Level operator M = diagonal md, x-neighbours mx, y-neighbours my,
+// inverse diagonal mdinv.
+// x_out = x_in + (w D^-1)(b - M x_in); x_in = NULL: the zero start (w D^-1) b
+void launch_st_mg_smooth(hipStream_t s, int nx, int ny, const StagCoef& k, double w, const double* x_in,
+                         const double* b, double* x_out);
+// r = b - M x
+void launch_st_mg_residual(hipStream_t s, int nx, int ny, const StagCoef& k, const double* b, const double* x,
+                           double* r);
+// b_c = P' r_f and x_f += P x_c for component `comp` (the face prolongation:
+// fine faces on coarse lines take the coarse face, mid-line faces the mean
+// of the two bracketing coarse faces; two fine faces along each coarse face)
+void launch_st_mg_restrict(hipStream_t s, int ncx, int ncy, int comp, const double* r_f, double* b_c);
+void launch_st_mg_prolong(hipStream_t s, int ncx, int ncy, int comp, const double* x_c, double* x_f);
+// x = inv b with a dense n x n row-major inverse
+void launch_st_dense_solve(hipStream_t s, int n, const double* inv, const double* b, double* x);
+// x += alpha p, r -= alpha Ap; partial maxima of |r| / V (no preconditioning)
+void launch_st_cg_update_xr(hipStream_t s, int n, double inv_vol, double* x, double* r, const double* p,
+                            const double* Ap, const double* scal, double* part);
+// partial sums of r z
+void launch_st_dot(hipStream_t s, int n, const double* r, const double* z, double* part);
 
 }  // namespace tpamr

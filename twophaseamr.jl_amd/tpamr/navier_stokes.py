@@ -45,6 +45,7 @@ _SIGS = {
     "tpamr_step_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_double,
                                     C.c_int, C.POINTER(L.Stats)]),
     "tpamr_step_viscous_stats": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "tpamr_momentum_levels": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_void_p]),
 }
 
 
@@ -64,7 +65,7 @@ def _viscous_stats(st: L.Stats) -> LinearSolverStats:
                              initial_l2=st.initial_l2, initial_linf=st.initial_linf, final_l2=st.final_l2,
                              final_linf=st.final_linf, target_linf=st.target_linf,
                              residual_scale=st.residual_scale, removed_rhs_mean=0.0, method="cg",
-                             preconditioner="jacobi", krylov_runs=st.krylov_runs)
+                             preconditioner="tree_staggered_geometric_multigrid", krylov_runs=st.krylov_runs)
 
 
 class NavierStokesSystem:
@@ -147,6 +148,14 @@ class NavierStokesSystem:
         out = C.c_double()
         L.check(_lib().tpamr_stable_timestep(self.ctx.h, L.ptr(u), float(cfl), C.byref(out)), self.ctx.h)
         return out.value
+
+    @property
+    def viscous_levels(self):
+        """Cells per level of the viscous solve's staggered multigrid."""
+        n = C.c_int()
+        cells = np.zeros(64, np.int32)
+        L.check(_lib().tpamr_momentum_levels(self.ctx.h, C.byref(n), L.ptr(cells)), self.ctx.h)
+        return [int(v) for v in cells[:n.value]]
 
     def viscous_stats(self):
         st = (L.Stats * 2)()
