@@ -2,8 +2,9 @@
 // periodic block mesh: the structured component grids, the transport
 // stages, the viscous operator's coefficients and the Krylov driver of the
 // implicit-midpoint viscous solve (the reference's _solve_krylov_with_stats!
-// with relative_to_rhs, elliptic_solver.jl:550-627, around PCG with the
-// shifted matrix's diagonal).
+// with relative_to_rhs, elliptic_solver.jl:550-627, around CG preconditioned
+// by one V-cycle of the staggered tree multigrid,
+// TreeStaggeredGeometricMultigridPreconditioner).
 #include "momentum.hpp"
 
 #include <algorithm>
