@@ -392,6 +392,63 @@ int tpamr_momentum_sweep_forward(const tpamr_ctx* ctx, int* forward);
  * NULL (at most 64 entries). */
 int tpamr_momentum_levels(const tpamr_ctx* ctx, int* count, int* cells);
 
+/* CMDGFM reduced-pressure/jump block solve (the two-velocity projection's
+ * linear solve, src/two_phase/projection/cmdgfm_projection.jl). */
+
+/* The solve fields of CMDGFMProjectionStats (cmdgfm_projection.jl:330-360):
+ * converged means both the pressure-volume and the jump-row defects are
+ * within the tolerance (the post-projection mixture and normal-velocity
+ * defects need the interface geometry and are the caller's). */
+typedef struct {
+  int converged, iterations, krylov_runs, initial_guess_used;
+  int pressure_unknowns, jump_unknowns;
+  double initial_residual, final_residual;
+  double pressure_defect, jump_defect, anchor_defect; /* dt^2-scaled defects */
+  double asymmetry, jump_diagonal_margin;
+} tpamr_cmdgfm_stats;
+
+/* The assembled system of _assemble_cmdgfm_system!
+ * (cmdgfm_projection.jl:1484-1523) on the context's mesh, whose pressure
+ * operator (tpamr_set_face_coefficients) is the block's shared hierarchy
+ * (assembled.shared_multigrid). CSR of the (np + nx)^2 matrix, columns
+ * increasing per row: unknowns 0..np-1 are reduced pressures (cell
+ * pressure_cells[i] minus anchor_cells[i], anchors not retained), np.. the
+ * mixed-face jumps. pressure_volumes: the retained cells' volumes;
+ * jump_lengths: _projection_face_spacing of each mixed face; the omitted
+ * (anchor and isolated) rows of the full system over the same unknowns with
+ * their rhs and volumes (for _cmdgfm_anchor_residual_defect, nomitted may
+ * be 0). Forms the jump Schur complement J - C D^-1 B (_cmdgfm_jump_schur!,
+ * cmdgfm_cache.jl:550-610) and its Crout ILU with drop_tolerance
+ * (incomplete_lu_preconditioner, solvers/multigrid/cycles.jl:390-396; the
+ * reference's default 1e-3, _CMDGFM_BLOCK_DEFAULTS). Works on a host-only
+ * context (factor inspection); a new mesh drops the system. */
+int tpamr_cmdgfm_set_system(tpamr_ctx* ctx, int np, int nx, const int* rowptr, const int* cols, const double* vals,
+                            const int* pressure_cells, const int* anchor_cells, const double* pressure_volumes,
+                            const double* jump_lengths, int nomitted, const int* omitted_rowptr,
+                            const int* omitted_cols, const double* omitted_vals, const double* omitted_rhs,
+                            const double* omitted_volumes, double drop_tolerance);
+/* The jump factor of the current system: sizes = {nx, nnz(S), nnz(L),
+ * nnz(U off the diagonal)}; S = J - C D^-1 B, L strictly lower (unit
+ * diagonal), U strictly upper plus udiag, all CSR by rows. Arrays may be
+ * NULL (sizes only). */
+int tpamr_cmdgfm_jump_factor(const tpamr_ctx* ctx, int* sizes, int* sptr, int* scol, double* sval, int* lptr,
+                             int* lcol, double* lval, int* uptr, int* ucol, double* uval, double* udiag);
+/* The solve of _project_cmdgfm_assembled! (cmdgfm_projection.jl:
+ * 2201-2318): Krylov scaling by the mesh's largest extent, `density` (the
+ * phases' largest) and dt; residual replacement around Krylov.jl's
+ * bicgstab! right-preconditioned by the CMDGFM block preconditioner
+ * (preconditioned = 0: none), one unpreconditioned retry after a nonfinite
+ * increment, at most max_iterations BiCGSTAB iterations in all; warm_start
+ * tries the zero-pressure and the previous-increment guesses
+ * (_cmdgfm_initial_solution!, previous_increment: a cell vector or NULL).
+ * solution (np + nx) receives the increment. */
+int tpamr_cmdgfm_solve(tpamr_ctx* ctx, const double* rhs, double* solution, double dt, double density,
+                       double tolerance, int max_iterations, int warm_start, const double* previous_increment,
+                       int preconditioned, tpamr_cmdgfm_stats* stats);
+int tpamr_cmdgfm_solve_device(tpamr_ctx* ctx, const double* d_rhs, double* d_solution, double dt, double density,
+                              double tolerance, int max_iterations, int warm_start, const double* d_previous,
+                              int preconditioned, tpamr_cmdgfm_stats* stats);
+
 /* Adjoint of one projection step (no reference counterpart). The forward
  * step is tpamr_project from u* = u + dt a (a = the step's force
  * accelerations sqrt(k) K (sqrt(k) jumps)): u' = u* - dt G phi with

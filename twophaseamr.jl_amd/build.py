@@ -24,8 +24,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
 # packed FP32 instructions stay disabled in device code
 DEVICE_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                 "-munsafe-fp-atomics"]
-SOURCES = ["hier.cpp", "kernels.hip", "staggered.hip", "momentum.cpp", "runtime.cpp"]
-HEADERS = ["hier.hpp", "kernels.hpp", "tile.hpp", "staggered.hpp", "momentum.hpp"]
+SOURCES = ["hier.cpp", "kernels.hip", "staggered.hip", "momentum.cpp", "cmdgfm.hip", "cmdgfm.cpp", "runtime.cpp"]
+HEADERS = ["hier.hpp", "kernels.hpp", "tile.hpp", "staggered.hpp", "momentum.hpp", "cmdgfm.hpp"]
 
 
 def _newer(target, deps):

@@ -20,6 +20,7 @@
 #include "../../include/tpamr.h"
 #include "hier.hpp"
 #include "kernels.hpp"
+#include "cmdgfm.hpp"
 #include "momentum.hpp"
 
 using namespace tpamr;
@@ -252,6 +253,10 @@ struct tpamr_ctx {
   std::unique_ptr<Momentum> mom;
   DevArray<double> mold, mconv, mp, mforce;
   tpamr_stats vstats[2] = {};
+  // assembled CMDGFM block system (tpamr_cmdgfm_set_system) and the device
+  // staging of its host-array solve
+  std::unique_ptr<CmdgfmSystem> cmd;
+  DevArray<double> cm_rhs, cm_x, cm_prev;
   const TreeMesh& mesh() const {
     if (H) return *H->levels[0]->mesh;
     if (mesh_pending) return *mesh_pending;
@@ -280,6 +285,9 @@ int guard(tpamr_ctx* ctx, F&& f) {
     if (ctx) ctx->err = e.what();
     return TPAMR_EDEVICE;
   } catch (const MomentumDeviceError& e) {
+    if (ctx) ctx->err = e.what();
+    return TPAMR_EDEVICE;
+  } catch (const CmdgfmDeviceError& e) {
     if (ctx) ctx->err = e.what();
     return TPAMR_EDEVICE;
   } catch (const StateError& e) {
@@ -1168,9 +1176,11 @@ tpamr_stats solve_dispatch(tpamr_ctx* c, double rtol, double atol, int max_cycle
   return solve_impl<double>(c, c->ld, rtol, atol, max_cycles, compat);
 }
 
+// z = ldiv!(P, r) of the V-cycle preconditioner on device vectors
+// (integrated r, rewritten with its own values; z with zero volume mean
+// where the operator is singular), enqueued on the context stream
 template <typename T>
-void vcycle_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv) {
-  // r in c->r (integrated); z into c->tmp with zero volume mean
+void vcycle_into(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double* r, double* zout) {
   hipStream_t s = c->stream;
   const int N = c->topo[0].ncells, nb = c->H->levels[0]->mesh->nb();
   CgState init{};
@@ -1180,15 +1190,21 @@ void vcycle_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv) {
   DevLevel<T>& L0 = lv[0];
   const double w0 = c->cfg.pre > 0 ? L0.omega : 0.0;
   launch_fill(s, N, 0.0, c->Ap.p);
-  launch_cg_update_xr<T>(s, N, nb, nullptr, c->Ap.p, nullptr, c->r.p, c->topo[0].volume.p, L0.b.p, L0.x.p,
+  launch_cg_update_xr<T>(s, N, nb, nullptr, c->Ap.p, nullptr, r, c->topo[0].volume.p, L0.b.p, L0.x.p,
                          L0.dinv.p, w0, c->rs.maxpart, c->rs.st, 1);
   T* z = enqueue_vcycle_any<T>(c, lv, &c->rs.st->done, ZStats{});
   if constexpr (std::is_same<T, float>::value)
-    launch_cast_f2d(s, N, z, c->tmp.p);
+    launch_cast_f2d(s, N, z, zout);
   else
-    launch_copy(s, N, z, c->tmp.p);
-  remove_mean(c, c->tmp.p);
-  HIPCHECK(hipStreamSynchronize(s));
+    launch_copy(s, N, z, zout);
+  remove_mean(c, zout);
+}
+
+// r in c->r (integrated); z into c->tmp
+template <typename T>
+void vcycle_impl(tpamr_ctx* c, std::vector<DevLevel<T>>& lv) {
+  vcycle_into<T>(c, lv, c->r.p, c->tmp.p);
+  HIPCHECK(hipStreamSynchronize(c->stream));
 }
 
 size_t upload_vec(tpamr_ctx* c, double* dst, const double* src, size_t n) {
@@ -1649,6 +1665,7 @@ void tpamr_destroy(tpamr_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->mom.reset();
+    c->cmd.reset();
     destroy_graph(c);
     if (c->blas) (void)rocblas_destroy_handle(c->blas);
     c->topo.clear();
@@ -1693,6 +1710,7 @@ int tpamr_set_mesh(tpamr_ctx* c, const int base_mesh[2], const int block_size[2]
     if (c->device >= 0) HIPCHECK(hipSetDevice(c->device));
     reset_topology_state(c);
     c->mom.reset();
+    c->cmd.reset();
     if (c->H) c->H_prev = std::move(c->H);
     c->have_coeff = false;
     c->mesh_pending = make_tree_mesh(base_mesh, block_size, origin, extent, per, max_level, keys);
@@ -3171,6 +3189,153 @@ int tpamr_step_viscous_stats(const tpamr_ctx* cc, tpamr_stats stats[2]) {
   return guard(c, [&] {
     stats[0] = c->vstats[0];
     stats[1] = c->vstats[1];
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// CMDGFM reduced-pressure/jump block solve on the context's pressure
+// hierarchy (cmdgfm.hpp)
+
+namespace {
+
+CmdgfmSystem& need_cmdgfm(tpamr_ctx* c) {
+  if (!c->cmd) throw StateError("no block system: call tpamr_cmdgfm_set_system first");
+  return *c->cmd;
+}
+
+// the shared hierarchy's V-cycle on full-cell device vectors
+CmdgfmSystem::Vcycle cmdgfm_vcycle(tpamr_ctx* c) {
+  return [c](double* r, double* z) {
+    if (c->fp32())
+      vcycle_into<float>(c, c->lf, r, z);
+    else
+      vcycle_into<double>(c, c->ld, r, z);
+  };
+}
+
+void cmdgfm_checks(tpamr_ctx* c, CmdgfmSystem& sys) {
+  need_device(c);
+  if (sys.pressure_size() > 0) need_solver(c);
+}
+
+void cmdgfm_run(tpamr_ctx* c, CmdgfmSystem& sys, const double* d_rhs, double* d_x, double dt, double density,
+                double tolerance, int max_iterations, int warm_start, const double* d_prev, int preconditioned,
+                tpamr_cmdgfm_stats* stats) {
+  const TreeMesh& m = c->mesh();
+  const double h = std::max(m.extent[0], m.extent[1]);
+  CmdgfmStats st;
+  sys.solve(d_rhs, d_x, dt, density, h, tolerance, max_iterations, warm_start != 0, d_prev, preconditioned != 0,
+            cmdgfm_vcycle(c), st);
+  if (stats) {
+    stats->converged = st.converged;
+    stats->iterations = st.iterations;
+    stats->krylov_runs = st.krylov_runs;
+    stats->initial_guess_used = st.initial_guess_used;
+    stats->pressure_unknowns = st.pressure_unknowns;
+    stats->jump_unknowns = st.jump_unknowns;
+    stats->initial_residual = st.initial_residual;
+    stats->final_residual = st.final_residual;
+    stats->pressure_defect = st.pressure_defect;
+    stats->jump_defect = st.jump_defect;
+    stats->anchor_defect = st.anchor_defect;
+    stats->asymmetry = st.asymmetry;
+    stats->jump_diagonal_margin = st.jump_diagonal_margin;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int tpamr_cmdgfm_set_system(tpamr_ctx* c, int np, int nx, const int* rowptr, const int* cols, const double* vals,
+                            const int* pressure_cells, const int* anchor_cells, const double* pressure_volumes,
+                            const double* jump_lengths, int nomitted, const int* omitted_rowptr,
+                            const int* omitted_cols, const double* omitted_vals, const double* omitted_rhs,
+                            const double* omitted_volumes, double drop_tolerance) {
+  if (!c) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    const int ncells = c->mesh().ncells();
+    if (c->device >= 0) HIPCHECK(hipSetDevice(c->device));
+    c->cmd.reset();
+    c->cmd.reset(new CmdgfmSystem(ncells, np, nx, rowptr, cols, vals, pressure_cells, anchor_cells,
+                                  pressure_volumes, jump_lengths, nomitted, omitted_rowptr, omitted_cols,
+                                  omitted_vals, omitted_rhs, omitted_volumes, drop_tolerance, c->device >= 0,
+                                  c->stream));
+  });
+}
+
+int tpamr_cmdgfm_jump_factor(const tpamr_ctx* cc, int* sizes, int* sptr, int* scol, double* sval, int* lptr,
+                             int* lcol, double* lval, int* uptr, int* ucol, double* uval, double* udiag) {
+  auto* c = const_cast<tpamr_ctx*>(cc);
+  if (!c || !sizes) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    const JumpFactor& f = need_cmdgfm(c).factor();
+    const int nx = need_cmdgfm(c).jump_size();
+    sizes[0] = nx;
+    sizes[1] = nx ? int(f.scol.size()) : 0;
+    sizes[2] = nx ? int(f.lcol.size()) : 0;
+    sizes[3] = nx ? int(f.ucol.size()) : 0;
+    if (!nx) return;
+    auto put_i = [](int* dst, const std::vector<int>& v) {
+      if (dst) std::copy(v.begin(), v.end(), dst);
+    };
+    auto put_d = [](double* dst, const std::vector<double>& v) {
+      if (dst) std::copy(v.begin(), v.end(), dst);
+    };
+    put_i(sptr, f.sptr);
+    put_i(scol, f.scol);
+    put_d(sval, f.sval);
+    put_i(lptr, f.lptr);
+    put_i(lcol, f.lcol);
+    put_d(lval, f.lval);
+    put_i(uptr, f.uptr);
+    put_i(ucol, f.ucol);
+    put_d(uval, f.uval);
+    put_d(udiag, f.udiag);
+  });
+}
+
+int tpamr_cmdgfm_solve(tpamr_ctx* c, const double* rhs, double* solution, double dt, double density,
+                       double tolerance, int max_iterations, int warm_start, const double* previous_increment,
+                       int preconditioned, tpamr_cmdgfm_stats* stats) {
+  if (!c || !solution) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    CmdgfmSystem& sys = need_cmdgfm(c);
+    cmdgfm_checks(c, sys);
+    const int n = sys.size();
+    if (n > 0 && !rhs) throw std::invalid_argument("null rhs");
+    HIPCHECK(hipSetDevice(c->device));
+    const int N = c->mesh().ncells();
+    c->cm_rhs.alloc(std::max(n, 1));
+    c->cm_x.alloc(std::max(n, 1));
+    if (n > 0) upload_vec(c, c->cm_rhs.p, rhs, n);
+    const double* prev = nullptr;
+    if (previous_increment) {
+      c->cm_prev.alloc(N);
+      upload_vec(c, c->cm_prev.p, previous_increment, N);
+      prev = c->cm_prev.p;
+    }
+    cmdgfm_run(c, sys, c->cm_rhs.p, c->cm_x.p, dt, density, tolerance, max_iterations, warm_start, prev,
+               preconditioned, stats);
+    if (n > 0) download_vec(c, solution, c->cm_x.p, n);
+  });
+}
+
+int tpamr_cmdgfm_solve_device(tpamr_ctx* c, const double* d_rhs, double* d_solution, double dt, double density,
+                              double tolerance, int max_iterations, int warm_start, const double* d_previous,
+                              int preconditioned, tpamr_cmdgfm_stats* stats) {
+  if (!c || !d_solution) return TPAMR_EINVAL;
+  return guard(c, [&] {
+    CmdgfmSystem& sys = need_cmdgfm(c);
+    cmdgfm_checks(c, sys);
+    if (sys.size() > 0 && !d_rhs) throw std::invalid_argument("null rhs");
+    HIPCHECK(hipSetDevice(c->device));
+    order_after_caller(c);
+    cmdgfm_run(c, sys, d_rhs, d_solution, dt, density, tolerance, max_iterations, warm_start, d_previous,
+               preconditioned, stats);
+    HIPCHECK(hipStreamSynchronize(c->stream));
   });
 }
 

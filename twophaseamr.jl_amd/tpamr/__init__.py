@@ -11,6 +11,7 @@ from .physics import (CGFM, CSF, divergence, fill_halo, gravity, gravity_jumps, 
                       refresh_two_phase, surface_tension, surface_tension_jumps, surface_tension_jumps_from_cells,
                       weighted_gradient,
                       weighted_jump_force)
+from .cmdgfm import CMDGFMBlockSystem, CMDGFMSolveStats
 from .navier_stokes import NavierStokesSystem, step, step_device
 from .stepping import DeviceProjectionRun, ProjectionRun, face_dual_volumes, kinetic_energy, staggered_liquid_fraction
 from .solver import (EllipticSolver, FreeSlip, LinearSolverOptions, LinearSolverStats, MultigridOptions, NoSlip,
@@ -18,7 +19,8 @@ from .solver import (EllipticSolver, FreeSlip, LinearSolverOptions, LinearSolver
                      pressure_boundary_sides, solve)
 
 __all__ = [
-    "BlockKey", "BlockTreeMesh", "CGFM", "CSF", "Context", "DeviceProjectionRun", "DimensionMismatch", "EllipticSolver",
+    "BlockKey", "BlockTreeMesh", "CGFM", "CMDGFMBlockSystem", "CMDGFMSolveStats", "CSF", "Context",
+    "DeviceProjectionRun", "DimensionMismatch", "EllipticSolver",
     "FreeSlip", "NoSlip", "Outflow", "VelocityBoundaryConditions", "ZeroGradientVelocity", "pressure_boundary_sides",
     "project_open", "weighted_gradient_boundary",
     "HostHierarchy",
