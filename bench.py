@@ -597,6 +597,90 @@ def taylor_green_main(args):
     print(json.dumps(out), flush=True)
 
 
+def cmdgfm_run(n=2048, steps=5, warmup=1, tol=1e-8, precision="fp64", sample=256):
+    """The CMDGFM reduced-pressure/jump block solve on the synthetic
+    n^2-cell system of tpamr.workloads.cmdgfm_block_workload (walls, liquid
+    disk r=0.27 at the centre, 1000:1, dt 0.02): `steps` timed solves from
+    the rhs and solution in HBM (the device entry point) after `warmup`. The
+    oracle solves the same workload at sample^2 cells on the host and the
+    GPU's answer there is compared with it (iterations, relative error); the
+    oracle's solve rate at that size is the CPU figure."""
+    import torch
+    import tpamr
+    from tpamr.workloads import cmdgfm_block_workload
+
+    def build(k):
+        t = time.perf_counter()
+        mesh, kappa, sysd, rhs, dt, rho = cmdgfm_block_workload(n=k)
+        solver = tpamr.EllipticSolver(tpamr.PoissonOperator(mesh, kappa), precision=precision)
+        gsys = tpamr.CMDGFMBlockSystem(solver, sysd["matrix"], sysd["pressure_indices"], sysd["pressure_anchors"],
+                                       sysd["pressure_volumes"], sysd["jump_lengths"], omitted=sysd["omitted"],
+                                       omitted_rhs=sysd["omitted_rhs"], omitted_volumes=sysd["omitted_volumes"])
+        log(f"cmdgfm {k}^2: {mesh.ncells} cells, {len(sysd['jump_lengths'])} jumps, setup "
+            f"{time.perf_counter() - t:.1f} s")
+        return mesh, kappa, sysd, rhs, dt, rho, solver, gsys
+
+    mesh, kappa, sysd, rhs, dt, rho, solver, gsys = build(n)
+    d_rhs = torch.tensor(rhs, device="cuda:0")
+    d_x = torch.zeros(len(rhs), dtype=torch.float64, device="cuda:0")
+    st = None
+    for _ in range(warmup):
+        st = gsys.solve_device(d_rhs, d_x, dt=dt, density=rho, tolerance=tol)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        st = gsys.solve_device(d_rhs, d_x, dt=dt, density=rho, tolerance=tol)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t) / steps
+    out = {"cells": mesh.ncells, "unknowns": len(rhs), "jumps": st.jump_unknowns, "ms_per_solve": wall * 1e3,
+           "mcells": mesh.ncells / wall / 1e6, "iterations": st.iterations, "krylov_runs": st.krylov_runs,
+           "converged": st.converged, "pressure_defect": st.pressure_defect, "jump_defect": st.jump_defect,
+           "anchor_defect": st.anchor_defect, "precision": precision}
+    solver.close()
+    if sample:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_cmdgfm as OC
+        O = _oracle()
+        model, threads = host_cpu()
+        O.set_threads(threads)
+        mesh, kappa, sysd, rhs, dt, rho, solver, gsys = build(sample)
+        x, gst = gsys.solve(rhs, dt=dt, density=rho, tolerance=tol)
+        solver.close()
+        om = O.OMesh(base_mesh=(sample, sample), block_size=(16, 16), periodic=(False, False))
+        osol = O.OSolver(om, kappa)
+        osys = OC.BlockSystem(sysd["matrix"], len(sysd["pressure_indices"]), sysd["pressure_indices"],
+                              sysd["pressure_anchors"], sysd["pressure_volumes"], sysd["jump_lengths"],
+                              sysd["omitted"], sysd["omitted_rhs"], sysd["omitted_volumes"], mesh.ncells,
+                              osol.vcycle)
+        t = time.perf_counter()
+        ox, ost = OC.solve(osys, rhs, dt=dt, density=rho, reference_length=1.0, tolerance=tol)
+        cpu = time.perf_counter() - t
+        out["sample"] = {"cells": mesh.ncells, "gpu_iterations": gst.iterations, "oracle_iterations": ost["iterations"],
+                         "error_vs_oracle": float(np.max(np.abs(x - ox)) / np.max(np.abs(ox))),
+                         "cpu_oracle_ms": cpu * 1e3, "cpu_oracle_mcells": mesh.ncells / cpu / 1e6,
+                         "cpu_threads": threads, "cpu_model": model,
+                         "cpu_note": "oracle restatement: numpy/SciPy BiCGSTAB, block preconditioner and Crout ILU, "
+                                     "C++ V-cycle on the host's threads, FP64"}
+    return out
+
+
+def cmdgfm_main(args):
+    n = args.cm_cells
+    res = cmdgfm_run(n=n, steps=args.steps, warmup=args.warmup, sample=args.cm_sample)
+    sample = res.get("sample", {})
+    out = {"metric": "Mcells/s of the CMDGFM block solve (pressure cells / solve wall time)", "value": res["mcells"],
+           "unit": "Mcells/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": res["ms_per_solve"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"cmdgfm_block_{n}x{n}_disk_rho1000", "cells": res["cells"],
+                      "jumps": res["jumps"], "tolerance": 1e-8},
+           "cpu_baseline": None if not sample else {
+               "value": sample["cpu_oracle_mcells"], "unit": "Mcells/s", "cores": sample["cpu_threads"],
+               "kind": "port", "sample": f"oracle block solve of the same workload at {sample['cells']} cells"},
+           "detail": res}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -612,10 +696,14 @@ def main():
     ap.add_argument("--probe", action="store_true", help="short run for rocprofv3 --pmc passes")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DB", "WRITE_DB"))
     ap.add_argument("--trace-summary", nargs=3, metavar=("DB", "TITLE", "COMMAND"))
-    ap.add_argument("--workload", choices=("circles", "wave", "taylor_green"), default="circles",
+    ap.add_argument("--workload", choices=("circles", "wave", "taylor_green", "cmdgfm"), default="circles",
                     help="circles: the headline solve benchmark; wave: the checkpointed multi-step run; "
-                         "taylor_green: one-velocity steps on the periodic Taylor-Green field")
+                         "taylor_green: one-velocity steps on the periodic Taylor-Green field; "
+                         "cmdgfm: the CMDGFM reduced-pressure/jump block solve")
     ap.add_argument("--tg-cells", type=int, default=128, help="taylor_green: cells per side")
+    ap.add_argument("--cm-cells", type=int, default=2048, help="cmdgfm: cells per side")
+    ap.add_argument("--cm-sample", type=int, default=256,
+                    help="cmdgfm: cells per side of the oracle comparison and CPU figure (0: none)")
     ap.add_argument("--wave-fd", action="store_true", help="wave: one finite-difference check of the gradient")
     ap.add_argument("--uniform", type=int, metavar="N",
                     help="time the finest-level sweep on a uniform N x N mesh of 16x16 blocks (no irregular rows)")
@@ -638,6 +726,9 @@ def main():
         return
     if args.workload == "taylor_green":
         taylor_green_main(args)
+        return
+    if args.workload == "cmdgfm":
+        cmdgfm_main(args)
         return
     if args.workload == "wave":
         if "--levels" not in sys.argv:

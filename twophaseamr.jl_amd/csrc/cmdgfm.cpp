@@ -258,17 +258,19 @@ struct CmdgfmSystem::Impl {
   int nanchors = 0, nchunks = 0;
   JumpFactor jf;
   double asym = 0, margin = 0;
-  // device copies
-  Buf<int> aptr, acol, cptr, ccol, role, idx, anc, list, chunk_ptr, anchor_cell, anchor_chunk_ptr, optr, ocol;
-  Buf<double> aval, cval, vol, len, oval, orhs, ovol;
+  // device copies: the system as sliced ELL, its C block as CSR
+  Buf<int> sptr, slen, scol, cptr, ccol, role, idx, anc, list, chunk_ptr, anchor_cell, anchor_chunk_ptr, optr, ocol;
+  Buf<double> sval, cval, vol, len, oval, orhs, ovol;
   Buf<int> lptr, lcol, llev, lorder, uptr, ucol, ulev, uorder;
   Buf<double> lval, uval, udiag;
   // work vectors
-  Buf<double> rows, cols, corr, srhs, inc, kr, kp, kv, ks, kt, kyz, scratch, cand, res, full, zfull, jr, jy;
+  Buf<double> corr, srhs, inc, kr, kp, kv, ks, kt, kyz, cand, res, full, jr, jy;
   Buf<double> part, apart, scal, out;
   Buf<int> flag;
   double host_out[8] = {};
   double host_scal[CM_SCALARS] = {};
+  // Krylov scaling of the current solve: rows prow / jrow, columns pcol / jcol
+  double prow = 1, pcol = 1, jrow = 1, jcol = 1;
 
   void read_out(int k) {
     CCHECK(hipMemcpyAsync(host_out, out.p, sizeof(double) * k, hipMemcpyDeviceToHost, s));
@@ -276,8 +278,8 @@ struct CmdgfmSystem::Impl {
   }
   // defects and norms of residual r (and finiteness of x): [0] pressure,
   // [1] jump, [2] sum r^2, [3] sum (rows r)^2, [4] x not finite
-  void check(const double* r, const double* x, bool scaled) {
-    launch_cm_check(s, n, np, r, vol.p, len.p, scaled ? rows.p : nullptr, x, part.p);
+  void check(const double* r, const double* x) {
+    launch_cm_check(s, n, np, r, vol.p, len.p, prow, jrow, x, part.p);
     launch_cm_reduce(s, cm_parts(n), 5, 0b10011, part.p, out.p);
     read_out(5);
   }
@@ -288,30 +290,38 @@ struct CmdgfmSystem::Impl {
     read_out(1);
     return dt * dt * host_out[0];
   }
+  void residual(const double* x, const double* b, double* r) {
+    launch_cm_residual(s, n, sptr.p, slen.p, scol.p, sval.p, x, b, r);
+  }
 
-  // dst = N^-1 src of the scaled block preconditioner
+  // dst = N^-1 src of the scaled block preconditioner: the source divided by
+  // the row scaling, the reduced-pressure block (V-cycle of the anchored
+  // full-cell rhs, differences to the anchors), the jump rows through the
+  // jump factor, each part divided by its column scaling
   void precondition(const double* src, double* dst, const Vcycle& vc) {
-    launch_cm_div(s, n, src, rows.p, scratch.p);
+    const double* z = nullptr;
     if (np) {
-      launch_cm_scatter(s, ncells, role.p, scratch.p, full.p);
-      launch_cm_anchor_chunks(s, nchunks, chunk_ptr.p, list.p, scratch.p, apart.p);
+      launch_cm_scatter(s, ncells, role.p, src, prow, full.p);
+      launch_cm_anchor_chunks(s, nchunks, chunk_ptr.p, list.p, src, prow, apart.p);
       launch_cm_anchor_totals(s, nanchors, anchor_cell.p, anchor_chunk_ptr.p, apart.p, full.p);
-      vc(full.p, zfull.p);
-      launch_cm_gather(s, np, idx.p, anc.p, zfull.p, dst);
+      z = vc(full.p);
+      launch_cm_gather(s, np, idx.p, anc.p, z, pcol, dst);
     }
-    if (nx) jump_solve(np ? dst : nullptr, scratch.p + np, dst + np, cols.p + np);
-    if (np) launch_cm_div(s, np, dst, cols.p, dst);
+    if (nx) {
+      launch_cm_jump_rhs(s, nx, np ? cptr.p : nullptr, ccol.p, cval.p, nullptr, z, idx.p, anc.p, src + np, jrow,
+                         jr.p);
+      ilu(dst + np, jcol);
+    }
   }
-  // xj = ILU^-1 (rhs_j - C zp) (zp null: no pressure part), xj /= cs
-  void jump_solve(const double* zp, const double* rhs_j, double* xj, const double* cs) {
-    launch_cm_jump_rhs(s, nx, zp ? cptr.p : nullptr, ccol.p, cval.p, zp, rhs_j, jr.p);
+  // x = ILU^-1 jr, then x /= d
+  void ilu(double* x, double d) {
     launch_cm_ilu(s, nx, lptr.p, lcol.p, lval.p, int(jf.llev.size()) - 1, llev.p, lorder.p, uptr.p, ucol.p, uval.p,
-                  udiag.p, int(jf.ulev.size()) - 1, ulev.p, uorder.p, jr.p, jy.p, xj, cs);
+                  udiag.p, int(jf.ulev.size()) - 1, ulev.p, uorder.p, jr.p, jy.p, x, d);
   }
 
-  // Krylov.jl bicgstab! (M = I, right preconditioner N when pc, zero start,
+  // Krylov.jl bicgstab! (M = I, right preconditioner N when use_pc, zero start,
   // c = b, atol = 0) on the scaled operator; the solution into inc
-  int bicgstab(const double* b, double rtol, int itmax, bool pc, const Vcycle& vc) {
+  int bicgstab(const double* b, double rtol, int itmax, bool use_pc, const Vcycle& vc) {
     CCHECK(hipMemsetAsync(inc.p, 0, sizeof(double) * n, s));
     CCHECK(hipMemcpyAsync(kr.p, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     CCHECK(hipMemcpyAsync(kp.p, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
@@ -330,20 +340,20 @@ struct CmdgfmSystem::Impl {
     while (!(solved || it >= itmax || breakdown)) {
       ++it;
       const double* y = kp.p;
-      if (pc) {
+      if (use_pc) {
         precondition(kp.p, kyz.p, vc);
         y = kyz.p;
       }
-      launch_cm_spmv(s, n, aptr.p, acol.p, aval.p, y, cols.p, rows.p, kv.p);
+      launch_cm_spmv(s, n, np, sptr.p, slen.p, scol.p, sval.p, y, pcol, jcol, prow, jrow, kv.p);
       launch_cm_dots(s, n, 1, b, kv.p, nullptr, nullptr, part.p);
       launch_cm_scalars(s, P, 1, part.p, scal.p, CM_STEP_ALPHA);
       launch_cm_update_s(s, n, kr.p, kv.p, y, ks.p, inc.p, scal.p);
       const double* z = ks.p;
-      if (pc) {
+      if (use_pc) {
         precondition(ks.p, kyz.p, vc);
         z = kyz.p;
       }
-      launch_cm_spmv(s, n, aptr.p, acol.p, aval.p, z, cols.p, rows.p, kt.p);
+      launch_cm_spmv(s, n, np, sptr.p, slen.p, scol.p, sval.p, z, pcol, jcol, prow, jrow, kt.p);
       launch_cm_dots(s, n, 2, kt.p, ks.p, kt.p, kt.p, part.p);
       launch_cm_scalars(s, P, 2, part.p, scal.p, CM_STEP_OMEGA);
       launch_cm_update_r(s, n, ks.p, kt.p, z, kr.p, inc.p, scal.p);
@@ -363,8 +373,8 @@ struct CmdgfmSystem::Impl {
 
   // _cmdgfm_initial_solution!: zero-pressure and previous-increment guesses
   // with the jump rows solved through the jump factor
-  bool initial_solution(const double* rhs, double* x, double dt, const double* previous, bool pc) {
-    check(rhs, nullptr, true);
+  bool initial_solution(const double* rhs, double* x, double dt, const double* previous, bool use_pc) {
+    check(rhs, nullptr);
     double best_defect = (dt * dt) * jmax(host_out[0], host_out[1]);
     double best_l2 = std::sqrt(host_out[3]);
     bool used = false;
@@ -372,18 +382,20 @@ struct CmdgfmSystem::Impl {
       CCHECK(hipMemsetAsync(cand.p, 0, sizeof(double) * n, s));
       if (use_previous) {
         if (!previous || np == 0) continue;
-        launch_cm_gather(s, np, idx.p, anc.p, previous, cand.p);
-        check(cand.p, cand.p, false);
+        launch_cm_gather(s, np, idx.p, anc.p, previous, 1.0, cand.p);
+        check(cand.p, cand.p);
         const bool nonfinite = host_out[4] != 0.0;
         if (!(host_out[2] > 0.0) && !nonfinite) continue;  // every entry zero
         if (nonfinite) continue;
       }
       if (nx) {
-        if (!pc) continue;
-        jump_solve(np ? cand.p : nullptr, rhs + np, cand.p + np, nullptr);
+        if (!use_pc) continue;
+        launch_cm_jump_rhs(s, nx, np ? cptr.p : nullptr, ccol.p, cval.p, cand.p, nullptr, idx.p, anc.p, rhs + np,
+                           1.0, jr.p);
+        ilu(cand.p + np, 1.0);
       }
-      launch_cm_residual(s, n, aptr.p, acol.p, aval.p, cand.p, rhs, res.p);
-      check(res.p, cand.p, true);
+      residual(cand.p, rhs, res.p);
+      check(res.p, cand.p);
       if (host_out[4] != 0.0) continue;
       const double defect = (dt * dt) * jmax(host_out[0], host_out[1]);
       const double l2 = std::sqrt(host_out[3]);
@@ -505,7 +517,6 @@ CmdgfmSystem::CmdgfmSystem(int ncells, int np, int nx, const int* ptr, const int
   }
   D.margin = margin;
   if (!device) return;
-  const int nnz = n > 0 ? ptr[n] : 0;
   auto up_i = [&](Buf<int>& b, const int* src, size_t k) {
     b.alloc(k);
     if (k) CCHECK(hipMemcpyAsync(b.p, src, k * sizeof(int), hipMemcpyHostToDevice, stream));
@@ -514,9 +525,32 @@ CmdgfmSystem::CmdgfmSystem(int ncells, int np, int nx, const int* ptr, const int
     b.alloc(k);
     if (k) CCHECK(hipMemcpyAsync(b.p, src, k * sizeof(double), hipMemcpyHostToDevice, stream));
   };
-  up_i(D.aptr, ptr, size_t(n) + 1);
-  up_i(D.acol, col, nnz);
-  up_d(D.aval, val, nnz);
+  // sliced ELL: CM_SLICE rows per slice, as wide as its longest row, entries
+  // column-major (a wavefront reads one entry of 64 rows per load)
+  {
+    const int nsl = (n + CM_SLICE - 1) / CM_SLICE;
+    std::vector<int> sp(nsl + 1, 0), sl(std::max(n, 1), 0);
+    for (int k = 0; k < nsl; ++k) {
+      int w = 0;
+      for (int i = k * CM_SLICE; i < std::min(n, (k + 1) * CM_SLICE); ++i) w = std::max(w, ptr[i + 1] - ptr[i]);
+      sp[k + 1] = sp[k] + w * CM_SLICE;
+    }
+    std::vector<int> sc(std::max(sp[nsl], 1), 0);
+    std::vector<double> sv(std::max(sp[nsl], 1), 0.0);
+    for (int i = 0; i < n; ++i) {
+      sl[i] = ptr[i + 1] - ptr[i];
+      const int base = sp[i / CM_SLICE] + i % CM_SLICE;
+      for (int e = 0; e < sl[i]; ++e) {
+        sc[base + e * CM_SLICE] = col[ptr[i] + e];
+        sv[base + e * CM_SLICE] = val[ptr[i] + e];
+      }
+    }
+    D.sptr.upload(sp, stream);
+    D.slen.upload(sl, stream);
+    D.scol.upload(sc, stream);
+    D.sval.upload(sv, stream);
+    CCHECK(hipStreamSynchronize(stream));
+  }
   D.cptr.upload(cptr, stream);
   D.ccol.upload(ccol, stream);
   D.cval.upload(cval, stream);
@@ -550,11 +584,9 @@ CmdgfmSystem::CmdgfmSystem(int ncells, int np, int nx, const int* ptr, const int
     D.ulev.upload(f.ulev, stream);
     D.uorder.upload(f.uorder, stream);
   }
-  for (Buf<double>* b : {&D.rows, &D.cols, &D.corr, &D.srhs, &D.inc, &D.kr, &D.kp, &D.kv, &D.ks, &D.kt, &D.kyz,
-                         &D.scratch, &D.cand, &D.res})
+  for (Buf<double>* b : {&D.corr, &D.srhs, &D.inc, &D.kr, &D.kp, &D.kv, &D.ks, &D.kt, &D.kyz, &D.cand, &D.res})
     b->alloc(std::max(n, 1));
   D.full.alloc(std::max(ncells, 1));
-  D.zfull.alloc(std::max(ncells, 1));
   D.jr.alloc(std::max(nx, 1));
   D.jy.alloc(std::max(nx, 1));
   D.part.alloc(size_t(CM_MAX_WG) * 5);
@@ -573,7 +605,7 @@ int CmdgfmSystem::jump_size() const { return d_->nx; }
 const JumpFactor& CmdgfmSystem::factor() const { return d_->jf; }
 
 void CmdgfmSystem::solve(const double* rhs, double* x, double dt, double density, double h, double tol,
-                         int max_iterations, bool warm_start, const double* previous, bool pc, const Vcycle& vc,
+                         int max_iterations, bool warm_start, const double* previous, bool use_pc, const Vcycle& vc,
                          CmdgfmStats& st) {
   Impl& D = *d_;
   if (!D.device) throw std::invalid_argument("host-only block system");
@@ -592,59 +624,61 @@ void CmdgfmSystem::solve(const double* rhs, double* x, double dt, double density
     st.converged = 1;
     return;
   }
+  // _cmdgfm_krylov_scaling!: pressure unknowns scale rho (L/dt)^2 (rows
+  // (dt/L)^2), jumps L/dt^2 (rows dt^2/L)
+  D.prow = (dt / h) * (dt / h);
+  D.pcol = density * ((h / dt) * (h / dt));
+  D.jrow = (dt * dt) / h;
+  D.jcol = h / (dt * dt);
   CCHECK(hipMemsetAsync(x, 0, sizeof(double) * n, s));
   CCHECK(hipMemcpyAsync(D.corr.p, rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-  D.check(rhs, nullptr, false);
+  D.check(rhs, nullptr);
   const double initial = std::sqrt(D.host_out[2]);
   st.initial_residual = initial;
   int iterations = 0, runs = 0;
   bool used = false;
   if (initial > 0) {
-    // _cmdgfm_krylov_scaling!: pressure scale rho (L/dt)^2, jump scale L/dt^2
-    const double pr = (dt / h) * (dt / h), pcs = density * ((h / dt) * (h / dt));
-    const double jr = (dt * dt) / h, jc = h / (dt * dt);
-    launch_cm_scaling(s, n, D.np, pr, pcs, jr, jc, D.rows.p, D.cols.p);
     if (warm_start) {
-      used = D.initial_solution(rhs, x, dt, previous, pc);
-      launch_cm_residual(s, n, D.aptr.p, D.acol.p, D.aval.p, x, rhs, D.corr.p);
+      used = D.initial_solution(rhs, x, dt, previous, use_pc);
+      D.residual(x, rhs, D.corr.p);
     }
     while (iterations < max_iterations) {
-      D.check(D.corr.p, nullptr, true);
+      D.check(D.corr.p, nullptr);
       const double pd = dt * dt * D.host_out[0], jd = dt * dt * D.host_out[1];
       const double rhs_norm = std::sqrt(D.host_out[3]);
       const double defect = jmax(jmax(pd, jd), D.anchor_defect(x, dt));
       if (!std::isfinite(defect) || defect <= tol) break;
       if (!std::isfinite(rhs_norm) || rhs_norm == 0) break;
-      launch_cm_mul(s, n, D.corr.p, D.rows.p, D.srhs.p);
+      launch_cm_scale_rows(s, n, D.np, D.corr.p, D.prow, D.jrow, D.srhs.p);
       const double requested = std::min(tol, tol / std::max(defect, EPS));
       double rhs_scale = 1.0;
       if (rhs_norm < 1.0 && rhs_norm * requested < EPS) {
         launch_cm_divs(s, n, D.srhs.p, rhs_norm);
         rhs_scale = rhs_norm;
       }
-      int its = D.bicgstab(D.srhs.p, requested, max_iterations - iterations, pc, vc);
+      int its = D.bicgstab(D.srhs.p, requested, max_iterations - iterations, use_pc, vc);
       ++runs;
       iterations += its;
-      D.check(D.inc.p, D.inc.p, false);
+      D.check(D.inc.p, D.inc.p);
       bool finite = D.host_out[4] == 0.0;
-      if (!finite && pc && iterations < max_iterations) {
+      if (!finite && use_pc && iterations < max_iterations) {
         its = D.bicgstab(D.srhs.p, requested, max_iterations - iterations, false, vc);
         ++runs;
         iterations += its;
-        D.check(D.inc.p, D.inc.p, false);
+        D.check(D.inc.p, D.inc.p);
         finite = D.host_out[4] == 0.0;
       }
       if (!finite || its == 0) break;
       CCHECK(hipMemsetAsync(D.flag.p, 0, sizeof(int), s));
-      launch_cm_increment(s, n, x, D.cols.p, rhs_scale, D.inc.p, D.flag.p);
+      launch_cm_increment(s, n, D.np, x, D.pcol, D.jcol, rhs_scale, D.inc.p, D.flag.p);
       int changed = 0;
       CCHECK(hipMemcpyAsync(&changed, D.flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
       CCHECK(hipStreamSynchronize(s));
       if (!changed) break;
-      launch_cm_residual(s, n, D.aptr.p, D.acol.p, D.aval.p, x, rhs, D.corr.p);
+      D.residual(x, rhs, D.corr.p);
     }
   }
-  D.check(D.corr.p, nullptr, false);
+  D.check(D.corr.p, nullptr);
   st.final_residual = std::sqrt(D.host_out[2]);
   st.pressure_defect = dt * dt * D.host_out[0];
   st.jump_defect = dt * dt * D.host_out[1];

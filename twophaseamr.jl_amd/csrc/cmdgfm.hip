@@ -1,4 +1,4 @@
-// Device kernels of the CMDGFM reduced-pressure/jump block solve: CSR
+// Device kernels of the CMDGFM reduced-pressure/jump block solve: sliced-ELL
 // products of the assembled system, BiCGSTAB vector updates and scalars, the
 // block preconditioner's scatter / anchor / gather passes, the level-scheduled
 // triangular solves of the jump factor, and the defect reductions.
@@ -61,33 +61,44 @@ __device__ __forceinline__ double row_dot(const int* __restrict__ ptr, const int
   return acc;
 }
 
-__global__ void __launch_bounds__(CM_WG) k_cm_spmv(int n, const int* __restrict__ ptr, const int* __restrict__ col,
+// Sliced ELL rows (CM_SLICE rows per slice, entries of a slice column-major,
+// a row's entries in CSR order): y_i = rs_i sum_j a_ij (cs_j x_j) with the
+// per-block scalings (pressure rows / columns before np, jump after).
+__global__ void __launch_bounds__(CM_WG) k_cm_spmv(int n, int np, const int* __restrict__ sptr,
+                                                   const int* __restrict__ len, const int* __restrict__ col,
                                                    const double* __restrict__ val, const double* __restrict__ x,
-                                                   const double* __restrict__ cs, const double* __restrict__ rs,
+                                                   double pc, double jc, double pr, double jr,
                                                    double* __restrict__ y) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const int base = sptr[i / CM_SLICE] + (i % CM_SLICE);
+  const int m = len[i];
   double acc = 0.0;
-  if (cs) {
-    for (int p = ptr[i]; p < ptr[i + 1]; ++p) {
-      const int j = col[p];
-      acc = acc + val[p] * (cs[j] * x[j]);
-    }
-  } else {
-    acc = row_dot(ptr, col, val, x, i);
+  for (int e = 0; e < m; ++e) {
+    const int q = base + e * CM_SLICE;
+    const int j = col[q];
+    acc = acc + val[q] * ((j < np ? pc : jc) * x[j]);
   }
-  y[i] = rs ? rs[i] * acc : acc;
+  y[i] = (i < np ? pr : jr) * acc;
 }
 
-__global__ void __launch_bounds__(CM_WG) k_cm_residual(int n, const int* __restrict__ ptr,
-                                                       const int* __restrict__ col, const double* __restrict__ val,
-                                                       const double* __restrict__ x, const double* __restrict__ b,
-                                                       double* __restrict__ r) {
+// r = b - A x over the sliced ELL rows
+__global__ void __launch_bounds__(CM_WG) k_cm_residual(int n, const int* __restrict__ sptr,
+                                                       const int* __restrict__ len, const int* __restrict__ col,
+                                                       const double* __restrict__ val, const double* __restrict__ x,
+                                                       const double* __restrict__ b, double* __restrict__ r) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  r[i] = b[i] - row_dot(ptr, col, val, x, i);
+  const int base = sptr[i / CM_SLICE] + (i % CM_SLICE);
+  const int m = len[i];
+  double acc = 0.0;
+  for (int e = 0; e < m; ++e) {
+    const int q = base + e * CM_SLICE;
+    acc = acc + val[q] * x[col[q]];
+  }
+  r[i] = b[i] - acc;
 }
 
 template <int ND>
@@ -190,16 +201,12 @@ __global__ void __launch_bounds__(CM_WG) k_cm_update_p(int n, const double* __re
   p[i] = r[i] + beta * q;
 }
 
-// out may alias a
-__global__ void __launch_bounds__(CM_WG) k_cm_div(int n, const double* a, const double* __restrict__ d, double* out) {
+// out = (rows) v: pressure entries times pr, jump entries times jr
+__global__ void __launch_bounds__(CM_WG) k_cm_scale_rows(int n, int np, const double* __restrict__ v, double pr,
+                                                         double jr, double* __restrict__ out) {
+#pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = a[i] / d[i];
-}
-
-// out = a * d (out may alias a)
-__global__ void __launch_bounds__(CM_WG) k_cm_mul(int n, const double* a, const double* __restrict__ d, double* out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = a[i] * d[i];
+  if (i < n) out[i] = (i < np ? pr : jr) * v[i];
 }
 
 // v = v / s
@@ -208,62 +215,71 @@ __global__ void __launch_bounds__(CM_WG) k_cm_divs(int n, double* v, double s) {
   if (i < n) v[i] = v[i] / s;
 }
 
-__global__ void __launch_bounds__(CM_WG) k_cm_scaling(int n, int np, double pr, double pc, double jr, double jc,
-                                                      double* __restrict__ rows, double* __restrict__ cols) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  rows[i] = i < np ? pr : jr;
-  cols[i] = i < np ? pc : jc;
-}
-
+// v / d: the preconditioner's input divided by its (pressure) row scaling
 __global__ void __launch_bounds__(CM_WG) k_cm_scatter(int ncells, const int* __restrict__ role,
-                                                      const double* __restrict__ v, double* __restrict__ full) {
+                                                      const double* __restrict__ v, double d,
+                                                      double* __restrict__ full) {
 #pragma clang fp contract(off)
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
   const int k = role[c];
-  full[c] = k >= 0 ? 0.0 + v[k] : 0.0;
+  full[c] = k >= 0 ? 0.0 + v[k] / d : 0.0;
 }
 
 __global__ void __launch_bounds__(CM_WG) k_cm_anchor_chunks(const int* __restrict__ chunk_ptr,
                                                             const int* __restrict__ list,
-                                                            const double* __restrict__ v,
+                                                            const double* __restrict__ v, double d,
                                                             double* __restrict__ part) {
 #pragma clang fp contract(off)
   double acc[1] = {0.0};
   const int b = chunk_ptr[blockIdx.x], e = chunk_ptr[blockIdx.x + 1];
-  for (int k = b + int(threadIdx.x); k < e; k += CM_WG) acc[0] = acc[0] + v[list[k]];
+  for (int k = b + int(threadIdx.x); k < e; k += CM_WG) acc[0] = acc[0] + v[list[k]] / d;
   wg_partials<1>(acc, 0, part);
 }
 
-__global__ void k_cm_anchor_totals(int nanchors, const int* __restrict__ anchor_cell,
-                                   const int* __restrict__ anchor_chunk_ptr, const double* __restrict__ part,
-                                   double* __restrict__ full) {
+// one workgroup per anchor: its chunks' sums folded in a fixed order and
+// subtracted from the anchor's (zero) entry
+__global__ void __launch_bounds__(CM_WG) k_cm_anchor_totals(const int* __restrict__ anchor_cell,
+                                                            const int* __restrict__ anchor_chunk_ptr,
+                                                            const double* __restrict__ part,
+                                                            double* __restrict__ full) {
 #pragma clang fp contract(off)
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= nanchors) return;
-  double acc = full[anchor_cell[a]];
-  for (int q = anchor_chunk_ptr[a]; q < anchor_chunk_ptr[a + 1]; ++q) acc = acc - part[q];
-  full[anchor_cell[a]] = acc;
+  __shared__ double red[8][CM_WG];
+  const int a = blockIdx.x;
+  const int b = anchor_chunk_ptr[a];
+  fold(anchor_chunk_ptr[a + 1] - b, 1, 0, part + b, red);
+  if (threadIdx.x == 0) full[anchor_cell[a]] = full[anchor_cell[a]] - red[0][0];
 }
 
 __global__ void __launch_bounds__(CM_WG) k_cm_gather(int np, const int* __restrict__ idx,
                                                      const int* __restrict__ anchor, const double* __restrict__ z,
-                                                     double* __restrict__ out) {
+                                                     double d, double* __restrict__ out) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < np) out[i] = z[idx[i]] - z[anchor[i]];
+  if (i < np) out[i] = (z[idx[i]] - z[anchor[i]]) / d;
 }
 
+// jr = -(C zp) + rhs / d; zp either a reduced vector or, with zfull, the
+// reduced-pressure block's answer zfull[idx] - zfull[anchor]
 __global__ void __launch_bounds__(CM_WG) k_cm_jump_rhs(int nx, const int* __restrict__ ptr,
                                                        const int* __restrict__ col, const double* __restrict__ val,
-                                                       const double* __restrict__ zp, const double* __restrict__ rhs,
+                                                       const double* __restrict__ zp,
+                                                       const double* __restrict__ zfull,
+                                                       const int* __restrict__ idx, const int* __restrict__ anchor,
+                                                       const double* __restrict__ rhs, double d,
                                                        double* __restrict__ jr) {
 #pragma clang fp contract(off)
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nx) return;
-  const double cz = ptr ? row_dot(ptr, col, val, zp, k) : 0.0;
-  jr[k] = -1.0 * cz + 1.0 * rhs[k];
+  double cz = 0.0;
+  if (ptr) {
+    for (int p = ptr[k]; p < ptr[k + 1]; ++p) {
+      const int q = col[p];
+      const double z = zfull ? zfull[idx[q]] - zfull[anchor[q]] : zp[q];
+      cz = cz + val[p] * z;
+    }
+  }
+  jr[k] = -1.0 * cz + 1.0 * (rhs[k] / d);
 }
 
 // One workgroup: rows of one level are independent; a barrier separates
@@ -276,7 +292,7 @@ __global__ void __launch_bounds__(CM_ILU_WG) k_cm_ilu(int nx, const int* __restr
                                                       const double* __restrict__ udiag, int nulev,
                                                       const int* __restrict__ ulev, const int* __restrict__ uorder,
                                                       const double* __restrict__ b, double* y, double* x,
-                                                      const double* __restrict__ cols) {
+                                                      double d) {
 #pragma clang fp contract(off)
   for (int l = 0; l < nllev; ++l) {
     for (int t = llev[l] + int(threadIdx.x); t < llev[l + 1]; t += CM_ILU_WG) {
@@ -296,13 +312,13 @@ __global__ void __launch_bounds__(CM_ILU_WG) k_cm_ilu(int nx, const int* __restr
     }
     __syncthreads();
   }
-  if (cols)
-    for (int i = threadIdx.x; i < nx; i += CM_ILU_WG) x[i] = x[i] / cols[i];
+  if (d != 1.0)
+    for (int i = threadIdx.x; i < nx; i += CM_ILU_WG) x[i] = x[i] / d;
 }
 
 __global__ void __launch_bounds__(CM_WG) k_cm_check(int n, int np, const double* __restrict__ r,
                                                     const double* __restrict__ vol, const double* __restrict__ len,
-                                                    const double* __restrict__ rows, const double* __restrict__ x,
+                                                    double pr, double jr, const double* __restrict__ x,
                                                     double* __restrict__ part) {
 #pragma clang fp contract(off)
   double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -314,7 +330,7 @@ __global__ void __launch_bounds__(CM_WG) k_cm_check(int n, int np, const double*
       v[1] = nanmax(v[1], fabs(ri) / len[i - np]);
     }
     v[2] = v[2] + ri * ri;
-    const double q = rows ? rows[i] * ri : 0.0;
+    const double q = (i < np ? pr : jr) * ri;
     v[3] = v[3] + q * q;
     if (x && !isfinite(x[i])) v[4] = 1.0;
   }
@@ -340,14 +356,14 @@ __global__ void __launch_bounds__(CM_WG) k_cm_reduce(int nparts, int nsets, int 
     for (int s = 0; s < nsets; ++s) out[s] = red[s][0];
 }
 
-__global__ void __launch_bounds__(CM_WG) k_cm_increment(int n, double* __restrict__ x, const double* __restrict__ cols,
+__global__ void __launch_bounds__(CM_WG) k_cm_increment(int n, int np, double* __restrict__ x, double pc, double jc,
                                                         double scale, const double* __restrict__ inc,
                                                         int* __restrict__ flag) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double previous = x[i];
-  const double updated = previous + cols[i] * (scale * inc[i]);
+  const double updated = previous + (i < np ? pc : jc) * (scale * inc[i]);
   x[i] = updated;
   if (updated != previous) *flag = 1;
 }
@@ -356,13 +372,16 @@ __global__ void __launch_bounds__(CM_WG) k_cm_increment(int n, double* __restric
 
 int cm_parts(int n) { return std::max(1, std::min(cdiv(n, CM_WG), CM_MAX_WG)); }
 
-void launch_cm_spmv(hipStream_t s, int n, const int* ptr, const int* col, const double* val, const double* x,
-                    const double* cs, const double* rs, double* y) {
-  if (n > 0) hipLaunchKernelGGL(k_cm_spmv, dim3(cdiv(n, CM_WG)), dim3(CM_WG), 0, s, n, ptr, col, val, x, cs, rs, y);
+void launch_cm_spmv(hipStream_t s, int n, int np, const int* sptr, const int* len, const int* col,
+                    const double* val, const double* x, double pc, double jc, double pr, double jr, double* y) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_cm_spmv, dim3(cdiv(n, CM_WG)), dim3(CM_WG), 0, s, n, np, sptr, len, col, val, x, pc, jc, pr,
+                       jr, y);
 }
-void launch_cm_residual(hipStream_t s, int n, const int* ptr, const int* col, const double* val, const double* x,
-                        const double* b, double* r) {
-  if (n > 0) hipLaunchKernelGGL(k_cm_residual, dim3(cdiv(n, CM_WG)), dim3(CM_WG), 0, s, n, ptr, col, val, x, b, r);
+void launch_cm_residual(hipStream_t s, int n, const int* sptr, const int* len, const int* col, const double* val,
+                        const double* x, const double* b, double* r) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_cm_residual, dim3(cdiv(n, CM_WG)), dim3(CM_WG), 0, s, n, sptr, len, col, val, x, b, r);
 }
 void launch_cm_dots(hipStream_t s, int n, int nd, const double* a0, const double* b0, const double* a1,
                     const double* b1, double* part) {
@@ -385,52 +404,49 @@ void launch_cm_update_r(hipStream_t s, int n, const double* sv, const double* t,
 void launch_cm_update_p(hipStream_t s, int n, const double* r, const double* v, double* p, const double* scal) {
   if (n > 0) hipLaunchKernelGGL(k_cm_update_p, dim3(cdiv(n, CM_WG)), dim3(CM_WG), 0, s, n, r, v, p, scal);
 }
-void launch_cm_div(hipStream_t s, int n, const double* a, const double* d, double* out) {
-  if (n > 0) hipLaunchKernelGGL(k_cm_div, dim3(cdiv(n, CM_WG)), dim3(CM_WG), 0, s, n, a, d, out);
-}
-void launch_cm_mul(hipStream_t s, int n, const double* a, const double* d, double* out) {
-  if (n > 0) hipLaunchKernelGGL(k_cm_mul, dim3(cdiv(n, CM_WG)), dim3(CM_WG), 0, s, n, a, d, out);
+void launch_cm_scale_rows(hipStream_t s, int n, int np, const double* v, double pr, double jr, double* out) {
+  if (n > 0) hipLaunchKernelGGL(k_cm_scale_rows, dim3(cdiv(n, CM_WG)), dim3(CM_WG), 0, s, n, np, v, pr, jr, out);
 }
 void launch_cm_divs(hipStream_t s, int n, double* v, double sc) {
   if (n > 0) hipLaunchKernelGGL(k_cm_divs, dim3(cdiv(n, CM_WG)), dim3(CM_WG), 0, s, n, v, sc);
 }
-void launch_cm_scaling(hipStream_t s, int n, int np, double pr, double pc, double jr, double jc, double* rows,
-                       double* cols) {
-  if (n > 0)
-    hipLaunchKernelGGL(k_cm_scaling, dim3(cdiv(n, CM_WG)), dim3(CM_WG), 0, s, n, np, pr, pc, jr, jc, rows, cols);
-}
-void launch_cm_scatter(hipStream_t s, int ncells, const int* role, const double* v, double* full) {
-  if (ncells > 0) hipLaunchKernelGGL(k_cm_scatter, dim3(cdiv(ncells, CM_WG)), dim3(CM_WG), 0, s, ncells, role, v, full);
+void launch_cm_scatter(hipStream_t s, int ncells, const int* role, const double* v, double d, double* full) {
+  if (ncells > 0)
+    hipLaunchKernelGGL(k_cm_scatter, dim3(cdiv(ncells, CM_WG)), dim3(CM_WG), 0, s, ncells, role, v, d, full);
 }
 void launch_cm_anchor_chunks(hipStream_t s, int nchunks, const int* chunk_ptr, const int* list, const double* v,
-                             double* part) {
-  if (nchunks > 0) hipLaunchKernelGGL(k_cm_anchor_chunks, dim3(nchunks), dim3(CM_WG), 0, s, chunk_ptr, list, v, part);
+                             double d, double* part) {
+  if (nchunks > 0)
+    hipLaunchKernelGGL(k_cm_anchor_chunks, dim3(nchunks), dim3(CM_WG), 0, s, chunk_ptr, list, v, d, part);
 }
 void launch_cm_anchor_totals(hipStream_t s, int nanchors, const int* anchor_cell, const int* anchor_chunk_ptr,
                              const double* part, double* full) {
   if (nanchors > 0)
-    hipLaunchKernelGGL(k_cm_anchor_totals, dim3(cdiv(nanchors, CM_WG)), dim3(CM_WG), 0, s, nanchors, anchor_cell,
-                       anchor_chunk_ptr, part, full);
+    hipLaunchKernelGGL(k_cm_anchor_totals, dim3(nanchors), dim3(CM_WG), 0, s, anchor_cell, anchor_chunk_ptr, part,
+                       full);
 }
-void launch_cm_gather(hipStream_t s, int np, const int* idx, const int* anchor, const double* z, double* out) {
-  if (np > 0) hipLaunchKernelGGL(k_cm_gather, dim3(cdiv(np, CM_WG)), dim3(CM_WG), 0, s, np, idx, anchor, z, out);
+void launch_cm_gather(hipStream_t s, int np, const int* idx, const int* anchor, const double* z, double d,
+                      double* out) {
+  if (np > 0) hipLaunchKernelGGL(k_cm_gather, dim3(cdiv(np, CM_WG)), dim3(CM_WG), 0, s, np, idx, anchor, z, d, out);
 }
 void launch_cm_jump_rhs(hipStream_t s, int nx, const int* ptr, const int* col, const double* val, const double* zp,
-                        const double* rhs, double* jr) {
+                        const double* zfull, const int* idx, const int* anchor, const double* rhs, double d,
+                        double* jr) {
   if (nx > 0)
-    hipLaunchKernelGGL(k_cm_jump_rhs, dim3(cdiv(nx, CM_WG)), dim3(CM_WG), 0, s, nx, ptr, col, val, zp, rhs, jr);
+    hipLaunchKernelGGL(k_cm_jump_rhs, dim3(cdiv(nx, CM_WG)), dim3(CM_WG), 0, s, nx, ptr, col, val, zp, zfull, idx,
+                       anchor, rhs, d, jr);
 }
 void launch_cm_ilu(hipStream_t s, int nx, const int* lptr, const int* lcol, const double* lval, int nllev,
                    const int* llev, const int* lorder, const int* uptr, const int* ucol, const double* uval,
                    const double* udiag, int nulev, const int* ulev, const int* uorder, const double* b, double* y,
-                   double* x, const double* cols) {
+                   double* x, double d) {
   if (nx > 0)
     hipLaunchKernelGGL(k_cm_ilu, dim3(1), dim3(CM_ILU_WG), 0, s, nx, lptr, lcol, lval, nllev, llev, lorder, uptr, ucol,
-                       uval, udiag, nulev, ulev, uorder, b, y, x, cols);
+                       uval, udiag, nulev, ulev, uorder, b, y, x, d);
 }
-void launch_cm_check(hipStream_t s, int n, int np, const double* r, const double* vol, const double* len,
-                     const double* rows, const double* x, double* part) {
-  hipLaunchKernelGGL(k_cm_check, dim3(cm_parts(n)), dim3(CM_WG), 0, s, n, np, r, vol, len, rows, x, part);
+void launch_cm_check(hipStream_t s, int n, int np, const double* r, const double* vol, const double* len, double pr,
+                     double jr, const double* x, double* part) {
+  hipLaunchKernelGGL(k_cm_check, dim3(cm_parts(n)), dim3(CM_WG), 0, s, n, np, r, vol, len, pr, jr, x, part);
 }
 void launch_cm_omitted(hipStream_t s, int m, const int* ptr, const int* col, const double* val, const double* x,
                        const double* rhs, const double* vol, double* part) {
@@ -439,10 +455,10 @@ void launch_cm_omitted(hipStream_t s, int m, const int* ptr, const int* col, con
 void launch_cm_reduce(hipStream_t s, int nparts, int nsets, int max_mask, const double* part, double* out) {
   hipLaunchKernelGGL(k_cm_reduce, dim3(1), dim3(CM_WG), 0, s, nparts, nsets, max_mask, part, out);
 }
-void launch_cm_increment(hipStream_t s, int n, double* x, const double* cols, double scale, const double* inc,
-                         int* flag) {
+void launch_cm_increment(hipStream_t s, int n, int np, double* x, double pc, double jc, double scale,
+                         const double* inc, int* flag) {
   if (n > 0)
-    hipLaunchKernelGGL(k_cm_increment, dim3(cdiv(n, CM_WG)), dim3(CM_WG), 0, s, n, x, cols, scale, inc, flag);
+    hipLaunchKernelGGL(k_cm_increment, dim3(cdiv(n, CM_WG)), dim3(CM_WG), 0, s, n, np, x, pc, jc, scale, inc, flag);
 }
 
 }  // namespace tpamr

@@ -22,9 +22,10 @@ struct CmdgfmDeviceError : std::runtime_error {
 };
 
 // Threads per workgroup of the vector and reduction kernels, their largest
-// grid, the one workgroup of the triangular solves, and the retained
-// entries one workgroup sums per anchor chunk.
+// grid, the one workgroup of the triangular solves, the retained entries one
+// workgroup sums per anchor chunk, and the rows of one sliced-ELL slice.
 constexpr int CM_WG = 256;
+constexpr int CM_SLICE = 64;
 constexpr int CM_MAX_WG = 1024;
 constexpr int CM_ILU_WG = 1024;
 constexpr int CM_ANCHOR_CHUNK = 4096;
@@ -44,12 +45,15 @@ enum CmStep { CM_STEP_INIT = 0, CM_STEP_ALPHA = 1, CM_STEP_OMEGA = 2, CM_STEP_RH
 
 int cm_parts(int n);
 
-// y = rs . (A (cs . x)) over CSR rows, cs / rs may be null
-void launch_cm_spmv(hipStream_t s, int n, const int* ptr, const int* col, const double* val, const double* x,
-                    const double* cs, const double* rs, double* y);
+// Sliced ELL (CM_SLICE rows per slice; slice k's entries start at sptr[k],
+// entry e of its row r at sptr[k] + e CM_SLICE + r; len[i] entries of row i
+// in CSR order). y = rows (A (cols x)) with the per-block scalings: rows pr
+// (pressure, i < np) or jr, columns pc (j < np) or jc.
+void launch_cm_spmv(hipStream_t s, int n, int np, const int* sptr, const int* len, const int* col,
+                    const double* val, const double* x, double pc, double jc, double pr, double jr, double* y);
 // r = b - A x
-void launch_cm_residual(hipStream_t s, int n, const int* ptr, const int* col, const double* val, const double* x,
-                        const double* b, double* r);
+void launch_cm_residual(hipStream_t s, int n, const int* sptr, const int* len, const int* col, const double* val,
+                        const double* x, const double* b, double* r);
 // per-workgroup partials of nd (1 or 2) dot products a_k . b_k
 void launch_cm_dots(hipStream_t s, int n, int nd, const double* a0, const double* b0, const double* a1,
                     const double* b1, double* part);
@@ -63,44 +67,46 @@ void launch_cm_update_r(hipStream_t s, int n, const double* sv, const double* t,
                         double* x, const double* scal);
 // p = r + beta (p - omega v)
 void launch_cm_update_p(hipStream_t s, int n, const double* r, const double* v, double* p, const double* scal);
-// out = a / d, out = a * d (out may alias a), v = v / sc
-void launch_cm_div(hipStream_t s, int n, const double* a, const double* d, double* out);
-void launch_cm_mul(hipStream_t s, int n, const double* a, const double* d, double* out);
+// out = rows v (pr before np, jr after); v = v / sc
+void launch_cm_scale_rows(hipStream_t s, int n, int np, const double* v, double pr, double jr, double* out);
 void launch_cm_divs(hipStream_t s, int n, double* v, double sc);
-// rows / columns of the Krylov scaling: the first np take (pr, pc), the rest (jr, jc)
-void launch_cm_scaling(hipStream_t s, int n, int np, double pr, double pc, double jr, double jc, double* rows,
-                       double* cols);
-// full-cell rhs of the reduced-pressure block: retained cells their value
-// (role = local index), other cells zero; then per-chunk sums of the
-// retained values in anchor order and each anchor's negated total
-void launch_cm_scatter(hipStream_t s, int ncells, const int* role, const double* v, double* full);
+// full-cell rhs of the reduced-pressure block from v / d (d the pressure
+// row scaling): retained cells their value (role = local index), other
+// cells zero; then per-chunk sums of the retained values in anchor order and
+// each anchor's negated total (one workgroup per anchor)
+void launch_cm_scatter(hipStream_t s, int ncells, const int* role, const double* v, double d, double* full);
 void launch_cm_anchor_chunks(hipStream_t s, int nchunks, const int* chunk_ptr, const int* list, const double* v,
-                             double* part);
+                             double d, double* part);
 void launch_cm_anchor_totals(hipStream_t s, int nanchors, const int* anchor_cell, const int* anchor_chunk_ptr,
                              const double* part, double* full);
-// out_i = z[idx_i] - z[anchor_i]
-void launch_cm_gather(hipStream_t s, int np, const int* idx, const int* anchor, const double* z, double* out);
-// jr = -(C zp) + rhs_j over the nx jump rows of C
+// out_i = (z[idx_i] - z[anchor_i]) / d
+void launch_cm_gather(hipStream_t s, int np, const int* idx, const int* anchor, const double* z, double d,
+                      double* out);
+// jr = -(C zp) + rhs_j / d over the nx jump rows of C (ptr null: no C);
+// with zfull, zp_q = zfull[idx_q] - zfull[anchor_q]
 void launch_cm_jump_rhs(hipStream_t s, int nx, const int* ptr, const int* col, const double* val, const double* zp,
-                        const double* rhs, double* jr);
-// x = U^-1 L^-1 b by level-scheduled rows in one workgroup; x /= cols if cols
+                        const double* zfull, const int* idx, const int* anchor, const double* rhs, double d,
+                        double* jr);
+// x = U^-1 L^-1 b by level-scheduled rows in one workgroup, then x /= d
 void launch_cm_ilu(hipStream_t s, int nx, const int* lptr, const int* lcol, const double* lval, int nllev,
                    const int* llev, const int* lorder, const int* uptr, const int* ucol, const double* uval,
                    const double* udiag, int nulev, const int* ulev, const int* uorder, const double* b, double* y,
-                   double* x, const double* cols);
+                   double* x, double d);
 // per-workgroup partials of the defects of a residual: [0] max |r_i| / vol_i
 // over the pressure rows with vol > 0, [1] max |r_k| / len_k over the jump
-// rows, [2] sum r^2, [3] sum (rows r)^2, [4] 1 where x is not finite
-void launch_cm_check(hipStream_t s, int n, int np, const double* r, const double* vol, const double* len,
-                     const double* rows, const double* x, double* part);
+// rows, [2] sum r^2, [3] sum (rows r)^2 (rows pr / jr), [4] 1 where x is not
+// finite
+void launch_cm_check(hipStream_t s, int n, int np, const double* r, const double* vol, const double* len, double pr,
+                     double jr, const double* x, double* part);
 // [0] max |rhs_a - (O x)_a| / vol_a over the omitted rows with vol > 0
 void launch_cm_omitted(hipStream_t s, int m, const int* ptr, const int* col, const double* val, const double* x,
                        const double* rhs, const double* vol, double* part);
 // final reduction: out[k] = sum or maximum (bit k of max_mask, NaN kept)
 void launch_cm_reduce(hipStream_t s, int nparts, int nsets, int max_mask, const double* part, double* out);
-// x += cols (scale inc); *flag = 1 when an entry changed
-void launch_cm_increment(hipStream_t s, int n, double* x, const double* cols, double scale, const double* inc,
-                         int* flag);
+// x += cols (scale inc) (cols pc before np, jc after); *flag = 1 when an
+// entry changed
+void launch_cm_increment(hipStream_t s, int n, int np, double* x, double pc, double jc, double scale,
+                         const double* inc, int* flag);
 
 // LinearSolverStats-like summary of one block solve (the solve fields of
 // CMDGFMProjectionStats).
@@ -133,10 +139,11 @@ void cmdgfm_crout_ilu(double tau, JumpFactor& f);
 
 class CmdgfmSystem {
  public:
-  // Enqueues z = V-cycle(r) of the shared pressure hierarchy on full-cell
-  // device vectors (integrated r; the V-cycle may rewrite r with its own
-  // values).
-  using Vcycle = std::function<void(double* r, double* z)>;
+  // Enqueues z = V-cycle(r) of the shared pressure hierarchy on a full-cell
+  // device vector (integrated r; the V-cycle may rewrite r with its own
+  // values) and returns z's device address (full-cell FP64, valid until the
+  // next call; defined up to a constant per pressure component).
+  using Vcycle = std::function<const double*(double* r)>;
 
   // Checks and factors the assembled system on the host; with a stream,
   // uploads it. Index arrays are 0-based; CSR column indices sorted per row.

@@ -256,7 +256,7 @@ struct tpamr_ctx {
   // assembled CMDGFM block system (tpamr_cmdgfm_set_system) and the device
   // staging of its host-array solve
   std::unique_ptr<CmdgfmSystem> cmd;
-  DevArray<double> cm_rhs, cm_x, cm_prev;
+  DevArray<double> cm_rhs, cm_x, cm_prev, cm_z;
   const TreeMesh& mesh() const {
     if (H) return *H->levels[0]->mesh;
     if (mesh_pending) return *mesh_pending;
@@ -3205,13 +3205,44 @@ CmdgfmSystem& need_cmdgfm(tpamr_ctx* c) {
   return *c->cmd;
 }
 
-// the shared hierarchy's V-cycle on full-cell device vectors
+// One V-cycle of the shared hierarchy for the reduced-pressure block: the
+// rhs goes straight into the finest level, whose zero-start iterate the
+// first pre-sweep forms, and the answer stays where the V-cycle leaves it
+// (its constant per component is irrelevant to the differences the block
+// takes). Hierarchies with several components, or fewer than two
+// pre-sweeps, take the preconditioner's full path.
+template <typename T>
+const double* cmdgfm_vcycle_t(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double* r, double* zout) {
+  if (!x0_fused(c)) {
+    vcycle_into<T>(c, lv, r, zout);
+    return zout;
+  }
+  hipStream_t s = c->stream;
+  const int N = c->topo[0].ncells;
+  if constexpr (std::is_same<T, float>::value)
+    launch_cast_d2f(s, N, r, lv[0].b.p);
+  else
+    launch_copy(s, N, r, lv[0].b.p);
+  T* z = enqueue_vcycle<T>(c, lv, &c->rs.st->done, ZStats{}, true);
+  if constexpr (std::is_same<T, float>::value) {
+    launch_cast_f2d(s, N, z, zout);
+    return zout;
+  } else {
+    return z;
+  }
+}
+
 CmdgfmSystem::Vcycle cmdgfm_vcycle(tpamr_ctx* c) {
-  return [c](double* r, double* z) {
-    if (c->fp32())
-      vcycle_into<float>(c, c->lf, r, z);
-    else
-      vcycle_into<double>(c, c->ld, r, z);
+  const int N = c->topo[0].ncells;
+  c->cm_z.alloc(N);
+  // the Krylov state's stop flag, which the V-cycle kernels read, is clear
+  CgState init{};
+  init.scale = 1.0;
+  *c->st_host = init;
+  HIPCHECK(hipMemcpyAsync(c->rs.st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+  return [c](double* r) -> const double* {
+    if (c->fp32()) return cmdgfm_vcycle_t<float>(c, c->lf, r, c->cm_z.p);
+    return cmdgfm_vcycle_t<double>(c, c->ld, r, c->cm_z.p);
   };
 }
 

@@ -398,3 +398,93 @@ def wave_steps(solver, steps, dt, density=WAVE_DENSITY, gravity=WAVE_GRAVITY):
         jumps = gravity_jumps(solver, la, ga, density=density, gravity=gravity, centroid_offsets=off)
         out.append((kappa, jumps))
     return out, frac
+
+
+def cmdgfm_block_workload(n=1024, block=16, dt=0.02, rho_liquid=1000.0, rho_gas=1.0, centre=(0.5, 0.5),
+                          radius=0.27, seed=SEED + 3):
+    """A CMDGFM-shaped reduced-pressure/jump system on the unit square (walls,
+    n^2 cells in block^2 blocks) with a liquid disk of `radius` at `centre`:
+    pressure operator 1/rho with the harmonic face mean (rho_liquid inside,
+    rho_gas outside), one anchor (cell 0), every face whose cells lie on
+    opposite sides of the circle a jump unknown. Blocks: the pressure
+    operator's stiffness restricted to the retained cells; B: -g / +g on the
+    face's minus / plus cell (g = 0.3 u |S_mp|); C: the pressure difference
+    across the face times u / (rho_liquid L); J: diagonal U(2, 4) plus
+    U(-0.4, 0.4) couplings of faces sharing a cell (u ~ U(0.5, 1.5)); rhs of
+    order-one volume and jump defects. The anchor's omitted row makes the
+    full system consistent. Returns (mesh, face coefficients, dict of the
+    CMDGFMBlockSystem arguments and rhs, dt, density)."""
+    import scipy.sparse as sp
+
+    from .hierarchy import HostHierarchy
+
+    drawn = [0]
+
+    def uniform(lo, hi, k):
+        # the next k numbers of the splitmix64 stream of `seed`, vectorised
+        step = np.uint64(0x9E3779B97F4A7C15)
+        with np.errstate(over="ignore"):
+            z = np.uint64(seed & MASK64) + (np.arange(drawn[0] + 1, drawn[0] + k + 1, dtype=np.uint64) * step)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        drawn[0] += k
+        return lo + (hi - lo) * ((z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53)))
+
+    mesh = BlockTreeMesh(base_mesh=(n, n), block_size=(block, block), periodic=(False, False))
+    x, y = mesh.cell_centers()
+    rho = np.where(np.hypot(x - centre[0], y - centre[1]) < radius, rho_liquid, rho_gas)
+    m, p = mesh.face_minus, mesh.face_plus
+    kappa = 2.0 / (rho[m] + rho[p])
+    s = HostHierarchy(mesh, kappa).level_matrix(0)
+    ncells = mesh.ncells
+    idx = np.arange(1, ncells)
+    anchor = np.zeros(ncells - 1, np.int64)
+    phi = np.hypot(x - centre[0], y - centre[1]) - radius
+    mixed = np.flatnonzero(np.sign(phi[m]) != np.sign(phi[p]))
+    nx = len(mixed)
+    fm, fp = m[mixed], p[mixed]
+    smp = np.asarray(s[fm, fp]).ravel() if nx else np.zeros(0)
+    g = 0.3 * np.abs(smp) * uniform(0.5, 1.5, nx)
+    b_full = sp.csr_matrix((np.concatenate([-g, g]), (np.concatenate([fm, fp]), np.tile(np.arange(nx), 2))),
+                           shape=(ncells, nx))
+    cw = uniform(0.5, 1.5, nx) / rho_liquid
+    local = np.concatenate([[-1], np.arange(ncells - 1)])
+    rows = np.tile(np.arange(nx), 2)
+    cols = np.concatenate([local[fm], local[fp]])
+    vals = np.concatenate([-cw, cw])
+    keep = cols >= 0
+    c_blk = sp.csr_matrix((vals[keep], (rows[keep], cols[keep])), shape=(nx, ncells - 1))
+    # faces sharing a cell
+    cell_face = np.concatenate([np.stack([fm, np.arange(nx)], 1), np.stack([fp, np.arange(nx)], 1)])
+    cell_face = cell_face[np.lexsort((cell_face[:, 1], cell_face[:, 0]))]
+    jr, jc = [np.arange(nx)], [np.arange(nx)]
+    start = 0
+    while start < len(cell_face):
+        end = start
+        while end < len(cell_face) and cell_face[end, 0] == cell_face[start, 0]:
+            end += 1
+        ks = cell_face[start:end, 1]
+        if len(ks) > 1:
+            a, b = np.meshgrid(ks, ks, indexing="ij")
+            off = a != b
+            jr.append(a[off])
+            jc.append(b[off])
+        start = end
+    jr, jc = np.concatenate(jr), np.concatenate(jc)
+    jv = np.concatenate([uniform(2.0, 4.0, nx), uniform(-0.4, 0.4, len(jr) - nx)])
+    j_blk = sp.csr_matrix((jv, (jr, jc)), shape=(nx, nx))
+    matrix = sp.bmat([[s[idx][:, idx], b_full[idx]], [c_blk, j_blk]], format="csr")
+    matrix.sum_duplicates()
+    matrix.sort_indices()
+    vol = mesh.volumes
+    rp = vol[idx] / (dt * dt) * (0.5 * np.sin(3.0 * x[idx] + 1.0) * np.cos(2.0 * y[idx]) +
+                                 0.1 * uniform(-1.0, 1.0, ncells - 1)) / rho_liquid
+    h = np.sqrt(vol)
+    lengths = np.minimum(h[fm], h[fp])
+    rj = lengths / (dt * dt) * uniform(-1.0, 1.0, nx)
+    omitted = sp.hstack([s[[0]][:, idx], b_full[[0]]], format="csr")
+    system = dict(matrix=matrix, pressure_indices=idx, pressure_anchors=anchor, pressure_volumes=vol[idx],
+                  jump_lengths=lengths, omitted=omitted, omitted_rhs=np.array([-rp.sum()]),
+                  omitted_volumes=vol[[0]])
+    return mesh, kappa, system, np.concatenate([rp, rj]), dt, max(rho_liquid, rho_gas)
