@@ -178,26 +178,46 @@ struct Momentum::Impl {
     }
     for (int l = 0; l + 1 < nl; ++l) {
       StagLevel& L = *lv[l];
-      // the first sweep starts from zero (x = 0 + w D^-1 (b - 0))
-      for (int q = 0; q < mg.pre; ++q) {
+      // the first sweep starts from zero (x = 0 + w D^-1 (b - 0)); with two
+      // or more pre-sweeps the first two run as one pass
+      int q = 0;
+      if (mg.pre >= 2) {
+        launch_st_mg_smooth2z(s, L.nx, L.ny, L.k, L.w, bl[l], oth[l]);
+        std::swap(cur[l], oth[l]);
+        q = 2;
+      }
+      for (; q < mg.pre; ++q) {
         launch_st_mg_smooth(s, L.nx, L.ny, L.k, L.w, q == 0 ? nullptr : cur[l], bl[l], oth[l]);
         std::swap(cur[l], oth[l]);
       }
       if (mg.pre == 0) launch_st_mg_smooth(s, L.nx, L.ny, L.k, 0.0, nullptr, bl[l], cur[l]);
-      launch_st_mg_residual(s, L.nx, L.ny, L.k, bl[l], cur[l], L.r.p);
-      launch_st_mg_restrict(s, lv[l + 1]->nx, lv[l + 1]->ny, comp, L.r.p, lv[l + 1]->b.p);
+      launch_st_mg_restrict_res(s, lv[l + 1]->nx, lv[l + 1]->ny, comp, L.k, bl[l], cur[l], lv[l + 1]->b.p);
     }
     const StagLevel& C = *lv.back();
     launch_st_dense_solve(s, C.nx * C.ny, coarse_inv.p, bl[nl - 1], cur[nl - 1]);
+    bool in_z = false;
     for (int l = nl - 2; l >= 0; --l) {
       StagLevel& L = *lv[l];
-      launch_st_mg_prolong(s, lv[l + 1]->nx, lv[l + 1]->ny, comp, cur[l + 1], cur[l]);
+      const int ncx = lv[l + 1]->nx, ncy = lv[l + 1]->ny;
+      if (mg.post == 0) {
+        launch_st_mg_prolong(s, ncx, ncy, comp, cur[l + 1], cur[l]);
+        continue;
+      }
+      // the finest level's last sweep writes z itself
       for (int q = 0; q < mg.post; ++q) {
-        launch_st_mg_smooth(s, L.nx, L.ny, L.k, L.w, cur[l], bl[l], oth[l]);
-        std::swap(cur[l], oth[l]);
+        double* out = (l == 0 && q == mg.post - 1) ? z : oth[l];
+        if (q == 0)
+          launch_st_mg_prolong_smooth(s, ncx, ncy, comp, L.k, L.w, cur[l + 1], cur[l], bl[l], out);
+        else
+          launch_st_mg_smooth(s, L.nx, L.ny, L.k, L.w, cur[l], bl[l], out);
+        if (out == z) {
+          in_z = true;
+        } else {
+          std::swap(cur[l], oth[l]);
+        }
       }
     }
-    launch_copy_grid(cur[0], z);
+    if (!in_z) launch_copy_grid(cur[0], z);
   }
   void launch_copy_grid(const double* from, double* to) {
     MCHECK(hipMemcpyAsync(to, from, sizeof(double) * n, hipMemcpyDeviceToDevice, s));

@@ -453,6 +453,88 @@ __global__ void k_st_mg_prolong(int ncx, int ncy, int comp, const double* __rest
   xf[t] = xf[t] + v;
 }
 
+// M Q at (i, j) for Q given pointwise by q(index), in mg_apply's order
+template <typename F>
+__device__ __forceinline__ double mg_apply_f(int nx, int ny, const StagCoef& k, F q, int i, int j) {
+#pragma clang fp contract(off)
+  double acc = k.md * q(j * nx + i);
+  acc = acc + k.mx * q(j * nx + wrap(i - 1, nx));
+  acc = acc + k.mx * q(j * nx + wrap(i + 1, nx));
+  acc = acc + k.my * q(wrap(j - 1, ny) * nx + i);
+  acc = acc + k.my * q(wrap(j + 1, ny) * nx + i);
+  return acc;
+}
+
+// the zero-start sweep and the next one in one pass: x1 = (w D^-1) b at the
+// five points, x2 = x1 + (w D^-1)(b - M x1); the same operations as two
+// k_st_mg_smooth launches
+__global__ void k_st_mg_smooth2z(int nx, int ny, StagCoef k, double w, const double* __restrict__ b,
+                                 double* __restrict__ x_out) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nx * ny) return;
+  const double wd = w * k.mdinv;
+  auto x1 = [&](int q) { return wd * b[q]; };
+  const double mx = mg_apply_f(nx, ny, k, x1, t % nx, t / nx);
+  x_out[t] = x1(t) + wd * (b[t] - mx);
+}
+
+// restriction of the fine residual b - M x computed on the fly (the fine
+// residuals of k_st_mg_restrict's six faces)
+__global__ void k_st_mg_restrict_res(int ncx, int ncy, int comp, StagCoef k, const double* __restrict__ bf,
+                                     const double* __restrict__ xf, double* __restrict__ bc) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncx * ncy) return;
+  const int ic = t % ncx, jc = t / ncx;
+  const int nfx = 2 * ncx, nfy = 2 * ncy;
+  auto res = [&](int i, int j) { return bf[j * nfx + i] - mg_apply(nfx, nfy, k, xf, i, j); };
+  double acc = 0.0;
+  for (int s = 0; s < 2; ++s) {
+    if (comp == 0) {
+      const int jf = 2 * jc + s, i0 = 2 * ic;
+      acc = acc + res(i0, jf);
+      acc = acc + 0.5 * res(i0 + 1, jf);
+      acc = acc + 0.5 * res(wrap(i0 - 1, nfx), jf);
+    } else {
+      const int ifc = 2 * ic + s, j0 = 2 * jc;
+      acc = acc + res(ifc, j0);
+      acc = acc + 0.5 * res(ifc, j0 + 1);
+      acc = acc + 0.5 * res(ifc, wrap(j0 - 1, nfy));
+    }
+  }
+  bc[t] = acc;
+}
+
+// the prolonged correction of fine unknown (i, j), as k_st_mg_prolong adds it
+__device__ __forceinline__ double prolonged(int ncx, int ncy, int comp, const double* __restrict__ xc, int i, int j) {
+#pragma clang fp contract(off)
+  if (comp == 0) {
+    const int jc = j >> 1;
+    if ((i & 1) == 0) return xc[jc * ncx + (i >> 1)];
+    const int lo = i >> 1;
+    return 0.5 * xc[jc * ncx + lo] + 0.5 * xc[jc * ncx + wrap(lo + 1, ncx)];
+  }
+  const int ic = i >> 1;
+  if ((j & 1) == 0) return xc[(j >> 1) * ncx + ic];
+  const int lo = j >> 1;
+  return 0.5 * xc[lo * ncx + ic] + 0.5 * xc[wrap(lo + 1, ncy) * ncx + ic];
+}
+
+// prolongation and the first post-sweep in one pass: xp = x + P xc at the
+// five points, out = xp + (w D^-1)(b - M xp)
+__global__ void k_st_mg_prolong_smooth(int ncx, int ncy, int comp, StagCoef k, double w,
+                                       const double* __restrict__ xc, const double* __restrict__ xf,
+                                       const double* __restrict__ b, double* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int nfx = 2 * ncx, nfy = 2 * ncy;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nfx * nfy) return;
+  auto xp = [&](int q) { return xf[q] + prolonged(ncx, ncy, comp, xc, q % nfx, q / nfx); };
+  const double mx = mg_apply_f(nfx, nfy, k, xp, t % nfx, t / nfx);
+  out[t] = xp(t) + (w * k.mdinv) * (b[t] - mx);
+}
+
 // one wavefront per row of the dense inverse
 __global__ void __launch_bounds__(256) k_st_dense_solve(int n, const double* __restrict__ inv,
                                                         const double* __restrict__ b, double* __restrict__ x) {
@@ -506,6 +588,20 @@ void launch_st_mg_restrict(hipStream_t s, int ncx, int ncy, int comp, const doub
 }
 void launch_st_mg_prolong(hipStream_t s, int ncx, int ncy, int comp, const double* x_c, double* x_f) {
   hipLaunchKernelGGL(k_st_mg_prolong, dim3(cdiv2(4 * ncx * ncy, 256)), dim3(256), 0, s, ncx, ncy, comp, x_c, x_f);
+}
+void launch_st_mg_smooth2z(hipStream_t s, int nx, int ny, const StagCoef& k, double w, const double* b,
+                           double* x_out) {
+  hipLaunchKernelGGL(k_st_mg_smooth2z, dim3(cdiv2(nx * ny, 256)), dim3(256), 0, s, nx, ny, k, w, b, x_out);
+}
+void launch_st_mg_restrict_res(hipStream_t s, int ncx, int ncy, int comp, const StagCoef& k, const double* b_f,
+                               const double* x_f, double* b_c) {
+  hipLaunchKernelGGL(k_st_mg_restrict_res, dim3(cdiv2(ncx * ncy, 256)), dim3(256), 0, s, ncx, ncy, comp, k, b_f, x_f,
+                     b_c);
+}
+void launch_st_mg_prolong_smooth(hipStream_t s, int ncx, int ncy, int comp, const StagCoef& k, double w,
+                                 const double* x_c, const double* x_f, const double* b, double* out) {
+  hipLaunchKernelGGL(k_st_mg_prolong_smooth, dim3(cdiv2(4 * ncx * ncy, 256)), dim3(256), 0, s, ncx, ncy, comp, k, w,
+                     x_c, x_f, b, out);
 }
 void launch_st_dense_solve(hipStream_t s, int n, const double* inv, const double* b, double* x) {
   hipLaunchKernelGGL(k_st_dense_solve, dim3(cdiv2(n, 4)), dim3(256), 0, s, n, inv, b, x);

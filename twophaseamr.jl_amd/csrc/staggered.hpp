@@ -100,6 +100,15 @@ void launch_st_mg_restrict(hipStream_t s, int ncx, int ncy, int comp, const doub
 void launch_st_mg_prolong(hipStream_t s, int ncx, int ncy, int comp, const double* x_c, double* x_f);
 // x = inv b with a dense n x n row-major inverse
 void launch_st_dense_solve(hipStream_t s, int n, const double* inv, const double* b, double* x);
+// fused passes with the operations of the unfused sequence: the zero-start
+// sweep and the next one; the restriction of the residual b_f - M x_f; the
+// prolongation (x_f + P x_c, not stored) and the first post-sweep into out
+void launch_st_mg_smooth2z(hipStream_t s, int nx, int ny, const StagCoef& k, double w, const double* b,
+                           double* x_out);
+void launch_st_mg_restrict_res(hipStream_t s, int ncx, int ncy, int comp, const StagCoef& k, const double* b_f,
+                               const double* x_f, double* b_c);
+void launch_st_mg_prolong_smooth(hipStream_t s, int ncx, int ncy, int comp, const StagCoef& k, double w,
+                                 const double* x_c, const double* x_f, const double* b, double* out);
 // x += alpha p, r -= alpha Ap; partial maxima of |r| / V (no preconditioning)
 void launch_st_cg_update_xr(hipStream_t s, int n, double inv_vol, double* x, double* r, const double* p,
                             const double* Ap, const double* scal, double* part);
