@@ -329,7 +329,10 @@ def trace_summary(db, title, command, top=30):
         if len(short) > 90:
             short = short[:87] + "..."
         lines.append(f"| `{short}` | {n} | {t / 1e3:.2f} | {t / n:.2f} | {lo:.2f} | {hi:.2f} | {g} |")
-    sweep = [(e0 - s0) / 1e3 for name, s0, e0, g in rows if SWEEP_KERNEL.replace(" ", "") in name.replace(" ", "")]
+    # the finest-level plain sweep (the template's trailing zero-start flag off)
+    target = SWEEP_KERNEL.replace(" ", "")[:-1] + ",false>"
+    sweep = [(e0 - s0) / 1e3 for name, s0, e0, g in rows
+             if name.replace(" ", "").replace("void", "", 1).replace("tpamr::", "").split("(")[0] == target]
     if sweep:
         med = statistics.median(sweep)
         full = [d for d in sweep if d >= 0.5 * med]
