@@ -834,6 +834,9 @@ def main():
     # the plain-method configuration's one-velocity step (128^2 cells, two
     # steps), field for field against the oracle
     out["taylor_green_step"] = taylor_green_run()
+    # the CMDGFM reduced-pressure/jump block solve (2048^2 cells), with the
+    # oracle's answer and rate at 256^2
+    out["cmdgfm_block_solve"] = cmdgfm_run(n=2048, steps=3, warmup=1)
     print(json.dumps(out), flush=True)
 
 

@@ -277,11 +277,11 @@ struct CmdgfmSystem::Impl {
     CCHECK(hipStreamSynchronize(s));
   }
   // defects and norms of residual r (and finiteness of x): [0] pressure,
-  // [1] jump, [2] sum r^2, [3] sum (rows r)^2, [4] x not finite
+  // [1] jump, [2] sum r^2, [3] sum (rows r)^2, [4] x not finite, [5] max |r|
   void check(const double* r, const double* x) {
     launch_cm_check(s, n, np, r, vol.p, len.p, prow, jrow, x, part.p);
-    launch_cm_reduce(s, cm_parts(n), 5, 0b10011, part.p, out.p);
-    read_out(5);
+    launch_cm_reduce(s, cm_parts(n), 6, 0b110011, part.p, out.p);
+    read_out(6);
   }
   double anchor_defect(const double* x, double dt) {
     if (nomit == 0) return 0.0;
@@ -384,9 +384,8 @@ struct CmdgfmSystem::Impl {
         if (!previous || np == 0) continue;
         launch_cm_gather(s, np, idx.p, anc.p, previous, 1.0, cand.p);
         check(cand.p, cand.p);
-        const bool nonfinite = host_out[4] != 0.0;
-        if (!(host_out[2] > 0.0) && !nonfinite) continue;  // every entry zero
-        if (nonfinite) continue;
+        // an all-zero increment is no guess, a nonfinite one is rejected
+        if (host_out[5] == 0.0 || host_out[4] != 0.0) continue;
       }
       if (nx) {
         if (!use_pc) continue;
@@ -589,7 +588,7 @@ CmdgfmSystem::CmdgfmSystem(int ncells, int np, int nx, const int* ptr, const int
   D.full.alloc(std::max(ncells, 1));
   D.jr.alloc(std::max(nx, 1));
   D.jy.alloc(std::max(nx, 1));
-  D.part.alloc(size_t(CM_MAX_WG) * 5);
+  D.part.alloc(size_t(CM_MAX_WG) * 6);
   D.apart.alloc(std::max(D.nchunks, 1));
   D.scal.alloc(CM_SCALARS);
   D.out.alloc(8);

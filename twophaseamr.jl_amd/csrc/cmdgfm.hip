@@ -321,7 +321,7 @@ __global__ void __launch_bounds__(CM_WG) k_cm_check(int n, int np, const double*
                                                     double pr, double jr, const double* __restrict__ x,
                                                     double* __restrict__ part) {
 #pragma clang fp contract(off)
-  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double ri = r[i];
     if (i < np) {
@@ -333,8 +333,9 @@ __global__ void __launch_bounds__(CM_WG) k_cm_check(int n, int np, const double*
     const double q = (i < np ? pr : jr) * ri;
     v[3] = v[3] + q * q;
     if (x && !isfinite(x[i])) v[4] = 1.0;
+    v[5] = nanmax(v[5], fabs(ri));
   }
-  wg_partials<5>(v, 0b10011, part);
+  wg_partials<6>(v, 0b110011, part);
 }
 
 __global__ void __launch_bounds__(CM_WG) k_cm_omitted(int m, const int* __restrict__ ptr, const int* __restrict__ col,

@@ -95,7 +95,7 @@ void launch_cm_ilu(hipStream_t s, int nx, const int* lptr, const int* lcol, cons
 // per-workgroup partials of the defects of a residual: [0] max |r_i| / vol_i
 // over the pressure rows with vol > 0, [1] max |r_k| / len_k over the jump
 // rows, [2] sum r^2, [3] sum (rows r)^2 (rows pr / jr), [4] 1 where x is not
-// finite
+// finite, [5] max |r_i|
 void launch_cm_check(hipStream_t s, int n, int np, const double* r, const double* vol, const double* len, double pr,
                      double jr, const double* x, double* part);
 // [0] max |rhs_a - (O x)_a| / vol_a over the omitted rows with vol > 0
