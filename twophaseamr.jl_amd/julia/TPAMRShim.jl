@@ -27,6 +27,7 @@ Loading the shim needs the built library at `ENV["TPAMR_LIB"]` or `../lib/libtpa
 module TPAMRShim
 
 using Krylov
+using SparseArrays
 using TwoPhaseAMR
 
 const LIB = get(ENV, "TPAMR_LIB", joinpath(@__DIR__, "..", "lib", "libtpamr_hip.so"))
@@ -76,6 +77,9 @@ mutable struct GpuCgWorkspace
     # open pressure sides and boundary coefficients last given to the context
     sides::Vector{Cint}
     boundary::Vector{Float64}
+    # the momentum operators the context holds (kinematic viscosity, scheme;
+    # NaN: none since the last mesh)
+    momentum::Tuple{Float64, Cint}
 end
 
 Krylov.solution(ws::GpuCgWorkspace) = ws.x
@@ -107,7 +111,7 @@ function gpu_krylov(; device::Integer = 0, precision::Symbol = :fp32_vcycle,
         _check(handle[], ccall((:tpamr_set_multigrid, LIB), Cint, (Ptr{Cvoid}, Ptr{Cint}), handle[], opts))
         n = length(rhs)
         ws = GpuCgWorkspace(handle[], n, n, zeros(n), 0, typemax(UInt), nothing, Float64[], Cint[0, 0, 0, 0],
-            Float64[])
+            Float64[], (NaN, Cint(-1)))
         finalizer(_destroy!, ws)
         return ws
     end
@@ -152,6 +156,7 @@ function _sync!(ws::GpuCgWorkspace, operator)
         ws.mesh_identity = mesh
         ws.mesh_generation = generation
         empty!(ws.coefficient)
+        ws.momentum = (NaN, Cint(-1))   # tpamr_set_mesh drops the momentum operators
     end
     # open pressure sides and their boundary coefficients (elliptic.jl:76-152):
     # given before the face coefficients they shape
@@ -365,6 +370,133 @@ function step_device!(ws::GpuCgWorkspace, d_u::Ptr{Float64}, d_fraction::Ptr{Flo
          Ref{TpamrStats}),
         ws.ctx, d_u, C_NULL, Float64(dt), Float64(tolerance), Cint(max_cycles), C_NULL, Float64(dt), d_phi, st))
     return st[]
+end
+
+# ---------------------------------------------------------------------------
+# The one-velocity step on a uniform mesh periodic in both directions
+
+"""
+    momentum_step!(ws, velocity, pressure, dt; viscosity, scheme = 2, force = nothing,
+                   tolerance = sqrt(eps()), max_cycles = 100)
+
+`step!(velocity, pressure, solver, dt, workspace)` of one phase
+(src/single_phase/stepping.jl:194-264) on the context of `ws`, whose mesh and
+pressure face coefficients `1/rho` `_sync!` has set: `velocity` holds the
+interior face velocities (the mesh's face order), `pressure` the cell
+pressures (cell order), both updated in place; `viscosity` is the kinematic
+viscosity mu/rho, `scheme` 0 Upwind, 1 Centred, 2 LaxWendroff; `force` the body
+force's face values at the midpoint time. The momentum operators are set when
+the viscosity, the scheme or the mesh changed. Returns the projection's
+`TpamrStats`.
+"""
+function momentum_step!(ws::GpuCgWorkspace, velocity::Vector{Float64}, pressure::Vector{Float64}, dt::Real;
+        viscosity::Real, scheme::Integer = 2, force = nothing, tolerance::Real = sqrt(eps()),
+        max_cycles::Integer = 100)
+    wanted = (Float64(viscosity), Cint(scheme))
+    if ws.momentum !== wanted
+        _check(ws.ctx, ccall((:tpamr_set_momentum, LIB), Cint, (Ptr{Cvoid}, Float64, Cint), ws.ctx, wanted...))
+        ws.momentum = wanted
+    end
+    f = force === nothing ? C_NULL : force
+    st = Ref{TpamrStats}()
+    GC.@preserve velocity pressure force _check(ws.ctx, ccall((:tpamr_step, LIB), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Float64, Ptr{Float64}, Float64, Cint, Ref{TpamrStats}),
+        ws.ctx, velocity, pressure, Float64(dt), f, Float64(tolerance), Cint(max_cycles), st))
+    return st[]
+end
+
+# ---------------------------------------------------------------------------
+# The CMDGFM reduced-pressure/jump block solve
+
+"""Mirror of `tpamr_cmdgfm_stats` (include/tpamr.h), field for field."""
+struct TpamrCmdgfmStats
+    converged::Cint
+    iterations::Cint
+    krylov_runs::Cint
+    initial_guess_used::Cint
+    pressure_unknowns::Cint
+    jump_unknowns::Cint
+    initial_residual::Float64
+    final_residual::Float64
+    pressure_defect::Float64
+    jump_defect::Float64
+    anchor_defect::Float64
+    asymmetry::Float64
+    jump_diagonal_margin::Float64
+end
+
+# rows of A (CSC) as 0-based CSR with increasing columns
+function _csr0(A)
+    T = SparseArrays.sparse(transpose(A))
+    return Cint.(T.colptr .- 1), Cint.(SparseArrays.rowvals(T) .- 1), SparseArrays.nonzeros(T)
+end
+
+# the equations the reduced system omits (anchors, isolated cells), as
+# _cmdgfm_anchor_residual_defect evaluates them (cmdgfm_projection.jl:1848-1893)
+function _cmdgfm_omitted_rows(workspace, assembled, dt)
+    op, cache = assembled.pressure_operator, workspace.pressure_cache
+    omitted = [c for c in eachindex(op.volumes) if op.volumes[c] > 0 && iszero(cache.pressure_lookup[c])]
+    row = Dict(c => k for (k, c) in pairs(omitted))
+    mesh = op.mesh
+    bx, by = mesh.block_size
+    I, J, V = Int[], Int[], Float64[]
+    for (k, f) in pairs(assembled.mixed_faces)
+        face = mesh.faces[f]
+        m = TwoPhaseAMR._cell_index(face.minus, bx, by)
+        p = TwoPhaseAMR._cell_index(face.plus, bx, by)
+        m == p && continue
+        g = assembled.data.coupling[f] * face.area
+        haskey(row, m) && (push!(I, row[m]); push!(J, k); push!(V, -g))
+        haskey(row, p) && (push!(I, row[p]); push!(J, k); push!(V, g))
+    end
+    jumps = SparseArrays.sparse(I, J, V, length(omitted), length(assembled.mixed_faces))
+    O = hcat(op.stiffness[omitted, assembled.pressure_indices], jumps)
+    w, inherited = assembled.inherited_weight, assembled.inherited_cell_flux
+    rhs = [cache.cell_flux[c] / dt - (iszero(w) || inherited === nothing ? 0.0 : w * inherited[c]) for c in omitted]
+    return O, rhs, op.volumes[omitted]
+end
+
+"""
+    cmdgfm_solve!(ws, workspace, assembled, solver, dt; tolerance, max_iterations,
+                  warm_start = true, previous_increment = nothing, drop_tolerance = 1e-3)
+
+The solve inside `_project_cmdgfm_assembled!` (src/two_phase/projection/
+cmdgfm_projection.jl:2201-2318) on the GPU: the assembled system is handed over
+(`tpamr_cmdgfm_set_system`, the jump Schur complement and its ILU formed by the
+library), the Krylov scaling, the block preconditioner over the context's
+pressure hierarchy (its coefficients those of `assembled.pressure_operator`),
+BiCGSTAB and the replacement loop run on the device. Returns the solution
+(reduced pressures, then jumps) and the `TpamrCmdgfmStats`; the reference's
+recovery (pressure increment, velocity correction, mixture and normal-velocity
+defects) follows unchanged.
+"""
+function cmdgfm_solve!(ws::GpuCgWorkspace, workspace, assembled, solver, dt::Real; tolerance::Real,
+        max_iterations::Integer, warm_start::Bool = true, previous_increment = nothing,
+        drop_tolerance::Real = 1.0e-3)
+    op = assembled.pressure_operator
+    np, nx = length(assembled.pressure_indices), length(assembled.mixed_faces)
+    ptr, col, val = _csr0(assembled.matrix)
+    O, orhs, ovol = _cmdgfm_omitted_rows(workspace, assembled, dt)
+    optr, ocol, oval = _csr0(O)
+    lengths = [TwoPhaseAMR._projection_face_spacing(op.mesh, f) for f in assembled.mixed_faces]
+    cells = Cint.(assembled.pressure_indices .- 1)
+    anchors = Cint.(assembled.pressure_anchors .- 1)
+    volumes = op.volumes[assembled.pressure_indices]
+    _check(ws.ctx, ccall((:tpamr_cmdgfm_set_system, LIB), Cint,
+        (Ptr{Cvoid}, Cint, Cint, Ptr{Cint}, Ptr{Cint}, Ptr{Float64}, Ptr{Cint}, Ptr{Cint}, Ptr{Float64},
+         Ptr{Float64}, Cint, Ptr{Cint}, Ptr{Cint}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64),
+        ws.ctx, np, nx, ptr, col, val, cells, anchors, volumes, lengths, size(O, 1), optr, ocol, oval, orhs,
+        ovol, Float64(drop_tolerance)))
+    x = zeros(np + nx)
+    prev = previous_increment === nothing ? C_NULL : previous_increment
+    st = Ref{TpamrCmdgfmStats}()
+    rhs = assembled.right_hand_side
+    GC.@preserve rhs x prev _check(ws.ctx, ccall((:tpamr_cmdgfm_solve, LIB), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Float64, Cint, Cint, Ptr{Float64}, Cint,
+         Ref{TpamrCmdgfmStats}),
+        ws.ctx, rhs, x, Float64(dt), Float64(maximum(solver.density)), Float64(tolerance), Cint(max_iterations),
+        Cint(warm_start), prev, Cint(solver.preconditioner === :block), st))
+    return x, st[]
 end
 
 end # module
