@@ -167,3 +167,29 @@ def test_new_mesh_drops_the_system():
     with pytest.raises(RuntimeError, match="block system"):
         gsys.solve(d["rhs"], dt=d["dt"], density=d["density"])
     solver.close()
+
+
+def test_block_solve_at_size():
+    """The 1024^2-cell system of tpamr.workloads.cmdgfm_block_workload
+    (beyond the oracle's reach): the solve converges and the residual
+    recomputed on the host from the assembled matrix meets the pressure and
+    jump defects the library reports."""
+    from tpamr.workloads import cmdgfm_block_workload
+
+    _print("case cmdgfm 1024^2")
+    mesh, kappa, sysd, rhs, dt, rho = cmdgfm_block_workload(n=1024)
+    solver = tpamr.EllipticSolver(tpamr.PoissonOperator(mesh, kappa))
+    gsys = tpamr.CMDGFMBlockSystem(solver, sysd["matrix"], sysd["pressure_indices"], sysd["pressure_anchors"],
+                                   sysd["pressure_volumes"], sysd["jump_lengths"], omitted=sysd["omitted"],
+                                   omitted_rhs=sysd["omitted_rhs"], omitted_volumes=sysd["omitted_volumes"])
+    x, st = gsys.solve(rhs, dt=dt, density=rho, tolerance=TOL)
+    solver.close()
+    assert st.converged and st.jump_unknowns > 1000
+    r = rhs - sysd["matrix"] @ x
+    npp = len(sysd["pressure_indices"])
+    pd = dt * dt * np.max(np.abs(r[:npp]) / sysd["pressure_volumes"])
+    jd = dt * dt * np.max(np.abs(r[npp:]) / sysd["jump_lengths"])
+    assert pd <= TOL and jd <= TOL
+    assert abs(pd - st.pressure_defect) <= 1e-3 * TOL and abs(jd - st.jump_defect) <= 1e-3 * TOL
+    ra = sysd["omitted_rhs"] - sysd["omitted"] @ x
+    assert dt * dt * np.max(np.abs(ra) / sysd["omitted_volumes"]) <= TOL

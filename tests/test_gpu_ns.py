@@ -302,3 +302,27 @@ def test_implicit_midpoint_amplification_on_one_root():
     assert np.max(np.abs(out - g * old)) <= 2e-13 + 2e-13 * np.max(np.abs(g * old))
     assert sx.converged and sy.converged and sysg.viscous_levels == [256]
     sysg.close()
+
+
+def test_taylor_green_at_1024_properties():
+    """At 1024^2 cells (beyond the oracle's reach per step) two steps keep the
+    size-independent properties: the projected velocity's divergence within
+    the solve's tolerance, the fields within the second-order truncation of
+    the exact decaying Taylor-Green solution, and the kinetic energy decaying
+    at the viscous rate exp(-4 nu t) (its transport error is of higher order)."""
+    kw = dict(base_mesh=(1024, 1024), block_size=(16, 16), extent=(2 * np.pi, 2 * np.pi), periodic=(True, True))
+    om, pm = OMesh(**kw), tpamr.BlockTreeMesh(**kw)
+    nu, dt = 0.02, 1e-3
+    sysg = tpamr.NavierStokesSystem(pm, dynamic_viscosity=nu)
+    u, p = tg(om), np.zeros(pm.ncells)
+    e0 = np.sum(u ** 2)
+    for _ in range(2):
+        st = tpamr.step(u, p, sysg, dt, tolerance=1e-12)
+        assert st.converged
+    vx, vy = sysg.viscous_stats()
+    assert vx.converged and vy.converged
+    sysg.close()
+    h = 2 * np.pi / 1024
+    assert np.max(np.abs(om.divergence(u))) <= 1e-8
+    assert rel(u, tg(om, 2 * dt, nu)) <= 4 * h * h
+    assert abs(np.sum(u ** 2) / e0 - np.exp(-4 * nu * 2 * dt)) <= 1e-5
