@@ -612,34 +612,51 @@ def cmdgfm_run(n=2048, steps=5, warmup=1, tol=1e-8, precision="fp64", sample=256
     import tpamr
     from tpamr.workloads import cmdgfm_block_workload
 
-    def build(k):
-        t = time.perf_counter()
-        mesh, kappa, sysd, rhs, dt, rho = cmdgfm_block_workload(n=k)
-        solver = tpamr.EllipticSolver(tpamr.PoissonOperator(mesh, kappa), precision=precision)
+    def system(mesh, kappa, sysd, prec):
+        solver = tpamr.EllipticSolver(tpamr.PoissonOperator(mesh, kappa), precision=prec)
         gsys = tpamr.CMDGFMBlockSystem(solver, sysd["matrix"], sysd["pressure_indices"], sysd["pressure_anchors"],
                                        sysd["pressure_volumes"], sysd["jump_lengths"], omitted=sysd["omitted"],
                                        omitted_rhs=sysd["omitted_rhs"], omitted_volumes=sysd["omitted_volumes"])
+        return solver, gsys
+
+    def build(k):
+        t = time.perf_counter()
+        mesh, kappa, sysd, rhs, dt, rho = cmdgfm_block_workload(n=k)
+        solver, gsys = system(mesh, kappa, sysd, precision)
         log(f"cmdgfm {k}^2: {mesh.ncells} cells, {len(sysd['jump_lengths'])} jumps, setup "
             f"{time.perf_counter() - t:.1f} s")
         return mesh, kappa, sysd, rhs, dt, rho, solver, gsys
 
+    def timed(gsys, d_rhs, d_x):
+        st = None
+        for _ in range(warmup):
+            st = gsys.solve_device(d_rhs, d_x, dt=dt, density=rho, tolerance=tol)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            st = gsys.solve_device(d_rhs, d_x, dt=dt, density=rho, tolerance=tol)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps, st
+
     mesh, kappa, sysd, rhs, dt, rho, solver, gsys = build(n)
     d_rhs = torch.tensor(rhs, device="cuda:0")
     d_x = torch.zeros(len(rhs), dtype=torch.float64, device="cuda:0")
-    st = None
-    for _ in range(warmup):
-        st = gsys.solve_device(d_rhs, d_x, dt=dt, density=rho, tolerance=tol)
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    for _ in range(steps):
-        st = gsys.solve_device(d_rhs, d_x, dt=dt, density=rho, tolerance=tol)
-    torch.cuda.synchronize()
-    wall = (time.perf_counter() - t) / steps
+    wall, st = timed(gsys, d_rhs, d_x)
     out = {"cells": mesh.ncells, "unknowns": len(rhs), "jumps": st.jump_unknowns, "ms_per_solve": wall * 1e3,
            "mcells": mesh.ncells / wall / 1e6, "iterations": st.iterations, "krylov_runs": st.krylov_runs,
            "converged": st.converged, "pressure_defect": st.pressure_defect, "jump_defect": st.jump_defect,
            "anchor_defect": st.anchor_defect, "precision": precision}
     solver.close()
+    if precision == "fp64":
+        # the pressure block's V-cycle in FP32 (BiCGSTAB, products and defects stay FP64)
+        x64 = d_x.clone()
+        solver, gsys = system(mesh, kappa, sysd, "fp32_vcycle")
+        wall, st = timed(gsys, d_rhs, d_x)
+        out["fp32_vcycle"] = {"ms_per_solve": wall * 1e3, "mcells": mesh.ncells / wall / 1e6,
+                              "iterations": st.iterations, "converged": st.converged,
+                              "pressure_defect": st.pressure_defect,
+                              "solution_vs_fp64": float((d_x - x64).abs().max() / x64.abs().max())}
+        solver.close()
     if sample:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_cmdgfm as OC

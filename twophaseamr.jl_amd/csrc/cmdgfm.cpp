@@ -249,6 +249,37 @@ void cmdgfm_crout_ilu(double tau, JumpFactor& f) {
   for (int i = n - 1; i >= 0; --i)
     for (int p = f.uptr[i]; p < f.uptr[i + 1]; ++p) lev[i] = std::max(lev[i], lev[f.ucol[p]] + 1);
   level_schedule(lev, f.ulev, f.uorder);
+  // rows packed in level order at the width of the longest row
+  int w = 0;
+  for (int i = 0; i < n; ++i) w = std::max({w, f.lptr[i + 1] - f.lptr[i], f.uptr[i + 1] - f.uptr[i]});
+  f.pw = w <= CM_ILU_PF ? std::max(w, 1) : 0;
+  f.flen.clear();
+  f.fcol.clear();
+  f.fval.clear();
+  f.blen.clear();
+  f.bcol.clear();
+  f.bval.clear();
+  f.bdiag.clear();
+  if (f.pw == 0) return;
+  f.fcol.assign(size_t(n) * f.pw, 0);
+  f.fval.assign(size_t(n) * f.pw, 0.0);
+  f.bcol.assign(size_t(n) * f.pw, 0);
+  f.bval.assign(size_t(n) * f.pw, 0.0);
+  for (int t = 0; t < n; ++t) {
+    const int i = f.lorder[t];
+    f.flen.push_back(f.lptr[i + 1] - f.lptr[i]);
+    for (int p = f.lptr[i]; p < f.lptr[i + 1]; ++p) {
+      f.fcol[size_t(t) * f.pw + (p - f.lptr[i])] = f.lcol[p];
+      f.fval[size_t(t) * f.pw + (p - f.lptr[i])] = f.lval[p];
+    }
+    const int j = f.uorder[t];
+    f.blen.push_back(f.uptr[j + 1] - f.uptr[j]);
+    for (int p = f.uptr[j]; p < f.uptr[j + 1]; ++p) {
+      f.bcol[size_t(t) * f.pw + (p - f.uptr[j])] = f.ucol[p];
+      f.bval[size_t(t) * f.pw + (p - f.uptr[j])] = f.uval[p];
+    }
+    f.bdiag.push_back(f.udiag[j]);
+  }
 }
 
 struct CmdgfmSystem::Impl {
@@ -261,8 +292,8 @@ struct CmdgfmSystem::Impl {
   // device copies: the system as sliced ELL, its C block as CSR
   Buf<int> sptr, slen, scol, cptr, ccol, role, idx, anc, list, chunk_ptr, anchor_cell, anchor_chunk_ptr, optr, ocol;
   Buf<double> sval, cval, vol, len, oval, orhs, ovol;
-  Buf<int> lptr, lcol, llev, lorder, uptr, ucol, ulev, uorder;
-  Buf<double> lval, uval, udiag;
+  Buf<int> lptr, lcol, llev, lorder, uptr, ucol, ulev, uorder, flen, fcol, blen, bcol;
+  Buf<double> lval, uval, udiag, fval, bval, bdiag, jbt;
   // work vectors
   Buf<double> corr, srhs, inc, kr, kp, kv, ks, kt, kyz, cand, res, full, jr, jy;
   Buf<double> part, apart, scal, out;
@@ -271,6 +302,8 @@ struct CmdgfmSystem::Impl {
   double host_scal[CM_SCALARS] = {};
   // Krylov scaling of the current solve: rows prow / jrow, columns pcol / jcol
   double prow = 1, pcol = 1, jrow = 1, jcol = 1;
+  // the V-cycle's own input buffer for the current solve (null: full)
+  double* vin_buf = nullptr;
 
   void read_out(int k) {
     CCHECK(hipMemcpyAsync(host_out, out.p, sizeof(double) * k, hipMemcpyDeviceToHost, s));
@@ -301,10 +334,11 @@ struct CmdgfmSystem::Impl {
   void precondition(const double* src, double* dst, const Vcycle& vc) {
     const double* z = nullptr;
     if (np) {
-      launch_cm_scatter(s, ncells, role.p, src, prow, full.p);
+      double* fb = vin_buf ? vin_buf : full.p;
+      launch_cm_scatter(s, ncells, role.p, src, prow, fb);
       launch_cm_anchor_chunks(s, nchunks, chunk_ptr.p, list.p, src, prow, apart.p);
-      launch_cm_anchor_totals(s, nanchors, anchor_cell.p, anchor_chunk_ptr.p, apart.p, full.p);
-      z = vc(full.p);
+      launch_cm_anchor_totals(s, nanchors, anchor_cell.p, anchor_chunk_ptr.p, apart.p, fb);
+      z = vc(fb);
       launch_cm_gather(s, np, idx.p, anc.p, z, pcol, dst);
     }
     if (nx) {
@@ -313,10 +347,15 @@ struct CmdgfmSystem::Impl {
       ilu(dst + np, jcol);
     }
   }
-  // x = ILU^-1 jr, then x /= d
+  // x = ILU^-1 jr, then x /= d (the iterate in LDS while it fits)
   void ilu(double* x, double d) {
-    launch_cm_ilu(s, nx, lptr.p, lcol.p, lval.p, int(jf.llev.size()) - 1, llev.p, lorder.p, uptr.p, ucol.p, uval.p,
-                  udiag.p, int(jf.ulev.size()) - 1, ulev.p, uorder.p, jr.p, jy.p, x, d);
+    const int nl = int(jf.llev.size()) - 1, nu = int(jf.ulev.size()) - 1;
+    if (jf.pw > 0 && size_t(nx) * sizeof(double) + size_t(nl + nu + 2) * sizeof(int) <= size_t(CM_ILU_LDS))
+      launch_cm_ilu_lds(s, nx, jf.pw, nl, llev.p, lorder.p, flen.p, fcol.p, fval.p, nu, ulev.p, uorder.p, blen.p,
+                        bcol.p, bval.p, bdiag.p, jr.p, jbt.p, x, d);
+    else
+      launch_cm_ilu(s, nx, lptr.p, lcol.p, lval.p, nl, llev.p, lorder.p, uptr.p, ucol.p, uval.p, udiag.p, nu, ulev.p,
+                    uorder.p, jr.p, jy.p, x, d);
   }
 
   // Krylov.jl bicgstab! (M = I, right preconditioner N when use_pc, zero start,
@@ -582,12 +621,20 @@ CmdgfmSystem::CmdgfmSystem(int ncells, int np, int nx, const int* ptr, const int
     D.udiag.upload(f.udiag, stream);
     D.ulev.upload(f.ulev, stream);
     D.uorder.upload(f.uorder, stream);
+    D.flen.upload(f.flen, stream);
+    D.fcol.upload(f.fcol, stream);
+    D.fval.upload(f.fval, stream);
+    D.blen.upload(f.blen, stream);
+    D.bcol.upload(f.bcol, stream);
+    D.bval.upload(f.bval, stream);
+    D.bdiag.upload(f.bdiag, stream);
   }
   for (Buf<double>* b : {&D.corr, &D.srhs, &D.inc, &D.kr, &D.kp, &D.kv, &D.ks, &D.kt, &D.kyz, &D.cand, &D.res})
     b->alloc(std::max(n, 1));
   D.full.alloc(std::max(ncells, 1));
   D.jr.alloc(std::max(nx, 1));
   D.jy.alloc(std::max(nx, 1));
+  D.jbt.alloc(std::max(nx, 1));
   D.part.alloc(size_t(CM_MAX_WG) * 6);
   D.apart.alloc(std::max(D.nchunks, 1));
   D.scal.alloc(CM_SCALARS);
@@ -605,7 +652,7 @@ const JumpFactor& CmdgfmSystem::factor() const { return d_->jf; }
 
 void CmdgfmSystem::solve(const double* rhs, double* x, double dt, double density, double h, double tol,
                          int max_iterations, bool warm_start, const double* previous, bool use_pc, const Vcycle& vc,
-                         CmdgfmStats& st) {
+                         const VcycleInput& vin, CmdgfmStats& st) {
   Impl& D = *d_;
   if (!D.device) throw std::invalid_argument("host-only block system");
   if (!(dt > 0)) throw std::invalid_argument("dt must be strictly positive");
@@ -629,6 +676,7 @@ void CmdgfmSystem::solve(const double* rhs, double* x, double dt, double density
   D.pcol = density * ((h / dt) * (h / dt));
   D.jrow = (dt * dt) / h;
   D.jcol = h / (dt * dt);
+  D.vin_buf = vin ? vin() : nullptr;
   CCHECK(hipMemsetAsync(x, 0, sizeof(double) * n, s));
   CCHECK(hipMemcpyAsync(D.corr.p, rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   D.check(rhs, nullptr);

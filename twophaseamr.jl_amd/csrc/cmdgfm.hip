@@ -316,6 +316,102 @@ __global__ void __launch_bounds__(CM_ILU_WG) k_cm_ilu(int nx, const int* __restr
     for (int i = threadIdx.x; i < nx; i += CM_ILU_WG) x[i] = x[i] / d;
 }
 
+// The same solves with the iterate and the level schedules in LDS and the
+// rows packed in level order at a fixed width W (entry k of packed row t at
+// t W + k, len[t] of them, in the global kernel's order): every load of a
+// row is independent of the others, and each thread fetches its row of the
+// next level into registers before the barrier that ends the current one.
+struct IluRow {
+  int row, m;
+  double rhs, diag;
+  int col[CM_ILU_PF];
+  double val[CM_ILU_PF];
+};
+
+__device__ __forceinline__ void ilu_fetch(int t, int w, const int* __restrict__ rows, const int* __restrict__ len,
+                                          const int* __restrict__ col, const double* __restrict__ val,
+                                          const double* __restrict__ rhs, const double* __restrict__ diag,
+                                          IluRow& f) {
+  f.row = rows[t];
+  f.m = len[t];
+  f.rhs = rhs ? rhs[t] : 0.0;
+  f.diag = diag ? diag[t] : 1.0;
+#pragma unroll
+  for (int k = 0; k < CM_ILU_PF; ++k)
+    if (k < w) {
+      f.col[k] = col[size_t(t) * w + k];
+      f.val[k] = val[size_t(t) * w + k];
+    }
+}
+
+// one triangular solve over the level schedule lev (nlev levels, in LDS) of
+// the packed rows; rhs_t: rhs by packed position (null: the LDS value of the
+// row itself); diag: divisor by packed position (null: unit)
+__device__ __forceinline__ void ilu_sweep(double* ys, int nlev, const int* lev, int w, const int* __restrict__ rows,
+                                          const int* __restrict__ len, const int* __restrict__ col,
+                                          const double* __restrict__ val, const double* __restrict__ rhs_t,
+                                          const double* __restrict__ diag) {
+#pragma clang fp contract(off)
+  const int tid = threadIdx.x;
+  IluRow cur, nxt;
+  bool have = nlev > 0 && lev[0] + tid < lev[1];
+  if (have) ilu_fetch(lev[0] + tid, w, rows, len, col, val, rhs_t, diag, cur);
+  for (int l = 0; l < nlev; ++l) {
+    bool nhave = false;
+    if (l + 1 < nlev && lev[l + 1] + tid < lev[l + 2]) {
+      ilu_fetch(lev[l + 1] + tid, w, rows, len, col, val, rhs_t, diag, nxt);
+      nhave = true;
+    }
+    if (have) {
+      double acc = rhs_t ? cur.rhs : ys[cur.row];
+#pragma unroll
+      for (int k = 0; k < CM_ILU_PF; ++k)
+        if (k < cur.m) acc = acc - cur.val[k] * ys[cur.col[k]];
+      ys[cur.row] = diag ? acc / cur.diag : acc;
+    }
+    // rows beyond one per thread in a wide level, straight from memory
+    for (int t = lev[l] + tid + CM_ILU_WG; t < lev[l + 1]; t += CM_ILU_WG) {
+      const int i = rows[t];
+      double acc = rhs_t ? rhs_t[t] : ys[i];
+      for (int k = 0; k < len[t]; ++k) acc = acc - val[size_t(t) * w + k] * ys[col[size_t(t) * w + k]];
+      ys[i] = diag ? acc / diag[t] : acc;
+    }
+    __syncthreads();
+    if (nhave) cur = nxt;
+    have = nhave;
+  }
+}
+
+__global__ void __launch_bounds__(CM_ILU_WG) k_cm_ilu_lds(int nx, int w, int nllev, const int* __restrict__ llev,
+                                                          const int* __restrict__ lrows, const int* __restrict__ llen,
+                                                          const int* __restrict__ lcol,
+                                                          const double* __restrict__ lval, int nulev,
+                                                          const int* __restrict__ ulev, const int* __restrict__ urows,
+                                                          const int* __restrict__ ulen, const int* __restrict__ ucol,
+                                                          const double* __restrict__ uval,
+                                                          const double* __restrict__ udiag,
+                                                          const double* __restrict__ b_t, double* __restrict__ x,
+                                                          double d) {
+#pragma clang fp contract(off)
+  // LDS: the iterate (nx doubles), then both level schedules
+  extern __shared__ double ys[];
+  int* ll = reinterpret_cast<int*>(ys + nx);
+  int* ul = ll + nllev + 1;
+  for (int k = threadIdx.x; k <= nllev; k += CM_ILU_WG) ll[k] = llev[k];
+  for (int k = threadIdx.x; k <= nulev; k += CM_ILU_WG) ul[k] = ulev[k];
+  __syncthreads();
+  ilu_sweep(ys, nllev, ll, w, lrows, llen, lcol, lval, b_t, nullptr);
+  ilu_sweep(ys, nulev, ul, w, urows, ulen, ucol, uval, nullptr, udiag);
+  for (int i = threadIdx.x; i < nx; i += CM_ILU_WG) x[i] = d != 1.0 ? ys[i] / d : ys[i];
+}
+
+// rhs in the forward solve's packed order: b_t[t] = b[lrows[t]]
+__global__ void __launch_bounds__(CM_WG) k_cm_ilu_rhs(int nx, const int* __restrict__ lrows,
+                                                      const double* __restrict__ b, double* __restrict__ b_t) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nx) b_t[t] = b[lrows[t]];
+}
+
 __global__ void __launch_bounds__(CM_WG) k_cm_check(int n, int np, const double* __restrict__ r,
                                                     const double* __restrict__ vol, const double* __restrict__ len,
                                                     double pr, double jr, const double* __restrict__ x,
@@ -444,6 +540,16 @@ void launch_cm_ilu(hipStream_t s, int nx, const int* lptr, const int* lcol, cons
   if (nx > 0)
     hipLaunchKernelGGL(k_cm_ilu, dim3(1), dim3(CM_ILU_WG), 0, s, nx, lptr, lcol, lval, nllev, llev, lorder, uptr, ucol,
                        uval, udiag, nulev, ulev, uorder, b, y, x, d);
+}
+void launch_cm_ilu_lds(hipStream_t s, int nx, int w, int nllev, const int* llev, const int* lrows, const int* llen,
+                       const int* lcol, const double* lval, int nulev, const int* ulev, const int* urows,
+                       const int* ulen, const int* ucol, const double* uval, const double* udiag, const double* b,
+                       double* b_t, double* x, double d) {
+  if (nx <= 0) return;
+  hipLaunchKernelGGL(k_cm_ilu_rhs, dim3(cdiv(nx, CM_WG)), dim3(CM_WG), 0, s, nx, lrows, b, b_t);
+  const size_t lds = size_t(nx) * sizeof(double) + size_t(nllev + nulev + 2) * sizeof(int);
+  hipLaunchKernelGGL(k_cm_ilu_lds, dim3(1), dim3(CM_ILU_WG), lds, s, nx, w, nllev, llev, lrows, llen, lcol, lval,
+                     nulev, ulev, urows, ulen, ucol, uval, udiag, b_t, x, d);
 }
 void launch_cm_check(hipStream_t s, int n, int np, const double* r, const double* vol, const double* len, double pr,
                      double jr, const double* x, double* part) {

@@ -26,6 +26,11 @@ struct CmdgfmDeviceError : std::runtime_error {
 // workgroup sums per anchor chunk, and the rows of one sliced-ELL slice.
 constexpr int CM_WG = 256;
 constexpr int CM_SLICE = 64;
+// The jump factor's solves keep the iterate and the level schedules in LDS
+// while they fit in CM_ILU_LDS bytes, and prefetch CM_ILU_PF entries of each
+// row into registers.
+constexpr int CM_ILU_LDS = 65536;
+constexpr int CM_ILU_PF = 8;
 constexpr int CM_MAX_WG = 1024;
 constexpr int CM_ILU_WG = 1024;
 constexpr int CM_ANCHOR_CHUNK = 4096;
@@ -92,6 +97,15 @@ void launch_cm_ilu(hipStream_t s, int nx, const int* lptr, const int* lcol, cons
                    const int* llev, const int* lorder, const int* uptr, const int* ucol, const double* uval,
                    const double* udiag, int nulev, const int* ulev, const int* uorder, const double* b, double* y,
                    double* x, double d);
+// the same with the iterate and level schedules in LDS (8 nx + 4 (nllev +
+// nulev + 2) <= CM_ILU_LDS bytes) and the rows packed in level order at width
+// w <= CM_ILU_PF (lrows / urows: row of each packed position, llen / ulen:
+// its entries, entry k of position t at t w + k; udiag by packed position);
+// b_t: nx doubles of scratch for the rhs in packed order
+void launch_cm_ilu_lds(hipStream_t s, int nx, int w, int nllev, const int* llev, const int* lrows, const int* llen,
+                       const int* lcol, const double* lval, int nulev, const int* ulev, const int* urows,
+                       const int* ulen, const int* ucol, const double* uval, const double* udiag, const double* b,
+                       double* b_t, double* x, double d);
 // per-workgroup partials of the defects of a residual: [0] max |r_i| / vol_i
 // over the pressure rows with vol > 0, [1] max |r_k| / len_k over the jump
 // rows, [2] sum r^2, [3] sum (rows r)^2 (rows pr / jr), [4] 1 where x is not
@@ -128,6 +142,12 @@ struct JumpFactor {
   std::vector<int> lptr, lcol, uptr, ucol;
   std::vector<double> lval, uval, udiag;
   std::vector<int> llev, lorder, ulev, uorder;
+  // rows packed in level order at width pw (the longest row; 0 when it
+  // exceeds CM_ILU_PF) for the LDS solves: lengths, entries of the forward
+  // (L) and backward (U off the diagonal) rows, U's diagonal by position
+  int pw = 0;
+  std::vector<int> flen, fcol, blen, bcol;
+  std::vector<double> fval, bval, bdiag;
 };
 
 // J - C D^-1 B of an assembled CSR system (rows sorted by column), D the
@@ -144,6 +164,9 @@ class CmdgfmSystem {
   // values) and returns z's device address (full-cell FP64, valid until the
   // next call; defined up to a constant per pressure component).
   using Vcycle = std::function<const double*(double* r)>;
+  // The V-cycle's own full-cell input buffer, when the rhs may be written
+  // there directly (null: the system's own buffer, copied in by the V-cycle).
+  using VcycleInput = std::function<double*()>;
 
   // Checks and factors the assembled system on the host; with a stream,
   // uploads it. Index arrays are 0-based; CSR column indices sorted per row.
@@ -164,7 +187,7 @@ class CmdgfmSystem {
   // solution (n, overwritten), previous full-cell increment (ncells) or null.
   void solve(const double* rhs, double* solution, double dt, double density, double reference_length,
              double tolerance, int max_iterations, bool warm_start, const double* previous, bool preconditioned,
-             const Vcycle& vcycle, CmdgfmStats& stats);
+             const Vcycle& vcycle, const VcycleInput& input, CmdgfmStats& stats);
 
  private:
   struct Impl;

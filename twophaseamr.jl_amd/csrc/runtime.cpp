@@ -3221,7 +3221,7 @@ const double* cmdgfm_vcycle_t(tpamr_ctx* c, std::vector<DevLevel<T>>& lv, double
   const int N = c->topo[0].ncells;
   if constexpr (std::is_same<T, float>::value)
     launch_cast_d2f(s, N, r, lv[0].b.p);
-  else
+  else if (r != lv[0].b.p)
     launch_copy(s, N, r, lv[0].b.p);
   T* z = enqueue_vcycle<T>(c, lv, &c->rs.st->done, ZStats{}, true);
   if constexpr (std::is_same<T, float>::value) {
@@ -3257,8 +3257,12 @@ void cmdgfm_run(tpamr_ctx* c, CmdgfmSystem& sys, const double* d_rhs, double* d_
   const TreeMesh& m = c->mesh();
   const double h = std::max(m.extent[0], m.extent[1]);
   CmdgfmStats st;
+  // in FP64 with the fused zero-start sweep the block writes its rhs
+  // straight into the finest level
+  CmdgfmSystem::VcycleInput input;
+  if (!c->fp32() && x0_fused(c)) input = [c]() { return c->ld[0].b.p; };
   sys.solve(d_rhs, d_x, dt, density, h, tolerance, max_iterations, warm_start != 0, d_prev, preconditioned != 0,
-            cmdgfm_vcycle(c), st);
+            cmdgfm_vcycle(c), input, st);
   if (stats) {
     stats->converged = st.converged;
     stats->iterations = st.iterations;
