@@ -524,10 +524,10 @@ def wave_main(args):
 
 def taylor_green_run(n=128, block=16, steps=2, warmup=1, nu=0.02, dt=1e-3, tol=1e-12, oracle_check=True):
     """The plain-method Taylor-Green configuration: periodic [0, 2 pi]^2, n^2
-    cells in block^2 blocks, density 1, viscosity nu, dt, `steps` timed
-    one-velocity steps (step! with velocity and pressure in HBM) after
-    `warmup` steps from the same initial field. With oracle_check, the
-    oracle runs the same steps on the host (its FP64 restatement: numpy
+    cells in block^2 blocks, density 1, viscosity nu, dt: a fresh system
+    runs `warmup` untimed one-velocity steps (step! with velocity and
+    pressure in HBM) and then `steps` timed ones. With oracle_check, the
+    oracle runs all of them on the host (its FP64 restatement: numpy
     transport and viscous operator, C++ pressure solver) and the velocity
     and pressure are compared field for field; its wall time per step is the
     CPU figure."""
@@ -542,9 +542,11 @@ def taylor_green_run(n=128, block=16, steps=2, warmup=1, nu=0.02, dt=1e-3, tol=1
     u0 = np.where(ax == 0, np.sin(c[:, 0]) * np.cos(c[:, 1]), -np.cos(c[:, 0]) * np.sin(c[:, 1]))
     system = tpamr.NavierStokesSystem(mesh, density=1.0, dynamic_viscosity=nu)
 
-    def run(k):
+    def run(k, untimed=0):
         du = torch.tensor(u0, device="cuda:0")
         dp = torch.zeros(mesh.ncells, dtype=torch.float64, device="cuda:0")
+        for _ in range(untimed):
+            tpamr.step_device(du, dp, system, dt, tolerance=tol)
         torch.cuda.synchronize()
         t = time.perf_counter()
         st = None
@@ -553,15 +555,16 @@ def taylor_green_run(n=128, block=16, steps=2, warmup=1, nu=0.02, dt=1e-3, tol=1
         torch.cuda.synchronize()
         return time.perf_counter() - t, du, dp, st
 
-    for _ in range(warmup):
-        run(steps)
+    run(steps)
     # a new system starts the transport's sweep order forward, as the
-    # oracle's does
+    # oracle's does; its first `warmup` steps (context set-up, first graph
+    # captures) are not timed, and the oracle runs them too
     system.close()
     system = tpamr.NavierStokesSystem(mesh, density=1.0, dynamic_viscosity=nu)
-    wall, du, dp, st = run(steps)
+    wall, du, dp, st = run(steps, untimed=warmup)
     vx, vy = system.viscous_stats()
-    out = {"cells": mesh.ncells, "block_size": [block, block], "steps": steps, "dt": dt, "viscosity": nu,
+    out = {"cells": mesh.ncells, "block_size": [block, block], "steps": steps, "untimed_first_steps": warmup,
+           "dt": dt, "viscosity": nu,
            "ms_per_step": wall / steps * 1e3, "mcells": mesh.ncells * steps / wall / 1e6,
            "pressure_cycles": st.cycles, "viscous_cycles": [vx.cycles, vy.cycles], "converged": bool(st.converged)}
     system.close()
@@ -574,9 +577,9 @@ def taylor_green_run(n=128, block=16, steps=2, warmup=1, nu=0.02, dt=1e-3, tol=1
         adv, D = N.StaggeredAdvection(om), N.StaggeredDiffusion(om, nu)
         uo, po = u0.copy(), np.zeros(om.ncells)
         t = time.perf_counter()
-        for _ in range(steps):
+        for _ in range(warmup + steps):
             uo, po, _ = N.step(uo, po, S, adv, D, dt, tolerance=tol)
-        cpu = (time.perf_counter() - t) / steps
+        cpu = (time.perf_counter() - t) / (warmup + steps)
         ug, pg = du.cpu().numpy(), dp.cpu().numpy()
         out["velocity_error_vs_oracle"] = float(np.max(np.abs(ug - uo)) / np.max(np.abs(uo)))
         out["pressure_error_vs_oracle"] = float(np.max(np.abs(pg - po)) / np.max(np.abs(po)))
@@ -850,7 +853,7 @@ def main():
         out["cpu_full"] = cpu_full(case, x_sol, args.rtol)
     # the plain-method configuration's one-velocity step (128^2 cells, two
     # steps), field for field against the oracle
-    out["taylor_green_step"] = taylor_green_run()
+    out["taylor_green_step"] = taylor_green_run(steps=4, warmup=1)
     # the CMDGFM reduced-pressure/jump block solve (2048^2 cells), with the
     # oracle's answer and rate at 256^2
     out["cmdgfm_block_solve"] = cmdgfm_run(n=2048, steps=3, warmup=1)

@@ -193,3 +193,33 @@ def test_block_solve_at_size():
     assert abs(pd - st.pressure_defect) <= 1e-3 * TOL and abs(jd - st.jump_defect) <= 1e-3 * TOL
     ra = sysd["omitted_rhs"] - sysd["omitted"] @ x
     assert dt * dt * np.max(np.abs(ra) / sysd["omitted_volumes"]) <= TOL
+
+
+def test_iteration_budget_and_pure_jump_system():
+    """An exhausted budget stops where the oracle stops (two BiCGSTAB
+    iterations, not converged, the same iterate), and a system of jumps only
+    (no reduced pressures) runs the jump factor alone."""
+    import scipy.sparse as sp
+
+    _print("case budget uniform32")
+    d, solver, gsys, osys = _setup("uniform32")
+    x, st = gsys.solve(d["rhs"], dt=d["dt"], density=d["density"], tolerance=TOL, max_iterations=2,
+                       warm_start=False)
+    ox, ost = _oracle(osys, d, max_iterations=2, warm_start=False)
+    assert st.iterations == ost["iterations"] == 2 and st.krylov_runs == ost["krylov_runs"] == 1
+    assert not st.converged and not ost["converged"]
+    assert rel(x, ox) <= 1e-10
+    _print("case pure jump system")
+    npp = d["np"]
+    j = sp.csr_matrix(d["matrix"])[npp:, npp:]
+    rj = d["rhs"][npp:]
+    empty = np.zeros(0, np.int64)
+    gj = tpamr.CMDGFMBlockSystem(solver, j, empty, empty, np.zeros(0), d["jump_lengths"])
+    xj, sj = gj.solve(rj, dt=d["dt"], density=d["density"], tolerance=TOL)
+    oj = OC.BlockSystem(j, 0, empty, empty, np.zeros(0), d["jump_lengths"], sp.csr_matrix((0, j.shape[0])),
+                        np.zeros(0), np.zeros(0), d["ncells"], None)
+    oxj, osj = OC.solve(oj, rj, dt=d["dt"], density=d["density"], reference_length=1.0, tolerance=TOL)
+    assert sj.converged and osj["converged"] and sj.pressure_unknowns == 0
+    assert sj.iterations == osj["iterations"]
+    assert rel(xj, oxj) <= 1e-10
+    solver.close()
