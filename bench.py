@@ -339,6 +339,15 @@ def trace_summary(db, title, command, top=30):
         lines += ["", f"`{SWEEP_KERNEL}` (finest level): mean {statistics.mean(sweep):.2f} us over all "
                       f"{len(sweep)} launches, {statistics.mean(full):.2f} us over the {len(full)} full launches "
                       f"(at least half the median), median {med:.2f} us."]
+    # the face and cell passes of a projection step around its solve
+    passes = ("k_mixture", "k_gravity_potential", "k_gravity_jump", "k_divergence", "k_projection_rhs",
+              "k_face_jump", "k_metric_apply", "k_face_scale", "k_axpby")
+    found = [(p, agg[n]) for p in passes for n in agg
+             if n.replace("tpamr::", "").split("(")[0].strip() == p]
+    if found:
+        lines += ["", "Face and cell passes around the solve (coefficients, jumps, force, divergence, correction):",
+                  "", "| kernel | calls | avg us |", "|---|---:|---:|"]
+        lines += [f"| `{p}` | {a[0]} | {a[1] / a[0]:.2f} |" for p, a in found]
     ends = [i for i, r in enumerate(rows) if "k_ir_update_p" in r[0]]
     its = []
     for i0, i1 in zip(ends, ends[1:]):
