@@ -283,6 +283,32 @@ def test_multigrid_viscous_solve_matches_oracle(name):
     sysg.close()
 
 
+@pytest.mark.parametrize("base, want_levels", [((96, 96), [9216, 2304]), ((48, 80), [3840])])
+def test_viscous_solve_with_large_coarsest_level_matches_oracle(base, want_levels):
+    """Root layouts whose staggered multigrid stops above 1024 unknowns per
+    component: 6x6 roots of 16^2 cells halve once to 3x3 (coarsest 2304),
+    3x5 roots do not halve (one level of 3840). The reference solves the
+    coarsest level exactly at any size (dense or sparse LU,
+    implicit_diffusion.jl:343-347); the library by the periodic convolution
+    with the first column of the circulant inverse. The oracle's exact coarse
+    LU gives the same iteration counts per component and the same solution
+    (1e-12)."""
+    kw = dict(base_mesh=base, block_size=(16, 16), extent=(1.0, 1.0), periodic=(True, True))
+    om, pm = OMesh(**kw), tpamr.BlockTreeMesh(**kw)
+    nu, dt = 0.5, 0.02
+    old = sample(om, lambda x, y: np.sin(2 * np.pi * x) * np.cos(4 * np.pi * y), lambda x, y: np.cos(2 * np.pi * x))
+    conv = old + 0.05 * np.sin(np.arange(om.nfaces) * 0.37)
+    sysg = tpamr.NavierStokesSystem(pm, dynamic_viscosity=nu)
+    assert sysg.viscous_levels == want_levels
+    dst = old.copy()
+    sx, sy = sysg.implicit_midpoint_diffusion(dst, conv, old, dt, tolerance=1e-11)
+    want, st = implicit_midpoint_mg(StaggeredDiffusion(om, nu), (base, (16, 16), 0), conv, old, old, dt, tol=1e-11)
+    assert (sx.cycles, sy.cycles) == (st[0]["cycles"], st[1]["cycles"])
+    assert sx.converged and sy.converged
+    assert rel(dst, want) < 1e-12
+    sysg.close()
+
+
 def test_implicit_midpoint_amplification_on_one_root():
     """'implicit midpoint staggered elliptic solve'
     (test/operators/staggered_diffusion.jl:1694-1728): a 16^2 root block

@@ -55,6 +55,59 @@ struct StagLevel {
   Buf<double> x, xs, b, r;
 };
 
+namespace {
+// The first column g = A^-1 e_0 of the inverse of a level's operator A =
+// md I + mx (S_x + S_x^T) + my (S_y + S_y^T) on its periodic grid (S the
+// periodic shifts; the coarse LU of the reference,
+// implicit_diffusion.jl:343-347, solves the same system exactly). A is
+// circulant with eigenvalues lambda = md + 2 mx cos(2 pi kx / nx) +
+// 2 my cos(2 pi ky / ny), even in kx and ky, so
+//   g[n, m] = 1 / (nx ny) sum_kx sum_ky cos(2 pi kx m / nx) cos(2 pi ky n / ny) / lambda
+// (row-major (n, m) = (j, i)), summed as two separable products. Checked on
+// the host: max |A g - e_0| must be round-off.
+std::vector<double> circulant_inverse_column(const StagLevel& C) {
+  const int nx = C.nx, ny = C.ny;
+  const double two_pi = 2.0 * std::acos(-1.0);
+  auto cos_table = [&](int n) {
+    std::vector<double> t(size_t(n) * n);
+    for (int k = 0; k < n; ++k)
+      for (int m = 0; m < n; ++m) t[size_t(k) * n + m] = std::cos(two_pi * double((int64_t(k) * m) % n) / n);
+    return t;
+  };
+  const std::vector<double> cx = cos_table(nx), cy = cos_table(ny);
+  std::vector<double> lam(size_t(nx) * ny);
+  for (int ky = 0; ky < ny; ++ky)
+    for (int kx = 0; kx < nx; ++kx) {
+      const double l = C.k.md + 2.0 * C.k.mx * cx[size_t(kx) * nx + 1 % nx] + 2.0 * C.k.my * cy[size_t(ky) * ny + 1 % ny];
+      if (!(l > 0)) throw std::invalid_argument("coarsest viscous operator is not positive definite");
+      lam[size_t(ky) * nx + kx] = 1.0 / l;
+    }
+  // t[kx, n] = sum_ky cy[ky, n] / lambda(kx, ky)
+  std::vector<double> t(size_t(nx) * ny, 0.0), g(size_t(nx) * ny, 0.0);
+  for (int ky = 0; ky < ny; ++ky)
+    for (int n = 0; n < ny; ++n) {
+      const double c = cy[size_t(ky) * ny + n];
+      for (int kx = 0; kx < nx; ++kx) t[size_t(kx) * ny + n] += c * lam[size_t(ky) * nx + kx];
+    }
+  const double scale = 1.0 / (double(nx) * ny);
+  for (int kx = 0; kx < nx; ++kx)
+    for (int n = 0; n < ny; ++n) {
+      const double tv = t[size_t(kx) * ny + n] * scale;
+      for (int m = 0; m < nx; ++m) g[size_t(n) * nx + m] += cx[size_t(kx) * nx + m] * tv;
+    }
+  double err = 0.0;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      auto at = [&](int ii, int jj) { return g[size_t((jj + ny) % ny) * nx + (ii + nx) % nx]; };
+      const double ag = C.k.md * at(i, j) + C.k.mx * (at(i - 1, j) + at(i + 1, j)) +
+                        C.k.my * (at(i, j - 1) + at(i, j + 1));
+      err = std::max(err, std::fabs(ag - (i == 0 && j == 0 ? 1.0 : 0.0)));
+    }
+  if (!(err <= 1e-10)) throw std::runtime_error("circulant coarse inverse: residual " + std::to_string(err));
+  return g;
+}
+}  // namespace
+
 struct Momentum::Impl {
   hipStream_t s = nullptr;
   StagGrid g{};
@@ -71,7 +124,8 @@ struct Momentum::Impl {
   Buf<double> part, scal;
   double host_scal[12] = {};
   // staggered multigrid: levels (level 0 shares the Krylov grid size) and
-  // the dense inverse of the coarsest operator for the current shift
+  // the coarsest operator's inverse for the current shift (dense, or the
+  // first column of the circulant inverse above STAG_DENSE_MAX)
   std::vector<std::unique_ptr<StagLevel>> lv;
   Buf<double> coarse_inv;
   double coarse_half = -1;
@@ -112,9 +166,18 @@ struct Momentum::Impl {
     k.mdinv = lv[0]->k.mdinv;
     coarse_half = half;
     if (!use_mg) return;
-    // dense inverse of the coarsest operator (Gauss-Jordan, partial pivoting)
     const StagLevel& C = *lv.back();
     const int nc = C.nx * C.ny;
+    if (nc > STAG_DENSE_MAX) {
+      // the first column of the inverse of the circulant coarsest operator;
+      // the coarse solve is the periodic convolution with it
+      const std::vector<double> col = circulant_inverse_column(C);
+      coarse_inv.alloc(col.size());
+      MCHECK(hipMemcpyAsync(coarse_inv.p, col.data(), col.size() * sizeof(double), hipMemcpyHostToDevice, s));
+      MCHECK(hipStreamSynchronize(s));
+      return;
+    }
+    // dense inverse of the coarsest operator (Gauss-Jordan, partial pivoting)
     std::vector<double> a(size_t(nc) * nc, 0.0), inv(size_t(nc) * nc, 0.0);
     for (int j = 0; j < C.ny; ++j)
       for (int i = 0; i < C.nx; ++i) {
@@ -166,7 +229,7 @@ struct Momentum::Impl {
   void vcycle(const double* rhs, double* z) {
     const int nl = int(lv.size());
     if (nl == 1) {
-      launch_st_dense_solve(s, n, coarse_inv.p, rhs, z);
+      coarse_solve(rhs, z);
       return;
     }
     std::vector<double*> cur(nl), oth(nl);
@@ -193,8 +256,7 @@ struct Momentum::Impl {
       if (mg.pre == 0) launch_st_mg_smooth(s, L.nx, L.ny, L.k, 0.0, nullptr, bl[l], cur[l]);
       launch_st_mg_restrict_res(s, lv[l + 1]->nx, lv[l + 1]->ny, comp, L.k, bl[l], cur[l], lv[l + 1]->b.p);
     }
-    const StagLevel& C = *lv.back();
-    launch_st_dense_solve(s, C.nx * C.ny, coarse_inv.p, bl[nl - 1], cur[nl - 1]);
+    coarse_solve(bl[nl - 1], cur[nl - 1]);
     bool in_z = false;
     for (int l = nl - 2; l >= 0; --l) {
       StagLevel& L = *lv[l];
@@ -218,6 +280,15 @@ struct Momentum::Impl {
       }
     }
     if (!in_z) launch_copy_grid(cur[0], z);
+  }
+  // x = A^-1 b on the coarsest level: its dense inverse, or the periodic
+  // convolution with the first column of its inverse above STAG_DENSE_MAX
+  void coarse_solve(const double* b, double* x) {
+    const StagLevel& C = *lv.back();
+    if (C.nx * C.ny > STAG_DENSE_MAX)
+      launch_st_circulant_solve(s, C.nx, C.ny, coarse_inv.p, b, x);
+    else
+      launch_st_dense_solve(s, C.nx * C.ny, coarse_inv.p, b, x);
   }
   void launch_copy_grid(const double* from, double* to) {
     MCHECK(hipMemcpyAsync(to, from, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
@@ -418,9 +489,11 @@ Momentum::Momentum(const TreeMesh& m, double viscosity, int scheme, const Multig
     const StagLevel& F = *D.lv.back();
     add_level(F.nx / 2, F.ny / 2, 2 * F.hx, 2 * F.hy);
   }
-  // the coarsest level's dense inverse is formed on the host; above this
-  // size the Krylov solve takes the operator's diagonal as preconditioner
-  D.use_mg = int64_t(D.lv.back()->nx) * D.lv.back()->ny <= STAG_DENSE_MAX;
+  // the coarsest level is solved exactly (dense inverse, or the convolution
+  // with the first column of its inverse, nc^2 operations per solve); above
+  // STAG_CIRC_MAX unknowns the Krylov solve takes the operator's diagonal as
+  // preconditioner instead
+  D.use_mg = int64_t(D.lv.back()->nx) * D.lv.back()->ny <= STAG_CIRC_MAX;
   MCHECK(hipStreamSynchronize(stream));
 }
 

@@ -546,6 +546,29 @@ __global__ void __launch_bounds__(256) k_st_dense_solve(int n, const double* __r
   if (lane == 0) x[row] = acc;
 }
 
+// x = A^-1 b for a symmetric circulant A on an nx x ny periodic grid, whose
+// inverse is the periodic convolution with its first column g (row-major
+// (j, i)): x[r] = sum_c g[(r_i - c_i) mod nx, (r_j - c_j) mod ny] b[c]. One
+// wavefront per output cell, lanes striding along the grid rows.
+__global__ void __launch_bounds__(256) k_st_circulant_solve(int nx, int ny, const double* __restrict__ g,
+                                                            const double* __restrict__ b, double* __restrict__ x) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= nx * ny) return;
+  const int ri = row % nx, rj = row / nx;
+  double acc = 0.0;
+  for (int cj = 0; cj < ny; ++cj) {
+    const int dj = rj >= cj ? rj - cj : rj - cj + ny;
+    const double* gr = g + size_t(dj) * nx;
+    const double* br = b + size_t(cj) * nx;
+    for (int ci = lane; ci < nx; ci += 64) {
+      const int di = ri >= ci ? ri - ci : ri - ci + nx;
+      acc += gr[di] * br[ci];
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (lane == 0) x[row] = acc;
+}
+
 __global__ void __launch_bounds__(STAG_WG) k_st_cg_update_xr(int n, double inv_vol, double* __restrict__ x,
                                                              double* __restrict__ r, const double* __restrict__ p,
                                                              const double* __restrict__ Ap,
@@ -605,6 +628,9 @@ void launch_st_mg_prolong_smooth(hipStream_t s, int ncx, int ncy, int comp, cons
 }
 void launch_st_dense_solve(hipStream_t s, int n, const double* inv, const double* b, double* x) {
   hipLaunchKernelGGL(k_st_dense_solve, dim3(cdiv2(n, 4)), dim3(256), 0, s, n, inv, b, x);
+}
+void launch_st_circulant_solve(hipStream_t s, int nx, int ny, const double* g, const double* b, double* x) {
+  hipLaunchKernelGGL(k_st_circulant_solve, dim3(cdiv2(nx * ny, 4)), dim3(256), 0, s, nx, ny, g, b, x);
 }
 void launch_st_cg_update_xr(hipStream_t s, int n, double inv_vol, double* x, double* r, const double* p,
                             const double* Ap, const double* scal, double* part) {

@@ -34,9 +34,12 @@ constexpr int STAG_UPWIND = 0, STAG_CENTRED = 1, STAG_LW = 2;
 // (grid-stride loops beyond it keep the partials few).
 constexpr int STAG_WG = 256;
 constexpr int STAG_MAX_WG = 1024;
-// Largest coarsest level of the viscous staggered multigrid (its dense
-// inverse is formed on the host).
+// Largest coarsest level of the viscous staggered multigrid whose dense
+// inverse is formed (Gauss-Jordan on the host); a larger one is solved by
+// the periodic convolution with the first column of its inverse, up to
+// STAG_CIRC_MAX unknowns.
 constexpr int STAG_DENSE_MAX = 1024;
+constexpr int STAG_CIRC_MAX = 65536;
 
 void launch_st_gather(hipStream_t s, int n, const int* fx, const int* fy, const double* u, double* X, double* Y);
 void launch_st_scatter(hipStream_t s, int n, const int* fx, const int* fy, const double* X, const double* Y,
@@ -100,6 +103,9 @@ void launch_st_mg_restrict(hipStream_t s, int ncx, int ncy, int comp, const doub
 void launch_st_mg_prolong(hipStream_t s, int ncx, int ncy, int comp, const double* x_c, double* x_f);
 // x = inv b with a dense n x n row-major inverse
 void launch_st_dense_solve(hipStream_t s, int n, const double* inv, const double* b, double* x);
+// x = A^-1 b with a symmetric circulant A on a periodic nx x ny grid, given
+// the first column g of A^-1 (nx * ny values, row-major)
+void launch_st_circulant_solve(hipStream_t s, int nx, int ny, const double* g, const double* b, double* x);
 // fused passes with the operations of the unfused sequence: the zero-start
 // sweep and the next one; the restriction of the residual b_f - M x_f; the
 // prolongation (x_f + P x_c, not stored) and the first post-sweep into out
