@@ -68,32 +68,32 @@ namespace {
 std::vector<double> circulant_inverse_column(const StagLevel& C) {
   const int nx = C.nx, ny = C.ny;
   const double two_pi = 2.0 * std::acos(-1.0);
+  // cos(2 pi q / n) for q = 0..n-1; cos(2 pi k m / n) is entry (k m) mod n
   auto cos_table = [&](int n) {
-    std::vector<double> t(size_t(n) * n);
-    for (int k = 0; k < n; ++k)
-      for (int m = 0; m < n; ++m) t[size_t(k) * n + m] = std::cos(two_pi * double((int64_t(k) * m) % n) / n);
+    std::vector<double> t(n);
+    for (int q = 0; q < n; ++q) t[q] = std::cos(two_pi * double(q) / n);
     return t;
   };
   const std::vector<double> cx = cos_table(nx), cy = cos_table(ny);
   std::vector<double> lam(size_t(nx) * ny);
   for (int ky = 0; ky < ny; ++ky)
     for (int kx = 0; kx < nx; ++kx) {
-      const double l = C.k.md + 2.0 * C.k.mx * cx[size_t(kx) * nx + 1 % nx] + 2.0 * C.k.my * cy[size_t(ky) * ny + 1 % ny];
+      const double l = C.k.md + 2.0 * C.k.mx * cx[kx] + 2.0 * C.k.my * cy[ky];
       if (!(l > 0)) throw std::invalid_argument("coarsest viscous operator is not positive definite");
       lam[size_t(ky) * nx + kx] = 1.0 / l;
     }
-  // t[kx, n] = sum_ky cy[ky, n] / lambda(kx, ky)
+  // t[kx, n] = sum_ky cos(2 pi ky n / ny) / lambda(kx, ky)
   std::vector<double> t(size_t(nx) * ny, 0.0), g(size_t(nx) * ny, 0.0);
   for (int ky = 0; ky < ny; ++ky)
     for (int n = 0; n < ny; ++n) {
-      const double c = cy[size_t(ky) * ny + n];
+      const double c = cy[(int64_t(ky) * n) % ny];
       for (int kx = 0; kx < nx; ++kx) t[size_t(kx) * ny + n] += c * lam[size_t(ky) * nx + kx];
     }
   const double scale = 1.0 / (double(nx) * ny);
   for (int kx = 0; kx < nx; ++kx)
     for (int n = 0; n < ny; ++n) {
       const double tv = t[size_t(kx) * ny + n] * scale;
-      for (int m = 0; m < nx; ++m) g[size_t(n) * nx + m] += cx[size_t(kx) * nx + m] * tv;
+      for (int m = 0; m < nx; ++m) g[size_t(n) * nx + m] += cx[(int64_t(kx) * m) % nx] * tv;
     }
   double err = 0.0;
   for (int j = 0; j < ny; ++j)
