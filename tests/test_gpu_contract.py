@@ -293,3 +293,33 @@ def test_coarsest_level_above_8192_cells():
         assert st.converged
         assert rel(x, exact) < 1e-9, (precision, rel(x, exact))
         gs.close()
+
+
+def test_coarsest_level_above_16384_cells():
+    """A 5 x 5 layout of 32^2 roots stops at 25 600 cells (the reference
+    factors any coarsest size, geometric_preconditioner.jl:6-24): a 25 601^2
+    dense inverse (5.2 GB) from the blocked Gauss-Jordan. The pressure equals
+    the exact solution of the oracle's assembled operator."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    _print("case coarse 25600")
+    kw = dict(base_mesh=(160, 160), block_size=(32, 32), periodic=(False, False), max_level=1,
+              refine=lambda k, o, e: k[0] == 0 and (k[1] + 2 * k[2]) % 3 == 0)
+    om, pm = OMesh(**kw), tpamr.BlockTreeMesh(**kw)
+    k = np.exp(0.5 * np.cos(np.arange(1, pm.nfaces + 1)))
+    A = OSolver(om, k, multigrid=False).level_matrix(0).tocsc()
+    V = om.volumes
+    n = pm.ncells
+    aug = sp.bmat([[A, sp.csc_matrix(V.reshape(-1, 1))], [sp.csc_matrix(V.reshape(1, -1)), None]]).tocsc()
+    b = techref_noise(n)
+    bi = b * V
+    bi -= V * bi.sum() / V.sum()
+    exact = spla.spsolve(aug, np.concatenate([bi, [0.0]]))[:n]
+    gs = tpamr.EllipticSolver(tpamr.PoissonOperator(pm, k), precision="fp32_refine")
+    sizes = [lv["ncells"] for lv in gs.levels]
+    assert sizes[-1] == 25600, sizes
+    x = np.zeros(n)
+    st = gs.solve(x, b, rtol=1e-12, max_cycles=100)
+    assert st.converged
+    assert rel(x, exact) < 1e-9, rel(x, exact)
+    gs.close()

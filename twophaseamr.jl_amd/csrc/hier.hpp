@@ -239,9 +239,10 @@ struct HaloSchedule {
 HaloSchedule halo_schedule(const TreeMesh& m, int halo);
 
 // Largest coarsest level (cells) of the dense coarse solve: its augmented
-// inverse is 2.1 GB of FP64 at this size, built by a blocked Gauss-Jordan
-// on the device and applied as one matrix-vector product per V-cycle.
-constexpr int COARSE_MAX = 16384;
+// inverse is 8.6 GB of FP64 at this size (N^2 stays below 2^31), built by a
+// blocked Gauss-Jordan on the device and applied as one matrix-vector
+// product per V-cycle.
+constexpr int COARSE_MAX = 32768;
 
 struct MultigridConfig {
   int coarse_size = 64, pre = 2, post = 2, max_levels = 32;
